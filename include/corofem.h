@@ -1,0 +1,233 @@
+/*
+ * corofem -- MI355X (gfx950) corotational tetrahedral FEM implicit step.
+ *
+ * C-ABI drop-in for the reference's hot path (Saphyre2):
+ *   element set + cache   <-> ms::fem::EditableTetrahedronElementSet3 / IForce3
+ *                             (ms/fem/TetrahedronElementSet3.h:29-57,70-261)
+ *   linear solver         <-> ns::IIterativeLinearSystemSolver / ILSS_CG
+ *                             (ns/IIterativeLinearSystemSolver.h:22-48, ns/ILSS_CG.h:19-151)
+ *   leaf step             <-> ds::LeafDSH_Solid3D_FEM::Create/Step/FixedStep/QueryStats
+ *                             (ds/dsh/LeafDSH_Solid3D_FEM.h:68-81, .cpp:125-256,898-990)
+ *   2D twin               <-> ms::fem::TriangleElementSet2 + ds::LeafDSH_Solid2D_FEM
+ *
+ * Conventions
+ *   - Every entry point returns an int status (CF_OK = 0, negative on error);
+ *     no C++ exception crosses the ABI. cf_last_error() describes the last error.
+ *   - Node vectors are flat FP32 arrays of 3*N values, node i at [3i, 3i+2]
+ *     (the reference's Vec3 arrays reinterpreted as Real*, as ILSS_CG sees them).
+ *   - Indices are uint32 (the reference's uint16 limit is not kept).
+ *   - Pointers named d_* are device pointers (from cf_device_alloc or any HIP
+ *     allocation on the current device); pointers named h_* are host pointers.
+ *   - Operator outputs ACCUMULATE (+=) like IForce3 (TetrahedronElementSet3.h:20-21).
+ *   - Each handle owns one HIP stream; calls on one handle are not thread-safe,
+ *     different handles are independent.
+ */
+#ifndef COROFEM_H
+#define COROFEM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- status */
+#define CF_OK 0
+#define CF_ERR_INVALID_ARG (-1)
+#define CF_ERR_HIP (-2)
+#define CF_ERR_OUT_OF_MEMORY (-3)
+#define CF_ERR_NAN (-5)
+#define CF_ERR_UNSUPPORTED (-6)
+#define CF_ERR_NO_DEVICE (-7)
+#define CF_ERR_CAPACITY (-8)
+
+/* CG termination reasons (cf_cg_result.reason) */
+#define CF_CG_CONVERGED_REL 1   /* delta <= rel^2 * delta_0          (ILSS_CG.h:86-88) */
+#define CF_CG_CONVERGED_ABS 2   /* delta <= abs^2                    (ILSS_CG.h:86)    */
+#define CF_CG_MAX_ITER 3        /* k == max_iterations               (ILSS_CG.h:88)    */
+#define CF_CG_BREAKDOWN 4       /* |d^T q| < abs^2                   (ILSS_CG.h:97)    */
+
+/* material methods (ms/fem/Params.h:50-62); only C_WRP is implemented */
+#define CF_MM_L 0
+#define CF_MM_C_WRP 1
+/* rotation methods (ms/fem/Params.h:63-67) */
+#define CF_RM_ID 0
+#define CF_RM_QR 1
+#define CF_RM_MSVD 2
+#define CF_RM_DAPD 3
+/* differential methods (ms/fem/Params.h:68-72); only Truncated is implemented */
+#define CF_DM_TRUNCATED 0
+
+/* ---------------------------------------------------------------- params */
+/* Mirrors ms::fem::Params (ms/fem/Params.h:10-99). */
+typedef struct cf_fem_params {
+    float young_modulus;              /* default 1000     */
+    float poisson_ratio;              /* default 0.25     */
+    float damping_ratio;              /* default 1.0      */
+    float rayleigh_freq1;             /* default 0.1      */
+    float rayleigh_freq2;             /* default 10000    */
+    float degenerate_threshold_det_f; /* default 0.1      */
+    int32_t mm;                       /* default CF_MM_C_WRP */
+    int32_t rm;                       /* default CF_RM_MSVD  */
+    int32_t dm;                       /* default CF_DM_TRUNCATED */
+} cf_fem_params;
+
+/* Mirrors the leaf's integrator/solver knobs (ds/dsh/LeafDSH_Solid3D_FEM.h:93-274). */
+typedef struct cf_solid_params {
+    cf_fem_params fem;
+    float fixed_dt;            /* default 0.001 (leaf default); configs use 1/60 */
+    float time_scale;          /* default 1 */
+    float density;             /* default 1000 */
+    float gravity[3];          /* default (0,0,0) */
+    float air_drag_coeff;      /* default 0 */
+    float cg_rel_epsilon;      /* default 1e-3 */
+    float cg_abs_epsilon;      /* default 1e-6 */
+    uint32_t cg_max_iter;      /* default 20   */
+    int32_t warm_start;        /* default 1    */
+    int32_t compute_energy;    /* 1: elastic energy each step like FixedStep (:977); default 1 */
+    float thickness;           /* 2D only (LeafDSH_Solid2D_FEM.h:217); default 1 */
+} cf_solid_params;
+
+/* Per-step statistics (LeafDSH_Solid3D_FEM::QueryStats fields, .cpp:698-770). */
+typedef struct cf_step_stats {
+    uint32_t num_fixed_steps;     /* fixed steps taken by this Step() call */
+    int32_t cg_iters;             /* last fixed step */
+    float cg_rel_residual;
+    float cg_abs_residual;
+    int32_t cg_reason;
+    uint32_t num_degenerate;      /* elements with det F <= 0 */
+    float sum_degenerate_det_f;
+    float elastic_energy;
+    float min_det_f;
+    double step_ms;               /* device time of the last fixed step */
+} cf_step_stats;
+
+typedef struct cf_cg_result {
+    int32_t iterations;
+    float rel_residual;
+    float abs_residual;
+    int32_t reason;
+} cf_cg_result;
+
+void cf_default_fem_params(cf_fem_params* p);
+void cf_default_solid_params(cf_solid_params* p);
+
+/* ---------------------------------------------------------------- runtime */
+const char* cf_last_error(void);
+int cf_version(void);
+int cf_device_count(int* n);
+int cf_set_device(int device);
+int cf_device_alloc(size_t bytes, void** d_ptr);
+int cf_device_free(void* d_ptr);
+int cf_memcpy_htod(void* d_dst, const void* h_src, size_t bytes);
+int cf_memcpy_dtoh(void* h_dst, const void* d_src, size_t bytes);
+int cf_memcpy_dtod(void* d_dst, const void* d_src, size_t bytes);
+int cf_memset_device(void* d_ptr, int value, size_t bytes);
+int cf_device_synchronize(void);
+
+/* ---------------------------------------------------------------- mesh input
+ * Make_TetSolidShape3_Box (Geo/shape/TetSolidShape3.cpp:921-999): nx,ny,nz are
+ * vertex counts per axis (cubes + 1); 5 tets per cube with mirrored parity.
+ * h_pos receives 3*nx*ny*nz floats, h_tets 4*5*(nx-1)(ny-1)(nz-1) indices. */
+int cf_mesh_box_counts(uint32_t nx, uint32_t ny, uint32_t nz, uint32_t* n_nodes, uint32_t* n_tets);
+int cf_mesh_box(const float half_sizes[3], uint32_t nx, uint32_t ny, uint32_t nz,
+                float* h_pos, uint32_t* h_tets);
+/* Make_MeshSolidShape2_Box (Geo/shape/MeshSolidShape2.cpp:1651-1675). */
+int cf_mesh_box2_counts(uint32_t nx, uint32_t ny, uint32_t* n_nodes, uint32_t* n_tris);
+int cf_mesh_box2(const float half_sizes[2], uint32_t nx, uint32_t ny, float* h_pos, uint32_t* h_tris);
+
+/* ---------------------------------------------------------------- element set
+ * EditableTetrahedronElementSet3::BeginEdition/AddElement/EndEdition
+ * (ms/fem/TetrahedronElementSet3.cpp:2084-2166). cf_tetset_create is the bulk
+ * form of BeginEdition + AddElement x T + EndEdition. */
+typedef struct cf_tetset cf_tetset;
+int cf_tetset_create(uint32_t n_nodes, const float* h_ref_pos, uint32_t n_tets, const uint32_t* h_tets,
+                     const cf_fem_params* params, cf_tetset** out);
+int cf_tetset_begin_edition(uint32_t n_nodes, const float* h_ref_pos, uint32_t n_tets,
+                            const cf_fem_params* params, cf_tetset** out);
+int cf_tetset_add_element(cf_tetset* set, uint32_t i, uint32_t j, uint32_t k, uint32_t l);
+int cf_tetset_end_edition(cf_tetset* set);
+int cf_tetset_set_params(cf_tetset* set, const cf_fem_params* params);      /* SetParams (:2168-2212) */
+int cf_tetset_destroy(cf_tetset* set);
+int cf_tetset_info(const cf_tetset* set, uint32_t* n_nodes, uint32_t* n_tets, float* total_volume);
+/* Per-element precompute in the caller's element order: h_inv_dm (9T, row-major), h_vol (T). */
+int cf_tetset_get_elements(const cf_tetset* set, float* h_inv_dm, float* h_vol);
+
+/* ---------------------------------------------------------------- IForce3 cache
+ * CreateCache(inv_mass) / BeginEvaluation / f / df_x / V / EndEvaluation
+ * (TetrahedronElementSet3.h:48-55,86-97). x, prev_x, v, f, dx, df: device,
+ * 3N floats. BeginEvaluation computes F, det F and the rotation R per element.
+ * With CF_RM_DAPD, elements with det F <= threshold take R = Id (the
+ * reference's invalid-DoC fallback, TetrahedronElementSet3.cpp:885-890); the
+ * DoC degenerate branch is not implemented and such elements are counted. */
+typedef struct cf_cache cf_cache;
+int cf_cache_create(cf_tetset* set, const float* h_inv_mass /* N, may be NULL */, cf_cache** out);
+int cf_cache_destroy(cf_cache* cache);
+int cf_begin_evaluation(cf_cache* cache, const float* d_prev_x, const float* d_x, const float* d_v, float t);
+int cf_f(cf_cache* cache, float* d_f);                       /* += f_e (+ f_p, a no-op) */
+int cf_f_e(cf_cache* cache, float* d_f);                     /* += elastic force (C_f_e :1053-1096) */
+int cf_df_x(cf_cache* cache, const float* d_dx, float* d_df); /* += K(x) dx (C_df_x :1761-1836)    */
+int cf_V(cf_cache* cache, double* h_energy);                 /* C_V (:1456-1488)                  */
+int cf_end_evaluation(cf_cache* cache);
+int cf_cache_stats(const cf_cache* cache, uint32_t* num_degenerate, float* sum_degenerate_det_f,
+                   float* min_det_f);
+/* R per element in the caller's element order (9T row-major) and det F (T). */
+int cf_cache_get_rotations(const cf_cache* cache, float* h_R, float* h_det_f);
+int cf_cache_synchronize(cf_cache* cache);
+
+/* ---------------------------------------------------------------- linear solver
+ * IIterativeLinearSystemSolver::Solve(D_Av, D_ApplyConstraints, x, b, rel, abs, max)
+ * (ns/IIterativeLinearSystemSolver.h:22-48) with the ILSS_CG algorithm
+ * (ns/ILSS_CG.h:38-151). The callbacks receive DEVICE pointers and must
+ * enqueue their work on the stream returned by cf_cg_stream(); x (in/out, warm
+ * start) and b are device pointers of length n. Dot products accumulate in
+ * FP64. */
+typedef void (*cf_av_fn)(void* user, float* d_Av, const float* d_v);
+typedef void (*cf_constraints_fn)(void* user, float* d_v);
+typedef struct cf_cg cf_cg;
+int cf_cg_create(uint32_t n, cf_cg** out);                   /* ILSS_CG::Init (:30-36) */
+int cf_cg_destroy(cf_cg* cg);
+void* cf_cg_stream(cf_cg* cg);
+int cf_cg_solve(cf_cg* cg, cf_av_fn av, cf_constraints_fn apply_constraints, void* user,
+                float* d_x, const float* d_b, float rel_epsilon, float abs_epsilon, uint32_t max_iterations,
+                cf_cg_result* result);
+
+/* ---------------------------------------------------------------- leaf solid (3D)
+ * LeafDSH_Solid3D_FEM: Create (.cpp:125-223), Step (:225-256: fixed-dt
+ * accumulator, strict '>'), FixedStep (:898-990) with QIE-v (:1205-1328),
+ * uniform node mass (RebuildMass :1591-1611), kinematic node constraints
+ * (KNC, :877-896,1045-1053,1268-1295). State stays resident on the device. */
+typedef struct cf_solid cf_solid;
+int cf_solid_create(uint32_t n_nodes, const float* h_ref_pos, uint32_t n_tets, const uint32_t* h_tets,
+                    const cf_solid_params* params, cf_solid** out);
+int cf_solid_destroy(cf_solid* s);
+/* Position KNC at the given targets (NULL: rest positions), like
+   CreateKinematicPointConstraint + eKM_Static. Replaces the current set. */
+int cf_solid_set_knc(cf_solid* s, uint32_t n, const uint32_t* h_nids, const float* h_targets);
+int cf_solid_step(cf_solid* s, float dt, cf_step_stats* stats);
+int cf_solid_fixed_step(cf_solid* s, float dt, cf_step_stats* stats);
+int cf_solid_get_state(const cf_solid* s, float* h_pos, float* h_vel);     /* QueryState (GLeafDSH_Model.h:34-43) */
+int cf_solid_set_state(cf_solid* s, const float* h_pos, const float* h_vel);
+int cf_solid_device_state(cf_solid* s, float** d_pos, float** d_vel);
+int cf_solid_info(const cf_solid* s, uint32_t* n_nodes, uint32_t* n_tets, float* node_mass,
+                  float rayleigh[2]);
+/* Inspection of the step's stages on the current state (no state change):
+   BeginEvaluation + QIE-v right-hand side b and warm start y0 (:1209-1295),
+   and q = S(fM M d - fK K d) on the current rotations (LS_A_times_d + LS_ZeroKNC,
+   :1045-1079). Host arrays of 3N floats. */
+int cf_solid_assemble_rhs(cf_solid* s, float dt, float* h_b, float* h_y0);
+int cf_solid_apply_A(cf_solid* s, float factor_m, float factor_k, const float* h_d, float* h_q);
+/* Time one K*x (df_x) on the solid's current rotations: mean device ms over reps. */
+int cf_solid_time_df_x(cf_solid* s, uint32_t reps, double* ms_per_call);
+
+/* ---------------------------------------------------------------- tuning */
+/* Elements per LDS tile of the node-gather K*x (0 = library default). Takes
+   effect for sets created afterwards. */
+int cf_set_tile_elems(uint32_t tile_elems);
+int cf_get_tile_elems(uint32_t* tile_elems);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* COROFEM_H */

@@ -1,0 +1,849 @@
+// corofem oracle -- CPU restatement of the reference's FP32 corotational FEM
+// step. Test infrastructure only (see corofem_oracle.h). Every function below
+// names the reference file:line whose arithmetic it restates; the evaluation
+// order of each FP32 expression follows the reference so that, compiled
+// without FMA contraction, results are reproducible bit for bit.
+#include "corofem_oracle.h"
+
+#include <cmath>
+#include <cstring>
+#include <cstdlib>
+#include <vector>
+#include <algorithm>
+#include <utility>
+
+namespace {
+
+// ---------------------------------------------------------------- small math
+// Row-major fixed-size matrices, products accumulate from 0 in k order
+// (Mal/GMat.h:135-145, Mal/GMatUtils.h:13-24).
+struct M3 { float a[3][3]; };
+struct V3 { float v[3]; };
+
+inline M3 mzero() { M3 m; std::memset(&m, 0, sizeof(m)); return m; }
+inline M3 mid() { M3 m = mzero(); m.a[0][0] = m.a[1][1] = m.a[2][2] = 1.0f; return m; }
+inline M3 mload(const float* p) { M3 m; std::memcpy(m.a, p, 9 * sizeof(float)); return m; }
+inline void mstore(const M3& m, float* p) { std::memcpy(p, m.a, 9 * sizeof(float)); }
+inline M3 mtr(const M3& m) { M3 t; for (int i = 0; i < 3; i++) for (int j = 0; j < 3; j++) t.a[i][j] = m.a[j][i]; return t; }
+inline M3 mmul(const M3& x, const M3& y) {
+    M3 p;
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+            float acc = 0.0f;
+            for (int k = 0; k < 3; k++) acc += x.a[i][k] * y.a[k][j];
+            p.a[i][j] = acc;
+        }
+    return p;
+}
+inline V3 mvec(const M3& m, const V3& v) {
+    V3 r;
+    for (int i = 0; i < 3; i++) {
+        float acc = 0.0f;
+        for (int j = 0; j < 3; j++) acc += m.a[i][j] * v.v[j];
+        r.v[i] = acc;
+    }
+    return r;
+}
+inline V3 vadd(const V3& a, const V3& b) { V3 r; for (int i = 0; i < 3; i++) r.v[i] = a.v[i] + b.v[i]; return r; }
+inline V3 vsub(const V3& a, const V3& b) { V3 r; for (int i = 0; i < 3; i++) r.v[i] = a.v[i] - b.v[i]; return r; }
+inline V3 vscale(const V3& a, float s) { V3 r; for (int i = 0; i < 3; i++) r.v[i] = s * a.v[i]; return r; }
+inline float vdot(const V3& a, const V3& b) { float acc = 0.0f; for (int i = 0; i < 3; i++) acc += a.v[i] * b.v[i]; return acc; }
+inline V3 vcross(const V3& a, const V3& b) {  // Mal/GVec.h:89-91
+    V3 r;
+    r.v[0] = a.v[1] * b.v[2] - a.v[2] * b.v[1];
+    r.v[1] = a.v[2] * b.v[0] - a.v[0] * b.v[2];
+    r.v[2] = a.v[0] * b.v[1] - a.v[1] * b.v[0];
+    return r;
+}
+inline float vnorm(const V3& a) { return std::sqrt(vdot(a, a)); }
+// v / s == v * (1/s)  (Mal/GVec.h:85)
+inline V3 vnormalized(const V3& a) { float inv = 1.0f / vnorm(a); return vscale(a, inv); }
+inline V3 vload(const float* p) { V3 r; r.v[0] = p[0]; r.v[1] = p[1]; r.v[2] = p[2]; return r; }
+inline M3 from_cols(const V3& c0, const V3& c1, const V3& c2) {
+    M3 m;
+    for (int i = 0; i < 3; i++) { m.a[i][0] = c0.v[i]; m.a[i][1] = c1.v[i]; m.a[i][2] = c2.v[i]; }
+    return m;
+}
+inline V3 col(const M3& m, int j) { V3 r; for (int i = 0; i < 3; i++) r.v[i] = m.a[i][j]; return r; }
+inline void setcol(M3& m, int j, const V3& c) { for (int i = 0; i < 3; i++) m.a[i][j] = c.v[i]; }
+
+// Mal/GMat.h:236-244
+inline float det3(const M3& m) {
+    return m.a[0][0] * m.a[1][1] * m.a[2][2]
+         - m.a[0][0] * m.a[2][1] * m.a[1][2]
+         + m.a[1][0] * m.a[2][1] * m.a[0][2]
+         - m.a[1][0] * m.a[0][1] * m.a[2][2]
+         + m.a[2][0] * m.a[0][1] * m.a[1][2]
+         - m.a[2][0] * m.a[1][1] * m.a[0][2];
+}
+// Mal/GMat.h:291-313
+inline M3 inv3(const M3& m, float det) {
+    float id = 1.0f / det;
+    M3 r;
+    r.a[0][0] = id * (m.a[1][1] * m.a[2][2] - m.a[1][2] * m.a[2][1]);
+    r.a[0][1] = id * (m.a[0][2] * m.a[2][1] - m.a[0][1] * m.a[2][2]);
+    r.a[0][2] = id * (m.a[0][1] * m.a[1][2] - m.a[0][2] * m.a[1][1]);
+    r.a[1][0] = id * (m.a[1][2] * m.a[2][0] - m.a[1][0] * m.a[2][2]);
+    r.a[1][1] = id * (m.a[0][0] * m.a[2][2] - m.a[0][2] * m.a[2][0]);
+    r.a[1][2] = id * (m.a[0][2] * m.a[1][0] - m.a[0][0] * m.a[1][2]);
+    r.a[2][0] = id * (m.a[1][0] * m.a[2][1] - m.a[1][1] * m.a[2][0]);
+    r.a[2][1] = id * (m.a[0][1] * m.a[2][0] - m.a[0][0] * m.a[2][1]);
+    r.a[2][2] = id * (m.a[0][0] * m.a[1][1] - m.a[0][1] * m.a[1][0]);
+    return r;
+}
+inline M3 inv3(const M3& m) { return inv3(m, det3(m)); }
+
+// Newton polar decomposition, Mal/GMatDecomposition.h:411-432:
+// H <- 0.5*(H + inv(H^T)), error = |H1-H0|_F^2 (sum in storage order),
+// stop when error <= 0.01 or after 10 iterations.
+inline M3 polar_newton(const M3& H, uint32_t* iters) {
+    M3 H0 = H;
+    float err = 0.0f;
+    unsigned n = 0;
+    do {
+        M3 it = inv3(mtr(H0));
+        M3 H1;
+        for (int i = 0; i < 3; i++) for (int j = 0; j < 3; j++) H1.a[i][j] = 0.5f * (H0.a[i][j] + it.a[i][j]);
+        err = 0.0f;
+        for (int i = 0; i < 3; i++) for (int j = 0; j < 3; j++) { float d = H1.a[i][j] - H0.a[i][j]; err += d * d; }
+        H0 = H1;
+        n++;
+    } while (n < 10u && err > 0.01f);
+    if (iters) *iters = n;
+    return H0;
+}
+
+// ---- PhysBAM-style 3x3 SVD, FP32, as adapted in Mal/GMatDecomposition_SVD3x3_PhysBAM.h
+struct Sym3 { float x11, x21, x31, x22, x32, x33; };
+struct Sym2 { float x11, x21, x22; };
+
+inline V3 orthogonal_vector(const V3& v) {  // :35-49
+    float ax = std::fabs(v.v[0]), ay = std::fabs(v.v[1]), az = std::fabs(v.v[2]);
+    V3 r;
+    if (ax < ay) {
+        if (ax < az) { r.v[0] = 0; r.v[1] = v.v[2]; r.v[2] = -v.v[1]; }
+        else { r.v[0] = v.v[1]; r.v[1] = -v.v[0]; r.v[2] = 0; }
+    } else {
+        if (ay < az) { r.v[0] = -v.v[2]; r.v[1] = 0; r.v[2] = v.v[0]; }
+        else { r.v[0] = v.v[1]; r.v[1] = -v.v[0]; r.v[2] = 0; }
+    }
+    return r;
+}
+inline Sym3 normal_equations(const M3& F) {  // :129-137
+    Sym3 s;
+    s.x11 = F.a[0][0] * F.a[0][0] + F.a[1][0] * F.a[1][0] + F.a[2][0] * F.a[2][0];
+    s.x21 = F.a[0][1] * F.a[0][0] + F.a[1][1] * F.a[1][0] + F.a[2][1] * F.a[2][0];
+    s.x31 = F.a[0][2] * F.a[0][0] + F.a[1][2] * F.a[1][0] + F.a[2][2] * F.a[2][0];
+    s.x22 = F.a[0][1] * F.a[0][1] + F.a[1][1] * F.a[1][1] + F.a[2][1] * F.a[2][1];
+    s.x32 = F.a[0][2] * F.a[0][1] + F.a[1][2] * F.a[1][1] + F.a[2][2] * F.a[2][1];
+    s.x33 = F.a[0][2] * F.a[0][2] + F.a[1][2] * F.a[1][2] + F.a[2][2] * F.a[2][2];
+    return s;
+}
+inline Sym3 cofactor(const Sym3& s) {  // :139-147
+    Sym3 c;
+    c.x11 = s.x22 * s.x33 - s.x32 * s.x32;
+    c.x21 = s.x32 * s.x31 - s.x21 * s.x33;
+    c.x31 = s.x21 * s.x32 - s.x22 * s.x31;
+    c.x22 = s.x11 * s.x33 - s.x31 * s.x31;
+    c.x32 = s.x21 * s.x31 - s.x11 * s.x32;
+    c.x33 = s.x11 * s.x22 - s.x21 * s.x21;
+    return c;
+}
+inline V3 largest_column_normalized(const Sym3& s) {  // :149-156
+    float s11 = s.x11 * s.x11, s12 = s.x21 * s.x21, s13 = s.x31 * s.x31;
+    float s22 = s.x22 * s.x22, s23 = s.x32 * s.x32, s33 = s.x33 * s.x33;
+    float c1 = s11 + s12 + s13, c2 = s12 + s22 + s23, c3 = s13 + s23 + s33;
+    V3 r;
+    if (c1 > c2) {
+        if (c1 > c3) { float inv = 1.0f / std::sqrt(c1); r.v[0] = s.x11 * inv; r.v[1] = s.x21 * inv; r.v[2] = s.x31 * inv; return r; }
+    } else if (c2 > c3) { float inv = 1.0f / std::sqrt(c2); r.v[0] = s.x21 * inv; r.v[1] = s.x22 * inv; r.v[2] = s.x32 * inv; return r; }
+    if (c3 > 0) { float inv = 1.0f / std::sqrt(c3); r.v[0] = s.x31 * inv; r.v[1] = s.x32 * inv; r.v[2] = s.x33 * inv; return r; }
+    r.v[0] = 1; r.v[1] = 0; r.v[2] = 0;
+    return r;
+}
+inline void largest_column_normalized2(const Sym2& s, float out[2]) {  // :75-82
+    float s11 = s.x11 * s.x11, s12 = s.x21 * s.x21, s22 = s.x22 * s.x22;
+    float c1 = s11 + s12, c2 = s12 + s22;
+    if (c1 > c2) { float inv = 1.0f / std::sqrt(c1); out[0] = s.x11 * inv; out[1] = s.x21 * inv; }
+    else if (c2 > 0) { float inv = 1.0f / std::sqrt(c2); out[0] = s.x21 * inv; out[1] = s.x22 * inv; }
+    else { out[0] = 1; out[1] = 0; }
+}
+// Smith (1961) trigonometric eigenvalues, sorted decreasingly (:161-197)
+// The reference writes Sqrt(3) with an int argument (:165), which deduces
+// mal::Sqrt<int> (Mal/RealUtils.h:26-27) and truncates sqrt(3) to 1.
+const float kRefRootThree = float(int(std::sqrt(3.0f)));
+inline V3 fast_eigenvalues(const Sym3& s, float root_three) {
+    const float one_third = 1.0f / 3.0f;
+    const float one_sixth = 1.0f / 6.0f;
+    float m = one_third * (s.x11 + s.x22 + s.x33);
+    float a11 = s.x11 - m, a22 = s.x22 - m, a33 = s.x33 - m;
+    float a12s = s.x21 * s.x21, a13s = s.x31 * s.x31, a23s = s.x32 * s.x32;
+    float p = one_sixth * (a11 * a11 + a22 * a22 + a33 * a33 + 2 * (a12s + a13s + a23s));
+    float q = 0.5f * (a11 * (a22 * a33 - a23s) - a22 * a13s - a33 * a12s) + s.x21 * s.x31 * s.x32;
+    float sqrt_p = std::sqrt(p), disc = p * p * p - q * q;
+    float phi = one_third * std::atan2(std::sqrt(std::max(0.0f, disc)), q);
+    float c = std::cos(phi), sn = std::sin(phi);
+    float spc = sqrt_p * c, r3sps = root_three * sqrt_p * sn;
+    float l0 = m + 2 * spc, l1 = m - spc - r3sps, l2 = m - spc + r3sps;
+    V3 r;
+    if (l0 < l1) {
+        if (l1 < l2) { r.v[0] = l2; r.v[1] = l1; r.v[2] = l0; }
+        else if (l0 < l2) { r.v[0] = l1; r.v[1] = l2; r.v[2] = l0; }
+        else { r.v[0] = l1; r.v[1] = l0; r.v[2] = l2; }
+    } else {
+        if (l1 > l2) { r.v[0] = l0; r.v[1] = l1; r.v[2] = l2; }
+        else if (l0 > l2) { r.v[0] = l0; r.v[1] = l2; r.v[2] = l1; }
+        else { r.v[0] = l2; r.v[1] = l0; r.v[2] = l1; }
+    }
+    return r;
+}
+// 3x2 matrices as two columns
+struct M32 { V3 c0, c1; };
+inline V3 m32_vec(const M32& m, const float v[2]) {  // GMat<3,2> * GVec<2>
+    V3 r;
+    for (int i = 0; i < 3; i++) { float acc = 0.0f; acc += m.c0.v[i] * v[0]; acc += m.c1.v[i] * v[1]; r.v[i] = acc; }
+    return r;
+}
+inline Sym2 conjugate_with_transpose(const M32& A, const Sym3& B) {  // :100-107
+    M3 fb;
+    fb.a[0][0] = B.x11; fb.a[0][1] = B.x21; fb.a[0][2] = B.x31;
+    fb.a[1][0] = B.x21; fb.a[1][1] = B.x22; fb.a[1][2] = B.x32;
+    fb.a[2][0] = B.x31; fb.a[2][1] = B.x32; fb.a[2][2] = B.x33;
+    // full_B * A (3x3 * 3x2), accumulate from 0
+    M32 BA;
+    BA.c0 = mvec(fb, A.c0);
+    BA.c1 = mvec(fb, A.c1);
+    // Transpose_Times_With_Symmetric_Result(BA, A)  (:86-96)
+    Sym2 r;
+    r.x11 = BA.c0.v[0] * A.c0.v[0] + BA.c0.v[1] * A.c0.v[1] + BA.c0.v[2] * A.c0.v[2];
+    r.x21 = BA.c1.v[0] * A.c0.v[0] + BA.c1.v[1] * A.c0.v[1] + BA.c1.v[2] * A.c0.v[2];
+    r.x22 = BA.c1.v[0] * A.c1.v[0] + BA.c1.v[1] * A.c1.v[1] + BA.c1.v[2] * A.c1.v[2];
+    return r;
+}
+inline M3 fast_eigenvectors(const Sym3& A, const V3& lambda) {  // :199-236
+    bool flipped = false;
+    V3 lf = lambda;
+    if (lambda.v[0] - lambda.v[1] < lambda.v[1] - lambda.v[2]) {
+        lf.v[0] = lambda.v[2];
+        lf.v[2] = lambda.v[0];
+        flipped = true;
+    }
+    Sym3 am = A;
+    am.x11 -= lf.v[0]; am.x22 -= lf.v[0]; am.x33 -= lf.v[0];
+    V3 v1 = largest_column_normalized(cofactor(am));
+    V3 v1o = vnormalized(orthogonal_vector(v1));
+    M32 other;
+    other.c0 = v1o;
+    other.c1 = vcross(v1, v1o);
+    Sym2 ar = conjugate_with_transpose(other, A);
+    ar.x11 -= lf.v[2]; ar.x22 -= lf.v[2];
+    Sym2 arc; arc.x11 = ar.x22; arc.x21 = -ar.x21; arc.x22 = ar.x11;  // GSymmetricMat2x2::Cofactor_Matrix
+    float w[2];
+    largest_column_normalized2(arc, w);
+    V3 v3 = m32_vec(other, w);
+    V3 v2 = vcross(v3, v1);
+    if (flipped) return from_cols(v3, v2, vscale(v1, -1.0f));
+    return from_cols(v1, v2, v3);
+}
+// GFastSVD3x3 (:246-280): F = U diag(s) V^T
+inline void fast_svd3(const M3& F, M3& U, V3& s, M3& Vt, float root_three = kRefRootThree) {
+    Sym3 ne = normal_equations(F);
+    V3 lambda = fast_eigenvalues(ne, root_three);
+    M3 V = fast_eigenvectors(ne, lambda);
+    Vt = mtr(V);
+    if (lambda.v[2] < 0) {
+        lambda.v[0] = std::max(lambda.v[0], 0.0f);
+        lambda.v[1] = std::max(lambda.v[1], 0.0f);
+        lambda.v[2] = std::max(lambda.v[2], 0.0f);
+    }
+    s.v[0] = std::sqrt(lambda.v[0]);
+    s.v[1] = std::sqrt(lambda.v[1]);
+    s.v[2] = std::sqrt(lambda.v[2]);
+    if (det3(F) < 0) s.v[2] = -s.v[2];
+    setcol(U, 0, vnormalized(mvec(F, col(V, 0))));
+    V3 v1o = vnormalized(orthogonal_vector(col(U, 0)));
+    M32 other;
+    other.c0 = v1o;
+    other.c1 = vcross(col(U, 0), v1o);
+    // other_v * ( other_v^T * Normalized(F * V.col1) )
+    V3 fv1 = vnormalized(mvec(F, col(V, 1)));
+    float t[2];
+    { float acc = 0.0f; for (int i = 0; i < 3; i++) acc += other.c0.v[i] * fv1.v[i]; t[0] = acc; }
+    { float acc = 0.0f; for (int i = 0; i < 3; i++) acc += other.c1.v[i] * fv1.v[i]; t[1] = acc; }
+    setcol(U, 1, m32_vec(other, t));
+    setcol(U, 2, vcross(col(U, 0), col(U, 1)));
+}
+
+// ---- element precompute (TetrahedronElement3.h:85-173) ----
+inline M3 compute_D(const V3& r0, const V3& r1, const V3& r2, const V3& r3) {
+    return from_cols(vsub(r1, r0), vsub(r2, r0), vsub(r3, r0));
+}
+float compute_K(const V3& r0, const V3& r1, const V3& r2, const V3& r3, float E, float nu, float K[144]) {
+    float volume = det3(compute_D(r0, r1, r2, r3)) / 6.0f;
+    float x1 = r0.v[0], x2 = r1.v[0], x3 = r2.v[0], x4 = r3.v[0];
+    float y1 = r0.v[1], y2 = r1.v[1], y3 = r2.v[1], y4 = r3.v[1];
+    float z1 = r0.v[2], z2 = r1.v[2], z3 = r2.v[2], z4 = r3.v[2];
+    float J[4][4];
+    // inverse Jacobian, AFEM-16 p.7 (TetrahedronElement3.h:116-123); column 0 unused by B
+    J[0][0] = x2 * (y3 * z4 - y4 * z3) + x3 * (y4 * z2 - y2 * z4) + x4 * (y2 * z3 - y3 * z2);
+    J[1][0] = x1 * (y4 * z3 - y3 * z4) + x3 * (y1 * z4 - y4 * z1) + x4 * (y3 * z1 - y1 * z3);
+    J[2][0] = x1 * (y2 * z4 - y4 * z2) + x2 * (y4 * z1 - y1 * z4) + x4 * (y1 * z2 - y2 * z1);
+    J[3][0] = x1 * (y3 * z2 - y2 * z3) + x2 * (y1 * z3 - y3 * z1) + x3 * (y2 * z1 - y1 * z2);
+    J[0][1] = (y4 - y2) * (z3 - z2) - (y3 - y2) * (z4 - z2); J[0][2] = (x3 - x2) * (z4 - z2) - (x4 - x2) * (z3 - z2); J[0][3] = (x4 - x2) * (y3 - y2) - (x3 - x2) * (y4 - y2);
+    J[1][1] = (y3 - y1) * (z4 - z3) - (y3 - y4) * (z1 - z3); J[1][2] = (x4 - x3) * (z3 - z1) - (x1 - x3) * (z3 - z4); J[1][3] = (x3 - x1) * (y4 - y3) - (x3 - x4) * (y1 - y3);
+    J[2][1] = (y2 - y4) * (z1 - z4) - (y1 - y4) * (z2 - z4); J[2][2] = (x1 - x4) * (z2 - z4) - (x2 - x4) * (z1 - z4); J[2][3] = (x2 - x4) * (y1 - y4) - (x1 - x4) * (y2 - y4);
+    J[3][1] = (y1 - y3) * (z2 - z1) - (y1 - y2) * (z3 - z1); J[3][2] = (x2 - x1) * (z1 - z3) - (x3 - x1) * (z1 - z2); J[3][3] = (x1 - x3) * (y2 - y1) - (x1 - x2) * (y3 - y1);
+    float s = 1.0f / (6.0f * volume);
+    for (int i = 0; i < 4; i++) for (int j = 0; j < 4; j++) J[i][j] = J[i][j] * s;
+    float a[4], b[4], c[4];
+    for (int n = 0; n < 4; n++) { a[n] = J[n][1]; b[n] = J[n][2]; c[n] = J[n][3]; }
+    float B[6][12];
+    std::memset(B, 0, sizeof(B));
+    for (int n = 0; n < 4; n++) {  // AFEM-16 p.13 (TetrahedronElement3.h:132-137)
+        B[0][3 * n + 0] = a[n];
+        B[1][3 * n + 1] = b[n];
+        B[2][3 * n + 2] = c[n];
+        B[3][3 * n + 0] = b[n]; B[3][3 * n + 1] = a[n];
+        B[4][3 * n + 1] = c[n]; B[4][3 * n + 2] = b[n];
+        B[5][3 * n + 0] = c[n]; B[5][3 * n + 2] = a[n];
+    }
+    // E (TetrahedronElement3.h:141-156)
+    float Em[6][6];
+    std::memset(Em, 0, sizeof(Em));
+    float aa = E / ((1.0f + nu) * (1.0f - 2.0f * nu));
+    float a_nu = aa * nu, a_k1 = aa * (1.0f - nu), a_k2 = aa * (0.5f - nu);
+    Em[0][0] = a_k1; Em[0][1] = a_nu; Em[0][2] = a_nu;
+    Em[1][0] = a_nu; Em[1][1] = a_k1; Em[1][2] = a_nu;
+    Em[2][0] = a_nu; Em[2][1] = a_nu; Em[2][2] = a_k1;
+    Em[3][3] = a_k2; Em[4][4] = a_k2; Em[5][5] = a_k2;
+    float EB[6][12];
+    for (int i = 0; i < 6; i++)
+        for (int j = 0; j < 12; j++) {
+            float acc = 0.0f;
+            for (int k = 0; k < 6; k++) acc += Em[i][k] * B[k][j];
+            EB[i][j] = acc;
+        }
+    for (int i = 0; i < 12; i++)
+        for (int j = 0; j < 12; j++) {
+            float acc = 0.0f;
+            for (int k = 0; k < 6; k++) acc += B[k][i] * EB[k][j];
+            K[i * 12 + j] = acc * volume;
+        }
+    return volume;
+}
+
+// 2D (TriangleElement2.h:71-139)
+float compute_K2(const float r0[2], const float r1[2], const float r2[2], float E, float nu, float K[36]) {
+    float area = 0.5f * ((r1[0] - r0[0]) * (r2[1] - r0[1]) - (r2[0] - r0[0]) * (r1[1] - r0[1]));
+    float rcp = 1.0f / (2.0f * area);
+    float x02 = r0[0] - r2[0]; x02 *= rcp;
+    float x10 = r1[0] - r0[0]; x10 *= rcp;
+    float x21 = r2[0] - r1[0]; x21 *= rcp;
+    float y01 = r0[1] - r1[1]; y01 *= rcp;
+    float y12 = r1[1] - r2[1]; y12 *= rcp;
+    float y20 = r2[1] - r0[1]; y20 *= rcp;
+    float B[3][6] = {{y12, 0, y20, 0, y01, 0}, {0, x21, 0, x02, 0, x10}, {x21, y12, x02, y20, x10, y01}};
+    float a = E / ((1 + nu) * (1 - 2 * nu));
+    float anu = a * nu;
+    float Em[3][3] = {{a - anu, anu, 0}, {anu, a - anu, 0}, {0, 0, 0.5f * a - anu}};
+    float EB[3][6];
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 6; j++) { float acc = 0.0f; for (int k = 0; k < 3; k++) acc += Em[i][k] * B[k][j]; EB[i][j] = acc; }
+    for (int i = 0; i < 6; i++)
+        for (int j = 0; j < 6; j++) { float acc = 0.0f; for (int k = 0; k < 3; k++) acc += B[k][i] * EB[k][j]; K[i * 6 + j] = acc * area; }
+    return area;
+}
+
+// ---- SSE real_array reductions (ns/real_array.h:34-49): each 4-lane chunk
+// is reduced by _mm_dp_ps as (p0+p1)+(p2+p3) and added serially to lane 0 of
+// the accumulator; lanes past n are taken as zero.
+inline float dp4(const float* a, const float* b, uint32_t i, uint32_t n) {
+    float p[4];
+    for (int l = 0; l < 4; l++) p[l] = (i + l < n) ? a[i + l] * b[i + l] : 0.0f;
+    return (p[0] + p[1]) + (p[2] + p[3]);
+}
+float sse_dot(const float* a, const float* b, uint32_t n) {
+    float acc = 0.0f;
+    uint32_t an = 4 * ((n + 3) / 4);
+    for (uint32_t i = 0; i < an; i += 4) acc = acc + dp4(a, b, i, n);
+    return acc;
+}
+
+// Same products accumulated in FP64 in index order, rounded once to FP32: the
+// reduction precision the GPU path uses (not the reference's; used to measure
+// how much of a GPU/oracle difference the reduction precision alone explains).
+float fp64_dot(const float* a, const float* b, uint32_t n) {
+    double acc = 0.0;
+    for (uint32_t i = 0; i < n; i++) acc += double(a[i]) * double(b[i]);
+    return float(acc);
+}
+
+}  // namespace
+
+// ================================================================ C API
+extern "C" {
+
+void orc_box_counts(uint32_t nx, uint32_t ny, uint32_t nz, uint32_t* n_nodes, uint32_t* n_tets) {
+    *n_nodes = nx * ny * nz;
+    *n_tets = 5u * (nx - 1) * (ny - 1) * (nz - 1);
+}
+
+// Geo/shape/TetSolidShape3.cpp:921-999 (counts are vertices per axis)
+void orc_mesh_box(float hx, float hy, float hz, uint32_t nx, uint32_t ny, uint32_t nz, float* pos, uint32_t* tets) {
+    float h[3] = {hx, hy, hz};
+    float sz[3] = {2 * hx, 2 * hy, 2 * hz};
+    uint32_t n[3] = {nx, ny, nz};
+    uint32_t v = 0;
+    for (uint32_t i = 0; i < nx; i++)
+        for (uint32_t j = 0; j < ny; j++)
+            for (uint32_t k = 0; k < nz; k++) {
+                uint32_t ijk[3] = {i, j, k};
+                for (int c = 0; c < 3; c++) pos[3 * v + c] = (-h[c]) + (sz[c] * float(ijk[c])) / float(n[c] - 1);
+                v++;
+            }
+    uint32_t t = 0;
+    auto add = [&](uint32_t a, uint32_t b, uint32_t c, uint32_t d) {
+        tets[4 * t + 0] = a; tets[4 * t + 1] = b; tets[4 * t + 2] = c; tets[4 * t + 3] = d; t++;
+    };
+    for (uint32_t i = 0; i + 1 < nx; i++)
+        for (uint32_t j = 0; j + 1 < ny; j++)
+            for (uint32_t k = 0; k + 1 < nz; k++) {
+                uint32_t v0 = i * ny * nz + j * nz + k;
+                uint32_t v1 = i * ny * nz + (j + 1) * nz + k;
+                uint32_t v2 = (i + 1) * ny * nz + (j + 1) * nz + k;
+                uint32_t v3 = (i + 1) * ny * nz + j * nz + k;
+                uint32_t v4 = i * ny * nz + j * nz + (k + 1);
+                uint32_t v5 = i * ny * nz + (j + 1) * nz + (k + 1);
+                uint32_t v6 = (i + 1) * ny * nz + (j + 1) * nz + (k + 1);
+                uint32_t v7 = (i + 1) * ny * nz + j * nz + (k + 1);
+                int mirrors = 0;
+                if (i % 2 == 1) { mirrors++; std::swap(v0, v3); std::swap(v1, v2); std::swap(v5, v6); std::swap(v4, v7); }
+                if (j % 2 == 1) { mirrors++; std::swap(v0, v1); std::swap(v3, v2); std::swap(v4, v5); std::swap(v7, v6); }
+                if (k % 2 == 1) { mirrors++; std::swap(v0, v4); std::swap(v1, v5); std::swap(v2, v6); std::swap(v3, v7); }
+                if (mirrors % 2 == 0) {
+                    add(v0, v1, v4, v3); add(v2, v1, v3, v6); add(v5, v1, v6, v4); add(v6, v1, v3, v4); add(v7, v3, v4, v6);
+                } else {
+                    add(v0, v1, v3, v4); add(v2, v1, v6, v3); add(v5, v1, v4, v6); add(v6, v1, v4, v3); add(v7, v3, v6, v4);
+                }
+            }
+}
+
+void orc_box2_counts(uint32_t nx, uint32_t ny, uint32_t* n_nodes, uint32_t* n_tris) {
+    *n_nodes = nx * ny;
+    *n_tris = 2u * (nx - 1) * (ny - 1);
+}
+
+// Geo/shape/MeshSolidShape2.cpp:1651-1675
+void orc_mesh_box2(float hx, float hy, uint32_t nx, uint32_t ny, float* pos, uint32_t* tris) {
+    float step_x = 2 * hx / float(nx - 1);
+    float step_y = 2 * hy / float(ny - 1);
+    uint32_t v = 0;
+    for (uint32_t y = 0; y < ny; y++)
+        for (uint32_t x = 0; x < nx; x++) {
+            pos[2 * v + 0] = (-hx) + float(x) * step_x;
+            pos[2 * v + 1] = (-hy) + float(y) * step_y;
+            v++;
+        }
+    uint32_t t = 0;
+    for (uint32_t y = 0; y + 1 < ny; y++)
+        for (uint32_t x = 0; x + 1 < nx; x++) {
+            uint32_t v0 = y * nx + x, v1 = y * nx + x + 1, v2 = (y + 1) * nx + x + 1, v3 = (y + 1) * nx + x;
+            tris[3 * t + 0] = v0; tris[3 * t + 1] = v1; tris[3 * t + 2] = v2; t++;
+            tris[3 * t + 0] = v2; tris[3 * t + 1] = v3; tris[3 * t + 2] = v0; t++;
+        }
+}
+
+float orc_det3(const float m[9]) { return det3(mload(m)); }
+void orc_inverse3(const float m[9], float out[9]) { mstore(inv3(mload(m)), out); }
+void orc_pd3(const float F[9], float detF, float R[9], uint32_t* iters) {
+    (void)detF;
+    mstore(polar_newton(mload(F), iters), R);
+}
+void orc_svd3(const float F[9], float U[9], float s[3], float Vt[9]) {
+    M3 u, vt; V3 sv;
+    fast_svd3(mload(F), u, sv, vt);
+    mstore(u, U); mstore(vt, Vt);
+    s[0] = sv.v[0]; s[1] = sv.v[1]; s[2] = sv.v[2];
+}
+// The same SVD with the true sqrt(3): a textbook Smith/cofactor SVD, used only
+// to pin the rest of the restated algorithm against an independent SVD.
+void orc_svd3_sqrt3(const float F[9], float U[9], float s[3], float Vt[9]) {
+    M3 u, vt; V3 sv;
+    fast_svd3(mload(F), u, sv, vt, std::sqrt(3.0f));
+    mstore(u, U); mstore(vt, Vt);
+    s[0] = sv.v[0]; s[1] = sv.v[1]; s[2] = sv.v[2];
+}
+// ms/fem/Rotation.cpp:39-57: R = U * V^T of the PhysBAM SVD
+void orc_svd_rotation3(const float F[9], float R[9]) {
+    M3 u, vt; V3 sv;
+    fast_svd3(mload(F), u, sv, vt);
+    mstore(mmul(u, vt), R);
+}
+// Mal/GMatDecomposition.h:17-30
+void orc_pd2(const float F[4], float detF, float R[4]) {
+    float sg = (detF > 0) ? 1.0f : ((detF < 0) ? -1.0f : 0.0f);
+    float r00 = F[0] + sg * F[3], r01 = F[1] + sg * (-F[2]);
+    float r10 = F[2] + sg * (-F[1]), r11 = F[3] + sg * F[0];
+    float i0 = 1.0f / std::sqrt(r00 * r00 + r10 * r10);
+    r00 *= i0; r10 *= i0;
+    float i1 = 1.0f / std::sqrt(r01 * r01 + r11 * r11);
+    r01 *= i1; r11 *= i1;
+    R[0] = r00; R[1] = r01; R[2] = r10; R[3] = r11;
+}
+float orc_compute_K(const float r[12], float E, float nu, float K[144]) {
+    return compute_K(vload(r), vload(r + 3), vload(r + 6), vload(r + 9), E, nu, K);
+}
+float orc_compute_K2(const float r[6], float E, float nu, float K[36]) {
+    return compute_K2(r, r + 2, r + 4, E, nu, K);
+}
+float orc_sse_dot(const float* a, const float* b, uint32_t n) { return sse_dot(a, b, n); }
+
+}  // extern "C"
+
+// ================================================================ element set
+struct orc_tetset {
+    uint32_t N = 0, T = 0;
+    std::vector<float> ref;          // 3N
+    std::vector<uint32_t> tet;       // 4T
+    std::vector<float> Kb;           // T * 16 blocks * 9, block (i,j) at (4*i+j)*9
+    std::vector<float> Dinv;         // 9T
+    std::vector<float> vol;          // T
+    float total_volume = 0.0f;
+    std::vector<float> R;            // 9T cache
+    std::vector<float> detF;         // T
+};
+
+extern "C" {
+
+// EditableTetrahedronElementSet3::BeginEdition/AddElement/EndEdition
+// (ms/fem/TetrahedronElementSet3.cpp:2084-2166)
+orc_tetset* orc_tetset_create(uint32_t n_nodes, const float* ref_pos, uint32_t n_tets, const uint32_t* tets,
+                              float young, float poisson) {
+    orc_tetset* s = new orc_tetset;
+    s->N = n_nodes; s->T = n_tets;
+    s->ref.assign(ref_pos, ref_pos + 3 * size_t(n_nodes));
+    s->tet.assign(tets, tets + 4 * size_t(n_tets));
+    s->Kb.resize(size_t(n_tets) * 144);
+    s->Dinv.resize(size_t(n_tets) * 9);
+    s->vol.resize(n_tets);
+    s->R.resize(size_t(n_tets) * 9);
+    s->detF.resize(n_tets);
+    float acc = 0.0f;
+    float K[144];
+    for (uint32_t e = 0; e < n_tets; e++) {
+        V3 r0 = vload(&s->ref[3 * size_t(tets[4 * e + 0])]);
+        V3 r1 = vload(&s->ref[3 * size_t(tets[4 * e + 1])]);
+        V3 r2 = vload(&s->ref[3 * size_t(tets[4 * e + 2])]);
+        V3 r3 = vload(&s->ref[3 * size_t(tets[4 * e + 3])]);
+        M3 Dm = compute_D(r0, r1, r2, r3);
+        float v = det3(Dm) / 6.0f;
+        s->vol[e] = v;
+        acc += v;
+        compute_K(r0, r1, r2, r3, young, poisson, K);
+        float* kb = &s->Kb[size_t(e) * 144];
+        for (int bi = 0; bi < 4; bi++)
+            for (int bj = 0; bj < 4; bj++)
+                for (int r = 0; r < 3; r++)
+                    for (int c = 0; c < 3; c++) kb[(4 * bi + bj) * 9 + 3 * r + c] = K[(3 * bi + r) * 12 + 3 * bj + c];
+        mstore(inv3(Dm), &s->Dinv[size_t(e) * 9]);
+        mstore(mid(), &s->R[size_t(e) * 9]);  // CreateCache: R = Id (:190-209)
+        s->detF[e] = 1.0f;
+    }
+    s->total_volume = acc;
+    return s;
+}
+
+void orc_tetset_destroy(orc_tetset* s) { delete s; }
+float orc_tetset_total_volume(const orc_tetset* s) { return s->total_volume; }
+
+void orc_tetset_element(const orc_tetset* s, uint32_t e, float K144[144], float Dinv[9], float* vol) {
+    if (K144) {
+        const float* kb = &s->Kb[size_t(e) * 144];
+        for (int bi = 0; bi < 4; bi++)
+            for (int bj = 0; bj < 4; bj++)
+                for (int r = 0; r < 3; r++)
+                    for (int c = 0; c < 3; c++) K144[(3 * bi + r) * 12 + 3 * bj + c] = kb[(4 * bi + bj) * 9 + 3 * r + c];
+    }
+    if (Dinv) std::memcpy(Dinv, &s->Dinv[size_t(e) * 9], 9 * sizeof(float));
+    if (vol) *vol = s->vol[e];
+}
+
+// BeginEvaluation (ms/fem/TetrahedronElementSet3.cpp:219-281) + ComputeRotations_*
+// (:748-892). The DAPD degenerate branch (det F <= threshold) is out of scope;
+// it falls back to R = Id like the reference's invalid-DoC path (:885-890).
+void orc_begin_evaluation(orc_tetset* s, const float* x, int rm, float thr, float* R_out, float* detF_out,
+                          uint32_t* num_degenerate, float* sum_degenerate_detF) {
+    uint32_t ndeg = 0;
+    float sdeg = 0.0f;
+    for (uint32_t e = 0; e < s->T; e++) {
+        const uint32_t* t = &s->tet[4 * size_t(e)];
+        V3 x0 = vload(&x[3 * size_t(t[0])]), x1 = vload(&x[3 * size_t(t[1])]);
+        V3 x2 = vload(&x[3 * size_t(t[2])]), x3 = vload(&x[3 * size_t(t[3])]);
+        M3 Ds = compute_D(x0, x1, x2, x3);
+        M3 F = mmul(Ds, mload(&s->Dinv[9 * size_t(e)]));
+        float dF = det3(F);
+        s->detF[e] = dF;
+        ndeg += (dF <= 0) ? 1u : 0u;
+        sdeg += (dF <= 0) ? dF : 0.0f;
+        M3 R;
+        switch (rm) {
+            case ORC_RM_MSVD: { M3 u, vt; V3 sv; fast_svd3(F, u, sv, vt); R = mmul(u, vt); } break;
+            case ORC_RM_DAPD:
+                if (dF > thr) R = polar_newton(F, nullptr);
+                else R = mid();
+                break;
+            case ORC_RM_ID:
+            default: R = mid(); break;
+        }
+        mstore(R, &s->R[9 * size_t(e)]);
+    }
+    if (R_out) std::memcpy(R_out, s->R.data(), s->R.size() * sizeof(float));
+    if (detF_out) std::memcpy(detF_out, s->detF.data(), s->detF.size() * sizeof(float));
+    if (num_degenerate) *num_degenerate = ndeg;
+    if (sum_degenerate_detF) *sum_degenerate_detF = sdeg;
+}
+
+void orc_set_rotations(orc_tetset* s, const float* R) { std::memcpy(s->R.data(), R, s->R.size() * sizeof(float)); }
+
+namespace {
+// Shared body of C_f_e / C_df_x: v_i -= R * sum_j K_ij * l_j, with l_j the
+// local-frame vectors, accumulated per element in element order.
+inline void corot_apply(const orc_tetset* s, uint32_t e, const V3 l[4], float* out) {
+    const uint32_t* t = &s->tet[4 * size_t(e)];
+    const float* kb = &s->Kb[size_t(e) * 144];
+    M3 R = mload(&s->R[9 * size_t(e)]);
+    for (int i = 0; i < 4; i++) {
+        V3 acc = mvec(mload(kb + (4 * i + 0) * 9), l[0]);
+        acc = vadd(acc, mvec(mload(kb + (4 * i + 1) * 9), l[1]));
+        acc = vadd(acc, mvec(mload(kb + (4 * i + 2) * 9), l[2]));
+        acc = vadd(acc, mvec(mload(kb + (4 * i + 3) * 9), l[3]));
+        V3 w = mvec(R, acc);
+        float* o = &out[3 * size_t(t[i])];
+        o[0] -= w.v[0]; o[1] -= w.v[1]; o[2] -= w.v[2];
+    }
+}
+}  // namespace
+
+// C_f_e (ms/fem/TetrahedronElementSet3.cpp:1053-1096): u_j = R^T x_j - r_j
+void orc_f_e(const orc_tetset* s, const float* x, float* f) {
+    for (uint32_t e = 0; e < s->T; e++) {
+        const uint32_t* t = &s->tet[4 * size_t(e)];
+        M3 Rt = mtr(mload(&s->R[9 * size_t(e)]));
+        V3 u[4];
+        for (int j = 0; j < 4; j++) u[j] = vsub(mvec(Rt, vload(&x[3 * size_t(t[j])])), vload(&s->ref[3 * size_t(t[j])]));
+        corot_apply(s, e, u, f);
+    }
+}
+
+// C_df_x with DM = Truncated (:1761-1836; the dR terms are skipped, :901)
+void orc_df_x(const orc_tetset* s, const float* dx, float* df) {
+    for (uint32_t e = 0; e < s->T; e++) {
+        const uint32_t* t = &s->tet[4 * size_t(e)];
+        M3 Rt = mtr(mload(&s->R[9 * size_t(e)]));
+        V3 l[4];
+        for (int j = 0; j < 4; j++) l[j] = mvec(Rt, vload(&dx[3 * size_t(t[j])]));
+        corot_apply(s, e, l, df);
+    }
+}
+
+// C_V (:1456-1488)
+float orc_V(const orc_tetset* s, const float* x) {
+    float acc2 = 0.0f;
+    for (uint32_t e = 0; e < s->T; e++) {
+        const uint32_t* t = &s->tet[4 * size_t(e)];
+        const float* kb = &s->Kb[size_t(e) * 144];
+        M3 Rt = mtr(mload(&s->R[9 * size_t(e)]));
+        V3 u[4];
+        for (int j = 0; j < 4; j++) u[j] = vsub(mvec(Rt, vload(&x[3 * size_t(t[j])])), vload(&s->ref[3 * size_t(t[j])]));
+        for (int i = 0; i < 4; i++) {
+            V3 acc = mvec(mload(kb + (4 * i + 0) * 9), u[0]);
+            acc = vadd(acc, mvec(mload(kb + (4 * i + 1) * 9), u[1]));
+            acc = vadd(acc, mvec(mload(kb + (4 * i + 2) * 9), u[2]));
+            acc = vadd(acc, mvec(mload(kb + (4 * i + 3) * 9), u[3]));
+            acc2 += vdot(u[i], acc);
+        }
+    }
+    return 0.5f * acc2;
+}
+
+}  // extern "C"
+
+// ================================================================ CG + leaf
+namespace {
+
+// Rayleigh coefficients c = inv(Omega) * 2*zeta (LeafDSH_Solid3D_FEM.h:347-354)
+void rayleigh(float f1, float f2, float zeta, float c[2]) {
+    float o00 = 1.0f / f1, o01 = f1, o10 = 1.0f / f2, o11 = f2;
+    float det = o00 * o11 - o10 * o01;
+    float id = 1.0f / det;
+    float i00 = id * o11, i01 = -id * o01, i10 = -id * o10, i11 = id * o00;
+    float z0 = 2 * zeta, z1 = 2 * zeta;
+    float acc = 0.0f; acc += i00 * z0; acc += i01 * z1; c[0] = acc;
+    acc = 0.0f; acc += i10 * z0; acc += i11 * z1; c[1] = acc;
+}
+
+// Unpreconditioned modified CG with projection S (ns/ILSS_CG.h:38-151).
+template <class AV, class SP>
+void cg_solve(uint32_t n, AV&& Av, SP&& S, float* x, const float* b, float rel, float abs_eps, uint32_t max_it,
+              std::vector<float>& r, std::vector<float>& d, std::vector<float>& q,
+              int32_t* iters, float* rel_res, float* abs_res, bool red64) {
+    auto sse_dot = [red64](const float* a, const float* c, uint32_t m) { return red64 ? fp64_dot(a, c, m) : ::sse_dot(a, c, m); };
+    float abs_sq = abs_eps * abs_eps;
+    Av(r.data(), x);
+    for (uint32_t i = 0; i < n; i++) r[i] = -1.0f * r[i] + b[i];  // muleq_addeq (real_array.h:78)
+    S(r.data());
+    std::memcpy(d.data(), r.data(), n * sizeof(float));
+    float delta0 = sse_dot(r.data(), r.data(), n);
+    float dnew = delta0, dold = delta0;
+    uint32_t k = 0;
+    float target = (rel * rel) * delta0;
+    while (dnew > abs_sq && dnew > target && k < max_it) {
+        Av(q.data(), d.data());
+        S(q.data());
+        float dq = sse_dot(d.data(), q.data(), n);
+        if (std::fabs(dq) < abs_sq) break;
+        float alpha = dnew / dq;
+        for (uint32_t i = 0; i < n; i++) x[i] = x[i] + d[i] * alpha;   // addeq_scaled (:66-67)
+        for (uint32_t i = 0; i < n; i++) r[i] = r[i] - q[i] * alpha;   // subeq_scaled (:69-70)
+        dold = dnew;
+        dnew = sse_dot(r.data(), r.data(), n);
+        float beta = dnew / dold;
+        for (uint32_t i = 0; i < n; i++) d[i] = beta * d[i] + r[i];    // muleq_addeq (:78)
+        S(d.data());
+        k++;
+    }
+    *iters = int32_t(k);
+    *rel_res = (delta0 > 0) ? std::sqrt(dnew / delta0) : 0.0f;
+    *abs_res = std::sqrt(dnew);
+}
+
+}  // namespace
+
+struct orc_solid {
+    orc_solid_params p;
+    orc_tetset* tes = nullptr;
+    uint32_t N = 0;
+    std::vector<float> x, v, x0;   // positions, velocities, previous positions (m_vecPos0)
+    float inv_mass = 0.0f;         // uniform
+    float c[2] = {0, 0};           // Rayleigh
+    std::vector<uint32_t> knc;
+    std::vector<float> b, y, r, d, q;
+    float acc_time = 0.0f;
+    float total_time = 0.0f;
+};
+
+extern "C" {
+
+// LeafDSH_Solid3D_FEM::Create (ds/dsh/LeafDSH_Solid3D_FEM.cpp:125-223) + RebuildMass (:1591-1611)
+orc_solid* orc_solid_create(uint32_t n_nodes, const float* ref_pos, uint32_t n_tets, const uint32_t* tets,
+                            const orc_solid_params* p) {
+    orc_solid* s = new orc_solid;
+    s->p = *p;
+    s->N = n_nodes;
+    s->tes = orc_tetset_create(n_nodes, ref_pos, n_tets, tets, p->young_modulus, p->poisson_ratio);
+    s->x.assign(ref_pos, ref_pos + 3 * size_t(n_nodes));
+    s->x0 = s->x;
+    s->v.assign(3 * size_t(n_nodes), 0.0f);
+    float total_mass = p->density * s->tes->total_volume;
+    float mass_per_node = total_mass / float(n_nodes);
+    s->inv_mass = 1.0f / mass_per_node;  // GParticleSystem::SetMass (ms/ParticleSystem.h:80)
+    rayleigh(p->rayleigh_freq1, p->rayleigh_freq2, p->damping_ratio, s->c);
+    size_t n3 = 3 * size_t(n_nodes);
+    s->b.resize(n3); s->y.resize(n3); s->r.resize(n3); s->d.resize(n3); s->q.resize(n3);
+    return s;
+}
+
+void orc_solid_destroy(orc_solid* s) { orc_tetset_destroy(s->tes); delete s; }
+void orc_solid_set_knc(orc_solid* s, uint32_t n, const uint32_t* nids) { s->knc.assign(nids, nids + n); }
+float orc_solid_node_mass(const orc_solid* s) { return 1.0f / s->inv_mass; }
+void orc_solid_rayleigh(const orc_solid* s, float c[2]) { c[0] = s->c[0]; c[1] = s->c[1]; }
+
+// FixedStep (:898-990) with Step_QuasiImplicitEuler_v (:1205-1328)
+void orc_solid_fixed_step(orc_solid* s, float dt, orc_step_stats* st) {
+    const uint32_t N = s->N;
+    const size_t n3 = 3 * size_t(N);
+    const float mass = 1.0f / s->inv_mass;  // GetMass (ms/ParticleSystem.h:74)
+    // UpdateKNC, static mode: KNC target = rest position (:877-896)
+    uint32_t ndeg = 0; float sdeg = 0.0f;
+    orc_begin_evaluation(s->tes, s->x.data(), s->p.rm, s->p.degenerate_threshold, nullptr, nullptr, &ndeg, &sdeg);
+    float min_det = 1e30f;
+    for (uint32_t e = 0; e < s->tes->T; e++) min_det = std::min(min_det, s->tes->detF[e]);
+    s->x0 = s->x;
+    float inv_dt = 1.0f / dt;
+    // b = M g
+    for (uint32_t i = 0; i < N; i++)
+        for (int c = 0; c < 3; c++) s->b[3 * i + c] = 0.0f + mass * s->p.gravity[c];
+    orc_f_e(s->tes, s->x.data(), s->b.data());
+    for (size_t i = 0; i < n3; i++) s->b[i] *= dt;
+    for (uint32_t i = 0; i < N; i++)
+        for (int c = 0; c < 3; c++) s->b[3 * i + c] += mass * s->v[3 * i + c];
+    if (s->p.warm_start) s->y = s->v; else std::fill(s->y.begin(), s->y.end(), 0.0f);
+    // KNC (:1268-1295): position constraints at the rest pose
+    for (uint32_t nid : s->knc) {
+        V3 vk;
+        for (int c = 0; c < 3; c++) vk.v[c] = (s->tes->ref[3 * size_t(nid) + c] - s->x[3 * size_t(nid) + c]) * inv_dt;
+        float nk = vnorm(vk);
+        V3 kv;
+        if (nk > 0.001f) kv = vscale(vnormalized(vk), std::min(nk, 50.0f));
+        else { kv.v[0] = kv.v[1] = kv.v[2] = 0.0f; }
+        for (int c = 0; c < 3; c++) { s->b[3 * size_t(nid) + c] = kv.v[c]; s->y[3 * size_t(nid) + c] = kv.v[c]; }
+    }
+    const float fM = 1.0f + dt * s->c[0];
+    const float fK = dt * dt + dt * s->c[1];
+    auto Av = [&](float* q, const float* d) {  // LS_A_times_d (:1056-1079)
+        std::memset(q, 0, n3 * sizeof(float));
+        orc_df_x(s->tes, d, q);
+        for (size_t i = 0; i < n3; i++) q[i] *= -fK;
+        for (uint32_t i = 0; i < N; i++) {
+            float factor = mass * fM;
+            q[3 * i] += factor * d[3 * i];
+            q[3 * i + 1] += factor * d[3 * i + 1];
+            q[3 * i + 2] += factor * d[3 * i + 2];
+        }
+    };
+    auto S = [&](float* w) {  // LS_ZeroKNC (:1045-1053)
+        for (uint32_t nid : s->knc) { w[3 * size_t(nid)] = 0; w[3 * size_t(nid) + 1] = 0; w[3 * size_t(nid) + 2] = 0; }
+    };
+    int32_t it = 0; float rr = 0, ar = 0;
+    cg_solve(uint32_t(n3), Av, S, s->y.data(), s->b.data(), s->p.cg_rel_epsilon, s->p.cg_abs_epsilon, s->p.cg_max_iter,
+             s->r, s->d, s->q, &it, &rr, &ar, s->p.reduce_fp64 != 0);
+    for (uint32_t i = 0; i < N; i++)
+        for (int c = 0; c < 3; c++) {
+            s->v[3 * i + c] = s->y[3 * i + c];
+            s->x[3 * i + c] = s->x[3 * i + c] + dt * s->y[3 * i + c];
+        }
+    float V = orc_V(s->tes, s->x.data());
+    if (st) {
+        st->num_fixed_steps = 1;
+        st->cg_iters = it; st->cg_rel = rr; st->cg_abs = ar;
+        st->num_degenerate = ndeg; st->sum_degenerate_detF = sdeg;
+        st->elastic_energy = V; st->min_detF = min_det;
+    }
+    s->total_time += dt;
+}
+
+// Step (:225-256): fixed-dt accumulator with strict '>'
+void orc_solid_step(orc_solid* s, float dt, orc_step_stats* st) {
+    s->acc_time += dt;
+    orc_step_stats last;
+    std::memset(&last, 0, sizeof(last));
+    uint32_t n = 0;
+    while (s->acc_time > s->p.fixed_dt) {
+        orc_solid_fixed_step(s, s->p.fixed_dt, &last);
+        s->acc_time -= s->p.fixed_dt;
+        n++;
+    }
+    last.num_fixed_steps = n;
+    if (st) *st = last;
+}
+
+void orc_solid_get_state(const orc_solid* s, float* x, float* v) {
+    if (x) std::memcpy(x, s->x.data(), s->x.size() * sizeof(float));
+    if (v) std::memcpy(v, s->v.data(), s->v.size() * sizeof(float));
+}
+void orc_solid_set_state(orc_solid* s, const float* x, const float* v) {
+    if (x) std::memcpy(s->x.data(), x, s->x.size() * sizeof(float));
+    if (v) std::memcpy(s->v.data(), v, s->v.size() * sizeof(float));
+}
+
+}  // extern "C"

@@ -1,0 +1,116 @@
+/*
+ * corofem oracle: CPU restatement of the reference's corotational tet FEM
+ * implicit step, used ONLY as a checker (tests, smoke, the CPU baseline leg of
+ * bench.py). Nothing in the shipped library links or calls it.
+ *
+ * It follows the reference's FP32 semantics operation by operation:
+ *   - box mesher        Geo/shape/TetSolidShape3.cpp:921-999
+ *   - element precompute ms/fem/TetrahedronElement3.h:85-173,
+ *                        ms/fem/TetrahedronElementSet3.cpp:2113-2166
+ *   - BeginEvaluation   ms/fem/TetrahedronElementSet3.cpp:219-281
+ *   - rotations         Mal/GMatDecomposition.h:411-432 (Newton polar),
+ *                        ms/fem/Rotation.cpp:39-57 + Mal/GMatDecomposition_SVD3x3_PhysBAM.h:246-280
+ *   - C_f_e / C_df_x / C_V  ms/fem/TetrahedronElementSet3.cpp:1053-1096,1761-1836,1456-1488
+ *     (stored 3x3 Ke blocks, serial scatter in element order)
+ *   - CG                ns/ILSS_CG.h:38-151 with ns/real_array.h:34-78 (SSE
+ *                        4-lane FP32 reduction order, zero padding lanes)
+ *   - QIE-v leaf step   ds/dsh/LeafDSH_Solid3D_FEM.cpp:225-256,898-990,1045-1079,1205-1328,1591-1611
+ *   - 2D twin           ms/fem/TriangleElement2.h:71-139, TriangleElementSet2.cpp,
+ *                        ds/dsh/LeafDSH_Solid2D_FEM.cpp:1101-1132,1287-1403,1842-1863
+ * Build with -ffp-contract=off (no FMA) to keep the reference's rounding.
+ */
+#ifndef COROFEM_ORACLE_H
+#define COROFEM_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* rotation methods (ms/fem/Params.h:63-67) */
+#define ORC_RM_ID 0
+#define ORC_RM_QR 1
+#define ORC_RM_MSVD 2
+#define ORC_RM_DAPD 3
+
+/* ---- mesh (Geo/shape/TetSolidShape3.cpp:921-999) ---- */
+void orc_box_counts(uint32_t nx, uint32_t ny, uint32_t nz, uint32_t* n_nodes, uint32_t* n_tets);
+void orc_mesh_box(float hx, float hy, float hz, uint32_t nx, uint32_t ny, uint32_t nz,
+                  float* pos, uint32_t* tets);
+/* 2D grid (Geo/shape/MeshSolidShape2.cpp:1651-1675) */
+void orc_box2_counts(uint32_t nx, uint32_t ny, uint32_t* n_nodes, uint32_t* n_tris);
+void orc_mesh_box2(float hx, float hy, uint32_t nx, uint32_t ny, float* pos, uint32_t* tris);
+
+/* ---- math kernels restated ---- */
+float orc_det3(const float m[9]);
+void orc_inverse3(const float m[9], float out[9]);
+void orc_pd3(const float F[9], float detF, float R[9], uint32_t* iters);
+void orc_svd3(const float F[9], float U[9], float s[3], float Vt[9]);
+void orc_svd_rotation3(const float F[9], float R[9]);
+void orc_svd3_sqrt3(const float F[9], float U[9], float s[3], float Vt[9]);
+void orc_pd2(const float F[4], float detF, float R[4]);
+float orc_compute_K(const float r[12], float E, float nu, float K[144]);
+float orc_compute_K2(const float r[6], float E, float nu, float K[36]);
+float orc_sse_dot(const float* a, const float* b, uint32_t n);
+
+/* ---- 3D element set + cache ---- */
+typedef struct orc_tetset orc_tetset;
+orc_tetset* orc_tetset_create(uint32_t n_nodes, const float* ref_pos, uint32_t n_tets,
+                              const uint32_t* tets, float young, float poisson);
+void orc_tetset_destroy(orc_tetset* s);
+float orc_tetset_total_volume(const orc_tetset* s);
+void orc_tetset_element(const orc_tetset* s, uint32_t e, float K144[144], float Dinv[9], float* vol);
+/* BeginEvaluation: computes F, det F, R per element into the set's cache.
+   R_out (9*T, row-major per element) and detF_out (T) may be NULL. */
+void orc_begin_evaluation(orc_tetset* s, const float* x, int rm, float degenerate_threshold,
+                          float* R_out, float* detF_out, uint32_t* num_degenerate,
+                          float* sum_degenerate_detF);
+void orc_set_rotations(orc_tetset* s, const float* R);  /* load R (9*T) directly */
+void orc_f_e(const orc_tetset* s, const float* x, float* f);      /* f  += f_e(x)  */
+void orc_df_x(const orc_tetset* s, const float* dx, float* df);   /* df += K'(dx) */
+float orc_V(const orc_tetset* s, const float* x);
+
+/* ---- 3D leaf solid (QIE-v) ---- */
+typedef struct orc_solid_params {
+    float fixed_dt;
+    float density;
+    float young_modulus;
+    float poisson_ratio;
+    float damping_ratio;
+    float rayleigh_freq1, rayleigh_freq2;
+    float gravity[3];
+    float cg_rel_epsilon, cg_abs_epsilon;
+    uint32_t cg_max_iter;
+    int rm;
+    float degenerate_threshold;
+    int warm_start;
+    int reduce_fp64;   /* 0: reference FP32 SSE reductions; 1: FP64 reductions */
+} orc_solid_params;
+
+typedef struct orc_step_stats {
+    uint32_t num_fixed_steps;
+    int32_t cg_iters;
+    float cg_rel;
+    float cg_abs;
+    uint32_t num_degenerate;
+    float sum_degenerate_detF;
+    float elastic_energy;
+    float min_detF;
+} orc_step_stats;
+
+typedef struct orc_solid orc_solid;
+orc_solid* orc_solid_create(uint32_t n_nodes, const float* ref_pos, uint32_t n_tets,
+                            const uint32_t* tets, const orc_solid_params* p);
+void orc_solid_destroy(orc_solid* s);
+void orc_solid_set_knc(orc_solid* s, uint32_t n, const uint32_t* nids);
+void orc_solid_fixed_step(orc_solid* s, float dt, orc_step_stats* st);
+void orc_solid_step(orc_solid* s, float dt, orc_step_stats* st);
+void orc_solid_get_state(const orc_solid* s, float* x, float* v);
+void orc_solid_set_state(orc_solid* s, const float* x, const float* v);
+float orc_solid_node_mass(const orc_solid* s);
+void orc_solid_rayleigh(const orc_solid* s, float c[2]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

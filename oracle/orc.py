@@ -1,0 +1,286 @@
+"""ctypes bindings for the CPU oracle (liborc.so) and the reference-pinning
+harness (_ref/libcorofem_ref.so).
+
+Test infrastructure only: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, always as the checker and never as the thing
+measured or shipped.
+"""
+import ctypes as C
+import os
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liborc.so")
+REF_PATH = os.path.join(HERE, "_ref", "libcorofem_ref.so")
+
+RM_ID, RM_QR, RM_MSVD, RM_DAPD = 0, 1, 2, 3
+
+_f = C.POINTER(C.c_float)
+_u = C.POINTER(C.c_uint32)
+
+
+def fp(a):
+    return a.ctypes.data_as(_f)
+
+
+def up(a):
+    return a.ctypes.data_as(_u)
+
+
+class SolidParams(C.Structure):
+    _fields_ = [
+        ("fixed_dt", C.c_float), ("density", C.c_float), ("young_modulus", C.c_float),
+        ("poisson_ratio", C.c_float), ("damping_ratio", C.c_float),
+        ("rayleigh_freq1", C.c_float), ("rayleigh_freq2", C.c_float),
+        ("gravity", C.c_float * 3), ("cg_rel_epsilon", C.c_float), ("cg_abs_epsilon", C.c_float),
+        ("cg_max_iter", C.c_uint32), ("rm", C.c_int), ("degenerate_threshold", C.c_float),
+        ("warm_start", C.c_int), ("reduce_fp64", C.c_int),
+    ]
+
+
+class StepStats(C.Structure):
+    _fields_ = [
+        ("num_fixed_steps", C.c_uint32), ("cg_iters", C.c_int32), ("cg_rel", C.c_float),
+        ("cg_abs", C.c_float), ("num_degenerate", C.c_uint32), ("sum_degenerate_detF", C.c_float),
+        ("elastic_energy", C.c_float), ("min_detF", C.c_float),
+    ]
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError("oracle not built: run `make -C oracle` (or __graft_entry__.build())")
+    lib = C.CDLL(LIB_PATH)
+    sig = {
+        "orc_box_counts": (None, [C.c_uint32] * 3 + [_u, _u]),
+        "orc_mesh_box": (None, [C.c_float] * 3 + [C.c_uint32] * 3 + [_f, _u]),
+        "orc_box2_counts": (None, [C.c_uint32] * 2 + [_u, _u]),
+        "orc_mesh_box2": (None, [C.c_float] * 2 + [C.c_uint32] * 2 + [_f, _u]),
+        "orc_det3": (C.c_float, [_f]),
+        "orc_inverse3": (None, [_f, _f]),
+        "orc_pd3": (None, [_f, C.c_float, _f, _u]),
+        "orc_svd3": (None, [_f, _f, _f, _f]),
+        "orc_svd_rotation3": (None, [_f, _f]),
+        "orc_svd3_sqrt3": (None, [_f, _f, _f, _f]),
+        "orc_pd2": (None, [_f, C.c_float, _f]),
+        "orc_compute_K": (C.c_float, [_f, C.c_float, C.c_float, _f]),
+        "orc_compute_K2": (C.c_float, [_f, C.c_float, C.c_float, _f]),
+        "orc_sse_dot": (C.c_float, [_f, _f, C.c_uint32]),
+        "orc_tetset_create": (C.c_void_p, [C.c_uint32, _f, C.c_uint32, _u, C.c_float, C.c_float]),
+        "orc_tetset_destroy": (None, [C.c_void_p]),
+        "orc_tetset_total_volume": (C.c_float, [C.c_void_p]),
+        "orc_tetset_element": (None, [C.c_void_p, C.c_uint32, _f, _f, _f]),
+        "orc_begin_evaluation": (None, [C.c_void_p, _f, C.c_int, C.c_float, _f, _f, _u, _f]),
+        "orc_set_rotations": (None, [C.c_void_p, _f]),
+        "orc_f_e": (None, [C.c_void_p, _f, _f]),
+        "orc_df_x": (None, [C.c_void_p, _f, _f]),
+        "orc_V": (C.c_float, [C.c_void_p, _f]),
+        "orc_solid_create": (C.c_void_p, [C.c_uint32, _f, C.c_uint32, _u, C.POINTER(SolidParams)]),
+        "orc_solid_destroy": (None, [C.c_void_p]),
+        "orc_solid_set_knc": (None, [C.c_void_p, C.c_uint32, _u]),
+        "orc_solid_fixed_step": (None, [C.c_void_p, C.c_float, C.POINTER(StepStats)]),
+        "orc_solid_step": (None, [C.c_void_p, C.c_float, C.POINTER(StepStats)]),
+        "orc_solid_get_state": (None, [C.c_void_p, _f, _f]),
+        "orc_solid_set_state": (None, [C.c_void_p, _f, _f]),
+        "orc_solid_node_mass": (C.c_float, [C.c_void_p]),
+        "orc_solid_rayleigh": (None, [C.c_void_p, _f]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+def _load_ref():
+    if not os.path.exists(REF_PATH):
+        return None
+    lib = C.CDLL(REF_PATH)
+    sig = {
+        "ref_compute_K": (C.c_float, [_f, C.c_float, C.c_float, _f]),
+        "ref_compute_volume": (C.c_float, [_f]),
+        "ref_inv_dm": (None, [_f, _f]),
+        "ref_compute_K2": (C.c_float, [_f, C.c_float, C.c_float, _f]),
+        "ref_det3": (C.c_float, [_f]),
+        "ref_inverse3": (None, [_f, _f]),
+        "ref_pd3": (None, [_f, C.c_float, _f]),
+        "ref_svd3": (None, [_f, _f, _f, _f]),
+        "ref_R_svd3": (None, [_f, C.c_float, C.c_float, _f]),
+        "ref_pd2": (None, [_f, C.c_float, _f]),
+        "ref_R_svd2": (None, [_f, C.c_float, C.c_float, _f]),
+        "ref_dot": (C.c_float, [_f, _f, C.c_uint32]),
+        "ref_has_sse_path": (C.c_int, []),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+_LIB = None
+_REF = None
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        _LIB = _load()
+    return _LIB
+
+
+def ref():
+    """The reference-pinning harness, or None where it was not built."""
+    global _REF
+    if _REF is None:
+        _REF = _load_ref()
+    return _REF
+
+
+# ---------------------------------------------------------------- helpers
+def mesh_box(half, counts):
+    nx, ny, nz = counts
+    L = lib()
+    nn = C.c_uint32()
+    nt = C.c_uint32()
+    L.orc_box_counts(nx, ny, nz, C.byref(nn), C.byref(nt))
+    pos = np.zeros(3 * nn.value, np.float32)
+    tets = np.zeros(4 * nt.value, np.uint32)
+    L.orc_mesh_box(half[0], half[1], half[2], nx, ny, nz, fp(pos), up(tets))
+    return pos, tets
+
+
+def mesh_box2(half, counts):
+    nx, ny = counts
+    L = lib()
+    nn = C.c_uint32()
+    nt = C.c_uint32()
+    L.orc_box2_counts(nx, ny, C.byref(nn), C.byref(nt))
+    pos = np.zeros(2 * nn.value, np.float32)
+    tris = np.zeros(3 * nt.value, np.uint32)
+    L.orc_mesh_box2(half[0], half[1], nx, ny, fp(pos), up(tris))
+    return pos, tris
+
+
+class TetSet:
+    """Oracle element set (EditableTetrahedronElementSet3 + its cache)."""
+
+    def __init__(self, ref_pos, tets, young, poisson):
+        self.ref_pos = np.ascontiguousarray(ref_pos, np.float32)
+        self.tets = np.ascontiguousarray(tets, np.uint32)
+        self.N = self.ref_pos.size // 3
+        self.T = self.tets.size // 4
+        self.h = lib().orc_tetset_create(self.N, fp(self.ref_pos), self.T, up(self.tets), young, poisson)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_tetset_destroy(self.h)
+            self.h = None
+
+    def total_volume(self):
+        return lib().orc_tetset_total_volume(self.h)
+
+    def element(self, e):
+        K = np.zeros(144, np.float32)
+        D = np.zeros(9, np.float32)
+        v = np.zeros(1, np.float32)
+        lib().orc_tetset_element(self.h, e, fp(K), fp(D), fp(v))
+        return K.reshape(12, 12), D.reshape(3, 3), float(v[0])
+
+    def begin_evaluation(self, x, rm=RM_MSVD, thr=0.1):
+        x = np.ascontiguousarray(x, np.float32)
+        R = np.zeros(9 * self.T, np.float32)
+        dF = np.zeros(self.T, np.float32)
+        nd = C.c_uint32()
+        sd = np.zeros(1, np.float32)
+        lib().orc_begin_evaluation(self.h, fp(x), rm, thr, fp(R), fp(dF), C.byref(nd), fp(sd))
+        return R, dF, nd.value, float(sd[0])
+
+    def set_rotations(self, R):
+        R = np.ascontiguousarray(R, np.float32)
+        lib().orc_set_rotations(self.h, fp(R))
+
+    def f_e(self, x, f=None):
+        x = np.ascontiguousarray(x, np.float32)
+        f = np.zeros(3 * self.N, np.float32) if f is None else f
+        lib().orc_f_e(self.h, fp(x), fp(f))
+        return f
+
+    def df_x(self, dx, df=None):
+        dx = np.ascontiguousarray(dx, np.float32)
+        df = np.zeros(3 * self.N, np.float32) if df is None else df
+        lib().orc_df_x(self.h, fp(dx), fp(df))
+        return df
+
+    def V(self, x):
+        x = np.ascontiguousarray(x, np.float32)
+        return lib().orc_V(self.h, fp(x))
+
+
+def default_params(**kw):
+    p = SolidParams()
+    p.fixed_dt = 1.0 / 60.0
+    p.density = 1000.0
+    p.young_modulus = 1e5
+    p.poisson_ratio = 0.3
+    p.damping_ratio = 0.0
+    p.rayleigh_freq1 = 0.1
+    p.rayleigh_freq2 = 10000.0
+    p.gravity[0], p.gravity[1], p.gravity[2] = 0.0, -9.81, 0.0
+    p.cg_rel_epsilon = 1e-4
+    p.cg_abs_epsilon = 1e-6
+    p.cg_max_iter = 50
+    p.rm = RM_MSVD
+    p.degenerate_threshold = 0.1
+    p.warm_start = 1
+    p.reduce_fp64 = 0
+    for k, v in kw.items():
+        if k == "gravity":
+            for i in range(3):
+                p.gravity[i] = v[i]
+        else:
+            setattr(p, k, v)
+    return p
+
+
+class Solid:
+    """Oracle leaf solid (LeafDSH_Solid3D_FEM, QIE-v)."""
+
+    def __init__(self, ref_pos, tets, params, knc=None):
+        self.ref_pos = np.ascontiguousarray(ref_pos, np.float32)
+        self.tets = np.ascontiguousarray(tets, np.uint32)
+        self.N = self.ref_pos.size // 3
+        self.params = params
+        self.h = lib().orc_solid_create(self.N, fp(self.ref_pos), self.tets.size // 4, up(self.tets),
+                                        C.byref(params))
+        if knc is not None:
+            k = np.ascontiguousarray(knc, np.uint32)
+            lib().orc_solid_set_knc(self.h, k.size, up(k))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_solid_destroy(self.h)
+            self.h = None
+
+    def fixed_step(self, dt=None):
+        st = StepStats()
+        lib().orc_solid_fixed_step(self.h, self.params.fixed_dt if dt is None else dt, C.byref(st))
+        return st
+
+    def step(self, dt):
+        st = StepStats()
+        lib().orc_solid_step(self.h, dt, C.byref(st))
+        return st
+
+    def state(self):
+        x = np.zeros(3 * self.N, np.float32)
+        v = np.zeros(3 * self.N, np.float32)
+        lib().orc_solid_get_state(self.h, fp(x), fp(v))
+        return x, v
+
+    def set_state(self, x, v):
+        x = np.ascontiguousarray(x, np.float32)
+        v = np.ascontiguousarray(v, np.float32)
+        lib().orc_solid_set_state(self.h, fp(x), fp(v))
+
+    def node_mass(self):
+        return lib().orc_solid_node_mass(self.h)

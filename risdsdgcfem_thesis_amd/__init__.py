@@ -1,0 +1,17 @@
+"""corofem for MI355X: the reference's corotational tet FEM implicit step
+(Saphyre2 LeafDSH_Solid3D_FEM + TetrahedronElementSet3 + ILSS_CG) as hand-written
+HIP kernels for gfx950 behind a C-ABI (include/corofem.h).
+
+This package is the Python mirror of that C-ABI (ctypes). It holds no compute
+of its own: every operation runs in libcorofem.so, and importing the package
+fails loudly when the library has not been built.
+"""
+from ._capi import (  # noqa: F401
+    lib, LIB_PATH, CorofemError, check, FemParams, SolidParams, StepStats, CgResult,
+    RM_ID, RM_QR, RM_MSVD, RM_DAPD, MM_C_WRP, DM_TRUNCATED,
+    CG_CONVERGED_REL, CG_CONVERGED_ABS, CG_MAX_ITER, CG_BREAKDOWN,
+)
+from .fem import (  # noqa: F401
+    DeviceArray, device_count, mesh_box, mesh_box2, TetrahedronElementSet3, ILSS_CG, LeafSolid3D,
+    default_fem_params, default_solid_params, set_tile_elems, get_tile_elems, synchronize,
+)

@@ -1,0 +1,137 @@
+"""ctypes declarations of include/corofem.h."""
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libcorofem.so")
+
+CF_OK = 0
+RM_ID, RM_QR, RM_MSVD, RM_DAPD = 0, 1, 2, 3
+MM_C_WRP = 1
+DM_TRUNCATED = 0
+CG_CONVERGED_REL, CG_CONVERGED_ABS, CG_MAX_ITER, CG_BREAKDOWN = 1, 2, 3, 4
+
+_f = C.POINTER(C.c_float)
+_u = C.POINTER(C.c_uint32)
+_vp = C.c_void_p
+
+
+class FemParams(C.Structure):
+    _fields_ = [
+        ("young_modulus", C.c_float), ("poisson_ratio", C.c_float), ("damping_ratio", C.c_float),
+        ("rayleigh_freq1", C.c_float), ("rayleigh_freq2", C.c_float),
+        ("degenerate_threshold_det_f", C.c_float), ("mm", C.c_int32), ("rm", C.c_int32), ("dm", C.c_int32),
+    ]
+
+
+class SolidParams(C.Structure):
+    _fields_ = [
+        ("fem", FemParams), ("fixed_dt", C.c_float), ("time_scale", C.c_float), ("density", C.c_float),
+        ("gravity", C.c_float * 3), ("air_drag_coeff", C.c_float), ("cg_rel_epsilon", C.c_float),
+        ("cg_abs_epsilon", C.c_float), ("cg_max_iter", C.c_uint32), ("warm_start", C.c_int32),
+        ("compute_energy", C.c_int32), ("thickness", C.c_float),
+    ]
+
+
+class StepStats(C.Structure):
+    _fields_ = [
+        ("num_fixed_steps", C.c_uint32), ("cg_iters", C.c_int32), ("cg_rel_residual", C.c_float),
+        ("cg_abs_residual", C.c_float), ("cg_reason", C.c_int32), ("num_degenerate", C.c_uint32),
+        ("sum_degenerate_det_f", C.c_float), ("elastic_energy", C.c_float), ("min_det_f", C.c_float),
+        ("step_ms", C.c_double),
+    ]
+
+
+class CgResult(C.Structure):
+    _fields_ = [("iterations", C.c_int32), ("rel_residual", C.c_float), ("abs_residual", C.c_float),
+                ("reason", C.c_int32)]
+
+
+AV_FN = C.CFUNCTYPE(None, _vp, _f, _f)
+CONS_FN = C.CFUNCTYPE(None, _vp, _f)
+
+# name -> (restype, argtypes); every function declared in include/corofem.h
+SIGNATURES = {
+    "cf_default_fem_params": (None, [C.POINTER(FemParams)]),
+    "cf_default_solid_params": (None, [C.POINTER(SolidParams)]),
+    "cf_last_error": (C.c_char_p, []),
+    "cf_version": (C.c_int, []),
+    "cf_device_count": (C.c_int, [C.POINTER(C.c_int)]),
+    "cf_set_device": (C.c_int, [C.c_int]),
+    "cf_device_alloc": (C.c_int, [C.c_size_t, C.POINTER(_vp)]),
+    "cf_device_free": (C.c_int, [_vp]),
+    "cf_memcpy_htod": (C.c_int, [_vp, _vp, C.c_size_t]),
+    "cf_memcpy_dtoh": (C.c_int, [_vp, _vp, C.c_size_t]),
+    "cf_memcpy_dtod": (C.c_int, [_vp, _vp, C.c_size_t]),
+    "cf_memset_device": (C.c_int, [_vp, C.c_int, C.c_size_t]),
+    "cf_device_synchronize": (C.c_int, []),
+    "cf_mesh_box_counts": (C.c_int, [C.c_uint32] * 3 + [_u, _u]),
+    "cf_mesh_box": (C.c_int, [_f, C.c_uint32, C.c_uint32, C.c_uint32, _f, _u]),
+    "cf_mesh_box2_counts": (C.c_int, [C.c_uint32] * 2 + [_u, _u]),
+    "cf_mesh_box2": (C.c_int, [_f, C.c_uint32, C.c_uint32, _f, _u]),
+    "cf_tetset_create": (C.c_int, [C.c_uint32, _f, C.c_uint32, _u, C.POINTER(FemParams), C.POINTER(_vp)]),
+    "cf_tetset_begin_edition": (C.c_int, [C.c_uint32, _f, C.c_uint32, C.POINTER(FemParams), C.POINTER(_vp)]),
+    "cf_tetset_add_element": (C.c_int, [_vp, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32]),
+    "cf_tetset_end_edition": (C.c_int, [_vp]),
+    "cf_tetset_set_params": (C.c_int, [_vp, C.POINTER(FemParams)]),
+    "cf_tetset_destroy": (C.c_int, [_vp]),
+    "cf_tetset_info": (C.c_int, [_vp, _u, _u, _f]),
+    "cf_tetset_get_elements": (C.c_int, [_vp, _f, _f]),
+    "cf_cache_create": (C.c_int, [_vp, _f, C.POINTER(_vp)]),
+    "cf_cache_destroy": (C.c_int, [_vp]),
+    "cf_begin_evaluation": (C.c_int, [_vp, _vp, _vp, _vp, C.c_float]),
+    "cf_f": (C.c_int, [_vp, _vp]),
+    "cf_f_e": (C.c_int, [_vp, _vp]),
+    "cf_df_x": (C.c_int, [_vp, _vp, _vp]),
+    "cf_V": (C.c_int, [_vp, C.POINTER(C.c_double)]),
+    "cf_end_evaluation": (C.c_int, [_vp]),
+    "cf_cache_stats": (C.c_int, [_vp, _u, _f, _f]),
+    "cf_cache_get_rotations": (C.c_int, [_vp, _f, _f]),
+    "cf_cache_synchronize": (C.c_int, [_vp]),
+    "cf_cg_create": (C.c_int, [C.c_uint32, C.POINTER(_vp)]),
+    "cf_cg_destroy": (C.c_int, [_vp]),
+    "cf_cg_stream": (_vp, [_vp]),
+    "cf_cg_solve": (C.c_int, [_vp, AV_FN, CONS_FN, _vp, _vp, _vp, C.c_float, C.c_float, C.c_uint32,
+                              C.POINTER(CgResult)]),
+    "cf_solid_create": (C.c_int, [C.c_uint32, _f, C.c_uint32, _u, C.POINTER(SolidParams), C.POINTER(_vp)]),
+    "cf_solid_destroy": (C.c_int, [_vp]),
+    "cf_solid_set_knc": (C.c_int, [_vp, C.c_uint32, _u, _f]),
+    "cf_solid_step": (C.c_int, [_vp, C.c_float, C.POINTER(StepStats)]),
+    "cf_solid_fixed_step": (C.c_int, [_vp, C.c_float, C.POINTER(StepStats)]),
+    "cf_solid_get_state": (C.c_int, [_vp, _f, _f]),
+    "cf_solid_set_state": (C.c_int, [_vp, _f, _f]),
+    "cf_solid_device_state": (C.c_int, [_vp, C.POINTER(_vp), C.POINTER(_vp)]),
+    "cf_solid_info": (C.c_int, [_vp, _u, _u, _f, _f]),
+    "cf_solid_assemble_rhs": (C.c_int, [_vp, C.c_float, _f, _f]),
+    "cf_solid_apply_A": (C.c_int, [_vp, C.c_float, C.c_float, _f, _f]),
+    "cf_solid_time_df_x": (C.c_int, [_vp, C.c_uint32, C.POINTER(C.c_double)]),
+    "cf_set_tile_elems": (C.c_int, [C.c_uint32]),
+    "cf_get_tile_elems": (C.c_int, [_u]),
+}
+
+
+class CorofemError(RuntimeError):
+    pass
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"libcorofem.so not found at {LIB_PATH}: build it with `python risdsdgcfem_thesis_amd/build.py` "
+            "(or __graft_entry__.build()); there is no fallback path")
+    L = C.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    return L
+
+
+lib = _load()
+
+
+def check(rc, what=""):
+    if rc != CF_OK:
+        msg = lib.cf_last_error().decode(errors="replace")
+        raise CorofemError(f"{what} failed with status {rc}: {msg}")
+    return rc

@@ -1,0 +1,1010 @@
+// corofem C-ABI: handles, device residency and the host-side orchestration of
+// the reference's element-set / solver / leaf-step interfaces (include/corofem.h).
+#include "cf_internal.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+namespace cf {
+
+static thread_local char g_err[512] = "";
+void set_error(const char* fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+}
+
+Material material_from(float E, float nu) {
+    // TetrahedronElement3::Compute_E (ms/fem/TetrahedronElement3.h:141-156)
+    Material m;
+    const float a = E / ((1.0f + nu) * (1.0f - 2.0f * nu));
+    m.a_nu = a * nu;
+    m.a_k1 = a * (1.0f - nu);
+    m.a_k2 = a * (0.5f - nu);
+    return m;
+}
+
+static uint32_t g_tile_elems = 0;
+
+template <class T>
+static int dalloc(T** p, size_t count) {
+    *p = nullptr;
+    if (count == 0) count = 1;
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(p), count * sizeof(T));
+    if (e != hipSuccess) {
+        set_error("hipMalloc(%zu bytes) failed: %s", count * sizeof(T), hipGetErrorString(e));
+        return (e == hipErrorOutOfMemory) ? CF_ERR_OUT_OF_MEMORY : CF_ERR_HIP;
+    }
+    return CF_OK;
+}
+template <class T>
+static void dfree(T*& p) {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+template <class T>
+static int upload(T** p, const std::vector<T>& v, hipStream_t s) {
+    CF_TRY(dalloc(p, v.size()));
+    if (!v.empty()) CF_CHECK_HIP(hipMemcpyAsync(*p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));
+    return CF_OK;
+}
+
+// Rayleigh coefficients c = inv(Omega) (2 zeta, 2 zeta) (LeafDSH_Solid3D_FEM.h:347-354)
+static void rayleigh_coeffs(float f1, float f2, float zeta, float c[2]) {
+    const float o00 = 1.0f / f1, o01 = f1, o10 = 1.0f / f2, o11 = f2;
+    const float det = o00 * o11 - o10 * o01;
+    const float id = 1.0f / det;
+    const float i00 = id * o11, i01 = -id * o01, i10 = -id * o10, i11 = id * o00;
+    const float z = 2 * zeta;
+    c[0] = i00 * z + i01 * z;
+    c[1] = i10 * z + i11 * z;
+}
+
+}  // namespace cf
+
+using namespace cf;
+
+// ================================================================ handles
+struct cf_tetset {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    cf_fem_params params{};
+    Material mat{};
+    TetDevice dev;
+    std::vector<uint32_t> perm;   // internal -> caller element
+    float total_volume = 0.f;
+    // edition
+    bool editing = false;
+    uint32_t expected_T = 0;
+    std::vector<float> ref_host;
+    std::vector<uint32_t> tets_host;
+};
+
+struct cf_cache {
+    cf_tetset* set = nullptr;
+    float* R = nullptr;
+    float* detF = nullptr;
+    float* inv_mass = nullptr;
+    uint32_t* blk_count = nullptr;
+    double* blk_sum = nullptr;
+    float* blk_min = nullptr;
+    double* blk_energy = nullptr;
+    int nblk = 1;
+    const float* x = nullptr;
+    const float* prev_x = nullptr;
+    const float* v = nullptr;
+    float time = 0.f, timestep = 0.f;
+    uint32_t num_degenerate = 0, num_fallback = 0;
+    float sum_degenerate = 0.f, min_det = 0.f;
+    bool stats_valid = false;
+};
+
+struct CgBuffers {
+    uint32_t n = 0;
+    float *r = nullptr, *d = nullptr, *q = nullptr;
+    CgScalars* st = nullptr;
+    double* part_a = nullptr;  // d.q partials
+    double* part_b = nullptr;  // r.r partials
+    int32_t* h_flag = nullptr; // pinned
+};
+
+struct cf_cg {
+    hipStream_t stream = nullptr;
+    CgBuffers b;
+};
+
+struct cf_solid {
+    cf_solid_params p{};
+    cf_tetset* set = nullptr;
+    cf_cache* cache = nullptr;
+    hipStream_t stream = nullptr;
+    uint32_t N = 0;
+    float *x = nullptr, *v = nullptr, *x0 = nullptr, *bvec = nullptr, *y = nullptr;
+    uint8_t* knc_mask = nullptr;
+    float* knc_target = nullptr;
+    uint32_t num_knc = 0;
+    float node_mass = 0.f;
+    float rayleigh[2] = {0, 0};
+    float acc_time = 0.f, total_time = 0.f;
+    float air_drag_per_second = 0.f;
+    CgBuffers cg;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    float* scratch_vec = nullptr;
+};
+
+// ================================================================ helpers
+static int cg_alloc(CgBuffers& b, uint32_t n) {
+    b.n = n;
+    CF_TRY(dalloc(&b.r, n));
+    CF_TRY(dalloc(&b.d, n));
+    CF_TRY(dalloc(&b.q, n));
+    CF_TRY(dalloc(&b.st, 1));
+    CF_TRY(dalloc(&b.part_a, kMaxVecBlocks));
+    CF_TRY(dalloc(&b.part_b, kMaxVecBlocks));
+    CF_CHECK_HIP(hipHostMalloc(reinterpret_cast<void**>(&b.h_flag), sizeof(int32_t)));
+    return CF_OK;
+}
+static void cg_free(CgBuffers& b) {
+    dfree(b.r); dfree(b.d); dfree(b.q); dfree(b.st); dfree(b.part_a); dfree(b.part_b);
+    if (b.h_flag) (void)hipHostFree(b.h_flag);
+    b.h_flag = nullptr;
+}
+
+static void tetset_free(cf_tetset* s) {
+    TetDevice& t = s->dev;
+    dfree(t.ref); dfree(t.conn); dfree(t.dinv); dfree(t.vol);
+    dfree(t.tile_elem_ptr); dfree(t.tile_node_ptr); dfree(t.tile_nodes); dfree(t.slot_ptr);
+    dfree(t.slots); dfree(t.lvid); dfree(t.node_part_ptr); dfree(t.node_parts); dfree(t.partials);
+    if (s->stream) (void)hipStreamDestroy(s->stream);
+    s->stream = nullptr;
+}
+
+static bool valid_params(const cf_fem_params* p) {
+    if (!p) return false;
+    if (!(p->young_modulus > 0) || !(p->poisson_ratio > -1.0f) || !(p->poisson_ratio < 0.5f)) {
+        set_error("invalid material: E=%g nu=%g", p->young_modulus, p->poisson_ratio);
+        return false;
+    }
+    if (p->mm != CF_MM_C_WRP) { set_error("material method %d not implemented (C_WRP only)", p->mm); return false; }
+    if (p->rm != CF_RM_ID && p->rm != CF_RM_MSVD && p->rm != CF_RM_DAPD) {
+        set_error("rotation method %d not implemented (Id, MSVD, DAPD)", p->rm);
+        return false;
+    }
+    if (p->dm != CF_DM_TRUNCATED) { set_error("differential method %d not implemented", p->dm); return false; }
+    return true;
+}
+
+// Builds device arrays from ref_host/tets_host (EndEdition).
+static int tetset_bake(cf_tetset* s) {
+    const uint32_t N = uint32_t(s->ref_host.size() / 3), T = uint32_t(s->tets_host.size() / 4);
+    for (size_t i = 0; i < s->tets_host.size(); i++)
+        if (s->tets_host[i] >= N) {
+            set_error("element %zu references node %u >= %u", i / 4, s->tets_host[i], N);
+            return CF_ERR_INVALID_ARG;
+        }
+    TileHost th;
+    CF_TRY(build_tiles(N, T, s->ref_host.data(), s->tets_host.data(), g_tile_elems, th));
+    TetDevice& t = s->dev;
+    t.N = N; t.T = T;
+    hipStream_t st = s->stream;
+    CF_TRY(upload(&t.ref, s->ref_host, st));
+    CF_TRY(upload(&t.conn, th.conn_planes, st));
+    CF_TRY(dalloc(&t.dinv, 9 * (size_t)T));
+    CF_TRY(dalloc(&t.vol, T));
+    CF_TRY(upload(&t.tile_elem_ptr, th.tile_elem_ptr, st));
+    CF_TRY(upload(&t.tile_node_ptr, th.tile_node_ptr, st));
+    CF_TRY(upload(&t.tile_nodes, th.tile_nodes, st));
+    CF_TRY(upload(&t.slot_ptr, th.slot_ptr, st));
+    CF_TRY(upload(&t.slots, th.slots, st));
+    CF_TRY(upload(&t.lvid, th.lvid_planes, st));
+    CF_TRY(upload(&t.node_part_ptr, th.node_part_ptr, st));
+    CF_TRY(upload(&t.node_parts, th.node_parts, st));
+    t.num_tiles = uint32_t(th.tile_elem_ptr.size() - 1);
+    t.max_tile_elems = th.max_tile_elems;
+    t.max_tile_nodes = th.max_tile_nodes;
+    t.num_parts = uint32_t(th.tile_nodes.size());
+    CF_TRY(dalloc(&t.partials, 3 * (size_t)t.num_parts));
+    CF_TRY(launch_precompute(t, st));
+    s->perm.swap(th.perm);
+    // total volume: FP32 serial sum in the caller's element order, like
+    // EndEdition's accumulation (TetrahedronElementSet3.cpp:2118-2162)
+    std::vector<float> vol(T);
+    if (T) CF_CHECK_HIP(hipMemcpyAsync(vol.data(), t.vol, T * sizeof(float), hipMemcpyDeviceToHost, st));
+    CF_CHECK_HIP(hipStreamSynchronize(st));
+    std::vector<float> vol_user(T);
+    for (uint32_t i = 0; i < T; i++) vol_user[s->perm[i]] = vol[i];
+    float acc = 0.f;
+    for (uint32_t i = 0; i < T; i++) acc += vol_user[i];
+    s->total_volume = acc;
+    s->mat = material_from(s->params.young_modulus, s->params.poisson_ratio);
+    s->ref_host.clear(); s->ref_host.shrink_to_fit();
+    s->tets_host.clear(); s->tets_host.shrink_to_fit();
+    s->editing = false;
+    return CF_OK;
+}
+
+static int cache_stats_fetch(cf_cache* c) {
+    if (c->stats_valid) return CF_OK;
+    hipStream_t st = c->set->stream;
+    std::vector<uint32_t> cnt(2 * c->nblk);
+    std::vector<double> sum(c->nblk);
+    std::vector<float> mn(c->nblk);
+    CF_CHECK_HIP(hipMemcpyAsync(cnt.data(), c->blk_count, cnt.size() * 4, hipMemcpyDeviceToHost, st));
+    CF_CHECK_HIP(hipMemcpyAsync(sum.data(), c->blk_sum, sum.size() * 8, hipMemcpyDeviceToHost, st));
+    CF_CHECK_HIP(hipMemcpyAsync(mn.data(), c->blk_min, mn.size() * 4, hipMemcpyDeviceToHost, st));
+    CF_CHECK_HIP(hipStreamSynchronize(st));
+    uint32_t a = 0, f = 0;
+    double s = 0;
+    float m = 3e38f;
+    for (int i = 0; i < c->nblk; i++) { a += cnt[i]; f += cnt[c->nblk + i]; s += sum[i]; m = std::min(m, mn[i]); }
+    c->num_degenerate = a;
+    c->num_fallback = f;
+    c->sum_degenerate = float(s);
+    c->min_det = m;
+    c->stats_valid = true;
+    return CF_OK;
+}
+
+// Device CG loop of ILSS_CG::Solve for the leaf operator A = fM M - fK K.
+// The loop runs on the device; the host enqueues chunks of iterations and
+// checks the device's done flag between chunks.
+static int solid_cg(cf_solid* s, float fM, float fK, cf_cg_result* res) {
+    cf_tetset* set = s->set;
+    const TetDevice& t = set->dev;
+    hipStream_t st = s->stream;
+    CgBuffers& b = s->cg;
+    const uint32_t n = 3 * s->N;
+    const int nvec = vec_blocks(n);
+    const int nnode = vec_blocks(s->N);
+    uint32_t max_it = std::min<uint32_t>(s->p.cg_max_iter, kMaxCgIter);
+    NodeTerms nt;
+    nt.knc_mask = s->num_knc ? s->knc_mask : nullptr;
+    nt.mass = s->node_mass;
+    nt.fM = fM; nt.fK = fK;
+    nt.partials = b.part_a;
+    // r = S(b - A y), d = r
+    nt.d = s->y; nt.out = b.q;
+    CF_TRY(launch_forces(t, kModeDfx, s->y, s->cache->R, set->mat, st));
+    CF_TRY(launch_gather(t, kGatherMatvec, nt, nnode, st));
+    CF_TRY(launch_cg_init(n, b.q, s->bvec, b.r, b.d, nt.knc_mask, b.part_b, nvec, st));
+    CF_TRY(launch_cg_start(b.st, b.part_b, nvec, s->p.cg_rel_epsilon, s->p.cg_abs_epsilon, max_it, st));
+    nt.d = b.d; nt.out = b.q;
+    uint32_t k = 0;
+    const uint32_t chunk = 8;
+    while (k < max_it) {
+        const uint32_t kend = std::min(max_it, k + chunk);
+        for (; k < kend; k++) {
+            const int32_t* skip = &b.st->done[k];
+            CF_TRY(launch_forces_skip(t, kModeDfx, b.d, s->cache->R, set->mat, skip, st));
+            CF_TRY(launch_gather_skip(t, kGatherMatvec, nt, nnode, skip, st));
+            CF_TRY(launch_cg_update_xr(n, k, b.st, b.part_a, nnode, s->y, b.r, b.d, b.q, b.part_b, nvec, st));
+            CF_TRY(launch_cg_update_d(n, k, b.st, b.part_b, nvec, b.d, b.r, nt.knc_mask, nvec, st));
+        }
+        CF_CHECK_HIP(hipMemcpyAsync(b.h_flag, &b.st->done[k], sizeof(int32_t), hipMemcpyDeviceToHost, st));
+        CF_CHECK_HIP(hipStreamSynchronize(st));
+        if (*b.h_flag) break;
+    }
+    // results
+    static thread_local CgScalars hs;
+    CF_CHECK_HIP(hipMemcpyAsync(&hs, b.st, sizeof(CgScalars), hipMemcpyDeviceToHost, st));
+    CF_CHECK_HIP(hipStreamSynchronize(st));
+    uint32_t j = 0;
+    while (j <= max_it && !hs.done[j]) j++;
+    if (j > max_it) j = max_it;
+    int reason = hs.reason[j];
+    uint32_t iters = j;
+    double delta = hs.delta[j];
+    if (reason == CF_CG_BREAKDOWN) { iters = j - 1; delta = hs.delta[j - 1]; }
+    res->iterations = int32_t(iters);
+    res->reason = reason;
+    res->rel_residual = (hs.delta0 > 0) ? float(std::sqrt(delta / hs.delta0)) : 0.f;
+    res->abs_residual = float(std::sqrt(delta));
+    return CF_OK;
+}
+
+// ================================================================ C API
+extern "C" {
+
+const char* cf_last_error(void) { return g_err; }
+int cf_version(void) { return 10000; }
+
+void cf_default_fem_params(cf_fem_params* p) {
+    if (!p) return;
+    // ms::fem::Params() (ms/fem/Params.h:74-98)
+    p->young_modulus = 1000.0f;
+    p->poisson_ratio = 0.25f;
+    p->damping_ratio = 1.0f;
+    p->rayleigh_freq1 = 0.1f;
+    p->rayleigh_freq2 = 10000.0f;
+    p->degenerate_threshold_det_f = 0.1f;
+    p->mm = CF_MM_C_WRP;
+    p->rm = CF_RM_MSVD;
+    p->dm = CF_DM_TRUNCATED;
+}
+
+void cf_default_solid_params(cf_solid_params* p) {
+    if (!p) return;
+    // LeafDSH_Solid3D_FEM::Params() (ds/dsh/LeafDSH_Solid3D_FEM.h:203-274)
+    cf_default_fem_params(&p->fem);
+    p->fixed_dt = 0.001f;
+    p->time_scale = 1.0f;
+    p->density = 1000.0f;
+    p->gravity[0] = p->gravity[1] = p->gravity[2] = 0.0f;
+    p->air_drag_coeff = 0.0f;
+    p->cg_rel_epsilon = 1e-3f;
+    p->cg_abs_epsilon = 1e-6f;
+    p->cg_max_iter = 20;
+    p->warm_start = 1;
+    p->compute_energy = 1;
+    p->thickness = 1.0f;
+}
+
+int cf_device_count(int* n) {
+    if (!n) return CF_ERR_INVALID_ARG;
+    hipError_t e = hipGetDeviceCount(n);
+    if (e != hipSuccess) { *n = 0; set_error("hipGetDeviceCount: %s", hipGetErrorString(e)); return CF_ERR_NO_DEVICE; }
+    return CF_OK;
+}
+int cf_set_device(int d) { CF_CHECK_HIP(hipSetDevice(d)); return CF_OK; }
+int cf_device_alloc(size_t bytes, void** p) {
+    if (!p) return CF_ERR_INVALID_ARG;
+    return dalloc(reinterpret_cast<uint8_t**>(p), bytes);
+}
+int cf_device_free(void* p) { if (p) CF_CHECK_HIP(hipFree(p)); return CF_OK; }
+// The library's handles run on non-blocking streams, which the legacy default
+// stream does not wait for: these convenience copies synchronize the device
+// first so that they observe every enqueued operation.
+int cf_memcpy_htod(void* d, const void* h, size_t n) {
+    CF_CHECK_HIP(hipDeviceSynchronize());
+    CF_CHECK_HIP(hipMemcpy(d, h, n, hipMemcpyHostToDevice));
+    return CF_OK;
+}
+int cf_memcpy_dtoh(void* h, const void* d, size_t n) {
+    CF_CHECK_HIP(hipDeviceSynchronize());
+    CF_CHECK_HIP(hipMemcpy(h, d, n, hipMemcpyDeviceToHost));
+    return CF_OK;
+}
+int cf_memcpy_dtod(void* d, const void* s, size_t n) {
+    CF_CHECK_HIP(hipDeviceSynchronize());
+    CF_CHECK_HIP(hipMemcpy(d, s, n, hipMemcpyDeviceToDevice));
+    CF_CHECK_HIP(hipDeviceSynchronize());
+    return CF_OK;
+}
+int cf_memset_device(void* d, int v, size_t n) {
+    CF_CHECK_HIP(hipDeviceSynchronize());
+    CF_CHECK_HIP(hipMemset(d, v, n));
+    CF_CHECK_HIP(hipDeviceSynchronize());
+    return CF_OK;
+}
+int cf_device_synchronize(void) { CF_CHECK_HIP(hipDeviceSynchronize()); return CF_OK; }
+
+// ---------------------------------------------------------------- meshes
+int cf_mesh_box_counts(uint32_t nx, uint32_t ny, uint32_t nz, uint32_t* nn, uint32_t* nt) {
+    if (nx < 2 || ny < 2 || nz < 2 || !nn || !nt) { set_error("box needs >= 2 vertices per axis"); return CF_ERR_INVALID_ARG; }
+    const uint64_t n = uint64_t(nx) * ny * nz, t = 5ull * (nx - 1) * (ny - 1) * (nz - 1);
+    if (n > 0xffffffffull || 4 * t > 0xffffffffull) { set_error("box too large for uint32 indices"); return CF_ERR_CAPACITY; }
+    *nn = uint32_t(n);
+    *nt = uint32_t(t);
+    return CF_OK;
+}
+
+// Make_TetSolidShape3_Box (Geo/shape/TetSolidShape3.cpp:921-999)
+int cf_mesh_box(const float half[3], uint32_t nx, uint32_t ny, uint32_t nz, float* pos, uint32_t* tets) {
+    uint32_t nn, nt;
+    CF_TRY(cf_mesh_box_counts(nx, ny, nz, &nn, &nt));
+    if (!half || !pos || !tets) return CF_ERR_INVALID_ARG;
+    const uint32_t cnt[3] = {nx, ny, nz};
+    float size[3];
+    for (int c = 0; c < 3; c++) size[c] = 2 * half[c];
+    size_t v = 0;
+    for (uint32_t i = 0; i < nx; i++)
+        for (uint32_t j = 0; j < ny; j++)
+            for (uint32_t k = 0; k < nz; k++, v++) {
+                const uint32_t ijk[3] = {i, j, k};
+                for (int c = 0; c < 3; c++) pos[3 * v + c] = (-half[c]) + (size[c] * float(ijk[c])) / float(cnt[c] - 1);
+            }
+    size_t t = 0;
+    auto emit = [&](uint32_t a, uint32_t b, uint32_t c, uint32_t d) {
+        tets[4 * t] = a; tets[4 * t + 1] = b; tets[4 * t + 2] = c; tets[4 * t + 3] = d; t++;
+    };
+    const uint32_t sx = ny * nz, sy = nz;
+    for (uint32_t i = 0; i + 1 < nx; i++)
+        for (uint32_t j = 0; j + 1 < ny; j++)
+            for (uint32_t k = 0; k + 1 < nz; k++) {
+                // cube corners: bit0 = +x, bit1 = +y, bit2 = +z, mirrored on odd i/j/k
+                uint32_t c[8];
+                for (uint32_t b = 0; b < 8; b++) {
+                    uint32_t di = (b & 1) ^ (i & 1), dj = ((b >> 1) & 1) ^ (j & 1), dk = ((b >> 2) & 1) ^ (k & 1);
+                    c[b] = (i + di) * sx + (j + dj) * sy + (k + dk);
+                }
+                // reference labels: v0=---, v1=-+-, v2=++-, v3=+--, v4=--+, v5=-++, v6=+++, v7=+-+
+                const uint32_t v0 = c[0], v1 = c[2], v2 = c[3], v3 = c[1], v4 = c[4], v5 = c[6], v6 = c[7], v7 = c[5];
+                const uint32_t mirrors = (i & 1) + (j & 1) + (k & 1);
+                if (mirrors % 2 == 0) {
+                    emit(v0, v1, v4, v3); emit(v2, v1, v3, v6); emit(v5, v1, v6, v4); emit(v6, v1, v3, v4); emit(v7, v3, v4, v6);
+                } else {
+                    emit(v0, v1, v3, v4); emit(v2, v1, v6, v3); emit(v5, v1, v4, v6); emit(v6, v1, v4, v3); emit(v7, v3, v6, v4);
+                }
+            }
+    return CF_OK;
+}
+
+int cf_mesh_box2_counts(uint32_t nx, uint32_t ny, uint32_t* nn, uint32_t* nt) {
+    if (nx < 2 || ny < 2 || !nn || !nt) { set_error("grid needs >= 2 vertices per axis"); return CF_ERR_INVALID_ARG; }
+    *nn = nx * ny;
+    *nt = 2 * (nx - 1) * (ny - 1);
+    return CF_OK;
+}
+
+// Make_MeshSolidShape2_Box (Geo/shape/MeshSolidShape2.cpp:1651-1675)
+int cf_mesh_box2(const float half[2], uint32_t nx, uint32_t ny, float* pos, uint32_t* tris) {
+    uint32_t nn, nt;
+    CF_TRY(cf_mesh_box2_counts(nx, ny, &nn, &nt));
+    if (!half || !pos || !tris) return CF_ERR_INVALID_ARG;
+    const float step_x = 2 * half[0] / float(nx - 1), step_y = 2 * half[1] / float(ny - 1);
+    size_t v = 0;
+    for (uint32_t iy = 0; iy < ny; iy++)
+        for (uint32_t ix = 0; ix < nx; ix++, v++) {
+            pos[2 * v] = (-half[0]) + float(ix) * step_x;
+            pos[2 * v + 1] = (-half[1]) + float(iy) * step_y;
+        }
+    size_t t = 0;
+    for (uint32_t iy = 0; iy + 1 < ny; iy++)
+        for (uint32_t ix = 0; ix + 1 < nx; ix++) {
+            const uint32_t a = iy * nx + ix, b = a + 1, c = (iy + 1) * nx + ix + 1, d = (iy + 1) * nx + ix;
+            tris[3 * t] = a; tris[3 * t + 1] = b; tris[3 * t + 2] = c; t++;
+            tris[3 * t] = c; tris[3 * t + 1] = d; tris[3 * t + 2] = a; t++;
+        }
+    return CF_OK;
+}
+
+// ---------------------------------------------------------------- element set
+int cf_tetset_begin_edition(uint32_t n_nodes, const float* ref, uint32_t n_tets, const cf_fem_params* params,
+                            cf_tetset** out) {
+    if (!out || !ref || n_nodes == 0 || !valid_params(params)) {
+        if (!out || !ref || n_nodes == 0) set_error("cf_tetset_begin_edition: invalid arguments");
+        return CF_ERR_INVALID_ARG;
+    }
+    *out = nullptr;
+    cf_tetset* s = new (std::nothrow) cf_tetset;
+    if (!s) return CF_ERR_OUT_OF_MEMORY;
+    hipError_t e = hipGetDevice(&s->device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        set_error("no HIP device: %s", hipGetErrorString(e));
+        delete s;
+        return CF_ERR_NO_DEVICE;
+    }
+    s->params = *params;
+    s->editing = true;
+    s->expected_T = n_tets;
+    s->ref_host.assign(ref, ref + 3 * (size_t)n_nodes);
+    s->tets_host.reserve(4 * (size_t)n_tets);
+    *out = s;
+    return CF_OK;
+}
+
+int cf_tetset_add_element(cf_tetset* s, uint32_t i, uint32_t j, uint32_t k, uint32_t l) {
+    if (!s || !s->editing) { set_error("set is not being edited"); return CF_ERR_INVALID_ARG; }
+    if (s->tets_host.size() / 4 >= s->expected_T) { set_error("more elements than announced"); return CF_ERR_CAPACITY; }
+    s->tets_host.push_back(i); s->tets_host.push_back(j); s->tets_host.push_back(k); s->tets_host.push_back(l);
+    return CF_OK;
+}
+
+int cf_tetset_end_edition(cf_tetset* s) {
+    if (!s || !s->editing) { set_error("set is not being edited"); return CF_ERR_INVALID_ARG; }
+    if (s->tets_host.size() / 4 != s->expected_T) {
+        set_error("EndEdition with %zu of %u elements", s->tets_host.size() / 4, s->expected_T);
+        return CF_ERR_INVALID_ARG;
+    }
+    return tetset_bake(s);
+}
+
+int cf_tetset_create(uint32_t n_nodes, const float* ref, uint32_t n_tets, const uint32_t* tets,
+                     const cf_fem_params* params, cf_tetset** out) {
+    if (!tets && n_tets) { set_error("null tets"); return CF_ERR_INVALID_ARG; }
+    CF_TRY(cf_tetset_begin_edition(n_nodes, ref, n_tets, params, out));
+    (*out)->tets_host.assign(tets, tets + 4 * (size_t)n_tets);
+    int rc = tetset_bake(*out);
+    if (rc != CF_OK) { cf_tetset_destroy(*out); *out = nullptr; }
+    return rc;
+}
+
+int cf_tetset_set_params(cf_tetset* s, const cf_fem_params* params) {
+    if (!s || !valid_params(params)) return CF_ERR_INVALID_ARG;
+    s->params = *params;
+    s->mat = material_from(params->young_modulus, params->poisson_ratio);
+    return CF_OK;
+}
+
+int cf_tetset_destroy(cf_tetset* s) {
+    if (!s) return CF_OK;
+    tetset_free(s);
+    delete s;
+    return CF_OK;
+}
+
+int cf_tetset_info(const cf_tetset* s, uint32_t* nn, uint32_t* nt, float* vol) {
+    if (!s) return CF_ERR_INVALID_ARG;
+    if (nn) *nn = s->dev.N;
+    if (nt) *nt = s->dev.T;
+    if (vol) *vol = s->total_volume;
+    return CF_OK;
+}
+
+int cf_tetset_get_elements(const cf_tetset* s, float* h_inv_dm, float* h_vol) {
+    if (!s || s->editing) return CF_ERR_INVALID_ARG;
+    const uint32_t T = s->dev.T;
+    std::vector<float> planes(9 * (size_t)T), vol(T);
+    CF_CHECK_HIP(hipMemcpyAsync(planes.data(), s->dev.dinv, planes.size() * 4, hipMemcpyDeviceToHost, s->stream));
+    CF_CHECK_HIP(hipMemcpyAsync(vol.data(), s->dev.vol, vol.size() * 4, hipMemcpyDeviceToHost, s->stream));
+    CF_CHECK_HIP(hipStreamSynchronize(s->stream));
+    for (uint32_t i = 0; i < T; i++) {
+        const uint32_t e = s->perm[i];
+        if (h_inv_dm)
+            for (int k = 0; k < 9; k++) h_inv_dm[9 * (size_t)e + k] = planes[k * (size_t)T + i];
+        if (h_vol) h_vol[e] = vol[i];
+    }
+    return CF_OK;
+}
+
+// ---------------------------------------------------------------- cache
+int cf_cache_create(cf_tetset* set, const float* inv_mass, cf_cache** out) {
+    if (!set || set->editing || !out) { set_error("cf_cache_create: invalid set"); return CF_ERR_INVALID_ARG; }
+    cf_cache* c = new (std::nothrow) cf_cache;
+    if (!c) return CF_ERR_OUT_OF_MEMORY;
+    c->set = set;
+    const uint32_t T = set->dev.T;
+    c->nblk = vec_blocks(T);
+    int rc = dalloc(&c->R, 9 * (size_t)T);
+    if (rc == CF_OK) rc = dalloc(&c->detF, T);
+    if (rc == CF_OK) rc = dalloc(&c->blk_count, 2 * (size_t)kMaxVecBlocks);
+    if (rc == CF_OK) rc = dalloc(&c->blk_sum, kMaxVecBlocks);
+    if (rc == CF_OK) rc = dalloc(&c->blk_min, kMaxVecBlocks);
+    if (rc == CF_OK) rc = dalloc(&c->blk_energy, kMaxVecBlocks);
+    if (rc == CF_OK && inv_mass) {
+        rc = dalloc(&c->inv_mass, set->dev.N);
+        if (rc == CF_OK && hipMemcpy(c->inv_mass, inv_mass, set->dev.N * 4, hipMemcpyHostToDevice) != hipSuccess)
+            rc = CF_ERR_HIP;
+    }
+    if (rc == CF_OK) {
+        // CreateCache: R = Id, det F = 1 (TetrahedronElementSet3.cpp:190-209)
+        std::vector<float> planes(9 * (size_t)T, 0.f), ones(T, 1.f);
+        for (size_t e = 0; e < T; e++) { planes[e] = 1.f; planes[4 * (size_t)T + e] = 1.f; planes[8 * (size_t)T + e] = 1.f; }
+        if (hipMemcpy(c->R, planes.data(), planes.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(c->detF, ones.data(), ones.size() * 4, hipMemcpyHostToDevice) != hipSuccess)
+            rc = CF_ERR_HIP;
+    }
+    if (rc != CF_OK) { cf_cache_destroy(c); return rc; }
+    *out = c;
+    return CF_OK;
+}
+
+int cf_cache_destroy(cf_cache* c) {
+    if (!c) return CF_OK;
+    dfree(c->R); dfree(c->detF); dfree(c->inv_mass); dfree(c->blk_count); dfree(c->blk_sum);
+    dfree(c->blk_min); dfree(c->blk_energy);
+    delete c;
+    return CF_OK;
+}
+
+int cf_begin_evaluation(cf_cache* c, const float* prev_x, const float* x, const float* v, float t) {
+    if (!c || !x) { set_error("cf_begin_evaluation: null cache or x"); return CF_ERR_INVALID_ARG; }
+    c->prev_x = prev_x; c->x = x; c->v = v;
+    c->timestep = t - c->time;
+    c->time = t;
+    const cf_tetset* s = c->set;
+    CF_TRY(launch_begin_eval(s->dev, x, s->params.rm, s->params.degenerate_threshold_det_f, c->R, c->detF,
+                             c->blk_count, c->blk_sum, c->blk_min, c->nblk, s->stream));
+    c->stats_valid = false;
+    return CF_OK;
+}
+
+int cf_f_e(cf_cache* c, float* f) {
+    if (!c || !c->x || !f) { set_error("cf_f_e outside Begin/EndEvaluation or null f"); return CF_ERR_INVALID_ARG; }
+    const cf_tetset* s = c->set;
+    CF_TRY(launch_forces(s->dev, kModeFe, c->x, c->R, s->mat, s->stream));
+    NodeTerms nt;
+    nt.out = f;
+    return launch_gather(s->dev, kGatherAccumulate, nt, vec_blocks(s->dev.N), s->stream);
+}
+
+int cf_f(cf_cache* c, float* f) { return cf_f_e(c, f); }  // f = f_e + f_p, f_p a no-op (:294-300,350-353)
+
+int cf_df_x(cf_cache* c, const float* dx, float* df) {
+    if (!c || !dx || !df) { set_error("cf_df_x: null argument"); return CF_ERR_INVALID_ARG; }
+    const cf_tetset* s = c->set;
+    CF_TRY(launch_forces(s->dev, kModeDfx, dx, c->R, s->mat, s->stream));
+    NodeTerms nt;
+    nt.out = df;
+    return launch_gather(s->dev, kGatherAccumulate, nt, vec_blocks(s->dev.N), s->stream);
+}
+
+int cf_V(cf_cache* c, double* energy) {
+    if (!c || !c->x || !energy) { set_error("cf_V outside Begin/EndEvaluation"); return CF_ERR_INVALID_ARG; }
+    const cf_tetset* s = c->set;
+    CF_TRY(launch_energy(s->dev, c->x, c->R, s->mat, c->blk_energy, c->nblk, s->stream));
+    std::vector<double> part(c->nblk);
+    CF_CHECK_HIP(hipMemcpyAsync(part.data(), c->blk_energy, part.size() * 8, hipMemcpyDeviceToHost, s->stream));
+    CF_CHECK_HIP(hipStreamSynchronize(s->stream));
+    double acc = 0;
+    for (double p : part) acc += p;
+    *energy = acc;
+    return CF_OK;
+}
+
+int cf_end_evaluation(cf_cache* c) {
+    if (!c) return CF_ERR_INVALID_ARG;
+    c->x = c->prev_x = c->v = nullptr;
+    return CF_OK;
+}
+
+int cf_cache_stats(const cf_cache* c, uint32_t* nd, float* sd, float* md) {
+    if (!c) return CF_ERR_INVALID_ARG;
+    CF_TRY(cache_stats_fetch(const_cast<cf_cache*>(c)));
+    if (nd) *nd = c->num_degenerate;
+    if (sd) *sd = c->sum_degenerate;
+    if (md) *md = c->min_det;
+    return CF_OK;
+}
+
+int cf_cache_get_rotations(const cf_cache* c, float* h_R, float* h_det) {
+    if (!c) return CF_ERR_INVALID_ARG;
+    const cf_tetset* s = c->set;
+    const uint32_t T = s->dev.T;
+    std::vector<float> planes(9 * (size_t)T), det(T);
+    CF_CHECK_HIP(hipMemcpyAsync(planes.data(), c->R, planes.size() * 4, hipMemcpyDeviceToHost, s->stream));
+    CF_CHECK_HIP(hipMemcpyAsync(det.data(), c->detF, det.size() * 4, hipMemcpyDeviceToHost, s->stream));
+    CF_CHECK_HIP(hipStreamSynchronize(s->stream));
+    for (uint32_t i = 0; i < T; i++) {
+        const uint32_t e = s->perm[i];
+        if (h_R)
+            for (int k = 0; k < 9; k++) h_R[9 * (size_t)e + k] = planes[k * (size_t)T + i];
+        if (h_det) h_det[e] = det[i];
+    }
+    return CF_OK;
+}
+
+int cf_cache_synchronize(cf_cache* c) {
+    if (!c) return CF_ERR_INVALID_ARG;
+    CF_CHECK_HIP(hipStreamSynchronize(c->set->stream));
+    return CF_OK;
+}
+
+// ---------------------------------------------------------------- generic CG
+int cf_cg_create(uint32_t n, cf_cg** out) {
+    if (!out || n == 0) return CF_ERR_INVALID_ARG;
+    cf_cg* c = new (std::nothrow) cf_cg;
+    if (!c) return CF_ERR_OUT_OF_MEMORY;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        set_error("hipStreamCreate failed");
+        return CF_ERR_NO_DEVICE;
+    }
+    int rc = cg_alloc(c->b, n);
+    if (rc != CF_OK) { cf_cg_destroy(c); return rc; }
+    *out = c;
+    return CF_OK;
+}
+
+int cf_cg_destroy(cf_cg* c) {
+    if (!c) return CF_OK;
+    cg_free(c->b);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return CF_OK;
+}
+
+void* cf_cg_stream(cf_cg* c) { return c ? (void*)c->stream : nullptr; }
+
+int cf_cg_solve(cf_cg* c, cf_av_fn av, cf_constraints_fn cons, void* user, float* x, const float* bvec, float rel,
+                float abs_eps, uint32_t max_it, cf_cg_result* res) {
+    if (!c || !av || !x || !bvec || !res || max_it == 0) { set_error("cf_cg_solve: invalid arguments"); return CF_ERR_INVALID_ARG; }
+    max_it = std::min(max_it, kMaxCgIter);
+    CgBuffers& b = c->b;
+    const uint32_t n = b.n;
+    const int nvec = vec_blocks(n);
+    hipStream_t st = c->stream;
+    // r = S(b - A x); d = r; delta0 = r.r   (ILSS_CG.h:53-65)
+    av(user, b.q, x);
+    CF_TRY(launch_cg_init(n, b.q, bvec, b.r, b.d, nullptr, b.part_b, nvec, st));
+    if (cons) {
+        cons(user, b.r);
+        CF_CHECK_HIP(hipMemcpyAsync(b.d, b.r, n * 4, hipMemcpyDeviceToDevice, st));
+        CF_TRY(launch_cg_dot(n, b.r, b.r, b.part_b, nvec, st));
+    }
+    CF_TRY(launch_cg_start(b.st, b.part_b, nvec, rel, abs_eps, max_it, st));
+    CF_CHECK_HIP(hipMemcpyAsync(b.h_flag, &b.st->done[0], 4, hipMemcpyDeviceToHost, st));
+    CF_CHECK_HIP(hipStreamSynchronize(st));
+    uint32_t k = 0;
+    while (!*b.h_flag && k < max_it) {
+        av(user, b.q, b.d);
+        if (cons) cons(user, b.q);
+        CF_TRY(launch_cg_dot(n, b.d, b.q, b.part_a, nvec, st));
+        CF_TRY(launch_cg_update_xr(n, k, b.st, b.part_a, nvec, x, b.r, b.d, b.q, b.part_b, nvec, st));
+        CF_TRY(launch_cg_update_d(n, k, b.st, b.part_b, nvec, b.d, b.r, nullptr, nvec, st));
+        if (cons) cons(user, b.d);
+        k++;
+        CF_CHECK_HIP(hipMemcpyAsync(b.h_flag, &b.st->done[k], 4, hipMemcpyDeviceToHost, st));
+        CF_CHECK_HIP(hipStreamSynchronize(st));
+    }
+    static thread_local CgScalars hs;
+    CF_CHECK_HIP(hipMemcpyAsync(&hs, b.st, sizeof(CgScalars), hipMemcpyDeviceToHost, st));
+    CF_CHECK_HIP(hipStreamSynchronize(st));
+    uint32_t j = 0;
+    while (j <= max_it && !hs.done[j]) j++;
+    if (j > max_it) j = max_it;
+    int reason = hs.reason[j];
+    uint32_t iters = j;
+    double delta = hs.delta[j];
+    if (reason == CF_CG_BREAKDOWN) { iters = j - 1; delta = hs.delta[j - 1]; }
+    res->iterations = int32_t(iters);
+    res->reason = reason;
+    res->rel_residual = (hs.delta0 > 0) ? float(std::sqrt(delta / hs.delta0)) : 0.f;
+    res->abs_residual = float(std::sqrt(delta));
+    return CF_OK;
+}
+
+// ---------------------------------------------------------------- leaf solid
+int cf_solid_destroy(cf_solid* s) {
+    if (!s) return CF_OK;
+    dfree(s->x); dfree(s->v); dfree(s->x0); dfree(s->bvec); dfree(s->y);
+    dfree(s->knc_mask); dfree(s->knc_target); dfree(s->scratch_vec);
+    cg_free(s->cg);
+    if (s->ev0) (void)hipEventDestroy(s->ev0);
+    if (s->ev1) (void)hipEventDestroy(s->ev1);
+    cf_cache_destroy(s->cache);
+    cf_tetset_destroy(s->set);
+    delete s;
+    return CF_OK;
+}
+
+int cf_solid_create(uint32_t N, const float* ref, uint32_t T, const uint32_t* tets, const cf_solid_params* p,
+                    cf_solid** out) {
+    if (!out || !ref || !tets || !p || N == 0 || T == 0) { set_error("cf_solid_create: invalid arguments"); return CF_ERR_INVALID_ARG; }
+    if (!(p->fixed_dt > 0) || !(p->density > 0)) { set_error("fixed_dt and density must be > 0"); return CF_ERR_INVALID_ARG; }
+    *out = nullptr;
+    cf_solid* s = new (std::nothrow) cf_solid;
+    if (!s) return CF_ERR_OUT_OF_MEMORY;
+    s->p = *p;
+    s->N = N;
+    int rc = cf_tetset_create(N, ref, T, tets, &p->fem, &s->set);
+    if (rc == CF_OK) rc = cf_cache_create(s->set, nullptr, &s->cache);
+    if (rc != CF_OK) { cf_solid_destroy(s); return rc; }
+    s->stream = s->set->stream;
+    const size_t n3 = 3 * (size_t)N;
+    if (rc == CF_OK) rc = dalloc(&s->x, n3);
+    if (rc == CF_OK) rc = dalloc(&s->v, n3);
+    if (rc == CF_OK) rc = dalloc(&s->x0, n3);
+    if (rc == CF_OK) rc = dalloc(&s->bvec, n3);
+    if (rc == CF_OK) rc = dalloc(&s->y, n3);
+    if (rc == CF_OK) rc = dalloc(&s->knc_mask, N);
+    if (rc == CF_OK) rc = dalloc(&s->knc_target, n3);
+    if (rc == CF_OK) rc = cg_alloc(s->cg, uint32_t(n3));
+    if (rc != CF_OK) { cf_solid_destroy(s); return rc; }
+    if (hipMemcpy(s->x, ref, n3 * 4, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(s->x0, ref, n3 * 4, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemset(s->v, 0, n3 * 4) != hipSuccess || hipMemset(s->knc_mask, 0, N) != hipSuccess ||
+        hipEventCreate(&s->ev0) != hipSuccess || hipEventCreate(&s->ev1) != hipSuccess) {
+        set_error("device initialisation of the solid failed");
+        cf_solid_destroy(s);
+        return CF_ERR_HIP;
+    }
+    // RebuildMass: uniform node mass rho*V/N stored as an inverse (:1591-1611,
+    // ms/ParticleSystem.h:74,80); GetMass returns 1/(1/m).
+    const float total_mass = p->density * s->set->total_volume;
+    const float mass_per_node = total_mass / float(N);
+    const float inv_mass = 1.0f / mass_per_node;
+    s->node_mass = 1.0f / inv_mass;
+    rayleigh_coeffs(p->fem.rayleigh_freq1, p->fem.rayleigh_freq2, p->fem.damping_ratio, s->rayleigh);
+    s->air_drag_per_second = -std::log(10e-6f) * p->air_drag_coeff;  // RebuildAirDrag (:1586-1589)
+    *out = s;
+    return CF_OK;
+}
+
+int cf_solid_set_knc(cf_solid* s, uint32_t n, const uint32_t* nids, const float* targets) {
+    if (!s || (n && !nids)) return CF_ERR_INVALID_ARG;
+    std::vector<uint8_t> mask(s->N, 0);
+    std::vector<float> tgt(3 * (size_t)s->N, 0.f);
+    std::vector<float> ref;
+    if (!targets) {
+        ref.resize(3 * (size_t)s->N);
+        CF_CHECK_HIP(hipMemcpy(ref.data(), s->set->dev.ref, ref.size() * 4, hipMemcpyDeviceToHost));
+    }
+    for (uint32_t i = 0; i < n; i++) {
+        const uint32_t nid = nids[i];
+        if (nid >= s->N) { set_error("KNC node %u out of range", nid); return CF_ERR_INVALID_ARG; }
+        mask[nid] = 1;
+        for (int c = 0; c < 3; c++) tgt[3 * (size_t)nid + c] = targets ? targets[3 * (size_t)i + c] : ref[3 * (size_t)nid + c];
+    }
+    CF_CHECK_HIP(hipMemcpy(s->knc_mask, mask.data(), mask.size(), hipMemcpyHostToDevice));
+    CF_CHECK_HIP(hipMemcpy(s->knc_target, tgt.data(), tgt.size() * 4, hipMemcpyHostToDevice));
+    s->num_knc = n;
+    return CF_OK;
+}
+
+int cf_solid_fixed_step(cf_solid* s, float dt, cf_step_stats* stats) {
+    if (!s || !(dt > 0)) return CF_ERR_INVALID_ARG;
+    cf_tetset* set = s->set;
+    const TetDevice& t = set->dev;
+    hipStream_t st = s->stream;
+    const uint32_t n3 = 3 * s->N;
+    CF_CHECK_HIP(hipEventRecord(s->ev0, st));
+    // BeginEvaluation(prev_x, x, v, t) (FixedStep :927)
+    CF_TRY(cf_begin_evaluation(s->cache, s->x0, s->x, s->v, s->total_time));
+    // m_vecPos0 <- x (:936-937)
+    CF_CHECK_HIP(hipMemcpyAsync(s->x0, s->x, n3 * 4, hipMemcpyDeviceToDevice, st));
+    // b = dt (M g + f_e(x)) + M v ; y = v ; KNC rows (Step_QuasiImplicitEuler_v :1209-1295)
+    CF_TRY(launch_forces(t, kModeFe, s->x, s->cache->R, set->mat, st));
+    NodeTerms nt;
+    nt.out = s->bvec; nt.y = s->y; nt.x = s->x; nt.v = s->v;
+    nt.knc_mask = s->num_knc ? s->knc_mask : nullptr;
+    nt.knc_target = s->knc_target;
+    nt.mass = s->node_mass;
+    nt.dt = dt; nt.inv_dt = 1.0f / dt;
+    nt.g[0] = s->p.gravity[0]; nt.g[1] = s->p.gravity[1]; nt.g[2] = s->p.gravity[2];
+    nt.warm_start = s->p.warm_start;
+    CF_TRY(launch_gather(t, kGatherRhs, nt, vec_blocks(s->N), st));
+    // A = (1 + dt c0) M + (dt^2 + dt c1) K  (:1300-1302)
+    const float fM = 1.0f + dt * s->rayleigh[0];
+    const float fK = dt * dt + dt * s->rayleigh[1];
+    cf_cg_result res{};
+    CF_TRY(solid_cg(s, fM, fK, &res));
+    // v = y ; x = x + dt y (:1322-1327) ; air drag (:982-984)
+    CF_TRY(launch_integrate(n3, s->x, s->v, s->y, dt, s->air_drag_per_second, st));
+    double energy = 0.0;
+    if (s->p.compute_energy) CF_TRY(cf_V(s->cache, &energy));  // V() with the new x (:977)
+    CF_TRY(cf_end_evaluation(s->cache));
+    CF_CHECK_HIP(hipEventRecord(s->ev1, st));
+    CF_CHECK_HIP(hipEventSynchronize(s->ev1));
+    s->total_time += dt;
+    if (stats) {
+        float ms = 0.f;
+        CF_CHECK_HIP(hipEventElapsedTime(&ms, s->ev0, s->ev1));
+        CF_TRY(cache_stats_fetch(s->cache));
+        stats->num_fixed_steps = 1;
+        stats->cg_iters = res.iterations;
+        stats->cg_rel_residual = res.rel_residual;
+        stats->cg_abs_residual = res.abs_residual;
+        stats->cg_reason = res.reason;
+        stats->num_degenerate = s->cache->num_degenerate;
+        stats->sum_degenerate_det_f = s->cache->sum_degenerate;
+        stats->elastic_energy = float(energy);
+        stats->min_det_f = s->cache->min_det;
+        stats->step_ms = ms;
+    }
+    return CF_OK;
+}
+
+// Step (:225-256): fixed-dt accumulator with strict '>'
+int cf_solid_step(cf_solid* s, float dt, cf_step_stats* stats) {
+    if (!s) return CF_ERR_INVALID_ARG;
+    const float scaled_dt = s->p.time_scale * dt;
+    s->acc_time += scaled_dt;
+    const float fixed = s->p.time_scale * s->p.fixed_dt;
+    cf_step_stats last{};
+    uint32_t n = 0;
+    while (s->acc_time > fixed) {
+        CF_TRY(cf_solid_fixed_step(s, fixed, &last));
+        s->acc_time -= fixed;
+        n++;
+    }
+    last.num_fixed_steps = n;
+    if (stats) *stats = last;
+    return CF_OK;
+}
+
+int cf_solid_get_state(const cf_solid* s, float* pos, float* vel) {
+    if (!s) return CF_ERR_INVALID_ARG;
+    CF_CHECK_HIP(hipStreamSynchronize(s->stream));
+    if (pos) CF_CHECK_HIP(hipMemcpy(pos, s->x, 3 * (size_t)s->N * 4, hipMemcpyDeviceToHost));
+    if (vel) CF_CHECK_HIP(hipMemcpy(vel, s->v, 3 * (size_t)s->N * 4, hipMemcpyDeviceToHost));
+    return CF_OK;
+}
+
+int cf_solid_set_state(cf_solid* s, const float* pos, const float* vel) {
+    if (!s) return CF_ERR_INVALID_ARG;
+    CF_CHECK_HIP(hipStreamSynchronize(s->stream));
+    if (pos) CF_CHECK_HIP(hipMemcpy(s->x, pos, 3 * (size_t)s->N * 4, hipMemcpyHostToDevice));
+    if (vel) CF_CHECK_HIP(hipMemcpy(s->v, vel, 3 * (size_t)s->N * 4, hipMemcpyHostToDevice));
+    return CF_OK;
+}
+
+int cf_solid_device_state(cf_solid* s, float** pos, float** vel) {
+    if (!s) return CF_ERR_INVALID_ARG;
+    if (pos) *pos = s->x;
+    if (vel) *vel = s->v;
+    return CF_OK;
+}
+
+int cf_solid_info(const cf_solid* s, uint32_t* nn, uint32_t* nt, float* mass, float rayleigh[2]) {
+    if (!s) return CF_ERR_INVALID_ARG;
+    if (nn) *nn = s->N;
+    if (nt) *nt = s->set->dev.T;
+    if (mass) *mass = s->node_mass;
+    if (rayleigh) { rayleigh[0] = s->rayleigh[0]; rayleigh[1] = s->rayleigh[1]; }
+    return CF_OK;
+}
+
+int cf_solid_assemble_rhs(cf_solid* s, float dt, float* h_b, float* h_y0) {
+    if (!s || !(dt > 0)) return CF_ERR_INVALID_ARG;
+    cf_tetset* set = s->set;
+    hipStream_t st = s->stream;
+    CF_TRY(cf_begin_evaluation(s->cache, s->x0, s->x, s->v, s->total_time));
+    CF_TRY(launch_forces(set->dev, kModeFe, s->x, s->cache->R, set->mat, st));
+    NodeTerms nt;
+    nt.out = s->bvec; nt.y = s->y; nt.x = s->x; nt.v = s->v;
+    nt.knc_mask = s->num_knc ? s->knc_mask : nullptr;
+    nt.knc_target = s->knc_target;
+    nt.mass = s->node_mass;
+    nt.dt = dt; nt.inv_dt = 1.0f / dt;
+    nt.g[0] = s->p.gravity[0]; nt.g[1] = s->p.gravity[1]; nt.g[2] = s->p.gravity[2];
+    nt.warm_start = s->p.warm_start;
+    CF_TRY(launch_gather(set->dev, kGatherRhs, nt, vec_blocks(s->N), st));
+    CF_TRY(cf_end_evaluation(s->cache));
+    CF_CHECK_HIP(hipStreamSynchronize(st));
+    if (h_b) CF_CHECK_HIP(hipMemcpy(h_b, s->bvec, 3 * (size_t)s->N * 4, hipMemcpyDeviceToHost));
+    if (h_y0) CF_CHECK_HIP(hipMemcpy(h_y0, s->y, 3 * (size_t)s->N * 4, hipMemcpyDeviceToHost));
+    return CF_OK;
+}
+
+int cf_solid_apply_A(cf_solid* s, float fM, float fK, const float* h_d, float* h_q) {
+    if (!s || !h_d || !h_q) return CF_ERR_INVALID_ARG;
+    cf_tetset* set = s->set;
+    hipStream_t st = s->stream;
+    CgBuffers& b = s->cg;
+    const size_t n3 = 3 * (size_t)s->N;
+    CF_CHECK_HIP(hipStreamSynchronize(st));
+    CF_CHECK_HIP(hipMemcpy(b.d, h_d, n3 * 4, hipMemcpyHostToDevice));
+    NodeTerms nt;
+    nt.knc_mask = s->num_knc ? s->knc_mask : nullptr;
+    nt.mass = s->node_mass;
+    nt.fM = fM; nt.fK = fK;
+    nt.partials = b.part_a;
+    nt.d = b.d; nt.out = b.q;
+    CF_TRY(launch_forces(set->dev, kModeDfx, b.d, s->cache->R, set->mat, st));
+    CF_TRY(launch_gather(set->dev, kGatherMatvec, nt, vec_blocks(s->N), st));
+    CF_CHECK_HIP(hipStreamSynchronize(st));
+    CF_CHECK_HIP(hipMemcpy(h_q, b.q, n3 * 4, hipMemcpyDeviceToHost));
+    return CF_OK;
+}
+
+int cf_solid_time_df_x(cf_solid* s, uint32_t reps, double* ms) {
+    if (!s || reps == 0 || !ms) return CF_ERR_INVALID_ARG;
+    cf_tetset* set = s->set;
+    hipStream_t st = s->stream;
+    if (!s->scratch_vec) CF_TRY(dalloc(&s->scratch_vec, 3 * (size_t)s->N));
+    NodeTerms nt;
+    nt.out = s->scratch_vec;
+    CF_CHECK_HIP(hipEventRecord(s->ev0, st));
+    for (uint32_t r = 0; r < reps; r++) {
+        CF_TRY(launch_forces(set->dev, kModeDfx, s->v, s->cache->R, set->mat, st));
+        CF_TRY(launch_gather(set->dev, kGatherAccumulate, nt, vec_blocks(s->N), st));
+    }
+    CF_CHECK_HIP(hipEventRecord(s->ev1, st));
+    CF_CHECK_HIP(hipEventSynchronize(s->ev1));
+    float t = 0.f;
+    CF_CHECK_HIP(hipEventElapsedTime(&t, s->ev0, s->ev1));
+    *ms = double(t) / reps;
+    return CF_OK;
+}
+
+int cf_set_tile_elems(uint32_t te) {
+    if (te != 0 && (te < 16 || te > 16383)) { set_error("tile_elems out of [16, 16383]"); return CF_ERR_INVALID_ARG; }
+    g_tile_elems = te;
+    return CF_OK;
+}
+int cf_get_tile_elems(uint32_t* te) {
+    if (!te) return CF_ERR_INVALID_ARG;
+    *te = g_tile_elems ? g_tile_elems : default_tile_elems();
+    return CF_OK;
+}
+
+}  // extern "C"

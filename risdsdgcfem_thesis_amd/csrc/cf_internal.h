@@ -1,0 +1,143 @@
+// Internal declarations shared by the corofem host and device sources.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstddef>
+#include <vector>
+#include "../../include/corofem.h"
+
+namespace cf {
+
+void set_error(const char* fmt, ...);
+
+#define CF_CHECK_HIP(expr)                                                                   \
+    do {                                                                                     \
+        hipError_t e_ = (expr);                                                              \
+        if (e_ != hipSuccess) {                                                              \
+            cf::set_error("%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, __LINE__); \
+            return CF_ERR_HIP;                                                               \
+        }                                                                                    \
+    } while (0)
+
+#define CF_TRY(expr)               \
+    do {                           \
+        int rc_ = (expr);          \
+        if (rc_ != CF_OK) return rc_; \
+    } while (0)
+
+// Threads per block of the element and node kernels (4 wave64s).
+constexpr int kBlock = 256;
+// Blocks of the grid-stride node/vector kernels; their FP64 partial sums are
+// reduced redundantly (and identically) by every block of the next kernel.
+constexpr int kMaxVecBlocks = 512;
+
+// Elastic constants of the reference's isotropic E matrix
+// (ms/fem/TetrahedronElement3.h:141-156): a = E/((1+nu)(1-2nu)).
+struct Material {
+    float a_k1;  // a(1-nu)   = lambda + 2 mu
+    float a_nu;  // a nu      = lambda
+    float a_k2;  // a(0.5-nu) = mu
+};
+Material material_from(float young, float poisson);
+
+// Device layout of an element set, elements in tile order (see cf_tiles.cpp).
+struct TetDevice {
+    uint32_t N = 0, T = 0;
+    float* ref = nullptr;          // 3N AoS rest positions
+    uint32_t* conn = nullptr;      // 4 planes of T global node ids
+    float* dinv = nullptr;         // 9 planes of T (row-major Dm^-1)
+    float* vol = nullptr;          // T
+    // LDS-tile node gather
+    uint32_t num_tiles = 0;
+    uint32_t max_tile_elems = 0;
+    uint32_t max_tile_nodes = 0;
+    uint32_t num_parts = 0;        // sum over tiles of tile node counts
+    uint32_t* tile_elem_ptr = nullptr;  // num_tiles+1
+    uint32_t* tile_node_ptr = nullptr;  // num_tiles+1
+    uint32_t* tile_nodes = nullptr;     // num_parts, global node ids
+    uint32_t* slot_ptr = nullptr;       // num_parts+1, into slots
+    uint16_t* slots = nullptr;          // 4T codes (local elem * 4 + vertex)
+    uint16_t* lvid = nullptr;           // 4 planes of T tile-local vertex ids
+    uint32_t* node_part_ptr = nullptr;  // N+1
+    uint32_t* node_parts = nullptr;     // num_parts
+    float* partials = nullptr;          // 3 * num_parts
+};
+
+// Host-side tile construction (cf_tiles.cpp).
+struct TileHost {
+    std::vector<uint32_t> perm;          // internal element -> caller element
+    std::vector<uint32_t> conn_planes;   // 4T
+    std::vector<uint16_t> lvid_planes;   // 4T
+    std::vector<uint32_t> tile_elem_ptr, tile_node_ptr, tile_nodes, slot_ptr;
+    std::vector<uint16_t> slots;
+    std::vector<uint32_t> node_part_ptr, node_parts;
+    uint32_t max_tile_elems = 0, max_tile_nodes = 0;
+};
+int build_tiles(uint32_t N, uint32_t T, const float* ref_pos, const uint32_t* tets, uint32_t tile_elems,
+                TileHost& out);
+uint32_t default_tile_elems();
+
+// CG scalar state in device memory, indexed by iteration so that no kernel
+// reads a slot that another block of the same launch writes.
+struct CgScalars {
+    double delta[1024 + 2];  // delta[k]: r^T r entering iteration k
+    double dq[1024 + 2];
+    int32_t done[1024 + 2];  // done[k]: iteration k does not run
+    int32_t reason[1024 + 2];
+    float alpha[1024 + 2];
+    double delta0;
+    double target;
+    double abs_sq;
+    uint32_t max_iter;
+};
+constexpr uint32_t kMaxCgIter = 1024;
+
+// ---------------------------------------------------------------- launchers
+enum ForceMode { kModeDfx = 0, kModeFe = 1 };
+enum GatherMode { kGatherAccumulate = 0, kGatherMatvec = 1, kGatherRhs = 2 };
+
+struct NodeTerms {  // per-node terms fused into the node gather
+    const float* d = nullptr;       // matvec: direction d (3N)
+    float* out = nullptr;           // accumulate: += ; matvec: q ; rhs: b
+    float* y = nullptr;             // rhs: warm start y
+    const float* x = nullptr;       // rhs: positions (for KNC velocity)
+    const float* v = nullptr;       // rhs: velocities
+    const uint8_t* knc_mask = nullptr;
+    const float* knc_target = nullptr;
+    float mass = 0.f;               // uniform node mass (3D)
+    const float* mass_array = nullptr;  // per-node masses (2D) or null
+    float fM = 0.f, fK = 0.f;
+    float dt = 0.f, inv_dt = 0.f;
+    float g[3] = {0, 0, 0};
+    int warm_start = 1;
+    double* partials = nullptr;     // matvec: per-block d.q
+};
+
+int launch_precompute(const TetDevice& t, hipStream_t s);
+// blk_count holds 2*nblk counters: [0,nblk) det F <= 0, [nblk,2nblk) DAPD R=Id fallbacks
+int launch_begin_eval(const TetDevice& t, const float* x, int rm, float thr, float* R, float* detF,
+                      uint32_t* blk_count, double* blk_sum, float* blk_min, int nblk, hipStream_t s);
+int launch_forces(const TetDevice& t, int mode, const float* in, const float* R, Material m, hipStream_t s);
+int launch_forces_skip(const TetDevice& t, int mode, const float* in, const float* R, Material m,
+                       const int32_t* skip, hipStream_t s);
+int launch_gather(const TetDevice& t, int mode, const NodeTerms& nt, int nblk, hipStream_t s);
+int launch_gather_skip(const TetDevice& t, int mode, const NodeTerms& nt, int nblk, const int32_t* skip,
+                       hipStream_t s);
+int launch_energy(const TetDevice& t, const float* x, const float* R, Material m, double* blk, int nblk,
+                  hipStream_t s);
+int vec_blocks(uint32_t n);
+
+// CG kernels over n = 3N floats
+int launch_cg_init(uint32_t n, const float* q, const float* b, float* r, float* d, const uint8_t* mask,
+                   double* rr_part, int nblk, hipStream_t s);
+int launch_cg_start(CgScalars* st, const double* rr_part, int nblk_prev, float rel, float abs_eps,
+                    uint32_t max_iter, hipStream_t s);
+int launch_cg_dot(uint32_t n, const float* a, const float* b, double* part, int nblk, hipStream_t s);
+int launch_cg_update_xr(uint32_t n, uint32_t k, CgScalars* st, const double* dq_part, int nblk_dq, float* x,
+                        float* r, const float* d, const float* q, double* rr_part, int nblk, hipStream_t s);
+int launch_cg_update_d(uint32_t n, uint32_t k, CgScalars* st, const double* rr_part, int nblk_rr, float* d,
+                       const float* r, const uint8_t* mask, int nblk, hipStream_t s);
+int launch_integrate(uint32_t n, float* x, float* v, const float* y, float dt, float drag, hipStream_t s);
+int launch_zero_mask(uint32_t n_nodes, float* w, const uint8_t* mask, hipStream_t s);
+
+}  // namespace cf

@@ -1,0 +1,144 @@
+// Host construction of the LDS tiles used by the atomics-free node gather.
+//
+// Elements are sorted along a Morton (Z-order) curve of their rest centroids
+// and cut into tiles of at most `tile_elems` elements and kMaxTileNodes
+// distinct nodes. For every tile we record
+//   - its distinct nodes (ascending global id) -> one partial per (tile, node)
+//   - per element, the tile-local index of each of its 4 vertices (uint16)
+//   - per tile node, the (local element, vertex) slots that touch it, in
+//     ascending element order (the order the reference's serial scatter would
+//     accumulate them), as uint16 codes elem*4+vertex
+// and, per global node, the list of its partials in ascending tile order.
+// All of this is integer work done once per mesh.
+#include "cf_internal.h"
+#include <algorithm>
+#include <numeric>
+#include <cstring>
+
+namespace cf {
+
+namespace {
+constexpr uint32_t kMaxTileNodes = 1024;
+// LDS bytes of one tile in k_tile_forces: node vectors and rest positions
+// (24 B), per-element nodal forces (36 B), slot offsets (4 B per node) and
+// slot codes (8 B per element).
+constexpr uint32_t kMaxTileLds = 64 * 1024 - 512;
+inline uint32_t tile_lds(uint32_t ne, uint32_t nn) { return 28 * nn + 44 * ne + 4; }
+
+inline uint64_t spread3(uint64_t v) {  // 21 bits -> every third bit
+    v &= 0x1fffff;
+    v = (v | v << 32) & 0x1f00000000ffffULL;
+    v = (v | v << 16) & 0x1f0000ff0000ffULL;
+    v = (v | v << 8) & 0x100f00f00f00f00fULL;
+    v = (v | v << 4) & 0x10c30c30c30c30c3ULL;
+    v = (v | v << 2) & 0x1249249249249249ULL;
+    return v;
+}
+}  // namespace
+
+uint32_t default_tile_elems() { return 640; }
+
+int build_tiles(uint32_t N, uint32_t T, const float* ref, const uint32_t* tets, uint32_t tile_elems,
+                TileHost& out) {
+    if (tile_elems == 0) tile_elems = default_tile_elems();
+    if (tile_elems > 16383) tile_elems = 16383;  // slot codes are uint16 (elem*4+vertex)
+    // ---- Morton order of centroids
+    float lo[3] = {3e38f, 3e38f, 3e38f}, hi[3] = {-3e38f, -3e38f, -3e38f};
+    for (uint32_t n = 0; n < N; n++)
+        for (int c = 0; c < 3; c++) {
+            lo[c] = std::min(lo[c], ref[3 * (size_t)n + c]);
+            hi[c] = std::max(hi[c], ref[3 * (size_t)n + c]);
+        }
+    float ext = std::max(hi[0] - lo[0], std::max(hi[1] - lo[1], hi[2] - lo[2]));
+    if (!(ext > 0)) ext = 1.0f;
+    const double scale = double((1u << 21) - 1) / double(ext);
+    std::vector<uint64_t> key(T);
+    for (uint32_t e = 0; e < T; e++) {
+        double c[3] = {0, 0, 0};
+        for (int j = 0; j < 4; j++)
+            for (int k = 0; k < 3; k++) c[k] += ref[3 * (size_t)tets[4 * (size_t)e + j] + k];
+        uint64_t q[3];
+        for (int k = 0; k < 3; k++) {
+            double v = (c[k] * 0.25 - lo[k]) * scale;
+            if (v < 0) v = 0;
+            if (v > double((1u << 21) - 1)) v = double((1u << 21) - 1);
+            q[k] = uint64_t(v);
+        }
+        key[e] = spread3(q[0]) | (spread3(q[1]) << 1) | (spread3(q[2]) << 2);
+    }
+    out.perm.resize(T);
+    std::iota(out.perm.begin(), out.perm.end(), 0u);
+    std::stable_sort(out.perm.begin(), out.perm.end(), [&](uint32_t a, uint32_t b) { return key[a] < key[b]; });
+
+    // ---- cut into tiles
+    out.tile_elem_ptr.assign(1, 0);
+    out.tile_node_ptr.assign(1, 0);
+    out.tile_nodes.clear();
+    out.slot_ptr.assign(1, 0);
+    out.slots.clear();
+    out.slots.reserve(4 * (size_t)T);
+    out.conn_planes.assign(4 * (size_t)T, 0);
+    out.lvid_planes.assign(4 * (size_t)T, 0);
+    out.max_tile_elems = 0;
+    out.max_tile_nodes = 0;
+    std::vector<int32_t> local(N, -1);
+    std::vector<uint32_t> nodes;  // distinct nodes of the current tile
+    uint32_t e = 0;
+    while (e < T) {
+        const uint32_t e0 = e;
+        nodes.clear();
+        while (e < T && e - e0 < tile_elems) {
+            const uint32_t* t = &tets[4 * (size_t)out.perm[e]];
+            uint32_t fresh = 0;
+            for (int j = 0; j < 4; j++) fresh += (local[t[j]] < 0) ? 1u : 0u;
+            const uint32_t nn1 = uint32_t(nodes.size()) + fresh;
+            if (nn1 > kMaxTileNodes || tile_lds(e - e0 + 1, nn1) > kMaxTileLds) break;
+            for (int j = 0; j < 4; j++)
+                if (local[t[j]] < 0) { local[t[j]] = 0; nodes.push_back(t[j]); }
+            e++;
+        }
+        const uint32_t ne = e - e0;
+        std::sort(nodes.begin(), nodes.end());
+        for (uint32_t l = 0; l < nodes.size(); l++) local[nodes[l]] = int32_t(l);
+        // per tile node slot lists in ascending local element order
+        std::vector<uint32_t> cnt(nodes.size() + 1, 0);
+        for (uint32_t i = 0; i < ne; i++) {
+            const uint32_t* t = &tets[4 * (size_t)out.perm[e0 + i]];
+            for (int j = 0; j < 4; j++) cnt[local[t[j]] + 1]++;
+        }
+        for (size_t l = 0; l < nodes.size(); l++) cnt[l + 1] += cnt[l];
+        const size_t sbase = out.slots.size();
+        out.slots.resize(sbase + 4 * (size_t)ne);
+        std::vector<uint32_t> fill(cnt.begin(), cnt.end() - 1);
+        for (uint32_t i = 0; i < ne; i++) {
+            const size_t ie = e0 + i;
+            const uint32_t* t = &tets[4 * (size_t)out.perm[ie]];
+            for (int j = 0; j < 4; j++) {
+                const uint32_t l = uint32_t(local[t[j]]);
+                out.slots[sbase + fill[l]++] = uint16_t(i * 4 + j);
+                out.conn_planes[j * (size_t)T + ie] = t[j];
+                out.lvid_planes[j * (size_t)T + ie] = uint16_t(l);
+            }
+        }
+        for (size_t l = 0; l < nodes.size(); l++) {
+            out.tile_nodes.push_back(nodes[l]);
+            out.slot_ptr.push_back(uint32_t(sbase + cnt[l + 1]));
+            local[nodes[l]] = -1;
+        }
+        out.tile_elem_ptr.push_back(e);
+        out.tile_node_ptr.push_back(uint32_t(out.tile_nodes.size()));
+        out.max_tile_elems = std::max(out.max_tile_elems, ne);
+        out.max_tile_nodes = std::max(out.max_tile_nodes, uint32_t(nodes.size()));
+    }
+    // ---- per global node: its partials in ascending tile order
+    const uint32_t P = uint32_t(out.tile_nodes.size());
+    out.node_part_ptr.assign((size_t)N + 1, 0);
+    for (uint32_t p = 0; p < P; p++) out.node_part_ptr[out.tile_nodes[p] + 1]++;
+    for (uint32_t n = 0; n < N; n++) out.node_part_ptr[n + 1] += out.node_part_ptr[n];
+    out.node_parts.assign(P, 0);
+    std::vector<uint32_t> fill(out.node_part_ptr.begin(), out.node_part_ptr.end() - 1);
+    for (uint32_t p = 0; p < P; p++) out.node_parts[fill[out.tile_nodes[p]]++] = p;
+    return CF_OK;
+}
+
+}  // namespace cf

@@ -1,0 +1,332 @@
+"""Python mirror of the reference's element-set / solver / leaf interfaces,
+over the corofem C-ABI.
+
+  TetrahedronElementSet3  <-> ms::fem::EditableTetrahedronElementSet3 + its
+                              IForce3 cache (ms/fem/TetrahedronElementSet3.h:29-261)
+  ILSS_CG                 <-> ns::ILSS_CG (ns/ILSS_CG.h:19-151)
+  LeafSolid3D             <-> ds::LeafDSH_Solid3D_FEM (ds/dsh/LeafDSH_Solid3D_FEM.h:68-81)
+
+Node vectors are flat float32 arrays of 3N values. Device vectors are
+DeviceArray objects (memory owned by libcorofem's allocator).
+"""
+import ctypes as C
+import numpy as np
+
+from ._capi import (lib, check, FemParams, SolidParams, StepStats, CgResult, AV_FN, CONS_FN,
+                    RM_MSVD, MM_C_WRP, DM_TRUNCATED)
+
+_f = C.POINTER(C.c_float)
+_u = C.POINTER(C.c_uint32)
+
+
+def _fp(a):
+    return a.ctypes.data_as(_f)
+
+
+def _up(a):
+    return a.ctypes.data_as(_u)
+
+
+def device_count():
+    n = C.c_int(0)
+    rc = lib.cf_device_count(C.byref(n))
+    return n.value if rc == 0 else 0
+
+
+def synchronize():
+    check(lib.cf_device_synchronize(), "cf_device_synchronize")
+
+
+class DeviceArray:
+    """A float32/uint32 buffer in device memory."""
+
+    def __init__(self, n, dtype=np.float32):
+        self.n = int(n)
+        self.dtype = np.dtype(dtype)
+        self.ptr = C.c_void_p()
+        check(lib.cf_device_alloc(max(1, self.n) * self.dtype.itemsize, C.byref(self.ptr)), "cf_device_alloc")
+
+    @classmethod
+    def from_host(cls, a, dtype=np.float32):
+        a = np.ascontiguousarray(a, dtype)
+        d = cls(a.size, dtype)
+        d.upload(a)
+        return d
+
+    @property
+    def nbytes(self):
+        return self.n * self.dtype.itemsize
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, self.dtype)
+        assert a.size == self.n
+        check(lib.cf_memcpy_htod(self.ptr, a.ctypes.data_as(C.c_void_p), self.nbytes), "cf_memcpy_htod")
+
+    def numpy(self):
+        out = np.empty(self.n, self.dtype)
+        check(lib.cf_memcpy_dtoh(out.ctypes.data_as(C.c_void_p), self.ptr, self.nbytes), "cf_memcpy_dtoh")
+        return out
+
+    def zero(self):
+        check(lib.cf_memset_device(self.ptr, 0, self.nbytes), "cf_memset_device")
+
+    def free(self):
+        if self.ptr:
+            lib.cf_device_free(self.ptr)
+            self.ptr = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def mesh_box(half_sizes, counts):
+    """Make_TetSolidShape3_Box: counts = vertices per axis (cubes + 1)."""
+    nx, ny, nz = (int(c) for c in counts)
+    nn, nt = C.c_uint32(), C.c_uint32()
+    check(lib.cf_mesh_box_counts(nx, ny, nz, C.byref(nn), C.byref(nt)), "cf_mesh_box_counts")
+    pos = np.empty(3 * nn.value, np.float32)
+    tets = np.empty(4 * nt.value, np.uint32)
+    h = np.asarray(half_sizes, np.float32)
+    check(lib.cf_mesh_box(_fp(h), nx, ny, nz, _fp(pos), _up(tets)), "cf_mesh_box")
+    return pos, tets
+
+
+def mesh_box2(half_sizes, counts):
+    """Make_MeshSolidShape2_Box: counts = vertices per axis."""
+    nx, ny = (int(c) for c in counts)
+    nn, nt = C.c_uint32(), C.c_uint32()
+    check(lib.cf_mesh_box2_counts(nx, ny, C.byref(nn), C.byref(nt)), "cf_mesh_box2_counts")
+    pos = np.empty(2 * nn.value, np.float32)
+    tris = np.empty(3 * nt.value, np.uint32)
+    h = np.asarray(half_sizes, np.float32)
+    check(lib.cf_mesh_box2(_fp(h), nx, ny, _fp(pos), _up(tris)), "cf_mesh_box2")
+    return pos, tris
+
+
+def default_fem_params(**kw):
+    p = FemParams()
+    lib.cf_default_fem_params(C.byref(p))
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+def default_solid_params(fem=None, **kw):
+    p = SolidParams()
+    lib.cf_default_solid_params(C.byref(p))
+    if fem is not None:
+        p.fem = fem
+    for k, v in kw.items():
+        if k == "gravity":
+            for i in range(3):
+                p.gravity[i] = v[i]
+        elif k in ("young_modulus", "poisson_ratio", "damping_ratio", "rm", "degenerate_threshold_det_f"):
+            setattr(p.fem, k, v)
+        else:
+            setattr(p, k, v)
+    return p
+
+
+def set_tile_elems(n):
+    check(lib.cf_set_tile_elems(int(n)), "cf_set_tile_elems")
+
+
+def get_tile_elems():
+    v = C.c_uint32()
+    check(lib.cf_get_tile_elems(C.byref(v)), "cf_get_tile_elems")
+    return v.value
+
+
+class TetrahedronElementSet3:
+    """Element set + IForce3 cache. Vectors passed to the operator methods are
+    DeviceArray objects of 3N floats; outputs accumulate (+=)."""
+
+    def __init__(self, ref_pos, tets, params=None, inv_mass=None):
+        self.ref_pos = np.ascontiguousarray(ref_pos, np.float32)
+        self.tets = np.ascontiguousarray(tets, np.uint32)
+        self.N = self.ref_pos.size // 3
+        self.T = self.tets.size // 4
+        self.params = params if params is not None else default_fem_params()
+        self.h = C.c_void_p()
+        check(lib.cf_tetset_create(self.N, _fp(self.ref_pos), self.T, _up(self.tets), C.byref(self.params),
+                                   C.byref(self.h)), "cf_tetset_create")
+        self.c = C.c_void_p()
+        im = None if inv_mass is None else np.ascontiguousarray(inv_mass, np.float32)
+        check(lib.cf_cache_create(self.h, None if im is None else _fp(im), C.byref(self.c)), "cf_cache_create")
+
+    def close(self):
+        if getattr(self, "c", None):
+            lib.cf_cache_destroy(self.c)
+            self.c = None
+        if getattr(self, "h", None):
+            lib.cf_tetset_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def info(self):
+        nn, nt, v = C.c_uint32(), C.c_uint32(), C.c_float()
+        check(lib.cf_tetset_info(self.h, C.byref(nn), C.byref(nt), C.byref(v)), "cf_tetset_info")
+        return nn.value, nt.value, v.value
+
+    def total_volume(self):
+        return self.info()[2]
+
+    def elements(self):
+        dinv = np.empty(9 * self.T, np.float32)
+        vol = np.empty(self.T, np.float32)
+        check(lib.cf_tetset_get_elements(self.h, _fp(dinv), _fp(vol)), "cf_tetset_get_elements")
+        return dinv.reshape(self.T, 3, 3), vol
+
+    def set_params(self, params):
+        self.params = params
+        check(lib.cf_tetset_set_params(self.h, C.byref(params)), "cf_tetset_set_params")
+
+    def begin_evaluation(self, x, prev_x=None, v=None, t=0.0):
+        self._x = x
+        check(lib.cf_begin_evaluation(self.c, prev_x.ptr if prev_x else None, x.ptr, v.ptr if v else None,
+                                      float(t)), "cf_begin_evaluation")
+
+    def f_e(self, f):
+        check(lib.cf_f_e(self.c, f.ptr), "cf_f_e")
+
+    def df_x(self, dx, df):
+        check(lib.cf_df_x(self.c, dx.ptr, df.ptr), "cf_df_x")
+
+    def V(self):
+        e = C.c_double()
+        check(lib.cf_V(self.c, C.byref(e)), "cf_V")
+        return e.value
+
+    def end_evaluation(self):
+        check(lib.cf_end_evaluation(self.c), "cf_end_evaluation")
+
+    def rotations(self):
+        R = np.empty(9 * self.T, np.float32)
+        d = np.empty(self.T, np.float32)
+        check(lib.cf_cache_get_rotations(self.c, _fp(R), _fp(d)), "cf_cache_get_rotations")
+        return R.reshape(self.T, 3, 3), d
+
+    def stats(self):
+        n, s, m = C.c_uint32(), C.c_float(), C.c_float()
+        check(lib.cf_cache_stats(self.c, C.byref(n), C.byref(s), C.byref(m)), "cf_cache_stats")
+        return n.value, s.value, m.value
+
+    def synchronize(self):
+        check(lib.cf_cache_synchronize(self.c), "cf_cache_synchronize")
+
+
+class ILSS_CG:
+    """IIterativeLinearSystemSolver::Solve(D_Av, D_ApplyConstraints, x, b, rel, abs, max)
+    with device vectors. `av(q_ptr, v_ptr)` and `apply_constraints(v_ptr)` receive raw
+    device pointers (ctypes float pointers) and must enqueue work on `self.stream`."""
+
+    def __init__(self, n):
+        self.n = int(n)
+        self.h = C.c_void_p()
+        check(lib.cf_cg_create(self.n, C.byref(self.h)), "cf_cg_create")
+        self.stream = lib.cf_cg_stream(self.h)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib.cf_cg_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def solve(self, av, apply_constraints, x, b, rel=1e-3, abs_eps=1e-6, max_iterations=20):
+        cav = AV_FN(lambda user, q, v: av(q, v))
+        ccons = CONS_FN(lambda user, v: apply_constraints(v)) if apply_constraints else CONS_FN(0)
+        res = CgResult()
+        check(lib.cf_cg_solve(self.h, cav, ccons, None, x.ptr, b.ptr, float(rel), float(abs_eps),
+                              int(max_iterations), C.byref(res)), "cf_cg_solve")
+        return res
+
+
+class LeafSolid3D:
+    """LeafDSH_Solid3D_FEM with device-resident state (QIE-v integrator)."""
+
+    def __init__(self, ref_pos, tets, params=None, knc=None, knc_targets=None):
+        self.ref_pos = np.ascontiguousarray(ref_pos, np.float32)
+        self.tets = np.ascontiguousarray(tets, np.uint32)
+        self.N = self.ref_pos.size // 3
+        self.T = self.tets.size // 4
+        self.params = params if params is not None else default_solid_params()
+        self.h = C.c_void_p()
+        check(lib.cf_solid_create(self.N, _fp(self.ref_pos), self.T, _up(self.tets), C.byref(self.params),
+                                  C.byref(self.h)), "cf_solid_create")
+        if knc is not None:
+            self.set_knc(knc, knc_targets)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib.cf_solid_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_knc(self, nids, targets=None):
+        k = np.ascontiguousarray(nids, np.uint32)
+        t = None if targets is None else np.ascontiguousarray(targets, np.float32)
+        check(lib.cf_solid_set_knc(self.h, k.size, _up(k), None if t is None else _fp(t)), "cf_solid_set_knc")
+
+    def fixed_step(self, dt=None):
+        st = StepStats()
+        check(lib.cf_solid_fixed_step(self.h, float(self.params.fixed_dt if dt is None else dt), C.byref(st)),
+              "cf_solid_fixed_step")
+        return st
+
+    def step(self, dt):
+        st = StepStats()
+        check(lib.cf_solid_step(self.h, float(dt), C.byref(st)), "cf_solid_step")
+        return st
+
+    def state(self):
+        x = np.empty(3 * self.N, np.float32)
+        v = np.empty(3 * self.N, np.float32)
+        check(lib.cf_solid_get_state(self.h, _fp(x), _fp(v)), "cf_solid_get_state")
+        return x, v
+
+    def set_state(self, x, v):
+        x = np.ascontiguousarray(x, np.float32)
+        v = np.ascontiguousarray(v, np.float32)
+        check(lib.cf_solid_set_state(self.h, _fp(x), _fp(v)), "cf_solid_set_state")
+
+    def info(self):
+        nn, nt, m = C.c_uint32(), C.c_uint32(), C.c_float()
+        r = np.zeros(2, np.float32)
+        check(lib.cf_solid_info(self.h, C.byref(nn), C.byref(nt), C.byref(m), _fp(r)), "cf_solid_info")
+        return {"nodes": nn.value, "tets": nt.value, "node_mass": m.value, "rayleigh": r}
+
+    def assemble_rhs(self, dt=None):
+        b = np.empty(3 * self.N, np.float32)
+        y = np.empty(3 * self.N, np.float32)
+        check(lib.cf_solid_assemble_rhs(self.h, float(self.params.fixed_dt if dt is None else dt), _fp(b), _fp(y)),
+              "cf_solid_assemble_rhs")
+        return b, y
+
+    def apply_A(self, d, factor_m, factor_k):
+        d = np.ascontiguousarray(d, np.float32)
+        q = np.empty(3 * self.N, np.float32)
+        check(lib.cf_solid_apply_A(self.h, float(factor_m), float(factor_k), _fp(d), _fp(q)), "cf_solid_apply_A")
+        return q
+
+    def time_df_x(self, reps=20):
+        ms = C.c_double()
+        check(lib.cf_solid_time_df_x(self.h, int(reps), C.byref(ms)), "cf_solid_time_df_x")
+        return ms.value
