@@ -1,0 +1,172 @@
+"""Benchmark: the corotational tet FEM implicit step on one MI355X.
+
+Workload (one "step" = one QIE-v implicit step, BeginEvaluation + f_e + CG +
+integrate + energy): the single large mesh, plain GPU drop-in -- a 64x64x256
+cube grid (65x65x257 vertices, 5,242,880 tets / 1,085,825 nodes), edge 0.01 m,
+E=1e5 Pa, nu=0.3, rho=1000, dt=1/60 s, gravity -9.81 y, CG rel 1e-4 / abs 1e-6 /
+max 50, polar-decomposition rotations, FP32 state with FP64 CG reductions. The
+initial pose is one seeded rigid rotation (uniform on SO(3)) times a stretch
+with diagonal entries in [0.7, 1.3] plus 1% edge-length per-node jitter; the
+z-min face is held at its initial pose.
+
+Prints ONE JSON line. value = tet-matvecs per second over the whole timed run
+(every K*x of every CG iteration of every step, divided by the wall time of the
+timed steps). The roofline object describes one K*x (element tile kernel +
+node gather) timed with HIP events on the solid's stream right after the timed
+steps; cpu_baseline times the oracle's K*x (stored Ke blocks, serial scatter,
+one core) on a bounded shard of the same mesh.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "K·x Gtets/s (%HBM peak); steps/s batched vs plain at 1e-4 residual; p99 ms"
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def uniform_so3(rng):
+    u1, u2, u3 = rng.random(3)
+    x, y, z, w = (np.sqrt(1 - u1) * np.sin(2 * np.pi * u2), np.sqrt(1 - u1) * np.cos(2 * np.pi * u2),
+                  np.sqrt(u1) * np.sin(2 * np.pi * u3), np.sqrt(u1) * np.cos(2 * np.pi * u3))
+    return np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)],
+                     [2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)],
+                     [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)]])
+
+
+def config2_state(pos, seed=2, edge=0.01):
+    """x0 = Q diag(s) X + jitter (one global rotation/stretch draw)."""
+    rng = np.random.Generator(np.random.Philox(seed))
+    Q = uniform_so3(rng)
+    s = rng.uniform(0.7, 1.3, 3)
+    X = pos.reshape(-1, 3).astype(np.float64)
+    x = X @ (Q @ np.diag(s)).T
+    x += rng.uniform(-0.01 * edge, 0.01 * edge, x.shape)
+    return x.astype(np.float32).ravel()
+
+
+def load_traffic():
+    p = os.path.join(ROOT, "profiles", "kx_traffic.json")
+    if not os.path.exists(p):
+        return None
+    with open(p) as f:
+        return json.load(f)
+
+
+def cpu_baseline(budget_s=12.0):
+    """The oracle's K*x (reference algorithm: stored 3x3 Ke blocks, serial
+    scatter-add, FP32, one core) on a 64x64x64-cube shard of the mesh."""
+    from oracle import orc
+    pos, tets = orc.mesh_box((0.32, 0.32, 0.32), (65, 65, 65))
+    T = tets.size // 4
+    ts = orc.TetSet(pos, tets, 1e5, 0.3)
+    x = config2_state(pos)
+    ts.begin_evaluation(x, rm=orc.RM_DAPD)
+    d = np.random.default_rng(1).standard_normal(pos.size).astype(np.float32)
+    df = np.zeros_like(d)
+    ts.df_x(d, df)  # warm
+    n = 0
+    t0 = time.perf_counter()
+    while True:
+        df[:] = 0
+        ts.df_x(d, df)
+        n += 1
+        if time.perf_counter() - t0 > budget_s or n >= 200:
+            break
+    dt = time.perf_counter() - t0
+    return {"value": T * n / dt / 1e9, "unit": "Gtet-matvec/s", "cores": 1, "kind": "port",
+            "sample": f"{n} K*x of the oracle on a 64x64x64-cube shard ({T} tets) of the same mesh, "
+                      f"{dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--kx-reps", type=int, default=50)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only: the workload is one GPU's mesh; --gpus {args.gpus} is not "
+              "supported (see the sharding design in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import risdsdgcfem_thesis_amd as cf
+    if cf.device_count() < 1:
+        print("no GPU visible", file=sys.stderr)
+        sys.exit(1)
+
+    counts = (65, 65, 257)
+    half = (0.32, 0.32, 1.28)
+    pos, tets = cf.mesh_box(half, counts)
+    N, T = pos.size // 3, tets.size // 4
+    x0 = config2_state(pos)
+    knc = np.nonzero(np.isclose(pos[2::3], -half[2]))[0].astype(np.uint32)
+    params = cf.default_solid_params(fixed_dt=1.0 / 60.0, gravity=(0.0, -9.81, 0.0), young_modulus=1e5,
+                                     poisson_ratio=0.3, damping_ratio=0.0, rm=cf.RM_DAPD, cg_rel_epsilon=1e-4,
+                                     cg_abs_epsilon=1e-6, cg_max_iter=50, density=1000.0)
+    log(f"[bench] mesh {N} nodes {T} tets; building solid")
+    t_build = time.perf_counter()
+    solid = cf.LeafSolid3D(pos, tets, params)
+    solid.set_state(x0, np.zeros_like(x0))
+    solid.set_knc(knc, x0.reshape(-1, 3)[knc].ravel())
+    log(f"[bench] setup {time.perf_counter() - t_build:.2f} s; warmup {args.warmup} steps")
+    for _ in range(args.warmup):
+        solid.fixed_step()
+    cf.synchronize()
+    log(f"[bench] timing {args.steps} steps")
+    iters = []
+    step_ms = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        st = solid.fixed_step()
+        iters.append(st.cg_iters)
+        step_ms.append(st.step_ms)
+    cf.synchronize()
+    wall = time.perf_counter() - t0
+    matvecs = sum(i + 1 for i in iters)
+    value = T * matvecs / wall / 1e9
+    log(f"[bench] {wall * 1e3 / args.steps:.2f} ms/step, cg iters {iters}")
+
+    ms_tile, ms_gather = solid.time_kx(args.kx_reps)
+    ms_kx = ms_tile + ms_gather
+    b_alg = 92 * T + 24 * N
+    achieved = b_alg / (ms_kx * 1e-3) / 1e9
+    traffic = load_traffic()
+    roofline = {
+        "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "traffic": traffic.get("bytes_per_kx") if traffic else None,
+        "kernels": {"k_tile_forces_ms": round(ms_tile, 5), "k_gather_ms": round(ms_gather, 5)},
+        "b_alg_bytes": b_alg, "kx_gtets": round(T / (ms_kx * 1e-3) / 1e9, 2),
+    }
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("[bench] cpu baseline (oracle K*x, one core)")
+        cpu = cpu_baseline()
+    out = {
+        "metric": METRIC, "value": round(value, 3), "unit": "Gtet-matvec/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(wall * 1e3 / args.steps, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded cube-grid mesh, seeded rotation+stretch initial pose)",
+        "config": {"workload": "single 64x64x256 cube-grid mesh, plain GPU drop-in (QIE-v, CG rel 1e-4 max 50)",
+                   "tets": T, "nodes": N, "rotations": "polar (DAPD)", "parallelism": "single GPU"},
+        "steps_per_s": round(args.steps / wall, 3), "cg_iters_mean": round(float(np.mean(iters)), 2),
+        "roofline": roofline, "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -220,6 +220,10 @@ int cf_solid_assemble_rhs(cf_solid* s, float dt, float* h_b, float* h_y0);
 int cf_solid_apply_A(cf_solid* s, float factor_m, float factor_k, const float* h_d, float* h_q);
 /* Time one K*x (df_x) on the solid's current rotations: mean device ms over reps. */
 int cf_solid_time_df_x(cf_solid* s, uint32_t reps, double* ms_per_call);
+/* Mean device ms of the two kernels of one K*x, each timed with HIP events on
+   the solid's stream over reps back-to-back launches: the LDS-tile element
+   kernel and the node gather (matvec form with the mass term and d.q). */
+int cf_solid_time_kx(cf_solid* s, uint32_t reps, double* ms_tile, double* ms_gather);
 
 /* ---------------------------------------------------------------- tuning */
 /* Elements per LDS tile of the node-gather K*x (0 = library default). Takes

@@ -105,6 +105,7 @@ SIGNATURES = {
     "cf_solid_assemble_rhs": (C.c_int, [_vp, C.c_float, _f, _f]),
     "cf_solid_apply_A": (C.c_int, [_vp, C.c_float, C.c_float, _f, _f]),
     "cf_solid_time_df_x": (C.c_int, [_vp, C.c_uint32, C.POINTER(C.c_double)]),
+    "cf_solid_time_kx": (C.c_int, [_vp, C.c_uint32, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "cf_set_tile_elems": (C.c_int, [C.c_uint32]),
     "cf_get_tile_elems": (C.c_int, [_u]),
 }

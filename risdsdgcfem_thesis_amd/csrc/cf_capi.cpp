@@ -996,6 +996,35 @@ int cf_solid_time_df_x(cf_solid* s, uint32_t reps, double* ms) {
     return CF_OK;
 }
 
+int cf_solid_time_kx(cf_solid* s, uint32_t reps, double* ms_tile, double* ms_gather) {
+    if (!s || reps == 0 || !ms_tile || !ms_gather) return CF_ERR_INVALID_ARG;
+    cf_tetset* set = s->set;
+    hipStream_t st = s->stream;
+    CgBuffers& b = s->cg;
+    CF_CHECK_HIP(hipMemcpyAsync(b.d, s->v, 3 * (size_t)s->N * 4, hipMemcpyDeviceToDevice, st));
+    NodeTerms nt;
+    nt.knc_mask = s->num_knc ? s->knc_mask : nullptr;
+    nt.mass = s->node_mass;
+    nt.fM = 1.0f; nt.fK = s->p.fixed_dt * s->p.fixed_dt;
+    nt.partials = b.part_a;
+    nt.d = b.d; nt.out = b.q;
+    const int nnode = vec_blocks(s->N);
+    float t = 0.f;
+    CF_CHECK_HIP(hipEventRecord(s->ev0, st));
+    for (uint32_t r = 0; r < reps; r++) CF_TRY(launch_forces(set->dev, kModeDfx, b.d, s->cache->R, set->mat, st));
+    CF_CHECK_HIP(hipEventRecord(s->ev1, st));
+    CF_CHECK_HIP(hipEventSynchronize(s->ev1));
+    CF_CHECK_HIP(hipEventElapsedTime(&t, s->ev0, s->ev1));
+    *ms_tile = double(t) / reps;
+    CF_CHECK_HIP(hipEventRecord(s->ev0, st));
+    for (uint32_t r = 0; r < reps; r++) CF_TRY(launch_gather(set->dev, kGatherMatvec, nt, nnode, st));
+    CF_CHECK_HIP(hipEventRecord(s->ev1, st));
+    CF_CHECK_HIP(hipEventSynchronize(s->ev1));
+    CF_CHECK_HIP(hipEventElapsedTime(&t, s->ev0, s->ev1));
+    *ms_gather = double(t) / reps;
+    return CF_OK;
+}
+
 int cf_set_tile_elems(uint32_t te) {
     if (te != 0 && (te < 16 || te > 16383)) { set_error("tile_elems out of [16, 16383]"); return CF_ERR_INVALID_ARG; }
     g_tile_elems = te;

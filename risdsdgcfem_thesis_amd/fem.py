@@ -326,6 +326,12 @@ class LeafSolid3D:
         check(lib.cf_solid_apply_A(self.h, float(factor_m), float(factor_k), _fp(d), _fp(q)), "cf_solid_apply_A")
         return q
 
+    def time_kx(self, reps=20):
+        """(ms per tile-kernel launch, ms per gather launch) of one K*x."""
+        a, b = C.c_double(), C.c_double()
+        check(lib.cf_solid_time_kx(self.h, int(reps), C.byref(a), C.byref(b)), "cf_solid_time_kx")
+        return a.value, b.value
+
     def time_df_x(self, reps=20):
         ms = C.c_double()
         check(lib.cf_solid_time_df_x(self.h, int(reps), C.byref(ms)), "cf_solid_time_df_x")
