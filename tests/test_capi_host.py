@@ -1,0 +1,91 @@
+"""CPU: the C-ABI library loads, exports every entry point that
+include/corofem.h declares, and its host-only parts (mesh input, defaults,
+argument and no-device error paths) behave like the reference's."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+from conftest import ROOT
+
+
+def _declared():
+    with open(os.path.join(ROOT, "include", "corofem.h")) as f:
+        text = f.read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = set(re.findall(r"\b(cf_[a-z0-9_]+)\s*\(", text))
+    return sorted(n for n in names if not n.endswith("_fn"))
+
+
+def test_library_exports_every_declared_symbol(cf):
+    lib = C.CDLL(cf.LIB_PATH)
+    missing = [n for n in _declared() if not hasattr(lib, n)]
+    assert not missing, missing
+    assert len(_declared()) >= 50
+
+
+def test_python_signatures_cover_header(cf):
+    from risdsdgcfem_thesis_amd._capi import SIGNATURES
+    missing = [n for n in _declared() if n not in SIGNATURES]
+    assert not missing, missing
+
+
+def test_defaults_mirror_reference(cf):
+    fem = cf.default_fem_params()
+    # ms::fem::Params() (ms/fem/Params.h:74-98)
+    assert (fem.young_modulus, fem.poisson_ratio, fem.damping_ratio) == (1000.0, 0.25, 1.0)
+    assert np.isclose(fem.rayleigh_freq1, 0.1) and fem.rayleigh_freq2 == 10000.0
+    assert np.isclose(fem.degenerate_threshold_det_f, 0.1)
+    assert (fem.mm, fem.rm, fem.dm) == (cf.MM_C_WRP, cf.RM_MSVD, cf.DM_TRUNCATED)
+    sp = cf.default_solid_params()
+    # LeafDSH_Solid3D_FEM::Params() (ds/dsh/LeafDSH_Solid3D_FEM.h:203-274)
+    assert np.isclose(sp.fixed_dt, 0.001) and sp.density == 1000.0
+    assert list(sp.gravity) == [0.0, 0.0, 0.0]
+    assert np.isclose(sp.cg_rel_epsilon, 1e-3) and np.isclose(sp.cg_abs_epsilon, 1e-6)
+    assert sp.cg_max_iter == 20 and sp.warm_start == 1
+
+
+@pytest.mark.parametrize("counts", [(2, 2, 2), (3, 4, 5), (11, 11, 41), (5, 3, 9)])
+def test_mesh_box_matches_oracle(cf, orc, counts):
+    half = (0.05, 0.07, 0.2)
+    p1, t1 = cf.mesh_box(half, counts)
+    p2, t2 = orc.mesh_box(half, counts)
+    assert np.array_equal(p1, p2) and np.array_equal(t1, t2)
+
+
+def test_mesh_box_config2_counts(cf):
+    nn, nt = C.c_uint32(), C.c_uint32()
+    assert cf.lib.cf_mesh_box_counts(65, 65, 257, C.byref(nn), C.byref(nt)) == 0
+    assert (nn.value, nt.value) == (1085825, 5242880)
+    assert cf.lib.cf_mesh_box_counts(1, 5, 5, C.byref(nn), C.byref(nt)) != 0
+
+
+def test_mesh_box2_matches_oracle(cf, orc):
+    for counts in [(2, 2), (5, 5), (33, 9)]:
+        p1, t1 = cf.mesh_box2((0.16, 0.04), counts)
+        p2, t2 = orc.mesh_box2((0.16, 0.04), counts)
+        assert np.array_equal(p1, p2) and np.array_equal(t1, t2)
+
+
+def test_invalid_arguments_return_status(cf):
+    out = C.c_void_p()
+    p = cf.default_fem_params()
+    assert cf.lib.cf_tetset_create(0, None, 0, None, C.byref(p), C.byref(out)) != 0
+    assert cf.lib.cf_set_tile_elems(5) != 0
+    assert cf.lib.cf_set_tile_elems(0) == 0
+    bad = cf.default_fem_params(poisson_ratio=0.5)
+    pos, tets = cf.mesh_box((0.01, 0.01, 0.01), (2, 2, 2))
+    from risdsdgcfem_thesis_amd.fem import _fp, _up
+    rc = cf.lib.cf_tetset_create(pos.size // 3, _fp(pos), tets.size // 4, _up(tets), C.byref(bad), C.byref(out))
+    assert rc != 0
+    assert cf.lib.cf_last_error()
+
+
+def test_no_device_is_an_error_not_a_crash(cf):
+    if cf.device_count() > 0:
+        pytest.skip("a GPU is visible")
+    pos, tets = cf.mesh_box((0.01, 0.01, 0.01), (2, 2, 2))
+    with pytest.raises(cf.CorofemError):
+        cf.TetrahedronElementSet3(pos, tets)
