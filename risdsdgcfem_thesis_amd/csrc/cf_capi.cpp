@@ -146,8 +146,8 @@ static int cg_alloc(CgBuffers& b, uint32_t n) {
     CF_TRY(dalloc(&b.d, n));
     CF_TRY(dalloc(&b.q, n));
     CF_TRY(dalloc(&b.st, 1));
-    CF_TRY(dalloc(&b.part_a, kMaxVecBlocks));
-    CF_TRY(dalloc(&b.part_b, kMaxVecBlocks));
+    CF_TRY(dalloc(&b.part_a, kMaxPartials));
+    CF_TRY(dalloc(&b.part_b, kMaxPartials));
     CF_CHECK_HIP(hipHostMalloc(reinterpret_cast<void**>(&b.h_flag), sizeof(int32_t)));
     return CF_OK;
 }
@@ -261,8 +261,8 @@ static int solid_cg(cf_solid* s, float fM, float fK, cf_cg_result* res) {
     hipStream_t st = s->stream;
     CgBuffers& b = s->cg;
     const uint32_t n = 3 * s->N;
-    const int nvec = vec_blocks(n);
-    const int nnode = vec_blocks(s->N);
+    const int nvec = node_blocks(n);
+    const int nnode = node_blocks(s->N);
     uint32_t max_it = std::min<uint32_t>(s->p.cg_max_iter, kMaxCgIter);
     NodeTerms nt;
     nt.knc_mask = s->num_knc ? s->knc_mask : nullptr;
@@ -284,8 +284,9 @@ static int solid_cg(cf_solid* s, float fM, float fK, cf_cg_result* res) {
             const int32_t* skip = &b.st->done[k];
             CF_TRY(launch_forces_skip(t, kModeDfx, b.d, s->cache->R, set->mat, skip, st));
             CF_TRY(launch_gather_skip(t, kGatherMatvec, nt, nnode, skip, st));
-            CF_TRY(launch_cg_update_xr(n, k, b.st, b.part_a, nnode, s->y, b.r, b.d, b.q, b.part_b, nvec, st));
-            CF_TRY(launch_cg_update_d(n, k, b.st, b.part_b, nvec, b.d, b.r, nt.knc_mask, nvec, st));
+            CF_TRY(launch_cg_alpha(b.st, k, b.part_a, nnode, st));
+            CF_TRY(launch_cg_update_xr(n, k, b.st, s->y, b.r, b.d, b.q, b.part_b, st));
+            CF_TRY(launch_cg_update_d(n, k, b.st, b.d, b.r, nt.knc_mask, st));
         }
         CF_CHECK_HIP(hipMemcpyAsync(b.h_flag, &b.st->done[k], sizeof(int32_t), hipMemcpyDeviceToHost, st));
         CF_CHECK_HIP(hipStreamSynchronize(st));
@@ -562,13 +563,13 @@ int cf_cache_create(cf_tetset* set, const float* inv_mass, cf_cache** out) {
     if (!c) return CF_ERR_OUT_OF_MEMORY;
     c->set = set;
     const uint32_t T = set->dev.T;
-    c->nblk = vec_blocks(T);
+    c->nblk = node_blocks(T);
     int rc = dalloc(&c->R, 9 * (size_t)T);
     if (rc == CF_OK) rc = dalloc(&c->detF, T);
-    if (rc == CF_OK) rc = dalloc(&c->blk_count, 2 * (size_t)kMaxVecBlocks);
-    if (rc == CF_OK) rc = dalloc(&c->blk_sum, kMaxVecBlocks);
-    if (rc == CF_OK) rc = dalloc(&c->blk_min, kMaxVecBlocks);
-    if (rc == CF_OK) rc = dalloc(&c->blk_energy, kMaxVecBlocks);
+    if (rc == CF_OK) rc = dalloc(&c->blk_count, 2 * (size_t)kMaxPartials);
+    if (rc == CF_OK) rc = dalloc(&c->blk_sum, kMaxPartials);
+    if (rc == CF_OK) rc = dalloc(&c->blk_min, kMaxPartials);
+    if (rc == CF_OK) rc = dalloc(&c->blk_energy, kMaxPartials);
     if (rc == CF_OK && inv_mass) {
         rc = dalloc(&c->inv_mass, set->dev.N);
         if (rc == CF_OK && hipMemcpy(c->inv_mass, inv_mass, set->dev.N * 4, hipMemcpyHostToDevice) != hipSuccess)
@@ -613,7 +614,7 @@ int cf_f_e(cf_cache* c, float* f) {
     CF_TRY(launch_forces(s->dev, kModeFe, c->x, c->R, s->mat, s->stream));
     NodeTerms nt;
     nt.out = f;
-    return launch_gather(s->dev, kGatherAccumulate, nt, vec_blocks(s->dev.N), s->stream);
+    return launch_gather(s->dev, kGatherAccumulate, nt, node_blocks(s->dev.N), s->stream);
 }
 
 int cf_f(cf_cache* c, float* f) { return cf_f_e(c, f); }  // f = f_e + f_p, f_p a no-op (:294-300,350-353)
@@ -624,7 +625,7 @@ int cf_df_x(cf_cache* c, const float* dx, float* df) {
     CF_TRY(launch_forces(s->dev, kModeDfx, dx, c->R, s->mat, s->stream));
     NodeTerms nt;
     nt.out = df;
-    return launch_gather(s->dev, kGatherAccumulate, nt, vec_blocks(s->dev.N), s->stream);
+    return launch_gather(s->dev, kGatherAccumulate, nt, node_blocks(s->dev.N), s->stream);
 }
 
 int cf_V(cf_cache* c, double* energy) {
@@ -710,7 +711,7 @@ int cf_cg_solve(cf_cg* c, cf_av_fn av, cf_constraints_fn cons, void* user, float
     max_it = std::min(max_it, kMaxCgIter);
     CgBuffers& b = c->b;
     const uint32_t n = b.n;
-    const int nvec = vec_blocks(n);
+    const int nvec = node_blocks(n);
     hipStream_t st = c->stream;
     // r = S(b - A x); d = r; delta0 = r.r   (ILSS_CG.h:53-65)
     av(user, b.q, x);
@@ -728,8 +729,9 @@ int cf_cg_solve(cf_cg* c, cf_av_fn av, cf_constraints_fn cons, void* user, float
         av(user, b.q, b.d);
         if (cons) cons(user, b.q);
         CF_TRY(launch_cg_dot(n, b.d, b.q, b.part_a, nvec, st));
-        CF_TRY(launch_cg_update_xr(n, k, b.st, b.part_a, nvec, x, b.r, b.d, b.q, b.part_b, nvec, st));
-        CF_TRY(launch_cg_update_d(n, k, b.st, b.part_b, nvec, b.d, b.r, nullptr, nvec, st));
+        CF_TRY(launch_cg_alpha(b.st, k, b.part_a, nvec, st));
+        CF_TRY(launch_cg_update_xr(n, k, b.st, x, b.r, b.d, b.q, b.part_b, st));
+        CF_TRY(launch_cg_update_d(n, k, b.st, b.d, b.r, nullptr, st));
         if (cons) cons(user, b.d);
         k++;
         CF_CHECK_HIP(hipMemcpyAsync(b.h_flag, &b.st->done[k], 4, hipMemcpyDeviceToHost, st));
@@ -851,7 +853,7 @@ int cf_solid_fixed_step(cf_solid* s, float dt, cf_step_stats* stats) {
     nt.dt = dt; nt.inv_dt = 1.0f / dt;
     nt.g[0] = s->p.gravity[0]; nt.g[1] = s->p.gravity[1]; nt.g[2] = s->p.gravity[2];
     nt.warm_start = s->p.warm_start;
-    CF_TRY(launch_gather(t, kGatherRhs, nt, vec_blocks(s->N), st));
+    CF_TRY(launch_gather(t, kGatherRhs, nt, node_blocks(s->N), st));
     // A = (1 + dt c0) M + (dt^2 + dt c1) K  (:1300-1302)
     const float fM = 1.0f + dt * s->rayleigh[0];
     const float fK = dt * dt + dt * s->rayleigh[1];
@@ -947,7 +949,7 @@ int cf_solid_assemble_rhs(cf_solid* s, float dt, float* h_b, float* h_y0) {
     nt.dt = dt; nt.inv_dt = 1.0f / dt;
     nt.g[0] = s->p.gravity[0]; nt.g[1] = s->p.gravity[1]; nt.g[2] = s->p.gravity[2];
     nt.warm_start = s->p.warm_start;
-    CF_TRY(launch_gather(set->dev, kGatherRhs, nt, vec_blocks(s->N), st));
+    CF_TRY(launch_gather(set->dev, kGatherRhs, nt, node_blocks(s->N), st));
     CF_TRY(cf_end_evaluation(s->cache));
     CF_CHECK_HIP(hipStreamSynchronize(st));
     if (h_b) CF_CHECK_HIP(hipMemcpy(h_b, s->bvec, 3 * (size_t)s->N * 4, hipMemcpyDeviceToHost));
@@ -970,7 +972,7 @@ int cf_solid_apply_A(cf_solid* s, float fM, float fK, const float* h_d, float* h
     nt.partials = b.part_a;
     nt.d = b.d; nt.out = b.q;
     CF_TRY(launch_forces(set->dev, kModeDfx, b.d, s->cache->R, set->mat, st));
-    CF_TRY(launch_gather(set->dev, kGatherMatvec, nt, vec_blocks(s->N), st));
+    CF_TRY(launch_gather(set->dev, kGatherMatvec, nt, node_blocks(s->N), st));
     CF_CHECK_HIP(hipStreamSynchronize(st));
     CF_CHECK_HIP(hipMemcpy(h_q, b.q, n3 * 4, hipMemcpyDeviceToHost));
     return CF_OK;
@@ -986,7 +988,7 @@ int cf_solid_time_df_x(cf_solid* s, uint32_t reps, double* ms) {
     CF_CHECK_HIP(hipEventRecord(s->ev0, st));
     for (uint32_t r = 0; r < reps; r++) {
         CF_TRY(launch_forces(set->dev, kModeDfx, s->v, s->cache->R, set->mat, st));
-        CF_TRY(launch_gather(set->dev, kGatherAccumulate, nt, vec_blocks(s->N), st));
+        CF_TRY(launch_gather(set->dev, kGatherAccumulate, nt, node_blocks(s->N), st));
     }
     CF_CHECK_HIP(hipEventRecord(s->ev1, st));
     CF_CHECK_HIP(hipEventSynchronize(s->ev1));
@@ -1008,7 +1010,7 @@ int cf_solid_time_kx(cf_solid* s, uint32_t reps, double* ms_tile, double* ms_gat
     nt.fM = 1.0f; nt.fK = s->p.fixed_dt * s->p.fixed_dt;
     nt.partials = b.part_a;
     nt.d = b.d; nt.out = b.q;
-    const int nnode = vec_blocks(s->N);
+    const int nnode = node_blocks(s->N);
     float t = 0.f;
     CF_CHECK_HIP(hipEventRecord(s->ev0, st));
     for (uint32_t r = 0; r < reps; r++) CF_TRY(launch_forces(set->dev, kModeDfx, b.d, s->cache->R, set->mat, st));

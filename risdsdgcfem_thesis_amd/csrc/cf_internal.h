@@ -27,9 +27,10 @@ void set_error(const char* fmt, ...);
 
 // Threads per block of the element and node kernels (4 wave64s).
 constexpr int kBlock = 256;
-// Blocks of the grid-stride node/vector kernels; their FP64 partial sums are
-// reduced redundantly (and identically) by every block of the next kernel.
-constexpr int kMaxVecBlocks = 512;
+// Upper bound on the blocks of the grid-stride node/vector/element kernels,
+// and so on the FP64 per-block partial sums they write. A one-block kernel
+// sums the partials in a fixed order, so reductions are bit-reproducible.
+constexpr int kMaxPartials = 8192;
 
 // Elastic constants of the reference's isotropic E matrix
 // (ms/fem/TetrahedronElement3.h:141-156): a = E/((1+nu)(1-2nu)).
@@ -85,6 +86,7 @@ struct CgScalars {
     int32_t done[1024 + 2];  // done[k]: iteration k does not run
     int32_t reason[1024 + 2];
     float alpha[1024 + 2];
+    float beta[1024 + 2];
     double delta0;
     double target;
     double abs_sq;
@@ -125,7 +127,8 @@ int launch_gather_skip(const TetDevice& t, int mode, const NodeTerms& nt, int nb
                        hipStream_t s);
 int launch_energy(const TetDevice& t, const float* x, const float* R, Material m, double* blk, int nblk,
                   hipStream_t s);
-int vec_blocks(uint32_t n);
+int vec_blocks(uint32_t n);    // vector kernels over n floats (float4 per thread)
+int node_blocks(uint32_t n);   // one node (or element) per thread
 
 // CG kernels over n = 3N floats
 int launch_cg_init(uint32_t n, const float* q, const float* b, float* r, float* d, const uint8_t* mask,
@@ -133,10 +136,12 @@ int launch_cg_init(uint32_t n, const float* q, const float* b, float* r, float* 
 int launch_cg_start(CgScalars* st, const double* rr_part, int nblk_prev, float rel, float abs_eps,
                     uint32_t max_iter, hipStream_t s);
 int launch_cg_dot(uint32_t n, const float* a, const float* b, double* part, int nblk, hipStream_t s);
-int launch_cg_update_xr(uint32_t n, uint32_t k, CgScalars* st, const double* dq_part, int nblk_dq, float* x,
-                        float* r, const float* d, const float* q, double* rr_part, int nblk, hipStream_t s);
-int launch_cg_update_d(uint32_t n, uint32_t k, CgScalars* st, const double* rr_part, int nblk_rr, float* d,
-                       const float* r, const uint8_t* mask, int nblk, hipStream_t s);
+int launch_cg_alpha(CgScalars* st, uint32_t k, const double* dq_part, int nblk_dq, hipStream_t s);
+// x, r update and per-block r.r partials, then the one-block delta/beta/stop kernel
+int launch_cg_update_xr(uint32_t n, uint32_t k, CgScalars* st, float* x, float* r, const float* d, const float* q,
+                        double* rr_part, hipStream_t s);
+int launch_cg_update_d(uint32_t n, uint32_t k, CgScalars* st, float* d, const float* r, const uint8_t* mask,
+                       hipStream_t s);
 int launch_integrate(uint32_t n, float* x, float* v, const float* y, float dt, float drag, hipStream_t s);
 int launch_zero_mask(uint32_t n_nodes, float* w, const uint8_t* mask, hipStream_t s);
 

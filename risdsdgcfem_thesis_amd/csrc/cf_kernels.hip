@@ -253,7 +253,7 @@ __device__ inline double block_sum(double v, double* sh) {
     for (int i = 0; i < kBlock / 64; i++) r += sh[i];
     return r;
 }
-// Deterministic all-reduce of n partials by one block (identical in every block).
+// Deterministic sum of n partials by one block (fixed order and tree).
 __device__ inline double reduce_partials(const double* part, int n, double* sh) {
     double v = 0.0;
     for (int i = threadIdx.x; i < n; i += kBlock) v += part[i];
@@ -407,11 +407,9 @@ __global__ __launch_bounds__(kBlock) void k_tile_forces(TileArgs a) {
         s_v[3 * l + 0] = a.in[3 * (size_t)g + 0];
         s_v[3 * l + 1] = a.in[3 * (size_t)g + 1];
         s_v[3 * l + 2] = a.in[3 * (size_t)g + 2];
-        if (MODE == kModeFe) {
-            s_r[3 * l + 0] = a.ref[3 * (size_t)g + 0];
-            s_r[3 * l + 1] = a.ref[3 * (size_t)g + 1];
-            s_r[3 * l + 2] = a.ref[3 * (size_t)g + 2];
-        }
+        s_r[3 * l + 0] = a.ref[3 * (size_t)g + 0];
+        s_r[3 * l + 1] = a.ref[3 * (size_t)g + 1];
+        s_r[3 * l + 2] = a.ref[3 * (size_t)g + 2];
     }
     for (uint32_t l = threadIdx.x; l <= nn; l += kBlock) s_sp[l] = a.slot_ptr[n0 + l] - sbase;
     for (uint32_t s = threadIdx.x; s < 4 * ne; s += kBlock) s_sl[s] = a.slots[sbase + s];
@@ -421,12 +419,24 @@ __global__ __launch_bounds__(kBlock) void k_tile_forces(TileArgs a) {
     for (uint32_t i = threadIdx.x; i < ne; i += kBlock) {
         const size_t e = e0 + i;
         const uint32_t l0 = a.lvid[e], l1 = a.lvid[T + e], l2 = a.lvid[2 * T + e], l3 = a.lvid[3 * T + e];
-        float R[9], G[9];
+        float R[9];
 #pragma unroll
         for (int k = 0; k < 9; k++) R[k] = a.R[k * T + e];
+        // Dm^-1 and V rebuilt from the rest positions staged in LDS with the
+        // same operations as k_precompute (bit-identical to the stored
+        // values): 12 B per tile node instead of 40 B per element from HBM.
+        float Dm[9];
 #pragma unroll
-        for (int k = 0; k < 9; k++) G[k] = a.dinv[k * T + e];
-        const float V = a.vol[e];
+        for (int c = 0; c < 3; c++) {
+            const float r0 = s_r[3 * l0 + c];
+            Dm[3 * c + 0] = s_r[3 * l1 + c] - r0;
+            Dm[3 * c + 1] = s_r[3 * l2 + c] - r0;
+            Dm[3 * c + 2] = s_r[3 * l3 + c] - r0;
+        }
+        const float detDm = refmath::det3(Dm);
+        const float V = detDm / 6.0f;
+        float G[9];
+        refmath::inv3(Dm, detDm, G);
         float D[9];  // columns v1-v0, v2-v0, v3-v0
 #pragma unroll
         for (int c = 0; c < 3; c++) {
@@ -443,12 +453,7 @@ __global__ __launch_bounds__(kBlock) void k_tile_forces(TileArgs a) {
                 RD[3 * i3 + j] = R[0 + i3] * D[0 + j] + R[3 + i3] * D[3 + j] + R[6 + i3] * D[6 + j];
         if (MODE == kModeFe) {
 #pragma unroll
-            for (int c = 0; c < 3; c++) {
-                const float r0 = s_r[3 * l0 + c];
-                RD[3 * c + 0] -= s_r[3 * l1 + c] - r0;
-                RD[3 * c + 1] -= s_r[3 * l2 + c] - r0;
-                RD[3 * c + 2] -= s_r[3 * l3 + c] - r0;
-            }
+            for (int k = 0; k < 9; k++) RD[k] -= Dm[k];
         }
         float H[9];  // RD G
 #pragma unroll
@@ -620,6 +625,8 @@ __global__ __launch_bounds__(kBlock) void k_energy(uint32_t T, const uint32_t* _
 
 // ============================================================ CG vector kernels
 // ILSS_CG::Solve (ns/ILSS_CG.h:38-151); the projection S zeroes KNC nodes.
+// Vector kernels walk n floats as float4 when every pointer is 16-byte
+// aligned (V4), with the n % 4 tail done by the first threads of the grid.
 __global__ __launch_bounds__(kBlock) void k_cg_init(uint32_t n, const float* __restrict__ q,
                                                     const float* __restrict__ b, float* __restrict__ r,
                                                     float* __restrict__ d, const uint8_t* __restrict__ mask,
@@ -647,6 +654,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_start(CgScalars* st, const double
         st->delta[k] = 0.0;
         st->dq[k] = 0.0;
         st->alpha[k] = 0.f;
+        st->beta[k] = 0.f;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -675,57 +683,71 @@ __global__ __launch_bounds__(kBlock) void k_dot(uint32_t n, const float* __restr
     if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
 
-// alpha = delta_k / d^T q ; x += alpha d ; r -= alpha q ; partial r^T r
-__global__ __launch_bounds__(kBlock) void k_cg_update_xr(uint32_t n, uint32_t k, CgScalars* st,
-                                                         const double* dq_part, int ndq, float* __restrict__ x,
-                                                         float* __restrict__ r, const float* __restrict__ d,
-                                                         const float* __restrict__ q, double* rr_part) {
+// alpha for iteration k from per-block d.q partials (generic solver path)
+__global__ __launch_bounds__(kBlock) void k_cg_alpha(CgScalars* st, uint32_t k, const double* part, int n) {
     __shared__ double sh[kBlock / 64];
     if (st->done[k]) return;
-    const double dq = reduce_partials(dq_part, ndq, sh);
-    if (fabs(dq) < st->abs_sq) {  // ILSS_CG.h:97
-        if (blockIdx.x == 0 && threadIdx.x == 0) {
-            st->dq[k] = dq;
+    const double dq = reduce_partials(part, n, sh);
+    if (threadIdx.x == 0) {
+        st->dq[k] = dq;
+        if (fabs(dq) < st->abs_sq) {
             st->reason[k + 1] = CF_CG_BREAKDOWN;
             st->done[k + 1] = 2;
+        } else {
+            st->alpha[k] = (float)(st->delta[k] / dq);
         }
-        return;
     }
-    const float alpha = (float)(st->delta[k] / dq);
+}
+
+// x += alpha d ; r -= alpha q ; per-block partials of r^T r (ILSS_CG.h:100-117)
+template <bool V4>
+__global__ __launch_bounds__(kBlock) void k_cg_update_xr(uint32_t n, uint32_t k, CgScalars* st,
+                                                         float* __restrict__ x, float* __restrict__ r,
+                                                         const float* __restrict__ d, const float* __restrict__ q,
+                                                         double* rr_part) {
+    __shared__ double sh[kBlock / 64];
+    if (st->done[k] || st->reason[k + 1] == CF_CG_BREAKDOWN) return;
+    const float alpha = st->alpha[k];
     double acc = 0.0;
-    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
-        const float di = d[i], qi = q[i];
-        x[i] = x[i] + di * alpha;
-        const float ri = r[i] - qi * alpha;
+    const uint32_t gid = blockIdx.x * kBlock + threadIdx.x, stride = gridDim.x * kBlock;
+    uint32_t tail0 = 0;
+    if (V4) {
+        const uint32_t n4 = n / 4;
+        tail0 = 4 * n4;
+        float4* x4 = reinterpret_cast<float4*>(x);
+        float4* r4 = reinterpret_cast<float4*>(r);
+        const float4* d4 = reinterpret_cast<const float4*>(d);
+        const float4* q4 = reinterpret_cast<const float4*>(q);
+        for (uint32_t i = gid; i < n4; i += stride) {
+            const float4 di = d4[i], qi = q4[i];
+            float4 xi = x4[i], ri = r4[i];
+            xi.x = xi.x + di.x * alpha; xi.y = xi.y + di.y * alpha;
+            xi.z = xi.z + di.z * alpha; xi.w = xi.w + di.w * alpha;
+            ri.x = ri.x - qi.x * alpha; ri.y = ri.y - qi.y * alpha;
+            ri.z = ri.z - qi.z * alpha; ri.w = ri.w - qi.w * alpha;
+            x4[i] = xi;
+            r4[i] = ri;
+            acc += (double)ri.x * ri.x + (double)ri.y * ri.y + (double)ri.z * ri.z + (double)ri.w * ri.w;
+        }
+    }
+    for (uint32_t i = tail0 + gid; i < n; i += stride) {
+        x[i] = x[i] + d[i] * alpha;
+        const float ri = r[i] - q[i] * alpha;
         r[i] = ri;
         acc += (double)ri * ri;
     }
     const double s = block_sum(acc, sh);
     if (threadIdx.x == 0) rr_part[blockIdx.x] = s;
-    if (blockIdx.x == 0 && threadIdx.x == 0) { st->dq[k] = dq; st->alpha[k] = alpha; }
 }
 
-// delta_{k+1} = r^T r ; beta = delta_{k+1}/delta_k ; d = S(beta d + r) ; test
-__global__ __launch_bounds__(kBlock) void k_cg_update_d(uint32_t n, uint32_t k, CgScalars* st,
-                                                        const double* rr_part, int nrr, float* __restrict__ d,
-                                                        const float* __restrict__ r,
-                                                        const uint8_t* __restrict__ mask) {
+// delta_{k+1} = r^T r from per-block partials ; beta ; stop test (ILSS_CG.h:86-88,115-133)
+__global__ __launch_bounds__(kBlock) void k_cg_beta(CgScalars* st, uint32_t k, const double* part, int n) {
     __shared__ double sh[kBlock / 64];
-    if (st->done[k]) {
-        // keep every later iteration switched off once the solve has ended
-        if (blockIdx.x == 0 && threadIdx.x == 0) st->done[k + 1] = 1;
-        return;
-    }
-    if (st->reason[k + 1] == CF_CG_BREAKDOWN) return;
-    const double dn = reduce_partials(rr_part, nrr, sh);
-    const float beta = (float)(dn / st->delta[k]);
-    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
-        float di = beta * d[i] + r[i];
-        if (mask && mask[i / 3]) di = 0.f;
-        d[i] = di;
-    }
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (st->done[k] || st->reason[k + 1] == CF_CG_BREAKDOWN) return;
+    const double dn = reduce_partials(part, n, sh);
+    if (threadIdx.x == 0) {
         st->delta[k + 1] = dn;
+        st->beta[k] = (float)(dn / st->delta[k]);
         const uint32_t k1 = k + 1;
         int reason = 0;
         if (!(dn > st->abs_sq)) reason = CF_CG_CONVERGED_ABS;
@@ -733,6 +755,47 @@ __global__ __launch_bounds__(kBlock) void k_cg_update_d(uint32_t n, uint32_t k, 
         else if (!(k1 < st->max_iter)) reason = CF_CG_MAX_ITER;
         st->reason[k1] = reason;
         st->done[k1] = reason != 0;
+    }
+}
+
+// d = S(beta d + r)
+template <bool V4>
+__global__ __launch_bounds__(kBlock) void k_cg_update_d(uint32_t n, uint32_t k, CgScalars* st,
+                                                        float* __restrict__ d, const float* __restrict__ r,
+                                                        const uint8_t* __restrict__ mask) {
+    if (st->done[k]) {
+        // keep every later iteration switched off once the solve has ended
+        if (blockIdx.x == 0 && threadIdx.x == 0) st->done[k + 1] = 1;
+        return;
+    }
+    if (st->done[k + 1]) return;  // converged or broke down at iteration k
+    const float beta = st->beta[k];
+    const uint32_t gid = blockIdx.x * kBlock + threadIdx.x, stride = gridDim.x * kBlock;
+    uint32_t tail0 = 0;
+    if (V4) {
+        const uint32_t n4 = n / 4;
+        tail0 = 4 * n4;
+        float4* d4 = reinterpret_cast<float4*>(d);
+        const float4* r4 = reinterpret_cast<const float4*>(r);
+        for (uint32_t i = gid; i < n4; i += stride) {
+            const float4 ri = r4[i];
+            float4 di = d4[i];
+            di.x = beta * di.x + ri.x; di.y = beta * di.y + ri.y;
+            di.z = beta * di.z + ri.z; di.w = beta * di.w + ri.w;
+            if (mask) {
+                const uint32_t e = 4 * i;
+                if (mask[e / 3]) di.x = 0.f;
+                if (mask[(e + 1) / 3]) di.y = 0.f;
+                if (mask[(e + 2) / 3]) di.z = 0.f;
+                if (mask[(e + 3) / 3]) di.w = 0.f;
+            }
+            d4[i] = di;
+        }
+    }
+    for (uint32_t i = tail0 + gid; i < n; i += stride) {
+        float di = beta * d[i] + r[i];
+        if (mask && mask[i / 3]) di = 0.f;
+        d[i] = di;
     }
 }
 
@@ -753,11 +816,20 @@ __global__ __launch_bounds__(kBlock) void k_zero_mask(uint32_t nn, float* __rest
 
 // ============================================================ launchers
 int vec_blocks(uint32_t n) {
-    uint32_t b = (n + kBlock - 1) / kBlock;
+    uint32_t b = (n + 4 * kBlock - 1) / (4 * kBlock);
     if (b < 1) b = 1;
-    if (b > (uint32_t)kMaxVecBlocks) b = kMaxVecBlocks;
+    if (b > (uint32_t)kMaxPartials) b = kMaxPartials;
     return (int)b;
 }
+
+int node_blocks(uint32_t n) {
+    uint32_t b = (n + kBlock - 1) / kBlock;
+    if (b < 1) b = 1;
+    if (b > (uint32_t)kMaxPartials) b = kMaxPartials;
+    return (int)b;
+}
+
+static bool aligned16(const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; }
 
 static int check_launch() {
     hipError_t e = hipGetLastError();
@@ -852,26 +924,40 @@ int launch_cg_dot(uint32_t n, const float* a, const float* b, double* part, int 
     return check_launch();
 }
 
-int launch_cg_update_xr(uint32_t n, uint32_t k, CgScalars* st, const double* dq_part, int nblk_dq, float* x,
-                        float* r, const float* d, const float* q, double* rr_part, int nblk, hipStream_t s) {
-    hipLaunchKernelGGL(k_cg_update_xr, dim3(nblk), dim3(kBlock), 0, s, n, k, st, dq_part, nblk_dq, x, r, d, q,
-                       rr_part);
+int launch_cg_alpha(CgScalars* st, uint32_t k, const double* dq_part, int nblk_dq, hipStream_t s) {
+    hipLaunchKernelGGL(k_cg_alpha, dim3(1), dim3(kBlock), 0, s, st, k, dq_part, nblk_dq);
     return check_launch();
 }
 
-int launch_cg_update_d(uint32_t n, uint32_t k, CgScalars* st, const double* rr_part, int nblk_rr, float* d,
-                       const float* r, const uint8_t* mask, int nblk, hipStream_t s) {
-    hipLaunchKernelGGL(k_cg_update_d, dim3(nblk), dim3(kBlock), 0, s, n, k, st, rr_part, nblk_rr, d, r, mask);
+int launch_cg_update_xr(uint32_t n, uint32_t k, CgScalars* st, float* x, float* r, const float* d, const float* q,
+                        double* rr_part, hipStream_t s) {
+    const int nb = vec_blocks(n);
+    if (aligned16(x) && aligned16(r) && aligned16(d) && aligned16(q))
+        hipLaunchKernelGGL(k_cg_update_xr<true>, dim3(nb), dim3(kBlock), 0, s, n, k, st, x, r, d, q, rr_part);
+    else
+        hipLaunchKernelGGL(k_cg_update_xr<false>, dim3(nb), dim3(kBlock), 0, s, n, k, st, x, r, d, q, rr_part);
+    CF_TRY(check_launch());
+    hipLaunchKernelGGL(k_cg_beta, dim3(1), dim3(kBlock), 0, s, st, k, rr_part, nb);
+    return check_launch();
+}
+
+int launch_cg_update_d(uint32_t n, uint32_t k, CgScalars* st, float* d, const float* r, const uint8_t* mask,
+                       hipStream_t s) {
+    const int nb = vec_blocks(n);
+    if (aligned16(d) && aligned16(r))
+        hipLaunchKernelGGL(k_cg_update_d<true>, dim3(nb), dim3(kBlock), 0, s, n, k, st, d, r, mask);
+    else
+        hipLaunchKernelGGL(k_cg_update_d<false>, dim3(nb), dim3(kBlock), 0, s, n, k, st, d, r, mask);
     return check_launch();
 }
 
 int launch_integrate(uint32_t n, float* x, float* v, const float* y, float dt, float drag, hipStream_t s) {
-    hipLaunchKernelGGL(k_integrate, dim3(vec_blocks(n)), dim3(kBlock), 0, s, n, x, v, y, dt, drag);
+    hipLaunchKernelGGL(k_integrate, dim3(node_blocks(n)), dim3(kBlock), 0, s, n, x, v, y, dt, drag);
     return check_launch();
 }
 
 int launch_zero_mask(uint32_t n_nodes, float* w, const uint8_t* mask, hipStream_t s) {
-    hipLaunchKernelGGL(k_zero_mask, dim3(vec_blocks(n_nodes)), dim3(kBlock), 0, s, n_nodes, w, mask);
+    hipLaunchKernelGGL(k_zero_mask, dim3(node_blocks(n_nodes)), dim3(kBlock), 0, s, n_nodes, w, mask);
     return check_launch();
 }
 
