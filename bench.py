@@ -95,7 +95,12 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--kx-reps", type=int, default=50)
+    ap.add_argument("--tile-elems", type=int, default=256)
+    ap.add_argument("--tile-threads", type=int, default=256)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--kx-only", type=int, default=0, metavar="REPS",
+                    help="profiling mode: one step, then REPS K*x launches and one 256 MiB device copy "
+                         "(the copy calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE); prints timings only")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only: the workload is one GPU's mesh; --gpus {args.gpus} is not "
@@ -107,6 +112,8 @@ def main():
         print("no GPU visible", file=sys.stderr)
         sys.exit(1)
 
+    cf.set_tile_elems(args.tile_elems)
+    cf.set_tile_threads(args.tile_threads)
     counts = (65, 65, 257)
     half = (0.32, 0.32, 1.28)
     pos, tets = cf.mesh_box(half, counts)
@@ -122,6 +129,16 @@ def main():
     solid.set_state(x0, np.zeros_like(x0))
     solid.set_knc(knc, x0.reshape(-1, 3)[knc].ravel())
     log(f"[bench] setup {time.perf_counter() - t_build:.2f} s; warmup {args.warmup} steps")
+    if args.kx_only:
+        solid.fixed_step()
+        ms_tile, ms_gather = solid.time_kx(args.kx_reps if args.kx_reps else args.kx_only)
+        nbytes = 256 << 20
+        a, b = cf.DeviceArray(nbytes // 4), cf.DeviceArray(nbytes // 4)
+        cf.lib.cf_memcpy_dtod(b.ptr, a.ptr, nbytes)
+        cf.synchronize()
+        print(json.dumps({"mode": "kx-only", "tile_ms": ms_tile, "gather_ms": ms_gather, "copy_bytes": nbytes,
+                          "tets": T, "nodes": N}), flush=True)
+        return
     for _ in range(args.warmup):
         solid.fixed_step()
     cf.synchronize()
@@ -150,6 +167,7 @@ def main():
         "traffic": traffic.get("bytes_per_kx") if traffic else None,
         "kernels": {"k_tile_forces_ms": round(ms_tile, 5), "k_gather_ms": round(ms_gather, 5)},
         "b_alg_bytes": b_alg, "kx_gtets": round(T / (ms_kx * 1e-3) / 1e9, 2),
+        "tile": {"elems": args.tile_elems, "threads": args.tile_threads},
     }
     cpu = None
     if not args.no_cpu_baseline:

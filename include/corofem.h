@@ -226,9 +226,15 @@ int cf_solid_time_df_x(cf_solid* s, uint32_t reps, double* ms_per_call);
 int cf_solid_time_kx(cf_solid* s, uint32_t reps, double* ms_tile, double* ms_gather);
 
 /* ---------------------------------------------------------------- tuning */
-/* Elements per LDS tile of the node-gather K*x (0 = library default). Takes
-   effect for sets created afterwards. */
+/* Elements per LDS tile of the node-gather K*x (0 = library default 256).
+   Takes effect for sets created afterwards. */
 int cf_set_tile_elems(uint32_t tile_elems);
+/* Workgroup size of the LDS-tile element kernel: 256, 512 or 1024 (0 = default
+   256). A tile holds at most this many elements and nodes. */
+int cf_set_tile_threads(uint32_t threads);
+/* 1: persistent grid (resident workgroups walk their tiles with the next
+   tile's loads in flight); 0 (default): one tile per workgroup. */
+int cf_set_tile_persistent(int persistent);
 int cf_get_tile_elems(uint32_t* tile_elems);
 
 #ifdef __cplusplus

@@ -108,6 +108,8 @@ SIGNATURES = {
     "cf_solid_time_kx": (C.c_int, [_vp, C.c_uint32, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "cf_set_tile_elems": (C.c_int, [C.c_uint32]),
     "cf_get_tile_elems": (C.c_int, [_u]),
+    "cf_set_tile_threads": (C.c_int, [C.c_uint32]),
+    "cf_set_tile_persistent": (C.c_int, [C.c_int]),
 }
 
 

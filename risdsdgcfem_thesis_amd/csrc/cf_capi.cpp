@@ -32,6 +32,8 @@ Material material_from(float E, float nu) {
 }
 
 static uint32_t g_tile_elems = 0;
+static uint32_t g_tile_threads = 256;
+static int g_tile_persistent = 0;
 
 template <class T>
 static int dalloc(T** p, size_t count) {
@@ -161,7 +163,7 @@ static void tetset_free(cf_tetset* s) {
     TetDevice& t = s->dev;
     dfree(t.ref); dfree(t.conn); dfree(t.dinv); dfree(t.vol);
     dfree(t.tile_elem_ptr); dfree(t.tile_node_ptr); dfree(t.tile_nodes); dfree(t.slot_ptr);
-    dfree(t.slots); dfree(t.lvid); dfree(t.node_part_ptr); dfree(t.node_parts); dfree(t.partials);
+    dfree(t.dst); dfree(t.lvid); dfree(t.node_part_ptr); dfree(t.part_dst); dfree(t.partials);
     if (s->stream) (void)hipStreamDestroy(s->stream);
     s->stream = nullptr;
 }
@@ -190,7 +192,7 @@ static int tetset_bake(cf_tetset* s) {
             return CF_ERR_INVALID_ARG;
         }
     TileHost th;
-    CF_TRY(build_tiles(N, T, s->ref_host.data(), s->tets_host.data(), g_tile_elems, th));
+    CF_TRY(build_tiles(N, T, s->ref_host.data(), s->tets_host.data(), g_tile_elems, g_tile_threads, th));
     TetDevice& t = s->dev;
     t.N = N; t.T = T;
     hipStream_t st = s->stream;
@@ -202,14 +204,16 @@ static int tetset_bake(cf_tetset* s) {
     CF_TRY(upload(&t.tile_node_ptr, th.tile_node_ptr, st));
     CF_TRY(upload(&t.tile_nodes, th.tile_nodes, st));
     CF_TRY(upload(&t.slot_ptr, th.slot_ptr, st));
-    CF_TRY(upload(&t.slots, th.slots, st));
+    CF_TRY(upload(&t.dst, th.dst_planes, st));
     CF_TRY(upload(&t.lvid, th.lvid_planes, st));
     CF_TRY(upload(&t.node_part_ptr, th.node_part_ptr, st));
-    CF_TRY(upload(&t.node_parts, th.node_parts, st));
+    CF_TRY(upload(&t.part_dst, th.part_dst, st));
     t.num_tiles = uint32_t(th.tile_elem_ptr.size() - 1);
     t.max_tile_elems = th.max_tile_elems;
     t.max_tile_nodes = th.max_tile_nodes;
     t.num_parts = uint32_t(th.tile_nodes.size());
+    t.tile_threads = g_tile_threads;
+    t.tile_grid = uint32_t(tile_grid_size(t, s->device, g_tile_persistent));
     CF_TRY(dalloc(&t.partials, 3 * (size_t)t.num_parts));
     CF_TRY(launch_precompute(t, st));
     s->perm.swap(th.perm);
@@ -1032,6 +1036,20 @@ int cf_set_tile_elems(uint32_t te) {
     g_tile_elems = te;
     return CF_OK;
 }
+int cf_set_tile_threads(uint32_t n) {
+    if (n != 0 && n != 256 && n != 512 && n != 1024) {
+        set_error("tile threads must be 256, 512 or 1024");
+        return CF_ERR_INVALID_ARG;
+    }
+    g_tile_threads = n ? n : 256;
+    return CF_OK;
+}
+
+int cf_set_tile_persistent(int persistent) {
+    g_tile_persistent = persistent ? 1 : 0;
+    return CF_OK;
+}
+
 int cf_get_tile_elems(uint32_t* te) {
     if (!te) return CF_ERR_INVALID_ARG;
     *te = g_tile_elems ? g_tile_elems : default_tile_elems();

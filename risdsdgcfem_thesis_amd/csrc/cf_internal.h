@@ -53,14 +53,16 @@ struct TetDevice {
     uint32_t max_tile_elems = 0;
     uint32_t max_tile_nodes = 0;
     uint32_t num_parts = 0;        // sum over tiles of tile node counts
+    uint32_t tile_threads = 256;   // workgroup size of k_tile_forces (256, 512 or 1024)
+    uint32_t tile_grid = 8;        // grid of k_tile_forces (persistent, or one tile per workgroup)
     uint32_t* tile_elem_ptr = nullptr;  // num_tiles+1
     uint32_t* tile_node_ptr = nullptr;  // num_tiles+1
     uint32_t* tile_nodes = nullptr;     // num_parts, global node ids
-    uint32_t* slot_ptr = nullptr;       // num_parts+1, into slots
-    uint16_t* slots = nullptr;          // 4T codes (local elem * 4 + vertex)
+    uint32_t* slot_ptr = nullptr;       // num_parts+1, slot runs (tile slots start at 4*e0)
+    uint16_t* dst = nullptr;            // 4 planes of T: tile-local force slot of (element, vertex)
     uint16_t* lvid = nullptr;           // 4 planes of T tile-local vertex ids
-    uint32_t* node_part_ptr = nullptr;  // N+1
-    uint32_t* node_parts = nullptr;     // num_parts
+    uint32_t* node_part_ptr = nullptr;  // N+1: node n's partials are [ptr[n], ptr[n+1])
+    uint32_t* part_dst = nullptr;       // num_parts: partial entry of (tile, tile node), tile-major
     float* partials = nullptr;          // 3 * num_parts
 };
 
@@ -70,12 +72,14 @@ struct TileHost {
     std::vector<uint32_t> conn_planes;   // 4T
     std::vector<uint16_t> lvid_planes;   // 4T
     std::vector<uint32_t> tile_elem_ptr, tile_node_ptr, tile_nodes, slot_ptr;
-    std::vector<uint16_t> slots;
-    std::vector<uint32_t> node_part_ptr, node_parts;
+    std::vector<uint16_t> dst_planes;    // 4T
+    std::vector<uint32_t> node_part_ptr, part_dst;
     uint32_t max_tile_elems = 0, max_tile_nodes = 0;
 };
+// Tiles hold at most min(tile_elems, max_per_tile) elements and max_per_tile nodes.
 int build_tiles(uint32_t N, uint32_t T, const float* ref_pos, const uint32_t* tets, uint32_t tile_elems,
-                TileHost& out);
+                uint32_t max_per_tile, TileHost& out);
+int tile_grid_size(const TetDevice& t, int device, int persistent);
 uint32_t default_tile_elems();
 
 // CG scalar state in device memory, indexed by iteration so that no kernel

@@ -366,8 +366,9 @@ struct TileArgs {
     const uint32_t* tile_elem_ptr;
     const uint32_t* tile_node_ptr;
     const uint32_t* tile_nodes;
+    const uint32_t* part_dst;
     const uint32_t* slot_ptr;
-    const uint16_t* slots;
+    const uint16_t* dst;
     const uint16_t* lvid;
     const float* R;
     const float* dinv;
@@ -379,128 +380,192 @@ struct TileArgs {
     const int32_t* skip;
 };
 
-// Workgroups are dealt round-robin to the 8 XCDs; give each XCD a contiguous
-// run of tiles so that tiles sharing halo nodes share an L2.
-__device__ inline uint32_t xcd_tile(uint32_t b, uint32_t n) {
-    const uint32_t per = (n + 7) / 8;
-    const uint32_t t = (b % 8) * per + b / 8;
-    return t;
+// LDS-only workgroup barrier: orders LDS accesses across the barrier without
+// waiting for outstanding global loads, so that prefetches stay in flight.
+__device__ inline void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
-template <int MODE>
-__global__ __launch_bounds__(kBlock) void k_tile_forces(TileArgs a) {
-    if (a.skip && *a.skip) return;
-    uint32_t tile = xcd_tile(blockIdx.x, a.num_tiles);
-    if (tile >= a.num_tiles) return;
-    extern __shared__ float smem[];
-    float* s_v = smem;                                   // 3*max_tn
-    float* s_r = s_v + 3 * a.max_tn;                     // 3*max_tn (rest positions, f_e)
-    float* s_P = s_r + 3 * a.max_tn;                     // 9*max_te
-    uint32_t* s_sp = reinterpret_cast<uint32_t*>(s_P + 9 * a.max_te);  // max_tn+1
-    uint16_t* s_sl = reinterpret_cast<uint16_t*>(s_sp + a.max_tn + 1);  // 4*max_te
+struct TileDesc { uint32_t e0, ne, n0, nn; };
 
-    const uint32_t e0 = a.tile_elem_ptr[tile], ne = a.tile_elem_ptr[tile + 1] - e0;
-    const uint32_t n0 = a.tile_node_ptr[tile], nn = a.tile_node_ptr[tile + 1] - n0;
-    const uint32_t sbase = a.slot_ptr[n0];
-    for (uint32_t l = threadIdx.x; l < nn; l += kBlock) {
-        const uint32_t g = a.tile_nodes[n0 + l];
-        s_v[3 * l + 0] = a.in[3 * (size_t)g + 0];
-        s_v[3 * l + 1] = a.in[3 * (size_t)g + 1];
-        s_v[3 * l + 2] = a.in[3 * (size_t)g + 2];
-        s_r[3 * l + 0] = a.ref[3 * (size_t)g + 0];
-        s_r[3 * l + 1] = a.ref[3 * (size_t)g + 1];
-        s_r[3 * l + 2] = a.ref[3 * (size_t)g + 2];
-    }
-    for (uint32_t l = threadIdx.x; l <= nn; l += kBlock) s_sp[l] = a.slot_ptr[n0 + l] - sbase;
-    for (uint32_t s = threadIdx.x; s < 4 * ne; s += kBlock) s_sl[s] = a.slots[sbase + s];
-    __syncthreads();
+__device__ inline TileDesc tile_desc(const TileArgs& a, uint32_t t) {
+    TileDesc d;
+    d.e0 = a.tile_elem_ptr[t];
+    d.ne = a.tile_elem_ptr[t + 1] - d.e0;
+    d.n0 = a.tile_node_ptr[t];
+    d.nn = a.tile_node_ptr[t + 1] - d.n0;
+    return d;
+}
 
+// Per-thread registers of one tile: this thread's tile node (vector, rest
+// position, slot offset) and element (local vertex ids, force slots, R).
+struct TileRegs {
+    float v[3], r[3], R[9];
+    uint32_t sp, pd;
+    uint32_t l[4];
+    uint32_t f[4];
+};
+
+__device__ inline void load_tile_regs(const TileArgs& a, const TileDesc& d, uint32_t gid, uint32_t tid,
+                                      TileRegs& g) {
     const size_t T = a.T;
-    for (uint32_t i = threadIdx.x; i < ne; i += kBlock) {
-        const size_t e = e0 + i;
-        const uint32_t l0 = a.lvid[e], l1 = a.lvid[T + e], l2 = a.lvid[2 * T + e], l3 = a.lvid[3 * T + e];
-        float R[9];
-#pragma unroll
-        for (int k = 0; k < 9; k++) R[k] = a.R[k * T + e];
-        // Dm^-1 and V rebuilt from the rest positions staged in LDS with the
-        // same operations as k_precompute (bit-identical to the stored
-        // values): 12 B per tile node instead of 40 B per element from HBM.
-        float Dm[9];
-#pragma unroll
-        for (int c = 0; c < 3; c++) {
-            const float r0 = s_r[3 * l0 + c];
-            Dm[3 * c + 0] = s_r[3 * l1 + c] - r0;
-            Dm[3 * c + 1] = s_r[3 * l2 + c] - r0;
-            Dm[3 * c + 2] = s_r[3 * l3 + c] - r0;
-        }
-        const float detDm = refmath::det3(Dm);
-        const float V = detDm / 6.0f;
-        float G[9];
-        refmath::inv3(Dm, detDm, G);
-        float D[9];  // columns v1-v0, v2-v0, v3-v0
-#pragma unroll
-        for (int c = 0; c < 3; c++) {
-            const float b0 = s_v[3 * l0 + c];
-            D[3 * c + 0] = s_v[3 * l1 + c] - b0;
-            D[3 * c + 1] = s_v[3 * l2 + c] - b0;
-            D[3 * c + 2] = s_v[3 * l3 + c] - b0;
-        }
-        float RD[9];  // R^T D  (f_e: minus the rest edges Dm, the local displacement differences)
-#pragma unroll
-        for (int i3 = 0; i3 < 3; i3++)
-#pragma unroll
-            for (int j = 0; j < 3; j++)
-                RD[3 * i3 + j] = R[0 + i3] * D[0 + j] + R[3 + i3] * D[3 + j] + R[6 + i3] * D[6 + j];
-        if (MODE == kModeFe) {
-#pragma unroll
-            for (int k = 0; k < 9; k++) RD[k] -= Dm[k];
-        }
-        float H[9];  // RD G
-#pragma unroll
-        for (int i3 = 0; i3 < 3; i3++)
-#pragma unroll
-            for (int j = 0; j < 3; j++)
-                H[3 * i3 + j] = RD[3 * i3 + 0] * G[0 + j] + RD[3 * i3 + 1] * G[3 + j] + RD[3 * i3 + 2] * G[6 + j];
-        const float sxx = a.m.a_k1 * H[0] + a.m.a_nu * (H[4] + H[8]);
-        const float syy = a.m.a_k1 * H[4] + a.m.a_nu * (H[0] + H[8]);
-        const float szz = a.m.a_k1 * H[8] + a.m.a_nu * (H[0] + H[4]);
-        const float sxy = a.m.a_k2 * (H[1] + H[3]);
-        const float syz = a.m.a_k2 * (H[5] + H[7]);
-        const float szx = a.m.a_k2 * (H[2] + H[6]);
-        const float S[9] = {sxx, sxy, szx, sxy, syy, syz, szx, syz, szz};
-        float W[9];  // -V sigma G^T
-#pragma unroll
-        for (int r = 0; r < 3; r++)
-#pragma unroll
-            for (int c = 0; c < 3; c++)
-                W[3 * r + c] = -V * (S[3 * r + 0] * G[3 * c + 0] + S[3 * r + 1] * G[3 * c + 1] + S[3 * r + 2] * G[3 * c + 2]);
-#pragma unroll
-        for (int r = 0; r < 3; r++)
-#pragma unroll
-            for (int c = 0; c < 3; c++)
-                s_P[(3 * r + c) * a.max_te + i] = R[3 * r + 0] * W[0 + c] + R[3 * r + 1] * W[3 + c] + R[3 * r + 2] * W[6 + c];
+    if (tid < d.nn) {
+        g.v[0] = a.in[3 * (size_t)gid + 0]; g.v[1] = a.in[3 * (size_t)gid + 1]; g.v[2] = a.in[3 * (size_t)gid + 2];
+        g.r[0] = a.ref[3 * (size_t)gid + 0]; g.r[1] = a.ref[3 * (size_t)gid + 1]; g.r[2] = a.ref[3 * (size_t)gid + 2];
     }
-    __syncthreads();
+    if (tid <= d.nn) g.sp = a.slot_ptr[d.n0 + tid] - 4 * d.e0;
+    if (tid < d.nn) g.pd = a.part_dst[d.n0 + tid];
+    if (tid < d.ne) {
+        const size_t e = d.e0 + tid;
+        g.l[0] = a.lvid[e]; g.l[1] = a.lvid[T + e]; g.l[2] = a.lvid[2 * T + e]; g.l[3] = a.lvid[3 * T + e];
+        g.f[0] = a.dst[e]; g.f[1] = a.dst[T + e]; g.f[2] = a.dst[2 * T + e]; g.f[3] = a.dst[3 * T + e];
+#pragma unroll
+        for (int k = 0; k < 9; k++) g.R[k] = a.R[k * T + e];
+    }
+}
 
-    for (uint32_t l = threadIdx.x; l < nn; l += kBlock) {
-        float fx = 0.f, fy = 0.f, fz = 0.f;
-        const uint32_t s1 = s_sp[l + 1];
-        for (uint32_t s = s_sp[l]; s < s1; s++) {
-            const uint32_t code = s_sl[s];
-            const uint32_t i = code >> 2, v = code & 3u;
-            if (v == 0) {
-                fx -= s_P[0 * a.max_te + i] + s_P[1 * a.max_te + i] + s_P[2 * a.max_te + i];
-                fy -= s_P[3 * a.max_te + i] + s_P[4 * a.max_te + i] + s_P[5 * a.max_te + i];
-                fz -= s_P[6 * a.max_te + i] + s_P[7 * a.max_te + i] + s_P[8 * a.max_te + i];
-            } else {
-                const uint32_t c = v - 1;
-                fx += s_P[(0 + c) * a.max_te + i];
-                fy += s_P[(3 + c) * a.max_te + i];
-                fz += s_P[(6 + c) * a.max_te + i];
-            }
+// Nodal forces of one element into its four tile force slots.
+template <int MODE>
+__device__ inline void element_force(const float4* s_v, const float4* s_r, float4* s_F, const uint32_t* lv,
+                                     const uint32_t* fs, const float* R, const Material& m) {
+    const float4 x0 = s_v[lv[0]], x1 = s_v[lv[1]], x2 = s_v[lv[2]], x3 = s_v[lv[3]];
+    const float4 r0 = s_r[lv[0]], r1 = s_r[lv[1]], r2 = s_r[lv[2]], r3 = s_r[lv[3]];
+    // Dm^-1 and V rebuilt from the rest positions staged in LDS with the same
+    // operations as k_precompute (bit-identical to the stored values): 12 B
+    // per tile node instead of 40 B per element from HBM.
+    const float Dm[9] = {r1.x - r0.x, r2.x - r0.x, r3.x - r0.x,
+                         r1.y - r0.y, r2.y - r0.y, r3.y - r0.y,
+                         r1.z - r0.z, r2.z - r0.z, r3.z - r0.z};
+    const float detDm = refmath::det3(Dm);
+    const float V = detDm / 6.0f;
+    float G[9];
+    refmath::inv3(Dm, detDm, G);
+    const float D[9] = {x1.x - x0.x, x2.x - x0.x, x3.x - x0.x,   // columns v1-v0, v2-v0, v3-v0
+                        x1.y - x0.y, x2.y - x0.y, x3.y - x0.y,
+                        x1.z - x0.z, x2.z - x0.z, x3.z - x0.z};
+    float RD[9];  // R^T D  (f_e: minus the rest edges Dm, the local displacement differences)
+#pragma unroll
+    for (int i3 = 0; i3 < 3; i3++)
+#pragma unroll
+        for (int j = 0; j < 3; j++)
+            RD[3 * i3 + j] = R[0 + i3] * D[0 + j] + R[3 + i3] * D[3 + j] + R[6 + i3] * D[6 + j];
+    if (MODE == kModeFe) {
+#pragma unroll
+        for (int k = 0; k < 9; k++) RD[k] -= Dm[k];
+    }
+    float H[9];  // RD G
+#pragma unroll
+    for (int i3 = 0; i3 < 3; i3++)
+#pragma unroll
+        for (int j = 0; j < 3; j++)
+            H[3 * i3 + j] = RD[3 * i3 + 0] * G[0 + j] + RD[3 * i3 + 1] * G[3 + j] + RD[3 * i3 + 2] * G[6 + j];
+    const float sxx = m.a_k1 * H[0] + m.a_nu * (H[4] + H[8]);
+    const float syy = m.a_k1 * H[4] + m.a_nu * (H[0] + H[8]);
+    const float szz = m.a_k1 * H[8] + m.a_nu * (H[0] + H[4]);
+    const float sxy = m.a_k2 * (H[1] + H[3]);
+    const float syz = m.a_k2 * (H[5] + H[7]);
+    const float szx = m.a_k2 * (H[2] + H[6]);
+    const float S[9] = {sxx, sxy, szx, sxy, syy, syz, szx, syz, szz};
+    float W[9];  // -V sigma G^T
+#pragma unroll
+    for (int r = 0; r < 3; r++)
+#pragma unroll
+        for (int c = 0; c < 3; c++)
+            W[3 * r + c] = -V * (S[3 * r + 0] * G[3 * c + 0] + S[3 * r + 1] * G[3 * c + 1] + S[3 * r + 2] * G[3 * c + 2]);
+    float P[9];  // R W: column c is the force on vertex c+1
+#pragma unroll
+    for (int r = 0; r < 3; r++)
+#pragma unroll
+        for (int c = 0; c < 3; c++)
+            P[3 * r + c] = R[3 * r + 0] * W[0 + c] + R[3 * r + 1] * W[3 + c] + R[3 * r + 2] * W[6 + c];
+    s_F[fs[1]] = make_float4(P[0], P[3], P[6], 0.f);
+    s_F[fs[2]] = make_float4(P[1], P[4], P[7], 0.f);
+    s_F[fs[3]] = make_float4(P[2], P[5], P[8], 0.f);
+    s_F[fs[0]] = make_float4(-(P[0] + P[1] + P[2]), -(P[3] + P[4] + P[5]), -(P[6] + P[7] + P[8]), 0.f);
+}
+
+// Tile kernel: one element and at most one tile node per thread. A workgroup
+// walks its tiles; while tile t is computed from LDS, the gathers of tile t+1
+// (node vectors, rest positions, element data) and the node ids of tile t+2
+// are in flight (LDS-only barriers do not wait for them). With a grid of one
+// workgroup per tile each workgroup handles a single tile. Workgroups are
+// dealt round-robin to the 8 XCDs; each XCD owns a contiguous range of tiles,
+// so tiles that share halo nodes share an L2.
+template <int MODE, int TB>
+__global__ __launch_bounds__(TB) void k_tile_forces(TileArgs a) {
+    if (a.skip && *a.skip) return;
+    extern __shared__ float4 smem4[];
+    float4* s_v = smem4;                                 // max_tn
+    float4* s_r = s_v + a.max_tn;                        // max_tn
+    float4* s_F = s_r + a.max_tn;                        // 4*max_te force slots
+    uint32_t* s_sp = reinterpret_cast<uint32_t*>(s_F + 4 * a.max_te);  // max_tn+1
+
+    const uint32_t tid = threadIdx.x;
+    const uint32_t gx = gridDim.x / 8, xcd = blockIdx.x % 8, j = blockIdx.x / 8;
+    const uint32_t per = (a.num_tiles + 7) / 8;
+    const uint32_t lo = xcd * per;
+    const uint32_t hi = min(a.num_tiles, lo + per);
+    uint32_t t = lo + j;
+    if (t >= hi) return;
+
+    TileDesc dt = tile_desc(a, t);
+    TileRegs g;
+    g.sp = 0;
+    g.pd = 0;
+    {
+        const uint32_t gid = (tid < dt.nn) ? a.tile_nodes[dt.n0 + tid] : 0u;
+        load_tile_regs(a, dt, gid, tid, g);
+    }
+    uint32_t t1 = t + gx;
+    TileDesc d1 = dt;
+    uint32_t gid1 = 0;
+    if (t1 < hi) {
+        d1 = tile_desc(a, t1);
+        gid1 = (tid < d1.nn) ? a.tile_nodes[d1.n0 + tid] : 0u;
+    }
+    while (true) {
+        // stage tile t into LDS
+        if (tid < dt.nn) {
+            s_v[tid] = make_float4(g.v[0], g.v[1], g.v[2], 0.f);
+            s_r[tid] = make_float4(g.r[0], g.r[1], g.r[2], 0.f);
         }
-        float* o = a.partials + 3 * (size_t)(n0 + l);
-        o[0] = fx; o[1] = fy; o[2] = fz;
+        if (tid <= dt.nn) s_sp[tid] = g.sp;
+        const uint32_t cpd = g.pd;
+        const uint32_t cl[4] = {g.l[0], g.l[1], g.l[2], g.l[3]};
+        const uint32_t cf[4] = {g.f[0], g.f[1], g.f[2], g.f[3]};
+        float cR[9];
+#pragma unroll
+        for (int k = 0; k < 9; k++) cR[k] = g.R[k];
+        lds_barrier();
+        // prefetch: gathers of tile t1, node ids of tile t2
+        const uint32_t t2 = t1 + gx;
+        TileDesc d2 = d1;
+        uint32_t gid2 = 0;
+        if (t1 < hi) load_tile_regs(a, d1, gid1, tid, g);
+        if (t2 < hi) {
+            d2 = tile_desc(a, t2);
+            gid2 = (tid < d2.nn) ? a.tile_nodes[d2.n0 + tid] : 0u;
+        }
+        // element forces of tile t into the force slots
+        if (tid < dt.ne) element_force<MODE>(s_v, s_r, s_F, cl, cf, cR, a.m);
+        lds_barrier();
+        // per tile node: its contiguous slot run (ascending element order) -> one partial
+        if (tid < dt.nn) {
+            float fx = 0.f, fy = 0.f, fz = 0.f;
+            const uint32_t s1 = s_sp[tid + 1];
+            for (uint32_t sidx = s_sp[tid]; sidx < s1; sidx++) {
+                const float4 f = s_F[sidx];
+                fx += f.x; fy += f.y; fz += f.z;
+            }
+            float* o = a.partials + 3 * (size_t)cpd;
+            o[0] = fx; o[1] = fy; o[2] = fz;
+        }
+        if (t1 >= hi) break;
+        lds_barrier();
+        t = t1; dt = d1;
+        t1 = t2; d1 = d2; gid1 = gid2;
     }
 }
 
@@ -508,7 +573,6 @@ __global__ __launch_bounds__(kBlock) void k_tile_forces(TileArgs a) {
 struct GatherArgs {
     uint32_t N;
     const uint32_t* node_part_ptr;
-    const uint32_t* node_parts;
     const float* partials;
     NodeTerms t;
     const int32_t* skip;
@@ -523,7 +587,7 @@ __global__ __launch_bounds__(kBlock) void k_gather(GatherArgs a) {
         float fx = 0.f, fy = 0.f, fz = 0.f;
         const uint32_t p1 = a.node_part_ptr[n + 1];
         for (uint32_t p = a.node_part_ptr[n]; p < p1; p++) {
-            const float* pp = a.partials + 3 * (size_t)a.node_parts[p];
+            const float* pp = a.partials + 3 * (size_t)p;
             fx += pp[0]; fy += pp[1]; fz += pp[2];
         }
         const size_t i = 3 * (size_t)n;
@@ -855,10 +919,9 @@ int launch_begin_eval(const TetDevice& t, const float* x, int rm, float thr, flo
 }
 
 static size_t tile_smem(const TetDevice& t) {
-    size_t b = sizeof(float) * (6 * (size_t)t.max_tile_nodes + 9 * (size_t)t.max_tile_elems);
+    size_t b = sizeof(float4) * (2 * (size_t)t.max_tile_nodes + 4 * (size_t)t.max_tile_elems);
     b += sizeof(uint32_t) * ((size_t)t.max_tile_nodes + 1);
-    b += sizeof(uint16_t) * 4 * (size_t)t.max_tile_elems;
-    return (b + 15) & ~size_t(15);
+    return (b + 63) & ~size_t(63);
 }
 
 int launch_forces_skip(const TetDevice& t, int mode, const float* in, const float* R, Material m,
@@ -867,16 +930,48 @@ int launch_forces_skip(const TetDevice& t, int mode, const float* in, const floa
     TileArgs a;
     a.T = t.T; a.num_tiles = t.num_tiles; a.max_te = t.max_tile_elems; a.max_tn = t.max_tile_nodes;
     a.tile_elem_ptr = t.tile_elem_ptr; a.tile_node_ptr = t.tile_node_ptr; a.tile_nodes = t.tile_nodes;
-    a.slot_ptr = t.slot_ptr; a.slots = t.slots; a.lvid = t.lvid; a.R = R; a.dinv = t.dinv; a.vol = t.vol;
+    a.part_dst = t.part_dst;
+    a.slot_ptr = t.slot_ptr; a.dst = t.dst; a.lvid = t.lvid; a.R = R; a.dinv = t.dinv; a.vol = t.vol;
     a.ref = t.ref;
     a.in = in; a.partials = t.partials; a.m = m; a.skip = skip;
     const size_t sm = tile_smem(t);
-    const uint32_t grid = ((t.num_tiles + 7) / 8) * 8;
-    if (mode == kModeFe)
-        hipLaunchKernelGGL(k_tile_forces<kModeFe>, dim3(grid), dim3(kBlock), sm, s, a);
-    else
-        hipLaunchKernelGGL(k_tile_forces<kModeDfx>, dim3(grid), dim3(kBlock), sm, s, a);
+    const uint32_t grid = t.tile_grid;
+    switch (t.tile_threads) {
+        case 1024:
+            if (mode == kModeFe) hipLaunchKernelGGL((k_tile_forces<kModeFe, 1024>), dim3(grid), dim3(1024), sm, s, a);
+            else hipLaunchKernelGGL((k_tile_forces<kModeDfx, 1024>), dim3(grid), dim3(1024), sm, s, a);
+            break;
+        case 512:
+            if (mode == kModeFe) hipLaunchKernelGGL((k_tile_forces<kModeFe, 512>), dim3(grid), dim3(512), sm, s, a);
+            else hipLaunchKernelGGL((k_tile_forces<kModeDfx, 512>), dim3(grid), dim3(512), sm, s, a);
+            break;
+        default:
+            if (mode == kModeFe) hipLaunchKernelGGL((k_tile_forces<kModeFe, 256>), dim3(grid), dim3(256), sm, s, a);
+            else hipLaunchKernelGGL((k_tile_forces<kModeDfx, 256>), dim3(grid), dim3(256), sm, s, a);
+            break;
+    }
     return check_launch();
+}
+
+// Grid of the tile kernel, a multiple of 8 (one share per XCD): one workgroup
+// per tile, or (persistent) the resident workgroups per CU from the occupancy
+// calculator for this LDS size times the CUs, at most one per tile.
+int tile_grid_size(const TetDevice& t, int device, int persistent) {
+    int cus = 0, per_cu = 0;
+    CF_CHECK_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
+    const size_t sm = tile_smem(t);
+    hipError_t e;
+    switch (t.tile_threads) {
+        case 1024: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_tile_forces<kModeDfx, 1024>, 1024, sm); break;
+        case 512: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_tile_forces<kModeDfx, 512>, 512, sm); break;
+        default: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_tile_forces<kModeDfx, 256>, 256, sm); break;
+    }
+    if (e != hipSuccess || per_cu < 1) per_cu = 1;
+    const uint32_t per_xcd_tiles = (t.num_tiles + 7) / 8;
+    uint32_t wg_per_xcd = persistent ? uint32_t(per_cu) * uint32_t(cus) / 8 : per_xcd_tiles;
+    if (wg_per_xcd < 1) wg_per_xcd = 1;
+    if (wg_per_xcd > per_xcd_tiles) wg_per_xcd = per_xcd_tiles;
+    return int(8 * wg_per_xcd);
 }
 
 int launch_forces(const TetDevice& t, int mode, const float* in, const float* R, Material m, hipStream_t s) {
@@ -886,7 +981,7 @@ int launch_forces(const TetDevice& t, int mode, const float* in, const float* R,
 int launch_gather_skip(const TetDevice& t, int mode, const NodeTerms& nt, int nblk, const int32_t* skip,
                        hipStream_t s) {
     GatherArgs a;
-    a.N = t.N; a.node_part_ptr = t.node_part_ptr; a.node_parts = t.node_parts; a.partials = t.partials;
+    a.N = t.N; a.node_part_ptr = t.node_part_ptr; a.partials = t.partials;
     a.t = nt; a.skip = skip;
     if (mode == kGatherAccumulate)
         hipLaunchKernelGGL(k_gather<kGatherAccumulate>, dim3(nblk), dim3(kBlock), 0, s, a);

@@ -1,14 +1,14 @@
 // Host construction of the LDS tiles used by the atomics-free node gather.
 //
 // Elements are sorted along a Morton (Z-order) curve of their rest centroids
-// and cut into tiles of at most `tile_elems` elements and kMaxTileNodes
-// distinct nodes. For every tile we record
+// and cut into tiles of at most `tile_elems` elements and as many distinct
+// nodes as the tile kernel has threads (one element and one node per thread). For every tile we record
 //   - its distinct nodes (ascending global id) -> one partial per (tile, node)
 //   - per element, the tile-local index of each of its 4 vertices (uint16)
 //   - per tile node, the (local element, vertex) slots that touch it, in
 //     ascending element order (the order the reference's serial scatter would
 //     accumulate them), as uint16 codes elem*4+vertex
-// and, per global node, the list of its partials in ascending tile order.
+// and, per global node, the contiguous run of its partials in ascending tile order.
 // All of this is integer work done once per mesh.
 #include "cf_internal.h"
 #include <algorithm>
@@ -20,10 +20,10 @@ namespace cf {
 namespace {
 constexpr uint32_t kMaxTileNodes = 1024;
 // LDS bytes of one tile in k_tile_forces: node vectors and rest positions
-// (24 B), per-element nodal forces (36 B), slot offsets (4 B per node) and
-// slot codes (8 B per element).
+// (2 x float4), 4 force slots per element (float4 each) and slot offsets
+// (4 B per node).
 constexpr uint32_t kMaxTileLds = 64 * 1024 - 512;
-inline uint32_t tile_lds(uint32_t ne, uint32_t nn) { return 28 * nn + 44 * ne + 4; }
+inline uint32_t tile_lds(uint32_t ne, uint32_t nn) { return 36 * nn + 64 * ne + 64; }
 
 inline uint64_t spread3(uint64_t v) {  // 21 bits -> every third bit
     v &= 0x1fffff;
@@ -36,12 +36,14 @@ inline uint64_t spread3(uint64_t v) {  // 21 bits -> every third bit
 }
 }  // namespace
 
-uint32_t default_tile_elems() { return 640; }
+uint32_t default_tile_elems() { return 256; }
 
 int build_tiles(uint32_t N, uint32_t T, const float* ref, const uint32_t* tets, uint32_t tile_elems,
-                TileHost& out) {
+                uint32_t max_per_tile, TileHost& out) {
     if (tile_elems == 0) tile_elems = default_tile_elems();
+    if (tile_elems > max_per_tile) tile_elems = max_per_tile;  // one element per thread
     if (tile_elems > 16383) tile_elems = 16383;  // slot codes are uint16 (elem*4+vertex)
+    const uint32_t max_nodes = std::min(kMaxTileNodes, max_per_tile);  // one node per thread
     // ---- Morton order of centroids
     float lo[3] = {3e38f, 3e38f, 3e38f}, hi[3] = {-3e38f, -3e38f, -3e38f};
     for (uint32_t n = 0; n < N; n++)
@@ -75,8 +77,7 @@ int build_tiles(uint32_t N, uint32_t T, const float* ref, const uint32_t* tets, 
     out.tile_node_ptr.assign(1, 0);
     out.tile_nodes.clear();
     out.slot_ptr.assign(1, 0);
-    out.slots.clear();
-    out.slots.reserve(4 * (size_t)T);
+    out.dst_planes.assign(4 * (size_t)T, 0);
     out.conn_planes.assign(4 * (size_t)T, 0);
     out.lvid_planes.assign(4 * (size_t)T, 0);
     out.max_tile_elems = 0;
@@ -92,30 +93,43 @@ int build_tiles(uint32_t N, uint32_t T, const float* ref, const uint32_t* tets, 
             uint32_t fresh = 0;
             for (int j = 0; j < 4; j++) fresh += (local[t[j]] < 0) ? 1u : 0u;
             const uint32_t nn1 = uint32_t(nodes.size()) + fresh;
-            if (nn1 > kMaxTileNodes || tile_lds(e - e0 + 1, nn1) > kMaxTileLds) break;
+            if (nn1 > max_nodes || tile_lds(e - e0 + 1, nn1) > kMaxTileLds) break;
             for (int j = 0; j < 4; j++)
                 if (local[t[j]] < 0) { local[t[j]] = 0; nodes.push_back(t[j]); }
             e++;
         }
         const uint32_t ne = e - e0;
+        // valence (slots) of each tile node; nodes ordered by descending valence
+        // (ties by id) so that the lanes of a wave sum runs of similar length
         std::sort(nodes.begin(), nodes.end());
         for (uint32_t l = 0; l < nodes.size(); l++) local[nodes[l]] = int32_t(l);
-        // per tile node slot lists in ascending local element order
+        std::vector<uint32_t> val(nodes.size(), 0);
+        for (uint32_t i = 0; i < ne; i++) {
+            const uint32_t* t = &tets[4 * (size_t)out.perm[e0 + i]];
+            for (int j = 0; j < 4; j++) val[local[t[j]]]++;
+        }
+        std::vector<uint32_t> order(nodes.size());
+        std::iota(order.begin(), order.end(), 0u);
+        std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return val[a] > val[b]; });
+        std::vector<uint32_t> sorted_nodes(nodes.size());
+        for (uint32_t r = 0; r < order.size(); r++) sorted_nodes[r] = nodes[order[r]];
+        nodes.swap(sorted_nodes);
+        for (uint32_t l = 0; l < nodes.size(); l++) local[nodes[l]] = int32_t(l);
+        // slot ranges per tile node; within a node, slots in ascending element order
         std::vector<uint32_t> cnt(nodes.size() + 1, 0);
         for (uint32_t i = 0; i < ne; i++) {
             const uint32_t* t = &tets[4 * (size_t)out.perm[e0 + i]];
             for (int j = 0; j < 4; j++) cnt[local[t[j]] + 1]++;
         }
         for (size_t l = 0; l < nodes.size(); l++) cnt[l + 1] += cnt[l];
-        const size_t sbase = out.slots.size();
-        out.slots.resize(sbase + 4 * (size_t)ne);
+        const size_t sbase = 4 * (size_t)e0;
         std::vector<uint32_t> fill(cnt.begin(), cnt.end() - 1);
         for (uint32_t i = 0; i < ne; i++) {
             const size_t ie = e0 + i;
             const uint32_t* t = &tets[4 * (size_t)out.perm[ie]];
             for (int j = 0; j < 4; j++) {
                 const uint32_t l = uint32_t(local[t[j]]);
-                out.slots[sbase + fill[l]++] = uint16_t(i * 4 + j);
+                out.dst_planes[j * (size_t)T + ie] = uint16_t(fill[l]++);
                 out.conn_planes[j * (size_t)T + ie] = t[j];
                 out.lvid_planes[j * (size_t)T + ie] = uint16_t(l);
             }
@@ -130,14 +144,16 @@ int build_tiles(uint32_t N, uint32_t T, const float* ref, const uint32_t* tets, 
         out.max_tile_elems = std::max(out.max_tile_elems, ne);
         out.max_tile_nodes = std::max(out.max_tile_nodes, uint32_t(nodes.size()));
     }
-    // ---- per global node: its partials in ascending tile order
+    // ---- partials stored node-major: node n owns [node_part_ptr[n], node_part_ptr[n+1]),
+    // one entry per tile touching it in ascending tile order; part_dst maps each
+    // (tile, tile node) to its entry
     const uint32_t P = uint32_t(out.tile_nodes.size());
     out.node_part_ptr.assign((size_t)N + 1, 0);
     for (uint32_t p = 0; p < P; p++) out.node_part_ptr[out.tile_nodes[p] + 1]++;
     for (uint32_t n = 0; n < N; n++) out.node_part_ptr[n + 1] += out.node_part_ptr[n];
-    out.node_parts.assign(P, 0);
+    out.part_dst.assign(P, 0);
     std::vector<uint32_t> fill(out.node_part_ptr.begin(), out.node_part_ptr.end() - 1);
-    for (uint32_t p = 0; p < P; p++) out.node_parts[fill[out.tile_nodes[p]]++] = p;
+    for (uint32_t p = 0; p < P; p++) out.part_dst[p] = fill[out.tile_nodes[p]]++;
     return CF_OK;
 }
 

@@ -134,6 +134,14 @@ def set_tile_elems(n):
     check(lib.cf_set_tile_elems(int(n)), "cf_set_tile_elems")
 
 
+def set_tile_threads(n):
+    check(lib.cf_set_tile_threads(int(n)), "cf_set_tile_threads")
+
+
+def set_tile_persistent(on):
+    check(lib.cf_set_tile_persistent(1 if on else 0), "cf_set_tile_persistent")
+
+
 def get_tile_elems():
     v = C.c_uint32()
     check(lib.cf_get_tile_elems(C.byref(v)), "cf_get_tile_elems")
