@@ -97,6 +97,7 @@ def main():
     ap.add_argument("--kx-reps", type=int, default=50)
     ap.add_argument("--tile-elems", type=int, default=256)
     ap.add_argument("--tile-threads", type=int, default=256)
+    ap.add_argument("--tile-schedule", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--kx-only", type=int, default=0, metavar="REPS",
                     help="profiling mode: one step, then REPS K*x launches and one 256 MiB device copy "
@@ -114,6 +115,7 @@ def main():
 
     cf.set_tile_elems(args.tile_elems)
     cf.set_tile_threads(args.tile_threads)
+    cf.set_tile_persistent(args.tile_schedule)
     counts = (65, 65, 257)
     half = (0.32, 0.32, 1.28)
     pos, tets = cf.mesh_box(half, counts)
@@ -167,7 +169,7 @@ def main():
         "traffic": traffic.get("bytes_per_kx") if traffic else None,
         "kernels": {"k_tile_forces_ms": round(ms_tile, 5), "k_gather_ms": round(ms_gather, 5)},
         "b_alg_bytes": b_alg, "kx_gtets": round(T / (ms_kx * 1e-3) / 1e9, 2),
-        "tile": {"elems": args.tile_elems, "threads": args.tile_threads},
+        "tile": {"elems": args.tile_elems, "threads": args.tile_threads, "schedule": args.tile_schedule},
     }
     cpu = None
     if not args.no_cpu_baseline:

@@ -33,7 +33,7 @@ Material material_from(float E, float nu) {
 
 static uint32_t g_tile_elems = 0;
 static uint32_t g_tile_threads = 256;
-static int g_tile_persistent = 0;
+static int g_tile_persistent = 2;
 
 template <class T>
 static int dalloc(T** p, size_t count) {
@@ -213,6 +213,7 @@ static int tetset_bake(cf_tetset* s) {
     t.max_tile_nodes = th.max_tile_nodes;
     t.num_parts = uint32_t(th.tile_nodes.size());
     t.tile_threads = g_tile_threads;
+    t.tile_persistent = g_tile_persistent;
     t.tile_grid = uint32_t(tile_grid_size(t, s->device, g_tile_persistent));
     CF_TRY(dalloc(&t.partials, 3 * (size_t)t.num_parts));
     CF_TRY(launch_precompute(t, st));
@@ -1037,8 +1038,8 @@ int cf_set_tile_elems(uint32_t te) {
     return CF_OK;
 }
 int cf_set_tile_threads(uint32_t n) {
-    if (n != 0 && n != 256 && n != 512 && n != 1024) {
-        set_error("tile threads must be 256, 512 or 1024");
+    if (n != 0 && n != 128 && n != 256 && n != 512 && n != 1024) {
+        set_error("tile threads must be 128, 256, 512 or 1024");
         return CF_ERR_INVALID_ARG;
     }
     g_tile_threads = n ? n : 256;
@@ -1046,7 +1047,8 @@ int cf_set_tile_threads(uint32_t n) {
 }
 
 int cf_set_tile_persistent(int persistent) {
-    g_tile_persistent = persistent ? 1 : 0;
+    if (persistent < 0 || persistent > 2) { set_error("tile schedule must be 0, 1 or 2"); return CF_ERR_INVALID_ARG; }
+    g_tile_persistent = persistent;
     return CF_OK;
 }
 

@@ -493,7 +493,7 @@ __device__ inline void element_force(const float4* s_v, const float4* s_r, float
 // workgroup per tile each workgroup handles a single tile. Workgroups are
 // dealt round-robin to the 8 XCDs; each XCD owns a contiguous range of tiles,
 // so tiles that share halo nodes share an L2.
-template <int MODE, int TB>
+template <int MODE, int TB, bool PIPE>
 __global__ __launch_bounds__(TB) void k_tile_forces(TileArgs a) {
     if (a.skip && *a.skip) return;
     extern __shared__ float4 smem4[];
@@ -521,7 +521,7 @@ __global__ __launch_bounds__(TB) void k_tile_forces(TileArgs a) {
     uint32_t t1 = t + gx;
     TileDesc d1 = dt;
     uint32_t gid1 = 0;
-    if (t1 < hi) {
+    if (PIPE && t1 < hi) {
         d1 = tile_desc(a, t1);
         gid1 = (tid < d1.nn) ? a.tile_nodes[d1.n0 + tid] : 0u;
     }
@@ -543,8 +543,8 @@ __global__ __launch_bounds__(TB) void k_tile_forces(TileArgs a) {
         const uint32_t t2 = t1 + gx;
         TileDesc d2 = d1;
         uint32_t gid2 = 0;
-        if (t1 < hi) load_tile_regs(a, d1, gid1, tid, g);
-        if (t2 < hi) {
+        if (PIPE && t1 < hi) load_tile_regs(a, d1, gid1, tid, g);
+        if (PIPE && t2 < hi) {
             d2 = tile_desc(a, t2);
             gid2 = (tid < d2.nn) ? a.tile_nodes[d2.n0 + tid] : 0u;
         }
@@ -562,7 +562,7 @@ __global__ __launch_bounds__(TB) void k_tile_forces(TileArgs a) {
             float* o = a.partials + 3 * (size_t)cpd;
             o[0] = fx; o[1] = fy; o[2] = fz;
         }
-        if (t1 >= hi) break;
+        if (!PIPE || t1 >= hi) break;
         lds_barrier();
         t = t1; dt = d1;
         t1 = t2; d1 = d2; gid1 = gid2;
@@ -936,20 +936,28 @@ int launch_forces_skip(const TetDevice& t, int mode, const float* in, const floa
     a.in = in; a.partials = t.partials; a.m = m; a.skip = skip;
     const size_t sm = tile_smem(t);
     const uint32_t grid = t.tile_grid;
-    switch (t.tile_threads) {
-        case 1024:
-            if (mode == kModeFe) hipLaunchKernelGGL((k_tile_forces<kModeFe, 1024>), dim3(grid), dim3(1024), sm, s, a);
-            else hipLaunchKernelGGL((k_tile_forces<kModeDfx, 1024>), dim3(grid), dim3(1024), sm, s, a);
-            break;
-        case 512:
-            if (mode == kModeFe) hipLaunchKernelGGL((k_tile_forces<kModeFe, 512>), dim3(grid), dim3(512), sm, s, a);
-            else hipLaunchKernelGGL((k_tile_forces<kModeDfx, 512>), dim3(grid), dim3(512), sm, s, a);
-            break;
-        default:
-            if (mode == kModeFe) hipLaunchKernelGGL((k_tile_forces<kModeFe, 256>), dim3(grid), dim3(256), sm, s, a);
-            else hipLaunchKernelGGL((k_tile_forces<kModeDfx, 256>), dim3(grid), dim3(256), sm, s, a);
-            break;
+    const bool fe = mode == kModeFe;
+#define CF_TILE_LAUNCH(TBV, PV)                                                                        \
+    do {                                                                                              \
+        if (fe) hipLaunchKernelGGL((k_tile_forces<kModeFe, TBV, PV>), dim3(grid), dim3(TBV), sm, s, a); \
+        else hipLaunchKernelGGL((k_tile_forces<kModeDfx, TBV, PV>), dim3(grid), dim3(TBV), sm, s, a);   \
+    } while (0)
+    if (t.tile_persistent) {
+        switch (t.tile_threads) {
+            case 1024: CF_TILE_LAUNCH(1024, true); break;
+            case 512: CF_TILE_LAUNCH(512, true); break;
+            case 128: CF_TILE_LAUNCH(128, true); break;
+            default: CF_TILE_LAUNCH(256, true); break;
+        }
+    } else {
+        switch (t.tile_threads) {
+            case 1024: CF_TILE_LAUNCH(1024, false); break;
+            case 512: CF_TILE_LAUNCH(512, false); break;
+            case 128: CF_TILE_LAUNCH(128, false); break;
+            default: CF_TILE_LAUNCH(256, false); break;
+        }
     }
+#undef CF_TILE_LAUNCH
     return check_launch();
 }
 
@@ -962,13 +970,14 @@ int tile_grid_size(const TetDevice& t, int device, int persistent) {
     const size_t sm = tile_smem(t);
     hipError_t e;
     switch (t.tile_threads) {
-        case 1024: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_tile_forces<kModeDfx, 1024>, 1024, sm); break;
-        case 512: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_tile_forces<kModeDfx, 512>, 512, sm); break;
-        default: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_tile_forces<kModeDfx, 256>, 256, sm); break;
+        case 1024: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_tile_forces<kModeDfx, 1024, true>, 1024, sm); break;
+        case 512: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_tile_forces<kModeDfx, 512, true>, 512, sm); break;
+        case 128: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_tile_forces<kModeDfx, 128, true>, 128, sm); break;
+        default: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_tile_forces<kModeDfx, 256, true>, 256, sm); break;
     }
     if (e != hipSuccess || per_cu < 1) per_cu = 1;
     const uint32_t per_xcd_tiles = (t.num_tiles + 7) / 8;
-    uint32_t wg_per_xcd = persistent ? uint32_t(per_cu) * uint32_t(cus) / 8 : per_xcd_tiles;
+    uint32_t wg_per_xcd = (persistent == 1) ? uint32_t(per_cu) * uint32_t(cus) / 8 : per_xcd_tiles;
     if (wg_per_xcd < 1) wg_per_xcd = 1;
     if (wg_per_xcd > per_xcd_tiles) wg_per_xcd = per_xcd_tiles;
     return int(8 * wg_per_xcd);

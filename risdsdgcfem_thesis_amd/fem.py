@@ -138,8 +138,8 @@ def set_tile_threads(n):
     check(lib.cf_set_tile_threads(int(n)), "cf_set_tile_threads")
 
 
-def set_tile_persistent(on):
-    check(lib.cf_set_tile_persistent(1 if on else 0), "cf_set_tile_persistent")
+def set_tile_persistent(mode):
+    check(lib.cf_set_tile_persistent(int(mode)), "cf_set_tile_persistent")
 
 
 def get_tile_elems():
