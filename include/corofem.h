@@ -238,6 +238,70 @@ int cf_set_tile_threads(uint32_t threads);
 int cf_set_tile_persistent(int persistent);
 int cf_get_tile_elems(uint32_t* tile_elems);
 
+/* ---------------------------------------------------------------- batches
+ * Many independent solids stepped together: each batched fixed step runs the
+ * step of cf_solid_fixed_step (QIE-v, LeafDSH_Solid3D_FEM.cpp:898-990,
+ * 1205-1328) for every active simulation of the batch in the same kernel
+ * launches. No reference counterpart: the reference steps one leaf at a time
+ * (SS_AggregateDSH_Basic.cpp:229-300 loops over leaves); this is the batching
+ * of many leaves of that loop.
+ *   - meshes (element-set topology, rest shape, tiles) are added once and
+ *     shared by any number of simulations;
+ *   - a simulation = its mesh + its own parameters, KNC and device state;
+ *   - each simulation's results are bit-identical to cf_solid_fixed_step on
+ *     the same mesh, parameters and state, whatever else is in the batch;
+ *   - device memory comes from the batch's own arena; simulations are
+ *     admitted while their bytes fit the HBM budget (CF_ERR_CAPACITY otherwise).
+ * The step is recorded as HIP graphs (prologue, a chunk of CG iterations,
+ * epilogue) re-recorded when the set of active simulations changes; the
+ * host reads one counter per chunk to stop when every solve has ended. */
+typedef struct cf_batch cf_batch;
+typedef struct cf_batch_info {
+    uint32_t num_meshes, num_sims, num_active;
+    uint64_t bytes_in_use;     /* meshes + simulations, against the budget */
+    uint64_t bytes_reserved;   /* device memory held by the arena */
+    uint64_t budget_bytes;
+    uint64_t active_tets, active_nodes;
+    uint32_t tile_threads, tile_elems, cg_chunk, use_graphs;
+} cf_batch_info;
+
+int cf_batch_create(uint64_t hbm_budget_bytes /* 0: no limit */, cf_batch** out);
+int cf_batch_destroy(cf_batch* b);
+/* Tiles of meshes added afterwards: threads 128/256/512/1024, elems <= threads (0: = threads). */
+int cf_batch_set_tiles(cf_batch* b, uint32_t tile_threads, uint32_t tile_elems);
+/* CG iterations per recorded chunk (1..1024; default 8) and graph use (default 1). */
+int cf_batch_set_schedule(cf_batch* b, uint32_t cg_chunk, int use_graphs);
+int cf_batch_add_mesh(cf_batch* b, uint32_t n_nodes, const float* h_ref_pos, uint32_t n_tets,
+                      const uint32_t* h_tets, uint32_t* mesh_id);
+/* count meshes at once, tiles built on up to `threads` host threads (0: hardware). */
+int cf_batch_add_meshes(cf_batch* b, uint32_t count, const uint32_t* n_nodes, const float* const* h_ref_pos,
+                        const uint32_t* n_tets, const uint32_t* const* h_tets, uint32_t threads,
+                        uint32_t* mesh_ids);
+/* Fails while a simulation uses the mesh. */
+int cf_batch_remove_mesh(cf_batch* b, uint32_t mesh_id);
+/* Bytes a simulation of this mesh takes from the budget. */
+int cf_batch_sim_bytes(const cf_batch* b, uint32_t mesh_id, uint64_t* bytes);
+/* New active simulation at rest (x = rest positions, v = 0) unless h_pos/h_vel
+   are given; KNC as cf_solid_set_knc (h_knc_targets NULL: rest positions). */
+int cf_batch_add_sim(cf_batch* b, uint32_t mesh_id, const cf_solid_params* params, uint32_t n_knc,
+                     const uint32_t* h_knc_nids, const float* h_knc_targets, const float* h_pos,
+                     const float* h_vel, uint32_t* sim_id);
+int cf_batch_remove_sim(cf_batch* b, uint32_t sim_id);
+/* Active simulations are stepped by cf_batch_fixed_step (new ones start active). */
+int cf_batch_set_active(cf_batch* b, uint32_t sim_id, int active);
+int cf_batch_get_state(cf_batch* b, uint32_t sim_id, float* h_pos, float* h_vel);
+int cf_batch_set_state(cf_batch* b, uint32_t sim_id, const float* h_pos, const float* h_vel);
+/* One fixed step (each simulation's own fixed_dt * time_scale) of every active
+   simulation. stats: NULL, or one entry per active simulation in the order of
+   cf_batch_active_ids (step_ms: device time of the whole batched step). */
+int cf_batch_fixed_step(cf_batch* b, cf_step_stats* stats);
+int cf_batch_active_ids(const cf_batch* b, uint32_t* ids /* num_active */);
+int cf_batch_info_get(const cf_batch* b, cf_batch_info* info);
+/* Mean device ms of one batched K*x over the active simulations (element
+   stage + node gather on the current rotations), reps launches. */
+int cf_batch_time_kx(cf_batch* b, uint32_t reps, double* ms);
+int cf_batch_synchronize(cf_batch* b);
+
 #ifdef __cplusplus
 }
 #endif

@@ -7,11 +7,11 @@ of its own: every operation runs in libcorofem.so, and importing the package
 fails loudly when the library has not been built.
 """
 from ._capi import (  # noqa: F401
-    lib, LIB_PATH, CorofemError, check, FemParams, SolidParams, StepStats, CgResult,
+    lib, LIB_PATH, CorofemError, check, FemParams, SolidParams, StepStats, CgResult, BatchInfo,
     RM_ID, RM_QR, RM_MSVD, RM_DAPD, MM_C_WRP, DM_TRUNCATED,
     CG_CONVERGED_REL, CG_CONVERGED_ABS, CG_MAX_ITER, CG_BREAKDOWN,
 )
 from .fem import (  # noqa: F401
-    DeviceArray, device_count, mesh_box, mesh_box2, TetrahedronElementSet3, ILSS_CG, LeafSolid3D,
+    DeviceArray, device_count, mesh_box, mesh_box2, TetrahedronElementSet3, ILSS_CG, LeafSolid3D, Batch,
     default_fem_params, default_solid_params, set_tile_elems, set_tile_threads, set_tile_persistent, get_tile_elems, synchronize,
 )

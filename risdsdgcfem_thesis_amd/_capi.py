@@ -47,6 +47,16 @@ class CgResult(C.Structure):
                 ("reason", C.c_int32)]
 
 
+class BatchInfo(C.Structure):
+    _fields_ = [
+        ("num_meshes", C.c_uint32), ("num_sims", C.c_uint32), ("num_active", C.c_uint32),
+        ("bytes_in_use", C.c_uint64), ("bytes_reserved", C.c_uint64), ("budget_bytes", C.c_uint64),
+        ("active_tets", C.c_uint64), ("active_nodes", C.c_uint64),
+        ("tile_threads", C.c_uint32), ("tile_elems", C.c_uint32), ("cg_chunk", C.c_uint32),
+        ("use_graphs", C.c_uint32),
+    ]
+
+
 AV_FN = C.CFUNCTYPE(None, _vp, _f, _f)
 CONS_FN = C.CFUNCTYPE(None, _vp, _f)
 
@@ -110,6 +120,24 @@ SIGNATURES = {
     "cf_get_tile_elems": (C.c_int, [_u]),
     "cf_set_tile_threads": (C.c_int, [C.c_uint32]),
     "cf_set_tile_persistent": (C.c_int, [C.c_int]),
+    "cf_batch_create": (C.c_int, [C.c_uint64, C.POINTER(_vp)]),
+    "cf_batch_destroy": (C.c_int, [_vp]),
+    "cf_batch_set_tiles": (C.c_int, [_vp, C.c_uint32, C.c_uint32]),
+    "cf_batch_set_schedule": (C.c_int, [_vp, C.c_uint32, C.c_int]),
+    "cf_batch_add_mesh": (C.c_int, [_vp, C.c_uint32, _f, C.c_uint32, _u, _u]),
+    "cf_batch_add_meshes": (C.c_int, [_vp, C.c_uint32, _u, C.POINTER(_f), _u, C.POINTER(_u), C.c_uint32, _u]),
+    "cf_batch_remove_mesh": (C.c_int, [_vp, C.c_uint32]),
+    "cf_batch_sim_bytes": (C.c_int, [_vp, C.c_uint32, C.POINTER(C.c_uint64)]),
+    "cf_batch_add_sim": (C.c_int, [_vp, C.c_uint32, C.POINTER(SolidParams), C.c_uint32, _u, _f, _f, _f, _u]),
+    "cf_batch_remove_sim": (C.c_int, [_vp, C.c_uint32]),
+    "cf_batch_set_active": (C.c_int, [_vp, C.c_uint32, C.c_int]),
+    "cf_batch_get_state": (C.c_int, [_vp, C.c_uint32, _f, _f]),
+    "cf_batch_set_state": (C.c_int, [_vp, C.c_uint32, _f, _f]),
+    "cf_batch_fixed_step": (C.c_int, [_vp, C.POINTER(StepStats)]),
+    "cf_batch_active_ids": (C.c_int, [_vp, _u]),
+    "cf_batch_info_get": (C.c_int, [_vp, C.POINTER(BatchInfo)]),
+    "cf_batch_time_kx": (C.c_int, [_vp, C.c_uint32, C.POINTER(C.c_double)]),
+    "cf_batch_synchronize": (C.c_int, [_vp]),
 }
 
 

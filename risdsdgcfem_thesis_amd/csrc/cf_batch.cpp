@@ -1,0 +1,893 @@
+// Batched stepping of many solids (cf_batch_* in include/corofem.h).
+//
+// Host side of the batch: a device arena that sub-allocates meshes and
+// simulations (so admission and removal do not call hipMalloc/hipFree, which
+// synchronise the device), shared mesh topologies, simulations in slots of a
+// device descriptor array, the block tables of the batched kernels, and the
+// step recorded as HIP graphs.
+#include "cf_internal.h"
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <new>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+namespace cf {
+
+// ---------------------------------------------------------------- arena
+// First-fit sub-allocator over device chunks of at least kChunk bytes, 256 B
+// alignment, neighbours coalesced on free. Chunks are returned to HIP only
+// when the batch is destroyed.
+class Arena {
+public:
+    static constexpr size_t kAlign = 256;
+    static constexpr size_t kChunk = size_t(1) << 30;
+    ~Arena() { release(); }
+    int alloc(size_t bytes, char** out) {
+        bytes = round(bytes ? bytes : 1);
+        for (size_t c = 0; c < chunks_.size(); c++) {
+            auto& fr = chunks_[c].free;
+            for (auto it = fr.begin(); it != fr.end(); ++it) {
+                if (it->second < bytes) continue;
+                const size_t off = it->first, sz = it->second;
+                fr.erase(it);
+                if (sz > bytes) fr[off + bytes] = sz - bytes;
+                char* p = chunks_[c].base + off;
+                live_[p] = {c, bytes};
+                in_use_ += bytes;
+                *out = p;
+                return CF_OK;
+            }
+        }
+        const size_t csz = std::max(kChunk, bytes);
+        char* base = nullptr;
+        hipError_t e = hipMalloc(reinterpret_cast<void**>(&base), csz);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            set_error("batch arena: hipMalloc(%zu) failed: %s", csz, hipGetErrorString(e));
+            return e == hipErrorOutOfMemory ? CF_ERR_OUT_OF_MEMORY : CF_ERR_HIP;
+        }
+        chunks_.push_back(Chunk{base, csz, {}});
+        reserved_ += csz;
+        if (csz > bytes) chunks_.back().free[bytes] = csz - bytes;
+        live_[base] = {chunks_.size() - 1, bytes};
+        in_use_ += bytes;
+        *out = base;
+        return CF_OK;
+    }
+    void free(char* p) {
+        if (!p) return;
+        auto it = live_.find(p);
+        if (it == live_.end()) return;
+        const size_t c = it->second.first, sz = it->second.second;
+        live_.erase(it);
+        in_use_ -= sz;
+        auto& fr = chunks_[c].free;
+        size_t off = size_t(p - chunks_[c].base), len = sz;
+        auto nx = fr.lower_bound(off);
+        if (nx != fr.end() && nx->first == off + len) { len += nx->second; nx = fr.erase(nx); }
+        if (nx != fr.begin()) {
+            auto pv = std::prev(nx);
+            if (pv->first + pv->second == off) { off = pv->first; len += pv->second; fr.erase(pv); }
+        }
+        fr[off] = len;
+    }
+    void release() {
+        for (auto& c : chunks_) (void)hipFree(c.base);
+        chunks_.clear();
+        live_.clear();
+        in_use_ = reserved_ = 0;
+    }
+    static size_t round(size_t n) { return (n + kAlign - 1) & ~(kAlign - 1); }
+    size_t reserved() const { return reserved_; }
+
+private:
+    struct Chunk {
+        char* base;
+        size_t size;
+        std::map<size_t, size_t> free;  // offset -> length
+    };
+    std::vector<Chunk> chunks_;
+    std::unordered_map<char*, std::pair<size_t, size_t>> live_;
+    size_t in_use_ = 0, reserved_ = 0;
+};
+
+// Offsets of sub-arrays inside one arena block.
+struct Layout {
+    size_t off = 0;
+    template <class T>
+    size_t add(size_t count) {
+        off = Arena::round(off);
+        const size_t o = off;
+        off += (count ? count : 1) * sizeof(T);
+        return o;
+    }
+};
+
+}  // namespace cf
+
+using namespace cf;
+
+namespace {
+
+struct BMesh {
+    uint32_t N = 0, T = 0, num_tiles = 0, num_parts = 0, max_te = 0, max_tn = 0, users = 0;
+    float total_volume = 0.f;
+    TetDevice dev;               // pointers into `block` (partials unused)
+    char* block = nullptr;
+    size_t bytes = 0;
+    std::vector<float> ref_host; // rest positions (default KNC targets)
+};
+
+struct BSim {
+    uint32_t mesh = 0, slot = 0;
+    bool active = true;
+    cf_solid_params p{};
+    char* block = nullptr;
+    size_t bytes = 0;
+    SimDev d{};
+};
+
+struct TileBuild {
+    TileHost th;
+    std::vector<float> ref;
+    float total_volume = 0.f;
+    int rc = CF_OK;
+};
+
+}  // namespace
+
+struct cf_batch {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    uint64_t budget = 0, used = 0;
+    Arena arena;
+    uint32_t tile_threads = 256, tile_elems = 256;
+    uint32_t cg_chunk = 8;
+    int use_graphs = 1;
+    std::map<uint32_t, BMesh> meshes;
+    std::map<uint32_t, BSim> sims;
+    uint32_t next_mesh = 1, next_sim = 1;
+    // slots
+    uint32_t cap = 0;
+    SimDev* d_sims = nullptr;
+    SimCg* d_cgs = nullptr;
+    std::vector<uint32_t> free_slots;
+    uint32_t slots_used = 0;
+    // tables
+    bool dirty = true;
+    std::vector<uint32_t> active_ids;  // sim ids in table order
+    BatchTables tab{};
+    BlockRef* d_tab = nullptr;
+    size_t tab_cap = 0;                // BlockRef entries
+    uint32_t* d_active = nullptr;
+    size_t active_cap = 0;
+    uint32_t* d_counter = nullptr;
+    uint32_t* h_counter = nullptr;     // pinned, 4 entries
+    uint32_t max_iter = 0;
+    uint64_t active_tets = 0, active_nodes = 0;
+    // graphs
+    hipGraphExec_t g_pro = nullptr, g_chunk = nullptr, g_epi = nullptr;
+    uint32_t g_chunk_len = 0;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, evc[2] = {nullptr, nullptr};
+    std::vector<SimCg> h_cgs;
+};
+
+namespace {
+
+void free_graphs(cf_batch* b) {
+    if (b->g_pro) (void)hipGraphExecDestroy(b->g_pro);
+    if (b->g_chunk) (void)hipGraphExecDestroy(b->g_chunk);
+    if (b->g_epi) (void)hipGraphExecDestroy(b->g_epi);
+    b->g_pro = b->g_chunk = b->g_epi = nullptr;
+}
+
+template <class T>
+T* at(char* base, size_t off) { return reinterpret_cast<T*>(base + off); }
+
+// Device layout of a mesh's topology (one arena block).
+struct MeshLayout {
+    size_t ref, conn, dinv, vol, tep, tnp, tn, sp, dst, lvid, npp, pd, total;
+    MeshLayout(uint32_t N, uint32_t T, uint32_t nt, uint32_t P) {
+        Layout l;
+        ref = l.add<float>(3 * (size_t)N);
+        conn = l.add<uint32_t>(4 * (size_t)T);
+        dinv = l.add<float>(9 * (size_t)T);
+        vol = l.add<float>(T);
+        tep = l.add<uint32_t>(nt + 1);
+        tnp = l.add<uint32_t>(nt + 1);
+        tn = l.add<uint32_t>(P);
+        sp = l.add<uint32_t>(P + 1);
+        dst = l.add<uint16_t>(4 * (size_t)T);
+        lvid = l.add<uint16_t>(4 * (size_t)T);
+        npp = l.add<uint32_t>((size_t)N + 1);
+        pd = l.add<uint32_t>(P);
+        total = Arena::round(l.off);
+    }
+};
+
+// Device layout of a simulation's state (one arena block).
+struct SimLayout {
+    size_t vec[8], R, detF, part, mask, pa, pb, pe, ec, es, em, total;
+    uint32_t nb_elem, nb_node, nb_init, nb_vec;
+    SimLayout(uint32_t N, uint32_t T, uint32_t P) {
+        nb_elem = uint32_t(node_blocks(T));
+        nb_node = uint32_t(node_blocks(N));
+        nb_init = uint32_t(node_blocks(3 * N));
+        nb_vec = uint32_t(vec_blocks(3 * N));
+        Layout l;
+        for (int i = 0; i < 8; i++) vec[i] = l.add<float>(3 * (size_t)N);
+        R = l.add<float>(9 * (size_t)T);
+        detF = l.add<float>(T);
+        part = l.add<float>(3 * (size_t)P);
+        mask = l.add<uint8_t>(N);
+        pa = l.add<double>(nb_node);
+        pb = l.add<double>(std::max(nb_init, nb_vec));
+        pe = l.add<double>(nb_elem);
+        ec = l.add<uint32_t>(2 * (size_t)nb_elem);
+        es = l.add<double>(nb_elem);
+        em = l.add<float>(nb_elem);
+        total = Arena::round(l.off);
+    }
+};
+
+int build_mesh_host(uint32_t N, const float* ref, uint32_t T, const uint32_t* tets, uint32_t tile_elems,
+                    uint32_t tile_threads, TileBuild& out) {
+    for (size_t i = 0; i < 4 * (size_t)T; i++)
+        if (tets[i] >= N) {
+            set_error("element %zu references node %u >= %u", i / 4, tets[i], N);
+            return CF_ERR_INVALID_ARG;
+        }
+    CF_TRY(build_tiles(N, T, ref, tets, tile_elems, tile_threads, out.th));
+    out.ref.assign(ref, ref + 3 * (size_t)N);
+    return CF_OK;
+}
+
+int upload_mesh(cf_batch* b, TileBuild& tb, uint32_t* mesh_id) {
+    const TileHost& th = tb.th;
+    const uint32_t N = uint32_t(tb.ref.size() / 3), T = uint32_t(th.perm.size());
+    const uint32_t nt = uint32_t(th.tile_elem_ptr.size() - 1), P = uint32_t(th.tile_nodes.size());
+    const MeshLayout L(N, T, nt, P);
+    if (b->budget && b->used + L.total > b->budget) {
+        set_error("mesh of %zu bytes exceeds the batch's HBM budget", L.total);
+        return CF_ERR_CAPACITY;
+    }
+    BMesh m;
+    m.N = N; m.T = T; m.num_tiles = nt; m.num_parts = P;
+    m.max_te = th.max_tile_elems; m.max_tn = th.max_tile_nodes;
+    CF_TRY(b->arena.alloc(L.total, &m.block));
+    m.bytes = L.total;
+    TetDevice& t = m.dev;
+    t.N = N; t.T = T;
+    t.ref = at<float>(m.block, L.ref);
+    t.conn = at<uint32_t>(m.block, L.conn);
+    t.dinv = at<float>(m.block, L.dinv);
+    t.vol = at<float>(m.block, L.vol);
+    t.tile_elem_ptr = at<uint32_t>(m.block, L.tep);
+    t.tile_node_ptr = at<uint32_t>(m.block, L.tnp);
+    t.tile_nodes = at<uint32_t>(m.block, L.tn);
+    t.slot_ptr = at<uint32_t>(m.block, L.sp);
+    t.dst = at<uint16_t>(m.block, L.dst);
+    t.lvid = at<uint16_t>(m.block, L.lvid);
+    t.node_part_ptr = at<uint32_t>(m.block, L.npp);
+    t.part_dst = at<uint32_t>(m.block, L.pd);
+    t.num_tiles = nt; t.max_tile_elems = m.max_te; t.max_tile_nodes = m.max_tn; t.num_parts = P;
+    hipStream_t s = b->stream;
+    auto put = [&](void* dst, const void* src, size_t bytes) -> int {
+        if (bytes) CF_CHECK_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s));
+        return CF_OK;
+    };
+    int rc = put(t.ref, tb.ref.data(), tb.ref.size() * 4);
+    if (rc == CF_OK) rc = put(t.conn, th.conn_planes.data(), th.conn_planes.size() * 4);
+    if (rc == CF_OK) rc = put(t.tile_elem_ptr, th.tile_elem_ptr.data(), th.tile_elem_ptr.size() * 4);
+    if (rc == CF_OK) rc = put(t.tile_node_ptr, th.tile_node_ptr.data(), th.tile_node_ptr.size() * 4);
+    if (rc == CF_OK) rc = put(t.tile_nodes, th.tile_nodes.data(), th.tile_nodes.size() * 4);
+    if (rc == CF_OK) rc = put(t.slot_ptr, th.slot_ptr.data(), th.slot_ptr.size() * 4);
+    if (rc == CF_OK) rc = put(t.dst, th.dst_planes.data(), th.dst_planes.size() * 2);
+    if (rc == CF_OK) rc = put(t.lvid, th.lvid_planes.data(), th.lvid_planes.size() * 2);
+    if (rc == CF_OK) rc = put(t.node_part_ptr, th.node_part_ptr.data(), th.node_part_ptr.size() * 4);
+    if (rc == CF_OK) rc = put(t.part_dst, th.part_dst.data(), th.part_dst.size() * 4);
+    if (rc == CF_OK) rc = launch_precompute(t, s);
+    // total volume: FP32 serial sum in the caller's element order (as cf_tetset)
+    std::vector<float> vol(T), vol_user(T);
+    if (rc == CF_OK && T && hipMemcpyAsync(vol.data(), t.vol, T * 4, hipMemcpyDeviceToHost, s) != hipSuccess)
+        rc = CF_ERR_HIP;
+    if (rc == CF_OK && hipStreamSynchronize(s) != hipSuccess) rc = CF_ERR_HIP;
+    if (rc != CF_OK) {
+        if (rc == CF_ERR_HIP) set_error("batch mesh upload failed");
+        b->arena.free(m.block);
+        return rc;
+    }
+    for (uint32_t i = 0; i < T; i++) vol_user[th.perm[i]] = vol[i];
+    float acc = 0.f;
+    for (uint32_t i = 0; i < T; i++) acc += vol_user[i];
+    m.total_volume = acc;
+    m.ref_host.swap(tb.ref);
+    b->used += m.bytes;
+    const uint32_t id = b->next_mesh++;
+    b->meshes.emplace(id, std::move(m));
+    *mesh_id = id;
+    return CF_OK;
+}
+
+int ensure_slots(cf_batch* b, uint32_t need) {
+    if (need <= b->cap) return CF_OK;
+    uint32_t cap = std::max<uint32_t>(64, b->cap);
+    while (cap < need) cap *= 2;
+    SimDev* ns = nullptr;
+    SimCg* nc = nullptr;
+    CF_CHECK_HIP(hipMalloc(reinterpret_cast<void**>(&ns), cap * sizeof(SimDev)));
+    if (hipMalloc(reinterpret_cast<void**>(&nc), cap * sizeof(SimCg)) != hipSuccess) {
+        (void)hipFree(ns);
+        set_error("batch slot arrays: hipMalloc failed");
+        return CF_ERR_OUT_OF_MEMORY;
+    }
+    CF_CHECK_HIP(hipMemsetAsync(nc, 0, cap * sizeof(SimCg), b->stream));
+    if (b->cap) {
+        CF_CHECK_HIP(hipMemcpyAsync(ns, b->d_sims, b->cap * sizeof(SimDev), hipMemcpyDeviceToDevice, b->stream));
+        CF_CHECK_HIP(hipMemcpyAsync(nc, b->d_cgs, b->cap * sizeof(SimCg), hipMemcpyDeviceToDevice, b->stream));
+    }
+    CF_CHECK_HIP(hipStreamSynchronize(b->stream));
+    if (b->d_sims) (void)hipFree(b->d_sims);
+    if (b->d_cgs) (void)hipFree(b->d_cgs);
+    b->d_sims = ns;
+    b->d_cgs = nc;
+    b->cap = cap;
+    b->dirty = true;
+    return CF_OK;
+}
+
+int ensure_device(void** p, size_t* cap, size_t need, size_t elem) {
+    if (need <= *cap && *p) return CF_OK;
+    size_t c = std::max<size_t>(1024, *cap);
+    while (c < need) c *= 2;
+    if (*p) (void)hipFree(*p);
+    *p = nullptr;
+    *cap = 0;
+    CF_CHECK_HIP(hipMalloc(p, c * elem));
+    *cap = c;
+    return CF_OK;
+}
+
+// Block tables of the active simulations, in ascending sim id. The tile
+// table is laid out so that the workgroups the hardware deals to one XCD
+// (blockIdx % 8) take a contiguous range of tiles.
+int rebuild_tables(cf_batch* b) {
+    std::vector<uint32_t> ids;
+    for (auto& kv : b->sims)
+        if (kv.second.active) ids.push_back(kv.first);
+    std::vector<BlockRef> tile, elem, node, init, vec;
+    std::vector<uint32_t> slots;
+    uint32_t max_tn = 1, max_te = 1, max_it = 0;
+    uint64_t tets = 0, nodes = 0;
+    for (uint32_t id : ids) {
+        const BSim& s = b->sims[id];
+        const BMesh& m = b->meshes[s.mesh];
+        const uint32_t sl = s.slot;
+        slots.push_back(sl);
+        for (uint32_t i = 0; i < m.num_tiles; i++) tile.push_back({sl, i});
+        for (uint32_t i = 0; i < s.d.nb_elem; i++) elem.push_back({sl, i});
+        for (uint32_t i = 0; i < s.d.nb_node; i++) node.push_back({sl, i});
+        for (uint32_t i = 0; i < s.d.nb_init; i++) init.push_back({sl, i});
+        for (uint32_t i = 0; i < s.d.nb_vec; i++) vec.push_back({sl, i});
+        max_tn = std::max(max_tn, m.max_tn);
+        max_te = std::max(max_te, m.max_te);
+        max_it = std::max(max_it, s.d.max_iter);
+        tets += m.T;
+        nodes += m.N;
+    }
+    // XCD-contiguous tile order
+    std::vector<BlockRef> tile_x;
+    if (!tile.empty()) {
+        const size_t per = (tile.size() + 7) / 8;
+        tile_x.assign(8 * per, BlockRef{kIdleSlot, 0});
+        for (size_t k = 0; k < tile_x.size(); k++) {
+            const size_t nat = (k % 8) * per + k / 8;
+            if (nat < tile.size()) tile_x[k] = tile[nat];
+        }
+    }
+    const size_t total = tile_x.size() + elem.size() + node.size() + init.size() + vec.size();
+    CF_TRY(ensure_device(reinterpret_cast<void**>(&b->d_tab), &b->tab_cap, total, sizeof(BlockRef)));
+    CF_TRY(ensure_device(reinterpret_cast<void**>(&b->d_active), &b->active_cap, slots.size(), sizeof(uint32_t)));
+    std::vector<BlockRef> all;
+    all.reserve(total);
+    all.insert(all.end(), tile_x.begin(), tile_x.end());
+    all.insert(all.end(), elem.begin(), elem.end());
+    all.insert(all.end(), node.begin(), node.end());
+    all.insert(all.end(), init.begin(), init.end());
+    all.insert(all.end(), vec.begin(), vec.end());
+    if (!all.empty())
+        CF_CHECK_HIP(hipMemcpyAsync(b->d_tab, all.data(), all.size() * sizeof(BlockRef), hipMemcpyHostToDevice,
+                                    b->stream));
+    if (!slots.empty())
+        CF_CHECK_HIP(hipMemcpyAsync(b->d_active, slots.data(), slots.size() * 4, hipMemcpyHostToDevice, b->stream));
+    CF_CHECK_HIP(hipStreamSynchronize(b->stream));
+    BatchTables& t = b->tab;
+    t.sims = b->d_sims;
+    t.cgs = b->d_cgs;
+    size_t o = 0;
+    t.tile = b->d_tab + o; t.n_tile = uint32_t(tile_x.size()); o += tile_x.size();
+    t.elem = b->d_tab + o; t.n_elem = uint32_t(elem.size()); o += elem.size();
+    t.node = b->d_tab + o; t.n_node = uint32_t(node.size()); o += node.size();
+    t.init = b->d_tab + o; t.n_init = uint32_t(init.size()); o += init.size();
+    t.vec = b->d_tab + o; t.n_vec = uint32_t(vec.size());
+    t.active = b->d_active;
+    t.n_active = uint32_t(slots.size());
+    t.counter = b->d_counter;
+    t.max_tn = max_tn;
+    t.max_te = max_te;
+    t.tile_threads = b->tile_threads;
+    b->active_ids.swap(ids);
+    b->max_iter = max_it;
+    b->active_tets = tets;
+    b->active_nodes = nodes;
+    free_graphs(b);
+    b->dirty = false;
+    return CF_OK;
+}
+
+int enqueue_prologue(cf_batch* b) {
+    const BatchTables& t = b->tab;
+    hipStream_t s = b->stream;
+    CF_TRY(launch_batch_begin_eval(t, s));               // BeginEvaluation (FixedStep :927)
+    CF_TRY(launch_batch_forces(t, kModeFe, kSelY, s));    // f_e(x)
+    CF_TRY(launch_batch_gather_rhs(t, s));                // b, y0, KNC (:1209-1295)
+    CF_TRY(launch_batch_forces(t, kModeDfx, kSelY, s));   // r = S(b - A y0)
+    CF_TRY(launch_batch_gather_matvec(t, kSelY, s));
+    CF_TRY(launch_batch_cg_init(t, s));
+    return launch_batch_cg_start(t, s);
+}
+
+int enqueue_chunk(cf_batch* b, uint32_t iters) {
+    const BatchTables& t = b->tab;
+    hipStream_t s = b->stream;
+    for (uint32_t k = 0; k < iters; k++) {
+        CF_TRY(launch_batch_forces(t, kModeDfx, kSelD, s));
+        CF_TRY(launch_batch_gather_matvec(t, kSelD, s));
+        CF_TRY(launch_batch_cg_alpha(t, s));
+        CF_TRY(launch_batch_update_xr(t, s));
+        CF_TRY(launch_batch_cg_beta(t, s));
+        CF_TRY(launch_batch_update_d(t, s));
+    }
+    return CF_OK;
+}
+
+int enqueue_epilogue(cf_batch* b) {
+    const BatchTables& t = b->tab;
+    hipStream_t s = b->stream;
+    CF_TRY(launch_batch_integrate(t, s));  // v = y, x += dt y (:1322-1327)
+    CF_TRY(launch_batch_energy(t, s));     // V() (:977)
+    return launch_batch_finish(t, s);
+}
+
+int record(cf_batch* b, int which, uint32_t iters, hipGraphExec_t* exec) {
+    CF_CHECK_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    int rc = which == 0 ? enqueue_prologue(b) : which == 1 ? enqueue_chunk(b, iters) : enqueue_epilogue(b);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(b->stream, &g);
+    if (rc != CF_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess) {
+        set_error("batch graph capture failed: %s", hipGetErrorString(e));
+        return CF_ERR_HIP;
+    }
+    e = hipGraphInstantiate(exec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) {
+        set_error("batch graph instantiate failed: %s", hipGetErrorString(e));
+        return CF_ERR_HIP;
+    }
+    return CF_OK;
+}
+
+int prepare(cf_batch* b) {
+    if (b->dirty) CF_TRY(rebuild_tables(b));
+    if (b->use_graphs && b->tab.n_active) {
+        const uint32_t chunk = std::min(b->cg_chunk, std::max<uint32_t>(1, b->max_iter));
+        if (b->g_chunk && b->g_chunk_len != chunk) free_graphs(b);
+        if (!b->g_pro) CF_TRY(record(b, 0, 0, &b->g_pro));
+        if (!b->g_chunk) CF_TRY(record(b, 1, chunk, &b->g_chunk));
+        if (!b->g_epi) CF_TRY(record(b, 2, 0, &b->g_epi));
+        b->g_chunk_len = chunk;
+    }
+    return CF_OK;
+}
+
+void sim_params(BSim& s, const BMesh& m) {
+    const cf_solid_params& p = s.p;
+    SimDev& d = s.d;
+    d.m = material_from(p.fem.young_modulus, p.fem.poisson_ratio);
+    // RebuildMass: uniform rho V / N stored as an inverse (LeafDSH_Solid3D_FEM.cpp:1591-1611)
+    const float inv_mass = 1.0f / (p.density * m.total_volume / float(m.N));
+    d.mass = 1.0f / inv_mass;
+    float c[2];
+    rayleigh_coeffs(p.fem.rayleigh_freq1, p.fem.rayleigh_freq2, p.fem.damping_ratio, c);
+    const float dt = p.time_scale * p.fixed_dt;
+    d.dt = dt;
+    d.inv_dt = 1.0f / dt;
+    d.fM = 1.0f + dt * c[0];
+    d.fK = dt * dt + dt * c[1];
+    d.drag = -std::log(10e-6f) * p.air_drag_coeff;
+    d.thr = p.fem.degenerate_threshold_det_f;
+    d.rel = p.cg_rel_epsilon;
+    d.abs_eps = p.cg_abs_epsilon;
+    d.g[0] = p.gravity[0]; d.g[1] = p.gravity[1]; d.g[2] = p.gravity[2];
+    d.rm = p.fem.rm;
+    d.warm_start = p.warm_start;
+    d.compute_energy = p.compute_energy;
+    d.max_iter = std::min<uint32_t>(p.cg_max_iter, kMaxCgIter);
+}
+
+}  // namespace
+
+extern "C" {
+
+int cf_batch_create(uint64_t budget, cf_batch** out) {
+    if (!out) return CF_ERR_INVALID_ARG;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n < 1) {
+        (void)hipGetLastError();
+        set_error("no HIP device");
+        return CF_ERR_NO_DEVICE;
+    }
+    cf_batch* b = new (std::nothrow) cf_batch;
+    if (!b) return CF_ERR_OUT_OF_MEMORY;
+    b->budget = budget;
+    (void)hipGetDevice(&b->device);
+    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipMalloc(reinterpret_cast<void**>(&b->d_counter), 16) != hipSuccess ||
+        hipHostMalloc(reinterpret_cast<void**>(&b->h_counter), 16) != hipSuccess ||
+        hipEventCreate(&b->ev0) != hipSuccess || hipEventCreate(&b->ev1) != hipSuccess ||
+        hipEventCreateWithFlags(&b->evc[0], hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&b->evc[1], hipEventDisableTiming) != hipSuccess) {
+        set_error("batch: HIP stream/event/memory setup failed");
+        cf_batch_destroy(b);
+        return CF_ERR_HIP;
+    }
+    *out = b;
+    return CF_OK;
+}
+
+int cf_batch_destroy(cf_batch* b) {
+    if (!b) return CF_OK;
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    free_graphs(b);
+    b->arena.release();
+    if (b->d_sims) (void)hipFree(b->d_sims);
+    if (b->d_cgs) (void)hipFree(b->d_cgs);
+    if (b->d_tab) (void)hipFree(b->d_tab);
+    if (b->d_active) (void)hipFree(b->d_active);
+    if (b->d_counter) (void)hipFree(b->d_counter);
+    if (b->h_counter) (void)hipHostFree(b->h_counter);
+    if (b->ev0) (void)hipEventDestroy(b->ev0);
+    if (b->ev1) (void)hipEventDestroy(b->ev1);
+    for (auto& e : b->evc)
+        if (e) (void)hipEventDestroy(e);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+    return CF_OK;
+}
+
+int cf_batch_set_tiles(cf_batch* b, uint32_t threads, uint32_t elems) {
+    if (!b || (threads != 128 && threads != 256 && threads != 512 && threads != 1024) || elems > threads) {
+        set_error("cf_batch_set_tiles: threads must be 128/256/512/1024 and elems <= threads");
+        return CF_ERR_INVALID_ARG;
+    }
+    if (!b->meshes.empty() && threads != b->tile_threads) {
+        set_error("cf_batch_set_tiles: the tile threads are fixed once meshes exist");
+        return CF_ERR_INVALID_ARG;
+    }
+    b->tile_threads = threads;
+    b->tile_elems = elems ? elems : threads;
+    return CF_OK;
+}
+
+int cf_batch_set_schedule(cf_batch* b, uint32_t chunk, int use_graphs) {
+    if (!b || chunk < 1 || chunk > kMaxCgIter) return CF_ERR_INVALID_ARG;
+    b->cg_chunk = chunk;
+    b->use_graphs = use_graphs ? 1 : 0;
+    free_graphs(b);
+    return CF_OK;
+}
+
+int cf_batch_add_mesh(cf_batch* b, uint32_t N, const float* ref, uint32_t T, const uint32_t* tets, uint32_t* id) {
+    if (!b || !ref || !tets || !id || N == 0 || T == 0) { set_error("cf_batch_add_mesh: invalid arguments"); return CF_ERR_INVALID_ARG; }
+    TileBuild tb;
+    CF_TRY(build_mesh_host(N, ref, T, tets, b->tile_elems, b->tile_threads, tb));
+    return upload_mesh(b, tb, id);
+}
+
+int cf_batch_add_meshes(cf_batch* b, uint32_t count, const uint32_t* N, const float* const* ref, const uint32_t* T,
+                        const uint32_t* const* tets, uint32_t threads, uint32_t* ids) {
+    if (!b || !N || !ref || !T || !tets || !ids) { set_error("cf_batch_add_meshes: invalid arguments"); return CF_ERR_INVALID_ARG; }
+    for (uint32_t i = 0; i < count; i++)
+        if (!ref[i] || !tets[i] || N[i] == 0 || T[i] == 0) { set_error("cf_batch_add_meshes: mesh %u is empty", i); return CF_ERR_INVALID_ARG; }
+    if (threads == 0) threads = std::max(1u, std::thread::hardware_concurrency());
+    threads = std::min(threads, std::max(1u, count));
+    std::vector<TileBuild> builds(count);
+    std::atomic<uint32_t> next{0};
+    auto work = [&]() {
+        for (uint32_t i = next++; i < count; i = next++)
+            builds[i].rc = build_mesh_host(N[i], ref[i], T[i], tets[i], b->tile_elems, b->tile_threads, builds[i]);
+    };
+    std::vector<std::thread> pool;
+    for (uint32_t k = 1; k < threads; k++) pool.emplace_back(work);
+    work();
+    for (auto& th : pool) th.join();
+    for (uint32_t i = 0; i < count; i++) {
+        if (builds[i].rc != CF_OK) { set_error("cf_batch_add_meshes: mesh %u is invalid", i); return builds[i].rc; }
+        CF_TRY(upload_mesh(b, builds[i], &ids[i]));
+        builds[i] = TileBuild();
+    }
+    return CF_OK;
+}
+
+int cf_batch_remove_mesh(cf_batch* b, uint32_t id) {
+    if (!b) return CF_ERR_INVALID_ARG;
+    auto it = b->meshes.find(id);
+    if (it == b->meshes.end()) { set_error("no mesh %u", id); return CF_ERR_INVALID_ARG; }
+    if (it->second.users) { set_error("mesh %u is used by %u simulations", id, it->second.users); return CF_ERR_INVALID_ARG; }
+    CF_CHECK_HIP(hipStreamSynchronize(b->stream));
+    b->arena.free(it->second.block);
+    b->used -= it->second.bytes;
+    b->meshes.erase(it);
+    return CF_OK;
+}
+
+int cf_batch_sim_bytes(const cf_batch* b, uint32_t mesh_id, uint64_t* bytes) {
+    if (!b || !bytes) return CF_ERR_INVALID_ARG;
+    auto it = b->meshes.find(mesh_id);
+    if (it == b->meshes.end()) { set_error("no mesh %u", mesh_id); return CF_ERR_INVALID_ARG; }
+    *bytes = SimLayout(it->second.N, it->second.T, it->second.num_parts).total;
+    return CF_OK;
+}
+
+int cf_batch_add_sim(cf_batch* b, uint32_t mesh_id, const cf_solid_params* p, uint32_t n_knc, const uint32_t* knc,
+                     const float* knc_tgt, const float* pos, const float* vel, uint32_t* sim_id) {
+    if (!b || !p || !sim_id || (n_knc && !knc)) { set_error("cf_batch_add_sim: invalid arguments"); return CF_ERR_INVALID_ARG; }
+    auto mit = b->meshes.find(mesh_id);
+    if (mit == b->meshes.end()) { set_error("no mesh %u", mesh_id); return CF_ERR_INVALID_ARG; }
+    BMesh& m = mit->second;
+    if (!(p->fixed_dt > 0) || !(p->density > 0) || !(p->time_scale > 0)) { set_error("fixed_dt, time_scale and density must be > 0"); return CF_ERR_INVALID_ARG; }
+    if (!(p->fem.young_modulus > 0) || !(p->fem.poisson_ratio > -1.0f) || !(p->fem.poisson_ratio < 0.5f) ||
+        p->fem.mm != CF_MM_C_WRP || p->fem.dm != CF_DM_TRUNCATED ||
+        (p->fem.rm != CF_RM_ID && p->fem.rm != CF_RM_MSVD && p->fem.rm != CF_RM_DAPD)) {
+        set_error("cf_batch_add_sim: unsupported material or method");
+        return CF_ERR_INVALID_ARG;
+    }
+    for (uint32_t i = 0; i < n_knc; i++)
+        if (knc[i] >= m.N) { set_error("KNC node %u out of range", knc[i]); return CF_ERR_INVALID_ARG; }
+    const SimLayout L(m.N, m.T, m.num_parts);
+    if (b->budget && b->used + L.total > b->budget) {
+        set_error("simulation of %zu bytes exceeds the HBM budget (%llu of %llu in use)", L.total,
+                  (unsigned long long)b->used, (unsigned long long)b->budget);
+        return CF_ERR_CAPACITY;
+    }
+    uint32_t slot;
+    if (!b->free_slots.empty()) {
+        slot = b->free_slots.back();
+    } else {
+        CF_TRY(ensure_slots(b, b->slots_used + 1));
+        slot = b->slots_used;
+    }
+    BSim s;
+    s.mesh = mesh_id;
+    s.slot = slot;
+    s.p = *p;
+    CF_TRY(b->arena.alloc(L.total, &s.block));
+    s.bytes = L.total;
+    SimDev& d = s.d;
+    std::memset(&d, 0, sizeof(d));
+    const TetDevice& t = m.dev;
+    d.N = m.N; d.T = m.T; d.num_tiles = m.num_tiles;
+    d.nb_elem = L.nb_elem; d.nb_node = L.nb_node; d.nb_init = L.nb_init; d.nb_vec = L.nb_vec;
+    d.conn = t.conn; d.tile_elem_ptr = t.tile_elem_ptr; d.tile_node_ptr = t.tile_node_ptr;
+    d.tile_nodes = t.tile_nodes; d.slot_ptr = t.slot_ptr; d.node_part_ptr = t.node_part_ptr;
+    d.part_dst = t.part_dst; d.dst = t.dst; d.lvid = t.lvid; d.ref = t.ref; d.dinv = t.dinv; d.vol = t.vol;
+    float* vecs[8];
+    for (int i = 0; i < 8; i++) vecs[i] = at<float>(s.block, L.vec[i]);
+    d.x = vecs[0]; d.v = vecs[1]; d.b = vecs[2]; d.y = vecs[3];
+    d.r = vecs[4]; d.d = vecs[5]; d.q = vecs[6]; d.knc_target = vecs[7];
+    d.R = at<float>(s.block, L.R);
+    d.detF = at<float>(s.block, L.detF);
+    d.partials = at<float>(s.block, L.part);
+    uint8_t* mask = at<uint8_t>(s.block, L.mask);
+    d.knc_mask = n_knc ? mask : nullptr;
+    d.part_a = at<double>(s.block, L.pa);
+    d.part_b = at<double>(s.block, L.pb);
+    d.part_e = at<double>(s.block, L.pe);
+    d.eval_count = at<uint32_t>(s.block, L.ec);
+    d.eval_sum = at<double>(s.block, L.es);
+    d.eval_min = at<float>(s.block, L.em);
+    sim_params(s, m);
+    // initial state, KNC, and R = Id (CreateCache, TetrahedronElementSet3.cpp:190-209)
+    hipStream_t st = b->stream;
+    const size_t n3 = 3 * (size_t)m.N;
+    std::vector<uint8_t> hmask(m.N, 0);
+    std::vector<float> tgt(n3, 0.f);
+    for (uint32_t i = 0; i < n_knc; i++) {
+        hmask[knc[i]] = 1;
+        for (int c = 0; c < 3; c++) tgt[3 * (size_t)knc[i] + c] = knc_tgt ? knc_tgt[3 * (size_t)i + c] : m.ref_host[3 * (size_t)knc[i] + c];
+    }
+    std::vector<float> Rid(9 * (size_t)m.T, 0.f);
+    for (size_t e = 0; e < m.T; e++) { Rid[e] = 1.f; Rid[4 * (size_t)m.T + e] = 1.f; Rid[8 * (size_t)m.T + e] = 1.f; }
+    int rc = CF_OK;
+    auto chk = [&](hipError_t e) { if (rc == CF_OK && e != hipSuccess) { set_error("batch sim upload: %s", hipGetErrorString(e)); rc = CF_ERR_HIP; } };
+    chk(hipMemsetAsync(s.block, 0, L.total, st));
+    chk(hipMemcpyAsync(d.x, pos ? pos : m.ref_host.data(), n3 * 4, hipMemcpyHostToDevice, st));
+    if (vel) chk(hipMemcpyAsync(d.v, vel, n3 * 4, hipMemcpyHostToDevice, st));
+    chk(hipMemcpyAsync(mask, hmask.data(), m.N, hipMemcpyHostToDevice, st));
+    chk(hipMemcpyAsync(d.knc_target, tgt.data(), n3 * 4, hipMemcpyHostToDevice, st));
+    chk(hipMemcpyAsync(d.R, Rid.data(), Rid.size() * 4, hipMemcpyHostToDevice, st));
+    chk(hipMemcpyAsync(b->d_sims + slot, &d, sizeof(SimDev), hipMemcpyHostToDevice, st));
+    chk(hipStreamSynchronize(st));
+    if (rc != CF_OK) {
+        b->arena.free(s.block);
+        return rc;
+    }
+    if (!b->free_slots.empty() && b->free_slots.back() == slot) b->free_slots.pop_back();
+    else b->slots_used++;
+    b->used += s.bytes;
+    m.users++;
+    const uint32_t id = b->next_sim++;
+    b->sims.emplace(id, s);
+    b->dirty = true;
+    *sim_id = id;
+    return CF_OK;
+}
+
+int cf_batch_remove_sim(cf_batch* b, uint32_t id) {
+    if (!b) return CF_ERR_INVALID_ARG;
+    auto it = b->sims.find(id);
+    if (it == b->sims.end()) { set_error("no simulation %u", id); return CF_ERR_INVALID_ARG; }
+    CF_CHECK_HIP(hipStreamSynchronize(b->stream));
+    b->arena.free(it->second.block);
+    b->used -= it->second.bytes;
+    b->meshes[it->second.mesh].users--;
+    b->free_slots.push_back(it->second.slot);
+    if (it->second.active) b->dirty = true;
+    b->sims.erase(it);
+    return CF_OK;
+}
+
+int cf_batch_set_active(cf_batch* b, uint32_t id, int active) {
+    if (!b) return CF_ERR_INVALID_ARG;
+    auto it = b->sims.find(id);
+    if (it == b->sims.end()) { set_error("no simulation %u", id); return CF_ERR_INVALID_ARG; }
+    const bool a = active != 0;
+    if (it->second.active != a) { it->second.active = a; b->dirty = true; }
+    return CF_OK;
+}
+
+int cf_batch_get_state(cf_batch* b, uint32_t id, float* pos, float* vel) {
+    if (!b) return CF_ERR_INVALID_ARG;
+    auto it = b->sims.find(id);
+    if (it == b->sims.end()) { set_error("no simulation %u", id); return CF_ERR_INVALID_ARG; }
+    const size_t n3 = 3 * (size_t)it->second.d.N;
+    if (pos) CF_CHECK_HIP(hipMemcpyAsync(pos, it->second.d.x, n3 * 4, hipMemcpyDeviceToHost, b->stream));
+    if (vel) CF_CHECK_HIP(hipMemcpyAsync(vel, it->second.d.v, n3 * 4, hipMemcpyDeviceToHost, b->stream));
+    CF_CHECK_HIP(hipStreamSynchronize(b->stream));
+    return CF_OK;
+}
+
+int cf_batch_set_state(cf_batch* b, uint32_t id, const float* pos, const float* vel) {
+    if (!b) return CF_ERR_INVALID_ARG;
+    auto it = b->sims.find(id);
+    if (it == b->sims.end()) { set_error("no simulation %u", id); return CF_ERR_INVALID_ARG; }
+    const size_t n3 = 3 * (size_t)it->second.d.N;
+    if (pos) CF_CHECK_HIP(hipMemcpyAsync(it->second.d.x, pos, n3 * 4, hipMemcpyHostToDevice, b->stream));
+    if (vel) CF_CHECK_HIP(hipMemcpyAsync(it->second.d.v, vel, n3 * 4, hipMemcpyHostToDevice, b->stream));
+    CF_CHECK_HIP(hipStreamSynchronize(b->stream));
+    return CF_OK;
+}
+
+int cf_batch_fixed_step(cf_batch* b, cf_step_stats* stats) {
+    if (!b) return CF_ERR_INVALID_ARG;
+    CF_TRY(prepare(b));
+    const BatchTables& t = b->tab;
+    if (!t.n_active) return CF_OK;
+    hipStream_t s = b->stream;
+    CF_CHECK_HIP(hipEventRecord(b->ev0, s));
+    if (b->use_graphs) CF_CHECK_HIP(hipGraphLaunch(b->g_pro, s));
+    else CF_TRY(enqueue_prologue(b));
+    // CG chunks: the counter written by each chunk is read while the next one runs
+    const uint32_t chunk = b->use_graphs ? b->g_chunk_len : std::min(b->cg_chunk, std::max<uint32_t>(1, b->max_iter));
+    const uint32_t nchunks = (b->max_iter + chunk - 1) / chunk;
+    for (uint32_t i = 0; i < nchunks; i++) {
+        if (b->use_graphs) CF_CHECK_HIP(hipGraphLaunch(b->g_chunk, s));
+        else CF_TRY(enqueue_chunk(b, chunk));
+        CF_CHECK_HIP(hipMemcpyAsync(&b->h_counter[i & 1], b->d_counter, 4, hipMemcpyDeviceToHost, s));
+        CF_CHECK_HIP(hipEventRecord(b->evc[i & 1], s));
+        if (i >= 1) {
+            CF_CHECK_HIP(hipEventSynchronize(b->evc[(i - 1) & 1]));
+            if (b->h_counter[(i - 1) & 1] == 0) break;
+        }
+    }
+    if (b->use_graphs) CF_CHECK_HIP(hipGraphLaunch(b->g_epi, s));
+    else CF_TRY(enqueue_epilogue(b));
+    CF_CHECK_HIP(hipEventRecord(b->ev1, s));
+    CF_CHECK_HIP(hipEventSynchronize(b->ev1));
+    if (stats) {
+        float ms = 0.f;
+        CF_CHECK_HIP(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+        b->h_cgs.resize(b->slots_used);
+        CF_CHECK_HIP(hipMemcpy(b->h_cgs.data(), b->d_cgs, b->slots_used * sizeof(SimCg), hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < b->active_ids.size(); i++) {
+            const BSim& sim = b->sims[b->active_ids[i]];
+            const SimCg& c = b->h_cgs[sim.slot];
+            cf_step_stats& o = stats[i];
+            o.num_fixed_steps = 1;
+            o.cg_iters = int32_t(c.iters);
+            o.cg_reason = c.reason;
+            o.cg_rel_residual = (c.delta0 > 0) ? float(std::sqrt(c.delta / c.delta0)) : 0.f;
+            o.cg_abs_residual = float(std::sqrt(c.delta));
+            o.num_degenerate = c.num_degenerate;
+            o.sum_degenerate_det_f = float(c.sum_degenerate);
+            o.elastic_energy = float(c.energy);
+            o.min_det_f = c.min_det;
+            o.step_ms = ms;
+        }
+    }
+    return CF_OK;
+}
+
+int cf_batch_active_ids(const cf_batch* b, uint32_t* ids) {
+    if (!b || !ids) return CF_ERR_INVALID_ARG;
+    CF_TRY(prepare(const_cast<cf_batch*>(b)));
+    std::copy(b->active_ids.begin(), b->active_ids.end(), ids);
+    return CF_OK;
+}
+
+int cf_batch_info_get(const cf_batch* b, cf_batch_info* info) {
+    if (!b || !info) return CF_ERR_INVALID_ARG;
+    CF_TRY(prepare(const_cast<cf_batch*>(b)));
+    info->num_meshes = uint32_t(b->meshes.size());
+    info->num_sims = uint32_t(b->sims.size());
+    info->num_active = uint32_t(b->active_ids.size());
+    info->bytes_in_use = b->used;
+    info->bytes_reserved = b->arena.reserved();
+    info->budget_bytes = b->budget;
+    info->active_tets = b->active_tets;
+    info->active_nodes = b->active_nodes;
+    info->tile_threads = b->tile_threads;
+    info->tile_elems = b->tile_elems;
+    info->cg_chunk = b->cg_chunk;
+    info->use_graphs = uint32_t(b->use_graphs);
+    return CF_OK;
+}
+
+int cf_batch_time_kx(cf_batch* b, uint32_t reps, double* ms) {
+    if (!b || !ms || reps == 0) return CF_ERR_INVALID_ARG;
+    CF_TRY(prepare(b));
+    const BatchTables& t = b->tab;
+    hipStream_t s = b->stream;
+    CF_TRY(launch_batch_forces(t, kModeDfx, kSelY, s));
+    CF_TRY(launch_batch_gather_matvec(t, kSelY, s));
+    CF_CHECK_HIP(hipEventRecord(b->ev0, s));
+    for (uint32_t i = 0; i < reps; i++) {
+        CF_TRY(launch_batch_forces(t, kModeDfx, kSelY, s));
+        CF_TRY(launch_batch_gather_matvec(t, kSelY, s));
+    }
+    CF_CHECK_HIP(hipEventRecord(b->ev1, s));
+    CF_CHECK_HIP(hipEventSynchronize(b->ev1));
+    float tot = 0.f;
+    CF_CHECK_HIP(hipEventElapsedTime(&tot, b->ev0, b->ev1));
+    *ms = double(tot) / reps;
+    return CF_OK;
+}
+
+int cf_batch_synchronize(cf_batch* b) {
+    if (!b) return CF_ERR_INVALID_ARG;
+    CF_CHECK_HIP(hipStreamSynchronize(b->stream));
+    return CF_OK;
+}
+
+}  // extern "C"

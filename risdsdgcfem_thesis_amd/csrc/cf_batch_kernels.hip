@@ -1,0 +1,383 @@
+// Batched kernels: one launch steps the same stage of every simulation of a
+// batch. A block reads its (simulation slot, block within that simulation's
+// grid) from a block table and runs the single-mesh block body with that
+// simulation's arrays; per-simulation scalars (the CG's delta, alpha, beta and
+// stop tests) are finalised by one wavefront per simulation that reproduces
+// the single-mesh one-block reduction. Simulations whose solve has stopped
+// skip the iteration kernels, so a batch iterates until its slowest member is
+// done while converged members cost one table and one flag read per block.
+#include "cf_device.h"
+
+namespace cf {
+
+namespace {
+
+__device__ inline TileArgs tile_args(const SimDev& s, const float* in, uint32_t max_tn, uint32_t max_te) {
+    TileArgs a;
+    a.T = s.T;
+    a.num_tiles = s.num_tiles;
+    a.max_te = max_te;
+    a.max_tn = max_tn;
+    a.tile_elem_ptr = s.tile_elem_ptr;
+    a.tile_node_ptr = s.tile_node_ptr;
+    a.tile_nodes = s.tile_nodes;
+    a.part_dst = s.part_dst;
+    a.slot_ptr = s.slot_ptr;
+    a.dst = s.dst;
+    a.lvid = s.lvid;
+    a.R = s.R;
+    a.dinv = s.dinv;
+    a.vol = s.vol;
+    a.ref = s.ref;
+    a.in = in;
+    a.partials = s.partials;
+    a.m = s.m;
+    a.skip = nullptr;
+    return a;
+}
+
+__device__ inline NodeTerms node_terms(const SimDev& s) {
+    NodeTerms t;
+    t.knc_mask = s.knc_mask;
+    t.knc_target = s.knc_target;
+    t.mass = s.mass;
+    t.mass_array = nullptr;
+    t.fM = s.fM;
+    t.fK = s.fK;
+    t.dt = s.dt;
+    t.inv_dt = s.inv_dt;
+    t.g[0] = s.g[0]; t.g[1] = s.g[1]; t.g[2] = s.g[2];
+    t.warm_start = s.warm_start;
+    return t;
+}
+
+// The slot of the wavefront's simulation in a per-simulation kernel
+// (4 wavefronts per block), or kIdleSlot.
+__device__ inline uint32_t wave_slot(const BatchTables& t) {
+    const uint32_t w = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+    return (w < t.n_active) ? t.active[w] : kIdleSlot;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------ step prologue
+__global__ __launch_bounds__(kBlock) void kb_begin_eval(BatchTables t) {
+    __shared__ double sh[kBlock / 64];
+    __shared__ float shm[kBlock / 64];
+    __shared__ uint32_t shc[kBlock / 64], shf[kBlock / 64];
+    if (blockIdx.x == 0 && threadIdx.x == 0) *t.counter = 0;
+    const BlockRef br = t.elem[blockIdx.x];
+    if (br.slot == kIdleSlot) return;
+    const SimDev& s = t.sims[br.slot];
+    begin_eval_block(s.T, s.conn, s.x, s.dinv, s.rm, s.thr, s.R, s.detF, s.eval_count, s.eval_sum, s.eval_min,
+                     br.blk, s.nb_elem, sh, shm, shc, shf);
+}
+
+// Element stage over every tile of the batch: f_e on x, or df_x on y (the
+// warm start, before the solve) or on d (inside the solve; skipped by
+// simulations whose solve has stopped).
+template <int MODE, int TB>
+__global__ __launch_bounds__(TB) void kb_tiles(BatchTables t, int sel) {
+    const BlockRef br = t.tile[blockIdx.x];
+    if (br.slot == kIdleSlot) return;
+    if (MODE == kModeDfx && sel == kSelD && t.cgs[br.slot].done) return;
+    extern __shared__ float4 smem4[];
+    float4* s_v = smem4;
+    float4* s_r = s_v + t.max_tn;
+    float4* s_F = s_r + t.max_tn;
+    uint32_t* s_sp = reinterpret_cast<uint32_t*>(s_F + 4 * t.max_te);
+    const SimDev& s = t.sims[br.slot];
+    const float* in = (MODE == kModeFe) ? s.x : (sel == kSelY ? s.y : s.d);
+    const TileArgs a = tile_args(s, in, t.max_tn, t.max_te);
+    tile_block<MODE>(a, br.blk, s_v, s_r, s_F, s_sp);
+}
+
+// b = dt (M g + f_e) + M v, y = v, KNC rows (Step_QuasiImplicitEuler_v,
+// ds/dsh/LeafDSH_Solid3D_FEM.cpp:1209-1295)
+__global__ __launch_bounds__(kBlock) void kb_gather_rhs(BatchTables t) {
+    __shared__ double sh[kBlock / 64];
+    const BlockRef br = t.node[blockIdx.x];
+    if (br.slot == kIdleSlot) return;
+    const SimDev& s = t.sims[br.slot];
+    NodeTerms nt = node_terms(s);
+    nt.out = s.b;
+    nt.y = s.y;
+    nt.x = s.x;
+    nt.v = s.v;
+    gather_block<kGatherRhs>(s.N, s.node_part_ptr, s.partials, nt, br.blk, s.nb_node, sh);
+}
+
+// q = S(fM M d - fK K d) and per-block d.q (LS_A_times_d :1056-1079)
+__global__ __launch_bounds__(kBlock) void kb_gather_matvec(BatchTables t, int sel) {
+    __shared__ double sh[kBlock / 64];
+    const BlockRef br = t.node[blockIdx.x];
+    if (br.slot == kIdleSlot) return;
+    if (sel == kSelD && t.cgs[br.slot].done) return;
+    const SimDev& s = t.sims[br.slot];
+    NodeTerms nt = node_terms(s);
+    nt.d = (sel == kSelY) ? s.y : s.d;
+    nt.out = s.q;
+    nt.partials = s.part_a;
+    gather_block<kGatherMatvec>(s.N, s.node_part_ptr, s.partials, nt, br.blk, s.nb_node, sh);
+}
+
+// ------------------------------------------------------------ CG (ns/ILSS_CG.h:38-151)
+__global__ __launch_bounds__(kBlock) void kb_cg_init(BatchTables t) {
+    __shared__ double sh[kBlock / 64];
+    const BlockRef br = t.init[blockIdx.x];
+    if (br.slot == kIdleSlot) return;
+    const SimDev& s = t.sims[br.slot];
+    cg_init_block(3 * s.N, s.q, s.b, s.r, s.d, s.knc_mask, s.part_b, br.blk, s.nb_init, sh);
+}
+
+__global__ __launch_bounds__(kBlock) void kb_cg_start(BatchTables t) {
+    const uint32_t slot = wave_slot(t);
+    if (slot == kIdleSlot) return;
+    const SimDev& s = t.sims[slot];
+    const double d0 = wave_reduce_partials(s.part_b, s.nb_init);
+    if ((threadIdx.x & 63) == 0) {
+        SimCg& c = t.cgs[slot];
+        const float abs_sq = s.abs_eps * s.abs_eps;
+        const float rel_sq = s.rel * s.rel;
+        c.delta0 = d0;
+        c.delta = d0;
+        c.target = (double)rel_sq * d0;
+        c.abs_sq = (double)abs_sq;
+        c.max_iter = s.max_iter;
+        c.iters = 0;
+        c.dq = 0.0;
+        c.alpha = 0.f;
+        c.beta = 0.f;
+        int reason = 0;
+        if (!(d0 > (double)abs_sq)) reason = CF_CG_CONVERGED_ABS;
+        else if (!(d0 > c.target)) reason = CF_CG_CONVERGED_REL;
+        else if (!(0u < s.max_iter)) reason = CF_CG_MAX_ITER;
+        c.done = reason != 0;
+        c.reason = reason;
+        if (!reason) atomicAdd(t.counter, 1u);
+    }
+}
+
+// alpha = delta / d.q, or breakdown when |d.q| < abs^2 (ILSS_CG.h:91-99)
+__global__ __launch_bounds__(kBlock) void kb_cg_alpha(BatchTables t) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *t.counter = 0;
+    const uint32_t slot = wave_slot(t);
+    if (slot == kIdleSlot) return;
+    SimCg& c = t.cgs[slot];
+    if (c.done) return;
+    const SimDev& s = t.sims[slot];
+    const double dq = wave_reduce_partials(s.part_a, s.nb_node);
+    if ((threadIdx.x & 63) == 0) {
+        c.dq = dq;
+        if (fabs(dq) < c.abs_sq) {
+            c.reason = CF_CG_BREAKDOWN;
+            c.done = 2;
+        } else {
+            c.alpha = (float)(c.delta / dq);
+        }
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void kb_update_xr(BatchTables t) {
+    __shared__ double sh[kBlock / 64];
+    const BlockRef br = t.vec[blockIdx.x];
+    if (br.slot == kIdleSlot) return;
+    const SimCg& c = t.cgs[br.slot];
+    if (c.done) return;
+    const SimDev& s = t.sims[br.slot];
+    update_xr_block<true>(3 * s.N, c.alpha, s.y, s.r, s.d, s.q, s.part_b, br.blk, s.nb_vec, sh);
+}
+
+// delta' = r.r ; beta = delta'/delta ; stop tests (ILSS_CG.h:86-88,115-133)
+__global__ __launch_bounds__(kBlock) void kb_cg_beta(BatchTables t) {
+    const uint32_t slot = wave_slot(t);
+    if (slot == kIdleSlot) return;
+    SimCg& c = t.cgs[slot];
+    if (c.done) return;
+    const SimDev& s = t.sims[slot];
+    const double dn = wave_reduce_partials(s.part_b, s.nb_vec);
+    if ((threadIdx.x & 63) == 0) {
+        c.beta = (float)(dn / c.delta);
+        c.delta = dn;
+        const uint32_t k1 = c.iters + 1;
+        c.iters = k1;
+        int reason = 0;
+        if (!(dn > c.abs_sq)) reason = CF_CG_CONVERGED_ABS;
+        else if (!(dn > c.target)) reason = CF_CG_CONVERGED_REL;
+        else if (!(k1 < c.max_iter)) reason = CF_CG_MAX_ITER;
+        c.reason = reason;
+        c.done = reason != 0;
+        if (!reason) atomicAdd(t.counter, 1u);
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void kb_update_d(BatchTables t) {
+    const BlockRef br = t.vec[blockIdx.x];
+    if (br.slot == kIdleSlot) return;
+    const SimCg& c = t.cgs[br.slot];
+    if (c.done) return;
+    const SimDev& s = t.sims[br.slot];
+    update_d_block<true>(3 * s.N, c.beta, s.d, s.r, s.knc_mask, br.blk, s.nb_vec);
+}
+
+// ------------------------------------------------------------ step epilogue
+__global__ __launch_bounds__(kBlock) void kb_integrate(BatchTables t) {
+    const BlockRef br = t.init[blockIdx.x];
+    if (br.slot == kIdleSlot) return;
+    const SimDev& s = t.sims[br.slot];
+    integrate_block(3 * s.N, s.x, s.v, s.y, s.dt, s.drag, br.blk, s.nb_init);
+}
+
+__global__ __launch_bounds__(kBlock) void kb_energy(BatchTables t) {
+    __shared__ double sh[kBlock / 64];
+    const BlockRef br = t.elem[blockIdx.x];
+    if (br.slot == kIdleSlot) return;
+    const SimDev& s = t.sims[br.slot];
+    if (!s.compute_energy) return;
+    energy_block(s.T, s.conn, s.ref, s.x, s.R, s.dinv, s.vol, s.m, s.part_e, br.blk, s.nb_elem, sh);
+}
+
+// Per-simulation step results: the block partials summed in block order
+// (the order the single-mesh path sums them on the host).
+__global__ __launch_bounds__(kBlock) void kb_finish(BatchTables t) {
+    const uint32_t slot = wave_slot(t);
+    if (slot == kIdleSlot || (threadIdx.x & 63) != 0) return;
+    const SimDev& s = t.sims[slot];
+    SimCg& c = t.cgs[slot];
+    uint32_t nd = 0, nf = 0;
+    double sd = 0.0, en = 0.0;
+    float mn = 3e38f;
+    for (uint32_t i = 0; i < s.nb_elem; i++) {
+        nd += s.eval_count[i];
+        nf += s.eval_count[s.nb_elem + i];
+        sd += s.eval_sum[i];
+        mn = fminf(mn, s.eval_min[i]);
+        if (s.compute_energy) en += s.part_e[i];
+    }
+    c.num_degenerate = nd;
+    c.num_fallback = nf;
+    c.sum_degenerate = sd;
+    c.min_det = mn;
+    c.energy = en;
+}
+
+__global__ void kb_fill_u32(uint32_t* p, uint32_t value, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = value;
+}
+
+// ------------------------------------------------------------ launchers
+namespace {
+int check_launch_b() {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        set_error("batched kernel launch failed: %s", hipGetErrorString(e));
+        return CF_ERR_HIP;
+    }
+    return CF_OK;
+}
+inline uint32_t mesh_grid(const BatchTables& t) { return (t.n_active + 3) / 4; }
+size_t tile_smem_b(const BatchTables& t) {
+    size_t b = sizeof(float4) * (2 * (size_t)t.max_tn + 4 * (size_t)t.max_te);
+    b += sizeof(uint32_t) * ((size_t)t.max_tn + 1);
+    return (b + 63) & ~size_t(63);
+}
+}  // namespace
+
+int launch_batch_begin_eval(const BatchTables& t, hipStream_t s) {
+    if (!t.n_elem) return CF_OK;
+    hipLaunchKernelGGL(kb_begin_eval, dim3(t.n_elem), dim3(kBlock), 0, s, t);
+    return check_launch_b();
+}
+
+int launch_batch_forces(const BatchTables& t, int mode, int sel, hipStream_t s) {
+    if (!t.n_tile) return CF_OK;
+    const size_t sm = tile_smem_b(t);
+#define CF_BTILE(TBV)                                                                                     \
+    do {                                                                                                  \
+        if (mode == kModeFe) hipLaunchKernelGGL((kb_tiles<kModeFe, TBV>), dim3(t.n_tile), dim3(TBV), sm, s, t, sel); \
+        else hipLaunchKernelGGL((kb_tiles<kModeDfx, TBV>), dim3(t.n_tile), dim3(TBV), sm, s, t, sel);      \
+    } while (0)
+    switch (t.tile_threads) {
+        case 1024: CF_BTILE(1024); break;
+        case 512: CF_BTILE(512); break;
+        case 128: CF_BTILE(128); break;
+        default: CF_BTILE(256); break;
+    }
+#undef CF_BTILE
+    return check_launch_b();
+}
+
+int launch_batch_gather_rhs(const BatchTables& t, hipStream_t s) {
+    if (!t.n_node) return CF_OK;
+    hipLaunchKernelGGL(kb_gather_rhs, dim3(t.n_node), dim3(kBlock), 0, s, t);
+    return check_launch_b();
+}
+
+int launch_batch_gather_matvec(const BatchTables& t, int sel, hipStream_t s) {
+    if (!t.n_node) return CF_OK;
+    hipLaunchKernelGGL(kb_gather_matvec, dim3(t.n_node), dim3(kBlock), 0, s, t, sel);
+    return check_launch_b();
+}
+
+int launch_batch_cg_init(const BatchTables& t, hipStream_t s) {
+    if (!t.n_init) return CF_OK;
+    hipLaunchKernelGGL(kb_cg_init, dim3(t.n_init), dim3(kBlock), 0, s, t);
+    return check_launch_b();
+}
+
+int launch_batch_cg_start(const BatchTables& t, hipStream_t s) {
+    if (!t.n_active) return CF_OK;
+    hipLaunchKernelGGL(kb_cg_start, dim3(mesh_grid(t)), dim3(kBlock), 0, s, t);
+    return check_launch_b();
+}
+
+int launch_batch_cg_alpha(const BatchTables& t, hipStream_t s) {
+    if (!t.n_active) return CF_OK;
+    hipLaunchKernelGGL(kb_cg_alpha, dim3(mesh_grid(t)), dim3(kBlock), 0, s, t);
+    return check_launch_b();
+}
+
+int launch_batch_update_xr(const BatchTables& t, hipStream_t s) {
+    if (!t.n_vec) return CF_OK;
+    hipLaunchKernelGGL(kb_update_xr, dim3(t.n_vec), dim3(kBlock), 0, s, t);
+    return check_launch_b();
+}
+
+int launch_batch_cg_beta(const BatchTables& t, hipStream_t s) {
+    if (!t.n_active) return CF_OK;
+    hipLaunchKernelGGL(kb_cg_beta, dim3(mesh_grid(t)), dim3(kBlock), 0, s, t);
+    return check_launch_b();
+}
+
+int launch_batch_update_d(const BatchTables& t, hipStream_t s) {
+    if (!t.n_vec) return CF_OK;
+    hipLaunchKernelGGL(kb_update_d, dim3(t.n_vec), dim3(kBlock), 0, s, t);
+    return check_launch_b();
+}
+
+int launch_batch_integrate(const BatchTables& t, hipStream_t s) {
+    if (!t.n_init) return CF_OK;
+    hipLaunchKernelGGL(kb_integrate, dim3(t.n_init), dim3(kBlock), 0, s, t);
+    return check_launch_b();
+}
+
+int launch_batch_energy(const BatchTables& t, hipStream_t s) {
+    if (!t.n_elem) return CF_OK;
+    hipLaunchKernelGGL(kb_energy, dim3(t.n_elem), dim3(kBlock), 0, s, t);
+    return check_launch_b();
+}
+
+int launch_batch_finish(const BatchTables& t, hipStream_t s) {
+    if (!t.n_active) return CF_OK;
+    hipLaunchKernelGGL(kb_finish, dim3(mesh_grid(t)), dim3(kBlock), 0, s, t);
+    return check_launch_b();
+}
+
+int launch_batch_fill_u32(uint32_t* p, uint32_t value, uint32_t n, hipStream_t s) {
+    if (!n) return CF_OK;
+    hipLaunchKernelGGL(kb_fill_u32, dim3((n + 255) / 256), dim3(256), 0, s, p, value, n);
+    return check_launch_b();
+}
+
+}  // namespace cf

@@ -59,7 +59,7 @@ static int upload(T** p, const std::vector<T>& v, hipStream_t s) {
 }
 
 // Rayleigh coefficients c = inv(Omega) (2 zeta, 2 zeta) (LeafDSH_Solid3D_FEM.h:347-354)
-static void rayleigh_coeffs(float f1, float f2, float zeta, float c[2]) {
+void rayleigh_coeffs(float f1, float f2, float zeta, float c[2]) {
     const float o00 = 1.0f / f1, o01 = f1, o10 = 1.0f / f2, o11 = f2;
     const float det = o00 * o11 - o10 * o01;
     const float id = 1.0f / det;

@@ -1,0 +1,737 @@
+// Device code shared by the single-mesh kernels (cf_kernels.hip) and the
+// batched kernels (cf_batch_kernels.hip): the reference-order FP32 math, the
+// fixed-order FP64 reductions, the tile staging and the element force.
+#pragma once
+#include "cf_internal.h"
+#include <hip/hip_runtime.h>
+#include <cmath>
+
+namespace cf {
+
+
+// ============================================================ FP32 math in the
+// reference's evaluation order. Contraction is disabled in these functions so
+// that, like the reference's SSE4.1 build (no FMA), each product is rounded.
+namespace refmath {
+
+__device__ inline float det3(const float* m) {  // Mal/GMat.h:236-244
+#pragma clang fp contract(off)
+    return m[0] * m[4] * m[8] - m[0] * m[7] * m[5] + m[3] * m[7] * m[2] - m[3] * m[1] * m[8] +
+           m[6] * m[1] * m[5] - m[6] * m[4] * m[2];
+}
+
+__device__ inline void inv3(const float* m, float det, float* r) {  // Mal/GMat.h:291-307
+#pragma clang fp contract(off)
+    float id = 1.0f / det;
+    r[0] = id * (m[4] * m[8] - m[5] * m[7]);
+    r[1] = id * (m[2] * m[7] - m[1] * m[8]);
+    r[2] = id * (m[1] * m[5] - m[2] * m[4]);
+    r[3] = id * (m[5] * m[6] - m[3] * m[8]);
+    r[4] = id * (m[0] * m[8] - m[2] * m[6]);
+    r[5] = id * (m[2] * m[3] - m[0] * m[5]);
+    r[6] = id * (m[3] * m[7] - m[4] * m[6]);
+    r[7] = id * (m[1] * m[6] - m[0] * m[7]);
+    r[8] = id * (m[0] * m[4] - m[1] * m[3]);
+}
+
+__device__ inline void mul3(const float* a, const float* b, float* p) {  // Mal/GMat.h:135-145
+#pragma clang fp contract(off)
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+            float acc = 0.0f;
+            acc += a[3 * i + 0] * b[0 + j];
+            acc += a[3 * i + 1] * b[3 + j];
+            acc += a[3 * i + 2] * b[6 + j];
+            p[3 * i + j] = acc;
+        }
+}
+
+// Newton polar decomposition with the reference's loose stop rule
+// (Mal/GMatDecomposition.h:411-432): error = |H1-H0|_F^2 <= 0.01 or 10 steps.
+__device__ inline void polar_newton(const float* H, float* R) {
+#pragma clang fp contract(off)
+    float H0[9];
+    for (int i = 0; i < 9; i++) H0[i] = H[i];
+    unsigned n = 0;
+    float err;
+    do {
+        float Ht[9] = {H0[0], H0[3], H0[6], H0[1], H0[4], H0[7], H0[2], H0[5], H0[8]};
+        float it[9];
+        inv3(Ht, det3(Ht), it);
+        float H1[9];
+        for (int i = 0; i < 9; i++) H1[i] = 0.5f * (H0[i] + it[i]);
+        err = 0.0f;
+        for (int i = 0; i < 9; i++) {
+            float d = H1[i] - H0[i];
+            err += d * d;
+        }
+        for (int i = 0; i < 9; i++) H0[i] = H1[i];
+        n++;
+    } while (n < 10u && err > 0.01f);
+    for (int i = 0; i < 9; i++) R[i] = H0[i];
+}
+
+// ---- PhysBAM-style FP32 SVD as adapted by the reference
+// (Mal/GMatDecomposition_SVD3x3_PhysBAM.h:35-280).
+struct v3 { float x, y, z; };
+
+__device__ inline v3 cross(v3 a, v3 b) {  // Mal/GVec.h:89-91
+#pragma clang fp contract(off)
+    return v3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+__device__ inline float dot(v3 a, v3 b) {
+#pragma clang fp contract(off)
+    float acc = 0.0f;
+    acc += a.x * b.x;
+    acc += a.y * b.y;
+    acc += a.z * b.z;
+    return acc;
+}
+__device__ inline v3 normalized(v3 a) {  // v * (1/|v|), Mal/GVec.h:85,97
+#pragma clang fp contract(off)
+    float inv = 1.0f / sqrtf(dot(a, a));
+    return v3{inv * a.x, inv * a.y, inv * a.z};
+}
+__device__ inline v3 orthogonal_vector(v3 v) {  // :35-49
+    float ax = fabsf(v.x), ay = fabsf(v.y), az = fabsf(v.z);
+    if (ax < ay) return (ax < az) ? v3{0.f, v.z, -v.y} : v3{v.y, -v.x, 0.f};
+    return (ay < az) ? v3{-v.z, 0.f, v.x} : v3{v.y, -v.x, 0.f};
+}
+struct sym3 { float x11, x21, x31, x22, x32, x33; };
+
+__device__ inline v3 largest_column_normalized(const sym3& s) {  // :149-156
+#pragma clang fp contract(off)
+    float s11 = s.x11 * s.x11, s12 = s.x21 * s.x21, s13 = s.x31 * s.x31;
+    float s22 = s.x22 * s.x22, s23 = s.x32 * s.x32, s33 = s.x33 * s.x33;
+    float c1 = s11 + s12 + s13, c2 = s12 + s22 + s23, c3 = s13 + s23 + s33;
+    if (c1 > c2) {
+        if (c1 > c3) { float inv = 1.0f / sqrtf(c1); return v3{inv * s.x11, inv * s.x21, inv * s.x31}; }
+    } else if (c2 > c3) { float inv = 1.0f / sqrtf(c2); return v3{inv * s.x21, inv * s.x22, inv * s.x32}; }
+    if (c3 > 0) { float inv = 1.0f / sqrtf(c3); return v3{inv * s.x31, inv * s.x32, inv * s.x33}; }
+    return v3{1.f, 0.f, 0.f};
+}
+
+__device__ inline void svd_rotation(const float* F, float* R) {
+#pragma clang fp contract(off)
+    // normal equations F^T F (:129-137)
+    sym3 A;
+    A.x11 = F[0] * F[0] + F[3] * F[3] + F[6] * F[6];
+    A.x21 = F[1] * F[0] + F[4] * F[3] + F[7] * F[6];
+    A.x31 = F[2] * F[0] + F[5] * F[3] + F[8] * F[6];
+    A.x22 = F[1] * F[1] + F[4] * F[4] + F[7] * F[7];
+    A.x32 = F[2] * F[1] + F[5] * F[4] + F[8] * F[7];
+    A.x33 = F[2] * F[2] + F[5] * F[5] + F[8] * F[8];
+    // Smith (1961) eigenvalues (:161-197). sqrt(3) is written Sqrt(3) with an
+    // int argument in the reference, which deduces Sqrt<int> and yields 1.
+    const float one_third = 1.0f / 3.0f;
+    const float one_sixth = 1.0f / 6.0f;
+    const float root_three = 1.0f;
+    float m = one_third * (A.x11 + A.x22 + A.x33);
+    float a11 = A.x11 - m, a22 = A.x22 - m, a33 = A.x33 - m;
+    float a12s = A.x21 * A.x21, a13s = A.x31 * A.x31, a23s = A.x32 * A.x32;
+    float p = one_sixth * (a11 * a11 + a22 * a22 + a33 * a33 + 2 * (a12s + a13s + a23s));
+    float q = 0.5f * (a11 * (a22 * a33 - a23s) - a22 * a13s - a33 * a12s) + A.x21 * A.x31 * A.x32;
+    float sqrt_p = sqrtf(p), disc = p * p * p - q * q;
+    float phi = one_third * atan2f(sqrtf(fmaxf(0.0f, disc)), q);
+    float c = cosf(phi), sn = sinf(phi);
+    float spc = sqrt_p * c, r3sps = root_three * sqrt_p * sn;
+    float l0 = m + 2 * spc, l1 = m - spc - r3sps, l2 = m - spc + r3sps;
+    float L0, L1, L2;
+    if (l0 < l1) {
+        if (l1 < l2) { L0 = l2; L1 = l1; L2 = l0; }
+        else if (l0 < l2) { L0 = l1; L1 = l2; L2 = l0; }
+        else { L0 = l1; L1 = l0; L2 = l2; }
+    } else {
+        if (l1 > l2) { L0 = l0; L1 = l1; L2 = l2; }
+        else if (l0 > l2) { L0 = l0; L1 = l2; L2 = l1; }
+        else { L0 = l2; L1 = l0; L2 = l1; }
+    }
+    // eigenvectors (:199-236)
+    bool flipped = false;
+    float f0 = L0, f2 = L2;
+    if (L0 - L1 < L1 - L2) { f0 = L2; f2 = L0; flipped = true; }
+    sym3 am = A;
+    am.x11 -= f0; am.x22 -= f0; am.x33 -= f0;
+    sym3 cf_;
+    cf_.x11 = am.x22 * am.x33 - am.x32 * am.x32;
+    cf_.x21 = am.x32 * am.x31 - am.x21 * am.x33;
+    cf_.x31 = am.x21 * am.x32 - am.x22 * am.x31;
+    cf_.x22 = am.x11 * am.x33 - am.x31 * am.x31;
+    cf_.x32 = am.x21 * am.x31 - am.x11 * am.x32;
+    cf_.x33 = am.x11 * am.x22 - am.x21 * am.x21;
+    v3 v1 = largest_column_normalized(cf_);
+    v3 o0 = normalized(orthogonal_vector(v1));
+    v3 o1 = cross(v1, o0);
+    // reduced 2x2: (full_A * other)^T other, accumulate from 0
+    auto Amul = [&](v3 u) {
+        v3 r;
+        float acc;
+        acc = 0.0f; acc += A.x11 * u.x; acc += A.x21 * u.y; acc += A.x31 * u.z; r.x = acc;
+        acc = 0.0f; acc += A.x21 * u.x; acc += A.x22 * u.y; acc += A.x32 * u.z; r.y = acc;
+        acc = 0.0f; acc += A.x31 * u.x; acc += A.x32 * u.y; acc += A.x33 * u.z; r.z = acc;
+        return r;
+    };
+    v3 b0 = Amul(o0), b1 = Amul(o1);
+    float r11 = b0.x * o0.x + b0.y * o0.y + b0.z * o0.z;
+    float r21 = b1.x * o0.x + b1.y * o0.y + b1.z * o0.z;
+    float r22 = b1.x * o1.x + b1.y * o1.y + b1.z * o1.z;
+    r11 -= f2; r22 -= f2;
+    // 2x2 cofactor (x22, -x21, x11) and its largest column (:70-82)
+    float c11 = r22, c21 = -r21, c22 = r11;
+    float w0, w1;
+    {
+        float s11 = c11 * c11, s12 = c21 * c21, s22 = c22 * c22;
+        float k1 = s11 + s12, k2 = s12 + s22;
+        if (k1 > k2) { float inv = 1.0f / sqrtf(k1); w0 = inv * c11; w1 = inv * c21; }
+        else if (k2 > 0) { float inv = 1.0f / sqrtf(k2); w0 = inv * c21; w1 = inv * c22; }
+        else { w0 = 1.f; w1 = 0.f; }
+    }
+    v3 v3v;
+    {
+        float acc;
+        acc = 0.0f; acc += o0.x * w0; acc += o1.x * w1; v3v.x = acc;
+        acc = 0.0f; acc += o0.y * w0; acc += o1.y * w1; v3v.y = acc;
+        acc = 0.0f; acc += o0.z * w0; acc += o1.z * w1; v3v.z = acc;
+    }
+    v3 v2 = cross(v3v, v1);
+    v3 V0, V1, V2;  // columns of V
+    if (flipped) { V0 = v3v; V1 = v2; V2 = v3{-v1.x, -v1.y, -v1.z}; }
+    else { V0 = v1; V1 = v2; V2 = v3v; }
+    // singular vectors (:259-279); the singular values only enter through
+    // the sign of det F, which does not change U or V here.
+    auto Fmul = [&](v3 u) {
+        v3 r;
+        float acc;
+        acc = 0.0f; acc += F[0] * u.x; acc += F[1] * u.y; acc += F[2] * u.z; r.x = acc;
+        acc = 0.0f; acc += F[3] * u.x; acc += F[4] * u.y; acc += F[5] * u.z; r.y = acc;
+        acc = 0.0f; acc += F[6] * u.x; acc += F[7] * u.y; acc += F[8] * u.z; r.z = acc;
+        return r;
+    };
+    v3 U0 = normalized(Fmul(V0));
+    v3 p0 = normalized(orthogonal_vector(U0));
+    v3 p1 = cross(U0, p0);
+    v3 fv1 = normalized(Fmul(V1));
+    float t0 = dot(p0, fv1), t1 = dot(p1, fv1);
+    v3 U1;
+    {
+        float acc;
+        acc = 0.0f; acc += p0.x * t0; acc += p1.x * t1; U1.x = acc;
+        acc = 0.0f; acc += p0.y * t0; acc += p1.y * t1; U1.y = acc;
+        acc = 0.0f; acc += p0.z * t0; acc += p1.z * t1; U1.z = acc;
+    }
+    v3 U2 = cross(U0, U1);
+    // R = U * V^T  (Rotation.cpp:55)
+    const float U[9] = {U0.x, U1.x, U2.x, U0.y, U1.y, U2.y, U0.z, U1.z, U2.z};
+    const float Vt[9] = {V0.x, V0.y, V0.z, V1.x, V1.y, V1.z, V2.x, V2.y, V2.z};
+    mul3(U, Vt, R);
+}
+
+}  // namespace refmath
+
+// ============================================================ reductions
+__device__ inline double wave_sum(double v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+// Sum over the block; result valid in every thread. sh: kBlock/64 doubles.
+__device__ inline double block_sum(double v, double* sh) {
+    v = wave_sum(v);
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    __syncthreads();
+    if (l == 0) sh[w] = v;
+    __syncthreads();
+    double r = 0.0;
+    for (int i = 0; i < kBlock / 64; i++) r += sh[i];
+    return r;
+}
+// Deterministic sum of n partials by one block (fixed order and tree).
+__device__ inline double reduce_partials(const double* part, int n, double* sh) {
+    double v = 0.0;
+    for (int i = threadIdx.x; i < n; i += kBlock) v += part[i];
+    return block_sum(v, sh);
+}
+// reduce_partials computed by one wavefront: lane l plays the threads l,
+// l+64, l+128, l+192 of the block and the four wave sums are added in the
+// block's order, so one wavefront per mesh reproduces the one-block sum bit
+// for bit. All 64 lanes must be active.
+__device__ inline double wave_reduce_partials(const double* part, int n) {
+    const int l = threadIdx.x & 63;
+    double r = 0.0;
+    for (int w = 0; w < kBlock / 64; w++) {
+        double v = 0.0;
+        for (int i = l + 64 * w; i < n; i += kBlock) v += part[i];
+        r += wave_sum(v);
+    }
+    return r;
+}
+
+// a0 b0 + a1 b1 + a2 b2 as fma(a2, b2, fma(a1, b1, a0 b0)): a fixed rounding
+// sequence, the same in every kernel.
+__device__ inline float fma3(float a0, float b0, float a1, float b1, float a2, float b2) {
+#pragma clang fp contract(off)
+    return fmaf(a2, b2, fmaf(a1, b1, a0 * b0));
+}
+
+// ============================================================ element forces
+// Corotated element force differential via the equivalent Piola form of
+// Ke = V B^T E B (TetrahedronElement3.h:157-173):
+//   H = R^T [v1-v0, v2-v0, v3-v0] Dm^-1            (df_x: v = dx)
+//   H = (R^T [x1-x0, x2-x0, x3-x0] - Dm) Dm^-1     (f_e:  v = x, u = R^T x - r)
+//   sigma = E(sym H) with the reference's a_k1, a_nu, a_k2
+//   P = -V R sigma Dm^-T ; node c+1 gets column c, node 0 gets -(sum of columns)
+// which equals -R * sum_j K_ij * (local vector j) of C_f_e / C_df_x.
+struct TileArgs {
+    uint32_t T;
+    uint32_t num_tiles;
+    uint32_t max_te, max_tn;
+    const uint32_t* tile_elem_ptr;
+    const uint32_t* tile_node_ptr;
+    const uint32_t* tile_nodes;
+    const uint32_t* part_dst;
+    const uint32_t* slot_ptr;
+    const uint16_t* dst;
+    const uint16_t* lvid;
+    const float* R;
+    const float* dinv;
+    const float* vol;
+    const float* ref;
+    const float* in;
+    float* partials;
+    Material m;
+    const int32_t* skip;
+};
+
+// LDS-only workgroup barrier: orders LDS accesses across the barrier without
+// waiting for outstanding global loads, so that prefetches stay in flight.
+__device__ inline void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+struct TileDesc { uint32_t e0, ne, n0, nn; };
+
+__device__ inline TileDesc tile_desc(const TileArgs& a, uint32_t t) {
+    TileDesc d;
+    d.e0 = a.tile_elem_ptr[t];
+    d.ne = a.tile_elem_ptr[t + 1] - d.e0;
+    d.n0 = a.tile_node_ptr[t];
+    d.nn = a.tile_node_ptr[t + 1] - d.n0;
+    return d;
+}
+
+// Per-thread registers of one tile: this thread's tile node (vector, rest
+// position, slot offset) and element (local vertex ids, force slots, R).
+struct TileRegs {
+    float v[3], r[3], R[9];
+    uint32_t sp, pd;
+    uint32_t l[4];
+    uint32_t f[4];
+};
+
+__device__ inline void load_tile_regs(const TileArgs& a, const TileDesc& d, uint32_t gid, uint32_t tid,
+                                      TileRegs& g) {
+    const size_t T = a.T;
+    if (tid < d.nn) {
+        g.v[0] = a.in[3 * (size_t)gid + 0]; g.v[1] = a.in[3 * (size_t)gid + 1]; g.v[2] = a.in[3 * (size_t)gid + 2];
+        g.r[0] = a.ref[3 * (size_t)gid + 0]; g.r[1] = a.ref[3 * (size_t)gid + 1]; g.r[2] = a.ref[3 * (size_t)gid + 2];
+    }
+    if (tid <= d.nn) g.sp = a.slot_ptr[d.n0 + tid] - 4 * d.e0;
+    if (tid < d.nn) g.pd = a.part_dst[d.n0 + tid];
+    if (tid < d.ne) {
+        const size_t e = d.e0 + tid;
+        g.l[0] = a.lvid[e]; g.l[1] = a.lvid[T + e]; g.l[2] = a.lvid[2 * T + e]; g.l[3] = a.lvid[3 * T + e];
+        g.f[0] = a.dst[e]; g.f[1] = a.dst[T + e]; g.f[2] = a.dst[2 * T + e]; g.f[3] = a.dst[3 * T + e];
+#pragma unroll
+        for (int k = 0; k < 9; k++) g.R[k] = a.R[k * T + e];
+    }
+}
+
+// Nodal forces of one element into its four tile force slots.
+template <int MODE>
+__device__ inline void element_force(const float4* s_v, const float4* s_r, float4* s_F, const uint32_t* lv,
+                                     const uint32_t* fs, const float* R, const Material& m) {
+    // Every fused multiply-add is written out (fma3); nothing else may be
+    // contracted, so the result is fixed by this source in every kernel that
+    // inlines it.
+#pragma clang fp contract(off)
+    const float4 x0 = s_v[lv[0]], x1 = s_v[lv[1]], x2 = s_v[lv[2]], x3 = s_v[lv[3]];
+    const float4 r0 = s_r[lv[0]], r1 = s_r[lv[1]], r2 = s_r[lv[2]], r3 = s_r[lv[3]];
+    // Dm^-1 and V rebuilt from the rest positions staged in LDS with the same
+    // operations as k_precompute (bit-identical to the stored values): 12 B
+    // per tile node instead of 40 B per element from HBM.
+    const float Dm[9] = {r1.x - r0.x, r2.x - r0.x, r3.x - r0.x,
+                         r1.y - r0.y, r2.y - r0.y, r3.y - r0.y,
+                         r1.z - r0.z, r2.z - r0.z, r3.z - r0.z};
+    const float detDm = refmath::det3(Dm);
+    const float V = detDm / 6.0f;
+    float G[9];
+    refmath::inv3(Dm, detDm, G);
+    const float D[9] = {x1.x - x0.x, x2.x - x0.x, x3.x - x0.x,   // columns v1-v0, v2-v0, v3-v0
+                        x1.y - x0.y, x2.y - x0.y, x3.y - x0.y,
+                        x1.z - x0.z, x2.z - x0.z, x3.z - x0.z};
+    float RD[9];  // R^T D  (f_e: minus the rest edges Dm, the local displacement differences)
+#pragma unroll
+    for (int i3 = 0; i3 < 3; i3++)
+#pragma unroll
+        for (int j = 0; j < 3; j++)
+            RD[3 * i3 + j] = fma3(R[0 + i3], D[0 + j], R[3 + i3], D[3 + j], R[6 + i3], D[6 + j]);
+    if (MODE == kModeFe) {
+#pragma unroll
+        for (int k = 0; k < 9; k++) RD[k] -= Dm[k];
+    }
+    float H[9];  // RD G
+#pragma unroll
+    for (int i3 = 0; i3 < 3; i3++)
+#pragma unroll
+        for (int j = 0; j < 3; j++)
+            H[3 * i3 + j] = fma3(RD[3 * i3 + 0], G[0 + j], RD[3 * i3 + 1], G[3 + j], RD[3 * i3 + 2], G[6 + j]);
+    const float sxx = fmaf(m.a_k1, H[0], m.a_nu * (H[4] + H[8]));
+    const float syy = fmaf(m.a_k1, H[4], m.a_nu * (H[0] + H[8]));
+    const float szz = fmaf(m.a_k1, H[8], m.a_nu * (H[0] + H[4]));
+    const float sxy = m.a_k2 * (H[1] + H[3]);
+    const float syz = m.a_k2 * (H[5] + H[7]);
+    const float szx = m.a_k2 * (H[2] + H[6]);
+    const float S[9] = {sxx, sxy, szx, sxy, syy, syz, szx, syz, szz};
+    float W[9];  // -V sigma G^T
+#pragma unroll
+    for (int r = 0; r < 3; r++)
+#pragma unroll
+        for (int c = 0; c < 3; c++)
+            W[3 * r + c] = -V * fma3(S[3 * r + 0], G[3 * c + 0], S[3 * r + 1], G[3 * c + 1], S[3 * r + 2], G[3 * c + 2]);
+    float P[9];  // R W: column c is the force on vertex c+1
+#pragma unroll
+    for (int r = 0; r < 3; r++)
+#pragma unroll
+        for (int c = 0; c < 3; c++)
+            P[3 * r + c] = fma3(R[3 * r + 0], W[0 + c], R[3 * r + 1], W[3 + c], R[3 * r + 2], W[6 + c]);
+    s_F[fs[1]] = make_float4(P[0], P[3], P[6], 0.f);
+    s_F[fs[2]] = make_float4(P[1], P[4], P[7], 0.f);
+    s_F[fs[3]] = make_float4(P[2], P[5], P[8], 0.f);
+    s_F[fs[0]] = make_float4(-(P[0] + P[1] + P[2]), -(P[3] + P[4] + P[5]), -(P[6] + P[7] + P[8]), 0.f);
+}
+
+
+// ============================================================ block bodies
+// Each body is the work of block `blk` of a grid of `nblk` blocks over one
+// mesh (grid-stride loops over that grid). The single-mesh kernels call them
+// with (blockIdx.x, gridDim.x); the batched kernels with the block's place in
+// its own mesh's grid, so a mesh's arithmetic, and the order of every sum, are
+// the same alone and inside a batch.
+
+// BeginEvaluation (ms/fem/TetrahedronElementSet3.cpp:219-281, rotations
+// :748-892): F = Ds Dm^-1, det F, R by the rotation method; per-block counts
+// of det F <= 0 (and their sum), of DAPD R = Id fallbacks, and min det F.
+// blk_count holds [0,nblk) degenerate counts and [nblk,2nblk) fallbacks.
+__device__ inline void begin_eval_block(uint32_t T, const uint32_t* __restrict__ conn, const float* __restrict__ x,
+                                        const float* __restrict__ dinv, int rm, float thr, float* __restrict__ R,
+                                        float* __restrict__ detF, uint32_t* blk_count, double* blk_sum,
+                                        float* blk_min, uint32_t blk, uint32_t nblk, double* sh, float* shm,
+                                        uint32_t* shc, uint32_t* shf) {
+#pragma clang fp contract(off)
+    uint32_t cnt = 0, fb = 0;
+    double sdet = 0.0;
+    float mdet = 3.0e38f;
+    for (uint32_t e = blk * kBlock + threadIdx.x; e < T; e += nblk * kBlock) {
+        float xs[4][3];
+        for (int j = 0; j < 4; j++) {
+            const uint32_t g = conn[j * (size_t)T + e];
+            xs[j][0] = x[3 * (size_t)g]; xs[j][1] = x[3 * (size_t)g + 1]; xs[j][2] = x[3 * (size_t)g + 2];
+        }
+        float Ds[9], G[9], F[9];
+        for (int i = 0; i < 3; i++) {
+            Ds[3 * i + 0] = xs[1][i] - xs[0][i];
+            Ds[3 * i + 1] = xs[2][i] - xs[0][i];
+            Ds[3 * i + 2] = xs[3][i] - xs[0][i];
+        }
+        for (int k = 0; k < 9; k++) G[k] = dinv[k * (size_t)T + e];
+        refmath::mul3(Ds, G, F);
+        const float d = refmath::det3(F);
+        float Rm[9] = {1.f, 0.f, 0.f, 0.f, 1.f, 0.f, 0.f, 0.f, 1.f};
+        if (rm == CF_RM_MSVD) {
+            refmath::svd_rotation(F, Rm);
+        } else if (rm == CF_RM_DAPD) {
+            if (d > thr) refmath::polar_newton(F, Rm);
+        }
+        for (int k = 0; k < 9; k++) R[k * (size_t)T + e] = Rm[k];
+        detF[e] = d;
+        if (d <= 0.0f) { cnt++; sdet += d; }
+        // DAPD elements at or below the threshold take R = Id: counted too.
+        if (rm == CF_RM_DAPD && !(d > thr)) fb++;
+        mdet = fminf(mdet, d);
+    }
+    double s = block_sum(sdet, sh);
+    for (int o = 32; o > 0; o >>= 1) {
+        mdet = fminf(mdet, __shfl_xor(mdet, o, 64));
+        cnt += __shfl_xor(cnt, o, 64);
+        fb += __shfl_xor(fb, o, 64);
+    }
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    if (l == 0) { shm[w] = mdet; shc[w] = cnt; shf[w] = fb; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float mm = shm[0];
+        uint32_t cc = shc[0], ff = shf[0];
+        for (int i = 1; i < kBlock / 64; i++) { mm = fminf(mm, shm[i]); cc += shc[i]; ff += shf[i]; }
+        blk_count[blk] = cc;
+        blk_count[nblk + blk] = ff;
+        blk_sum[blk] = s;
+        blk_min[blk] = mm;
+    }
+}
+
+// One tile of the element stage without prefetching: stage the tile's node
+// vectors and rest positions in LDS, write each element's nodal forces to its
+// force slots, sum each tile node's contiguous slot run into its partial.
+template <int MODE>
+__device__ inline void tile_block(const TileArgs& a, uint32_t t, float4* s_v, float4* s_r, float4* s_F,
+                                  uint32_t* s_sp) {
+#pragma clang fp contract(off)
+    const uint32_t tid = threadIdx.x;
+    const TileDesc dt = tile_desc(a, t);
+    TileRegs g;
+    g.sp = 0;
+    g.pd = 0;
+    const uint32_t gid = (tid < dt.nn) ? a.tile_nodes[dt.n0 + tid] : 0u;
+    load_tile_regs(a, dt, gid, tid, g);
+    if (tid < dt.nn) {
+        s_v[tid] = make_float4(g.v[0], g.v[1], g.v[2], 0.f);
+        s_r[tid] = make_float4(g.r[0], g.r[1], g.r[2], 0.f);
+    }
+    if (tid <= dt.nn) s_sp[tid] = g.sp;
+    lds_barrier();
+    if (tid < dt.ne) element_force<MODE>(s_v, s_r, s_F, g.l, g.f, g.R, a.m);
+    lds_barrier();
+    if (tid < dt.nn) {
+        float fx = 0.f, fy = 0.f, fz = 0.f;
+        const uint32_t s1 = s_sp[tid + 1];
+        for (uint32_t sidx = s_sp[tid]; sidx < s1; sidx++) {
+            const float4 f = s_F[sidx];
+            fx += f.x; fy += f.y; fz += f.z;
+        }
+        float* o = a.partials + 3 * (size_t)g.pd;
+        o[0] = fx; o[1] = fy; o[2] = fz;
+    }
+}
+
+// Node gather: node n sums its tile partials in tile order, then the caller's
+// per-node terms. Returns the node's d.q contribution (matvec), else 0.
+template <int MODE>
+__device__ inline double gather_node(uint32_t n, const uint32_t* __restrict__ node_part_ptr,
+                                     const float* __restrict__ partials, const NodeTerms& t) {
+#pragma clang fp contract(off)
+    float fx = 0.f, fy = 0.f, fz = 0.f;
+    const uint32_t p1 = node_part_ptr[n + 1];
+    for (uint32_t p = node_part_ptr[n]; p < p1; p++) {
+        const float* pp = partials + 3 * (size_t)p;
+        fx += pp[0]; fy += pp[1]; fz += pp[2];
+    }
+    const size_t i = 3 * (size_t)n;
+    if (MODE == kGatherAccumulate) {
+        t.out[i] += fx; t.out[i + 1] += fy; t.out[i + 2] += fz;
+        return 0.0;
+    } else if (MODE == kGatherMatvec) {
+        // LS_A_times_d (LeafDSH_Solid3D_FEM.cpp:1056-1079) + LS_ZeroKNC (:1045-1053)
+        const float mass = t.mass_array ? t.mass_array[n] : t.mass;
+        const float fac = mass * t.fM;
+        const float d0 = t.d[i], d1 = t.d[i + 1], d2 = t.d[i + 2];
+        float q0 = -t.fK * fx + fac * d0;
+        float q1 = -t.fK * fy + fac * d1;
+        float q2 = -t.fK * fz + fac * d2;
+        if (t.knc_mask && t.knc_mask[n]) { q0 = 0.f; q1 = 0.f; q2 = 0.f; }
+        t.out[i] = q0; t.out[i + 1] = q1; t.out[i + 2] = q2;
+        return (double)d0 * q0 + (double)d1 * q1 + (double)d2 * q2;
+    } else {
+        // b = dt (M g + f_e) + M v ; y = v ; KNC rows take the constraint
+        // velocity (Step_QuasiImplicitEuler_v :1209-1295).
+        const float mass = t.mass_array ? t.mass_array[n] : t.mass;
+        const float f[3] = {fx, fy, fz};
+        float b[3], y[3];
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const float vc = t.v[i + c];
+            b[c] = (mass * t.g[c] + f[c]) * t.dt + mass * vc;
+            y[c] = t.warm_start ? vc : 0.0f;
+        }
+        if (t.knc_mask && t.knc_mask[n]) {
+            float vk[3];
+#pragma unroll
+            for (int c = 0; c < 3; c++) vk[c] = (t.knc_target[i + c] - t.x[i + c]) * t.inv_dt;
+            const float nk = sqrtf(vk[0] * vk[0] + vk[1] * vk[1] + vk[2] * vk[2]);
+            if (nk > 0.001f) {
+                const float inv = 1.0f / nk, s = fminf(nk, 50.0f);
+#pragma unroll
+                for (int c = 0; c < 3; c++) vk[c] = s * (vk[c] * inv);
+            } else {
+                vk[0] = vk[1] = vk[2] = 0.f;
+            }
+#pragma unroll
+            for (int c = 0; c < 3; c++) { b[c] = vk[c]; y[c] = vk[c]; }
+        }
+#pragma unroll
+        for (int c = 0; c < 3; c++) { t.out[i + c] = b[c]; t.y[i + c] = y[c]; }
+        return 0.0;
+    }
+}
+
+template <int MODE>
+__device__ inline void gather_block(uint32_t N, const uint32_t* __restrict__ node_part_ptr,
+                                    const float* __restrict__ partials, const NodeTerms& t, uint32_t blk,
+                                    uint32_t nblk, double* sh) {
+#pragma clang fp contract(off)
+    double local = 0.0;
+    for (uint32_t n = blk * kBlock + threadIdx.x; n < N; n += nblk * kBlock)
+        local += gather_node<MODE>(n, node_part_ptr, partials, t);
+    if (MODE == kGatherMatvec) {
+        const double s = block_sum(local, sh);
+        if (threadIdx.x == 0) t.partials[blk] = s;
+    }
+}
+
+// C_V (ms/fem/TetrahedronElementSet3.cpp:1456-1488): 1/2 u^T Ke u = 1/2 V sigma:H
+__device__ inline void energy_block(uint32_t T, const uint32_t* __restrict__ conn, const float* __restrict__ ref,
+                                    const float* __restrict__ x, const float* __restrict__ R,
+                                    const float* __restrict__ dinv, const float* __restrict__ vol, Material m,
+                                    double* blk_part, uint32_t blk, uint32_t nblk, double* sh) {
+#pragma clang fp contract(off)
+    double acc = 0.0;
+    for (uint32_t e = blk * kBlock + threadIdx.x; e < T; e += nblk * kBlock) {
+        float xs[4][3], rs[4][3];
+        for (int j = 0; j < 4; j++) {
+            const uint32_t g = conn[j * (size_t)T + e];
+            xs[j][0] = x[3 * (size_t)g]; xs[j][1] = x[3 * (size_t)g + 1]; xs[j][2] = x[3 * (size_t)g + 2];
+            rs[j][0] = ref[3 * (size_t)g]; rs[j][1] = ref[3 * (size_t)g + 1]; rs[j][2] = ref[3 * (size_t)g + 2];
+        }
+        float D[9], Rm[9], G[9], RD[9], H[9];
+        for (int c = 0; c < 3; c++) {
+            D[3 * c + 0] = xs[1][c] - xs[0][c];
+            D[3 * c + 1] = xs[2][c] - xs[0][c];
+            D[3 * c + 2] = xs[3][c] - xs[0][c];
+        }
+        for (int k = 0; k < 9; k++) { Rm[k] = R[k * (size_t)T + e]; G[k] = dinv[k * (size_t)T + e]; }
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++)
+                RD[3 * i + j] = Rm[i] * D[j] + Rm[3 + i] * D[3 + j] + Rm[6 + i] * D[6 + j] - (rs[j + 1][i] - rs[0][i]);
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) H[3 * i + j] = RD[3 * i] * G[j] + RD[3 * i + 1] * G[3 + j] + RD[3 * i + 2] * G[6 + j];
+        const float sxx = m.a_k1 * H[0] + m.a_nu * (H[4] + H[8]);
+        const float syy = m.a_k1 * H[4] + m.a_nu * (H[0] + H[8]);
+        const float szz = m.a_k1 * H[8] + m.a_nu * (H[0] + H[4]);
+        const float sxy = m.a_k2 * (H[1] + H[3]);
+        const float syz = m.a_k2 * (H[5] + H[7]);
+        const float szx = m.a_k2 * (H[2] + H[6]);
+        const float sH = sxx * H[0] + syy * H[4] + szz * H[8] + sxy * (H[1] + H[3]) + syz * (H[5] + H[7]) +
+                         szx * (H[2] + H[6]);
+        acc += 0.5 * (double)vol[e] * (double)sH;
+    }
+    const double s = block_sum(acc, sh);
+    if (threadIdx.x == 0) blk_part[blk] = s;
+}
+
+// ------------------------------------------------------------ CG bodies
+// ILSS_CG::Solve (ns/ILSS_CG.h:38-151) over n = 3N floats; the projection S
+// zeroes the rows of KNC nodes. Vector bodies walk n floats as float4 when
+// every pointer is 16-byte aligned (V4), the n % 4 tail by block 0.
+
+// r = S(b - q) ; d = r ; per-block r.r (one float per thread)
+__device__ inline void cg_init_block(uint32_t n, const float* __restrict__ q, const float* __restrict__ b,
+                                     float* __restrict__ r, float* __restrict__ d, const uint8_t* __restrict__ mask,
+                                     double* rr_part, uint32_t blk, uint32_t nblk, double* sh) {
+#pragma clang fp contract(off)
+    double acc = 0.0;
+    for (uint32_t i = blk * kBlock + threadIdx.x; i < n; i += nblk * kBlock) {
+        float ri = -1.0f * q[i] + b[i];  // muleq_addeq(r, -1, b)
+        if (mask && mask[i / 3]) ri = 0.f;
+        r[i] = ri;
+        d[i] = ri;
+        acc += (double)ri * ri;
+    }
+    const double s = block_sum(acc, sh);
+    if (threadIdx.x == 0) rr_part[blk] = s;
+}
+
+// x += alpha d ; r -= alpha q ; per-block r.r (ILSS_CG.h:100-117)
+template <bool V4>
+__device__ inline void update_xr_block(uint32_t n, float alpha, float* __restrict__ x, float* __restrict__ r,
+                                       const float* __restrict__ d, const float* __restrict__ q, double* rr_part,
+                                       uint32_t blk, uint32_t nblk, double* sh) {
+#pragma clang fp contract(off)
+    double acc = 0.0;
+    const uint32_t gid = blk * kBlock + threadIdx.x, stride = nblk * kBlock;
+    uint32_t tail0 = 0;
+    if (V4) {
+        const uint32_t n4 = n / 4;
+        tail0 = 4 * n4;
+        float4* x4 = reinterpret_cast<float4*>(x);
+        float4* r4 = reinterpret_cast<float4*>(r);
+        const float4* d4 = reinterpret_cast<const float4*>(d);
+        const float4* q4 = reinterpret_cast<const float4*>(q);
+        for (uint32_t i = gid; i < n4; i += stride) {
+            const float4 di = d4[i], qi = q4[i];
+            float4 xi = x4[i], ri = r4[i];
+            xi.x = xi.x + di.x * alpha; xi.y = xi.y + di.y * alpha;
+            xi.z = xi.z + di.z * alpha; xi.w = xi.w + di.w * alpha;
+            ri.x = ri.x - qi.x * alpha; ri.y = ri.y - qi.y * alpha;
+            ri.z = ri.z - qi.z * alpha; ri.w = ri.w - qi.w * alpha;
+            x4[i] = xi;
+            r4[i] = ri;
+            acc += (double)ri.x * ri.x + (double)ri.y * ri.y + (double)ri.z * ri.z + (double)ri.w * ri.w;
+        }
+    }
+    for (uint32_t i = tail0 + gid; i < n; i += stride) {
+        x[i] = x[i] + d[i] * alpha;
+        const float ri = r[i] - q[i] * alpha;
+        r[i] = ri;
+        acc += (double)ri * ri;
+    }
+    const double s = block_sum(acc, sh);
+    if (threadIdx.x == 0) rr_part[blk] = s;
+}
+
+// d = S(beta d + r)
+template <bool V4>
+__device__ inline void update_d_block(uint32_t n, float beta, float* __restrict__ d, const float* __restrict__ r,
+                                      const uint8_t* __restrict__ mask, uint32_t blk, uint32_t nblk) {
+#pragma clang fp contract(off)
+    const uint32_t gid = blk * kBlock + threadIdx.x, stride = nblk * kBlock;
+    uint32_t tail0 = 0;
+    if (V4) {
+        const uint32_t n4 = n / 4;
+        tail0 = 4 * n4;
+        float4* d4 = reinterpret_cast<float4*>(d);
+        const float4* r4 = reinterpret_cast<const float4*>(r);
+        for (uint32_t i = gid; i < n4; i += stride) {
+            const float4 ri = r4[i];
+            float4 di = d4[i];
+            di.x = beta * di.x + ri.x; di.y = beta * di.y + ri.y;
+            di.z = beta * di.z + ri.z; di.w = beta * di.w + ri.w;
+            if (mask) {
+                const uint32_t e = 4 * i;
+                if (mask[e / 3]) di.x = 0.f;
+                if (mask[(e + 1) / 3]) di.y = 0.f;
+                if (mask[(e + 2) / 3]) di.z = 0.f;
+                if (mask[(e + 3) / 3]) di.w = 0.f;
+            }
+            d4[i] = di;
+        }
+    }
+    for (uint32_t i = tail0 + gid; i < n; i += stride) {
+        float di = beta * d[i] + r[i];
+        if (mask && mask[i / 3]) di = 0.f;
+        d[i] = di;
+    }
+}
+
+// v = y ; x = x + dt y (Step_QuasiImplicitEuler_v :1322-1327), air drag (FixedStep :982-984)
+__device__ inline void integrate_block(uint32_t n, float* __restrict__ x, float* __restrict__ v,
+                                       const float* __restrict__ y, float dt, float drag, uint32_t blk,
+                                       uint32_t nblk) {
+#pragma clang fp contract(off)
+    for (uint32_t i = blk * kBlock + threadIdx.x; i < n; i += nblk * kBlock) {
+        const float yi = y[i];
+        x[i] = x[i] + dt * yi;
+        v[i] = (drag > 0.f) ? (1.0f - dt * drag) * yi : yi;
+    }
+}
+
+}  // namespace cf

@@ -40,6 +40,8 @@ struct Material {
     float a_k2;  // a(0.5-nu) = mu
 };
 Material material_from(float young, float poisson);
+// Rayleigh coefficients c = inv(Omega) (2 zeta, 2 zeta) (LeafDSH_Solid3D_FEM.h:347-354)
+void rayleigh_coeffs(float f1, float f2, float zeta, float c[2]);
 
 // Device layout of an element set, elements in tile order (see cf_tiles.cpp).
 struct TetDevice {
@@ -77,7 +79,7 @@ struct TileHost {
     std::vector<uint32_t> node_part_ptr, part_dst;
     uint32_t max_tile_elems = 0, max_tile_nodes = 0;
 };
-// Tiles hold at most min(tile_elems, max_per_tile) elements and max_per_tile nodes.
+// Tiles hold at most min(tile_elems, max_per_tile) elements and max_per_tile - 1 nodes.
 int build_tiles(uint32_t N, uint32_t T, const float* ref_pos, const uint32_t* tets, uint32_t tile_elems,
                 uint32_t max_per_tile, TileHost& out);
 int tile_grid_size(const TetDevice& t, int device, int persistent);
@@ -150,4 +152,91 @@ int launch_cg_update_d(uint32_t n, uint32_t k, CgScalars* st, float* d, const fl
 int launch_integrate(uint32_t n, float* x, float* v, const float* y, float dt, float drag, hipStream_t s);
 int launch_zero_mask(uint32_t n_nodes, float* w, const uint8_t* mask, hipStream_t s);
 
+
+// ---------------------------------------------------------------- batches
+// A batch steps many simulations together. Every simulation keeps the arrays
+// of the single-mesh path (tiles, partials, CG vectors) and its own FP64
+// partial buffers; the batched kernels find their simulation and their block
+// within that simulation's grid through block tables, and run the same block
+// bodies, so a simulation's arithmetic does not depend on the batch around it.
+struct SimDev {
+    // sizes and per-simulation grids (same grids as the single-mesh path)
+    uint32_t N, T, num_tiles;
+    uint32_t nb_elem;  // node_blocks(T): BeginEvaluation, energy
+    uint32_t nb_node;  // node_blocks(N): gathers
+    uint32_t nb_init;  // node_blocks(3N): CG init, integration
+    uint32_t nb_vec;   // vec_blocks(3N): CG vector updates
+    // topology (may be shared by several simulations)
+    const uint32_t* conn;
+    const uint32_t* tile_elem_ptr;
+    const uint32_t* tile_node_ptr;
+    const uint32_t* tile_nodes;
+    const uint32_t* slot_ptr;
+    const uint32_t* node_part_ptr;
+    const uint32_t* part_dst;
+    const uint16_t* dst;
+    const uint16_t* lvid;
+    const float* ref;
+    const float* dinv;
+    const float* vol;
+    // state of this simulation
+    float *x, *v, *b, *y, *r, *d, *q, *R, *detF, *partials, *knc_target;
+    const uint8_t* knc_mask;  // null without KNC
+    double* part_a;           // nb_node d.q partials
+    double* part_b;           // max(nb_init, nb_vec) r.r partials
+    double* part_e;           // nb_elem energy partials
+    uint32_t* eval_count;     // 2 nb_elem
+    double* eval_sum;         // nb_elem
+    float* eval_min;          // nb_elem
+    // parameters
+    Material m;
+    float mass, fM, fK, dt, inv_dt, drag, thr, rel, abs_eps;
+    float g[3];
+    int32_t rm, warm_start, compute_energy;
+    uint32_t max_iter;
+};
+
+// CG scalars of one simulation for the current solve, plus its step results.
+struct SimCg {
+    double delta, delta0, target, abs_sq, dq;
+    float alpha, beta;
+    uint32_t iters, max_iter;
+    int32_t done, reason;  // done: 1 stopped by the tests, 2 breakdown
+    double energy, sum_degenerate;
+    uint32_t num_degenerate, num_fallback;
+    float min_det;
+    uint32_t pad;
+};
+
+struct BlockRef { uint32_t slot, blk; };  // slot 0xffffffff: idle block
+constexpr uint32_t kIdleSlot = 0xffffffffu;
+
+struct BatchTables {
+    const SimDev* sims;
+    SimCg* cgs;
+    const BlockRef* tile; uint32_t n_tile;
+    const BlockRef* elem; uint32_t n_elem;
+    const BlockRef* node; uint32_t n_node;
+    const BlockRef* init; uint32_t n_init;
+    const BlockRef* vec; uint32_t n_vec;
+    const uint32_t* active; uint32_t n_active;  // slots of the stepped simulations
+    uint32_t* counter;                          // simulations still iterating
+    uint32_t max_tn, max_te, tile_threads;
+};
+
+enum BatchVec { kSelY = 0, kSelD = 1 };
+int launch_batch_begin_eval(const BatchTables& t, hipStream_t s);
+int launch_batch_forces(const BatchTables& t, int mode, int sel, hipStream_t s);
+int launch_batch_gather_rhs(const BatchTables& t, hipStream_t s);
+int launch_batch_gather_matvec(const BatchTables& t, int sel, hipStream_t s);
+int launch_batch_cg_init(const BatchTables& t, hipStream_t s);
+int launch_batch_cg_start(const BatchTables& t, hipStream_t s);
+int launch_batch_cg_alpha(const BatchTables& t, hipStream_t s);
+int launch_batch_update_xr(const BatchTables& t, hipStream_t s);
+int launch_batch_cg_beta(const BatchTables& t, hipStream_t s);
+int launch_batch_update_d(const BatchTables& t, hipStream_t s);
+int launch_batch_integrate(const BatchTables& t, hipStream_t s);
+int launch_batch_energy(const BatchTables& t, hipStream_t s);
+int launch_batch_finish(const BatchTables& t, hipStream_t s);
+int launch_batch_fill_u32(uint32_t* p, uint32_t value, uint32_t n, hipStream_t s);
 }  // namespace cf

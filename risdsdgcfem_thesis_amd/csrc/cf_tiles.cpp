@@ -1,13 +1,14 @@
 // Host construction of the LDS tiles used by the atomics-free node gather.
 //
 // Elements are sorted along a Morton (Z-order) curve of their rest centroids
-// and cut into tiles of at most `tile_elems` elements and as many distinct
-// nodes as the tile kernel has threads (one element and one node per thread). For every tile we record
+// and cut into tiles of at most `tile_elems` elements and fewer distinct nodes
+// than the tile kernel has threads (one element and one node per thread, plus
+// the end of the last node's slot run). For every tile we record
 //   - its distinct nodes (ascending global id) -> one partial per (tile, node)
 //   - per element, the tile-local index of each of its 4 vertices (uint16)
-//   - per tile node, the (local element, vertex) slots that touch it, in
-//     ascending element order (the order the reference's serial scatter would
-//     accumulate them), as uint16 codes elem*4+vertex
+//   - per (element, vertex) its force slot: the slots of one tile node form a
+//     contiguous run in ascending element order (the order the reference's
+//     serial scatter accumulates them), and tile nodes are ordered by valence
 // and, per global node, the contiguous run of its partials in ascending tile order.
 // All of this is integer work done once per mesh.
 #include "cf_internal.h"
@@ -43,7 +44,9 @@ int build_tiles(uint32_t N, uint32_t T, const float* ref, const uint32_t* tets, 
     if (tile_elems == 0) tile_elems = default_tile_elems();
     if (tile_elems > max_per_tile) tile_elems = max_per_tile;  // one element per thread
     if (tile_elems > 16383) tile_elems = 16383;  // slot codes are uint16 (elem*4+vertex)
-    const uint32_t max_nodes = std::min(kMaxTileNodes, max_per_tile);  // one node per thread
+    // one node per thread, and one more thread stages the end of the last
+    // node's slot run: at most max_per_tile - 1 nodes
+    const uint32_t max_nodes = std::min(kMaxTileNodes, max_per_tile) - 1;
     // ---- Morton order of centroids
     float lo[3] = {3e38f, 3e38f, 3e38f}, hi[3] = {-3e38f, -3e38f, -3e38f};
     for (uint32_t n = 0; n < N; n++)

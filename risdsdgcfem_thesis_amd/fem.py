@@ -5,6 +5,8 @@ over the corofem C-ABI.
                               IForce3 cache (ms/fem/TetrahedronElementSet3.h:29-261)
   ILSS_CG                 <-> ns::ILSS_CG (ns/ILSS_CG.h:19-151)
   LeafSolid3D             <-> ds::LeafDSH_Solid3D_FEM (ds/dsh/LeafDSH_Solid3D_FEM.h:68-81)
+  Batch                   <-> many LeafDSH_Solid3D_FEM leaves stepped together (no
+                              reference counterpart; the reference loops over leaves)
 
 Node vectors are flat float32 arrays of 3N values. Device vectors are
 DeviceArray objects (memory owned by libcorofem's allocator).
@@ -12,7 +14,7 @@ DeviceArray objects (memory owned by libcorofem's allocator).
 import ctypes as C
 import numpy as np
 
-from ._capi import (lib, check, FemParams, SolidParams, StepStats, CgResult, AV_FN, CONS_FN,
+from ._capi import (lib, check, FemParams, SolidParams, StepStats, CgResult, BatchInfo, AV_FN, CONS_FN,
                     RM_MSVD, MM_C_WRP, DM_TRUNCATED)
 
 _f = C.POINTER(C.c_float)
@@ -344,3 +346,122 @@ class LeafSolid3D:
         ms = C.c_double()
         check(lib.cf_solid_time_df_x(self.h, int(reps), C.byref(ms)), "cf_solid_time_df_x")
         return ms.value
+
+
+class Batch:
+    """Many solids stepped together (cf_batch_*): meshes are shared
+    topologies, simulations carry their own parameters, KNC and state. Each
+    simulation's step equals LeafSolid3D.fixed_step on the same inputs."""
+
+    def __init__(self, hbm_budget_bytes=0, tile_threads=256, tile_elems=0, cg_chunk=8, use_graphs=True):
+        self.h = C.c_void_p()
+        check(lib.cf_batch_create(int(hbm_budget_bytes), C.byref(self.h)), "cf_batch_create")
+        check(lib.cf_batch_set_tiles(self.h, int(tile_threads), int(tile_elems)), "cf_batch_set_tiles")
+        check(lib.cf_batch_set_schedule(self.h, int(cg_chunk), int(bool(use_graphs))), "cf_batch_set_schedule")
+        self.mesh_nodes = {}
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib.cf_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_schedule(self, cg_chunk=8, use_graphs=True):
+        check(lib.cf_batch_set_schedule(self.h, int(cg_chunk), int(bool(use_graphs))), "cf_batch_set_schedule")
+
+    def add_mesh(self, ref_pos, tets):
+        r = np.ascontiguousarray(ref_pos, np.float32)
+        t = np.ascontiguousarray(tets, np.uint32)
+        mid = C.c_uint32()
+        check(lib.cf_batch_add_mesh(self.h, r.size // 3, _fp(r), t.size // 4, _up(t), C.byref(mid)),
+              "cf_batch_add_mesh")
+        self.mesh_nodes[mid.value] = r.size // 3
+        return mid.value
+
+    def add_meshes(self, meshes, threads=0):
+        """meshes: list of (ref_pos, tets); tiles built on host threads."""
+        rs = [np.ascontiguousarray(r, np.float32) for r, _ in meshes]
+        ts = [np.ascontiguousarray(t, np.uint32) for _, t in meshes]
+        n = len(rs)
+        nn = np.array([r.size // 3 for r in rs], np.uint32)
+        nt = np.array([t.size // 4 for t in ts], np.uint32)
+        rp = (_f * n)(*[_fp(r) for r in rs])
+        tp = (_u * n)(*[_up(t) for t in ts])
+        ids = np.zeros(n, np.uint32)
+        check(lib.cf_batch_add_meshes(self.h, n, _up(nn), rp, _up(nt), tp, int(threads), _up(ids)),
+              "cf_batch_add_meshes")
+        for i, mid in enumerate(ids):
+            self.mesh_nodes[int(mid)] = int(nn[i])
+        return [int(i) for i in ids]
+
+    def remove_mesh(self, mesh_id):
+        check(lib.cf_batch_remove_mesh(self.h, int(mesh_id)), "cf_batch_remove_mesh")
+        self.mesh_nodes.pop(mesh_id, None)
+
+    def sim_bytes(self, mesh_id):
+        b = C.c_uint64()
+        check(lib.cf_batch_sim_bytes(self.h, int(mesh_id), C.byref(b)), "cf_batch_sim_bytes")
+        return b.value
+
+    def add_sim(self, mesh_id, params=None, knc=None, knc_targets=None, x=None, v=None):
+        p = params if params is not None else default_solid_params()
+        k = np.ascontiguousarray(knc if knc is not None else [], np.uint32)
+        kt = None if knc_targets is None else np.ascontiguousarray(knc_targets, np.float32)
+        xa = None if x is None else np.ascontiguousarray(x, np.float32)
+        va = None if v is None else np.ascontiguousarray(v, np.float32)
+        sid = C.c_uint32()
+        check(lib.cf_batch_add_sim(self.h, int(mesh_id), C.byref(p), k.size, _up(k) if k.size else None,
+                                   None if kt is None else _fp(kt), None if xa is None else _fp(xa),
+                                   None if va is None else _fp(va), C.byref(sid)), "cf_batch_add_sim")
+        return sid.value
+
+    def remove_sim(self, sim_id):
+        check(lib.cf_batch_remove_sim(self.h, int(sim_id)), "cf_batch_remove_sim")
+
+    def set_active(self, sim_id, active=True):
+        check(lib.cf_batch_set_active(self.h, int(sim_id), int(bool(active))), "cf_batch_set_active")
+
+    def info(self):
+        i = BatchInfo()
+        check(lib.cf_batch_info_get(self.h, C.byref(i)), "cf_batch_info_get")
+        return {k: getattr(i, k) for k, _ in BatchInfo._fields_}
+
+    def active_ids(self):
+        n = self.info()["num_active"]
+        ids = np.zeros(max(n, 1), np.uint32)
+        check(lib.cf_batch_active_ids(self.h, _up(ids)), "cf_batch_active_ids")
+        return [int(i) for i in ids[:n]]
+
+    def fixed_step(self, stats=True):
+        """One fixed step of every active simulation; returns {sim_id: StepStats}."""
+        if not stats:
+            check(lib.cf_batch_fixed_step(self.h, None), "cf_batch_fixed_step")
+            return None
+        ids = self.active_ids()
+        st = (StepStats * max(len(ids), 1))()
+        check(lib.cf_batch_fixed_step(self.h, st), "cf_batch_fixed_step")
+        return {sid: st[i] for i, sid in enumerate(ids)}
+
+    def state(self, sim_id, n_nodes):
+        x = np.empty(3 * n_nodes, np.float32)
+        v = np.empty(3 * n_nodes, np.float32)
+        check(lib.cf_batch_get_state(self.h, int(sim_id), _fp(x), _fp(v)), "cf_batch_get_state")
+        return x, v
+
+    def set_state(self, sim_id, x, v):
+        x = np.ascontiguousarray(x, np.float32)
+        v = np.ascontiguousarray(v, np.float32)
+        check(lib.cf_batch_set_state(self.h, int(sim_id), _fp(x), _fp(v)), "cf_batch_set_state")
+
+    def time_kx(self, reps=20):
+        ms = C.c_double()
+        check(lib.cf_batch_time_kx(self.h, int(reps), C.byref(ms)), "cf_batch_time_kx")
+        return ms.value
+
+    def synchronize(self):
+        check(lib.cf_batch_synchronize(self.h), "cf_batch_synchronize")

@@ -269,3 +269,24 @@ def test_generic_cg_solver_against_numpy(gpu):
     want = np.linalg.solve(A, b.astype(np.float64))
     assert np.abs(x.numpy() - want).max() <= 1e-4 * np.abs(want).max()
     del tmp_in, tmp_out
+
+
+def test_df_x_disjoint_tets_vs_oracle(gpu, orc):
+    """Disjoint tets fill a tile's node budget (4 nodes per element): the tile
+    builder must leave one thread to stage the end of the last slot run."""
+    rng = np.random.default_rng(7)
+    n = 1000
+    F = np.eye(3, dtype=np.float32)[None] + 0.1 * rng.standard_normal((n, 3, 3)).astype(np.float32)
+    ref, tets, x = unit_tets_from_F(F.reshape(n, 9))
+    s = gpu.TetrahedronElementSet3(ref, tets, gpu.default_fem_params(young_modulus=1e5, poisson_ratio=0.3,
+                                                                     rm=gpu.RM_DAPD))
+    o = orc.TetSet(ref, tets, 1e5, 0.3)
+    s.begin_evaluation(gpu.DeviceArray.from_host(x))
+    R, _ = s.rotations()
+    o.set_rotations(R.reshape(-1))
+    v = rng.standard_normal(x.size).astype(np.float32)
+    df = gpu.DeviceArray.from_host(np.zeros(x.size, np.float32))
+    s.df_x(gpu.DeviceArray.from_host(v), df)
+    dg, do = df.numpy(), o.df_x(v)
+    assert np.linalg.norm(dg - do) <= 1e-6 * np.linalg.norm(do)
+    s.close()
