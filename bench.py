@@ -89,6 +89,99 @@ def cpu_baseline(budget_s=12.0):
                       f"{dt:.1f} s"}
 
 
+def bench_batch3d(cf, args):
+    """Uneven 3D batch: 1,024 cube grids (cubes per axis uniform in [4,40],
+    seed 4), E log-uniform [1e4,1e7], nu [0.2,0.45], a random face held. The
+    batched step (all meshes in the same launches, graph-recorded) against the
+    plain single-mesh path run mesh by mesh on every 8th mesh."""
+    from risdsdgcfem_thesis_amd import workloads
+    t0 = time.perf_counter()
+    meshes = workloads.uneven_batch(count=args.batch_count, cg_max_iter=args.batch_max_iter)
+    T = sum(m["tets"].size // 4 for m in meshes)
+    N = sum(m["pos"].size // 3 for m in meshes)
+    b = cf.Batch(tile_threads=args.tile_threads, tile_elems=args.tile_elems, cg_chunk=args.cg_chunk)
+    mids = b.add_meshes([(m["pos"], m["tets"]) for m in meshes], threads=16)
+    sids = [b.add_sim(mid, m["params"], knc=m["knc"]) for mid, m in zip(mids, meshes)]
+    log(f"[batch3d] {len(meshes)} meshes, {T} tets, {N} nodes; setup {time.perf_counter() - t0:.1f} s")
+    for _ in range(args.warmup):
+        b.fixed_step(stats=False)
+    b.synchronize()
+    iters = []
+    t1 = time.perf_counter()
+    for _ in range(args.batch_steps):
+        st = b.fixed_step()
+        iters.append([st[s].cg_iters for s in sids])
+    b.synchronize()
+    wall = time.perf_counter() - t1
+    it = np.array(iters)
+    reasons = np.array([st[s].cg_reason for s in sids])
+    batched_sps = len(meshes) * args.batch_steps / wall
+    kx_ms = b.time_kx(args.kx_reps)
+    b_alg = 92 * T + 24 * N
+    log(f"[batch3d] batched {wall / args.batch_steps * 1e3:.1f} ms/step, iters mean {it.mean():.1f} "
+        f"max {it.max()}, batched K*x {kx_ms * 1e3:.0f} us")
+    b.close()
+    # plain path, one mesh at a time
+    sample = meshes[::8]
+    plain_wall = 0.0
+    for m in sample:
+        s = cf.LeafSolid3D(m["pos"], m["tets"], m["params"], knc=m["knc"])
+        for _ in range(args.warmup):
+            s.fixed_step()
+        cf.synchronize()
+        t2 = time.perf_counter()
+        for _ in range(args.batch_steps):
+            s.fixed_step()
+        cf.synchronize()
+        plain_wall += time.perf_counter() - t2
+        s.close()
+    plain_sps = len(sample) * args.batch_steps / plain_wall
+    log(f"[batch3d] plain {plain_wall / (len(sample) * args.batch_steps) * 1e3:.2f} ms per mesh-step")
+    return {
+        "workload": "uneven 3D batch: 1,024 cube grids, 4-40 cubes per axis (seed 4), QIE-v, CG rel 1e-4",
+        "meshes": len(meshes), "tets": T, "nodes": N, "steps": args.batch_steps, "warmup": args.warmup,
+        "cg_max_iter": args.batch_max_iter, "cg_iters_mean": round(float(it.mean()), 2),
+        "cg_iters_max": int(it.max()), "converged_frac": round(float(np.isin(reasons, [1, 2]).mean()), 4),
+        "batched_steps_per_s": round(batched_sps, 1), "batched_ms_per_step": round(wall / args.batch_steps * 1e3, 2),
+        "plain_steps_per_s": round(plain_sps, 1), "plain_sample": f"{len(sample)} of {len(meshes)} meshes (every 8th)",
+        "speedup": round(batched_sps / plain_sps, 2),
+        "kx": {"ms": round(kx_ms, 4), "gtets": round(T / (kx_ms * 1e-3) / 1e9, 2),
+               "achieved_gbs": round(b_alg / (kx_ms * 1e-3) / 1e9, 1),
+               "frac": round(b_alg / (kx_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
+        "tile": {"threads": args.tile_threads, "elems": args.tile_elems, "cg_chunk": args.cg_chunk},
+    }
+
+
+def bench_serve(cf, args):
+    """Serving: Poisson arrivals at each rate for a fixed real-time window;
+    requests are seeded cube grids of 500-200,000 tets (log-uniform) with 1-60
+    steps and config-3 materials, admitted by a 200 GB HBM budget and a cap on
+    the tets stepped per tick."""
+    from risdsdgcfem_thesis_amd import workloads
+    out = []
+    for rate in args.serve_rates:
+        meshes, reqs = workloads.serving_trace(rate=rate, duration_s=args.serve_seconds, seed=5)
+        base = workloads.solid_params(1e5, 0.3, args.batch_max_iter)
+        res, lat = cf.serve(meshes, reqs, base, hbm_budget_bytes=int(args.serve_budget_gb * 1e9),
+                            duration_s=args.serve_seconds, drain=False, max_tick_tets=args.serve_tick_tets,
+                            cg_chunk=args.cg_chunk)
+        log(f"[serve] rate {rate}/s: {res['requests']} requests, {res['admitted']} admitted, {res['completed']} done, "
+            f"{res['steps']} steps in {res['wall_s']:.2f} s, p50 {res['step_p50_ms']:.2f} ms, "
+            f"p99 {res['step_p99_ms']:.2f} ms, max active {res['max_active']}")
+        out.append({"rate_per_s": rate, "requests": res["requests"], "admitted": res["admitted"],
+                    "completed": res["completed"], "steps": res["steps"],
+                    "steps_per_s": round(res["steps_per_s"], 1),
+                    "gtet_steps_per_s": round(res["tet_steps"] / res["wall_s"] / 1e9, 4) if res["wall_s"] else 0,
+                    "step_p50_ms": round(res["step_p50_ms"], 3), "step_p99_ms": round(res["step_p99_ms"], 3),
+                    "first_step_p50_ms": round(res["first_p50_ms"], 3),
+                    "first_step_p99_ms": round(res["first_p99_ms"], 3),
+                    "tick_mean_ms": round(res["tick_mean_ms"], 3), "max_active": res["max_active"],
+                    "peak_gb": round(res["peak_bytes"] / 1e9, 3), "window_s": args.serve_seconds})
+    return {"workload": "serving: Poisson arrivals, cube grids 500-200,000 tets, 1-60 steps, config-3 materials",
+            "hbm_budget_gb": args.serve_budget_gb, "max_tick_tets": args.serve_tick_tets,
+            "cg_max_iter": args.batch_max_iter, "rates": out}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -99,6 +192,15 @@ def main():
     ap.add_argument("--tile-threads", type=int, default=256)
     ap.add_argument("--tile-schedule", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--workload", default="single", choices=["single", "batch3d", "serve"])
+    ap.add_argument("--batch-count", type=int, default=1024)
+    ap.add_argument("--batch-steps", type=int, default=3)
+    ap.add_argument("--batch-max-iter", type=int, default=50)
+    ap.add_argument("--cg-chunk", type=int, default=8)
+    ap.add_argument("--serve-rates", type=float, nargs="+", default=[100.0, 1000.0, 10000.0])
+    ap.add_argument("--serve-seconds", type=float, default=3.0)
+    ap.add_argument("--serve-budget-gb", type=float, default=200.0)
+    ap.add_argument("--serve-tick-tets", type=int, default=16_000_000)
     ap.add_argument("--kx-only", type=int, default=0, metavar="REPS",
                     help="profiling mode: one step, then REPS K*x launches and one 256 MiB device copy "
                          "(the copy calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE); prints timings only")
@@ -113,6 +215,12 @@ def main():
         print("no GPU visible", file=sys.stderr)
         sys.exit(1)
 
+    if args.workload == "batch3d":
+        print(json.dumps(bench_batch3d(cf, args)), flush=True)
+        return
+    if args.workload == "serve":
+        print(json.dumps(bench_serve(cf, args)), flush=True)
+        return
     cf.set_tile_elems(args.tile_elems)
     cf.set_tile_threads(args.tile_threads)
     cf.set_tile_persistent(args.tile_schedule)

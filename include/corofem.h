@@ -302,6 +302,55 @@ int cf_batch_info_get(const cf_batch* b, cf_batch_info* info);
 int cf_batch_time_kx(cf_batch* b, uint32_t reps, double* ms);
 int cf_batch_synchronize(cf_batch* b);
 
+/* ---------------------------------------------------------------- serving
+ * A real-time request loop over a batch (no reference counterpart: the
+ * reference steps the leaves of one scene; this serves many independent
+ * leaves). Requests arrive at given times; each asks for `steps` fixed steps
+ * of a solid on one of the given meshes. The loop admits waiting requests in
+ * arrival order while their bytes fit the HBM budget and the tets stepped per
+ * tick stay under max_tick_tets, keeps their state on the device, and steps
+ * every admitted request once per tick in one batched step (continuous
+ * batching: a request admitted while others run joins the next tick; a
+ * finished one leaves and frees its memory). Meshes are uploaded on first use
+ * and evicted, least recently used first, when unused and memory is needed.
+ * A step's latency runs from the moment it was requested (arrival for the
+ * first step, the previous step's result for the others) to its result. */
+typedef struct cf_serve_mesh {
+    uint32_t n_nodes;
+    const float* ref_pos;
+    uint32_t n_tets;
+    const uint32_t* tets;
+} cf_serve_mesh;
+typedef struct cf_serve_request {
+    double arrival_s;          /* seconds after the start of the run */
+    uint32_t mesh;             /* index into the mesh list */
+    uint32_t steps;            /* fixed steps requested (>= 1) */
+    float young_modulus, poisson_ratio;
+    uint32_t n_knc;            /* nodes held at their rest positions */
+    const uint32_t* knc_nids;
+} cf_serve_request;
+typedef struct cf_serve_config {
+    cf_solid_params base;      /* dt, density, gravity, CG settings (material from each request) */
+    uint64_t hbm_budget_bytes; /* meshes + simulations resident at once */
+    uint64_t max_tick_tets;    /* 0: no cap */
+    double duration_s;         /* arrivals after this are not served */
+    int32_t drain;             /* 1: run until every admitted and waiting request is done */
+    uint32_t tile_threads, cg_chunk;
+} cf_serve_config;
+typedef struct cf_serve_result {
+    uint32_t requests, admitted, completed, max_active, ticks, mesh_uploads;
+    uint64_t steps, tet_steps, peak_bytes;
+    double wall_s, steps_per_s, tick_mean_ms;
+    double step_p50_ms, step_p99_ms, step_mean_ms, step_max_ms;
+    double first_p50_ms, first_p99_ms;  /* arrival -> first step result */
+} cf_serve_result;
+/* h_final_pos: NULL or one pointer per request (NULL entries skipped) that
+   receives the final positions (3 n_nodes floats) of completed requests.
+   step_latency_ms: NULL or room for sum(steps) values, filled in completion order. */
+int cf_serve_trace(const cf_serve_mesh* meshes, uint32_t n_meshes, const cf_serve_request* reqs, uint32_t n_reqs,
+                   const cf_serve_config* cfg, float* const* h_final_pos, double* step_latency_ms,
+                   cf_serve_result* result);
+
 #ifdef __cplusplus
 }
 #endif

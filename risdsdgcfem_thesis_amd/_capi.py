@@ -57,6 +57,33 @@ class BatchInfo(C.Structure):
     ]
 
 
+class ServeMesh(C.Structure):
+    _fields_ = [("n_nodes", C.c_uint32), ("ref_pos", _f), ("n_tets", C.c_uint32), ("tets", _u)]
+
+
+class ServeRequest(C.Structure):
+    _fields_ = [("arrival_s", C.c_double), ("mesh", C.c_uint32), ("steps", C.c_uint32),
+                ("young_modulus", C.c_float), ("poisson_ratio", C.c_float), ("n_knc", C.c_uint32),
+                ("knc_nids", _u)]
+
+
+class ServeConfig(C.Structure):
+    _fields_ = [("base", SolidParams), ("hbm_budget_bytes", C.c_uint64), ("max_tick_tets", C.c_uint64),
+                ("duration_s", C.c_double), ("drain", C.c_int32), ("tile_threads", C.c_uint32),
+                ("cg_chunk", C.c_uint32)]
+
+
+class ServeResult(C.Structure):
+    _fields_ = [
+        ("requests", C.c_uint32), ("admitted", C.c_uint32), ("completed", C.c_uint32), ("max_active", C.c_uint32),
+        ("ticks", C.c_uint32), ("mesh_uploads", C.c_uint32), ("steps", C.c_uint64), ("tet_steps", C.c_uint64),
+        ("peak_bytes", C.c_uint64), ("wall_s", C.c_double), ("steps_per_s", C.c_double),
+        ("tick_mean_ms", C.c_double), ("step_p50_ms", C.c_double), ("step_p99_ms", C.c_double),
+        ("step_mean_ms", C.c_double), ("step_max_ms", C.c_double), ("first_p50_ms", C.c_double),
+        ("first_p99_ms", C.c_double),
+    ]
+
+
 AV_FN = C.CFUNCTYPE(None, _vp, _f, _f)
 CONS_FN = C.CFUNCTYPE(None, _vp, _f)
 
@@ -138,6 +165,9 @@ SIGNATURES = {
     "cf_batch_info_get": (C.c_int, [_vp, C.POINTER(BatchInfo)]),
     "cf_batch_time_kx": (C.c_int, [_vp, C.c_uint32, C.POINTER(C.c_double)]),
     "cf_batch_synchronize": (C.c_int, [_vp]),
+    "cf_serve_trace": (C.c_int, [C.POINTER(ServeMesh), C.c_uint32, C.POINTER(ServeRequest), C.c_uint32,
+                                 C.POINTER(ServeConfig), C.POINTER(_f), C.POINTER(C.c_double),
+                                 C.POINTER(ServeResult)]),
 }
 
 

@@ -465,3 +465,48 @@ class Batch:
 
     def synchronize(self):
         check(lib.cf_batch_synchronize(self.h), "cf_batch_synchronize")
+
+
+def serve(meshes, requests, params, hbm_budget_bytes, duration_s, drain=False, max_tick_tets=0, tile_threads=256,
+          cg_chunk=8, final_positions=None):
+    """Runs the real-time serving loop (cf_serve_trace).
+
+    meshes: list of (ref_pos, tets); requests: list of dicts with arrival (s),
+    mesh (index), steps, E, nu, knc (node ids held at rest), sorted by
+    arrival; params: base SolidParams (dt, density, gravity, CG settings).
+    final_positions: None or a list (one entry per request, None to skip) of
+    float32 arrays that receive the final positions.
+    Returns (result dict, per-step latencies in ms)."""
+    from ._capi import ServeMesh, ServeRequest, ServeConfig, ServeResult
+    keep = []
+    ms = (ServeMesh * len(meshes))()
+    for i, (r, t) in enumerate(meshes):
+        r = np.ascontiguousarray(r, np.float32)
+        t = np.ascontiguousarray(t, np.uint32)
+        keep += [r, t]
+        ms[i].n_nodes, ms[i].ref_pos, ms[i].n_tets, ms[i].tets = r.size // 3, _fp(r), t.size // 4, _up(t)
+    rq = (ServeRequest * max(len(requests), 1))()
+    for i, q in enumerate(requests):
+        k = np.ascontiguousarray(q.get("knc", []), np.uint32)
+        keep.append(k)
+        rq[i].arrival_s, rq[i].mesh, rq[i].steps = float(q["arrival"]), int(q["mesh"]), int(q["steps"])
+        rq[i].young_modulus, rq[i].poisson_ratio = float(q["E"]), float(q["nu"])
+        rq[i].n_knc, rq[i].knc_nids = k.size, (_up(k) if k.size else None)
+    cfg = ServeConfig()
+    cfg.base = params
+    cfg.hbm_budget_bytes = int(hbm_budget_bytes)
+    cfg.max_tick_tets = int(max_tick_tets)
+    cfg.duration_s = float(duration_s)
+    cfg.drain = int(bool(drain))
+    cfg.tile_threads = int(tile_threads)
+    cfg.cg_chunk = int(cg_chunk)
+    fp = None
+    if final_positions is not None:
+        fp = (_f * len(requests))(*[(_fp(a) if a is not None else None) for a in final_positions])
+    total_steps = sum(int(q["steps"]) for q in requests)
+    lat = np.zeros(max(total_steps, 1), np.float64)
+    res = ServeResult()
+    check(lib.cf_serve_trace(ms, len(meshes), rq, len(requests), C.byref(cfg), fp,
+                             lat.ctypes.data_as(C.POINTER(C.c_double)), C.byref(res)), "cf_serve_trace")
+    out = {k: getattr(res, k) for k, _ in ServeResult._fields_}
+    return out, lat[:res.steps].copy()
