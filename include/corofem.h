@@ -302,6 +302,45 @@ int cf_batch_info_get(const cf_batch* b, cf_batch_info* info);
 int cf_batch_time_kx(cf_batch* b, uint32_t reps, double* ms);
 int cf_batch_synchronize(cf_batch* b);
 
+/* ---------------------------------------------------------------- 2D solids
+ * The 2D twin: TriangleElementSet2 + LeafDSH_Solid2D_FEM (QIE-v, plane-strain
+ * linear triangles, Ke = A B^T E B, ms/fem/TriangleElement2.h:71-139;
+ * BeginEvaluation ms/fem/TriangleElementSet2.cpp:183-262; C_f_e :1253-1286;
+ * C_df_x :1859-1915; lumped masses ds/dsh/LeafDSH_Solid2D_FEM.cpp:1842-1863;
+ * Step_QuasiImplicitEuler_v :1287-1403). Node vectors are 2N floats.
+ * Rotations: the closed-form 2x2 polar factor for det F > threshold (the
+ * reference's MSVD and DAPD both take it there); below it R = Id (the 2D SVD
+ * and DAPD degenerate branches are not implemented).
+ * Each mesh is stepped inside one workgroup by a group of 1, 2, 4 or 8
+ * wavefronts (up to 2048 triangles and 1536 nodes; CF_ERR_UNSUPPORTED beyond).
+ * A batch steps all its simulations in ONE launch; each simulation's result is
+ * independent of the others in the batch. */
+typedef struct cf_batch2 cf_batch2;
+int cf_batch2_create(cf_batch2** out);
+int cf_batch2_destroy(cf_batch2* b);
+/* A simulation at rest (x = rest positions, v = 0), KNC nodes held at rest.
+   Simulation ids are 0, 1, 2, ... in order of addition. */
+int cf_batch2_add_sim(cf_batch2* b, uint32_t n_nodes, const float* h_ref_pos2, uint32_t n_tris,
+                      const uint32_t* h_tris, const cf_solid_params* params, uint32_t n_knc,
+                      const uint32_t* h_knc_nids, uint32_t* sim_id);
+/* One fixed step of every simulation; stats: NULL or one per simulation (id order). */
+int cf_batch2_fixed_step(cf_batch2* b, cf_step_stats* stats);
+int cf_batch2_get_state(cf_batch2* b, uint32_t sim_id, float* h_pos2, float* h_vel2);
+int cf_batch2_set_state(cf_batch2* b, uint32_t sim_id, const float* h_pos2, const float* h_vel2);
+int cf_batch2_node_masses(const cf_batch2* b, uint32_t sim_id, float* h_mass);
+int cf_batch2_info(const cf_batch2* b, uint32_t* num_sims, uint64_t* tris, uint64_t* nodes, uint32_t* workgroups);
+
+/* LeafDSH_Solid2D_FEM drop-in (Create / Step / FixedStep / QueryState): one
+   simulation, the same kernel launched for this mesh alone. */
+typedef struct cf_solid2 cf_solid2;
+int cf_solid2_create(uint32_t n_nodes, const float* h_ref_pos2, uint32_t n_tris, const uint32_t* h_tris,
+                     const cf_solid_params* params, uint32_t n_knc, const uint32_t* h_knc_nids, cf_solid2** out);
+int cf_solid2_destroy(cf_solid2* s);
+int cf_solid2_fixed_step(cf_solid2* s, cf_step_stats* stats);
+int cf_solid2_step(cf_solid2* s, float dt, cf_step_stats* stats);
+int cf_solid2_get_state(const cf_solid2* s, float* h_pos2, float* h_vel2);
+int cf_solid2_set_state(cf_solid2* s, const float* h_pos2, const float* h_vel2);
+
 /* ---------------------------------------------------------------- serving
  * A real-time request loop over a batch (no reference counterpart: the
  * reference steps the leaves of one scene; this serves many independent

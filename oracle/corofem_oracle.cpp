@@ -847,3 +847,270 @@ void orc_solid_set_state(orc_solid* s, const float* x, const float* v) {
 }
 
 }  // extern "C"
+
+// ================================================================ 2D twin
+struct orc_solid2 {
+    orc_solid_params p;
+    float thickness = 1.0f;
+    uint32_t N = 0, T = 0;
+    std::vector<float> ref;       // 2N
+    std::vector<uint32_t> tri;    // 3T
+    std::vector<float> Kb;        // T * 9 blocks * 4, block (i,j) at (3*i+j)*4, row-major 2x2
+    std::vector<float> Dinv;      // 4T
+    std::vector<float> area;      // T
+    float total_area = 0.0f;
+    std::vector<float> R, detF;   // 4T, T
+    std::vector<float> inv_mass;  // N
+    float c[2] = {0, 0};
+    std::vector<uint32_t> knc;
+    std::vector<float> x, v, b, y, r, d, q;
+};
+
+namespace {
+struct M2 { float a[2][2]; };
+inline M2 m2load(const float* p) { M2 m; m.a[0][0] = p[0]; m.a[0][1] = p[1]; m.a[1][0] = p[2]; m.a[1][1] = p[3]; return m; }
+inline void m2store(const M2& m, float* p) { p[0] = m.a[0][0]; p[1] = m.a[0][1]; p[2] = m.a[1][0]; p[3] = m.a[1][1]; }
+inline M2 m2mul(const M2& x, const M2& y) {  // Mal/GMat.h:135-145 (accumulate from 0)
+    M2 r;
+    for (int i = 0; i < 2; i++)
+        for (int j = 0; j < 2; j++) { float acc = 0.0f; acc += x.a[i][0] * y.a[0][j]; acc += x.a[i][1] * y.a[1][j]; r.a[i][j] = acc; }
+    return r;
+}
+inline void m2vec(const M2& m, const float v[2], float out[2]) {
+    for (int i = 0; i < 2; i++) { float acc = 0.0f; acc += m.a[i][0] * v[0]; acc += m.a[i][1] * v[1]; out[i] = acc; }
+}
+inline float det2(const M2& m) { return m.a[0][0] * m.a[1][1] - m.a[1][0] * m.a[0][1]; }  // Mal/GMat.h:230-233
+inline M2 inv2(const M2& m) {  // Mal/GMat.h:260-288
+    float id = 1.0f / det2(m);
+    M2 r;
+    r.a[0][0] = id * m.a[1][1]; r.a[0][1] = -id * m.a[0][1];
+    r.a[1][0] = -id * m.a[1][0]; r.a[1][1] = id * m.a[0][0];
+    return r;
+}
+inline M2 d2(const float* r0, const float* r1, const float* r2) {  // TriangleElement2::Compute_D
+    M2 D;
+    D.a[0][0] = r1[0] - r0[0]; D.a[0][1] = r2[0] - r0[0];
+    D.a[1][0] = r1[1] - r0[1]; D.a[1][1] = r2[1] - r0[1];
+    return D;
+}
+// v_i -= R * sum_j K_ij l_j (C_f_e / C_df_x, TriangleElementSet2.cpp:1253-1286,1859-1915)
+inline void corot_apply2(const orc_solid2* s, uint32_t e, const float l[3][2], float* out) {
+    const uint32_t* t = &s->tri[3 * size_t(e)];
+    const float* kb = &s->Kb[size_t(e) * 36];
+    M2 R = m2load(&s->R[4 * size_t(e)]);
+    for (int i = 0; i < 3; i++) {
+        float acc[2], w[2], tmp[2];
+        m2vec(m2load(kb + (3 * i + 0) * 4), l[0], acc);
+        m2vec(m2load(kb + (3 * i + 1) * 4), l[1], tmp); acc[0] += tmp[0]; acc[1] += tmp[1];
+        m2vec(m2load(kb + (3 * i + 2) * 4), l[2], tmp); acc[0] += tmp[0]; acc[1] += tmp[1];
+        m2vec(R, acc, w);
+        float* o = &out[2 * size_t(t[i])];
+        o[0] -= w[0]; o[1] -= w[1];
+    }
+}
+inline M2 m2tr(const M2& m) { M2 t; t.a[0][0] = m.a[0][0]; t.a[0][1] = m.a[1][0]; t.a[1][0] = m.a[0][1]; t.a[1][1] = m.a[1][1]; return t; }
+}  // namespace
+
+extern "C" {
+
+// EditableTriangleElementSet2::EndEdition (TriangleElementSet2.cpp:2187-2239) +
+// LeafDSH_Solid2D_FEM::RebuildMass (LeafDSH_Solid2D_FEM.cpp:1842-1863)
+orc_solid2* orc_solid2_create(uint32_t n_nodes, const float* ref2, uint32_t n_tris, const uint32_t* tris,
+                              const orc_solid_params* p, float thickness) {
+    orc_solid2* s = new orc_solid2;
+    s->p = *p;
+    s->thickness = thickness;
+    s->N = n_nodes; s->T = n_tris;
+    s->ref.assign(ref2, ref2 + 2 * size_t(n_nodes));
+    s->tri.assign(tris, tris + 3 * size_t(n_tris));
+    s->Kb.resize(size_t(n_tris) * 36);
+    s->Dinv.resize(4 * size_t(n_tris));
+    s->area.resize(n_tris);
+    s->R.assign(4 * size_t(n_tris), 0.0f);
+    s->detF.assign(n_tris, 1.0f);
+    float acc = 0.0f, K[36];
+    for (uint32_t e = 0; e < n_tris; e++) {
+        const float* r0 = &s->ref[2 * size_t(tris[3 * e])];
+        const float* r1 = &s->ref[2 * size_t(tris[3 * e + 1])];
+        const float* r2 = &s->ref[2 * size_t(tris[3 * e + 2])];
+        float a = 0.5f * ((r1[0] - r0[0]) * (r2[1] - r0[1]) - (r2[0] - r0[0]) * (r1[1] - r0[1]));  // Compute_Area
+        s->area[e] = a;
+        acc += a;
+        compute_K2(r0, r1, r2, p->young_modulus, p->poisson_ratio, K);
+        float* kb = &s->Kb[size_t(e) * 36];
+        for (int bi = 0; bi < 3; bi++)
+            for (int bj = 0; bj < 3; bj++)
+                for (int rr = 0; rr < 2; rr++)
+                    for (int cc = 0; cc < 2; cc++) kb[(3 * bi + bj) * 4 + 2 * rr + cc] = K[(2 * bi + rr) * 6 + 2 * bj + cc];
+        m2store(inv2(d2(r0, r1, r2)), &s->Dinv[4 * size_t(e)]);
+        s->R[4 * size_t(e)] = 1.0f; s->R[4 * size_t(e) + 3] = 1.0f;  // CreateCache: R = Id
+    }
+    s->total_area = acc;
+    // lumped masses: weighted area Sum(Rcp(3) * area) over the node's triangles
+    const float density_per_area = p->density * thickness;
+    std::vector<float> wa(n_nodes, 0.0f);
+    const float third = 1.0f / 3.0f;
+    for (uint32_t e = 0; e < n_tris; e++)
+        for (int j = 0; j < 3; j++) wa[tris[3 * e + j]] += third * s->area[e];
+    s->inv_mass.resize(n_nodes);
+    for (uint32_t n = 0; n < n_nodes; n++) s->inv_mass[n] = 1.0f / (wa[n] * density_per_area);  // SetMass stores 1/m
+    rayleigh(p->rayleigh_freq1, p->rayleigh_freq2, p->damping_ratio, s->c);
+    const size_t n2 = 2 * size_t(n_nodes);
+    s->x.assign(ref2, ref2 + n2);
+    s->v.assign(n2, 0.0f);
+    s->b.resize(n2); s->y.resize(n2); s->r.resize(n2); s->d.resize(n2); s->q.resize(n2);
+    return s;
+}
+
+void orc_solid2_destroy(orc_solid2* s) { delete s; }
+void orc_solid2_set_knc(orc_solid2* s, uint32_t n, const uint32_t* nids) { s->knc.assign(nids, nids + n); }
+float orc_solid2_total_area(const orc_solid2* s) { return s->total_area; }
+void orc_solid2_node_masses(const orc_solid2* s, float* m) {
+    for (uint32_t n = 0; n < s->N; n++) m[n] = 1.0f / s->inv_mass[n];
+}
+
+// BeginEvaluation (TriangleElementSet2.cpp:183-262) + ComputeRotations_MSVD/DAPD
+// (:1007-1018,1078-1110) with Compute_R_SVD / Compute_R_PD for det F > threshold
+// (Rotation.cpp:19-37, GMatDecomposition.h:17-30)
+void orc_solid2_begin_evaluation(orc_solid2* s, const float* x, float* R_out, float* detF_out) {
+    for (uint32_t e = 0; e < s->T; e++) {
+        const uint32_t* t = &s->tri[3 * size_t(e)];
+        M2 F = m2mul(d2(&x[2 * size_t(t[0])], &x[2 * size_t(t[1])], &x[2 * size_t(t[2])]), m2load(&s->Dinv[4 * size_t(e)]));
+        float dF = det2(F);
+        s->detF[e] = dF;
+        float R[4] = {1.0f, 0.0f, 0.0f, 1.0f};
+        if (s->p.rm != ORC_RM_ID && dF > s->p.degenerate_threshold) {
+            float Fa[4];
+            m2store(F, Fa);
+            orc_pd2(Fa, dF, R);
+        }
+        std::memcpy(&s->R[4 * size_t(e)], R, sizeof(R));
+    }
+    if (R_out) std::memcpy(R_out, s->R.data(), s->R.size() * sizeof(float));
+    if (detF_out) std::memcpy(detF_out, s->detF.data(), s->detF.size() * sizeof(float));
+}
+
+void orc_solid2_f_e(const orc_solid2* s, const float* x, float* f) {
+    for (uint32_t e = 0; e < s->T; e++) {
+        const uint32_t* t = &s->tri[3 * size_t(e)];
+        M2 Rt = m2tr(m2load(&s->R[4 * size_t(e)]));
+        float u[3][2];
+        for (int j = 0; j < 3; j++) {
+            m2vec(Rt, &x[2 * size_t(t[j])], u[j]);
+            u[j][0] -= s->ref[2 * size_t(t[j])]; u[j][1] -= s->ref[2 * size_t(t[j]) + 1];
+        }
+        corot_apply2(s, e, u, f);
+    }
+}
+
+void orc_solid2_df_x(const orc_solid2* s, const float* dx, float* df) {
+    for (uint32_t e = 0; e < s->T; e++) {
+        const uint32_t* t = &s->tri[3 * size_t(e)];
+        M2 Rt = m2tr(m2load(&s->R[4 * size_t(e)]));
+        float l[3][2];
+        for (int j = 0; j < 3; j++) m2vec(Rt, &dx[2 * size_t(t[j])], l[j]);
+        corot_apply2(s, e, l, df);
+    }
+}
+
+// C_V (TriangleElementSet2.cpp:1620-1645)
+float orc_solid2_V(const orc_solid2* s, const float* x) {
+    float acc2 = 0.0f;
+    for (uint32_t e = 0; e < s->T; e++) {
+        const uint32_t* t = &s->tri[3 * size_t(e)];
+        const float* kb = &s->Kb[size_t(e) * 36];
+        M2 Rt = m2tr(m2load(&s->R[4 * size_t(e)]));
+        float u[3][2];
+        for (int j = 0; j < 3; j++) {
+            m2vec(Rt, &x[2 * size_t(t[j])], u[j]);
+            u[j][0] -= s->ref[2 * size_t(t[j])]; u[j][1] -= s->ref[2 * size_t(t[j]) + 1];
+        }
+        for (int i = 0; i < 3; i++) {
+            float acc[2], tmp[2];
+            m2vec(m2load(kb + (3 * i + 0) * 4), u[0], acc);
+            m2vec(m2load(kb + (3 * i + 1) * 4), u[1], tmp); acc[0] += tmp[0]; acc[1] += tmp[1];
+            m2vec(m2load(kb + (3 * i + 2) * 4), u[2], tmp); acc[0] += tmp[0]; acc[1] += tmp[1];
+            float dd = 0.0f; dd += u[i][0] * acc[0]; dd += u[i][1] * acc[1];
+            acc2 += dd;
+        }
+    }
+    return 0.5f * acc2;
+}
+
+// FixedStep with Step_QuasiImplicitEuler_v (LeafDSH_Solid2D_FEM.cpp:1287-1403)
+void orc_solid2_fixed_step(orc_solid2* s, float dt, orc_step_stats* st) {
+    const uint32_t N = s->N;
+    const size_t n2 = 2 * size_t(N);
+    orc_solid2_begin_evaluation(s, s->x.data(), nullptr, nullptr);
+    uint32_t ndeg = 0; float sdeg = 0.0f, min_det = 1e30f;
+    for (uint32_t e = 0; e < s->T; e++) {
+        if (s->detF[e] <= 0) { ndeg++; sdeg += s->detF[e]; }
+        min_det = std::min(min_det, s->detF[e]);
+    }
+    const float inv_dt = 1.0f / dt;
+    for (uint32_t i = 0; i < N; i++) {
+        const float m = 1.0f / s->inv_mass[i];  // GetMass
+        s->b[2 * i] = 0.0f + m * s->p.gravity[0];
+        s->b[2 * i + 1] = 0.0f + m * s->p.gravity[1];
+    }
+    orc_solid2_f_e(s, s->x.data(), s->b.data());
+    for (size_t i = 0; i < n2; i++) s->b[i] *= dt;
+    for (uint32_t i = 0; i < N; i++) {
+        const float m = 1.0f / s->inv_mass[i];
+        s->b[2 * i] += m * s->v[2 * i];
+        s->b[2 * i + 1] += m * s->v[2 * i + 1];
+    }
+    if (s->p.warm_start) s->y = s->v; else std::fill(s->y.begin(), s->y.end(), 0.0f);
+    for (uint32_t nid : s->knc) {  // KNC (:1352-1372): position constraints at the rest pose
+        float vk[2] = {(s->ref[2 * size_t(nid)] - s->x[2 * size_t(nid)]) * inv_dt,
+                       (s->ref[2 * size_t(nid) + 1] - s->x[2 * size_t(nid) + 1]) * inv_dt};
+        float nk = std::sqrt(vk[0] * vk[0] + vk[1] * vk[1]);
+        float kv[2] = {0.0f, 0.0f};
+        if (nk > 0.001f) {
+            float inv = 1.0f / nk, sc = std::min(nk, 50.0f);
+            kv[0] = sc * (vk[0] * inv); kv[1] = sc * (vk[1] * inv);
+        }
+        s->b[2 * size_t(nid)] = kv[0]; s->b[2 * size_t(nid) + 1] = kv[1];
+        s->y[2 * size_t(nid)] = kv[0]; s->y[2 * size_t(nid) + 1] = kv[1];
+    }
+    const float fM = 1.0f + dt * s->c[0];
+    const float fK = dt * dt + dt * s->c[1];
+    auto Av = [&](float* q, const float* d) {  // LS_A_times_d (:1101-1132)
+        std::memset(q, 0, n2 * sizeof(float));
+        orc_solid2_df_x(s, d, q);
+        for (size_t i = 0; i < n2; i++) q[i] *= -fK;
+        for (uint32_t i = 0; i < N; i++) {
+            float factor = (1.0f / s->inv_mass[i]) * fM;
+            q[2 * i] += factor * d[2 * i];
+            q[2 * i + 1] += factor * d[2 * i + 1];
+        }
+    };
+    auto S = [&](float* w) {
+        for (uint32_t nid : s->knc) { w[2 * size_t(nid)] = 0; w[2 * size_t(nid) + 1] = 0; }
+    };
+    int32_t it = 0; float rr = 0, ar = 0;
+    cg_solve(uint32_t(n2), Av, S, s->y.data(), s->b.data(), s->p.cg_rel_epsilon, s->p.cg_abs_epsilon,
+             s->p.cg_max_iter, s->r, s->d, s->q, &it, &rr, &ar, s->p.reduce_fp64 != 0);
+    for (size_t i = 0; i < n2; i++) {
+        s->v[i] = s->y[i];
+        s->x[i] = s->x[i] + dt * s->y[i];
+    }
+    float V = orc_solid2_V(s, s->x.data());
+    if (st) {
+        st->num_fixed_steps = 1;
+        st->cg_iters = it; st->cg_rel = rr; st->cg_abs = ar;
+        st->num_degenerate = ndeg; st->sum_degenerate_detF = sdeg;
+        st->elastic_energy = V; st->min_detF = min_det;
+    }
+}
+
+void orc_solid2_get_state(const orc_solid2* s, float* x, float* v) {
+    if (x) std::memcpy(x, s->x.data(), s->x.size() * sizeof(float));
+    if (v) std::memcpy(v, s->v.data(), s->v.size() * sizeof(float));
+}
+void orc_solid2_set_state(orc_solid2* s, const float* x, const float* v) {
+    if (x) std::memcpy(s->x.data(), x, s->x.size() * sizeof(float));
+    if (v) std::memcpy(s->v.data(), v, s->v.size() * sizeof(float));
+}
+
+}  // extern "C"

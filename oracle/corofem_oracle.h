@@ -110,6 +110,28 @@ void orc_solid_set_state(orc_solid* s, const float* x, const float* v);
 float orc_solid_node_mass(const orc_solid* s);
 void orc_solid_rayleigh(const orc_solid* s, float c[2]);
 
+/* ---- 2D leaf solid (QIE-v, TriangleElementSet2 + LeafDSH_Solid2D_FEM) ----
+   Same parameters (gravity[0..1] used); lumped masses Sum(area/3) * density *
+   thickness per node, summed over the node's triangles in ascending triangle
+   order. Rotations: closed-form 2x2 polar factor for det F > threshold (both
+   MSVD and DAPD); below it R = Id (the 2D SVD and DAPD degenerate branches
+   are not restated) and the element is counted in num_degenerate when det F <= 0. */
+typedef struct orc_solid2 orc_solid2;
+orc_solid2* orc_solid2_create(uint32_t n_nodes, const float* ref_pos2, uint32_t n_tris, const uint32_t* tris,
+                              const orc_solid_params* p, float thickness);
+void orc_solid2_destroy(orc_solid2* s);
+void orc_solid2_set_knc(orc_solid2* s, uint32_t n, const uint32_t* nids);
+void orc_solid2_fixed_step(orc_solid2* s, float dt, orc_step_stats* st);
+void orc_solid2_get_state(const orc_solid2* s, float* x, float* v);
+void orc_solid2_set_state(orc_solid2* s, const float* x, const float* v);
+void orc_solid2_node_masses(const orc_solid2* s, float* m);
+float orc_solid2_total_area(const orc_solid2* s);
+/* element operators on the current rotations (set by the last step or begin_evaluation) */
+void orc_solid2_begin_evaluation(orc_solid2* s, const float* x, float* R_out, float* detF_out);
+void orc_solid2_f_e(const orc_solid2* s, const float* x, float* f);
+void orc_solid2_df_x(const orc_solid2* s, const float* dx, float* df);
+float orc_solid2_V(const orc_solid2* s, const float* x);
+
 #ifdef __cplusplus
 }
 #endif

@@ -83,6 +83,18 @@ def _load():
         "orc_solid_set_state": (None, [C.c_void_p, _f, _f]),
         "orc_solid_node_mass": (C.c_float, [C.c_void_p]),
         "orc_solid_rayleigh": (None, [C.c_void_p, _f]),
+        "orc_solid2_create": (C.c_void_p, [C.c_uint32, _f, C.c_uint32, _u, C.POINTER(SolidParams), C.c_float]),
+        "orc_solid2_destroy": (None, [C.c_void_p]),
+        "orc_solid2_set_knc": (None, [C.c_void_p, C.c_uint32, _u]),
+        "orc_solid2_fixed_step": (None, [C.c_void_p, C.c_float, C.POINTER(StepStats)]),
+        "orc_solid2_get_state": (None, [C.c_void_p, _f, _f]),
+        "orc_solid2_set_state": (None, [C.c_void_p, _f, _f]),
+        "orc_solid2_node_masses": (None, [C.c_void_p, _f]),
+        "orc_solid2_total_area": (C.c_float, [C.c_void_p]),
+        "orc_solid2_begin_evaluation": (None, [C.c_void_p, _f, _f, _f]),
+        "orc_solid2_f_e": (None, [C.c_void_p, _f, _f]),
+        "orc_solid2_df_x": (None, [C.c_void_p, _f, _f]),
+        "orc_solid2_V": (C.c_float, [C.c_void_p, _f]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -284,3 +296,72 @@ class Solid:
 
     def node_mass(self):
         return lib().orc_solid_node_mass(self.h)
+
+
+class Solid2:
+    """Oracle 2D leaf solid (TriangleElementSet2 + LeafDSH_Solid2D_FEM, QIE-v,
+    lumped masses)."""
+
+    def __init__(self, ref_pos, tris, params, knc=None, thickness=1.0):
+        self.ref_pos = np.ascontiguousarray(ref_pos, np.float32)
+        self.tris = np.ascontiguousarray(tris, np.uint32)
+        self.N = self.ref_pos.size // 2
+        self.T = self.tris.size // 3
+        self.params = params
+        self.h = lib().orc_solid2_create(self.N, fp(self.ref_pos), self.T, up(self.tris), C.byref(params),
+                                         float(thickness))
+        if knc is not None:
+            k = np.ascontiguousarray(knc, np.uint32)
+            lib().orc_solid2_set_knc(self.h, k.size, up(k))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_solid2_destroy(self.h)
+            self.h = None
+
+    def fixed_step(self, dt=None):
+        st = StepStats()
+        lib().orc_solid2_fixed_step(self.h, self.params.fixed_dt if dt is None else dt, C.byref(st))
+        return st
+
+    def state(self):
+        x = np.zeros(2 * self.N, np.float32)
+        v = np.zeros(2 * self.N, np.float32)
+        lib().orc_solid2_get_state(self.h, fp(x), fp(v))
+        return x, v
+
+    def set_state(self, x, v):
+        x = np.ascontiguousarray(x, np.float32)
+        v = np.ascontiguousarray(v, np.float32)
+        lib().orc_solid2_set_state(self.h, fp(x), fp(v))
+
+    def node_masses(self):
+        m = np.zeros(self.N, np.float32)
+        lib().orc_solid2_node_masses(self.h, fp(m))
+        return m
+
+    def total_area(self):
+        return lib().orc_solid2_total_area(self.h)
+
+    def begin_evaluation(self, x):
+        x = np.ascontiguousarray(x, np.float32)
+        R = np.zeros(4 * self.T, np.float32)
+        dF = np.zeros(self.T, np.float32)
+        lib().orc_solid2_begin_evaluation(self.h, fp(x), fp(R), fp(dF))
+        return R, dF
+
+    def f_e(self, x):
+        x = np.ascontiguousarray(x, np.float32)
+        f = np.zeros(2 * self.N, np.float32)
+        lib().orc_solid2_f_e(self.h, fp(x), fp(f))
+        return f
+
+    def df_x(self, dx):
+        dx = np.ascontiguousarray(dx, np.float32)
+        df = np.zeros(2 * self.N, np.float32)
+        lib().orc_solid2_df_x(self.h, fp(dx), fp(df))
+        return df
+
+    def V(self, x):
+        x = np.ascontiguousarray(x, np.float32)
+        return lib().orc_solid2_V(self.h, fp(x))

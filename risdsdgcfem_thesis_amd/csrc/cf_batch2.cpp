@@ -1,0 +1,441 @@
+// 2D solids (cf_batch2_*, cf_solid2_* in include/corofem.h): host-side mesh
+// preparation for the workgroup-resident step kernel (cf_kernels2d.hip).
+//
+// Per mesh, once: element precompute in the reference's FP32 order (area,
+// Dm^-1: ms/fem/TriangleElement2.h:71-96, Mal/GMat.h:260-288), lumped masses
+// (LeafDSH_Solid2D_FEM::RebuildMass, ds/dsh/LeafDSH_Solid2D_FEM.cpp:1842-1863,
+// summed over the node's triangles in ascending triangle order), a greedy
+// triangle colouring, and the triangles sorted by colour.
+#include "cf_internal.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <numeric>
+#include <vector>
+
+using namespace cf;
+
+struct cf_batch2 {
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::vector<Mesh2Dev> meshes;
+    std::vector<float2> ref, x, v;
+    std::vector<float> mass, area;
+    std::vector<uint8_t> knc;
+    std::vector<uint32_t> tri_ids;
+    std::vector<float4> dinv;
+    std::vector<uint16_t> cstart;
+    std::vector<Work2> work;
+    uint64_t tris = 0;
+    bool dirty = true;        // static data or state must be uploaded
+    bool on_device = false;   // the device holds the newest state
+    // device copies
+    Mesh2Dev* d_meshes = nullptr;
+    float2 *d_ref = nullptr, *d_x = nullptr, *d_v = nullptr;
+    float *d_mass = nullptr, *d_area = nullptr;
+    uint8_t* d_knc = nullptr;
+    uint32_t* d_tri = nullptr;
+    float4* d_dinv = nullptr;
+    uint16_t* d_cs = nullptr;
+    Work2* d_work = nullptr;
+    Sim2Out* d_out = nullptr;
+    std::vector<Sim2Out> h_out;
+};
+
+namespace {
+
+template <class T>
+void dfree(T*& p) {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+void free_device(cf_batch2* b) {
+    dfree(b->d_meshes); dfree(b->d_ref); dfree(b->d_x); dfree(b->d_v); dfree(b->d_mass); dfree(b->d_area);
+    dfree(b->d_knc); dfree(b->d_tri); dfree(b->d_dinv); dfree(b->d_cs); dfree(b->d_work); dfree(b->d_out);
+}
+
+template <class T>
+int put(T** d, const std::vector<T>& h, hipStream_t s) {
+    CF_CHECK_HIP(hipMalloc(reinterpret_cast<void**>(d), std::max<size_t>(1, h.size()) * sizeof(T)));
+    if (!h.empty()) CF_CHECK_HIP(hipMemcpyAsync(*d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, s));
+    return CF_OK;
+}
+
+// Pulls the device state into the host mirrors (before a re-upload).
+int pull_state(cf_batch2* b) {
+    if (!b->on_device) return CF_OK;
+    CF_CHECK_HIP(hipMemcpyAsync(b->x.data(), b->d_x, b->x.size() * sizeof(float2), hipMemcpyDeviceToHost, b->stream));
+    CF_CHECK_HIP(hipMemcpyAsync(b->v.data(), b->d_v, b->v.size() * sizeof(float2), hipMemcpyDeviceToHost, b->stream));
+    CF_CHECK_HIP(hipStreamSynchronize(b->stream));
+    b->on_device = false;
+    return CF_OK;
+}
+
+// Work list: meshes of one group size share workgroups (8/gw meshes each),
+// largest first so that the tail is short.
+void build_work(cf_batch2* b) {
+    std::vector<uint32_t> cls[4];
+    auto slot = [](uint32_t gw) { return gw == 1 ? 0 : gw == 2 ? 1 : gw == 4 ? 2 : 3; };
+    for (uint32_t i = 0; i < b->meshes.size(); i++) cls[slot(b->meshes[i].gw)].push_back(i);
+    auto by_size = [&](uint32_t p, uint32_t q) {
+        return b->meshes[p].T != b->meshes[q].T ? b->meshes[p].T > b->meshes[q].T : p < q;
+    };
+    b->work.clear();
+    for (int c = 3; c >= 0; c--) {
+        std::stable_sort(cls[c].begin(), cls[c].end(), by_size);
+        const uint32_t gw = 1u << c, per = 8 / gw;
+        for (size_t i = 0; i < cls[c].size(); i += per) {
+            Work2 w{};
+            w.gw = gw;
+            w.cmax = 0;
+            for (uint32_t j = 0; j < 8; j++) w.sim[j] = kIdleSlot;
+            for (uint32_t j = 0; j < per && i + j < cls[c].size(); j++) {
+                w.sim[j] = cls[c][i + j];
+                w.cmax = std::max(w.cmax, b->meshes[cls[c][i + j]].C);
+            }
+            b->work.push_back(w);
+        }
+    }
+}
+
+int upload(cf_batch2* b) {
+    CF_TRY(pull_state(b));
+    free_device(b);
+    build_work(b);
+    hipStream_t s = b->stream;
+    CF_TRY(put(&b->d_meshes, b->meshes, s));
+    CF_TRY(put(&b->d_ref, b->ref, s));
+    CF_TRY(put(&b->d_x, b->x, s));
+    CF_TRY(put(&b->d_v, b->v, s));
+    CF_TRY(put(&b->d_mass, b->mass, s));
+    CF_TRY(put(&b->d_area, b->area, s));
+    CF_TRY(put(&b->d_knc, b->knc, s));
+    CF_TRY(put(&b->d_tri, b->tri_ids, s));
+    CF_TRY(put(&b->d_dinv, b->dinv, s));
+    CF_TRY(put(&b->d_cs, b->cstart, s));
+    CF_TRY(put(&b->d_work, b->work, s));
+    b->h_out.assign(b->meshes.size(), Sim2Out{});
+    CF_TRY(put(&b->d_out, b->h_out, s));
+    CF_CHECK_HIP(hipStreamSynchronize(s));
+    b->dirty = false;
+    b->on_device = true;
+    return CF_OK;
+}
+
+Batch2Args args(const cf_batch2* b) {
+    Batch2Args a;
+    a.work = b->d_work;
+    a.meshes = b->d_meshes;
+    a.ref = b->d_ref;
+    a.mass = b->d_mass;
+    a.knc = b->d_knc;
+    a.x = b->d_x;
+    a.v = b->d_v;
+    a.tri_ids = b->d_tri;
+    a.dinv = b->d_dinv;
+    a.area = b->d_area;
+    a.color_start = b->d_cs;
+    a.out = b->d_out;
+    return a;
+}
+
+Material material2(float E, float nu) {
+    // TriangleElement2::Compute_E (ms/fem/TriangleElement2.h:121-129), plane strain
+    const float a = E / ((1 + nu) * (1 - 2 * nu));
+    const float anu = a * nu;
+    Material m;
+    m.a_k1 = a - anu;
+    m.a_nu = anu;
+    m.a_k2 = 0.5f * a - anu;
+    return m;
+}
+
+}  // namespace
+
+extern "C" {
+
+int cf_batch2_create(cf_batch2** out) {
+    if (!out) return CF_ERR_INVALID_ARG;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n < 1) {
+        (void)hipGetLastError();
+        set_error("no HIP device");
+        return CF_ERR_NO_DEVICE;
+    }
+    cf_batch2* b = new (std::nothrow) cf_batch2;
+    if (!b) return CF_ERR_OUT_OF_MEMORY;
+    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&b->ev0) != hipSuccess || hipEventCreate(&b->ev1) != hipSuccess) {
+        set_error("2D batch: stream/event creation failed");
+        cf_batch2_destroy(b);
+        return CF_ERR_HIP;
+    }
+    *out = b;
+    return CF_OK;
+}
+
+int cf_batch2_destroy(cf_batch2* b) {
+    if (!b) return CF_OK;
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    free_device(b);
+    if (b->ev0) (void)hipEventDestroy(b->ev0);
+    if (b->ev1) (void)hipEventDestroy(b->ev1);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+    return CF_OK;
+}
+
+int cf_batch2_add_sim(cf_batch2* b, uint32_t N, const float* ref, uint32_t T, const uint32_t* tris,
+                      const cf_solid_params* p, uint32_t n_knc, const uint32_t* knc, uint32_t* sim_id) {
+    if (!b || !ref || !tris || !p || !sim_id || N == 0 || T == 0 || (n_knc && !knc)) {
+        set_error("cf_batch2_add_sim: invalid arguments");
+        return CF_ERR_INVALID_ARG;
+    }
+    if (!(p->fixed_dt > 0) || !(p->density > 0) || !(p->time_scale > 0) || !(p->thickness > 0) ||
+        !(p->fem.young_modulus > 0) || !(p->fem.poisson_ratio > -1.0f) || !(p->fem.poisson_ratio < 0.5f) ||
+        p->fem.mm != CF_MM_C_WRP || p->fem.dm != CF_DM_TRUNCATED ||
+        (p->fem.rm != CF_RM_ID && p->fem.rm != CF_RM_MSVD && p->fem.rm != CF_RM_DAPD)) {
+        set_error("cf_batch2_add_sim: unsupported parameters");
+        return CF_ERR_INVALID_ARG;
+    }
+    // smallest group of wavefronts that holds the mesh in registers
+    uint32_t gw = 1;
+    while (gw <= 8 && (T > gw * uint32_t(kGroupTris) || N > gw * uint32_t(kGroupNodes))) gw *= 2;
+    if (gw > 8) {
+        set_error("2D mesh of %u triangles / %u nodes exceeds one workgroup (%d / %d)", T, N, 8 * kGroupTris,
+                  8 * kGroupNodes);
+        return CF_ERR_UNSUPPORTED;
+    }
+    for (size_t i = 0; i < 3 * (size_t)T; i++)
+        if (tris[i] >= N) { set_error("triangle %zu references node %u >= %u", i / 3, tris[i], N); return CF_ERR_INVALID_ARG; }
+    for (uint32_t i = 0; i < n_knc; i++)
+        if (knc[i] >= N) { set_error("KNC node %u out of range", knc[i]); return CF_ERR_INVALID_ARG; }
+    // element precompute (EndEdition, TriangleElementSet2.cpp:2187-2239)
+    std::vector<float> area(T);
+    std::vector<float4> dinv(T);
+    for (uint32_t e = 0; e < T; e++) {
+        const float* r0 = &ref[2 * (size_t)tris[3 * e]];
+        const float* r1 = &ref[2 * (size_t)tris[3 * e + 1]];
+        const float* r2 = &ref[2 * (size_t)tris[3 * e + 2]];
+        area[e] = 0.5f * ((r1[0] - r0[0]) * (r2[1] - r0[1]) - (r2[0] - r0[0]) * (r1[1] - r0[1]));
+        const float D00 = r1[0] - r0[0], D01 = r2[0] - r0[0], D10 = r1[1] - r0[1], D11 = r2[1] - r0[1];
+        const float det = D00 * D11 - D10 * D01;
+        if (!(det > 0)) { set_error("triangle %u is degenerate or inverted at rest", e); return CF_ERR_INVALID_ARG; }
+        const float id = 1.0f / det;
+        dinv[e] = make_float4(id * D11, -id * D01, -id * D10, id * D00);
+    }
+    // lumped masses: Sum(Rcp(3) area) * density * thickness, stored as 1/m (GetMass = 1/(1/m))
+    const float density_per_area = p->density * p->thickness;
+    const float third = 1.0f / 3.0f;
+    std::vector<float> wa(N, 0.0f);
+    for (uint32_t e = 0; e < T; e++)
+        for (int j = 0; j < 3; j++) wa[tris[3 * e + j]] += third * area[e];
+    // greedy colouring: no two triangles of one colour share a node
+    std::vector<uint32_t> used(N, 0), color(T);
+    uint32_t C = 0;
+    for (uint32_t e = 0; e < T; e++) {
+        uint32_t mask = used[tris[3 * e]] | used[tris[3 * e + 1]] | used[tris[3 * e + 2]];
+        uint32_t c = 0;
+        while (c < 32 && (mask >> c & 1u)) c++;
+        if (c >= uint32_t(kMaxColors)) { set_error("2D mesh needs more than %d colours", kMaxColors); return CF_ERR_UNSUPPORTED; }
+        color[e] = c;
+        for (int j = 0; j < 3; j++) used[tris[3 * e + j]] |= 1u << c;
+        C = std::max(C, c + 1);
+    }
+    std::vector<uint32_t> order(T);
+    std::iota(order.begin(), order.end(), 0u);
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t q) { return color[a] < color[q]; });
+    CF_TRY(pull_state(b));
+    Mesh2Dev m{};
+    m.N = N; m.T = T; m.C = C; m.gw = gw;
+    m.node_off = uint32_t(b->ref.size());
+    m.tri_off = uint32_t(b->area.size());
+    m.color_off = uint32_t(b->cstart.size());
+    m.m = material2(p->fem.young_modulus, p->fem.poisson_ratio);
+    float c[2];
+    rayleigh_coeffs(p->fem.rayleigh_freq1, p->fem.rayleigh_freq2, p->fem.damping_ratio, c);
+    const float dt = p->time_scale * p->fixed_dt;
+    m.dt = dt;
+    m.inv_dt = 1.0f / dt;
+    m.fM = 1.0f + dt * c[0];
+    m.fK = dt * dt + dt * c[1];
+    m.g0 = p->gravity[0];
+    m.g1 = p->gravity[1];
+    m.rel = p->cg_rel_epsilon;
+    m.abs_eps = p->cg_abs_epsilon;
+    m.thr = p->fem.degenerate_threshold_det_f;
+    m.drag = -std::log(10e-6f) * p->air_drag_coeff;
+    m.rm = p->fem.rm;
+    m.warm_start = p->warm_start;
+    m.compute_energy = p->compute_energy;
+    m.max_iter = std::min<uint32_t>(p->cg_max_iter, kMaxCgIter);
+    for (uint32_t n = 0; n < N; n++) {
+        b->ref.push_back(make_float2(ref[2 * (size_t)n], ref[2 * (size_t)n + 1]));
+        b->x.push_back(b->ref.back());
+        b->v.push_back(make_float2(0.f, 0.f));
+        const float inv_mass = 1.0f / (wa[n] * density_per_area);
+        b->mass.push_back(1.0f / inv_mass);
+        b->knc.push_back(0);
+    }
+    for (uint32_t i = 0; i < n_knc; i++) b->knc[m.node_off + knc[i]] = 1;
+    std::vector<uint32_t> cstart(C + 1, 0);
+    for (uint32_t e = 0; e < T; e++) cstart[color[e] + 1]++;
+    for (uint32_t k = 0; k < C; k++) cstart[k + 1] += cstart[k];
+    for (uint32_t k = 0; k <= C; k++) b->cstart.push_back(uint16_t(cstart[k]));
+    for (uint32_t i = 0; i < T; i++) {
+        const uint32_t e = order[i];
+        for (int j = 0; j < 3; j++) b->tri_ids.push_back(tris[3 * e + j]);
+        b->dinv.push_back(dinv[e]);
+        b->area.push_back(area[e]);
+    }
+    b->meshes.push_back(m);
+    b->tris += T;
+    b->dirty = true;
+    *sim_id = uint32_t(b->meshes.size() - 1);
+    return CF_OK;
+}
+
+int cf_batch2_fixed_step(cf_batch2* b, cf_step_stats* stats) {
+    if (!b) return CF_ERR_INVALID_ARG;
+    if (b->meshes.empty()) return CF_OK;
+    if (b->dirty) CF_TRY(upload(b));
+    CF_CHECK_HIP(hipEventRecord(b->ev0, b->stream));
+    CF_TRY(launch_batch2_step(args(b), uint32_t(b->work.size()), b->stream));
+    CF_CHECK_HIP(hipEventRecord(b->ev1, b->stream));
+    CF_CHECK_HIP(hipEventSynchronize(b->ev1));
+    if (stats) {
+        float ms = 0.f;
+        CF_CHECK_HIP(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+        CF_CHECK_HIP(hipMemcpy(b->h_out.data(), b->d_out, b->h_out.size() * sizeof(Sim2Out), hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < b->h_out.size(); i++) {
+            const Sim2Out& o = b->h_out[i];
+            cf_step_stats& s = stats[i];
+            s.num_fixed_steps = 1;
+            s.cg_iters = o.iters;
+            s.cg_reason = o.reason;
+            s.cg_rel_residual = o.rel;
+            s.cg_abs_residual = o.abs_res;
+            s.num_degenerate = o.num_degenerate;
+            s.sum_degenerate_det_f = float(o.sum_degenerate);
+            s.elastic_energy = float(o.energy);
+            s.min_det_f = o.min_det;
+            s.step_ms = ms;
+        }
+    }
+    return CF_OK;
+}
+
+int cf_batch2_get_state(cf_batch2* b, uint32_t id, float* pos, float* vel) {
+    if (!b || id >= b->meshes.size()) { set_error("no 2D simulation %u", id); return CF_ERR_INVALID_ARG; }
+    const Mesh2Dev& m = b->meshes[id];
+    if (b->on_device) {
+        CF_CHECK_HIP(hipStreamSynchronize(b->stream));
+        if (pos) CF_CHECK_HIP(hipMemcpy(pos, b->d_x + m.node_off, m.N * sizeof(float2), hipMemcpyDeviceToHost));
+        if (vel) CF_CHECK_HIP(hipMemcpy(vel, b->d_v + m.node_off, m.N * sizeof(float2), hipMemcpyDeviceToHost));
+    } else {
+        if (pos) std::memcpy(pos, &b->x[m.node_off], m.N * sizeof(float2));
+        if (vel) std::memcpy(vel, &b->v[m.node_off], m.N * sizeof(float2));
+    }
+    return CF_OK;
+}
+
+int cf_batch2_set_state(cf_batch2* b, uint32_t id, const float* pos, const float* vel) {
+    if (!b || id >= b->meshes.size()) { set_error("no 2D simulation %u", id); return CF_ERR_INVALID_ARG; }
+    const Mesh2Dev& m = b->meshes[id];
+    if (b->on_device) {
+        CF_CHECK_HIP(hipStreamSynchronize(b->stream));
+        if (pos) CF_CHECK_HIP(hipMemcpy(b->d_x + m.node_off, pos, m.N * sizeof(float2), hipMemcpyHostToDevice));
+        if (vel) CF_CHECK_HIP(hipMemcpy(b->d_v + m.node_off, vel, m.N * sizeof(float2), hipMemcpyHostToDevice));
+    } else {
+        if (pos) std::memcpy(&b->x[m.node_off], pos, m.N * sizeof(float2));
+        if (vel) std::memcpy(&b->v[m.node_off], vel, m.N * sizeof(float2));
+    }
+    return CF_OK;
+}
+
+int cf_batch2_node_masses(const cf_batch2* b, uint32_t id, float* mass) {
+    if (!b || !mass || id >= b->meshes.size()) { set_error("no 2D simulation %u", id); return CF_ERR_INVALID_ARG; }
+    const Mesh2Dev& m = b->meshes[id];
+    std::memcpy(mass, &b->mass[m.node_off], m.N * sizeof(float));
+    return CF_OK;
+}
+
+int cf_batch2_info(const cf_batch2* b, uint32_t* num_sims, uint64_t* tris, uint64_t* nodes, uint32_t* workgroups) {
+    if (!b) return CF_ERR_INVALID_ARG;
+    if (num_sims) *num_sims = uint32_t(b->meshes.size());
+    if (tris) *tris = b->tris;
+    if (nodes) *nodes = b->ref.size();
+    if (workgroups) {
+        uint32_t n[9] = {0};
+        for (const auto& m : b->meshes) n[m.gw]++;
+        *workgroups = (n[1] + 7) / 8 + (n[2] + 3) / 4 + (n[4] + 1) / 2 + n[8];
+    }
+    return CF_OK;
+}
+
+// ---------------------------------------------------------------- single 2D solid
+struct cf_solid2 {
+    cf_batch2* b = nullptr;
+    float acc_time = 0.f;
+    cf_solid_params p{};
+};
+
+int cf_solid2_create(uint32_t N, const float* ref, uint32_t T, const uint32_t* tris, const cf_solid_params* p,
+                     uint32_t n_knc, const uint32_t* knc, cf_solid2** out) {
+    if (!out || !p) return CF_ERR_INVALID_ARG;
+    *out = nullptr;
+    cf_solid2* s = new (std::nothrow) cf_solid2;
+    if (!s) return CF_ERR_OUT_OF_MEMORY;
+    s->p = *p;
+    int rc = cf_batch2_create(&s->b);
+    uint32_t id = 0;
+    if (rc == CF_OK) rc = cf_batch2_add_sim(s->b, N, ref, T, tris, p, n_knc, knc, &id);
+    if (rc != CF_OK) { cf_solid2_destroy(s); return rc; }
+    *out = s;
+    return CF_OK;
+}
+
+int cf_solid2_destroy(cf_solid2* s) {
+    if (!s) return CF_OK;
+    cf_batch2_destroy(s->b);
+    delete s;
+    return CF_OK;
+}
+
+int cf_solid2_fixed_step(cf_solid2* s, cf_step_stats* stats) {
+    if (!s) return CF_ERR_INVALID_ARG;
+    return cf_batch2_fixed_step(s->b, stats);
+}
+
+// Step (LeafDSH_Solid2D_FEM: fixed-dt accumulator with strict '>')
+int cf_solid2_step(cf_solid2* s, float dt, cf_step_stats* stats) {
+    if (!s) return CF_ERR_INVALID_ARG;
+    s->acc_time += s->p.time_scale * dt;
+    const float fixed = s->p.time_scale * s->p.fixed_dt;
+    cf_step_stats last{};
+    uint32_t n = 0;
+    while (s->acc_time > fixed) {
+        CF_TRY(cf_batch2_fixed_step(s->b, &last));
+        s->acc_time -= fixed;
+        n++;
+    }
+    last.num_fixed_steps = n;
+    if (stats) *stats = last;
+    return CF_OK;
+}
+
+int cf_solid2_get_state(const cf_solid2* s, float* pos, float* vel) {
+    if (!s) return CF_ERR_INVALID_ARG;
+    return cf_batch2_get_state(s->b, 0, pos, vel);
+}
+
+int cf_solid2_set_state(cf_solid2* s, const float* pos, const float* vel) {
+    if (!s) return CF_ERR_INVALID_ARG;
+    return cf_batch2_set_state(s->b, 0, pos, vel);
+}
+
+}  // extern "C"

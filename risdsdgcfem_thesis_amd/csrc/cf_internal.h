@@ -239,4 +239,58 @@ int launch_batch_integrate(const BatchTables& t, hipStream_t s);
 int launch_batch_energy(const BatchTables& t, hipStream_t s);
 int launch_batch_finish(const BatchTables& t, hipStream_t s);
 int launch_batch_fill_u32(uint32_t* p, uint32_t value, uint32_t n, hipStream_t s);
+
+// ---------------------------------------------------------------- 2D resident step
+// A 2D mesh small enough to live in one workgroup is stepped entirely inside
+// it (BeginEvaluation, f_e, RHS, the whole CG loop, integration, energy).
+// Workgroups have 512 threads (8 wavefronts) split into groups of 1, 2, 4 or
+// 8 wavefronts, one mesh per group; each thread keeps up to kTriRegs
+// triangles and kNodeRegs nodes in registers, node vectors shared between
+// threads live in LDS. Groups of several wavefronts share the workgroup's
+// barriers: they step in lockstep (a finished group waits through the
+// barriers) but never share data, so a mesh's arithmetic does not depend on
+// its neighbours. Triangles are sorted by a greedy colouring (no two
+// triangles of one colour share a node), so nodal forces are accumulated in
+// LDS colour by colour without atomics, in a fixed order.
+constexpr int kTriRegs = 4, kNodeRegs = 3, kMaxColors = 16, k2Threads = 512;
+constexpr int kGroupTris = 64 * kTriRegs, kGroupNodes = 64 * kNodeRegs;  // per wavefront of a group
+
+struct Mesh2Dev {
+    uint32_t N, T, C, gw;          // gw: wavefronts of the mesh's group (1, 2, 4 or 8)
+    uint32_t node_off, tri_off, color_off, pad;
+    Material m;                    // plane strain: a_k1 = a - a nu, a_nu = a nu, a_k2 = a/2 - a nu
+    float fM, fK, dt, inv_dt, g0, g1, rel, abs_eps, thr, drag;
+    int32_t rm, warm_start, compute_energy;
+    uint32_t max_iter;
+};
+
+struct Sim2Out {
+    int32_t iters, reason;
+    float rel, abs_res;
+    double energy, sum_degenerate;
+    uint32_t num_degenerate, num_fallback;
+    float min_det, pad;
+};
+
+struct Work2 {  // one workgroup: 8/gw meshes of group size gw (kIdleSlot: empty group)
+    uint32_t gw, cmax;  // cmax: most colours among the workgroup's meshes
+    uint32_t sim[8];
+};
+
+struct Batch2Args {
+    const Work2* work;
+    const Mesh2Dev* meshes;
+    const float2* ref;       // per node (node_off)
+    const float* mass;
+    const uint8_t* knc;
+    float2* x;
+    float2* v;
+    const uint32_t* tri_ids; // per triangle: 3 local node ids, colour-sorted (tri_off)
+    const float4* dinv;      // Dm^-1 row-major
+    const float* area;
+    const uint16_t* color_start;  // per mesh C+1 entries (color_off)
+    Sim2Out* out;
+};
+
+int launch_batch2_step(const Batch2Args& a, uint32_t n_work, hipStream_t s);
 }  // namespace cf

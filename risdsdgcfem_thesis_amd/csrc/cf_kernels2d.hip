@@ -1,0 +1,466 @@
+// 2D corotational triangle FEM: the whole QIE-v fixed step of a mesh inside
+// one wavefront or one workgroup (TriangleElementSet2 + LeafDSH_Solid2D_FEM).
+//
+// Reference path: BeginEvaluation (ms/fem/TriangleElementSet2.cpp:183-262),
+// rotations (:1007-1018,1078-1110 -> Mal/GMatDecomposition.h:17-30 for
+// det F > threshold), C_f_e (:1253-1286), C_df_x (:1859-1915), C_V
+// (:1620-1645), LS_A_times_d (ds/dsh/LeafDSH_Solid2D_FEM.cpp:1101-1132),
+// Step_QuasiImplicitEuler_v (:1287-1403) with the CG of ns/ILSS_CG.h:38-151.
+//
+// Ke = A B^T E B (ms/fem/TriangleElement2.h:98-139) is applied in the
+// equivalent Piola form from Dm^-1 and the area:
+//   H = R^T [v1-v0, v2-v0] Dm^-1         (df_x: v = d)
+//   H = (R^T [x1-x0, x2-x0] - Dm) Dm^-1  (f_e)
+//   sigma = E(sym H), P = -A R sigma Dm^-T ; vertex c+1 gets column c,
+//   vertex 0 minus their sum.
+// No memory traffic inside the CG loop: triangle data in registers, the CG
+// direction and the nodal forces in LDS, per-node CG vectors in registers.
+#include "cf_device.h"
+
+namespace cf {
+
+namespace {
+
+// A group of GW wavefronts stepping one mesh inside a 512-thread workgroup.
+// GW == 1: wave-level barriers and reductions, groups fully independent.
+// GW > 1: workgroup barriers shared by all groups (same barrier sequence in
+// every group), reductions over the group's own wavefronts in fixed order.
+template <int GW>
+struct Grp {
+    static constexpr int G = 64 * GW;
+    __device__ static uint32_t lane() { return threadIdx.x % G; }
+    __device__ static uint32_t group() { return threadIdx.x / G; }
+    __device__ static void bar() {
+        if (GW == 1) {
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        } else {
+            __syncthreads();
+        }
+    }
+    __device__ static double sum(double v, double* red) {
+        v = wave_sum(v);
+        if (GW == 1) return v;
+        __syncthreads();
+        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+        __syncthreads();
+        const uint32_t w0 = group() * GW;
+        double r = 0.0;
+        for (int i = 0; i < GW; i++) r += red[w0 + i];
+        return r;
+    }
+    __device__ static float min(float v, float* red) {
+        for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o, 64));
+        if (GW == 1) return v;
+        __syncthreads();
+        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+        __syncthreads();
+        const uint32_t w0 = group() * GW;
+        float r = red[w0];
+        for (int i = 1; i < GW; i++) r = fminf(r, red[w0 + i]);
+        return r;
+    }
+    // whether any group of the workgroup is still active (its own flag for GW == 1)
+    __device__ static bool any(bool active, uint32_t* flags) {
+        if (GW == 1) return active;
+        __syncthreads();
+        if (lane() == 0) flags[group()] = active ? 1u : 0u;
+        __syncthreads();
+        uint32_t f = 0;
+        for (int i = 0; i < 8 / GW; i++) f |= flags[i];
+        return f != 0;
+    }
+};
+
+struct Tri {
+    uint32_t i0, i1, i2;
+    float g00, g01, g10, g11;  // Dm^-1
+    float A;
+    float r00, r01, r10, r11;  // R
+};
+
+// Closed-form 2x2 polar factor (Mal/GMatDecomposition.h:17-30)
+__device__ inline void polar2(float f00, float f01, float f10, float f11, float det, Tri& t) {
+#pragma clang fp contract(off)
+    const float sg = (det > 0.f) ? 1.0f : ((det < 0.f) ? -1.0f : 0.0f);
+    float r00 = f00 + sg * f11, r01 = f01 + sg * (-f10);
+    float r10 = f10 + sg * (-f01), r11 = f11 + sg * f00;
+    const float i0 = 1.0f / sqrtf(r00 * r00 + r10 * r10);
+    r00 *= i0; r10 *= i0;
+    const float i1 = 1.0f / sqrtf(r01 * r01 + r11 * r11);
+    r01 *= i1; r11 *= i1;
+    t.r00 = r00; t.r01 = r01; t.r10 = r10; t.r11 = r11;
+}
+
+// Displacement gradient H of a triangle: R^T D Dm^-1, or for f_e
+// (R^T D - Dm) Dm^-1 so that a rest state gives exactly 0.
+template <bool FE>
+__device__ inline void tri_gradient(const Tri& t, float2 p0, float2 p1, float2 p2, float2 q0, float2 q1,
+                                    float2 q2, float H[4]) {
+#pragma clang fp contract(off)
+    const float D00 = p1.x - p0.x, D01 = p2.x - p0.x, D10 = p1.y - p0.y, D11 = p2.y - p0.y;
+    float RD00 = fmaf(t.r10, D10, t.r00 * D00), RD01 = fmaf(t.r10, D11, t.r00 * D01);
+    float RD10 = fmaf(t.r11, D10, t.r01 * D00), RD11 = fmaf(t.r11, D11, t.r01 * D01);
+    if (FE) {
+        RD00 -= q1.x - q0.x; RD01 -= q2.x - q0.x;
+        RD10 -= q1.y - q0.y; RD11 -= q2.y - q0.y;
+    }
+    H[0] = fmaf(RD01, t.g10, RD00 * t.g00);
+    H[1] = fmaf(RD01, t.g11, RD00 * t.g01);
+    H[2] = fmaf(RD11, t.g10, RD10 * t.g00);
+    H[3] = fmaf(RD11, t.g11, RD10 * t.g01);
+}
+
+// Nodal forces of a triangle from its displacement gradient.
+__device__ inline void tri_forces(const Tri& t, const float H[4], const Material& m, float2& f0, float2& f1,
+                                  float2& f2) {
+#pragma clang fp contract(off)
+    const float sxx = fmaf(m.a_k1, H[0], m.a_nu * H[3]);
+    const float syy = fmaf(m.a_k1, H[3], m.a_nu * H[0]);
+    const float sxy = m.a_k2 * (H[1] + H[2]);
+    const float W00 = -t.A * fmaf(sxy, t.g01, sxx * t.g00), W01 = -t.A * fmaf(sxy, t.g11, sxx * t.g10);
+    const float W10 = -t.A * fmaf(syy, t.g01, sxy * t.g00), W11 = -t.A * fmaf(syy, t.g11, sxy * t.g10);
+    const float P00 = fmaf(t.r01, W10, t.r00 * W00), P01 = fmaf(t.r01, W11, t.r00 * W01);
+    const float P10 = fmaf(t.r11, W10, t.r10 * W00), P11 = fmaf(t.r11, W11, t.r10 * W01);
+    f1 = make_float2(P00, P10);
+    f2 = make_float2(P01, P11);
+    f0 = make_float2(-(P00 + P01), -(P10 + P11));
+}
+
+// Element pass: s_f = sum of the group's triangle forces on the vectors in
+// s_v (FE: f_e on positions, with the rest positions), colour by colour.
+// `run` false: the group only takes part in the barriers.
+template <int GW, bool FE>
+__device__ inline void accumulate_forces(const Tri* tr, uint32_t lane, uint32_t N, uint32_t T, uint32_t C,
+                                         uint32_t CL, bool run, const uint16_t* cs, const float2* ref,
+                                         const Material& m, const float2* s_v, float2* s_f) {
+#pragma clang fp contract(off)
+    constexpr int G = Grp<GW>::G;
+    if (run) {
+#pragma unroll
+        for (int k = 0; k < kNodeRegs; k++) {
+            const uint32_t n = lane + k * G;
+            if (n < N) s_f[n] = make_float2(0.f, 0.f);
+        }
+    }
+    Grp<GW>::bar();
+    for (uint32_t c = 0; c < CL; c++) {
+        if (run && c < C) {
+            const uint32_t lo = cs[c], hi = cs[c + 1];
+#pragma unroll
+            for (int k = 0; k < kTriRegs; k++) {
+                const uint32_t e = lane + k * G;
+                if (e >= T || e < lo || e >= hi) continue;
+                const Tri& t = tr[k];
+                float H[4];
+                const float2 z = make_float2(0.f, 0.f);
+                if (FE) tri_gradient<true>(t, s_v[t.i0], s_v[t.i1], s_v[t.i2], ref[t.i0], ref[t.i1], ref[t.i2], H);
+                else tri_gradient<false>(t, s_v[t.i0], s_v[t.i1], s_v[t.i2], z, z, z, H);
+                float2 f0, f1, f2;
+                tri_forces(t, H, m, f0, f1, f2);
+                float2 a0 = s_f[t.i0], a1 = s_f[t.i1], a2 = s_f[t.i2];
+                a0.x += f0.x; a0.y += f0.y;
+                a1.x += f1.x; a1.y += f1.y;
+                a2.x += f2.x; a2.y += f2.y;
+                s_f[t.i0] = a0; s_f[t.i1] = a1; s_f[t.i2] = a2;
+            }
+        }
+        Grp<GW>::bar();
+    }
+}
+
+template <int GW>
+__device__ void mesh_step(const Batch2Args& a, uint32_t cmax, float2* s_v_all, float2* s_f_all, double* red,
+                          float* redm, uint32_t* flags) {
+#pragma clang fp contract(off)
+    using Gp = Grp<GW>;
+    constexpr int G = Gp::G;
+    // the group index is the same in every lane of a wavefront: make it (and
+    // so the mesh descriptor) scalar
+    const uint32_t lane = Gp::lane();
+    const uint32_t grp = __builtin_amdgcn_readfirstlane(Gp::group());
+    const uint32_t sim = __builtin_amdgcn_readfirstlane(a.work[blockIdx.x].sim[grp]);
+    const bool real = sim != kIdleSlot;
+    if (GW == 1 && !real) return;  // independent wavefronts: nothing to wait for
+    Mesh2Dev M;
+    if (real) {
+        M = a.meshes[sim];
+    } else {
+        M.N = M.T = M.C = 0;
+        M.node_off = M.tri_off = M.color_off = 0;
+        M.m = Material{0.f, 0.f, 0.f};
+        M.fM = M.fK = M.dt = M.inv_dt = M.g0 = M.g1 = M.rel = M.abs_eps = M.thr = M.drag = 0.f;
+        M.rm = M.warm_start = M.compute_energy = 0;
+        M.max_iter = 0;
+    }
+    const uint32_t N = M.N, T = M.T, C = M.C;
+    const uint32_t CL = (GW == 1) ? C : cmax;  // colour passes: the same count in every group
+    float2* s_v = s_v_all + grp * (G * kNodeRegs);
+    float2* s_f = s_f_all + grp * (G * kNodeRegs);
+    const float2* ref = a.ref + M.node_off;
+    float2* xg = a.x + M.node_off;
+    float2* vg = a.v + M.node_off;
+    const uint16_t* cs = a.color_start + M.color_off;
+
+    // ---- own triangles and nodes
+    Tri tr[kTriRegs];
+#pragma unroll
+    for (int k = 0; k < kTriRegs; k++) {
+        const uint32_t e = lane + k * G;
+        if (e < T) {
+            const size_t ge = M.tri_off + (size_t)e;
+            tr[k].i0 = a.tri_ids[3 * ge]; tr[k].i1 = a.tri_ids[3 * ge + 1]; tr[k].i2 = a.tri_ids[3 * ge + 2];
+            const float4 g = a.dinv[ge];
+            tr[k].g00 = g.x; tr[k].g01 = g.y; tr[k].g10 = g.z; tr[k].g11 = g.w;
+            tr[k].A = a.area[ge];
+        }
+    }
+    float2 y[kNodeRegs], r[kNodeRegs], d[kNodeRegs];
+    float mass[kNodeRegs];
+    bool kn[kNodeRegs];
+#pragma unroll
+    for (int k = 0; k < kNodeRegs; k++) {
+        const uint32_t n = lane + k * G;
+        if (n < N) {
+            mass[k] = a.mass[M.node_off + n];
+            kn[k] = a.knc[M.node_off + n] != 0;
+            s_v[n] = xg[n];
+        }
+    }
+    Gp::bar();
+
+    // ---- BeginEvaluation: F = Ds Dm^-1, det F, R
+    double ndeg = 0.0, sdeg = 0.0, nfb = 0.0;
+    float mind = 3.0e38f;
+#pragma unroll
+    for (int k = 0; k < kTriRegs; k++) {
+        const uint32_t e = lane + k * G;
+        if (e >= T) continue;
+        Tri& t = tr[k];
+        const float2 p0 = s_v[t.i0], p1 = s_v[t.i1], p2 = s_v[t.i2];
+        const float D00 = p1.x - p0.x, D01 = p2.x - p0.x, D10 = p1.y - p0.y, D11 = p2.y - p0.y;
+        float f00 = 0.0f; f00 += D00 * t.g00; f00 += D01 * t.g10;  // Mal GMat product, accumulated from 0
+        float f01 = 0.0f; f01 += D00 * t.g01; f01 += D01 * t.g11;
+        float f10 = 0.0f; f10 += D10 * t.g00; f10 += D11 * t.g10;
+        float f11 = 0.0f; f11 += D10 * t.g01; f11 += D11 * t.g11;
+        const float det = f00 * f11 - f10 * f01;  // Mal/GMat.h:230-233
+        t.r00 = 1.f; t.r01 = 0.f; t.r10 = 0.f; t.r11 = 1.f;
+        if (M.rm != CF_RM_ID) {
+            if (det > M.thr) polar2(f00, f01, f10, f11, det, t);
+            else nfb += 1.0;  // below the threshold: R = Id (degenerate branches not implemented)
+        }
+        if (det <= 0.f) { ndeg += 1.0; sdeg += det; }
+        mind = fminf(mind, det);
+    }
+
+    // ---- b = dt (M g + f_e) + M v ; y = v ; KNC rows take the constraint velocity
+    accumulate_forces<GW, true>(tr, lane, N, T, C, CL, real, cs, ref, M.m, s_v, s_f);
+    float2 b[kNodeRegs];
+#pragma unroll
+    for (int k = 0; k < kNodeRegs; k++) {
+        const uint32_t n = lane + k * G;
+        if (n >= N) continue;
+        const float2 f = s_f[n], vv = vg[n];
+        const float m = mass[k];
+        b[k].x = (m * M.g0 + f.x) * M.dt + m * vv.x;
+        b[k].y = (m * M.g1 + f.y) * M.dt + m * vv.y;
+        y[k] = M.warm_start ? vv : make_float2(0.f, 0.f);
+        if (kn[k]) {
+            const float2 q = ref[n], xx = s_v[n];
+            float vx = (q.x - xx.x) * M.inv_dt, vy = (q.y - xx.y) * M.inv_dt;
+            const float nk = sqrtf(vx * vx + vy * vy);
+            if (nk > 0.001f) {
+                const float inv = 1.0f / nk, s = fminf(nk, 50.0f);
+                vx = s * (vx * inv); vy = s * (vy * inv);
+            } else {
+                vx = 0.f; vy = 0.f;
+            }
+            b[k] = make_float2(vx, vy);
+            y[k] = b[k];
+        }
+    }
+    Gp::bar();  // every read of s_v (positions) is done
+#pragma unroll
+    for (int k = 0; k < kNodeRegs; k++) {
+        const uint32_t n = lane + k * G;
+        if (n < N) s_v[n] = y[k];
+    }
+    Gp::bar();
+
+    // ---- CG (ns/ILSS_CG.h:38-151): r = S(b - A y), d = r
+    accumulate_forces<GW, false>(tr, lane, N, T, C, CL, real, cs, ref, M.m, s_v, s_f);
+    double acc = 0.0;
+#pragma unroll
+    for (int k = 0; k < kNodeRegs; k++) {
+        const uint32_t n = lane + k * G;
+        if (n >= N) continue;
+        const float2 f = s_f[n];
+        const float fac = mass[k] * M.fM;
+        float q0 = -M.fK * f.x + fac * y[k].x, q1 = -M.fK * f.y + fac * y[k].y;
+        if (kn[k]) { q0 = 0.f; q1 = 0.f; }
+        float r0 = -1.0f * q0 + b[k].x, r1 = -1.0f * q1 + b[k].y;
+        if (kn[k]) { r0 = 0.f; r1 = 0.f; }
+        r[k] = make_float2(r0, r1);
+        d[k] = r[k];
+        acc += (double)r0 * r0 + (double)r1 * r1;
+    }
+    const double delta0 = Gp::sum(acc, red);
+    double delta = delta0;
+    const float abs_sq = M.abs_eps * M.abs_eps;
+    const float rel_sq = M.rel * M.rel;
+    const double target = (double)rel_sq * delta0;
+    const double abs2 = (double)abs_sq;
+    int reason = 0;
+    uint32_t it = 0;
+    if (!real) reason = -1;
+    else if (!(delta0 > abs2)) reason = CF_CG_CONVERGED_ABS;
+    else if (!(delta0 > target)) reason = CF_CG_CONVERGED_REL;
+    else if (!(0u < M.max_iter)) reason = CF_CG_MAX_ITER;
+    Gp::bar();  // every read of s_v (y) is done
+#pragma unroll
+    for (int k = 0; k < kNodeRegs; k++) {
+        const uint32_t n = lane + k * G;
+        if (n < N) s_v[n] = d[k];
+    }
+    while (Gp::any(reason == 0, flags)) {
+        const bool run = reason == 0;
+        Gp::bar();
+        accumulate_forces<GW, false>(tr, lane, N, T, C, CL, run, cs, ref, M.m, s_v, s_f);
+        float2 q[kNodeRegs];
+        acc = 0.0;
+        if (run) {
+#pragma unroll
+            for (int k = 0; k < kNodeRegs; k++) {
+                const uint32_t n = lane + k * G;
+                if (n >= N) continue;
+                const float2 f = s_f[n];
+                const float fac = mass[k] * M.fM;
+                float q0 = -M.fK * f.x + fac * d[k].x, q1 = -M.fK * f.y + fac * d[k].y;
+                if (kn[k]) { q0 = 0.f; q1 = 0.f; }
+                q[k] = make_float2(q0, q1);
+                acc += (double)d[k].x * q0 + (double)d[k].y * q1;
+            }
+        }
+        const double dq = Gp::sum(acc, red);
+        if (run && fabs(dq) < abs2) reason = CF_CG_BREAKDOWN;
+        const bool upd = run && reason == 0;
+        const float alpha = upd ? (float)(delta / dq) : 0.f;
+        acc = 0.0;
+        if (upd) {
+#pragma unroll
+            for (int k = 0; k < kNodeRegs; k++) {
+                const uint32_t n = lane + k * G;
+                if (n >= N) continue;
+                y[k].x = y[k].x + d[k].x * alpha; y[k].y = y[k].y + d[k].y * alpha;
+                r[k].x = r[k].x - q[k].x * alpha; r[k].y = r[k].y - q[k].y * alpha;
+                acc += (double)r[k].x * r[k].x + (double)r[k].y * r[k].y;
+            }
+        }
+        const double dn = Gp::sum(acc, red);
+        float beta = 0.f;
+        if (upd) {
+            beta = (float)(dn / delta);
+            delta = dn;
+            it++;
+            if (!(dn > abs2)) reason = CF_CG_CONVERGED_ABS;
+            else if (!(dn > target)) reason = CF_CG_CONVERGED_REL;
+            else if (!(it < M.max_iter)) reason = CF_CG_MAX_ITER;
+        }
+        Gp::bar();  // every read of s_v (d) is done
+        if (upd && reason == 0) {
+#pragma unroll
+            for (int k = 0; k < kNodeRegs; k++) {
+                const uint32_t n = lane + k * G;
+                if (n >= N) continue;
+                float d0 = beta * d[k].x + r[k].x, d1 = beta * d[k].y + r[k].y;
+                if (kn[k]) { d0 = 0.f; d1 = 0.f; }
+                d[k] = make_float2(d0, d1);
+                s_v[n] = d[k];
+            }
+        }
+    }
+
+    // ---- integrate: v = y, x = x + dt y (LeafDSH_Solid2D_FEM.cpp:1394-1398)
+    Gp::bar();
+#pragma unroll
+    for (int k = 0; k < kNodeRegs; k++) {
+        const uint32_t n = lane + k * G;
+        if (n >= N) continue;
+        const float2 yy = y[k], xx = xg[n];
+        const float2 xn = make_float2(xx.x + M.dt * yy.x, xx.y + M.dt * yy.y);
+        const float2 vn = (M.drag > 0.f) ? make_float2((1.0f - M.dt * M.drag) * yy.x, (1.0f - M.dt * M.drag) * yy.y)
+                                         : yy;
+        xg[n] = xn;
+        vg[n] = vn;
+        s_v[n] = xn;
+    }
+    Gp::bar();
+
+    // ---- elastic energy on the new positions with the step's rotations (C_V)
+    double en = 0.0;
+    if (M.compute_energy) {
+#pragma unroll
+        for (int k = 0; k < kTriRegs; k++) {
+            const uint32_t e = lane + k * G;
+            if (e >= T) continue;
+            const Tri& t = tr[k];
+            float H[4];
+            tri_gradient<true>(t, s_v[t.i0], s_v[t.i1], s_v[t.i2], ref[t.i0], ref[t.i1], ref[t.i2], H);
+            const float sxx = fmaf(M.m.a_k1, H[0], M.m.a_nu * H[3]);
+            const float syy = fmaf(M.m.a_k1, H[3], M.m.a_nu * H[0]);
+            const float sxy = M.m.a_k2 * (H[1] + H[2]);
+            const float sH = sxx * H[0] + syy * H[3] + sxy * (H[1] + H[2]);
+            en += 0.5 * (double)t.A * (double)sH;
+        }
+    }
+    const double energy = Gp::sum(en, red);
+    const double nd = Gp::sum(ndeg, red);
+    const double sd = Gp::sum(sdeg, red);
+    const double nf = Gp::sum(nfb, red);
+    const float md = Gp::min(mind, redm);
+    if (real && lane == 0) {
+        Sim2Out& o = a.out[sim];
+        o.iters = int32_t(it);
+        o.reason = reason;
+        o.rel = (delta0 > 0) ? float(sqrt(delta / delta0)) : 0.f;
+        o.abs_res = float(sqrt(delta));
+        o.energy = energy;
+        o.sum_degenerate = sd;
+        o.num_degenerate = uint32_t(nd);
+        o.num_fallback = uint32_t(nf);
+        o.min_det = md;
+    }
+}
+
+}  // namespace
+
+// One workgroup per work item: 8/gw meshes, one per group of gw wavefronts.
+// LDS: per group two float2 node arrays of 64 gw kNodeRegs entries.
+__global__ __launch_bounds__(k2Threads) void k2_step(Batch2Args a) {
+    __shared__ float2 s_v[k2Threads * kNodeRegs];
+    __shared__ float2 s_f[k2Threads * kNodeRegs];
+    __shared__ double red[8];
+    __shared__ float redm[8];
+    __shared__ uint32_t flags[8];
+    const uint32_t gw = a.work[blockIdx.x].gw, cmax = a.work[blockIdx.x].cmax;
+    switch (gw) {
+        case 1: mesh_step<1>(a, cmax, s_v, s_f, red, redm, flags); break;
+        case 2: mesh_step<2>(a, cmax, s_v, s_f, red, redm, flags); break;
+        case 4: mesh_step<4>(a, cmax, s_v, s_f, red, redm, flags); break;
+        default: mesh_step<8>(a, cmax, s_v, s_f, red, redm, flags); break;
+    }
+}
+
+int launch_batch2_step(const Batch2Args& a, uint32_t n_work, hipStream_t s) {
+    if (!n_work) return CF_OK;
+    hipLaunchKernelGGL(k2_step, dim3(n_work), dim3(k2Threads), 0, s, a);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        set_error("2D step launch failed: %s", hipGetErrorString(e));
+        return CF_ERR_HIP;
+    }
+    return CF_OK;
+}
+
+}  // namespace cf

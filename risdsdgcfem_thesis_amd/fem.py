@@ -7,6 +7,8 @@ over the corofem C-ABI.
   LeafSolid3D             <-> ds::LeafDSH_Solid3D_FEM (ds/dsh/LeafDSH_Solid3D_FEM.h:68-81)
   Batch                   <-> many LeafDSH_Solid3D_FEM leaves stepped together (no
                               reference counterpart; the reference loops over leaves)
+  LeafSolid2D, Batch2     <-> ds::LeafDSH_Solid2D_FEM + ms::fem::TriangleElementSet2, one
+                              or many 2D solids (2N-float node vectors)
 
 Node vectors are flat float32 arrays of 3N values. Device vectors are
 DeviceArray objects (memory owned by libcorofem's allocator).
@@ -510,3 +512,111 @@ def serve(meshes, requests, params, hbm_budget_bytes, duration_s, drain=False, m
                              lat.ctypes.data_as(C.POINTER(C.c_double)), C.byref(res)), "cf_serve_trace")
     out = {k: getattr(res, k) for k, _ in ServeResult._fields_}
     return out, lat[:res.steps].copy()
+
+
+class Batch2:
+    """Many 2D solids (TriangleElementSet2 + LeafDSH_Solid2D_FEM) stepped in one
+    launch; node vectors are 2N floats."""
+
+    def __init__(self):
+        self.h = C.c_void_p()
+        check(lib.cf_batch2_create(C.byref(self.h)), "cf_batch2_create")
+        self.nodes = []
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib.cf_batch2_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add_sim(self, ref_pos, tris, params, knc=None):
+        r = np.ascontiguousarray(ref_pos, np.float32)
+        t = np.ascontiguousarray(tris, np.uint32)
+        k = np.ascontiguousarray(knc if knc is not None else [], np.uint32)
+        sid = C.c_uint32()
+        check(lib.cf_batch2_add_sim(self.h, r.size // 2, _fp(r), t.size // 3, _up(t), C.byref(params), k.size,
+                                    _up(k) if k.size else None, C.byref(sid)), "cf_batch2_add_sim")
+        self.nodes.append(r.size // 2)
+        return sid.value
+
+    def fixed_step(self, stats=True):
+        if not stats:
+            check(lib.cf_batch2_fixed_step(self.h, None), "cf_batch2_fixed_step")
+            return None
+        st = (StepStats * max(len(self.nodes), 1))()
+        check(lib.cf_batch2_fixed_step(self.h, st), "cf_batch2_fixed_step")
+        return [st[i] for i in range(len(self.nodes))]
+
+    def state(self, sim_id):
+        n = self.nodes[sim_id]
+        x = np.empty(2 * n, np.float32)
+        v = np.empty(2 * n, np.float32)
+        check(lib.cf_batch2_get_state(self.h, int(sim_id), _fp(x), _fp(v)), "cf_batch2_get_state")
+        return x, v
+
+    def set_state(self, sim_id, x, v):
+        x = np.ascontiguousarray(x, np.float32)
+        v = np.ascontiguousarray(v, np.float32)
+        check(lib.cf_batch2_set_state(self.h, int(sim_id), _fp(x), _fp(v)), "cf_batch2_set_state")
+
+    def node_masses(self, sim_id):
+        m = np.empty(self.nodes[sim_id], np.float32)
+        check(lib.cf_batch2_node_masses(self.h, int(sim_id), _fp(m)), "cf_batch2_node_masses")
+        return m
+
+    def info(self):
+        n, w = C.c_uint32(), C.c_uint32()
+        t, nn = C.c_uint64(), C.c_uint64()
+        check(lib.cf_batch2_info(self.h, C.byref(n), C.byref(t), C.byref(nn), C.byref(w)), "cf_batch2_info")
+        return {"sims": n.value, "tris": t.value, "nodes": nn.value, "workgroups": w.value}
+
+
+class LeafSolid2D:
+    """LeafDSH_Solid2D_FEM drop-in: one 2D solid (the batched kernel on one mesh)."""
+
+    def __init__(self, ref_pos, tris, params, knc=None):
+        r = np.ascontiguousarray(ref_pos, np.float32)
+        t = np.ascontiguousarray(tris, np.uint32)
+        k = np.ascontiguousarray(knc if knc is not None else [], np.uint32)
+        self.N = r.size // 2
+        self.params = params
+        self.h = C.c_void_p()
+        check(lib.cf_solid2_create(self.N, _fp(r), t.size // 3, _up(t), C.byref(params), k.size,
+                                   _up(k) if k.size else None, C.byref(self.h)), "cf_solid2_create")
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib.cf_solid2_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def fixed_step(self):
+        st = StepStats()
+        check(lib.cf_solid2_fixed_step(self.h, C.byref(st)), "cf_solid2_fixed_step")
+        return st
+
+    def step(self, dt):
+        st = StepStats()
+        check(lib.cf_solid2_step(self.h, float(dt), C.byref(st)), "cf_solid2_step")
+        return st
+
+    def state(self):
+        x = np.empty(2 * self.N, np.float32)
+        v = np.empty(2 * self.N, np.float32)
+        check(lib.cf_solid2_get_state(self.h, _fp(x), _fp(v)), "cf_solid2_get_state")
+        return x, v
+
+    def set_state(self, x, v):
+        x = np.ascontiguousarray(x, np.float32)
+        v = np.ascontiguousarray(v, np.float32)
+        check(lib.cf_solid2_set_state(self.h, _fp(x), _fp(v)), "cf_solid2_set_state")
