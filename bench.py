@@ -1,20 +1,24 @@
-"""Benchmark: the corotational tet FEM implicit step on one MI355X.
+"""Benchmark: the corotational FEM implicit step on one MI355X.
 
-Workload (one "step" = one QIE-v implicit step, BeginEvaluation + f_e + CG +
-integrate + energy): the single large mesh, plain GPU drop-in -- a 64x64x256
-cube grid (65x65x257 vertices, 5,242,880 tets / 1,085,825 nodes), edge 0.01 m,
-E=1e5 Pa, nu=0.3, rho=1000, dt=1/60 s, gravity -9.81 y, CG rel 1e-4 / abs 1e-6 /
-max 50, polar-decomposition rotations, FP32 state with FP64 CG reductions. The
-initial pose is one seeded rigid rotation (uniform on SO(3)) times a stretch
-with diagonal entries in [0.7, 1.3] plus 1% edge-length per-node jitter; the
-z-min face is held at its initial pose.
-
-Prints ONE JSON line. value = tet-matvecs per second over the whole timed run
-(every K*x of every CG iteration of every step, divided by the wall time of the
-timed steps). The roofline object describes one K*x (element tile kernel +
-node gather) timed with HIP events on the solid's stream right after the timed
-steps; cpu_baseline times the oracle's K*x (stored Ke blocks, serial scatter,
-one core) on a bounded shard of the same mesh.
+Default run (one JSON line): the four workloads of the hot path.
+  single      the large mesh, plain GPU drop-in -- a 64x64x256 cube grid
+              (65x65x257 vertices, 5,242,880 tets / 1,085,825 nodes), edge
+              0.01 m, E=1e5 Pa, nu=0.3, rho=1000, dt=1/60 s, gravity -9.81 y, CG
+              rel 1e-4 / abs 1e-6 / max 50, polar-decomposition rotations, FP32
+              state with FP64 CG reductions; initial pose one seeded rigid
+              rotation times a stretch in [0.7, 1.3] plus 1% jitter, z-min face
+              held. One "step" = one QIE-v fixed step. Top-level `value` =
+              tet-matvecs per second over the timed steps; `roofline` = one K*x
+              (element tile kernel + node gather) timed with HIP events on the
+              solid's stream; `cpu_baseline` = the oracle's K*x (stored Ke,
+              serial scatter, one core) on a bounded shard of the same mesh.
+  batched_2d  65,536 2D grids (n x n, n in [4,32]) stepped by ONE launch per
+              step, against the plain single-mesh path on every 64th mesh.
+  batched_3d  1,024 uneven 3D cube grids (4-40 cubes per axis) in one
+              graph-recorded batched step, against the plain path on every 8th.
+  serving     Poisson arrivals at 100 / 1k / 10k requests/s, HBM budget 200 GB,
+              continuous batching; per-step latency p50/p99.
+`--workload single|batch2d|batch3d|serve` runs one of them alone.
 """
 import argparse
 import json
@@ -152,6 +156,67 @@ def bench_batch3d(cf, args):
     }
 
 
+def bench_batch2d(cf, args):
+    """2D batch: 65,536 n x n grids (n uniform in [4,32], seed 3), E
+    log-uniform [1e4,1e7], nu [0.2,0.45], bottom edge held. The batched step is
+    ONE kernel launch for all meshes; the plain single-mesh path (the same
+    kernel, one mesh per launch) runs on every 64th mesh."""
+    from risdsdgcfem_thesis_amd import workloads
+    t0 = time.perf_counter()
+    meshes = workloads.batch2d(count=args.batch2d_count, cg_max_iter=args.batch_max_iter)
+    b = cf.Batch2()
+    for m in meshes:
+        b.add_sim(m["pos"], m["tris"], m["params"], knc=m["knc"])
+    info = b.info()
+    b.fixed_step(stats=False)  # uploads; first step
+    log(f"[batch2d] {info['sims']} meshes, {info['tris']} tris, {info['nodes']} nodes, {info['workgroups']} "
+        f"workgroups; setup {time.perf_counter() - t0:.1f} s")
+    for _ in range(args.warmup):
+        b.fixed_step(stats=False)
+    cf.synchronize()
+    t1 = time.perf_counter()
+    for _ in range(args.batch_steps):
+        b.fixed_step(stats=False)
+    wall = time.perf_counter() - t1
+    st = b.fixed_step()  # one more step, untimed, for the CG statistics
+    it = np.array([x.cg_iters for x in st])
+    reasons = np.array([x.cg_reason for x in st])
+    dev_ms = st[0].step_ms
+    T = np.array([m["tris"].size // 3 for m in meshes])
+    tri_matvecs = float(((it + 1) * T).sum())
+    batched_sps = len(meshes) * args.batch_steps / wall
+    log(f"[batch2d] batched {wall / args.batch_steps * 1e3:.2f} ms/step (device {dev_ms:.2f} ms), "
+        f"iters mean {it.mean():.1f}")
+    b.close()
+    sample = meshes[::64]
+    plain_wall = 0.0
+    for m in sample:
+        s = cf.LeafSolid2D(m["pos"], m["tris"], m["params"], knc=m["knc"])
+        s.fixed_step()
+        for _ in range(args.warmup):
+            s.fixed_step()
+        t2 = time.perf_counter()
+        for _ in range(args.batch_steps):
+            s.fixed_step()
+        plain_wall += time.perf_counter() - t2
+        s.close()
+    plain_sps = len(sample) * args.batch_steps / plain_wall
+    log(f"[batch2d] plain {plain_wall / (len(sample) * args.batch_steps) * 1e3:.3f} ms per mesh-step")
+    return {
+        "workload": "2D batch: 65,536 n x n grids, n in [4,32] (seed 3), QIE-v, CG rel 1e-4, one launch per step",
+        "meshes": len(meshes), "tris": int(info["tris"]), "nodes": int(info["nodes"]),
+        "workgroups": int(info["workgroups"]), "steps": args.batch_steps, "warmup": args.warmup,
+        "cg_max_iter": args.batch_max_iter, "cg_iters_mean": round(float(it.mean()), 2),
+        "cg_iters_max": int(it.max()),
+        "cg_reasons": {str(k): int((reasons == k).sum()) for k in (1, 2, 3, 4)},
+        "batched_steps_per_s": round(batched_sps, 1), "batched_ms_per_step": round(wall / args.batch_steps * 1e3, 3),
+        "batched_device_ms_per_step": round(float(dev_ms), 3),
+        "gtri_matvecs_per_s": round(tri_matvecs / (dev_ms * 1e-3) / 1e9, 2),
+        "plain_steps_per_s": round(plain_sps, 1), "plain_sample": f"{len(sample)} of {len(meshes)} meshes (every 64th)",
+        "speedup": round(batched_sps / plain_sps, 1),
+    }
+
+
 def bench_serve(cf, args):
     """Serving: Poisson arrivals at each rate for a fixed real-time window;
     requests are seeded cube grids of 500-200,000 tets (log-uniform) with 1-60
@@ -192,11 +257,12 @@ def main():
     ap.add_argument("--tile-threads", type=int, default=256)
     ap.add_argument("--tile-schedule", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", default="single", choices=["single", "batch3d", "serve"])
+    ap.add_argument("--workload", default="all", choices=["all", "single", "batch3d", "batch2d", "serve"])
     ap.add_argument("--batch-count", type=int, default=1024)
     ap.add_argument("--batch-steps", type=int, default=3)
     ap.add_argument("--batch-max-iter", type=int, default=50)
     ap.add_argument("--cg-chunk", type=int, default=8)
+    ap.add_argument("--batch2d-count", type=int, default=65536)
     ap.add_argument("--serve-rates", type=float, nargs="+", default=[100.0, 1000.0, 10000.0])
     ap.add_argument("--serve-seconds", type=float, default=3.0)
     ap.add_argument("--serve-budget-gb", type=float, default=200.0)
@@ -218,9 +284,30 @@ def main():
     if args.workload == "batch3d":
         print(json.dumps(bench_batch3d(cf, args)), flush=True)
         return
+    if args.workload == "batch2d":
+        print(json.dumps(bench_batch2d(cf, args)), flush=True)
+        return
     if args.workload == "serve":
         print(json.dumps(bench_serve(cf, args)), flush=True)
         return
+    if args.workload == "single" or args.kx_only:
+        print(json.dumps(bench_single(cf, args)), flush=True)
+        return
+    # default: every workload, one JSON line
+    out = bench_single(cf, args)
+    out["batched_2d"] = bench_batch2d(cf, args)
+    out["batched_3d"] = bench_batch3d(cf, args)
+    out["serving"] = bench_serve(cf, args)
+    out["headline"] = {
+        "kx_gtets": out["roofline"]["kx_gtets"], "kx_frac_hbm_peak": out["roofline"]["frac"],
+        "batched_vs_plain_2d": out["batched_2d"]["speedup"], "batched_vs_plain_3d": out["batched_3d"]["speedup"],
+        "serve_p99_ms": {str(int(r["rate_per_s"])): r["step_p99_ms"] for r in out["serving"]["rates"]},
+    }
+    print(json.dumps(out), flush=True)
+
+
+def bench_single(cf, args):
+    """The single large mesh (64x64x256 cube grid), plain GPU drop-in."""
     cf.set_tile_elems(args.tile_elems)
     cf.set_tile_threads(args.tile_threads)
     cf.set_tile_persistent(args.tile_schedule)
@@ -246,9 +333,8 @@ def main():
         a, b = cf.DeviceArray(nbytes // 4), cf.DeviceArray(nbytes // 4)
         cf.lib.cf_memcpy_dtod(b.ptr, a.ptr, nbytes)
         cf.synchronize()
-        print(json.dumps({"mode": "kx-only", "tile_ms": ms_tile, "gather_ms": ms_gather, "copy_bytes": nbytes,
-                          "tets": T, "nodes": N}), flush=True)
-        return
+        return {"mode": "kx-only", "tile_ms": ms_tile, "gather_ms": ms_gather, "copy_bytes": nbytes,
+                "tets": T, "nodes": N}
     for _ in range(args.warmup):
         solid.fixed_step()
     cf.synchronize()
@@ -293,7 +379,9 @@ def main():
         "steps_per_s": round(args.steps / wall, 3), "cg_iters_mean": round(float(np.mean(iters)), 2),
         "roofline": roofline, "cpu_baseline": cpu,
     }
-    print(json.dumps(out), flush=True)
+    solid.close()
+    return out
+
 
 
 if __name__ == "__main__":

@@ -96,3 +96,33 @@ def serving_trace(rate, duration_s, seed=5, **kw):
         q["mesh"] = m
         q["knc"] = face_nodes(meshes[m][0], halves[m], q["face"])
     return meshes, reqs
+
+
+def grid2(n):
+    """n x n square grid (2 n^2 triangles, (n+1)^2 nodes), edge 0.01 m,
+    centred at the origin; the bottom edge's nodes."""
+    from .fem import mesh_box2
+    half = (0.5 * EDGE * n, 0.5 * EDGE * n)
+    pos, tris = mesh_box2(half, (n + 1, n + 1))
+    knc = np.nonzero(np.isclose(pos[1::2], -half[1]))[0].astype(np.uint32)
+    return pos, tris, knc
+
+
+def batch2d(count=65536, lo=4, hi=32, seed=3, cg_max_iter=50):
+    """2D batch: `count` n x n grids with n uniform in [lo, hi], E log-uniform
+    in [1e4, 1e7] Pa, nu uniform in [0.2, 0.45], bottom edge held, thickness 1.
+    Meshes of one size share their arrays. Returns a list of dicts."""
+    g = rng(seed)
+    ns = g.integers(lo, hi + 1, size=count)
+    E = np.exp(g.uniform(np.log(1e4), np.log(1e7), size=count))
+    nu = g.uniform(0.2, 0.45, size=count)
+    cache = {}
+    out = []
+    for i in range(count):
+        n = int(ns[i])
+        if n not in cache:
+            cache[n] = grid2(n)
+        pos, tris, knc = cache[n]
+        out.append({"n": n, "pos": pos, "tris": tris, "knc": knc,
+                    "params": solid_params(E[i], nu[i], cg_max_iter), "E": float(E[i]), "nu": float(nu[i])})
+    return out
