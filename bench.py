@@ -98,12 +98,18 @@ def bench_batch3d(cf, args):
     seed 4), E log-uniform [1e4,1e7], nu [0.2,0.45], a random face held. The
     batched step (all meshes in the same launches, graph-recorded) against the
     plain single-mesh path run mesh by mesh on every 8th mesh."""
-    from risdsdgcfem_thesis_amd import workloads
+    from risdsdgcfem_thesis_amd import workloads, autotune
     t0 = time.perf_counter()
     meshes = workloads.uneven_batch(count=args.batch_count, cg_max_iter=args.batch_max_iter)
     T = sum(m["tets"].size // 4 for m in meshes)
     N = sum(m["pos"].size // 3 for m in meshes)
-    b = cf.Batch(tile_threads=args.tile_threads, tile_elems=args.tile_elems, cg_chunk=args.cg_chunk)
+    if args.no_autotune:
+        tuned = {"tile_threads": args.tile_threads, "tile_elems": args.tile_elems, "cg_chunk": args.cg_chunk}
+    else:
+        tuned = autotune.tune(meshes)
+        log(f"[batch3d] autotuned in {tuned['seconds']} s: tiles {tuned['tile_threads']}/{tuned['tile_elems']} "
+            f"(ns/tet {tuned['kx_ns_per_tet']}), chunk {tuned['cg_chunk']} (step ms {tuned['step_ms']})")
+    b = cf.Batch(tile_threads=tuned["tile_threads"], tile_elems=tuned["tile_elems"], cg_chunk=tuned["cg_chunk"])
     mids = b.add_meshes([(m["pos"], m["tets"]) for m in meshes], threads=16)
     sids = [b.add_sim(mid, m["params"], knc=m["knc"]) for mid, m in zip(mids, meshes)]
     log(f"[batch3d] {len(meshes)} meshes, {T} tets, {N} nodes; setup {time.perf_counter() - t0:.1f} s")
@@ -152,7 +158,9 @@ def bench_batch3d(cf, args):
         "kx": {"ms": round(kx_ms, 4), "gtets": round(T / (kx_ms * 1e-3) / 1e9, 2),
                "achieved_gbs": round(b_alg / (kx_ms * 1e-3) / 1e9, 1),
                "frac": round(b_alg / (kx_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
-        "tile": {"threads": args.tile_threads, "elems": args.tile_elems, "cg_chunk": args.cg_chunk},
+        "tile": {"threads": tuned["tile_threads"], "elems": tuned["tile_elems"], "cg_chunk": tuned["cg_chunk"]},
+        "autotune": None if args.no_autotune else {k: tuned[k] for k in ("kx_ns_per_tet", "step_ms", "sample",
+                                                                          "seconds")},
     }
 
 
@@ -262,6 +270,7 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=3)
     ap.add_argument("--batch-max-iter", type=int, default=50)
     ap.add_argument("--cg-chunk", type=int, default=8)
+    ap.add_argument("--no-autotune", action="store_true", help="batched 3D: use --tile-threads/--tile-elems/--cg-chunk")
     ap.add_argument("--batch2d-count", type=int, default=65536)
     ap.add_argument("--serve-rates", type=float, nargs="+", default=[100.0, 1000.0, 10000.0])
     ap.add_argument("--serve-seconds", type=float, default=3.0)
