@@ -375,6 +375,7 @@ typedef struct cf_serve_config {
     double duration_s;         /* arrivals after this are not served */
     int32_t drain;             /* 1: run until every admitted and waiting request is done */
     uint32_t tile_threads, cg_chunk;
+    int32_t use_graphs;        /* 1: record the batched step as HIP graphs */
 } cf_serve_config;
 typedef struct cf_serve_result {
     uint32_t requests, admitted, completed, max_active, ticks, mesh_uploads;

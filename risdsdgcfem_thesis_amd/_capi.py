@@ -70,7 +70,7 @@ class ServeRequest(C.Structure):
 class ServeConfig(C.Structure):
     _fields_ = [("base", SolidParams), ("hbm_budget_bytes", C.c_uint64), ("max_tick_tets", C.c_uint64),
                 ("duration_s", C.c_double), ("drain", C.c_int32), ("tile_threads", C.c_uint32),
-                ("cg_chunk", C.c_uint32)]
+                ("cg_chunk", C.c_uint32), ("use_graphs", C.c_int32)]
 
 
 class ServeResult(C.Structure):

@@ -35,20 +35,20 @@ def _sample(meshes, n):
     return [meshes[i] for i in order[::step]]
 
 
-def _batch(sample, threads, elems, chunk):
-    b = Batch(tile_threads=threads, tile_elems=elems, cg_chunk=chunk)
+def _batch(sample, threads, elems, chunk, use_graphs):
+    b = Batch(tile_threads=threads, tile_elems=elems, cg_chunk=chunk, use_graphs=use_graphs)
     mids = b.add_meshes([(m["pos"], m["tets"]) for m in sample], threads=16)
     for mid, m in zip(mids, sample):
         b.add_sim(mid, m["params"], knc=m["knc"])
     return b
 
 
-def tune(meshes, sample_size=128, kx_reps=10, cache_path=None):
+def tune(meshes, sample_size=128, kx_reps=10, cache_path=None, use_graphs=True):
     """Returns {"tile_threads", "tile_elems", "cg_chunk", "kx_ns_per_tet": {...},
     "step_ms": {...}, "seconds"}."""
     if device_count() < 1:
         raise RuntimeError("autotuning needs a GPU")
-    key = signature(meshes)
+    key = signature(meshes) + (":graphs" if use_graphs else ":direct")
     if key in _MEMO:
         return _MEMO[key]
     if cache_path and os.path.exists(cache_path):
@@ -62,7 +62,7 @@ def tune(meshes, sample_size=128, kx_reps=10, cache_path=None):
     tets = sum(m["tets"].size // 4 for m in sample)
     kx = {}
     for threads, elems in TILE_CANDIDATES:
-        b = _batch(sample, threads, elems, 8)
+        b = _batch(sample, threads, elems, 8, use_graphs)
         b.fixed_step(stats=False)  # rotations of a real state
         ms = b.time_kx(kx_reps)
         kx[f"{threads}/{elems}"] = round(ms * 1e6 / tets, 4)
@@ -70,10 +70,10 @@ def tune(meshes, sample_size=128, kx_reps=10, cache_path=None):
     best = min(kx, key=kx.get)
     threads, elems = (int(v) for v in best.split("/"))
     steps = {}
-    b = _batch(sample, threads, elems, CHUNK_CANDIDATES[0])
+    b = _batch(sample, threads, elems, CHUNK_CANDIDATES[0], use_graphs)
     b.fixed_step(stats=False)
     for chunk in CHUNK_CANDIDATES:
-        b.set_schedule(cg_chunk=chunk, use_graphs=True)
+        b.set_schedule(cg_chunk=chunk, use_graphs=use_graphs)
         b.fixed_step(stats=False)  # records the graphs
         b.synchronize()
         t = time.perf_counter()

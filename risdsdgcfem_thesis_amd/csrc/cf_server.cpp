@@ -71,7 +71,7 @@ extern "C" int cf_serve_trace(const cf_serve_mesh* meshes, uint32_t n_meshes, co
         ~Guard() { cf_batch_destroy(b); }
     } guard{b};
     CF_TRY(cf_batch_set_tiles(b, cfg->tile_threads ? cfg->tile_threads : 256, 0));
-    CF_TRY(cf_batch_set_schedule(b, cfg->cg_chunk ? cfg->cg_chunk : 8, 1));
+    CF_TRY(cf_batch_set_schedule(b, cfg->cg_chunk ? cfg->cg_chunk : 8, cfg->use_graphs));
 
     cf_serve_result r{};
     std::map<uint32_t, CachedMesh> cache;  // mesh index -> batch mesh

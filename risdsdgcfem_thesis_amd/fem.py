@@ -470,7 +470,7 @@ class Batch:
 
 
 def serve(meshes, requests, params, hbm_budget_bytes, duration_s, drain=False, max_tick_tets=0, tile_threads=256,
-          cg_chunk=8, final_positions=None):
+          cg_chunk=8, final_positions=None, use_graphs=True):
     """Runs the real-time serving loop (cf_serve_trace).
 
     meshes: list of (ref_pos, tets); requests: list of dicts with arrival (s),
@@ -502,6 +502,7 @@ def serve(meshes, requests, params, hbm_budget_bytes, duration_s, drain=False, m
     cfg.drain = int(bool(drain))
     cfg.tile_threads = int(tile_threads)
     cfg.cg_chunk = int(cg_chunk)
+    cfg.use_graphs = int(bool(use_graphs))
     fp = None
     if final_positions is not None:
         fp = (_f * len(requests))(*[(_fp(a) if a is not None else None) for a in final_positions])
