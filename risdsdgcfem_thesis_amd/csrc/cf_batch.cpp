@@ -169,6 +169,15 @@ struct cf_batch {
     size_t active_cap = 0;
     uint32_t* d_counter = nullptr;
     uint32_t* h_counter = nullptr;     // pinned, 4 entries
+    // compacted tables of the simulations still iterating (CG loop only)
+    BlockRef* d_tab_cg = nullptr;
+    size_t tab_cg_cap = 0;
+    uint32_t* d_active_cg = nullptr;
+    size_t active_cg_cap = 0;
+    std::vector<BlockRef> h_tab_cg;
+    std::vector<uint32_t> h_active_cg;
+    SimCg* h_cgs_pin[2] = {nullptr, nullptr};  // pinned copies of the CG scalars, per chunk parity
+    uint32_t h_cgs_cap = 0;
     uint32_t max_iter = 0;
     uint64_t active_tets = 0, active_nodes = 0;
     // graphs
@@ -354,80 +363,136 @@ int ensure_device(void** p, size_t* cap, size_t need, size_t elem) {
     return CF_OK;
 }
 
-// Block tables of the active simulations, in ascending sim id. The tile
-// table is laid out so that the workgroups the hardware deals to one XCD
-// (blockIdx % 8) take a contiguous range of tiles.
-int rebuild_tables(cf_batch* b) {
-    std::vector<uint32_t> ids;
-    for (auto& kv : b->sims)
-        if (kv.second.active) ids.push_back(kv.first);
+// Block tables of the given simulations, in the given order. The tile table
+// is laid out so that the workgroups the hardware deals to one XCD
+// (blockIdx % 8) take a contiguous range of tiles. cg_only: only the tables
+// of the CG-loop kernels (tile, node, vec) and the slot list.
+struct TableLists {
     std::vector<BlockRef> tile, elem, node, init, vec;
     std::vector<uint32_t> slots;
     uint32_t max_tn = 1, max_te = 1, max_it = 0;
     uint64_t tets = 0, nodes = 0;
+};
+
+void table_lists(cf_batch* b, const std::vector<uint32_t>& ids, bool cg_only, TableLists& L) {
+    std::vector<BlockRef> tile;
     for (uint32_t id : ids) {
         const BSim& s = b->sims[id];
         const BMesh& m = b->meshes[s.mesh];
         const uint32_t sl = s.slot;
-        slots.push_back(sl);
+        L.slots.push_back(sl);
         for (uint32_t i = 0; i < m.num_tiles; i++) tile.push_back({sl, i});
-        for (uint32_t i = 0; i < s.d.nb_elem; i++) elem.push_back({sl, i});
-        for (uint32_t i = 0; i < s.d.nb_node; i++) node.push_back({sl, i});
-        for (uint32_t i = 0; i < s.d.nb_init; i++) init.push_back({sl, i});
-        for (uint32_t i = 0; i < s.d.nb_vec; i++) vec.push_back({sl, i});
-        max_tn = std::max(max_tn, m.max_tn);
-        max_te = std::max(max_te, m.max_te);
-        max_it = std::max(max_it, s.d.max_iter);
-        tets += m.T;
-        nodes += m.N;
+        for (uint32_t i = 0; i < s.d.nb_node; i++) L.node.push_back({sl, i});
+        for (uint32_t i = 0; i < s.d.nb_vec; i++) L.vec.push_back({sl, i});
+        if (!cg_only) {
+            for (uint32_t i = 0; i < s.d.nb_elem; i++) L.elem.push_back({sl, i});
+            for (uint32_t i = 0; i < s.d.nb_init; i++) L.init.push_back({sl, i});
+        }
+        L.max_tn = std::max(L.max_tn, m.max_tn);
+        L.max_te = std::max(L.max_te, m.max_te);
+        L.max_it = std::max(L.max_it, s.d.max_iter);
+        L.tets += m.T;
+        L.nodes += m.N;
     }
-    // XCD-contiguous tile order
-    std::vector<BlockRef> tile_x;
     if (!tile.empty()) {
         const size_t per = (tile.size() + 7) / 8;
-        tile_x.assign(8 * per, BlockRef{kIdleSlot, 0});
-        for (size_t k = 0; k < tile_x.size(); k++) {
+        L.tile.assign(8 * per, BlockRef{kIdleSlot, 0});
+        for (size_t k = 0; k < L.tile.size(); k++) {
             const size_t nat = (k % 8) * per + k / 8;
-            if (nat < tile.size()) tile_x[k] = tile[nat];
+            if (nat < tile.size()) L.tile[k] = tile[nat];
         }
     }
-    const size_t total = tile_x.size() + elem.size() + node.size() + init.size() + vec.size();
-    CF_TRY(ensure_device(reinterpret_cast<void**>(&b->d_tab), &b->tab_cap, total, sizeof(BlockRef)));
-    CF_TRY(ensure_device(reinterpret_cast<void**>(&b->d_active), &b->active_cap, slots.size(), sizeof(uint32_t)));
+}
+
+// Lays the lists out in one buffer (tile, elem, node, init, vec) and points t at it.
+void table_views(const TableLists& L, BlockRef* base, uint32_t* active, BatchTables& t) {
+    size_t o = 0;
+    t.tile = base + o; t.n_tile = uint32_t(L.tile.size()); o += L.tile.size();
+    t.elem = base + o; t.n_elem = uint32_t(L.elem.size()); o += L.elem.size();
+    t.node = base + o; t.n_node = uint32_t(L.node.size()); o += L.node.size();
+    t.init = base + o; t.n_init = uint32_t(L.init.size()); o += L.init.size();
+    t.vec = base + o; t.n_vec = uint32_t(L.vec.size());
+    t.active = active;
+    t.n_active = uint32_t(L.slots.size());
+}
+
+void table_concat(const TableLists& L, std::vector<BlockRef>& all) {
+    all.clear();
+    all.reserve(L.tile.size() + L.elem.size() + L.node.size() + L.init.size() + L.vec.size());
+    all.insert(all.end(), L.tile.begin(), L.tile.end());
+    all.insert(all.end(), L.elem.begin(), L.elem.end());
+    all.insert(all.end(), L.node.begin(), L.node.end());
+    all.insert(all.end(), L.init.begin(), L.init.end());
+    all.insert(all.end(), L.vec.begin(), L.vec.end());
+}
+
+int rebuild_tables(cf_batch* b) {
+    std::vector<uint32_t> ids;
+    for (auto& kv : b->sims)
+        if (kv.second.active) ids.push_back(kv.first);
+    TableLists L;
+    table_lists(b, ids, false, L);
     std::vector<BlockRef> all;
-    all.reserve(total);
-    all.insert(all.end(), tile_x.begin(), tile_x.end());
-    all.insert(all.end(), elem.begin(), elem.end());
-    all.insert(all.end(), node.begin(), node.end());
-    all.insert(all.end(), init.begin(), init.end());
-    all.insert(all.end(), vec.begin(), vec.end());
+    table_concat(L, all);
+    CF_TRY(ensure_device(reinterpret_cast<void**>(&b->d_tab), &b->tab_cap, all.size(), sizeof(BlockRef)));
+    CF_TRY(ensure_device(reinterpret_cast<void**>(&b->d_active), &b->active_cap, L.slots.size(), sizeof(uint32_t)));
+    // room for the compacted tables (never larger than the full ones)
+    CF_TRY(ensure_device(reinterpret_cast<void**>(&b->d_tab_cg), &b->tab_cg_cap, all.size(), sizeof(BlockRef)));
+    CF_TRY(ensure_device(reinterpret_cast<void**>(&b->d_active_cg), &b->active_cg_cap, L.slots.size(),
+                         sizeof(uint32_t)));
     if (!all.empty())
         CF_CHECK_HIP(hipMemcpyAsync(b->d_tab, all.data(), all.size() * sizeof(BlockRef), hipMemcpyHostToDevice,
                                     b->stream));
-    if (!slots.empty())
-        CF_CHECK_HIP(hipMemcpyAsync(b->d_active, slots.data(), slots.size() * 4, hipMemcpyHostToDevice, b->stream));
+    if (!L.slots.empty())
+        CF_CHECK_HIP(hipMemcpyAsync(b->d_active, L.slots.data(), L.slots.size() * 4, hipMemcpyHostToDevice,
+                                    b->stream));
     CF_CHECK_HIP(hipStreamSynchronize(b->stream));
+    if (b->h_cgs_cap < b->cap) {
+        for (auto& p : b->h_cgs_pin)
+            if (p) (void)hipHostFree(p);
+        for (auto& p : b->h_cgs_pin) {
+            p = nullptr;
+            CF_CHECK_HIP(hipHostMalloc(reinterpret_cast<void**>(&p), std::max<size_t>(1, b->cap) * sizeof(SimCg)));
+        }
+        b->h_cgs_cap = b->cap;
+    }
     BatchTables& t = b->tab;
     t.sims = b->d_sims;
     t.cgs = b->d_cgs;
-    size_t o = 0;
-    t.tile = b->d_tab + o; t.n_tile = uint32_t(tile_x.size()); o += tile_x.size();
-    t.elem = b->d_tab + o; t.n_elem = uint32_t(elem.size()); o += elem.size();
-    t.node = b->d_tab + o; t.n_node = uint32_t(node.size()); o += node.size();
-    t.init = b->d_tab + o; t.n_init = uint32_t(init.size()); o += init.size();
-    t.vec = b->d_tab + o; t.n_vec = uint32_t(vec.size());
-    t.active = b->d_active;
-    t.n_active = uint32_t(slots.size());
+    table_views(L, b->d_tab, b->d_active, t);
     t.counter = b->d_counter;
-    t.max_tn = max_tn;
-    t.max_te = max_te;
+    t.max_tn = L.max_tn;
+    t.max_te = L.max_te;
     t.tile_threads = b->tile_threads;
     b->active_ids.swap(ids);
-    b->max_iter = max_it;
-    b->active_tets = tets;
-    b->active_nodes = nodes;
+    b->max_iter = L.max_it;
+    b->active_tets = L.tets;
+    b->active_nodes = L.nodes;
     free_graphs(b);
     b->dirty = false;
+    return CF_OK;
+}
+
+// Tables of the CG loop over the simulations still iterating, read from the
+// pinned copy of the CG scalars; uploaded on the stream (after the chunk in
+// flight) into the compacted buffers.
+int compact_tables(cf_batch* b, const SimCg* cgs, BatchTables& out, uint32_t* n_sims) {
+    std::vector<uint32_t> ids;
+    for (uint32_t id : b->active_ids)
+        if (!cgs[b->sims[id].slot].done) ids.push_back(id);
+    TableLists L;
+    table_lists(b, ids, true, L);
+    table_concat(L, b->h_tab_cg);
+    b->h_active_cg = L.slots;
+    if (!b->h_tab_cg.empty())
+        CF_CHECK_HIP(hipMemcpyAsync(b->d_tab_cg, b->h_tab_cg.data(), b->h_tab_cg.size() * sizeof(BlockRef),
+                                    hipMemcpyHostToDevice, b->stream));
+    if (!b->h_active_cg.empty())
+        CF_CHECK_HIP(hipMemcpyAsync(b->d_active_cg, b->h_active_cg.data(), b->h_active_cg.size() * 4,
+                                    hipMemcpyHostToDevice, b->stream));
+    out = b->tab;
+    table_views(L, b->d_tab_cg, b->d_active_cg, out);
+    *n_sims = uint32_t(ids.size());
     return CF_OK;
 }
 
@@ -443,8 +508,7 @@ int enqueue_prologue(cf_batch* b) {
     return launch_batch_cg_start(t, s);
 }
 
-int enqueue_chunk(cf_batch* b, uint32_t iters) {
-    const BatchTables& t = b->tab;
+int enqueue_chunk(cf_batch* b, uint32_t iters, const BatchTables& t) {
     hipStream_t s = b->stream;
     for (uint32_t k = 0; k < iters; k++) {
         CF_TRY(launch_batch_forces(t, kModeDfx, kSelD, s));
@@ -467,7 +531,7 @@ int enqueue_epilogue(cf_batch* b) {
 
 int record(cf_batch* b, int which, uint32_t iters, hipGraphExec_t* exec) {
     CF_CHECK_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-    int rc = which == 0 ? enqueue_prologue(b) : which == 1 ? enqueue_chunk(b, iters) : enqueue_epilogue(b);
+    int rc = which == 0 ? enqueue_prologue(b) : which == 1 ? enqueue_chunk(b, iters, b->tab) : enqueue_epilogue(b);
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(b->stream, &g);
     if (rc != CF_OK) {
@@ -567,6 +631,10 @@ int cf_batch_destroy(cf_batch* b) {
     if (b->d_active) (void)hipFree(b->d_active);
     if (b->d_counter) (void)hipFree(b->d_counter);
     if (b->h_counter) (void)hipHostFree(b->h_counter);
+    for (auto& p : b->h_cgs_pin)
+        if (p) (void)hipHostFree(p);
+    if (b->d_tab_cg) (void)hipFree(b->d_tab_cg);
+    if (b->d_active_cg) (void)hipFree(b->d_active_cg);
     if (b->ev0) (void)hipEventDestroy(b->ev0);
     if (b->ev1) (void)hipEventDestroy(b->ev1);
     for (auto& e : b->evc)
@@ -798,17 +866,30 @@ int cf_batch_fixed_step(cf_batch* b, cf_step_stats* stats) {
     CF_CHECK_HIP(hipEventRecord(b->ev0, s));
     if (b->use_graphs) CF_CHECK_HIP(hipGraphLaunch(b->g_pro, s));
     else CF_TRY(enqueue_prologue(b));
-    // CG chunks: the counter written by each chunk is read while the next one runs
+    // CG chunks: the counter and the CG scalars written by each chunk are read
+    // while the next one runs. Once the simulations still iterating are at
+    // most half of those in the tables, the remaining chunks run on compacted
+    // tables of just those (launched directly).
     const uint32_t chunk = b->use_graphs ? b->g_chunk_len : std::min(b->cg_chunk, std::max<uint32_t>(1, b->max_iter));
     const uint32_t nchunks = (b->max_iter + chunk - 1) / chunk;
+    BatchTables cur = b->tab;
+    bool compacted = false;
+    uint32_t cur_sims = t.n_active;
+    const size_t cg_bytes = b->slots_used * sizeof(SimCg);
     for (uint32_t i = 0; i < nchunks; i++) {
-        if (b->use_graphs) CF_CHECK_HIP(hipGraphLaunch(b->g_chunk, s));
-        else CF_TRY(enqueue_chunk(b, chunk));
+        if (b->use_graphs && !compacted) CF_CHECK_HIP(hipGraphLaunch(b->g_chunk, s));
+        else CF_TRY(enqueue_chunk(b, chunk, cur));
         CF_CHECK_HIP(hipMemcpyAsync(&b->h_counter[i & 1], b->d_counter, 4, hipMemcpyDeviceToHost, s));
+        CF_CHECK_HIP(hipMemcpyAsync(b->h_cgs_pin[i & 1], b->d_cgs, cg_bytes, hipMemcpyDeviceToHost, s));
         CF_CHECK_HIP(hipEventRecord(b->evc[i & 1], s));
         if (i >= 1) {
             CF_CHECK_HIP(hipEventSynchronize(b->evc[(i - 1) & 1]));
-            if (b->h_counter[(i - 1) & 1] == 0) break;
+            const uint32_t left = b->h_counter[(i - 1) & 1];
+            if (left == 0) break;
+            if (2 * left <= cur_sims && i + 1 < nchunks) {
+                CF_TRY(compact_tables(b, b->h_cgs_pin[(i - 1) & 1], cur, &cur_sims));
+                compacted = true;
+            }
         }
     }
     if (b->use_graphs) CF_CHECK_HIP(hipGraphLaunch(b->g_epi, s));
