@@ -201,7 +201,7 @@ T* at(char* base, size_t off) { return reinterpret_cast<T*>(base + off); }
 
 // Device layout of a mesh's topology (one arena block).
 struct MeshLayout {
-    size_t ref, conn, dinv, vol, tep, tnp, tn, sp, dst, lvid, npp, pd, total;
+    size_t ref, conn, dinv, vol, tep, tnp, tn, sp, lv, rk, npp, pd, total;
     MeshLayout(uint32_t N, uint32_t T, uint32_t nt, uint32_t P) {
         Layout l;
         ref = l.add<float>(3 * (size_t)N);
@@ -212,8 +212,8 @@ struct MeshLayout {
         tnp = l.add<uint32_t>(nt + 1);
         tn = l.add<uint32_t>(P);
         sp = l.add<uint32_t>(P + 1);
-        dst = l.add<uint16_t>(4 * (size_t)T);
-        lvid = l.add<uint16_t>(4 * (size_t)T);
+        lv = l.add<uint32_t>(T);
+        rk = l.add<uint32_t>(T);
         npp = l.add<uint32_t>((size_t)N + 1);
         pd = l.add<uint32_t>(P);
         total = Arena::round(l.off);
@@ -281,8 +281,8 @@ int upload_mesh(cf_batch* b, TileBuild& tb, uint32_t* mesh_id) {
     t.tile_node_ptr = at<uint32_t>(m.block, L.tnp);
     t.tile_nodes = at<uint32_t>(m.block, L.tn);
     t.slot_ptr = at<uint32_t>(m.block, L.sp);
-    t.dst = at<uint16_t>(m.block, L.dst);
-    t.lvid = at<uint16_t>(m.block, L.lvid);
+    t.lv = at<uint32_t>(m.block, L.lv);
+    t.rk = at<uint32_t>(m.block, L.rk);
     t.node_part_ptr = at<uint32_t>(m.block, L.npp);
     t.part_dst = at<uint32_t>(m.block, L.pd);
     t.num_tiles = nt; t.max_tile_elems = m.max_te; t.max_tile_nodes = m.max_tn; t.num_parts = P;
@@ -297,8 +297,8 @@ int upload_mesh(cf_batch* b, TileBuild& tb, uint32_t* mesh_id) {
     if (rc == CF_OK) rc = put(t.tile_node_ptr, th.tile_node_ptr.data(), th.tile_node_ptr.size() * 4);
     if (rc == CF_OK) rc = put(t.tile_nodes, th.tile_nodes.data(), th.tile_nodes.size() * 4);
     if (rc == CF_OK) rc = put(t.slot_ptr, th.slot_ptr.data(), th.slot_ptr.size() * 4);
-    if (rc == CF_OK) rc = put(t.dst, th.dst_planes.data(), th.dst_planes.size() * 2);
-    if (rc == CF_OK) rc = put(t.lvid, th.lvid_planes.data(), th.lvid_planes.size() * 2);
+    if (rc == CF_OK) rc = put(t.lv, th.lv.data(), th.lv.size() * 4);
+    if (rc == CF_OK) rc = put(t.rk, th.rk.data(), th.rk.size() * 4);
     if (rc == CF_OK) rc = put(t.node_part_ptr, th.node_part_ptr.data(), th.node_part_ptr.size() * 4);
     if (rc == CF_OK) rc = put(t.part_dst, th.part_dst.data(), th.part_dst.size() * 4);
     if (rc == CF_OK) rc = launch_precompute(t, s);
@@ -759,7 +759,7 @@ int cf_batch_add_sim(cf_batch* b, uint32_t mesh_id, const cf_solid_params* p, ui
     d.nb_elem = L.nb_elem; d.nb_node = L.nb_node; d.nb_init = L.nb_init; d.nb_vec = L.nb_vec;
     d.conn = t.conn; d.tile_elem_ptr = t.tile_elem_ptr; d.tile_node_ptr = t.tile_node_ptr;
     d.tile_nodes = t.tile_nodes; d.slot_ptr = t.slot_ptr; d.node_part_ptr = t.node_part_ptr;
-    d.part_dst = t.part_dst; d.dst = t.dst; d.lvid = t.lvid; d.ref = t.ref; d.dinv = t.dinv; d.vol = t.vol;
+    d.part_dst = t.part_dst; d.lv = t.lv; d.rk = t.rk; d.ref = t.ref; d.dinv = t.dinv; d.vol = t.vol;
     float* vecs[8];
     for (int i = 0; i < 8; i++) vecs[i] = at<float>(s.block, L.vec[i]);
     d.x = vecs[0]; d.v = vecs[1]; d.b = vecs[2]; d.y = vecs[3];

@@ -23,8 +23,8 @@ __device__ inline TileArgs tile_args(const SimDev& s, const float* in, uint32_t 
     a.tile_nodes = s.tile_nodes;
     a.part_dst = s.part_dst;
     a.slot_ptr = s.slot_ptr;
-    a.dst = s.dst;
-    a.lvid = s.lvid;
+    a.lv = s.lv;
+    a.rk = s.rk;
     a.R = s.R;
     a.dinv = s.dinv;
     a.vol = s.vol;

@@ -163,7 +163,7 @@ static void tetset_free(cf_tetset* s) {
     TetDevice& t = s->dev;
     dfree(t.ref); dfree(t.conn); dfree(t.dinv); dfree(t.vol);
     dfree(t.tile_elem_ptr); dfree(t.tile_node_ptr); dfree(t.tile_nodes); dfree(t.slot_ptr);
-    dfree(t.dst); dfree(t.lvid); dfree(t.node_part_ptr); dfree(t.part_dst); dfree(t.partials);
+    dfree(t.lv); dfree(t.rk); dfree(t.node_part_ptr); dfree(t.part_dst); dfree(t.partials);
     if (s->stream) (void)hipStreamDestroy(s->stream);
     s->stream = nullptr;
 }
@@ -204,8 +204,8 @@ static int tetset_bake(cf_tetset* s) {
     CF_TRY(upload(&t.tile_node_ptr, th.tile_node_ptr, st));
     CF_TRY(upload(&t.tile_nodes, th.tile_nodes, st));
     CF_TRY(upload(&t.slot_ptr, th.slot_ptr, st));
-    CF_TRY(upload(&t.dst, th.dst_planes, st));
-    CF_TRY(upload(&t.lvid, th.lvid_planes, st));
+    CF_TRY(upload(&t.lv, th.lv, st));
+    CF_TRY(upload(&t.rk, th.rk, st));
     CF_TRY(upload(&t.node_part_ptr, th.node_part_ptr, st));
     CF_TRY(upload(&t.part_dst, th.part_dst, st));
     t.num_tiles = uint32_t(th.tile_elem_ptr.size() - 1);

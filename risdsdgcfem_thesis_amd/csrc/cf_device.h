@@ -289,8 +289,8 @@ struct TileArgs {
     const uint32_t* tile_nodes;
     const uint32_t* part_dst;
     const uint32_t* slot_ptr;
-    const uint16_t* dst;
-    const uint16_t* lvid;
+    const uint32_t* lv;
+    const uint32_t* rk;
     const float* R;
     const float* dinv;
     const float* vol;
@@ -321,7 +321,9 @@ __device__ inline TileDesc tile_desc(const TileArgs& a, uint32_t t) {
 }
 
 // Per-thread registers of one tile: this thread's tile node (vector, rest
-// position, slot offset) and element (local vertex ids, force slots, R).
+// position, slot offset) and element (local vertex ids, its rank in each
+// vertex's slot run, R). The element's force slot at vertex j is the run start
+// of node l[j] (staged in LDS) plus f[j].
 struct TileRegs {
     float v[3], r[3], R[9];
     uint32_t sp, pd;
@@ -340,8 +342,9 @@ __device__ inline void load_tile_regs(const TileArgs& a, const TileDesc& d, uint
     if (tid < d.nn) g.pd = a.part_dst[d.n0 + tid];
     if (tid < d.ne) {
         const size_t e = d.e0 + tid;
-        g.l[0] = a.lvid[e]; g.l[1] = a.lvid[T + e]; g.l[2] = a.lvid[2 * T + e]; g.l[3] = a.lvid[3 * T + e];
-        g.f[0] = a.dst[e]; g.f[1] = a.dst[T + e]; g.f[2] = a.dst[2 * T + e]; g.f[3] = a.dst[3 * T + e];
+        const uint32_t lv = a.lv[e], rk = a.rk[e];
+#pragma unroll
+        for (int j = 0; j < 4; j++) { g.l[j] = (lv >> (8 * j)) & 255u; g.f[j] = (rk >> (8 * j)) & 255u; }
 #pragma unroll
         for (int k = 0; k < 9; k++) g.R[k] = a.R[k * T + e];
     }
@@ -500,7 +503,12 @@ __device__ inline void tile_block(const TileArgs& a, uint32_t t, float4* s_v, fl
     }
     if (tid <= dt.nn) s_sp[tid] = g.sp;
     lds_barrier();
-    if (tid < dt.ne) element_force<MODE>(s_v, s_r, s_F, g.l, g.f, g.R, a.m);
+    if (tid < dt.ne) {
+        uint32_t fs[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) fs[j] = s_sp[g.l[j]] + g.f[j];
+        element_force<MODE>(s_v, s_r, s_F, g.l, fs, g.R, a.m);
+    }
     lds_barrier();
     if (tid < dt.nn) {
         float fx = 0.f, fy = 0.f, fz = 0.f;

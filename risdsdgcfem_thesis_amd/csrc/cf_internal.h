@@ -62,8 +62,8 @@ struct TetDevice {
     uint32_t* tile_node_ptr = nullptr;  // num_tiles+1
     uint32_t* tile_nodes = nullptr;     // num_parts, global node ids
     uint32_t* slot_ptr = nullptr;       // num_parts+1, slot runs (tile slots start at 4*e0)
-    uint16_t* dst = nullptr;            // 4 planes of T: tile-local force slot of (element, vertex)
-    uint16_t* lvid = nullptr;           // 4 planes of T tile-local vertex ids
+    uint32_t* lv = nullptr;             // T: the element's 4 tile-local node ids, one byte each
+    uint32_t* rk = nullptr;             // T: per vertex, the element's rank in that node's slot run (byte each)
     uint32_t* node_part_ptr = nullptr;  // N+1: node n's partials are [ptr[n], ptr[n+1])
     uint32_t* part_dst = nullptr;       // num_parts: partial entry of (tile, tile node), tile-major
     float* partials = nullptr;          // 3 * num_parts
@@ -73,9 +73,8 @@ struct TetDevice {
 struct TileHost {
     std::vector<uint32_t> perm;          // internal element -> caller element
     std::vector<uint32_t> conn_planes;   // 4T
-    std::vector<uint16_t> lvid_planes;   // 4T
+    std::vector<uint32_t> lv, rk;        // T each: 4 bytes per element
     std::vector<uint32_t> tile_elem_ptr, tile_node_ptr, tile_nodes, slot_ptr;
-    std::vector<uint16_t> dst_planes;    // 4T
     std::vector<uint32_t> node_part_ptr, part_dst;
     uint32_t max_tile_elems = 0, max_tile_nodes = 0;
 };
@@ -174,8 +173,8 @@ struct SimDev {
     const uint32_t* slot_ptr;
     const uint32_t* node_part_ptr;
     const uint32_t* part_dst;
-    const uint16_t* dst;
-    const uint16_t* lvid;
+    const uint32_t* lv;
+    const uint32_t* rk;
     const float* ref;
     const float* dinv;
     const float* vol;

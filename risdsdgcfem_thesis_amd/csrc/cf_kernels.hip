@@ -120,7 +120,12 @@ __global__ __launch_bounds__(TB) void k_tile_forces(TileArgs a) {
             gid2 = (tid < d2.nn) ? a.tile_nodes[d2.n0 + tid] : 0u;
         }
         // element forces of tile t into the force slots
-        if (tid < dt.ne) element_force<MODE>(s_v, s_r, s_F, cl, cf, cR, a.m);
+        if (tid < dt.ne) {
+            uint32_t fs[4];
+#pragma unroll
+            for (int q = 0; q < 4; q++) fs[q] = s_sp[cl[q]] + cf[q];
+            element_force<MODE>(s_v, s_r, s_F, cl, fs, cR, a.m);
+        }
         lds_barrier();
         // per tile node: its contiguous slot run (ascending element order) -> one partial
         if (tid < dt.nn) {
@@ -340,7 +345,7 @@ int launch_forces_skip(const TetDevice& t, int mode, const float* in, const floa
     a.T = t.T; a.num_tiles = t.num_tiles; a.max_te = t.max_tile_elems; a.max_tn = t.max_tile_nodes;
     a.tile_elem_ptr = t.tile_elem_ptr; a.tile_node_ptr = t.tile_node_ptr; a.tile_nodes = t.tile_nodes;
     a.part_dst = t.part_dst;
-    a.slot_ptr = t.slot_ptr; a.dst = t.dst; a.lvid = t.lvid; a.R = R; a.dinv = t.dinv; a.vol = t.vol;
+    a.slot_ptr = t.slot_ptr; a.lv = t.lv; a.rk = t.rk; a.R = R; a.dinv = t.dinv; a.vol = t.vol;
     a.ref = t.ref;
     a.in = in; a.partials = t.partials; a.m = m; a.skip = skip;
     const size_t sm = tile_smem(t);

@@ -19,7 +19,8 @@
 namespace cf {
 
 namespace {
-constexpr uint32_t kMaxTileNodes = 1024;
+constexpr uint32_t kMaxTileNodes = 256;  // tile-local node ids are one byte
+constexpr uint32_t kMaxRank = 63;        // elements per node within one tile
 // LDS bytes of one tile in k_tile_forces: node vectors and rest positions
 // (2 x float4), 4 force slots per element (float4 each) and slot offsets
 // (4 B per node).
@@ -80,12 +81,13 @@ int build_tiles(uint32_t N, uint32_t T, const float* ref, const uint32_t* tets, 
     out.tile_node_ptr.assign(1, 0);
     out.tile_nodes.clear();
     out.slot_ptr.assign(1, 0);
-    out.dst_planes.assign(4 * (size_t)T, 0);
     out.conn_planes.assign(4 * (size_t)T, 0);
-    out.lvid_planes.assign(4 * (size_t)T, 0);
+    out.lv.assign(T, 0);
+    out.rk.assign(T, 0);
     out.max_tile_elems = 0;
     out.max_tile_nodes = 0;
     std::vector<int32_t> local(N, -1);
+    std::vector<uint32_t> in_tile(N, 0);  // elements of the current tile at each node
     std::vector<uint32_t> nodes;  // distinct nodes of the current tile
     uint32_t e = 0;
     while (e < T) {
@@ -97,8 +99,14 @@ int build_tiles(uint32_t N, uint32_t T, const float* ref, const uint32_t* tets, 
             for (int j = 0; j < 4; j++) fresh += (local[t[j]] < 0) ? 1u : 0u;
             const uint32_t nn1 = uint32_t(nodes.size()) + fresh;
             if (nn1 > max_nodes || tile_lds(e - e0 + 1, nn1) > kMaxTileLds) break;
-            for (int j = 0; j < 4; j++)
+            // an element's rank in a node's slot run is stored in 6 bits
+            bool full = false;
+            for (int j = 0; j < 4; j++) full |= in_tile[t[j]] >= kMaxRank;
+            if (full) break;
+            for (int j = 0; j < 4; j++) {
                 if (local[t[j]] < 0) { local[t[j]] = 0; nodes.push_back(t[j]); }
+                in_tile[t[j]]++;
+            }
             e++;
         }
         const uint32_t ne = e - e0;
@@ -130,17 +138,22 @@ int build_tiles(uint32_t N, uint32_t T, const float* ref, const uint32_t* tets, 
         for (uint32_t i = 0; i < ne; i++) {
             const size_t ie = e0 + i;
             const uint32_t* t = &tets[4 * (size_t)out.perm[ie]];
+            uint32_t lv = 0, rk = 0;
             for (int j = 0; j < 4; j++) {
                 const uint32_t l = uint32_t(local[t[j]]);
-                out.dst_planes[j * (size_t)T + ie] = uint16_t(fill[l]++);
+                const uint32_t rank = fill[l]++ - cnt[l];  // slot = run start + rank
                 out.conn_planes[j * (size_t)T + ie] = t[j];
-                out.lvid_planes[j * (size_t)T + ie] = uint16_t(l);
+                lv |= l << (8 * j);
+                rk |= rank << (8 * j);
             }
+            out.lv[ie] = lv;
+            out.rk[ie] = rk;
         }
         for (size_t l = 0; l < nodes.size(); l++) {
             out.tile_nodes.push_back(nodes[l]);
             out.slot_ptr.push_back(uint32_t(sbase + cnt[l + 1]));
             local[nodes[l]] = -1;
+            in_tile[nodes[l]] = 0;
         }
         out.tile_elem_ptr.push_back(e);
         out.tile_node_ptr.push_back(uint32_t(out.tile_nodes.size()));
