@@ -296,6 +296,18 @@ int cf_batch_set_state(cf_batch* b, uint32_t sim_id, const float* h_pos, const f
    cf_batch_active_ids (step_ms: device time of the whole batched step). */
 int cf_batch_fixed_step(cf_batch* b, cf_step_stats* stats);
 int cf_batch_active_ids(const cf_batch* b, uint32_t* ids /* num_active */);
+/* Iteration-level batching (the serving loop's engine): queue `steps` more
+   fixed steps for a simulation; a round then runs, in the same launches, the
+   prologue of every simulation starting a step, `iters` CG iterations of every
+   simulation iterating, and the epilogue of every simulation whose solve
+   stopped, which starts its next queued step in the following round. Each
+   step's arithmetic is the same as in cf_batch_fixed_step (bit-identical
+   results). steps_done: NULL or, per active simulation in the order of
+   cf_batch_active_ids, its completed steps since it was added; busy: NULL or
+   the simulations with work left. Do not mix with cf_batch_fixed_step while
+   steps are queued or running. */
+int cf_batch_request_steps(cf_batch* b, uint32_t sim_id, uint32_t steps);
+int cf_batch_round(cf_batch* b, uint32_t iters, uint32_t* steps_done, uint32_t* busy);
 int cf_batch_info_get(const cf_batch* b, cf_batch_info* info);
 /* Mean device ms of one batched K*x over the active simulations (element
    stage + node gather on the current rotations), reps launches. */
@@ -346,14 +358,16 @@ int cf_solid2_set_state(cf_solid2* s, const float* h_pos2, const float* h_vel2);
  * reference steps the leaves of one scene; this serves many independent
  * leaves). Requests arrive at given times; each asks for `steps` fixed steps
  * of a solid on one of the given meshes. The loop admits waiting requests in
- * arrival order while their bytes fit the HBM budget and the tets stepped per
- * tick stay under max_tick_tets, keeps their state on the device, and steps
- * every admitted request once per tick in one batched step (continuous
- * batching: a request admitted while others run joins the next tick; a
- * finished one leaves and frees its memory). Meshes are uploaded on first use
- * and evicted, least recently used first, when unused and memory is needed.
- * A step's latency runs from the moment it was requested (arrival for the
- * first step, the previous step's result for the others) to its result. */
+ * arrival order while their bytes fit the HBM budget and the tets in flight
+ * stay under max_tick_tets, keeps their state on the device, and runs rounds
+ * (cf_batch_round with cg_chunk iterations): every round advances every
+ * admitted request, and a step completes in the round its own CG stops
+ * (continuous batching at the iteration level: a request admitted while others
+ * run joins the next round; a finished one leaves and frees its memory).
+ * Meshes are uploaded on first use and evicted, least recently used first,
+ * when unused and memory is needed. A step's latency runs from the moment it
+ * was requested (arrival for the first step, the previous step's result for
+ * the others) to the end of the round that completes it. */
 typedef struct cf_serve_mesh {
     uint32_t n_nodes;
     const float* ref_pos;

@@ -162,6 +162,8 @@ SIGNATURES = {
     "cf_batch_set_state": (C.c_int, [_vp, C.c_uint32, _f, _f]),
     "cf_batch_fixed_step": (C.c_int, [_vp, C.POINTER(StepStats)]),
     "cf_batch_active_ids": (C.c_int, [_vp, _u]),
+    "cf_batch_request_steps": (C.c_int, [_vp, C.c_uint32, C.c_uint32]),
+    "cf_batch_round": (C.c_int, [_vp, C.c_uint32, _u, _u]),
     "cf_batch_info_get": (C.c_int, [_vp, C.POINTER(BatchInfo)]),
     "cf_batch_time_kx": (C.c_int, [_vp, C.c_uint32, C.POINTER(C.c_double)]),
     "cf_batch_synchronize": (C.c_int, [_vp]),

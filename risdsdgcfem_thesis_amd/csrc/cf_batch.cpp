@@ -181,8 +181,12 @@ struct cf_batch {
     uint32_t max_iter = 0;
     uint64_t active_tets = 0, active_nodes = 0;
     // graphs
-    hipGraphExec_t g_pro = nullptr, g_chunk = nullptr, g_epi = nullptr;
-    uint32_t g_chunk_len = 0;
+    hipGraphExec_t g_pro = nullptr, g_chunk = nullptr, g_epi = nullptr, g_round = nullptr;
+    uint32_t g_chunk_len = 0, g_round_len = 0;
+    // steps queued for the next round: (slot, steps)
+    std::vector<uint2> pending;
+    uint2* d_arm = nullptr;
+    size_t arm_cap = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr, evc[2] = {nullptr, nullptr};
     std::vector<SimCg> h_cgs;
 };
@@ -193,7 +197,8 @@ void free_graphs(cf_batch* b) {
     if (b->g_pro) (void)hipGraphExecDestroy(b->g_pro);
     if (b->g_chunk) (void)hipGraphExecDestroy(b->g_chunk);
     if (b->g_epi) (void)hipGraphExecDestroy(b->g_epi);
-    b->g_pro = b->g_chunk = b->g_epi = nullptr;
+    if (b->g_round) (void)hipGraphExecDestroy(b->g_round);
+    b->g_pro = b->g_chunk = b->g_epi = b->g_round = nullptr;
 }
 
 template <class T>
@@ -461,6 +466,7 @@ int rebuild_tables(cf_batch* b) {
     t.cgs = b->d_cgs;
     table_views(L, b->d_tab, b->d_active, t);
     t.counter = b->d_counter;
+    t.busy = b->d_counter + 1;
     t.max_tn = L.max_tn;
     t.max_te = L.max_te;
     t.tile_threads = b->tile_threads;
@@ -496,9 +502,10 @@ int compact_tables(cf_batch* b, const SimCg* cgs, BatchTables& out, uint32_t* n_
     return CF_OK;
 }
 
-int enqueue_prologue(cf_batch* b) {
+int enqueue_prologue(cf_batch* b, bool arm_all) {
     const BatchTables& t = b->tab;
     hipStream_t s = b->stream;
+    if (arm_all) CF_TRY(launch_batch_arm_all(t, s));      // every active simulation starts a step
     CF_TRY(launch_batch_begin_eval(t, s));               // BeginEvaluation (FixedStep :927)
     CF_TRY(launch_batch_forces(t, kModeFe, kSelY, s));    // f_e(x)
     CF_TRY(launch_batch_gather_rhs(t, s));                // b, y0, KNC (:1209-1295)
@@ -529,9 +536,22 @@ int enqueue_epilogue(cf_batch* b) {
     return launch_batch_finish(t, s);
 }
 
+// One round of iteration-level batching: starting simulations run their
+// prologue, iterating ones `iters` CG iterations, those whose solve stopped
+// their epilogue (and start their next queued step in the following round).
+int enqueue_round(cf_batch* b, uint32_t iters) {
+    CF_TRY(enqueue_prologue(b, false));
+    CF_TRY(enqueue_chunk(b, iters, b->tab));
+    CF_TRY(enqueue_epilogue(b));
+    return launch_batch_count_busy(b->tab, b->stream);
+}
+
 int record(cf_batch* b, int which, uint32_t iters, hipGraphExec_t* exec) {
     CF_CHECK_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-    int rc = which == 0 ? enqueue_prologue(b) : which == 1 ? enqueue_chunk(b, iters, b->tab) : enqueue_epilogue(b);
+    int rc = which == 0   ? enqueue_prologue(b, true)
+             : which == 1 ? enqueue_chunk(b, iters, b->tab)
+             : which == 2 ? enqueue_epilogue(b)
+                          : enqueue_round(b, iters);
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(b->stream, &g);
     if (rc != CF_OK) {
@@ -551,9 +571,9 @@ int record(cf_batch* b, int which, uint32_t iters, hipGraphExec_t* exec) {
     return CF_OK;
 }
 
-int prepare(cf_batch* b) {
+int prepare(cf_batch* b, bool step_graphs = false) {
     if (b->dirty) CF_TRY(rebuild_tables(b));
-    if (b->use_graphs && b->tab.n_active) {
+    if (step_graphs && b->use_graphs && b->tab.n_active) {
         const uint32_t chunk = std::min(b->cg_chunk, std::max<uint32_t>(1, b->max_iter));
         if (b->g_chunk && b->g_chunk_len != chunk) free_graphs(b);
         if (!b->g_pro) CF_TRY(record(b, 0, 0, &b->g_pro));
@@ -634,6 +654,7 @@ int cf_batch_destroy(cf_batch* b) {
     for (auto& p : b->h_cgs_pin)
         if (p) (void)hipHostFree(p);
     if (b->d_tab_cg) (void)hipFree(b->d_tab_cg);
+    if (b->d_arm) (void)hipFree(b->d_arm);
     if (b->d_active_cg) (void)hipFree(b->d_active_cg);
     if (b->ev0) (void)hipEventDestroy(b->ev0);
     if (b->ev1) (void)hipEventDestroy(b->ev1);
@@ -796,6 +817,7 @@ int cf_batch_add_sim(cf_batch* b, uint32_t mesh_id, const cf_solid_params* p, ui
     chk(hipMemcpyAsync(d.knc_target, tgt.data(), n3 * 4, hipMemcpyHostToDevice, st));
     chk(hipMemcpyAsync(d.R, Rid.data(), Rid.size() * 4, hipMemcpyHostToDevice, st));
     chk(hipMemcpyAsync(b->d_sims + slot, &d, sizeof(SimDev), hipMemcpyHostToDevice, st));
+    chk(hipMemsetAsync(b->d_cgs + slot, 0, sizeof(SimCg), st));  // idle, no steps done
     chk(hipStreamSynchronize(st));
     if (rc != CF_OK) {
         b->arena.free(s.block);
@@ -821,6 +843,10 @@ int cf_batch_remove_sim(cf_batch* b, uint32_t id) {
     b->used -= it->second.bytes;
     b->meshes[it->second.mesh].users--;
     b->free_slots.push_back(it->second.slot);
+    const uint32_t slot = it->second.slot;
+    b->pending.erase(std::remove_if(b->pending.begin(), b->pending.end(),
+                                    [slot](const uint2& p) { return p.x == slot; }),
+                     b->pending.end());
     if (it->second.active) b->dirty = true;
     b->sims.erase(it);
     return CF_OK;
@@ -859,13 +885,13 @@ int cf_batch_set_state(cf_batch* b, uint32_t id, const float* pos, const float* 
 
 int cf_batch_fixed_step(cf_batch* b, cf_step_stats* stats) {
     if (!b) return CF_ERR_INVALID_ARG;
-    CF_TRY(prepare(b));
+    CF_TRY(prepare(b, true));
     const BatchTables& t = b->tab;
     if (!t.n_active) return CF_OK;
     hipStream_t s = b->stream;
     CF_CHECK_HIP(hipEventRecord(b->ev0, s));
     if (b->use_graphs) CF_CHECK_HIP(hipGraphLaunch(b->g_pro, s));
-    else CF_TRY(enqueue_prologue(b));
+    else CF_TRY(enqueue_prologue(b, true));
     // CG chunks: the counter and the CG scalars written by each chunk are read
     // while the next one runs. Once the simulations still iterating are at
     // most half of those in the tables, the remaining chunks run on compacted
@@ -920,6 +946,51 @@ int cf_batch_fixed_step(cf_batch* b, cf_step_stats* stats) {
     return CF_OK;
 }
 
+int cf_batch_request_steps(cf_batch* b, uint32_t id, uint32_t steps) {
+    if (!b) return CF_ERR_INVALID_ARG;
+    auto it = b->sims.find(id);
+    if (it == b->sims.end()) { set_error("no simulation %u", id); return CF_ERR_INVALID_ARG; }
+    if (steps) b->pending.push_back(make_uint2(it->second.slot, steps));
+    return CF_OK;
+}
+
+int cf_batch_round(cf_batch* b, uint32_t iters, uint32_t* steps_done, uint32_t* busy) {
+    if (!b || iters == 0 || iters > kMaxCgIter) return CF_ERR_INVALID_ARG;
+    CF_TRY(prepare(b));
+    const BatchTables& t = b->tab;
+    hipStream_t s = b->stream;
+    if (!b->pending.empty()) {
+        CF_TRY(ensure_device(reinterpret_cast<void**>(&b->d_arm), &b->arm_cap, b->pending.size(), sizeof(uint2)));
+        CF_CHECK_HIP(hipMemcpyAsync(b->d_arm, b->pending.data(), b->pending.size() * sizeof(uint2),
+                                    hipMemcpyHostToDevice, s));
+        CF_TRY(launch_batch_arm(t, b->d_arm, uint32_t(b->pending.size()), s));
+        CF_CHECK_HIP(hipStreamSynchronize(s));  // the host list is reused
+        b->pending.clear();
+    }
+    if (t.n_active) {
+        if (b->use_graphs) {
+            if (b->g_round && b->g_round_len != iters) {
+                (void)hipGraphExecDestroy(b->g_round);
+                b->g_round = nullptr;
+            }
+            if (!b->g_round) CF_TRY(record(b, 3, iters, &b->g_round));
+            b->g_round_len = iters;
+            CF_CHECK_HIP(hipGraphLaunch(b->g_round, s));
+        } else {
+            CF_TRY(enqueue_round(b, iters));
+        }
+    }
+    const size_t cg_bytes = b->slots_used * sizeof(SimCg);
+    CF_CHECK_HIP(hipMemcpyAsync(&b->h_counter[2], b->d_counter + 1, 4, hipMemcpyDeviceToHost, s));
+    CF_CHECK_HIP(hipMemcpyAsync(b->h_cgs_pin[0], b->d_cgs, cg_bytes, hipMemcpyDeviceToHost, s));
+    CF_CHECK_HIP(hipStreamSynchronize(s));
+    if (steps_done)
+        for (size_t i = 0; i < b->active_ids.size(); i++)
+            steps_done[i] = b->h_cgs_pin[0][b->sims[b->active_ids[i]].slot].steps_done;
+    if (busy) *busy = t.n_active ? b->h_counter[2] : 0;
+    return CF_OK;
+}
+
 int cf_batch_active_ids(const cf_batch* b, uint32_t* ids) {
     if (!b || !ids) return CF_ERR_INVALID_ARG;
     CF_TRY(prepare(const_cast<cf_batch*>(b)));
@@ -950,12 +1021,12 @@ int cf_batch_time_kx(cf_batch* b, uint32_t reps, double* ms) {
     CF_TRY(prepare(b));
     const BatchTables& t = b->tab;
     hipStream_t s = b->stream;
-    CF_TRY(launch_batch_forces(t, kModeDfx, kSelY, s));
-    CF_TRY(launch_batch_gather_matvec(t, kSelY, s));
+    CF_TRY(launch_batch_forces(t, kModeDfx, kSelYAll, s));
+    CF_TRY(launch_batch_gather_matvec(t, kSelYAll, s));
     CF_CHECK_HIP(hipEventRecord(b->ev0, s));
     for (uint32_t i = 0; i < reps; i++) {
-        CF_TRY(launch_batch_forces(t, kModeDfx, kSelY, s));
-        CF_TRY(launch_batch_gather_matvec(t, kSelY, s));
+        CF_TRY(launch_batch_forces(t, kModeDfx, kSelYAll, s));
+        CF_TRY(launch_batch_gather_matvec(t, kSelYAll, s));
     }
     CF_CHECK_HIP(hipEventRecord(b->ev1, s));
     CF_CHECK_HIP(hipEventSynchronize(b->ev1));

@@ -65,29 +65,29 @@ __global__ __launch_bounds__(kBlock) void kb_begin_eval(BatchTables t) {
     __shared__ double sh[kBlock / 64];
     __shared__ float shm[kBlock / 64];
     __shared__ uint32_t shc[kBlock / 64], shf[kBlock / 64];
-    if (blockIdx.x == 0 && threadIdx.x == 0) *t.counter = 0;
+    if (blockIdx.x == 0 && threadIdx.x == 0) { *t.counter = 0; *t.busy = 0; }
     const BlockRef br = t.elem[blockIdx.x];
-    if (br.slot == kIdleSlot) return;
+    if (br.slot == kIdleSlot || t.cgs[br.slot].phase != kPhaseStart) return;
     const SimDev& s = t.sims[br.slot];
     begin_eval_block(s.T, s.conn, s.x, s.dinv, s.rm, s.thr, s.R, s.detF, s.eval_count, s.eval_sum, s.eval_min,
                      br.blk, s.nb_elem, sh, shm, shc, shf);
 }
 
 // Element stage over every tile of the batch: f_e on x, or df_x on y (the
-// warm start, before the solve) or on d (inside the solve; skipped by
-// simulations whose solve has stopped).
+// warm start, before the solve) for starting simulations, or df_x on d for
+// iterating ones; kSelYAll: df_x on y for every simulation.
 template <int MODE, int TB>
 __global__ __launch_bounds__(TB) void kb_tiles(BatchTables t, int sel) {
     const BlockRef br = t.tile[blockIdx.x];
     if (br.slot == kIdleSlot) return;
-    if (MODE == kModeDfx && sel == kSelD && t.cgs[br.slot].done) return;
+    if (sel != kSelYAll && t.cgs[br.slot].phase != (sel == kSelD ? kPhaseIter : kPhaseStart)) return;
     extern __shared__ float4 smem4[];
     float4* s_v = smem4;
     float4* s_r = s_v + t.max_tn;
     float4* s_F = s_r + t.max_tn;
     uint32_t* s_sp = reinterpret_cast<uint32_t*>(s_F + 4 * t.max_te);
     const SimDev& s = t.sims[br.slot];
-    const float* in = (MODE == kModeFe) ? s.x : (sel == kSelY ? s.y : s.d);
+    const float* in = (MODE == kModeFe) ? s.x : (sel == kSelD ? s.d : s.y);
     const TileArgs a = tile_args(s, in, t.max_tn, t.max_te);
     tile_block<MODE>(a, br.blk, s_v, s_r, s_F, s_sp);
 }
@@ -97,7 +97,7 @@ __global__ __launch_bounds__(TB) void kb_tiles(BatchTables t, int sel) {
 __global__ __launch_bounds__(kBlock) void kb_gather_rhs(BatchTables t) {
     __shared__ double sh[kBlock / 64];
     const BlockRef br = t.node[blockIdx.x];
-    if (br.slot == kIdleSlot) return;
+    if (br.slot == kIdleSlot || t.cgs[br.slot].phase != kPhaseStart) return;
     const SimDev& s = t.sims[br.slot];
     NodeTerms nt = node_terms(s);
     nt.out = s.b;
@@ -112,10 +112,10 @@ __global__ __launch_bounds__(kBlock) void kb_gather_matvec(BatchTables t, int se
     __shared__ double sh[kBlock / 64];
     const BlockRef br = t.node[blockIdx.x];
     if (br.slot == kIdleSlot) return;
-    if (sel == kSelD && t.cgs[br.slot].done) return;
+    if (sel != kSelYAll && t.cgs[br.slot].phase != (sel == kSelD ? kPhaseIter : kPhaseStart)) return;
     const SimDev& s = t.sims[br.slot];
     NodeTerms nt = node_terms(s);
-    nt.d = (sel == kSelY) ? s.y : s.d;
+    nt.d = (sel == kSelD) ? s.d : s.y;
     nt.out = s.q;
     nt.partials = s.part_a;
     gather_block<kGatherMatvec>(s.N, s.node_part_ptr, s.partials, nt, br.blk, s.nb_node, sh);
@@ -125,14 +125,14 @@ __global__ __launch_bounds__(kBlock) void kb_gather_matvec(BatchTables t, int se
 __global__ __launch_bounds__(kBlock) void kb_cg_init(BatchTables t) {
     __shared__ double sh[kBlock / 64];
     const BlockRef br = t.init[blockIdx.x];
-    if (br.slot == kIdleSlot) return;
+    if (br.slot == kIdleSlot || t.cgs[br.slot].phase != kPhaseStart) return;
     const SimDev& s = t.sims[br.slot];
     cg_init_block(3 * s.N, s.q, s.b, s.r, s.d, s.knc_mask, s.part_b, br.blk, s.nb_init, sh);
 }
 
 __global__ __launch_bounds__(kBlock) void kb_cg_start(BatchTables t) {
     const uint32_t slot = wave_slot(t);
-    if (slot == kIdleSlot) return;
+    if (slot == kIdleSlot || t.cgs[slot].phase != kPhaseStart) return;
     const SimDev& s = t.sims[slot];
     const double d0 = wave_reduce_partials(s.part_b, s.nb_init);
     if ((threadIdx.x & 63) == 0) {
@@ -154,6 +154,7 @@ __global__ __launch_bounds__(kBlock) void kb_cg_start(BatchTables t) {
         else if (!(0u < s.max_iter)) reason = CF_CG_MAX_ITER;
         c.done = reason != 0;
         c.reason = reason;
+        c.phase = reason ? kPhaseEnd : kPhaseIter;
         if (!reason) atomicAdd(t.counter, 1u);
     }
 }
@@ -164,7 +165,7 @@ __global__ __launch_bounds__(kBlock) void kb_cg_alpha(BatchTables t) {
     const uint32_t slot = wave_slot(t);
     if (slot == kIdleSlot) return;
     SimCg& c = t.cgs[slot];
-    if (c.done) return;
+    if (c.phase != kPhaseIter) return;
     const SimDev& s = t.sims[slot];
     const double dq = wave_reduce_partials(s.part_a, s.nb_node);
     if ((threadIdx.x & 63) == 0) {
@@ -172,6 +173,7 @@ __global__ __launch_bounds__(kBlock) void kb_cg_alpha(BatchTables t) {
         if (fabs(dq) < c.abs_sq) {
             c.reason = CF_CG_BREAKDOWN;
             c.done = 2;
+            c.phase = kPhaseEnd;
         } else {
             c.alpha = (float)(c.delta / dq);
         }
@@ -183,7 +185,7 @@ __global__ __launch_bounds__(kBlock) void kb_update_xr(BatchTables t) {
     const BlockRef br = t.vec[blockIdx.x];
     if (br.slot == kIdleSlot) return;
     const SimCg& c = t.cgs[br.slot];
-    if (c.done) return;
+    if (c.phase != kPhaseIter) return;
     const SimDev& s = t.sims[br.slot];
     update_xr_block<true>(3 * s.N, c.alpha, s.y, s.r, s.d, s.q, s.part_b, br.blk, s.nb_vec, sh);
 }
@@ -193,7 +195,7 @@ __global__ __launch_bounds__(kBlock) void kb_cg_beta(BatchTables t) {
     const uint32_t slot = wave_slot(t);
     if (slot == kIdleSlot) return;
     SimCg& c = t.cgs[slot];
-    if (c.done) return;
+    if (c.phase != kPhaseIter) return;
     const SimDev& s = t.sims[slot];
     const double dn = wave_reduce_partials(s.part_b, s.nb_vec);
     if ((threadIdx.x & 63) == 0) {
@@ -207,7 +209,8 @@ __global__ __launch_bounds__(kBlock) void kb_cg_beta(BatchTables t) {
         else if (!(k1 < c.max_iter)) reason = CF_CG_MAX_ITER;
         c.reason = reason;
         c.done = reason != 0;
-        if (!reason) atomicAdd(t.counter, 1u);
+        if (reason) c.phase = kPhaseEnd;
+        else atomicAdd(t.counter, 1u);
     }
 }
 
@@ -215,7 +218,7 @@ __global__ __launch_bounds__(kBlock) void kb_update_d(BatchTables t) {
     const BlockRef br = t.vec[blockIdx.x];
     if (br.slot == kIdleSlot) return;
     const SimCg& c = t.cgs[br.slot];
-    if (c.done) return;
+    if (c.phase != kPhaseIter) return;
     const SimDev& s = t.sims[br.slot];
     update_d_block<true>(3 * s.N, c.beta, s.d, s.r, s.knc_mask, br.blk, s.nb_vec);
 }
@@ -223,7 +226,7 @@ __global__ __launch_bounds__(kBlock) void kb_update_d(BatchTables t) {
 // ------------------------------------------------------------ step epilogue
 __global__ __launch_bounds__(kBlock) void kb_integrate(BatchTables t) {
     const BlockRef br = t.init[blockIdx.x];
-    if (br.slot == kIdleSlot) return;
+    if (br.slot == kIdleSlot || t.cgs[br.slot].phase != kPhaseEnd) return;
     const SimDev& s = t.sims[br.slot];
     integrate_block(3 * s.N, s.x, s.v, s.y, s.dt, s.drag, br.blk, s.nb_init);
 }
@@ -231,7 +234,7 @@ __global__ __launch_bounds__(kBlock) void kb_integrate(BatchTables t) {
 __global__ __launch_bounds__(kBlock) void kb_energy(BatchTables t) {
     __shared__ double sh[kBlock / 64];
     const BlockRef br = t.elem[blockIdx.x];
-    if (br.slot == kIdleSlot) return;
+    if (br.slot == kIdleSlot || t.cgs[br.slot].phase != kPhaseEnd) return;
     const SimDev& s = t.sims[br.slot];
     if (!s.compute_energy) return;
     energy_block(s.T, s.conn, s.ref, s.x, s.R, s.dinv, s.vol, s.m, s.part_e, br.blk, s.nb_elem, sh);
@@ -244,6 +247,7 @@ __global__ __launch_bounds__(kBlock) void kb_finish(BatchTables t) {
     if (slot == kIdleSlot || (threadIdx.x & 63) != 0) return;
     const SimDev& s = t.sims[slot];
     SimCg& c = t.cgs[slot];
+    if (c.phase != kPhaseEnd) return;
     uint32_t nd = 0, nf = 0;
     double sd = 0.0, en = 0.0;
     float mn = 3e38f;
@@ -259,6 +263,41 @@ __global__ __launch_bounds__(kBlock) void kb_finish(BatchTables t) {
     c.sum_degenerate = sd;
     c.min_det = mn;
     c.energy = en;
+    c.steps_done++;
+    if (c.steps_left) {
+        c.steps_left--;
+        c.phase = kPhaseStart;
+    } else {
+        c.phase = kPhaseIdle;
+    }
+}
+
+// Every active simulation starts one step (cf_batch_fixed_step).
+__global__ void kb_arm_all(BatchTables t) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= t.n_active) return;
+    SimCg& c = t.cgs[t.active[i]];
+    c.phase = kPhaseStart;
+    c.steps_left = 0;
+}
+
+// Queued steps: an idle simulation starts its first one at once.
+__global__ void kb_arm(BatchTables t, const uint2* list, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || list[i].y == 0) return;
+    SimCg& c = t.cgs[list[i].x];
+    if (c.phase == kPhaseIdle) {
+        c.phase = kPhaseStart;
+        c.steps_left += list[i].y - 1;
+    } else {
+        c.steps_left += list[i].y;
+    }
+}
+
+// Simulations with work left after a round.
+__global__ void kb_count_busy(BatchTables t) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < t.n_active && t.cgs[t.active[i]].phase != kPhaseIdle) atomicAdd(t.busy, 1u);
 }
 
 __global__ void kb_fill_u32(uint32_t* p, uint32_t value, uint32_t n) {
@@ -371,6 +410,24 @@ int launch_batch_energy(const BatchTables& t, hipStream_t s) {
 int launch_batch_finish(const BatchTables& t, hipStream_t s) {
     if (!t.n_active) return CF_OK;
     hipLaunchKernelGGL(kb_finish, dim3(mesh_grid(t)), dim3(kBlock), 0, s, t);
+    return check_launch_b();
+}
+
+int launch_batch_arm_all(const BatchTables& t, hipStream_t s) {
+    if (!t.n_active) return CF_OK;
+    hipLaunchKernelGGL(kb_arm_all, dim3((t.n_active + 255) / 256), dim3(256), 0, s, t);
+    return check_launch_b();
+}
+
+int launch_batch_arm(const BatchTables& t, const uint2* list, uint32_t n, hipStream_t s) {
+    if (!n) return CF_OK;
+    hipLaunchKernelGGL(kb_arm, dim3((n + 255) / 256), dim3(256), 0, s, t, list, n);
+    return check_launch_b();
+}
+
+int launch_batch_count_busy(const BatchTables& t, hipStream_t s) {
+    if (!t.n_active) return CF_OK;
+    hipLaunchKernelGGL(kb_count_busy, dim3((t.n_active + 255) / 256), dim3(256), 0, s, t);
     return check_launch_b();
 }
 

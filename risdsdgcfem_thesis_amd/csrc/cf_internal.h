@@ -195,7 +195,11 @@ struct SimDev {
     uint32_t max_iter;
 };
 
-// CG scalars of one simulation for the current solve, plus its step results.
+// CG scalars of one simulation for the current solve, its step results, and
+// where it is in its step: idle, starting (prologue due), iterating (CG), or
+// ending (epilogue due). Every batched kernel only touches simulations in its
+// own phase, so steps of different simulations may overlap in one round.
+enum SimPhase { kPhaseIdle = 0, kPhaseStart = 1, kPhaseIter = 2, kPhaseEnd = 3 };
 struct SimCg {
     double delta, delta0, target, abs_sq, dq;
     float alpha, beta;
@@ -204,7 +208,9 @@ struct SimCg {
     double energy, sum_degenerate;
     uint32_t num_degenerate, num_fallback;
     float min_det;
-    uint32_t pad;
+    int32_t phase;
+    uint32_t steps_left;   // steps to start after the current one
+    uint32_t steps_done;   // steps completed since the simulation was added
 };
 
 struct BlockRef { uint32_t slot, blk; };  // slot 0xffffffff: idle block
@@ -220,10 +226,13 @@ struct BatchTables {
     const BlockRef* vec; uint32_t n_vec;
     const uint32_t* active; uint32_t n_active;  // slots of the stepped simulations
     uint32_t* counter;                          // simulations still iterating
+    uint32_t* busy;                             // simulations not idle after a round
     uint32_t max_tn, max_te, tile_threads;
 };
 
-enum BatchVec { kSelY = 0, kSelD = 1 };
+// element-stage input: y in the prologue, d in the CG loop, y for every
+// simulation regardless of phase (K*x timing)
+enum BatchVec { kSelY = 0, kSelD = 1, kSelYAll = 2 };
 int launch_batch_begin_eval(const BatchTables& t, hipStream_t s);
 int launch_batch_forces(const BatchTables& t, int mode, int sel, hipStream_t s);
 int launch_batch_gather_rhs(const BatchTables& t, hipStream_t s);
@@ -238,6 +247,11 @@ int launch_batch_integrate(const BatchTables& t, hipStream_t s);
 int launch_batch_energy(const BatchTables& t, hipStream_t s);
 int launch_batch_finish(const BatchTables& t, hipStream_t s);
 int launch_batch_fill_u32(uint32_t* p, uint32_t value, uint32_t n, hipStream_t s);
+// every active simulation: start one step now (phase start, no further steps)
+int launch_batch_arm_all(const BatchTables& t, hipStream_t s);
+// (slot, steps) pairs: queue steps; an idle simulation starts at once
+int launch_batch_arm(const BatchTables& t, const uint2* list, uint32_t n, hipStream_t s);
+int launch_batch_count_busy(const BatchTables& t, hipStream_t s);
 
 // ---------------------------------------------------------------- 2D resident step
 // A 2D mesh small enough to live in one workgroup is stepped entirely inside

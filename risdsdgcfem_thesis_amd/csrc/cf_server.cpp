@@ -1,7 +1,10 @@
 // Real-time serving loop over a batch (cf_serve_trace in include/corofem.h):
-// arrivals, FIFO admission by HBM budget and per-tick tets cap, one batched
-// fixed step per tick for every admitted request, departures that free their
-// memory, lazily uploaded and LRU-evicted meshes, per-step latencies.
+// arrivals, FIFO admission by HBM budget and a cap on the tets in flight,
+// iteration-level batching (every round runs, in the same launches, the step
+// prologues, a few CG iterations and the step epilogues of all admitted
+// requests, so each step completes in the round its own solve stops),
+// departures that free their memory, lazily uploaded and LRU-evicted meshes,
+// per-step latencies.
 #include "cf_internal.h"
 
 #include <algorithm>
@@ -10,6 +13,7 @@
 #include <deque>
 #include <map>
 #include <thread>
+#include <unordered_map>
 #include <vector>
 
 namespace {
@@ -78,6 +82,7 @@ extern "C" int cf_serve_trace(const cf_serve_mesh* meshes, uint32_t n_meshes, co
     std::deque<uint32_t> waiting;
     std::vector<Live> live;
     std::vector<double> lat, first;
+    std::vector<uint32_t> ids, done;
     lat.reserve(1024);
     uint64_t live_tets = 0;
     uint32_t next = 0;
@@ -117,6 +122,7 @@ extern "C" int cf_serve_trace(const cf_serve_mesh* meshes, uint32_t n_meshes, co
                 if (!evict_one(b, cache, q.mesh)) break;
             if (rc == CF_ERR_CAPACITY) break;
             if (rc != CF_OK) return rc;
+            CF_TRY(cf_batch_request_steps(b, sim, q.steps));
             it->second.users++;
             live.push_back(Live{waiting.front(), sim, 0, q.arrival_s});
             live_tets += m.n_tets;
@@ -137,25 +143,35 @@ extern "C" int cf_serve_trace(const cf_serve_mesh* meshes, uint32_t n_meshes, co
             continue;
         }
         if (!cfg->drain && t >= cfg->duration_s) break;
-        // one tick: every admitted request takes one step
+        // one round: prologues, cfg->cg_chunk CG iterations and epilogues of
+        // every admitted request; a request's step completes in the round its
+        // own solve stops
         cf_batch_info info{};
         CF_TRY(cf_batch_info_get(b, &info));
         r.peak_bytes = std::max<uint64_t>(r.peak_bytes, info.bytes_in_use);
         const double ts = now();
-        CF_TRY(cf_batch_fixed_step(b, nullptr));
+        uint32_t busy = 0;
+        ids.resize(info.num_active);
+        done.resize(info.num_active);
+        CF_TRY(cf_batch_round(b, cfg->cg_chunk ? cfg->cg_chunk : 8, done.data(), &busy));
+        CF_TRY(cf_batch_active_ids(b, ids.data()));
         const double te = now();
         tick_total += te - ts;
         r.ticks++;
+        std::unordered_map<uint32_t, uint32_t> done_of;
+        for (size_t i = 0; i < ids.size(); i++) done_of[ids[i]] = done[i];
         std::vector<Live> keep;
         keep.reserve(live.size());
         for (Live& l : live) {
             const cf_serve_request& q = reqs[l.req];
-            lat.push_back((te - l.enqueued) * 1e3);
-            if (l.done == 0) first.push_back((te - q.arrival_s) * 1e3);
-            l.done++;
-            l.enqueued = te;
-            r.steps++;
-            r.tet_steps += meshes[q.mesh].n_tets;
+            const uint32_t now_done = done_of[l.sim];
+            for (; l.done < now_done; l.done++) {
+                lat.push_back((te - l.enqueued) * 1e3);
+                if (l.done == 0) first.push_back((te - q.arrival_s) * 1e3);
+                l.enqueued = te;
+                r.steps++;
+                r.tet_steps += meshes[q.mesh].n_tets;
+            }
             if (l.done < q.steps) {
                 keep.push_back(l);
                 continue;

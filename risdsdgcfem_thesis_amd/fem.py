@@ -449,6 +449,17 @@ class Batch:
         check(lib.cf_batch_fixed_step(self.h, st), "cf_batch_fixed_step")
         return {sid: st[i] for i, sid in enumerate(ids)}
 
+    def request_steps(self, sim_id, steps):
+        check(lib.cf_batch_request_steps(self.h, int(sim_id), int(steps)), "cf_batch_request_steps")
+
+    def round(self, iters=8):
+        """One round of iteration-level batching; returns ({sim_id: steps done}, busy)."""
+        ids = self.active_ids()
+        done = np.zeros(max(len(ids), 1), np.uint32)
+        busy = C.c_uint32()
+        check(lib.cf_batch_round(self.h, int(iters), _up(done), C.byref(busy)), "cf_batch_round")
+        return {sid: int(done[i]) for i, sid in enumerate(ids)}, busy.value
+
     def state(self, sim_id, n_nodes):
         x = np.empty(3 * n_nodes, np.float32)
         v = np.empty(3 * n_nodes, np.float32)
