@@ -267,7 +267,7 @@ def main():
     ap.add_argument("--kx-reps", type=int, default=50)
     ap.add_argument("--tile-elems", type=int, default=256)
     ap.add_argument("--tile-threads", type=int, default=256)
-    ap.add_argument("--tile-schedule", type=int, default=2)
+    ap.add_argument("--tile-schedule", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--workload", default="all", choices=["all", "single", "batch3d", "batch2d", "serve"])
     ap.add_argument("--batch-count", type=int, default=1024)

@@ -232,9 +232,9 @@ int cf_set_tile_elems(uint32_t tile_elems);
 /* Workgroup size of the LDS-tile element kernel: 128, 256, 512 or 1024 (0 = default
    256). A tile holds at most this many elements and nodes. */
 int cf_set_tile_threads(uint32_t threads);
-/* Tile schedule. 0: one tile per workgroup; 1: persistent grid of resident
-   workgroups that walk their tiles with the next tile's loads in flight;
-   2 (default): the pipelined kernel of 1 with one tile per workgroup. */
+/* Tile schedule. 0 (default): one tile per workgroup; 1: persistent grid of
+   resident workgroups that walk their tiles with the next tile's loads in
+   flight; 2: the pipelined kernel of 1 with one tile per workgroup. */
 int cf_set_tile_persistent(int persistent);
 int cf_get_tile_elems(uint32_t* tile_elems);
 

@@ -81,11 +81,11 @@ __global__ __launch_bounds__(TB) void kb_tiles(BatchTables t, int sel) {
     const BlockRef br = t.tile[blockIdx.x];
     if (br.slot == kIdleSlot) return;
     if (sel != kSelYAll && t.cgs[br.slot].phase != (sel == kSelD ? kPhaseIter : kPhaseStart)) return;
-    extern __shared__ float4 smem4[];
-    float4* s_v = smem4;
-    float4* s_r = s_v + t.max_tn;
-    float4* s_F = s_r + t.max_tn;
-    uint32_t* s_sp = reinterpret_cast<uint32_t*>(s_F + 4 * t.max_te);
+    extern __shared__ float smem_f[];
+    const Slots s_v = make_slots(smem_f, t.max_tn);
+    const Slots s_r = make_slots(smem_f + 3 * t.max_tn, t.max_tn);
+    const Slots s_F = make_slots(smem_f + 6 * t.max_tn, 4 * t.max_te);
+    uint32_t* s_sp = reinterpret_cast<uint32_t*>(s_F.z + 4 * t.max_te);
     const SimDev& s = t.sims[br.slot];
     const float* in = (MODE == kModeFe) ? s.x : (sel == kSelD ? s.d : s.y);
     const TileArgs a = tile_args(s, in, t.max_tn, t.max_te);
@@ -317,7 +317,7 @@ int check_launch_b() {
 }
 inline uint32_t mesh_grid(const BatchTables& t) { return (t.n_active + 3) / 4; }
 size_t tile_smem_b(const BatchTables& t) {
-    size_t b = sizeof(float4) * (2 * (size_t)t.max_tn + 4 * (size_t)t.max_te);
+    size_t b = sizeof(float) * (6 * (size_t)t.max_tn + 12 * (size_t)t.max_te);
     b += sizeof(uint32_t) * ((size_t)t.max_tn + 1);
     return (b + 63) & ~size_t(63);
 }

@@ -33,7 +33,7 @@ Material material_from(float E, float nu) {
 
 static uint32_t g_tile_elems = 0;
 static uint32_t g_tile_threads = 256;
-static int g_tile_persistent = 2;
+static int g_tile_persistent = 0;
 
 template <class T>
 static int dalloc(T** p, size_t count) {

@@ -351,15 +351,31 @@ __device__ inline void load_tile_regs(const TileArgs& a, const TileDesc& d, uint
 }
 
 // Nodal forces of one element into its four tile force slots.
+// Tile data in LDS as float planes (x, y, z): the tile's node vectors and rest
+// positions (one entry per tile node) and its force slots (4 per element),
+// 12 bytes per entry.
+struct Slots {
+    float* x;
+    float* y;
+    float* z;
+};
+__device__ inline Slots make_slots(float* base, uint32_t n) { return Slots{base, base + n, base + 2 * n}; }
+
 template <int MODE>
-__device__ inline void element_force(const float4* s_v, const float4* s_r, float4* s_F, const uint32_t* lv,
+__device__ inline void element_force(const Slots& s_v, const Slots& s_r, const Slots& s_F, const uint32_t* lv,
                                      const uint32_t* fs, const float* R, const Material& m) {
     // Every fused multiply-add is written out (fma3); nothing else may be
     // contracted, so the result is fixed by this source in every kernel that
     // inlines it.
 #pragma clang fp contract(off)
-    const float4 x0 = s_v[lv[0]], x1 = s_v[lv[1]], x2 = s_v[lv[2]], x3 = s_v[lv[3]];
-    const float4 r0 = s_r[lv[0]], r1 = s_r[lv[1]], r2 = s_r[lv[2]], r3 = s_r[lv[3]];
+    const float3 x0 = make_float3(s_v.x[lv[0]], s_v.y[lv[0]], s_v.z[lv[0]]);
+    const float3 x1 = make_float3(s_v.x[lv[1]], s_v.y[lv[1]], s_v.z[lv[1]]);
+    const float3 x2 = make_float3(s_v.x[lv[2]], s_v.y[lv[2]], s_v.z[lv[2]]);
+    const float3 x3 = make_float3(s_v.x[lv[3]], s_v.y[lv[3]], s_v.z[lv[3]]);
+    const float3 r0 = make_float3(s_r.x[lv[0]], s_r.y[lv[0]], s_r.z[lv[0]]);
+    const float3 r1 = make_float3(s_r.x[lv[1]], s_r.y[lv[1]], s_r.z[lv[1]]);
+    const float3 r2 = make_float3(s_r.x[lv[2]], s_r.y[lv[2]], s_r.z[lv[2]]);
+    const float3 r3 = make_float3(s_r.x[lv[3]], s_r.y[lv[3]], s_r.z[lv[3]]);
     // Dm^-1 and V rebuilt from the rest positions staged in LDS with the same
     // operations as k_precompute (bit-identical to the stored values): 12 B
     // per tile node instead of 40 B per element from HBM.
@@ -408,10 +424,10 @@ __device__ inline void element_force(const float4* s_v, const float4* s_r, float
 #pragma unroll
         for (int c = 0; c < 3; c++)
             P[3 * r + c] = fma3(R[3 * r + 0], W[0 + c], R[3 * r + 1], W[3 + c], R[3 * r + 2], W[6 + c]);
-    s_F[fs[1]] = make_float4(P[0], P[3], P[6], 0.f);
-    s_F[fs[2]] = make_float4(P[1], P[4], P[7], 0.f);
-    s_F[fs[3]] = make_float4(P[2], P[5], P[8], 0.f);
-    s_F[fs[0]] = make_float4(-(P[0] + P[1] + P[2]), -(P[3] + P[4] + P[5]), -(P[6] + P[7] + P[8]), 0.f);
+    s_F.x[fs[1]] = P[0]; s_F.y[fs[1]] = P[3]; s_F.z[fs[1]] = P[6];
+    s_F.x[fs[2]] = P[1]; s_F.y[fs[2]] = P[4]; s_F.z[fs[2]] = P[7];
+    s_F.x[fs[3]] = P[2]; s_F.y[fs[3]] = P[5]; s_F.z[fs[3]] = P[8];
+    s_F.x[fs[0]] = -(P[0] + P[1] + P[2]); s_F.y[fs[0]] = -(P[3] + P[4] + P[5]); s_F.z[fs[0]] = -(P[6] + P[7] + P[8]);
 }
 
 
@@ -487,7 +503,7 @@ __device__ inline void begin_eval_block(uint32_t T, const uint32_t* __restrict__
 // vectors and rest positions in LDS, write each element's nodal forces to its
 // force slots, sum each tile node's contiguous slot run into its partial.
 template <int MODE>
-__device__ inline void tile_block(const TileArgs& a, uint32_t t, float4* s_v, float4* s_r, float4* s_F,
+__device__ inline void tile_block(const TileArgs& a, uint32_t t, const Slots& s_v, const Slots& s_r, const Slots& s_F,
                                   uint32_t* s_sp) {
 #pragma clang fp contract(off)
     const uint32_t tid = threadIdx.x;
@@ -498,8 +514,8 @@ __device__ inline void tile_block(const TileArgs& a, uint32_t t, float4* s_v, fl
     const uint32_t gid = (tid < dt.nn) ? a.tile_nodes[dt.n0 + tid] : 0u;
     load_tile_regs(a, dt, gid, tid, g);
     if (tid < dt.nn) {
-        s_v[tid] = make_float4(g.v[0], g.v[1], g.v[2], 0.f);
-        s_r[tid] = make_float4(g.r[0], g.r[1], g.r[2], 0.f);
+        s_v.x[tid] = g.v[0]; s_v.y[tid] = g.v[1]; s_v.z[tid] = g.v[2];
+        s_r.x[tid] = g.r[0]; s_r.y[tid] = g.r[1]; s_r.z[tid] = g.r[2];
     }
     if (tid <= dt.nn) s_sp[tid] = g.sp;
     lds_barrier();
@@ -514,8 +530,7 @@ __device__ inline void tile_block(const TileArgs& a, uint32_t t, float4* s_v, fl
         float fx = 0.f, fy = 0.f, fz = 0.f;
         const uint32_t s1 = s_sp[tid + 1];
         for (uint32_t sidx = s_sp[tid]; sidx < s1; sidx++) {
-            const float4 f = s_F[sidx];
-            fx += f.x; fy += f.y; fz += f.z;
+            fx += s_F.x[sidx]; fy += s_F.y[sidx]; fz += s_F.z[sidx];
         }
         float* o = a.partials + 3 * (size_t)g.pd;
         o[0] = fx; o[1] = fy; o[2] = fz;

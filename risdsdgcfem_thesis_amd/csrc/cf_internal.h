@@ -57,7 +57,7 @@ struct TetDevice {
     uint32_t num_parts = 0;        // sum over tiles of tile node counts
     uint32_t tile_threads = 256;   // workgroup size of k_tile_forces (256, 512 or 1024)
     uint32_t tile_grid = 8;        // grid of k_tile_forces (persistent, or one tile per workgroup)
-    int tile_persistent = 2;       // 0: plain kernel, 1: persistent pipelined, 2: pipelined kernel, one tile per workgroup
+    int tile_persistent = 0;       // 0: one tile per workgroup, 1: persistent pipelined, 2: pipelined kernel, one tile per workgroup
     uint32_t* tile_elem_ptr = nullptr;  // num_tiles+1
     uint32_t* tile_node_ptr = nullptr;  // num_tiles+1
     uint32_t* tile_nodes = nullptr;     // num_parts, global node ids

@@ -67,11 +67,11 @@ __global__ __launch_bounds__(kBlock) void k_begin_eval(uint32_t T, const uint32_
 template <int MODE, int TB, bool PIPE>
 __global__ __launch_bounds__(TB) void k_tile_forces(TileArgs a) {
     if (a.skip && *a.skip) return;
-    extern __shared__ float4 smem4[];
-    float4* s_v = smem4;                                 // max_tn
-    float4* s_r = s_v + a.max_tn;                        // max_tn
-    float4* s_F = s_r + a.max_tn;                        // 4*max_te force slots
-    uint32_t* s_sp = reinterpret_cast<uint32_t*>(s_F + 4 * a.max_te);  // max_tn+1
+    extern __shared__ float smem_f[];
+    const Slots s_v = make_slots(smem_f, a.max_tn);                     // tile node vectors
+    const Slots s_r = make_slots(smem_f + 3 * a.max_tn, a.max_tn);      // tile node rest positions
+    const Slots s_F = make_slots(smem_f + 6 * a.max_tn, 4 * a.max_te);  // 4 force slots per element
+    uint32_t* s_sp = reinterpret_cast<uint32_t*>(s_F.z + 4 * a.max_te);  // max_tn+1 slot-run starts
 
     const uint32_t tid = threadIdx.x;
     const uint32_t gx = gridDim.x / 8, xcd = blockIdx.x % 8, j = blockIdx.x / 8;
@@ -99,8 +99,8 @@ __global__ __launch_bounds__(TB) void k_tile_forces(TileArgs a) {
     while (true) {
         // stage tile t into LDS
         if (tid < dt.nn) {
-            s_v[tid] = make_float4(g.v[0], g.v[1], g.v[2], 0.f);
-            s_r[tid] = make_float4(g.r[0], g.r[1], g.r[2], 0.f);
+            s_v.x[tid] = g.v[0]; s_v.y[tid] = g.v[1]; s_v.z[tid] = g.v[2];
+            s_r.x[tid] = g.r[0]; s_r.y[tid] = g.r[1]; s_r.z[tid] = g.r[2];
         }
         if (tid <= dt.nn) s_sp[tid] = g.sp;
         const uint32_t cpd = g.pd;
@@ -132,8 +132,7 @@ __global__ __launch_bounds__(TB) void k_tile_forces(TileArgs a) {
             float fx = 0.f, fy = 0.f, fz = 0.f;
             const uint32_t s1 = s_sp[tid + 1];
             for (uint32_t sidx = s_sp[tid]; sidx < s1; sidx++) {
-                const float4 f = s_F[sidx];
-                fx += f.x; fy += f.y; fz += f.z;
+                fx += s_F.x[sidx]; fy += s_F.y[sidx]; fz += s_F.z[sidx];
             }
             float* o = a.partials + 3 * (size_t)cpd;
             o[0] = fx; o[1] = fy; o[2] = fz;
@@ -333,7 +332,7 @@ int launch_begin_eval(const TetDevice& t, const float* x, int rm, float thr, flo
 }
 
 static size_t tile_smem(const TetDevice& t) {
-    size_t b = sizeof(float4) * (2 * (size_t)t.max_tile_nodes + 4 * (size_t)t.max_tile_elems);
+    size_t b = sizeof(float) * (6 * (size_t)t.max_tile_nodes + 12 * (size_t)t.max_tile_elems);
     b += sizeof(uint32_t) * ((size_t)t.max_tile_nodes + 1);
     return (b + 63) & ~size_t(63);
 }

@@ -22,10 +22,10 @@ namespace {
 constexpr uint32_t kMaxTileNodes = 256;  // tile-local node ids are one byte
 constexpr uint32_t kMaxRank = 63;        // elements per node within one tile
 // LDS bytes of one tile in k_tile_forces: node vectors and rest positions
-// (2 x float4), 4 force slots per element (float4 each) and slot offsets
+// (2 x 3 floats), 4 force slots per element (3 floats each) and slot offsets
 // (4 B per node).
 constexpr uint32_t kMaxTileLds = 64 * 1024 - 512;
-inline uint32_t tile_lds(uint32_t ne, uint32_t nn) { return 36 * nn + 64 * ne + 64; }
+inline uint32_t tile_lds(uint32_t ne, uint32_t nn) { return 28 * nn + 48 * ne + 64; }
 
 inline uint64_t spread3(uint64_t v) {  // 21 bits -> every third bit
     v &= 0x1fffff;
