@@ -321,13 +321,23 @@ int cf_batch_synchronize(cf_batch* b);
  * C_df_x :1859-1915; lumped masses ds/dsh/LeafDSH_Solid2D_FEM.cpp:1842-1863;
  * Step_QuasiImplicitEuler_v :1287-1403). Node vectors are 2N floats.
  * Rotations: the closed-form 2x2 polar factor for det F > threshold (the
- * reference's MSVD and DAPD both take it there); below it R = Id (the 2D SVD
- * and DAPD degenerate branches are not implemented).
+ * reference's MSVD and DAPD both take it there); below it MSVD takes the 2x2
+ * SVD of ms/fem/Rotation.cpp:27-36 (Mal/GMatDecomposition.h:255-345) and DAPD
+ * R = Id, its invalid-NoC branch (TriangleElementSet2.cpp:1100-1105; the NoC
+ * degenerate rotation is not implemented), counted in num_fallback.
  * Each mesh is stepped inside one workgroup by a group of 1, 2, 4 or 8
  * wavefronts (up to 2048 triangles and 1536 nodes; CF_ERR_UNSUPPORTED beyond).
  * A batch steps all its simulations in ONE launch; each simulation's result is
  * independent of the others in the batch. */
 typedef struct cf_batch2 cf_batch2;
+/* Rotations of n triangles from their deformation gradients F (4 floats each,
+   row-major) and det F, as the 2D step computes them: ms/fem/Rotation.cpp:19-37
+   (Compute_R_SVD, rm = CF_RM_MSVD) and Rotation.h:20 (Compute_R_PD, rm =
+   CF_RM_DAPD) as called by TriangleElementSet2::ComputeRotations_MSVD/DAPD
+   (TriangleElementSet2.cpp:1007-1018,1078-1110). Host arrays; R is 4n floats;
+   fallback (may be NULL) gets 1 where DAPD fell back to R = Id. */
+int cf_rotations2(const float* h_F, const float* h_detF, uint32_t n, int32_t rm, float threshold, float* h_R,
+                  uint32_t* h_fallback);
 int cf_batch2_create(cf_batch2** out);
 int cf_batch2_destroy(cf_batch2* b);
 /* A simulation at rest (x = rest positions, v = 0), KNC nodes held at rest.
@@ -340,6 +350,9 @@ int cf_batch2_fixed_step(cf_batch2* b, cf_step_stats* stats);
 int cf_batch2_get_state(cf_batch2* b, uint32_t sim_id, float* h_pos2, float* h_vel2);
 int cf_batch2_set_state(cf_batch2* b, uint32_t sim_id, const float* h_pos2, const float* h_vel2);
 int cf_batch2_node_masses(const cf_batch2* b, uint32_t sim_id, float* h_mass);
+/* Elements of the simulation's last fixed step whose DAPD rotation fell back
+   to R = Id (det F <= threshold; TriangleElementSet2.cpp:1100-1105). */
+int cf_batch2_fallbacks(cf_batch2* b, uint32_t sim_id, uint32_t* num_fallback);
 int cf_batch2_info(const cf_batch2* b, uint32_t* num_sims, uint64_t* tris, uint64_t* nodes, uint32_t* workgroups);
 
 /* LeafDSH_Solid2D_FEM drop-in (Create / Step / FixedStep / QueryState): one
@@ -352,6 +365,7 @@ int cf_solid2_fixed_step(cf_solid2* s, cf_step_stats* stats);
 int cf_solid2_step(cf_solid2* s, float dt, cf_step_stats* stats);
 int cf_solid2_get_state(const cf_solid2* s, float* h_pos2, float* h_vel2);
 int cf_solid2_set_state(cf_solid2* s, const float* h_pos2, const float* h_vel2);
+int cf_solid2_fallbacks(cf_solid2* s, uint32_t* num_fallback);  /* as cf_batch2_fallbacks */
 
 /* ---------------------------------------------------------------- serving
  * A real-time request loop over a batch (no reference counterpart: the

@@ -909,9 +909,96 @@ inline void corot_apply2(const orc_solid2* s, uint32_t e, const float l[3][2], f
     }
 }
 inline M2 m2tr(const M2& m) { M2 t; t.a[0][0] = m.a[0][0]; t.a[0][1] = m.a[1][0]; t.a[1][0] = m.a[0][1]; t.a[1][1] = m.a[1][1]; return t; }
+
+// Mal/GVec.h:193-199: squared norm accumulated from 0, strictly below
+// epsilon^2 with epsilon = 1e-4 (Mal/RealUtils.h:110)
+inline bool zero2(float a, float b) {
+    float acc = 0.0f;
+    acc += a * a;
+    acc += b * b;
+    return acc < 0.0001f * 0.0001f;
+}
+// Mal/GMatEigen.h:172-184: unit eigenvector of the symmetric m for lambda
+// from the first non-negligible row of m - lambda*Id, normalised as
+// v * (1 / sqrt(v.v)) (Mal/GVec.h:85,97,103)
+inline void eigvec2(const M2& m, float lam, float& x, float& y) {
+    const float e00 = m.a[0][0] - lam, e01 = m.a[0][1], e10 = m.a[1][0], e11 = m.a[1][1] - lam;
+    if (!zero2(e00, e01)) { x = e01; y = -e00; }
+    else if (!zero2(e10, e11)) { x = -e11; y = e10; }
+    else { x = 0.0f; y = 0.0f; return; }
+    float nsq = 0.0f;
+    nsq += x * x;
+    nsq += y * y;
+    const float inv = 1.0f / std::sqrt(nsq);
+    x = inv * x;
+    y = inv * y;
+}
+// Degenerate branch of the 2D Compute_R_SVD (ms/fem/Rotation.cpp:27-36):
+// SVD F = U diag(s) V^T (Mal/GMatDecomposition.h:255-345) from the
+// eigenvalues of F^T F by the quadratic formula (Mal/GMatEigen.h:151-164),
+// U = F V diag(s)^+, null singular directions completed by the CCW
+// perpendicular; then U and V^T are made proper rotations and R = U V^T.
+// Release-build behaviour: a NaN eigenvalue pair with |det F - 1| >= 0.05 is
+// not caught and flows into R. Only the second eigenvalue and the second
+// singular value are NaN-tested (Mal/GVec.h:201-208 starts at index 1).
+M2 svd_rotation2_degenerate(const M2& F) {
+    M2 U, Vt;
+    const M2 A = m2mul(m2tr(F), F);
+    float tr = 0.0f;
+    tr += A.a[0][0];
+    tr += A.a[1][1];
+    const float b = -tr, c = det2(A);
+    const float pm = std::sqrt(b * b - 4.0f * c);
+    const float ev0 = 0.5f * (-b + pm), ev1 = 0.5f * (-b - pm);
+    bool identity = false;
+    if (ev1 != ev1) {
+        const float d = det2(F) - 1.0f;
+        identity = double(d > 0.0f ? d : -d) < 0.05;
+    }
+    if (identity) {
+        U = M2{{{1.0f, 0.0f}, {0.0f, 1.0f}}};
+        Vt = U;
+    } else {
+        float s0 = std::sqrt(ev0), s1 = std::sqrt(ev1);
+        if (s1 != s1) {
+            if (ev0 < 0.0f) s0 = 0.0f;
+            if (ev1 < 0.0f) s1 = 0.0f;
+        }
+        const bool z0 = (s0 > 0.0f ? s0 : -s0) <= 0.0001f;  // Mal/RealUtils.h:114
+        const bool z1 = (s1 > 0.0f ? s1 : -s1) <= 0.0001f;
+        M2 V;
+        eigvec2(A, ev0, V.a[0][0], V.a[1][0]);
+        eigvec2(A, ev1, V.a[0][1], V.a[1][1]);
+        M2 P{{{z0 ? 0.0f : 1.0f / s0, 0.0f}, {0.0f, z1 ? 0.0f : 1.0f / s1}}};
+        U = m2mul(m2mul(F, V), P);
+        Vt = m2tr(V);
+        if (z0 && z1) {
+            U = M2{{{1.0f, 0.0f}, {0.0f, 1.0f}}};
+            Vt = U;
+        } else if (z0) {
+            Vt.a[0][0] = Vt.a[1][1]; Vt.a[0][1] = -Vt.a[1][0];
+            U.a[0][0] = U.a[1][1]; U.a[1][0] = -U.a[0][1];
+        } else if (z1) {
+            Vt.a[1][0] = Vt.a[0][1]; Vt.a[1][1] = -Vt.a[0][0];
+            U.a[0][1] = U.a[1][0]; U.a[1][1] = -U.a[0][0];
+        }
+    }
+    if (det2(U) < 0.0f) { U.a[0][1] = -U.a[0][1]; U.a[1][1] = -U.a[1][1]; }
+    if (det2(Vt) < 0.0f) { Vt.a[1][0] = -Vt.a[1][0]; Vt.a[1][1] = -Vt.a[1][1]; }
+    return m2mul(U, Vt);
+}
 }  // namespace
 
 extern "C" {
+
+// ms/fem/Rotation.cpp:19-37 (2D Compute_R_SVD)
+void orc_svd_rotation2(const float F[4], float detF, float thr, float R[4]) {
+    if (detF > thr) {
+        orc_pd2(F, detF, R);
+        return;
+    }
+    m2store(svd_rotation2_degenerate(m2load(F)), R);
+}
 
 // EditableTriangleElementSet2::EndEdition (TriangleElementSet2.cpp:2187-2239) +
 // LeafDSH_Solid2D_FEM::RebuildMass (LeafDSH_Solid2D_FEM.cpp:1842-1863)
@@ -970,8 +1057,9 @@ void orc_solid2_node_masses(const orc_solid2* s, float* m) {
 }
 
 // BeginEvaluation (TriangleElementSet2.cpp:183-262) + ComputeRotations_MSVD/DAPD
-// (:1007-1018,1078-1110) with Compute_R_SVD / Compute_R_PD for det F > threshold
-// (Rotation.cpp:19-37, GMatDecomposition.h:17-30)
+// (:1007-1018,1078-1110): Compute_R_PD for det F > threshold
+// (GMatDecomposition.h:17-30); below it MSVD takes the 2x2 SVD
+// (Rotation.cpp:19-37) and DAPD R = Id (its invalid-NoC branch, :1100-1105)
 void orc_solid2_begin_evaluation(orc_solid2* s, const float* x, float* R_out, float* detF_out) {
     for (uint32_t e = 0; e < s->T; e++) {
         const uint32_t* t = &s->tri[3 * size_t(e)];
@@ -983,6 +1071,8 @@ void orc_solid2_begin_evaluation(orc_solid2* s, const float* x, float* R_out, fl
             float Fa[4];
             m2store(F, Fa);
             orc_pd2(Fa, dF, R);
+        } else if (s->p.rm == ORC_RM_MSVD) {
+            m2store(svd_rotation2_degenerate(F), R);
         }
         std::memcpy(&s->R[4 * size_t(e)], R, sizeof(R));
     }

@@ -49,6 +49,7 @@ void orc_svd3(const float F[9], float U[9], float s[3], float Vt[9]);
 void orc_svd_rotation3(const float F[9], float R[9]);
 void orc_svd3_sqrt3(const float F[9], float U[9], float s[3], float Vt[9]);
 void orc_pd2(const float F[4], float detF, float R[4]);
+void orc_svd_rotation2(const float F[4], float detF, float thr, float R[4]);
 float orc_compute_K(const float r[12], float E, float nu, float K[144]);
 float orc_compute_K2(const float r[6], float E, float nu, float K[36]);
 float orc_sse_dot(const float* a, const float* b, uint32_t n);

@@ -62,6 +62,7 @@ def _load():
         "orc_svd_rotation3": (None, [_f, _f]),
         "orc_svd3_sqrt3": (None, [_f, _f, _f, _f]),
         "orc_pd2": (None, [_f, C.c_float, _f]),
+        "orc_svd_rotation2": (None, [_f, C.c_float, C.c_float, _f]),
         "orc_compute_K": (C.c_float, [_f, C.c_float, C.c_float, _f]),
         "orc_compute_K2": (C.c_float, [_f, C.c_float, C.c_float, _f]),
         "orc_sse_dot": (C.c_float, [_f, _f, C.c_uint32]),

@@ -167,6 +167,7 @@ SIGNATURES = {
     "cf_batch_info_get": (C.c_int, [_vp, C.POINTER(BatchInfo)]),
     "cf_batch_time_kx": (C.c_int, [_vp, C.c_uint32, C.POINTER(C.c_double)]),
     "cf_batch_synchronize": (C.c_int, [_vp]),
+    "cf_rotations2": (C.c_int, [_f, _f, C.c_uint32, C.c_int32, C.c_float, _f, _u]),
     "cf_batch2_create": (C.c_int, [C.POINTER(_vp)]),
     "cf_batch2_destroy": (C.c_int, [_vp]),
     "cf_batch2_add_sim": (C.c_int, [_vp, C.c_uint32, _f, C.c_uint32, _u, C.POINTER(SolidParams), C.c_uint32, _u, _u]),
@@ -174,6 +175,8 @@ SIGNATURES = {
     "cf_batch2_get_state": (C.c_int, [_vp, C.c_uint32, _f, _f]),
     "cf_batch2_set_state": (C.c_int, [_vp, C.c_uint32, _f, _f]),
     "cf_batch2_node_masses": (C.c_int, [_vp, C.c_uint32, _f]),
+    "cf_batch2_fallbacks": (C.c_int, [_vp, C.c_uint32, _u]),
+    "cf_solid2_fallbacks": (C.c_int, [_vp, _u]),
     "cf_batch2_info": (C.c_int, [_vp, _u, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), _u]),
     "cf_solid2_create": (C.c_int, [C.c_uint32, _f, C.c_uint32, _u, C.POINTER(SolidParams), C.c_uint32, _u,
                                    C.POINTER(_vp)]),

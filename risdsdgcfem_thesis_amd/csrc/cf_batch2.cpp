@@ -157,6 +157,44 @@ Material material2(float E, float nu) {
 
 extern "C" {
 
+int cf_rotations2(const float* h_F, const float* h_detF, uint32_t n, int32_t rm, float threshold, float* h_R,
+                  uint32_t* h_fallback) {
+    using namespace cf;
+    if (n && (!h_F || !h_detF || !h_R)) {
+        set_error("cf_rotations2: null array");
+        return CF_ERR_INVALID_ARG;
+    }
+    if (rm != CF_RM_ID && rm != CF_RM_MSVD && rm != CF_RM_DAPD) {
+        set_error("cf_rotations2: rotation method %d is not supported", rm);
+        return CF_ERR_UNSUPPORTED;
+    }
+    if (!n) return CF_OK;
+    float *d_F = nullptr, *d_det = nullptr, *d_R = nullptr;
+    uint32_t* d_fb = nullptr;
+    struct Free {
+        void* p[4];
+        ~Free() {
+            for (void* q : p)
+                if (q) (void)hipFree(q);
+        }
+    } guard{{nullptr, nullptr, nullptr, nullptr}};
+    CF_CHECK_HIP(hipMalloc(&d_F, size_t(n) * 16));
+    guard.p[0] = d_F;
+    CF_CHECK_HIP(hipMalloc(&d_det, size_t(n) * 4));
+    guard.p[1] = d_det;
+    CF_CHECK_HIP(hipMalloc(&d_R, size_t(n) * 16));
+    guard.p[2] = d_R;
+    CF_CHECK_HIP(hipMalloc(&d_fb, size_t(n) * 4));
+    guard.p[3] = d_fb;
+    CF_CHECK_HIP(hipMemcpy(d_F, h_F, size_t(n) * 16, hipMemcpyHostToDevice));
+    CF_CHECK_HIP(hipMemcpy(d_det, h_detF, size_t(n) * 4, hipMemcpyHostToDevice));
+    CF_TRY(launch_rotations2(d_F, d_det, n, rm, threshold, d_R, d_fb, nullptr));
+    CF_CHECK_HIP(hipDeviceSynchronize());
+    CF_CHECK_HIP(hipMemcpy(h_R, d_R, size_t(n) * 16, hipMemcpyDeviceToHost));
+    if (h_fallback) CF_CHECK_HIP(hipMemcpy(h_fallback, d_fb, size_t(n) * 4, hipMemcpyDeviceToHost));
+    return CF_OK;
+}
+
 int cf_batch2_create(cf_batch2** out) {
     if (!out) return CF_ERR_INVALID_ARG;
     *out = nullptr;
@@ -357,6 +395,20 @@ int cf_batch2_set_state(cf_batch2* b, uint32_t id, const float* pos, const float
     return CF_OK;
 }
 
+int cf_batch2_fallbacks(cf_batch2* b, uint32_t id, uint32_t* num_fallback) {
+    if (!b || !num_fallback || id >= b->meshes.size()) {
+        set_error("no 2D simulation %u", id);
+        return CF_ERR_INVALID_ARG;
+    }
+    *num_fallback = 0;
+    if (!b->on_device) return CF_OK;  // not stepped since it was added
+    Sim2Out o;
+    CF_CHECK_HIP(hipStreamSynchronize(b->stream));
+    CF_CHECK_HIP(hipMemcpy(&o, b->d_out + id, sizeof(Sim2Out), hipMemcpyDeviceToHost));
+    *num_fallback = o.num_fallback;
+    return CF_OK;
+}
+
 int cf_batch2_node_masses(const cf_batch2* b, uint32_t id, float* mass) {
     if (!b || !mass || id >= b->meshes.size()) { set_error("no 2D simulation %u", id); return CF_ERR_INVALID_ARG; }
     const Mesh2Dev& m = b->meshes[id];
@@ -426,6 +478,11 @@ int cf_solid2_step(cf_solid2* s, float dt, cf_step_stats* stats) {
     last.num_fixed_steps = n;
     if (stats) *stats = last;
     return CF_OK;
+}
+
+int cf_solid2_fallbacks(cf_solid2* s, uint32_t* num_fallback) {
+    if (!s) return CF_ERR_INVALID_ARG;
+    return cf_batch2_fallbacks(s->b, 0, num_fallback);
 }
 
 int cf_solid2_get_state(const cf_solid2* s, float* pos, float* vel) {

@@ -306,4 +306,6 @@ struct Batch2Args {
 };
 
 int launch_batch2_step(const Batch2Args& a, uint32_t n_work, hipStream_t s);
+int launch_rotations2(const float* F, const float* det, uint32_t n, int rm, float thr, float* R, uint32_t* fallback,
+                      hipStream_t s);
 }  // namespace cf

@@ -3,7 +3,8 @@
 //
 // Reference path: BeginEvaluation (ms/fem/TriangleElementSet2.cpp:183-262),
 // rotations (:1007-1018,1078-1110 -> Mal/GMatDecomposition.h:17-30 for
-// det F > threshold), C_f_e (:1253-1286), C_df_x (:1859-1915), C_V
+// det F > threshold, ms/fem/Rotation.cpp:19-37 for MSVD below it),
+// C_f_e (:1253-1286), C_df_x (:1859-1915), C_V
 // (:1620-1645), LS_A_times_d (ds/dsh/LeafDSH_Solid2D_FEM.cpp:1101-1132),
 // Step_QuasiImplicitEuler_v (:1287-1403) with the CG of ns/ILSS_CG.h:38-151.
 //
@@ -91,6 +92,125 @@ __device__ inline void polar2(float f00, float f01, float f10, float f11, float 
     const float i1 = 1.0f / sqrtf(r01 * r01 + r11 * r11);
     r01 *= i1; r11 *= i1;
     t.r00 = r00; t.r01 = r01; t.r10 = r10; t.r11 = r11;
+}
+
+// Degenerate branch of the 2D Compute_R_SVD (ms/fem/Rotation.cpp:27-36), in
+// the reference's FP32 operation order: eigenvalues of F^T F by the quadratic
+// formula (Mal/GMatEigen.h:151-164), eigenvectors from the first row of
+// F^T F - lambda Id whose squared norm is not below 1e-8 (:172-184,
+// Mal/GVec.h:193-199), U = F V diag(s)^+ with singular values <= 1e-4 taken
+// as 0 and their directions completed by the CCW perpendicular
+// (Mal/GMatDecomposition.h:255-345), reflections removed from U and V^T,
+// R = U V^T. Only the second eigenvalue and singular value are NaN-tested
+// (Mal/GVec.h:201-208 starts at index 1); a NaN pair with |det F - 1| >= 0.05
+// flows into R, as in the reference's release build.
+struct Mat2 {
+    float m00, m01, m10, m11;
+};
+
+__device__ inline Mat2 mul2(const Mat2& x, const Mat2& y) {  // Mal/GMat.h:135-145, accumulated from 0
+#pragma clang fp contract(off)
+    Mat2 r;
+    float a;
+    a = 0.0f; a += x.m00 * y.m00; a += x.m01 * y.m10; r.m00 = a;
+    a = 0.0f; a += x.m00 * y.m01; a += x.m01 * y.m11; r.m01 = a;
+    a = 0.0f; a += x.m10 * y.m00; a += x.m11 * y.m10; r.m10 = a;
+    a = 0.0f; a += x.m10 * y.m01; a += x.m11 * y.m11; r.m11 = a;
+    return r;
+}
+
+__device__ inline float det2m(const Mat2& m) {
+#pragma clang fp contract(off)
+    return m.m00 * m.m11 - m.m10 * m.m01;
+}
+
+__device__ inline bool zero2(float a, float b) {
+#pragma clang fp contract(off)
+    float acc = 0.0f;
+    acc += a * a;
+    acc += b * b;
+    return acc < 0.0001f * 0.0001f;
+}
+
+__device__ inline void eigvec2(const Mat2& m, float lam, float& x, float& y) {
+#pragma clang fp contract(off)
+    const float e00 = m.m00 - lam, e01 = m.m01, e10 = m.m10, e11 = m.m11 - lam;
+    if (!zero2(e00, e01)) { x = e01; y = -e00; }
+    else if (!zero2(e10, e11)) { x = -e11; y = e10; }
+    else { x = 0.0f; y = 0.0f; return; }
+    float nsq = 0.0f;
+    nsq += x * x;
+    nsq += y * y;
+    const float inv = 1.0f / sqrtf(nsq);
+    x = inv * x;
+    y = inv * y;
+}
+
+// Not inlined: the branch is rare, and inlining it into the step kernel's
+// unrolled triangle loop would cost the kernel registers it needs elsewhere.
+__device__ __noinline__ float4 svd2_degenerate(float f00, float f01, float f10, float f11) {
+#pragma clang fp contract(off)
+    const Mat2 F{f00, f01, f10, f11};
+    const Mat2 A = mul2(Mat2{f00, f10, f01, f11}, F);
+    float tr = 0.0f;
+    tr += A.m00;
+    tr += A.m11;
+    const float b = -tr, c = det2m(A);
+    const float pm = sqrtf(b * b - 4.0f * c);
+    const float ev0 = 0.5f * (-b + pm), ev1 = 0.5f * (-b - pm);
+    bool identity = false;
+    if (ev1 != ev1) {
+        const float d = det2m(F) - 1.0f;
+        identity = double(d > 0.0f ? d : -d) < 0.05;
+    }
+    Mat2 U{1.0f, 0.0f, 0.0f, 1.0f}, Vt{1.0f, 0.0f, 0.0f, 1.0f};
+    if (!identity) {
+        float s0 = sqrtf(ev0), s1 = sqrtf(ev1);
+        if (s1 != s1) {
+            if (ev0 < 0.0f) s0 = 0.0f;
+            if (ev1 < 0.0f) s1 = 0.0f;
+        }
+        const bool z0 = (s0 > 0.0f ? s0 : -s0) <= 0.0001f;  // Mal/RealUtils.h:114
+        const bool z1 = (s1 > 0.0f ? s1 : -s1) <= 0.0001f;
+        Mat2 V;
+        eigvec2(A, ev0, V.m00, V.m10);
+        eigvec2(A, ev1, V.m01, V.m11);
+        const Mat2 P{z0 ? 0.0f : 1.0f / s0, 0.0f, 0.0f, z1 ? 0.0f : 1.0f / s1};
+        if (!(z0 && z1)) {
+            U = mul2(mul2(F, V), P);
+            Vt = Mat2{V.m00, V.m10, V.m01, V.m11};
+            if (z0) {
+                Vt.m00 = Vt.m11; Vt.m01 = -Vt.m10;
+                U.m00 = U.m11; U.m10 = -U.m01;
+            } else if (z1) {
+                Vt.m10 = Vt.m01; Vt.m11 = -Vt.m00;
+                U.m01 = U.m10; U.m11 = -U.m00;
+            }
+        }
+    }
+    if (det2m(U) < 0.0f) { U.m01 = -U.m01; U.m11 = -U.m11; }
+    if (det2m(Vt) < 0.0f) { Vt.m10 = -Vt.m10; Vt.m11 = -Vt.m11; }
+    const Mat2 R = mul2(U, Vt);
+    return make_float4(R.m00, R.m01, R.m10, R.m11);
+}
+
+// Rotation of one triangle (ComputeRotations_MSVD/DAPD,
+// TriangleElementSet2.cpp:1007-1018,1078-1110): polar factor above the
+// threshold; below it MSVD's 2x2 SVD, DAPD R = Id (its invalid-NoC branch).
+// Returns 1 for a DAPD fallback.
+__device__ inline float rotation2(int rm, float thr, float f00, float f01, float f10, float f11, float det, Tri& t) {
+    t.r00 = 1.f; t.r01 = 0.f; t.r10 = 0.f; t.r11 = 1.f;
+    if (rm == CF_RM_ID) return 0.0f;
+    if (det > thr) {
+        polar2(f00, f01, f10, f11, det, t);
+        return 0.0f;
+    }
+    if (rm == CF_RM_MSVD) {
+        const float4 r = svd2_degenerate(f00, f01, f10, f11);
+        t.r00 = r.x; t.r01 = r.y; t.r10 = r.z; t.r11 = r.w;
+        return 0.0f;
+    }
+    return 1.0f;
 }
 
 // Displacement gradient H of a triangle: R^T D Dm^-1, or for f_e
@@ -245,11 +365,7 @@ __device__ void mesh_step(const Batch2Args& a, uint32_t cmax, float2* s_v_all, f
         float f10 = 0.0f; f10 += D10 * t.g00; f10 += D11 * t.g10;
         float f11 = 0.0f; f11 += D10 * t.g01; f11 += D11 * t.g11;
         const float det = f00 * f11 - f10 * f01;  // Mal/GMat.h:230-233
-        t.r00 = 1.f; t.r01 = 0.f; t.r10 = 0.f; t.r11 = 1.f;
-        if (M.rm != CF_RM_ID) {
-            if (det > M.thr) polar2(f00, f01, f10, f11, det, t);
-            else nfb += 1.0;  // below the threshold: R = Id (degenerate branches not implemented)
-        }
+        nfb += rotation2(M.rm, M.thr, f00, f01, f10, f11, det, t);
         if (det <= 0.f) { ndeg += 1.0; sdeg += det; }
         mind = fminf(mind, det);
     }
@@ -450,6 +566,34 @@ __global__ __launch_bounds__(k2Threads) void k2_step(Batch2Args a) {
         case 4: mesh_step<4>(a, cmax, s_v, s_f, red, redm, flags); break;
         default: mesh_step<8>(a, cmax, s_v, s_f, red, redm, flags); break;
     }
+}
+
+namespace {
+// One thread per deformation gradient: the triangle rotation of the step kernel.
+__global__ void __launch_bounds__(256) k2_rotations(const float4* __restrict__ F, const float* __restrict__ det,
+                                                    uint32_t n, int rm, float thr, float4* __restrict__ R,
+                                                    uint32_t* __restrict__ fallback) {
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= n) return;
+    const float4 f = F[i];
+    Tri t;
+    const float fb = rotation2(rm, thr, f.x, f.y, f.z, f.w, det[i], t);
+    R[i] = make_float4(t.r00, t.r01, t.r10, t.r11);
+    fallback[i] = fb != 0.0f;
+}
+}  // namespace
+
+int launch_rotations2(const float* F, const float* det, uint32_t n, int rm, float thr, float* R, uint32_t* fallback,
+                      hipStream_t s) {
+    if (!n) return CF_OK;
+    hipLaunchKernelGGL(k2_rotations, dim3((n + 255) / 256), dim3(256), 0, s, reinterpret_cast<const float4*>(F),
+                       det, n, rm, thr, reinterpret_cast<float4*>(R), fallback);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        set_error("2D rotation launch failed: %s", hipGetErrorString(e));
+        return CF_ERR_HIP;
+    }
+    return CF_OK;
 }
 
 int launch_batch2_step(const Batch2Args& a, uint32_t n_work, hipStream_t s) {

@@ -526,6 +526,21 @@ def serve(meshes, requests, params, hbm_budget_bytes, duration_s, drain=False, m
     return out, lat[:res.steps].copy()
 
 
+def rotations2(F, det, rm, threshold=0.1):
+    """Triangle rotations from deformation gradients F (n x 4, row-major) and
+    det F, as the 2D step computes them (Compute_R_SVD for RM_MSVD,
+    Compute_R_PD with the R = Id fallback below the threshold for RM_DAPD).
+    Returns (R as n x 4, fallback flags)."""
+    F = np.ascontiguousarray(F, np.float32).reshape(-1, 4)
+    det = np.ascontiguousarray(det, np.float32).ravel()
+    if det.size != F.shape[0]:
+        raise ValueError("one det F per F")
+    R = np.zeros_like(F)
+    fb = np.zeros(det.size, np.uint32)
+    check(lib.cf_rotations2(_fp(F), _fp(det), det.size, int(rm), float(threshold), _fp(R), _up(fb)), "cf_rotations2")
+    return R, fb
+
+
 class Batch2:
     """Many 2D solids (TriangleElementSet2 + LeafDSH_Solid2D_FEM) stepped in one
     launch; node vectors are 2N floats."""
@@ -581,6 +596,12 @@ class Batch2:
         check(lib.cf_batch2_node_masses(self.h, int(sim_id), _fp(m)), "cf_batch2_node_masses")
         return m
 
+    def num_fallback(self, sim_id):
+        """DAPD R = Id fallbacks of the simulation's last fixed step."""
+        n = C.c_uint32()
+        check(lib.cf_batch2_fallbacks(self.h, int(sim_id), C.byref(n)), "cf_batch2_fallbacks")
+        return n.value
+
     def info(self):
         n, w = C.c_uint32(), C.c_uint32()
         t, nn = C.c_uint64(), C.c_uint64()
@@ -632,3 +653,9 @@ class LeafSolid2D:
         x = np.ascontiguousarray(x, np.float32)
         v = np.ascontiguousarray(v, np.float32)
         check(lib.cf_solid2_set_state(self.h, _fp(x), _fp(v)), "cf_solid2_set_state")
+
+    def num_fallback(self):
+        """DAPD R = Id fallbacks of the last fixed step."""
+        n = C.c_uint32()
+        check(lib.cf_solid2_fallbacks(self.h, C.byref(n)), "cf_solid2_fallbacks")
+        return n.value

@@ -10,7 +10,7 @@ bit-identical to the same simulation alone (same kernel, its own group).
 import numpy as np
 import pytest
 
-from conftest import rel_node_err
+from conftest import golden, rel_node_err
 
 pytestmark = pytest.mark.gpu
 
@@ -24,15 +24,16 @@ def grid(cf, n):
     return pos, tris, knc
 
 
-def params(cf, E=1e5, nu=0.3, max_iter=50, rel=1e-4):
+def params(cf, E=1e5, nu=0.3, max_iter=50, rel=1e-4, rm=None):
     return cf.default_solid_params(fixed_dt=1.0 / 60.0, gravity=(0.0, -9.81, 0.0), young_modulus=E,
-                                   poisson_ratio=nu, damping_ratio=0.0, rm=cf.RM_DAPD, cg_rel_epsilon=rel,
-                                   cg_abs_epsilon=1e-6, cg_max_iter=max_iter, density=1000.0)
+                                   poisson_ratio=nu, damping_ratio=0.0, rm=cf.RM_DAPD if rm is None else rm,
+                                   cg_rel_epsilon=rel, cg_abs_epsilon=1e-6, cg_max_iter=max_iter, density=1000.0)
 
 
-def oracle_params(orc, E=1e5, nu=0.3, max_iter=50, rel=1e-4):
-    return orc.default_params(young_modulus=E, poisson_ratio=nu, rm=orc.RM_DAPD, cg_rel_epsilon=rel,
-                              cg_abs_epsilon=1e-6, cg_max_iter=max_iter, gravity=(0.0, -9.81, 0.0), reduce_fp64=1)
+def oracle_params(orc, E=1e5, nu=0.3, max_iter=50, rel=1e-4, rm=None):
+    return orc.default_params(young_modulus=E, poisson_ratio=nu, rm=orc.RM_DAPD if rm is None else rm,
+                              cg_rel_epsilon=rel, cg_abs_epsilon=1e-6, cg_max_iter=max_iter,
+                              gravity=(0.0, -9.81, 0.0), reduce_fp64=1)
 
 
 def test_lumped_masses_bitexact(gpu, orc):
@@ -100,6 +101,95 @@ def test_batch_equals_alone_bitexact(gpu):
                    (hist[j][i].cg_iters, hist[j][i].cg_reason, hist[j][i].cg_rel_residual)
         s.close()
     b.close()
+
+
+def test_rotations2_against_reference_golden(gpu):
+    """1024 golden F (56 with det F <= 0.1, 29 inverted): MSVD bit-exact with
+    the reference's Compute_R_SVD including its 2x2 SVD branch; DAPD polar
+    above the threshold and R = Id (flagged) below."""
+    g = golden("tris.npz")
+    R, fb = gpu.rotations2(g["F"], g["det"], gpu.RM_MSVD)
+    assert np.array_equal(R, g["R_svd"]) and not fb.any()
+    R, fb = gpu.rotations2(g["F"], g["det"], gpu.RM_DAPD)
+    ok = g["det"] > 0.1
+    assert np.array_equal(R[ok], g["R_pd"][ok])
+    assert np.array_equal(R[~ok], np.tile(np.eye(2, dtype=np.float32).ravel(), ((~ok).sum(), 1)))
+    assert np.array_equal(fb.astype(bool), ~ok)
+
+
+def test_rotations2_degenerate_vs_oracle(gpu, orc):
+    """The 2x2 SVD branch on generic, small, inverted, rank-1, near rank-1 and
+    zero F: bit-exact with the oracle (NaN where the reference yields NaN)."""
+    rng = np.random.default_rng(12)
+    F = [rng.uniform(-1, 1, (4000, 4)), rng.uniform(-0.3, 0.3, (4000, 4))]
+    u, v = rng.normal(size=(1000, 2)), rng.normal(size=(1000, 2))
+    r1 = np.einsum("ni,nj->nij", u, v).reshape(-1, 4)
+    F += [r1, r1 + 1e-4 * rng.normal(size=r1.shape), np.zeros((1, 4)), [[1e-5, 0, 0, 1e-5]], [[0, 0, 0, 1.0]]]
+    F = np.ascontiguousarray(np.concatenate([np.asarray(f, np.float64) for f in F]).astype(np.float32))
+    det = (F[:, 0] * F[:, 3] - F[:, 2] * F[:, 1]).astype(np.float32)
+    assert (det <= 0.1).sum() > 5000
+    R, _ = gpu.rotations2(F, det, gpu.RM_MSVD)
+    want = np.zeros_like(F)
+    for i in range(len(F)):
+        orc.lib().orc_svd_rotation2(orc.fp(F[i]), float(det[i]), 0.1, orc.fp(want[i]))
+    np.testing.assert_array_equal(R, want)
+
+
+def _squashed_state(pos):
+    """Upper half compressed to 4% of its height (det F = 0.04 <= 0.1), its top
+    quarter also mirrored in x (det F = -0.04, inverted)."""
+    x = pos.reshape(-1, 2).copy()
+    up = x[:, 1] > 0
+    x[up, 1] *= 0.04
+    top = pos.reshape(-1, 2)[:, 1] > 0.5 * pos.reshape(-1, 2)[:, 1].max()
+    x[top, 0] *= -1.0
+    return np.ascontiguousarray(x.ravel())
+
+
+@pytest.mark.parametrize("rm", ["MSVD", "DAPD"])
+def test_degenerate_step_vs_oracle(gpu, orc, rm):
+    """A converged step from a state with degenerate and inverted triangles:
+    MSVD takes the 2x2 SVD rotation there, DAPD R = Id (counted as fallbacks);
+    positions within 1e-4 of the oracle relative to the displacement."""
+    pos, tris, knc = grid(gpu, 12)
+    x0 = _squashed_state(pos)
+    s = gpu.LeafSolid2D(pos, tris, params(gpu, max_iter=1000, rel=1e-6, rm=getattr(gpu, "RM_" + rm)), knc=knc)
+    o = orc.Solid2(pos, tris, oracle_params(orc, max_iter=1000, rel=1e-6, rm=getattr(orc, "RM_" + rm)), knc=knc)
+    v0 = np.zeros_like(x0)
+    s.set_state(x0, v0)
+    o.set_state(x0, v0)
+    sg, so = s.fixed_step(), o.fixed_step()
+    assert sg.num_degenerate == so.num_degenerate > 0
+    X = x0.reshape(-1, 2)
+    t = tris.reshape(-1, 3)
+    P = pos.reshape(-1, 2)
+    def cross(a, b):
+        return a[:, 0] * b[:, 1] - a[:, 1] * b[:, 0]
+
+    dF = cross(X[t[:, 1]] - X[t[:, 0]], X[t[:, 2]] - X[t[:, 0]]) / cross(P[t[:, 1]] - P[t[:, 0]], P[t[:, 2]] - P[t[:, 0]])
+    n_low = int((dF <= 0.1).sum())
+    assert n_low > 20
+    assert s.num_fallback() == (n_low if rm == "DAPD" else 0)
+    xg, _ = s.state()
+    xo, _ = o.state()
+    d = np.abs(xg - xo).max() / np.abs(xo - x0).max()
+    assert d <= 1e-4, d
+    s.close()
+
+
+def test_msvd_and_dapd_differ_on_degenerate_elements(gpu):
+    """The SVD branch is exercised: the same degenerate state steps differently
+    under MSVD (SVD rotation) and DAPD (R = Id)."""
+    pos, tris, knc = grid(gpu, 12)
+    x0 = _squashed_state(pos)
+    out = []
+    for rm in (gpu.RM_MSVD, gpu.RM_DAPD):
+        s = gpu.LeafSolid2D(pos, tris, params(gpu, rm=rm), knc=knc)
+        s.set_state(x0, np.zeros_like(x0))
+        s.fixed_step()
+        out.append(s.state()[0])
+        s.close()
+    assert np.abs(out[0] - out[1]).max() > 1e-3 * np.abs(out[1] - x0).max()
 
 
 def test_too_large_mesh_is_refused(gpu):

@@ -61,14 +61,15 @@ def test_2d_matches_reference_golden(orc):
         K = np.zeros(36, np.float32)
         a = L.orc_compute_K2(orc.fp(r), float(g["E"][i]), float(g["nu"][i]), orc.fp(K))
         assert a == g["A"][i] and np.array_equal(K, g["K"][i]), i
+    assert (g["det"] <= 0.1).sum() >= 50  # the fixture exercises the 2x2 SVD branch
     for i in range(len(g["det"])):
-        if not g["det"][i] > 0.1:
-            continue  # the degenerate 2x2 SVD branch is not restated
         F = np.ascontiguousarray(g["F"][i])
         R = np.zeros(4, np.float32)
-        L.orc_pd2(orc.fp(F), float(g["det"][i]), orc.fp(R))
-        assert np.array_equal(R, g["R_pd"][i]), i
-        assert np.array_equal(R, g["R_svd"][i]), i  # Compute_R_SVD takes the polar branch
+        if g["det"][i] > 0.1:
+            L.orc_pd2(orc.fp(F), float(g["det"][i]), orc.fp(R))
+            assert np.array_equal(R, g["R_pd"][i]), i
+        L.orc_svd_rotation2(orc.fp(F), float(g["det"][i]), 0.1, orc.fp(R))
+        assert np.array_equal(R, g["R_svd"][i]), i  # polar above the threshold, 2x2 SVD below
 
 
 def test_oracle_against_compiled_reference_live(orc):
@@ -92,6 +93,20 @@ def test_oracle_against_compiled_reference_live(orc):
         L.orc_svd_rotation3(orc.fp(F), orc.fp(a))
         R.ref_R_svd3(orc.fp(F), d, 0.1, orc.fp(b))
         assert np.array_equal(a, b)
+    # 2x2 SVD branch: generic, small, inverted, rank-1 and near rank-1 F
+    F2 = [rng.uniform(-1, 1, 4) for _ in range(2000)] + [rng.uniform(-0.3, 0.3, 4) for _ in range(2000)]
+    for _ in range(500):
+        u, v = rng.normal(size=2), rng.normal(size=2)
+        F2.append(np.outer(u, v).ravel())
+        F2.append((np.outer(u, v) + 1e-4 * rng.normal(size=(2, 2))).ravel())
+    F2 += [np.zeros(4), np.array([1e-5, 0, 0, 1e-5]), np.array([0, 0, 0, 1.0]), np.array([-1, 0, 0, 1.0])]
+    for f in np.array(F2, np.float32):
+        f = np.ascontiguousarray(f)
+        d = np.float32(f[0] * f[3] - f[2] * f[1])
+        a, b = np.zeros(4, np.float32), np.zeros(4, np.float32)
+        L.orc_svd_rotation2(orc.fp(f), float(d), 0.1, orc.fp(a))
+        R.ref_R_svd2(orc.fp(f), float(d), 0.1, orc.fp(b))
+        assert np.array_equal(a, b), f
 
 
 # ------------------------------------------------------------ closed forms
@@ -169,6 +184,50 @@ def test_polar_newton_closed_form(orc):
         assert g["det"][i] > 0.1
         assert 1 <= it[0] <= 10
         assert np.abs(R.reshape(3, 3) - Q).max() < 0.06
+
+
+def test_svd2_degenerate_closed_forms(orc):
+    """The 2x2 SVD branch (det F <= 0.1) against what the mathematics fixes:
+    F = Q1 diag(s0, +-s1) Q2^T with s0 > s1 >= 0 and Q1, Q2 proper rotations
+    gives R = Q1 Q2^T (the polar factor when det F > 0; with the smallest
+    singular direction flipped when F is inverted); a rank-1 F = s u v^T maps
+    v onto u for 0.05 <= s <= 0.1 (the branch tests singular values and rows
+    of F^T F - lambda Id against absolute epsilons: above s = 0.1 the float32
+    noise of the null singular value, about 3.5e-4 s, passes its 1e-4 zero
+    test, below s = 0.05 whole rows fall under it; outside the window the
+    reference itself returns no rotation, and only the bit-exact comparison
+    with the compiled reference covers it); F = 0 gives the identity. The branch works on the eigenvectors of
+    F^T F in float32, so its error grows with the squared condition number
+    kappa = s0 / s1: the tolerance is 2e-4 + 60 eps kappa^2 (kappa <= 50),
+    far below the O(1) error of a flipped sign or a transposed factor."""
+    rng = np.random.default_rng(21)
+    L = orc.lib()
+
+    def rot(t):
+        return np.array([[np.cos(t), -np.sin(t)], [np.sin(t), np.cos(t)]])
+
+    def R_of(F):
+        F = np.ascontiguousarray(F, np.float32).ravel()
+        d = np.float32(F[0] * F[3] - F[2] * F[1])
+        assert d <= 0.1
+        R = np.zeros(4, np.float32)
+        L.orc_svd_rotation2(orc.fp(F), float(d), 0.1, orc.fp(R))
+        return R.reshape(2, 2).astype(np.float64)
+
+    for _ in range(2000):
+        Q1, Q2 = rot(rng.uniform(0, 2 * np.pi)), rot(rng.uniform(0, 2 * np.pi))
+        s0 = rng.uniform(0.3, 1.5)
+        s1 = rng.uniform(0.02 * s0, min(0.09 / s0, 0.5 * s0))
+        tol = 2e-4 + 60 * np.finfo(np.float32).eps * (s0 / s1) ** 2
+        for sign in (1.0, -1.0):
+            R = R_of(Q1 @ np.diag([s0, sign * s1]) @ Q2.T)
+            assert np.abs(R - Q1 @ Q2.T).max() <= tol, (s0, sign * s1)
+            assert abs(np.linalg.det(R) - 1.0) <= tol and np.abs(R @ R.T - np.eye(2)).max() <= tol
+        u, v = Q1[:, 0], Q2[:, 0]
+        R = R_of(rng.uniform(0.05, 0.1) * np.outer(u, v))
+        assert np.abs(R @ v - u).max() <= 2e-4
+        assert abs(np.linalg.det(R) - 1.0) <= 1e-5
+    assert np.array_equal(R_of(np.zeros((2, 2))), np.eye(2))
 
 
 def test_mesh_box_conformity(orc, cf):
