@@ -82,14 +82,11 @@ __global__ __launch_bounds__(TB) void kb_tiles(BatchTables t, int sel) {
     if (br.slot == kIdleSlot) return;
     if (sel != kSelYAll && t.cgs[br.slot].phase != (sel == kSelD ? kPhaseIter : kPhaseStart)) return;
     extern __shared__ float smem_f[];
-    const Slots s_v = make_slots(smem_f, t.max_tn);
-    const Slots s_r = make_slots(smem_f + 3 * t.max_tn, t.max_tn);
-    const Slots s_F = make_slots(smem_f + 6 * t.max_tn, 4 * t.max_te);
-    uint32_t* s_sp = reinterpret_cast<uint32_t*>(s_F.z + 4 * t.max_te);
+    const TileLds<TB> L(smem_f);
     const SimDev& s = t.sims[br.slot];
     const float* in = (MODE == kModeFe) ? s.x : (sel == kSelD ? s.d : s.y);
     const TileArgs a = tile_args(s, in, t.max_tn, t.max_te);
-    tile_block<MODE>(a, br.blk, s_v, s_r, s_F, s_sp);
+    tile_block<MODE>(a, br.blk, L.v, L.r, L.F, L.sp);
 }
 
 // b = dt (M g + f_e) + M v, y = v, KNC rows (Step_QuasiImplicitEuler_v,
@@ -316,11 +313,6 @@ int check_launch_b() {
     return CF_OK;
 }
 inline uint32_t mesh_grid(const BatchTables& t) { return (t.n_active + 3) / 4; }
-size_t tile_smem_b(const BatchTables& t) {
-    size_t b = sizeof(float) * (6 * (size_t)t.max_tn + 12 * (size_t)t.max_te);
-    b += sizeof(uint32_t) * ((size_t)t.max_tn + 1);
-    return (b + 63) & ~size_t(63);
-}
 }  // namespace
 
 int launch_batch_begin_eval(const BatchTables& t, hipStream_t s) {
@@ -331,9 +323,9 @@ int launch_batch_begin_eval(const BatchTables& t, hipStream_t s) {
 
 int launch_batch_forces(const BatchTables& t, int mode, int sel, hipStream_t s) {
     if (!t.n_tile) return CF_OK;
-    const size_t sm = tile_smem_b(t);
 #define CF_BTILE(TBV)                                                                                     \
     do {                                                                                                  \
+        const size_t sm = TileLds<TBV>::kBytes;                                                           \
         if (mode == kModeFe) hipLaunchKernelGGL((kb_tiles<kModeFe, TBV>), dim3(t.n_tile), dim3(TBV), sm, s, t, sel); \
         else hipLaunchKernelGGL((kb_tiles<kModeDfx, TBV>), dim3(t.n_tile), dim3(TBV), sm, s, t, sel);      \
     } while (0)

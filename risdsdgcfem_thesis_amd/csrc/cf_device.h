@@ -331,23 +331,54 @@ struct TileRegs {
     uint32_t f[4];
 };
 
+// Element i of an array addressed by a 32-bit byte offset from a uniform
+// base: the load takes the base in scalar registers and one 32-bit vector
+// offset, with no 64-bit address arithmetic per lane. Every per-mesh array
+// the tile stage reads is below 4 GiB (T, N < 2^28 are checked on the host).
+template <typename V>
+__device__ inline V ld32(const V* base, uint32_t i) {
+    return *reinterpret_cast<const V*>(reinterpret_cast<const char*>(base) + i * uint32_t(sizeof(V)));
+}
+
 __device__ inline void load_tile_regs(const TileArgs& a, const TileDesc& d, uint32_t gid, uint32_t tid,
                                       TileRegs& g) {
-    const size_t T = a.T;
     if (tid < d.nn) {
-        g.v[0] = a.in[3 * (size_t)gid + 0]; g.v[1] = a.in[3 * (size_t)gid + 1]; g.v[2] = a.in[3 * (size_t)gid + 2];
-        g.r[0] = a.ref[3 * (size_t)gid + 0]; g.r[1] = a.ref[3 * (size_t)gid + 1]; g.r[2] = a.ref[3 * (size_t)gid + 2];
+        g.v[0] = ld32(a.in, 3 * gid + 0); g.v[1] = ld32(a.in, 3 * gid + 1); g.v[2] = ld32(a.in, 3 * gid + 2);
+        g.r[0] = ld32(a.ref, 3 * gid + 0); g.r[1] = ld32(a.ref, 3 * gid + 1); g.r[2] = ld32(a.ref, 3 * gid + 2);
     }
-    if (tid <= d.nn) g.sp = a.slot_ptr[d.n0 + tid] - 4 * d.e0;
-    if (tid < d.nn) g.pd = a.part_dst[d.n0 + tid];
+    if (tid <= d.nn) g.sp = ld32(a.slot_ptr, d.n0 + tid) - 4 * d.e0;
+    if (tid < d.nn) g.pd = ld32(a.part_dst, d.n0 + tid);
     if (tid < d.ne) {
-        const size_t e = d.e0 + tid;
-        const uint32_t lv = a.lv[e], rk = a.rk[e];
+        const uint32_t e = d.e0 + tid;
+        const uint32_t lv = ld32(a.lv, e), rk = ld32(a.rk, e);
 #pragma unroll
         for (int j = 0; j < 4; j++) { g.l[j] = (lv >> (8 * j)) & 255u; g.f[j] = (rk >> (8 * j)) & 255u; }
 #pragma unroll
-        for (int k = 0; k < 9; k++) g.R[k] = a.R[k * T + e];
+        for (int k = 0; k < 9; k++) g.R[k] = ld32(a.R + k * (size_t)a.T, e);
     }
+}
+
+// The same registers for a tile whose loads start cold, in an order that
+// keeps bytes in flight: the tile's node ids first, then the element stream
+// (R and the packed ids/ranks) and the slot offsets while the ids are in
+// flight, then the node gathers, which need the ids. Lanes past the tile's
+// elements or nodes load a valid duplicate instead of branching around the
+// loads (their values are not used).
+__device__ inline void load_tile_regs_cold(const TileArgs& a, const TileDesc& d, uint32_t tid, TileRegs& g) {
+    const uint32_t tn = min(tid, d.nn - 1);
+    const uint32_t gid = ld32(a.tile_nodes, d.n0 + tn);
+    const uint32_t e = d.e0 + min(tid, d.ne - 1);
+    const uint32_t lv = ld32(a.lv, e), rk = ld32(a.rk, e);
+#pragma unroll
+    for (int k = 0; k < 9; k++) g.R[k] = ld32(a.R + k * (size_t)a.T, e);
+    g.sp = ld32(a.slot_ptr, d.n0 + min(tid, d.nn)) - 4 * d.e0;
+    g.pd = ld32(a.part_dst, d.n0 + tn);
+    if (tid < d.nn) {
+        g.v[0] = ld32(a.in, 3 * gid + 0); g.v[1] = ld32(a.in, 3 * gid + 1); g.v[2] = ld32(a.in, 3 * gid + 2);
+        g.r[0] = ld32(a.ref, 3 * gid + 0); g.r[1] = ld32(a.ref, 3 * gid + 1); g.r[2] = ld32(a.ref, 3 * gid + 2);
+    }
+#pragma unroll
+    for (int j = 0; j < 4; j++) { g.l[j] = (lv >> (8 * j)) & 255u; g.f[j] = (rk >> (8 * j)) & 255u; }
 }
 
 // Nodal forces of one element into its four tile force slots.
@@ -360,6 +391,54 @@ struct Slots {
     float* z;
 };
 __device__ inline Slots make_slots(float* base, uint32_t n) { return Slots{base, base + n, base + 2 * n}; }
+
+// Physical LDS word of force slot s: bits 0-4 XORed with bits 5-9. Tile
+// nodes are sorted by valence, so the slot runs summed by neighbouring lanes
+// often start 32, 16 or 8 slots apart; unswizzled, those lanes would hit the
+// same few LDS banks on every step of the run. The XOR stays inside each
+// 32-slot block (the slot planes are multiples of 32 long).
+__device__ inline uint32_t slot_swizzle(uint32_t s) { return s ^ ((s >> 5) & 31u); }
+
+
+// LDS layout of a tile workgroup of TB threads (at most TB - 1 tile nodes and
+// TB elements): node vectors and rest positions as planes of TB floats, the
+// 4 TB force slots as planes of 4 TB floats (swizzled, slot_swizzle), then
+// TB + 1 slot-run starts.
+// Compile-time plane strides let one LDS address serve the x, y and z planes
+// (immediate offsets).
+template <int TB>
+struct TileLds {
+    static_assert(TB % 8 == 0, "slot planes must be whole 32-slot swizzle blocks");
+    static constexpr uint32_t kFloats = 18 * TB;
+    static constexpr size_t kBytes = (sizeof(float) * kFloats + sizeof(uint32_t) * (TB + 1) + 63) & ~size_t(63);
+    Slots v, r, F;
+    uint32_t* sp;
+    __device__ explicit TileLds(float* base)
+        : v(make_slots(base, TB)), r(make_slots(base + 3 * TB, TB)), F(make_slots(base + 6 * TB, 4 * TB)),
+          sp(reinterpret_cast<uint32_t*>(base + kFloats)) {}
+};
+
+// Sum of one node's slot run [s0, s1) in ascending slot order; the loads of
+// four slots are issued together, the adds stay sequential.
+__device__ inline void sum_slot_run(const Slots& s_F, uint32_t s0, uint32_t s1, float& fx, float& fy, float& fz) {
+#pragma clang fp contract(off)
+    fx = 0.f; fy = 0.f; fz = 0.f;
+    uint32_t i = s0;
+    for (; i + 4 <= s1; i += 4) {
+        float ax[4], ay[4], az[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const uint32_t p = slot_swizzle(i + q);
+            ax[q] = s_F.x[p]; ay[q] = s_F.y[p]; az[q] = s_F.z[p];
+        }
+#pragma unroll
+        for (int q = 0; q < 4; q++) { fx += ax[q]; fy += ay[q]; fz += az[q]; }
+    }
+    for (; i < s1; i++) {
+        const uint32_t p = slot_swizzle(i);
+        fx += s_F.x[p]; fy += s_F.y[p]; fz += s_F.z[p];
+    }
+}
 
 template <int MODE>
 __device__ inline void element_force(const Slots& s_v, const Slots& s_r, const Slots& s_F, const uint32_t* lv,
@@ -424,10 +503,12 @@ __device__ inline void element_force(const Slots& s_v, const Slots& s_r, const S
 #pragma unroll
         for (int c = 0; c < 3; c++)
             P[3 * r + c] = fma3(R[3 * r + 0], W[0 + c], R[3 * r + 1], W[3 + c], R[3 * r + 2], W[6 + c]);
-    s_F.x[fs[1]] = P[0]; s_F.y[fs[1]] = P[3]; s_F.z[fs[1]] = P[6];
-    s_F.x[fs[2]] = P[1]; s_F.y[fs[2]] = P[4]; s_F.z[fs[2]] = P[7];
-    s_F.x[fs[3]] = P[2]; s_F.y[fs[3]] = P[5]; s_F.z[fs[3]] = P[8];
-    s_F.x[fs[0]] = -(P[0] + P[1] + P[2]); s_F.y[fs[0]] = -(P[3] + P[4] + P[5]); s_F.z[fs[0]] = -(P[6] + P[7] + P[8]);
+    const uint32_t f0 = slot_swizzle(fs[0]), f1 = slot_swizzle(fs[1]), f2 = slot_swizzle(fs[2]),
+                   f3 = slot_swizzle(fs[3]);
+    s_F.x[f1] = P[0]; s_F.y[f1] = P[3]; s_F.z[f1] = P[6];
+    s_F.x[f2] = P[1]; s_F.y[f2] = P[4]; s_F.z[f2] = P[7];
+    s_F.x[f3] = P[2]; s_F.y[f3] = P[5]; s_F.z[f3] = P[8];
+    s_F.x[f0] = -(P[0] + P[1] + P[2]); s_F.y[f0] = -(P[3] + P[4] + P[5]); s_F.z[f0] = -(P[6] + P[7] + P[8]);
 }
 
 
@@ -509,10 +590,7 @@ __device__ inline void tile_block(const TileArgs& a, uint32_t t, const Slots& s_
     const uint32_t tid = threadIdx.x;
     const TileDesc dt = tile_desc(a, t);
     TileRegs g;
-    g.sp = 0;
-    g.pd = 0;
-    const uint32_t gid = (tid < dt.nn) ? a.tile_nodes[dt.n0 + tid] : 0u;
-    load_tile_regs(a, dt, gid, tid, g);
+    load_tile_regs_cold(a, dt, tid, g);
     if (tid < dt.nn) {
         s_v.x[tid] = g.v[0]; s_v.y[tid] = g.v[1]; s_v.z[tid] = g.v[2];
         s_r.x[tid] = g.r[0]; s_r.y[tid] = g.r[1]; s_r.z[tid] = g.r[2];
@@ -527,11 +605,8 @@ __device__ inline void tile_block(const TileArgs& a, uint32_t t, const Slots& s_
     }
     lds_barrier();
     if (tid < dt.nn) {
-        float fx = 0.f, fy = 0.f, fz = 0.f;
-        const uint32_t s1 = s_sp[tid + 1];
-        for (uint32_t sidx = s_sp[tid]; sidx < s1; sidx++) {
-            fx += s_F.x[sidx]; fy += s_F.y[sidx]; fz += s_F.z[sidx];
-        }
+        float fx, fy, fz;
+        sum_slot_run(s_F, s_sp[tid], s_sp[tid + 1], fx, fy, fz);
         float* o = a.partials + 3 * (size_t)g.pd;
         o[0] = fx; o[1] = fy; o[2] = fz;
     }

@@ -68,10 +68,11 @@ template <int MODE, int TB, bool PIPE>
 __global__ __launch_bounds__(TB) void k_tile_forces(TileArgs a) {
     if (a.skip && *a.skip) return;
     extern __shared__ float smem_f[];
-    const Slots s_v = make_slots(smem_f, a.max_tn);                     // tile node vectors
-    const Slots s_r = make_slots(smem_f + 3 * a.max_tn, a.max_tn);      // tile node rest positions
-    const Slots s_F = make_slots(smem_f + 6 * a.max_tn, 4 * a.max_te);  // 4 force slots per element
-    uint32_t* s_sp = reinterpret_cast<uint32_t*>(s_F.z + 4 * a.max_te);  // max_tn+1 slot-run starts
+    const TileLds<TB> L(smem_f);
+    const Slots& s_v = L.v;  // tile node vectors
+    const Slots& s_r = L.r;  // tile node rest positions
+    const Slots& s_F = L.F;  // 4 force slots per element
+    uint32_t* s_sp = L.sp;   // slot-run starts
 
     const uint32_t tid = threadIdx.x;
     const uint32_t gx = gridDim.x / 8, xcd = blockIdx.x % 8, j = blockIdx.x / 8;
@@ -83,18 +84,13 @@ __global__ __launch_bounds__(TB) void k_tile_forces(TileArgs a) {
 
     TileDesc dt = tile_desc(a, t);
     TileRegs g;
-    g.sp = 0;
-    g.pd = 0;
-    {
-        const uint32_t gid = (tid < dt.nn) ? a.tile_nodes[dt.n0 + tid] : 0u;
-        load_tile_regs(a, dt, gid, tid, g);
-    }
+    load_tile_regs_cold(a, dt, tid, g);
     uint32_t t1 = t + gx;
     TileDesc d1 = dt;
     uint32_t gid1 = 0;
     if (PIPE && t1 < hi) {
         d1 = tile_desc(a, t1);
-        gid1 = (tid < d1.nn) ? a.tile_nodes[d1.n0 + tid] : 0u;
+        gid1 = (tid < d1.nn) ? ld32(a.tile_nodes, d1.n0 + tid) : 0u;
     }
     while (true) {
         // stage tile t into LDS
@@ -117,7 +113,7 @@ __global__ __launch_bounds__(TB) void k_tile_forces(TileArgs a) {
         if (PIPE && t1 < hi) load_tile_regs(a, d1, gid1, tid, g);
         if (PIPE && t2 < hi) {
             d2 = tile_desc(a, t2);
-            gid2 = (tid < d2.nn) ? a.tile_nodes[d2.n0 + tid] : 0u;
+            gid2 = (tid < d2.nn) ? ld32(a.tile_nodes, d2.n0 + tid) : 0u;
         }
         // element forces of tile t into the force slots
         if (tid < dt.ne) {
@@ -129,11 +125,8 @@ __global__ __launch_bounds__(TB) void k_tile_forces(TileArgs a) {
         lds_barrier();
         // per tile node: its contiguous slot run (ascending element order) -> one partial
         if (tid < dt.nn) {
-            float fx = 0.f, fy = 0.f, fz = 0.f;
-            const uint32_t s1 = s_sp[tid + 1];
-            for (uint32_t sidx = s_sp[tid]; sidx < s1; sidx++) {
-                fx += s_F.x[sidx]; fy += s_F.y[sidx]; fz += s_F.z[sidx];
-            }
+            float fx, fy, fz;
+            sum_slot_run(s_F, s_sp[tid], s_sp[tid + 1], fx, fy, fz);
             float* o = a.partials + 3 * (size_t)cpd;
             o[0] = fx; o[1] = fy; o[2] = fz;
         }
@@ -331,11 +324,6 @@ int launch_begin_eval(const TetDevice& t, const float* x, int rm, float thr, flo
     return check_launch();
 }
 
-static size_t tile_smem(const TetDevice& t) {
-    size_t b = sizeof(float) * (6 * (size_t)t.max_tile_nodes + 12 * (size_t)t.max_tile_elems);
-    b += sizeof(uint32_t) * ((size_t)t.max_tile_nodes + 1);
-    return (b + 63) & ~size_t(63);
-}
 
 int launch_forces_skip(const TetDevice& t, int mode, const float* in, const float* R, Material m,
                        const int32_t* skip, hipStream_t s) {
@@ -347,11 +335,11 @@ int launch_forces_skip(const TetDevice& t, int mode, const float* in, const floa
     a.slot_ptr = t.slot_ptr; a.lv = t.lv; a.rk = t.rk; a.R = R; a.dinv = t.dinv; a.vol = t.vol;
     a.ref = t.ref;
     a.in = in; a.partials = t.partials; a.m = m; a.skip = skip;
-    const size_t sm = tile_smem(t);
     const uint32_t grid = t.tile_grid;
     const bool fe = mode == kModeFe;
-#define CF_TILE_LAUNCH(TBV, PV)                                                                        \
-    do {                                                                                              \
+#define CF_TILE_LAUNCH(TBV, PV)                                                                          \
+    do {                                                                                                \
+        const size_t sm = TileLds<TBV>::kBytes;                                                         \
         if (fe) hipLaunchKernelGGL((k_tile_forces<kModeFe, TBV, PV>), dim3(grid), dim3(TBV), sm, s, a); \
         else hipLaunchKernelGGL((k_tile_forces<kModeDfx, TBV, PV>), dim3(grid), dim3(TBV), sm, s, a);   \
     } while (0)
@@ -380,13 +368,12 @@ int launch_forces_skip(const TetDevice& t, int mode, const float* in, const floa
 int tile_grid_size(const TetDevice& t, int device, int persistent) {
     int cus = 0, per_cu = 0;
     CF_CHECK_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
-    const size_t sm = tile_smem(t);
     hipError_t e;
     switch (t.tile_threads) {
-        case 1024: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_tile_forces<kModeDfx, 1024, true>, 1024, sm); break;
-        case 512: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_tile_forces<kModeDfx, 512, true>, 512, sm); break;
-        case 128: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_tile_forces<kModeDfx, 128, true>, 128, sm); break;
-        default: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_tile_forces<kModeDfx, 256, true>, 256, sm); break;
+        case 1024: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_tile_forces<kModeDfx, 1024, true>, 1024, TileLds<1024>::kBytes); break;
+        case 512: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_tile_forces<kModeDfx, 512, true>, 512, TileLds<512>::kBytes); break;
+        case 128: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_tile_forces<kModeDfx, 128, true>, 128, TileLds<128>::kBytes); break;
+        default: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_tile_forces<kModeDfx, 256, true>, 256, TileLds<256>::kBytes); break;
     }
     if (e != hipSuccess || per_cu < 1) per_cu = 1;
     const uint32_t per_xcd_tiles = (t.num_tiles + 7) / 8;

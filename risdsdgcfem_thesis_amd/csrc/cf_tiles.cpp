@@ -42,9 +42,13 @@ uint32_t default_tile_elems() { return 256; }
 
 int build_tiles(uint32_t N, uint32_t T, const float* ref, const uint32_t* tets, uint32_t tile_elems,
                 uint32_t max_per_tile, TileHost& out) {
+    // the tile stage addresses per-mesh arrays with 32-bit byte offsets
+    if (N >= (1u << 28) || T >= (1u << 28)) {
+        set_error("mesh too large for one element set: %u nodes, %u elements (limit 2^28 each)", N, T);
+        return CF_ERR_UNSUPPORTED;
+    }
     if (tile_elems == 0) tile_elems = default_tile_elems();
     if (tile_elems > max_per_tile) tile_elems = max_per_tile;  // one element per thread
-    if (tile_elems > 16383) tile_elems = 16383;  // slot codes are uint16 (elem*4+vertex)
     // one node per thread, and one more thread stages the end of the last
     // node's slot run: at most max_per_tile - 1 nodes
     const uint32_t max_nodes = std::min(kMaxTileNodes, max_per_tile) - 1;
