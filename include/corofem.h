@@ -87,7 +87,18 @@ typedef struct cf_solid_params {
     int32_t warm_start;        /* default 1    */
     int32_t compute_energy;    /* 1: elastic energy each step like FixedStep (:977); default 1 */
     float thickness;           /* 2D only (LeafDSH_Solid2D_FEM.h:217); default 1 */
+    int32_t integrator;        /* CF_IT_QIE_V (default) or CF_IT_FIE_DX (.h:109, dispatch .cpp:944-963) */
+    uint32_t nr_max_iter;      /* FIE-dx Newton iterations (m_SolverNR_MaxIter, .h:233); default 3 */
+    float nr_abs_epsilon;      /* FIE-dx stop: |b| < this (m_SolverNR_AbsEpsilon, .h:236-238); default 1e-6 */
 } cf_solid_params;
+
+/* Integrators (LeafDSH_Solid3D_FEM.h:109). CF_IT_FIE_DX is
+   Step_FullyImplicitEuler_dx (.cpp:1357-1552): Newton iterations in position
+   increments with A = (1/dt^2 + c0/dt) M + (1 + c1/dt) K, a BeginEvaluation per
+   iteration and KNC rows in increment form; 3D leaves and the multi-solid
+   step only (batches and 2D solids take CF_IT_QIE_V). */
+#define CF_IT_QIE_V 1
+#define CF_IT_FIE_DX 3
 
 /* Per-step statistics (LeafDSH_Solid3D_FEM::QueryStats fields, .cpp:698-770). */
 typedef struct cf_step_stats {
@@ -101,6 +112,9 @@ typedef struct cf_step_stats {
     float elastic_energy;
     float min_det_f;
     double step_ms;               /* device time of the last fixed step */
+    int32_t nr_iters;             /* FIE-dx: Newton iterations of the last fixed step */
+    float nr_rel_residual;        /* FIE-dx: last |b| or |y| over the first |b| (m_SolverNR_RelPrec) */
+    float nr_abs_residual;        /* FIE-dx: last |b| or |y| (m_SolverNR_AbsPrec) */
 } cf_step_stats;
 
 typedef struct cf_cg_result {
@@ -224,6 +238,21 @@ int cf_solid_time_df_x(cf_solid* s, uint32_t reps, double* ms_per_call);
    the solid's stream over reps back-to-back launches: the LDS-tile element
    kernel and the node gather (matvec form with the mass term and d.q). */
 int cf_solid_time_kx(cf_solid* s, uint32_t reps, double* ms_tile, double* ms_gather);
+
+/* ---------------------------------------------------------------- multi-solid
+ * SS_AggregateDSH_Basic::MultiDSH_Solid3D_Update (ds/ss/SS_AggregateDSH_Basic.cpp:229-300;
+ * the universe step of several Solid3D leaves, __ENABLE_MULTI_SOLID3D), without
+ * the contact passes: fixed dt = the smallest of the solids' fixed_dt, Newton
+ * passes = the largest nr_max_iter, one time accumulator with strict '>'. Each
+ * fixed step runs the leaves' NR_Init / NR_PreStep / NR_MidStep / NR_PostStep /
+ * NR_End (ds/dsh/LeafDSH_Solid3D_FEM.cpp:2824-3039) with the frame dt, whatever
+ * each leaf's integrator. The solids stay owned by the caller. */
+typedef struct cf_multi cf_multi;
+int cf_multi_create(cf_solid* const* solids, uint32_t n, cf_multi** out);
+int cf_multi_destroy(cf_multi* m);
+/* stats: NULL or one per solid (its last fixed step; elastic_energy 0, as the
+   NR_End hook does not evaluate V). */
+int cf_multi_update(cf_multi* m, float dt, uint32_t* num_fixed_steps, cf_step_stats* stats);
 
 /* ---------------------------------------------------------------- tuning */
 /* Elements per LDS tile of the node-gather K*x (0 = library default 256).

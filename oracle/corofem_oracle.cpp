@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstring>
 #include <cstdlib>
+#include <limits>
 #include <vector>
 #include <algorithm>
 #include <utility>
@@ -728,9 +729,86 @@ struct orc_solid {
     float c[2] = {0, 0};           // Rayleigh
     std::vector<uint32_t> knc;
     std::vector<float> b, y, r, d, q;
+    std::vector<float> xk, vk;     // FIE-dx iterates (m_IE_vec_x_k, m_IE_vec_v_k)
     float acc_time = 0.0f;
     float total_time = 0.0f;
+    // last BeginEvaluation's counts
+    uint32_t ndeg = 0;
+    float sdeg = 0.0f, min_det = 0.0f;
+    int32_t ls_iters = 0;
+    float ls_rel = 0.0f, ls_abs = 0.0f;
 };
+
+namespace {
+
+void solid_begin_evaluation(orc_solid* s, const float* x) {
+    orc_begin_evaluation(s->tes, x, s->p.rm, s->p.degenerate_threshold, nullptr, nullptr, &s->ndeg, &s->sdeg);
+    s->min_det = 1e30f;
+    for (uint32_t e = 0; e < s->tes->T; e++) s->min_det = std::min(s->min_det, s->tes->detF[e]);
+}
+
+// LS_Solve (ds/dsh/LeafDSH_Solid3D_FEM.cpp:1081-1155): CG on A y = b with
+// A = fM M - fK df_x (LS_A_times_d :1056-1079) and KNC rows zeroed (:1045-1053).
+void solid_solve(orc_solid* s, float fM, float fK) {
+    const uint32_t N = s->N;
+    const size_t n3 = 3 * size_t(N);
+    const float mass = 1.0f / s->inv_mass;
+    auto Av = [&](float* q, const float* d) {
+        std::memset(q, 0, n3 * sizeof(float));
+        orc_df_x(s->tes, d, q);
+        for (size_t i = 0; i < n3; i++) q[i] *= -fK;
+        for (uint32_t i = 0; i < N; i++) {
+            float factor = mass * fM;
+            q[3 * i] += factor * d[3 * i];
+            q[3 * i + 1] += factor * d[3 * i + 1];
+            q[3 * i + 2] += factor * d[3 * i + 2];
+        }
+    };
+    auto S = [&](float* w) {
+        for (uint32_t nid : s->knc) { w[3 * size_t(nid)] = 0; w[3 * size_t(nid) + 1] = 0; w[3 * size_t(nid) + 2] = 0; }
+    };
+    cg_solve(uint32_t(n3), Av, S, s->y.data(), s->b.data(), s->p.cg_rel_epsilon, s->p.cg_abs_epsilon, s->p.cg_max_iter,
+             s->r, s->d, s->q, &s->ls_iters, &s->ls_rel, &s->ls_abs, s->p.reduce_fp64 != 0);
+}
+
+float solid_norm(const orc_solid* s, const std::vector<float>& w) {  // ns::real_array::norm_2 (real_array.h:42-50)
+    const float sq = s->p.reduce_fp64 ? fp64_dot(w.data(), w.data(), uint32_t(w.size()))
+                                      : sse_dot(w.data(), w.data(), uint32_t(w.size()));
+    return std::sqrt(sq);
+}
+
+// Right-hand side of one FIE-dx Newton iteration at the iterate x_k, with the
+// elastic force evaluated at x_eval under the current rotations
+// (Step_FullyImplicitEuler_dx :1378-1465): b = M g + f_e + (1/dt) M (v - (x_k - x)/dt),
+// y = dt v on the first iteration (warm start) else 0, KNC rows b = y = target - x_k.
+void fie_rhs(orc_solid* s, float dt, float inv_dt, const float* x_eval, bool first) {
+    const uint32_t N = s->N;
+    const float mass = 1.0f / s->inv_mass;
+    for (uint32_t i = 0; i < N; i++)
+        for (int c = 0; c < 3; c++) s->b[3 * i + c] = 0.0f + mass * s->p.gravity[c];
+    orc_f_e(s->tes, x_eval, s->b.data());
+    const float sm = inv_dt * mass;
+    for (size_t i = 0; i < s->b.size(); i++) s->b[i] += sm * (s->v[i] - inv_dt * (s->xk[i] - s->x[i]));
+    if (first && s->p.warm_start) {
+        for (size_t i = 0; i < s->y.size(); i++) s->y[i] = dt * s->v[i];
+    } else {
+        std::fill(s->y.begin(), s->y.end(), 0.0f);
+    }
+    for (uint32_t nid : s->knc)
+        for (int c = 0; c < 3; c++) {
+            const size_t i = 3 * size_t(nid) + c;
+            s->b[i] = s->tes->ref[i] - s->xk[i];
+            s->y[i] = s->tes->ref[i] - s->xk[i];
+        }
+}
+
+// x_k += y ; v_k = (x_k - x) / dt (:1500-1511)
+void fie_update(orc_solid* s, float inv_dt) {
+    for (size_t i = 0; i < s->xk.size(); i++) s->xk[i] = s->xk[i] + s->y[i];
+    for (size_t i = 0; i < s->vk.size(); i++) s->vk[i] = inv_dt * (s->xk[i] - s->x[i]);
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -758,8 +836,60 @@ void orc_solid_set_knc(orc_solid* s, uint32_t n, const uint32_t* nids) { s->knc.
 float orc_solid_node_mass(const orc_solid* s) { return 1.0f / s->inv_mass; }
 void orc_solid_rayleigh(const orc_solid* s, float c[2]) { c[0] = s->c[0]; c[1] = s->c[1]; }
 
-// FixedStep (:898-990) with Step_QuasiImplicitEuler_v (:1205-1328)
+// Step_FullyImplicitEuler_dx (:1357-1552): Newton iterations on the implicit
+// Euler equations in position increments; dt = 0 is the quasi-static variant
+// (:957-959). Returns the iterations taken and the last |b| or |y|.
+static void step_fie_dx(orc_solid* s, float dt, int32_t* nr_iters, float* nr_rel, float* nr_abs, int32_t* ls_total) {
+    const float inv_dt = (dt == 0.0f) ? 0.0f : 1.0f / dt;
+    s->xk = s->x;
+    s->vk = s->v;
+    uint32_t nr_k = 0;
+    float delta_new = 0.0f, delta0 = -1.0f;
+    bool conv = false;
+    *ls_total = 0;
+    do {
+        fie_rhs(s, dt, inv_dt, s->xk.data(), nr_k == 0);
+        delta_new = solid_norm(s, s->b);  // after KNC (:1468-1474)
+        if (delta0 < 0) delta0 = delta_new;
+        conv = delta_new < s->p.nr_abs_epsilon;
+        if (!conv) {
+            const float fM = inv_dt * inv_dt + inv_dt * s->c[0];
+            const float fK = 1.0f + inv_dt * s->c[1];
+            solid_solve(s, fM, fK);
+            *ls_total += s->ls_iters;
+            fie_update(s, inv_dt);
+            delta_new = solid_norm(s, s->y);
+            nr_k++;
+            solid_begin_evaluation(s, s->xk.data());  // End/BeginEvaluation at x_k (:1526-1530)
+        }
+    } while (!conv && nr_k < s->p.nr_max_iter);
+    *nr_iters = int32_t(nr_k);
+    *nr_rel = (delta0 > 0) ? delta_new / delta0 : 0.0f;
+    *nr_abs = delta_new;
+    s->x = s->xk;
+    s->v = s->vk;
+}
+
+// FixedStep (:898-990) with Step_QuasiImplicitEuler_v (:1205-1328) or
+// Step_FullyImplicitEuler_dx (:1357-1552)
 void orc_solid_fixed_step(orc_solid* s, float dt, orc_step_stats* st) {
+    if (s->p.integrator == ORC_IT_FIE_DX) {
+        solid_begin_evaluation(s, s->x.data());
+        s->x0 = s->x;
+        int32_t nr = 0, ls = 0;
+        float nr_rel = 0, nr_abs = 0;
+        step_fie_dx(s, dt, &nr, &nr_rel, &nr_abs, &ls);
+        const float V = orc_V(s->tes, s->x.data());  // V() at the final iterate (:977)
+        if (st) {
+            st->num_fixed_steps = 1;
+            st->cg_iters = ls; st->cg_rel = s->ls_rel; st->cg_abs = s->ls_abs;
+            st->num_degenerate = s->ndeg; st->sum_degenerate_detF = s->sdeg;
+            st->elastic_energy = V; st->min_detF = s->min_det;
+            st->nr_iters = nr; st->nr_rel = nr_rel; st->nr_abs = nr_abs;
+        }
+        s->total_time += dt;
+        return;
+    }
     const uint32_t N = s->N;
     const size_t n3 = 3 * size_t(N);
     const float mass = 1.0f / s->inv_mass;  // GetMass (ms/ParticleSystem.h:74)
@@ -818,8 +948,65 @@ void orc_solid_fixed_step(orc_solid* s, float dt, orc_step_stats* st) {
         st->cg_iters = it; st->cg_rel = rr; st->cg_abs = ar;
         st->num_degenerate = ndeg; st->sum_degenerate_detF = sdeg;
         st->elastic_energy = V; st->min_detF = min_det;
+        st->nr_iters = 0; st->nr_rel = 0; st->nr_abs = 0;
     }
     s->total_time += dt;
+}
+
+// SS_AggregateDSH_Basic::MultiDSH_Solid3D_Update (ds/ss/SS_AggregateDSH_Basic.cpp:229-300)
+// without contacts, with the leaves' NR hooks (LeafDSH_Solid3D_FEM.cpp:2824-3039).
+// NR_PreStep resets x_k = x, v_k = v and the iteration count on every pass, and
+// the cache keeps pointing at x_k, so each pass evaluates f_e at x under the
+// rotations of the previous pass's end positions and warm-starts y = dt v.
+uint32_t orc_multi_update(orc_solid* const* solids, uint32_t n, float dt, float* acc_time, orc_step_stats* stats) {
+    float fixed_dt = std::numeric_limits<float>::infinity();
+    uint32_t max_nr = 0;
+    for (uint32_t i = 0; i < n; i++) {
+        fixed_dt = std::min(fixed_dt, solids[i]->p.fixed_dt);
+        max_nr = std::max(max_nr, solids[i]->p.nr_max_iter);
+        solids[i]->total_time += dt;
+    }
+    *acc_time += dt;
+    const float inv_dt = (dt == 0.0f) ? 0.0f : 1.0f / dt;
+    uint32_t steps = 0;
+    while (*acc_time > fixed_dt) {
+        for (uint32_t i = 0; i < n; i++) {  // NR_Init (:2824-2836)
+            orc_solid* s = solids[i];
+            solid_begin_evaluation(s, s->x.data());
+            s->x0 = s->x;
+        }
+        uint32_t nr = 0;
+        do {
+            for (uint32_t i = 0; i < n; i++) {  // NR_PreStep + NR_MidStep (:2857-2985)
+                orc_solid* s = solids[i];
+                s->xk = s->x;
+                s->vk = s->v;
+                fie_rhs(s, dt, inv_dt, s->xk.data(), true);
+                solid_solve(s, inv_dt * inv_dt + inv_dt * s->c[0], 1.0f + inv_dt * s->c[1]);
+            }
+            for (uint32_t i = 0; i < n; i++) {  // NR_PostStep (:2987-3037)
+                orc_solid* s = solids[i];
+                fie_update(s, inv_dt);
+                solid_begin_evaluation(s, s->xk.data());
+            }
+        } while (++nr < max_nr);
+        for (uint32_t i = 0; i < n; i++) {  // NR_End (:2838-2852)
+            orc_solid* s = solids[i];
+            s->x = s->xk;
+            s->v = s->vk;
+            if (stats) {
+                orc_step_stats& st = stats[i];
+                st.num_fixed_steps = 1;
+                st.cg_iters = s->ls_iters; st.cg_rel = s->ls_rel; st.cg_abs = s->ls_abs;
+                st.num_degenerate = s->ndeg; st.sum_degenerate_detF = s->sdeg;
+                st.elastic_energy = 0.0f; st.min_detF = s->min_det;
+                st.nr_iters = int32_t(nr); st.nr_rel = 0.0f; st.nr_abs = 0.0f;
+            }
+        }
+        *acc_time -= fixed_dt;
+        steps++;
+    }
+    return steps;
 }
 
 // Step (:225-256): fixed-dt accumulator with strict '>'

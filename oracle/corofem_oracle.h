@@ -71,7 +71,7 @@ void orc_f_e(const orc_tetset* s, const float* x, float* f);      /* f  += f_e(x
 void orc_df_x(const orc_tetset* s, const float* dx, float* df);   /* df += K'(dx) */
 float orc_V(const orc_tetset* s, const float* x);
 
-/* ---- 3D leaf solid (QIE-v) ---- */
+/* ---- 3D leaf solid (QIE-v, or FIE-dx with params.integrator) ---- */
 typedef struct orc_solid_params {
     float fixed_dt;
     float density;
@@ -86,7 +86,12 @@ typedef struct orc_solid_params {
     float degenerate_threshold;
     int warm_start;
     int reduce_fp64;   /* 0: reference FP32 SSE reductions; 1: FP64 reductions */
+    int integrator;    /* ORC_IT_QIE_V (0, default) or ORC_IT_FIE_DX (1) */
+    uint32_t nr_max_iter;  /* FIE-dx Newton iterations (reference default 3) */
+    float nr_abs_epsilon;  /* FIE-dx: stop when |b| < this (reference default 1e-6) */
 } orc_solid_params;
+#define ORC_IT_QIE_V 0
+#define ORC_IT_FIE_DX 1
 
 typedef struct orc_step_stats {
     uint32_t num_fixed_steps;
@@ -97,6 +102,9 @@ typedef struct orc_step_stats {
     float sum_degenerate_detF;
     float elastic_energy;
     float min_detF;
+    int32_t nr_iters;  /* FIE-dx: Newton iterations taken */
+    float nr_rel;      /* FIE-dx: last |b| or |y| over the first |b| */
+    float nr_abs;      /* FIE-dx: last |b| or |y| */
 } orc_step_stats;
 
 typedef struct orc_solid orc_solid;
@@ -110,13 +118,21 @@ void orc_solid_get_state(const orc_solid* s, float* x, float* v);
 void orc_solid_set_state(orc_solid* s, const float* x, const float* v);
 float orc_solid_node_mass(const orc_solid* s);
 void orc_solid_rayleigh(const orc_solid* s, float c[2]);
+/* Multi-solid universe step (SS_AggregateDSH_Basic::MultiDSH_Solid3D_Update):
+   fixed dt = min over the solids, Newton passes = max of their nr_max_iter,
+   one accumulator (*acc_time, strict '>'); each fixed step runs the leaves'
+   NR_Init / NR_PreStep / NR_MidStep / NR_PostStep / NR_End hooks with the
+   frame dt. stats: NULL or one per solid (its last fixed step). Returns the
+   number of fixed steps taken. */
+uint32_t orc_multi_update(orc_solid* const* solids, uint32_t n, float dt, float* acc_time, orc_step_stats* stats);
 
 /* ---- 2D leaf solid (QIE-v, TriangleElementSet2 + LeafDSH_Solid2D_FEM) ----
    Same parameters (gravity[0..1] used); lumped masses Sum(area/3) * density *
    thickness per node, summed over the node's triangles in ascending triangle
    order. Rotations: closed-form 2x2 polar factor for det F > threshold (both
-   MSVD and DAPD); below it R = Id (the 2D SVD and DAPD degenerate branches
-   are not restated) and the element is counted in num_degenerate when det F <= 0. */
+   MSVD and DAPD); below it MSVD takes the 2x2 SVD branch of Compute_R_SVD and
+   DAPD R = Id (its invalid-NoC branch); elements with det F <= 0 are counted in
+   num_degenerate. */
 typedef struct orc_solid2 orc_solid2;
 orc_solid2* orc_solid2_create(uint32_t n_nodes, const float* ref_pos2, uint32_t n_tris, const uint32_t* tris,
                               const orc_solid_params* p, float thickness);

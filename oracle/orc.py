@@ -14,6 +14,7 @@ LIB_PATH = os.path.join(HERE, "liborc.so")
 REF_PATH = os.path.join(HERE, "_ref", "libcorofem_ref.so")
 
 RM_ID, RM_QR, RM_MSVD, RM_DAPD = 0, 1, 2, 3
+IT_QIE_V, IT_FIE_DX = 0, 1
 
 _f = C.POINTER(C.c_float)
 _u = C.POINTER(C.c_uint32)
@@ -35,6 +36,7 @@ class SolidParams(C.Structure):
         ("gravity", C.c_float * 3), ("cg_rel_epsilon", C.c_float), ("cg_abs_epsilon", C.c_float),
         ("cg_max_iter", C.c_uint32), ("rm", C.c_int), ("degenerate_threshold", C.c_float),
         ("warm_start", C.c_int), ("reduce_fp64", C.c_int),
+        ("integrator", C.c_int), ("nr_max_iter", C.c_uint32), ("nr_abs_epsilon", C.c_float),
     ]
 
 
@@ -43,6 +45,7 @@ class StepStats(C.Structure):
         ("num_fixed_steps", C.c_uint32), ("cg_iters", C.c_int32), ("cg_rel", C.c_float),
         ("cg_abs", C.c_float), ("num_degenerate", C.c_uint32), ("sum_degenerate_detF", C.c_float),
         ("elastic_energy", C.c_float), ("min_detF", C.c_float),
+        ("nr_iters", C.c_int32), ("nr_rel", C.c_float), ("nr_abs", C.c_float),
     ]
 
 
@@ -84,6 +87,8 @@ def _load():
         "orc_solid_set_state": (None, [C.c_void_p, _f, _f]),
         "orc_solid_node_mass": (C.c_float, [C.c_void_p]),
         "orc_solid_rayleigh": (None, [C.c_void_p, _f]),
+        "orc_multi_update": (C.c_uint32, [C.POINTER(C.c_void_p), C.c_uint32, C.c_float, _f,
+                                          C.POINTER(StepStats)]),
         "orc_solid2_create": (C.c_void_p, [C.c_uint32, _f, C.c_uint32, _u, C.POINTER(SolidParams), C.c_float]),
         "orc_solid2_destroy": (None, [C.c_void_p]),
         "orc_solid2_set_knc": (None, [C.c_void_p, C.c_uint32, _u]),
@@ -246,6 +251,9 @@ def default_params(**kw):
     p.degenerate_threshold = 0.1
     p.warm_start = 1
     p.reduce_fp64 = 0
+    p.integrator = IT_QIE_V
+    p.nr_max_iter = 3
+    p.nr_abs_epsilon = 1e-6
     for k, v in kw.items():
         if k == "gravity":
             for i in range(3):
@@ -255,8 +263,22 @@ def default_params(**kw):
     return p
 
 
+class Multi:
+    """Oracle multi-solid universe step (SS_AggregateDSH_Basic::MultiDSH_Solid3D_Update)."""
+
+    def __init__(self, solids):
+        self.solids = list(solids)
+        self.acc = np.zeros(1, np.float32)
+
+    def update(self, dt):
+        hs = (C.c_void_p * len(self.solids))(*[s.h for s in self.solids])
+        st = (StepStats * len(self.solids))()
+        n = lib().orc_multi_update(hs, len(self.solids), dt, fp(self.acc), st)
+        return n, list(st)
+
+
 class Solid:
-    """Oracle leaf solid (LeafDSH_Solid3D_FEM, QIE-v)."""
+    """Oracle leaf solid (LeafDSH_Solid3D_FEM: QIE-v, or FIE-dx by params.integrator)."""
 
     def __init__(self, ref_pos, tets, params, knc=None):
         self.ref_pos = np.ascontiguousarray(ref_pos, np.float32)

@@ -8,10 +8,11 @@ fails loudly when the library has not been built.
 """
 from ._capi import (  # noqa: F401
     lib, LIB_PATH, CorofemError, check, FemParams, SolidParams, StepStats, CgResult, BatchInfo,
-    RM_ID, RM_QR, RM_MSVD, RM_DAPD, MM_C_WRP, DM_TRUNCATED,
+    RM_ID, RM_QR, RM_MSVD, RM_DAPD, MM_C_WRP, DM_TRUNCATED, IT_QIE_V, IT_FIE_DX,
     CG_CONVERGED_REL, CG_CONVERGED_ABS, CG_MAX_ITER, CG_BREAKDOWN,
 )
 from .fem import (  # noqa: F401
-    DeviceArray, device_count, mesh_box, mesh_box2, TetrahedronElementSet3, ILSS_CG, LeafSolid3D, Batch, Batch2, LeafSolid2D,
+    DeviceArray, device_count, mesh_box, mesh_box2, TetrahedronElementSet3, ILSS_CG, LeafSolid3D, MultiSolid3D, Batch,
+    Batch2, LeafSolid2D,
     default_fem_params, default_solid_params, serve, rotations2, set_tile_elems, set_tile_threads, set_tile_persistent, get_tile_elems, synchronize,
 )

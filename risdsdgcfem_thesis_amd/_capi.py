@@ -9,6 +9,7 @@ CF_OK = 0
 RM_ID, RM_QR, RM_MSVD, RM_DAPD = 0, 1, 2, 3
 MM_C_WRP = 1
 DM_TRUNCATED = 0
+IT_QIE_V, IT_FIE_DX = 1, 3
 CG_CONVERGED_REL, CG_CONVERGED_ABS, CG_MAX_ITER, CG_BREAKDOWN = 1, 2, 3, 4
 
 _f = C.POINTER(C.c_float)
@@ -30,6 +31,7 @@ class SolidParams(C.Structure):
         ("gravity", C.c_float * 3), ("air_drag_coeff", C.c_float), ("cg_rel_epsilon", C.c_float),
         ("cg_abs_epsilon", C.c_float), ("cg_max_iter", C.c_uint32), ("warm_start", C.c_int32),
         ("compute_energy", C.c_int32), ("thickness", C.c_float),
+        ("integrator", C.c_int32), ("nr_max_iter", C.c_uint32), ("nr_abs_epsilon", C.c_float),
     ]
 
 
@@ -39,6 +41,7 @@ class StepStats(C.Structure):
         ("cg_abs_residual", C.c_float), ("cg_reason", C.c_int32), ("num_degenerate", C.c_uint32),
         ("sum_degenerate_det_f", C.c_float), ("elastic_energy", C.c_float), ("min_det_f", C.c_float),
         ("step_ms", C.c_double),
+        ("nr_iters", C.c_int32), ("nr_rel_residual", C.c_float), ("nr_abs_residual", C.c_float),
     ]
 
 
@@ -143,6 +146,9 @@ SIGNATURES = {
     "cf_solid_apply_A": (C.c_int, [_vp, C.c_float, C.c_float, _f, _f]),
     "cf_solid_time_df_x": (C.c_int, [_vp, C.c_uint32, C.POINTER(C.c_double)]),
     "cf_solid_time_kx": (C.c_int, [_vp, C.c_uint32, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "cf_multi_create": (C.c_int, [C.POINTER(_vp), C.c_uint32, C.POINTER(_vp)]),
+    "cf_multi_destroy": (C.c_int, [_vp]),
+    "cf_multi_update": (C.c_int, [_vp, C.c_float, _u, C.POINTER(StepStats)]),
     "cf_set_tile_elems": (C.c_int, [C.c_uint32]),
     "cf_get_tile_elems": (C.c_int, [_u]),
     "cf_set_tile_threads": (C.c_int, [C.c_uint32]),

@@ -9,6 +9,7 @@
 #include <cstring>
 #include <new>
 #include <string>
+#include <limits>
 #include <vector>
 
 namespace cf {
@@ -139,6 +140,15 @@ struct cf_solid {
     CgBuffers cg;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     float* scratch_vec = nullptr;
+    float *xk = nullptr, *vk = nullptr;  // FIE-dx iterates (m_IE_vec_x_k, m_IE_vec_v_k)
+    double* h_part = nullptr;             // pinned: block partials of |b|^2, |y|^2
+    cf_cg_result last_ls{};               // last linear solve
+    int32_t ls_total = 0;                 // linear-solve iterations of the last fixed step
+};
+
+struct cf_multi {
+    std::vector<cf_solid*> solids;
+    float acc_time = 0.f;
 };
 
 // ================================================================ helpers
@@ -315,6 +325,104 @@ static int solid_cg(cf_solid* s, float fM, float fK, cf_cg_result* res) {
     return CF_OK;
 }
 
+// ================================================================ FIE-dx
+// Sum of nblk FP64 block partials in block order, read back to the host.
+static int host_sum(cf_solid* s, const double* d_part, int nblk, double* out) {
+    CF_CHECK_HIP(hipMemcpyAsync(s->h_part, d_part, size_t(nblk) * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+    CF_CHECK_HIP(hipStreamSynchronize(s->stream));
+    double acc = 0.0;
+    for (int i = 0; i < nblk; i++) acc += s->h_part[i];
+    *out = acc;
+    return CF_OK;
+}
+
+// Right-hand side of one Newton iteration at the iterate x_k (f_e evaluated at
+// x_k under the current rotations), warm start on the first iteration only,
+// KNC rows in increment form; |b| when bnorm is not null.
+static int fie_rhs(cf_solid* s, float dt, float inv_dt, bool first, float* bnorm) {
+    cf_tetset* set = s->set;
+    const TetDevice& t = set->dev;
+    hipStream_t st = s->stream;
+    CF_TRY(launch_forces(t, kModeFe, s->xk, s->cache->R, set->mat, st));
+    NodeTerms nt;
+    nt.out = s->bvec; nt.y = s->y; nt.x = s->x; nt.xk = s->xk; nt.v = s->v;
+    nt.knc_mask = s->num_knc ? s->knc_mask : nullptr;
+    nt.knc_target = s->knc_target;
+    nt.mass = s->node_mass;
+    nt.dt = dt; nt.inv_dt = inv_dt;
+    nt.g[0] = s->p.gravity[0]; nt.g[1] = s->p.gravity[1]; nt.g[2] = s->p.gravity[2];
+    nt.warm_start = (first && s->p.warm_start) ? 1 : 0;
+    nt.partials = s->cg.part_b;
+    const int nb = node_blocks(s->N);
+    CF_TRY(launch_gather(t, kGatherRhsDx, nt, nb, st));
+    if (bnorm) {
+        double bb = 0.0;
+        CF_TRY(host_sum(s, s->cg.part_b, nb, &bb));
+        *bnorm = float(std::sqrt(bb));
+    }
+    return CF_OK;
+}
+
+// Solve, x_k += y, v_k = (x_k - x)/dt, re-open the evaluation at x_k
+// (:1479-1530, NR_MidStep/NR_PostStep :2967-3037); |y| when ynorm is not null.
+static int fie_solve_update(cf_solid* s, float inv_dt, float* ynorm) {
+    const float fM = inv_dt * inv_dt + inv_dt * s->rayleigh[0];
+    const float fK = 1.0f + inv_dt * s->rayleigh[1];
+    CF_TRY(solid_cg(s, fM, fK, &s->last_ls));
+    s->ls_total += s->last_ls.iterations;
+    const uint32_t n3 = 3 * s->N;
+    const int nb = node_blocks(n3);
+    CF_TRY(launch_nr_update(n3, s->xk, s->vk, s->x, s->y, inv_dt, s->cg.part_b, nb, s->stream));
+    if (ynorm) {
+        double yy = 0.0;
+        CF_TRY(host_sum(s, s->cg.part_b, nb, &yy));
+        *ynorm = float(std::sqrt(yy));
+    }
+    return cf_begin_evaluation(s->cache, s->x, s->xk, s->vk, s->total_time);
+}
+
+// Step_FullyImplicitEuler_dx (ds/dsh/LeafDSH_Solid3D_FEM.cpp:1357-1552); dt = 0
+// is the quasi-static variant (:957-959). The evaluation is open at x on entry.
+static int step_fie_dx(cf_solid* s, float dt, int32_t* nr_iters, float* nr_rel, float* nr_abs) {
+    const float inv_dt = (dt == 0.0f) ? 0.0f : 1.0f / dt;
+    const size_t bytes = 3 * (size_t)s->N * sizeof(float);
+    CF_CHECK_HIP(hipMemcpyAsync(s->xk, s->x, bytes, hipMemcpyDeviceToDevice, s->stream));
+    CF_CHECK_HIP(hipMemcpyAsync(s->vk, s->v, bytes, hipMemcpyDeviceToDevice, s->stream));
+    uint32_t nr_k = 0;
+    float delta_new = 0.0f, delta0 = -1.0f;
+    bool conv = false;
+    s->ls_total = 0;
+    s->last_ls = cf_cg_result{};
+    do {
+        CF_TRY(fie_rhs(s, dt, inv_dt, nr_k == 0, &delta_new));  // |b| after KNC (:1468-1474)
+        if (delta0 < 0) delta0 = delta_new;
+        conv = delta_new < s->p.nr_abs_epsilon;
+        if (!conv) {
+            CF_TRY(fie_solve_update(s, inv_dt, &delta_new));
+            nr_k++;
+        }
+    } while (!conv && nr_k < s->p.nr_max_iter);
+    *nr_iters = int32_t(nr_k);
+    *nr_rel = (delta0 > 0) ? delta_new / delta0 : 0.0f;
+    *nr_abs = delta_new;
+    return CF_OK;
+}
+
+static int solid_fill_stats(cf_solid* s, double energy, float ms, cf_step_stats* stats) {
+    CF_TRY(cache_stats_fetch(s->cache));
+    stats->num_fixed_steps = 1;
+    stats->cg_iters = s->ls_total;
+    stats->cg_rel_residual = s->last_ls.rel_residual;
+    stats->cg_abs_residual = s->last_ls.abs_residual;
+    stats->cg_reason = s->last_ls.reason;
+    stats->num_degenerate = s->cache->num_degenerate;
+    stats->sum_degenerate_det_f = s->cache->sum_degenerate;
+    stats->elastic_energy = float(energy);
+    stats->min_det_f = s->cache->min_det;
+    stats->step_ms = ms;
+    return CF_OK;
+}
+
 // ================================================================ C API
 extern "C" {
 
@@ -350,6 +458,9 @@ void cf_default_solid_params(cf_solid_params* p) {
     p->warm_start = 1;
     p->compute_energy = 1;
     p->thickness = 1.0f;
+    p->integrator = CF_IT_QIE_V;
+    p->nr_max_iter = 3;
+    p->nr_abs_epsilon = 1e-6f;
 }
 
 int cf_device_count(int* n) {
@@ -764,6 +875,8 @@ int cf_solid_destroy(cf_solid* s) {
     if (!s) return CF_OK;
     dfree(s->x); dfree(s->v); dfree(s->x0); dfree(s->bvec); dfree(s->y);
     dfree(s->knc_mask); dfree(s->knc_target); dfree(s->scratch_vec);
+    dfree(s->xk); dfree(s->vk);
+    if (s->h_part) (void)hipHostFree(s->h_part);
     cg_free(s->cg);
     if (s->ev0) (void)hipEventDestroy(s->ev0);
     if (s->ev1) (void)hipEventDestroy(s->ev1);
@@ -777,6 +890,10 @@ int cf_solid_create(uint32_t N, const float* ref, uint32_t T, const uint32_t* te
                     cf_solid** out) {
     if (!out || !ref || !tets || !p || N == 0 || T == 0) { set_error("cf_solid_create: invalid arguments"); return CF_ERR_INVALID_ARG; }
     if (!(p->fixed_dt > 0) || !(p->density > 0)) { set_error("fixed_dt and density must be > 0"); return CF_ERR_INVALID_ARG; }
+    if (p->integrator != CF_IT_QIE_V && p->integrator != CF_IT_FIE_DX) {
+        set_error("integrator %d is not supported (CF_IT_QIE_V or CF_IT_FIE_DX)", p->integrator);
+        return CF_ERR_UNSUPPORTED;
+    }
     *out = nullptr;
     cf_solid* s = new (std::nothrow) cf_solid;
     if (!s) return CF_ERR_OUT_OF_MEMORY;
@@ -795,6 +912,12 @@ int cf_solid_create(uint32_t N, const float* ref, uint32_t T, const uint32_t* te
     if (rc == CF_OK) rc = dalloc(&s->knc_mask, N);
     if (rc == CF_OK) rc = dalloc(&s->knc_target, n3);
     if (rc == CF_OK) rc = cg_alloc(s->cg, uint32_t(n3));
+    if (rc == CF_OK) rc = dalloc(&s->xk, n3);
+    if (rc == CF_OK) rc = dalloc(&s->vk, n3);
+    if (rc == CF_OK && hipHostMalloc(reinterpret_cast<void**>(&s->h_part), kMaxPartials * sizeof(double)) != hipSuccess) {
+        set_error("pinned allocation failed");
+        rc = CF_ERR_HIP;
+    }
     if (rc != CF_OK) { cf_solid_destroy(s); return rc; }
     if (hipMemcpy(s->x, ref, n3 * 4, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(s->x0, ref, n3 * 4, hipMemcpyHostToDevice) != hipSuccess ||
@@ -838,7 +961,7 @@ int cf_solid_set_knc(cf_solid* s, uint32_t n, const uint32_t* nids, const float*
 }
 
 int cf_solid_fixed_step(cf_solid* s, float dt, cf_step_stats* stats) {
-    if (!s || !(dt > 0)) return CF_ERR_INVALID_ARG;
+    if (!s || !(dt >= 0) || (dt == 0 && s->p.integrator != CF_IT_FIE_DX)) return CF_ERR_INVALID_ARG;
     cf_tetset* set = s->set;
     const TetDevice& t = set->dev;
     hipStream_t st = s->stream;
@@ -848,6 +971,28 @@ int cf_solid_fixed_step(cf_solid* s, float dt, cf_step_stats* stats) {
     CF_TRY(cf_begin_evaluation(s->cache, s->x0, s->x, s->v, s->total_time));
     // m_vecPos0 <- x (:936-937)
     CF_CHECK_HIP(hipMemcpyAsync(s->x0, s->x, n3 * 4, hipMemcpyDeviceToDevice, st));
+    if (s->p.integrator == CF_IT_FIE_DX) {
+        int32_t nr = 0;
+        float nr_rel = 0.f, nr_abs = 0.f;
+        CF_TRY(step_fie_dx(s, dt, &nr, &nr_rel, &nr_abs));
+        // pos = x_k, vel = v_k (:1546-1551), air drag (:982-984)
+        CF_TRY(launch_nr_finish(n3, s->x, s->v, s->xk, s->vk, dt, s->air_drag_per_second, st));
+        double energy = 0.0;
+        if (s->p.compute_energy) CF_TRY(cf_V(s->cache, &energy));  // V() at the final iterate (:977)
+        CF_TRY(cf_end_evaluation(s->cache));
+        CF_CHECK_HIP(hipEventRecord(s->ev1, st));
+        CF_CHECK_HIP(hipEventSynchronize(s->ev1));
+        s->total_time += dt;
+        if (stats) {
+            float ms = 0.f;
+            CF_CHECK_HIP(hipEventElapsedTime(&ms, s->ev0, s->ev1));
+            CF_TRY(solid_fill_stats(s, energy, ms, stats));
+            stats->nr_iters = nr;
+            stats->nr_rel_residual = nr_rel;
+            stats->nr_abs_residual = nr_abs;
+        }
+        return CF_OK;
+    }
     // b = dt (M g + f_e(x)) + M v ; y = v ; KNC rows (Step_QuasiImplicitEuler_v :1209-1295)
     CF_TRY(launch_forces(t, kModeFe, s->x, s->cache->R, set->mat, st));
     NodeTerms nt;
@@ -886,7 +1031,93 @@ int cf_solid_fixed_step(cf_solid* s, float dt, cf_step_stats* stats) {
         stats->elastic_energy = float(energy);
         stats->min_det_f = s->cache->min_det;
         stats->step_ms = ms;
+        stats->nr_iters = 0;
+        stats->nr_rel_residual = 0.f;
+        stats->nr_abs_residual = 0.f;
     }
+    return CF_OK;
+}
+
+// ---------------------------------------------------------------- multi-solid
+int cf_multi_create(cf_solid* const* solids, uint32_t n, cf_multi** out) {
+    if (!out || !solids || n == 0) { set_error("cf_multi_create: invalid arguments"); return CF_ERR_INVALID_ARG; }
+    *out = nullptr;
+    cf_multi* m = new (std::nothrow) cf_multi;
+    if (!m) return CF_ERR_OUT_OF_MEMORY;
+    for (uint32_t i = 0; i < n; i++) {
+        if (!solids[i]) { delete m; set_error("cf_multi_create: solid %u is null", i); return CF_ERR_INVALID_ARG; }
+        m->solids.push_back(solids[i]);
+    }
+    *out = m;
+    return CF_OK;
+}
+
+int cf_multi_destroy(cf_multi* m) {
+    delete m;
+    return CF_OK;
+}
+
+// SS_AggregateDSH_Basic::MultiDSH_Solid3D_Update (ds/ss/SS_AggregateDSH_Basic.cpp:229-300)
+// without contacts: fixed dt = min of the solids' fixed_dt, Newton passes = max
+// of their nr_max_iter, one accumulator with strict '>'; each fixed step runs
+// the leaves' NR hooks (LeafDSH_Solid3D_FEM.cpp:2824-3039) with the frame dt.
+// NR_PreStep resets x_k = x, v_k = v and the pass count every pass, so each
+// pass evaluates f_e at x under the rotations of the previous pass's end
+// positions and warm-starts y = dt v.
+int cf_multi_update(cf_multi* m, float dt, uint32_t* num_fixed_steps, cf_step_stats* stats) {
+    if (!m || !(dt >= 0)) return CF_ERR_INVALID_ARG;
+    float fixed_dt = std::numeric_limits<float>::infinity();
+    uint32_t max_nr = 0;
+    for (cf_solid* s : m->solids) {
+        fixed_dt = std::min(fixed_dt, s->p.fixed_dt);
+        max_nr = std::max(max_nr, s->p.nr_max_iter);
+        s->total_time += dt;
+    }
+    m->acc_time += dt;
+    const float inv_dt = (dt == 0.0f) ? 0.0f : 1.0f / dt;
+    const size_t n = m->solids.size();
+    uint32_t steps = 0;
+    while (m->acc_time > fixed_dt) {
+        for (cf_solid* s : m->solids) {  // NR_Init (:2824-2836)
+            CF_TRY(cf_begin_evaluation(s->cache, s->x0, s->x, s->v, s->total_time));
+            CF_CHECK_HIP(hipMemcpyAsync(s->x0, s->x, 3 * (size_t)s->N * 4, hipMemcpyDeviceToDevice, s->stream));
+            s->ls_total = 0;
+        }
+        uint32_t nr = 0;
+        do {
+            for (cf_solid* s : m->solids) {  // NR_PreStep + NR_MidStep (:2857-2985)
+                const size_t bytes = 3 * (size_t)s->N * 4;
+                CF_CHECK_HIP(hipMemcpyAsync(s->xk, s->x, bytes, hipMemcpyDeviceToDevice, s->stream));
+                CF_CHECK_HIP(hipMemcpyAsync(s->vk, s->v, bytes, hipMemcpyDeviceToDevice, s->stream));
+                CF_TRY(fie_rhs(s, dt, inv_dt, true, nullptr));
+                s->ls_total = 0;
+                const float fM = inv_dt * inv_dt + inv_dt * s->rayleigh[0];
+                const float fK = 1.0f + inv_dt * s->rayleigh[1];
+                CF_TRY(solid_cg(s, fM, fK, &s->last_ls));
+                s->ls_total = s->last_ls.iterations;
+            }
+            for (cf_solid* s : m->solids) {  // NR_PostStep (:2987-3037)
+                const uint32_t n3 = 3 * s->N;
+                CF_TRY(launch_nr_update(n3, s->xk, s->vk, s->x, s->y, inv_dt, s->cg.part_b, node_blocks(n3), s->stream));
+                CF_TRY(cf_begin_evaluation(s->cache, s->x, s->xk, s->vk, s->total_time));
+            }
+        } while (++nr < max_nr);
+        for (size_t i = 0; i < n; i++) {  // NR_End (:2838-2852)
+            cf_solid* s = m->solids[i];
+            CF_TRY(launch_nr_finish(3 * s->N, s->x, s->v, s->xk, s->vk, dt, s->air_drag_per_second, s->stream));
+            CF_TRY(cf_end_evaluation(s->cache));
+            CF_CHECK_HIP(hipStreamSynchronize(s->stream));
+            if (stats) {
+                CF_TRY(solid_fill_stats(s, 0.0, 0.f, &stats[i]));
+                stats[i].nr_iters = int32_t(nr);
+                stats[i].nr_rel_residual = 0.f;
+                stats[i].nr_abs_residual = 0.f;
+            }
+        }
+        m->acc_time -= fixed_dt;
+        steps++;
+    }
+    if (num_fixed_steps) *num_fixed_steps = steps;
     return CF_OK;
 }
 

@@ -639,6 +639,31 @@ __device__ inline double gather_node(uint32_t n, const uint32_t* __restrict__ no
         if (t.knc_mask && t.knc_mask[n]) { q0 = 0.f; q1 = 0.f; q2 = 0.f; }
         t.out[i] = q0; t.out[i + 1] = q1; t.out[i + 2] = q2;
         return (double)d0 * q0 + (double)d1 * q1 + (double)d2 * q2;
+    } else if (MODE == kGatherRhsDx) {
+        // One Newton iteration of Step_FullyImplicitEuler_dx (:1378-1465):
+        // b = M g + f_e + (1/dt) M (v - (x_k - x)/dt); y = dt v when warm_start
+        // (set by the caller for the first iteration only) else 0; KNC rows
+        // b = y = target - x_k. Returns the node's b.b.
+        const float mass = t.mass;
+        const float sm = t.inv_dt * mass;
+        const float f[3] = {fx, fy, fz};
+        float b[3], y[3];
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const float vc = t.v[i + c];
+            b[c] = (mass * t.g[c] + f[c]) + sm * (vc - t.inv_dt * (t.xk[i + c] - t.x[i + c]));
+            y[c] = t.warm_start ? t.dt * vc : 0.0f;
+        }
+        if (t.knc_mask && t.knc_mask[n]) {
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                b[c] = t.knc_target[i + c] - t.xk[i + c];
+                y[c] = b[c];
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < 3; c++) { t.out[i + c] = b[c]; t.y[i + c] = y[c]; }
+        return (double)b[0] * b[0] + (double)b[1] * b[1] + (double)b[2] * b[2];
     } else {
         // b = dt (M g + f_e) + M v ; y = v ; KNC rows take the constraint
         // velocity (Step_QuasiImplicitEuler_v :1209-1295).
@@ -680,7 +705,7 @@ __device__ inline void gather_block(uint32_t N, const uint32_t* __restrict__ nod
     double local = 0.0;
     for (uint32_t n = blk * kBlock + threadIdx.x; n < N; n += nblk * kBlock)
         local += gather_node<MODE>(n, node_part_ptr, partials, t);
-    if (MODE == kGatherMatvec) {
+    if (MODE == kGatherMatvec || MODE == kGatherRhsDx) {
         const double s = block_sum(local, sh);
         if (threadIdx.x == 0) t.partials[blk] = s;
     }
@@ -821,6 +846,35 @@ __device__ inline void update_d_block(uint32_t n, float beta, float* __restrict_
 }
 
 // v = y ; x = x + dt y (Step_QuasiImplicitEuler_v :1322-1327), air drag (FixedStep :982-984)
+// FIE-dx iterate update (:1500-1511): x_k += y ; v_k = (x_k - x) / dt ;
+// per-block sum of y.y (FP64) into part[blk].
+__device__ inline void nr_update_block(uint32_t n, float* __restrict__ xk, float* __restrict__ vk,
+                                       const float* __restrict__ x, const float* __restrict__ y, float inv_dt,
+                                       double* part, uint32_t blk, uint32_t nblk, double* sh) {
+#pragma clang fp contract(off)
+    double acc = 0.0;
+    for (uint32_t i = blk * kBlock + threadIdx.x; i < n; i += nblk * kBlock) {
+        const float yi = y[i];
+        const float xn = xk[i] + yi;
+        xk[i] = xn;
+        vk[i] = inv_dt * (xn - x[i]);
+        acc += (double)yi * yi;
+    }
+    const double s = block_sum(acc, sh);
+    if (threadIdx.x == 0) part[blk] = s;
+}
+
+// End of a FIE-dx step (:1546-1551, air drag :982-984): x = x_k ; v = v_k.
+__device__ inline void nr_finish_block(uint32_t n, float* __restrict__ x, float* __restrict__ v,
+                                       const float* __restrict__ xk, const float* __restrict__ vk, float dt,
+                                       float drag, uint32_t blk, uint32_t nblk) {
+#pragma clang fp contract(off)
+    for (uint32_t i = blk * kBlock + threadIdx.x; i < n; i += nblk * kBlock) {
+        x[i] = xk[i];
+        v[i] = (drag > 0.f) ? (1.0f - dt * drag) * vk[i] : vk[i];
+    }
+}
+
 __device__ inline void integrate_block(uint32_t n, float* __restrict__ x, float* __restrict__ v,
                                        const float* __restrict__ y, float dt, float drag, uint32_t blk,
                                        uint32_t nblk) {

@@ -102,13 +102,14 @@ constexpr uint32_t kMaxCgIter = 1024;
 
 // ---------------------------------------------------------------- launchers
 enum ForceMode { kModeDfx = 0, kModeFe = 1 };
-enum GatherMode { kGatherAccumulate = 0, kGatherMatvec = 1, kGatherRhs = 2 };
+enum GatherMode { kGatherAccumulate = 0, kGatherMatvec = 1, kGatherRhs = 2, kGatherRhsDx = 3 };
 
 struct NodeTerms {  // per-node terms fused into the node gather
     const float* d = nullptr;       // matvec: direction d (3N)
     float* out = nullptr;           // accumulate: += ; matvec: q ; rhs: b
     float* y = nullptr;             // rhs: warm start y
     const float* x = nullptr;       // rhs: positions (for KNC velocity)
+    const float* xk = nullptr;      // FIE-dx rhs: the Newton iterate x_k
     const float* v = nullptr;       // rhs: velocities
     const uint8_t* knc_mask = nullptr;
     const float* knc_target = nullptr;
@@ -118,7 +119,7 @@ struct NodeTerms {  // per-node terms fused into the node gather
     float dt = 0.f, inv_dt = 0.f;
     float g[3] = {0, 0, 0};
     int warm_start = 1;
-    double* partials = nullptr;     // matvec: per-block d.q
+    double* partials = nullptr;     // matvec: per-block d.q ; FIE-dx rhs: per-block b.b
 };
 
 int launch_precompute(const TetDevice& t, hipStream_t s);
@@ -148,6 +149,12 @@ int launch_cg_update_xr(uint32_t n, uint32_t k, CgScalars* st, float* x, float* 
                         double* rr_part, hipStream_t s);
 int launch_cg_update_d(uint32_t n, uint32_t k, CgScalars* st, float* d, const float* r, const uint8_t* mask,
                        hipStream_t s);
+// FIE-dx: x_k += y ; v_k = (x_k - x) / dt ; per-block y.y into part (nblk blocks)
+int launch_nr_update(uint32_t n, float* xk, float* vk, const float* x, const float* y, float inv_dt, double* part,
+                     int nblk, hipStream_t s);
+// x = x_k ; v = v_k, scaled by (1 - dt drag) when drag > 0
+int launch_nr_finish(uint32_t n, float* x, float* v, const float* xk, const float* vk, float dt, float drag,
+                     hipStream_t s);
 int launch_integrate(uint32_t n, float* x, float* v, const float* y, float dt, float drag, hipStream_t s);
 int launch_zero_mask(uint32_t n_nodes, float* w, const uint8_t* mask, hipStream_t s);
 

@@ -278,6 +278,19 @@ __global__ __launch_bounds__(kBlock) void k_integrate(uint32_t n, float* __restr
     integrate_block(n, x, v, y, dt, drag, blockIdx.x, gridDim.x);
 }
 
+__global__ __launch_bounds__(kBlock) void k_nr_update(uint32_t n, float* __restrict__ xk, float* __restrict__ vk,
+                                                      const float* __restrict__ x, const float* __restrict__ y,
+                                                      float inv_dt, double* part) {
+    __shared__ double sh[kBlock / 64];
+    nr_update_block(n, xk, vk, x, y, inv_dt, part, blockIdx.x, gridDim.x, sh);
+}
+
+__global__ __launch_bounds__(kBlock) void k_nr_finish(uint32_t n, float* __restrict__ x, float* __restrict__ v,
+                                                      const float* __restrict__ xk, const float* __restrict__ vk,
+                                                      float dt, float drag) {
+    nr_finish_block(n, x, v, xk, vk, dt, drag, blockIdx.x, gridDim.x);
+}
+
 __global__ __launch_bounds__(kBlock) void k_zero_mask(uint32_t nn, float* __restrict__ w,
                                                       const uint8_t* __restrict__ mask) {
     for (uint32_t n = blockIdx.x * kBlock + threadIdx.x; n < nn; n += gridDim.x * kBlock)
@@ -396,6 +409,8 @@ int launch_gather_skip(const TetDevice& t, int mode, const NodeTerms& nt, int nb
         hipLaunchKernelGGL(k_gather<kGatherAccumulate>, dim3(nblk), dim3(kBlock), 0, s, a);
     else if (mode == kGatherMatvec)
         hipLaunchKernelGGL(k_gather<kGatherMatvec>, dim3(nblk), dim3(kBlock), 0, s, a);
+    else if (mode == kGatherRhsDx)
+        hipLaunchKernelGGL(k_gather<kGatherRhsDx>, dim3(nblk), dim3(kBlock), 0, s, a);
     else
         hipLaunchKernelGGL(k_gather<kGatherRhs>, dim3(nblk), dim3(kBlock), 0, s, a);
     return check_launch();
@@ -452,6 +467,18 @@ int launch_cg_update_d(uint32_t n, uint32_t k, CgScalars* st, float* d, const fl
         hipLaunchKernelGGL(k_cg_update_d<true>, dim3(nb), dim3(kBlock), 0, s, n, k, st, d, r, mask);
     else
         hipLaunchKernelGGL(k_cg_update_d<false>, dim3(nb), dim3(kBlock), 0, s, n, k, st, d, r, mask);
+    return check_launch();
+}
+
+int launch_nr_update(uint32_t n, float* xk, float* vk, const float* x, const float* y, float inv_dt, double* part,
+                     int nblk, hipStream_t s) {
+    hipLaunchKernelGGL(k_nr_update, dim3(nblk), dim3(kBlock), 0, s, n, xk, vk, x, y, inv_dt, part);
+    return check_launch();
+}
+
+int launch_nr_finish(uint32_t n, float* x, float* v, const float* xk, const float* vk, float dt, float drag,
+                     hipStream_t s) {
+    hipLaunchKernelGGL(k_nr_finish, dim3(node_blocks(n)), dim3(kBlock), 0, s, n, x, v, xk, vk, dt, drag);
     return check_launch();
 }
 

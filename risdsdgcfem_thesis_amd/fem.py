@@ -267,7 +267,8 @@ class ILSS_CG:
 
 
 class LeafSolid3D:
-    """LeafDSH_Solid3D_FEM with device-resident state (QIE-v integrator)."""
+    """LeafDSH_Solid3D_FEM with device-resident state (QIE-v, or FIE-dx with
+    params.integrator = IT_FIE_DX)."""
 
     def __init__(self, ref_pos, tets, params=None, knc=None, knc_targets=None):
         self.ref_pos = np.ascontiguousarray(ref_pos, np.float32)
@@ -539,6 +540,36 @@ def rotations2(F, det, rm, threshold=0.1):
     fb = np.zeros(det.size, np.uint32)
     check(lib.cf_rotations2(_fp(F), _fp(det), det.size, int(rm), float(threshold), _fp(R), _up(fb)), "cf_rotations2")
     return R, fb
+
+
+class MultiSolid3D:
+    """Universe step of several LeafSolid3D (SS_AggregateDSH_Basic::MultiDSH_Solid3D_Update
+    without contacts): shared fixed dt (the smallest), Newton passes (the largest
+    nr_max_iter) and time accumulator. The solids stay owned by the caller."""
+
+    def __init__(self, solids):
+        self.solids = list(solids)
+        hs = (C.c_void_p * len(self.solids))(*[sd.h for sd in self.solids])
+        self.h = C.c_void_p()
+        check(lib.cf_multi_create(hs, len(self.solids), C.byref(self.h)), "cf_multi_create")
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib.cf_multi_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def update(self, dt):
+        """Returns (fixed steps taken, per-solid stats of the last one)."""
+        n = C.c_uint32()
+        st = (StepStats * len(self.solids))()
+        check(lib.cf_multi_update(self.h, float(dt), C.byref(n), st), "cf_multi_update")
+        return n.value, list(st)
 
 
 class Batch2:
