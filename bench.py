@@ -269,7 +269,8 @@ def main():
     ap.add_argument("--tile-threads", type=int, default=256)
     ap.add_argument("--tile-schedule", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", default="all", choices=["all", "single", "batch3d", "batch2d", "serve"])
+    ap.add_argument("--workload", default="all", choices=["all", "single", "batch3d", "batch2d", "serve", "fie"])
+    ap.add_argument("--fie-steps", type=int, default=5)
     ap.add_argument("--batch-count", type=int, default=1024)
     ap.add_argument("--batch-steps", type=int, default=3)
     ap.add_argument("--batch-max-iter", type=int, default=1000,
@@ -308,6 +309,9 @@ def main():
     if args.workload == "serve":
         print(json.dumps(bench_serve(cf, args)), flush=True)
         return
+    if args.workload == "fie":
+        print(json.dumps(bench_fie(cf, args)), flush=True)
+        return
     if args.workload == "single" or args.kx_only:
         print(json.dumps(bench_single(cf, args)), flush=True)
         return
@@ -316,6 +320,7 @@ def main():
     out["batched_2d"] = bench_batch2d(cf, args)
     out["batched_3d"] = bench_batch3d(cf, args)
     out["serving"] = bench_serve(cf, args)
+    out["fie_dx"] = bench_fie(cf, args)
     out["headline"] = {
         "kx_gtets": out["roofline"]["kx_gtets"], "kx_frac_hbm_peak": out["roofline"]["frac"],
         "batched_vs_plain_2d": out["batched_2d"]["speedup"], "batched_vs_plain_3d": out["batched_3d"]["speedup"],
@@ -400,6 +405,66 @@ def bench_single(cf, args):
     solid.close()
     return out
 
+
+def bench_fie(cf, args):
+    """FIE-dx on the single large mesh (reference defaults: 3 Newton iterations,
+    CG rel 1e-4 max 50), and the multi-solid universe step over 16 config-1
+    beams (20,000 tets each)."""
+    counts = (65, 65, 257)
+    half = (0.32, 0.32, 1.28)
+    pos, tets = cf.mesh_box(half, counts)
+    N, T = pos.size // 3, tets.size // 4
+    x0 = config2_state(pos)
+    knc = np.nonzero(np.isclose(pos[2::3], -half[2]))[0].astype(np.uint32)
+    params = cf.default_solid_params(fixed_dt=1.0 / 60.0, gravity=(0.0, -9.81, 0.0), young_modulus=1e5,
+                                     poisson_ratio=0.3, damping_ratio=0.0, rm=cf.RM_DAPD, cg_rel_epsilon=1e-4,
+                                     cg_abs_epsilon=1e-6, cg_max_iter=50, density=1000.0,
+                                     integrator=cf.IT_FIE_DX, nr_max_iter=3)
+    log(f"[bench] FIE-dx on {T} tets")
+    solid = cf.LeafSolid3D(pos, tets, params)
+    solid.set_state(x0, np.zeros_like(x0))
+    solid.set_knc(knc, x0.reshape(-1, 3)[knc].ravel())
+    solid.fixed_step()
+    cf.synchronize()
+    nr, ls = [], []
+    t0 = time.perf_counter()
+    for _ in range(args.fie_steps):
+        st = solid.fixed_step()
+        nr.append(st.nr_iters)
+        ls.append(st.cg_iters)
+    cf.synchronize()
+    wall = time.perf_counter() - t0
+    solid.close()
+    matvecs = sum(ls) + sum(nr)  # CG iterations + one residual matvec per solve
+    single = {"workload": "FIE-dx, 64x64x256 cube grid, 3 Newton iterations, CG rel 1e-4 max 50",
+              "steps": args.fie_steps, "ms_per_step": round(wall * 1e3 / args.fie_steps, 3),
+              "steps_per_s": round(args.fie_steps / wall, 3), "nr_iters": nr, "cg_iters": ls,
+              "gtet_matvecs_per_s": round(T * matvecs / wall / 1e9, 3)}
+    bpos, btets = cf.mesh_box((0.05, 0.05, 0.2), (11, 11, 41))
+    bknc = np.nonzero(np.isclose(bpos[2::3], -0.2))[0].astype(np.uint32)
+    bp = cf.default_solid_params(fixed_dt=1.0 / 60.0, gravity=(0.0, -9.81, 0.0), young_modulus=1e5,
+                                 poisson_ratio=0.3, damping_ratio=0.0, rm=cf.RM_DAPD, cg_rel_epsilon=1e-4,
+                                 cg_abs_epsilon=1e-6, cg_max_iter=50, density=1000.0, nr_max_iter=3)
+    solids = [cf.LeafSolid3D(bpos, btets, bp, knc=bknc) for _ in range(16)]
+    multi = cf.MultiSolid3D(solids)
+    multi.update(1.0 / 60.0)
+    multi.update(1.0 / 60.0)
+    cf.synchronize()
+    frames, steps = 5, 0
+    t0 = time.perf_counter()
+    for _ in range(frames):
+        n, _ = multi.update(1.0 / 60.0)
+        steps += n
+    cf.synchronize()
+    mwall = time.perf_counter() - t0
+    multi.close()
+    for sd in solids:
+        sd.close()
+    log(f"[bench] FIE-dx {single['ms_per_step']} ms/step; multi-solid {mwall * 1e3 / max(steps, 1):.2f} ms/fixed step")
+    return {"single": single,
+            "multi": {"workload": "multi-solid universe step, 16 config-1 beams (20,000 tets each), 3 Newton passes",
+                      "solids": 16, "fixed_steps": steps, "ms_per_fixed_step": round(mwall * 1e3 / max(steps, 1), 3),
+                      "solid_steps_per_s": round(16 * steps / mwall, 2)}}
 
 
 if __name__ == "__main__":
