@@ -51,6 +51,13 @@ extern "C" {
 /* material methods (ms/fem/Params.h:50-62); only C_WRP is implemented */
 #define CF_MM_L 0
 #define CF_MM_C_WRP 1
+/* LCM and the LCM/WRP hybrid (ms/fem/Params.h:54-55): Piola forces
+   C_LCM_f_e / C_LCM_df_x / C_LCMH_df_x / C_LCM_V
+   (ms/fem/TetrahedronElementSet3.cpp:1155-1178,1489-1508,1838-1929), from the
+   F and R of the last BeginEvaluation; single element sets and leaves only
+   (batches take C_WRP). */
+#define CF_MM_C_LCM 2
+#define CF_MM_C_LCMH 3
 /* rotation methods (ms/fem/Params.h:63-67) */
 #define CF_RM_ID 0
 #define CF_RM_QR 1
@@ -71,6 +78,7 @@ typedef struct cf_fem_params {
     int32_t mm;                       /* default CF_MM_C_WRP */
     int32_t rm;                       /* default CF_RM_MSVD  */
     int32_t dm;                       /* default CF_DM_TRUNCATED */
+    float ecie_k_factor;              /* LCMH blend radius k (m_ECIE_k_factor); default 1 */
 } cf_fem_params;
 
 /* Mirrors the leaf's integrator/solver knobs (ds/dsh/LeafDSH_Solid3D_FEM.h:93-274). */

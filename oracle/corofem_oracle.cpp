@@ -512,6 +512,10 @@ struct orc_tetset {
     float total_volume = 0.0f;
     std::vector<float> R;            // 9T cache
     std::vector<float> detF;         // T
+    std::vector<float> F;            // 9T cache (LCM / LCMH forces)
+    int mm = ORC_MM_C_WRP;           // material model (ms/fem/Params.h:51-61)
+    float ecie_k = 1.0f;             // m_ECIE_k_factor (Params.h:89)
+    float mu = 0.0f, lambda = 0.0f;  // Lame parameters (TetrahedronElementSet3.cpp:143-148)
 };
 
 extern "C" {
@@ -522,6 +526,10 @@ orc_tetset* orc_tetset_create(uint32_t n_nodes, const float* ref_pos, uint32_t n
                               float young, float poisson) {
     orc_tetset* s = new orc_tetset;
     s->N = n_nodes; s->T = n_tets;
+    // LameParameters_From_YoungAndPoisson (TetrahedronElementSet3.cpp:143-148)
+    s->mu = 0.5f * young / (1 + poisson);
+    s->lambda = young * poisson / ((1.0f + poisson) * (1.0f - 2.0f * poisson));
+    s->F.assign(9 * size_t(n_tets), 0.0f);
     s->ref.assign(ref_pos, ref_pos + 3 * size_t(n_nodes));
     s->tet.assign(tets, tets + 4 * size_t(n_tets));
     s->Kb.resize(size_t(n_tets) * 144);
@@ -584,6 +592,7 @@ void orc_begin_evaluation(orc_tetset* s, const float* x, int rm, float thr, floa
         M3 F = mmul(Ds, mload(&s->Dinv[9 * size_t(e)]));
         float dF = det3(F);
         s->detF[e] = dF;
+        mstore(F, &s->F[9 * size_t(e)]);
         ndeg += (dF <= 0) ? 1u : 0u;
         sdeg += (dF <= 0) ? dF : 0.0f;
         M3 R;
@@ -625,8 +634,94 @@ inline void corot_apply(const orc_tetset* s, uint32_t e, const V3 l[4], float* o
 }
 }  // namespace
 
+void orc_tetset_set_material(orc_tetset* s, int mm, float ecie_k_factor) {
+    s->mm = mm;
+    s->ecie_k = ecie_k_factor;
+}
+}  // extern "C"
+
+namespace {
+inline M3 mlin(float a, const M3& x, float b, const M3& y) {  // a x + b y, element-wise
+    M3 r;
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) r.a[i][j] = a * x.a[i][j] + b * y.a[i][j];
+    return r;
+}
+inline M3 madd(const M3& x, const M3& y) { M3 r; for (int i = 0; i < 3; i++) for (int j = 0; j < 3; j++) r.a[i][j] = x.a[i][j] + y.a[i][j]; return r; }
+inline M3 mscale(float a, const M3& x) { M3 r; for (int i = 0; i < 3; i++) for (int j = 0; j < 3; j++) r.a[i][j] = a * x.a[i][j]; return r; }
+inline float mtrace(const M3& m) { float t = 0.0f; for (int i = 0; i < 3; i++) t += m.a[i][i]; return t; }
+
+// Piola force of one element, H = -V P Dm^-T, scattered as the reference
+// does: h0 = -h1 - h2 - h3 to vertex 0, column c to vertex c+1
+// (TetrahedronElementSet3.cpp:1171-1176).
+inline void scatter_piola(const orc_tetset* s, uint32_t e, const M3& P, float* out) {
+    const uint32_t* t = &s->tet[4 * size_t(e)];
+    const M3 H = mmul(mscale(-s->vol[e], P), mtr(mload(&s->Dinv[9 * size_t(e)])));
+    const V3 c0 = col(H, 0), c1 = col(H, 1), c2 = col(H, 2);
+    const V3 h0 = vsub(vsub(vscale(c0, -1.0f), c1), c2);
+    float* o0 = &out[3 * size_t(t[0])];
+    for (int i = 0; i < 3; i++) o0[i] += h0.v[i];
+    for (int k = 0; k < 3; k++) {
+        float* o = &out[3 * size_t(t[k + 1])];
+        const V3 c = col(H, k);
+        for (int i = 0; i < 3; i++) o[i] += c.v[i];
+    }
+}
+
+// C_LCM_f_e (:1155-1178): S = R^T F, P = R (2 mu (S - I) + lambda tr(S - I) I)
+void lcm_f_e(const orc_tetset* s, float* f) {
+    for (uint32_t e = 0; e < s->T; e++) {
+        const M3 R = mload(&s->R[9 * size_t(e)]);
+        const M3 S = mmul(mtr(R), mload(&s->F[9 * size_t(e)]));
+        M3 SmI = S;
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) SmI.a[i][j] = S.a[i][j] - mid().a[i][j];
+        const M3 P = mmul(R, mlin(2 * s->mu, SmI, s->lambda * mtrace(SmI), mid()));
+        scatter_piola(s, e, P, f);
+    }
+}
+
+// C_LCM_df_x (:1838-1872) and C_LCMH_df_x (:1874-1929), DM = Truncated:
+// dF = dDs Dm^-1; LCM dP = 2 mu dF + lambda tr(R^T dF) R; LCMH blends 2 mu dF
+// with 2 mu R sym(R^T dF) by w = clamp01(tr|S - I| / (3 k)) when k > 0.1.
+void lcm_df_x(const orc_tetset* s, const float* dx, float* df, bool hybrid) {
+    for (uint32_t e = 0; e < s->T; e++) {
+        const uint32_t* t = &s->tet[4 * size_t(e)];
+        const M3 R = mload(&s->R[9 * size_t(e)]);
+        const M3 Rt = mtr(R);
+        const M3 dDs = compute_D(vload(&dx[3 * size_t(t[0])]), vload(&dx[3 * size_t(t[1])]),
+                                 vload(&dx[3 * size_t(t[2])]), vload(&dx[3 * size_t(t[3])]));
+        const M3 dF = mmul(dDs, mload(&s->Dinv[9 * size_t(e)]));
+        const M3 RtdF = mmul(Rt, dF);
+        M3 dP;
+        if (hybrid && s->ecie_k > 0.1f) {
+            const M3 S = mmul(Rt, mload(&s->F[9 * size_t(e)]));
+            float tr_abs = 0.0f;
+            for (int i = 0; i < 3; i++) tr_abs += std::fabs(S.a[i][i] - 1.0f);
+            float w = (1.0f / s->ecie_k) * tr_abs / 3.0f;
+            w = std::max(0.0f, std::min(w, 1.0f));
+            M3 sym;
+            for (int i = 0; i < 3; i++)
+                for (int j = 0; j < 3; j++) sym.a[i][j] = 0.5f * (RtdF.a[i][j] + RtdF.a[j][i]);
+            const M3 blend = madd(mscale(w, dF), mmul(mscale(1.0f - w, R), sym));
+            dP = madd(mscale(2 * s->mu, blend), mscale(s->lambda * mtrace(RtdF), R));
+        } else {
+            dP = madd(mscale(2 * s->mu, dF), mscale(s->lambda * mtrace(RtdF), R));
+        }
+        scatter_piola(s, e, dP, df);
+    }
+}
+}  // namespace
+
+extern "C" {
+
 // C_f_e (ms/fem/TetrahedronElementSet3.cpp:1053-1096): u_j = R^T x_j - r_j
+// (LCM / LCMH: C_LCM_f_e from the F cached at BeginEvaluation)
 void orc_f_e(const orc_tetset* s, const float* x, float* f) {
+    if (s->mm == ORC_MM_C_LCM || s->mm == ORC_MM_C_LCMH) {
+        lcm_f_e(s, f);
+        return;
+    }
     for (uint32_t e = 0; e < s->T; e++) {
         const uint32_t* t = &s->tet[4 * size_t(e)];
         M3 Rt = mtr(mload(&s->R[9 * size_t(e)]));
@@ -638,6 +733,10 @@ void orc_f_e(const orc_tetset* s, const float* x, float* f) {
 
 // C_df_x with DM = Truncated (:1761-1836; the dR terms are skipped, :901)
 void orc_df_x(const orc_tetset* s, const float* dx, float* df) {
+    if (s->mm == ORC_MM_C_LCM || s->mm == ORC_MM_C_LCMH) {
+        lcm_df_x(s, dx, df, s->mm == ORC_MM_C_LCMH);
+        return;
+    }
     for (uint32_t e = 0; e < s->T; e++) {
         const uint32_t* t = &s->tet[4 * size_t(e)];
         M3 Rt = mtr(mload(&s->R[9 * size_t(e)]));
@@ -647,8 +746,24 @@ void orc_df_x(const orc_tetset* s, const float* dx, float* df) {
     }
 }
 
-// C_V (:1456-1488)
+// C_V (:1456-1488); LCM / LCMH: C_LCM_V (:1489-1508),
+// sum of V (mu |S - I|_F^2 + lambda/2 tr(S - I)^2) with S = R^T F
 float orc_V(const orc_tetset* s, const float* x) {
+    if (s->mm == ORC_MM_C_LCM || s->mm == ORC_MM_C_LCMH) {
+        float acc = 0.0f;
+        for (uint32_t e = 0; e < s->T; e++) {
+            const M3 S = mmul(mtr(mload(&s->R[9 * size_t(e)])), mload(&s->F[9 * size_t(e)]));
+            float nsq = 0.0f, tr = 0.0f;
+            for (int i = 0; i < 3; i++)
+                for (int j = 0; j < 3; j++) {
+                    const float d = S.a[i][j] - ((i == j) ? 1.0f : 0.0f);
+                    nsq += d * d;
+                }
+            for (int i = 0; i < 3; i++) tr += S.a[i][i] - 1.0f;
+            acc += s->vol[e] * (s->mu * nsq + 0.5f * s->lambda * (tr * tr));
+        }
+        return acc;
+    }
     float acc2 = 0.0f;
     for (uint32_t e = 0; e < s->T; e++) {
         const uint32_t* t = &s->tet[4 * size_t(e)];
@@ -819,6 +934,7 @@ orc_solid* orc_solid_create(uint32_t n_nodes, const float* ref_pos, uint32_t n_t
     s->p = *p;
     s->N = n_nodes;
     s->tes = orc_tetset_create(n_nodes, ref_pos, n_tets, tets, p->young_modulus, p->poisson_ratio);
+    orc_tetset_set_material(s->tes, p->mm ? p->mm : ORC_MM_C_WRP, p->ecie_k_factor);
     s->x.assign(ref_pos, ref_pos + 3 * size_t(n_nodes));
     s->x0 = s->x;
     s->v.assign(3 * size_t(n_nodes), 0.0f);

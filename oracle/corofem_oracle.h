@@ -67,6 +67,13 @@ void orc_begin_evaluation(orc_tetset* s, const float* x, int rm, float degenerat
                           float* R_out, float* detF_out, uint32_t* num_degenerate,
                           float* sum_degenerate_detF);
 void orc_set_rotations(orc_tetset* s, const float* R);  /* load R (9*T) directly */
+/* Material model (ms/fem/Params.h:51-61): ORC_MM_C_WRP (default, stored Ke),
+   ORC_MM_C_LCM or ORC_MM_C_LCMH (Piola forms from the F of the last
+   BeginEvaluation), and m_ECIE_k_factor (LCMH blend radius, default 1). */
+#define ORC_MM_C_WRP 1
+#define ORC_MM_C_LCM 2
+#define ORC_MM_C_LCMH 3
+void orc_tetset_set_material(orc_tetset* s, int mm, float ecie_k_factor);
 void orc_f_e(const orc_tetset* s, const float* x, float* f);      /* f  += f_e(x)  */
 void orc_df_x(const orc_tetset* s, const float* dx, float* df);   /* df += K'(dx) */
 float orc_V(const orc_tetset* s, const float* x);
@@ -89,6 +96,8 @@ typedef struct orc_solid_params {
     int integrator;    /* ORC_IT_QIE_V (0, default) or ORC_IT_FIE_DX (1) */
     uint32_t nr_max_iter;  /* FIE-dx Newton iterations (reference default 3) */
     float nr_abs_epsilon;  /* FIE-dx: stop when |b| < this (reference default 1e-6) */
+    int mm;                /* material model, ORC_MM_C_* (0 = C_WRP) */
+    float ecie_k_factor;   /* LCMH blend radius (reference default 1) */
 } orc_solid_params;
 #define ORC_IT_QIE_V 0
 #define ORC_IT_FIE_DX 1

@@ -15,6 +15,7 @@ REF_PATH = os.path.join(HERE, "_ref", "libcorofem_ref.so")
 
 RM_ID, RM_QR, RM_MSVD, RM_DAPD = 0, 1, 2, 3
 IT_QIE_V, IT_FIE_DX = 0, 1
+MM_C_WRP, MM_C_LCM, MM_C_LCMH = 1, 2, 3
 
 _f = C.POINTER(C.c_float)
 _u = C.POINTER(C.c_uint32)
@@ -37,6 +38,7 @@ class SolidParams(C.Structure):
         ("cg_max_iter", C.c_uint32), ("rm", C.c_int), ("degenerate_threshold", C.c_float),
         ("warm_start", C.c_int), ("reduce_fp64", C.c_int),
         ("integrator", C.c_int), ("nr_max_iter", C.c_uint32), ("nr_abs_epsilon", C.c_float),
+        ("mm", C.c_int), ("ecie_k_factor", C.c_float),
     ]
 
 
@@ -87,6 +89,7 @@ def _load():
         "orc_solid_set_state": (None, [C.c_void_p, _f, _f]),
         "orc_solid_node_mass": (C.c_float, [C.c_void_p]),
         "orc_solid_rayleigh": (None, [C.c_void_p, _f]),
+        "orc_tetset_set_material": (None, [C.c_void_p, C.c_int, C.c_float]),
         "orc_multi_update": (C.c_uint32, [C.POINTER(C.c_void_p), C.c_uint32, C.c_float, _f,
                                           C.POINTER(StepStats)]),
         "orc_solid2_create": (C.c_void_p, [C.c_uint32, _f, C.c_uint32, _u, C.POINTER(SolidParams), C.c_float]),
@@ -217,6 +220,9 @@ class TetSet:
         R = np.ascontiguousarray(R, np.float32)
         lib().orc_set_rotations(self.h, fp(R))
 
+    def set_material(self, mm, ecie_k_factor=1.0):
+        lib().orc_tetset_set_material(self.h, mm, ecie_k_factor)
+
     def f_e(self, x, f=None):
         x = np.ascontiguousarray(x, np.float32)
         f = np.zeros(3 * self.N, np.float32) if f is None else f
@@ -254,6 +260,8 @@ def default_params(**kw):
     p.integrator = IT_QIE_V
     p.nr_max_iter = 3
     p.nr_abs_epsilon = 1e-6
+    p.mm = MM_C_WRP
+    p.ecie_k_factor = 1.0
     for k, v in kw.items():
         if k == "gravity":
             for i in range(3):

@@ -7,7 +7,7 @@ LIB_PATH = os.path.join(HERE, "libcorofem.so")
 
 CF_OK = 0
 RM_ID, RM_QR, RM_MSVD, RM_DAPD = 0, 1, 2, 3
-MM_C_WRP = 1
+MM_C_WRP, MM_C_LCM, MM_C_LCMH = 1, 2, 3
 DM_TRUNCATED = 0
 IT_QIE_V, IT_FIE_DX = 1, 3
 CG_CONVERGED_REL, CG_CONVERGED_ABS, CG_MAX_ITER, CG_BREAKDOWN = 1, 2, 3, 4
@@ -22,6 +22,7 @@ class FemParams(C.Structure):
         ("young_modulus", C.c_float), ("poisson_ratio", C.c_float), ("damping_ratio", C.c_float),
         ("rayleigh_freq1", C.c_float), ("rayleigh_freq2", C.c_float),
         ("degenerate_threshold_det_f", C.c_float), ("mm", C.c_int32), ("rm", C.c_int32), ("dm", C.c_int32),
+        ("ecie_k_factor", C.c_float),
     ]
 
 

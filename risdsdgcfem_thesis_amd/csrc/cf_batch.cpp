@@ -745,6 +745,10 @@ int cf_batch_add_sim(cf_batch* b, uint32_t mesh_id, const cf_solid_params* p, ui
     auto mit = b->meshes.find(mesh_id);
     if (mit == b->meshes.end()) { set_error("no mesh %u", mesh_id); return CF_ERR_INVALID_ARG; }
     BMesh& m = mit->second;
+    if (p->fem.mm != CF_MM_C_WRP) {
+        set_error("%s: only the C_WRP material is batched", "cf_batch_add_sim");
+        return CF_ERR_UNSUPPORTED;
+    }
     if (p->integrator != CF_IT_QIE_V) {
         set_error("%s: only the QIE-v integrator is batched", "cf_batch_add_sim");
         return CF_ERR_UNSUPPORTED;

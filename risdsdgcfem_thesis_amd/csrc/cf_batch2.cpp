@@ -233,6 +233,10 @@ int cf_batch2_add_sim(cf_batch2* b, uint32_t N, const float* ref, uint32_t T, co
         set_error("cf_batch2_add_sim: invalid arguments");
         return CF_ERR_INVALID_ARG;
     }
+    if (p->fem.mm != CF_MM_C_WRP) {
+        set_error("%s: only the C_WRP material is batched", "cf_batch2_add_sim");
+        return CF_ERR_UNSUPPORTED;
+    }
     if (p->integrator != CF_IT_QIE_V) {
         set_error("%s: only the QIE-v integrator is batched", "cf_batch2_add_sim");
         return CF_ERR_UNSUPPORTED;

@@ -22,13 +22,17 @@ void set_error(const char* fmt, ...) {
     va_end(ap);
 }
 
-Material material_from(float E, float nu) {
+Material material_from(float E, float nu, int32_t mm) {
     // TetrahedronElement3::Compute_E (ms/fem/TetrahedronElement3.h:141-156)
     Material m;
     const float a = E / ((1.0f + nu) * (1.0f - 2.0f * nu));
     m.a_nu = a * nu;
     m.a_k1 = a * (1.0f - nu);
     m.a_k2 = a * (0.5f - nu);
+    // LameParameters_From_YoungAndPoisson (TetrahedronElementSet3.cpp:143-148)
+    m.mu = 0.5f * E / (1 + nu);
+    m.lambda = E * nu / ((1.0f + nu) * (1.0f - 2.0f * nu));
+    m.mm = mm;
     return m;
 }
 
@@ -107,7 +111,20 @@ struct cf_cache {
     uint32_t num_degenerate = 0, num_fallback = 0;
     float sum_degenerate = 0.f, min_det = 0.f;
     bool stats_valid = false;
+    float* F = nullptr;   // LCM / LCMH: 9 planes of F at the evaluation
+    float* w = nullptr;   // LCMH: per-element blend weight
 };
+
+// The set's material with the cache's per-evaluation arrays (LCM: F for f_e;
+// LCMH: the blend weights for df_x).
+static Material cache_mat(const cf_cache* c) {
+    Material m = c->set->mat;
+    if (m.mm != CF_MM_C_WRP) {
+        m.F = c->F;
+        m.w = (m.mm == CF_MM_C_LCMH) ? c->w : nullptr;
+    }
+    return m;
+}
 
 struct CgBuffers {
     uint32_t n = 0;
@@ -184,7 +201,10 @@ static bool valid_params(const cf_fem_params* p) {
         set_error("invalid material: E=%g nu=%g", p->young_modulus, p->poisson_ratio);
         return false;
     }
-    if (p->mm != CF_MM_C_WRP) { set_error("material method %d not implemented (C_WRP only)", p->mm); return false; }
+    if (p->mm != CF_MM_C_WRP && p->mm != CF_MM_C_LCM && p->mm != CF_MM_C_LCMH) {
+        set_error("material method %d not implemented (C_WRP, C_LCM, C_LCMH)", p->mm);
+        return false;
+    }
     if (p->rm != CF_RM_ID && p->rm != CF_RM_MSVD && p->rm != CF_RM_DAPD) {
         set_error("rotation method %d not implemented (Id, MSVD, DAPD)", p->rm);
         return false;
@@ -238,7 +258,7 @@ static int tetset_bake(cf_tetset* s) {
     float acc = 0.f;
     for (uint32_t i = 0; i < T; i++) acc += vol_user[i];
     s->total_volume = acc;
-    s->mat = material_from(s->params.young_modulus, s->params.poisson_ratio);
+    s->mat = material_from(s->params.young_modulus, s->params.poisson_ratio, s->params.mm);
     s->ref_host.clear(); s->ref_host.shrink_to_fit();
     s->tets_host.clear(); s->tets_host.shrink_to_fit();
     s->editing = false;
@@ -286,7 +306,7 @@ static int solid_cg(cf_solid* s, float fM, float fK, cf_cg_result* res) {
     nt.partials = b.part_a;
     // r = S(b - A y), d = r
     nt.d = s->y; nt.out = b.q;
-    CF_TRY(launch_forces(t, kModeDfx, s->y, s->cache->R, set->mat, st));
+    CF_TRY(launch_forces(t, kModeDfx, s->y, s->cache->R, cache_mat(s->cache), st));
     CF_TRY(launch_gather(t, kGatherMatvec, nt, nnode, st));
     CF_TRY(launch_cg_init(n, b.q, s->bvec, b.r, b.d, nt.knc_mask, b.part_b, nvec, st));
     CF_TRY(launch_cg_start(b.st, b.part_b, nvec, s->p.cg_rel_epsilon, s->p.cg_abs_epsilon, max_it, st));
@@ -297,7 +317,7 @@ static int solid_cg(cf_solid* s, float fM, float fK, cf_cg_result* res) {
         const uint32_t kend = std::min(max_it, k + chunk);
         for (; k < kend; k++) {
             const int32_t* skip = &b.st->done[k];
-            CF_TRY(launch_forces_skip(t, kModeDfx, b.d, s->cache->R, set->mat, skip, st));
+            CF_TRY(launch_forces_skip(t, kModeDfx, b.d, s->cache->R, cache_mat(s->cache), skip, st));
             CF_TRY(launch_gather_skip(t, kGatherMatvec, nt, nnode, skip, st));
             CF_TRY(launch_cg_alpha(b.st, k, b.part_a, nnode, st));
             CF_TRY(launch_cg_update_xr(n, k, b.st, s->y, b.r, b.d, b.q, b.part_b, st));
@@ -343,7 +363,7 @@ static int fie_rhs(cf_solid* s, float dt, float inv_dt, bool first, float* bnorm
     cf_tetset* set = s->set;
     const TetDevice& t = set->dev;
     hipStream_t st = s->stream;
-    CF_TRY(launch_forces(t, kModeFe, s->xk, s->cache->R, set->mat, st));
+    CF_TRY(launch_forces(t, kModeFe, s->xk, s->cache->R, cache_mat(s->cache), st));
     NodeTerms nt;
     nt.out = s->bvec; nt.y = s->y; nt.x = s->x; nt.xk = s->xk; nt.v = s->v;
     nt.knc_mask = s->num_knc ? s->knc_mask : nullptr;
@@ -439,6 +459,7 @@ void cf_default_fem_params(cf_fem_params* p) {
     p->rayleigh_freq2 = 10000.0f;
     p->degenerate_threshold_det_f = 0.1f;
     p->mm = CF_MM_C_WRP;
+    p->ecie_k_factor = 1.0f;
     p->rm = CF_RM_MSVD;
     p->dm = CF_DM_TRUNCATED;
 }
@@ -637,7 +658,7 @@ int cf_tetset_create(uint32_t n_nodes, const float* ref, uint32_t n_tets, const 
 int cf_tetset_set_params(cf_tetset* s, const cf_fem_params* params) {
     if (!s || !valid_params(params)) return CF_ERR_INVALID_ARG;
     s->params = *params;
-    s->mat = material_from(params->young_modulus, params->poisson_ratio);
+    s->mat = material_from(params->young_modulus, params->poisson_ratio, params->mm);
     return CF_OK;
 }
 
@@ -707,7 +728,7 @@ int cf_cache_create(cf_tetset* set, const float* inv_mass, cf_cache** out) {
 int cf_cache_destroy(cf_cache* c) {
     if (!c) return CF_OK;
     dfree(c->R); dfree(c->detF); dfree(c->inv_mass); dfree(c->blk_count); dfree(c->blk_sum);
-    dfree(c->blk_min); dfree(c->blk_energy);
+    dfree(c->blk_min); dfree(c->blk_energy); dfree(c->F); dfree(c->w);
     delete c;
     return CF_OK;
 }
@@ -718,8 +739,18 @@ int cf_begin_evaluation(cf_cache* c, const float* prev_x, const float* x, const 
     c->timestep = t - c->time;
     c->time = t;
     const cf_tetset* s = c->set;
+    const bool lcm = s->params.mm != CF_MM_C_WRP;
+    if (lcm && !c->F) CF_TRY(dalloc(&c->F, 9 * (size_t)s->dev.T));
+    if (lcm && !c->w) CF_TRY(dalloc(&c->w, (size_t)s->dev.T));
+    LcmOut lo;
+    if (lcm) {
+        lo.F = c->F;
+        lo.w = (s->params.mm == CF_MM_C_LCMH) ? c->w : nullptr;
+        lo.inv_k = 1.0f / s->params.ecie_k_factor;  // mal::Rcp (:1898)
+        lo.hybrid = s->params.ecie_k_factor > 0.1f;   // LCMH blend branch (:1893-1897)
+    }
     CF_TRY(launch_begin_eval(s->dev, x, s->params.rm, s->params.degenerate_threshold_det_f, c->R, c->detF,
-                             c->blk_count, c->blk_sum, c->blk_min, c->nblk, s->stream));
+                             c->blk_count, c->blk_sum, c->blk_min, c->nblk, s->stream, lo));
     c->stats_valid = false;
     return CF_OK;
 }
@@ -727,7 +758,7 @@ int cf_begin_evaluation(cf_cache* c, const float* prev_x, const float* x, const 
 int cf_f_e(cf_cache* c, float* f) {
     if (!c || !c->x || !f) { set_error("cf_f_e outside Begin/EndEvaluation or null f"); return CF_ERR_INVALID_ARG; }
     const cf_tetset* s = c->set;
-    CF_TRY(launch_forces(s->dev, kModeFe, c->x, c->R, s->mat, s->stream));
+    CF_TRY(launch_forces(s->dev, kModeFe, c->x, c->R, cache_mat(c), s->stream));
     NodeTerms nt;
     nt.out = f;
     return launch_gather(s->dev, kGatherAccumulate, nt, node_blocks(s->dev.N), s->stream);
@@ -738,7 +769,7 @@ int cf_f(cf_cache* c, float* f) { return cf_f_e(c, f); }  // f = f_e + f_p, f_p 
 int cf_df_x(cf_cache* c, const float* dx, float* df) {
     if (!c || !dx || !df) { set_error("cf_df_x: null argument"); return CF_ERR_INVALID_ARG; }
     const cf_tetset* s = c->set;
-    CF_TRY(launch_forces(s->dev, kModeDfx, dx, c->R, s->mat, s->stream));
+    CF_TRY(launch_forces(s->dev, kModeDfx, dx, c->R, cache_mat(c), s->stream));
     NodeTerms nt;
     nt.out = df;
     return launch_gather(s->dev, kGatherAccumulate, nt, node_blocks(s->dev.N), s->stream);
@@ -747,7 +778,7 @@ int cf_df_x(cf_cache* c, const float* dx, float* df) {
 int cf_V(cf_cache* c, double* energy) {
     if (!c || !c->x || !energy) { set_error("cf_V outside Begin/EndEvaluation"); return CF_ERR_INVALID_ARG; }
     const cf_tetset* s = c->set;
-    CF_TRY(launch_energy(s->dev, c->x, c->R, s->mat, c->blk_energy, c->nblk, s->stream));
+    CF_TRY(launch_energy(s->dev, c->x, c->R, cache_mat(c), c->blk_energy, c->nblk, s->stream));
     std::vector<double> part(c->nblk);
     CF_CHECK_HIP(hipMemcpyAsync(part.data(), c->blk_energy, part.size() * 8, hipMemcpyDeviceToHost, s->stream));
     CF_CHECK_HIP(hipStreamSynchronize(s->stream));
@@ -994,7 +1025,7 @@ int cf_solid_fixed_step(cf_solid* s, float dt, cf_step_stats* stats) {
         return CF_OK;
     }
     // b = dt (M g + f_e(x)) + M v ; y = v ; KNC rows (Step_QuasiImplicitEuler_v :1209-1295)
-    CF_TRY(launch_forces(t, kModeFe, s->x, s->cache->R, set->mat, st));
+    CF_TRY(launch_forces(t, kModeFe, s->x, s->cache->R, cache_mat(s->cache), st));
     NodeTerms nt;
     nt.out = s->bvec; nt.y = s->y; nt.x = s->x; nt.v = s->v;
     nt.knc_mask = s->num_knc ? s->knc_mask : nullptr;
@@ -1176,7 +1207,7 @@ int cf_solid_assemble_rhs(cf_solid* s, float dt, float* h_b, float* h_y0) {
     cf_tetset* set = s->set;
     hipStream_t st = s->stream;
     CF_TRY(cf_begin_evaluation(s->cache, s->x0, s->x, s->v, s->total_time));
-    CF_TRY(launch_forces(set->dev, kModeFe, s->x, s->cache->R, set->mat, st));
+    CF_TRY(launch_forces(set->dev, kModeFe, s->x, s->cache->R, cache_mat(s->cache), st));
     NodeTerms nt;
     nt.out = s->bvec; nt.y = s->y; nt.x = s->x; nt.v = s->v;
     nt.knc_mask = s->num_knc ? s->knc_mask : nullptr;
@@ -1207,7 +1238,7 @@ int cf_solid_apply_A(cf_solid* s, float fM, float fK, const float* h_d, float* h
     nt.fM = fM; nt.fK = fK;
     nt.partials = b.part_a;
     nt.d = b.d; nt.out = b.q;
-    CF_TRY(launch_forces(set->dev, kModeDfx, b.d, s->cache->R, set->mat, st));
+    CF_TRY(launch_forces(set->dev, kModeDfx, b.d, s->cache->R, cache_mat(s->cache), st));
     CF_TRY(launch_gather(set->dev, kGatherMatvec, nt, node_blocks(s->N), st));
     CF_CHECK_HIP(hipStreamSynchronize(st));
     CF_CHECK_HIP(hipMemcpy(h_q, b.q, n3 * 4, hipMemcpyDeviceToHost));
@@ -1223,7 +1254,7 @@ int cf_solid_time_df_x(cf_solid* s, uint32_t reps, double* ms) {
     nt.out = s->scratch_vec;
     CF_CHECK_HIP(hipEventRecord(s->ev0, st));
     for (uint32_t r = 0; r < reps; r++) {
-        CF_TRY(launch_forces(set->dev, kModeDfx, s->v, s->cache->R, set->mat, st));
+        CF_TRY(launch_forces(set->dev, kModeDfx, s->v, s->cache->R, cache_mat(s->cache), st));
         CF_TRY(launch_gather(set->dev, kGatherAccumulate, nt, node_blocks(s->N), st));
     }
     CF_CHECK_HIP(hipEventRecord(s->ev1, st));
@@ -1249,7 +1280,7 @@ int cf_solid_time_kx(cf_solid* s, uint32_t reps, double* ms_tile, double* ms_gat
     const int nnode = node_blocks(s->N);
     float t = 0.f;
     CF_CHECK_HIP(hipEventRecord(s->ev0, st));
-    for (uint32_t r = 0; r < reps; r++) CF_TRY(launch_forces(set->dev, kModeDfx, b.d, s->cache->R, set->mat, st));
+    for (uint32_t r = 0; r < reps; r++) CF_TRY(launch_forces(set->dev, kModeDfx, b.d, s->cache->R, cache_mat(s->cache), st));
     CF_CHECK_HIP(hipEventRecord(s->ev1, st));
     CF_CHECK_HIP(hipEventSynchronize(s->ev1));
     CF_CHECK_HIP(hipEventElapsedTime(&t, s->ev0, s->ev1));

@@ -512,6 +512,123 @@ __device__ inline void element_force(const Slots& s_v, const Slots& s_r, const S
 }
 
 
+// LCM / LCMH nodal forces of one element into its four tile force slots
+// (C_LCM_f_e :1155-1178, C_LCM_df_x :1838-1872, C_LCMH_df_x :1874-1929, with
+// DM = Truncated), in the reference's operation order: f_e from the F of the
+// evaluation, P = R (2 mu (S - I) + lambda tr(S - I) I), S = R^T F; df_x with
+// dF = dDs Dm^-1, dP = 2 mu dF + lambda tr(R^T dF) R, or for LCMH
+// 2 mu (w dF + (1 - w) R sym(R^T dF)) + lambda tr(R^T dF) R; then
+// H = -V P Dm^-T, column c to vertex c+1 and -(sum) to vertex 0. Dm^-1 and V
+// are rebuilt from the rest positions in LDS as for C_WRP.
+template <int MODE>
+__device__ inline void element_force_lcm(const Slots& s_v, const Slots& s_r, const Slots& s_F, const uint32_t* lv,
+                                         const uint32_t* fs, const float* R, const Material& m, const float* Fe,
+                                         float w) {
+#pragma clang fp contract(off)
+    const float3 r0 = make_float3(s_r.x[lv[0]], s_r.y[lv[0]], s_r.z[lv[0]]);
+    const float3 r1 = make_float3(s_r.x[lv[1]], s_r.y[lv[1]], s_r.z[lv[1]]);
+    const float3 r2 = make_float3(s_r.x[lv[2]], s_r.y[lv[2]], s_r.z[lv[2]]);
+    const float3 r3 = make_float3(s_r.x[lv[3]], s_r.y[lv[3]], s_r.z[lv[3]]);
+    const float Dm[9] = {r1.x - r0.x, r2.x - r0.x, r3.x - r0.x,
+                         r1.y - r0.y, r2.y - r0.y, r3.y - r0.y,
+                         r1.z - r0.z, r2.z - r0.z, r3.z - r0.z};
+    const float detDm = refmath::det3(Dm);
+    const float V = detDm / 6.0f;
+    float G[9];
+    refmath::inv3(Dm, detDm, G);
+    float P[9];
+    if (MODE == kModeFe) {
+        float SmI[9];
+        float tr = 0.0f;
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 3; j++) {
+                float a = 0.0f;
+                a += R[i] * Fe[j]; a += R[3 + i] * Fe[3 + j]; a += R[6 + i] * Fe[6 + j];
+                SmI[3 * i + j] = a - ((i == j) ? 1.0f : 0.0f);
+            }
+        tr += SmI[0]; tr += SmI[4]; tr += SmI[8];
+        const float c = 2 * m.mu, lt = m.lambda * tr;
+        float Q[9];
+#pragma unroll
+        for (int k = 0; k < 9; k++) Q[k] = c * SmI[k] + lt * ((k % 4 == 0) ? 1.0f : 0.0f);
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 3; j++) {
+                float a = 0.0f;
+                a += R[3 * i] * Q[j]; a += R[3 * i + 1] * Q[3 + j]; a += R[3 * i + 2] * Q[6 + j];
+                P[3 * i + j] = a;
+            }
+    } else {
+        const float3 x0 = make_float3(s_v.x[lv[0]], s_v.y[lv[0]], s_v.z[lv[0]]);
+        const float3 x1 = make_float3(s_v.x[lv[1]], s_v.y[lv[1]], s_v.z[lv[1]]);
+        const float3 x2 = make_float3(s_v.x[lv[2]], s_v.y[lv[2]], s_v.z[lv[2]]);
+        const float3 x3 = make_float3(s_v.x[lv[3]], s_v.y[lv[3]], s_v.z[lv[3]]);
+        const float D[9] = {x1.x - x0.x, x2.x - x0.x, x3.x - x0.x,
+                            x1.y - x0.y, x2.y - x0.y, x3.y - x0.y,
+                            x1.z - x0.z, x2.z - x0.z, x3.z - x0.z};
+        float dF[9], RtdF[9];
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 3; j++) {
+                float a = 0.0f;
+                a += D[3 * i] * G[j]; a += D[3 * i + 1] * G[3 + j]; a += D[3 * i + 2] * G[6 + j];
+                dF[3 * i + j] = a;
+            }
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 3; j++) {
+                float a = 0.0f;
+                a += R[i] * dF[j]; a += R[3 + i] * dF[3 + j]; a += R[6 + i] * dF[6 + j];
+                RtdF[3 * i + j] = a;
+            }
+        float tr = 0.0f;
+        tr += RtdF[0]; tr += RtdF[4]; tr += RtdF[8];
+        const float c = 2 * m.mu, lt = m.lambda * tr;
+        if (m.w) {
+            float Sy[9], B[9];
+#pragma unroll
+            for (int i = 0; i < 3; i++)
+#pragma unroll
+                for (int j = 0; j < 3; j++) Sy[3 * i + j] = 0.5f * (RtdF[3 * i + j] + RtdF[3 * j + i]);
+            const float ow = 1 - w;
+#pragma unroll
+            for (int i = 0; i < 3; i++)
+#pragma unroll
+                for (int j = 0; j < 3; j++) {
+                    float a = 0.0f;
+                    a += (ow * R[3 * i]) * Sy[j]; a += (ow * R[3 * i + 1]) * Sy[3 + j]; a += (ow * R[3 * i + 2]) * Sy[6 + j];
+                    B[3 * i + j] = w * dF[3 * i + j] + a;
+                }
+#pragma unroll
+            for (int k = 0; k < 9; k++) P[k] = c * B[k] + lt * R[k];
+        } else {
+#pragma unroll
+            for (int k = 0; k < 9; k++) P[k] = c * dF[k] + lt * R[k];
+        }
+    }
+    float H[9];  // (-V P) Dm^-T
+    const float nv = -V;
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            float a = 0.0f;
+            a += (nv * P[3 * i]) * G[3 * j]; a += (nv * P[3 * i + 1]) * G[3 * j + 1]; a += (nv * P[3 * i + 2]) * G[3 * j + 2];
+            H[3 * i + j] = a;
+        }
+    const uint32_t f0 = slot_swizzle(fs[0]), f1 = slot_swizzle(fs[1]), f2 = slot_swizzle(fs[2]),
+                   f3 = slot_swizzle(fs[3]);
+    s_F.x[f1] = H[0]; s_F.y[f1] = H[3]; s_F.z[f1] = H[6];
+    s_F.x[f2] = H[1]; s_F.y[f2] = H[4]; s_F.z[f2] = H[7];
+    s_F.x[f3] = H[2]; s_F.y[f3] = H[5]; s_F.z[f3] = H[8];
+    s_F.x[f0] = (-H[0] - H[1]) - H[2]; s_F.y[f0] = (-H[3] - H[4]) - H[5]; s_F.z[f0] = (-H[6] - H[7]) - H[8];
+}
+
 // ============================================================ block bodies
 // Each body is the work of block `blk` of a grid of `nblk` blocks over one
 // mesh (grid-stride loops over that grid). The single-mesh kernels call them
@@ -527,7 +644,7 @@ __device__ inline void begin_eval_block(uint32_t T, const uint32_t* __restrict__
                                         const float* __restrict__ dinv, int rm, float thr, float* __restrict__ R,
                                         float* __restrict__ detF, uint32_t* blk_count, double* blk_sum,
                                         float* blk_min, uint32_t blk, uint32_t nblk, double* sh, float* shm,
-                                        uint32_t* shc, uint32_t* shf) {
+                                        uint32_t* shc, uint32_t* shf, const LcmOut lcm = LcmOut()) {
 #pragma clang fp contract(off)
     uint32_t cnt = 0, fb = 0;
     double sdet = 0.0;
@@ -555,6 +672,25 @@ __device__ inline void begin_eval_block(uint32_t T, const uint32_t* __restrict__
         }
         for (int k = 0; k < 9; k++) R[k * (size_t)T + e] = Rm[k];
         detF[e] = d;
+        if (lcm.F) {
+            for (int k = 0; k < 9; k++) lcm.F[k * (size_t)T + e] = F[k];
+        }
+        if (lcm.w) {
+            // LCMH weight (C_LCMH_df_x :1893-1899): S = R^T F, only its diagonal enters
+            float wt = 1.0f;
+            if (lcm.hybrid) {
+                float tr = 0.0f;
+                for (int i = 0; i < 3; i++) {
+                    float sii = 0.0f;
+                    for (int k = 0; k < 3; k++) sii += Rm[3 * k + i] * F[3 * k + i];
+                    const float dii = sii - 1.0f;
+                    tr += (dii > 0.0f) ? dii : -dii;
+                }
+                wt = lcm.inv_k * tr / 3.0f;
+                wt = fmaxf(0.0f, fminf(wt, 1.0f));
+            }
+            lcm.w[e] = wt;
+        }
         if (d <= 0.0f) { cnt++; sdet += d; }
         // DAPD elements at or below the threshold take R = Id: counted too.
         if (rm == CF_RM_DAPD && !(d > thr)) fb++;
@@ -583,7 +719,7 @@ __device__ inline void begin_eval_block(uint32_t T, const uint32_t* __restrict__
 // One tile of the element stage without prefetching: stage the tile's node
 // vectors and rest positions in LDS, write each element's nodal forces to its
 // force slots, sum each tile node's contiguous slot run into its partial.
-template <int MODE>
+template <int MODE, bool LCM = false>
 __device__ inline void tile_block(const TileArgs& a, uint32_t t, const Slots& s_v, const Slots& s_r, const Slots& s_F,
                                   uint32_t* s_sp) {
 #pragma clang fp contract(off)
@@ -591,6 +727,17 @@ __device__ inline void tile_block(const TileArgs& a, uint32_t t, const Slots& s_
     const TileDesc dt = tile_desc(a, t);
     TileRegs g;
     load_tile_regs_cold(a, dt, tid, g);
+    float Fe[9];
+    float w = 1.0f;
+    if (LCM) {
+        const uint32_t e = dt.e0 + min(tid, dt.ne - 1);
+        if (MODE == kModeFe) {
+#pragma unroll
+            for (int k = 0; k < 9; k++) Fe[k] = ld32(a.m.F + k * (size_t)a.T, e);
+        } else if (a.m.w) {
+            w = ld32(a.m.w, e);
+        }
+    }
     if (tid < dt.nn) {
         s_v.x[tid] = g.v[0]; s_v.y[tid] = g.v[1]; s_v.z[tid] = g.v[2];
         s_r.x[tid] = g.r[0]; s_r.y[tid] = g.r[1]; s_r.z[tid] = g.r[2];
@@ -601,7 +748,8 @@ __device__ inline void tile_block(const TileArgs& a, uint32_t t, const Slots& s_
         uint32_t fs[4];
 #pragma unroll
         for (int j = 0; j < 4; j++) fs[j] = s_sp[g.l[j]] + g.f[j];
-        element_force<MODE>(s_v, s_r, s_F, g.l, fs, g.R, a.m);
+        if (LCM) element_force_lcm<MODE>(s_v, s_r, s_F, g.l, fs, g.R, a.m, Fe, w);
+        else element_force<MODE>(s_v, s_r, s_F, g.l, fs, g.R, a.m);
     }
     lds_barrier();
     if (tid < dt.nn) {
@@ -718,6 +866,26 @@ __device__ inline void energy_block(uint32_t T, const uint32_t* __restrict__ con
                                     double* blk_part, uint32_t blk, uint32_t nblk, double* sh) {
 #pragma clang fp contract(off)
     double acc = 0.0;
+    if (m.mm != CF_MM_C_WRP && m.F) {
+        // C_LCM_V (TetrahedronElementSet3.cpp:1489-1508): V (mu |S - I|_F^2 + lambda/2 tr(S - I)^2), S = R^T F
+        for (uint32_t e = blk * kBlock + threadIdx.x; e < T; e += nblk * kBlock) {
+            float Rm[9], F[9];
+            for (int k = 0; k < 9; k++) { Rm[k] = R[k * (size_t)T + e]; F[k] = m.F[k * (size_t)T + e]; }
+            float nsq = 0.0f, tr = 0.0f;
+            for (int i = 0; i < 3; i++)
+                for (int j = 0; j < 3; j++) {
+                    float sij = 0.0f;
+                    for (int k = 0; k < 3; k++) sij += Rm[3 * k + i] * F[3 * k + j];
+                    const float d = sij - ((i == j) ? 1.0f : 0.0f);
+                    nsq += d * d;
+                    if (i == j) tr += d;
+                }
+            acc += (double)(vol[e] * (m.mu * nsq + 0.5f * m.lambda * (tr * tr)));
+        }
+        const double s = block_sum(acc, sh);
+        if (threadIdx.x == 0) blk_part[blk] = s;
+        return;
+    }
     for (uint32_t e = blk * kBlock + threadIdx.x; e < T; e += nblk * kBlock) {
         float xs[4][3], rs[4][3];
         for (int j = 0; j < 4; j++) {

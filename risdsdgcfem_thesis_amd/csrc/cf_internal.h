@@ -38,8 +38,12 @@ struct Material {
     float a_k1;  // a(1-nu)   = lambda + 2 mu
     float a_nu;  // a nu      = lambda
     float a_k2;  // a(0.5-nu) = mu
+    int32_t mm = CF_MM_C_WRP;           // material model
+    float mu = 0.f, lambda = 0.f;       // Lame parameters (TetrahedronElementSet3.cpp:143-148)
+    const float* F = nullptr;           // LCM: 9 planes of the evaluation's F (f_e)
+    const float* w = nullptr;           // LCMH: per-element blend weight (df_x), null = LCM
 };
-Material material_from(float young, float poisson);
+Material material_from(float young, float poisson, int32_t mm = CF_MM_C_WRP);
 // Rayleigh coefficients c = inv(Omega) (2 zeta, 2 zeta) (LeafDSH_Solid3D_FEM.h:347-354)
 void rayleigh_coeffs(float f1, float f2, float zeta, float c[2]);
 
@@ -123,9 +127,18 @@ struct NodeTerms {  // per-node terms fused into the node gather
 };
 
 int launch_precompute(const TetDevice& t, hipStream_t s);
+// Per-evaluation outputs of the LCM materials: F (9 planes) and, for LCMH,
+// the blend weight w = clamp01(tr|R^T F - I| inv_k / 3) (1 when !hybrid).
+struct LcmOut {
+    float* F = nullptr;
+    float* w = nullptr;
+    float inv_k = 1.f;
+    bool hybrid = false;
+};
 // blk_count holds 2*nblk counters: [0,nblk) det F <= 0, [nblk,2nblk) DAPD R=Id fallbacks
 int launch_begin_eval(const TetDevice& t, const float* x, int rm, float thr, float* R, float* detF,
-                      uint32_t* blk_count, double* blk_sum, float* blk_min, int nblk, hipStream_t s);
+                      uint32_t* blk_count, double* blk_sum, float* blk_min, int nblk, hipStream_t s,
+                      const LcmOut& lcm = LcmOut());
 int launch_forces(const TetDevice& t, int mode, const float* in, const float* R, Material m, hipStream_t s);
 int launch_forces_skip(const TetDevice& t, int mode, const float* in, const float* R, Material m,
                        const int32_t* skip, hipStream_t s);

@@ -49,12 +49,12 @@ __global__ __launch_bounds__(kBlock) void k_begin_eval(uint32_t T, const uint32_
                                                        const float* __restrict__ x, const float* __restrict__ dinv,
                                                        int rm, float thr, float* __restrict__ R,
                                                        float* __restrict__ detF, uint32_t* blk_count,
-                                                       double* blk_sum, float* blk_min) {
+                                                       double* blk_sum, float* blk_min, LcmOut lcm) {
     __shared__ double sh[kBlock / 64];
     __shared__ float shm[kBlock / 64];
     __shared__ uint32_t shc[kBlock / 64], shf[kBlock / 64];
     begin_eval_block(T, conn, x, dinv, rm, thr, R, detF, blk_count, blk_sum, blk_min, blockIdx.x, gridDim.x, sh,
-                     shm, shc, shf);
+                     shm, shc, shf, lcm);
 }
 
 // Tile kernel: one element and at most one tile node per thread. A workgroup
@@ -135,6 +135,21 @@ __global__ __launch_bounds__(TB) void k_tile_forces(TileArgs a) {
         t = t1; dt = d1;
         t1 = t2; d1 = d2; gid1 = gid2;
     }
+}
+
+// LCM / LCMH element stage: one tile per workgroup, same tile mapping.
+template <int MODE, int TB>
+__global__ __launch_bounds__(TB) void k_tile_forces_lcm(TileArgs a) {
+    if (a.skip && *a.skip) return;
+    extern __shared__ float smem_f[];
+    const TileLds<TB> L(smem_f);
+    const uint32_t xcd = blockIdx.x % 8, j = blockIdx.x / 8;
+    const uint32_t per = (a.num_tiles + 7) / 8;
+    const uint32_t lo = xcd * per;
+    const uint32_t hi = min(a.num_tiles, lo + per);
+    const uint32_t t = lo + j;
+    if (t >= hi) return;
+    tile_block<MODE, true>(a, t, L.v, L.r, L.F, L.sp);
 }
 
 // ============================================================ node gather
@@ -331,9 +346,10 @@ int launch_precompute(const TetDevice& t, hipStream_t s) {
 }
 
 int launch_begin_eval(const TetDevice& t, const float* x, int rm, float thr, float* R, float* detF,
-                      uint32_t* blk_count, double* blk_sum, float* blk_min, int nblk, hipStream_t s) {
+                      uint32_t* blk_count, double* blk_sum, float* blk_min, int nblk, hipStream_t s,
+                      const LcmOut& lcm) {
     hipLaunchKernelGGL(k_begin_eval, dim3(nblk), dim3(kBlock), 0, s, t.T, t.conn, x, t.dinv, rm, thr, R, detF,
-                       blk_count, blk_sum, blk_min);
+                       blk_count, blk_sum, blk_min, lcm);
     return check_launch();
 }
 
@@ -350,6 +366,23 @@ int launch_forces_skip(const TetDevice& t, int mode, const float* in, const floa
     a.in = in; a.partials = t.partials; a.m = m; a.skip = skip;
     const uint32_t grid = t.tile_grid;
     const bool fe = mode == kModeFe;
+    if (m.mm != CF_MM_C_WRP) {
+        const uint32_t g1 = 8 * ((t.num_tiles + 7) / 8);  // one workgroup per tile
+#define CF_TILE_LCM(TBV)                                                                                   \
+    do {                                                                                                   \
+        const size_t sm = TileLds<TBV>::kBytes;                                                            \
+        if (fe) hipLaunchKernelGGL((k_tile_forces_lcm<kModeFe, TBV>), dim3(g1), dim3(TBV), sm, s, a);      \
+        else hipLaunchKernelGGL((k_tile_forces_lcm<kModeDfx, TBV>), dim3(g1), dim3(TBV), sm, s, a);        \
+    } while (0)
+        switch (t.tile_threads) {
+            case 1024: CF_TILE_LCM(1024); break;
+            case 512: CF_TILE_LCM(512); break;
+            case 128: CF_TILE_LCM(128); break;
+            default: CF_TILE_LCM(256); break;
+        }
+#undef CF_TILE_LCM
+        return check_launch();
+    }
 #define CF_TILE_LAUNCH(TBV, PV)                                                                          \
     do {                                                                                                \
         const size_t sm = TileLds<TBV>::kBytes;                                                         \

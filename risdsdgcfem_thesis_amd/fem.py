@@ -127,7 +127,8 @@ def default_solid_params(fem=None, **kw):
         if k == "gravity":
             for i in range(3):
                 p.gravity[i] = v[i]
-        elif k in ("young_modulus", "poisson_ratio", "damping_ratio", "rm", "degenerate_threshold_det_f"):
+        elif k in ("young_modulus", "poisson_ratio", "damping_ratio", "rm", "degenerate_threshold_det_f", "mm",
+                   "ecie_k_factor"):
             setattr(p.fem, k, v)
         else:
             setattr(p, k, v)
