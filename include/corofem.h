@@ -134,6 +134,9 @@ typedef struct cf_cg_result {
 
 void cf_default_fem_params(cf_fem_params* p);
 void cf_default_solid_params(cf_solid_params* p);
+/* Rayleigh coefficients (c0, c1) from damping ratio zeta and frequencies f1, f2
+   (LeafDSH_Solid3D_FEM::RebuildRayleighCoeffsFromFreqs, LeafDSH_Solid3D_FEM.h:347-354). */
+void cf_rayleigh_coeffs(float f1, float f2, float zeta, float c[2]);
 
 /* ---------------------------------------------------------------- runtime */
 const char* cf_last_error(void);
@@ -190,6 +193,12 @@ int cf_begin_evaluation(cf_cache* cache, const float* d_prev_x, const float* d_x
 int cf_f(cf_cache* cache, float* d_f);                       /* += f_e (+ f_p, a no-op) */
 int cf_f_e(cf_cache* cache, float* d_f);                     /* += elastic force (C_f_e :1053-1096) */
 int cf_df_x(cf_cache* cache, const float* d_dx, float* d_df); /* += K(x) dx (C_df_x :1761-1836)    */
+/* += C_f_d (:1098-1150): corotational Rayleigh damping of the evaluation's
+   velocities v, -R (c0 m R^T v + c1 Ke R^T v) per element, with c from the
+   set's damping ratio and frequencies (LeafDSH_Solid3D_FEM.h:347-354) and
+   m = 1 / inv_mass (the cache must have been created with inv_mass). The
+   reference's f() leaves it out (:294-300); its symplectic Euler adds it. */
+int cf_f_d(cf_cache* cache, float* d_f);
 int cf_V(cf_cache* cache, double* h_energy);                 /* C_V (:1456-1488)                  */
 int cf_end_evaluation(cf_cache* cache);
 int cf_cache_stats(const cf_cache* cache, uint32_t* num_degenerate, float* sum_degenerate_det_f,

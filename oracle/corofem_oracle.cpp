@@ -746,6 +746,30 @@ void orc_df_x(const orc_tetset* s, const float* dx, float* df) {
     }
 }
 
+// C_f_d (:1098-1150): corotational Rayleigh damping from the stored Ke (for
+// every corotational material), f_i -= R ((c0 m_i) R^T v_i + c1 sum_j K_ij R^T v_j)
+// per element, m_i = 1 / inv_mass_i.
+void orc_f_d(const orc_tetset* s, const float* v, const float* inv_mass, float c0, float c1, float* f) {
+    for (uint32_t e = 0; e < s->T; e++) {
+        const uint32_t* t = &s->tet[4 * size_t(e)];
+        const float* kb = &s->Kb[size_t(e) * 144];
+        const M3 R = mload(&s->R[9 * size_t(e)]);
+        const M3 Rt = mtr(R);
+        V3 l[4];
+        for (int j = 0; j < 4; j++) l[j] = mvec(Rt, vload(&v[3 * size_t(t[j])]));
+        for (int i = 0; i < 4; i++) {
+            V3 acc = mvec(mload(kb + (4 * i + 0) * 9), l[0]);
+            acc = vadd(acc, mvec(mload(kb + (4 * i + 1) * 9), l[1]));
+            acc = vadd(acc, mvec(mload(kb + (4 * i + 2) * 9), l[2]));
+            acc = vadd(acc, mvec(mload(kb + (4 * i + 3) * 9), l[3]));
+            const float sm = c0 * (1.0f / inv_mass[t[i]]);
+            const V3 w = mvec(R, vadd(vscale(l[i], sm), vscale(acc, c1)));
+            float* o = &f[3 * size_t(t[i])];
+            o[0] -= w.v[0]; o[1] -= w.v[1]; o[2] -= w.v[2];
+        }
+    }
+}
+
 // C_V (:1456-1488); LCM / LCMH: C_LCM_V (:1489-1508),
 // sum of V (mu |S - I|_F^2 + lambda/2 tr(S - I)^2) with S = R^T F
 float orc_V(const orc_tetset* s, const float* x) {

@@ -77,6 +77,9 @@ void orc_tetset_set_material(orc_tetset* s, int mm, float ecie_k_factor);
 void orc_f_e(const orc_tetset* s, const float* x, float* f);      /* f  += f_e(x)  */
 void orc_df_x(const orc_tetset* s, const float* dx, float* df);   /* df += K'(dx) */
 float orc_V(const orc_tetset* s, const float* x);
+/* f += C_f_d(v): corotational Rayleigh damping with coefficients c0 (mass) and
+   c1 (stiffness) and per-node inverse masses (N). */
+void orc_f_d(const orc_tetset* s, const float* v, const float* inv_mass, float c0, float c1, float* f);
 
 /* ---- 3D leaf solid (QIE-v, or FIE-dx with params.integrator) ---- */
 typedef struct orc_solid_params {

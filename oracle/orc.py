@@ -90,6 +90,7 @@ def _load():
         "orc_solid_node_mass": (C.c_float, [C.c_void_p]),
         "orc_solid_rayleigh": (None, [C.c_void_p, _f]),
         "orc_tetset_set_material": (None, [C.c_void_p, C.c_int, C.c_float]),
+        "orc_f_d": (None, [C.c_void_p, _f, _f, C.c_float, C.c_float, _f]),
         "orc_multi_update": (C.c_uint32, [C.POINTER(C.c_void_p), C.c_uint32, C.c_float, _f,
                                           C.POINTER(StepStats)]),
         "orc_solid2_create": (C.c_void_p, [C.c_uint32, _f, C.c_uint32, _u, C.POINTER(SolidParams), C.c_float]),
@@ -222,6 +223,13 @@ class TetSet:
 
     def set_material(self, mm, ecie_k_factor=1.0):
         lib().orc_tetset_set_material(self.h, mm, ecie_k_factor)
+
+    def f_d(self, v, inv_mass, c0, c1):
+        v = np.ascontiguousarray(v, np.float32)
+        im = np.ascontiguousarray(inv_mass, np.float32)
+        f = np.zeros(3 * self.N, np.float32)
+        lib().orc_f_d(self.h, fp(v), fp(im), c0, c1, fp(f))
+        return f
 
     def f_e(self, x, f=None):
         x = np.ascontiguousarray(x, np.float32)

@@ -124,6 +124,8 @@ SIGNATURES = {
     "cf_f": (C.c_int, [_vp, _vp]),
     "cf_f_e": (C.c_int, [_vp, _vp]),
     "cf_df_x": (C.c_int, [_vp, _vp, _vp]),
+    "cf_f_d": (C.c_int, [_vp, _vp]),
+    "cf_rayleigh_coeffs": (None, [C.c_float, C.c_float, C.c_float, _f]),
     "cf_V": (C.c_int, [_vp, C.POINTER(C.c_double)]),
     "cf_end_evaluation": (C.c_int, [_vp]),
     "cf_cache_stats": (C.c_int, [_vp, _u, _f, _f]),

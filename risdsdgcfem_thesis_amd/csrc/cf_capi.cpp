@@ -464,6 +464,10 @@ void cf_default_fem_params(cf_fem_params* p) {
     p->dm = CF_DM_TRUNCATED;
 }
 
+void cf_rayleigh_coeffs(float f1, float f2, float zeta, float c[2]) {
+    if (c) rayleigh_coeffs(f1, f2, zeta, c);
+}
+
 void cf_default_solid_params(cf_solid_params* p) {
     if (!p) return;
     // LeafDSH_Solid3D_FEM::Params() (ds/dsh/LeafDSH_Solid3D_FEM.h:203-274)
@@ -764,7 +768,30 @@ int cf_f_e(cf_cache* c, float* f) {
     return launch_gather(s->dev, kGatherAccumulate, nt, node_blocks(s->dev.N), s->stream);
 }
 
-int cf_f(cf_cache* c, float* f) { return cf_f_e(c, f); }  // f = f_e + f_p, f_p a no-op (:294-300,350-353)
+int cf_f(cf_cache* c, float* f) { return cf_f_e(c, f); }
+
+// C_f_d (TetrahedronElementSet3.cpp:1098-1150) with the set's Rayleigh coefficients
+int cf_f_d(cf_cache* c, float* f) {
+    if (!c || !f || !c->x) { set_error("cf_f_d outside Begin/EndEvaluation"); return CF_ERR_INVALID_ARG; }
+    if (!c->v || !c->inv_mass) {
+        set_error("cf_f_d needs the evaluation's velocities and the cache's inverse masses");
+        return CF_ERR_INVALID_ARG;
+    }
+    const cf_tetset* s = c->set;
+    DampArgs d;
+    float coef[2];
+    rayleigh_coeffs(s->params.rayleigh_freq1, s->params.rayleigh_freq2, s->params.damping_ratio, coef);
+    d.inv_mass = c->inv_mass;
+    d.c0 = coef[0];
+    d.c1 = coef[1];
+    Material m = s->mat;  // the stored-Ke (C_WRP) stiffness for every corotational material
+    m.mm = CF_MM_C_WRP;
+    m.F = m.w = nullptr;
+    CF_TRY(launch_damping(s->dev, c->v, c->R, m, d, s->stream));
+    NodeTerms nt;
+    nt.out = f;
+    return launch_gather(s->dev, kGatherAccumulate, nt, node_blocks(s->dev.N), s->stream);
+}  // f = f_e + f_p, f_p a no-op (:294-300,350-353)
 
 int cf_df_x(cf_cache* c, const float* dx, float* df) {
     if (!c || !dx || !df) { set_error("cf_df_x: null argument"); return CF_ERR_INVALID_ARG; }

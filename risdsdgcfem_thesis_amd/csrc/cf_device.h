@@ -406,16 +406,17 @@ __device__ inline uint32_t slot_swizzle(uint32_t s) { return s ^ ((s >> 5) & 31u
 // TB + 1 slot-run starts.
 // Compile-time plane strides let one LDS address serve the x, y and z planes
 // (immediate offsets).
-template <int TB>
+template <int TB, int EXTRA = 0>
 struct TileLds {
     static_assert(TB % 8 == 0, "slot planes must be whole 32-slot swizzle blocks");
-    static constexpr uint32_t kFloats = 18 * TB;
+    static constexpr uint32_t kFloats = (18 + EXTRA) * TB;
     static constexpr size_t kBytes = (sizeof(float) * kFloats + sizeof(uint32_t) * (TB + 1) + 63) & ~size_t(63);
     Slots v, r, F;
+    float* extra;  // EXTRA planes of TB floats (per tile node)
     uint32_t* sp;
     __device__ explicit TileLds(float* base)
         : v(make_slots(base, TB)), r(make_slots(base + 3 * TB, TB)), F(make_slots(base + 6 * TB, 4 * TB)),
-          sp(reinterpret_cast<uint32_t*>(base + kFloats)) {}
+          extra(base + 18 * TB), sp(reinterpret_cast<uint32_t*>(base + kFloats)) {}
 };
 
 // Sum of one node's slot run [s0, s1) in ascending slot order; the loads of
@@ -440,9 +441,10 @@ __device__ inline void sum_slot_run(const Slots& s_F, uint32_t s0, uint32_t s1, 
     }
 }
 
+// C_WRP element force in Piola form: P (column c is the force on vertex c+1).
 template <int MODE>
-__device__ inline void element_force(const Slots& s_v, const Slots& s_r, const Slots& s_F, const uint32_t* lv,
-                                     const uint32_t* fs, const float* R, const Material& m) {
+__device__ inline void wrp_forces(const Slots& s_v, const Slots& s_r, const uint32_t* lv, const float* R,
+                                  const Material& m, float P[9]) {
     // Every fused multiply-add is written out (fma3); nothing else may be
     // contracted, so the result is fixed by this source in every kernel that
     // inlines it.
@@ -497,12 +499,19 @@ __device__ inline void element_force(const Slots& s_v, const Slots& s_r, const S
 #pragma unroll
         for (int c = 0; c < 3; c++)
             W[3 * r + c] = -V * fma3(S[3 * r + 0], G[3 * c + 0], S[3 * r + 1], G[3 * c + 1], S[3 * r + 2], G[3 * c + 2]);
-    float P[9];  // R W: column c is the force on vertex c+1
 #pragma unroll
     for (int r = 0; r < 3; r++)
 #pragma unroll
         for (int c = 0; c < 3; c++)
             P[3 * r + c] = fma3(R[3 * r + 0], W[0 + c], R[3 * r + 1], W[3 + c], R[3 * r + 2], W[6 + c]);
+}
+
+template <int MODE>
+__device__ inline void element_force(const Slots& s_v, const Slots& s_r, const Slots& s_F, const uint32_t* lv,
+                                     const uint32_t* fs, const float* R, const Material& m) {
+#pragma clang fp contract(off)
+    float P[9];  // R W: column c is the force on vertex c+1
+    wrp_forces<MODE>(s_v, s_r, lv, R, m, P);
     const uint32_t f0 = slot_swizzle(fs[0]), f1 = slot_swizzle(fs[1]), f2 = slot_swizzle(fs[2]),
                    f3 = slot_swizzle(fs[3]);
     s_F.x[f1] = P[0]; s_F.y[f1] = P[3]; s_F.z[f1] = P[6];
@@ -511,6 +520,38 @@ __device__ inline void element_force(const Slots& s_v, const Slots& s_r, const S
     s_F.x[f0] = -(P[0] + P[1] + P[2]); s_F.y[f0] = -(P[3] + P[4] + P[5]); s_F.z[f0] = -(P[6] + P[7] + P[8]);
 }
 
+
+// C_f_d (TetrahedronElementSet3.cpp:1098-1150) of one element into its force
+// slots: f_j = c1 (C_WRP stiffness force of the velocities)_j - R (s_m[j] R^T v_j),
+// with s_m holding c0 m per tile node and s_v the velocities.
+__device__ inline void element_force_fd(const Slots& s_v, const Slots& s_r, const Slots& s_F, const float* s_m,
+                                        const uint32_t* lv, const uint32_t* fs, const float* R, const Material& m,
+                                        float c1) {
+#pragma clang fp contract(off)
+    float P[9];
+    wrp_forces<kModeDfx>(s_v, s_r, lv, R, m, P);
+    float fk[4][3];
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        fk[1][i] = c1 * P[3 * i];
+        fk[2][i] = c1 * P[3 * i + 1];
+        fk[3][i] = c1 * P[3 * i + 2];
+        fk[0][i] = c1 * (-(P[3 * i] + P[3 * i + 1] + P[3 * i + 2]));
+    }
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        const uint32_t l = lv[j];
+        const float v0 = s_v.x[l], v1 = s_v.y[l], v2 = s_v.z[l];
+        const float sm = s_m[l];
+        float mv[3];
+#pragma unroll
+        for (int i = 0; i < 3; i++) mv[i] = sm * fma3(R[i], v0, R[3 + i], v1, R[6 + i], v2);
+        const uint32_t f = slot_swizzle(fs[j]);
+        s_F.x[f] = fk[j][0] - fma3(R[0], mv[0], R[1], mv[1], R[2], mv[2]);
+        s_F.y[f] = fk[j][1] - fma3(R[3], mv[0], R[4], mv[1], R[5], mv[2]);
+        s_F.z[f] = fk[j][2] - fma3(R[6], mv[0], R[7], mv[1], R[8], mv[2]);
+    }
+}
 
 // LCM / LCMH nodal forces of one element into its four tile force slots
 // (C_LCM_f_e :1155-1178, C_LCM_df_x :1838-1872, C_LCMH_df_x :1874-1929, with

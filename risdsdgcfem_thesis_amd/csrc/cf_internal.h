@@ -140,6 +140,12 @@ int launch_begin_eval(const TetDevice& t, const float* x, int rm, float thr, flo
                       uint32_t* blk_count, double* blk_sum, float* blk_min, int nblk, hipStream_t s,
                       const LcmOut& lcm = LcmOut());
 int launch_forces(const TetDevice& t, int mode, const float* in, const float* R, Material m, hipStream_t s);
+// C_f_d element stage (Rayleigh damping of the velocities v) into the tile partials
+struct DampArgs {
+    const float* inv_mass;  // N
+    float c0, c1;
+};
+int launch_damping(const TetDevice& t, const float* v, const float* R, Material m, const DampArgs& d, hipStream_t s);
 int launch_forces_skip(const TetDevice& t, int mode, const float* in, const float* R, Material m,
                        const int32_t* skip, hipStream_t s);
 int launch_gather(const TetDevice& t, int mode, const NodeTerms& nt, int nblk, hipStream_t s);

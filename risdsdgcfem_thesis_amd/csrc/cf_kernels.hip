@@ -152,6 +152,46 @@ __global__ __launch_bounds__(TB) void k_tile_forces_lcm(TileArgs a) {
     tile_block<MODE, true>(a, t, L.v, L.r, L.F, L.sp);
 }
 
+// C_f_d element stage (TetrahedronElementSet3.cpp:1098-1150): the C_WRP
+// stiffness force of the velocities (a.in) scaled by c1, minus the
+// per-vertex mass term R (c0 m_j R^T v_j); tile node masses staged in LDS.
+template <int TB>
+__global__ __launch_bounds__(TB) void k_tile_damping(TileArgs a, DampArgs d) {
+    extern __shared__ float smem_f[];
+    const TileLds<TB, 1> L(smem_f);
+    const uint32_t xcd = blockIdx.x % 8, j = blockIdx.x / 8;
+    const uint32_t per = (a.num_tiles + 7) / 8;
+    const uint32_t lo = xcd * per;
+    const uint32_t hi = min(a.num_tiles, lo + per);
+    const uint32_t t = lo + j;
+    if (t >= hi) return;
+    const uint32_t tid = threadIdx.x;
+    const TileDesc dt = tile_desc(a, t);
+    TileRegs g;
+    load_tile_regs_cold(a, dt, tid, g);
+    if (tid < dt.nn) {
+        const uint32_t gid = ld32(a.tile_nodes, dt.n0 + tid);
+        L.v.x[tid] = g.v[0]; L.v.y[tid] = g.v[1]; L.v.z[tid] = g.v[2];
+        L.r.x[tid] = g.r[0]; L.r.y[tid] = g.r[1]; L.r.z[tid] = g.r[2];
+        L.extra[tid] = d.c0 * (1.0f / ld32(d.inv_mass, gid));  // c0 mal::Rcp(inv_mass) (:1126)
+    }
+    if (tid <= dt.nn) L.sp[tid] = g.sp;
+    lds_barrier();
+    if (tid < dt.ne) {
+        uint32_t fs[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) fs[q] = L.sp[g.l[q]] + g.f[q];
+        element_force_fd(L.v, L.r, L.F, L.extra, g.l, fs, g.R, a.m, d.c1);
+    }
+    lds_barrier();
+    if (tid < dt.nn) {
+        float fx, fy, fz;
+        sum_slot_run(L.F, L.sp[tid], L.sp[tid + 1], fx, fy, fz);
+        float* o = a.partials + 3 * (size_t)g.pd;
+        o[0] = fx; o[1] = fy; o[2] = fz;
+    }
+}
+
 // ============================================================ node gather
 struct GatherArgs {
     uint32_t N;
@@ -427,6 +467,26 @@ int tile_grid_size(const TetDevice& t, int device, int persistent) {
     if (wg_per_xcd < 1) wg_per_xcd = 1;
     if (wg_per_xcd > per_xcd_tiles) wg_per_xcd = per_xcd_tiles;
     return int(8 * wg_per_xcd);
+}
+
+int launch_damping(const TetDevice& t, const float* v, const float* R, Material m, const DampArgs& d,
+                   hipStream_t s) {
+    if (t.num_tiles == 0) return CF_OK;
+    TileArgs a;
+    a.T = t.T; a.num_tiles = t.num_tiles; a.max_te = t.max_tile_elems; a.max_tn = t.max_tile_nodes;
+    a.tile_elem_ptr = t.tile_elem_ptr; a.tile_node_ptr = t.tile_node_ptr; a.tile_nodes = t.tile_nodes;
+    a.part_dst = t.part_dst;
+    a.slot_ptr = t.slot_ptr; a.lv = t.lv; a.rk = t.rk; a.R = R; a.dinv = t.dinv; a.vol = t.vol;
+    a.ref = t.ref;
+    a.in = v; a.partials = t.partials; a.m = m; a.skip = nullptr;
+    const uint32_t g1 = 8 * ((t.num_tiles + 7) / 8);
+    switch (t.tile_threads) {
+        case 1024: hipLaunchKernelGGL(k_tile_damping<1024>, dim3(g1), dim3(1024), (TileLds<1024, 1>::kBytes), s, a, d); break;
+        case 512: hipLaunchKernelGGL(k_tile_damping<512>, dim3(g1), dim3(512), (TileLds<512, 1>::kBytes), s, a, d); break;
+        case 128: hipLaunchKernelGGL(k_tile_damping<128>, dim3(g1), dim3(128), (TileLds<128, 1>::kBytes), s, a, d); break;
+        default: hipLaunchKernelGGL(k_tile_damping<256>, dim3(g1), dim3(256), (TileLds<256, 1>::kBytes), s, a, d); break;
+    }
+    return check_launch();
 }
 
 int launch_forces(const TetDevice& t, int mode, const float* in, const float* R, Material m, hipStream_t s) {

@@ -110,6 +110,13 @@ def mesh_box2(half_sizes, counts):
     return pos, tris
 
 
+def rayleigh_coefficients(f1, f2, zeta):
+    """(c0, c1) of LeafDSH_Solid3D_FEM::RebuildRayleighCoeffsFromFreqs."""
+    c = np.zeros(2, np.float32)
+    lib.cf_rayleigh_coeffs(float(f1), float(f2), float(zeta), _fp(c))
+    return float(c[0]), float(c[1])
+
+
 def default_fem_params(**kw):
     p = FemParams()
     lib.cf_default_fem_params(C.byref(p))
@@ -203,7 +210,7 @@ class TetrahedronElementSet3:
         check(lib.cf_tetset_set_params(self.h, C.byref(params)), "cf_tetset_set_params")
 
     def begin_evaluation(self, x, prev_x=None, v=None, t=0.0):
-        self._x = x
+        self._x, self._v, self._prev_x = x, v, prev_x  # the cache keeps these device pointers
         check(lib.cf_begin_evaluation(self.c, prev_x.ptr if prev_x else None, x.ptr, v.ptr if v else None,
                                       float(t)), "cf_begin_evaluation")
 
@@ -212,6 +219,10 @@ class TetrahedronElementSet3:
 
     def df_x(self, dx, df):
         check(lib.cf_df_x(self.c, dx.ptr, df.ptr), "cf_df_x")
+
+    def f_d(self, f):
+        """+= the Rayleigh damping force of the evaluation's velocities (needs inv_mass)."""
+        check(lib.cf_f_d(self.c, f.ptr), "cf_f_d")
 
     def V(self):
         e = C.c_double()

@@ -74,3 +74,25 @@ def test_batches_refuse_lcm(gpu):
         b.add_sim(mid, gpu.default_solid_params(mm=gpu.MM_C_LCMH), knc=knc)
     assert "status -6" in str(e.value)
     b.close()
+
+
+def test_damping_force_vs_oracle(gpu, orc):
+    pos, tets, _ = beam_config1(gpu)
+    N = pos.size // 3
+    rng = np.random.default_rng(7)
+    x = (pos + rng.standard_normal(pos.size).astype(np.float32) * 2e-3).astype(np.float32)
+    v = rng.standard_normal(pos.size).astype(np.float32)
+    inv_mass = rng.uniform(50.0, 200.0, N).astype(np.float32)
+    fem = gpu.default_fem_params(young_modulus=1e5, poisson_ratio=0.3, rm=gpu.RM_DAPD, damping_ratio=0.5)
+    s = gpu.TetrahedronElementSet3(pos, tets, fem, inv_mass=inv_mass)
+    dxa, dva = gpu.DeviceArray.from_host(x), gpu.DeviceArray.from_host(v)
+    s.begin_evaluation(dxa, v=dva)
+    o = orc.TetSet(pos, tets, 1e5, 0.3)
+    o.begin_evaluation(x, rm=orc.RM_DAPD)
+    c = gpu.rayleigh_coefficients(0.1, 10000.0, 0.5)
+    f = gpu.DeviceArray.from_host(np.zeros(3 * N, np.float32))
+    s.f_d(f)
+    fo = o.f_d(v, inv_mass, c[0], c[1])
+    assert np.linalg.norm(f.numpy() - fo) <= 1e-5 * np.linalg.norm(fo)
+    s.end_evaluation()
+    s.close()

@@ -119,3 +119,26 @@ def test_translations_in_null_space(orc, cf, mm):
     f = ts.df_x(dx)
     scale = np.abs(ts.df_x(rng.normal(size=pos.size).astype(np.float32))).max()
     assert np.abs(f).max() <= 1e-5 * scale
+
+
+def test_damping_force_identities(orc, cf):
+    """C_f_d (TetrahedronElementSet3.cpp:1098-1150): with c0 = 0 it is c1 times
+    the K*x of the velocities; with c1 = 0 and orthonormal rotations each node
+    gets -c0 m_i v_i once per incident element."""
+    pos, tets = mesh(cf)
+    rng = np.random.default_rng(3)
+    x = (pos + 0.002 * rng.normal(size=pos.size)).astype(np.float32)
+    R = exact_polar(pos, tets, x)
+    ts = tetset(orc, pos, tets, orc.MM_C_WRP, x=x, R=R)
+    N = pos.size // 3
+    inv_mass = rng.uniform(50.0, 200.0, N).astype(np.float32)
+    v = rng.normal(size=pos.size).astype(np.float32)
+    c1 = 0.02
+    fd = ts.f_d(v, inv_mass, 0.0, c1).astype(np.float64)
+    kv = ts.df_x(v).astype(np.float64)
+    assert np.abs(fd - c1 * kv).max() <= 1e-6 * np.abs(c1 * kv).max()
+    c0 = 0.7
+    fd = ts.f_d(v, inv_mass, c0, 0.0).astype(np.float64).reshape(-1, 3)
+    valence = np.bincount(tets.astype(np.int64), minlength=N)
+    want = -c0 * (1.0 / inv_mass.astype(np.float64))[:, None] * valence[:, None] * v.reshape(-1, 3)
+    assert np.abs(fd - want).max() <= 1e-5 * np.abs(want).max()
