@@ -65,6 +65,11 @@ extern "C" {
 #define CF_RM_DAPD 3
 /* differential methods (ms/fem/Params.h:68-72); only Truncated is implemented */
 #define CF_DM_TRUNCATED 0
+/* Exact differentials (ms/fem/Params.h:69): K*x adds the McAdams rotation
+   differential dR (ms/fem/Rotation.cpp:622-644; C_df_x dR terms
+   TetrahedronElementSet3.cpp:1804-1834, LCM :1862-1864) for MSVD, and for
+   DAPD above the degeneration threshold; single element sets and leaves. */
+#define CF_DM_EXACT 1
 
 /* ---------------------------------------------------------------- params */
 /* Mirrors ms::fem::Params (ms/fem/Params.h:10-99). */

@@ -22,6 +22,7 @@ struct M3 { float a[3][3]; };
 struct V3 { float v[3]; };
 
 inline M3 mzero() { M3 m; std::memset(&m, 0, sizeof(m)); return m; }
+inline float mtrace_(const M3& m) { float t = 0.0f; for (int i = 0; i < 3; i++) t += m.a[i][i]; return t; }  // Mal/GMat.h:221-227
 inline M3 mid() { M3 m = mzero(); m.a[0][0] = m.a[1][1] = m.a[2][2] = 1.0f; return m; }
 inline M3 mload(const float* p) { M3 m; std::memcpy(m.a, p, 9 * sizeof(float)); return m; }
 inline void mstore(const M3& m, float* p) { std::memcpy(p, m.a, 9 * sizeof(float)); }
@@ -514,6 +515,10 @@ struct orc_tetset {
     std::vector<float> detF;         // T
     std::vector<float> F;            // 9T cache (LCM / LCMH forces)
     int mm = ORC_MM_C_WRP;           // material model (ms/fem/Params.h:51-61)
+    int dm = ORC_DM_TRUNCATED;       // differentials method (Params.h:68-72)
+    int rm_eval = ORC_RM_MSVD;       // rotation method and threshold of the last BeginEvaluation
+    float thr_eval = 0.1f;
+    std::vector<float> X;            // 3N positions of the last BeginEvaluation (cache.m_vecX)
     float ecie_k = 1.0f;             // m_ECIE_k_factor (Params.h:89)
     float mu = 0.0f, lambda = 0.0f;  // Lame parameters (TetrahedronElementSet3.cpp:143-148)
 };
@@ -582,6 +587,9 @@ void orc_tetset_element(const orc_tetset* s, uint32_t e, float K144[144], float 
 // it falls back to R = Id like the reference's invalid-DoC path (:885-890).
 void orc_begin_evaluation(orc_tetset* s, const float* x, int rm, float thr, float* R_out, float* detF_out,
                           uint32_t* num_degenerate, float* sum_degenerate_detF) {
+    s->rm_eval = rm;
+    s->thr_eval = thr;
+    s->X.assign(x, x + 3 * size_t(s->N));
     uint32_t ndeg = 0;
     float sdeg = 0.0f;
     for (uint32_t e = 0; e < s->T; e++) {
@@ -638,9 +646,58 @@ void orc_tetset_set_material(orc_tetset* s, int mm, float ecie_k_factor) {
     s->mm = mm;
     s->ecie_k = ecie_k_factor;
 }
+void orc_tetset_set_dm(orc_tetset* s, int dm) { s->dm = dm; }
 }  // extern "C"
+namespace {
+bool compute_dR(const M3& dDs, const M3& invDm, const M3& F, const M3& R, const M3& Rt, M3& dR);
+}
+extern "C" int orc_dR(const float F[9], const float dF[9], const float R[9], float dR[9]) {
+    M3 out = mzero();
+    const M3 Rm = mload(R);
+    const bool ok = compute_dR(mload(dF), mid(), mload(F), Rm, mtr(Rm), out);
+    mstore(out, dR);
+    return ok ? 1 : 0;
+}
 
 namespace {
+// Compute_dR (ms/fem/Rotation.cpp:622-644, McAdams 2011): dF = dDs Dm^-1,
+// S = R^T F, W = R^T dF, G = S - tr(S) I, w = axial(W - W^T);
+// dR = R skew(G^-1 w) when |det G| > 1e-4 (mal::Epsilon), else no dR.
+bool compute_dR(const M3& dDs, const M3& invDm, const M3& F, const M3& R, const M3& Rt, M3& dR) {
+    const M3 dF = mmul(dDs, invDm);
+    const M3 S = mmul(Rt, F);
+    const M3 W = mmul(Rt, dF);
+    const float tr = mtrace_(S);
+    M3 G = S;
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) G.a[i][j] = S.a[i][j] - tr * ((i == j) ? 1.0f : 0.0f);
+    V3 w;
+    w.v[0] = W.a[1][2] - W.a[2][1];
+    w.v[1] = W.a[2][0] - W.a[0][2];
+    w.v[2] = W.a[0][1] - W.a[1][0];
+    const float d = det3(G);
+    if (!((d > 0.0f ? d : -d) > 0.0001f)) return false;
+    const V3 r = mvec(inv3(G, d), w);
+    M3 K = mzero();
+    K.a[0][1] = -r.v[2]; K.a[0][2] = r.v[1];
+    K.a[1][0] = r.v[2];  K.a[1][2] = -r.v[0];
+    K.a[2][0] = -r.v[1]; K.a[2][1] = r.v[0];
+    dR = mmul(R, K);
+    return true;
+}
+
+// Compute_dR_From_RM (TetrahedronElementSet3.cpp:894-953) for DM = Exact:
+// MSVD always, DAPD above the degeneration threshold.
+bool element_dR(const orc_tetset* s, uint32_t e, const float* dx, M3& dR) {
+    if (s->dm != ORC_DM_EXACT) return false;
+    if (s->rm_eval == ORC_RM_DAPD && !(s->detF[e] > s->thr_eval)) return false;
+    if (s->rm_eval != ORC_RM_DAPD && s->rm_eval != ORC_RM_MSVD) return false;
+    const uint32_t* t = &s->tet[4 * size_t(e)];
+    const M3 R = mload(&s->R[9 * size_t(e)]);
+    const M3 dDs = compute_D(vload(&dx[3 * size_t(t[0])]), vload(&dx[3 * size_t(t[1])]),
+                             vload(&dx[3 * size_t(t[2])]), vload(&dx[3 * size_t(t[3])]));
+    return compute_dR(dDs, mload(&s->Dinv[9 * size_t(e)]), mload(&s->F[9 * size_t(e)]), R, mtr(R), dR);
+}
 inline M3 mlin(float a, const M3& x, float b, const M3& y) {  // a x + b y, element-wise
     M3 r;
     for (int i = 0; i < 3; i++)
@@ -708,6 +765,13 @@ void lcm_df_x(const orc_tetset* s, const float* dx, float* df, bool hybrid) {
         } else {
             dP = madd(mscale(2 * s->mu, dF), mscale(s->lambda * mtrace(RtdF), R));
         }
+        M3 dR;
+        if (element_dR(s, e, dx, dR)) {  // dP += (lambda tr(S - I) - 2 mu) dR (:1862-1864)
+            const M3 S = mmul(Rt, mload(&s->F[9 * size_t(e)]));
+            M3 SmI = S;
+            for (int i = 0; i < 3; i++) SmI.a[i][i] = S.a[i][i] - 1.0f;
+            dP = madd(dP, mscale(s->lambda * mtrace(SmI) - 2 * s->mu, dR));
+        }
         scatter_piola(s, e, dP, df);
     }
 }
@@ -743,6 +807,35 @@ void orc_df_x(const orc_tetset* s, const float* dx, float* df) {
         V3 l[4];
         for (int j = 0; j < 4; j++) l[j] = mvec(Rt, vload(&dx[3 * size_t(t[j])]));
         corot_apply(s, e, l, df);
+        M3 dR;
+        if (element_dR(s, e, dx, dR)) {  // dR terms of C_df_x (:1804-1834)
+            const float* kb = &s->Kb[size_t(e) * 144];
+            V3 u[4], q[4];
+            const M3 dRt = mtr(dR);
+            for (int j = 0; j < 4; j++) {
+                const V3 xj = vload(&s->X[3 * size_t(t[j])]);
+                u[j] = vsub(mvec(Rt, xj), vload(&s->ref[3 * size_t(t[j])]));
+                q[j] = mvec(dRt, xj);
+            }
+            for (int i = 0; i < 4; i++) {  // df -= dR K u
+                V3 acc = mvec(mload(kb + (4 * i + 0) * 9), u[0]);
+                acc = vadd(acc, mvec(mload(kb + (4 * i + 1) * 9), u[1]));
+                acc = vadd(acc, mvec(mload(kb + (4 * i + 2) * 9), u[2]));
+                acc = vadd(acc, mvec(mload(kb + (4 * i + 3) * 9), u[3]));
+                const V3 w = mvec(dR, acc);
+                float* o = &df[3 * size_t(t[i])];
+                o[0] -= w.v[0]; o[1] -= w.v[1]; o[2] -= w.v[2];
+            }
+            for (int i = 0; i < 4; i++) {  // df -= R K dR^T x
+                V3 acc = mvec(mload(kb + (4 * i + 0) * 9), q[0]);
+                acc = vadd(acc, mvec(mload(kb + (4 * i + 1) * 9), q[1]));
+                acc = vadd(acc, mvec(mload(kb + (4 * i + 2) * 9), q[2]));
+                acc = vadd(acc, mvec(mload(kb + (4 * i + 3) * 9), q[3]));
+                const V3 w = mvec(mtr(Rt), acc);
+                float* o = &df[3 * size_t(t[i])];
+                o[0] -= w.v[0]; o[1] -= w.v[1]; o[2] -= w.v[2];
+            }
+        }
     }
 }
 
@@ -959,6 +1052,7 @@ orc_solid* orc_solid_create(uint32_t n_nodes, const float* ref_pos, uint32_t n_t
     s->N = n_nodes;
     s->tes = orc_tetset_create(n_nodes, ref_pos, n_tets, tets, p->young_modulus, p->poisson_ratio);
     orc_tetset_set_material(s->tes, p->mm ? p->mm : ORC_MM_C_WRP, p->ecie_k_factor);
+    orc_tetset_set_dm(s->tes, p->dm);
     s->x.assign(ref_pos, ref_pos + 3 * size_t(n_nodes));
     s->x0 = s->x;
     s->v.assign(3 * size_t(n_nodes), 0.0f);

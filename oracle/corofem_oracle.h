@@ -74,6 +74,15 @@ void orc_set_rotations(orc_tetset* s, const float* R);  /* load R (9*T) directly
 #define ORC_MM_C_LCM 2
 #define ORC_MM_C_LCMH 3
 void orc_tetset_set_material(orc_tetset* s, int mm, float ecie_k_factor);
+/* Differentials method (Params.h:68-72): ORC_DM_TRUNCATED (default) or
+   ORC_DM_EXACT (the McAdams rotation differential dR in df_x for MSVD, and for
+   DAPD above the degeneration threshold). */
+#define ORC_DM_TRUNCATED 0
+#define ORC_DM_EXACT 1
+void orc_tetset_set_dm(orc_tetset* s, int dm);
+/* Compute_dR for one element with Dm^-1 = Id (dDs = dF); returns 0 where the
+   reference skips dR (|det(S - tr(S) I)| <= 1e-4). */
+int orc_dR(const float F[9], const float dF[9], const float R[9], float dR[9]);
 void orc_f_e(const orc_tetset* s, const float* x, float* f);      /* f  += f_e(x)  */
 void orc_df_x(const orc_tetset* s, const float* dx, float* df);   /* df += K'(dx) */
 float orc_V(const orc_tetset* s, const float* x);
@@ -101,6 +110,7 @@ typedef struct orc_solid_params {
     float nr_abs_epsilon;  /* FIE-dx: stop when |b| < this (reference default 1e-6) */
     int mm;                /* material model, ORC_MM_C_* (0 = C_WRP) */
     float ecie_k_factor;   /* LCMH blend radius (reference default 1) */
+    int dm;                /* differentials method, ORC_DM_* (0 = Truncated) */
 } orc_solid_params;
 #define ORC_IT_QIE_V 0
 #define ORC_IT_FIE_DX 1

@@ -16,6 +16,7 @@ REF_PATH = os.path.join(HERE, "_ref", "libcorofem_ref.so")
 RM_ID, RM_QR, RM_MSVD, RM_DAPD = 0, 1, 2, 3
 IT_QIE_V, IT_FIE_DX = 0, 1
 MM_C_WRP, MM_C_LCM, MM_C_LCMH = 1, 2, 3
+DM_TRUNCATED, DM_EXACT = 0, 1
 
 _f = C.POINTER(C.c_float)
 _u = C.POINTER(C.c_uint32)
@@ -38,7 +39,7 @@ class SolidParams(C.Structure):
         ("cg_max_iter", C.c_uint32), ("rm", C.c_int), ("degenerate_threshold", C.c_float),
         ("warm_start", C.c_int), ("reduce_fp64", C.c_int),
         ("integrator", C.c_int), ("nr_max_iter", C.c_uint32), ("nr_abs_epsilon", C.c_float),
-        ("mm", C.c_int), ("ecie_k_factor", C.c_float),
+        ("mm", C.c_int), ("ecie_k_factor", C.c_float), ("dm", C.c_int),
     ]
 
 
@@ -90,6 +91,8 @@ def _load():
         "orc_solid_node_mass": (C.c_float, [C.c_void_p]),
         "orc_solid_rayleigh": (None, [C.c_void_p, _f]),
         "orc_tetset_set_material": (None, [C.c_void_p, C.c_int, C.c_float]),
+        "orc_tetset_set_dm": (None, [C.c_void_p, C.c_int]),
+        "orc_dR": (C.c_int, [_f, _f, _f, _f]),
         "orc_f_d": (None, [C.c_void_p, _f, _f, C.c_float, C.c_float, _f]),
         "orc_multi_update": (C.c_uint32, [C.POINTER(C.c_void_p), C.c_uint32, C.c_float, _f,
                                           C.POINTER(StepStats)]),
@@ -221,6 +224,9 @@ class TetSet:
         R = np.ascontiguousarray(R, np.float32)
         lib().orc_set_rotations(self.h, fp(R))
 
+    def set_dm(self, dm):
+        lib().orc_tetset_set_dm(self.h, dm)
+
     def set_material(self, mm, ecie_k_factor=1.0):
         lib().orc_tetset_set_material(self.h, mm, ecie_k_factor)
 
@@ -270,6 +276,7 @@ def default_params(**kw):
     p.nr_abs_epsilon = 1e-6
     p.mm = MM_C_WRP
     p.ecie_k_factor = 1.0
+    p.dm = DM_TRUNCATED
     for k, v in kw.items():
         if k == "gravity":
             for i in range(3):

@@ -8,7 +8,7 @@ fails loudly when the library has not been built.
 """
 from ._capi import (  # noqa: F401
     lib, LIB_PATH, CorofemError, check, FemParams, SolidParams, StepStats, CgResult, BatchInfo,
-    RM_ID, RM_QR, RM_MSVD, RM_DAPD, MM_C_WRP, MM_C_LCM, MM_C_LCMH, DM_TRUNCATED, IT_QIE_V, IT_FIE_DX,
+    RM_ID, RM_QR, RM_MSVD, RM_DAPD, MM_C_WRP, MM_C_LCM, MM_C_LCMH, DM_TRUNCATED, DM_EXACT, IT_QIE_V, IT_FIE_DX,
     CG_CONVERGED_REL, CG_CONVERGED_ABS, CG_MAX_ITER, CG_BREAKDOWN,
 )
 from .fem import (  # noqa: F401

@@ -8,7 +8,7 @@ LIB_PATH = os.path.join(HERE, "libcorofem.so")
 CF_OK = 0
 RM_ID, RM_QR, RM_MSVD, RM_DAPD = 0, 1, 2, 3
 MM_C_WRP, MM_C_LCM, MM_C_LCMH = 1, 2, 3
-DM_TRUNCATED = 0
+DM_TRUNCATED, DM_EXACT = 0, 1
 IT_QIE_V, IT_FIE_DX = 1, 3
 CG_CONVERGED_REL, CG_CONVERGED_ABS, CG_MAX_ITER, CG_BREAKDOWN = 1, 2, 3, 4
 

@@ -118,10 +118,14 @@ struct cf_cache {
 // The set's material with the cache's per-evaluation arrays (LCM: F for f_e;
 // LCMH: the blend weights for df_x).
 static Material cache_mat(const cf_cache* c) {
+    const cf_fem_params& p = c->set->params;
     Material m = c->set->mat;
-    if (m.mm != CF_MM_C_WRP) {
-        m.F = c->F;
-        m.w = (m.mm == CF_MM_C_LCMH) ? c->w : nullptr;
+    if (m.mm != CF_MM_C_WRP || p.dm == CF_DM_EXACT) m.F = c->F;
+    if (m.mm == CF_MM_C_LCMH) m.w = c->w;
+    if (p.dm == CF_DM_EXACT) {  // Compute_dR_From_RM (TetrahedronElementSet3.cpp:894-953)
+        m.dr = (p.rm == CF_RM_MSVD) ? 1 : (p.rm == CF_RM_DAPD ? 2 : 0);
+        m.thr = p.degenerate_threshold_det_f;
+        m.detF = c->detF;
     }
     return m;
 }
@@ -205,11 +209,14 @@ static bool valid_params(const cf_fem_params* p) {
         set_error("material method %d not implemented (C_WRP, C_LCM, C_LCMH)", p->mm);
         return false;
     }
+    if (p->dm != CF_DM_TRUNCATED && p->dm != CF_DM_EXACT) {
+        set_error("differentials method %d not implemented (Truncated, Exact)", p->dm);
+        return false;
+    }
     if (p->rm != CF_RM_ID && p->rm != CF_RM_MSVD && p->rm != CF_RM_DAPD) {
         set_error("rotation method %d not implemented (Id, MSVD, DAPD)", p->rm);
         return false;
     }
-    if (p->dm != CF_DM_TRUNCATED) { set_error("differential method %d not implemented", p->dm); return false; }
     return true;
 }
 
@@ -744,11 +751,12 @@ int cf_begin_evaluation(cf_cache* c, const float* prev_x, const float* x, const 
     c->time = t;
     const cf_tetset* s = c->set;
     const bool lcm = s->params.mm != CF_MM_C_WRP;
-    if (lcm && !c->F) CF_TRY(dalloc(&c->F, 9 * (size_t)s->dev.T));
+    const bool keep_F = lcm || s->params.dm == CF_DM_EXACT;
+    if (keep_F && !c->F) CF_TRY(dalloc(&c->F, 9 * (size_t)s->dev.T));
     if (lcm && !c->w) CF_TRY(dalloc(&c->w, (size_t)s->dev.T));
     LcmOut lo;
+    if (keep_F) lo.F = c->F;
     if (lcm) {
-        lo.F = c->F;
         lo.w = (s->params.mm == CF_MM_C_LCMH) ? c->w : nullptr;
         lo.inv_k = 1.0f / s->params.ecie_k_factor;  // mal::Rcp (:1898)
         lo.hybrid = s->params.ecie_k_factor > 0.1f;   // LCMH blend branch (:1893-1897)

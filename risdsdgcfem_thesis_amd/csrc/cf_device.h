@@ -553,6 +553,148 @@ __device__ inline void element_force_fd(const Slots& s_v, const Slots& s_r, cons
     }
 }
 
+// Compute_dR (ms/fem/Rotation.cpp:622-644, McAdams 2011) in the reference's
+// order: S = R^T F, W = R^T dF, G = S - tr(S) I, w = (W12 - W21, W20 - W02,
+// W01 - W10); dR = R skew(G^-1 w) when |det G| > 1e-4, else no dR.
+__device__ inline bool compute_dR(const float dF[9], const float F[9], const float R[9], float dR[9]) {
+#pragma clang fp contract(off)
+    float S[9], W[9];
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            float a = 0.0f, b = 0.0f;
+            a += R[i] * F[j]; a += R[3 + i] * F[3 + j]; a += R[6 + i] * F[6 + j];
+            b += R[i] * dF[j]; b += R[3 + i] * dF[3 + j]; b += R[6 + i] * dF[6 + j];
+            S[3 * i + j] = a;
+            W[3 * i + j] = b;
+        }
+    float tr = 0.0f;
+    tr += S[0]; tr += S[4]; tr += S[8];
+    float G[9];
+#pragma unroll
+    for (int k = 0; k < 9; k++) G[k] = S[k] - tr * ((k % 4 == 0) ? 1.0f : 0.0f);
+    const float w0 = W[5] - W[7], w1 = W[6] - W[2], w2 = W[1] - W[3];
+    const float d = refmath::det3(G);
+    if (!((d > 0.0f ? d : -d) > 0.0001f)) return false;
+    float Gi[9];
+    refmath::inv3(G, d, Gi);
+    float r[3];
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        float a = 0.0f;
+        a += Gi[3 * i] * w0; a += Gi[3 * i + 1] * w1; a += Gi[3 * i + 2] * w2;
+        r[i] = a;
+    }
+    const float K[9] = {0.0f, -r[2], r[1], r[2], 0.0f, -r[0], -r[1], r[0], 0.0f};
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            float a = 0.0f;
+            a += R[3 * i] * K[j]; a += R[3 * i + 1] * K[3 + j]; a += R[3 * i + 2] * K[6 + j];
+            dR[3 * i + j] = a;
+        }
+    return true;
+}
+
+// Isotropic stress of the Ke-based C_WRP material for a (non-symmetric)
+// displacement gradient H (TetrahedronElement3::Compute_E, engineering shear).
+__device__ inline void wrp_stress(const Material& m, const float H[9], float S[9]) {
+#pragma clang fp contract(off)
+    const float sxx = fmaf(m.a_k1, H[0], m.a_nu * (H[4] + H[8]));
+    const float syy = fmaf(m.a_k1, H[4], m.a_nu * (H[0] + H[8]));
+    const float szz = fmaf(m.a_k1, H[8], m.a_nu * (H[0] + H[4]));
+    const float sxy = m.a_k2 * (H[1] + H[3]);
+    const float syz = m.a_k2 * (H[5] + H[7]);
+    const float szx = m.a_k2 * (H[2] + H[6]);
+    S[0] = sxx; S[1] = sxy; S[2] = szx; S[3] = sxy; S[4] = syy; S[5] = syz; S[6] = szx; S[7] = syz; S[8] = szz;
+}
+
+// C_df_x with DM = Exact (TetrahedronElementSet3.cpp:1761-1836) in Piola form:
+// df = R W(R^T dF + dR^T F) + dR W(R^T F - I), W(H) = -V sigma(H) Dm^-T, where
+// the first term is the truncated K*x plus -R K dR^T x and the second is
+// -dR K u; F is the evaluation's (cached), dR from Compute_dR.
+__device__ inline void element_force_exact(const Slots& s_v, const Slots& s_r, const Slots& s_F, const uint32_t* lv,
+                                           const uint32_t* fs, const float* R, const Material& m, const float* Fe,
+                                           bool allow_dR) {
+#pragma clang fp contract(off)
+    const float3 r0 = make_float3(s_r.x[lv[0]], s_r.y[lv[0]], s_r.z[lv[0]]);
+    const float3 r1 = make_float3(s_r.x[lv[1]], s_r.y[lv[1]], s_r.z[lv[1]]);
+    const float3 r2 = make_float3(s_r.x[lv[2]], s_r.y[lv[2]], s_r.z[lv[2]]);
+    const float3 r3 = make_float3(s_r.x[lv[3]], s_r.y[lv[3]], s_r.z[lv[3]]);
+    const float Dm[9] = {r1.x - r0.x, r2.x - r0.x, r3.x - r0.x,
+                         r1.y - r0.y, r2.y - r0.y, r3.y - r0.y,
+                         r1.z - r0.z, r2.z - r0.z, r3.z - r0.z};
+    const float detDm = refmath::det3(Dm);
+    const float V = detDm / 6.0f;
+    float Gi[9];
+    refmath::inv3(Dm, detDm, Gi);
+    const float3 x0 = make_float3(s_v.x[lv[0]], s_v.y[lv[0]], s_v.z[lv[0]]);
+    const float3 x1 = make_float3(s_v.x[lv[1]], s_v.y[lv[1]], s_v.z[lv[1]]);
+    const float3 x2 = make_float3(s_v.x[lv[2]], s_v.y[lv[2]], s_v.z[lv[2]]);
+    const float3 x3 = make_float3(s_v.x[lv[3]], s_v.y[lv[3]], s_v.z[lv[3]]);
+    const float D[9] = {x1.x - x0.x, x2.x - x0.x, x3.x - x0.x,
+                        x1.y - x0.y, x2.y - x0.y, x3.y - x0.y,
+                        x1.z - x0.z, x2.z - x0.z, x3.z - x0.z};
+    float dF[9], H1[9];
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            float a = 0.0f;
+            a += D[3 * i] * Gi[j]; a += D[3 * i + 1] * Gi[3 + j]; a += D[3 * i + 2] * Gi[6 + j];
+            dF[3 * i + j] = a;
+        }
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) H1[3 * i + j] = fma3(R[i], dF[j], R[3 + i], dF[3 + j], R[6 + i], dF[6 + j]);
+    float dR[9];
+    const bool ok = allow_dR && compute_dR(dF, Fe, R, dR);
+    float H2[9];
+    if (ok) {
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 3; j++) {
+                H1[3 * i + j] += fma3(dR[i], Fe[j], dR[3 + i], Fe[3 + j], dR[6 + i], Fe[6 + j]);
+                H2[3 * i + j] = fma3(R[i], Fe[j], R[3 + i], Fe[3 + j], R[6 + i], Fe[6 + j]) - ((i == j) ? 1.0f : 0.0f);
+            }
+    }
+    float S[9], W[9], P[9];
+    wrp_stress(m, H1, S);
+#pragma unroll
+    for (int r = 0; r < 3; r++)
+#pragma unroll
+        for (int c = 0; c < 3; c++)
+            W[3 * r + c] = -V * fma3(S[3 * r + 0], Gi[3 * c + 0], S[3 * r + 1], Gi[3 * c + 1], S[3 * r + 2], Gi[3 * c + 2]);
+#pragma unroll
+    for (int r = 0; r < 3; r++)
+#pragma unroll
+        for (int c = 0; c < 3; c++)
+            P[3 * r + c] = fma3(R[3 * r + 0], W[0 + c], R[3 * r + 1], W[3 + c], R[3 * r + 2], W[6 + c]);
+    if (ok) {
+        wrp_stress(m, H2, S);
+#pragma unroll
+        for (int r = 0; r < 3; r++)
+#pragma unroll
+            for (int c = 0; c < 3; c++)
+                W[3 * r + c] = -V * fma3(S[3 * r + 0], Gi[3 * c + 0], S[3 * r + 1], Gi[3 * c + 1], S[3 * r + 2], Gi[3 * c + 2]);
+#pragma unroll
+        for (int r = 0; r < 3; r++)
+#pragma unroll
+            for (int c = 0; c < 3; c++)
+                P[3 * r + c] += fma3(dR[3 * r + 0], W[0 + c], dR[3 * r + 1], W[3 + c], dR[3 * r + 2], W[6 + c]);
+    }
+    const uint32_t f0 = slot_swizzle(fs[0]), f1 = slot_swizzle(fs[1]), f2 = slot_swizzle(fs[2]),
+                   f3 = slot_swizzle(fs[3]);
+    s_F.x[f1] = P[0]; s_F.y[f1] = P[3]; s_F.z[f1] = P[6];
+    s_F.x[f2] = P[1]; s_F.y[f2] = P[4]; s_F.z[f2] = P[7];
+    s_F.x[f3] = P[2]; s_F.y[f3] = P[5]; s_F.z[f3] = P[8];
+    s_F.x[f0] = -(P[0] + P[1] + P[2]); s_F.y[f0] = -(P[3] + P[4] + P[5]); s_F.z[f0] = -(P[6] + P[7] + P[8]);
+}
+
 // LCM / LCMH nodal forces of one element into its four tile force slots
 // (C_LCM_f_e :1155-1178, C_LCM_df_x :1838-1872, C_LCMH_df_x :1874-1929, with
 // DM = Truncated), in the reference's operation order: f_e from the F of the
@@ -564,7 +706,7 @@ __device__ inline void element_force_fd(const Slots& s_v, const Slots& s_r, cons
 template <int MODE>
 __device__ inline void element_force_lcm(const Slots& s_v, const Slots& s_r, const Slots& s_F, const uint32_t* lv,
                                          const uint32_t* fs, const float* R, const Material& m, const float* Fe,
-                                         float w) {
+                                         float w, bool allow_dR = false) {
 #pragma clang fp contract(off)
     const float3 r0 = make_float3(s_r.x[lv[0]], s_r.y[lv[0]], s_r.z[lv[0]]);
     const float3 r1 = make_float3(s_r.x[lv[1]], s_r.y[lv[1]], s_r.z[lv[1]]);
@@ -650,6 +792,20 @@ __device__ inline void element_force_lcm(const Slots& s_v, const Slots& s_r, con
         } else {
 #pragma unroll
             for (int k = 0; k < 9; k++) P[k] = c * dF[k] + lt * R[k];
+        }
+        float dR[9];
+        if (allow_dR && compute_dR(dF, Fe, R, dR)) {
+            // dP += (lambda tr(S - I) - 2 mu) dR (C_LCM_df_x :1862-1864), S = R^T F
+            float trs = 0.0f;
+#pragma unroll
+            for (int i = 0; i < 3; i++) {
+                float a = 0.0f;
+                a += R[i] * Fe[i]; a += R[3 + i] * Fe[3 + i]; a += R[6 + i] * Fe[6 + i];
+                trs += a - 1.0f;
+            }
+            const float coef = m.lambda * trs - 2 * m.mu;
+#pragma unroll
+            for (int k = 0; k < 9; k++) P[k] = P[k] + coef * dR[k];
         }
     }
     float H[9];  // (-V P) Dm^-T
@@ -760,7 +916,11 @@ __device__ inline void begin_eval_block(uint32_t T, const uint32_t* __restrict__
 // One tile of the element stage without prefetching: stage the tile's node
 // vectors and rest positions in LDS, write each element's nodal forces to its
 // force slots, sum each tile node's contiguous slot run into its partial.
-template <int MODE, bool LCM = false>
+// Element kinds of the tile stage: the stored-Ke C_WRP forces, the LCM / LCMH
+// Piola forces, and C_WRP's K*x with the exact rotation differential.
+enum TileKind { kTileWrp = 0, kTileLcm = 1, kTileWrpExact = 2 };
+
+template <int MODE, int KIND = kTileWrp>
 __device__ inline void tile_block(const TileArgs& a, uint32_t t, const Slots& s_v, const Slots& s_r, const Slots& s_F,
                                   uint32_t* s_sp) {
 #pragma clang fp contract(off)
@@ -770,14 +930,15 @@ __device__ inline void tile_block(const TileArgs& a, uint32_t t, const Slots& s_
     load_tile_regs_cold(a, dt, tid, g);
     float Fe[9];
     float w = 1.0f;
-    if (LCM) {
+    bool allow_dR = false;
+    if (KIND != kTileWrp) {
         const uint32_t e = dt.e0 + min(tid, dt.ne - 1);
-        if (MODE == kModeFe) {
+        if (MODE == kModeFe || a.m.dr) {
 #pragma unroll
             for (int k = 0; k < 9; k++) Fe[k] = ld32(a.m.F + k * (size_t)a.T, e);
-        } else if (a.m.w) {
-            w = ld32(a.m.w, e);
         }
+        if (MODE == kModeDfx && a.m.w) w = ld32(a.m.w, e);
+        if (MODE == kModeDfx && a.m.dr) allow_dR = (a.m.dr == 1) || (ld32(a.m.detF, e) > a.m.thr);
     }
     if (tid < dt.nn) {
         s_v.x[tid] = g.v[0]; s_v.y[tid] = g.v[1]; s_v.z[tid] = g.v[2];
@@ -789,7 +950,8 @@ __device__ inline void tile_block(const TileArgs& a, uint32_t t, const Slots& s_
         uint32_t fs[4];
 #pragma unroll
         for (int j = 0; j < 4; j++) fs[j] = s_sp[g.l[j]] + g.f[j];
-        if (LCM) element_force_lcm<MODE>(s_v, s_r, s_F, g.l, fs, g.R, a.m, Fe, w);
+        if (KIND == kTileLcm) element_force_lcm<MODE>(s_v, s_r, s_F, g.l, fs, g.R, a.m, Fe, w, allow_dR);
+        else if (KIND == kTileWrpExact) element_force_exact(s_v, s_r, s_F, g.l, fs, g.R, a.m, Fe, allow_dR);
         else element_force<MODE>(s_v, s_r, s_F, g.l, fs, g.R, a.m);
     }
     lds_barrier();

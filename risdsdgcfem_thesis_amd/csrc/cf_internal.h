@@ -40,8 +40,11 @@ struct Material {
     float a_k2;  // a(0.5-nu) = mu
     int32_t mm = CF_MM_C_WRP;           // material model
     float mu = 0.f, lambda = 0.f;       // Lame parameters (TetrahedronElementSet3.cpp:143-148)
-    const float* F = nullptr;           // LCM: 9 planes of the evaluation's F (f_e)
+    const float* F = nullptr;           // LCM or DM = Exact: 9 planes of the evaluation's F
     const float* w = nullptr;           // LCMH: per-element blend weight (df_x), null = LCM
+    int32_t dr = 0;                     // DM = Exact: 0 none, 1 every element (MSVD), 2 det F > thr (DAPD)
+    float thr = 0.f;                    // degeneration threshold (dr == 2)
+    const float* detF = nullptr;        // per-element det F of the evaluation (dr == 2)
 };
 Material material_from(float young, float poisson, int32_t mm = CF_MM_C_WRP);
 // Rayleigh coefficients c = inv(Omega) (2 zeta, 2 zeta) (LeafDSH_Solid3D_FEM.h:347-354)

@@ -137,9 +137,10 @@ __global__ __launch_bounds__(TB) void k_tile_forces(TileArgs a) {
     }
 }
 
-// LCM / LCMH element stage: one tile per workgroup, same tile mapping.
-template <int MODE, int TB>
-__global__ __launch_bounds__(TB) void k_tile_forces_lcm(TileArgs a) {
+// LCM / LCMH element stage, and C_WRP's K*x with the exact rotation
+// differential: one tile per workgroup, same tile mapping.
+template <int MODE, int TB, int KIND>
+__global__ __launch_bounds__(TB) void k_tile_forces_kind(TileArgs a) {
     if (a.skip && *a.skip) return;
     extern __shared__ float smem_f[];
     const TileLds<TB> L(smem_f);
@@ -149,7 +150,7 @@ __global__ __launch_bounds__(TB) void k_tile_forces_lcm(TileArgs a) {
     const uint32_t hi = min(a.num_tiles, lo + per);
     const uint32_t t = lo + j;
     if (t >= hi) return;
-    tile_block<MODE, true>(a, t, L.v, L.r, L.F, L.sp);
+    tile_block<MODE, KIND>(a, t, L.v, L.r, L.F, L.sp);
 }
 
 // C_f_d element stage (TetrahedronElementSet3.cpp:1098-1150): the C_WRP
@@ -406,13 +407,15 @@ int launch_forces_skip(const TetDevice& t, int mode, const float* in, const floa
     a.in = in; a.partials = t.partials; a.m = m; a.skip = skip;
     const uint32_t grid = t.tile_grid;
     const bool fe = mode == kModeFe;
-    if (m.mm != CF_MM_C_WRP) {
+    if (m.mm != CF_MM_C_WRP || (m.dr && !fe)) {
         const uint32_t g1 = 8 * ((t.num_tiles + 7) / 8);  // one workgroup per tile
-#define CF_TILE_LCM(TBV)                                                                                   \
-    do {                                                                                                   \
-        const size_t sm = TileLds<TBV>::kBytes;                                                            \
-        if (fe) hipLaunchKernelGGL((k_tile_forces_lcm<kModeFe, TBV>), dim3(g1), dim3(TBV), sm, s, a);      \
-        else hipLaunchKernelGGL((k_tile_forces_lcm<kModeDfx, TBV>), dim3(g1), dim3(TBV), sm, s, a);        \
+        const bool lcm = m.mm != CF_MM_C_WRP;
+#define CF_TILE_LCM(TBV)                                                                                           \
+    do {                                                                                                           \
+        const size_t sm = TileLds<TBV>::kBytes;                                                                    \
+        if (fe) hipLaunchKernelGGL((k_tile_forces_kind<kModeFe, TBV, kTileLcm>), dim3(g1), dim3(TBV), sm, s, a);   \
+        else if (lcm) hipLaunchKernelGGL((k_tile_forces_kind<kModeDfx, TBV, kTileLcm>), dim3(g1), dim3(TBV), sm, s, a); \
+        else hipLaunchKernelGGL((k_tile_forces_kind<kModeDfx, TBV, kTileWrpExact>), dim3(g1), dim3(TBV), sm, s, a); \
     } while (0)
         switch (t.tile_threads) {
             case 1024: CF_TILE_LCM(1024); break;

@@ -135,7 +135,7 @@ def default_solid_params(fem=None, **kw):
             for i in range(3):
                 p.gravity[i] = v[i]
         elif k in ("young_modulus", "poisson_ratio", "damping_ratio", "rm", "degenerate_threshold_det_f", "mm",
-                   "ecie_k_factor"):
+                   "ecie_k_factor", "dm", "rayleigh_freq1", "rayleigh_freq2"):
             setattr(p.fem, k, v)
         else:
             setattr(p, k, v)
