@@ -280,6 +280,193 @@ inline void fast_svd3(const M3& F, M3& U, V3& s, M3& Vt, float root_three = kRef
 inline M3 compute_D(const V3& r0, const V3& r1, const V3& r2, const V3& r3) {
     return from_cols(vsub(r1, r0), vsub(r2, r0), vsub(r3, r0));
 }
+// ---- degenerate elements: direction of collapse and the DAPD rotation ----
+// DoC (ms/fem/TetrahedronElement3.h:28-38) is kept as one byte, 4*vit0 + vit1:
+// vertex-face when vit0 == vit1, edge-edge when vit0 < vit1, invalid (3,0) = 12.
+constexpr uint8_t kDocInvalid = 12;
+inline bool doc_valid(uint8_t c) { return (c >> 2) <= (c & 3); }
+
+// gsl_poly_solve_cubic (GNU Scientific Library, poly/solve_cubic.c; the
+// reference links the system GSL and pins no version, the routine is the same
+// in GSL 1.x and 2.x): x^3 + a x^2 + b x + c = 0 in double. One real root by
+// Cardano's formula, three by Viete's trigonometric form, sorted ascending;
+// the exact double root and the triple root are caught by the discriminant
+// tests. Returns the number of roots written.
+int gsl_cubic(double a, double b, double c, double* x0, double* x1, double* x2) {
+    const double q = a * a - 3 * b;
+    const double r = 2 * a * a * a - 9 * a * b + 27 * c;
+    const double Q = q / 9, R = r / 54;
+    const double Q3 = Q * Q * Q, R2 = R * R;
+    const double CR2 = 729 * r * r, CQ3 = 2916 * q * q * q;
+    if (R == 0 && Q == 0) {
+        *x0 = *x1 = *x2 = -a / 3;
+        return 3;
+    }
+    if (CR2 == CQ3) {  // R^2 == Q^3 in integer-exact form: a double root
+        const double sq = std::sqrt(Q);
+        if (R > 0) { *x0 = -2 * sq - a / 3; *x1 = sq - a / 3; *x2 = sq - a / 3; }
+        else { *x0 = -sq - a / 3; *x1 = -sq - a / 3; *x2 = 2 * sq - a / 3; }
+        return 3;
+    }
+    if (R2 < Q3) {
+        const double sgn = (R >= 0) ? 1.0 : -1.0;
+        const double theta = std::acos(sgn * std::sqrt(R2 / Q3));
+        const double norm = -2 * std::sqrt(Q);
+        double r0 = norm * std::cos(theta / 3) - a / 3;
+        double r1 = norm * std::cos((theta + 2.0 * M_PI) / 3) - a / 3;
+        double r2 = norm * std::cos((theta - 2.0 * M_PI) / 3) - a / 3;
+        if (r0 > r1) std::swap(r0, r1);
+        if (r1 > r2) { std::swap(r1, r2); if (r0 > r1) std::swap(r0, r1); }
+        *x0 = r0; *x1 = r1; *x2 = r2;
+        return 3;
+    }
+    const double sgn = (R >= 0) ? 1.0 : -1.0;
+    const double A = -sgn * std::pow(std::fabs(R) + std::sqrt(R2 - Q3), 1.0 / 3.0);
+    *x0 = A + Q / A - a / 3;
+    return 1;
+}
+
+// mal::GSolvePolynomialEq1/2 (Mal/GSolvePolynomialEq.h:11-38) in float, with
+// IsZero's absolute epsilon 1e-4 (Mal/RealUtils.h:110-114).
+inline int solve_eq1(float a, float b, float& x) {
+    if (std::fabs(a) <= 0.0001f) return 0;
+    x = -b / a;
+    return 1;
+}
+inline int solve_eq2(float a, float b, float c, float& x1, float& x2) {
+    if (std::fabs(a) <= 0.0001f) {
+        const int n = solve_eq1(b, c, x1);
+        x2 = x1;
+        return n;
+    }
+    const float disc = b * b - 4 * a * c;
+    if (disc < 0) return 0;
+    const float root = std::sqrt(disc);
+    const float inv = 1.0f / (2.0f * a);
+    x1 = (-b - root) * inv;
+    x2 = (-b + root) * inv;
+    if (x1 > x2) std::swap(x1, x2);
+    return (x1 == x2) ? 1 : 2;
+}
+// mal::GSolvePolynomialEq3 (ms/fem/TetrahedronElementSet3.cpp:50-69):
+// coefficients normalised in float, GSL's cubic in double, roots to float.
+inline int solve_eq3(float a, float b, float c, float d, float x[3]) {
+    if (a != 0.0f) {
+        double r0 = 0, r1 = 0, r2 = 0;
+        const int n = gsl_cubic(double(b / a), double(c / a), double(d / a), &r0, &r1, &r2);
+        x[0] = float(r0); x[1] = float(r1); x[2] = float(r2);
+        return n;
+    }
+    x[2] = 0.0f;
+    return solve_eq2(b, c, d, x[0], x[1]);
+}
+
+// det Ds(t) of the linear path Ds(t) = Da + t Dd is the cubic
+// A t^3 + B t^2 + C t + D'. Its coefficients are sums over the six permutation
+// terms u_x v_y w_z of the determinant, (u, v, w) running over the edge
+// columns 0, 1, 2; a term takes k of its factors from Da and 3-k from Dd.
+// The sums run in the reference's order (:607-636): terms + + + - - -, and in
+// B and C the Da factor (resp. the Dd factor) moving from x to y to z.
+template <class Term>
+inline float perm6(Term f) {
+    return f(0, 1, 2) + f(1, 2, 0) + f(2, 0, 1) - f(2, 1, 0) - f(1, 0, 2) - f(0, 2, 1);
+}
+
+// ComputeDoC (ms/fem/TetrahedronElementSet3.cpp:589-743): the time of
+// collapse toc in (0,1] at which det Ds on the path a -> b reaches
+// thr * 6 V, the collapse "normal" N (row 2 of V^T in Ds(toc) = U S V^T: the
+// right singular vector of the smallest singular value, as the reference
+// takes it, not the left one normal to the flattened element), then the
+// crossing: vertex-face when one face's
+// N-projected area is half the sum of all four (absolute epsilon 1e-4), else
+// the first edge pair (0j, kl) whose triangle normals around edge 0j disagree.
+uint8_t compute_doc(const V3 a[4], const V3 b[4], float volume, float thr, float* toc_out = nullptr) {
+    const V3 d0 = vsub(b[0], a[0]), d1 = vsub(b[1], a[1]), d2 = vsub(b[2], a[2]), d3 = vsub(b[3], a[3]);
+    const V3 dd[3] = {vsub(d1, d0), vsub(d2, d0), vsub(d3, d0)};
+    const V3 da[3] = {vsub(a[1], a[0]), vsub(a[2], a[0]), vsub(a[3], a[0])};
+    const float K = thr * 6.0f * volume;
+    const float A = perm6([&](int p, int q, int r) { return dd[p].v[0] * dd[q].v[1] * dd[r].v[2]; });
+    const float B = perm6([&](int p, int q, int r) {
+        return da[p].v[0] * dd[q].v[1] * dd[r].v[2] + dd[p].v[0] * da[q].v[1] * dd[r].v[2] +
+               dd[p].v[0] * dd[q].v[1] * da[r].v[2];
+    });
+    const float C = perm6([&](int p, int q, int r) {
+        return da[p].v[0] * da[q].v[1] * dd[r].v[2] + da[p].v[0] * dd[q].v[1] * da[r].v[2] +
+               dd[p].v[0] * da[q].v[1] * da[r].v[2];
+    });
+    const float D = perm6([&](int p, int q, int r) { return da[p].v[0] * da[q].v[1] * da[r].v[2]; }) - K;
+    float roots[3] = {0.0f, 0.0f, 0.0f};
+    const int n = solve_eq3(A, B, C, D, roots);
+    if (n <= 0) return kDocInvalid;
+    float toc = 2.0f;
+    int n01 = 0;
+    for (int i = 0; i < n; i++)
+        if (roots[i] > 0.0f && roots[i] <= 1.0f && roots[i] < toc) { toc = roots[i]; n01++; }
+    if (n01 != 1 && n01 != 3) return kDocInvalid;
+    if (toc_out) *toc_out = toc;
+    V3 q[4];
+    q[0] = vadd(a[0], vscale(d0, toc)); q[1] = vadd(a[1], vscale(d1, toc));
+    q[2] = vadd(a[2], vscale(d2, toc)); q[3] = vadd(a[3], vscale(d3, toc));
+    const V3 q10 = vsub(q[1], q[0]), q20 = vsub(q[2], q[0]), q30 = vsub(q[3], q[0]);
+    M3 U, Vt;
+    V3 sv;
+    fast_svd3(from_cols(q10, q20, q30), U, sv, Vt);
+    V3 N;
+    for (int i = 0; i < 3; i++) N.v[i] = Vt.a[2][i];
+    float area[4];
+    area[0] = std::fabs(vdot(N, vcross(vsub(q[2], q[1]), vsub(q[3], q[1]))));
+    area[1] = std::fabs(vdot(N, vcross(q20, q30)));
+    area[2] = std::fabs(vdot(N, vcross(q10, q30)));
+    area[3] = std::fabs(vdot(N, vcross(q10, q20)));
+    const float sum = area[0] + area[1] + area[2] + area[3];
+    uint8_t doc = kDocInvalid;
+    for (int i = 0; i < 4; i++)
+        if (std::fabs(area[i] - 0.5f * sum) <= 0.0001f) doc = uint8_t(5 * i);
+    if (doc_valid(doc)) return doc;
+    if (vdot(vcross(q10, q20), vcross(q10, q30)) < 0.0f) return 1;  // (0,1)
+    if (vdot(vcross(q20, q10), vcross(q20, q30)) < 0.0f) return 2;  // (0,2)
+    if (vdot(vcross(q30, q10), vcross(q30, q20)) < 0.0f) return 3;  // (0,3)
+    return 0;                                                        // forced vertex-face 0
+}
+
+// Compute_R_DAPD_Degenerate (ms/fem/Rotation.cpp:364-545): move the collapsing
+// vertex (vertex-face) or edge (edge-edge) along the unit collapse axis past the
+// collapse by lambda_u (cubic Hermite for 0 < w <= dh, linear plus optional
+// power term for w <= 0), then R = Newton polar factor of the projected F.
+// Axis shorter than 1e-3 (squared length 1e-6): R = Id.
+M3 r_dapd_degenerate(float detF, float thr, uint8_t doc, const V3 p[4], float volume, const M3& invDm,
+                     float fL, float fNL, float eNL, bool* ok) {
+    V3 x[4] = {p[0], p[1], p[2], p[3]};
+    const int v0 = doc >> 2, v1 = doc & 3;
+    int i0, i1, i2, i3;  // moved vertex/vertices, then the ones defining the axis
+    V3 axis;
+    if (v0 == v1) {  // vertex-face: vertex v0 against the face of the next three
+        i0 = v0; i1 = (v0 + 1) % 4; i2 = (v0 + 2) % 4; i3 = (v0 + 3) % 4;
+        axis = vcross(vsub(x[i2], x[i1]), vsub(x[i3], x[i1]));
+    } else {         // edge-edge: edge (0, v1) against the opposite edge
+        i0 = v0; i1 = v1;
+        i2 = (v1 == 1) ? 2 : 1;
+        i3 = (v1 == 3) ? 2 : 3;
+        axis = vcross(vsub(x[i1], x[i0]), vsub(x[i3], x[i2]));
+    }
+    const float len_sq = vdot(axis, axis);
+    if (!(len_sq > 0.000001f)) { *ok = false; return mid(); }
+    const float len = std::sqrt(len_sq);
+    axis = vscale(axis, 1.0f / len);
+    const float dh = thr * 6.0f * volume / len;
+    // the moved vertex's height over the axis' plane
+    float w = (v0 == v1) ? vdot(vsub(x[i0], x[i1]), axis) : vdot(vsub(x[i0], x[i2]), axis);
+    if (w * detF < 0.0f) { w = -w; axis = vscale(axis, -1.0f); }
+    const float s = w / dh;
+    const float t = 1.0f - s;
+    const float lambda = (w > 0.0f) ? fL * dh * ((2.0f - t) * t * t) : fL * dh * (t + fNL * std::pow(-s, eNL));
+    x[i0] = vadd(x[i0], vscale(axis, lambda));
+    if (v0 != v1) x[i1] = vadd(x[i1], vscale(axis, lambda));
+    const M3 projF = mmul(compute_D(x[0], x[1], x[2], x[3]), invDm);
+    *ok = true;
+    return polar_newton(projF, nullptr);
+}
+
 float compute_K(const V3& r0, const V3& r1, const V3& r2, const V3& r3, float E, float nu, float K[144]) {
     float volume = det3(compute_D(r0, r1, r2, r3)) / 6.0f;
     float x1 = r0.v[0], x2 = r1.v[0], x3 = r2.v[0], x4 = r3.v[0];
@@ -521,6 +708,9 @@ struct orc_tetset {
     std::vector<float> X;            // 3N positions of the last BeginEvaluation (cache.m_vecX)
     float ecie_k = 1.0f;             // m_ECIE_k_factor (Params.h:89)
     float mu = 0.0f, lambda = 0.0f;  // Lame parameters (TetrahedronElementSet3.cpp:143-148)
+    std::vector<uint8_t> doc;        // T directions of collapse (ElementCache3::m_DoC), kept across evaluations
+    float dapd_l = 1.0f, dapd_nl = 0.0f, dapd_e = 2.0f;  // m_DAPD_L/NL_Factor, m_DAPD_NL_Exponent (Params.h:88)
+    uint32_t n_fallback = 0;         // DAPD elements at or below the threshold left with R = Id
 };
 
 extern "C" {
@@ -542,6 +732,7 @@ orc_tetset* orc_tetset_create(uint32_t n_nodes, const float* ref_pos, uint32_t n
     s->vol.resize(n_tets);
     s->R.resize(size_t(n_tets) * 9);
     s->detF.resize(n_tets);
+    s->doc.assign(n_tets, kDocInvalid);  // CreateCache: DoC() is invalid (TetrahedronElement3.h:30)
     float acc = 0.0f;
     float K[144];
     for (uint32_t e = 0; e < n_tets; e++) {
@@ -582,33 +773,57 @@ void orc_tetset_element(const orc_tetset* s, uint32_t e, float K144[144], float 
     if (vol) *vol = s->vol[e];
 }
 
-// BeginEvaluation (ms/fem/TetrahedronElementSet3.cpp:219-281) + ComputeRotations_*
-// (:748-892). The DAPD degenerate branch (det F <= threshold) is out of scope;
-// it falls back to R = Id like the reference's invalid-DoC path (:885-890).
-void orc_begin_evaluation(orc_tetset* s, const float* x, int rm, float thr, float* R_out, float* detF_out,
-                          uint32_t* num_degenerate, float* sum_degenerate_detF) {
+// BeginEvaluation (ms/fem/TetrahedronElementSet3.cpp:219-281), its
+// UpdateDegeneration (:477-532, the __NEW_CROSSING_DETECTION variant) and
+// ComputeRotations_* (:748-892). prev_x may be NULL (then x). Under DAPD an
+// element at or below the threshold with a valid DoC takes
+// Compute_R_DAPD_Degenerate; one without (or with a collapsed axis) takes
+// R = Id (:885-890) and is counted in n_fallback.
+void orc_begin_evaluation(orc_tetset* s, const float* prev_x, const float* x, int rm, float thr, float* R_out,
+                          float* detF_out, uint32_t* num_degenerate, float* sum_degenerate_detF) {
+    if (!prev_x) prev_x = x;
     s->rm_eval = rm;
     s->thr_eval = thr;
     s->X.assign(x, x + 3 * size_t(s->N));
     uint32_t ndeg = 0;
     float sdeg = 0.0f;
+    s->n_fallback = 0;
     for (uint32_t e = 0; e < s->T; e++) {
         const uint32_t* t = &s->tet[4 * size_t(e)];
-        V3 x0 = vload(&x[3 * size_t(t[0])]), x1 = vload(&x[3 * size_t(t[1])]);
-        V3 x2 = vload(&x[3 * size_t(t[2])]), x3 = vload(&x[3 * size_t(t[3])]);
-        M3 Ds = compute_D(x0, x1, x2, x3);
-        M3 F = mmul(Ds, mload(&s->Dinv[9 * size_t(e)]));
+        const V3 xs[4] = {vload(&x[3 * size_t(t[0])]), vload(&x[3 * size_t(t[1])]), vload(&x[3 * size_t(t[2])]),
+                          vload(&x[3 * size_t(t[3])])};
+        const M3 Dinv = mload(&s->Dinv[9 * size_t(e)]);
+        M3 Ds = compute_D(xs[0], xs[1], xs[2], xs[3]);
+        M3 F = mmul(Ds, Dinv);
         float dF = det3(F);
         s->detF[e] = dF;
         mstore(F, &s->F[9 * size_t(e)]);
         ndeg += (dF <= 0) ? 1u : 0u;
         sdeg += (dF <= 0) ? dF : 0.0f;
+        // UpdateDegeneration: undegenerate clears the DoC; an element that was
+        // undegenerate at prev_x and is degenerate now gets a new one; one that
+        // was degenerate already keeps its DoC
+        if (dF > thr) {
+            s->doc[e] = kDocInvalid;
+        } else {
+            const V3 ps[4] = {vload(&prev_x[3 * size_t(t[0])]), vload(&prev_x[3 * size_t(t[1])]),
+                              vload(&prev_x[3 * size_t(t[2])]), vload(&prev_x[3 * size_t(t[3])])};
+            if (det3(mmul(compute_D(ps[0], ps[1], ps[2], ps[3]), Dinv)) > thr)
+                s->doc[e] = compute_doc(ps, xs, s->vol[e], thr);
+        }
         M3 R;
         switch (rm) {
             case ORC_RM_MSVD: { M3 u, vt; V3 sv; fast_svd3(F, u, sv, vt); R = mmul(u, vt); } break;
             case ORC_RM_DAPD:
-                if (dF > thr) R = polar_newton(F, nullptr);
-                else R = mid();
+                if (dF > thr) {
+                    R = polar_newton(F, nullptr);
+                } else {
+                    bool ok = false;
+                    if (doc_valid(s->doc[e]))
+                        R = r_dapd_degenerate(dF, thr, s->doc[e], xs, s->vol[e], Dinv, s->dapd_l, s->dapd_nl,
+                                              s->dapd_e, &ok);
+                    if (!ok) { R = mid(); s->n_fallback++; }
+                }
                 break;
             case ORC_RM_ID:
             default: R = mid(); break;
@@ -622,6 +837,31 @@ void orc_begin_evaluation(orc_tetset* s, const float* x, int rm, float thr, floa
 }
 
 void orc_set_rotations(orc_tetset* s, const float* R) { std::memcpy(s->R.data(), R, s->R.size() * sizeof(float)); }
+
+void orc_tetset_get_doc(const orc_tetset* s, uint8_t* doc) { std::memcpy(doc, s->doc.data(), s->doc.size()); }
+void orc_tetset_set_doc(orc_tetset* s, const uint8_t* doc) { std::memcpy(s->doc.data(), doc, s->doc.size()); }
+void orc_tetset_set_dapd(orc_tetset* s, float l_factor, float nl_factor, float nl_exponent) {
+    s->dapd_l = l_factor; s->dapd_nl = nl_factor; s->dapd_e = nl_exponent;
+}
+uint32_t orc_tetset_num_fallback(const orc_tetset* s) { return s->n_fallback; }
+
+int orc_gsl_cubic(double a, double b, double c, double roots[3]) {
+    roots[0] = roots[1] = roots[2] = 0.0;
+    return gsl_cubic(a, b, c, &roots[0], &roots[1], &roots[2]);
+}
+int orc_solve_eq3(float a, float b, float c, float d, float roots[3]) { return solve_eq3(a, b, c, d, roots); }
+uint8_t orc_compute_doc(const float a[12], const float b[12], float volume, float thr, float* toc) {
+    const V3 av[4] = {vload(a), vload(a + 3), vload(a + 6), vload(a + 9)};
+    const V3 bv[4] = {vload(b), vload(b + 3), vload(b + 6), vload(b + 9)};
+    return compute_doc(av, bv, volume, thr, toc);
+}
+int orc_r_dapd_degenerate(float detF, float thr, uint8_t doc, const float p[12], float volume, const float inv_dm[9],
+                          float l_factor, float nl_factor, float nl_exponent, float R[9]) {
+    const V3 pv[4] = {vload(p), vload(p + 3), vload(p + 6), vload(p + 9)};
+    bool ok = false;
+    mstore(r_dapd_degenerate(detF, thr, doc, pv, volume, mload(inv_dm), l_factor, nl_factor, nl_exponent, &ok), R);
+    return ok ? 1 : 0;
+}
 
 namespace {
 // Shared body of C_f_e / C_df_x: v_i -= R * sum_j K_ij * l_j, with l_j the
@@ -973,8 +1213,8 @@ struct orc_solid {
 
 namespace {
 
-void solid_begin_evaluation(orc_solid* s, const float* x) {
-    orc_begin_evaluation(s->tes, x, s->p.rm, s->p.degenerate_threshold, nullptr, nullptr, &s->ndeg, &s->sdeg);
+void solid_begin_evaluation(orc_solid* s, const float* prev_x, const float* x) {
+    orc_begin_evaluation(s->tes, prev_x, x, s->p.rm, s->p.degenerate_threshold, nullptr, nullptr, &s->ndeg, &s->sdeg);
     s->min_det = 1e30f;
     for (uint32_t e = 0; e < s->tes->T; e++) s->min_det = std::min(s->min_det, s->tes->detF[e]);
 }
@@ -1094,7 +1334,7 @@ static void step_fie_dx(orc_solid* s, float dt, int32_t* nr_iters, float* nr_rel
             fie_update(s, inv_dt);
             delta_new = solid_norm(s, s->y);
             nr_k++;
-            solid_begin_evaluation(s, s->xk.data());  // End/BeginEvaluation at x_k (:1526-1530)
+            solid_begin_evaluation(s, s->x.data(), s->xk.data());  // End/BeginEvaluation at x_k (:1526-1530)
         }
     } while (!conv && nr_k < s->p.nr_max_iter);
     *nr_iters = int32_t(nr_k);
@@ -1108,7 +1348,7 @@ static void step_fie_dx(orc_solid* s, float dt, int32_t* nr_iters, float* nr_rel
 // Step_FullyImplicitEuler_dx (:1357-1552)
 void orc_solid_fixed_step(orc_solid* s, float dt, orc_step_stats* st) {
     if (s->p.integrator == ORC_IT_FIE_DX) {
-        solid_begin_evaluation(s, s->x.data());
+        solid_begin_evaluation(s, s->x0.data(), s->x.data());
         s->x0 = s->x;
         int32_t nr = 0, ls = 0;
         float nr_rel = 0, nr_abs = 0;
@@ -1129,7 +1369,8 @@ void orc_solid_fixed_step(orc_solid* s, float dt, orc_step_stats* st) {
     const float mass = 1.0f / s->inv_mass;  // GetMass (ms/ParticleSystem.h:74)
     // UpdateKNC, static mode: KNC target = rest position (:877-896)
     uint32_t ndeg = 0; float sdeg = 0.0f;
-    orc_begin_evaluation(s->tes, s->x.data(), s->p.rm, s->p.degenerate_threshold, nullptr, nullptr, &ndeg, &sdeg);
+    orc_begin_evaluation(s->tes, s->x0.data(), s->x.data(), s->p.rm, s->p.degenerate_threshold, nullptr, nullptr, &ndeg,
+                         &sdeg);
     float min_det = 1e30f;
     for (uint32_t e = 0; e < s->tes->T; e++) min_det = std::min(min_det, s->tes->detF[e]);
     s->x0 = s->x;
@@ -1206,7 +1447,7 @@ uint32_t orc_multi_update(orc_solid* const* solids, uint32_t n, float dt, float*
     while (*acc_time > fixed_dt) {
         for (uint32_t i = 0; i < n; i++) {  // NR_Init (:2824-2836)
             orc_solid* s = solids[i];
-            solid_begin_evaluation(s, s->x.data());
+            solid_begin_evaluation(s, s->x0.data(), s->x.data());
             s->x0 = s->x;
         }
         uint32_t nr = 0;
@@ -1221,7 +1462,7 @@ uint32_t orc_multi_update(orc_solid* const* solids, uint32_t n, float dt, float*
             for (uint32_t i = 0; i < n; i++) {  // NR_PostStep (:2987-3037)
                 orc_solid* s = solids[i];
                 fie_update(s, inv_dt);
-                solid_begin_evaluation(s, s->xk.data());
+                solid_begin_evaluation(s, s->x.data(), s->xk.data());
             }
         } while (++nr < max_nr);
         for (uint32_t i = 0; i < n; i++) {  // NR_End (:2838-2852)

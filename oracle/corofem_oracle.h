@@ -61,12 +61,26 @@ orc_tetset* orc_tetset_create(uint32_t n_nodes, const float* ref_pos, uint32_t n
 void orc_tetset_destroy(orc_tetset* s);
 float orc_tetset_total_volume(const orc_tetset* s);
 void orc_tetset_element(const orc_tetset* s, uint32_t e, float K144[144], float Dinv[9], float* vol);
-/* BeginEvaluation: computes F, det F, R per element into the set's cache.
-   R_out (9*T, row-major per element) and detF_out (T) may be NULL. */
-void orc_begin_evaluation(orc_tetset* s, const float* x, int rm, float degenerate_threshold,
-                          float* R_out, float* detF_out, uint32_t* num_degenerate,
-                          float* sum_degenerate_detF);
+/* BeginEvaluation: computes F, det F, R per element into the set's cache and
+   updates the per-element direction of collapse (DoC) from prev_x -> x
+   (prev_x may be NULL: x). R_out (9*T, row-major per element) and detF_out (T)
+   may be NULL. */
+void orc_begin_evaluation(orc_tetset* s, const float* prev_x, const float* x, int rm,
+                          float degenerate_threshold, float* R_out, float* detF_out,
+                          uint32_t* num_degenerate, float* sum_degenerate_detF);
 void orc_set_rotations(orc_tetset* s, const float* R);  /* load R (9*T) directly */
+/* Degenerate elements (DAPD). DoC byte = 4*vit0 + vit1; 12 = invalid. */
+void orc_tetset_get_doc(const orc_tetset* s, uint8_t* doc);
+void orc_tetset_set_doc(orc_tetset* s, const uint8_t* doc);
+void orc_tetset_set_dapd(orc_tetset* s, float l_factor, float nl_factor, float nl_exponent);
+uint32_t orc_tetset_num_fallback(const orc_tetset* s);  /* last BeginEvaluation's R = Id count */
+int orc_gsl_cubic(double a, double b, double c, double roots[3]);          /* x^3 + a x^2 + b x + c */
+int orc_solve_eq3(float a, float b, float c, float d, float roots[3]);     /* a x^3 + b x^2 + c x + d */
+uint8_t orc_compute_doc(const float a[12], const float b[12], float volume, float thr,
+                        float* toc /* may be NULL; set when a crossing is found */);
+int orc_r_dapd_degenerate(float detF, float thr, uint8_t doc, const float p[12], float volume,
+                          const float inv_dm[9], float l_factor, float nl_factor,
+                          float nl_exponent, float R[9]);
 /* Material model (ms/fem/Params.h:51-61): ORC_MM_C_WRP (default, stored Ke),
    ORC_MM_C_LCM or ORC_MM_C_LCMH (Piola forms from the F of the last
    BeginEvaluation), and m_ECIE_k_factor (LCMH blend radius, default 1). */

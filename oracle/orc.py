@@ -20,6 +20,9 @@ DM_TRUNCATED, DM_EXACT = 0, 1
 
 _f = C.POINTER(C.c_float)
 _u = C.POINTER(C.c_uint32)
+_b = C.POINTER(C.c_uint8)
+_d = C.POINTER(C.c_double)
+DOC_INVALID = 12  # direction-of-collapse byte 4*vit0 + vit1; (3,0) is invalid
 
 
 def fp(a):
@@ -76,7 +79,16 @@ def _load():
         "orc_tetset_destroy": (None, [C.c_void_p]),
         "orc_tetset_total_volume": (C.c_float, [C.c_void_p]),
         "orc_tetset_element": (None, [C.c_void_p, C.c_uint32, _f, _f, _f]),
-        "orc_begin_evaluation": (None, [C.c_void_p, _f, C.c_int, C.c_float, _f, _f, _u, _f]),
+        "orc_begin_evaluation": (None, [C.c_void_p, _f, _f, C.c_int, C.c_float, _f, _f, _u, _f]),
+        "orc_tetset_get_doc": (None, [C.c_void_p, _b]),
+        "orc_tetset_set_doc": (None, [C.c_void_p, _b]),
+        "orc_tetset_set_dapd": (None, [C.c_void_p, C.c_float, C.c_float, C.c_float]),
+        "orc_tetset_num_fallback": (C.c_uint32, [C.c_void_p]),
+        "orc_gsl_cubic": (C.c_int, [C.c_double, C.c_double, C.c_double, _d]),
+        "orc_solve_eq3": (C.c_int, [C.c_float] * 4 + [_f]),
+        "orc_compute_doc": (C.c_uint8, [_f, _f, C.c_float, C.c_float, _f]),
+        "orc_r_dapd_degenerate": (C.c_int, [C.c_float, C.c_float, C.c_uint8, _f, C.c_float, _f,
+                                            C.c_float, C.c_float, C.c_float, _f]),
         "orc_set_rotations": (None, [C.c_void_p, _f]),
         "orc_f_e": (None, [C.c_void_p, _f, _f]),
         "orc_df_x": (None, [C.c_void_p, _f, _f]),
@@ -211,14 +223,31 @@ class TetSet:
         lib().orc_tetset_element(self.h, e, fp(K), fp(D), fp(v))
         return K.reshape(12, 12), D.reshape(3, 3), float(v[0])
 
-    def begin_evaluation(self, x, rm=RM_MSVD, thr=0.1):
+    def begin_evaluation(self, x, rm=RM_MSVD, thr=0.1, prev_x=None):
         x = np.ascontiguousarray(x, np.float32)
+        px = None if prev_x is None else np.ascontiguousarray(prev_x, np.float32)
         R = np.zeros(9 * self.T, np.float32)
         dF = np.zeros(self.T, np.float32)
         nd = C.c_uint32()
         sd = np.zeros(1, np.float32)
-        lib().orc_begin_evaluation(self.h, fp(x), rm, thr, fp(R), fp(dF), C.byref(nd), fp(sd))
+        lib().orc_begin_evaluation(self.h, None if px is None else fp(px), fp(x), rm, thr, fp(R), fp(dF),
+                                   C.byref(nd), fp(sd))
         return R, dF, nd.value, float(sd[0])
+
+    def doc(self):
+        out = np.zeros(self.T, np.uint8)
+        lib().orc_tetset_get_doc(self.h, out.ctypes.data_as(_b))
+        return out
+
+    def set_doc(self, doc):
+        doc = np.ascontiguousarray(doc, np.uint8)
+        lib().orc_tetset_set_doc(self.h, doc.ctypes.data_as(_b))
+
+    def set_dapd(self, l_factor=1.0, nl_factor=0.0, nl_exponent=2.0):
+        lib().orc_tetset_set_dapd(self.h, l_factor, nl_factor, nl_exponent)
+
+    def num_fallback(self):
+        return lib().orc_tetset_num_fallback(self.h)
 
     def set_rotations(self, R):
         R = np.ascontiguousarray(R, np.float32)
@@ -252,6 +281,39 @@ class TetSet:
     def V(self, x):
         x = np.ascontiguousarray(x, np.float32)
         return lib().orc_V(self.h, fp(x))
+
+
+def gsl_cubic(a, b, c):
+    """Real roots of x^3 + a x^2 + b x + c (ascending), GSL's method in double."""
+    r = np.zeros(3, np.float64)
+    n = lib().orc_gsl_cubic(a, b, c, r.ctypes.data_as(_d))
+    return r[:n]
+
+
+def solve_eq3(a, b, c, d):
+    """mal::GSolvePolynomialEq3 in float: (number of roots, the three slots)."""
+    r = np.zeros(3, np.float32)
+    n = lib().orc_solve_eq3(a, b, c, d, fp(r))
+    return n, r
+
+
+def compute_doc(a, b, volume, thr=0.1, with_toc=False):
+    """ComputeDoC for the linear path a -> b (4x3 each): the DoC byte, and
+    the time of collapse (NaN when none was found) with with_toc."""
+    a = np.ascontiguousarray(a, np.float32).ravel()
+    b = np.ascontiguousarray(b, np.float32).ravel()
+    toc = np.full(1, np.nan, np.float32)
+    doc = int(lib().orc_compute_doc(fp(a), fp(b), volume, thr, fp(toc)))
+    return (doc, float(toc[0])) if with_toc else doc
+
+
+def r_dapd_degenerate(det_f, thr, doc, p, volume, inv_dm, l_factor=1.0, nl_factor=0.0, nl_exponent=2.0):
+    p = np.ascontiguousarray(p, np.float32).ravel()
+    inv_dm = np.ascontiguousarray(inv_dm, np.float32).ravel()
+    R = np.zeros(9, np.float32)
+    ok = lib().orc_r_dapd_degenerate(det_f, thr, doc, fp(p), volume, fp(inv_dm), l_factor, nl_factor,
+                                     nl_exponent, fp(R))
+    return bool(ok), R.reshape(3, 3)
 
 
 def default_params(**kw):
