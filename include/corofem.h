@@ -84,6 +84,9 @@ typedef struct cf_fem_params {
     int32_t rm;                       /* default CF_RM_MSVD  */
     int32_t dm;                       /* default CF_DM_TRUNCATED */
     float ecie_k_factor;              /* LCMH blend radius k (m_ECIE_k_factor); default 1 */
+    float dapd_l_factor;              /* DAPD degenerate projection: m_DAPD_L_Factor, default 1 */
+    float dapd_nl_factor;             /* m_DAPD_NL_Factor, default 0 */
+    float dapd_nl_exponent;           /* m_DAPD_NL_Exponent, default 2 (ms/fem/Params.h:29,88) */
 } cf_fem_params;
 
 /* Mirrors the leaf's integrator/solver knobs (ds/dsh/LeafDSH_Solid3D_FEM.h:93-274). */
@@ -187,14 +190,22 @@ int cf_tetset_get_elements(const cf_tetset* set, float* h_inv_dm, float* h_vol);
 /* ---------------------------------------------------------------- IForce3 cache
  * CreateCache(inv_mass) / BeginEvaluation / f / df_x / V / EndEvaluation
  * (TetrahedronElementSet3.h:48-55,86-97). x, prev_x, v, f, dx, df: device,
- * 3N floats. BeginEvaluation computes F, det F and the rotation R per element.
- * With CF_RM_DAPD, elements with det F <= threshold take R = Id (the
- * reference's invalid-DoC fallback, TetrahedronElementSet3.cpp:885-890); the
- * DoC degenerate branch is not implemented and such elements are counted. */
+ * 3N floats. BeginEvaluation computes F, det F and the rotation R per element
+ * and updates each element's direction of collapse (DoC) from prev_x -> x
+ * (UpdateDegeneration, TetrahedronElementSet3.cpp:477-743; prev_x may be NULL:
+ * x). With CF_RM_DAPD an element with det F <= threshold takes the projected
+ * rotation of Compute_R_DAPD_Degenerate (Rotation.cpp:364-545) when it has a
+ * DoC, else R = Id (the reference's invalid-DoC path, :885-890), counted as a
+ * fallback. */
 typedef struct cf_cache cf_cache;
 int cf_cache_create(cf_tetset* set, const float* h_inv_mass /* N, may be NULL */, cf_cache** out);
 int cf_cache_destroy(cf_cache* cache);
 int cf_begin_evaluation(cf_cache* cache, const float* d_prev_x, const float* d_x, const float* d_v, float t);
+/* The cache's per-element DoC bytes (ElementCache3::m_DoC, TetrahedronElement3.h:28-38):
+   4*vit0 + vit1, vertex-face when vit0 == vit1, edge-edge (0, vit1) when vit0 < vit1,
+   12 = invalid. T bytes, host memory. set rejects other codes. */
+int cf_cache_get_doc(cf_cache* cache, uint8_t* h_doc);
+int cf_cache_set_doc(cf_cache* cache, const uint8_t* h_doc);
 int cf_f(cf_cache* cache, float* d_f);                       /* += f_e (+ f_p, a no-op) */
 int cf_f_e(cf_cache* cache, float* d_f);                     /* += elastic force (C_f_e :1053-1096) */
 int cf_df_x(cf_cache* cache, const float* d_dx, float* d_df); /* += K(x) dx (C_df_x :1761-1836)    */
@@ -208,6 +219,9 @@ int cf_V(cf_cache* cache, double* h_energy);                 /* C_V (:1456-1488)
 int cf_end_evaluation(cf_cache* cache);
 int cf_cache_stats(const cf_cache* cache, uint32_t* num_degenerate, float* sum_degenerate_det_f,
                    float* min_det_f);
+/* Elements of the last BeginEvaluation that DAPD left at R = Id (det F <= threshold
+   without a DoC or with a collapsed axis; TetrahedronElementSet3.cpp:885-890). */
+int cf_cache_num_fallback(const cf_cache* cache, uint32_t* num_fallback);
 /* R per element in the caller's element order (9T row-major) and det F (T). */
 int cf_cache_get_rotations(const cf_cache* cache, float* h_R, float* h_det_f);
 int cf_cache_synchronize(cf_cache* cache);

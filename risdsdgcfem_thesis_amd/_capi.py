@@ -22,7 +22,8 @@ class FemParams(C.Structure):
         ("young_modulus", C.c_float), ("poisson_ratio", C.c_float), ("damping_ratio", C.c_float),
         ("rayleigh_freq1", C.c_float), ("rayleigh_freq2", C.c_float),
         ("degenerate_threshold_det_f", C.c_float), ("mm", C.c_int32), ("rm", C.c_int32), ("dm", C.c_int32),
-        ("ecie_k_factor", C.c_float),
+        ("ecie_k_factor", C.c_float), ("dapd_l_factor", C.c_float), ("dapd_nl_factor", C.c_float),
+        ("dapd_nl_exponent", C.c_float),
     ]
 
 
@@ -130,6 +131,9 @@ SIGNATURES = {
     "cf_end_evaluation": (C.c_int, [_vp]),
     "cf_cache_stats": (C.c_int, [_vp, _u, _f, _f]),
     "cf_cache_get_rotations": (C.c_int, [_vp, _f, _f]),
+    "cf_cache_get_doc": (C.c_int, [_vp, C.POINTER(C.c_uint8)]),
+    "cf_cache_set_doc": (C.c_int, [_vp, C.POINTER(C.c_uint8)]),
+    "cf_cache_num_fallback": (C.c_int, [_vp, _u]),
     "cf_cache_synchronize": (C.c_int, [_vp]),
     "cf_cg_create": (C.c_int, [C.c_uint32, C.POINTER(_vp)]),
     "cf_cg_destroy": (C.c_int, [_vp]),

@@ -227,7 +227,7 @@ struct MeshLayout {
 
 // Device layout of a simulation's state (one arena block).
 struct SimLayout {
-    size_t vec[8], R, detF, part, mask, pa, pb, pe, ec, es, em, total;
+    size_t vec[8], R, detF, part, mask, pa, pb, pe, ec, es, em, x0, doc, total;
     uint32_t nb_elem, nb_node, nb_init, nb_vec;
     SimLayout(uint32_t N, uint32_t T, uint32_t P) {
         nb_elem = uint32_t(node_blocks(T));
@@ -246,6 +246,8 @@ struct SimLayout {
         ec = l.add<uint32_t>(2 * (size_t)nb_elem);
         es = l.add<double>(nb_elem);
         em = l.add<float>(nb_elem);
+        x0 = l.add<float>(3 * (size_t)N);  // DAPD: previous positions and DoC
+        doc = l.add<uint8_t>(T);
         total = Arena::round(l.off);
     }
 };
@@ -604,6 +606,9 @@ void sim_params(BSim& s, const BMesh& m) {
     d.abs_eps = p.cg_abs_epsilon;
     d.g[0] = p.gravity[0]; d.g[1] = p.gravity[1]; d.g[2] = p.gravity[2];
     d.rm = p.fem.rm;
+    d.dapd_l = p.fem.dapd_l_factor;
+    d.dapd_nl = p.fem.dapd_nl_factor;
+    d.dapd_e = p.fem.dapd_nl_exponent;
     d.warm_start = p.warm_start;
     d.compute_energy = p.compute_energy;
     d.max_iter = std::min<uint32_t>(p.cg_max_iter, kMaxCgIter);
@@ -804,6 +809,10 @@ int cf_batch_add_sim(cf_batch* b, uint32_t mesh_id, const cf_solid_params* p, ui
     d.eval_count = at<uint32_t>(s.block, L.ec);
     d.eval_sum = at<double>(s.block, L.es);
     d.eval_min = at<float>(s.block, L.em);
+    if (p->fem.rm == CF_RM_DAPD) {
+        d.x0 = at<float>(s.block, L.x0);
+        d.doc = at<uint8_t>(s.block, L.doc);
+    }
     sim_params(s, m);
     // initial state, KNC, and R = Id (CreateCache, TetrahedronElementSet3.cpp:190-209)
     hipStream_t st = b->stream;
@@ -824,6 +833,11 @@ int cf_batch_add_sim(cf_batch* b, uint32_t mesh_id, const cf_solid_params* p, ui
     chk(hipMemcpyAsync(mask, hmask.data(), m.N, hipMemcpyHostToDevice, st));
     chk(hipMemcpyAsync(d.knc_target, tgt.data(), n3 * 4, hipMemcpyHostToDevice, st));
     chk(hipMemcpyAsync(d.R, Rid.data(), Rid.size() * 4, hipMemcpyHostToDevice, st));
+    if (d.x0) {  // m_vecPos0 starts at the rest pose like a leaf's (a given initial
+                 // state is a set_state, which leaves it), every DoC invalid
+        chk(hipMemcpyAsync(d.x0, m.ref_host.data(), n3 * 4, hipMemcpyHostToDevice, st));
+        chk(hipMemsetAsync(d.doc, kDocInvalid, m.T, st));
+    }
     chk(hipMemcpyAsync(b->d_sims + slot, &d, sizeof(SimDev), hipMemcpyHostToDevice, st));
     chk(hipMemsetAsync(b->d_cgs + slot, 0, sizeof(SimCg), st));  // idle, no steps done
     chk(hipStreamSynchronize(st));

@@ -61,7 +61,7 @@ __device__ inline uint32_t wave_slot(const BatchTables& t) {
 }  // namespace
 
 // ------------------------------------------------------------ step prologue
-__global__ __launch_bounds__(kBlock) void kb_begin_eval(BatchTables t) {
+__global__ __launch_bounds__(kBlock, 8) void kb_begin_eval(BatchTables t) {
     __shared__ double sh[kBlock / 64];
     __shared__ float shm[kBlock / 64];
     __shared__ uint32_t shc[kBlock / 64], shf[kBlock / 64];
@@ -69,8 +69,15 @@ __global__ __launch_bounds__(kBlock) void kb_begin_eval(BatchTables t) {
     const BlockRef br = t.elem[blockIdx.x];
     if (br.slot == kIdleSlot || t.cgs[br.slot].phase != kPhaseStart) return;
     const SimDev& s = t.sims[br.slot];
+    DegenArgs dg;
+    dg.prev_x = s.x0;
+    dg.doc = s.doc;
+    dg.vol = s.vol;
+    dg.l_factor = s.dapd_l;
+    dg.nl_factor = s.dapd_nl;
+    dg.nl_exponent = s.dapd_e;
     begin_eval_block(s.T, s.conn, s.x, s.dinv, s.rm, s.thr, s.R, s.detF, s.eval_count, s.eval_sum, s.eval_min,
-                     br.blk, s.nb_elem, sh, shm, shc, shf);
+                     br.blk, s.nb_elem, sh, shm, shc, shf, LcmOut(), dg);
 }
 
 // Element stage over every tile of the batch: f_e on x, or df_x on y (the
@@ -225,7 +232,7 @@ __global__ __launch_bounds__(kBlock) void kb_integrate(BatchTables t) {
     const BlockRef br = t.init[blockIdx.x];
     if (br.slot == kIdleSlot || t.cgs[br.slot].phase != kPhaseEnd) return;
     const SimDev& s = t.sims[br.slot];
-    integrate_block(3 * s.N, s.x, s.v, s.y, s.dt, s.drag, br.blk, s.nb_init);
+    integrate_block(3 * s.N, s.x, s.v, s.y, s.dt, s.drag, br.blk, s.nb_init, s.x0);
 }
 
 __global__ __launch_bounds__(kBlock) void kb_energy(BatchTables t) {

@@ -113,6 +113,7 @@ struct cf_cache {
     bool stats_valid = false;
     float* F = nullptr;   // LCM / LCMH: 9 planes of F at the evaluation
     float* w = nullptr;   // LCMH: per-element blend weight
+    uint8_t* doc = nullptr;  // per-element direction of collapse (ElementCache3::m_DoC)
 };
 
 // The set's material with the cache's per-evaluation arrays (LCM: F for f_e;
@@ -215,6 +216,11 @@ static bool valid_params(const cf_fem_params* p) {
     }
     if (p->rm != CF_RM_ID && p->rm != CF_RM_MSVD && p->rm != CF_RM_DAPD) {
         set_error("rotation method %d not implemented (Id, MSVD, DAPD)", p->rm);
+        return false;
+    }
+    if (!std::isfinite(p->dapd_l_factor) || !std::isfinite(p->dapd_nl_factor) ||
+        !std::isfinite(p->dapd_nl_exponent)) {
+        set_error("DAPD factors must be finite");
         return false;
     }
     return true;
@@ -469,6 +475,9 @@ void cf_default_fem_params(cf_fem_params* p) {
     p->ecie_k_factor = 1.0f;
     p->rm = CF_RM_MSVD;
     p->dm = CF_DM_TRUNCATED;
+    p->dapd_l_factor = 1.0f;   // Params.h:88
+    p->dapd_nl_factor = 0.0f;
+    p->dapd_nl_exponent = 2.0f;
 }
 
 void cf_rayleigh_coeffs(float f1, float f2, float zeta, float c[2]) {
@@ -718,6 +727,7 @@ int cf_cache_create(cf_tetset* set, const float* inv_mass, cf_cache** out) {
     if (rc == CF_OK) rc = dalloc(&c->blk_sum, kMaxPartials);
     if (rc == CF_OK) rc = dalloc(&c->blk_min, kMaxPartials);
     if (rc == CF_OK) rc = dalloc(&c->blk_energy, kMaxPartials);
+    if (rc == CF_OK) rc = dalloc(&c->doc, std::max<size_t>(1, T));
     if (rc == CF_OK && inv_mass) {
         rc = dalloc(&c->inv_mass, set->dev.N);
         if (rc == CF_OK && hipMemcpy(c->inv_mass, inv_mass, set->dev.N * 4, hipMemcpyHostToDevice) != hipSuccess)
@@ -727,8 +737,10 @@ int cf_cache_create(cf_tetset* set, const float* inv_mass, cf_cache** out) {
         // CreateCache: R = Id, det F = 1 (TetrahedronElementSet3.cpp:190-209)
         std::vector<float> planes(9 * (size_t)T, 0.f), ones(T, 1.f);
         for (size_t e = 0; e < T; e++) { planes[e] = 1.f; planes[4 * (size_t)T + e] = 1.f; planes[8 * (size_t)T + e] = 1.f; }
+        // and every DoC invalid (TetrahedronElement3.h:30)
         if (hipMemcpy(c->R, planes.data(), planes.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
-            hipMemcpy(c->detF, ones.data(), ones.size() * 4, hipMemcpyHostToDevice) != hipSuccess)
+            hipMemcpy(c->detF, ones.data(), ones.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemset(c->doc, kDocInvalid, std::max<size_t>(1, T)) != hipSuccess)
             rc = CF_ERR_HIP;
     }
     if (rc != CF_OK) { cf_cache_destroy(c); return rc; }
@@ -739,7 +751,7 @@ int cf_cache_create(cf_tetset* set, const float* inv_mass, cf_cache** out) {
 int cf_cache_destroy(cf_cache* c) {
     if (!c) return CF_OK;
     dfree(c->R); dfree(c->detF); dfree(c->inv_mass); dfree(c->blk_count); dfree(c->blk_sum);
-    dfree(c->blk_min); dfree(c->blk_energy); dfree(c->F); dfree(c->w);
+    dfree(c->blk_min); dfree(c->blk_energy); dfree(c->F); dfree(c->w); dfree(c->doc);
     delete c;
     return CF_OK;
 }
@@ -761,9 +773,46 @@ int cf_begin_evaluation(cf_cache* c, const float* prev_x, const float* x, const 
         lo.inv_k = 1.0f / s->params.ecie_k_factor;  // mal::Rcp (:1898)
         lo.hybrid = s->params.ecie_k_factor > 0.1f;   // LCMH blend branch (:1893-1897)
     }
+    // UpdateDegeneration runs under every rotation method (:261); its DoC
+    // only enters the DAPD rotations
+    DegenArgs dg;
+    dg.prev_x = prev_x ? prev_x : x;
+    dg.doc = c->doc;
+    dg.vol = s->dev.vol;
+    dg.l_factor = s->params.dapd_l_factor;
+    dg.nl_factor = s->params.dapd_nl_factor;
+    dg.nl_exponent = s->params.dapd_nl_exponent;
     CF_TRY(launch_begin_eval(s->dev, x, s->params.rm, s->params.degenerate_threshold_det_f, c->R, c->detF,
-                             c->blk_count, c->blk_sum, c->blk_min, c->nblk, s->stream, lo));
+                             c->blk_count, c->blk_sum, c->blk_min, c->nblk, s->stream, lo, dg));
     c->stats_valid = false;
+    return CF_OK;
+}
+
+// DoC bytes in the caller's element order (the device keeps them in tile order;
+// vertex order within an element is the caller's).
+int cf_cache_get_doc(cf_cache* c, uint8_t* h_doc) {
+    if (!c || !h_doc) return CF_ERR_INVALID_ARG;
+    const cf_tetset* s = c->set;
+    std::vector<uint8_t> dev(s->dev.T);
+    CF_CHECK_HIP(hipStreamSynchronize(s->stream));
+    CF_CHECK_HIP(hipMemcpy(dev.data(), c->doc, s->dev.T, hipMemcpyDeviceToHost));
+    for (uint32_t i = 0; i < s->dev.T; i++) h_doc[s->perm[i]] = dev[i];
+    return CF_OK;
+}
+
+int cf_cache_set_doc(cf_cache* c, const uint8_t* h_doc) {
+    if (!c || !h_doc) return CF_ERR_INVALID_ARG;
+    const cf_tetset* s = c->set;
+    for (uint32_t e = 0; e < s->dev.T; e++)
+        if ((h_doc[e] >> 2) > 3 || (h_doc[e] != kDocInvalid && (h_doc[e] >> 2) > (h_doc[e] & 3))) {
+            set_error("cf_cache_set_doc: element %u has DoC byte %u (4*vit0 + vit1 with vit0 <= vit1, or 12)", e,
+                      unsigned(h_doc[e]));
+            return CF_ERR_INVALID_ARG;
+        }
+    std::vector<uint8_t> dev(s->dev.T);
+    for (uint32_t i = 0; i < s->dev.T; i++) dev[i] = h_doc[s->perm[i]];
+    CF_CHECK_HIP(hipStreamSynchronize(s->stream));
+    CF_CHECK_HIP(hipMemcpy(c->doc, dev.data(), s->dev.T, hipMemcpyHostToDevice));
     return CF_OK;
 }
 
@@ -835,6 +884,13 @@ int cf_cache_stats(const cf_cache* c, uint32_t* nd, float* sd, float* md) {
     if (nd) *nd = c->num_degenerate;
     if (sd) *sd = c->sum_degenerate;
     if (md) *md = c->min_det;
+    return CF_OK;
+}
+
+int cf_cache_num_fallback(const cf_cache* c, uint32_t* n) {
+    if (!c || !n) return CF_ERR_INVALID_ARG;
+    CF_TRY(cache_stats_fetch(const_cast<cf_cache*>(c)));
+    *n = c->num_fallback;
     return CF_OK;
 }
 

@@ -111,7 +111,9 @@ __device__ inline v3 largest_column_normalized(const sym3& s) {  // :149-156
     return v3{1.f, 0.f, 0.f};
 }
 
-__device__ inline void svd_rotation(const float* F, float* R) {
+// Right singular vectors V0, V1, V2 of F (descending singular values): the
+// eigenvectors of the normal equations F^T F.
+__device__ inline void svd_right_vectors(const float* F, v3& V0, v3& V1, v3& V2) {
 #pragma clang fp contract(off)
     // normal equations F^T F (:129-137)
     sym3 A;
@@ -194,9 +196,14 @@ __device__ inline void svd_rotation(const float* F, float* R) {
         acc = 0.0f; acc += o0.z * w0; acc += o1.z * w1; v3v.z = acc;
     }
     v3 v2 = cross(v3v, v1);
-    v3 V0, V1, V2;  // columns of V
     if (flipped) { V0 = v3v; V1 = v2; V2 = v3{-v1.x, -v1.y, -v1.z}; }
     else { V0 = v1; V1 = v2; V2 = v3v; }
+}
+
+__device__ inline void svd_rotation(const float* F, float* R) {
+#pragma clang fp contract(off)
+    v3 V0, V1, V2;  // columns of V
+    svd_right_vectors(F, V0, V1, V2);
     // singular vectors (:259-279); the singular values only enter through
     // the sign of det F, which does not change U or V here.
     auto Fmul = [&](v3 u) {
@@ -224,6 +231,108 @@ __device__ inline void svd_rotation(const float* F, float* R) {
     const float U[9] = {U0.x, U1.x, U2.x, U0.y, U1.y, U2.y, U0.z, U1.z, U2.z};
     const float Vt[9] = {V0.x, V0.y, V0.z, V1.x, V1.y, V1.z, V2.x, V2.y, V2.z};
     mul3(U, Vt, R);
+}
+
+// ---- degenerate elements (DAPD) --------------------------------------------
+
+// gsl_poly_solve_cubic (GNU Scientific Library, poly/solve_cubic.c, the same
+// in GSL 1.x and 2.x; the reference links the system GSL): real roots of
+// x^3 + a x^2 + b x + c in double, Cardano for one, Viete's trigonometric form
+// for three (ascending), the exact double and triple roots by the
+// discriminant tests. Returns the number of roots.
+__device__ inline int gsl_cubic(double a, double b, double c, double* x) {
+#pragma clang fp contract(off)
+    const double q = a * a - 3 * b;
+    const double r = 2 * a * a * a - 9 * a * b + 27 * c;
+    const double Q = q / 9, R = r / 54;
+    const double Q3 = Q * Q * Q, R2 = R * R;
+    const double CR2 = 729 * r * r, CQ3 = 2916 * q * q * q;
+    if (R == 0 && Q == 0) {
+        x[0] = x[1] = x[2] = -a / 3;
+        return 3;
+    }
+    if (CR2 == CQ3) {
+        const double sq = sqrt(Q);
+        if (R > 0) { x[0] = -2 * sq - a / 3; x[1] = sq - a / 3; x[2] = sq - a / 3; }
+        else { x[0] = -sq - a / 3; x[1] = -sq - a / 3; x[2] = 2 * sq - a / 3; }
+        return 3;
+    }
+    if (R2 < Q3) {
+        const double sgn = (R >= 0) ? 1.0 : -1.0;
+        const double theta = acos(sgn * sqrt(R2 / Q3));
+        const double norm = -2 * sqrt(Q);
+        double r0 = norm * cos(theta / 3) - a / 3;
+        double r1 = norm * cos((theta + 2.0 * M_PI) / 3) - a / 3;
+        double r2 = norm * cos((theta - 2.0 * M_PI) / 3) - a / 3;
+        double t;
+        if (r0 > r1) { t = r0; r0 = r1; r1 = t; }
+        if (r1 > r2) { t = r1; r1 = r2; r2 = t; if (r0 > r1) { t = r0; r0 = r1; r1 = t; } }
+        x[0] = r0; x[1] = r1; x[2] = r2;
+        return 3;
+    }
+    const double sgn = (R >= 0) ? 1.0 : -1.0;
+    const double A = -sgn * pow(fabs(R) + sqrt(R2 - Q3), 1.0 / 3.0);
+    x[0] = A + Q / A - a / 3;
+    return 1;
+}
+
+// mal::GSolvePolynomialEq2 (Mal/GSolvePolynomialEq.h:11-38) in float;
+// IsZero takes an absolute epsilon of 1e-4 (Mal/RealUtils.h:110-114).
+__device__ inline int solve_eq2(float a, float b, float c, float& x1, float& x2) {
+#pragma clang fp contract(off)
+    if (fabsf(a) <= 0.0001f) {
+        if (fabsf(b) <= 0.0001f) return 0;
+        x1 = x2 = -c / b;
+        return 1;
+    }
+    const float disc = b * b - 4 * a * c;
+    if (disc < 0) return 0;
+    const float root = sqrtf(disc);
+    const float inv = 1.0f / (2.0f * a);
+    x1 = (-b - root) * inv;
+    x2 = (-b + root) * inv;
+    if (x1 > x2) { const float t = x1; x1 = x2; x2 = t; }
+    return (x1 == x2) ? 1 : 2;
+}
+
+// mal::GSolvePolynomialEq3 (ms/fem/TetrahedronElementSet3.cpp:50-69):
+// coefficients normalised in float, the cubic in double, roots to float.
+__device__ inline int solve_eq3(float a, float b, float c, float d, float* x) {
+#pragma clang fp contract(off)
+    if (a != 0.0f) {
+        double r[3] = {0.0, 0.0, 0.0};
+        const int n = gsl_cubic(double(b / a), double(c / a), double(d / a), r);
+        x[0] = float(r[0]); x[1] = float(r[1]); x[2] = float(r[2]);
+        return n;
+    }
+    x[2] = 0.0f;
+    return solve_eq2(b, c, d, x[0], x[1]);
+}
+
+// One permutation term u_x v_y w_z of a 3x3 determinant whose columns p, q, r
+// are taken from the edge-difference sets u, v, w.
+__device__ inline float perm_term(const float (*u)[3], const float (*v)[3], const float (*w)[3], int p, int q,
+                                  int r) {
+#pragma clang fp contract(off)
+    return u[p][0] * v[q][1] * w[r][2];
+}
+// det Ds(t) on the linear path Ds(t) = Da + t Dd as A t^3 + B t^2 + C t + D:
+// each coefficient sums the six permutation terms + + + - - - (column
+// triples 012, 120, 201, 210, 102, 021), a term taking its factors from Da
+// (a) or Dd (d); in B (C) the single Da (Dd) factor moves from x to y to z.
+__device__ inline float det_coef(const float (*a)[3], const float (*d)[3], int k, int p, int q, int r) {
+#pragma clang fp contract(off)
+    switch (k) {
+        case 3: return perm_term(d, d, d, p, q, r);
+        case 2: return perm_term(a, d, d, p, q, r) + perm_term(d, a, d, p, q, r) + perm_term(d, d, a, p, q, r);
+        case 1: return perm_term(a, a, d, p, q, r) + perm_term(a, d, a, p, q, r) + perm_term(d, a, a, p, q, r);
+        default: return perm_term(a, a, a, p, q, r);
+    }
+}
+__device__ inline float det_poly(const float (*a)[3], const float (*d)[3], int k) {
+#pragma clang fp contract(off)
+    return det_coef(a, d, k, 0, 1, 2) + det_coef(a, d, k, 1, 2, 0) + det_coef(a, d, k, 2, 0, 1) -
+           det_coef(a, d, k, 2, 1, 0) - det_coef(a, d, k, 1, 0, 2) - det_coef(a, d, k, 0, 2, 1);
 }
 
 }  // namespace refmath
@@ -837,11 +946,156 @@ __device__ inline void element_force_lcm(const Slots& s_v, const Slots& s_r, con
 // :748-892): F = Ds Dm^-1, det F, R by the rotation method; per-block counts
 // of det F <= 0 (and their sum), of DAPD R = Id fallbacks, and min det F.
 // blk_count holds [0,nblk) degenerate counts and [nblk,2nblk) fallbacks.
+// ComputeDoC (ms/fem/TetrahedronElementSet3.cpp:589-743) for the path a -> b
+// of one element: the time of collapse toc in (0,1] where det Ds(t) reaches
+// thr * 6 V, the reference's collapse direction N at toc (row 2 of V^T,
+// Ds = U S V^T), then vertex-face when a face's N-projected area is half the
+// sum of all four (absolute epsilon 1e-4, the last such face wins), else the
+// first edge pair (0j, kl) whose triangle normals around edge 0j disagree.
+// Returns the DoC byte (4*vit0 + vit1, kDocInvalid without a crossing).
+__device__ inline uint8_t compute_doc(const float (*a)[3], const float (*b)[3], float volume, float thr) {
+#pragma clang fp contract(off)
+    float d[4][3], dd[3][3], da[3][3];
+    for (int i = 0; i < 4; i++)
+        for (int c = 0; c < 3; c++) d[i][c] = b[i][c] - a[i][c];
+    for (int i = 0; i < 3; i++)
+        for (int c = 0; c < 3; c++) { dd[i][c] = d[i + 1][c] - d[0][c]; da[i][c] = a[i + 1][c] - a[0][c]; }
+    const float K = thr * 6.0f * volume;
+    float roots[3] = {0.0f, 0.0f, 0.0f};
+    const int n = refmath::solve_eq3(refmath::det_poly(da, dd, 3), refmath::det_poly(da, dd, 2),
+                                     refmath::det_poly(da, dd, 1), refmath::det_poly(da, dd, 0) - K, roots);
+    if (n <= 0) return kDocInvalid;
+    float toc = 2.0f;
+    int n01 = 0;
+    for (int i = 0; i < n; i++)
+        if (roots[i] > 0.0f && roots[i] <= 1.0f && roots[i] < toc) { toc = roots[i]; n01++; }
+    if (n01 != 1 && n01 != 3) return kDocInvalid;
+    refmath::v3 q[4];
+    for (int i = 0; i < 4; i++)
+        q[i] = refmath::v3{a[i][0] + toc * d[i][0], a[i][1] + toc * d[i][1], a[i][2] + toc * d[i][2]};
+    auto sub = [](refmath::v3 u, refmath::v3 v) { return refmath::v3{u.x - v.x, u.y - v.y, u.z - v.z}; };
+    const refmath::v3 q10 = sub(q[1], q[0]), q20 = sub(q[2], q[0]), q30 = sub(q[3], q[0]);
+    const float Q[9] = {q10.x, q20.x, q30.x, q10.y, q20.y, q30.y, q10.z, q20.z, q30.z};
+    refmath::v3 V0, V1, N;
+    refmath::svd_right_vectors(Q, V0, V1, N);
+    float area[4];
+    area[0] = fabsf(refmath::dot(N, refmath::cross(sub(q[2], q[1]), sub(q[3], q[1]))));
+    area[1] = fabsf(refmath::dot(N, refmath::cross(q20, q30)));
+    area[2] = fabsf(refmath::dot(N, refmath::cross(q10, q30)));
+    area[3] = fabsf(refmath::dot(N, refmath::cross(q10, q20)));
+    const float sum = area[0] + area[1] + area[2] + area[3];
+    uint8_t doc = kDocInvalid;
+    for (int i = 0; i < 4; i++)
+        if (fabsf(area[i] - 0.5f * sum) <= 0.0001f) doc = uint8_t(5 * i);
+    if (doc != kDocInvalid) return doc;
+    if (refmath::dot(refmath::cross(q10, q20), refmath::cross(q10, q30)) < 0.0f) return 1;
+    if (refmath::dot(refmath::cross(q20, q10), refmath::cross(q20, q30)) < 0.0f) return 2;
+    if (refmath::dot(refmath::cross(q30, q10), refmath::cross(q30, q20)) < 0.0f) return 3;
+    return 0;
+}
+
+// Compute_R_DAPD_Degenerate (ms/fem/Rotation.cpp:364-545): the collapsing
+// vertex (vertex-face DoC) or edge (edge-edge DoC) moves along the unit
+// collapse axis by lambda_u, a cubic Hermite in w / dh for 0 < w and linear
+// (plus the optional power term) for w <= 0, dh = thr 6 V / |axis|; R is the
+// Newton polar factor of the projected F. Returns false (R untouched) when the
+// axis is shorter than 1e-3.
+__device__ inline bool r_dapd_degenerate(float (*x)[3], float detF, float thr, uint8_t doc, float volume,
+                                         const float* inv_dm, float fL, float fNL, float eNL, float* R) {
+#pragma clang fp contract(off)
+    using refmath::v3;
+    auto P = [&](int i) { return v3{x[i][0], x[i][1], x[i][2]}; };
+    auto sub = [](v3 u, v3 v) { return v3{u.x - v.x, u.y - v.y, u.z - v.z}; };
+    const int v0 = doc >> 2, v1 = doc & 3;
+    int i1, i2;  // the axis' reference vertex (vertex-face: x1; edge-edge: x2)
+    v3 axis;
+    if (v0 == v1) {
+        const int a1 = (v0 + 1) & 3, a2 = (v0 + 2) & 3, a3 = (v0 + 3) & 3;
+        axis = refmath::cross(sub(P(a2), P(a1)), sub(P(a3), P(a1)));
+        i1 = a1; i2 = a1;
+    } else {
+        const int k = (v1 == 1) ? 2 : 1, l = (v1 == 3) ? 2 : 3;
+        axis = refmath::cross(sub(P(v1), P(v0)), sub(P(l), P(k)));
+        i1 = v1; i2 = k;
+    }
+    const float len_sq = refmath::dot(axis, axis);
+    if (!(len_sq > 0.000001f)) return false;
+    const float len = sqrtf(len_sq);
+    const float inv = 1.0f / len;
+    axis = v3{inv * axis.x, inv * axis.y, inv * axis.z};
+    const float dh = thr * 6.0f * volume / len;
+    float w = refmath::dot(sub(P(v0), P(i2)), axis);
+    if (w * detF < 0.0f) { w = -w; axis = v3{-axis.x, -axis.y, -axis.z}; }
+    const float s = w / dh;
+    const float t = 1.0f - s;
+    const float lambda = (w > 0.0f) ? fL * dh * ((2.0f - t) * t * t) : fL * dh * (t + fNL * powf(-s, eNL));
+    const v3 mv{lambda * axis.x, lambda * axis.y, lambda * axis.z};
+    x[v0][0] = x[v0][0] + mv.x; x[v0][1] = x[v0][1] + mv.y; x[v0][2] = x[v0][2] + mv.z;
+    if (v0 != v1) { x[i1][0] = x[i1][0] + mv.x; x[i1][1] = x[i1][1] + mv.y; x[i1][2] = x[i1][2] + mv.z; }
+    float Ds[9], F[9];
+    for (int i = 0; i < 3; i++) {
+        Ds[3 * i + 0] = x[1][i] - x[0][i];
+        Ds[3 * i + 1] = x[2][i] - x[0][i];
+        Ds[3 * i + 2] = x[3][i] - x[0][i];
+    }
+    refmath::mul3(Ds, inv_dm, F);
+    refmath::polar_newton(F, R);
+    return true;
+}
+
+// The degenerate branch of BeginEvaluation for one element (det F <= thr),
+// kept out of line so that the common path's registers are not sized for it:
+// UpdateDegeneration (TetrahedronElementSet3.cpp:477-532) — a new DoC when the
+// element was undegenerate at prev_x, else the DoC is kept — and, under DAPD,
+// the rotation (ComputeRotations_DAPD :868-892), written to R. Returns 1 when
+// DAPD fell back to R = Id (no DoC, or a collapsed axis).
+__device__ __noinline__ uint32_t degenerate_element(uint32_t e, uint32_t T, const uint32_t* conn, const float* x,
+                                                    const float* prev_x, uint8_t* doc, const float* vol,
+                                                    const float* dinv, int rm, float thr, float detF, float fL,
+                                                    float fNL, float eNL, float* R) {
+#pragma clang fp contract(off)
+    uint32_t g[4];
+    for (int j = 0; j < 4; j++) g[j] = conn[j * (size_t)T + e];
+    float G[9];
+    for (int k = 0; k < 9; k++) G[k] = dinv[k * (size_t)T + e];
+    float xs[4][3], ps[4][3];
+    for (int j = 0; j < 4; j++)
+        for (int c = 0; c < 3; c++) {
+            xs[j][c] = x[3 * (size_t)g[j] + c];
+            ps[j][c] = prev_x[3 * (size_t)g[j] + c];
+        }
+    const float V = vol[e];
+    uint8_t dc = doc[e];
+    {
+        float Ds[9], F[9];
+        for (int i = 0; i < 3; i++) {
+            Ds[3 * i + 0] = ps[1][i] - ps[0][i];
+            Ds[3 * i + 1] = ps[2][i] - ps[0][i];
+            Ds[3 * i + 2] = ps[3][i] - ps[0][i];
+        }
+        refmath::mul3(Ds, G, F);
+        if (refmath::det3(F) > thr) {
+            const uint8_t nd = compute_doc(ps, xs, V, thr);
+            if (nd != dc) { doc[e] = nd; dc = nd; }
+        }
+    }
+    if (rm != CF_RM_DAPD) return 0;
+    float Rm[9] = {1.f, 0.f, 0.f, 0.f, 1.f, 0.f, 0.f, 0.f, 1.f};
+    bool ok = false;
+    if ((dc >> 2) <= (dc & 3)) ok = r_dapd_degenerate(xs, detF, thr, dc, V, G, fL, fNL, eNL, Rm);
+    if (!ok) {
+        for (int k = 0; k < 9; k++) Rm[k] = (k % 4 == 0) ? 1.f : 0.f;
+    }
+    for (int k = 0; k < 9; k++) R[k * (size_t)T + e] = Rm[k];
+    return ok ? 0u : 1u;
+}
+
 __device__ inline void begin_eval_block(uint32_t T, const uint32_t* __restrict__ conn, const float* __restrict__ x,
                                         const float* __restrict__ dinv, int rm, float thr, float* __restrict__ R,
                                         float* __restrict__ detF, uint32_t* blk_count, double* blk_sum,
                                         float* blk_min, uint32_t blk, uint32_t nblk, double* sh, float* shm,
-                                        uint32_t* shc, uint32_t* shf, const LcmOut lcm = LcmOut()) {
+                                        uint32_t* shc, uint32_t* shf, const LcmOut lcm = LcmOut(),
+                                        const DegenArgs dg = DegenArgs()) {
 #pragma clang fp contract(off)
     uint32_t cnt = 0, fb = 0;
     double sdet = 0.0;
@@ -867,7 +1121,26 @@ __device__ inline void begin_eval_block(uint32_t T, const uint32_t* __restrict__
         } else if (rm == CF_RM_DAPD) {
             if (d > thr) refmath::polar_newton(F, Rm);
         }
-        for (int k = 0; k < 9; k++) R[k * (size_t)T + e] = Rm[k];
+        // UpdateDegeneration (:477-532): undegenerate clears the DoC; the
+        // degenerate branch (rare) updates it and, under DAPD, writes R itself
+        bool r_stored = false;
+        if (dg.doc) {
+            if (d > thr) {
+                if (dg.doc[e] != kDocInvalid) dg.doc[e] = kDocInvalid;
+            } else {
+                const uint32_t f = degenerate_element(e, T, conn, x, dg.prev_x ? dg.prev_x : x, dg.doc, dg.vol, dinv,
+                                                      rm, thr, d, dg.l_factor, dg.nl_factor, dg.nl_exponent, R);
+                if (rm == CF_RM_DAPD) {
+                    fb += f;
+                    r_stored = true;
+                    for (int k = 0; k < 9; k++) Rm[k] = R[k * (size_t)T + e];
+                }
+            }
+        } else if (rm == CF_RM_DAPD && !(d > thr)) {
+            fb++;  // no DoC tracking: R = Id
+        }
+        if (!r_stored)
+            for (int k = 0; k < 9; k++) R[k * (size_t)T + e] = Rm[k];
         detF[e] = d;
         if (lcm.F) {
             for (int k = 0; k < 9; k++) lcm.F[k * (size_t)T + e] = F[k];
@@ -889,8 +1162,6 @@ __device__ inline void begin_eval_block(uint32_t T, const uint32_t* __restrict__
             lcm.w[e] = wt;
         }
         if (d <= 0.0f) { cnt++; sdet += d; }
-        // DAPD elements at or below the threshold take R = Id: counted too.
-        if (rm == CF_RM_DAPD && !(d > thr)) fb++;
         mdet = fminf(mdet, d);
     }
     double s = block_sum(sdet, sh);
@@ -1246,13 +1517,18 @@ __device__ inline void nr_finish_block(uint32_t n, float* __restrict__ x, float*
     }
 }
 
+// x0 (may be null) receives x before the update: the leaf's m_vecPos0 = x,
+// taken after BeginEvaluation (LeafDSH_Solid3D_FEM.cpp:936-937) and not read
+// again before the next one.
 __device__ inline void integrate_block(uint32_t n, float* __restrict__ x, float* __restrict__ v,
                                        const float* __restrict__ y, float dt, float drag, uint32_t blk,
-                                       uint32_t nblk) {
+                                       uint32_t nblk, float* __restrict__ x0 = nullptr) {
 #pragma clang fp contract(off)
     for (uint32_t i = blk * kBlock + threadIdx.x; i < n; i += nblk * kBlock) {
         const float yi = y[i];
-        x[i] = x[i] + dt * yi;
+        const float xi = x[i];
+        if (x0) x0[i] = xi;
+        x[i] = xi + dt * yi;
         v[i] = (drag > 0.f) ? (1.0f - dt * drag) * yi : yi;
     }
 }

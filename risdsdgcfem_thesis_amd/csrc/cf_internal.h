@@ -138,10 +138,22 @@ struct LcmOut {
     float inv_k = 1.f;
     bool hybrid = false;
 };
+// Degenerate-element state of BeginEvaluation: the per-element direction of
+// collapse (one byte, 4*vit0 + vit1; kDocInvalid = (3,0)), kept across
+// evaluations and updated from prev_x -> x, and the DAPD projection factors
+// (m_DAPD_L_Factor, m_DAPD_NL_Factor, m_DAPD_NL_Exponent). doc == nullptr: no
+// tracking, DAPD elements at or below the threshold take R = Id.
+constexpr uint8_t kDocInvalid = 12;
+struct DegenArgs {
+    const float* prev_x = nullptr;  // nullptr: x
+    uint8_t* doc = nullptr;
+    const float* vol = nullptr;
+    float l_factor = 1.f, nl_factor = 0.f, nl_exponent = 2.f;
+};
 // blk_count holds 2*nblk counters: [0,nblk) det F <= 0, [nblk,2nblk) DAPD R=Id fallbacks
 int launch_begin_eval(const TetDevice& t, const float* x, int rm, float thr, float* R, float* detF,
                       uint32_t* blk_count, double* blk_sum, float* blk_min, int nblk, hipStream_t s,
-                      const LcmOut& lcm = LcmOut());
+                      const LcmOut& lcm = LcmOut(), const DegenArgs& dg = DegenArgs());
 int launch_forces(const TetDevice& t, int mode, const float* in, const float* R, Material m, hipStream_t s);
 // C_f_d element stage (Rayleigh damping of the velocities v) into the tile partials
 struct DampArgs {
@@ -222,6 +234,11 @@ struct SimDev {
     float g[3];
     int32_t rm, warm_start, compute_energy;
     uint32_t max_iter;
+    // DAPD degenerate elements: previous positions (m_vecPos0, written by the
+    // integration) and per-element DoC; null unless rm is DAPD
+    float* x0;
+    uint8_t* doc;
+    float dapd_l, dapd_nl, dapd_e;
 };
 
 // CG scalars of one simulation for the current solve, its step results, and

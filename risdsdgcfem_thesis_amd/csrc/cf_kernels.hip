@@ -45,16 +45,17 @@ __global__ __launch_bounds__(kBlock) void k_precompute(uint32_t T, const uint32_
 }
 
 // ============================================================ BeginEvaluation
-__global__ __launch_bounds__(kBlock) void k_begin_eval(uint32_t T, const uint32_t* __restrict__ conn,
+__global__ __launch_bounds__(kBlock, 8) void k_begin_eval(uint32_t T, const uint32_t* __restrict__ conn,
                                                        const float* __restrict__ x, const float* __restrict__ dinv,
                                                        int rm, float thr, float* __restrict__ R,
                                                        float* __restrict__ detF, uint32_t* blk_count,
-                                                       double* blk_sum, float* blk_min, LcmOut lcm) {
+                                                       double* blk_sum, float* blk_min, LcmOut lcm,
+                                                       DegenArgs dg) {
     __shared__ double sh[kBlock / 64];
     __shared__ float shm[kBlock / 64];
     __shared__ uint32_t shc[kBlock / 64], shf[kBlock / 64];
     begin_eval_block(T, conn, x, dinv, rm, thr, R, detF, blk_count, blk_sum, blk_min, blockIdx.x, gridDim.x, sh,
-                     shm, shc, shf, lcm);
+                     shm, shc, shf, lcm, dg);
 }
 
 // Tile kernel: one element and at most one tile node per thread. A workgroup
@@ -388,9 +389,9 @@ int launch_precompute(const TetDevice& t, hipStream_t s) {
 
 int launch_begin_eval(const TetDevice& t, const float* x, int rm, float thr, float* R, float* detF,
                       uint32_t* blk_count, double* blk_sum, float* blk_min, int nblk, hipStream_t s,
-                      const LcmOut& lcm) {
+                      const LcmOut& lcm, const DegenArgs& dg) {
     hipLaunchKernelGGL(k_begin_eval, dim3(nblk), dim3(kBlock), 0, s, t.T, t.conn, x, t.dinv, rm, thr, R, detF,
-                       blk_count, blk_sum, blk_min, lcm);
+                       blk_count, blk_sum, blk_min, lcm, dg);
     return check_launch();
 }
 

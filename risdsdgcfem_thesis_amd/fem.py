@@ -135,7 +135,8 @@ def default_solid_params(fem=None, **kw):
             for i in range(3):
                 p.gravity[i] = v[i]
         elif k in ("young_modulus", "poisson_ratio", "damping_ratio", "rm", "degenerate_threshold_det_f", "mm",
-                   "ecie_k_factor", "dm", "rayleigh_freq1", "rayleigh_freq2"):
+                   "ecie_k_factor", "dm", "rayleigh_freq1", "rayleigh_freq2", "dapd_l_factor", "dapd_nl_factor",
+                   "dapd_nl_exponent"):
             setattr(p.fem, k, v)
         else:
             setattr(p, k, v)
@@ -242,6 +243,22 @@ class TetrahedronElementSet3:
         n, s, m = C.c_uint32(), C.c_float(), C.c_float()
         check(lib.cf_cache_stats(self.c, C.byref(n), C.byref(s), C.byref(m)), "cf_cache_stats")
         return n.value, s.value, m.value
+
+    def doc(self):
+        """Per-element direction-of-collapse bytes (4*vit0 + vit1; 12 = invalid)."""
+        out = np.empty(self.T, np.uint8)
+        check(lib.cf_cache_get_doc(self.c, out.ctypes.data_as(C.POINTER(C.c_uint8))), "cf_cache_get_doc")
+        return out
+
+    def set_doc(self, doc):
+        doc = np.ascontiguousarray(doc, np.uint8)
+        check(lib.cf_cache_set_doc(self.c, doc.ctypes.data_as(C.POINTER(C.c_uint8))), "cf_cache_set_doc")
+
+    def num_fallback(self):
+        """DAPD elements of the last evaluation left at R = Id."""
+        n = C.c_uint32()
+        check(lib.cf_cache_num_fallback(self.c, C.byref(n)), "cf_cache_num_fallback")
+        return n.value
 
     def synchronize(self):
         check(lib.cf_cache_synchronize(self.c), "cf_cache_synchronize")
