@@ -388,8 +388,11 @@ int cf_batch_synchronize(cf_batch* b);
  * Rotations: the closed-form 2x2 polar factor for det F > threshold (the
  * reference's MSVD and DAPD both take it there); below it MSVD takes the 2x2
  * SVD of ms/fem/Rotation.cpp:27-36 (Mal/GMatDecomposition.h:255-345) and DAPD
- * R = Id, its invalid-NoC branch (TriangleElementSet2.cpp:1100-1105; the NoC
- * degenerate rotation is not implemented), counted in num_fallback.
+ * the projected rotation of the triangle's node of collapse (NoC; ComputeNoC
+ * TriangleElementSet2.cpp:739-835, Compute_R_DAPD_Degenerate Rotation.cpp:300-362),
+ * kept per triangle and updated from the previous positions as
+ * UpdateDegeneration (:583-640) does; without a NoC R = Id (:1100-1105), counted
+ * in num_fallback. Previous positions start at the rest pose; set_state leaves them.
  * Each mesh is stepped inside one workgroup by a group of 1, 2, 4 or 8
  * wavefronts (up to 2048 triangles and 1536 nodes; CF_ERR_UNSUPPORTED beyond).
  * A batch steps all its simulations in ONE launch; each simulation's result is
@@ -399,8 +402,9 @@ typedef struct cf_batch2 cf_batch2;
    row-major) and det F, as the 2D step computes them: ms/fem/Rotation.cpp:19-37
    (Compute_R_SVD, rm = CF_RM_MSVD) and Rotation.h:20 (Compute_R_PD, rm =
    CF_RM_DAPD) as called by TriangleElementSet2::ComputeRotations_MSVD/DAPD
-   (TriangleElementSet2.cpp:1007-1018,1078-1110). Host arrays; R is 4n floats;
-   fallback (may be NULL) gets 1 where DAPD fell back to R = Id. */
+   (TriangleElementSet2.cpp:1007-1018,1078-1110). Host arrays; R is 4n floats.
+   Stateless: no node of collapse, so DAPD below the threshold gives R = Id and
+   fallback (may be NULL) gets 1 there. */
 int cf_rotations2(const float* h_F, const float* h_detF, uint32_t n, int32_t rm, float threshold, float* h_R,
                   uint32_t* h_fallback);
 int cf_batch2_create(cf_batch2** out);
@@ -416,8 +420,11 @@ int cf_batch2_get_state(cf_batch2* b, uint32_t sim_id, float* h_pos2, float* h_v
 int cf_batch2_set_state(cf_batch2* b, uint32_t sim_id, const float* h_pos2, const float* h_vel2);
 int cf_batch2_node_masses(const cf_batch2* b, uint32_t sim_id, float* h_mass);
 /* Elements of the simulation's last fixed step whose DAPD rotation fell back
-   to R = Id (det F <= threshold; TriangleElementSet2.cpp:1100-1105). */
+   to R = Id (det F <= threshold without a NoC; TriangleElementSet2.cpp:1100-1105). */
 int cf_batch2_fallbacks(cf_batch2* b, uint32_t sim_id, uint32_t* num_fallback);
+/* The simulation's node of collapse per triangle (0, 1, 2; 3 = none), in the
+   caller's triangle order; tracked for DAPD simulations only. */
+int cf_batch2_get_noc(cf_batch2* b, uint32_t sim_id, uint8_t* h_noc);
 int cf_batch2_info(const cf_batch2* b, uint32_t* num_sims, uint64_t* tris, uint64_t* nodes, uint32_t* workgroups);
 
 /* LeafDSH_Solid2D_FEM drop-in (Create / Step / FixedStep / QueryState): one
@@ -431,6 +438,7 @@ int cf_solid2_step(cf_solid2* s, float dt, cf_step_stats* stats);
 int cf_solid2_get_state(const cf_solid2* s, float* h_pos2, float* h_vel2);
 int cf_solid2_set_state(cf_solid2* s, const float* h_pos2, const float* h_vel2);
 int cf_solid2_fallbacks(cf_solid2* s, uint32_t* num_fallback);  /* as cf_batch2_fallbacks */
+int cf_solid2_get_noc(cf_solid2* s, uint8_t* h_noc);               /* as cf_batch2_get_noc */
 
 /* ---------------------------------------------------------------- serving
  * A real-time request loop over a batch (no reference counterpart: the

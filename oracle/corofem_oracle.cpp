@@ -1526,6 +1526,10 @@ struct orc_solid2 {
     float c[2] = {0, 0};
     std::vector<uint32_t> knc;
     std::vector<float> x, v, b, y, r, d, q;
+    std::vector<float> x0;           // 2N previous positions (m_vecPos0)
+    std::vector<uint8_t> noc;        // T nodes of collapse (ElementCache::m_NoC), 3 = invalid
+    float dapd_l = 1.0f, dapd_nl = 0.0f, dapd_e = 2.0f;
+    uint32_t n_fallback = 0;         // last BeginEvaluation's DAPD R = Id count
 };
 
 namespace {
@@ -1603,7 +1607,10 @@ inline void eigvec2(const M2& m, float lam, float& x, float& y) {
 // Release-build behaviour: a NaN eigenvalue pair with |det F - 1| >= 0.05 is
 // not caught and flows into R. Only the second eigenvalue and the second
 // singular value are NaN-tested (Mal/GVec.h:201-208 starts at index 1).
-M2 svd_rotation2_degenerate(const M2& F) {
+// fix_reflection: Compute_R_SVD's removal of reflections from U and V^T
+// (Rotation.cpp:31-33); GRotation2x2_PolarDecomposition_From_SVD returns U V^T
+// as it is (Mal/GMatDecomposition.h:365-379).
+M2 svd_rotation2_degenerate(const M2& F, bool fix_reflection = true) {
     M2 U, Vt;
     const M2 A = m2mul(m2tr(F), F);
     float tr = 0.0f;
@@ -1645,9 +1652,85 @@ M2 svd_rotation2_degenerate(const M2& F) {
             U.a[0][1] = U.a[1][0]; U.a[1][1] = -U.a[0][0];
         }
     }
-    if (det2(U) < 0.0f) { U.a[0][1] = -U.a[0][1]; U.a[1][1] = -U.a[1][1]; }
-    if (det2(Vt) < 0.0f) { Vt.a[1][0] = -Vt.a[1][0]; Vt.a[1][1] = -Vt.a[1][1]; }
+    if (fix_reflection) {
+        if (det2(U) < 0.0f) { U.a[0][1] = -U.a[0][1]; U.a[1][1] = -U.a[1][1]; }
+        if (det2(Vt) < 0.0f) { Vt.a[1][0] = -Vt.a[1][0]; Vt.a[1][1] = -Vt.a[1][1]; }
+    }
     return m2mul(U, Vt);
+}
+
+// ---- 2D degenerate triangles (DAPD)
+constexpr uint8_t kNocInvalid = 3;  // cInvalidNoC (TriangleElementSet2.h:114)
+
+// ComputeNoC (ms/fem/TriangleElementSet2.cpp:739-835): the time of collapse as
+// a root of the quadratic det Ds(t) = thr 2 A (mal::GSolvePolynomialEq2 in
+// float, the first positive root, else the second, even above 1), then the
+// node of collapse: the vertex opposite the longest edge at that time.
+uint8_t compute_noc2(const float a[3][2], const float b[3][2], float area, float thr, float* toc_out = nullptr) {
+    float d[3][2];
+    for (int i = 0; i < 3; i++) { d[i][0] = b[i][0] - a[i][0]; d[i][1] = b[i][1] - a[i][1]; }
+    const float K = thr * 2.0f * area;
+    const float A = d[1][0] * d[2][1] + d[2][0] * d[0][1] + d[0][0] * d[1][1] - d[1][0] * d[0][1] -
+                    d[0][0] * d[2][1] - d[2][0] * d[1][1];
+    const float B = (d[1][0] * a[2][1] + a[1][0] * d[2][1]) + (d[2][0] * a[0][1] + a[2][0] * d[0][1]) +
+                    (d[0][0] * a[1][1] + a[0][0] * d[1][1]) - (d[1][0] * a[0][1] + a[1][0] * d[0][1]) -
+                    (d[0][0] * a[2][1] + a[0][0] * d[2][1]) - (d[2][0] * a[1][1] + a[2][0] * d[1][1]);
+    const float C = a[1][0] * a[2][1] + a[2][0] * a[0][1] + a[0][0] * a[1][1] - a[1][0] * a[0][1] -
+                    a[0][0] * a[2][1] - a[2][0] * a[1][1] - K;
+    float t1 = 0.0f, t2 = 1.0f;
+    if (solve_eq2(A, B, C, t1, t2) <= 0) return kNocInvalid;
+    const float toc = (t1 > 0.0f) ? t1 : t2;
+    if (toc_out) *toc_out = toc;
+    float q[3][2];
+    for (int i = 0; i < 3; i++) { q[i][0] = a[i][0] + toc * d[i][0]; q[i][1] = a[i][1] + toc * d[i][1]; }
+    auto nsq = [&](int i, int j) {
+        const float ex = q[i][0] - q[j][0], ey = q[i][1] - q[j][1];
+        float acc = 0.0f;
+        acc += ex * ex;
+        acc += ey * ey;
+        return acc;
+    };
+    const float l01 = nsq(1, 0), l12 = nsq(2, 1), l20 = nsq(0, 2);
+    return (l01 >= l12) ? ((l01 >= l20) ? 2 : 1) : ((l12 >= l20) ? 0 : 1);
+}
+
+// Compute_R_DAPD_Degenerate, 2D (ms/fem/Rotation.cpp:300-362, with
+// __ENABLE_REGULARIZE_DAPD, :6-8): the collapsing node moves along the unit
+// perpendicular of its opposite edge (squared length regularised by 1e-6) by
+// lambda_u; R is the closed-form polar factor of the projected F, or
+// GRotation2x2_PolarDecomposition_From_SVD when its det is <= 0.001.
+M2 r_dapd_degenerate2(float thr, uint8_t noc, const float p[3][2], float area, const M2& invDm, float fL, float fNL,
+                      float eNL) {
+    float x[3][2];
+    std::memcpy(x, p, sizeof(x));
+    const int i0 = noc, i1 = (noc + 1) % 3, i2 = (noc + 2) % 3;
+    const float ex = x[i2][0] - x[i1][0], ey = x[i2][1] - x[i1][1];
+    float dsq = 0.0f;
+    dsq += ex * ex;
+    dsq += ey * ey;
+    dsq += 0.001f * 0.001f;
+    const float dist = std::sqrt(dsq);
+    const float inv = 1.0f / dist;
+    const float ax = inv * -ey, ay = inv * ex;  // PerpendicularCW (Mal/GVec.h:223-226) / dist
+    const float dh = thr * 2.0f * area / dist;
+    const float mx = 0.5f * (x[i1][0] + x[i2][0]), my = 0.5f * (x[i1][1] + x[i2][1]);
+    float w = 0.0f;
+    w += (x[i0][0] - mx) * ax;
+    w += (x[i0][1] - my) * ay;
+    const float s = w / dh;
+    const float t = 1.0f - s;
+    const float lambda = (w >= 0.0f) ? fL * dh * ((2.0f - t) * t * t) : fL * dh * (t + fNL * std::pow(-s, eNL));
+    x[i0][0] = x[i0][0] + lambda * ax;
+    x[i0][1] = x[i0][1] + lambda * ay;
+    const M2 projF = m2mul(d2(x[0], x[1], x[2]), invDm);
+    const float dp = det2(projF);
+    float Fa[4], R[4];
+    m2store(projF, Fa);
+    if (dp > 0.001f || std::fabs(dp - 1.0f) <= 0.0001f) {
+        orc_pd2(Fa, dp, R);
+        return m2load(R);
+    }
+    return svd_rotation2_degenerate(projF, false);
 }
 }  // namespace
 
@@ -1707,6 +1790,8 @@ orc_solid2* orc_solid2_create(uint32_t n_nodes, const float* ref2, uint32_t n_tr
     const size_t n2 = 2 * size_t(n_nodes);
     s->x.assign(ref2, ref2 + n2);
     s->v.assign(n2, 0.0f);
+    s->x0 = s->x;                       // m_vecPos0 starts at the rest pose
+    s->noc.assign(n_tris, kNocInvalid);  // CreateCache: m_NoC = cInvalidNoC
     s->b.resize(n2); s->y.resize(n2); s->r.resize(n2); s->d.resize(n2); s->q.resize(n2);
     return s;
 }
@@ -1718,23 +1803,45 @@ void orc_solid2_node_masses(const orc_solid2* s, float* m) {
     for (uint32_t n = 0; n < s->N; n++) m[n] = 1.0f / s->inv_mass[n];
 }
 
-// BeginEvaluation (TriangleElementSet2.cpp:183-262) + ComputeRotations_MSVD/DAPD
-// (:1007-1018,1078-1110): Compute_R_PD for det F > threshold
-// (GMatDecomposition.h:17-30); below it MSVD takes the 2x2 SVD
-// (Rotation.cpp:19-37) and DAPD R = Id (its invalid-NoC branch, :1100-1105)
-void orc_solid2_begin_evaluation(orc_solid2* s, const float* x, float* R_out, float* detF_out) {
+// BeginEvaluation (TriangleElementSet2.cpp:183-262), its UpdateDegeneration
+// (:583-640) and ComputeRotations_MSVD/DAPD (:1007-1018,1078-1110):
+// Compute_R_PD for det F > threshold (GMatDecomposition.h:17-30); below it
+// MSVD takes the 2x2 SVD (Rotation.cpp:19-37) and DAPD the projected rotation
+// of the triangle's node of collapse, or R = Id without one (:1100-1105).
+// prev_x may be NULL (then x).
+void orc_solid2_begin_evaluation(orc_solid2* s, const float* prev_x, const float* x, float* R_out, float* detF_out) {
+    if (!prev_x) prev_x = x;
+    const float thr = s->p.degenerate_threshold;
+    s->n_fallback = 0;
     for (uint32_t e = 0; e < s->T; e++) {
         const uint32_t* t = &s->tri[3 * size_t(e)];
-        M2 F = m2mul(d2(&x[2 * size_t(t[0])], &x[2 * size_t(t[1])], &x[2 * size_t(t[2])]), m2load(&s->Dinv[4 * size_t(e)]));
+        const M2 Dinv = m2load(&s->Dinv[4 * size_t(e)]);
+        M2 F = m2mul(d2(&x[2 * size_t(t[0])], &x[2 * size_t(t[1])], &x[2 * size_t(t[2])]), Dinv);
         float dF = det2(F);
         s->detF[e] = dF;
+        const float xs[3][2] = {{x[2 * size_t(t[0])], x[2 * size_t(t[0]) + 1]},
+                                {x[2 * size_t(t[1])], x[2 * size_t(t[1]) + 1]},
+                                {x[2 * size_t(t[2])], x[2 * size_t(t[2]) + 1]}};
+        if (dF > thr) {
+            s->noc[e] = kNocInvalid;
+        } else {
+            const float ps[3][2] = {{prev_x[2 * size_t(t[0])], prev_x[2 * size_t(t[0]) + 1]},
+                                    {prev_x[2 * size_t(t[1])], prev_x[2 * size_t(t[1]) + 1]},
+                                    {prev_x[2 * size_t(t[2])], prev_x[2 * size_t(t[2]) + 1]}};
+            if (det2(m2mul(d2(ps[0], ps[1], ps[2]), Dinv)) > thr) s->noc[e] = compute_noc2(ps, xs, s->area[e], thr);
+        }
         float R[4] = {1.0f, 0.0f, 0.0f, 1.0f};
-        if (s->p.rm != ORC_RM_ID && dF > s->p.degenerate_threshold) {
+        if (s->p.rm != ORC_RM_ID && dF > thr) {
             float Fa[4];
             m2store(F, Fa);
             orc_pd2(Fa, dF, R);
         } else if (s->p.rm == ORC_RM_MSVD) {
             m2store(svd_rotation2_degenerate(F), R);
+        } else if (s->p.rm == ORC_RM_DAPD) {
+            if (s->noc[e] != kNocInvalid)
+                m2store(r_dapd_degenerate2(thr, s->noc[e], xs, s->area[e], Dinv, s->dapd_l, s->dapd_nl, s->dapd_e), R);
+            else
+                s->n_fallback++;
         }
         std::memcpy(&s->R[4 * size_t(e)], R, sizeof(R));
     }
@@ -1793,7 +1900,8 @@ float orc_solid2_V(const orc_solid2* s, const float* x) {
 void orc_solid2_fixed_step(orc_solid2* s, float dt, orc_step_stats* st) {
     const uint32_t N = s->N;
     const size_t n2 = 2 * size_t(N);
-    orc_solid2_begin_evaluation(s, s->x.data(), nullptr, nullptr);
+    orc_solid2_begin_evaluation(s, s->x0.data(), s->x.data(), nullptr, nullptr);  // FixedStep :974
+    s->x0 = s->x;
     uint32_t ndeg = 0; float sdeg = 0.0f, min_det = 1e30f;
     for (uint32_t e = 0; e < s->T; e++) {
         if (s->detF[e] <= 0) { ndeg++; sdeg += s->detF[e]; }
@@ -1863,6 +1971,22 @@ void orc_solid2_get_state(const orc_solid2* s, float* x, float* v) {
 void orc_solid2_set_state(orc_solid2* s, const float* x, const float* v) {
     if (x) std::memcpy(s->x.data(), x, s->x.size() * sizeof(float));
     if (v) std::memcpy(s->v.data(), v, s->v.size() * sizeof(float));
+}
+
+void orc_solid2_get_noc(const orc_solid2* s, uint8_t* noc) { std::memcpy(noc, s->noc.data(), s->noc.size()); }
+void orc_solid2_set_dapd(orc_solid2* s, float l_factor, float nl_factor, float nl_exponent) {
+    s->dapd_l = l_factor; s->dapd_nl = nl_factor; s->dapd_e = nl_exponent;
+}
+uint32_t orc_solid2_num_fallback(const orc_solid2* s) { return s->n_fallback; }
+uint8_t orc_compute_noc2(const float a[6], const float b[6], float area, float thr, float* toc) {
+    const float av[3][2] = {{a[0], a[1]}, {a[2], a[3]}, {a[4], a[5]}};
+    const float bv[3][2] = {{b[0], b[1]}, {b[2], b[3]}, {b[4], b[5]}};
+    return compute_noc2(av, bv, area, thr, toc);
+}
+void orc_r_dapd_degenerate2(float thr, uint8_t noc, const float p[6], float area, const float inv_dm[4],
+                            float l_factor, float nl_factor, float nl_exponent, float R[4]) {
+    const float pv[3][2] = {{p[0], p[1]}, {p[2], p[3]}, {p[4], p[5]}};
+    m2store(r_dapd_degenerate2(thr, noc, pv, area, m2load(inv_dm), l_factor, nl_factor, nl_exponent), R);
 }
 
 }  // extern "C"

@@ -180,7 +180,15 @@ void orc_solid2_set_state(orc_solid2* s, const float* x, const float* v);
 void orc_solid2_node_masses(const orc_solid2* s, float* m);
 float orc_solid2_total_area(const orc_solid2* s);
 /* element operators on the current rotations (set by the last step or begin_evaluation) */
-void orc_solid2_begin_evaluation(orc_solid2* s, const float* x, float* R_out, float* detF_out);
+void orc_solid2_begin_evaluation(orc_solid2* s, const float* prev_x /* may be NULL */, const float* x,
+                                 float* R_out, float* detF_out);
+/* 2D degenerate triangles (DAPD): node of collapse per triangle (3 = invalid) */
+void orc_solid2_get_noc(const orc_solid2* s, uint8_t* noc);
+void orc_solid2_set_dapd(orc_solid2* s, float l_factor, float nl_factor, float nl_exponent);
+uint32_t orc_solid2_num_fallback(const orc_solid2* s);
+uint8_t orc_compute_noc2(const float a[6], const float b[6], float area, float thr, float* toc);
+void orc_r_dapd_degenerate2(float thr, uint8_t noc, const float p[6], float area, const float inv_dm[4],
+                            float l_factor, float nl_factor, float nl_exponent, float R[4]);
 void orc_solid2_f_e(const orc_solid2* s, const float* x, float* f);
 void orc_solid2_df_x(const orc_solid2* s, const float* dx, float* df);
 float orc_solid2_V(const orc_solid2* s, const float* x);

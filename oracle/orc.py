@@ -116,7 +116,13 @@ def _load():
         "orc_solid2_set_state": (None, [C.c_void_p, _f, _f]),
         "orc_solid2_node_masses": (None, [C.c_void_p, _f]),
         "orc_solid2_total_area": (C.c_float, [C.c_void_p]),
-        "orc_solid2_begin_evaluation": (None, [C.c_void_p, _f, _f, _f]),
+        "orc_solid2_begin_evaluation": (None, [C.c_void_p, _f, _f, _f, _f]),
+        "orc_solid2_get_noc": (None, [C.c_void_p, _b]),
+        "orc_solid2_set_dapd": (None, [C.c_void_p, C.c_float, C.c_float, C.c_float]),
+        "orc_solid2_num_fallback": (C.c_uint32, [C.c_void_p]),
+        "orc_compute_noc2": (C.c_uint8, [_f, _f, C.c_float, C.c_float, _f]),
+        "orc_r_dapd_degenerate2": (None, [C.c_float, C.c_uint8, _f, C.c_float, _f, C.c_float, C.c_float,
+                                          C.c_float, _f]),
         "orc_solid2_f_e": (None, [C.c_void_p, _f, _f]),
         "orc_solid2_df_x": (None, [C.c_void_p, _f, _f]),
         "orc_solid2_V": (C.c_float, [C.c_void_p, _f]),
@@ -316,6 +322,26 @@ def r_dapd_degenerate(det_f, thr, doc, p, volume, inv_dm, l_factor=1.0, nl_facto
     return bool(ok), R.reshape(3, 3)
 
 
+NOC_INVALID = 3  # 2D node of collapse: 0, 1, 2 or invalid
+
+
+def compute_noc2(a, b, area, thr=0.1, with_toc=False):
+    """ComputeNoC for the linear path a -> b (3x2 each)."""
+    a = np.ascontiguousarray(a, np.float32).ravel()
+    b = np.ascontiguousarray(b, np.float32).ravel()
+    toc = np.full(1, np.nan, np.float32)
+    noc = int(lib().orc_compute_noc2(fp(a), fp(b), area, thr, fp(toc)))
+    return (noc, float(toc[0])) if with_toc else noc
+
+
+def r_dapd_degenerate2(thr, noc, p, area, inv_dm, l_factor=1.0, nl_factor=0.0, nl_exponent=2.0):
+    p = np.ascontiguousarray(p, np.float32).ravel()
+    inv_dm = np.ascontiguousarray(inv_dm, np.float32).ravel()
+    R = np.zeros(4, np.float32)
+    lib().orc_r_dapd_degenerate2(thr, noc, fp(p), area, fp(inv_dm), l_factor, nl_factor, nl_exponent, fp(R))
+    return R.reshape(2, 2)
+
+
 def default_params(**kw):
     p = SolidParams()
     p.fixed_dt = 1.0 / 60.0
@@ -451,12 +477,24 @@ class Solid2:
     def total_area(self):
         return lib().orc_solid2_total_area(self.h)
 
-    def begin_evaluation(self, x):
+    def begin_evaluation(self, x, prev_x=None):
         x = np.ascontiguousarray(x, np.float32)
+        px = None if prev_x is None else np.ascontiguousarray(prev_x, np.float32)
         R = np.zeros(4 * self.T, np.float32)
         dF = np.zeros(self.T, np.float32)
-        lib().orc_solid2_begin_evaluation(self.h, fp(x), fp(R), fp(dF))
+        lib().orc_solid2_begin_evaluation(self.h, None if px is None else fp(px), fp(x), fp(R), fp(dF))
         return R, dF
+
+    def noc(self):
+        out = np.zeros(self.T, np.uint8)
+        lib().orc_solid2_get_noc(self.h, out.ctypes.data_as(_b))
+        return out
+
+    def set_dapd(self, l_factor=1.0, nl_factor=0.0, nl_exponent=2.0):
+        lib().orc_solid2_set_dapd(self.h, l_factor, nl_factor, nl_exponent)
+
+    def num_fallback(self):
+        return lib().orc_solid2_num_fallback(self.h)
 
     def f_e(self, x):
         x = np.ascontiguousarray(x, np.float32)

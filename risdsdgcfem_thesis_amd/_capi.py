@@ -190,6 +190,8 @@ SIGNATURES = {
     "cf_batch2_node_masses": (C.c_int, [_vp, C.c_uint32, _f]),
     "cf_batch2_fallbacks": (C.c_int, [_vp, C.c_uint32, _u]),
     "cf_solid2_fallbacks": (C.c_int, [_vp, _u]),
+    "cf_batch2_get_noc": (C.c_int, [_vp, C.c_uint32, C.POINTER(C.c_uint8)]),
+    "cf_solid2_get_noc": (C.c_int, [_vp, C.POINTER(C.c_uint8)]),
     "cf_batch2_info": (C.c_int, [_vp, _u, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), _u]),
     "cf_solid2_create": (C.c_int, [C.c_uint32, _f, C.c_uint32, _u, C.POINTER(SolidParams), C.c_uint32, _u,
                                    C.POINTER(_vp)]),

@@ -27,6 +27,9 @@ struct cf_batch2 {
     std::vector<uint32_t> tri_ids;
     std::vector<float4> dinv;
     std::vector<uint16_t> cstart;
+    std::vector<float2> x0;          // previous positions (DAPD node-of-collapse tracking)
+    std::vector<uint8_t> noc;        // per triangle (colour order), 3 = none
+    std::vector<uint32_t> tri_src;   // colour order -> the caller's triangle index
     std::vector<Work2> work;
     uint64_t tris = 0;
     bool dirty = true;        // static data or state must be uploaded
@@ -39,12 +42,16 @@ struct cf_batch2 {
     uint32_t* d_tri = nullptr;
     float4* d_dinv = nullptr;
     uint16_t* d_cs = nullptr;
+    float2* d_x0 = nullptr;
+    uint8_t* d_noc = nullptr;
     Work2* d_work = nullptr;
     Sim2Out* d_out = nullptr;
     std::vector<Sim2Out> h_out;
 };
 
 namespace {
+
+constexpr uint32_t kNocNone = 3;  // cInvalidNoC (TriangleElementSet2.h:114)
 
 template <class T>
 void dfree(T*& p) {
@@ -55,6 +62,7 @@ void dfree(T*& p) {
 void free_device(cf_batch2* b) {
     dfree(b->d_meshes); dfree(b->d_ref); dfree(b->d_x); dfree(b->d_v); dfree(b->d_mass); dfree(b->d_area);
     dfree(b->d_knc); dfree(b->d_tri); dfree(b->d_dinv); dfree(b->d_cs); dfree(b->d_work); dfree(b->d_out);
+    dfree(b->d_x0); dfree(b->d_noc);
 }
 
 template <class T>
@@ -69,6 +77,8 @@ int pull_state(cf_batch2* b) {
     if (!b->on_device) return CF_OK;
     CF_CHECK_HIP(hipMemcpyAsync(b->x.data(), b->d_x, b->x.size() * sizeof(float2), hipMemcpyDeviceToHost, b->stream));
     CF_CHECK_HIP(hipMemcpyAsync(b->v.data(), b->d_v, b->v.size() * sizeof(float2), hipMemcpyDeviceToHost, b->stream));
+    CF_CHECK_HIP(hipMemcpyAsync(b->x0.data(), b->d_x0, b->x0.size() * sizeof(float2), hipMemcpyDeviceToHost, b->stream));
+    CF_CHECK_HIP(hipMemcpyAsync(b->noc.data(), b->d_noc, b->noc.size(), hipMemcpyDeviceToHost, b->stream));
     CF_CHECK_HIP(hipStreamSynchronize(b->stream));
     b->on_device = false;
     return CF_OK;
@@ -116,6 +126,8 @@ int upload(cf_batch2* b) {
     CF_TRY(put(&b->d_tri, b->tri_ids, s));
     CF_TRY(put(&b->d_dinv, b->dinv, s));
     CF_TRY(put(&b->d_cs, b->cstart, s));
+    CF_TRY(put(&b->d_x0, b->x0, s));
+    CF_TRY(put(&b->d_noc, b->noc, s));
     CF_TRY(put(&b->d_work, b->work, s));
     b->h_out.assign(b->meshes.size(), Sim2Out{});
     CF_TRY(put(&b->d_out, b->h_out, s));
@@ -139,6 +151,8 @@ Batch2Args args(const cf_batch2* b) {
     a.area = b->d_area;
     a.color_start = b->d_cs;
     a.out = b->d_out;
+    a.x0 = b->d_x0;
+    a.noc = b->d_noc;
     return a;
 }
 
@@ -319,9 +333,13 @@ int cf_batch2_add_sim(cf_batch2* b, uint32_t N, const float* ref, uint32_t T, co
     m.warm_start = p->warm_start;
     m.compute_energy = p->compute_energy;
     m.max_iter = std::min<uint32_t>(p->cg_max_iter, kMaxCgIter);
+    m.dapd_l = p->fem.dapd_l_factor;
+    m.dapd_nl = p->fem.dapd_nl_factor;
+    m.dapd_e = p->fem.dapd_nl_exponent;
     for (uint32_t n = 0; n < N; n++) {
         b->ref.push_back(make_float2(ref[2 * (size_t)n], ref[2 * (size_t)n + 1]));
         b->x.push_back(b->ref.back());
+        b->x0.push_back(b->ref.back());  // m_vecPos0 starts at the rest pose
         b->v.push_back(make_float2(0.f, 0.f));
         const float inv_mass = 1.0f / (wa[n] * density_per_area);
         b->mass.push_back(1.0f / inv_mass);
@@ -337,6 +355,8 @@ int cf_batch2_add_sim(cf_batch2* b, uint32_t N, const float* ref, uint32_t T, co
         for (int j = 0; j < 3; j++) b->tri_ids.push_back(tris[3 * e + j]);
         b->dinv.push_back(dinv[e]);
         b->area.push_back(area[e]);
+        b->noc.push_back(uint8_t(kNocNone));
+        b->tri_src.push_back(e);
     }
     b->meshes.push_back(m);
     b->tris += T;
@@ -400,6 +420,23 @@ int cf_batch2_set_state(cf_batch2* b, uint32_t id, const float* pos, const float
         if (pos) std::memcpy(&b->x[m.node_off], pos, m.N * sizeof(float2));
         if (vel) std::memcpy(&b->v[m.node_off], vel, m.N * sizeof(float2));
     }
+    return CF_OK;
+}
+
+int cf_batch2_get_noc(cf_batch2* b, uint32_t id, uint8_t* h_noc) {
+    if (!b || !h_noc || id >= b->meshes.size()) {
+        set_error("no 2D simulation %u", id);
+        return CF_ERR_INVALID_ARG;
+    }
+    const Mesh2Dev& m = b->meshes[id];
+    std::vector<uint8_t> sorted(m.T);
+    if (b->on_device) {
+        CF_CHECK_HIP(hipStreamSynchronize(b->stream));
+        CF_CHECK_HIP(hipMemcpy(sorted.data(), b->d_noc + m.tri_off, m.T, hipMemcpyDeviceToHost));
+    } else {
+        std::memcpy(sorted.data(), &b->noc[m.tri_off], m.T);
+    }
+    for (uint32_t i = 0; i < m.T; i++) h_noc[b->tri_src[m.tri_off + i]] = sorted[i];
     return CF_OK;
 }
 
@@ -486,6 +523,11 @@ int cf_solid2_step(cf_solid2* s, float dt, cf_step_stats* stats) {
     last.num_fixed_steps = n;
     if (stats) *stats = last;
     return CF_OK;
+}
+
+int cf_solid2_get_noc(cf_solid2* s, uint8_t* h_noc) {
+    if (!s) return CF_ERR_INVALID_ARG;
+    return cf_batch2_get_noc(s->b, 0, h_noc);
 }
 
 int cf_solid2_fallbacks(cf_solid2* s, uint32_t* num_fallback) {

@@ -321,6 +321,7 @@ struct Mesh2Dev {
     float fM, fK, dt, inv_dt, g0, g1, rel, abs_eps, thr, drag;
     int32_t rm, warm_start, compute_energy;
     uint32_t max_iter;
+    float dapd_l, dapd_nl, dapd_e;  // DAPD degenerate projection factors
 };
 
 struct Sim2Out {
@@ -349,6 +350,10 @@ struct Batch2Args {
     const float* area;
     const uint16_t* color_start;  // per mesh C+1 entries (color_off)
     Sim2Out* out;
+    // DAPD degenerate triangles: previous positions (m_vecPos0, node_off) and
+    // node of collapse per triangle (tri_off, 3 = none); used by DAPD meshes
+    float2* x0;
+    uint8_t* noc;
 };
 
 int launch_batch2_step(const Batch2Args& a, uint32_t n_work, hipStream_t s);

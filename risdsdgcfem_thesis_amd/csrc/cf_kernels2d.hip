@@ -148,7 +148,9 @@ __device__ inline void eigvec2(const Mat2& m, float lam, float& x, float& y) {
 
 // Not inlined: the branch is rare, and inlining it into the step kernel's
 // unrolled triangle loop would cost the kernel registers it needs elsewhere.
-__device__ __noinline__ float4 svd2_degenerate(float f00, float f01, float f10, float f11) {
+// fix: remove reflections from U and V^T as Compute_R_SVD does (Rotation.cpp:31-33);
+// GRotation2x2_PolarDecomposition_From_SVD (GMatDecomposition.h:365-379) does not.
+__device__ __noinline__ float4 svd2_degenerate(float f00, float f01, float f10, float f11, bool fix = true) {
 #pragma clang fp contract(off)
     const Mat2 F{f00, f01, f10, f11};
     const Mat2 A = mul2(Mat2{f00, f10, f01, f11}, F);
@@ -188,10 +190,102 @@ __device__ __noinline__ float4 svd2_degenerate(float f00, float f01, float f10, 
             }
         }
     }
-    if (det2m(U) < 0.0f) { U.m01 = -U.m01; U.m11 = -U.m11; }
-    if (det2m(Vt) < 0.0f) { Vt.m10 = -Vt.m10; Vt.m11 = -Vt.m11; }
+    if (fix) {
+        if (det2m(U) < 0.0f) { U.m01 = -U.m01; U.m11 = -U.m11; }
+        if (det2m(Vt) < 0.0f) { Vt.m10 = -Vt.m10; Vt.m11 = -Vt.m11; }
+    }
     const Mat2 R = mul2(U, Vt);
     return make_float4(R.m00, R.m01, R.m10, R.m11);
+}
+
+// ---- DAPD degenerate triangles (rare path, out of line)
+constexpr uint32_t kNocInvalid = 3;  // cInvalidNoC (TriangleElementSet2.h:114)
+
+// UpdateDegeneration for one triangle at or below the threshold
+// (TriangleElementSet2.cpp:583-640): a new node of collapse when the triangle
+// was undegenerate at prev_x (ComputeNoC, :739-835: the time of collapse as a
+// root of the quadratic det Ds(t) = thr 2 A in float, the first positive one
+// else the second, then the vertex opposite the longest edge), else the old.
+__device__ __noinline__ uint32_t update_noc2(float2 a0, float2 a1, float2 a2, float2 b0, float2 b1, float2 b2,
+                                             float g00, float g01, float g10, float g11, float area, float thr,
+                                             uint32_t old) {
+#pragma clang fp contract(off)
+    {
+        const float D00 = a1.x - a0.x, D01 = a2.x - a0.x, D10 = a1.y - a0.y, D11 = a2.y - a0.y;
+        float f00 = 0.0f; f00 += D00 * g00; f00 += D01 * g10;
+        float f01 = 0.0f; f01 += D00 * g01; f01 += D01 * g11;
+        float f10 = 0.0f; f10 += D10 * g00; f10 += D11 * g10;
+        float f11 = 0.0f; f11 += D10 * g01; f11 += D11 * g11;
+        if (!(f00 * f11 - f10 * f01 > thr)) return old;  // degenerate already: keep
+    }
+    const float a[3][2] = {{a0.x, a0.y}, {a1.x, a1.y}, {a2.x, a2.y}};
+    float d[3][2] = {{b0.x - a0.x, b0.y - a0.y}, {b1.x - a1.x, b1.y - a1.y}, {b2.x - a2.x, b2.y - a2.y}};
+    const float K = thr * 2.0f * area;
+    const float A = d[1][0] * d[2][1] + d[2][0] * d[0][1] + d[0][0] * d[1][1] - d[1][0] * d[0][1] -
+                    d[0][0] * d[2][1] - d[2][0] * d[1][1];
+    const float B = (d[1][0] * a[2][1] + a[1][0] * d[2][1]) + (d[2][0] * a[0][1] + a[2][0] * d[0][1]) +
+                    (d[0][0] * a[1][1] + a[0][0] * d[1][1]) - (d[1][0] * a[0][1] + a[1][0] * d[0][1]) -
+                    (d[0][0] * a[2][1] + a[0][0] * d[2][1]) - (d[2][0] * a[1][1] + a[2][0] * d[1][1]);
+    const float C = a[1][0] * a[2][1] + a[2][0] * a[0][1] + a[0][0] * a[1][1] - a[1][0] * a[0][1] -
+                    a[0][0] * a[2][1] - a[2][0] * a[1][1] - K;
+    float t1 = 0.0f, t2 = 1.0f;
+    if (refmath::solve_eq2(A, B, C, t1, t2) <= 0) return kNocInvalid;
+    const float toc = (t1 > 0.0f) ? t1 : t2;
+    float q[3][2];
+    for (int i = 0; i < 3; i++) { q[i][0] = a[i][0] + toc * d[i][0]; q[i][1] = a[i][1] + toc * d[i][1]; }
+    float l[3];  // squared lengths of edges 01, 12, 20
+    for (int i = 0; i < 3; i++) {
+        const int j = (i + 1) % 3;
+        const float ex = q[j][0] - q[i][0], ey = q[j][1] - q[i][1];
+        float acc = 0.0f;
+        acc += ex * ex;
+        acc += ey * ey;
+        l[i] = acc;
+    }
+    return (l[0] >= l[1]) ? ((l[0] >= l[2]) ? 2u : 1u) : ((l[1] >= l[2]) ? 0u : 1u);
+}
+
+// Compute_R_DAPD_Degenerate, 2D (Rotation.cpp:300-362, __ENABLE_REGULARIZE_DAPD
+// :6-8): the node of collapse moves along the perpendicular of its opposite
+// edge (squared length + 1e-6) by lambda_u; R is the closed-form polar factor
+// of the projected F, or its SVD-based form when det <= 0.001.
+__device__ __noinline__ float4 r_dapd_degenerate2(float2 p0, float2 p1, float2 p2, uint32_t noc, float area,
+                                                  float g00, float g01, float g10, float g11, float thr, float fL,
+                                                  float fNL, float eNL) {
+#pragma clang fp contract(off)
+    float x[3][2] = {{p0.x, p0.y}, {p1.x, p1.y}, {p2.x, p2.y}};
+    const uint32_t i0 = noc, i1 = (noc + 1) % 3, i2 = (noc + 2) % 3;
+    const float ex = x[i2][0] - x[i1][0], ey = x[i2][1] - x[i1][1];
+    float dsq = 0.0f;
+    dsq += ex * ex;
+    dsq += ey * ey;
+    dsq += 0.001f * 0.001f;
+    const float dist = sqrtf(dsq);
+    const float inv = 1.0f / dist;
+    const float ax = inv * -ey, ay = inv * ex;  // PerpendicularCW (Mal/GVec.h:223-226) / dist
+    const float dh = thr * 2.0f * area / dist;
+    const float mx = 0.5f * (x[i1][0] + x[i2][0]), my = 0.5f * (x[i1][1] + x[i2][1]);
+    float w = 0.0f;
+    w += (x[i0][0] - mx) * ax;
+    w += (x[i0][1] - my) * ay;
+    const float s = w / dh;
+    const float t = 1.0f - s;
+    const float lambda = (w >= 0.0f) ? fL * dh * ((2.0f - t) * t * t) : fL * dh * (t + fNL * powf(-s, eNL));
+    x[i0][0] = x[i0][0] + lambda * ax;
+    x[i0][1] = x[i0][1] + lambda * ay;
+    const float D00 = x[1][0] - x[0][0], D01 = x[2][0] - x[0][0], D10 = x[1][1] - x[0][1], D11 = x[2][1] - x[0][1];
+    float f00 = 0.0f; f00 += D00 * g00; f00 += D01 * g10;
+    float f01 = 0.0f; f01 += D00 * g01; f01 += D01 * g11;
+    float f10 = 0.0f; f10 += D10 * g00; f10 += D11 * g10;
+    float f11 = 0.0f; f11 += D10 * g01; f11 += D11 * g11;
+    const float dp = f00 * f11 - f10 * f01;
+    const float dd = dp - 1.0f;
+    if (dp > 0.001f || (dd > 0.0f ? dd : -dd) <= 0.0001f) {
+        Tri t_;
+        polar2(f00, f01, f10, f11, dp, t_);
+        return make_float4(t_.r00, t_.r01, t_.r10, t_.r11);
+    }
+    return svd2_degenerate(f00, f01, f10, f11, false);
 }
 
 // Rotation of one triangle (ComputeRotations_MSVD/DAPD,
@@ -313,6 +407,7 @@ __device__ void mesh_step(const Batch2Args& a, uint32_t cmax, float2* s_v_all, f
         M.fM = M.fK = M.dt = M.inv_dt = M.g0 = M.g1 = M.rel = M.abs_eps = M.thr = M.drag = 0.f;
         M.rm = M.warm_start = M.compute_energy = 0;
         M.max_iter = 0;
+        M.dapd_l = M.dapd_nl = M.dapd_e = 0.f;
     }
     const uint32_t N = M.N, T = M.T, C = M.C;
     const uint32_t CL = (GW == 1) ? C : cmax;  // colour passes: the same count in every group
@@ -321,6 +416,9 @@ __device__ void mesh_step(const Batch2Args& a, uint32_t cmax, float2* s_v_all, f
     const float2* ref = a.ref + M.node_off;
     float2* xg = a.x + M.node_off;
     float2* vg = a.v + M.node_off;
+    // DoC-style state only matters to DAPD: other meshes neither read nor write it
+    const bool track = real && a.noc && M.rm == CF_RM_DAPD;
+    float2* x0g = track ? a.x0 + M.node_off : nullptr;
     const uint16_t* cs = a.color_start + M.color_off;
 
     // ---- own triangles and nodes
@@ -365,7 +463,24 @@ __device__ void mesh_step(const Batch2Args& a, uint32_t cmax, float2* s_v_all, f
         float f10 = 0.0f; f10 += D10 * t.g00; f10 += D11 * t.g10;
         float f11 = 0.0f; f11 += D10 * t.g01; f11 += D11 * t.g11;
         const float det = f00 * f11 - f10 * f01;  // Mal/GMat.h:230-233
-        nfb += rotation2(M.rm, M.thr, f00, f01, f10, f11, det, t);
+        // UpdateDegeneration (TriangleElementSet2.cpp:583-640) and the DAPD
+        // rotation of a degenerate triangle with a node of collapse
+        uint32_t nc = kNocInvalid;
+        if (track) {
+            const size_t ge = M.tri_off + (size_t)e;
+            const uint32_t old = a.noc[ge];
+            nc = (det > M.thr) ? kNocInvalid
+                               : update_noc2(x0g[t.i0], x0g[t.i1], x0g[t.i2], p0, p1, p2, t.g00, t.g01, t.g10, t.g11,
+                                             t.A, M.thr, old);
+            if (nc != old) a.noc[ge] = uint8_t(nc);
+        }
+        if (track && !(det > M.thr) && nc != kNocInvalid) {
+            const float4 R = r_dapd_degenerate2(p0, p1, p2, nc, t.A, t.g00, t.g01, t.g10, t.g11, M.thr, M.dapd_l,
+                                                M.dapd_nl, M.dapd_e);
+            t.r00 = R.x; t.r01 = R.y; t.r10 = R.z; t.r11 = R.w;
+        } else {
+            nfb += rotation2(M.rm, M.thr, f00, f01, f10, f11, det, t);
+        }
         if (det <= 0.f) { ndeg += 1.0; sdeg += det; }
         mind = fminf(mind, det);
     }
@@ -504,6 +619,7 @@ __device__ void mesh_step(const Batch2Args& a, uint32_t cmax, float2* s_v_all, f
         const uint32_t n = lane + k * G;
         if (n >= N) continue;
         const float2 yy = y[k], xx = xg[n];
+        if (track) x0g[n] = xx;  // m_vecPos0 = x after BeginEvaluation (LeafDSH_Solid2D_FEM.cpp:974-985)
         const float2 xn = make_float2(xx.x + M.dt * yy.x, xx.y + M.dt * yy.y);
         const float2 vn = (M.drag > 0.f) ? make_float2((1.0f - M.dt * M.drag) * yy.x, (1.0f - M.dt * M.drag) * yy.y)
                                          : yy;

@@ -609,6 +609,7 @@ class Batch2:
         self.h = C.c_void_p()
         check(lib.cf_batch2_create(C.byref(self.h)), "cf_batch2_create")
         self.nodes = []
+        self.tris = []
 
     def close(self):
         if getattr(self, "h", None):
@@ -629,6 +630,7 @@ class Batch2:
         check(lib.cf_batch2_add_sim(self.h, r.size // 2, _fp(r), t.size // 3, _up(t), C.byref(params), k.size,
                                     _up(k) if k.size else None, C.byref(sid)), "cf_batch2_add_sim")
         self.nodes.append(r.size // 2)
+        self.tris.append(t.size // 3)
         return sid.value
 
     def fixed_step(self, stats=True):
@@ -662,6 +664,13 @@ class Batch2:
         check(lib.cf_batch2_fallbacks(self.h, int(sim_id), C.byref(n)), "cf_batch2_fallbacks")
         return n.value
 
+    def noc(self, sim_id):
+        """Node of collapse per triangle (3 = none), DAPD simulations."""
+        out = np.empty(self.tris[sim_id], np.uint8)
+        check(lib.cf_batch2_get_noc(self.h, int(sim_id), out.ctypes.data_as(C.POINTER(C.c_uint8))),
+              "cf_batch2_get_noc")
+        return out
+
     def info(self):
         n, w = C.c_uint32(), C.c_uint32()
         t, nn = C.c_uint64(), C.c_uint64()
@@ -677,6 +686,7 @@ class LeafSolid2D:
         t = np.ascontiguousarray(tris, np.uint32)
         k = np.ascontiguousarray(knc if knc is not None else [], np.uint32)
         self.N = r.size // 2
+        self.T = t.size // 3
         self.params = params
         self.h = C.c_void_p()
         check(lib.cf_solid2_create(self.N, _fp(r), t.size // 3, _up(t), C.byref(params), k.size,
@@ -719,3 +729,9 @@ class LeafSolid2D:
         n = C.c_uint32()
         check(lib.cf_solid2_fallbacks(self.h, C.byref(n)), "cf_solid2_fallbacks")
         return n.value
+
+    def noc(self):
+        """Node of collapse per triangle (3 = none), DAPD only."""
+        out = np.empty(self.T, np.uint8)
+        check(lib.cf_solid2_get_noc(self.h, out.ctypes.data_as(C.POINTER(C.c_uint8))), "cf_solid2_get_noc")
+        return out

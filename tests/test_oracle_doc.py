@@ -302,3 +302,107 @@ def test_dapd_degenerate_continuity(orc):
         fr = np.linspace(-0.5, 0.2, 141) + 0.0025
         Rs = np.array([rot(f) for f in fr])
         assert np.abs(np.diff(Rs, axis=0)).max() < 0.05
+
+
+# ------------------------------------------------------------------ 2D (TriangleElementSet2 / Rotation.cpp:300-362)
+def rest_tri(rng):
+    while True:
+        a = rng.normal(size=(3, 2))
+        D = (a[1:] - a[0]).T
+        if np.linalg.det(D) > 0.4 and np.linalg.cond(D) < 5:
+            return a
+
+
+def pd2(F):
+    """Exact 2x2 polar factor in float64."""
+    U, _, Vt = np.linalg.svd(F)
+    R = U @ Vt
+    if np.linalg.det(R) < 0:
+        U[:, -1] *= -1
+        R = U @ Vt
+    return R
+
+
+def test_noc2_time_and_node(orc):
+    """ComputeNoC: toc is the first positive root of det Ds(t) = thr 2 A on the
+    linear path (brute force in float64) and a vertex moved through its
+    opposite edge is the node of collapse (its opposite edge is the longest
+    once the triangle is nearly flat)."""
+    rng = np.random.default_rng(7)
+    for i in range(3):
+        for _ in range(100):
+            a = rest_tri(rng)
+            j, k = [m for m in range(3) if m != i]
+            c = 0.5 * (a[j] + a[k]) + rng.uniform(-0.2, 0.2) * (a[k] - a[j])
+            b = a.copy()
+            b[i] = c - rng.uniform(0.05, 0.8) * (a[i] - c)
+            A = np.linalg.det((a[1:] - a[0]).T) / 2
+            a32, b32 = a.astype(np.float32), b.astype(np.float32)
+            noc, toc = orc.compute_noc2(a32, b32, np.float32(A), THR, with_toc=True)
+            assert noc == i
+            aa, bb = a32.astype(np.float64), b32.astype(np.float64)
+            ts = np.array([0.0, 0.5, 1.0])
+            vals = [np.linalg.det(((aa + t * (bb - aa))[1:] - (aa + t * (bb - aa))[0]).T) - THR * 2 * np.float32(A)
+                    for t in ts]
+            r = np.sort(np.roots(np.polyfit(ts, vals, 2)).real)
+            first = r[r > 0][0]
+            assert abs(toc - first) <= 1e-5 * (1 + abs(first))
+
+
+def test_dapd_degenerate2_projects_inverted_to_threshold(orc):
+    rng = np.random.default_rng(8)
+    cases = 0
+    for i in range(3):
+        for _ in range(50):
+            a = rest_tri(rng)
+            j, k = [m for m in range(3) if m != i]
+            c = 0.5 * (a[j] + a[k])
+            b = a.copy()
+            b[i] = c - rng.uniform(0.1, 0.9) * (a[i] - c) + 0.05 * rng.normal(size=2)
+            Dm = (a[1:] - a[0]).T
+            A = np.float32(np.linalg.det(Dm) / 2)
+            Dinv = np.linalg.inv(Dm).astype(np.float32)
+            b32 = b.astype(np.float32)
+            if np.linalg.det((b32[1:] - b32[0]).T.astype(np.float64)) >= 0:
+                continue  # not inverted: the Hermite branch, not a projection to thr
+            cases += 1
+            R = orc.r_dapd_degenerate2(THR, i, b32, A, Dinv)
+            # float64: the node moved perpendicular to its opposite edge to height
+            # thr 2 A / |edge| on the triangle's side, its foot point kept
+            q = b32.astype(np.float64)
+            e = q[(i + 2) % 3] - q[(i + 1) % 3]
+            n = np.array([-e[1], e[0]]) / np.linalg.norm(e)
+            h = THR * 2 * A / np.linalg.norm(e)
+            q[i] = q[i] - np.dot(q[i] - q[(i + 1) % 3], n) * n + h * n
+            projF = (q[1:] - q[0]).T @ Dinv.astype(np.float64)
+            assert abs(np.linalg.det(projF) - THR) < 1e-4
+            assert np.abs(R - pd2(projF)).max() < 2e-5
+    assert cases > 130
+
+
+def test_dapd_degenerate2_continuity(orc):
+    rng = np.random.default_rng(9)
+    for i in range(3):
+        a = rest_tri(rng)
+        Dm = (a[1:] - a[0]).T
+        A = np.float32(np.linalg.det(Dm) / 2)
+        Dinv = np.linalg.inv(Dm).astype(np.float32)
+        j, k = (i + 1) % 3, (i + 2) % 3
+        e = a[k] - a[j]
+        n = np.array([-e[1], e[0]]) / np.linalg.norm(e)
+        h0 = np.dot(a[i] - a[j], n)
+        slide = 0.3 * rng.normal() * e / np.linalg.norm(e)
+
+        def rot(frac):
+            p = a.copy()
+            p[i] = a[i] - (1 - frac) * h0 * n + slide
+            p = p.astype(np.float32)
+            F = (p[1:] - p[0]).T.astype(np.float64) @ Dinv.astype(np.float64)
+            if np.linalg.det(F) > THR:
+                return pd2(F)
+            return orc.r_dapd_degenerate2(THR, i, p, A, Dinv)
+
+        for edge in (THR, 0.0):
+            assert np.abs(rot(edge - 1e-4) - rot(edge + 1e-4)).max() < 2e-3, (i, edge)
+        Rs = np.array([rot(f) for f in np.linspace(-0.5, 0.2, 141) + 0.0025])
+        assert np.abs(np.diff(Rs, axis=0)).max() < 0.05
