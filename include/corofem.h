@@ -167,6 +167,32 @@ int cf_mesh_box_counts(uint32_t nx, uint32_t ny, uint32_t nz, uint32_t* n_nodes,
 int cf_mesh_box(const float half_sizes[3], uint32_t nx, uint32_t ny, uint32_t nz,
                 float* h_pos, uint32_t* h_tets);
 /* Make_MeshSolidShape2_Box (Geo/shape/MeshSolidShape2.cpp:1651-1675). */
+/* ---------------------------------------------------------------- tet shapes
+ * EditableTetSolidShape3's edit-then-bake pipeline (Geo/shape/TetSolidShape3.cpp):
+ * EndEdition (:262-268, :558-566) drops tetrahedra with det Ds <= 0
+ * (AddTetrahedron :278-307), vertices no tetrahedron uses (FixDegenerateVertices
+ * :809-845), builds face neighbours and oriented boundary faces (RebuildTopology
+ * :619-749) and sorts the tetrahedra stably by layer, the least number of
+ * tetrahedron hops from the boundary (RebuildLayers :752-807). The baked order is
+ * the element order of the reference's element sets. Make_TetSolidShape3_Box
+ * (:921-999) is cf_mesh_box followed by cf_tetshape_bake. Host memory; indices
+ * uint32 (the reference stores uint16). */
+typedef struct cf_tetshape cf_tetshape;
+int cf_tetshape_bake(uint32_t n_vertices, const float* h_pos, uint32_t n_tets, const uint32_t* h_tets,
+                     cf_tetshape** out);
+/* TetGen <base>.node / <base>.ele as examples/tetgen2geo/main.cpp:80-180 reads them,
+   then baked; natural_scale > 0: its -nsac option (centre, largest extent = natural_scale). */
+int cf_tetshape_load_tetgen(const char* base_name, float natural_scale, cf_tetshape** out);
+int cf_tetshape_info(const cf_tetshape* shape, uint32_t* n_vertices, uint32_t* n_tets,
+                     uint32_t* n_boundary_faces, uint32_t* n_layers);
+/* Any output may be NULL: positions 3V, tetrahedra 4T, face neighbours 4T
+   (0xFFFFFFFF = boundary; entry n is the tet across the face opposite vertex n),
+   boundary faces 3BF (outward), layers 2L (first tet, count), and per baked tet
+   its index in the input (before dropping and sorting). */
+int cf_tetshape_get(const cf_tetshape* shape, float* h_pos, uint32_t* h_tets, uint32_t* h_ntid, uint32_t* h_bf,
+                    uint32_t* h_layers, uint32_t* h_src);
+int cf_tetshape_destroy(cf_tetshape* shape);
+
 int cf_mesh_box2_counts(uint32_t nx, uint32_t ny, uint32_t* n_nodes, uint32_t* n_tris);
 int cf_mesh_box2(const float half_sizes[2], uint32_t nx, uint32_t ny, float* h_pos, uint32_t* h_tris);
 

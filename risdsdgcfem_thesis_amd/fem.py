@@ -86,8 +86,57 @@ class DeviceArray:
             pass
 
 
+class TetShape:
+    """A tet shape after EditableTetSolidShape3::EndEdition: vertices that some
+    tetrahedron uses, positively oriented tetrahedra in layer order (the
+    reference's element order), face neighbours (-1 = boundary), outward
+    boundary faces, layers (first, count) and, per baked tetrahedron, its index
+    in the input."""
+
+    def __init__(self, handle):
+        try:
+            nv, nt, nbf, nl = C.c_uint32(), C.c_uint32(), C.c_uint32(), C.c_uint32()
+            check(lib.cf_tetshape_info(handle, C.byref(nv), C.byref(nt), C.byref(nbf), C.byref(nl)),
+                  "cf_tetshape_info")
+            self.pos = np.empty(3 * nv.value, np.float32)
+            self.tets = np.empty(4 * nt.value, np.uint32)
+            ntid = np.empty(4 * nt.value, np.uint32)
+            self.boundary_faces = np.empty(3 * nbf.value, np.uint32)
+            self.layers = np.empty(2 * nl.value, np.uint32).reshape(-1, 2)
+            self.src = np.empty(nt.value, np.uint32)
+            check(lib.cf_tetshape_get(handle, _fp(self.pos), _up(self.tets), _up(ntid), _up(self.boundary_faces),
+                                      _up(self.layers), _up(self.src)), "cf_tetshape_get")
+            self.ntid = ntid.astype(np.int64)
+            self.ntid[ntid == 0xFFFFFFFF] = -1
+        finally:
+            lib.cf_tetshape_destroy(handle)
+
+    @classmethod
+    def bake(cls, pos, tets):
+        p = np.ascontiguousarray(pos, np.float32)
+        t = np.ascontiguousarray(tets, np.uint32)
+        h = C.c_void_p()
+        check(lib.cf_tetshape_bake(p.size // 3, _fp(p), t.size // 4, _up(t), C.byref(h)), "cf_tetshape_bake")
+        return cls(h)
+
+    @classmethod
+    def from_tetgen(cls, base_name, natural_scale=0.0):
+        h = C.c_void_p()
+        check(lib.cf_tetshape_load_tetgen(str(base_name).encode(), float(natural_scale), C.byref(h)),
+              "cf_tetshape_load_tetgen")
+        return cls(h)
+
+
+def mesh_box_baked(half_sizes, counts):
+    """Make_TetSolidShape3_Box as the reference's scenes see it: the box mesh
+    after EndEdition (tetrahedra in layer order)."""
+    return TetShape.bake(*mesh_box(half_sizes, counts))
+
+
 def mesh_box(half_sizes, counts):
-    """Make_TetSolidShape3_Box: counts = vertices per axis (cubes + 1)."""
+    """The vertices and tetrahedra Make_TetSolidShape3_Box adds, in the order it
+    adds them (before EndEdition reorders the tetrahedra by layer, which
+    mesh_box_baked applies); counts = vertices per axis (cubes + 1)."""
     nx, ny, nz = (int(c) for c in counts)
     nn, nt = C.c_uint32(), C.c_uint32()
     check(lib.cf_mesh_box_counts(nx, ny, nz, C.byref(nn), C.byref(nt)), "cf_mesh_box_counts")
