@@ -192,6 +192,14 @@ int cf_tetshape_info(const cf_tetshape* shape, uint32_t* n_vertices, uint32_t* n
 int cf_tetshape_get(const cf_tetshape* shape, float* h_pos, uint32_t* h_tets, uint32_t* h_ntid, uint32_t* h_bf,
                     uint32_t* h_layers, uint32_t* h_src);
 int cf_tetshape_destroy(cf_tetshape* shape);
+/* Shape library files (ShapeLibrary::Load/Save, Geo/shape/ShapeLibrary.cpp:332-359,446-468;
+   the ItemStream container of util/ItemStream.cpp and util/ItemStreamSerialization.cpp).
+   load: the first TetSolidShape3 (version 4) item of a binary or text file (told
+   apart by the magic number); its arrays are already baked (h_src = identity).
+   save: a one-shape library, binary (binary != 0) or text; 16-bit indices, so
+   at most 65,535 vertices and tetrahedra (CF_ERR_UNSUPPORTED beyond). */
+int cf_tetshape_load(const char* path, cf_tetshape** out);
+int cf_tetshape_save(const cf_tetshape* shape, const char* path, int binary);
 
 int cf_mesh_box2_counts(uint32_t nx, uint32_t ny, uint32_t* n_nodes, uint32_t* n_tris);
 int cf_mesh_box2(const float half_sizes[2], uint32_t nx, uint32_t ny, float* h_pos, uint32_t* h_tris);

@@ -114,6 +114,8 @@ SIGNATURES = {
     "cf_tetshape_info": (C.c_int, [_vp, _u, _u, _u, _u]),
     "cf_tetshape_get": (C.c_int, [_vp, _f, _u, _u, _u, _u, _u]),
     "cf_tetshape_destroy": (C.c_int, [_vp]),
+    "cf_tetshape_load": (C.c_int, [C.c_char_p, C.POINTER(_vp)]),
+    "cf_tetshape_save": (C.c_int, [_vp, C.c_char_p, C.c_int]),
     "cf_mesh_box2_counts": (C.c_int, [C.c_uint32] * 2 + [_u, _u]),
     "cf_mesh_box2": (C.c_int, [_f, C.c_uint32, C.c_uint32, _f, _u]),
     "cf_tetset_create": (C.c_int, [C.c_uint32, _f, C.c_uint32, _u, C.POINTER(FemParams), C.POINTER(_vp)]),

@@ -24,6 +24,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <fstream>
+#include <iterator>
 #include <sstream>
 #include <string>
 #include <unordered_map>
@@ -267,6 +268,229 @@ struct Tokens {
     }
 };
 
+// ---- ItemStream shape files (util/ItemStream.h, util/ItemStream.cpp,
+// util/ItemStreamSerialization.cpp; the shape item of
+// Geo/shape/ShapeLibrary.cpp:332-359).
+// Binary: uint32 magic 'UISM', version 0, begin, end, string_end, data size
+// and string size (both rounded up to 4), the data block, the string block.
+// Items: int32 id (> 0 numeric, < 0 minus the offset of the name in the
+// string block, whose byte 0 is unused), uint16 type (low 11 bits the value
+// type, 0x1000 simple, 0x4000 array, 0x8000 complex), two pad bytes, uint32
+// size including the header; arrays add a uint32 count; payloads are padded
+// to 4 bytes; a complex item's size covers its sub-items.
+// Text: one item per line, "name value" with value 21u / -3i / 1.5f / 2.0d,
+// ^HEX for byte arrays, and "name" followed by { ... } for complex items,
+// indented 4 spaces per level; numeric ids are written as #id.
+constexpr uint32_t kIsMagic = (uint32_t('U') << 24) | (uint32_t('I') << 16) | (uint32_t('S') << 8) | uint32_t('M');
+constexpr uint16_t kIsSimple = 0x1000, kIsArray = 0x4000, kIsComplex = 0x8000, kIsTypeMask = 0x07FF;
+constexpr uint16_t kPlaUInt32 = 5, kPlaUInt8 = 9;          // pla_types.h enum order
+constexpr uint16_t kTypeShapeDef = 40;                      // cNumPlaTypes + 1 (ShapeLibrary.h:9)
+constexpr uint32_t kShapeTetSolid3 = 21;                    // EShapeType (Geo/ShapeTypes.h)
+constexpr uint32_t kTetSolidVersion = 4;                    // TetSolidShape3::cVersion
+constexpr uint16_t kNone16 = 0xFFFF;
+
+struct IsItem {
+    std::string name;  // empty for numeric ids
+    int32_t id = 0;
+    uint16_t kind = 0, type = 0;
+    double number = 0.0;  // simple numeric value
+    std::vector<uint8_t> bytes;
+    std::vector<IsItem> sub;
+    const IsItem* find(const char* n) const {
+        for (const IsItem& it : sub)
+            if (it.name == n) return &it;
+        return nullptr;
+    }
+};
+
+bool is_parse_bin(const std::vector<uint8_t>& data, const std::vector<char>& strings, uint32_t b, uint32_t e,
+                  std::vector<IsItem>& out, int depth) {
+    if (depth > 32) return false;
+    while (b < e) {
+        if (e - b < 12) return false;
+        IsItem it;
+        int32_t id;
+        uint16_t ty;
+        uint32_t size;
+        std::memcpy(&id, &data[b], 4);
+        std::memcpy(&ty, &data[b + 4], 2);
+        std::memcpy(&size, &data[b + 8], 4);
+        if (size < 12 || size > e - b) return false;
+        it.id = id;
+        if (id < 0) {
+            const size_t off = size_t(-(int64_t)id);
+            if (off >= strings.size()) return false;
+            const char* str = &strings[off];
+            it.name.assign(str, strnlen(str, strings.size() - off));
+        }
+        it.type = ty & kIsTypeMask;
+        if (ty & kIsComplex) {
+            it.kind = kIsComplex;
+            if (!is_parse_bin(data, strings, b + 12, b + size, it.sub, depth + 1)) return false;
+        } else if (ty & kIsArray) {
+            it.kind = kIsArray;
+            if (size < 16) return false;
+            uint32_t count;
+            std::memcpy(&count, &data[b + 12], 4);
+            if (it.type == kPlaUInt8 || it.type == 8) {  // uint8 or int8 (strings) arrays
+                if (count > size - 16) return false;
+                it.bytes.assign(&data[b + 16], &data[b + 16] + count);
+            }
+        } else if (ty & kIsSimple) {
+            it.kind = kIsSimple;
+            const uint8_t* p = &data[b + 12];
+            switch (it.type) {
+                case 4: { int32_t v; std::memcpy(&v, p, 4); it.number = v; } break;
+                case kPlaUInt32: { uint32_t v; std::memcpy(&v, p, 4); it.number = v; } break;
+                case 11: { float v; std::memcpy(&v, p, 4); it.number = v; } break;
+                case 12: if (size >= 20) { double v; std::memcpy(&v, p, 8); it.number = v; } break;
+                default: break;
+            }
+        }
+        out.push_back(std::move(it));
+        b += size;
+    }
+    return true;
+}
+
+// ItemStream::Parse subset (util/ItemStream.cpp:305-624): names, numbers with
+// an i/u/f/d suffix (none: double), ^hex byte arrays, nested { } items, '/'
+// comment lines, optional '=' and ',' separators.
+struct IsText {
+    const std::string& s;
+    size_t at = 0;
+    explicit IsText(const std::string& str) : s(str) {}
+    void skip(const char* set) {
+        while (at < s.size() && std::strchr(set, s[at]) && s[at]) at++;
+    }
+    bool items(std::vector<IsItem>& out, bool nested, int depth) {
+        if (depth > 32) return false;
+        for (;;) {
+            skip(" \t\r\n,");
+            if (at < s.size() && s[at] == '/') {
+                while (at < s.size() && s[at] != '\n') at++;
+                continue;
+            }
+            if (at >= s.size()) return !nested;
+            if (s[at] == '}') {
+                if (!nested) return false;
+                at++;
+                return true;
+            }
+            IsItem it;
+            const size_t nb = at;
+            while (at < s.size() && !std::strchr(" \t\r\n=", s[at])) at++;
+            std::string name = s.substr(nb, at - nb);
+            if (name.empty()) return false;
+            if (name[0] == '#') {
+                char* end = nullptr;
+                it.id = int32_t(std::strtol(name.c_str() + 1, &end, 10));
+                if (!end || *end) it.name = name;
+            } else {
+                it.name = name;
+            }
+            skip(" \t\r\n=");
+            if (at >= s.size()) return false;
+            const char c = s[at];
+            if (c == '{') {
+                at++;
+                it.kind = kIsComplex;
+                if (!items(it.sub, true, depth + 1)) return false;
+            } else if (c == '^') {
+                at++;
+                skip(" ");
+                it.kind = kIsArray;
+                it.type = kPlaUInt8;
+                auto hex = [](char h) -> int {
+                    return (h >= '0' && h <= '9') ? h - '0' : (h >= 'A' && h <= 'F') ? h - 'A' + 10 : -1;
+                };
+                while (at + 1 < s.size() && hex(s[at]) >= 0 && hex(s[at + 1]) >= 0) {
+                    it.bytes.push_back(uint8_t(hex(s[at]) * 16 + hex(s[at + 1])));
+                    at += 2;
+                }
+            } else if (std::isdigit((unsigned char)c) || c == '+' || c == '-' || c == '.') {
+                const size_t b = at;
+                while (at < s.size() && std::strchr("0123456789+-.", s[at]) && s[at]) at++;
+                const std::string num = s.substr(b, at - b);
+                it.kind = kIsSimple;
+                const char suf = (at < s.size()) ? s[at] : '\0';
+                if (suf == 'u') { it.type = kPlaUInt32; it.number = double(uint32_t(std::atoi(num.c_str()))); at++; }
+                else if (suf == 'i') { it.type = 4; it.number = std::atoi(num.c_str()); at++; }
+                else if (suf == 'f') { it.type = 11; it.number = float(std::atof(num.c_str())); at++; }
+                else if (suf == 'd') { it.type = 12; it.number = std::atof(num.c_str()); at++; }
+                else { it.type = 12; it.number = std::atof(num.c_str()); }
+            } else {
+                return false;  // keywords, strings, arrays, NiR: not part of a shape item
+            }
+            out.push_back(std::move(it));
+        }
+    }
+};
+
+int is_read_file(const char* path, std::vector<IsItem>& items) {
+    std::ifstream f(path, std::ios::binary);
+    if (!f) { set_error("cannot open %s", path); return CF_ERR_INVALID_ARG; }
+    std::vector<uint8_t> raw((std::istreambuf_iterator<char>(f)), std::istreambuf_iterator<char>());
+    uint32_t magic = 0;
+    if (raw.size() >= 4) std::memcpy(&magic, raw.data(), 4);
+    if (magic == kIsMagic) {
+        if (raw.size() < 28) { set_error("%s: truncated item stream", path); return CF_ERR_INVALID_ARG; }
+        uint32_t h[7];
+        std::memcpy(h, raw.data(), 28);  // magic, version, begin, end, string_end, data size, string size
+        if (h[1] != 0 || h[2] > h[3] || h[3] > h[5] || h[4] > h[6] || 28ull + h[5] + h[6] > raw.size()) {
+            set_error("%s: bad item stream header", path);
+            return CF_ERR_INVALID_ARG;
+        }
+        std::vector<uint8_t> data(raw.begin() + 28, raw.begin() + 28 + h[5]);
+        std::vector<char> strings(raw.begin() + 28 + h[5], raw.begin() + 28 + h[5] + h[6]);
+        strings.push_back('\0');
+        if (!is_parse_bin(data, strings, h[2], h[3], items, 0)) {
+            set_error("%s: malformed binary item stream", path);
+            return CF_ERR_INVALID_ARG;
+        }
+        return CF_OK;
+    }
+    const std::string text(raw.begin(), raw.end());
+    IsText p(text);
+    if (!p.items(items, false, 0)) {
+        set_error("%s: malformed text item stream (at byte %zu)", path, p.at);
+        return CF_ERR_INVALID_ARG;
+    }
+    return CF_OK;
+}
+
+// the arrays of a baked shape in the reference's 16-bit layout
+struct Shape16 {
+    std::vector<uint8_t> points, v, t, bf, l;
+};
+
+int shape16(const cf_tetshape* s, Shape16& o) {
+    const size_t V = s->pos.size() / 3, T = s->src.size(), BF = s->bf.size() / 3, L = s->layers.size() / 2;
+    if (V > 0xFFFF || T > 0xFFFF || L > 0xFFFF) {
+        set_error("the reference's shape files hold 16-bit indices: %zu vertices, %zu tetrahedra", V, T);
+        return CF_ERR_UNSUPPORTED;
+    }
+    o.points.resize(12 * V);
+    std::memcpy(o.points.data(), s->pos.data(), 12 * V);
+    o.v.assign(V, 0);
+    std::vector<uint16_t> t(8 * T), bf(3 * BF), l(2 * L);
+    for (size_t e = 0; e < T; e++)
+        for (int j = 0; j < 4; j++) {
+            t[8 * e + j] = uint16_t(s->vid[4 * e + j]);
+            t[8 * e + 4 + j] = (s->ntid[4 * e + j] == kNone) ? kNone16 : uint16_t(s->ntid[4 * e + j]);
+        }
+    for (size_t i = 0; i < 3 * BF; i++) bf[i] = uint16_t(s->bf[i]);
+    for (size_t i = 0; i < 2 * L; i++) l[i] = uint16_t(s->layers[i]);
+    auto bytes = [](const std::vector<uint16_t>& x, std::vector<uint8_t>& out) {
+        out.resize(2 * x.size());
+        std::memcpy(out.data(), x.data(), out.size());
+    };
+    bytes(t, o.t);
+    bytes(bf, o.bf);
+    bytes(l, o.l);
+    return CF_OK;
+}
+
 }  // namespace
 }  // namespace cf
 
@@ -394,6 +618,158 @@ int cf_tetshape_get(const cf_tetshape* s, float* h_pos, uint32_t* h_tets, uint32
 
 int cf_tetshape_destroy(cf_tetshape* s) {
     delete s;
+    return CF_OK;
+}
+
+// ShapeLibrary::Load (Geo/shape/ShapeLibrary.cpp:460-468) of a library file
+// holding a TetSolidShape3 (binary or text, told apart by the magic number):
+// the first top-level complex item with shape_type == eShape_TetSolid3.
+int cf_tetshape_load(const char* path, cf_tetshape** out) {
+    if (!path || !out) return CF_ERR_INVALID_ARG;
+    std::vector<IsItem> items;
+    CF_TRY(is_read_file(path, items));
+    for (const IsItem& it : items) {
+        if (it.kind != kIsComplex) continue;
+        const IsItem* st = it.find("shape_type");
+        if (!st || st->kind != kIsSimple || st->number != double(kShapeTetSolid3)) continue;
+        const IsItem* ver = it.find("version");
+        if (!ver || ver->number != double(kTetSolidVersion)) {
+            set_error("%s: TetSolidShape3 version %g (only version %u is read)", path, ver ? ver->number : -1.0,
+                      kTetSolidVersion);
+            return CF_ERR_UNSUPPORTED;
+        }
+        const char* names[5] = {"vec_points", "vec_v", "vec_t", "vec_bf", "vec_l"};
+        const char* counts[4] = {"num_v", "num_t", "num_bf", "num_l"};
+        double n[4];
+        for (int k = 0; k < 4; k++) {
+            const IsItem* c = it.find(counts[k]);
+            if (!c || c->kind != kIsSimple || !(c->number >= 0) || c->number > 4294967295.0) {
+                set_error("%s: missing or bad %s", path, counts[k]);
+                return CF_ERR_INVALID_ARG;
+            }
+            n[k] = c->number;
+        }
+        const size_t V = size_t(n[0]), T = size_t(n[1]), BF = size_t(n[2]), L = size_t(n[3]);
+        const size_t want[5] = {12 * V, V, 16 * T, 6 * BF, 4 * L};
+        const IsItem* arr[5];
+        for (int k = 0; k < 5; k++) {
+            arr[k] = it.find(names[k]);
+            if (!arr[k] || arr[k]->kind != kIsArray || arr[k]->bytes.size() != want[k]) {
+                set_error("%s: missing or bad %s", path, names[k]);
+                return CF_ERR_INVALID_ARG;
+            }
+        }
+        cf_tetshape* s = new cf_tetshape;
+        s->pos.resize(3 * V);
+        std::memcpy(s->pos.data(), arr[0]->bytes.data(), 12 * V);
+        s->vid.resize(4 * T);
+        s->ntid.resize(4 * T);
+        s->src.resize(T);
+        for (size_t e = 0; e < T; e++) {
+            uint16_t q[8];
+            std::memcpy(q, &arr[2]->bytes[16 * e], 16);
+            for (int j = 0; j < 4; j++) {
+                s->vid[4 * e + j] = q[j];
+                s->ntid[4 * e + j] = (q[4 + j] == kNone16) ? kNone : q[4 + j];
+            }
+            s->src[e] = uint32_t(e);
+        }
+        s->bf.resize(3 * BF);
+        for (size_t i = 0; i < 3 * BF; i++) {
+            uint16_t x;
+            std::memcpy(&x, &arr[3]->bytes[2 * i], 2);
+            s->bf[i] = x;
+        }
+        s->layers.resize(2 * L);
+        for (size_t i = 0; i < 2 * L; i++) {
+            uint16_t x;
+            std::memcpy(&x, &arr[4]->bytes[2 * i], 2);
+            s->layers[i] = x;
+        }
+        for (size_t i = 0; i < 4 * T; i++)
+            if (s->vid[i] >= V || (s->ntid[i] != kNone && s->ntid[i] >= T)) {
+                delete s;
+                set_error("%s: tetrahedron index out of range", path);
+                return CF_ERR_INVALID_ARG;
+            }
+        *out = s;
+        return CF_OK;
+    }
+    set_error("%s: no TetSolidShape3 item", path);
+    return CF_ERR_INVALID_ARG;
+}
+
+// ShapeLibrary::Register_TetSolidShape3 + Save (ShapeLibrary.cpp:332-359,
+// :446-450): a one-shape library, the shape as item #1.
+int cf_tetshape_save(const cf_tetshape* s, const char* path, int binary) {
+    if (!s || !path) return CF_ERR_INVALID_ARG;
+    Shape16 a;
+    CF_TRY(shape16(s, a));
+    const uint32_t counts[4] = {uint32_t(s->pos.size() / 3), uint32_t(s->src.size()), uint32_t(s->bf.size() / 3),
+                                uint32_t(s->layers.size() / 2)};
+    const char* cnames[6] = {"shape_type", "version", "num_v", "num_t", "num_bf", "num_l"};
+    const uint32_t cvals[6] = {kShapeTetSolid3, kTetSolidVersion, counts[0], counts[1], counts[2], counts[3]};
+    const char* anames[5] = {"vec_points", "vec_v", "vec_t", "vec_bf", "vec_l"};
+    const std::vector<uint8_t>* arrs[5] = {&a.points, &a.v, &a.t, &a.bf, &a.l};
+    std::string out;
+    if (binary) {
+        std::vector<char> strings(1, '\0');  // offset 0 is never a name
+        std::vector<uint32_t> off;
+        for (const char* nm : cnames) { off.push_back(uint32_t(strings.size())); strings.insert(strings.end(), nm, nm + std::strlen(nm) + 1); }
+        for (const char* nm : anames) { off.push_back(uint32_t(strings.size())); strings.insert(strings.end(), nm, nm + std::strlen(nm) + 1); }
+        std::vector<uint8_t> data;
+        auto put32 = [&](uint32_t x) { const uint8_t* p = reinterpret_cast<const uint8_t*>(&x); data.insert(data.end(), p, p + 4); };
+        auto header = [&](int32_t id, uint16_t type, uint32_t size) {
+            put32(uint32_t(id));
+            data.push_back(uint8_t(type & 0xFF));
+            data.push_back(uint8_t(type >> 8));
+            data.push_back(0);
+            data.push_back(0);
+            put32(size);
+        };
+        header(1, uint16_t(kTypeShapeDef | kIsComplex), 0);
+        for (int k = 0; k < 6; k++) {
+            header(-int32_t(off[k]), uint16_t(kPlaUInt32 | kIsSimple), 16);
+            put32(cvals[k]);
+        }
+        for (int k = 0; k < 5; k++) {
+            const std::vector<uint8_t>& b = *arrs[k];
+            const uint32_t padded = uint32_t(4 * ((b.size() + 3) / 4));
+            header(-int32_t(off[6 + k]), uint16_t(kPlaUInt8 | kIsArray), 16 + padded);
+            put32(uint32_t(b.size()));
+            data.insert(data.end(), b.begin(), b.end());
+            data.resize(data.size() + (padded - b.size()), 0);
+        }
+        const uint32_t total = uint32_t(data.size());
+        std::memcpy(&data[8], &total, 4);
+        const uint32_t send = uint32_t(strings.size());
+        strings.resize(4 * ((strings.size() + 3) / 4), '\0');
+        const uint32_t hdr[7] = {kIsMagic, 0u, 0u, total, send, total, uint32_t(strings.size())};
+        out.append(reinterpret_cast<const char*>(hdr), 28);
+        out.append(reinterpret_cast<const char*>(data.data()), data.size());
+        out.append(strings.data(), strings.size());
+    } else {  // ItemToFileTxt (ItemStreamSerialization.cpp:14-173), 4-space indentation
+        static const char hexd[] = "0123456789ABCDEF";
+        out += "#1\n{\n";
+        char line[64];
+        for (int k = 0; k < 6; k++) {
+            std::snprintf(line, sizeof(line), "    %s %du\n", cnames[k], int(cvals[k]));
+            out += line;
+        }
+        for (int k = 0; k < 5; k++) {
+            out += "    ";
+            out += anames[k];
+            out += " ^";
+            for (uint8_t b : *arrs[k]) { out += hexd[b >> 4]; out += hexd[b & 15]; }
+            out += "\n";
+        }
+        out += "}\n";
+    }
+    std::ofstream f(path, std::ios::binary);
+    if (!f || !f.write(out.data(), std::streamsize(out.size()))) {
+        set_error("cannot write %s", path);
+        return CF_ERR_INVALID_ARG;
+    }
     return CF_OK;
 }
 

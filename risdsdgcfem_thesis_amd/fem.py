@@ -94,6 +94,7 @@ class TetShape:
     in the input."""
 
     def __init__(self, handle):
+        self.h = handle
         try:
             nv, nt, nbf, nl = C.c_uint32(), C.c_uint32(), C.c_uint32(), C.c_uint32()
             check(lib.cf_tetshape_info(handle, C.byref(nv), C.byref(nt), C.byref(nbf), C.byref(nl)),
@@ -108,8 +109,28 @@ class TetShape:
                                       _up(self.layers), _up(self.src)), "cf_tetshape_get")
             self.ntid = ntid.astype(np.int64)
             self.ntid[ntid == 0xFFFFFFFF] = -1
-        finally:
-            lib.cf_tetshape_destroy(handle)
+        except Exception:
+            self.close()
+            raise
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib.cf_tetshape_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def save(self, path, binary=True):
+        """ShapeLibrary::Save of a one-shape library (binary or text)."""
+        check(lib.cf_tetshape_save(self.h, str(path).encode(), int(bool(binary))), "cf_tetshape_save")
+
+    @classmethod
+    def load(cls, path):
+        """The first TetSolidShape3 item of a shape library file (binary or text)."""
+        h = C.c_void_p()
+        check(lib.cf_tetshape_load(str(path).encode(), C.byref(h)), "cf_tetshape_load")
+        return cls(h)
 
     @classmethod
     def bake(cls, pos, tets):
