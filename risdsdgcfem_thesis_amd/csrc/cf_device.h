@@ -353,11 +353,26 @@ __device__ inline double block_sum(double v, double* sh) {
     for (int i = 0; i < kBlock / 64; i++) r += sh[i];
     return r;
 }
+// v += part[i], i = first, first + stride, ... < n, in that order. The loads
+// of eight terms are issued before their adds: the partials come from blocks
+// on every XCD, so most are misses in this XCD's L2, and one latency per
+// eight terms replaces one per term. The additions keep their order.
+__device__ inline double strided_sum(const double* part, int first, int n, int stride) {
+    double v = 0.0;
+    int i = first;
+    for (; i + 7 * stride < n; i += 8 * stride) {
+        double t[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) t[u] = part[i + u * stride];
+#pragma unroll
+        for (int u = 0; u < 8; u++) v += t[u];
+    }
+    for (; i < n; i += stride) v += part[i];
+    return v;
+}
 // Deterministic sum of n partials by one block (fixed order and tree).
 __device__ inline double reduce_partials(const double* part, int n, double* sh) {
-    double v = 0.0;
-    for (int i = threadIdx.x; i < n; i += kBlock) v += part[i];
-    return block_sum(v, sh);
+    return block_sum(strided_sum(part, threadIdx.x, n, kBlock), sh);
 }
 // reduce_partials computed by one wavefront: lane l plays the threads l,
 // l+64, l+128, l+192 of the block and the four wave sums are added in the
@@ -366,11 +381,7 @@ __device__ inline double reduce_partials(const double* part, int n, double* sh) 
 __device__ inline double wave_reduce_partials(const double* part, int n) {
     const int l = threadIdx.x & 63;
     double r = 0.0;
-    for (int w = 0; w < kBlock / 64; w++) {
-        double v = 0.0;
-        for (int i = l + 64 * w; i < n; i += kBlock) v += part[i];
-        r += wave_sum(v);
-    }
+    for (int w = 0; w < kBlock / 64; w++) r += wave_sum(strided_sum(part, l + 64 * w, n, kBlock));
     return r;
 }
 
