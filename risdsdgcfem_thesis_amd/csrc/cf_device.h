@@ -1251,25 +1251,40 @@ template <int MODE>
 __device__ inline double gather_node(uint32_t n, const uint32_t* __restrict__ node_part_ptr,
                                      const float* __restrict__ partials, const NodeTerms& t) {
 #pragma clang fp contract(off)
+    const size_t i = 3 * (size_t)n;
+    // the matvec's node data does not depend on the partials: load it first
+    float d0 = 0.f, d1 = 0.f, d2 = 0.f, mass = 0.f;
+    bool knc = false;
+    if (MODE == kGatherMatvec) {
+        mass = t.mass_array ? t.mass_array[n] : t.mass;
+        d0 = t.d[i]; d1 = t.d[i + 1]; d2 = t.d[i + 2];
+        knc = t.knc_mask && t.knc_mask[n];
+    }
+    // partials in ascending order, loaded two at a time (most nodes have one
+    // or two), added one after the other as before
     float fx = 0.f, fy = 0.f, fz = 0.f;
     const uint32_t p1 = node_part_ptr[n + 1];
-    for (uint32_t p = node_part_ptr[n]; p < p1; p++) {
+    uint32_t p = node_part_ptr[n];
+    for (; p + 1 < p1; p += 2) {
+        const float* pp = partials + 3 * (size_t)p;
+        const float ax = pp[0], ay = pp[1], az = pp[2], bx = pp[3], by = pp[4], bz = pp[5];
+        fx += ax; fy += ay; fz += az;
+        fx += bx; fy += by; fz += bz;
+    }
+    if (p < p1) {
         const float* pp = partials + 3 * (size_t)p;
         fx += pp[0]; fy += pp[1]; fz += pp[2];
     }
-    const size_t i = 3 * (size_t)n;
     if (MODE == kGatherAccumulate) {
         t.out[i] += fx; t.out[i + 1] += fy; t.out[i + 2] += fz;
         return 0.0;
     } else if (MODE == kGatherMatvec) {
         // LS_A_times_d (LeafDSH_Solid3D_FEM.cpp:1056-1079) + LS_ZeroKNC (:1045-1053)
-        const float mass = t.mass_array ? t.mass_array[n] : t.mass;
         const float fac = mass * t.fM;
-        const float d0 = t.d[i], d1 = t.d[i + 1], d2 = t.d[i + 2];
         float q0 = -t.fK * fx + fac * d0;
         float q1 = -t.fK * fy + fac * d1;
         float q2 = -t.fK * fz + fac * d2;
-        if (t.knc_mask && t.knc_mask[n]) { q0 = 0.f; q1 = 0.f; q2 = 0.f; }
+        if (knc) { q0 = 0.f; q1 = 0.f; q2 = 0.f; }
         t.out[i] = q0; t.out[i + 1] = q1; t.out[i + 2] = q2;
         return (double)d0 * q0 + (double)d1 * q1 + (double)d2 * q2;
     } else if (MODE == kGatherRhsDx) {
