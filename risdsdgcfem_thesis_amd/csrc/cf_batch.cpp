@@ -206,7 +206,7 @@ T* at(char* base, size_t off) { return reinterpret_cast<T*>(base + off); }
 
 // Device layout of a mesh's topology (one arena block).
 struct MeshLayout {
-    size_t ref, conn, dinv, vol, tep, tnp, tn, sp, lv, rk, npp, pd, total;
+    size_t ref, conn, dinv, vol, tep, tnp, tn, sp, lv, rk, npp, pd, rt, total;
     MeshLayout(uint32_t N, uint32_t T, uint32_t nt, uint32_t P) {
         Layout l;
         ref = l.add<float>(3 * (size_t)N);
@@ -221,6 +221,7 @@ struct MeshLayout {
         rk = l.add<uint32_t>(T);
         npp = l.add<uint32_t>((size_t)N + 1);
         pd = l.add<uint32_t>(P);
+        rt = l.add<float>(3 * (size_t)P);  // rest positions per tile node, tile-major
         total = Arena::round(l.off);
     }
 };
@@ -292,6 +293,7 @@ int upload_mesh(cf_batch* b, TileBuild& tb, uint32_t* mesh_id) {
     t.rk = at<uint32_t>(m.block, L.rk);
     t.node_part_ptr = at<uint32_t>(m.block, L.npp);
     t.part_dst = at<uint32_t>(m.block, L.pd);
+    t.ref_tile = at<float>(m.block, L.rt);
     t.num_tiles = nt; t.max_tile_elems = m.max_te; t.max_tile_nodes = m.max_tn; t.num_parts = P;
     hipStream_t s = b->stream;
     auto put = [&](void* dst, const void* src, size_t bytes) -> int {
@@ -308,6 +310,10 @@ int upload_mesh(cf_batch* b, TileBuild& tb, uint32_t* mesh_id) {
     if (rc == CF_OK) rc = put(t.rk, th.rk.data(), th.rk.size() * 4);
     if (rc == CF_OK) rc = put(t.node_part_ptr, th.node_part_ptr.data(), th.node_part_ptr.size() * 4);
     if (rc == CF_OK) rc = put(t.part_dst, th.part_dst.data(), th.part_dst.size() * 4);
+    std::vector<float> rt(3 * (size_t)P);
+    for (size_t q = 0; q < P; q++)
+        for (int c = 0; c < 3; c++) rt[3 * q + c] = tb.ref[3 * (size_t)th.tile_nodes[q] + c];
+    if (rc == CF_OK) rc = put(t.ref_tile, rt.data(), rt.size() * 4);
     if (rc == CF_OK) rc = launch_precompute(t, s);
     // total volume: FP32 serial sum in the caller's element order (as cf_tetset)
     std::vector<float> vol(T), vol_user(T);
@@ -794,6 +800,7 @@ int cf_batch_add_sim(cf_batch* b, uint32_t mesh_id, const cf_solid_params* p, ui
     d.conn = t.conn; d.tile_elem_ptr = t.tile_elem_ptr; d.tile_node_ptr = t.tile_node_ptr;
     d.tile_nodes = t.tile_nodes; d.slot_ptr = t.slot_ptr; d.node_part_ptr = t.node_part_ptr;
     d.part_dst = t.part_dst; d.lv = t.lv; d.rk = t.rk; d.ref = t.ref; d.dinv = t.dinv; d.vol = t.vol;
+    d.ref_tile = t.ref_tile;
     float* vecs[8];
     for (int i = 0; i < 8; i++) vecs[i] = at<float>(s.block, L.vec[i]);
     d.x = vecs[0]; d.v = vecs[1]; d.b = vecs[2]; d.y = vecs[3];

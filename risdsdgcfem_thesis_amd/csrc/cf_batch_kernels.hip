@@ -29,6 +29,7 @@ __device__ inline TileArgs tile_args(const SimDev& s, const float* in, uint32_t 
     a.dinv = s.dinv;
     a.vol = s.vol;
     a.ref = s.ref;
+    a.ref_tile = s.ref_tile;
     a.in = in;
     a.partials = s.partials;
     a.m = s.m;

@@ -194,7 +194,7 @@ static void cg_free(CgBuffers& b) {
 static void tetset_free(cf_tetset* s) {
     TetDevice& t = s->dev;
     dfree(t.ref); dfree(t.conn); dfree(t.dinv); dfree(t.vol);
-    dfree(t.tile_elem_ptr); dfree(t.tile_node_ptr); dfree(t.tile_nodes); dfree(t.slot_ptr);
+    dfree(t.tile_elem_ptr); dfree(t.tile_node_ptr); dfree(t.tile_nodes); dfree(t.slot_ptr); dfree(t.ref_tile);
     dfree(t.lv); dfree(t.rk); dfree(t.node_part_ptr); dfree(t.part_dst); dfree(t.partials);
     if (s->stream) (void)hipStreamDestroy(s->stream);
     s->stream = nullptr;
@@ -251,6 +251,13 @@ static int tetset_bake(cf_tetset* s) {
     CF_TRY(upload(&t.rk, th.rk, st));
     CF_TRY(upload(&t.node_part_ptr, th.node_part_ptr, st));
     CF_TRY(upload(&t.part_dst, th.part_dst, st));
+    {
+        // rest positions per tile node, tile-major: the tile stage streams them
+        std::vector<float> rt(3 * th.tile_nodes.size());
+        for (size_t q = 0; q < th.tile_nodes.size(); q++)
+            for (int c = 0; c < 3; c++) rt[3 * q + c] = s->ref_host[3 * (size_t)th.tile_nodes[q] + c];
+        CF_TRY(upload(&t.ref_tile, rt, st));
+    }
     t.num_tiles = uint32_t(th.tile_elem_ptr.size() - 1);
     t.max_tile_elems = th.max_tile_elems;
     t.max_tile_nodes = th.max_tile_nodes;

@@ -415,6 +415,7 @@ struct TileArgs {
     const float* dinv;
     const float* vol;
     const float* ref;
+    const float* ref_tile;  // rest positions per tile node (tile-major), or null: gather from ref
     const float* in;
     float* partials;
     Material m;
@@ -493,9 +494,15 @@ __device__ inline void load_tile_regs_cold(const TileArgs& a, const TileDesc& d,
     for (int k = 0; k < 9; k++) g.R[k] = ld32(a.R + k * (size_t)a.T, e);
     g.sp = ld32(a.slot_ptr, d.n0 + min(tid, d.nn)) - 4 * d.e0;
     g.pd = ld32(a.part_dst, d.n0 + tn);
+    if (a.ref_tile) {  // static: streamed in tile order, not gathered behind the ids
+        const uint32_t q = 3 * (d.n0 + tn);
+        g.r[0] = ld32(a.ref_tile, q + 0); g.r[1] = ld32(a.ref_tile, q + 1); g.r[2] = ld32(a.ref_tile, q + 2);
+    }
     if (tid < d.nn) {
         g.v[0] = ld32(a.in, 3 * gid + 0); g.v[1] = ld32(a.in, 3 * gid + 1); g.v[2] = ld32(a.in, 3 * gid + 2);
-        g.r[0] = ld32(a.ref, 3 * gid + 0); g.r[1] = ld32(a.ref, 3 * gid + 1); g.r[2] = ld32(a.ref, 3 * gid + 2);
+        if (!a.ref_tile) {
+            g.r[0] = ld32(a.ref, 3 * gid + 0); g.r[1] = ld32(a.ref, 3 * gid + 1); g.r[2] = ld32(a.ref, 3 * gid + 2);
+        }
     }
 #pragma unroll
     for (int j = 0; j < 4; j++) { g.l[j] = (lv >> (8 * j)) & 255u; g.f[j] = (rk >> (8 * j)) & 255u; }

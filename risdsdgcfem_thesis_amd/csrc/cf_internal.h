@@ -68,6 +68,7 @@ struct TetDevice {
     uint32_t* tile_elem_ptr = nullptr;  // num_tiles+1
     uint32_t* tile_node_ptr = nullptr;  // num_tiles+1
     uint32_t* tile_nodes = nullptr;     // num_parts, global node ids
+    float* ref_tile = nullptr;          // 3 num_parts: rest position of each tile node, tile-major
     uint32_t* slot_ptr = nullptr;       // num_parts+1, slot runs (tile slots start at 4*e0)
     uint32_t* lv = nullptr;             // T: the element's 4 tile-local node ids, one byte each
     uint32_t* rk = nullptr;             // T: per vertex, the element's rank in that node's slot run (byte each)
@@ -217,6 +218,7 @@ struct SimDev {
     const uint32_t* lv;
     const uint32_t* rk;
     const float* ref;
+    const float* ref_tile;  // rest positions per tile node, tile-major
     const float* dinv;
     const float* vol;
     // state of this simulation

@@ -405,6 +405,7 @@ int launch_forces_skip(const TetDevice& t, int mode, const float* in, const floa
     a.part_dst = t.part_dst;
     a.slot_ptr = t.slot_ptr; a.lv = t.lv; a.rk = t.rk; a.R = R; a.dinv = t.dinv; a.vol = t.vol;
     a.ref = t.ref;
+    a.ref_tile = t.ref_tile;
     a.in = in; a.partials = t.partials; a.m = m; a.skip = skip;
     const uint32_t grid = t.tile_grid;
     const bool fe = mode == kModeFe;
@@ -482,6 +483,7 @@ int launch_damping(const TetDevice& t, const float* v, const float* R, Material 
     a.part_dst = t.part_dst;
     a.slot_ptr = t.slot_ptr; a.lv = t.lv; a.rk = t.rk; a.R = R; a.dinv = t.dinv; a.vol = t.vol;
     a.ref = t.ref;
+    a.ref_tile = t.ref_tile;
     a.in = v; a.partials = t.partials; a.m = m; a.skip = nullptr;
     const uint32_t g1 = 8 * ((t.num_tiles + 7) / 8);
     switch (t.tile_threads) {
