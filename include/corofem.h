@@ -323,6 +323,12 @@ int cf_multi_destroy(cf_multi* m);
 /* stats: NULL or one per solid (its last fixed step; elastic_energy 0, as the
    NR_End hook does not evaluate V). */
 int cf_multi_update(cf_multi* m, float dt, uint32_t* num_fixed_steps, cf_step_stats* stats);
+/* Each leaf's CG runs in chunks of eight iterations, every leaf's chunk
+   enqueued on its own stream before the host waits on any. With graphs on (the
+   default) each chunk is a HIP graph recorded per leaf on first use and kept
+   with the leaf until its linear-system coefficients, iteration cap or fixed
+   nodes change; off, the same kernels are launched directly (bit-identical). */
+int cf_multi_set_graphs(cf_multi* m, int use_graphs);
 
 /* ---------------------------------------------------------------- tuning */
 /* Elements per LDS tile of the node-gather K*x (0 = library default 256).

@@ -163,6 +163,7 @@ SIGNATURES = {
     "cf_multi_create": (C.c_int, [C.POINTER(_vp), C.c_uint32, C.POINTER(_vp)]),
     "cf_multi_destroy": (C.c_int, [_vp]),
     "cf_multi_update": (C.c_int, [_vp, C.c_float, _u, C.POINTER(StepStats)]),
+    "cf_multi_set_graphs": (C.c_int, [_vp, C.c_int]),
     "cf_set_tile_elems": (C.c_int, [C.c_uint32]),
     "cf_get_tile_elems": (C.c_int, [_u]),
     "cf_set_tile_threads": (C.c_int, [C.c_uint32]),

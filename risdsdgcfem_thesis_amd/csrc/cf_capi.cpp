@@ -145,6 +145,22 @@ struct cf_cg {
     CgBuffers b;
 };
 
+// Recorded CG chunks of one leaf: graph c runs iterations 8c .. 8c+7 (clipped
+// to the iteration cap). Every kernel argument is fixed for the leaf's
+// lifetime except those in the key, so the graphs are re-recorded only when
+// the key changes.
+struct CgGraphs {
+    std::vector<hipGraphExec_t> chunk;
+    float fM = 0.f, fK = 0.f;
+    uint32_t max_it = 0;
+    const uint8_t* knc_mask = nullptr;
+    void clear() {
+        for (hipGraphExec_t g : chunk)
+            if (g) (void)hipGraphExecDestroy(g);
+        chunk.clear();
+    }
+};
+
 struct cf_solid {
     cf_solid_params p{};
     cf_tetset* set = nullptr;
@@ -166,11 +182,13 @@ struct cf_solid {
     double* h_part = nullptr;             // pinned: block partials of |b|^2, |y|^2
     cf_cg_result last_ls{};               // last linear solve
     int32_t ls_total = 0;                 // linear-solve iterations of the last fixed step
+    CgGraphs graphs;                      // recorded CG chunks (multi-solid step)
 };
 
 struct cf_multi {
     std::vector<cf_solid*> solids;
     float acc_time = 0.f;
+    bool use_graphs = true;
 };
 
 // ================================================================ helpers
@@ -310,47 +328,116 @@ static int cache_stats_fetch(cf_cache* c) {
 // Device CG loop of ILSS_CG::Solve for the leaf operator A = fM M - fK K.
 // The loop runs on the device; the host enqueues chunks of iterations and
 // checks the device's done flag between chunks.
-static int solid_cg(cf_solid* s, float fM, float fK, cf_cg_result* res) {
-    cf_tetset* set = s->set;
-    const TetDevice& t = set->dev;
+// One leaf's CG in resumable phases: cg_begin enqueues r = S(b - A y), d = r
+// and the start test; cg_launch enqueues up to eight iterations (each skipped
+// on the device once the solve has stopped) and the read-back of the stop
+// flag; cg_poll waits for that flag; cg_finish reads the scalars. The
+// multi-solid step interleaves several leaves' phases on their own streams.
+struct CgRun {
+    NodeTerms nt;
+    uint32_t k = 0, max_it = 0;
+    bool done = false;
+};
+
+static int cg_begin(cf_solid* s, float fM, float fK, CgRun* run) {
+    const TetDevice& t = s->set->dev;
     hipStream_t st = s->stream;
     CgBuffers& b = s->cg;
     const uint32_t n = 3 * s->N;
     const int nvec = node_blocks(n);
     const int nnode = node_blocks(s->N);
-    uint32_t max_it = std::min<uint32_t>(s->p.cg_max_iter, kMaxCgIter);
-    NodeTerms nt;
+    run->max_it = std::min<uint32_t>(s->p.cg_max_iter, kMaxCgIter);
+    run->k = 0;
+    run->done = false;
+    NodeTerms& nt = run->nt;
+    nt = NodeTerms{};
     nt.knc_mask = s->num_knc ? s->knc_mask : nullptr;
     nt.mass = s->node_mass;
     nt.fM = fM; nt.fK = fK;
     nt.partials = b.part_a;
-    // r = S(b - A y), d = r
     nt.d = s->y; nt.out = b.q;
     CF_TRY(launch_forces(t, kModeDfx, s->y, s->cache->R, cache_mat(s->cache), st));
     CF_TRY(launch_gather(t, kGatherMatvec, nt, nnode, st));
     CF_TRY(launch_cg_init(n, b.q, s->bvec, b.r, b.d, nt.knc_mask, b.part_b, nvec, st));
-    CF_TRY(launch_cg_start(b.st, b.part_b, nvec, s->p.cg_rel_epsilon, s->p.cg_abs_epsilon, max_it, st));
+    CF_TRY(launch_cg_start(b.st, b.part_b, nvec, s->p.cg_rel_epsilon, s->p.cg_abs_epsilon, run->max_it, st));
     nt.d = b.d; nt.out = b.q;
-    uint32_t k = 0;
-    const uint32_t chunk = 8;
-    while (k < max_it) {
-        const uint32_t kend = std::min(max_it, k + chunk);
-        for (; k < kend; k++) {
-            const int32_t* skip = &b.st->done[k];
-            CF_TRY(launch_forces_skip(t, kModeDfx, b.d, s->cache->R, cache_mat(s->cache), skip, st));
-            CF_TRY(launch_gather_skip(t, kGatherMatvec, nt, nnode, skip, st));
-            CF_TRY(launch_cg_alpha(b.st, k, b.part_a, nnode, st));
-            CF_TRY(launch_cg_update_xr(n, k, b.st, s->y, b.r, b.d, b.q, b.part_b, st));
-            CF_TRY(launch_cg_update_d(n, k, b.st, b.d, b.r, nt.knc_mask, st));
-        }
-        CF_CHECK_HIP(hipMemcpyAsync(b.h_flag, &b.st->done[k], sizeof(int32_t), hipMemcpyDeviceToHost, st));
-        CF_CHECK_HIP(hipStreamSynchronize(st));
-        if (*b.h_flag) break;
+    return CF_OK;
+}
+
+static int cg_enqueue(cf_solid* s, const CgRun& run, uint32_t k0, uint32_t kend) {
+    const TetDevice& t = s->set->dev;
+    hipStream_t st = s->stream;
+    CgBuffers& b = s->cg;
+    const uint32_t n = 3 * s->N;
+    const int nnode = node_blocks(s->N);
+    for (uint32_t k = k0; k < kend; k++) {
+        const int32_t* skip = &b.st->done[k];
+        CF_TRY(launch_forces_skip(t, kModeDfx, b.d, s->cache->R, cache_mat(s->cache), skip, st));
+        CF_TRY(launch_gather_skip(t, kGatherMatvec, run.nt, nnode, skip, st));
+        CF_TRY(launch_cg_alpha(b.st, k, b.part_a, nnode, st));
+        CF_TRY(launch_cg_update_xr(n, k, b.st, s->y, b.r, b.d, b.q, b.part_b, st));
+        CF_TRY(launch_cg_update_d(n, k, b.st, b.d, b.r, run.nt.knc_mask, st));
     }
-    // results
+    return CF_OK;
+}
+
+// The recorded chunk starting at iteration k0 (recorded on first use).
+static int cg_graph(cf_solid* s, const CgRun& run, uint32_t k0, uint32_t kend, hipGraphExec_t* out) {
+    CgGraphs& g = s->graphs;
+    if (g.fM != run.nt.fM || g.fK != run.nt.fK || g.max_it != run.max_it || g.knc_mask != run.nt.knc_mask) {
+        g.clear();
+        g.fM = run.nt.fM; g.fK = run.nt.fK; g.max_it = run.max_it; g.knc_mask = run.nt.knc_mask;
+    }
+    const size_t c = k0 / 8;
+    if (g.chunk.size() <= c) g.chunk.resize(c + 1, nullptr);
+    if (!g.chunk[c]) {
+        CF_CHECK_HIP(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+        const int rc = cg_enqueue(s, run, k0, kend);
+        hipGraph_t graph = nullptr;
+        hipError_t e = hipStreamEndCapture(s->stream, &graph);
+        if (rc != CF_OK) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return rc;
+        }
+        if (e != hipSuccess) { set_error("CG graph capture failed: %s", hipGetErrorString(e)); return CF_ERR_HIP; }
+        e = hipGraphInstantiate(&g.chunk[c], graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (e != hipSuccess) {
+            g.chunk[c] = nullptr;
+            set_error("CG graph instantiate failed: %s", hipGetErrorString(e));
+            return CF_ERR_HIP;
+        }
+    }
+    *out = g.chunk[c];
+    return CF_OK;
+}
+
+static int cg_launch(cf_solid* s, CgRun* run, bool graphs = false) {
+    const uint32_t kend = std::min(run->max_it, run->k + 8);
+    if (graphs && kend > run->k) {
+        hipGraphExec_t g = nullptr;
+        CF_TRY(cg_graph(s, *run, run->k, kend, &g));
+        CF_CHECK_HIP(hipGraphLaunch(g, s->stream));
+    } else {
+        CF_TRY(cg_enqueue(s, *run, run->k, kend));
+    }
+    run->k = kend;
+    CgBuffers& b = s->cg;
+    CF_CHECK_HIP(hipMemcpyAsync(b.h_flag, &b.st->done[run->k], sizeof(int32_t), hipMemcpyDeviceToHost, s->stream));
+    return CF_OK;
+}
+
+static int cg_poll(cf_solid* s, CgRun* run) {
+    CF_CHECK_HIP(hipStreamSynchronize(s->stream));
+    if (*s->cg.h_flag || run->k >= run->max_it) run->done = true;
+    return CF_OK;
+}
+
+static int cg_finish(cf_solid* s, const CgRun& run, cf_cg_result* res) {
     static thread_local CgScalars hs;
-    CF_CHECK_HIP(hipMemcpyAsync(&hs, b.st, sizeof(CgScalars), hipMemcpyDeviceToHost, st));
-    CF_CHECK_HIP(hipStreamSynchronize(st));
+    CF_CHECK_HIP(hipMemcpyAsync(&hs, s->cg.st, sizeof(CgScalars), hipMemcpyDeviceToHost, s->stream));
+    CF_CHECK_HIP(hipStreamSynchronize(s->stream));
+    const uint32_t max_it = run.max_it;
     uint32_t j = 0;
     while (j <= max_it && !hs.done[j]) j++;
     if (j > max_it) j = max_it;
@@ -363,6 +450,16 @@ static int solid_cg(cf_solid* s, float fM, float fK, cf_cg_result* res) {
     res->rel_residual = (hs.delta0 > 0) ? float(std::sqrt(delta / hs.delta0)) : 0.f;
     res->abs_residual = float(std::sqrt(delta));
     return CF_OK;
+}
+
+static int solid_cg(cf_solid* s, float fM, float fK, cf_cg_result* res) {
+    CgRun run;
+    CF_TRY(cg_begin(s, fM, fK, &run));
+    while (!run.done) {
+        CF_TRY(cg_launch(s, &run));
+        CF_TRY(cg_poll(s, &run));
+    }
+    return cg_finish(s, run, res);
 }
 
 // ================================================================ FIE-dx
@@ -1009,6 +1106,7 @@ int cf_solid_destroy(cf_solid* s) {
     cg_free(s->cg);
     if (s->ev0) (void)hipEventDestroy(s->ev0);
     if (s->ev1) (void)hipEventDestroy(s->ev1);
+    s->graphs.clear();
     cf_cache_destroy(s->cache);
     cf_tetset_destroy(s->set);
     delete s;
@@ -1186,6 +1284,12 @@ int cf_multi_destroy(cf_multi* m) {
     return CF_OK;
 }
 
+int cf_multi_set_graphs(cf_multi* m, int use_graphs) {
+    if (!m) return CF_ERR_INVALID_ARG;
+    m->use_graphs = use_graphs != 0;
+    return CF_OK;
+}
+
 // SS_AggregateDSH_Basic::MultiDSH_Solid3D_Update (ds/ss/SS_AggregateDSH_Basic.cpp:229-300)
 // without contacts: fixed dt = min of the solids' fixed_dt, Newton passes = max
 // of their nr_max_iter, one accumulator with strict '>'; each fixed step runs
@@ -1206,6 +1310,7 @@ int cf_multi_update(cf_multi* m, float dt, uint32_t* num_fixed_steps, cf_step_st
     const float inv_dt = (dt == 0.0f) ? 0.0f : 1.0f / dt;
     const size_t n = m->solids.size();
     uint32_t steps = 0;
+    std::vector<CgRun> runs(n);
     while (m->acc_time > fixed_dt) {
         for (cf_solid* s : m->solids) {  // NR_Init (:2824-2836)
             CF_TRY(cf_begin_evaluation(s->cache, s->x0, s->x, s->v, s->total_time));
@@ -1214,7 +1319,12 @@ int cf_multi_update(cf_multi* m, float dt, uint32_t* num_fixed_steps, cf_step_st
         }
         uint32_t nr = 0;
         do {
-            for (cf_solid* s : m->solids) {  // NR_PreStep + NR_MidStep (:2857-2985)
+            // NR_PreStep + NR_MidStep (:2857-2985): every leaf's solve is
+            // enqueued on its own stream, eight iterations at a time (one
+            // recorded graph per chunk unless graphs are off), before the
+            // host waits on any of them
+            for (size_t i = 0; i < n; i++) {
+                cf_solid* s = m->solids[i];
                 const size_t bytes = 3 * (size_t)s->N * 4;
                 CF_CHECK_HIP(hipMemcpyAsync(s->xk, s->x, bytes, hipMemcpyDeviceToDevice, s->stream));
                 CF_CHECK_HIP(hipMemcpyAsync(s->vk, s->v, bytes, hipMemcpyDeviceToDevice, s->stream));
@@ -1222,7 +1332,21 @@ int cf_multi_update(cf_multi* m, float dt, uint32_t* num_fixed_steps, cf_step_st
                 s->ls_total = 0;
                 const float fM = inv_dt * inv_dt + inv_dt * s->rayleigh[0];
                 const float fK = 1.0f + inv_dt * s->rayleigh[1];
-                CF_TRY(solid_cg(s, fM, fK, &s->last_ls));
+                CF_TRY(cg_begin(s, fM, fK, &runs[i]));
+            }
+            for (size_t active = n; active > 0;) {
+                for (size_t i = 0; i < n; i++)
+                    if (!runs[i].done) CF_TRY(cg_launch(m->solids[i], &runs[i], m->use_graphs));
+                active = 0;
+                for (size_t i = 0; i < n; i++) {
+                    if (runs[i].done) continue;
+                    CF_TRY(cg_poll(m->solids[i], &runs[i]));
+                    active += runs[i].done ? 0 : 1;
+                }
+            }
+            for (size_t i = 0; i < n; i++) {
+                cf_solid* s = m->solids[i];
+                CF_TRY(cg_finish(s, runs[i], &s->last_ls));
                 s->ls_total = s->last_ls.iterations;
             }
             for (cf_solid* s : m->solids) {  // NR_PostStep (:2987-3037)

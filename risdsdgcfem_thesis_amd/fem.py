@@ -646,11 +646,12 @@ class MultiSolid3D:
     without contacts): shared fixed dt (the smallest), Newton passes (the largest
     nr_max_iter) and time accumulator. The solids stay owned by the caller."""
 
-    def __init__(self, solids):
+    def __init__(self, solids, use_graphs=True):
         self.solids = list(solids)
         hs = (C.c_void_p * len(self.solids))(*[sd.h for sd in self.solids])
         self.h = C.c_void_p()
         check(lib.cf_multi_create(hs, len(self.solids), C.byref(self.h)), "cf_multi_create")
+        check(lib.cf_multi_set_graphs(self.h, int(bool(use_graphs))), "cf_multi_set_graphs")
 
     def close(self):
         if getattr(self, "h", None):

@@ -113,6 +113,35 @@ def test_multi_vs_oracle(gpu, orc):
         g.close()
 
 
+def test_multi_interleaved_equals_one_leaf_universes(gpu):
+    """The universe step enqueues every leaf's CG on its own stream before it
+    waits on any, as recorded graphs: three leaves of different sizes and
+    iteration caps stepped together end bit-identical to each leaf stepped by a
+    universe of its own, and to the same step with the kernels launched
+    directly, also after the fixed nodes change."""
+    meshes = [beam(gpu), beam(gpu, (6, 4, 9), (0.05, 0.03, 0.16)), beam(gpu, (3, 3, 12), (0.02, 0.02, 0.3))]
+    params = [gpu_params(gpu, fixed_dt=1.0 / 60.0, nr_max_iter=3, cg_max_iter=m) for m in (200, 7, 40)]
+    runs = []
+    for together, graphs in ((True, True), (False, True), (True, False)):
+        leaves = [gpu.LeafSolid3D(p, t, q, knc=c) for (p, t, c), q in zip(meshes, params)]
+        multis = ([gpu.MultiSolid3D(leaves, use_graphs=graphs)] if together else
+                  [gpu.MultiSolid3D([s], use_graphs=graphs) for s in leaves])
+        for frame in range(3):
+            if frame == 2:  # new fixed nodes: the recorded chunks are re-recorded
+                for s, (p, _, c) in zip(leaves, meshes):
+                    s.set_knc(c[: len(c) // 2])
+            for m in multis:
+                m.update(DT)
+        runs.append([s.state() for s in leaves])
+        for m in multis:
+            m.close()
+        for s in leaves:
+            s.close()
+    for other in runs[1:]:
+        for (xa, va), (xb, vb) in zip(runs[0], other):
+            assert np.array_equal(xa, xb) and np.array_equal(va, vb)
+
+
 def test_batches_and_2d_refuse_fie_dx(gpu):
     pos, tets, knc = beam(gpu)
     b = gpu.Batch()
