@@ -268,6 +268,8 @@ def main():
     ap.add_argument("--tile-elems", type=int, default=256)
     ap.add_argument("--tile-threads", type=int, default=256)
     ap.add_argument("--tile-schedule", type=int, default=0)
+    ap.add_argument("--cg-workgroups", type=int, default=0,
+                    help="workgroups of the leaf CG iteration kernels (0 = library default)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--workload", default="all", choices=["all", "single", "batch3d", "batch2d", "serve", "fie"])
     ap.add_argument("--fie-steps", type=int, default=5)
@@ -334,6 +336,7 @@ def bench_single(cf, args):
     cf.set_tile_elems(args.tile_elems)
     cf.set_tile_threads(args.tile_threads)
     cf.set_tile_persistent(args.tile_schedule)
+    cf.set_cg_workgroups(args.cg_workgroups)
     counts = (65, 65, 257)
     half = (0.32, 0.32, 1.28)
     pos, tets = cf.mesh_box(half, counts)
@@ -387,6 +390,7 @@ def bench_single(cf, args):
         "kernels": {"k_tile_forces_ms": round(ms_tile, 5), "k_gather_ms": round(ms_gather, 5)},
         "b_alg_bytes": b_alg, "kx_gtets": round(T / (ms_kx * 1e-3) / 1e9, 2),
         "tile": {"elems": args.tile_elems, "threads": args.tile_threads, "schedule": args.tile_schedule},
+        "cg_workgroups": cf.get_cg_workgroups(),
     }
     cpu = None
     if not args.no_cpu_baseline:

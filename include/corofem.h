@@ -342,6 +342,12 @@ int cf_set_tile_threads(uint32_t threads);
    flight; 2: the pipelined kernel of 1 with one tile per workgroup. */
 int cf_set_tile_persistent(int persistent);
 int cf_get_tile_elems(uint32_t* tile_elems);
+/* Workgroups of the two CG iteration kernels of the leaf solvers (alpha with
+   x, r; beta and the stop test with d), each summing the iteration's FP64
+   partials itself (0 = default 1,024; at most 8,192). Results do not depend
+   on it. */
+int cf_set_cg_workgroups(uint32_t workgroups);
+int cf_get_cg_workgroups(uint32_t* workgroups);
 
 /* ---------------------------------------------------------------- batches
  * Many independent solids stepped together: each batched fixed step runs the

@@ -374,9 +374,8 @@ static int cg_enqueue(cf_solid* s, const CgRun& run, uint32_t k0, uint32_t kend)
         const int32_t* skip = &b.st->done[k];
         CF_TRY(launch_forces_skip(t, kModeDfx, b.d, s->cache->R, cache_mat(s->cache), skip, st));
         CF_TRY(launch_gather_skip(t, kGatherMatvec, run.nt, nnode, skip, st));
-        CF_TRY(launch_cg_alpha(b.st, k, b.part_a, nnode, st));
-        CF_TRY(launch_cg_update_xr(n, k, b.st, s->y, b.r, b.d, b.q, b.part_b, st));
-        CF_TRY(launch_cg_update_d(n, k, b.st, b.d, b.r, run.nt.knc_mask, st));
+        CF_TRY(launch_cg_step_xr(n, k, b.st, s->y, b.r, b.d, b.q, b.part_a, nnode, b.part_b, st));
+        CF_TRY(launch_cg_step_d(n, k, b.st, b.d, b.r, run.nt.knc_mask, b.part_b, st));
     }
     return CF_OK;
 }
@@ -1071,9 +1070,8 @@ int cf_cg_solve(cf_cg* c, cf_av_fn av, cf_constraints_fn cons, void* user, float
         av(user, b.q, b.d);
         if (cons) cons(user, b.q);
         CF_TRY(launch_cg_dot(n, b.d, b.q, b.part_a, nvec, st));
-        CF_TRY(launch_cg_alpha(b.st, k, b.part_a, nvec, st));
-        CF_TRY(launch_cg_update_xr(n, k, b.st, x, b.r, b.d, b.q, b.part_b, st));
-        CF_TRY(launch_cg_update_d(n, k, b.st, b.d, b.r, nullptr, st));
+        CF_TRY(launch_cg_step_xr(n, k, b.st, x, b.r, b.d, b.q, b.part_a, nvec, b.part_b, st));
+        CF_TRY(launch_cg_step_d(n, k, b.st, b.d, b.r, nullptr, b.part_b, st));
         if (cons) cons(user, b.d);
         k++;
         CF_CHECK_HIP(hipMemcpyAsync(b.h_flag, &b.st->done[k], 4, hipMemcpyDeviceToHost, st));
@@ -1539,6 +1537,20 @@ int cf_set_tile_persistent(int persistent) {
 int cf_get_tile_elems(uint32_t* te) {
     if (!te) return CF_ERR_INVALID_ARG;
     *te = g_tile_elems ? g_tile_elems : default_tile_elems();
+    return CF_OK;
+}
+
+int cf_set_cg_workgroups(uint32_t n) {
+    if (set_cg_workgroups(n) != CF_OK) {
+        set_error("CG workgroups must be at most %d", kMaxPartials);
+        return CF_ERR_INVALID_ARG;
+    }
+    return CF_OK;
+}
+
+int cf_get_cg_workgroups(uint32_t* n) {
+    if (!n) return CF_ERR_INVALID_ARG;
+    *n = get_cg_workgroups();
     return CF_OK;
 }
 

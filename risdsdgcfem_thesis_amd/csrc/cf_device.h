@@ -353,13 +353,11 @@ __device__ inline double block_sum(double v, double* sh) {
     for (int i = 0; i < kBlock / 64; i++) r += sh[i];
     return r;
 }
-// v += part[i], i = first, first + stride, ... < n, in that order. The loads
+// v += part[j], j = i, i + stride, ... < n, in that order. The loads
 // of eight terms are issued before their adds: the partials come from blocks
 // on every XCD, so most are misses in this XCD's L2, and one latency per
 // eight terms replaces one per term. The additions keep their order.
-__device__ inline double strided_sum(const double* part, int first, int n, int stride) {
-    double v = 0.0;
-    int i = first;
+__device__ inline double strided_sum_from(double v, const double* part, int i, int n, int stride) {
     for (; i + 7 * stride < n; i += 8 * stride) {
         double t[8];
 #pragma unroll
@@ -370,9 +368,31 @@ __device__ inline double strided_sum(const double* part, int first, int n, int s
     for (; i < n; i += stride) v += part[i];
     return v;
 }
+__device__ inline double strided_sum(const double* part, int first, int n, int stride) {
+    return strided_sum_from(0.0, part, first, n, stride);
+}
+// The same sum with the terms of the first 24 strides all loaded before the
+// first add (one latency for up to 24 terms; more registers).
+__device__ inline double strided_sum_wide(const double* part, int first, int n, int stride) {
+    constexpr int M = 24;
+    double t[M];
+#pragma unroll
+    for (int u = 0; u < M; u++) {
+        const int j = first + u * stride;
+        t[u] = j < n ? part[j] : 0.0;
+    }
+    double v = 0.0;
+#pragma unroll
+    for (int u = 0; u < M; u++)
+        if (first + u * stride < n) v += t[u];
+    return strided_sum_from(v, part, first + M * stride, n, stride);
+}
 // Deterministic sum of n partials by one block (fixed order and tree).
 __device__ inline double reduce_partials(const double* part, int n, double* sh) {
     return block_sum(strided_sum(part, threadIdx.x, n, kBlock), sh);
+}
+__device__ inline double reduce_partials_wide(const double* part, int n, double* sh) {
+    return block_sum(strided_sum_wide(part, threadIdx.x, n, kBlock), sh);
 }
 // reduce_partials computed by one wavefront: lane l plays the threads l,
 // l+64, l+128, l+192 of the block and the four wave sums are added in the
@@ -1448,6 +1468,42 @@ __device__ inline void cg_init_block(uint32_t n, const float* __restrict__ q, co
     if (threadIdx.x == 0) rr_part[blk] = s;
 }
 
+// One float4 of the CG vector updates: the operands, then the arithmetic
+// (shared by the block helpers below and the CG step kernels, which may load
+// the operands early).
+struct Xr4 { float4 x, r, d, q; };
+__device__ inline Xr4 load_xr4(const float* x, const float* r, const float* d, const float* q, uint32_t i) {
+    Xr4 v;
+    v.d = reinterpret_cast<const float4*>(d)[i];
+    v.q = reinterpret_cast<const float4*>(q)[i];
+    v.x = reinterpret_cast<const float4*>(x)[i];
+    v.r = reinterpret_cast<const float4*>(r)[i];
+    return v;
+}
+// x += alpha d ; r -= alpha q for one float4, stored; returns its r.r terms
+__device__ inline double apply_xr4(Xr4 v, float alpha, float* x, float* r, uint32_t i) {
+#pragma clang fp contract(off)
+    v.x.x = v.x.x + v.d.x * alpha; v.x.y = v.x.y + v.d.y * alpha;
+    v.x.z = v.x.z + v.d.z * alpha; v.x.w = v.x.w + v.d.w * alpha;
+    v.r.x = v.r.x - v.q.x * alpha; v.r.y = v.r.y - v.q.y * alpha;
+    v.r.z = v.r.z - v.q.z * alpha; v.r.w = v.r.w - v.q.w * alpha;
+    reinterpret_cast<float4*>(x)[i] = v.x;
+    reinterpret_cast<float4*>(r)[i] = v.r;
+    return (double)v.r.x * v.r.x + (double)v.r.y * v.r.y + (double)v.r.z * v.r.z + (double)v.r.w * v.r.w;
+}
+// the n % 4 floats after the float4s, for the threads gid of a stride
+__device__ inline double tail_xr(uint32_t n, float alpha, float* x, float* r, const float* d, const float* q,
+                                 uint32_t gid, uint32_t stride, double acc) {
+#pragma clang fp contract(off)
+    for (uint32_t i = 4 * (n / 4) + gid; i < n; i += stride) {
+        x[i] = x[i] + d[i] * alpha;
+        const float ri = r[i] - q[i] * alpha;
+        r[i] = ri;
+        acc += (double)ri * ri;
+    }
+    return acc;
+}
+
 // x += alpha d ; r -= alpha q ; per-block r.r (ILSS_CG.h:100-117)
 template <bool V4>
 __device__ inline void update_xr_block(uint32_t n, float alpha, float* __restrict__ x, float* __restrict__ r,
@@ -1456,67 +1512,63 @@ __device__ inline void update_xr_block(uint32_t n, float alpha, float* __restric
 #pragma clang fp contract(off)
     double acc = 0.0;
     const uint32_t gid = blk * kBlock + threadIdx.x, stride = nblk * kBlock;
-    uint32_t tail0 = 0;
     if (V4) {
-        const uint32_t n4 = n / 4;
-        tail0 = 4 * n4;
-        float4* x4 = reinterpret_cast<float4*>(x);
-        float4* r4 = reinterpret_cast<float4*>(r);
-        const float4* d4 = reinterpret_cast<const float4*>(d);
-        const float4* q4 = reinterpret_cast<const float4*>(q);
-        for (uint32_t i = gid; i < n4; i += stride) {
-            const float4 di = d4[i], qi = q4[i];
-            float4 xi = x4[i], ri = r4[i];
-            xi.x = xi.x + di.x * alpha; xi.y = xi.y + di.y * alpha;
-            xi.z = xi.z + di.z * alpha; xi.w = xi.w + di.w * alpha;
-            ri.x = ri.x - qi.x * alpha; ri.y = ri.y - qi.y * alpha;
-            ri.z = ri.z - qi.z * alpha; ri.w = ri.w - qi.w * alpha;
-            x4[i] = xi;
-            r4[i] = ri;
-            acc += (double)ri.x * ri.x + (double)ri.y * ri.y + (double)ri.z * ri.z + (double)ri.w * ri.w;
+        for (uint32_t i = gid; i < n / 4; i += stride) acc += apply_xr4(load_xr4(x, r, d, q, i), alpha, x, r, i);
+        acc = tail_xr(n, alpha, x, r, d, q, gid, stride, acc);
+    } else {
+        for (uint32_t i = gid; i < n; i += stride) {
+            x[i] = x[i] + d[i] * alpha;
+            const float ri = r[i] - q[i] * alpha;
+            r[i] = ri;
+            acc += (double)ri * ri;
         }
-    }
-    for (uint32_t i = tail0 + gid; i < n; i += stride) {
-        x[i] = x[i] + d[i] * alpha;
-        const float ri = r[i] - q[i] * alpha;
-        r[i] = ri;
-        acc += (double)ri * ri;
     }
     const double s = block_sum(acc, sh);
     if (threadIdx.x == 0) rr_part[blk] = s;
+}
+
+struct D4 { float4 d, r; };
+__device__ inline D4 load_d4(const float* d, const float* r, uint32_t i) {
+    D4 v;
+    v.r = reinterpret_cast<const float4*>(r)[i];
+    v.d = reinterpret_cast<const float4*>(d)[i];
+    return v;
+}
+// d = S(beta d + r) for one float4, stored
+__device__ inline void apply_d4(D4 v, float beta, float* d, const uint8_t* mask, uint32_t i) {
+#pragma clang fp contract(off)
+    float4 di = v.d;
+    di.x = beta * di.x + v.r.x; di.y = beta * di.y + v.r.y;
+    di.z = beta * di.z + v.r.z; di.w = beta * di.w + v.r.w;
+    if (mask) {
+        const uint32_t e = 4 * i;
+        if (mask[e / 3]) di.x = 0.f;
+        if (mask[(e + 1) / 3]) di.y = 0.f;
+        if (mask[(e + 2) / 3]) di.z = 0.f;
+        if (mask[(e + 3) / 3]) di.w = 0.f;
+    }
+    reinterpret_cast<float4*>(d)[i] = di;
+}
+__device__ inline void tail_d(uint32_t n, float beta, float* d, const float* r, const uint8_t* mask, uint32_t first,
+                              uint32_t stride) {
+#pragma clang fp contract(off)
+    for (uint32_t i = first; i < n; i += stride) {
+        float di = beta * d[i] + r[i];
+        if (mask && mask[i / 3]) di = 0.f;
+        d[i] = di;
+    }
 }
 
 // d = S(beta d + r)
 template <bool V4>
 __device__ inline void update_d_block(uint32_t n, float beta, float* __restrict__ d, const float* __restrict__ r,
                                       const uint8_t* __restrict__ mask, uint32_t blk, uint32_t nblk) {
-#pragma clang fp contract(off)
     const uint32_t gid = blk * kBlock + threadIdx.x, stride = nblk * kBlock;
-    uint32_t tail0 = 0;
     if (V4) {
-        const uint32_t n4 = n / 4;
-        tail0 = 4 * n4;
-        float4* d4 = reinterpret_cast<float4*>(d);
-        const float4* r4 = reinterpret_cast<const float4*>(r);
-        for (uint32_t i = gid; i < n4; i += stride) {
-            const float4 ri = r4[i];
-            float4 di = d4[i];
-            di.x = beta * di.x + ri.x; di.y = beta * di.y + ri.y;
-            di.z = beta * di.z + ri.z; di.w = beta * di.w + ri.w;
-            if (mask) {
-                const uint32_t e = 4 * i;
-                if (mask[e / 3]) di.x = 0.f;
-                if (mask[(e + 1) / 3]) di.y = 0.f;
-                if (mask[(e + 2) / 3]) di.z = 0.f;
-                if (mask[(e + 3) / 3]) di.w = 0.f;
-            }
-            d4[i] = di;
-        }
-    }
-    for (uint32_t i = tail0 + gid; i < n; i += stride) {
-        float di = beta * d[i] + r[i];
-        if (mask && mask[i / 3]) di = 0.f;
-        d[i] = di;
+        for (uint32_t i = gid; i < n / 4; i += stride) apply_d4(load_d4(d, r, i), beta, d, mask, i);
+        tail_d(n, beta, d, r, mask, 4 * (n / 4) + gid, stride);
+    } else {
+        tail_d(n, beta, d, r, mask, gid, stride);
     }
 }
 

@@ -178,12 +178,15 @@ int launch_cg_init(uint32_t n, const float* q, const float* b, float* r, float* 
 int launch_cg_start(CgScalars* st, const double* rr_part, int nblk_prev, float rel, float abs_eps,
                     uint32_t max_iter, hipStream_t s);
 int launch_cg_dot(uint32_t n, const float* a, const float* b, double* part, int nblk, hipStream_t s);
-int launch_cg_alpha(CgScalars* st, uint32_t k, const double* dq_part, int nblk_dq, hipStream_t s);
-// x, r update and per-block r.r partials, then the one-block delta/beta/stop kernel
-int launch_cg_update_xr(uint32_t n, uint32_t k, CgScalars* st, float* x, float* r, const float* d, const float* q,
-                        double* rr_part, hipStream_t s);
-int launch_cg_update_d(uint32_t n, uint32_t k, CgScalars* st, float* d, const float* r, const uint8_t* mask,
-                       hipStream_t s);
+// One CG iteration after q = S A d (alpha, x and r, then beta, the stop
+// test and d); r.r partials per vec_blocks(n) chunk.
+int launch_cg_step_xr(uint32_t n, uint32_t k, CgScalars* st, float* x, float* r, const float* d, const float* q,
+                      const double* dq_part, int ndq, double* rr_part, hipStream_t s);
+int launch_cg_step_d(uint32_t n, uint32_t k, CgScalars* st, float* d, const float* r, const uint8_t* mask,
+                     const double* rr_part, hipStream_t s);
+// Workgroups of the two CG step kernels (0 = default 1024; at most kMaxPartials).
+int set_cg_workgroups(uint32_t n);
+uint32_t get_cg_workgroups();
 // FIE-dx: x_k += y ; v_k = (x_k - x) / dt ; per-block y.y into part (nblk blocks)
 int launch_nr_update(uint32_t n, float* xk, float* vk, const float* x, const float* y, float inv_dt, double* part,
                      int nblk, hipStream_t s);

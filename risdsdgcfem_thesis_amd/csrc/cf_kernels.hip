@@ -271,63 +271,140 @@ __global__ __launch_bounds__(kBlock) void k_dot(uint32_t n, const float* __restr
     if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
 
-// alpha for iteration k from per-block d.q partials (generic solver path)
-__global__ __launch_bounds__(kBlock) void k_cg_alpha(CgScalars* st, uint32_t k, const double* part, int n) {
+// One CG iteration's scalar and vector work after q = S A d, in two kernels
+// (ILSS_CG.h:86-133). Every workgroup sums the same FP64 partials in the same
+// fixed order as a one-block reduction (so all agree on alpha / beta to the
+// bit), workgroup 0 records the scalars and stop tests, and the vector work is
+// split into nchunk chunks laid out as a grid of nchunk blocks would be: chunk
+// c's r.r partial is the same value whichever workgroup computes it.
+
+// Chunks per workgroup whose operands are loaded before the partial sum (one:
+// with more, the held registers and the chunks' serial barriers cost more
+// than the latency they hide on the 5.24M-tet mesh's 3.26M-float vectors),
+// and the default grid (1,024 workgroups, four per CU; swept 256-8,192).
+constexpr int kCgAhead = 1;
+constexpr uint32_t kCgWorkgroups = 1024;
+
+// Chunk c (one float4 per thread, its operands v already loaded when the
+// thread has one, plus the n % 4 tail in chunk 0), as update_xr_block(c,
+// nchunk) / update_d_block(c, nchunk) compute it.
+__device__ inline void xr_chunk(uint32_t n, uint32_t c, uint32_t nchunk, float alpha, float* x, float* r,
+                                const float* d, const float* q, double* rr_part, const Xr4& v, double* sh) {
+    const uint32_t i = c * kBlock + threadIdx.x;
+    double acc = 0.0;
+    if (i < n / 4) acc += apply_xr4(v, alpha, x, r, i);
+    acc = tail_xr(n, alpha, x, r, d, q, i, nchunk * kBlock, acc);
+    const double s = block_sum(acc, sh);
+    if (threadIdx.x == 0) rr_part[c] = s;
+}
+__device__ inline void d_chunk(uint32_t n, uint32_t c, uint32_t nchunk, float beta, float* d, const float* r,
+                               const uint8_t* mask, const D4& v) {
+    const uint32_t i = c * kBlock + threadIdx.x;
+    if (i < n / 4) apply_d4(v, beta, d, mask, i);
+    tail_d(n, beta, d, r, mask, 4 * (n / 4) + i, nchunk * kBlock);
+}
+
+// alpha = delta_k / d.q (break down if |d.q| < abs^2) ; x += alpha d ;
+// r -= alpha q ; per-chunk partials of r.r
+template <bool V4>
+__global__ __launch_bounds__(kBlock) void k_cg_step_xr(uint32_t n, uint32_t k, CgScalars* st,
+                                                       float* __restrict__ x, float* __restrict__ r,
+                                                       const float* __restrict__ d, const float* __restrict__ q,
+                                                       const double* dq_part, int ndq, double* rr_part,
+                                                       uint32_t nchunk) {
     __shared__ double sh[kBlock / 64];
     if (st->done[k]) return;
-    const double dq = reduce_partials(part, n, sh);
-    if (threadIdx.x == 0) {
-        st->dq[k] = dq;
-        if (fabs(dq) < st->abs_sq) {
+    const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+    // When every chunk holds at most one float4 per thread, the float4s of
+    // this workgroup's first kCgAhead chunks are loaded before the sum, whose
+    // latency they then overlap.
+    const uint32_t n4 = n / 4;
+    const bool early = V4 && n4 <= nchunk * kBlock;
+    Xr4 v[kCgAhead];
+#pragma unroll
+    for (int j = 0; j < kCgAhead; j++) {
+        const uint32_t i = (blockIdx.x + j * gridDim.x) * kBlock + threadIdx.x;
+        v[j] = (early && i < n4) ? load_xr4(x, r, d, q, i) : Xr4{};
+    }
+    const double dq = reduce_partials_wide(dq_part, ndq, sh);
+    if (fabs(dq) < st->abs_sq) {
+        if (lead) {
+            st->dq[k] = dq;
             st->reason[k + 1] = CF_CG_BREAKDOWN;
             st->done[k + 1] = 2;
-        } else {
-            st->alpha[k] = (float)(st->delta[k] / dq);
         }
+        return;
+    }
+    const float alpha = (float)(st->delta[k] / dq);
+    if (lead) {
+        st->dq[k] = dq;
+        st->alpha[k] = alpha;
+    }
+    if (!early) {
+        for (uint32_t c = blockIdx.x; c < nchunk; c += gridDim.x)
+            update_xr_block<V4>(n, alpha, x, r, d, q, rr_part, c, nchunk, sh);
+        return;
+    }
+#pragma unroll
+    for (int j = 0; j < kCgAhead; j++) {
+        const uint32_t c = blockIdx.x + j * gridDim.x;
+        if (c < nchunk) xr_chunk(n, c, nchunk, alpha, x, r, d, q, rr_part, v[j], sh);
+    }
+    for (uint32_t c = blockIdx.x + kCgAhead * gridDim.x; c < nchunk; c += gridDim.x) {
+        const uint32_t i = c * kBlock + threadIdx.x;
+        xr_chunk(n, c, nchunk, alpha, x, r, d, q, rr_part, i < n4 ? load_xr4(x, r, d, q, i) : Xr4{}, sh);
     }
 }
 
-// x += alpha d ; r -= alpha q ; per-block partials of r^T r (ILSS_CG.h:100-117)
+// delta_{k+1} = r.r ; beta ; stop test ; d = S(beta d + r)
 template <bool V4>
-__global__ __launch_bounds__(kBlock) void k_cg_update_xr(uint32_t n, uint32_t k, CgScalars* st,
-                                                         float* __restrict__ x, float* __restrict__ r,
-                                                         const float* __restrict__ d, const float* __restrict__ q,
-                                                         double* rr_part) {
+__global__ __launch_bounds__(kBlock) void k_cg_step_d(uint32_t n, uint32_t k, CgScalars* st,
+                                                      float* __restrict__ d, const float* __restrict__ r,
+                                                      const uint8_t* __restrict__ mask, const double* rr_part,
+                                                      uint32_t nchunk) {
     __shared__ double sh[kBlock / 64];
-    if (st->done[k] || st->reason[k + 1] == CF_CG_BREAKDOWN) return;
-    update_xr_block<V4>(n, st->alpha[k], x, r, d, q, rr_part, blockIdx.x, gridDim.x, sh);
-}
-
-// delta_{k+1} = r^T r from per-block partials ; beta ; stop test (ILSS_CG.h:86-88,115-133)
-__global__ __launch_bounds__(kBlock) void k_cg_beta(CgScalars* st, uint32_t k, const double* part, int n) {
-    __shared__ double sh[kBlock / 64];
-    if (st->done[k] || st->reason[k + 1] == CF_CG_BREAKDOWN) return;
-    const double dn = reduce_partials(part, n, sh);
-    if (threadIdx.x == 0) {
-        st->delta[k + 1] = dn;
-        st->beta[k] = (float)(dn / st->delta[k]);
-        const uint32_t k1 = k + 1;
-        int reason = 0;
-        if (!(dn > st->abs_sq)) reason = CF_CG_CONVERGED_ABS;
-        else if (!(dn > st->target)) reason = CF_CG_CONVERGED_REL;
-        else if (!(k1 < st->max_iter)) reason = CF_CG_MAX_ITER;
+    const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+    if (st->done[k]) {
+        // keep every later iteration switched off once the solve has ended
+        if (lead) st->done[k + 1] = 1;
+        return;
+    }
+    if (st->reason[k + 1] == CF_CG_BREAKDOWN) return;  // done[k + 1] is set
+    const uint32_t n4 = n / 4;
+    const bool early = V4 && n4 <= nchunk * kBlock;  // as in k_cg_step_xr
+    D4 v[kCgAhead];
+#pragma unroll
+    for (int j = 0; j < kCgAhead; j++) {
+        const uint32_t i = (blockIdx.x + j * gridDim.x) * kBlock + threadIdx.x;
+        v[j] = (early && i < n4) ? load_d4(d, r, i) : D4{};
+    }
+    const double dn = reduce_partials_wide(rr_part, (int)nchunk, sh);
+    const uint32_t k1 = k + 1;
+    int reason = 0;
+    if (!(dn > st->abs_sq)) reason = CF_CG_CONVERGED_ABS;
+    else if (!(dn > st->target)) reason = CF_CG_CONVERGED_REL;
+    else if (!(k1 < st->max_iter)) reason = CF_CG_MAX_ITER;
+    const float beta = (float)(dn / st->delta[k]);
+    if (lead) {
+        st->delta[k1] = dn;
+        st->beta[k] = beta;
         st->reason[k1] = reason;
         st->done[k1] = reason != 0;
     }
-}
-
-// d = S(beta d + r)
-template <bool V4>
-__global__ __launch_bounds__(kBlock) void k_cg_update_d(uint32_t n, uint32_t k, CgScalars* st,
-                                                        float* __restrict__ d, const float* __restrict__ r,
-                                                        const uint8_t* __restrict__ mask) {
-    if (st->done[k]) {
-        // keep every later iteration switched off once the solve has ended
-        if (blockIdx.x == 0 && threadIdx.x == 0) st->done[k + 1] = 1;
+    if (reason) return;
+    if (!early) {
+        for (uint32_t c = blockIdx.x; c < nchunk; c += gridDim.x) update_d_block<V4>(n, beta, d, r, mask, c, nchunk);
         return;
     }
-    if (st->done[k + 1]) return;  // converged or broke down at iteration k
-    update_d_block<V4>(n, st->beta[k], d, r, mask, blockIdx.x, gridDim.x);
+#pragma unroll
+    for (int j = 0; j < kCgAhead; j++) {
+        const uint32_t c = blockIdx.x + j * gridDim.x;
+        if (c < nchunk) d_chunk(n, c, nchunk, beta, d, r, mask, v[j]);
+    }
+    for (uint32_t c = blockIdx.x + kCgAhead * gridDim.x; c < nchunk; c += gridDim.x) {
+        const uint32_t i = c * kBlock + threadIdx.x;
+        d_chunk(n, c, nchunk, beta, d, r, mask, i < n4 ? load_d4(d, r, i) : D4{});
+    }
 }
 
 __global__ __launch_bounds__(kBlock) void k_integrate(uint32_t n, float* __restrict__ x, float* __restrict__ v,
@@ -542,30 +619,41 @@ int launch_cg_dot(uint32_t n, const float* a, const float* b, double* part, int 
     return check_launch();
 }
 
-int launch_cg_alpha(CgScalars* st, uint32_t k, const double* dq_part, int nblk_dq, hipStream_t s) {
-    hipLaunchKernelGGL(k_cg_alpha, dim3(1), dim3(kBlock), 0, s, st, k, dq_part, nblk_dq);
-    return check_launch();
+static uint32_t g_cg_workgroups = 0;  // 0: kCgWorkgroups
+
+int set_cg_workgroups(uint32_t n) {
+    if (n > (uint32_t)kMaxPartials) return CF_ERR_INVALID_ARG;
+    g_cg_workgroups = n;
+    return CF_OK;
+}
+uint32_t get_cg_workgroups() { return g_cg_workgroups ? g_cg_workgroups : kCgWorkgroups; }
+
+static dim3 cg_step_grid(uint32_t nchunk) {
+    const uint32_t g = g_cg_workgroups ? g_cg_workgroups : kCgWorkgroups;
+    return dim3(std::max(1u, std::min(nchunk, g)));
 }
 
-int launch_cg_update_xr(uint32_t n, uint32_t k, CgScalars* st, float* x, float* r, const float* d, const float* q,
-                        double* rr_part, hipStream_t s) {
-    const int nb = vec_blocks(n);
+int launch_cg_step_xr(uint32_t n, uint32_t k, CgScalars* st, float* x, float* r, const float* d, const float* q,
+                      const double* dq_part, int ndq, double* rr_part, hipStream_t s) {
+    const uint32_t nchunk = (uint32_t)vec_blocks(n);
+    const dim3 grid = cg_step_grid(nchunk);
     if (aligned16(x) && aligned16(r) && aligned16(d) && aligned16(q))
-        hipLaunchKernelGGL(k_cg_update_xr<true>, dim3(nb), dim3(kBlock), 0, s, n, k, st, x, r, d, q, rr_part);
+        hipLaunchKernelGGL(k_cg_step_xr<true>, grid, dim3(kBlock), 0, s, n, k, st, x, r, d, q, dq_part, ndq, rr_part,
+                           nchunk);
     else
-        hipLaunchKernelGGL(k_cg_update_xr<false>, dim3(nb), dim3(kBlock), 0, s, n, k, st, x, r, d, q, rr_part);
-    CF_TRY(check_launch());
-    hipLaunchKernelGGL(k_cg_beta, dim3(1), dim3(kBlock), 0, s, st, k, rr_part, nb);
+        hipLaunchKernelGGL(k_cg_step_xr<false>, grid, dim3(kBlock), 0, s, n, k, st, x, r, d, q, dq_part, ndq, rr_part,
+                           nchunk);
     return check_launch();
 }
 
-int launch_cg_update_d(uint32_t n, uint32_t k, CgScalars* st, float* d, const float* r, const uint8_t* mask,
-                       hipStream_t s) {
-    const int nb = vec_blocks(n);
+int launch_cg_step_d(uint32_t n, uint32_t k, CgScalars* st, float* d, const float* r, const uint8_t* mask,
+                     const double* rr_part, hipStream_t s) {
+    const uint32_t nchunk = (uint32_t)vec_blocks(n);
+    const dim3 grid = cg_step_grid(nchunk);
     if (aligned16(d) && aligned16(r))
-        hipLaunchKernelGGL(k_cg_update_d<true>, dim3(nb), dim3(kBlock), 0, s, n, k, st, d, r, mask);
+        hipLaunchKernelGGL(k_cg_step_d<true>, grid, dim3(kBlock), 0, s, n, k, st, d, r, mask, rr_part, nchunk);
     else
-        hipLaunchKernelGGL(k_cg_update_d<false>, dim3(nb), dim3(kBlock), 0, s, n, k, st, d, r, mask);
+        hipLaunchKernelGGL(k_cg_step_d<false>, grid, dim3(kBlock), 0, s, n, k, st, d, r, mask, rr_part, nchunk);
     return check_launch();
 }
 

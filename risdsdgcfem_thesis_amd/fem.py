@@ -231,6 +231,16 @@ def get_tile_elems():
     return v.value
 
 
+def set_cg_workgroups(n):
+    check(lib.cf_set_cg_workgroups(int(n)), "cf_set_cg_workgroups")
+
+
+def get_cg_workgroups():
+    v = C.c_uint32()
+    check(lib.cf_get_cg_workgroups(C.byref(v)), "cf_get_cg_workgroups")
+    return v.value
+
+
 class TetrahedronElementSet3:
     """Element set + IForce3 cache. Vectors passed to the operator methods are
     DeviceArray objects of 3N floats; outputs accumulate (+=)."""
