@@ -52,3 +52,30 @@ def beam_config1(mod):
     pos, tets = mod.mesh_box((0.05, 0.05, 0.2), (11, 11, 41))
     knc = np.nonzero(np.isclose(pos[2::3], -0.2))[0].astype(np.uint32)
     return pos, tets, knc
+
+
+def special_elements(rng, T, rm):
+    """T disjoint elements, every other one deformed so that no dR term
+    applies: for MSVD an inverted F with singular values (2, 1, 1) (S = R^T F
+    then has the eigenvalues (2, 1, -1), so G = S - tr(S) I has the eigenvalue
+    0); for DAPD det F = 0.05, below the degeneration threshold 0.1. The
+    others: F = I + U(-0.3, 0.3), where the dR terms are there."""
+    A = np.zeros((T, 4, 3))
+    B = np.zeros((T, 4, 3))
+    special = np.arange(T) % 2 == 0
+    for e in range(T):
+        while True:
+            a = rng.normal(size=(4, 3))
+            Dm = (a[1:] - a[0]).T
+            if np.linalg.det(Dm) > 0.3 and np.linalg.cond(Dm) < 6:
+                break
+        F = np.eye(3) + rng.uniform(-0.3, 0.3, (3, 3))
+        if special[e] or np.linalg.det(F) < 0.3:
+            Q, P = (np.linalg.qr(rng.normal(size=(3, 3)))[0] for _ in range(2))
+            Q *= np.sign(np.linalg.det(Q))
+            P *= np.sign(np.linalg.det(P))
+            sv = [2.0, 1.0, -1.0] if rm == "RM_MSVD" else [1.0, 1.0, 0.05]
+            F = Q @ np.diag(sv) @ P.T if special[e] else Q @ P.T
+        A[e], B[e] = a, (a - a[0]) @ F.T + a[0] + rng.normal(size=3)
+    return A.astype(np.float32).reshape(-1), B.astype(np.float32).reshape(-1), special
+

@@ -44,6 +44,68 @@ def test_dR_is_derivative_of_polar_rotation(orc):
     assert checked > 300
 
 
+def random_rotation(rng):
+    q, r = np.linalg.qr(rng.normal(size=(3, 3)))
+    q = q * np.sign(np.diag(r))
+    return q if np.linalg.det(q) > 0 else -q
+
+
+def test_dR_singular_branch(orc):
+    """The branch |det G| <= 1e-4 -> no dR. With R given and S = R^T F =
+    P diag(s) P^T symmetric, G = S - tr(S) I has the eigenvalues -(s_j + s_k),
+    so det G = -(s0 + s1)(s1 + s2)(s0 + s2): F is built for a chosen det G
+    (s = (a, b, -b + e), e solving (a + b) e (a - b + e) = -det G). Below the
+    threshold the oracle reports no dR and leaves it zero; above it, dR is
+    finite and R^T dR is skew (dR = R skew(.), a rotation's derivative)."""
+    rng = np.random.default_rng(12)
+    L = orc.lib()
+    for _ in range(40):
+        Q, P = random_rotation(rng), random_rotation(rng)
+        a, b = rng.uniform(1.2, 2.0), rng.uniform(0.3, 0.6)
+        dF = rng.uniform(-1, 1, (3, 3))
+        for det_g, expect in ((0.0, 0), (5e-5, 0), (-5e-5, 0), (5e-4, 1), (-5e-4, 1), (1e-2, 1)):
+            e = 0.5 * (-(a - b) + np.sqrt((a - b) ** 2 - 4 * det_g / (a + b)))
+            sv = np.array([a, b, -b + e])
+            assert abs(-(sv[0] + sv[1]) * (sv[1] + sv[2]) * (sv[0] + sv[2]) - det_g) < 1e-12
+            F = Q @ P @ np.diag(sv) @ P.T
+            out = np.full(9, np.nan, np.float32)
+            ok = L.orc_dR(orc.fp(np.ascontiguousarray(F.astype(np.float32).ravel())),
+                          orc.fp(np.ascontiguousarray(dF.astype(np.float32).ravel())),
+                          orc.fp(np.ascontiguousarray(Q.astype(np.float32).ravel())), orc.fp(out))
+            assert ok == expect, (det_g, ok)
+            if not expect:
+                assert np.all(out == 0.0)
+                continue
+            dR = out.reshape(3, 3).astype(np.float64)
+            assert np.all(np.isfinite(dR)) and np.abs(dR).max() > 0
+            K = Q.T @ dR
+            assert np.abs(K + K.T).max() <= 1e-5 * np.abs(K).max()
+
+
+@pytest.mark.parametrize("rm", ["RM_MSVD", "RM_DAPD"])
+def test_exact_df_x_without_dR_branches(orc, rm):
+    """The oracle's K*x with DM = Exact drops exactly the dR terms where the
+    reference computes none: under MSVD on inverted elements with singular
+    values (2, 1, 1) (G singular, see test_dR_singular_branch), under DAPD on
+    elements with det F = 0.05 <= 0.1; there it equals the truncated K*x bit
+    for bit, and on the regular elements beside them it does not."""
+    from conftest import special_elements
+    rng = np.random.default_rng(14)
+    T = 256
+    A, B, special = special_elements(rng, T, rm)
+    tets = np.arange(4 * T, dtype=np.uint32)
+    v = rng.standard_normal(A.size).astype(np.float32)
+    o = orc.TetSet(A, tets, E, NU)
+    o.begin_evaluation(B, rm=getattr(orc, rm))
+    o.set_dm(orc.DM_EXACT)
+    ex = o.df_x(v).reshape(T, 12)
+    o.set_dm(orc.DM_TRUNCATED)
+    tr = o.df_x(v).reshape(T, 12)
+    assert np.array_equal(ex[special], tr[special])
+    scale = np.abs(tr).max(axis=1)
+    assert np.mean(np.abs(ex - tr)[~special].max(axis=1) > 1e-3 * scale[~special]) > 0.95
+
+
 E, NU = 1e5, 0.3
 MU = 0.5 * E / (1 + NU)
 LAM = E * NU / ((1 + NU) * (1 - 2 * NU))
