@@ -118,7 +118,7 @@ def test_multi_interleaved_equals_one_leaf_universes(gpu):
     waits on any, as recorded graphs: three leaves of different sizes and
     iteration caps stepped together end bit-identical to each leaf stepped by a
     universe of its own, and to the same step with the kernels launched
-    directly, also after the fixed nodes change."""
+    directly, also after the frame dt and the fixed nodes change."""
     meshes = [beam(gpu), beam(gpu, (6, 4, 9), (0.05, 0.03, 0.16)), beam(gpu, (3, 3, 12), (0.02, 0.02, 0.3))]
     params = [gpu_params(gpu, fixed_dt=1.0 / 60.0, nr_max_iter=3, cg_max_iter=m) for m in (200, 7, 40)]
     runs = []
@@ -126,12 +126,14 @@ def test_multi_interleaved_equals_one_leaf_universes(gpu):
         leaves = [gpu.LeafSolid3D(p, t, q, knc=c) for (p, t, c), q in zip(meshes, params)]
         multis = ([gpu.MultiSolid3D(leaves, use_graphs=graphs)] if together else
                   [gpu.MultiSolid3D([s], use_graphs=graphs) for s in leaves])
-        for frame in range(3):
-            if frame == 2:  # new fixed nodes: the recorded chunks are re-recorded
+        for frame, dt in enumerate((DT, 2 * DT, DT, DT)):
+            # a new frame dt (new fM, fK) and new fixed nodes: the recorded
+            # chunks are re-recorded
+            if frame == 3:
                 for s, (p, _, c) in zip(leaves, meshes):
                     s.set_knc(c[: len(c) // 2])
             for m in multis:
-                m.update(DT)
+                m.update(dt)
         runs.append([s.state() for s in leaves])
         for m in multis:
             m.close()
