@@ -9,6 +9,7 @@
 #include <cstring>
 #include <cstdlib>
 #include <limits>
+#include <map>
 #include <vector>
 #include <algorithm>
 #include <utility>
@@ -1747,6 +1748,51 @@ void orc_svd_rotation2(const float F[4], float detF, float thr, float R[4]) {
 
 // EditableTriangleElementSet2::EndEdition (TriangleElementSet2.cpp:2187-2239) +
 // LeafDSH_Solid2D_FEM::RebuildMass (LeafDSH_Solid2D_FEM.cpp:1842-1863)
+
+// RebuildMass's weighted areas (ds/dsh/LeafDSH_Solid2D_FEM.cpp:1842-1863):
+// weighted_area += Rcp(3) * Area(pid) over iterator_polygons_around_vertex_ccw
+// (Geo/shape/MeshSolidShape2.h:587-619). Half-edge 3p+k of triangle p runs
+// from its vertex k to vertex k+1 (AddPolygonN, in the order the triangles
+// were added; baking remaps polygon ids, not half-edges); a vertex's first
+// half-edge is its lowest-numbered outgoing one, or its outgoing boundary
+// half-edge (RebuildTopology, MeshSolidShape2.cpp:1330-1356); the fan advances
+// h <- Sym(Prev(h)) until it reaches the boundary or comes back.
+static std::vector<float> fan_weighted_areas(uint32_t n_nodes, uint32_t n_tris, const uint32_t* tris,
+                                             const float* area) {
+    const uint32_t none = 0xffffffffu;
+    const uint32_t nhe = 3 * n_tris;
+    auto origin = [&](uint32_t h) { return tris[h]; };
+    auto next = [&](uint32_t h) { return 3 * (h / 3) + (h % 3 + 1) % 3; };
+    auto prev = [&](uint32_t h) { return 3 * (h / 3) + (h % 3 + 2) % 3; };
+    std::map<std::pair<uint32_t, uint32_t>, uint32_t> by_ends;
+    for (uint32_t h = 0; h < nhe; h++) by_ends[{origin(h), origin(next(h))}] = h;
+    std::vector<uint32_t> sym(nhe, none), out(n_nodes, none);
+    for (uint32_t h = 0; h < nhe; h++) {
+        auto it = by_ends.find({origin(next(h)), origin(h)});
+        if (it != by_ends.end()) sym[h] = it->second;
+    }
+    std::vector<uint32_t> deg(n_nodes, 0);
+    for (uint32_t h = 0; h < nhe; h++) {
+        if (out[origin(h)] == none || sym[h] == none) out[origin(h)] = h;
+        deg[origin(h)]++;
+    }
+    const float third = 1.0f / 3.0f;
+    std::vector<float> wa(n_nodes, 0.0f);
+    for (uint32_t v = 0; v < n_nodes; v++) {
+        if (out[v] == none) continue;
+        float w = 0.0f;
+        uint32_t h = out[v];
+        for (uint32_t steps = deg[v]; steps > 0; steps--) {  // a manifold fan ends within its degree
+            w += third * area[h / 3];
+            const uint32_t nh = sym[prev(h)];
+            if (nh == none || nh == out[v]) break;
+            h = nh;
+        }
+        wa[v] = w;
+    }
+    return wa;
+}
+
 orc_solid2* orc_solid2_create(uint32_t n_nodes, const float* ref2, uint32_t n_tris, const uint32_t* tris,
                               const orc_solid_params* p, float thickness) {
     orc_solid2* s = new orc_solid2;
@@ -1779,11 +1825,9 @@ orc_solid2* orc_solid2_create(uint32_t n_nodes, const float* ref2, uint32_t n_tr
     }
     s->total_area = acc;
     // lumped masses: weighted area Sum(Rcp(3) * area) over the node's triangles
+    // in the mesh's CCW fan order
     const float density_per_area = p->density * thickness;
-    std::vector<float> wa(n_nodes, 0.0f);
-    const float third = 1.0f / 3.0f;
-    for (uint32_t e = 0; e < n_tris; e++)
-        for (int j = 0; j < 3; j++) wa[tris[3 * e + j]] += third * s->area[e];
+    const std::vector<float> wa = fan_weighted_areas(n_nodes, n_tris, tris, s->area.data());
     s->inv_mass.resize(n_nodes);
     for (uint32_t n = 0; n < n_nodes; n++) s->inv_mass[n] = 1.0f / (wa[n] * density_per_area);  // SetMass stores 1/m
     rayleigh(p->rayleigh_freq1, p->rayleigh_freq2, p->damping_ratio, s->c);
@@ -1799,6 +1843,11 @@ orc_solid2* orc_solid2_create(uint32_t n_nodes, const float* ref2, uint32_t n_tr
 void orc_solid2_destroy(orc_solid2* s) { delete s; }
 void orc_solid2_set_knc(orc_solid2* s, uint32_t n, const uint32_t* nids) { s->knc.assign(nids, nids + n); }
 float orc_solid2_total_area(const orc_solid2* s) { return s->total_area; }
+// The fan-ordered weighted areas for caller-given triangle areas.
+void orc_fan_weighted_areas(uint32_t n_nodes, uint32_t n_tris, const uint32_t* tris, const float* area, float* wa) {
+    const std::vector<float> w = fan_weighted_areas(n_nodes, n_tris, tris, area);
+    std::copy(w.begin(), w.end(), wa);
+}
 void orc_solid2_node_masses(const orc_solid2* s, float* m) {
     for (uint32_t n = 0; n < s->N; n++) m[n] = 1.0f / s->inv_mass[n];
 }

@@ -179,6 +179,7 @@ void orc_solid2_get_state(const orc_solid2* s, float* x, float* v);
 void orc_solid2_set_state(orc_solid2* s, const float* x, const float* v);
 void orc_solid2_node_masses(const orc_solid2* s, float* m);
 float orc_solid2_total_area(const orc_solid2* s);
+void orc_fan_weighted_areas(uint32_t n_nodes, uint32_t n_tris, const uint32_t* tris, const float* area, float* wa);
 /* element operators on the current rotations (set by the last step or begin_evaluation) */
 void orc_solid2_begin_evaluation(orc_solid2* s, const float* prev_x /* may be NULL */, const float* x,
                                  float* R_out, float* detF_out);

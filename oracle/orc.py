@@ -116,6 +116,7 @@ def _load():
         "orc_solid2_set_state": (None, [C.c_void_p, _f, _f]),
         "orc_solid2_node_masses": (None, [C.c_void_p, _f]),
         "orc_solid2_total_area": (C.c_float, [C.c_void_p]),
+        "orc_fan_weighted_areas": (None, [C.c_uint32, C.c_uint32, _u, _f, _f]),
         "orc_solid2_begin_evaluation": (None, [C.c_void_p, _f, _f, _f, _f]),
         "orc_solid2_get_noc": (None, [C.c_void_p, _b]),
         "orc_solid2_set_dapd": (None, [C.c_void_p, C.c_float, C.c_float, C.c_float]),

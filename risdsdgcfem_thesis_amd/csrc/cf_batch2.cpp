@@ -241,6 +241,51 @@ int cf_batch2_destroy(cf_batch2* b) {
     return CF_OK;
 }
 
+
+// RebuildMass's weighted areas (ds/dsh/LeafDSH_Solid2D_FEM.cpp:1842-1863):
+// Rcp(3) * area summed over iterator_polygons_around_vertex_ccw
+// (Geo/shape/MeshSolidShape2.h:587-619). Half-edge 3p+k of triangle p runs
+// from its vertex k to vertex k+1, numbered in the order the triangles are
+// given (the reference's AddPolygonN order; its baking remaps polygon ids but
+// keeps half-edges); a vertex's fan starts at its lowest-numbered outgoing
+// half-edge, or at its outgoing boundary half-edge (RebuildTopology,
+// MeshSolidShape2.cpp:1330-1356), and advances h <- Sym(Prev(h)) until the
+// boundary or the start, at most once per outgoing half-edge.
+static void fan_weighted_areas(uint32_t N, uint32_t T, const uint32_t* tris, const float* area,
+                               std::vector<float>& wa) {
+    const uint32_t none = 0xffffffffu, nhe = 3 * T;
+    auto next = [](uint32_t h) { return h - h % 3 + (h % 3 + 1) % 3; };
+    auto prev = [](uint32_t h) { return h - h % 3 + (h % 3 + 2) % 3; };
+    std::vector<uint32_t> first(N + 1, 0), outs(nhe);  // outgoing half-edges per vertex, ascending
+    for (uint32_t h = 0; h < nhe; h++) first[tris[h] + 1]++;
+    for (uint32_t v = 0; v < N; v++) first[v + 1] += first[v];
+    std::vector<uint32_t> fill(first.begin(), first.end() - 1);
+    for (uint32_t h = 0; h < nhe; h++) outs[fill[tris[h]]++] = h;
+    std::vector<uint32_t> sym(nhe, none);
+    for (uint32_t h = 0; h < nhe; h++) {
+        const uint32_t o = tris[h], d = tris[next(h)];
+        for (uint32_t i = first[d]; i < first[d + 1]; i++)
+            if (tris[next(outs[i])] == o) { sym[h] = outs[i]; break; }
+    }
+    const float third = 1.0f / 3.0f;
+    wa.assign(N, 0.0f);
+    for (uint32_t v = 0; v < N; v++) {
+        uint32_t start = none;
+        for (uint32_t i = first[v]; i < first[v + 1]; i++)
+            if (start == none || sym[outs[i]] == none) start = outs[i];
+        if (start == none) continue;
+        float w = 0.0f;
+        uint32_t h = start;
+        for (uint32_t steps = first[v + 1] - first[v]; steps > 0; steps--) {
+            w += third * area[h / 3];
+            const uint32_t nh = sym[prev(h)];
+            if (nh == none || nh == start) break;
+            h = nh;
+        }
+        wa[v] = w;
+    }
+}
+
 int cf_batch2_add_sim(cf_batch2* b, uint32_t N, const float* ref, uint32_t T, const uint32_t* tris,
                       const cf_solid_params* p, uint32_t n_knc, const uint32_t* knc, uint32_t* sim_id) {
     if (!b || !ref || !tris || !p || !sim_id || N == 0 || T == 0 || (n_knc && !knc)) {
@@ -288,12 +333,11 @@ int cf_batch2_add_sim(cf_batch2* b, uint32_t N, const float* ref, uint32_t T, co
         const float id = 1.0f / det;
         dinv[e] = make_float4(id * D11, -id * D01, -id * D10, id * D00);
     }
-    // lumped masses: Sum(Rcp(3) area) * density * thickness, stored as 1/m (GetMass = 1/(1/m))
+    // lumped masses: Sum(Rcp(3) area) * density * thickness in the mesh's CCW
+    // fan order, stored as 1/m (GetMass = 1/(1/m))
     const float density_per_area = p->density * p->thickness;
-    const float third = 1.0f / 3.0f;
-    std::vector<float> wa(N, 0.0f);
-    for (uint32_t e = 0; e < T; e++)
-        for (int j = 0; j < 3; j++) wa[tris[3 * e + j]] += third * area[e];
+    std::vector<float> wa;
+    fan_weighted_areas(N, T, tris, area.data(), wa);
     // greedy colouring: no two triangles of one colour share a node
     std::vector<uint32_t> used(N, 0), color(T);
     uint32_t C = 0;

@@ -4,6 +4,7 @@ triangle stiffness, dense assembly, a dense solve of one QIE-v step, the
 lumped-mass closed form and free fall. (The 2D Ke and polar factor themselves
 are pinned bit-exactly to the reference in test_oracle.py.)"""
 import numpy as np
+import pytest
 
 
 def ke2_float64(r, E, nu):
@@ -68,6 +69,70 @@ def test_lumped_masses_closed_form(orc):
     inc = np.bincount(tris, minlength=pos.size // 2)
     tri_area = area / (tris.size // 3)
     assert np.allclose(m, inc * tri_area / 3 * 1000.0, rtol=1e-5)
+
+
+def fan_sums_geometric(pos, tris, area):
+    """RebuildMass's weighted areas, Sum Rcp(3) * area in float32, with each
+    vertex's triangles ordered by the angle of their centroids around it
+    (counter-clockwise): an interior vertex's fan starts at its lowest-numbered
+    triangle (which holds its lowest-numbered outgoing half-edge), a boundary
+    vertex's just after the empty sector."""
+    P = pos.reshape(-1, 2).astype(np.float64)
+    T3 = tris.reshape(-1, 3)
+    third = np.float32(1.0) / np.float32(3.0)
+    inc = [[] for _ in range(len(P))]
+    edges = {}
+    for t, tri in enumerate(T3):
+        for k in range(3):
+            inc[tri[k]].append(t)
+            e = tuple(sorted((tri[k], tri[(k + 1) % 3])))
+            edges[e] = edges.get(e, 0) + 1
+    wa = np.zeros(len(P), np.float32)
+    for v, ts in enumerate(inc):
+        if not ts:
+            continue
+        c = P[T3[ts]].mean(axis=1) - P[v]
+        ang = np.arctan2(c[:, 1], c[:, 0])
+        order = np.argsort(ang)
+        boundary = any(n == 1 for e, n in edges.items() if v in e)
+        if boundary:
+            a = ang[order]
+            gaps = np.diff(np.r_[a, a[0] + 2 * np.pi])
+            start = (int(np.argmax(gaps)) + 1) % len(order)
+        else:
+            start = int(np.nonzero(np.array(ts)[order] == min(ts))[0][0])
+        w = np.float32(0.0)
+        for i in np.roll(order, -start):
+            w = np.float32(w + np.float32(third * area[ts[i]]))
+        wa[v] = w
+    return wa
+
+
+@pytest.mark.parametrize("counts,hole", [((9, 7), None), ((6, 6), (2, 3)), ((12, 5), (7, 2))])
+def test_lumped_mass_fan_order(orc, counts, hole):
+    """RebuildMass sums over iterator_polygons_around_vertex_ccw
+    (LeafDSH_Solid2D_FEM.cpp:1842-1863, MeshSolidShape2.h:587-619): with
+    random areas of several magnitudes, where any other order would round
+    differently, the oracle's sums equal the geometric CCW fan sums bit for
+    bit, on boxes and on boxes with a one-quad hole."""
+    pos, tris = orc.mesh_box2((0.4, 0.3), counts)
+    T3 = tris.reshape(-1, 3)
+    if hole is not None:  # the two triangles of the quad at (i, j): quads are added row-major
+        q = hole[0] * (counts[1] - 1) + hole[1]
+        T3 = np.delete(T3, [2 * q, 2 * q + 1], axis=0)
+    tris = np.ascontiguousarray(T3.ravel(), np.uint32)
+    rng = np.random.default_rng(6)
+    area = (10.0 ** rng.uniform(-3, 0, len(T3))).astype(np.float32)
+    wa = np.zeros(pos.size // 2, np.float32)
+    orc.lib().orc_fan_weighted_areas(pos.size // 2, len(T3), orc.up(tris), orc.fp(area), orc.fp(wa))
+    expect = fan_sums_geometric(pos, tris, area)
+    assert np.array_equal(wa, expect)
+    # the order matters: summing in ascending triangle order differs somewhere
+    asc = np.zeros_like(wa)
+    for t, tri in enumerate(T3):
+        for v in tri:
+            asc[v] = np.float32(asc[v] + np.float32(np.float32(1.0) / np.float32(3.0) * area[t]))
+    assert not np.array_equal(asc, expect)
 
 
 def test_operators_against_dense_assembly(orc):
