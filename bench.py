@@ -265,8 +265,9 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--kx-reps", type=int, default=50)
-    ap.add_argument("--tile-elems", type=int, default=256)
-    ap.add_argument("--tile-threads", type=int, default=256)
+    ap.add_argument("--tile-elems", type=int, default=512,
+                    help="elements per tile of the 5.24M-tet mesh's element kernel (swept: 512 x 512 best)")
+    ap.add_argument("--tile-threads", type=int, default=512)
     ap.add_argument("--tile-schedule", type=int, default=0)
     ap.add_argument("--cg-workgroups", type=int, default=0,
                     help="workgroups of the leaf CG iteration kernels (0 = library default)")
@@ -331,6 +332,12 @@ def main():
     print(json.dumps(out), flush=True)
 
 
+def reset_tiles(cf):
+    cf.set_tile_elems(0)
+    cf.set_tile_threads(0)
+    cf.set_tile_persistent(0)
+
+
 def bench_single(cf, args):
     """The single large mesh (64x64x256 cube grid), plain GPU drop-in."""
     cf.set_tile_elems(args.tile_elems)
@@ -349,6 +356,7 @@ def bench_single(cf, args):
     log(f"[bench] mesh {N} nodes {T} tets; building solid")
     t_build = time.perf_counter()
     solid = cf.LeafSolid3D(pos, tets, params)
+    reset_tiles(cf)  # the solid keeps its tiles; later workloads build theirs with the library default
     solid.set_state(x0, np.zeros_like(x0))
     solid.set_knc(knc, x0.reshape(-1, 3)[knc].ravel())
     log(f"[bench] setup {time.perf_counter() - t_build:.2f} s; warmup {args.warmup} steps")
@@ -425,7 +433,10 @@ def bench_fie(cf, args):
                                      cg_abs_epsilon=1e-6, cg_max_iter=50, density=1000.0,
                                      integrator=cf.IT_FIE_DX, nr_max_iter=3)
     log(f"[bench] FIE-dx on {T} tets")
+    cf.set_tile_elems(args.tile_elems)
+    cf.set_tile_threads(args.tile_threads)
     solid = cf.LeafSolid3D(pos, tets, params)
+    reset_tiles(cf)
     solid.set_state(x0, np.zeros_like(x0))
     solid.set_knc(knc, x0.reshape(-1, 3)[knc].ravel())
     solid.fixed_step()
