@@ -328,6 +328,7 @@ static int cache_stats_fetch(cf_cache* c) {
 // Device CG loop of ILSS_CG::Solve for the leaf operator A = fM M - fK K.
 // The loop runs on the device; the host enqueues chunks of iterations and
 // checks the device's done flag between chunks.
+//
 // One leaf's CG in resumable phases: cg_begin enqueues r = S(b - A y), d = r
 // and the start test; cg_launch enqueues up to eight iterations (each skipped
 // on the device once the solve has stopped) and the read-back of the stop
