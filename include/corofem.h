@@ -37,7 +37,7 @@ extern "C" {
 #define CF_ERR_INVALID_ARG (-1)
 #define CF_ERR_HIP (-2)
 #define CF_ERR_OUT_OF_MEMORY (-3)
-#define CF_ERR_NAN (-5)
+#define CF_ERR_NAN (-5)            /* a CG or Newton reduction was NaN or Inf (see CF_CG_NONFINITE) */
 #define CF_ERR_UNSUPPORTED (-6)
 #define CF_ERR_NO_DEVICE (-7)
 #define CF_ERR_CAPACITY (-8)
@@ -47,8 +47,15 @@ extern "C" {
 #define CF_CG_CONVERGED_ABS 2   /* delta <= abs^2                    (ILSS_CG.h:86)    */
 #define CF_CG_MAX_ITER 3        /* k == max_iterations               (ILSS_CG.h:88)    */
 #define CF_CG_BREAKDOWN 4       /* |d^T q| < abs^2                   (ILSS_CG.h:97)    */
+/* r^T r or d^T q is NaN or Inf: the state, the right-hand side or the operator
+   holds non-finite values (the reference asserts they do not,
+   TetrahedronElementSet3.cpp:253-255). The step still runs to its end; the
+   entry point returns CF_ERR_NAN. */
+#define CF_CG_NONFINITE 5
 
-/* material methods (ms/fem/Params.h:50-62); only C_WRP is implemented */
+/* material methods (ms/fem/Params.h:50-62): C_WRP (the stored-Ke corotational
+   method, default), C_LCM and C_LCMH are implemented; CF_MM_L and the
+   hyperelastic methods are not (CF_ERR_UNSUPPORTED) */
 #define CF_MM_L 0
 #define CF_MM_C_WRP 1
 /* LCM and the LCM/WRP hybrid (ms/fem/Params.h:54-55): Piola forces
@@ -63,7 +70,8 @@ extern "C" {
 #define CF_RM_QR 1
 #define CF_RM_MSVD 2
 #define CF_RM_DAPD 3
-/* differential methods (ms/fem/Params.h:68-72); only Truncated is implemented */
+/* differential methods (ms/fem/Params.h:68-72): Truncated (default) and Exact
+   are implemented; Exact_Inv and Numerical are not (CF_ERR_UNSUPPORTED) */
 #define CF_DM_TRUNCATED 0
 /* Exact differentials (ms/fem/Params.h:69): K*x adds the McAdams rotation
    differential dR (ms/fem/Rotation.cpp:622-644; C_df_x dR terms
@@ -289,8 +297,17 @@ int cf_solid_destroy(cf_solid* s);
 /* Position KNC at the given targets (NULL: rest positions), like
    CreateKinematicPointConstraint + eKM_Static. Replaces the current set. */
 int cf_solid_set_knc(cf_solid* s, uint32_t n, const uint32_t* h_nids, const float* h_targets);
+/* Step(dt) (:225-256): the total time grows by time_scale * dt once per call
+   (:229-230); fixed steps of time_scale * fixed_dt run while the accumulator is
+   strictly greater than that (:239). stats: the last fixed step, with
+   num_fixed_steps = how many ran. */
 int cf_solid_step(cf_solid* s, float dt, cf_step_stats* stats);
+/* One FixedStep(dt) (:898-990); advances the total time by dt. Both return
+   CF_ERR_NAN (after completing the step, stats->cg_reason = CF_CG_NONFINITE)
+   when a CG or Newton reduction was NaN or Inf: the state is not finite. */
 int cf_solid_fixed_step(cf_solid* s, float dt, cf_step_stats* stats);
+/* m_TotalTime: the t handed to BeginEvaluation (:927). */
+int cf_solid_total_time(const cf_solid* s, float* t);
 int cf_solid_get_state(const cf_solid* s, float* h_pos, float* h_vel);     /* QueryState (GLeafDSH_Model.h:34-43) */
 int cf_solid_set_state(cf_solid* s, const float* h_pos, const float* h_vel);
 int cf_solid_device_state(cf_solid* s, float** d_pos, float** d_vel);

@@ -1346,8 +1346,9 @@ static void step_fie_dx(orc_solid* s, float dt, int32_t* nr_iters, float* nr_rel
 }
 
 // FixedStep (:898-990) with Step_QuasiImplicitEuler_v (:1205-1328) or
-// Step_FullyImplicitEuler_dx (:1357-1552)
-void orc_solid_fixed_step(orc_solid* s, float dt, orc_step_stats* st) {
+// Step_FullyImplicitEuler_dx (:1357-1552). FixedStep itself leaves the total
+// time alone: Step advances it once per call (:229-230).
+static void solid_fixed_step(orc_solid* s, float dt, orc_step_stats* st) {
     if (s->p.integrator == ORC_IT_FIE_DX) {
         solid_begin_evaluation(s, s->x0.data(), s->x.data());
         s->x0 = s->x;
@@ -1362,7 +1363,6 @@ void orc_solid_fixed_step(orc_solid* s, float dt, orc_step_stats* st) {
             st->elastic_energy = V; st->min_detF = s->min_det;
             st->nr_iters = nr; st->nr_rel = nr_rel; st->nr_abs = nr_abs;
         }
-        s->total_time += dt;
         return;
     }
     const uint32_t N = s->N;
@@ -1426,6 +1426,10 @@ void orc_solid_fixed_step(orc_solid* s, float dt, orc_step_stats* st) {
         st->elastic_energy = V; st->min_detF = min_det;
         st->nr_iters = 0; st->nr_rel = 0; st->nr_abs = 0;
     }
+}
+
+void orc_solid_fixed_step(orc_solid* s, float dt, orc_step_stats* st) {
+    solid_fixed_step(s, dt, st);
     s->total_time += dt;
 }
 
@@ -1485,20 +1489,29 @@ uint32_t orc_multi_update(orc_solid* const* solids, uint32_t n, float dt, float*
     return steps;
 }
 
-// Step (:225-256): fixed-dt accumulator with strict '>'
+// Step (:225-256): scaled_dt = time_scale dt is added to the total time and
+// to the accumulator; fixed steps of time_scale fixed_dt run while the
+// accumulator is strictly greater (:239)
 void orc_solid_step(orc_solid* s, float dt, orc_step_stats* st) {
-    s->acc_time += dt;
+    const float scaled_dt = s->p.time_scale * dt;
+    s->total_time += scaled_dt;
+    s->acc_time += scaled_dt;
+    const float scaled_fixed_dt = s->p.time_scale * s->p.fixed_dt;
     orc_step_stats last;
     std::memset(&last, 0, sizeof(last));
     uint32_t n = 0;
-    while (s->acc_time > s->p.fixed_dt) {
-        orc_solid_fixed_step(s, s->p.fixed_dt, &last);
-        s->acc_time -= s->p.fixed_dt;
+    while (s->acc_time > scaled_fixed_dt) {
+        solid_fixed_step(s, scaled_fixed_dt, &last);
+        s->acc_time -= scaled_fixed_dt;
         n++;
     }
     last.num_fixed_steps = n;
     if (st) *st = last;
 }
+
+float orc_solid_total_time(const orc_solid* s) { return s->total_time; }
+
+void orc_rayleigh(float f1, float f2, float zeta, float c[2]) { rayleigh(f1, f2, zeta, c); }
 
 void orc_solid_get_state(const orc_solid* s, float* x, float* v) {
     if (x) std::memcpy(x, s->x.data(), s->x.size() * sizeof(float));
@@ -1530,6 +1543,7 @@ struct orc_solid2 {
     std::vector<float> x0;           // 2N previous positions (m_vecPos0)
     std::vector<uint8_t> noc;        // T nodes of collapse (ElementCache::m_NoC), 3 = invalid
     float dapd_l = 1.0f, dapd_nl = 0.0f, dapd_e = 2.0f;
+    float acc_time = 0.0f, total_time = 0.0f;
     uint32_t n_fallback = 0;         // last BeginEvaluation's DAPD R = Id count
 };
 
@@ -2012,6 +2026,26 @@ void orc_solid2_fixed_step(orc_solid2* s, float dt, orc_step_stats* st) {
         st->elastic_energy = V; st->min_detF = min_det;
     }
 }
+
+// Step (LeafDSH_Solid2D_FEM.cpp:264-294): the 3D leaf's accumulator
+void orc_solid2_step(orc_solid2* s, float dt, orc_step_stats* st) {
+    const float scaled_dt = s->p.time_scale * dt;
+    s->total_time += scaled_dt;
+    s->acc_time += scaled_dt;
+    const float scaled_fixed_dt = s->p.time_scale * s->p.fixed_dt;
+    orc_step_stats last;
+    std::memset(&last, 0, sizeof(last));
+    uint32_t n = 0;
+    while (s->acc_time > scaled_fixed_dt) {
+        orc_solid2_fixed_step(s, scaled_fixed_dt, &last);
+        s->acc_time -= scaled_fixed_dt;
+        n++;
+    }
+    last.num_fixed_steps = n;
+    if (st) *st = last;
+}
+
+float orc_solid2_total_time(const orc_solid2* s) { return s->total_time; }
 
 void orc_solid2_get_state(const orc_solid2* s, float* x, float* v) {
     if (x) std::memcpy(x, s->x.data(), s->x.size() * sizeof(float));

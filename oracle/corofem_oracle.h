@@ -125,6 +125,7 @@ typedef struct orc_solid_params {
     int mm;                /* material model, ORC_MM_C_* (0 = C_WRP) */
     float ecie_k_factor;   /* LCMH blend radius (reference default 1) */
     int dm;                /* differentials method, ORC_DM_* (0 = Truncated) */
+    float time_scale;      /* m_TimeScale (reference default 1): Step scales dt and fixed_dt by it */
 } orc_solid_params;
 #define ORC_IT_QIE_V 0
 #define ORC_IT_FIE_DX 1
@@ -148,8 +149,16 @@ orc_solid* orc_solid_create(uint32_t n_nodes, const float* ref_pos, uint32_t n_t
                             const uint32_t* tets, const orc_solid_params* p);
 void orc_solid_destroy(orc_solid* s);
 void orc_solid_set_knc(orc_solid* s, uint32_t n, const uint32_t* nids);
+/* One FixedStep(dt); advances the solid's total time by dt. */
 void orc_solid_fixed_step(orc_solid* s, float dt, orc_step_stats* st);
+/* Step(dt) (LeafDSH_Solid3D_FEM.cpp:225-256): total time += time_scale dt,
+   then FixedStep(time_scale fixed_dt) while the accumulator is strictly
+   greater than time_scale fixed_dt; num_fixed_steps counts them. */
 void orc_solid_step(orc_solid* s, float dt, orc_step_stats* st);
+float orc_solid_total_time(const orc_solid* s);
+/* Rayleigh coefficients c = inv(Omega) (2 zeta, 2 zeta), Omega = [[1/f1, f1], [1/f2, f2]]
+   (LeafDSH_Solid3D_FEM.h:347-354, LeafDSH_Solid2D_FEM.h:333-341). */
+void orc_rayleigh(float f1, float f2, float zeta, float c[2]);
 void orc_solid_get_state(const orc_solid* s, float* x, float* v);
 void orc_solid_set_state(orc_solid* s, const float* x, const float* v);
 float orc_solid_node_mass(const orc_solid* s);
@@ -175,6 +184,9 @@ orc_solid2* orc_solid2_create(uint32_t n_nodes, const float* ref_pos2, uint32_t 
 void orc_solid2_destroy(orc_solid2* s);
 void orc_solid2_set_knc(orc_solid2* s, uint32_t n, const uint32_t* nids);
 void orc_solid2_fixed_step(orc_solid2* s, float dt, orc_step_stats* st);
+/* Step(dt) of the 2D leaf (LeafDSH_Solid2D_FEM.cpp:264-294), as orc_solid_step. */
+void orc_solid2_step(orc_solid2* s, float dt, orc_step_stats* st);
+float orc_solid2_total_time(const orc_solid2* s);
 void orc_solid2_get_state(const orc_solid2* s, float* x, float* v);
 void orc_solid2_set_state(orc_solid2* s, const float* x, const float* v);
 void orc_solid2_node_masses(const orc_solid2* s, float* m);

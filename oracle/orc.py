@@ -42,7 +42,7 @@ class SolidParams(C.Structure):
         ("cg_max_iter", C.c_uint32), ("rm", C.c_int), ("degenerate_threshold", C.c_float),
         ("warm_start", C.c_int), ("reduce_fp64", C.c_int),
         ("integrator", C.c_int), ("nr_max_iter", C.c_uint32), ("nr_abs_epsilon", C.c_float),
-        ("mm", C.c_int), ("ecie_k_factor", C.c_float), ("dm", C.c_int),
+        ("mm", C.c_int), ("ecie_k_factor", C.c_float), ("dm", C.c_int), ("time_scale", C.c_float),
     ]
 
 
@@ -98,6 +98,10 @@ def _load():
         "orc_solid_set_knc": (None, [C.c_void_p, C.c_uint32, _u]),
         "orc_solid_fixed_step": (None, [C.c_void_p, C.c_float, C.POINTER(StepStats)]),
         "orc_solid_step": (None, [C.c_void_p, C.c_float, C.POINTER(StepStats)]),
+        "orc_solid_total_time": (C.c_float, [C.c_void_p]),
+        "orc_rayleigh": (None, [C.c_float, C.c_float, C.c_float, _f]),
+        "orc_solid2_step": (None, [C.c_void_p, C.c_float, C.POINTER(StepStats)]),
+        "orc_solid2_total_time": (C.c_float, [C.c_void_p]),
         "orc_solid_get_state": (None, [C.c_void_p, _f, _f]),
         "orc_solid_set_state": (None, [C.c_void_p, _f, _f]),
         "orc_solid_node_mass": (C.c_float, [C.c_void_p]),
@@ -153,6 +157,7 @@ def _load_ref():
         "ref_R_svd2": (None, [_f, C.c_float, C.c_float, _f]),
         "ref_dot": (C.c_float, [_f, _f, C.c_uint32]),
         "ref_has_sse_path": (C.c_int, []),
+        "ref_rayleigh": (None, [C.c_float, C.c_float, C.c_float, _f]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -290,6 +295,13 @@ class TetSet:
         return lib().orc_V(self.h, fp(x))
 
 
+def rayleigh(f1, f2, zeta):
+    """Rayleigh coefficients (c0, c1) of the leaves (LeafDSH_Solid3D_FEM.h:347-354)."""
+    c = np.zeros(2, np.float32)
+    lib().orc_rayleigh(f1, f2, zeta, fp(c))
+    return c
+
+
 def gsl_cubic(a, b, c):
     """Real roots of x^3 + a x^2 + b x + c (ascending), GSL's method in double."""
     r = np.zeros(3, np.float64)
@@ -366,6 +378,7 @@ def default_params(**kw):
     p.mm = MM_C_WRP
     p.ecie_k_factor = 1.0
     p.dm = DM_TRUNCATED
+    p.time_scale = 1.0
     for k, v in kw.items():
         if k == "gravity":
             for i in range(3):
@@ -418,6 +431,14 @@ class Solid:
         lib().orc_solid_step(self.h, dt, C.byref(st))
         return st
 
+    def total_time(self):
+        return lib().orc_solid_total_time(self.h)
+
+    def rayleigh(self):
+        c = np.zeros(2, np.float32)
+        lib().orc_solid_rayleigh(self.h, fp(c))
+        return c
+
     def state(self):
         x = np.zeros(3 * self.N, np.float32)
         v = np.zeros(3 * self.N, np.float32)
@@ -458,6 +479,14 @@ class Solid2:
         st = StepStats()
         lib().orc_solid2_fixed_step(self.h, self.params.fixed_dt if dt is None else dt, C.byref(st))
         return st
+
+    def step(self, dt):
+        st = StepStats()
+        lib().orc_solid2_step(self.h, dt, C.byref(st))
+        return st
+
+    def total_time(self):
+        return lib().orc_solid2_total_time(self.h)
 
     def state(self):
         x = np.zeros(2 * self.N, np.float32)

@@ -12,6 +12,9 @@
 //   mal::GSingularValueDecomposition_USVt (PhysBAM 3x3)         (Mal/GMatDecomposition.h:351-361)
 //   S2::ms::fem::Compute_R_SVD (2D and 3D)                      (ms/fem/Rotation.cpp:19-57)
 //   S2::ns::real_array::dot / sq_norm_2 (SSE4.1)                (ns/real_array.h:34-49)
+//   the leaves' Rayleigh coefficients, mal::Inverse(Omega) * (2*zeta) on Mal's
+//   Mat2x2 / Vec2 (LeafDSH_Solid3D_FEM.h:347-354; the leaf header itself needs
+//   boost, so the four statements are driven here on the reference's types)
 #include <cstring>
 #include <cstdint>
 #include <cstdlib>
@@ -107,6 +110,16 @@ float ref_dot(const float* a, const float* b, uint32_t n) {
     ns::real_array::dealloc(pa);
     ns::real_array::dealloc(pb);
     return r;
+}
+
+void ref_rayleigh(float f1, float f2, float zeta, float c[2]) {
+    Mat2x2 Omega;
+    Omega(0, 0) = mal::Rcp(f1); Omega(0, 1) = f1;
+    Omega(1, 0) = mal::Rcp(f2); Omega(1, 1) = f2;
+    Vec2 z(zeta, zeta);
+    Vec2 r = mal::Inverse(Omega) * (2 * z);
+    c[0] = r[0];
+    c[1] = r[1];
 }
 
 int ref_has_sse_path(void) {

@@ -10,7 +10,9 @@ RM_ID, RM_QR, RM_MSVD, RM_DAPD = 0, 1, 2, 3
 MM_C_WRP, MM_C_LCM, MM_C_LCMH = 1, 2, 3
 DM_TRUNCATED, DM_EXACT = 0, 1
 IT_QIE_V, IT_FIE_DX = 1, 3
-CG_CONVERGED_REL, CG_CONVERGED_ABS, CG_MAX_ITER, CG_BREAKDOWN = 1, 2, 3, 4
+CG_CONVERGED_REL, CG_CONVERGED_ABS, CG_MAX_ITER, CG_BREAKDOWN, CG_NONFINITE = 1, 2, 3, 4, 5
+ERR_INVALID_ARG, ERR_HIP, ERR_OUT_OF_MEMORY, ERR_NAN, ERR_UNSUPPORTED, ERR_NO_DEVICE, ERR_CAPACITY = (
+    -1, -2, -3, -5, -6, -7, -8)
 
 _f = C.POINTER(C.c_float)
 _u = C.POINTER(C.c_uint32)
@@ -156,6 +158,7 @@ SIGNATURES = {
     "cf_solid_set_state": (C.c_int, [_vp, _f, _f]),
     "cf_solid_device_state": (C.c_int, [_vp, C.POINTER(_vp), C.POINTER(_vp)]),
     "cf_solid_info": (C.c_int, [_vp, _u, _u, _f, _f]),
+    "cf_solid_total_time": (C.c_int, [_vp, _f]),
     "cf_solid_assemble_rhs": (C.c_int, [_vp, C.c_float, _f, _f]),
     "cf_solid_apply_A": (C.c_int, [_vp, C.c_float, C.c_float, _f, _f]),
     "cf_solid_time_df_x": (C.c_int, [_vp, C.c_uint32, C.POINTER(C.c_double)]),
@@ -217,7 +220,11 @@ SIGNATURES = {
 
 
 class CorofemError(RuntimeError):
-    pass
+    """A non-zero status from the C-ABI; .status holds the code (ERR_*)."""
+
+    def __init__(self, msg, status=0):
+        super().__init__(msg)
+        self.status = status
 
 
 def _load():
@@ -239,5 +246,5 @@ lib = _load()
 def check(rc, what=""):
     if rc != CF_OK:
         msg = lib.cf_last_error().decode(errors="replace")
-        raise CorofemError(f"{what} failed with status {rc}: {msg}")
+        raise CorofemError(f"{what} failed with status {rc}: {msg}", rc)
     return rc

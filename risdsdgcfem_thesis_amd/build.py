@@ -27,18 +27,24 @@ def hipcc():
     return "hipcc"
 
 
-def build(verbose=True):
-    objs = []
+def build(verbose=True, jobs=None):
+    """Compiles every source (in parallel, `jobs` at once; default: the CPU
+    count, at most 8) and links the library."""
+    from concurrent.futures import ThreadPoolExecutor
     bdir = os.path.join(HERE, "_build")
     os.makedirs(bdir, exist_ok=True)
+    cmds, objs = [], []
     for src in SOURCES:
         path = os.path.join(CSRC, src)
         obj = os.path.join(bdir, os.path.splitext(src)[0] + ".o")
-        cmd = [hipcc(), "-x", "hip", f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",
-               "-Wall", "-Wno-unused-result", "-I", os.path.join(HERE, "..", "include"),
-               "-c", path, "-o", obj] + DEVICE_FLAGS
-        _run(cmd, verbose)
+        cmds.append([hipcc(), "-x", "hip", f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",
+                     "-Wall", "-Wno-unused-result", "-I", os.path.join(HERE, "..", "include"),
+                     "-c", path, "-o", obj] + DEVICE_FLAGS)
         objs.append(obj)
+    jobs = jobs or min(8, os.cpu_count() or 1)
+    with ThreadPoolExecutor(max_workers=jobs) as ex:
+        for f in [ex.submit(_run, c, verbose) for c in cmds]:
+            f.result()
     cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", OUT] + objs
     _run(cmd, verbose)
     return OUT

@@ -490,6 +490,12 @@ int rebuild_tables(cf_batch* b) {
 // Tables of the CG loop over the simulations still iterating, read from the
 // pinned copy of the CG scalars; uploaded on the stream (after the chunk in
 // flight) into the compacted buffers.
+// A simulation whose last solve met a NaN or Inf in r.r or d.q (its state,
+// forces or operator hold non-finite values).
+static bool sim_cg_nonfinite(const SimCg& c) {
+    return !std::isfinite(c.delta0) || !std::isfinite(c.delta) || !std::isfinite(c.dq);
+}
+
 int compact_tables(cf_batch* b, const SimCg* cgs, BatchTables& out, uint32_t* n_sims) {
     std::vector<uint32_t> ids;
     for (uint32_t id : b->active_ids)
@@ -951,18 +957,21 @@ int cf_batch_fixed_step(cf_batch* b, cf_step_stats* stats) {
     else CF_TRY(enqueue_epilogue(b));
     CF_CHECK_HIP(hipEventRecord(b->ev1, s));
     CF_CHECK_HIP(hipEventSynchronize(b->ev1));
+    // every step's CG records: the stats, and the non-finite test
+    b->h_cgs.resize(b->slots_used);
+    CF_CHECK_HIP(hipMemcpy(b->h_cgs.data(), b->d_cgs, b->slots_used * sizeof(SimCg), hipMemcpyDeviceToHost));
+    uint32_t nonfinite = 0;
+    for (uint32_t id : b->active_ids) nonfinite += sim_cg_nonfinite(b->h_cgs[b->sims[id].slot]) ? 1 : 0;
     if (stats) {
         float ms = 0.f;
         CF_CHECK_HIP(hipEventElapsedTime(&ms, b->ev0, b->ev1));
-        b->h_cgs.resize(b->slots_used);
-        CF_CHECK_HIP(hipMemcpy(b->h_cgs.data(), b->d_cgs, b->slots_used * sizeof(SimCg), hipMemcpyDeviceToHost));
         for (size_t i = 0; i < b->active_ids.size(); i++) {
             const BSim& sim = b->sims[b->active_ids[i]];
             const SimCg& c = b->h_cgs[sim.slot];
             cf_step_stats& o = stats[i];
             o.num_fixed_steps = 1;
             o.cg_iters = int32_t(c.iters);
-            o.cg_reason = c.reason;
+            o.cg_reason = sim_cg_nonfinite(c) ? CF_CG_NONFINITE : c.reason;
             o.cg_rel_residual = (c.delta0 > 0) ? float(std::sqrt(c.delta / c.delta0)) : 0.f;
             o.cg_abs_residual = float(std::sqrt(c.delta));
             o.num_degenerate = c.num_degenerate;
@@ -971,6 +980,10 @@ int cf_batch_fixed_step(cf_batch* b, cf_step_stats* stats) {
             o.min_det_f = c.min_det;
             o.step_ms = ms;
         }
+    }
+    if (nonfinite) {
+        set_error("batched step: %u simulation(s) with a NaN or Inf CG reduction", nonfinite);
+        return CF_ERR_NAN;
     }
     return CF_OK;
 }
@@ -1017,6 +1030,12 @@ int cf_batch_round(cf_batch* b, uint32_t iters, uint32_t* steps_done, uint32_t* 
         for (size_t i = 0; i < b->active_ids.size(); i++)
             steps_done[i] = b->h_cgs_pin[0][b->sims[b->active_ids[i]].slot].steps_done;
     if (busy) *busy = t.n_active ? b->h_counter[2] : 0;
+    uint32_t nonfinite = 0;
+    for (uint32_t id : b->active_ids) nonfinite += sim_cg_nonfinite(b->h_cgs_pin[0][b->sims[id].slot]) ? 1 : 0;
+    if (nonfinite) {
+        set_error("batch round: %u simulation(s) with a NaN or Inf CG reduction", nonfinite);
+        return CF_ERR_NAN;
+    }
     return CF_OK;
 }
 

@@ -47,6 +47,7 @@ struct cf_batch2 {
     Work2* d_work = nullptr;
     Sim2Out* d_out = nullptr;
     std::vector<Sim2Out> h_out;
+    int32_t* d_flag = nullptr;  // a mesh's CG met a NaN or Inf
 };
 
 namespace {
@@ -62,7 +63,7 @@ void dfree(T*& p) {
 void free_device(cf_batch2* b) {
     dfree(b->d_meshes); dfree(b->d_ref); dfree(b->d_x); dfree(b->d_v); dfree(b->d_mass); dfree(b->d_area);
     dfree(b->d_knc); dfree(b->d_tri); dfree(b->d_dinv); dfree(b->d_cs); dfree(b->d_work); dfree(b->d_out);
-    dfree(b->d_x0); dfree(b->d_noc);
+    dfree(b->d_x0); dfree(b->d_noc); dfree(b->d_flag);
 }
 
 template <class T>
@@ -131,6 +132,7 @@ int upload(cf_batch2* b) {
     CF_TRY(put(&b->d_work, b->work, s));
     b->h_out.assign(b->meshes.size(), Sim2Out{});
     CF_TRY(put(&b->d_out, b->h_out, s));
+    CF_CHECK_HIP(hipMalloc(reinterpret_cast<void**>(&b->d_flag), sizeof(int32_t)));
     CF_CHECK_HIP(hipStreamSynchronize(s));
     b->dirty = false;
     b->on_device = true;
@@ -153,6 +155,7 @@ Batch2Args args(const cf_batch2* b) {
     a.out = b->d_out;
     a.x0 = b->d_x0;
     a.noc = b->d_noc;
+    a.nonfinite = b->d_flag;
     return a;
 }
 
@@ -413,10 +416,13 @@ int cf_batch2_fixed_step(cf_batch2* b, cf_step_stats* stats) {
     if (!b) return CF_ERR_INVALID_ARG;
     if (b->meshes.empty()) return CF_OK;
     if (b->dirty) CF_TRY(upload(b));
+    CF_CHECK_HIP(hipMemsetAsync(b->d_flag, 0, sizeof(int32_t), b->stream));
     CF_CHECK_HIP(hipEventRecord(b->ev0, b->stream));
     CF_TRY(launch_batch2_step(args(b), uint32_t(b->work.size()), b->stream));
     CF_CHECK_HIP(hipEventRecord(b->ev1, b->stream));
     CF_CHECK_HIP(hipEventSynchronize(b->ev1));
+    int32_t nonfinite = 0;
+    CF_CHECK_HIP(hipMemcpy(&nonfinite, b->d_flag, sizeof(int32_t), hipMemcpyDeviceToHost));
     if (stats) {
         float ms = 0.f;
         CF_CHECK_HIP(hipEventElapsedTime(&ms, b->ev0, b->ev1));
@@ -426,7 +432,7 @@ int cf_batch2_fixed_step(cf_batch2* b, cf_step_stats* stats) {
             cf_step_stats& s = stats[i];
             s.num_fixed_steps = 1;
             s.cg_iters = o.iters;
-            s.cg_reason = o.reason;
+            s.cg_reason = (std::isfinite(o.rel) && std::isfinite(o.abs_res)) ? o.reason : CF_CG_NONFINITE;
             s.cg_rel_residual = o.rel;
             s.cg_abs_residual = o.abs_res;
             s.num_degenerate = o.num_degenerate;
@@ -435,6 +441,10 @@ int cf_batch2_fixed_step(cf_batch2* b, cf_step_stats* stats) {
             s.min_det_f = o.min_det;
             s.step_ms = ms;
         }
+    }
+    if (nonfinite) {
+        set_error("2D step: a CG reduction is NaN or Inf (non-finite state)");
+        return CF_ERR_NAN;
     }
     return CF_OK;
 }
@@ -559,14 +569,17 @@ int cf_solid2_step(cf_solid2* s, float dt, cf_step_stats* stats) {
     const float fixed = s->p.time_scale * s->p.fixed_dt;
     cf_step_stats last{};
     uint32_t n = 0;
+    int rc = CF_OK;
     while (s->acc_time > fixed) {
-        CF_TRY(cf_batch2_fixed_step(s->b, &last));
+        rc = cf_batch2_fixed_step(s->b, &last);
+        if (rc != CF_OK && rc != CF_ERR_NAN) return rc;
         s->acc_time -= fixed;
         n++;
+        if (rc == CF_ERR_NAN) break;
     }
     last.num_fixed_steps = n;
     if (stats) *stats = last;
-    return CF_OK;
+    return rc;
 }
 
 int cf_solid2_get_noc(cf_solid2* s, uint8_t* h_noc) {

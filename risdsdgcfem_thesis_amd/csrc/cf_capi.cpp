@@ -182,6 +182,7 @@ struct cf_solid {
     double* h_part = nullptr;             // pinned: block partials of |b|^2, |y|^2
     cf_cg_result last_ls{};               // last linear solve
     int32_t ls_total = 0;                 // linear-solve iterations of the last fixed step
+    bool nonfinite = false;               // a reduction of the last fixed step was NaN or Inf
     CgGraphs graphs;                      // recorded CG chunks (multi-solid step)
 };
 
@@ -433,11 +434,12 @@ static int cg_poll(cf_solid* s, CgRun* run) {
     return CF_OK;
 }
 
-static int cg_finish(cf_solid* s, const CgRun& run, cf_cg_result* res) {
-    static thread_local CgScalars hs;
-    CF_CHECK_HIP(hipMemcpyAsync(&hs, s->cg.st, sizeof(CgScalars), hipMemcpyDeviceToHost, s->stream));
-    CF_CHECK_HIP(hipStreamSynchronize(s->stream));
-    const uint32_t max_it = run.max_it;
+// The solve's result from its scalar record: the iteration that stopped, its
+// reason and residuals. A NaN or Inf in r.r or d.q (a non-finite state or
+// right-hand side, which the reference only asserts against,
+// TetrahedronElementSet3.cpp:253-255) stops the tests at once; it is
+// reported as CF_CG_NONFINITE.
+static void cg_result_from(const CgScalars& hs, uint32_t max_it, cf_cg_result* res) {
     uint32_t j = 0;
     while (j <= max_it && !hs.done[j]) j++;
     if (j > max_it) j = max_it;
@@ -445,10 +447,22 @@ static int cg_finish(cf_solid* s, const CgRun& run, cf_cg_result* res) {
     uint32_t iters = j;
     double delta = hs.delta[j];
     if (reason == CF_CG_BREAKDOWN) { iters = j - 1; delta = hs.delta[j - 1]; }
+    // r.r entering iterations 0..iters and d.q of iterations 0..j-1 were computed
+    bool finite = std::isfinite(hs.delta0);
+    for (uint32_t i = 0; i <= iters && finite; i++) finite = std::isfinite(hs.delta[i]);
+    for (uint32_t i = 0; i < j && finite; i++) finite = std::isfinite(hs.dq[i]);
     res->iterations = int32_t(iters);
-    res->reason = reason;
+    res->reason = finite ? reason : CF_CG_NONFINITE;
     res->rel_residual = (hs.delta0 > 0) ? float(std::sqrt(delta / hs.delta0)) : 0.f;
     res->abs_residual = float(std::sqrt(delta));
+}
+
+static int cg_finish(cf_solid* s, const CgRun& run, cf_cg_result* res) {
+    static thread_local CgScalars hs;
+    CF_CHECK_HIP(hipMemcpyAsync(&hs, s->cg.st, sizeof(CgScalars), hipMemcpyDeviceToHost, s->stream));
+    CF_CHECK_HIP(hipStreamSynchronize(s->stream));
+    cg_result_from(hs, run.max_it, res);
+    if (res->reason == CF_CG_NONFINITE) s->nonfinite = true;
     return CF_OK;
 }
 
@@ -496,6 +510,7 @@ static int fie_rhs(cf_solid* s, float dt, float inv_dt, bool first, float* bnorm
         double bb = 0.0;
         CF_TRY(host_sum(s, s->cg.part_b, nb, &bb));
         *bnorm = float(std::sqrt(bb));
+        if (!std::isfinite(bb)) s->nonfinite = true;
     }
     return CF_OK;
 }
@@ -514,6 +529,7 @@ static int fie_solve_update(cf_solid* s, float inv_dt, float* ynorm) {
         double yy = 0.0;
         CF_TRY(host_sum(s, s->cg.part_b, nb, &yy));
         *ynorm = float(std::sqrt(yy));
+        if (!std::isfinite(yy)) s->nonfinite = true;
     }
     return cf_begin_evaluation(s->cache, s->x, s->xk, s->vk, s->total_time);
 }
@@ -1081,17 +1097,11 @@ int cf_cg_solve(cf_cg* c, cf_av_fn av, cf_constraints_fn cons, void* user, float
     static thread_local CgScalars hs;
     CF_CHECK_HIP(hipMemcpyAsync(&hs, b.st, sizeof(CgScalars), hipMemcpyDeviceToHost, st));
     CF_CHECK_HIP(hipStreamSynchronize(st));
-    uint32_t j = 0;
-    while (j <= max_it && !hs.done[j]) j++;
-    if (j > max_it) j = max_it;
-    int reason = hs.reason[j];
-    uint32_t iters = j;
-    double delta = hs.delta[j];
-    if (reason == CF_CG_BREAKDOWN) { iters = j - 1; delta = hs.delta[j - 1]; }
-    res->iterations = int32_t(iters);
-    res->reason = reason;
-    res->rel_residual = (hs.delta0 > 0) ? float(std::sqrt(delta / hs.delta0)) : 0.f;
-    res->abs_residual = float(std::sqrt(delta));
+    cg_result_from(hs, max_it, res);
+    if (res->reason == CF_CG_NONFINITE) {
+        set_error("cf_cg_solve: r.r or d.q is not finite (NaN or Inf in x, b or A)");
+        return CF_ERR_NAN;
+    }
     return CF_OK;
 }
 
@@ -1186,8 +1196,17 @@ int cf_solid_set_knc(cf_solid* s, uint32_t n, const uint32_t* nids, const float*
     return CF_OK;
 }
 
-int cf_solid_fixed_step(cf_solid* s, float dt, cf_step_stats* stats) {
+static int nonfinite_status(cf_solid* s) {
+    if (!s->nonfinite) return CF_OK;
+    set_error("fixed step: a CG or Newton reduction is NaN or Inf (non-finite state or forces)");
+    return CF_ERR_NAN;
+}
+
+// FixedStep (:898-990). The total time is the caller's: Step advances it once
+// per call (:229-230), a direct fixed step by its own dt.
+static int solid_fixed_step(cf_solid* s, float dt, cf_step_stats* stats) {
     if (!s || !(dt >= 0) || (dt == 0 && s->p.integrator != CF_IT_FIE_DX)) return CF_ERR_INVALID_ARG;
+    s->nonfinite = false;
     cf_tetset* set = s->set;
     const TetDevice& t = set->dev;
     hipStream_t st = s->stream;
@@ -1208,7 +1227,6 @@ int cf_solid_fixed_step(cf_solid* s, float dt, cf_step_stats* stats) {
         CF_TRY(cf_end_evaluation(s->cache));
         CF_CHECK_HIP(hipEventRecord(s->ev1, st));
         CF_CHECK_HIP(hipEventSynchronize(s->ev1));
-        s->total_time += dt;
         if (stats) {
             float ms = 0.f;
             CF_CHECK_HIP(hipEventElapsedTime(&ms, s->ev0, s->ev1));
@@ -1216,8 +1234,9 @@ int cf_solid_fixed_step(cf_solid* s, float dt, cf_step_stats* stats) {
             stats->nr_iters = nr;
             stats->nr_rel_residual = nr_rel;
             stats->nr_abs_residual = nr_abs;
+            if (s->nonfinite) stats->cg_reason = CF_CG_NONFINITE;
         }
-        return CF_OK;
+        return nonfinite_status(s);
     }
     // b = dt (M g + f_e(x)) + M v ; y = v ; KNC rows (Step_QuasiImplicitEuler_v :1209-1295)
     CF_TRY(launch_forces(t, kModeFe, s->x, s->cache->R, cache_mat(s->cache), st));
@@ -1242,7 +1261,6 @@ int cf_solid_fixed_step(cf_solid* s, float dt, cf_step_stats* stats) {
     CF_TRY(cf_end_evaluation(s->cache));
     CF_CHECK_HIP(hipEventRecord(s->ev1, st));
     CF_CHECK_HIP(hipEventSynchronize(s->ev1));
-    s->total_time += dt;
     if (stats) {
         float ms = 0.f;
         CF_CHECK_HIP(hipEventElapsedTime(&ms, s->ev0, s->ev1));
@@ -1261,6 +1279,18 @@ int cf_solid_fixed_step(cf_solid* s, float dt, cf_step_stats* stats) {
         stats->nr_rel_residual = 0.f;
         stats->nr_abs_residual = 0.f;
     }
+    return nonfinite_status(s);
+}
+
+int cf_solid_fixed_step(cf_solid* s, float dt, cf_step_stats* stats) {
+    const int rc = solid_fixed_step(s, dt, stats);
+    if (s && (rc == CF_OK || rc == CF_ERR_NAN)) s->total_time += dt;
+    return rc;
+}
+
+int cf_solid_total_time(const cf_solid* s, float* t) {
+    if (!s || !t) return CF_ERR_INVALID_ARG;
+    *t = s->total_time;
     return CF_OK;
 }
 
@@ -1309,9 +1339,11 @@ int cf_multi_update(cf_multi* m, float dt, uint32_t* num_fixed_steps, cf_step_st
     const float inv_dt = (dt == 0.0f) ? 0.0f : 1.0f / dt;
     const size_t n = m->solids.size();
     uint32_t steps = 0;
+    bool nonfinite = false;
     std::vector<CgRun> runs(n);
     while (m->acc_time > fixed_dt) {
         for (cf_solid* s : m->solids) {  // NR_Init (:2824-2836)
+            s->nonfinite = false;
             CF_TRY(cf_begin_evaluation(s->cache, s->x0, s->x, s->v, s->total_time));
             CF_CHECK_HIP(hipMemcpyAsync(s->x0, s->x, 3 * (size_t)s->N * 4, hipMemcpyDeviceToDevice, s->stream));
             s->ls_total = 0;
@@ -1359,6 +1391,7 @@ int cf_multi_update(cf_multi* m, float dt, uint32_t* num_fixed_steps, cf_step_st
             CF_TRY(launch_nr_finish(3 * s->N, s->x, s->v, s->xk, s->vk, dt, s->air_drag_per_second, s->stream));
             CF_TRY(cf_end_evaluation(s->cache));
             CF_CHECK_HIP(hipStreamSynchronize(s->stream));
+            nonfinite = nonfinite || s->nonfinite;
             if (stats) {
                 CF_TRY(solid_fill_stats(s, 0.0, 0.f, &stats[i]));
                 stats[i].nr_iters = int32_t(nr);
@@ -1370,25 +1403,36 @@ int cf_multi_update(cf_multi* m, float dt, uint32_t* num_fixed_steps, cf_step_st
         steps++;
     }
     if (num_fixed_steps) *num_fixed_steps = steps;
+    if (nonfinite) {
+        set_error("multi-solid step: a CG reduction is NaN or Inf (non-finite state or forces)");
+        return CF_ERR_NAN;
+    }
     return CF_OK;
 }
 
 // Step (:225-256): fixed-dt accumulator with strict '>'
+// Step (:225-256): scaled_dt = time_scale dt goes to the total time once per
+// call (:229-230) and to the accumulator; fixed steps of time_scale fixed_dt
+// run while the accumulator is strictly greater (:239)
 int cf_solid_step(cf_solid* s, float dt, cf_step_stats* stats) {
-    if (!s) return CF_ERR_INVALID_ARG;
+    if (!s || !(dt >= 0)) return CF_ERR_INVALID_ARG;
     const float scaled_dt = s->p.time_scale * dt;
+    s->total_time += scaled_dt;
     s->acc_time += scaled_dt;
     const float fixed = s->p.time_scale * s->p.fixed_dt;
     cf_step_stats last{};
     uint32_t n = 0;
+    int rc = CF_OK;
     while (s->acc_time > fixed) {
-        CF_TRY(cf_solid_fixed_step(s, fixed, &last));
+        rc = solid_fixed_step(s, fixed, &last);
+        if (rc != CF_OK && rc != CF_ERR_NAN) return rc;
         s->acc_time -= fixed;
         n++;
+        if (rc == CF_ERR_NAN) break;
     }
     last.num_fixed_steps = n;
     if (stats) *stats = last;
-    return CF_OK;
+    return rc;
 }
 
 int cf_solid_get_state(const cf_solid* s, float* pos, float* vel) {

@@ -359,6 +359,7 @@ struct Batch2Args {
     // node of collapse per triangle (tri_off, 3 = none); used by DAPD meshes
     float2* x0;
     uint8_t* noc;
+    int32_t* nonfinite;  // set to 1 by any mesh whose CG met a NaN or Inf r.r
 };
 
 int launch_batch2_step(const Batch2Args& a, uint32_t n_work, hipStream_t s);

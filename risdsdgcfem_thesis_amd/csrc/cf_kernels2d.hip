@@ -662,6 +662,7 @@ __device__ void mesh_step(const Batch2Args& a, uint32_t cmax, float2* s_v_all, f
         o.num_degenerate = uint32_t(nd);
         o.num_fallback = uint32_t(nf);
         o.min_det = md;
+        if (!(isfinite(delta0) && isfinite(delta))) *a.nonfinite = 1;
     }
 }
 

@@ -245,15 +245,25 @@ class TetrahedronElementSet3:
     """Element set + IForce3 cache. Vectors passed to the operator methods are
     DeviceArray objects of 3N floats; outputs accumulate (+=)."""
 
-    def __init__(self, ref_pos, tets, params=None, inv_mass=None):
+    def __init__(self, ref_pos, tets, params=None, inv_mass=None, incremental=False):
+        """incremental: build through BeginEdition / AddElement per tet /
+        EndEdition (TetrahedronElementSet3.cpp:2084-2166) instead of the bulk
+        cf_tetset_create; the two give the same set."""
         self.ref_pos = np.ascontiguousarray(ref_pos, np.float32)
         self.tets = np.ascontiguousarray(tets, np.uint32)
         self.N = self.ref_pos.size // 3
         self.T = self.tets.size // 4
         self.params = params if params is not None else default_fem_params()
         self.h = C.c_void_p()
-        check(lib.cf_tetset_create(self.N, _fp(self.ref_pos), self.T, _up(self.tets), C.byref(self.params),
-                                   C.byref(self.h)), "cf_tetset_create")
+        if incremental:
+            check(lib.cf_tetset_begin_edition(self.N, _fp(self.ref_pos), self.T, C.byref(self.params),
+                                              C.byref(self.h)), "cf_tetset_begin_edition")
+            for i, j, k, l in self.tets.reshape(-1, 4).tolist():
+                check(lib.cf_tetset_add_element(self.h, i, j, k, l), "cf_tetset_add_element")
+            check(lib.cf_tetset_end_edition(self.h), "cf_tetset_end_edition")
+        else:
+            check(lib.cf_tetset_create(self.N, _fp(self.ref_pos), self.T, _up(self.tets), C.byref(self.params),
+                                       C.byref(self.h)), "cf_tetset_create")
         self.c = C.c_void_p()
         im = None if inv_mass is None else np.ascontiguousarray(inv_mass, np.float32)
         check(lib.cf_cache_create(self.h, None if im is None else _fp(im), C.byref(self.c)), "cf_cache_create")
@@ -428,6 +438,11 @@ class LeafSolid3D:
         x = np.ascontiguousarray(x, np.float32)
         v = np.ascontiguousarray(v, np.float32)
         check(lib.cf_solid_set_state(self.h, _fp(x), _fp(v)), "cf_solid_set_state")
+
+    def total_time(self):
+        t = C.c_float()
+        check(lib.cf_solid_total_time(self.h, C.byref(t)), "cf_solid_total_time")
+        return t.value
 
     def info(self):
         nn, nt, m = C.c_uint32(), C.c_uint32(), C.c_float()
