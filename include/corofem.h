@@ -114,6 +114,17 @@ typedef struct cf_solid_params {
     int32_t integrator;        /* CF_IT_QIE_V (default) or CF_IT_FIE_DX (.h:109, dispatch .cpp:944-963) */
     uint32_t nr_max_iter;      /* FIE-dx Newton iterations (m_SolverNR_MaxIter, .h:233); default 3 */
     float nr_abs_epsilon;      /* FIE-dx stop: |b| < this (m_SolverNR_AbsEpsilon, .h:236-238); default 1e-6 */
+    /* 1: 3D leaves compute each fixed step in the reference's FP32 order, bit for
+       bit: stored 16-block Ke per element (TetrahedronElement3::Compute_K), the
+       element-ordered serial scatter of C_f_e / C_df_x
+       (TetrahedronElementSet3.cpp:1053-1096,1761-1836), ns::real_array's SSE dot
+       order with FP32 CG scalars (ILSS_CG.h:38-151) and the QIE-v vector terms
+       (LeafDSH_Solid3D_FEM.cpp:1205-1328). QIE-v, C_WRP, DM Truncated only
+       (CF_ERR_UNSUPPORTED otherwise); rotations by DAPD are bit-exact, by MSVD
+       within the SVD's ulps. For parity checks: 576 B of Ke per element and one
+       host round trip per CG dot product. Default 0 (the fast path: FP64
+       reductions, Piola-form element forces). */
+    int32_t reference_order;
 } cf_solid_params;
 
 /* Integrators (LeafDSH_Solid3D_FEM.h:109). CF_IT_FIE_DX is
@@ -228,6 +239,10 @@ int cf_tetset_destroy(cf_tetset* set);
 int cf_tetset_info(const cf_tetset* set, uint32_t* n_nodes, uint32_t* n_tets, float* total_volume);
 /* Per-element precompute in the caller's element order: h_inv_dm (9T, row-major), h_vol (T). */
 int cf_tetset_get_elements(const cf_tetset* set, float* h_inv_dm, float* h_vol);
+/* TetrahedronElement3::Compute_K (ms/fem/TetrahedronElement3.h:85-173) for one
+   element with corners r (12 floats): the 12x12 stiffness matrix, row-major, in
+   the reference's FP32 order, and its volume. Host code; needs no device. */
+int cf_tet_compute_k(const float h_r[12], float young_modulus, float poisson_ratio, float* h_K144, float* volume);
 
 /* ---------------------------------------------------------------- IForce3 cache
  * CreateCache(inv_mass) / BeginEvaluation / f / df_x / V / EndEvaluation

@@ -36,6 +36,7 @@ class SolidParams(C.Structure):
         ("cg_abs_epsilon", C.c_float), ("cg_max_iter", C.c_uint32), ("warm_start", C.c_int32),
         ("compute_energy", C.c_int32), ("thickness", C.c_float),
         ("integrator", C.c_int32), ("nr_max_iter", C.c_uint32), ("nr_abs_epsilon", C.c_float),
+        ("reference_order", C.c_int32),
     ]
 
 
@@ -128,6 +129,7 @@ SIGNATURES = {
     "cf_tetset_destroy": (C.c_int, [_vp]),
     "cf_tetset_info": (C.c_int, [_vp, _u, _u, _f]),
     "cf_tetset_get_elements": (C.c_int, [_vp, _f, _f]),
+    "cf_tet_compute_k": (C.c_int, [_f, C.c_float, C.c_float, _f, _f]),
     "cf_cache_create": (C.c_int, [_vp, _f, C.POINTER(_vp)]),
     "cf_cache_destroy": (C.c_int, [_vp]),
     "cf_begin_evaluation": (C.c_int, [_vp, _vp, _vp, _vp, C.c_float]),

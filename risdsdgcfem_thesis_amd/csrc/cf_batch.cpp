@@ -771,6 +771,7 @@ int cf_batch_add_sim(cf_batch* b, uint32_t mesh_id, const cf_solid_params* p, ui
         return CF_ERR_UNSUPPORTED;
     }
     if (!(p->fixed_dt > 0) || !(p->density > 0) || !(p->time_scale > 0)) { set_error("fixed_dt, time_scale and density must be > 0"); return CF_ERR_INVALID_ARG; }
+    if (p->reference_order) { set_error("batches take the fast path (reference_order = 0)"); return CF_ERR_UNSUPPORTED; }
     if (!(p->fem.young_modulus > 0) || !(p->fem.poisson_ratio > -1.0f) || !(p->fem.poisson_ratio < 0.5f) ||
         p->fem.mm != CF_MM_C_WRP || p->fem.dm != CF_DM_TRUNCATED ||
         (p->fem.rm != CF_RM_ID && p->fem.rm != CF_RM_MSVD && p->fem.rm != CF_RM_DAPD)) {

@@ -303,6 +303,10 @@ int cf_batch2_add_sim(cf_batch2* b, uint32_t N, const float* ref, uint32_t T, co
         set_error("%s: only the QIE-v integrator is batched", "cf_batch2_add_sim");
         return CF_ERR_UNSUPPORTED;
     }
+    if (p->reference_order) {
+        set_error("cf_batch2_add_sim: 2D solids take the resident step kernel (reference_order = 0)");
+        return CF_ERR_UNSUPPORTED;
+    }
     if (!(p->fixed_dt > 0) || !(p->density > 0) || !(p->time_scale > 0) || !(p->thickness > 0) ||
         !(p->fem.young_modulus > 0) || !(p->fem.poisson_ratio > -1.0f) || !(p->fem.poisson_ratio < 0.5f) ||
         p->fem.mm != CF_MM_C_WRP || p->fem.dm != CF_DM_TRUNCATED ||

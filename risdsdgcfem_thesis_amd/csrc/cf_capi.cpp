@@ -183,6 +183,7 @@ struct cf_solid {
     cf_cg_result last_ls{};               // last linear solve
     int32_t ls_total = 0;                 // linear-solve iterations of the last fixed step
     bool nonfinite = false;               // a reduction of the last fixed step was NaN or Inf
+    ExactDevice ex;                       // reference_order: the reference-order FP32 step's data
     CgGraphs graphs;                      // recorded CG chunks (multi-solid step)
 };
 
@@ -622,6 +623,7 @@ void cf_default_solid_params(cf_solid_params* p) {
     p->integrator = CF_IT_QIE_V;
     p->nr_max_iter = 3;
     p->nr_abs_epsilon = 1e-6f;
+    p->reference_order = 0;
 }
 
 int cf_device_count(int* n) {
@@ -806,6 +808,13 @@ int cf_tetset_destroy(cf_tetset* s) {
     if (!s) return CF_OK;
     tetset_free(s);
     delete s;
+    return CF_OK;
+}
+
+int cf_tet_compute_k(const float* r, float E, float nu, float* K, float* volume) {
+    if (!r || !K) { set_error("cf_tet_compute_k: null argument"); return CF_ERR_INVALID_ARG; }
+    const float v = exact_element_stiffness(r, r + 3, r + 6, r + 9, E, nu, K);
+    if (volume) *volume = v;
     return CF_OK;
 }
 
@@ -1116,6 +1125,7 @@ int cf_solid_destroy(cf_solid* s) {
     if (s->ev0) (void)hipEventDestroy(s->ev0);
     if (s->ev1) (void)hipEventDestroy(s->ev1);
     s->graphs.clear();
+    exact_free(s->ex);
     cf_cache_destroy(s->cache);
     cf_tetset_destroy(s->set);
     delete s;
@@ -1128,6 +1138,11 @@ int cf_solid_create(uint32_t N, const float* ref, uint32_t T, const uint32_t* te
     if (!(p->fixed_dt > 0) || !(p->density > 0)) { set_error("fixed_dt and density must be > 0"); return CF_ERR_INVALID_ARG; }
     if (p->integrator != CF_IT_QIE_V && p->integrator != CF_IT_FIE_DX) {
         set_error("integrator %d is not supported (CF_IT_QIE_V or CF_IT_FIE_DX)", p->integrator);
+        return CF_ERR_UNSUPPORTED;
+    }
+    if (p->reference_order &&
+        (p->integrator != CF_IT_QIE_V || p->fem.mm != CF_MM_C_WRP || p->fem.dm != CF_DM_TRUNCATED)) {
+        set_error("reference_order takes QIE-v, C_WRP and DM Truncated");
         return CF_ERR_UNSUPPORTED;
     }
     *out = nullptr;
@@ -1171,6 +1186,11 @@ int cf_solid_create(uint32_t N, const float* ref, uint32_t T, const uint32_t* te
     s->node_mass = 1.0f / inv_mass;
     rayleigh_coeffs(p->fem.rayleigh_freq1, p->fem.rayleigh_freq2, p->fem.damping_ratio, s->rayleigh);
     s->air_drag_per_second = -std::log(10e-6f) * p->air_drag_coeff;  // RebuildAirDrag (:1586-1589)
+    if (p->reference_order) {
+        rc = exact_build(N, ref, T, tets, s->set->perm.data(), p->fem.young_modulus, p->fem.poisson_ratio, s->stream,
+                         s->ex);
+        if (rc != CF_OK) { cf_solid_destroy(s); return rc; }
+    }
     *out = s;
     return CF_OK;
 }
@@ -1196,6 +1216,99 @@ int cf_solid_set_knc(cf_solid* s, uint32_t n, const uint32_t* nids, const float*
     return CF_OK;
 }
 
+// FixedStep with QIE-v in the reference's FP32 order (reference_order = 1):
+// BeginEvaluation as the fast path (bit-exact F, det F and DAPD rotations), then
+// b, y, the CG and the integration term by term as Step_QuasiImplicitEuler_v
+// (:1209-1327), LS_Solve (:1081-1155) and ILSS_CG::Solve (ns/ILSS_CG.h:38-151)
+// compute them, with float scalars on the host.
+static int solid_fixed_step_exact(cf_solid* s, float dt, cf_step_stats* stats) {
+#pragma clang fp contract(off)
+    if (!(dt > 0)) return CF_ERR_INVALID_ARG;
+    s->nonfinite = false;
+    const TetDevice& t = s->set->dev;
+    const ExactDevice& ex = s->ex;
+    hipStream_t st = s->stream;
+    const uint32_t n3 = 3 * s->N;
+    CgBuffers& b = s->cg;
+    CF_CHECK_HIP(hipEventRecord(s->ev0, st));
+    CF_TRY(cf_begin_evaluation(s->cache, s->x0, s->x, s->v, s->total_time));
+    CF_CHECK_HIP(hipMemcpyAsync(s->x0, s->x, n3 * 4, hipMemcpyDeviceToDevice, st));
+    const float* R = s->cache->R;
+    const uint8_t* knc = s->num_knc ? s->knc_mask : nullptr;
+    // b = dt (M g + f_e) + M v, y = v, KNC rows (:1209-1295)
+    CF_TRY(launch_exact_elem(ex, kExactFe, R, t.T, t.ref, s->x, nullptr, st));
+    ExactNodeTerms rhs;
+    rhs.knc = knc; rhs.knc_target = s->knc_target; rhs.x = s->x; rhs.v = s->v; rhs.out = s->bvec; rhs.y = s->y;
+    rhs.mass = s->node_mass; rhs.dt = dt; rhs.inv_dt = 1.0f / dt; rhs.warm_start = s->p.warm_start;
+    for (int c = 0; c < 3; c++) rhs.g[c] = s->p.gravity[c];
+    CF_TRY(launch_exact_node(ex, kExactRhs, rhs, st));
+    // A = (1 + dt c0) M + (dt^2 + dt c1) K (:1300-1301); LS_A_times_d scales K d
+    // by -factor_K and adds (m factor_M) d (:1056-1079)
+    const float fM = 1.0f + dt * s->rayleigh[0];
+    const float fK = dt * dt + dt * s->rayleigh[1];
+    ExactNodeTerms av;
+    av.knc = knc; av.neg_fk = -fK; av.m_fm = s->node_mass * fM;
+    const float rel = s->p.cg_rel_epsilon, abs_sq = s->p.cg_abs_epsilon * s->p.cg_abs_epsilon;
+    // r = S(b - A y); d = r; delta0 = r.r (ILSS_CG.h:53-65)
+    CF_TRY(launch_exact_elem(ex, kExactDfx, R, t.T, t.ref, s->y, nullptr, st));
+    av.d = s->y; av.out = b.r;
+    CF_TRY(launch_exact_node(ex, kExactAv, av, st));
+    CF_TRY(launch_exact_vec(0, n3, 0.0f, knc, nullptr, b.r, b.d, s->bvec, st));
+    float delta0 = 0.0f;
+    CF_TRY(exact_dot(ex, 0, n3, b.r, b.r, st, &delta0));
+    float dnew = delta0;
+    const float target = (rel * rel) * delta0;
+    bool finite = std::isfinite(delta0);
+    uint32_t k = 0;
+    int reason = 0;
+    av.d = b.d; av.out = b.q;
+    while (dnew > abs_sq && dnew > target && k < s->p.cg_max_iter) {
+        CF_TRY(launch_exact_elem(ex, kExactDfx, R, t.T, t.ref, b.d, nullptr, st));
+        CF_TRY(launch_exact_node(ex, kExactAv, av, st));
+        float dq = 0.0f;
+        CF_TRY(exact_dot(ex, 0, n3, b.d, b.q, st, &dq));
+        finite = finite && std::isfinite(dq);
+        if (std::fabs(dq) < abs_sq) { reason = CF_CG_BREAKDOWN; break; }  // ILSS_CG.h:97
+        const float alpha = dnew / dq;
+        CF_TRY(launch_exact_vec(1, n3, alpha, knc, s->y, b.r, b.d, b.q, st));
+        const float dold = dnew;
+        CF_TRY(exact_dot(ex, 0, n3, b.r, b.r, st, &dnew));
+        finite = finite && std::isfinite(dnew);
+        const float beta = dnew / dold;
+        CF_TRY(launch_exact_vec(2, n3, beta, knc, nullptr, b.r, b.d, nullptr, st));
+        k++;
+    }
+    if (!reason) reason = !(dnew > abs_sq) ? CF_CG_CONVERGED_ABS : !(dnew > target) ? CF_CG_CONVERGED_REL : CF_CG_MAX_ITER;
+    if (!finite) { reason = CF_CG_NONFINITE; s->nonfinite = true; }
+    // v = y, x = x + dt y (:1322-1327)
+    CF_TRY(launch_exact_vec(3, n3, dt, nullptr, s->x, s->v, nullptr, s->y, st));
+    // V() with the new x (:977; C_V :1456-1488): 0.5 * the element-ordered sum
+    float energy = 0.0f;
+    if (s->p.compute_energy) {
+        CF_TRY(launch_exact_elem(ex, kExactV, R, t.T, t.ref, s->x, nullptr, st));
+        CF_TRY(exact_dot(ex, 1, 4 * t.T, ex.w, nullptr, st, &energy));
+    }
+    CF_TRY(cf_end_evaluation(s->cache));
+    CF_CHECK_HIP(hipEventRecord(s->ev1, st));
+    CF_CHECK_HIP(hipEventSynchronize(s->ev1));
+    s->last_ls.iterations = int32_t(k);
+    s->last_ls.reason = reason;
+    s->last_ls.rel_residual = (delta0 > 0) ? std::sqrt(dnew / delta0) : 0.0f;
+    s->last_ls.abs_residual = std::sqrt(dnew);
+    s->ls_total = int32_t(k);
+    if (stats) {
+        float ms = 0.f;
+        CF_CHECK_HIP(hipEventElapsedTime(&ms, s->ev0, s->ev1));
+        CF_TRY(solid_fill_stats(s, energy, ms, stats));
+        stats->nr_iters = 0;
+        stats->nr_rel_residual = 0.f;
+        stats->nr_abs_residual = 0.f;
+    }
+    if (!s->nonfinite) return CF_OK;
+    set_error("fixed step: a CG reduction is NaN or Inf (non-finite state or forces)");
+    return CF_ERR_NAN;
+}
+
 static int nonfinite_status(cf_solid* s) {
     if (!s->nonfinite) return CF_OK;
     set_error("fixed step: a CG or Newton reduction is NaN or Inf (non-finite state or forces)");
@@ -1206,6 +1319,7 @@ static int nonfinite_status(cf_solid* s) {
 // per call (:229-230), a direct fixed step by its own dt.
 static int solid_fixed_step(cf_solid* s, float dt, cf_step_stats* stats) {
     if (!s || !(dt >= 0) || (dt == 0 && s->p.integrator != CF_IT_FIE_DX)) return CF_ERR_INVALID_ARG;
+    if (s->p.reference_order) return solid_fixed_step_exact(s, dt, stats);
     s->nonfinite = false;
     cf_tetset* set = s->set;
     const TetDevice& t = set->dev;
@@ -1302,6 +1416,11 @@ int cf_multi_create(cf_solid* const* solids, uint32_t n, cf_multi** out) {
     if (!m) return CF_ERR_OUT_OF_MEMORY;
     for (uint32_t i = 0; i < n; i++) {
         if (!solids[i]) { delete m; set_error("cf_multi_create: solid %u is null", i); return CF_ERR_INVALID_ARG; }
+        if (solids[i]->p.reference_order) {
+            delete m;
+            set_error("cf_multi_create: solid %u uses reference_order (leaf steps only)", i);
+            return CF_ERR_UNSUPPORTED;
+        }
         m->solids.push_back(solids[i]);
     }
     *out = m;

@@ -77,6 +77,46 @@ struct TetDevice {
     float* partials = nullptr;          // 3 * num_parts
 };
 
+// Reference-order FP32 evaluation of one solid (cf_exact.hip): stored Ke blocks
+// in caller element order, node -> (element, vertex) runs in ascending element
+// order for the serial scatter, and the scratch of the element and dot kernels.
+struct ExactDevice {
+    uint32_t N = 0, T = 0;
+    float* kb = nullptr;        // 144 planes of T: block (bi, bj) entry (r, c) at plane (4 bi + bj) 9 + 3 r + c
+    uint32_t* tets = nullptr;   // 4T node ids, caller order
+    uint32_t* tile_of = nullptr;  // T: caller element -> index of its R in the cache planes
+    uint32_t* ptr = nullptr;    // N+1: node n's incident (element, vertex) codes are [ptr[n], ptr[n+1])
+    uint32_t* code = nullptr;   // 4T codes 4 e + vertex, ascending e per node
+    float* w = nullptr;         // 12T vertex terms (also 4T energy terms)
+    float* scal = nullptr;      // the dot kernel's result
+};
+enum ExactElemMode { kExactFe = 0, kExactDfx = 1, kExactV = 2 };
+enum ExactNodeMode { kExactRhs = 0, kExactAv = 1 };
+struct ExactNodeTerms {
+    const uint8_t* knc = nullptr;
+    const float* knc_target = nullptr;
+    const float *x = nullptr, *v = nullptr, *d = nullptr;
+    float* out = nullptr;
+    float* y = nullptr;
+    float mass = 0.f, dt = 0.f, inv_dt = 0.f, g[3] = {0, 0, 0}, neg_fk = 0.f, m_fm = 0.f;
+    int warm_start = 1;
+};
+int exact_build(uint32_t N, const float* ref, uint32_t T, const uint32_t* tets, const uint32_t* perm, float E,
+                float nu, hipStream_t st, ExactDevice& out);
+void exact_free(ExactDevice& d);
+// Element terms into `out` (null: d.w); R: the cache's rotation planes, r_stride apart.
+int launch_exact_elem(const ExactDevice& d, int mode, const float* R, uint32_t r_stride, const float* ref,
+                      const float* in, float* out, hipStream_t s);
+int launch_exact_node(const ExactDevice& d, int mode, const ExactNodeTerms& t, hipStream_t s);
+// mode 0: ns::real_array::dot(a, b, n); mode 1: 0.5 * the ordered sum of a[0..n).
+// Synchronizes the stream and returns the value on the host.
+int exact_dot(const ExactDevice& d, int mode, uint32_t n, const float* a, const float* b, hipStream_t s, float* h_out);
+int launch_exact_vec(int mode, uint32_t n3, float scalar, const uint8_t* knc, float* x, float* r, float* dvec,
+                     const float* q, hipStream_t s);
+// TetrahedronElement3::Compute_K in the reference's order (host); returns the volume.
+float exact_element_stiffness(const float* r0, const float* r1, const float* r2, const float* r3, float E, float nu,
+                              float K[144]);
+
 // Host-side tile construction (cf_tiles.cpp).
 struct TileHost {
     std::vector<uint32_t> perm;          // internal element -> caller element

@@ -89,3 +89,48 @@ def test_no_device_is_an_error_not_a_crash(cf):
     pos, tets = cf.mesh_box((0.01, 0.01, 0.01), (2, 2, 2))
     with pytest.raises(cf.CorofemError):
         cf.TetrahedronElementSet3(pos, tets)
+
+
+def test_tet_compute_k_reference_order(cf, orc):
+    """cf_tet_compute_k (the Ke the reference-order leaf mode stores) equals
+    TetrahedronElement3::Compute_K bit for bit: the golden vectors written with
+    the compiled reference, and the oracle's restatement on fresh elements
+    (and the compiled reference itself where oracle/_ref is built)."""
+    from conftest import golden
+    g = golden("ke_tets.npz")
+    K = np.zeros(144, np.float32)
+    v = C.c_float()
+    fp = lambda a: a.ctypes.data_as(C.POINTER(C.c_float))
+    for r, E, nu, Kw, Vw in zip(g["r"], g["E"], g["nu"], g["K"], g["V"]):
+        r = np.ascontiguousarray(r, np.float32)
+        cf.check(cf.lib.cf_tet_compute_k(fp(r), float(E), float(nu), fp(K), C.byref(v)))
+        assert np.array_equal(K, Kw) and np.float32(v.value) == Vw
+    rng = np.random.default_rng(12)
+    ref = orc.ref()
+    for _ in range(200):
+        r = rng.uniform(-0.05, 0.05, 12).astype(np.float32)
+        if np.linalg.det((r.reshape(4, 3)[1:] - r.reshape(4, 3)[0]).T) <= 1e-7:
+            continue
+        E, nu = float(rng.uniform(1e3, 1e7)), float(rng.uniform(0.0, 0.49))
+        cf.check(cf.lib.cf_tet_compute_k(fp(r), E, nu, fp(K), C.byref(v)))
+        Ko = np.zeros(144, np.float32)
+        vo = orc.lib().orc_compute_K(fp(r), E, nu, fp(Ko))
+        assert np.array_equal(K, Ko) and np.float32(v.value) == np.float32(vo)
+        if ref is not None:
+            Kr = np.zeros(144, np.float32)
+            ref.ref_compute_K(fp(r), E, nu, fp(Kr))
+            assert np.array_equal(K, Kr)
+
+
+def test_reference_order_rejects_other_paths(cf):
+    """The reference-order leaf mode takes QIE-v, C_WRP and DM Truncated only:
+    other combinations are refused before any device work."""
+    pos, tets = cf.mesh_box((0.01, 0.01, 0.01), (2, 2, 2))
+    fp = lambda a: a.ctypes.data_as(C.POINTER(C.c_float))
+    up = lambda a: a.ctypes.data_as(C.POINTER(C.c_uint32))
+    for kw in (dict(integrator=cf.IT_FIE_DX), dict(mm=cf.MM_C_LCM), dict(dm=cf.DM_EXACT)):
+        p = cf.default_solid_params(reference_order=1, **kw)
+        h = C.c_void_p()
+        rc = cf.lib.cf_solid_create(pos.size // 3, fp(pos), tets.size // 4, up(tets), C.byref(p), C.byref(h))
+        assert rc == cf.ERR_UNSUPPORTED, (kw, rc)
+    assert cf.default_solid_params().reference_order == 0
