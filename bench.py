@@ -31,32 +31,14 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
+from risdsdgcfem_thesis_amd.workloads import large_mesh_state  # noqa: E402
+
 HBM_PEAK_GBS = 8000.0
 METRIC = "K·x Gtets/s (%HBM peak); steps/s batched vs plain at 1e-4 residual; p99 ms"
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
-
-
-def uniform_so3(rng):
-    u1, u2, u3 = rng.random(3)
-    x, y, z, w = (np.sqrt(1 - u1) * np.sin(2 * np.pi * u2), np.sqrt(1 - u1) * np.cos(2 * np.pi * u2),
-                  np.sqrt(u1) * np.sin(2 * np.pi * u3), np.sqrt(u1) * np.cos(2 * np.pi * u3))
-    return np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)],
-                     [2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)],
-                     [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)]])
-
-
-def config2_state(pos, seed=2, edge=0.01):
-    """x0 = Q diag(s) X + jitter (one global rotation/stretch draw)."""
-    rng = np.random.Generator(np.random.Philox(seed))
-    Q = uniform_so3(rng)
-    s = rng.uniform(0.7, 1.3, 3)
-    X = pos.reshape(-1, 3).astype(np.float64)
-    x = X @ (Q @ np.diag(s)).T
-    x += rng.uniform(-0.01 * edge, 0.01 * edge, x.shape)
-    return x.astype(np.float32).ravel()
 
 
 def load_traffic():
@@ -74,7 +56,7 @@ def cpu_baseline(budget_s=12.0):
     pos, tets = orc.mesh_box((0.32, 0.32, 0.32), (65, 65, 65))
     T = tets.size // 4
     ts = orc.TetSet(pos, tets, 1e5, 0.3)
-    x = config2_state(pos)
+    x = large_mesh_state(pos)
     ts.begin_evaluation(x, rm=orc.RM_DAPD)
     d = np.random.default_rng(1).standard_normal(pos.size).astype(np.float32)
     df = np.zeros_like(d)
@@ -348,7 +330,7 @@ def bench_single(cf, args):
     half = (0.32, 0.32, 1.28)
     pos, tets = cf.mesh_box(half, counts)
     N, T = pos.size // 3, tets.size // 4
-    x0 = config2_state(pos)
+    x0 = large_mesh_state(pos)
     knc = np.nonzero(np.isclose(pos[2::3], -half[2]))[0].astype(np.uint32)
     params = cf.default_solid_params(fixed_dt=1.0 / 60.0, gravity=(0.0, -9.81, 0.0), young_modulus=1e5,
                                      poisson_ratio=0.3, damping_ratio=0.0, rm=cf.RM_DAPD, cg_rel_epsilon=1e-4,
@@ -426,7 +408,7 @@ def bench_fie(cf, args):
     half = (0.32, 0.32, 1.28)
     pos, tets = cf.mesh_box(half, counts)
     N, T = pos.size // 3, tets.size // 4
-    x0 = config2_state(pos)
+    x0 = large_mesh_state(pos)
     knc = np.nonzero(np.isclose(pos[2::3], -half[2]))[0].astype(np.uint32)
     params = cf.default_solid_params(fixed_dt=1.0 / 60.0, gravity=(0.0, -9.81, 0.0), young_modulus=1e5,
                                      poisson_ratio=0.3, damping_ratio=0.0, rm=cf.RM_DAPD, cg_rel_epsilon=1e-4,

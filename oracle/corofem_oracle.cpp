@@ -1201,6 +1201,7 @@ struct orc_solid {
     float inv_mass = 0.0f;         // uniform
     float c[2] = {0, 0};           // Rayleigh
     std::vector<uint32_t> knc;
+    std::vector<float> knc_pos;    // 3N: each KNC node's target position (KNC::m_Pos), the rest pose by default
     std::vector<float> b, y, r, d, q;
     std::vector<float> xk, vk;     // FIE-dx iterates (m_IE_vec_x_k, m_IE_vec_v_k)
     float acc_time = 0.0f;
@@ -1270,8 +1271,8 @@ void fie_rhs(orc_solid* s, float dt, float inv_dt, const float* x_eval, bool fir
     for (uint32_t nid : s->knc)
         for (int c = 0; c < 3; c++) {
             const size_t i = 3 * size_t(nid) + c;
-            s->b[i] = s->tes->ref[i] - s->xk[i];
-            s->y[i] = s->tes->ref[i] - s->xk[i];
+            s->b[i] = s->knc_pos[i] - s->xk[i];
+            s->y[i] = s->knc_pos[i] - s->xk[i];
         }
 }
 
@@ -1307,7 +1308,15 @@ orc_solid* orc_solid_create(uint32_t n_nodes, const float* ref_pos, uint32_t n_t
 }
 
 void orc_solid_destroy(orc_solid* s) { orc_tetset_destroy(s->tes); delete s; }
-void orc_solid_set_knc(orc_solid* s, uint32_t n, const uint32_t* nids) { s->knc.assign(nids, nids + n); }
+void orc_solid_set_knc(orc_solid* s, uint32_t n, const uint32_t* nids) {
+    s->knc.assign(nids, nids + n);
+    s->knc_pos = s->tes->ref;
+}
+void orc_solid_set_knc_targets(orc_solid* s, uint32_t n, const uint32_t* nids, const float* targets) {
+    orc_solid_set_knc(s, n, nids);
+    for (uint32_t i = 0; i < n; i++)
+        for (int c = 0; c < 3; c++) s->knc_pos[3 * size_t(nids[i]) + c] = targets[3 * size_t(i) + c];
+}
 float orc_solid_node_mass(const orc_solid* s) { return 1.0f / s->inv_mass; }
 void orc_solid_rayleigh(const orc_solid* s, float c[2]) { c[0] = s->c[0]; c[1] = s->c[1]; }
 
@@ -1384,10 +1393,10 @@ static void solid_fixed_step(orc_solid* s, float dt, orc_step_stats* st) {
     for (uint32_t i = 0; i < N; i++)
         for (int c = 0; c < 3; c++) s->b[3 * i + c] += mass * s->v[3 * i + c];
     if (s->p.warm_start) s->y = s->v; else std::fill(s->y.begin(), s->y.end(), 0.0f);
-    // KNC (:1268-1295): position constraints at the rest pose
+    // KNC (:1268-1295): position constraints at their targets
     for (uint32_t nid : s->knc) {
         V3 vk;
-        for (int c = 0; c < 3; c++) vk.v[c] = (s->tes->ref[3 * size_t(nid) + c] - s->x[3 * size_t(nid) + c]) * inv_dt;
+        for (int c = 0; c < 3; c++) vk.v[c] = (s->knc_pos[3 * size_t(nid) + c] - s->x[3 * size_t(nid) + c]) * inv_dt;
         float nk = vnorm(vk);
         V3 kv;
         if (nk > 0.001f) kv = vscale(vnormalized(vk), std::min(nk, 50.0f));

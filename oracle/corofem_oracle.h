@@ -148,7 +148,9 @@ typedef struct orc_solid orc_solid;
 orc_solid* orc_solid_create(uint32_t n_nodes, const float* ref_pos, uint32_t n_tets,
                             const uint32_t* tets, const orc_solid_params* p);
 void orc_solid_destroy(orc_solid* s);
-void orc_solid_set_knc(orc_solid* s, uint32_t n, const uint32_t* nids);
+void orc_solid_set_knc(orc_solid* s, uint32_t n, const uint32_t* nids);  /* held at the rest pose */
+/* KNC held at given targets (3 floats per listed node), like SetPosKNC */
+void orc_solid_set_knc_targets(orc_solid* s, uint32_t n, const uint32_t* nids, const float* targets);
 /* One FixedStep(dt); advances the solid's total time by dt. */
 void orc_solid_fixed_step(orc_solid* s, float dt, orc_step_stats* st);
 /* Step(dt) (LeafDSH_Solid3D_FEM.cpp:225-256): total time += time_scale dt,

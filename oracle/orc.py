@@ -96,6 +96,7 @@ def _load():
         "orc_solid_create": (C.c_void_p, [C.c_uint32, _f, C.c_uint32, _u, C.POINTER(SolidParams)]),
         "orc_solid_destroy": (None, [C.c_void_p]),
         "orc_solid_set_knc": (None, [C.c_void_p, C.c_uint32, _u]),
+        "orc_solid_set_knc_targets": (None, [C.c_void_p, C.c_uint32, _u, _f]),
         "orc_solid_fixed_step": (None, [C.c_void_p, C.c_float, C.POINTER(StepStats)]),
         "orc_solid_step": (None, [C.c_void_p, C.c_float, C.POINTER(StepStats)]),
         "orc_solid_total_time": (C.c_float, [C.c_void_p]),
@@ -405,7 +406,7 @@ class Multi:
 class Solid:
     """Oracle leaf solid (LeafDSH_Solid3D_FEM: QIE-v, or FIE-dx by params.integrator)."""
 
-    def __init__(self, ref_pos, tets, params, knc=None):
+    def __init__(self, ref_pos, tets, params, knc=None, knc_targets=None):
         self.ref_pos = np.ascontiguousarray(ref_pos, np.float32)
         self.tets = np.ascontiguousarray(tets, np.uint32)
         self.N = self.ref_pos.size // 3
@@ -414,7 +415,11 @@ class Solid:
                                         C.byref(params))
         if knc is not None:
             k = np.ascontiguousarray(knc, np.uint32)
-            lib().orc_solid_set_knc(self.h, k.size, up(k))
+            if knc_targets is None:
+                lib().orc_solid_set_knc(self.h, k.size, up(k))
+            else:
+                t = np.ascontiguousarray(knc_targets, np.float32)
+                lib().orc_solid_set_knc_targets(self.h, k.size, up(k), fp(t))
 
     def __del__(self):
         if getattr(self, "h", None):

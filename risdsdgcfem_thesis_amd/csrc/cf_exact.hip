@@ -247,7 +247,8 @@ __global__ __launch_bounds__(kBlock) void k_exact_node(int mode, uint32_t N, Exa
 __global__ __launch_bounds__(kBlock) void k_exact_dot(int mode, uint32_t n, const float* __restrict__ a,
                                                       const float* __restrict__ b, float* __restrict__ out) {
 #pragma clang fp contract(off)
-    __shared__ float chunk[kBlock];
+    __shared__ float4 chunk4[kBlock / 4];
+    float* chunk = reinterpret_cast<float*>(chunk4);
     const uint32_t nchunk = mode == 1 ? n : (n + 3) / 4;
     float acc = 0.0f;
     for (uint32_t base = 0; base < nchunk; base += kBlock) {
@@ -267,8 +268,18 @@ __global__ __launch_bounds__(kBlock) void k_exact_dot(int mode, uint32_t n, cons
         }
         __syncthreads();
         if (threadIdx.x == 0) {
+            // in chunk order; the float4 reads only let the loads run ahead of the adds
             const uint32_t m = std::min<uint32_t>(kBlock, nchunk - base);
-            for (uint32_t j = 0; j < m; j++) acc = acc + chunk[j];
+            uint32_t j = 0;
+#pragma unroll 8
+            for (; j + 4 <= m; j += 4) {
+                const float4 c = chunk4[j / 4];
+                acc = acc + c.x;
+                acc = acc + c.y;
+                acc = acc + c.z;
+                acc = acc + c.w;
+            }
+            for (; j < m; j++) acc = acc + chunk[j];
         }
         __syncthreads();
     }

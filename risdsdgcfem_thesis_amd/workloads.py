@@ -38,23 +38,56 @@ def solid_params(E, nu, cg_max_iter, rm=RM_DAPD):
                                 cg_abs_epsilon=1e-6, cg_max_iter=int(cg_max_iter), density=1000.0)
 
 
-def uneven_batch(count=1024, lo=4, hi=40, seed=4, cg_max_iter=50):
+def uniform_so3(g):
+    """A rotation uniform on SO(3) from three uniforms (Shoemake's quaternion method)."""
+    u1, u2, u3 = g.random(3)
+    x, y, z, w = (np.sqrt(1 - u1) * np.sin(2 * np.pi * u2), np.sqrt(1 - u1) * np.cos(2 * np.pi * u2),
+                  np.sqrt(u1) * np.sin(2 * np.pi * u3), np.sqrt(u1) * np.cos(2 * np.pi * u3))
+    return np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)],
+                     [2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)],
+                     [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)]])
+
+
+def large_mesh_state(pos, seed=2, edge=EDGE):
+    """Initial pose of the single large mesh: x0 = Q diag(s) X with one seeded
+    uniform rotation Q and stretches s uniform in [0.7, 1.3], plus a per-node
+    jitter uniform within 1% of the edge length."""
+    g = rng(seed)
+    Q = uniform_so3(g)
+    s = g.uniform(0.7, 1.3, 3)
+    X = pos.reshape(-1, 3).astype(np.float64)
+    x = X @ (Q @ np.diag(s)).T
+    x += g.uniform(-0.01 * edge, 0.01 * edge, x.shape)
+    return x.astype(np.float32).ravel()
+
+
+def large_mesh(cubes=(64, 64, 256)):
+    """The single large mesh (64x64x256 cubes: 5,242,880 tets, 1,085,825 nodes
+    at the default size), its initial pose and its z-min face (held at that pose)."""
+    pos, tets, half = box_mesh(cubes)
+    x0 = large_mesh_state(pos)
+    knc = face_nodes(pos, half, (2, -1))
+    return pos, tets, x0, knc
+
+
+def uneven_batch(count=1024, lo=4, hi=40, seed=4, cg_max_iter=50, only=None):
     """3D uneven batch: `count` cube grids with each axis' cube count uniform
     in [lo, hi], E log-uniform in [1e4, 1e7] Pa, nu uniform in [0.2, 0.45], one
     random face held at rest. Returns a list of dicts (pos, tets, knc, params,
-    cubes, face)."""
+    cubes, face, index); only: the indices of the meshes to build (the draws
+    are those of the whole batch)."""
     g = rng(seed)
     cubes = g.integers(lo, hi + 1, size=(count, 3))
     E = np.exp(g.uniform(np.log(1e4), np.log(1e7), size=count))
     nu = g.uniform(0.2, 0.45, size=count)
     faces = g.integers(0, 6, size=count)
     out = []
-    for i in range(count):
+    for i in (range(count) if only is None else only):
         pos, tets, half = box_mesh(cubes[i])
         face = FACES[int(faces[i])]
         out.append({"pos": pos, "tets": tets, "knc": face_nodes(pos, half, face),
                     "params": solid_params(E[i], nu[i], cg_max_iter), "cubes": tuple(int(c) for c in cubes[i]),
-                    "face": face, "E": float(E[i]), "nu": float(nu[i])})
+                    "face": face, "E": float(E[i]), "nu": float(nu[i]), "index": int(i)})
     return out
 
 
