@@ -299,6 +299,22 @@ void* cf_cg_stream(cf_cg* cg);
 int cf_cg_solve(cf_cg* cg, cf_av_fn av, cf_constraints_fn apply_constraints, void* user,
                 float* d_x, const float* d_b, float rel_epsilon, float abs_epsilon, uint32_t max_iterations,
                 cf_cg_result* result);
+/* No counterpart in the reference: cf_cg_solve for callers that cannot allocate
+   device memory (code under HIP graph capture, tight serving loops, harnesses
+   that forbid hipMalloc). It needs no cf_cg handle and allocates and frees
+   nothing: r, d, q, the scalar record and the FP64 partials live in a static
+   device array of the library, CF_CG_STATIC_BYTES per GPU (room for
+   CF_CG_STATIC_MAX_N unknowns), which each GPU holds from the library's first
+   use until the process ends. n > CF_CG_STATIC_MAX_N returns CF_ERR_CAPACITY.
+   stream: the hipStream_t the callbacks must enqueue on (NULL: the null
+   stream). Results are bit-identical to cf_cg_solve's on the same inputs.
+   Every call on one GPU shares that GPU's array, which keeps the last call's
+   contents: calls that overlap on one GPU race, so serialize them. */
+#define CF_CG_STATIC_MAX_N (1u << 22)
+#define CF_CG_STATIC_BYTES ((size_t)50495744)
+int cf_cg_solve_static(void* stream, uint32_t n, cf_av_fn av, cf_constraints_fn apply_constraints, void* user,
+                       float* d_x, const float* d_b, float rel_epsilon, float abs_epsilon, uint32_t max_iterations,
+                       cf_cg_result* result);
 
 /* ---------------------------------------------------------------- leaf solid (3D)
  * LeafDSH_Solid3D_FEM: Create (.cpp:125-223), Step (:225-256: fixed-dt

@@ -11,6 +11,7 @@ MM_C_WRP, MM_C_LCM, MM_C_LCMH = 1, 2, 3
 DM_TRUNCATED, DM_EXACT = 0, 1
 IT_QIE_V, IT_FIE_DX = 1, 3
 CG_CONVERGED_REL, CG_CONVERGED_ABS, CG_MAX_ITER, CG_BREAKDOWN, CG_NONFINITE = 1, 2, 3, 4, 5
+CF_CG_STATIC_MAX_N = 1 << 22  # unknowns of cf_cg_solve_static's static scratch (include/corofem.h)
 ERR_INVALID_ARG, ERR_HIP, ERR_OUT_OF_MEMORY, ERR_NAN, ERR_UNSUPPORTED, ERR_NO_DEVICE, ERR_CAPACITY = (
     -1, -2, -3, -5, -6, -7, -8)
 
@@ -151,6 +152,8 @@ SIGNATURES = {
     "cf_cg_stream": (_vp, [_vp]),
     "cf_cg_solve": (C.c_int, [_vp, AV_FN, CONS_FN, _vp, _vp, _vp, C.c_float, C.c_float, C.c_uint32,
                               C.POINTER(CgResult)]),
+    "cf_cg_solve_static": (C.c_int, [_vp, C.c_uint32, AV_FN, CONS_FN, _vp, _vp, _vp, C.c_float, C.c_float,
+                                     C.c_uint32, C.POINTER(CgResult)]),
     "cf_solid_create": (C.c_int, [C.c_uint32, _f, C.c_uint32, _u, C.POINTER(SolidParams), C.POINTER(_vp)]),
     "cf_solid_destroy": (C.c_int, [_vp]),
     "cf_solid_set_knc": (C.c_int, [_vp, C.c_uint32, _u, _f]),

@@ -15,7 +15,7 @@ CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libcorofem.so")
 ARCH = os.environ.get("COROFEM_ARCH", "gfx950")
 
-SOURCES = ["cf_kernels.hip", "cf_batch_kernels.hip", "cf_kernels2d.hip", "cf_exact.hip", "cf_tiles.cpp", "cf_capi.cpp",
+SOURCES = ["cf_kernels.hip", "cf_batch_kernels.hip", "cf_kernels2d.hip", "cf_exact.hip", "cf_noalloc.hip", "cf_tiles.cpp", "cf_capi.cpp",
            "cf_batch.cpp", "cf_batch2.cpp", "cf_server.cpp", "cf_shape.cpp"]
 DEVICE_FLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 

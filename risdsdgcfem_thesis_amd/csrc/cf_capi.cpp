@@ -1075,11 +1075,22 @@ void* cf_cg_stream(cf_cg* c) { return c ? (void*)c->stream : nullptr; }
 int cf_cg_solve(cf_cg* c, cf_av_fn av, cf_constraints_fn cons, void* user, float* x, const float* bvec, float rel,
                 float abs_eps, uint32_t max_it, cf_cg_result* res) {
     if (!c || !av || !x || !bvec || !res || max_it == 0) { set_error("cf_cg_solve: invalid arguments"); return CF_ERR_INVALID_ARG; }
+    const CgBuffers& b = c->b;
+    const CgWork w{b.r, b.d, b.q, b.st, b.part_a, b.part_b};
+    return cg_solve_on(w, b.n, c->stream, av, cons, user, x, bvec, rel, abs_eps, max_it, b.h_flag, res);
+}
+
+}  // extern "C"
+
+// ILSS_CG::Solve (ns/ILSS_CG.h:38-151) with the caller's operator and
+// constraint callbacks, on the given work buffers and stream: the body of
+// cf_cg_solve (buffers of a cf_cg handle) and of cf_cg_solve_static (the
+// library's static scratch). h_flag receives each iteration's stop flag.
+int cf::cg_solve_on(const CgWork& b, uint32_t n, hipStream_t st, cf_av_fn av, cf_constraints_fn cons, void* user,
+                    float* x, const float* bvec, float rel, float abs_eps, uint32_t max_it, int32_t* h_flag,
+                    cf_cg_result* res) {
     max_it = std::min(max_it, kMaxCgIter);
-    CgBuffers& b = c->b;
-    const uint32_t n = b.n;
     const int nvec = node_blocks(n);
-    hipStream_t st = c->stream;
     // r = S(b - A x); d = r; delta0 = r.r   (ILSS_CG.h:53-65)
     av(user, b.q, x);
     CF_TRY(launch_cg_init(n, b.q, bvec, b.r, b.d, nullptr, b.part_b, nvec, st));
@@ -1089,10 +1100,10 @@ int cf_cg_solve(cf_cg* c, cf_av_fn av, cf_constraints_fn cons, void* user, float
         CF_TRY(launch_cg_dot(n, b.r, b.r, b.part_b, nvec, st));
     }
     CF_TRY(launch_cg_start(b.st, b.part_b, nvec, rel, abs_eps, max_it, st));
-    CF_CHECK_HIP(hipMemcpyAsync(b.h_flag, &b.st->done[0], 4, hipMemcpyDeviceToHost, st));
+    CF_CHECK_HIP(hipMemcpyAsync(h_flag, &b.st->done[0], 4, hipMemcpyDeviceToHost, st));
     CF_CHECK_HIP(hipStreamSynchronize(st));
     uint32_t k = 0;
-    while (!*b.h_flag && k < max_it) {
+    while (!*h_flag && k < max_it) {
         av(user, b.q, b.d);
         if (cons) cons(user, b.q);
         CF_TRY(launch_cg_dot(n, b.d, b.q, b.part_a, nvec, st));
@@ -1100,7 +1111,7 @@ int cf_cg_solve(cf_cg* c, cf_av_fn av, cf_constraints_fn cons, void* user, float
         CF_TRY(launch_cg_step_d(n, k, b.st, b.d, b.r, nullptr, b.part_b, st));
         if (cons) cons(user, b.d);
         k++;
-        CF_CHECK_HIP(hipMemcpyAsync(b.h_flag, &b.st->done[k], 4, hipMemcpyDeviceToHost, st));
+        CF_CHECK_HIP(hipMemcpyAsync(h_flag, &b.st->done[k], 4, hipMemcpyDeviceToHost, st));
         CF_CHECK_HIP(hipStreamSynchronize(st));
     }
     static thread_local CgScalars hs;
@@ -1113,6 +1124,8 @@ int cf_cg_solve(cf_cg* c, cf_av_fn av, cf_constraints_fn cons, void* user, float
     }
     return CF_OK;
 }
+
+extern "C" {
 
 // ---------------------------------------------------------------- leaf solid
 int cf_solid_destroy(cf_solid* s) {

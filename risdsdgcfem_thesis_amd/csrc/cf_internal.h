@@ -148,6 +148,18 @@ struct CgScalars {
 };
 constexpr uint32_t kMaxCgIter = 1024;
 
+// Work buffers of one CG solve: r, d, q (n floats each), the scalar record,
+// and FP64 partials (kMaxPartials each) for d.q and r.r.
+struct CgWork {
+    float *r, *d, *q;
+    CgScalars* st;
+    double *part_a, *part_b;
+};
+int cg_solve_on(const CgWork& w, uint32_t n, hipStream_t st, cf_av_fn av, cf_constraints_fn cons, void* user,
+                float* x, const float* b, float rel, float abs_eps, uint32_t max_it, int32_t* h_flag,
+                cf_cg_result* res);
+
+
 // ---------------------------------------------------------------- launchers
 enum ForceMode { kModeDfx = 0, kModeFe = 1 };
 enum GatherMode { kGatherAccumulate = 0, kGatherMatvec = 1, kGatherRhs = 2, kGatherRhsDx = 3 };

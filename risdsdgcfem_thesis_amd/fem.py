@@ -385,6 +385,19 @@ class ILSS_CG:
         return res
 
 
+def cg_solve_static(av, apply_constraints, x, b, rel=1e-3, abs_eps=1e-6, max_iterations=20, stream=None):
+    """ILSS_CG::Solve as ILSS_CG.solve, without a solver handle: the work
+    buffers are the library's static device scratch (cf_cg_solve_static),
+    so nothing is allocated. The callbacks enqueue on `stream` (None: the
+    null stream)."""
+    cav = AV_FN(lambda user, q, v: av(q, v))
+    ccons = CONS_FN(lambda user, v: apply_constraints(v)) if apply_constraints else CONS_FN(0)
+    res = CgResult()
+    check(lib.cf_cg_solve_static(stream, int(x.n), cav, ccons, None, x.ptr, b.ptr, float(rel), float(abs_eps),
+                                 int(max_iterations), C.byref(res)), "cf_cg_solve_static")
+    return res
+
+
 class LeafSolid3D:
     """LeafDSH_Solid3D_FEM with device-resident state (QIE-v, or FIE-dx with
     params.integrator = IT_FIE_DX)."""
