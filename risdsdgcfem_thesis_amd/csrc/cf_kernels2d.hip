@@ -74,15 +74,50 @@ struct Grp {
     }
 };
 
+// A triangle's constant data, kept in registers by the thread that owns it;
+// its rotation of the step lives in LDS (one float4 per owned triangle).
 struct Tri {
-    uint32_t i0, i1, i2;
+    uint32_t i01, i2;          // local node ids: i0 | i1 << 16, i2 (meshes hold at most 1,536 nodes)
     float g00, g01, g10, g11;  // Dm^-1
     float A;
-    float r00, r01, r10, r11;  // R
+    __device__ uint32_t n0() const { return i01 & 0xffffu; }
+    __device__ uint32_t n1() const { return i01 >> 16; }
+    __device__ uint32_t n2() const { return i2; }
 };
 
+// The thread's k-th triangle for a k known only at run time. The fields are
+// combined by masks on their bits: a register array indexed at run time (or
+// a select chain the compiler turns into one) would live in scratch memory.
+__device__ inline uint32_t pick_bits(const uint32_t (&v)[kTriRegs], uint32_t k) {
+    uint32_t r = 0;
+#pragma unroll
+    for (uint32_t j = 0; j < uint32_t(kTriRegs); j++) r |= v[j] & (0u - uint32_t(k == j));
+    return r;
+}
+__device__ inline Tri pick(const Tri (&tr)[kTriRegs], uint32_t k) {
+    uint32_t i01[kTriRegs], i2[kTriRegs], g00[kTriRegs], g01[kTriRegs], g10[kTriRegs], g11[kTriRegs], A[kTriRegs];
+#pragma unroll
+    for (int j = 0; j < kTriRegs; j++) {
+        i01[j] = tr[j].i01; i2[j] = tr[j].i2;
+        g00[j] = __float_as_uint(tr[j].g00); g01[j] = __float_as_uint(tr[j].g01);
+        g10[j] = __float_as_uint(tr[j].g10); g11[j] = __float_as_uint(tr[j].g11);
+        A[j] = __float_as_uint(tr[j].A);
+    }
+    Tri t;
+    t.i01 = pick_bits(i01, k); t.i2 = pick_bits(i2, k);
+    t.g00 = __uint_as_float(pick_bits(g00, k)); t.g01 = __uint_as_float(pick_bits(g01, k));
+    t.g10 = __uint_as_float(pick_bits(g10, k)); t.g11 = __uint_as_float(pick_bits(g11, k));
+    t.A = __uint_as_float(pick_bits(A, k));
+    return t;
+}
+
+struct Rot {
+    float r00, r01, r10, r11;  // R, row-major
+};
+__device__ inline Rot rot_of(float4 v) { return Rot{v.x, v.y, v.z, v.w}; }
+
 // Closed-form 2x2 polar factor (Mal/GMatDecomposition.h:17-30)
-__device__ inline void polar2(float f00, float f01, float f10, float f11, float det, Tri& t) {
+__device__ inline void polar2(float f00, float f01, float f10, float f11, float det, Rot& t) {
 #pragma clang fp contract(off)
     const float sg = (det > 0.f) ? 1.0f : ((det < 0.f) ? -1.0f : 0.0f);
     float r00 = f00 + sg * f11, r01 = f01 + sg * (-f10);
@@ -281,7 +316,7 @@ __device__ __noinline__ float4 r_dapd_degenerate2(float2 p0, float2 p1, float2 p
     const float dp = f00 * f11 - f10 * f01;
     const float dd = dp - 1.0f;
     if (dp > 0.001f || (dd > 0.0f ? dd : -dd) <= 0.0001f) {
-        Tri t_;
+        Rot t_;
         polar2(f00, f01, f10, f11, dp, t_);
         return make_float4(t_.r00, t_.r01, t_.r10, t_.r11);
     }
@@ -292,7 +327,7 @@ __device__ __noinline__ float4 r_dapd_degenerate2(float2 p0, float2 p1, float2 p
 // TriangleElementSet2.cpp:1007-1018,1078-1110): polar factor above the
 // threshold; below it MSVD's 2x2 SVD, DAPD R = Id (its invalid-NoC branch).
 // Returns 1 for a DAPD fallback.
-__device__ inline float rotation2(int rm, float thr, float f00, float f01, float f10, float f11, float det, Tri& t) {
+__device__ inline float rotation2(int rm, float thr, float f00, float f01, float f10, float f11, float det, Rot& t) {
     t.r00 = 1.f; t.r01 = 0.f; t.r10 = 0.f; t.r11 = 1.f;
     if (rm == CF_RM_ID) return 0.0f;
     if (det > thr) {
@@ -310,12 +345,12 @@ __device__ inline float rotation2(int rm, float thr, float f00, float f01, float
 // Displacement gradient H of a triangle: R^T D Dm^-1, or for f_e
 // (R^T D - Dm) Dm^-1 so that a rest state gives exactly 0.
 template <bool FE>
-__device__ inline void tri_gradient(const Tri& t, float2 p0, float2 p1, float2 p2, float2 q0, float2 q1,
+__device__ inline void tri_gradient(const Tri& t, const Rot& R, float2 p0, float2 p1, float2 p2, float2 q0, float2 q1,
                                     float2 q2, float H[4]) {
 #pragma clang fp contract(off)
     const float D00 = p1.x - p0.x, D01 = p2.x - p0.x, D10 = p1.y - p0.y, D11 = p2.y - p0.y;
-    float RD00 = fmaf(t.r10, D10, t.r00 * D00), RD01 = fmaf(t.r10, D11, t.r00 * D01);
-    float RD10 = fmaf(t.r11, D10, t.r01 * D00), RD11 = fmaf(t.r11, D11, t.r01 * D01);
+    float RD00 = fmaf(R.r10, D10, R.r00 * D00), RD01 = fmaf(R.r10, D11, R.r00 * D01);
+    float RD10 = fmaf(R.r11, D10, R.r01 * D00), RD11 = fmaf(R.r11, D11, R.r01 * D01);
     if (FE) {
         RD00 -= q1.x - q0.x; RD01 -= q2.x - q0.x;
         RD10 -= q1.y - q0.y; RD11 -= q2.y - q0.y;
@@ -327,16 +362,16 @@ __device__ inline void tri_gradient(const Tri& t, float2 p0, float2 p1, float2 p
 }
 
 // Nodal forces of a triangle from its displacement gradient.
-__device__ inline void tri_forces(const Tri& t, const float H[4], const Material& m, float2& f0, float2& f1,
-                                  float2& f2) {
+__device__ inline void tri_forces(const Tri& t, const Rot& R, const float H[4], const Material& m, float2& f0,
+                                  float2& f1, float2& f2) {
 #pragma clang fp contract(off)
     const float sxx = fmaf(m.a_k1, H[0], m.a_nu * H[3]);
     const float syy = fmaf(m.a_k1, H[3], m.a_nu * H[0]);
     const float sxy = m.a_k2 * (H[1] + H[2]);
     const float W00 = -t.A * fmaf(sxy, t.g01, sxx * t.g00), W01 = -t.A * fmaf(sxy, t.g11, sxx * t.g10);
     const float W10 = -t.A * fmaf(syy, t.g01, sxy * t.g00), W11 = -t.A * fmaf(syy, t.g11, sxy * t.g10);
-    const float P00 = fmaf(t.r01, W10, t.r00 * W00), P01 = fmaf(t.r01, W11, t.r00 * W01);
-    const float P10 = fmaf(t.r11, W10, t.r10 * W00), P11 = fmaf(t.r11, W11, t.r10 * W01);
+    const float P00 = fmaf(R.r01, W10, R.r00 * W00), P01 = fmaf(R.r01, W11, R.r00 * W01);
+    const float P10 = fmaf(R.r11, W10, R.r10 * W00), P11 = fmaf(R.r11, W11, R.r10 * W01);
     f1 = make_float2(P00, P10);
     f2 = make_float2(P01, P11);
     f0 = make_float2(-(P00 + P01), -(P10 + P11));
@@ -344,11 +379,15 @@ __device__ inline void tri_forces(const Tri& t, const float H[4], const Material
 
 // Element pass: s_f = sum of the group's triangle forces on the vectors in
 // s_v (FE: f_e on positions, with the rest positions), colour by colour.
-// `run` false: the group only takes part in the barriers.
+// Triangles are sorted by colour and thread `lane` owns triangles lane + k G,
+// so its triangles of one colour are a run of consecutive k (on a grid mesh
+// at most one); no two triangles of a colour share a node, so the order
+// within a colour does not change a node's sum. `run` false: the group only
+// takes part in the barriers.
 template <int GW, bool FE>
-__device__ inline void accumulate_forces(const Tri* tr, uint32_t lane, uint32_t N, uint32_t T, uint32_t C,
-                                         uint32_t CL, bool run, const uint16_t* cs, const float2* ref,
-                                         const Material& m, const float2* s_v, float2* s_f) {
+__device__ inline void accumulate_forces(const Tri (&tr)[kTriRegs], const float4* s_R, uint32_t lane, uint32_t N,
+                                         uint32_t T, uint32_t C, uint32_t CL, bool run, const uint16_t* cs,
+                                         const float2* ref, const Material& m, const float2* s_v, float2* s_f) {
 #pragma clang fp contract(off)
     constexpr int G = Grp<GW>::G;
     if (run) {
@@ -361,23 +400,23 @@ __device__ inline void accumulate_forces(const Tri* tr, uint32_t lane, uint32_t 
     Grp<GW>::bar();
     for (uint32_t c = 0; c < CL; c++) {
         if (run && c < C) {
-            const uint32_t lo = cs[c], hi = cs[c + 1];
-#pragma unroll
-            for (int k = 0; k < kTriRegs; k++) {
-                const uint32_t e = lane + k * G;
-                if (e >= T || e < lo || e >= hi) continue;
-                const Tri& t = tr[k];
+            const uint32_t lo = cs[c], hi = min(uint32_t(cs[c + 1]), T);
+            uint32_t k = (lane >= lo) ? 0u : (lo - lane + G - 1) / G;  // first k with lane + k G >= lo
+#pragma unroll 1
+            for (; k < uint32_t(kTriRegs) && lane + k * G < hi; k++) {
+                const Tri t = pick(tr, k);
+                const Rot R = rot_of(s_R[k * G + lane]);
                 float H[4];
                 const float2 z = make_float2(0.f, 0.f);
-                if (FE) tri_gradient<true>(t, s_v[t.i0], s_v[t.i1], s_v[t.i2], ref[t.i0], ref[t.i1], ref[t.i2], H);
-                else tri_gradient<false>(t, s_v[t.i0], s_v[t.i1], s_v[t.i2], z, z, z, H);
+                if (FE) tri_gradient<true>(t, R, s_v[t.n0()], s_v[t.n1()], s_v[t.n2()], ref[t.n0()], ref[t.n1()], ref[t.n2()], H);
+                else tri_gradient<false>(t, R, s_v[t.n0()], s_v[t.n1()], s_v[t.n2()], z, z, z, H);
                 float2 f0, f1, f2;
-                tri_forces(t, H, m, f0, f1, f2);
-                float2 a0 = s_f[t.i0], a1 = s_f[t.i1], a2 = s_f[t.i2];
+                tri_forces(t, R, H, m, f0, f1, f2);
+                float2 a0 = s_f[t.n0()], a1 = s_f[t.n1()], a2 = s_f[t.n2()];
                 a0.x += f0.x; a0.y += f0.y;
                 a1.x += f1.x; a1.y += f1.y;
                 a2.x += f2.x; a2.y += f2.y;
-                s_f[t.i0] = a0; s_f[t.i1] = a1; s_f[t.i2] = a2;
+                s_f[t.n0()] = a0; s_f[t.n1()] = a1; s_f[t.n2()] = a2;
             }
         }
         Grp<GW>::bar();
@@ -385,8 +424,8 @@ __device__ inline void accumulate_forces(const Tri* tr, uint32_t lane, uint32_t 
 }
 
 template <int GW>
-__device__ void mesh_step(const Batch2Args& a, uint32_t cmax, float2* s_v_all, float2* s_f_all, double* red,
-                          float* redm, uint32_t* flags) {
+__device__ void mesh_step(const Batch2Args& a, uint32_t cmax, float2* s_v_all, float2* s_f_all, float2* s_y_all,
+                          float4* s_R_all, double* red, float* redm, uint32_t* flags) {
 #pragma clang fp contract(off)
     using Gp = Grp<GW>;
     constexpr int G = Gp::G;
@@ -413,6 +452,8 @@ __device__ void mesh_step(const Batch2Args& a, uint32_t cmax, float2* s_v_all, f
     const uint32_t CL = (GW == 1) ? C : cmax;  // colour passes: the same count in every group
     float2* s_v = s_v_all + grp * (G * kNodeRegs);
     float2* s_f = s_f_all + grp * (G * kNodeRegs);
+    float2* s_y = s_y_all + grp * (G * kNodeRegs);  // the solution y; s_v holds the CG direction d
+    float4* s_R = s_R_all + grp * (G * kTriRegs);   // the rotation of triangle lane + k G at k G + lane
     const float2* ref = a.ref + M.node_off;
     float2* xg = a.x + M.node_off;
     float2* vg = a.v + M.node_off;
@@ -428,13 +469,14 @@ __device__ void mesh_step(const Batch2Args& a, uint32_t cmax, float2* s_v_all, f
         const uint32_t e = lane + k * G;
         if (e < T) {
             const size_t ge = M.tri_off + (size_t)e;
-            tr[k].i0 = a.tri_ids[3 * ge]; tr[k].i1 = a.tri_ids[3 * ge + 1]; tr[k].i2 = a.tri_ids[3 * ge + 2];
+            tr[k].i01 = a.tri_ids[3 * ge] | (a.tri_ids[3 * ge + 1] << 16);
+            tr[k].i2 = a.tri_ids[3 * ge + 2];
             const float4 g = a.dinv[ge];
             tr[k].g00 = g.x; tr[k].g01 = g.y; tr[k].g10 = g.z; tr[k].g11 = g.w;
             tr[k].A = a.area[ge];
         }
     }
-    float2 y[kNodeRegs], r[kNodeRegs], d[kNodeRegs];
+    float2 r[kNodeRegs];
     float mass[kNodeRegs];
     bool kn[kNodeRegs];
 #pragma unroll
@@ -451,12 +493,14 @@ __device__ void mesh_step(const Batch2Args& a, uint32_t cmax, float2* s_v_all, f
     // ---- BeginEvaluation: F = Ds Dm^-1, det F, R
     double ndeg = 0.0, sdeg = 0.0, nfb = 0.0;
     float mind = 3.0e38f;
-#pragma unroll
-    for (int k = 0; k < kTriRegs; k++) {
+    // one triangle at a time (k at run time): the rotation's temporaries are
+    // live for one triangle only
+#pragma unroll 1
+    for (uint32_t k = 0; k < uint32_t(kTriRegs); k++) {
         const uint32_t e = lane + k * G;
-        if (e >= T) continue;
-        Tri& t = tr[k];
-        const float2 p0 = s_v[t.i0], p1 = s_v[t.i1], p2 = s_v[t.i2];
+        if (e >= T) break;
+        const Tri t = pick(tr, k);
+        const float2 p0 = s_v[t.n0()], p1 = s_v[t.n1()], p2 = s_v[t.n2()];
         const float D00 = p1.x - p0.x, D01 = p2.x - p0.x, D10 = p1.y - p0.y, D11 = p2.y - p0.y;
         float f00 = 0.0f; f00 += D00 * t.g00; f00 += D01 * t.g10;  // Mal GMat product, accumulated from 0
         float f01 = 0.0f; f01 += D00 * t.g01; f01 += D01 * t.g11;
@@ -470,23 +514,24 @@ __device__ void mesh_step(const Batch2Args& a, uint32_t cmax, float2* s_v_all, f
             const size_t ge = M.tri_off + (size_t)e;
             const uint32_t old = a.noc[ge];
             nc = (det > M.thr) ? kNocInvalid
-                               : update_noc2(x0g[t.i0], x0g[t.i1], x0g[t.i2], p0, p1, p2, t.g00, t.g01, t.g10, t.g11,
+                               : update_noc2(x0g[t.n0()], x0g[t.n1()], x0g[t.n2()], p0, p1, p2, t.g00, t.g01, t.g10, t.g11,
                                              t.A, M.thr, old);
             if (nc != old) a.noc[ge] = uint8_t(nc);
         }
         if (track && !(det > M.thr) && nc != kNocInvalid) {
-            const float4 R = r_dapd_degenerate2(p0, p1, p2, nc, t.A, t.g00, t.g01, t.g10, t.g11, M.thr, M.dapd_l,
-                                                M.dapd_nl, M.dapd_e);
-            t.r00 = R.x; t.r01 = R.y; t.r10 = R.z; t.r11 = R.w;
+            s_R[k * G + lane] = r_dapd_degenerate2(p0, p1, p2, nc, t.A, t.g00, t.g01, t.g10, t.g11, M.thr, M.dapd_l,
+                                                   M.dapd_nl, M.dapd_e);
         } else {
-            nfb += rotation2(M.rm, M.thr, f00, f01, f10, f11, det, t);
+            Rot rt;
+            nfb += rotation2(M.rm, M.thr, f00, f01, f10, f11, det, rt);
+            s_R[k * G + lane] = make_float4(rt.r00, rt.r01, rt.r10, rt.r11);
         }
         if (det <= 0.f) { ndeg += 1.0; sdeg += det; }
         mind = fminf(mind, det);
     }
 
     // ---- b = dt (M g + f_e) + M v ; y = v ; KNC rows take the constraint velocity
-    accumulate_forces<GW, true>(tr, lane, N, T, C, CL, real, cs, ref, M.m, s_v, s_f);
+    accumulate_forces<GW, true>(tr, s_R, lane, N, T, C, CL, real, cs, ref, M.m, s_v, s_f);
     float2 b[kNodeRegs];
 #pragma unroll
     for (int k = 0; k < kNodeRegs; k++) {
@@ -496,7 +541,7 @@ __device__ void mesh_step(const Batch2Args& a, uint32_t cmax, float2* s_v_all, f
         const float m = mass[k];
         b[k].x = (m * M.g0 + f.x) * M.dt + m * vv.x;
         b[k].y = (m * M.g1 + f.y) * M.dt + m * vv.y;
-        y[k] = M.warm_start ? vv : make_float2(0.f, 0.f);
+        float2 yk = M.warm_start ? vv : make_float2(0.f, 0.f);
         if (kn[k]) {
             const float2 q = ref[n], xx = s_v[n];
             float vx = (q.x - xx.x) * M.inv_dt, vy = (q.y - xx.y) * M.inv_dt;
@@ -508,32 +553,32 @@ __device__ void mesh_step(const Batch2Args& a, uint32_t cmax, float2* s_v_all, f
                 vx = 0.f; vy = 0.f;
             }
             b[k] = make_float2(vx, vy);
-            y[k] = b[k];
+            yk = b[k];
         }
+        s_y[n] = yk;
     }
     Gp::bar();  // every read of s_v (positions) is done
 #pragma unroll
     for (int k = 0; k < kNodeRegs; k++) {
         const uint32_t n = lane + k * G;
-        if (n < N) s_v[n] = y[k];
+        if (n < N) s_v[n] = s_y[n];
     }
     Gp::bar();
 
     // ---- CG (ns/ILSS_CG.h:38-151): r = S(b - A y), d = r
-    accumulate_forces<GW, false>(tr, lane, N, T, C, CL, real, cs, ref, M.m, s_v, s_f);
+    accumulate_forces<GW, false>(tr, s_R, lane, N, T, C, CL, real, cs, ref, M.m, s_v, s_f);
     double acc = 0.0;
 #pragma unroll
     for (int k = 0; k < kNodeRegs; k++) {
         const uint32_t n = lane + k * G;
         if (n >= N) continue;
-        const float2 f = s_f[n];
+        const float2 f = s_f[n], yk = s_y[n];
         const float fac = mass[k] * M.fM;
-        float q0 = -M.fK * f.x + fac * y[k].x, q1 = -M.fK * f.y + fac * y[k].y;
+        float q0 = -M.fK * f.x + fac * yk.x, q1 = -M.fK * f.y + fac * yk.y;
         if (kn[k]) { q0 = 0.f; q1 = 0.f; }
         float r0 = -1.0f * q0 + b[k].x, r1 = -1.0f * q1 + b[k].y;
         if (kn[k]) { r0 = 0.f; r1 = 0.f; }
         r[k] = make_float2(r0, r1);
-        d[k] = r[k];
         acc += (double)r0 * r0 + (double)r1 * r1;
     }
     const double delta0 = Gp::sum(acc, red);
@@ -552,12 +597,12 @@ __device__ void mesh_step(const Batch2Args& a, uint32_t cmax, float2* s_v_all, f
 #pragma unroll
     for (int k = 0; k < kNodeRegs; k++) {
         const uint32_t n = lane + k * G;
-        if (n < N) s_v[n] = d[k];
+        if (n < N) s_v[n] = r[k];  // d = r
     }
     while (Gp::any(reason == 0, flags)) {
         const bool run = reason == 0;
         Gp::bar();
-        accumulate_forces<GW, false>(tr, lane, N, T, C, CL, run, cs, ref, M.m, s_v, s_f);
+        accumulate_forces<GW, false>(tr, s_R, lane, N, T, C, CL, run, cs, ref, M.m, s_v, s_f);
         float2 q[kNodeRegs];
         acc = 0.0;
         if (run) {
@@ -565,12 +610,12 @@ __device__ void mesh_step(const Batch2Args& a, uint32_t cmax, float2* s_v_all, f
             for (int k = 0; k < kNodeRegs; k++) {
                 const uint32_t n = lane + k * G;
                 if (n >= N) continue;
-                const float2 f = s_f[n];
+                const float2 f = s_f[n], dk = s_v[n];
                 const float fac = mass[k] * M.fM;
-                float q0 = -M.fK * f.x + fac * d[k].x, q1 = -M.fK * f.y + fac * d[k].y;
+                float q0 = -M.fK * f.x + fac * dk.x, q1 = -M.fK * f.y + fac * dk.y;
                 if (kn[k]) { q0 = 0.f; q1 = 0.f; }
                 q[k] = make_float2(q0, q1);
-                acc += (double)d[k].x * q0 + (double)d[k].y * q1;
+                acc += (double)dk.x * q0 + (double)dk.y * q1;
             }
         }
         const double dq = Gp::sum(acc, red);
@@ -583,7 +628,10 @@ __device__ void mesh_step(const Batch2Args& a, uint32_t cmax, float2* s_v_all, f
             for (int k = 0; k < kNodeRegs; k++) {
                 const uint32_t n = lane + k * G;
                 if (n >= N) continue;
-                y[k].x = y[k].x + d[k].x * alpha; y[k].y = y[k].y + d[k].y * alpha;
+                const float2 dk = s_v[n];
+                float2 yk = s_y[n];
+                yk.x = yk.x + dk.x * alpha; yk.y = yk.y + dk.y * alpha;
+                s_y[n] = yk;
                 r[k].x = r[k].x - q[k].x * alpha; r[k].y = r[k].y - q[k].y * alpha;
                 acc += (double)r[k].x * r[k].x + (double)r[k].y * r[k].y;
             }
@@ -604,10 +652,10 @@ __device__ void mesh_step(const Batch2Args& a, uint32_t cmax, float2* s_v_all, f
             for (int k = 0; k < kNodeRegs; k++) {
                 const uint32_t n = lane + k * G;
                 if (n >= N) continue;
-                float d0 = beta * d[k].x + r[k].x, d1 = beta * d[k].y + r[k].y;
+                const float2 dk = s_v[n];
+                float d0 = beta * dk.x + r[k].x, d1 = beta * dk.y + r[k].y;
                 if (kn[k]) { d0 = 0.f; d1 = 0.f; }
-                d[k] = make_float2(d0, d1);
-                s_v[n] = d[k];
+                s_v[n] = make_float2(d0, d1);
             }
         }
     }
@@ -618,7 +666,7 @@ __device__ void mesh_step(const Batch2Args& a, uint32_t cmax, float2* s_v_all, f
     for (int k = 0; k < kNodeRegs; k++) {
         const uint32_t n = lane + k * G;
         if (n >= N) continue;
-        const float2 yy = y[k], xx = xg[n];
+        const float2 yy = s_y[n], xx = xg[n];
         if (track) x0g[n] = xx;  // m_vecPos0 = x after BeginEvaluation (LeafDSH_Solid2D_FEM.cpp:974-985)
         const float2 xn = make_float2(xx.x + M.dt * yy.x, xx.y + M.dt * yy.y);
         const float2 vn = (M.drag > 0.f) ? make_float2((1.0f - M.dt * M.drag) * yy.x, (1.0f - M.dt * M.drag) * yy.y)
@@ -632,13 +680,14 @@ __device__ void mesh_step(const Batch2Args& a, uint32_t cmax, float2* s_v_all, f
     // ---- elastic energy on the new positions with the step's rotations (C_V)
     double en = 0.0;
     if (M.compute_energy) {
-#pragma unroll
-        for (int k = 0; k < kTriRegs; k++) {
+#pragma unroll 1
+        for (uint32_t k = 0; k < uint32_t(kTriRegs); k++) {
             const uint32_t e = lane + k * G;
-            if (e >= T) continue;
-            const Tri& t = tr[k];
+            if (e >= T) break;
+            const Tri t = pick(tr, k);
+            const Rot R = rot_of(s_R[k * G + lane]);
             float H[4];
-            tri_gradient<true>(t, s_v[t.i0], s_v[t.i1], s_v[t.i2], ref[t.i0], ref[t.i1], ref[t.i2], H);
+            tri_gradient<true>(t, R, s_v[t.n0()], s_v[t.n1()], s_v[t.n2()], ref[t.n0()], ref[t.n1()], ref[t.n2()], H);
             const float sxx = fmaf(M.m.a_k1, H[0], M.m.a_nu * H[3]);
             const float syy = fmaf(M.m.a_k1, H[3], M.m.a_nu * H[0]);
             const float sxy = M.m.a_k2 * (H[1] + H[2]);
@@ -669,19 +718,25 @@ __device__ void mesh_step(const Batch2Args& a, uint32_t cmax, float2* s_v_all, f
 }  // namespace
 
 // One workgroup per work item: 8/gw meshes, one per group of gw wavefronts.
-// LDS: per group two float2 node arrays of 64 gw kNodeRegs entries.
-__global__ __launch_bounds__(k2Threads) void k2_step(Batch2Args a) {
+// LDS per group: three float2 node arrays of 64 gw kNodeRegs entries (d or
+// positions, forces, the solution y) and one float4 rotation per owned
+// triangle, 68.7 KB per workgroup. Bounded to 4 waves per SIMD (at most 128
+// VGPRs), so that two workgroups share a CU and hide each other's barrier
+// and LDS latency.
+__global__ __launch_bounds__(k2Threads, 4) void k2_step(Batch2Args a) {
     __shared__ float2 s_v[k2Threads * kNodeRegs];
     __shared__ float2 s_f[k2Threads * kNodeRegs];
+    __shared__ float2 s_y[k2Threads * kNodeRegs];
+    __shared__ float4 s_R[k2Threads * kTriRegs];
     __shared__ double red[8];
     __shared__ float redm[8];
     __shared__ uint32_t flags[8];
     const uint32_t gw = a.work[blockIdx.x].gw, cmax = a.work[blockIdx.x].cmax;
     switch (gw) {
-        case 1: mesh_step<1>(a, cmax, s_v, s_f, red, redm, flags); break;
-        case 2: mesh_step<2>(a, cmax, s_v, s_f, red, redm, flags); break;
-        case 4: mesh_step<4>(a, cmax, s_v, s_f, red, redm, flags); break;
-        default: mesh_step<8>(a, cmax, s_v, s_f, red, redm, flags); break;
+        case 1: mesh_step<1>(a, cmax, s_v, s_f, s_y, s_R, red, redm, flags); break;
+        case 2: mesh_step<2>(a, cmax, s_v, s_f, s_y, s_R, red, redm, flags); break;
+        case 4: mesh_step<4>(a, cmax, s_v, s_f, s_y, s_R, red, redm, flags); break;
+        default: mesh_step<8>(a, cmax, s_v, s_f, s_y, s_R, red, redm, flags); break;
     }
 }
 
@@ -693,7 +748,7 @@ __global__ void __launch_bounds__(256) k2_rotations(const float4* __restrict__ F
     const uint32_t i = blockIdx.x * 256u + threadIdx.x;
     if (i >= n) return;
     const float4 f = F[i];
-    Tri t;
+    Rot t;
     const float fb = rotation2(rm, thr, f.x, f.y, f.z, f.w, det[i], t);
     R[i] = make_float4(t.r00, t.r01, t.r10, t.r11);
     fallback[i] = fb != 0.0f;
