@@ -11,11 +11,19 @@ Default run (one JSON line): the four workloads of the hot path.
               tet-matvecs per second over the timed steps; `roofline` = one K*x
               (element tile kernel + node gather) timed with HIP events on the
               solid's stream; `cpu_baseline` = the oracle's K*x (stored Ke,
-              serial scatter, one core) on a bounded shard of the same mesh.
+              serial scatter, one core) on a bounded shard of the same mesh,
+              with the host's CPU model and, for config 1 (the 20,000-tet
+              beam), per-step medians of the oracle on one core and of the
+              GPU leaf (fast path and reference order).
   batched_2d  65,536 2D grids (n x n, n in [4,32]) stepped by ONE launch per
               step, against the plain single-mesh path on every 64th mesh.
   batched_3d  1,024 uneven 3D cube grids (4-40 cubes per axis) in one
               graph-recorded batched step, against the plain path on every 8th.
+              Both batches run under two CG settings (the reference's abs
+              floor 1e-6, and the relative test alone, where every solve
+              stops at rel 1e-4: the headline), each with its achieved
+              relative residuals; `cpu_baseline` = the oracle stepping a
+              sample of the meshes on all host threads (up to 16).
   serving     Poisson arrivals at 100 / 1k / 10k requests/s, HBM budget 200 GB,
               continuous batching; per-step latency p50/p99.
 `--workload single|batch2d|batch3d|serve` runs one of them alone.
@@ -49,6 +57,78 @@ def load_traffic():
         return json.load(f)
 
 
+def host_cpu():
+    """The host's CPU model and the cores this process may use."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    return {"model": model, "cores_available": cores}
+
+
+def cpu_threads():
+    """Host threads for the mesh-parallel CPU baselines: the cores this
+    process may use, at most 16 (one GPU's share of a box's CPUs)."""
+    return max(1, min(16, host_cpu()["cores_available"]))
+
+
+def config1_steps(cf, steps=10, warmup=2):
+    """Config 1 (the 10x10x40-cube beam in the reference's baked element
+    order, CG rel 1e-4 max 50, undamped): per-step wall time, median over
+    `steps` after `warmup`, of the oracle's FixedStep on one core (the
+    reference's own per-step statistic, LeafDSH_Solid3D_FEM.h:599-621) and of
+    the GPU leaf, fast path and reference order."""
+    from oracle import orc
+    sh = cf.mesh_box_baked((0.05, 0.05, 0.2), (11, 11, 41))
+    pos, tets = sh.pos.copy(), sh.tets.copy()
+    knc = np.nonzero(np.isclose(pos[2::3], -0.2))[0].astype(np.uint32)
+
+    def timed(step):
+        for _ in range(warmup):
+            step()
+        ts = []
+        for _ in range(steps):
+            t = time.perf_counter()
+            step()
+            ts.append(time.perf_counter() - t)
+        return round(float(np.median(ts)) * 1e3, 3)
+
+    o = orc.Solid(pos, tets, orc.default_params(rm=orc.RM_DAPD, cg_max_iter=50), knc)
+    out = {"workload": "config 1: 10x10x40-cube beam, 20,000 tets, baked order, CG rel 1e-4 max 50",
+           "cpu_oracle_ms_median": timed(o.fixed_step), "cpu_cores": 1}
+    for name, ro in (("gpu_fast_ms_median", 0), ("gpu_reference_order_ms_median", 1)):
+        p = cf.default_solid_params(fixed_dt=1.0 / 60.0, gravity=(0.0, -9.81, 0.0), young_modulus=1e5,
+                                    poisson_ratio=0.3, damping_ratio=0.0, rm=cf.RM_DAPD, cg_rel_epsilon=1e-4,
+                                    cg_abs_epsilon=1e-6, cg_max_iter=50, density=1000.0, reference_order=ro)
+        g = cf.LeafSolid3D(pos, tets, p, knc=knc)
+        out[name] = timed(g.fixed_step)
+        g.close()
+    return out
+
+
+def cpu_batch_baseline(meshes, make_solid, unit, what, steps=1):
+    """The oracle stepping a sample of a batch's meshes, the meshes spread over
+    the host threads (each stepped whole by one thread)."""
+    from oracle import orc
+    solids = [make_solid(m) for m in meshes]
+    threads = cpu_threads()
+    t = time.perf_counter()
+    for _ in range(steps):
+        orc.step_parallel(solids, threads)
+    dt = time.perf_counter() - t
+    return {"value": round(len(meshes) * steps / dt, 2), "unit": unit, "cores": threads, "kind": "port",
+            "sample": f"{what}: {len(meshes)} meshes x {steps} step(s) in {dt:.1f} s on {threads} threads"}
+
+
 def cpu_baseline(budget_s=12.0):
     """The oracle's K*x (reference algorithm: stored 3x3 Ke blocks, serial
     scatter-add, FP32, one core) on a 64x64x64-cube shard of the mesh."""
@@ -75,11 +155,35 @@ def cpu_baseline(budget_s=12.0):
                       f"{dt:.1f} s"}
 
 
+def residual_stats(stats):
+    """Achieved relative CG residuals of a step's solves (sqrt(delta/delta0),
+    ILSS_CG.h:148-150) and why they stopped."""
+    rel = np.array([s.cg_rel_residual for s in stats], np.float64)
+    reasons = np.array([s.cg_reason for s in stats])
+    return {"rel_p50": float(np.percentile(rel, 50)), "rel_p99": float(np.percentile(rel, 99)),
+            "rel_max": float(rel.max()), "solves": int(rel.size), "at_or_below_1e-4": int((rel <= 1e-4).sum()),
+            "cg_reasons": {str(k): int((reasons == k).sum()) for k in (1, 2, 3, 4)}}
+
+
+def with_abs(params, abs_eps):
+    p = type(params).from_buffer_copy(params)
+    p.cg_abs_epsilon = abs_eps
+    return p
+
+
+# The two CG settings of the batched workloads: the reference's defaults (rel
+# 1e-4 and the absolute floor 1e-6, below which a solve also stops, as does a
+# |d.q| < abs^2 breakdown), and the relative test alone (abs 0), where every
+# solve stops at rel 1e-4 (or the iteration cap).
+CG_VARIANTS = [("reference_abs_1e-6", 1e-6), ("rel_1e-4_only", 0.0)]
+
+
 def bench_batch3d(cf, args):
     """Uneven 3D batch: 1,024 cube grids (cubes per axis uniform in [4,40],
     seed 4), E log-uniform [1e4,1e7], nu [0.2,0.45], a random face held. The
     batched step (all meshes in the same launches, graph-recorded) against the
-    plain single-mesh path run mesh by mesh on every 8th mesh."""
+    plain single-mesh path run mesh by mesh on every 8th mesh, under both CG
+    settings."""
     from risdsdgcfem_thesis_amd import workloads, autotune
     t0 = time.perf_counter()
     meshes = workloads.uneven_batch(count=args.batch_count, cg_max_iter=args.batch_max_iter)
@@ -94,121 +198,155 @@ def bench_batch3d(cf, args):
     b = cf.Batch(tile_threads=tuned["tile_threads"], tile_elems=tuned["tile_elems"], cg_chunk=tuned["cg_chunk"],
                  use_graphs=not args.no_graphs)
     mids = b.add_meshes([(m["pos"], m["tets"]) for m in meshes], threads=16)
-    sids = [b.add_sim(mid, m["params"], knc=m["knc"]) for mid, m in zip(mids, meshes)]
     log(f"[batch3d] {len(meshes)} meshes, {T} tets, {N} nodes; setup {time.perf_counter() - t0:.1f} s")
-    for _ in range(args.warmup):
-        b.fixed_step(stats=False)
-    b.synchronize()
-    iters = []
-    t1 = time.perf_counter()
-    for _ in range(args.batch_steps):
-        st = b.fixed_step()
-        iters.append([st[s].cg_iters for s in sids])
-    b.synchronize()
-    wall = time.perf_counter() - t1
-    it = np.array(iters)
-    reasons = np.array([st[s].cg_reason for s in sids])
-    batched_sps = len(meshes) * args.batch_steps / wall
-    kx_ms = b.time_kx(args.kx_reps)
     b_alg = 92 * T + 24 * N
-    log(f"[batch3d] batched {wall / args.batch_steps * 1e3:.1f} ms/step, iters mean {it.mean():.1f} "
-        f"max {it.max()}, batched K*x {kx_ms * 1e3:.0f} us")
-    b.close()
-    # plain path, one mesh at a time
-    sample = meshes[::8]
-    plain_wall = 0.0
-    for m in sample:
-        s = cf.LeafSolid3D(m["pos"], m["tets"], m["params"], knc=m["knc"])
-        for _ in range(args.warmup):
-            s.fixed_step()
-        cf.synchronize()
-        t2 = time.perf_counter()
-        for _ in range(args.batch_steps):
-            s.fixed_step()
-        cf.synchronize()
-        plain_wall += time.perf_counter() - t2
-        s.close()
-    plain_sps = len(sample) * args.batch_steps / plain_wall
-    log(f"[batch3d] plain {plain_wall / (len(sample) * args.batch_steps) * 1e3:.2f} ms per mesh-step")
-    return {
+    out = {
         "workload": "uneven 3D batch: 1,024 cube grids, 4-40 cubes per axis (seed 4), QIE-v, CG rel 1e-4",
         "cg_reason_codes": "1 rel, 2 abs, 3 max_iter, 4 breakdown (|d.q| < abs^2, ILSS_CG.h:97)",
         "meshes": len(meshes), "tets": T, "nodes": N, "steps": args.batch_steps, "warmup": args.warmup,
-        "cg_max_iter": args.batch_max_iter, "cg_iters_mean": round(float(it.mean()), 2),
-        "cg_iters_max": int(it.max()),
-        "cg_reasons": {str(k): int((reasons == k).sum()) for k in (1, 2, 3, 4)},
-        "batched_steps_per_s": round(batched_sps, 1), "batched_ms_per_step": round(wall / args.batch_steps * 1e3, 2),
-        "plain_steps_per_s": round(plain_sps, 1), "plain_sample": f"{len(sample)} of {len(meshes)} meshes (every 8th)",
-        "speedup": round(batched_sps / plain_sps, 2),
-        "kx": {"ms": round(kx_ms, 4), "gtets": round(T / (kx_ms * 1e-3) / 1e9, 2),
-               "achieved_gbs": round(b_alg / (kx_ms * 1e-3) / 1e9, 1),
-               "frac": round(b_alg / (kx_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
+        "cg_max_iter": args.batch_max_iter,
         "tile": {"threads": tuned["tile_threads"], "elems": tuned["tile_elems"], "cg_chunk": tuned["cg_chunk"],
                  "graphs": not args.no_graphs},
         "autotune": None if args.no_autotune else {k: tuned[k] for k in ("kx_ns_per_tet", "step_ms", "sample",
                                                                           "seconds")},
+        "variants": {},
     }
+    for name, abs_eps in CG_VARIANTS:
+        sids = [b.add_sim(mid, with_abs(m["params"], abs_eps), knc=m["knc"]) for mid, m in zip(mids, meshes)]
+        for _ in range(args.warmup):
+            b.fixed_step(stats=False)
+        b.synchronize()
+        iters = []
+        t1 = time.perf_counter()
+        for _ in range(args.batch_steps):
+            st = b.fixed_step()
+            iters.append([st[s].cg_iters for s in sids])
+        b.synchronize()
+        wall = time.perf_counter() - t1
+        it = np.array(iters)
+        res = residual_stats([st[s] for s in sids])
+        batched_sps = len(meshes) * args.batch_steps / wall
+        if name == CG_VARIANTS[0][0]:
+            kx_ms = b.time_kx(args.kx_reps)
+            out["kx"] = {"ms": round(kx_ms, 4), "gtets": round(T / (kx_ms * 1e-3) / 1e9, 2),
+                         "achieved_gbs": round(b_alg / (kx_ms * 1e-3) / 1e9, 1),
+                         "frac": round(b_alg / (kx_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+        for s in sids:
+            b.remove_sim(s)
+        # plain path, one mesh at a time
+        sample = meshes[::8]
+        plain_wall = 0.0
+        for m in sample:
+            s = cf.LeafSolid3D(m["pos"], m["tets"], with_abs(m["params"], abs_eps), knc=m["knc"])
+            for _ in range(args.warmup):
+                s.fixed_step()
+            cf.synchronize()
+            t2 = time.perf_counter()
+            for _ in range(args.batch_steps):
+                s.fixed_step()
+            cf.synchronize()
+            plain_wall += time.perf_counter() - t2
+            s.close()
+        plain_sps = len(sample) * args.batch_steps / plain_wall
+        log(f"[batch3d] {name}: batched {wall / args.batch_steps * 1e3:.1f} ms/step, iters mean {it.mean():.1f} "
+            f"max {it.max()}, rel p99 {res['rel_p99']:.2e}; plain "
+            f"{plain_wall / (len(sample) * args.batch_steps) * 1e3:.2f} ms per mesh-step")
+        out["variants"][name] = {
+            "cg_abs_epsilon": abs_eps, "cg_iters_mean": round(float(it.mean()), 2), "cg_iters_max": int(it.max()),
+            "residual": res,
+            "batched_steps_per_s": round(batched_sps, 1), "batched_ms_per_step": round(wall / args.batch_steps * 1e3, 2),
+            "plain_steps_per_s": round(plain_sps, 1), "plain_sample": f"{len(sample)} of {len(meshes)} meshes (every 8th)",
+            "speedup": round(batched_sps / plain_sps, 2)}
+    b.close()
+    head = out["variants"][CG_VARIANTS[1][0]]
+    out.update({k: head[k] for k in ("batched_steps_per_s", "plain_steps_per_s", "speedup")})
+    out["headline_variant"] = CG_VARIANTS[1][0]
+    if not args.no_cpu_baseline:
+        from oracle import orc
+        abs_eps = CG_VARIANTS[1][1]
+        out["cpu_baseline"] = cpu_batch_baseline(
+            meshes[::8], lambda m: orc.Solid(m["pos"], m["tets"], orc.default_params(
+                young_modulus=m["E"], poisson_ratio=m["nu"], rm=orc.RM_DAPD, cg_max_iter=args.batch_max_iter,
+                cg_abs_epsilon=abs_eps), m["knc"]),
+            "steps/s", "the oracle (stored Ke, FP32) on every 8th mesh, CG rel 1e-4 only")
+    return out
 
 
 def bench_batch2d(cf, args):
     """2D batch: 65,536 n x n grids (n uniform in [4,32], seed 3), E
     log-uniform [1e4,1e7], nu [0.2,0.45], bottom edge held. The batched step is
     ONE kernel launch for all meshes; the plain single-mesh path (the same
-    kernel, one mesh per launch) runs on every 64th mesh."""
+    kernel, one mesh per launch) runs on every 64th mesh; under both CG
+    settings."""
     from risdsdgcfem_thesis_amd import workloads
-    t0 = time.perf_counter()
     meshes = workloads.batch2d(count=args.batch2d_count, cg_max_iter=args.batch_max_iter)
-    b = cf.Batch2()
-    for m in meshes:
-        b.add_sim(m["pos"], m["tris"], m["params"], knc=m["knc"])
-    info = b.info()
-    b.fixed_step(stats=False)  # uploads; first step
-    log(f"[batch2d] {info['sims']} meshes, {info['tris']} tris, {info['nodes']} nodes, {info['workgroups']} "
-        f"workgroups; setup {time.perf_counter() - t0:.1f} s")
-    for _ in range(args.warmup):
-        b.fixed_step(stats=False)
-    cf.synchronize()
-    t1 = time.perf_counter()
-    for _ in range(args.batch_steps):
-        b.fixed_step(stats=False)
-    wall = time.perf_counter() - t1
-    st = b.fixed_step()  # one more step, untimed, for the CG statistics
-    it = np.array([x.cg_iters for x in st])
-    reasons = np.array([x.cg_reason for x in st])
-    dev_ms = st[0].step_ms
     T = np.array([m["tris"].size // 3 for m in meshes])
-    tri_matvecs = float(((it + 1) * T).sum())
-    batched_sps = len(meshes) * args.batch_steps / wall
-    log(f"[batch2d] batched {wall / args.batch_steps * 1e3:.2f} ms/step (device {dev_ms:.2f} ms), "
-        f"iters mean {it.mean():.1f}")
-    b.close()
-    sample = meshes[::64]
-    plain_wall = 0.0
-    for m in sample:
-        s = cf.LeafSolid2D(m["pos"], m["tris"], m["params"], knc=m["knc"])
-        s.fixed_step()
+    out = {"workload": "2D batch: 65,536 n x n grids, n in [4,32] (seed 3), QIE-v, CG rel 1e-4, one launch per step",
+           "meshes": len(meshes), "steps": args.batch_steps, "warmup": args.warmup,
+           "cg_max_iter": args.batch_max_iter, "variants": {}}
+    for name, abs_eps in CG_VARIANTS:
+        t0 = time.perf_counter()
+        b = cf.Batch2()
+        for m in meshes:
+            b.add_sim(m["pos"], m["tris"], with_abs(m["params"], abs_eps), knc=m["knc"])
+        info = b.info()
+        b.fixed_step(stats=False)  # uploads; first step
+        log(f"[batch2d] {name}: {info['sims']} meshes, {info['tris']} tris, {info['nodes']} nodes, "
+            f"{info['workgroups']} workgroups; setup {time.perf_counter() - t0:.1f} s")
+        out.update({"tris": int(info["tris"]), "nodes": int(info["nodes"]), "workgroups": int(info["workgroups"])})
         for _ in range(args.warmup):
-            s.fixed_step()
-        t2 = time.perf_counter()
+            b.fixed_step(stats=False)
+        cf.synchronize()
+        t1 = time.perf_counter()
         for _ in range(args.batch_steps):
+            b.fixed_step(stats=False)
+        wall = time.perf_counter() - t1
+        st = b.fixed_step()  # one more step, untimed, for the CG statistics
+        it = np.array([x.cg_iters for x in st])
+        res = residual_stats(st)
+        dev_ms = st[0].step_ms
+        tri_matvecs = float(((it + 1) * T).sum())
+        batched_sps = len(meshes) * args.batch_steps / wall
+        b.close()
+        sample = meshes[::64]
+        plain_wall = 0.0
+        for m in sample:
+            s = cf.LeafSolid2D(m["pos"], m["tris"], with_abs(m["params"], abs_eps), knc=m["knc"])
             s.fixed_step()
-        plain_wall += time.perf_counter() - t2
-        s.close()
-    plain_sps = len(sample) * args.batch_steps / plain_wall
-    log(f"[batch2d] plain {plain_wall / (len(sample) * args.batch_steps) * 1e3:.3f} ms per mesh-step")
-    return {
-        "workload": "2D batch: 65,536 n x n grids, n in [4,32] (seed 3), QIE-v, CG rel 1e-4, one launch per step",
-        "meshes": len(meshes), "tris": int(info["tris"]), "nodes": int(info["nodes"]),
-        "workgroups": int(info["workgroups"]), "steps": args.batch_steps, "warmup": args.warmup,
-        "cg_max_iter": args.batch_max_iter, "cg_iters_mean": round(float(it.mean()), 2),
-        "cg_iters_max": int(it.max()),
-        "cg_reasons": {str(k): int((reasons == k).sum()) for k in (1, 2, 3, 4)},
-        "batched_steps_per_s": round(batched_sps, 1), "batched_ms_per_step": round(wall / args.batch_steps * 1e3, 3),
-        "batched_device_ms_per_step": round(float(dev_ms), 3),
-        "gtri_matvecs_per_s": round(tri_matvecs / (dev_ms * 1e-3) / 1e9, 2),
-        "plain_steps_per_s": round(plain_sps, 1), "plain_sample": f"{len(sample)} of {len(meshes)} meshes (every 64th)",
-        "speedup": round(batched_sps / plain_sps, 1),
-    }
+            for _ in range(args.warmup):
+                s.fixed_step()
+            t2 = time.perf_counter()
+            for _ in range(args.batch_steps):
+                s.fixed_step()
+            plain_wall += time.perf_counter() - t2
+            s.close()
+        plain_sps = len(sample) * args.batch_steps / plain_wall
+        log(f"[batch2d] {name}: batched {wall / args.batch_steps * 1e3:.2f} ms/step (device {dev_ms:.2f} ms), "
+            f"iters mean {it.mean():.1f}, rel p99 {res['rel_p99']:.2e}; plain "
+            f"{plain_wall / (len(sample) * args.batch_steps) * 1e3:.3f} ms per mesh-step")
+        out["variants"][name] = {
+            "cg_abs_epsilon": abs_eps, "cg_iters_mean": round(float(it.mean()), 2), "cg_iters_max": int(it.max()),
+            "residual": res,
+            "batched_steps_per_s": round(batched_sps, 1),
+            "batched_ms_per_step": round(wall / args.batch_steps * 1e3, 3),
+            "batched_device_ms_per_step": round(float(dev_ms), 3),
+            "gtri_matvecs_per_s": round(tri_matvecs / (dev_ms * 1e-3) / 1e9, 2),
+            "plain_steps_per_s": round(plain_sps, 1),
+            "plain_sample": f"{len(sample)} of {len(meshes)} meshes (every 64th)",
+            "speedup": round(batched_sps / plain_sps, 1)}
+    head = out["variants"][CG_VARIANTS[1][0]]
+    out.update({k: head[k] for k in ("batched_steps_per_s", "plain_steps_per_s", "speedup",
+                                     "batched_device_ms_per_step", "gtri_matvecs_per_s")})
+    out["headline_variant"] = CG_VARIANTS[1][0]
+    if not args.no_cpu_baseline:
+        from oracle import orc
+        abs_eps = CG_VARIANTS[1][1]
+        out["cpu_baseline"] = cpu_batch_baseline(
+            meshes[::64], lambda m: orc.Solid2(m["pos"], m["tris"], orc.default_params(
+                young_modulus=m["E"], poisson_ratio=m["nu"], rm=orc.RM_DAPD, cg_max_iter=args.batch_max_iter,
+                cg_abs_epsilon=abs_eps, gravity=(0.0, -9.81, 0.0)), knc=m["knc"]),
+            "steps/s", "the oracle (stored Ke, FP32) on every 64th mesh, CG rel 1e-4 only", steps=3)
+    return out
 
 
 def bench_serve(cf, args):
@@ -386,6 +524,9 @@ def bench_single(cf, args):
     if not args.no_cpu_baseline:
         log("[bench] cpu baseline (oracle K*x, one core)")
         cpu = cpu_baseline()
+        cpu["host"] = host_cpu()
+        log("[bench] config 1: per-step medians, oracle and GPU")
+        cpu["config1_step"] = config1_steps(cf)
     out = {
         "metric": METRIC, "value": round(value, 3), "unit": "Gtet-matvec/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(wall * 1e3 / args.steps, 3),

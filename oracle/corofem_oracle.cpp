@@ -11,6 +11,8 @@
 #include <limits>
 #include <map>
 #include <vector>
+#include <thread>
+#include <atomic>
 #include <algorithm>
 #include <utility>
 
@@ -1520,6 +1522,20 @@ void orc_solid_step(orc_solid* s, float dt, orc_step_stats* st) {
 
 float orc_solid_total_time(const orc_solid* s) { return s->total_time; }
 
+// Many independent solids, one fixed step each, spread over `threads` host
+// threads (each solid stepped whole by one thread): the mesh-parallel CPU
+// baseline of the batched workloads.
+void orc_solids_fixed_step_parallel(orc_solid* const* solids, uint32_t n, uint32_t threads) {
+    std::atomic<uint32_t> next{0};
+    auto work = [&]() {
+        for (uint32_t i = next++; i < n; i = next++) orc_solid_fixed_step(solids[i], solids[i]->p.fixed_dt, nullptr);
+    };
+    std::vector<std::thread> pool;
+    for (uint32_t t = 1; t < std::max(1u, threads); t++) pool.emplace_back(work);
+    work();
+    for (auto& t : pool) t.join();
+}
+
 void orc_rayleigh(float f1, float f2, float zeta, float c[2]) { rayleigh(f1, f2, zeta, c); }
 
 void orc_solid_get_state(const orc_solid* s, float* x, float* v) {
@@ -2055,6 +2071,17 @@ void orc_solid2_step(orc_solid2* s, float dt, orc_step_stats* st) {
 }
 
 float orc_solid2_total_time(const orc_solid2* s) { return s->total_time; }
+
+void orc_solids2_fixed_step_parallel(orc_solid2* const* solids, uint32_t n, uint32_t threads) {
+    std::atomic<uint32_t> next{0};
+    auto work = [&]() {
+        for (uint32_t i = next++; i < n; i = next++) orc_solid2_fixed_step(solids[i], solids[i]->p.fixed_dt, nullptr);
+    };
+    std::vector<std::thread> pool;
+    for (uint32_t t = 1; t < std::max(1u, threads); t++) pool.emplace_back(work);
+    work();
+    for (auto& t : pool) t.join();
+}
 
 void orc_solid2_get_state(const orc_solid2* s, float* x, float* v) {
     if (x) std::memcpy(x, s->x.data(), s->x.size() * sizeof(float));

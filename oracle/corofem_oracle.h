@@ -158,6 +158,9 @@ void orc_solid_fixed_step(orc_solid* s, float dt, orc_step_stats* st);
    greater than time_scale fixed_dt; num_fixed_steps counts them. */
 void orc_solid_step(orc_solid* s, float dt, orc_step_stats* st);
 float orc_solid_total_time(const orc_solid* s);
+/* One fixed step (each solid's own fixed_dt) of n independent solids over
+   `threads` host threads: the mesh-parallel CPU baseline of batches. */
+void orc_solids_fixed_step_parallel(orc_solid* const* solids, uint32_t n, uint32_t threads);
 /* Rayleigh coefficients c = inv(Omega) (2 zeta, 2 zeta), Omega = [[1/f1, f1], [1/f2, f2]]
    (LeafDSH_Solid3D_FEM.h:347-354, LeafDSH_Solid2D_FEM.h:333-341). */
 void orc_rayleigh(float f1, float f2, float zeta, float c[2]);
@@ -189,6 +192,7 @@ void orc_solid2_fixed_step(orc_solid2* s, float dt, orc_step_stats* st);
 /* Step(dt) of the 2D leaf (LeafDSH_Solid2D_FEM.cpp:264-294), as orc_solid_step. */
 void orc_solid2_step(orc_solid2* s, float dt, orc_step_stats* st);
 float orc_solid2_total_time(const orc_solid2* s);
+void orc_solids2_fixed_step_parallel(orc_solid2* const* solids, uint32_t n, uint32_t threads);
 void orc_solid2_get_state(const orc_solid2* s, float* x, float* v);
 void orc_solid2_set_state(orc_solid2* s, const float* x, const float* v);
 void orc_solid2_node_masses(const orc_solid2* s, float* m);

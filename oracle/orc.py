@@ -103,6 +103,8 @@ def _load():
         "orc_rayleigh": (None, [C.c_float, C.c_float, C.c_float, _f]),
         "orc_solid2_step": (None, [C.c_void_p, C.c_float, C.POINTER(StepStats)]),
         "orc_solid2_total_time": (C.c_float, [C.c_void_p]),
+        "orc_solids_fixed_step_parallel": (None, [C.POINTER(C.c_void_p), C.c_uint32, C.c_uint32]),
+        "orc_solids2_fixed_step_parallel": (None, [C.POINTER(C.c_void_p), C.c_uint32, C.c_uint32]),
         "orc_solid_get_state": (None, [C.c_void_p, _f, _f]),
         "orc_solid_set_state": (None, [C.c_void_p, _f, _f]),
         "orc_solid_node_mass": (C.c_float, [C.c_void_p]),
@@ -387,6 +389,15 @@ def default_params(**kw):
         else:
             setattr(p, k, v)
     return p
+
+
+def step_parallel(solids, threads):
+    """One fixed step of each of `solids` (all Solid or all Solid2) over `threads` host threads."""
+    hs = (C.c_void_p * len(solids))(*[s.h for s in solids])
+    if isinstance(solids[0], Solid2):
+        lib().orc_solids2_fixed_step_parallel(hs, len(solids), int(threads))
+    else:
+        lib().orc_solids_fixed_step_parallel(hs, len(solids), int(threads))
 
 
 class Multi:
