@@ -39,6 +39,7 @@ Material material_from(float E, float nu, int32_t mm) {
 static uint32_t g_tile_elems = 0;
 static uint32_t g_tile_threads = 256;
 static int g_tile_persistent = 0;
+static int g_tile_alternate = 1;  // odd CG iterations walk the tiles backwards
 
 template <class T>
 static int dalloc(T** p, size_t count) {
@@ -138,6 +139,9 @@ struct CgBuffers {
     double* part_a = nullptr;  // d.q partials
     double* part_b = nullptr;  // r.r partials
     int32_t* h_flag = nullptr; // pinned
+    int32_t* h_done = nullptr; // pinned, kMaxCgIter + 2: done[k + 1], written by the iteration-k d kernel
+    int32_t* d_done = nullptr; // h_done's device address
+    hipEvent_t ev_chunk[2] = {nullptr, nullptr};  // ends of the two chunks in flight
 };
 
 struct cf_cg {
@@ -203,12 +207,23 @@ static int cg_alloc(CgBuffers& b, uint32_t n) {
     CF_TRY(dalloc(&b.part_a, kMaxPartials));
     CF_TRY(dalloc(&b.part_b, kMaxPartials));
     CF_CHECK_HIP(hipHostMalloc(reinterpret_cast<void**>(&b.h_flag), sizeof(int32_t)));
+    CF_CHECK_HIP(hipHostMalloc(reinterpret_cast<void**>(&b.h_done), (kMaxCgIter + 2) * sizeof(int32_t)));
+    std::memset(b.h_done, 0, (kMaxCgIter + 2) * sizeof(int32_t));
+    CF_CHECK_HIP(hipHostGetDevicePointer(reinterpret_cast<void**>(&b.d_done), b.h_done, 0));
+    for (hipEvent_t& e : b.ev_chunk) CF_CHECK_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     return CF_OK;
 }
 static void cg_free(CgBuffers& b) {
     dfree(b.r); dfree(b.d); dfree(b.q); dfree(b.st); dfree(b.part_a); dfree(b.part_b);
     if (b.h_flag) (void)hipHostFree(b.h_flag);
+    if (b.h_done) (void)hipHostFree(b.h_done);
+    for (hipEvent_t& e : b.ev_chunk) {
+        if (e) (void)hipEventDestroy(e);
+        e = nullptr;
+    }
     b.h_flag = nullptr;
+    b.h_done = nullptr;
+    b.d_done = nullptr;
 }
 
 static void tetset_free(cf_tetset* s) {
@@ -332,10 +347,14 @@ static int cache_stats_fetch(cf_cache* c) {
 // checks the device's done flag between chunks.
 //
 // One leaf's CG in resumable phases: cg_begin enqueues r = S(b - A y), d = r
-// and the start test; cg_launch enqueues up to eight iterations (each skipped
-// on the device once the solve has stopped) and the read-back of the stop
-// flag; cg_poll waits for that flag; cg_finish reads the scalars. The
-// multi-solid step interleaves several leaves' phases on their own streams.
+// and the start test; cg_launch enqueues up to kCgChunk iterations (each
+// skipped on the device once the solve has stopped), whose last d kernel
+// writes the stop flag into pinned host memory; cg_poll waits for the stream
+// and reads it; cg_finish reads the scalars. The multi-solid step interleaves
+// several leaves' phases on their own streams; the single-leaf solve keeps
+// one chunk queued behind the one it waits for.
+constexpr uint32_t kCgChunk = 8;
+
 struct CgRun {
     NodeTerms nt;
     uint32_t k = 0, max_it = 0;
@@ -375,10 +394,12 @@ static int cg_enqueue(cf_solid* s, const CgRun& run, uint32_t k0, uint32_t kend)
     const int nnode = node_blocks(s->N);
     for (uint32_t k = k0; k < kend; k++) {
         const int32_t* skip = &b.st->done[k];
-        CF_TRY(launch_forces_skip(t, kModeDfx, b.d, s->cache->R, cache_mat(s->cache), skip, st));
+        CF_TRY(launch_forces_skip(t, kModeDfx, b.d, s->cache->R, cache_mat(s->cache), skip, st,
+                                  g_tile_alternate && (k & 1)));
         CF_TRY(launch_gather_skip(t, kGatherMatvec, run.nt, nnode, skip, st));
-        CF_TRY(launch_cg_step_xr(n, k, b.st, s->y, b.r, b.d, b.q, b.part_a, nnode, b.part_b, st));
-        CF_TRY(launch_cg_step_d(n, k, b.st, b.d, b.r, run.nt.knc_mask, b.part_b, st));
+        CF_TRY(launch_cg_step_xr(n, k, b.st, b.r, b.q, b.part_a, nnode, b.part_b, st));
+        CF_TRY(launch_cg_step_d(n, k, b.st, s->y, b.d, b.r, run.nt.knc_mask, b.part_b, st,
+                                k + 1 == kend ? b.d_done : nullptr));
     }
     return CF_OK;
 }
@@ -390,7 +411,7 @@ static int cg_graph(cf_solid* s, const CgRun& run, uint32_t k0, uint32_t kend, h
         g.clear();
         g.fM = run.nt.fM; g.fK = run.nt.fK; g.max_it = run.max_it; g.knc_mask = run.nt.knc_mask;
     }
-    const size_t c = k0 / 8;
+    const size_t c = k0 / kCgChunk;
     if (g.chunk.size() <= c) g.chunk.resize(c + 1, nullptr);
     if (!g.chunk[c]) {
         CF_CHECK_HIP(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
@@ -415,7 +436,7 @@ static int cg_graph(cf_solid* s, const CgRun& run, uint32_t k0, uint32_t kend, h
 }
 
 static int cg_launch(cf_solid* s, CgRun* run, bool graphs = false) {
-    const uint32_t kend = std::min(run->max_it, run->k + 8);
+    const uint32_t kend = std::min(run->max_it, run->k + kCgChunk);
     if (graphs && kend > run->k) {
         hipGraphExec_t g = nullptr;
         CF_TRY(cg_graph(s, *run, run->k, kend, &g));
@@ -424,14 +445,18 @@ static int cg_launch(cf_solid* s, CgRun* run, bool graphs = false) {
         CF_TRY(cg_enqueue(s, *run, run->k, kend));
     }
     run->k = kend;
-    CgBuffers& b = s->cg;
-    CF_CHECK_HIP(hipMemcpyAsync(b.h_flag, &b.st->done[run->k], sizeof(int32_t), hipMemcpyDeviceToHost, s->stream));
     return CF_OK;
+}
+
+// Reads the stop flag of the chunk ending at iteration kend once the device
+// has finished that chunk (host_done[kend] was written by its last d kernel).
+static bool cg_chunk_stopped(const cf_solid* s, const CgRun& run, uint32_t kend) {
+    return kend >= run.max_it || reinterpret_cast<volatile const int32_t*>(s->cg.h_done)[kend] != 0;
 }
 
 static int cg_poll(cf_solid* s, CgRun* run) {
     CF_CHECK_HIP(hipStreamSynchronize(s->stream));
-    if (*s->cg.h_flag || run->k >= run->max_it) run->done = true;
+    if (cg_chunk_stopped(s, *run, run->k)) run->done = true;
     return CF_OK;
 }
 
@@ -467,13 +492,29 @@ static int cg_finish(cf_solid* s, const CgRun& run, cf_cg_result* res) {
     return CF_OK;
 }
 
+// The single-leaf solve: chunk c + 1 is queued before the host waits for
+// chunk c, so the device does not idle while the host reads a stop flag and
+// enqueues; a solve that stops inside chunk c leaves chunk c + 1 to run as
+// skipped launches.
 static int solid_cg(cf_solid* s, float fM, float fK, cf_cg_result* res) {
     CgRun run;
     CF_TRY(cg_begin(s, fM, fK, &run));
-    while (!run.done) {
-        CF_TRY(cg_launch(s, &run));
-        CF_TRY(cg_poll(s, &run));
+    CgBuffers& b = s->cg;
+    uint32_t kend[2] = {0, 0};
+    CF_TRY(cg_launch(s, &run));
+    kend[0] = run.k;
+    CF_CHECK_HIP(hipEventRecord(b.ev_chunk[0], s->stream));
+    for (uint32_t c = 0;; c++) {
+        const uint32_t cur = c & 1, nxt = cur ^ 1;
+        if (run.k < run.max_it) {
+            CF_TRY(cg_launch(s, &run));
+            kend[nxt] = run.k;
+            CF_CHECK_HIP(hipEventRecord(b.ev_chunk[nxt], s->stream));
+        }
+        CF_CHECK_HIP(hipEventSynchronize(b.ev_chunk[cur]));
+        if (cg_chunk_stopped(s, run, kend[cur])) break;
     }
+    run.done = true;
     return cg_finish(s, run, res);
 }
 
@@ -1107,8 +1148,8 @@ int cf::cg_solve_on(const CgWork& b, uint32_t n, hipStream_t st, cf_av_fn av, cf
         av(user, b.q, b.d);
         if (cons) cons(user, b.q);
         CF_TRY(launch_cg_dot(n, b.d, b.q, b.part_a, nvec, st));
-        CF_TRY(launch_cg_step_xr(n, k, b.st, x, b.r, b.d, b.q, b.part_a, nvec, b.part_b, st));
-        CF_TRY(launch_cg_step_d(n, k, b.st, b.d, b.r, nullptr, b.part_b, st));
+        CF_TRY(launch_cg_step_xr(n, k, b.st, b.r, b.q, b.part_a, nvec, b.part_b, st));
+        CF_TRY(launch_cg_step_d(n, k, b.st, x, b.d, b.r, nullptr, b.part_b, st));
         if (cons) cons(user, b.d);
         k++;
         CF_CHECK_HIP(hipMemcpyAsync(h_flag, &b.st->done[k], 4, hipMemcpyDeviceToHost, st));
@@ -1708,6 +1749,12 @@ int cf_set_tile_threads(uint32_t n) {
 int cf_set_tile_persistent(int persistent) {
     if (persistent < 0 || persistent > 2) { set_error("tile schedule must be 0, 1 or 2"); return CF_ERR_INVALID_ARG; }
     g_tile_persistent = persistent;
+    return CF_OK;
+}
+
+int cf_set_tile_alternate(int alternate) {
+    if (alternate != 0 && alternate != 1) { set_error("tile alternation must be 0 or 1"); return CF_ERR_INVALID_ARG; }
+    g_tile_alternate = alternate;
     return CF_OK;
 }
 

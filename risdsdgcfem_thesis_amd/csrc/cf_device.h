@@ -440,6 +440,11 @@ struct TileArgs {
     float* partials;
     Material m;
     const int32_t* skip;
+    // Walk each XCD's tile range from its end (one tile per workgroup only).
+    // Consecutive K*x passes in opposite directions start on the tiles the
+    // previous pass touched last, which the memory-side cache may still hold;
+    // every tile writes the same partials either way.
+    uint32_t reverse = 0;
 };
 
 // LDS-only workgroup barrier: orders LDS accesses across the barrier without

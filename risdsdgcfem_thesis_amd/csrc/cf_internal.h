@@ -215,7 +215,7 @@ struct DampArgs {
 };
 int launch_damping(const TetDevice& t, const float* v, const float* R, Material m, const DampArgs& d, hipStream_t s);
 int launch_forces_skip(const TetDevice& t, int mode, const float* in, const float* R, Material m,
-                       const int32_t* skip, hipStream_t s);
+                       const int32_t* skip, hipStream_t s, bool reverse = false);
 int launch_gather(const TetDevice& t, int mode, const NodeTerms& nt, int nblk, hipStream_t s);
 int launch_gather_skip(const TetDevice& t, int mode, const NodeTerms& nt, int nblk, const int32_t* skip,
                        hipStream_t s);
@@ -232,10 +232,13 @@ int launch_cg_start(CgScalars* st, const double* rr_part, int nblk_prev, float r
 int launch_cg_dot(uint32_t n, const float* a, const float* b, double* part, int nblk, hipStream_t s);
 // One CG iteration after q = S A d (alpha, x and r, then beta, the stop
 // test and d); r.r partials per vec_blocks(n) chunk.
-int launch_cg_step_xr(uint32_t n, uint32_t k, CgScalars* st, float* x, float* r, const float* d, const float* q,
-                      const double* dq_part, int ndq, double* rr_part, hipStream_t s);
-int launch_cg_step_d(uint32_t n, uint32_t k, CgScalars* st, float* d, const float* r, const uint8_t* mask,
-                     const double* rr_part, hipStream_t s);
+// alpha ; r -= alpha q ; r.r partials (x is updated by the d kernel)
+int launch_cg_step_xr(uint32_t n, uint32_t k, CgScalars* st, float* r, const float* q, const double* dq_part,
+                      int ndq, double* rr_part, hipStream_t s);
+// beta and the stop test ; x += alpha d ; d = S(beta d + r) unless stopped.
+// host_done (pinned host memory, or null) receives the stop flag done[k + 1].
+int launch_cg_step_d(uint32_t n, uint32_t k, CgScalars* st, float* x, float* d, const float* r,
+                     const uint8_t* mask, const double* rr_part, hipStream_t s, int32_t* host_done = nullptr);
 // Workgroups of the two CG step kernels (0 = default 1024; at most kMaxPartials).
 int set_cg_workgroups(uint32_t n);
 uint32_t get_cg_workgroups();

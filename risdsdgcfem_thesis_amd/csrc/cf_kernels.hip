@@ -82,6 +82,7 @@ __global__ __launch_bounds__(TB) void k_tile_forces(TileArgs a) {
     const uint32_t hi = min(a.num_tiles, lo + per);
     uint32_t t = lo + j;
     if (t >= hi) return;
+    if (!PIPE && a.reverse) t = hi - 1 - j;
 
     TileDesc dt = tile_desc(a, t);
     TileRegs g;
@@ -149,8 +150,8 @@ __global__ __launch_bounds__(TB) void k_tile_forces_kind(TileArgs a) {
     const uint32_t per = (a.num_tiles + 7) / 8;
     const uint32_t lo = xcd * per;
     const uint32_t hi = min(a.num_tiles, lo + per);
-    const uint32_t t = lo + j;
-    if (t >= hi) return;
+    if (lo + j >= hi) return;
+    const uint32_t t = a.reverse ? hi - 1 - j : lo + j;
     tile_block<MODE, KIND>(a, t, L.v, L.r, L.F, L.sp);
 }
 
@@ -278,54 +279,127 @@ __global__ __launch_bounds__(kBlock) void k_dot(uint32_t n, const float* __restr
 // split into nchunk chunks laid out as a grid of nchunk blocks would be: chunk
 // c's r.r partial is the same value whichever workgroup computes it.
 
-// Chunks per workgroup whose operands are loaded before the partial sum (one:
-// with more, the held registers and the chunks' serial barriers cost more
-// than the latency they hide on the 5.24M-tet mesh's 3.26M-float vectors),
-// and the default grid (1,024 workgroups, four per CU; swept 256-8,192).
-constexpr int kCgAhead = 1;
+// Default grid of the two CG step kernels: 1,024 workgroups, four per CU
+// (swept 256-8,192). Each workgroup loads its first chunk's operands before
+// the partial sum; loading more chunks early cost more in held registers and
+// serial barriers than the latency they hid on the 5.24M-tet mesh's
+// 3.26M-float vectors.
 constexpr uint32_t kCgWorkgroups = 1024;
 
-// Chunk c (one float4 per thread, its operands v already loaded when the
-// thread has one, plus the n % 4 tail in chunk 0), as update_xr_block(c,
-// nchunk) / update_d_block(c, nchunk) compute it.
-__device__ inline void xr_chunk(uint32_t n, uint32_t c, uint32_t nchunk, float alpha, float* x, float* r,
-                                const float* d, const float* q, double* rr_part, const Xr4& v, double* sh) {
+// One float4 of the two CG step kernels. The x update of iteration k moves
+// from the first kernel (which then streams r and q only) to the second, which
+// loads d_k anyway: x += alpha_k d_k there, before d is overwritten. The
+// arithmetic of every element is the reference's (ILSS_CG.h:100-133).
+struct R4 { float4 r, q; };
+__device__ inline R4 load_r4(const float* r, const float* q, uint32_t i) {
+    R4 v;
+    v.q = reinterpret_cast<const float4*>(q)[i];
+    v.r = reinterpret_cast<const float4*>(r)[i];
+    return v;
+}
+// r -= alpha q, stored; returns the float4's r.r terms
+__device__ inline double apply_r4(R4 v, float alpha, float* r, uint32_t i) {
+#pragma clang fp contract(off)
+    v.r.x = v.r.x - v.q.x * alpha; v.r.y = v.r.y - v.q.y * alpha;
+    v.r.z = v.r.z - v.q.z * alpha; v.r.w = v.r.w - v.q.w * alpha;
+    reinterpret_cast<float4*>(r)[i] = v.r;
+    return (double)v.r.x * v.r.x + (double)v.r.y * v.r.y + (double)v.r.z * v.r.z + (double)v.r.w * v.r.w;
+}
+// scalar elements first, first + stride, ... below n
+__device__ inline double tail_r(uint32_t n, float alpha, float* r, const float* q, uint32_t first, uint32_t stride,
+                                double acc) {
+#pragma clang fp contract(off)
+    for (uint32_t i = first; i < n; i += stride) {
+        const float ri = r[i] - q[i] * alpha;
+        r[i] = ri;
+        acc += (double)ri * ri;
+    }
+    return acc;
+}
+
+struct Dx4 { float4 d, r, x; };
+__device__ inline Dx4 load_dx4(const float* d, const float* r, const float* x, uint32_t i) {
+    Dx4 v;
+    v.d = reinterpret_cast<const float4*>(d)[i];
+    v.x = reinterpret_cast<const float4*>(x)[i];
+    v.r = reinterpret_cast<const float4*>(r)[i];
+    return v;
+}
+// x += alpha d ; then, unless the solve stopped, d = S(beta d + r)
+__device__ inline void apply_dx4(Dx4 v, float alpha, float beta, bool upd_d, float* x, float* d,
+                                 const uint8_t* mask, uint32_t i) {
+#pragma clang fp contract(off)
+    float4 xi = v.x;
+    xi.x = xi.x + v.d.x * alpha; xi.y = xi.y + v.d.y * alpha;
+    xi.z = xi.z + v.d.z * alpha; xi.w = xi.w + v.d.w * alpha;
+    reinterpret_cast<float4*>(x)[i] = xi;
+    if (upd_d) apply_d4(D4{v.d, v.r}, beta, d, mask, i);
+}
+__device__ inline void tail_dx(uint32_t n, float alpha, float beta, bool upd_d, float* x, float* d, const float* r,
+                               const uint8_t* mask, uint32_t first, uint32_t stride) {
+#pragma clang fp contract(off)
+    for (uint32_t i = first; i < n; i += stride) {
+        const float di = d[i];
+        x[i] = x[i] + di * alpha;
+        if (upd_d) {
+            float dn = beta * di + r[i];
+            if (mask && mask[i / 3]) dn = 0.f;
+            d[i] = dn;
+        }
+    }
+}
+
+// Chunk c: kBlock consecutive float4s (or, without float4 access, kBlock
+// consecutive floats), plus the n % 4 tail in chunk 0 under float4 access.
+// Its r.r partial is the same value whichever workgroup computes it.
+// (The operands are taken by value and replaced by a plain assignment: a
+// conditional expression over a struct reference put them on the stack.)
+template <bool V4>
+__device__ inline void r_chunk(uint32_t n, uint32_t c, uint32_t nchunk, float alpha, float* r, const float* q,
+                               double* rr_part, R4 v, bool loaded, double* sh) {
     const uint32_t i = c * kBlock + threadIdx.x;
     double acc = 0.0;
-    if (i < n / 4) acc += apply_xr4(v, alpha, x, r, i);
-    acc = tail_xr(n, alpha, x, r, d, q, i, nchunk * kBlock, acc);
+    if (V4) {
+        if (i < n / 4) {
+            if (!loaded) v = load_r4(r, q, i);
+            acc += apply_r4(v, alpha, r, i);
+        }
+        acc = tail_r(n, alpha, r, q, 4 * (n / 4) + i, nchunk * kBlock, acc);
+    } else {
+        acc = tail_r(n, alpha, r, q, i, nchunk * kBlock, acc);
+    }
     const double s = block_sum(acc, sh);
     if (threadIdx.x == 0) rr_part[c] = s;
 }
-__device__ inline void d_chunk(uint32_t n, uint32_t c, uint32_t nchunk, float beta, float* d, const float* r,
-                               const uint8_t* mask, const D4& v) {
+template <bool V4>
+__device__ inline void dx_chunk(uint32_t n, uint32_t c, uint32_t nchunk, float alpha, float beta, bool upd_d,
+                                float* x, float* d, const float* r, const uint8_t* mask, Dx4 v, bool loaded) {
     const uint32_t i = c * kBlock + threadIdx.x;
-    if (i < n / 4) apply_d4(v, beta, d, mask, i);
-    tail_d(n, beta, d, r, mask, 4 * (n / 4) + i, nchunk * kBlock);
+    if (V4) {
+        if (i < n / 4) {
+            if (!loaded) v = load_dx4(d, r, x, i);
+            apply_dx4(v, alpha, beta, upd_d, x, d, mask, i);
+        }
+        tail_dx(n, alpha, beta, upd_d, x, d, r, mask, 4 * (n / 4) + i, nchunk * kBlock);
+    } else {
+        tail_dx(n, alpha, beta, upd_d, x, d, r, mask, i, nchunk * kBlock);
+    }
 }
 
-// alpha = delta_k / d.q (break down if |d.q| < abs^2) ; x += alpha d ;
-// r -= alpha q ; per-chunk partials of r.r
+// alpha = delta_k / d.q (break down if |d.q| < abs^2) ; r -= alpha q ;
+// per-chunk partials of r.r. Workgroup g walks chunks g, g + grid, ...; the
+// operands of its first chunk are loaded before the sum of the d.q partials,
+// whose latency they then overlap.
 template <bool V4>
-__global__ __launch_bounds__(kBlock) void k_cg_step_xr(uint32_t n, uint32_t k, CgScalars* st,
-                                                       float* __restrict__ x, float* __restrict__ r,
-                                                       const float* __restrict__ d, const float* __restrict__ q,
-                                                       const double* dq_part, int ndq, double* rr_part,
-                                                       uint32_t nchunk) {
+__global__ __launch_bounds__(kBlock) void k_cg_step_xr(uint32_t n, uint32_t k, CgScalars* st, float* __restrict__ r,
+                                                       const float* __restrict__ q, const double* dq_part, int ndq,
+                                                       double* rr_part, uint32_t nchunk) {
     __shared__ double sh[kBlock / 64];
     if (st->done[k]) return;
     const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
-    // When every chunk holds at most one float4 per thread, the float4s of
-    // this workgroup's first kCgAhead chunks are loaded before the sum, whose
-    // latency they then overlap.
-    const uint32_t n4 = n / 4;
-    const bool early = V4 && n4 <= nchunk * kBlock;
-    Xr4 v[kCgAhead];
-#pragma unroll
-    for (int j = 0; j < kCgAhead; j++) {
-        const uint32_t i = (blockIdx.x + j * gridDim.x) * kBlock + threadIdx.x;
-        v[j] = (early && i < n4) ? load_xr4(x, r, d, q, i) : Xr4{};
-    }
+    const uint32_t i0 = blockIdx.x * kBlock + threadIdx.x;
+    const bool first = V4 && blockIdx.x < nchunk && i0 < n / 4;
+    const R4 v = first ? load_r4(r, q, i0) : R4{};
     const double dq = reduce_partials_wide(dq_part, ndq, sh);
     if (fabs(dq) < st->abs_sq) {
         if (lead) {
@@ -340,44 +414,34 @@ __global__ __launch_bounds__(kBlock) void k_cg_step_xr(uint32_t n, uint32_t k, C
         st->dq[k] = dq;
         st->alpha[k] = alpha;
     }
-    if (!early) {
-        for (uint32_t c = blockIdx.x; c < nchunk; c += gridDim.x)
-            update_xr_block<V4>(n, alpha, x, r, d, q, rr_part, c, nchunk, sh);
-        return;
-    }
-#pragma unroll
-    for (int j = 0; j < kCgAhead; j++) {
-        const uint32_t c = blockIdx.x + j * gridDim.x;
-        if (c < nchunk) xr_chunk(n, c, nchunk, alpha, x, r, d, q, rr_part, v[j], sh);
-    }
-    for (uint32_t c = blockIdx.x + kCgAhead * gridDim.x; c < nchunk; c += gridDim.x) {
-        const uint32_t i = c * kBlock + threadIdx.x;
-        xr_chunk(n, c, nchunk, alpha, x, r, d, q, rr_part, i < n4 ? load_xr4(x, r, d, q, i) : Xr4{}, sh);
-    }
+    for (uint32_t c = blockIdx.x; c < nchunk; c += gridDim.x)
+        r_chunk<V4>(n, c, nchunk, alpha, r, q, rr_part, v, first && c == blockIdx.x, sh);
 }
 
-// delta_{k+1} = r.r ; beta ; stop test ; d = S(beta d + r)
+// delta_{k+1} = r.r ; beta ; stop test ; x += alpha_k d ; d = S(beta d + r)
+// unless the solve stopped. host_done, when not null, receives done[k + 1].
 template <bool V4>
-__global__ __launch_bounds__(kBlock) void k_cg_step_d(uint32_t n, uint32_t k, CgScalars* st,
+__global__ __launch_bounds__(kBlock) void k_cg_step_d(uint32_t n, uint32_t k, CgScalars* st, float* __restrict__ x,
                                                       float* __restrict__ d, const float* __restrict__ r,
                                                       const uint8_t* __restrict__ mask, const double* rr_part,
-                                                      uint32_t nchunk) {
+                                                      uint32_t nchunk, int32_t* host_done) {
     __shared__ double sh[kBlock / 64];
     const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
     if (st->done[k]) {
         // keep every later iteration switched off once the solve has ended
-        if (lead) st->done[k + 1] = 1;
+        if (lead) {
+            st->done[k + 1] = 1;
+            if (host_done) host_done[k + 1] = 1;
+        }
         return;
     }
-    if (st->reason[k + 1] == CF_CG_BREAKDOWN) return;  // done[k + 1] is set
-    const uint32_t n4 = n / 4;
-    const bool early = V4 && n4 <= nchunk * kBlock;  // as in k_cg_step_xr
-    D4 v[kCgAhead];
-#pragma unroll
-    for (int j = 0; j < kCgAhead; j++) {
-        const uint32_t i = (blockIdx.x + j * gridDim.x) * kBlock + threadIdx.x;
-        v[j] = (early && i < n4) ? load_d4(d, r, i) : D4{};
+    if (st->reason[k + 1] == CF_CG_BREAKDOWN) {  // done[k + 1] is set; x keeps its value
+        if (lead && host_done) host_done[k + 1] = 2;
+        return;
     }
+    const uint32_t i0 = blockIdx.x * kBlock + threadIdx.x;
+    const bool first = V4 && blockIdx.x < nchunk && i0 < n / 4;
+    const Dx4 v = first ? load_dx4(d, r, x, i0) : Dx4{};
     const double dn = reduce_partials_wide(rr_part, (int)nchunk, sh);
     const uint32_t k1 = k + 1;
     int reason = 0;
@@ -385,26 +449,16 @@ __global__ __launch_bounds__(kBlock) void k_cg_step_d(uint32_t n, uint32_t k, Cg
     else if (!(dn > st->target)) reason = CF_CG_CONVERGED_REL;
     else if (!(k1 < st->max_iter)) reason = CF_CG_MAX_ITER;
     const float beta = (float)(dn / st->delta[k]);
+    const float alpha = st->alpha[k];
     if (lead) {
         st->delta[k1] = dn;
         st->beta[k] = beta;
         st->reason[k1] = reason;
         st->done[k1] = reason != 0;
+        if (host_done) host_done[k1] = reason != 0;
     }
-    if (reason) return;
-    if (!early) {
-        for (uint32_t c = blockIdx.x; c < nchunk; c += gridDim.x) update_d_block<V4>(n, beta, d, r, mask, c, nchunk);
-        return;
-    }
-#pragma unroll
-    for (int j = 0; j < kCgAhead; j++) {
-        const uint32_t c = blockIdx.x + j * gridDim.x;
-        if (c < nchunk) d_chunk(n, c, nchunk, beta, d, r, mask, v[j]);
-    }
-    for (uint32_t c = blockIdx.x + kCgAhead * gridDim.x; c < nchunk; c += gridDim.x) {
-        const uint32_t i = c * kBlock + threadIdx.x;
-        d_chunk(n, c, nchunk, beta, d, r, mask, i < n4 ? load_d4(d, r, i) : D4{});
-    }
+    for (uint32_t c = blockIdx.x; c < nchunk; c += gridDim.x)
+        dx_chunk<V4>(n, c, nchunk, alpha, beta, reason == 0, x, d, r, mask, v, first && c == blockIdx.x);
 }
 
 __global__ __launch_bounds__(kBlock) void k_integrate(uint32_t n, float* __restrict__ x, float* __restrict__ v,
@@ -474,9 +528,10 @@ int launch_begin_eval(const TetDevice& t, const float* x, int rm, float thr, flo
 
 
 int launch_forces_skip(const TetDevice& t, int mode, const float* in, const float* R, Material m,
-                       const int32_t* skip, hipStream_t s) {
+                       const int32_t* skip, hipStream_t s, bool reverse) {
     if (t.num_tiles == 0) return CF_OK;
     TileArgs a;
+    a.reverse = reverse ? 1u : 0u;
     a.T = t.T; a.num_tiles = t.num_tiles; a.max_te = t.max_tile_elems; a.max_tn = t.max_tile_nodes;
     a.tile_elem_ptr = t.tile_elem_ptr; a.tile_node_ptr = t.tile_node_ptr; a.tile_nodes = t.tile_nodes;
     a.part_dst = t.part_dst;
@@ -633,27 +688,29 @@ static dim3 cg_step_grid(uint32_t nchunk) {
     return dim3(std::max(1u, std::min(nchunk, g)));
 }
 
-int launch_cg_step_xr(uint32_t n, uint32_t k, CgScalars* st, float* x, float* r, const float* d, const float* q,
-                      const double* dq_part, int ndq, double* rr_part, hipStream_t s) {
-    const uint32_t nchunk = (uint32_t)vec_blocks(n);
+int launch_cg_step_xr(uint32_t n, uint32_t k, CgScalars* st, float* r, const float* q, const double* dq_part,
+                      int ndq, double* rr_part, hipStream_t s) {
+    const uint32_t nchunk = (uint32_t)vec_blocks(n);  // the d kernel sums this many r.r partials
     const dim3 grid = cg_step_grid(nchunk);
-    if (aligned16(x) && aligned16(r) && aligned16(d) && aligned16(q))
-        hipLaunchKernelGGL(k_cg_step_xr<true>, grid, dim3(kBlock), 0, s, n, k, st, x, r, d, q, dq_part, ndq, rr_part,
+    if (aligned16(r) && aligned16(q))
+        hipLaunchKernelGGL(k_cg_step_xr<true>, grid, dim3(kBlock), 0, s, n, k, st, r, q, dq_part, ndq, rr_part,
                            nchunk);
     else
-        hipLaunchKernelGGL(k_cg_step_xr<false>, grid, dim3(kBlock), 0, s, n, k, st, x, r, d, q, dq_part, ndq, rr_part,
+        hipLaunchKernelGGL(k_cg_step_xr<false>, grid, dim3(kBlock), 0, s, n, k, st, r, q, dq_part, ndq, rr_part,
                            nchunk);
     return check_launch();
 }
 
-int launch_cg_step_d(uint32_t n, uint32_t k, CgScalars* st, float* d, const float* r, const uint8_t* mask,
-                     const double* rr_part, hipStream_t s) {
+int launch_cg_step_d(uint32_t n, uint32_t k, CgScalars* st, float* x, float* d, const float* r,
+                     const uint8_t* mask, const double* rr_part, hipStream_t s, int32_t* host_done) {
     const uint32_t nchunk = (uint32_t)vec_blocks(n);
     const dim3 grid = cg_step_grid(nchunk);
-    if (aligned16(d) && aligned16(r))
-        hipLaunchKernelGGL(k_cg_step_d<true>, grid, dim3(kBlock), 0, s, n, k, st, d, r, mask, rr_part, nchunk);
+    if (aligned16(x) && aligned16(d) && aligned16(r))
+        hipLaunchKernelGGL(k_cg_step_d<true>, grid, dim3(kBlock), 0, s, n, k, st, x, d, r, mask, rr_part, nchunk,
+                           host_done);
     else
-        hipLaunchKernelGGL(k_cg_step_d<false>, grid, dim3(kBlock), 0, s, n, k, st, d, r, mask, rr_part, nchunk);
+        hipLaunchKernelGGL(k_cg_step_d<false>, grid, dim3(kBlock), 0, s, n, k, st, x, d, r, mask, rr_part, nchunk,
+                           host_done);
     return check_launch();
 }
 

@@ -225,6 +225,10 @@ def set_tile_persistent(mode):
     check(lib.cf_set_tile_persistent(int(mode)), "cf_set_tile_persistent")
 
 
+def set_tile_alternate(on):
+    check(lib.cf_set_tile_alternate(int(bool(on))), "cf_set_tile_alternate")
+
+
 def get_tile_elems():
     v = C.c_uint32()
     check(lib.cf_get_tile_elems(C.byref(v)), "cf_get_tile_elems")
