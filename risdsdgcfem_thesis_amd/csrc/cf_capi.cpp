@@ -230,7 +230,7 @@ static void tetset_free(cf_tetset* s) {
     TetDevice& t = s->dev;
     dfree(t.ref); dfree(t.conn); dfree(t.dinv); dfree(t.vol);
     dfree(t.tile_elem_ptr); dfree(t.tile_node_ptr); dfree(t.tile_nodes); dfree(t.slot_ptr); dfree(t.ref_tile);
-    dfree(t.lv); dfree(t.rk); dfree(t.node_part_ptr); dfree(t.part_dst); dfree(t.partials);
+    dfree(t.lv); dfree(t.rk); dfree(t.node_part_ptr); dfree(t.part_dst); dfree(t.partials); dfree(t.tile_part);
     if (s->stream) (void)hipStreamDestroy(s->stream);
     s->stream = nullptr;
 }
@@ -301,6 +301,7 @@ static int tetset_bake(cf_tetset* s) {
     t.tile_persistent = g_tile_persistent;
     t.tile_grid = uint32_t(tile_grid_size(t, s->device, g_tile_persistent));
     CF_TRY(dalloc(&t.partials, 3 * (size_t)t.num_parts));
+    CF_TRY(dalloc(&t.tile_part, std::max<size_t>(1, t.num_tiles)));
     CF_TRY(launch_precompute(t, st));
     s->perm.swap(th.perm);
     // total volume: FP32 serial sum in the caller's element order, like
@@ -896,7 +897,7 @@ int cf_cache_create(cf_tetset* set, const float* inv_mass, cf_cache** out) {
     if (rc == CF_OK) rc = dalloc(&c->blk_count, 2 * (size_t)kMaxPartials);
     if (rc == CF_OK) rc = dalloc(&c->blk_sum, kMaxPartials);
     if (rc == CF_OK) rc = dalloc(&c->blk_min, kMaxPartials);
-    if (rc == CF_OK) rc = dalloc(&c->blk_energy, kMaxPartials);
+    if (rc == CF_OK) rc = dalloc(&c->blk_energy, kMaxPartials + 1);  // block partials, then their total
     if (rc == CF_OK) rc = dalloc(&c->doc, std::max<size_t>(1, T));
     if (rc == CF_OK && inv_mass) {
         rc = dalloc(&c->inv_mass, set->dev.N);
@@ -1032,13 +1033,12 @@ int cf_df_x(cf_cache* c, const float* dx, float* df) {
 int cf_V(cf_cache* c, double* energy) {
     if (!c || !c->x || !energy) { set_error("cf_V outside Begin/EndEvaluation"); return CF_ERR_INVALID_ARG; }
     const cf_tetset* s = c->set;
-    CF_TRY(launch_energy(s->dev, c->x, c->R, cache_mat(c), c->blk_energy, c->nblk, s->stream));
-    std::vector<double> part(c->nblk);
-    CF_CHECK_HIP(hipMemcpyAsync(part.data(), c->blk_energy, part.size() * 8, hipMemcpyDeviceToHost, s->stream));
+    double* total = c->blk_energy + kMaxPartials;
+    CF_TRY(launch_energy(s->dev, c->x, c->R, cache_mat(c), c->blk_energy, c->nblk, total, s->stream));
+    double h = 0.0;
+    CF_CHECK_HIP(hipMemcpyAsync(&h, total, sizeof(double), hipMemcpyDeviceToHost, s->stream));
     CF_CHECK_HIP(hipStreamSynchronize(s->stream));
-    double acc = 0;
-    for (double p : part) acc += p;
-    *energy = acc;
+    *energy = h;
     return CF_OK;
 }
 

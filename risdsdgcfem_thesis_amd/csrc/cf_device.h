@@ -1391,6 +1391,30 @@ __device__ inline void gather_block(uint32_t N, const uint32_t* __restrict__ nod
     }
 }
 
+// One element's C_V term (ms/fem/TetrahedronElementSet3.cpp:1456-1488),
+// 1/2 u^T Ke u = 1/2 V sigma:H with H = (R^T D - Dm) Dm^-1: D and Dm hold the
+// current and rest edges (entry 3 c + j: component c of vertex j+1 minus
+// vertex 0), G = Dm^-1, V the rest volume.
+__device__ inline double wrp_energy(const float D[9], const float Dm[9], const float Rm[9], const float G[9],
+                                    float V, const Material& m) {
+#pragma clang fp contract(off)
+    float RD[9], H[9];
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++)
+            RD[3 * i + j] = Rm[i] * D[j] + Rm[3 + i] * D[3 + j] + Rm[6 + i] * D[6 + j] - Dm[3 * i + j];
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) H[3 * i + j] = RD[3 * i] * G[j] + RD[3 * i + 1] * G[3 + j] + RD[3 * i + 2] * G[6 + j];
+    const float sxx = m.a_k1 * H[0] + m.a_nu * (H[4] + H[8]);
+    const float syy = m.a_k1 * H[4] + m.a_nu * (H[0] + H[8]);
+    const float szz = m.a_k1 * H[8] + m.a_nu * (H[0] + H[4]);
+    const float sxy = m.a_k2 * (H[1] + H[3]);
+    const float syz = m.a_k2 * (H[5] + H[7]);
+    const float szx = m.a_k2 * (H[2] + H[6]);
+    const float sH = sxx * H[0] + syy * H[4] + szz * H[8] + sxy * (H[1] + H[3]) + syz * (H[5] + H[7]) +
+                     szx * (H[2] + H[6]);
+    return 0.5 * (double)V * (double)sH;
+}
+
 // C_V (ms/fem/TetrahedronElementSet3.cpp:1456-1488): 1/2 u^T Ke u = 1/2 V sigma:H
 __device__ inline void energy_block(uint32_t T, const uint32_t* __restrict__ conn, const float* __restrict__ ref,
                                     const float* __restrict__ x, const float* __restrict__ R,
@@ -1425,27 +1449,14 @@ __device__ inline void energy_block(uint32_t T, const uint32_t* __restrict__ con
             xs[j][0] = x[3 * (size_t)g]; xs[j][1] = x[3 * (size_t)g + 1]; xs[j][2] = x[3 * (size_t)g + 2];
             rs[j][0] = ref[3 * (size_t)g]; rs[j][1] = ref[3 * (size_t)g + 1]; rs[j][2] = ref[3 * (size_t)g + 2];
         }
-        float D[9], Rm[9], G[9], RD[9], H[9];
-        for (int c = 0; c < 3; c++) {
-            D[3 * c + 0] = xs[1][c] - xs[0][c];
-            D[3 * c + 1] = xs[2][c] - xs[0][c];
-            D[3 * c + 2] = xs[3][c] - xs[0][c];
-        }
+        float D[9], Dm[9], Rm[9], G[9];
+        for (int c = 0; c < 3; c++)
+            for (int j = 0; j < 3; j++) {
+                D[3 * c + j] = xs[j + 1][c] - xs[0][c];
+                Dm[3 * c + j] = rs[j + 1][c] - rs[0][c];
+            }
         for (int k = 0; k < 9; k++) { Rm[k] = R[k * (size_t)T + e]; G[k] = dinv[k * (size_t)T + e]; }
-        for (int i = 0; i < 3; i++)
-            for (int j = 0; j < 3; j++)
-                RD[3 * i + j] = Rm[i] * D[j] + Rm[3 + i] * D[3 + j] + Rm[6 + i] * D[6 + j] - (rs[j + 1][i] - rs[0][i]);
-        for (int i = 0; i < 3; i++)
-            for (int j = 0; j < 3; j++) H[3 * i + j] = RD[3 * i] * G[j] + RD[3 * i + 1] * G[3 + j] + RD[3 * i + 2] * G[6 + j];
-        const float sxx = m.a_k1 * H[0] + m.a_nu * (H[4] + H[8]);
-        const float syy = m.a_k1 * H[4] + m.a_nu * (H[0] + H[8]);
-        const float szz = m.a_k1 * H[8] + m.a_nu * (H[0] + H[4]);
-        const float sxy = m.a_k2 * (H[1] + H[3]);
-        const float syz = m.a_k2 * (H[5] + H[7]);
-        const float szx = m.a_k2 * (H[2] + H[6]);
-        const float sH = sxx * H[0] + syy * H[4] + szz * H[8] + sxy * (H[1] + H[3]) + syz * (H[5] + H[7]) +
-                         szx * (H[2] + H[6]);
-        acc += 0.5 * (double)vol[e] * (double)sH;
+        acc += wrp_energy(D, Dm, Rm, G, vol[e], m);
     }
     const double s = block_sum(acc, sh);
     if (threadIdx.x == 0) blk_part[blk] = s;

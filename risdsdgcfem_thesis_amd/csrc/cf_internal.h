@@ -75,6 +75,7 @@ struct TetDevice {
     uint32_t* node_part_ptr = nullptr;  // N+1: node n's partials are [ptr[n], ptr[n+1])
     uint32_t* part_dst = nullptr;       // num_parts: partial entry of (tile, tile node), tile-major
     float* partials = nullptr;          // 3 * num_parts
+    double* tile_part = nullptr;        // num_tiles: per-tile FP64 sums (elastic energy)
 };
 
 // Reference-order FP32 evaluation of one solid (cf_exact.hip): stored Ke blocks
@@ -220,7 +221,7 @@ int launch_gather(const TetDevice& t, int mode, const NodeTerms& nt, int nblk, h
 int launch_gather_skip(const TetDevice& t, int mode, const NodeTerms& nt, int nblk, const int32_t* skip,
                        hipStream_t s);
 int launch_energy(const TetDevice& t, const float* x, const float* R, Material m, double* blk, int nblk,
-                  hipStream_t s);
+                  double* total, hipStream_t s);
 int vec_blocks(uint32_t n);    // vector kernels over n floats (float4 per thread)
 int node_blocks(uint32_t n);   // one node (or element) per thread
 

@@ -221,6 +221,64 @@ __global__ __launch_bounds__(kBlock) void k_energy(uint32_t T, const uint32_t* _
     energy_block(T, conn, ref, x, R, dinv, vol, m, blk, blockIdx.x, gridDim.x, sh);
 }
 
+// C_V of C_WRP per tile (ms/fem/TetrahedronElementSet3.cpp:1456-1488): the
+// tile's positions (a.in) and rest positions staged in LDS planes as for K*x,
+// Dm^-1 and V rebuilt from them as k_precompute computes them, one element per
+// thread; part[t] is the tile's FP64 sum (each wave's terms summed by
+// wave_sum, the waves in order).
+template <int TB>
+__global__ __launch_bounds__(TB) void k_tile_energy(TileArgs a, double* part) {
+#pragma clang fp contract(off)
+    extern __shared__ float smem_f[];
+    __shared__ double sh[TB / 64];
+    const Slots s_x = make_slots(smem_f, TB), s_r = make_slots(smem_f + 3 * TB, TB);
+    const uint32_t xcd = blockIdx.x % 8, j = blockIdx.x / 8;
+    const uint32_t per = (a.num_tiles + 7) / 8;
+    const uint32_t lo = xcd * per;
+    const uint32_t hi = min(a.num_tiles, lo + per);
+    if (lo + j >= hi) return;
+    const uint32_t t = lo + j;
+    const uint32_t tid = threadIdx.x;
+    const TileDesc dt = tile_desc(a, t);
+    TileRegs g;
+    load_tile_regs_cold(a, dt, tid, g);
+    if (tid < dt.nn) {
+        s_x.x[tid] = g.v[0]; s_x.y[tid] = g.v[1]; s_x.z[tid] = g.v[2];
+        s_r.x[tid] = g.r[0]; s_r.y[tid] = g.r[1]; s_r.z[tid] = g.r[2];
+    }
+    __syncthreads();
+    double e = 0.0;
+    if (tid < dt.ne) {
+        const uint32_t l0 = g.l[0];
+        float D[9], Dm[9], G[9];
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+            const uint32_t l = g.l[q + 1];
+            D[q] = s_x.x[l] - s_x.x[l0]; D[3 + q] = s_x.y[l] - s_x.y[l0]; D[6 + q] = s_x.z[l] - s_x.z[l0];
+            Dm[q] = s_r.x[l] - s_r.x[l0]; Dm[3 + q] = s_r.y[l] - s_r.y[l0]; Dm[6 + q] = s_r.z[l] - s_r.z[l0];
+        }
+        const float detDm = refmath::det3(Dm);
+        const float V = detDm / 6.0f;
+        refmath::inv3(Dm, detDm, G);
+        e = wrp_energy(D, Dm, g.R, G, V, a.m);
+    }
+    e = wave_sum(e);
+    if ((tid & 63) == 0) sh[tid >> 6] = e;
+    __syncthreads();
+    if (tid == 0) {
+        double sum = 0.0;
+        for (int w = 0; w < TB / 64; w++) sum += sh[w];
+        part[t] = sum;
+    }
+}
+
+// *out = the n partials summed in the one-block order (one workgroup).
+__global__ __launch_bounds__(kBlock) void k_sum_partials(const double* part, int n, double* out) {
+    __shared__ double sh[kBlock / 64];
+    const double s = reduce_partials(part, n, sh);
+    if (threadIdx.x == 0) *out = s;
+}
+
 // ============================================================ CG vector kernels
 // ILSS_CG::Solve (ns/ILSS_CG.h:38-151); the projection S zeroes KNC nodes.
 // Vector kernels walk n floats as float4 when every pointer is 16-byte
@@ -652,8 +710,30 @@ int launch_gather(const TetDevice& t, int mode, const NodeTerms& nt, int nblk, h
 }
 
 int launch_energy(const TetDevice& t, const float* x, const float* R, Material m, double* blk, int nblk,
-                  hipStream_t s) {
+                  double* total, hipStream_t s) {
+    if (m.mm == CF_MM_C_WRP && t.num_tiles > 0 && t.tile_part) {
+        TileArgs a;
+        a.T = t.T; a.num_tiles = t.num_tiles; a.max_te = t.max_tile_elems; a.max_tn = t.max_tile_nodes;
+        a.tile_elem_ptr = t.tile_elem_ptr; a.tile_node_ptr = t.tile_node_ptr; a.tile_nodes = t.tile_nodes;
+        a.part_dst = t.part_dst;
+        a.slot_ptr = t.slot_ptr; a.lv = t.lv; a.rk = t.rk; a.R = R; a.dinv = t.dinv; a.vol = t.vol;
+        a.ref = t.ref;
+        a.ref_tile = t.ref_tile;
+        a.in = x; a.partials = t.partials; a.m = m; a.skip = nullptr;
+        const uint32_t g1 = 8 * ((t.num_tiles + 7) / 8);  // one workgroup per tile
+        switch (t.tile_threads) {
+            case 1024: hipLaunchKernelGGL(k_tile_energy<1024>, dim3(g1), dim3(1024), 6 * 1024 * 4, s, a, t.tile_part); break;
+            case 512: hipLaunchKernelGGL(k_tile_energy<512>, dim3(g1), dim3(512), 6 * 512 * 4, s, a, t.tile_part); break;
+            case 128: hipLaunchKernelGGL(k_tile_energy<128>, dim3(g1), dim3(128), 6 * 128 * 4, s, a, t.tile_part); break;
+            default: hipLaunchKernelGGL(k_tile_energy<256>, dim3(g1), dim3(256), 6 * 256 * 4, s, a, t.tile_part); break;
+        }
+        CF_TRY(check_launch());
+        hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(kBlock), 0, s, t.tile_part, int(t.num_tiles), total);
+        return check_launch();
+    }
     hipLaunchKernelGGL(k_energy, dim3(nblk), dim3(kBlock), 0, s, t.T, t.conn, t.ref, x, R, t.dinv, t.vol, m, blk);
+    CF_TRY(check_launch());
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(kBlock), 0, s, blk, nblk, total);
     return check_launch();
 }
 
