@@ -400,9 +400,9 @@ def main():
                     help="CG iteration cap of the batched and served workloads (high enough that every solve "
                          "reaches the 1e-4 residual)")
     ap.add_argument("--cg-chunk", type=int, default=8)
-    ap.add_argument("--tile-forward", action="store_true",
-                    help="single mesh: every CG iteration walks the tiles forwards (default: odd iterations "
-                         "backwards, starting on the tiles the previous K*x touched last)")
+    ap.add_argument("--tile-alternate", type=int, default=1, choices=[0, 1],
+                    help="single mesh: 0 every CG iteration walks the tiles forwards; 1 odd iterations walk them "
+                         "backwards, starting on the tiles the previous K*x touched last")
     ap.add_argument("--no-autotune", action="store_true", help="batched 3D: use --tile-threads/--tile-elems/--cg-chunk")
     ap.add_argument("--no-graphs", action="store_true",
                     help="batched 3D and serving: launch the step's kernels directly instead of recorded HIP graphs "
@@ -467,7 +467,7 @@ def bench_single(cf, args):
     cf.set_tile_threads(args.tile_threads)
     cf.set_tile_persistent(args.tile_schedule)
     cf.set_cg_workgroups(args.cg_workgroups)
-    cf.set_tile_alternate(not args.tile_forward)
+    cf.set_tile_alternate(args.tile_alternate)
     counts = (65, 65, 257)
     half = (0.32, 0.32, 1.28)
     pos, tets = cf.mesh_box(half, counts)
@@ -522,7 +522,7 @@ def bench_single(cf, args):
         "kernels": {"k_tile_forces_ms": round(ms_tile, 5), "k_gather_ms": round(ms_gather, 5)},
         "b_alg_bytes": b_alg, "kx_gtets": round(T / (ms_kx * 1e-3) / 1e9, 2),
         "tile": {"elems": args.tile_elems, "threads": args.tile_threads, "schedule": args.tile_schedule,
-                 "alternate": not args.tile_forward},
+                 "alternate": args.tile_alternate},
         "cg_workgroups": cf.get_cg_workgroups(),
     }
     cpu = None

@@ -297,6 +297,14 @@ static int tetset_bake(cf_tetset* s) {
     t.max_tile_elems = th.max_tile_elems;
     t.max_tile_nodes = th.max_tile_nodes;
     t.num_parts = uint32_t(th.tile_nodes.size());
+    t.uniform_te = 0;
+    if (t.num_tiles > 0) {
+        const uint32_t te = th.tile_elem_ptr[1] - th.tile_elem_ptr[0];
+        bool uniform = te > 0 && th.tile_elem_ptr[t.num_tiles] - th.tile_elem_ptr[t.num_tiles - 1] <= te;
+        for (uint32_t i = 0; uniform && i + 1 < t.num_tiles; i++)
+            uniform = th.tile_elem_ptr[i] == i * te && th.tile_elem_ptr[i + 1] - th.tile_elem_ptr[i] == te;
+        if (uniform) t.uniform_te = te;
+    }
     t.tile_threads = g_tile_threads;
     t.tile_persistent = g_tile_persistent;
     t.tile_grid = uint32_t(tile_grid_size(t, s->device, g_tile_persistent));
@@ -398,7 +406,7 @@ static int cg_enqueue(cf_solid* s, const CgRun& run, uint32_t k0, uint32_t kend)
         CF_TRY(launch_forces_skip(t, kModeDfx, b.d, s->cache->R, cache_mat(s->cache), skip, st,
                                   g_tile_alternate && (k & 1)));
         CF_TRY(launch_gather_skip(t, kGatherMatvec, run.nt, nnode, skip, st));
-        CF_TRY(launch_cg_step_xr(n, k, b.st, b.r, b.q, b.part_a, nnode, b.part_b, st));
+        CF_TRY(launch_cg_step_xr(n, k, b.st, b.r, b.q, b.part_a, matvec_chunks(s->N), b.part_b, st));
         CF_TRY(launch_cg_step_d(n, k, b.st, s->y, b.d, b.r, run.nt.knc_mask, b.part_b, st,
                                 k + 1 == kend ? b.d_done : nullptr));
     }

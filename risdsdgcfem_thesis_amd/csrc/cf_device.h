@@ -445,6 +445,10 @@ struct TileArgs {
     // previous pass touched last, which the memory-side cache may still hold;
     // every tile writes the same partials either way.
     uint32_t reverse = 0;
+    // When every tile but the last holds exactly uniform_te elements, tile t's
+    // elements are [t uniform_te, ...) and their loads need not wait for the
+    // tile's descriptor (0: read tile_elem_ptr).
+    uint32_t uniform_te = 0;
 };
 
 // LDS-only workgroup barrier: orders LDS accesses across the barrier without
@@ -459,8 +463,13 @@ struct TileDesc { uint32_t e0, ne, n0, nn; };
 
 __device__ inline TileDesc tile_desc(const TileArgs& a, uint32_t t) {
     TileDesc d;
-    d.e0 = a.tile_elem_ptr[t];
-    d.ne = a.tile_elem_ptr[t + 1] - d.e0;
+    if (a.uniform_te) {
+        d.e0 = t * a.uniform_te;
+        d.ne = min(a.uniform_te, a.T - d.e0);
+    } else {
+        d.e0 = a.tile_elem_ptr[t];
+        d.ne = a.tile_elem_ptr[t + 1] - d.e0;
+    }
     d.n0 = a.tile_node_ptr[t];
     d.nn = a.tile_node_ptr[t + 1] - d.n0;
     return d;
@@ -1389,6 +1398,79 @@ __device__ inline void gather_block(uint32_t N, const uint32_t* __restrict__ nod
         const double s = block_sum(local, sh);
         if (threadIdx.x == 0) t.partials[blk] = s;
     }
+}
+
+// The matvec gather (LS_A_times_d, LeafDSH_Solid3D_FEM.cpp:1056-1079, and
+// LS_ZeroKNC, :1045-1053) over chunk c of `npt` kBlock nodes: node
+// (c npt + j) kBlock + thread for j < npt. For the first kGatherNpt nodes of a
+// thread the node terms and the first two partials are loaded before any sum,
+// so their latencies overlap; each node still adds its partials in ascending
+// order. Returns the thread's d.q terms, summed in j order.
+constexpr uint32_t kGatherNpt = 2;
+__device__ inline double gather_matvec_chunk(uint32_t N, uint32_t c, uint32_t npt,
+                                             const uint32_t* __restrict__ node_part_ptr,
+                                             const float* __restrict__ partials, const NodeTerms& t) {
+#pragma clang fp contract(off)
+    const float* __restrict__ dv = t.d;
+    float* __restrict__ out = t.out;
+    const uint32_t tid = threadIdx.x;
+    uint32_t nn[kGatherNpt], p0[kGatherNpt], p1[kGatherNpt];
+    float dd[kGatherNpt][3], ms[kGatherNpt], a[kGatherNpt][3], b[kGatherNpt][3];
+    bool kn[kGatherNpt];
+#pragma unroll
+    for (uint32_t j = 0; j < kGatherNpt; j++) {
+        const uint32_t n = (c * npt + j) * kBlock + tid;
+        nn[j] = n;
+        p0[j] = p1[j] = 0;
+        dd[j][0] = dd[j][1] = dd[j][2] = 0.f;
+        ms[j] = 0.f;
+        kn[j] = false;
+        if (n < N) {
+            p0[j] = node_part_ptr[n];
+            p1[j] = node_part_ptr[n + 1];
+            ms[j] = t.mass_array ? t.mass_array[n] : t.mass;
+            dd[j][0] = dv[3 * (size_t)n]; dd[j][1] = dv[3 * (size_t)n + 1]; dd[j][2] = dv[3 * (size_t)n + 2];
+            kn[j] = t.knc_mask && t.knc_mask[n];
+        }
+    }
+#pragma unroll
+    for (uint32_t j = 0; j < kGatherNpt; j++) {
+        a[j][0] = a[j][1] = a[j][2] = 0.f;
+        b[j][0] = b[j][1] = b[j][2] = 0.f;
+        if (p0[j] < p1[j]) {
+            const float* pp = partials + 3 * (size_t)p0[j];
+            a[j][0] = pp[0]; a[j][1] = pp[1]; a[j][2] = pp[2];
+        }
+        if (p0[j] + 1 < p1[j]) {
+            const float* pp = partials + 3 * (size_t)p0[j] + 3;
+            b[j][0] = pp[0]; b[j][1] = pp[1]; b[j][2] = pp[2];
+        }
+    }
+    double acc = 0.0;
+#pragma unroll
+    for (uint32_t j = 0; j < kGatherNpt; j++) {
+        if (nn[j] >= N) continue;
+        float fx = 0.f, fy = 0.f, fz = 0.f;
+        if (p0[j] < p1[j]) { fx += a[j][0]; fy += a[j][1]; fz += a[j][2]; }
+        if (p0[j] + 1 < p1[j]) { fx += b[j][0]; fy += b[j][1]; fz += b[j][2]; }
+        for (uint32_t p = p0[j] + 2; p < p1[j]; p++) {
+            const float* pp = partials + 3 * (size_t)p;
+            fx += pp[0]; fy += pp[1]; fz += pp[2];
+        }
+        const float fac = ms[j] * t.fM;
+        float q0 = -t.fK * fx + fac * dd[j][0];
+        float q1 = -t.fK * fy + fac * dd[j][1];
+        float q2 = -t.fK * fz + fac * dd[j][2];
+        if (kn[j]) { q0 = 0.f; q1 = 0.f; q2 = 0.f; }
+        const size_t i = 3 * (size_t)nn[j];
+        out[i] = q0; out[i + 1] = q1; out[i + 2] = q2;
+        acc += (double)dd[j][0] * q0 + (double)dd[j][1] * q1 + (double)dd[j][2] * q2;
+    }
+    for (uint32_t j = kGatherNpt; j < npt; j++) {
+        const uint32_t n = (c * npt + j) * kBlock + tid;
+        if (n < N) acc += gather_node<kGatherMatvec>(n, node_part_ptr, partials, t);
+    }
+    return acc;
 }
 
 // One element's C_V term (ms/fem/TetrahedronElementSet3.cpp:1456-1488),

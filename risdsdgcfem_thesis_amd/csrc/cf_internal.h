@@ -62,6 +62,7 @@ struct TetDevice {
     uint32_t max_tile_elems = 0;
     uint32_t max_tile_nodes = 0;
     uint32_t num_parts = 0;        // sum over tiles of tile node counts
+    uint32_t uniform_te = 0;       // every tile but the last has exactly this many elements (0: not so)
     uint32_t tile_threads = 256;   // workgroup size of k_tile_forces (256, 512 or 1024)
     uint32_t tile_grid = 8;        // grid of k_tile_forces (persistent, or one tile per workgroup)
     int tile_persistent = 0;       // 0: one tile per workgroup, 1: persistent pipelined, 2: pipelined kernel, one tile per workgroup
@@ -218,8 +219,11 @@ int launch_damping(const TetDevice& t, const float* v, const float* R, Material 
 int launch_forces_skip(const TetDevice& t, int mode, const float* in, const float* R, Material m,
                        const int32_t* skip, hipStream_t s, bool reverse = false);
 int launch_gather(const TetDevice& t, int mode, const NodeTerms& nt, int nblk, hipStream_t s);
+// kGatherMatvec launches its own grid of matvec_chunks(N) workgroups (nblk is
+// ignored) and writes that many d.q partials.
 int launch_gather_skip(const TetDevice& t, int mode, const NodeTerms& nt, int nblk, const int32_t* skip,
                        hipStream_t s);
+int matvec_chunks(uint32_t N);
 int launch_energy(const TetDevice& t, const float* x, const float* R, Material m, double* blk, int nblk,
                   double* total, hipStream_t s);
 int vec_blocks(uint32_t n);    // vector kernels over n floats (float4 per thread)

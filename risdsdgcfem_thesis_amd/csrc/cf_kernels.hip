@@ -211,6 +211,15 @@ __global__ __launch_bounds__(kBlock) void k_gather(GatherArgs a) {
     gather_block<MODE>(a.N, a.node_part_ptr, a.partials, a.t, blockIdx.x, gridDim.x, sh);
 }
 
+// The matvec gather in chunks of npt kBlock nodes, one chunk per workgroup;
+// chunk c's d.q partial in t.partials[c].
+__global__ __launch_bounds__(kBlock) void k_gather_matvec(GatherArgs a, uint32_t npt) {
+    __shared__ double sh[kBlock / 64];
+    if (a.skip && *a.skip) return;
+    const double s = block_sum(gather_matvec_chunk(a.N, blockIdx.x, npt, a.node_part_ptr, a.partials, a.t), sh);
+    if (threadIdx.x == 0) a.t.partials[blockIdx.x] = s;
+}
+
 // ============================================================ energy
 __global__ __launch_bounds__(kBlock) void k_energy(uint32_t T, const uint32_t* __restrict__ conn,
                                                    const float* __restrict__ ref,
@@ -407,57 +416,89 @@ __device__ inline void tail_dx(uint32_t n, float alpha, float beta, bool upd_d, 
     }
 }
 
-// Chunk c: kBlock consecutive float4s (or, without float4 access, kBlock
-// consecutive floats), plus the n % 4 tail in chunk 0 under float4 access.
-// Its r.r partial is the same value whichever workgroup computes it.
-// (The operands are taken by value and replaced by a plain assignment: a
-// conditional expression over a struct reference put them on the stack.)
-template <bool V4>
-__device__ inline void r_chunk(uint32_t n, uint32_t c, uint32_t nchunk, float alpha, float* r, const float* q,
-                               double* rr_part, R4 v, bool loaded, double* sh) {
-    const uint32_t i = c * kBlock + threadIdx.x;
+// Chunk c of the two CG step kernels: `rows` float4s per thread (kCgVpt, or
+// more where kCgVpt would make more than kMaxPartials chunks), float4
+// (c rows + j) kBlock + thread for j < rows (each j a coalesced row), plus the
+// n % 4 tail in chunk 0; without float4 access, the floats
+// c kBlock + thread + m nchunk kBlock. Chunk c's r.r partial is the same value
+// whichever workgroup computes it, so results do not depend on the grid. A
+// workgroup loads the first kCgVpt rows of its first chunk before it sums the
+// previous kernel's partials (PRE: chunk c's operands are in v).
+constexpr uint32_t kCgVpt = 4;
+struct R4Rows { R4 v[kCgVpt]; };
+struct Dx4Rows { Dx4 v[kCgVpt]; };
+
+template <bool V4, bool PRE>
+__device__ inline double r_chunk(uint32_t n, uint32_t c, uint32_t nchunk, uint32_t rows, float alpha, float* r,
+                                 const float* q, const R4Rows& v) {
+    const uint32_t tid = threadIdx.x;
     double acc = 0.0;
     if (V4) {
-        if (i < n / 4) {
-            if (!loaded) v = load_r4(r, q, i);
-            acc += apply_r4(v, alpha, r, i);
+        const uint32_t n4 = n / 4;
+#pragma unroll
+        for (uint32_t j = 0; j < kCgVpt; j++) {
+            const uint32_t i = (c * rows + j) * kBlock + tid;
+            if (i < n4) {
+                R4 w;
+                if (PRE) w = v.v[j];
+                else w = load_r4(r, q, i);
+                acc += apply_r4(w, alpha, r, i);
+            }
         }
-        acc = tail_r(n, alpha, r, q, 4 * (n / 4) + i, nchunk * kBlock, acc);
+        for (uint32_t j = kCgVpt; j < rows; j++) {
+            const uint32_t i = (c * rows + j) * kBlock + tid;
+            if (i < n4) acc += apply_r4(load_r4(r, q, i), alpha, r, i);
+        }
+        if (c == 0) acc = tail_r(n, alpha, r, q, 4 * n4 + tid, kBlock, acc);
     } else {
-        acc = tail_r(n, alpha, r, q, i, nchunk * kBlock, acc);
+        acc = tail_r(n, alpha, r, q, c * kBlock + tid, nchunk * kBlock, acc);
     }
-    const double s = block_sum(acc, sh);
-    if (threadIdx.x == 0) rr_part[c] = s;
+    return acc;
 }
-template <bool V4>
-__device__ inline void dx_chunk(uint32_t n, uint32_t c, uint32_t nchunk, float alpha, float beta, bool upd_d,
-                                float* x, float* d, const float* r, const uint8_t* mask, Dx4 v, bool loaded) {
-    const uint32_t i = c * kBlock + threadIdx.x;
+template <bool V4, bool PRE>
+__device__ inline void dx_chunk(uint32_t n, uint32_t c, uint32_t nchunk, uint32_t rows, float alpha, float beta,
+                                bool upd_d, float* x, float* d, const float* r, const uint8_t* mask,
+                                const Dx4Rows& v) {
+    const uint32_t tid = threadIdx.x;
     if (V4) {
-        if (i < n / 4) {
-            if (!loaded) v = load_dx4(d, r, x, i);
-            apply_dx4(v, alpha, beta, upd_d, x, d, mask, i);
+        const uint32_t n4 = n / 4;
+#pragma unroll
+        for (uint32_t j = 0; j < kCgVpt; j++) {
+            const uint32_t i = (c * rows + j) * kBlock + tid;
+            if (i < n4) {
+                Dx4 w;
+                if (PRE) w = v.v[j];
+                else w = load_dx4(d, r, x, i);
+                apply_dx4(w, alpha, beta, upd_d, x, d, mask, i);
+            }
         }
-        tail_dx(n, alpha, beta, upd_d, x, d, r, mask, 4 * (n / 4) + i, nchunk * kBlock);
+        for (uint32_t j = kCgVpt; j < rows; j++) {
+            const uint32_t i = (c * rows + j) * kBlock + tid;
+            if (i < n4) apply_dx4(load_dx4(d, r, x, i), alpha, beta, upd_d, x, d, mask, i);
+        }
+        if (c == 0) tail_dx(n, alpha, beta, upd_d, x, d, r, mask, 4 * n4 + tid, kBlock);
     } else {
-        tail_dx(n, alpha, beta, upd_d, x, d, r, mask, i, nchunk * kBlock);
+        tail_dx(n, alpha, beta, upd_d, x, d, r, mask, c * kBlock + tid, nchunk * kBlock);
     }
 }
 
 // alpha = delta_k / d.q (break down if |d.q| < abs^2) ; r -= alpha q ;
-// per-chunk partials of r.r. Workgroup g walks chunks g, g + grid, ...; the
-// operands of its first chunk are loaded before the sum of the d.q partials,
-// whose latency they then overlap.
+// per-chunk partials of r.r. Workgroup g walks chunks g, g + grid, ...
 template <bool V4>
 __global__ __launch_bounds__(kBlock) void k_cg_step_xr(uint32_t n, uint32_t k, CgScalars* st, float* __restrict__ r,
                                                        const float* __restrict__ q, const double* dq_part, int ndq,
-                                                       double* rr_part, uint32_t nchunk) {
+                                                       double* rr_part, uint32_t nchunk, uint32_t rows) {
     __shared__ double sh[kBlock / 64];
     if (st->done[k]) return;
     const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
-    const uint32_t i0 = blockIdx.x * kBlock + threadIdx.x;
-    const bool first = V4 && blockIdx.x < nchunk && i0 < n / 4;
-    const R4 v = first ? load_r4(r, q, i0) : R4{};
+    const uint32_t c0 = blockIdx.x;
+    R4Rows v;
+#pragma unroll
+    for (uint32_t j = 0; j < kCgVpt; j++) {
+        const uint32_t i = (c0 * rows + j) * kBlock + threadIdx.x;
+        v.v[j] = R4{};
+        if (V4 && blockIdx.x < nchunk && i < n / 4) v.v[j] = load_r4(r, q, i);
+    }
     const double dq = reduce_partials_wide(dq_part, ndq, sh);
     if (fabs(dq) < st->abs_sq) {
         if (lead) {
@@ -472,8 +513,14 @@ __global__ __launch_bounds__(kBlock) void k_cg_step_xr(uint32_t n, uint32_t k, C
         st->dq[k] = dq;
         st->alpha[k] = alpha;
     }
-    for (uint32_t c = blockIdx.x; c < nchunk; c += gridDim.x)
-        r_chunk<V4>(n, c, nchunk, alpha, r, q, rr_part, v, first && c == blockIdx.x, sh);
+    if (c0 < nchunk) {
+        const double s = block_sum(r_chunk<V4, true>(n, c0, nchunk, rows, alpha, r, q, v), sh);
+        if (threadIdx.x == 0) rr_part[c0] = s;
+    }
+    for (uint32_t c = c0 + gridDim.x; c < nchunk; c += gridDim.x) {
+        const double s = block_sum(r_chunk<V4, false>(n, c, nchunk, rows, alpha, r, q, v), sh);
+        if (threadIdx.x == 0) rr_part[c] = s;
+    }
 }
 
 // delta_{k+1} = r.r ; beta ; stop test ; x += alpha_k d ; d = S(beta d + r)
@@ -482,7 +529,7 @@ template <bool V4>
 __global__ __launch_bounds__(kBlock) void k_cg_step_d(uint32_t n, uint32_t k, CgScalars* st, float* __restrict__ x,
                                                       float* __restrict__ d, const float* __restrict__ r,
                                                       const uint8_t* __restrict__ mask, const double* rr_part,
-                                                      uint32_t nchunk, int32_t* host_done) {
+                                                      uint32_t nchunk, uint32_t rows, int32_t* host_done) {
     __shared__ double sh[kBlock / 64];
     const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
     if (st->done[k]) {
@@ -497,9 +544,14 @@ __global__ __launch_bounds__(kBlock) void k_cg_step_d(uint32_t n, uint32_t k, Cg
         if (lead && host_done) host_done[k + 1] = 2;
         return;
     }
-    const uint32_t i0 = blockIdx.x * kBlock + threadIdx.x;
-    const bool first = V4 && blockIdx.x < nchunk && i0 < n / 4;
-    const Dx4 v = first ? load_dx4(d, r, x, i0) : Dx4{};
+    const uint32_t c0 = blockIdx.x;
+    Dx4Rows v;
+#pragma unroll
+    for (uint32_t j = 0; j < kCgVpt; j++) {
+        const uint32_t i = (c0 * rows + j) * kBlock + threadIdx.x;
+        v.v[j] = Dx4{};
+        if (V4 && blockIdx.x < nchunk && i < n / 4) v.v[j] = load_dx4(d, r, x, i);
+    }
     const double dn = reduce_partials_wide(rr_part, (int)nchunk, sh);
     const uint32_t k1 = k + 1;
     int reason = 0;
@@ -515,8 +567,9 @@ __global__ __launch_bounds__(kBlock) void k_cg_step_d(uint32_t n, uint32_t k, Cg
         st->done[k1] = reason != 0;
         if (host_done) host_done[k1] = reason != 0;
     }
-    for (uint32_t c = blockIdx.x; c < nchunk; c += gridDim.x)
-        dx_chunk<V4>(n, c, nchunk, alpha, beta, reason == 0, x, d, r, mask, v, first && c == blockIdx.x);
+    if (c0 < nchunk) dx_chunk<V4, true>(n, c0, nchunk, rows, alpha, beta, reason == 0, x, d, r, mask, v);
+    for (uint32_t c = c0 + gridDim.x; c < nchunk; c += gridDim.x)
+        dx_chunk<V4, false>(n, c, nchunk, rows, alpha, beta, reason == 0, x, d, r, mask, v);
 }
 
 __global__ __launch_bounds__(kBlock) void k_integrate(uint32_t n, float* __restrict__ x, float* __restrict__ v,
@@ -596,6 +649,7 @@ int launch_forces_skip(const TetDevice& t, int mode, const float* in, const floa
     a.slot_ptr = t.slot_ptr; a.lv = t.lv; a.rk = t.rk; a.R = R; a.dinv = t.dinv; a.vol = t.vol;
     a.ref = t.ref;
     a.ref_tile = t.ref_tile;
+    a.uniform_te = t.uniform_te;
     a.in = in; a.partials = t.partials; a.m = m; a.skip = skip;
     const uint32_t grid = t.tile_grid;
     const bool fe = mode == kModeFe;
@@ -674,6 +728,7 @@ int launch_damping(const TetDevice& t, const float* v, const float* R, Material 
     a.slot_ptr = t.slot_ptr; a.lv = t.lv; a.rk = t.rk; a.R = R; a.dinv = t.dinv; a.vol = t.vol;
     a.ref = t.ref;
     a.ref_tile = t.ref_tile;
+    a.uniform_te = t.uniform_te;
     a.in = v; a.partials = t.partials; a.m = m; a.skip = nullptr;
     const uint32_t g1 = 8 * ((t.num_tiles + 7) / 8);
     switch (t.tile_threads) {
@@ -689,6 +744,18 @@ int launch_forces(const TetDevice& t, int mode, const float* in, const float* R,
     return launch_forces_skip(t, mode, in, R, m, nullptr, s);
 }
 
+// Nodes per thread of the matvec gather's chunks over N nodes (kGatherNpt, or
+// more where that would make more than kMaxPartials chunks), and the chunk
+// count: the number of d.q partials it writes.
+static uint32_t matvec_npt(uint32_t N) {
+    const uint32_t cap = (uint32_t)kMaxPartials * kBlock;
+    return std::max(kGatherNpt, (N + cap - 1) / cap);
+}
+int matvec_chunks(uint32_t N) {
+    const uint32_t per = matvec_npt(N) * kBlock;
+    return (int)std::max(1u, (N + per - 1) / per);
+}
+
 int launch_gather_skip(const TetDevice& t, int mode, const NodeTerms& nt, int nblk, const int32_t* skip,
                        hipStream_t s) {
     GatherArgs a;
@@ -697,7 +764,7 @@ int launch_gather_skip(const TetDevice& t, int mode, const NodeTerms& nt, int nb
     if (mode == kGatherAccumulate)
         hipLaunchKernelGGL(k_gather<kGatherAccumulate>, dim3(nblk), dim3(kBlock), 0, s, a);
     else if (mode == kGatherMatvec)
-        hipLaunchKernelGGL(k_gather<kGatherMatvec>, dim3(nblk), dim3(kBlock), 0, s, a);
+        hipLaunchKernelGGL(k_gather_matvec, dim3(matvec_chunks(t.N)), dim3(kBlock), 0, s, a, matvec_npt(t.N));
     else if (mode == kGatherRhsDx)
         hipLaunchKernelGGL(k_gather<kGatherRhsDx>, dim3(nblk), dim3(kBlock), 0, s, a);
     else
@@ -719,6 +786,7 @@ int launch_energy(const TetDevice& t, const float* x, const float* R, Material m
         a.slot_ptr = t.slot_ptr; a.lv = t.lv; a.rk = t.rk; a.R = R; a.dinv = t.dinv; a.vol = t.vol;
         a.ref = t.ref;
         a.ref_tile = t.ref_tile;
+        a.uniform_te = t.uniform_te;
         a.in = x; a.partials = t.partials; a.m = m; a.skip = nullptr;
         const uint32_t g1 = 8 * ((t.num_tiles + 7) / 8);  // one workgroup per tile
         switch (t.tile_threads) {
@@ -763,6 +831,19 @@ int set_cg_workgroups(uint32_t n) {
 }
 uint32_t get_cg_workgroups() { return g_cg_workgroups ? g_cg_workgroups : kCgWorkgroups; }
 
+// Float4 rows per thread of one CG step chunk and the chunk count over n
+// floats: kCgVpt rows, or more where that would make more than kMaxPartials
+// chunks (the r.r partial buffer's size).
+static uint32_t cg_rows(uint32_t n) {
+    const uint32_t n4 = (n + 3) / 4;
+    const uint32_t cap = (uint32_t)kMaxPartials * kBlock;  // float4 rows of kBlock over all chunks
+    return std::max(kCgVpt, (n4 + cap - 1) / cap);
+}
+static uint32_t cg_chunks(uint32_t n) {
+    const uint32_t per = cg_rows(n) * kBlock;
+    return std::max(1u, ((n + 3) / 4 + per - 1) / per);
+}
+
 static dim3 cg_step_grid(uint32_t nchunk) {
     const uint32_t g = g_cg_workgroups ? g_cg_workgroups : kCgWorkgroups;
     return dim3(std::max(1u, std::min(nchunk, g)));
@@ -770,27 +851,27 @@ static dim3 cg_step_grid(uint32_t nchunk) {
 
 int launch_cg_step_xr(uint32_t n, uint32_t k, CgScalars* st, float* r, const float* q, const double* dq_part,
                       int ndq, double* rr_part, hipStream_t s) {
-    const uint32_t nchunk = (uint32_t)vec_blocks(n);  // the d kernel sums this many r.r partials
+    const uint32_t nchunk = cg_chunks(n);  // the d kernel sums this many r.r partials
     const dim3 grid = cg_step_grid(nchunk);
     if (aligned16(r) && aligned16(q))
         hipLaunchKernelGGL(k_cg_step_xr<true>, grid, dim3(kBlock), 0, s, n, k, st, r, q, dq_part, ndq, rr_part,
-                           nchunk);
+                           nchunk, cg_rows(n));
     else
         hipLaunchKernelGGL(k_cg_step_xr<false>, grid, dim3(kBlock), 0, s, n, k, st, r, q, dq_part, ndq, rr_part,
-                           nchunk);
+                           nchunk, cg_rows(n));
     return check_launch();
 }
 
 int launch_cg_step_d(uint32_t n, uint32_t k, CgScalars* st, float* x, float* d, const float* r,
                      const uint8_t* mask, const double* rr_part, hipStream_t s, int32_t* host_done) {
-    const uint32_t nchunk = (uint32_t)vec_blocks(n);
+    const uint32_t nchunk = cg_chunks(n);
     const dim3 grid = cg_step_grid(nchunk);
     if (aligned16(x) && aligned16(d) && aligned16(r))
         hipLaunchKernelGGL(k_cg_step_d<true>, grid, dim3(kBlock), 0, s, n, k, st, x, d, r, mask, rr_part, nchunk,
-                           host_done);
+                           cg_rows(n), host_done);
     else
         hipLaunchKernelGGL(k_cg_step_d<false>, grid, dim3(kBlock), 0, s, n, k, st, x, d, r, mask, rr_part, nchunk,
-                           host_done);
+                           cg_rows(n), host_done);
     return check_launch();
 }
 

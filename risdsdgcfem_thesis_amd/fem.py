@@ -225,8 +225,8 @@ def set_tile_persistent(mode):
     check(lib.cf_set_tile_persistent(int(mode)), "cf_set_tile_persistent")
 
 
-def set_tile_alternate(on):
-    check(lib.cf_set_tile_alternate(int(bool(on))), "cf_set_tile_alternate")
+def set_tile_alternate(level):
+    check(lib.cf_set_tile_alternate(int(level)), "cf_set_tile_alternate")
 
 
 def get_tile_elems():
