@@ -374,9 +374,6 @@ static int cg_begin(cf_solid* s, float fM, float fK, CgRun* run) {
     const TetDevice& t = s->set->dev;
     hipStream_t st = s->stream;
     CgBuffers& b = s->cg;
-    const uint32_t n = 3 * s->N;
-    const int nvec = node_blocks(n);
-    const int nnode = node_blocks(s->N);
     run->max_it = std::min<uint32_t>(s->p.cg_max_iter, kMaxCgIter);
     run->k = 0;
     run->done = false;
@@ -385,12 +382,14 @@ static int cg_begin(cf_solid* s, float fM, float fK, CgRun* run) {
     nt.knc_mask = s->num_knc ? s->knc_mask : nullptr;
     nt.mass = s->node_mass;
     nt.fM = fM; nt.fK = fK;
-    nt.partials = b.part_a;
-    nt.d = s->y; nt.out = b.q;
+    // r = S(b - A y), d = r and the r.r partials in the gather of A y
+    NodeTerms ni = nt;
+    ni.d = s->y; ni.b = s->bvec; ni.r_out = b.r; ni.d_out = b.d; ni.partials = b.part_b;
     CF_TRY(launch_forces(t, kModeDfx, s->y, s->cache->R, cache_mat(s->cache), st));
-    CF_TRY(launch_gather(t, kGatherMatvec, nt, nnode, st));
-    CF_TRY(launch_cg_init(n, b.q, s->bvec, b.r, b.d, nt.knc_mask, b.part_b, nvec, st));
-    CF_TRY(launch_cg_start(b.st, b.part_b, nvec, s->p.cg_rel_epsilon, s->p.cg_abs_epsilon, run->max_it, st));
+    CF_TRY(launch_gather(t, kGatherCgInit, ni, 0, st));
+    CF_TRY(launch_cg_start(b.st, b.part_b, matvec_chunks(s->N), s->p.cg_rel_epsilon, s->p.cg_abs_epsilon,
+                           run->max_it, st));
+    nt.partials = b.part_a;
     nt.d = b.d; nt.out = b.q;
     return CF_OK;
 }

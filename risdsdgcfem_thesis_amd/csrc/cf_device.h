@@ -1405,8 +1405,26 @@ __device__ inline void gather_block(uint32_t N, const uint32_t* __restrict__ nod
 // (c npt + j) kBlock + thread for j < npt. For the first kGatherNpt nodes of a
 // thread the node terms and the first two partials are loaded before any sum,
 // so their latencies overlap; each node still adds its partials in ascending
-// order. Returns the thread's d.q terms, summed in j order.
+// order. Returns the thread's d.q terms, summed in j order. INIT: the CG start
+// instead, r = S(b - q) and d = r written for each node (q is not stored),
+// returning the r.r terms.
 constexpr uint32_t kGatherNpt = 2;
+__device__ inline double cg_start_node(uint32_t n, const float q[3], const NodeTerms& t) {
+#pragma clang fp contract(off)
+    const size_t i = 3 * (size_t)n;
+    const bool knc = t.knc_mask && t.knc_mask[n];
+    double acc = 0.0;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        float ri = -1.0f * q[k] + t.b[i + k];  // muleq_addeq(r, -1, b) (ILSS_CG.h:55-63)
+        if (knc) ri = 0.f;
+        t.r_out[i + k] = ri;
+        t.d_out[i + k] = ri;
+        acc += (double)ri * ri;
+    }
+    return acc;
+}
+template <bool INIT>
 __device__ inline double gather_matvec_chunk(uint32_t N, uint32_t c, uint32_t npt,
                                              const uint32_t* __restrict__ node_part_ptr,
                                              const float* __restrict__ partials, const NodeTerms& t) {
@@ -1462,13 +1480,36 @@ __device__ inline double gather_matvec_chunk(uint32_t N, uint32_t c, uint32_t np
         float q1 = -t.fK * fy + fac * dd[j][1];
         float q2 = -t.fK * fz + fac * dd[j][2];
         if (kn[j]) { q0 = 0.f; q1 = 0.f; q2 = 0.f; }
+        if (INIT) {
+            const float q[3] = {q0, q1, q2};
+            acc += cg_start_node(nn[j], q, t);
+            continue;
+        }
         const size_t i = 3 * (size_t)nn[j];
         out[i] = q0; out[i + 1] = q1; out[i + 2] = q2;
         acc += (double)dd[j][0] * q0 + (double)dd[j][1] * q1 + (double)dd[j][2] * q2;
     }
     for (uint32_t j = kGatherNpt; j < npt; j++) {
         const uint32_t n = (c * npt + j) * kBlock + tid;
-        if (n < N) acc += gather_node<kGatherMatvec>(n, node_part_ptr, partials, t);
+        if (n >= N) continue;
+        if (INIT) {
+            // q of node n alone, then the CG start terms
+            float q[3];
+            const float* pd = dv + 3 * (size_t)n;
+            float fx = 0.f, fy = 0.f, fz = 0.f;
+            for (uint32_t p = node_part_ptr[n]; p < node_part_ptr[n + 1]; p++) {
+                const float* pp = partials + 3 * (size_t)p;
+                fx += pp[0]; fy += pp[1]; fz += pp[2];
+            }
+            const float fac = (t.mass_array ? t.mass_array[n] : t.mass) * t.fM;
+            q[0] = -t.fK * fx + fac * pd[0];
+            q[1] = -t.fK * fy + fac * pd[1];
+            q[2] = -t.fK * fz + fac * pd[2];
+            if (t.knc_mask && t.knc_mask[n]) q[0] = q[1] = q[2] = 0.f;
+            acc += cg_start_node(n, q, t);
+        } else {
+            acc += gather_node<kGatherMatvec>(n, node_part_ptr, partials, t);
+        }
     }
     return acc;
 }

@@ -164,7 +164,9 @@ int cg_solve_on(const CgWork& w, uint32_t n, hipStream_t st, cf_av_fn av, cf_con
 
 // ---------------------------------------------------------------- launchers
 enum ForceMode { kModeDfx = 0, kModeFe = 1 };
-enum GatherMode { kGatherAccumulate = 0, kGatherMatvec = 1, kGatherRhs = 2, kGatherRhsDx = 3 };
+// kGatherCgInit: the matvec q = S A d followed by the CG start r = S(b - q),
+// d = r (ILSS_CG.h:55-63) with per-chunk r.r partials; q is not stored.
+enum GatherMode { kGatherAccumulate = 0, kGatherMatvec = 1, kGatherRhs = 2, kGatherRhsDx = 3, kGatherCgInit = 4 };
 
 struct NodeTerms {  // per-node terms fused into the node gather
     const float* d = nullptr;       // matvec: direction d (3N)
@@ -181,7 +183,10 @@ struct NodeTerms {  // per-node terms fused into the node gather
     float dt = 0.f, inv_dt = 0.f;
     float g[3] = {0, 0, 0};
     int warm_start = 1;
-    double* partials = nullptr;     // matvec: per-block d.q ; FIE-dx rhs: per-block b.b
+    double* partials = nullptr;     // matvec: per-block d.q ; FIE-dx rhs: per-block b.b ; CG start: r.r
+    const float* b = nullptr;       // CG start: right-hand side
+    float* r_out = nullptr;         // CG start: residual r
+    float* d_out = nullptr;         // CG start: direction d = r
 };
 
 int launch_precompute(const TetDevice& t, hipStream_t s);

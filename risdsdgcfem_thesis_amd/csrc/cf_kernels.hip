@@ -213,10 +213,12 @@ __global__ __launch_bounds__(kBlock) void k_gather(GatherArgs a) {
 
 // The matvec gather in chunks of npt kBlock nodes, one chunk per workgroup;
 // chunk c's d.q partial in t.partials[c].
+template <bool INIT>
 __global__ __launch_bounds__(kBlock) void k_gather_matvec(GatherArgs a, uint32_t npt) {
     __shared__ double sh[kBlock / 64];
     if (a.skip && *a.skip) return;
-    const double s = block_sum(gather_matvec_chunk(a.N, blockIdx.x, npt, a.node_part_ptr, a.partials, a.t), sh);
+    const double s =
+        block_sum(gather_matvec_chunk<INIT>(a.N, blockIdx.x, npt, a.node_part_ptr, a.partials, a.t), sh);
     if (threadIdx.x == 0) a.t.partials[blockIdx.x] = s;
 }
 
@@ -764,7 +766,9 @@ int launch_gather_skip(const TetDevice& t, int mode, const NodeTerms& nt, int nb
     if (mode == kGatherAccumulate)
         hipLaunchKernelGGL(k_gather<kGatherAccumulate>, dim3(nblk), dim3(kBlock), 0, s, a);
     else if (mode == kGatherMatvec)
-        hipLaunchKernelGGL(k_gather_matvec, dim3(matvec_chunks(t.N)), dim3(kBlock), 0, s, a, matvec_npt(t.N));
+        hipLaunchKernelGGL(k_gather_matvec<false>, dim3(matvec_chunks(t.N)), dim3(kBlock), 0, s, a, matvec_npt(t.N));
+    else if (mode == kGatherCgInit)
+        hipLaunchKernelGGL(k_gather_matvec<true>, dim3(matvec_chunks(t.N)), dim3(kBlock), 0, s, a, matvec_npt(t.N));
     else if (mode == kGatherRhsDx)
         hipLaunchKernelGGL(k_gather<kGatherRhsDx>, dim3(nblk), dim3(kBlock), 0, s, a);
     else
