@@ -67,7 +67,6 @@ __global__ __launch_bounds__(kBlock, 8) void k_begin_eval(uint32_t T, const uint
 // so tiles that share halo nodes share an L2.
 template <int MODE, int TB, bool PIPE>
 __global__ __launch_bounds__(TB) void k_tile_forces(TileArgs a) {
-    if (a.skip && *a.skip) return;
     extern __shared__ float smem_f[];
     const TileLds<TB> L(smem_f);
     const Slots& s_v = L.v;  // tile node vectors
@@ -84,7 +83,13 @@ __global__ __launch_bounds__(TB) void k_tile_forces(TileArgs a) {
     if (t >= hi) return;
     if (!PIPE && a.reverse) t = hi - 1 - j;
 
+    // The skip flag and the tile's descriptor are loaded together: the empty
+    // asm consumes the descriptor before the branch, so the compiler cannot
+    // sink its loads behind the skip test (one scalar round trip, not two).
     TileDesc dt = tile_desc(a, t);
+    const int32_t sk = a.skip ? *a.skip : 0;
+    asm volatile("" ::"s"(dt.n0), "s"(dt.nn), "s"(dt.e0), "s"(dt.ne));
+    if (sk) return;
     TileRegs g;
     load_tile_regs_cold(a, dt, tid, g);
     uint32_t t1 = t + gx;
