@@ -229,21 +229,22 @@ struct MeshLayout {
 // Device layout of a simulation's state (one arena block).
 struct SimLayout {
     size_t vec[8], R, detF, part, mask, pa, pb, pe, ec, es, em, x0, doc, total;
-    uint32_t nb_elem, nb_node, nb_init, nb_vec;
-    SimLayout(uint32_t N, uint32_t T, uint32_t P) {
+    uint32_t nb_elem, nb_node, nb_init, nb_vec, nb_mv;
+    SimLayout(uint32_t N, uint32_t T, uint32_t P, uint32_t num_tiles) {
         nb_elem = uint32_t(node_blocks(T));
         nb_node = uint32_t(node_blocks(N));
         nb_init = uint32_t(node_blocks(3 * N));
-        nb_vec = uint32_t(vec_blocks(3 * N));
+        nb_vec = cg_chunks(3 * N);
+        nb_mv = matvec_chunks_of(N);
         Layout l;
         for (int i = 0; i < 8; i++) vec[i] = l.add<float>(3 * (size_t)N);
         R = l.add<float>(9 * (size_t)T);
         detF = l.add<float>(T);
         part = l.add<float>(3 * (size_t)P);
         mask = l.add<uint8_t>(N);
-        pa = l.add<double>(nb_node);
-        pb = l.add<double>(std::max(nb_init, nb_vec));
-        pe = l.add<double>(nb_elem);
+        pa = l.add<double>(nb_mv);
+        pb = l.add<double>(std::max(nb_mv, nb_vec));
+        pe = l.add<double>(std::max(std::max(num_tiles, nb_elem), 1u));
         ec = l.add<uint32_t>(2 * (size_t)nb_elem);
         es = l.add<double>(nb_elem);
         em = l.add<float>(nb_elem);
@@ -381,7 +382,7 @@ int ensure_device(void** p, size_t* cap, size_t need, size_t elem) {
 // (blockIdx % 8) take a contiguous range of tiles. cg_only: only the tables
 // of the CG-loop kernels (tile, node, vec) and the slot list.
 struct TableLists {
-    std::vector<BlockRef> tile, elem, node, init, vec;
+    std::vector<BlockRef> tile, elem, node, init, vec, mv;
     std::vector<uint32_t> slots;
     uint32_t max_tn = 1, max_te = 1, max_it = 0;
     uint64_t tets = 0, nodes = 0;
@@ -397,6 +398,7 @@ void table_lists(cf_batch* b, const std::vector<uint32_t>& ids, bool cg_only, Ta
         for (uint32_t i = 0; i < m.num_tiles; i++) tile.push_back({sl, i});
         for (uint32_t i = 0; i < s.d.nb_node; i++) L.node.push_back({sl, i});
         for (uint32_t i = 0; i < s.d.nb_vec; i++) L.vec.push_back({sl, i});
+        for (uint32_t i = 0; i < s.d.nb_mv; i++) L.mv.push_back({sl, i});
         if (!cg_only) {
             for (uint32_t i = 0; i < s.d.nb_elem; i++) L.elem.push_back({sl, i});
             for (uint32_t i = 0; i < s.d.nb_init; i++) L.init.push_back({sl, i});
@@ -424,19 +426,21 @@ void table_views(const TableLists& L, BlockRef* base, uint32_t* active, BatchTab
     t.elem = base + o; t.n_elem = uint32_t(L.elem.size()); o += L.elem.size();
     t.node = base + o; t.n_node = uint32_t(L.node.size()); o += L.node.size();
     t.init = base + o; t.n_init = uint32_t(L.init.size()); o += L.init.size();
-    t.vec = base + o; t.n_vec = uint32_t(L.vec.size());
+    t.vec = base + o; t.n_vec = uint32_t(L.vec.size()); o += L.vec.size();
+    t.mv = base + o; t.n_mv = uint32_t(L.mv.size());
     t.active = active;
     t.n_active = uint32_t(L.slots.size());
 }
 
 void table_concat(const TableLists& L, std::vector<BlockRef>& all) {
     all.clear();
-    all.reserve(L.tile.size() + L.elem.size() + L.node.size() + L.init.size() + L.vec.size());
+    all.reserve(L.tile.size() + L.elem.size() + L.node.size() + L.init.size() + L.vec.size() + L.mv.size());
     all.insert(all.end(), L.tile.begin(), L.tile.end());
     all.insert(all.end(), L.elem.begin(), L.elem.end());
     all.insert(all.end(), L.node.begin(), L.node.end());
     all.insert(all.end(), L.init.begin(), L.init.end());
     all.insert(all.end(), L.vec.begin(), L.vec.end());
+    all.insert(all.end(), L.mv.begin(), L.mv.end());
 }
 
 int rebuild_tables(cf_batch* b) {
@@ -524,8 +528,7 @@ int enqueue_prologue(cf_batch* b, bool arm_all) {
     CF_TRY(launch_batch_forces(t, kModeFe, kSelY, s));    // f_e(x)
     CF_TRY(launch_batch_gather_rhs(t, s));                // b, y0, KNC (:1209-1295)
     CF_TRY(launch_batch_forces(t, kModeDfx, kSelY, s));   // r = S(b - A y0)
-    CF_TRY(launch_batch_gather_matvec(t, kSelY, s));
-    CF_TRY(launch_batch_cg_init(t, s));
+    CF_TRY(launch_batch_gather_matvec(t, kSelY, s));  // fused: r = S(b - A y0), d = r, r.r partials
     return launch_batch_cg_start(t, s);
 }
 
@@ -752,7 +755,7 @@ int cf_batch_sim_bytes(const cf_batch* b, uint32_t mesh_id, uint64_t* bytes) {
     if (!b || !bytes) return CF_ERR_INVALID_ARG;
     auto it = b->meshes.find(mesh_id);
     if (it == b->meshes.end()) { set_error("no mesh %u", mesh_id); return CF_ERR_INVALID_ARG; }
-    *bytes = SimLayout(it->second.N, it->second.T, it->second.num_parts).total;
+    *bytes = SimLayout(it->second.N, it->second.T, it->second.num_parts, it->second.num_tiles).total;
     return CF_OK;
 }
 
@@ -780,7 +783,7 @@ int cf_batch_add_sim(cf_batch* b, uint32_t mesh_id, const cf_solid_params* p, ui
     }
     for (uint32_t i = 0; i < n_knc; i++)
         if (knc[i] >= m.N) { set_error("KNC node %u out of range", knc[i]); return CF_ERR_INVALID_ARG; }
-    const SimLayout L(m.N, m.T, m.num_parts);
+    const SimLayout L(m.N, m.T, m.num_parts, m.num_tiles);
     if (b->budget && b->used + L.total > b->budget) {
         set_error("simulation of %zu bytes exceeds the HBM budget (%llu of %llu in use)", L.total,
                   (unsigned long long)b->used, (unsigned long long)b->budget);
@@ -804,6 +807,7 @@ int cf_batch_add_sim(cf_batch* b, uint32_t mesh_id, const cf_solid_params* p, ui
     const TetDevice& t = m.dev;
     d.N = m.N; d.T = m.T; d.num_tiles = m.num_tiles;
     d.nb_elem = L.nb_elem; d.nb_node = L.nb_node; d.nb_init = L.nb_init; d.nb_vec = L.nb_vec;
+    d.nb_mv = L.nb_mv; d.npt_mv = matvec_npt(m.N); d.rows_cg = cg_rows(3 * m.N);
     d.conn = t.conn; d.tile_elem_ptr = t.tile_elem_ptr; d.tile_node_ptr = t.tile_node_ptr;
     d.tile_nodes = t.tile_nodes; d.slot_ptr = t.slot_ptr; d.node_part_ptr = t.node_part_ptr;
     d.part_dst = t.part_dst; d.lv = t.lv; d.rk = t.rk; d.ref = t.ref; d.dinv = t.dinv; d.vol = t.vol;

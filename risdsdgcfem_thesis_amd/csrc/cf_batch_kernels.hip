@@ -112,34 +112,39 @@ __global__ __launch_bounds__(kBlock) void kb_gather_rhs(BatchTables t) {
     gather_block<kGatherRhs>(s.N, s.node_part_ptr, s.partials, nt, br.blk, s.nb_node, sh);
 }
 
-// q = S(fM M d - fK K d) and per-block d.q (LS_A_times_d :1056-1079)
+// Over the leaf's matvec chunks (one chunk per block): q = S(fM M d - fK K d)
+// and per-chunk d.q (LS_A_times_d :1056-1079) for iterating simulations
+// (kSelD) or, on y, for every simulation (kSelYAll); for starting simulations
+// (kSelY) the CG start r = S(b - A y), d = r with per-chunk r.r instead
+// (ILSS_CG.h:55-63), as the leaf's fused start gather computes it.
 __global__ __launch_bounds__(kBlock) void kb_gather_matvec(BatchTables t, int sel) {
     __shared__ double sh[kBlock / 64];
-    const BlockRef br = t.node[blockIdx.x];
+    const BlockRef br = t.mv[blockIdx.x];
     if (br.slot == kIdleSlot) return;
     if (sel != kSelYAll && t.cgs[br.slot].phase != (sel == kSelD ? kPhaseIter : kPhaseStart)) return;
     const SimDev& s = t.sims[br.slot];
     NodeTerms nt = node_terms(s);
-    nt.d = (sel == kSelD) ? s.d : s.y;
-    nt.out = s.q;
-    nt.partials = s.part_a;
-    gather_block<kGatherMatvec>(s.N, s.node_part_ptr, s.partials, nt, br.blk, s.nb_node, sh);
+    double part;
+    if (sel == kSelY) {
+        nt.d = s.y;
+        nt.b = s.b; nt.r_out = s.r; nt.d_out = s.d;
+        part = block_sum(gather_matvec_chunk<true>(s.N, br.blk, s.npt_mv, s.node_part_ptr, s.partials, nt), sh);
+        if (threadIdx.x == 0) s.part_b[br.blk] = part;
+    } else {
+        nt.d = (sel == kSelD) ? s.d : s.y;
+        nt.out = s.q;
+        part = block_sum(gather_matvec_chunk<false>(s.N, br.blk, s.npt_mv, s.node_part_ptr, s.partials, nt), sh);
+        if (threadIdx.x == 0) s.part_a[br.blk] = part;
+    }
 }
 
 // ------------------------------------------------------------ CG (ns/ILSS_CG.h:38-151)
-__global__ __launch_bounds__(kBlock) void kb_cg_init(BatchTables t) {
-    __shared__ double sh[kBlock / 64];
-    const BlockRef br = t.init[blockIdx.x];
-    if (br.slot == kIdleSlot || t.cgs[br.slot].phase != kPhaseStart) return;
-    const SimDev& s = t.sims[br.slot];
-    cg_init_block(3 * s.N, s.q, s.b, s.r, s.d, s.knc_mask, s.part_b, br.blk, s.nb_init, sh);
-}
 
 __global__ __launch_bounds__(kBlock) void kb_cg_start(BatchTables t) {
     const uint32_t slot = wave_slot(t);
     if (slot == kIdleSlot || t.cgs[slot].phase != kPhaseStart) return;
     const SimDev& s = t.sims[slot];
-    const double d0 = wave_reduce_partials(s.part_b, s.nb_init);
+    const double d0 = wave_reduce_partials(s.part_b, s.nb_mv);
     if ((threadIdx.x & 63) == 0) {
         SimCg& c = t.cgs[slot];
         const float abs_sq = s.abs_eps * s.abs_eps;
@@ -172,7 +177,7 @@ __global__ __launch_bounds__(kBlock) void kb_cg_alpha(BatchTables t) {
     SimCg& c = t.cgs[slot];
     if (c.phase != kPhaseIter) return;
     const SimDev& s = t.sims[slot];
-    const double dq = wave_reduce_partials(s.part_a, s.nb_node);
+    const double dq = wave_reduce_partials(s.part_a, s.nb_mv);
     if ((threadIdx.x & 63) == 0) {
         c.dq = dq;
         if (fabs(dq) < c.abs_sq) {
@@ -185,6 +190,7 @@ __global__ __launch_bounds__(kBlock) void kb_cg_alpha(BatchTables t) {
     }
 }
 
+// x += alpha d ; r -= alpha q over the leaf's CG chunks, with per-chunk r.r
 __global__ __launch_bounds__(kBlock) void kb_update_xr(BatchTables t) {
     __shared__ double sh[kBlock / 64];
     const BlockRef br = t.vec[blockIdx.x];
@@ -192,7 +198,8 @@ __global__ __launch_bounds__(kBlock) void kb_update_xr(BatchTables t) {
     const SimCg& c = t.cgs[br.slot];
     if (c.phase != kPhaseIter) return;
     const SimDev& s = t.sims[br.slot];
-    update_xr_block<true>(3 * s.N, c.alpha, s.y, s.r, s.d, s.q, s.part_b, br.blk, s.nb_vec, sh);
+    const double part = block_sum(xr_rows_chunk(3 * s.N, br.blk, s.rows_cg, c.alpha, s.y, s.r, s.d, s.q), sh);
+    if (threadIdx.x == 0) s.part_b[br.blk] = part;
 }
 
 // delta' = r.r ; beta = delta'/delta ; stop tests (ILSS_CG.h:86-88,115-133)
@@ -236,33 +243,63 @@ __global__ __launch_bounds__(kBlock) void kb_integrate(BatchTables t) {
     integrate_block(3 * s.N, s.x, s.v, s.y, s.dt, s.drag, br.blk, s.nb_init, s.x0);
 }
 
+// V() with the new positions (FixedStep :977): C_WRP per tile as the leaf's
+// tile energy computes it (part_e[tile]); the LCM materials per element block
+// (part_e[block]).
+template <int TB>
+__global__ __launch_bounds__(TB) void kb_tile_energy(BatchTables t) {
+    extern __shared__ float smem_f[];
+    __shared__ double sh[TB / 64];
+    const BlockRef br = t.tile[blockIdx.x];
+    if (br.slot == kIdleSlot || t.cgs[br.slot].phase != kPhaseEnd) return;
+    const SimDev& s = t.sims[br.slot];
+    if (!s.compute_energy || s.m.mm != CF_MM_C_WRP) return;
+    const TileArgs a = tile_args(s, s.x, t.max_tn, t.max_te);
+    const double e = tile_energy_block<TB>(a, br.blk, smem_f, sh);
+    if (threadIdx.x == 0) s.part_e[br.blk] = e;
+}
+
 __global__ __launch_bounds__(kBlock) void kb_energy(BatchTables t) {
     __shared__ double sh[kBlock / 64];
     const BlockRef br = t.elem[blockIdx.x];
     if (br.slot == kIdleSlot || t.cgs[br.slot].phase != kPhaseEnd) return;
     const SimDev& s = t.sims[br.slot];
-    if (!s.compute_energy) return;
+    if (!s.compute_energy || s.m.mm == CF_MM_C_WRP) return;
     energy_block(s.T, s.conn, s.ref, s.x, s.R, s.dinv, s.vol, s.m, s.part_e, br.blk, s.nb_elem, sh);
 }
 
-// Per-simulation step results: the block partials summed in block order
-// (the order the single-mesh path sums them on the host).
+// Per-simulation step results, one wavefront per simulation: the
+// BeginEvaluation counts and minimum over the element blocks (any order gives
+// the same values), the sum of degenerate det F in block order (as the leaf's
+// host sum; only when there are degenerate elements, else it is 0), and the
+// energy partials in the leaf's one-block order (tiles for C_WRP, element
+// blocks for the LCM materials).
 __global__ __launch_bounds__(kBlock) void kb_finish(BatchTables t) {
     const uint32_t slot = wave_slot(t);
-    if (slot == kIdleSlot || (threadIdx.x & 63) != 0) return;
+    if (slot == kIdleSlot) return;
     const SimDev& s = t.sims[slot];
     SimCg& c = t.cgs[slot];
     if (c.phase != kPhaseEnd) return;
+    const uint32_t lane = threadIdx.x & 63;
     uint32_t nd = 0, nf = 0;
-    double sd = 0.0, en = 0.0;
     float mn = 3e38f;
-    for (uint32_t i = 0; i < s.nb_elem; i++) {
+    for (uint32_t i = lane; i < s.nb_elem; i += 64) {
         nd += s.eval_count[i];
         nf += s.eval_count[s.nb_elem + i];
-        sd += s.eval_sum[i];
         mn = fminf(mn, s.eval_min[i]);
-        if (s.compute_energy) en += s.part_e[i];
     }
+    for (int o = 32; o > 0; o >>= 1) {
+        nd += __shfl_xor(nd, o);
+        nf += __shfl_xor(nf, o);
+        mn = fminf(mn, __shfl_xor(mn, o));
+    }
+    double en = 0.0;
+    if (s.compute_energy)
+        en = wave_reduce_partials(s.part_e, (int)(s.m.mm == CF_MM_C_WRP ? s.num_tiles : s.nb_elem));
+    if (lane != 0) return;
+    double sd = 0.0;
+    if (nd)
+        for (uint32_t i = 0; i < s.nb_elem; i++) sd += s.eval_sum[i];
     c.num_degenerate = nd;
     c.num_fallback = nf;
     c.sum_degenerate = sd;
@@ -354,14 +391,8 @@ int launch_batch_gather_rhs(const BatchTables& t, hipStream_t s) {
 }
 
 int launch_batch_gather_matvec(const BatchTables& t, int sel, hipStream_t s) {
-    if (!t.n_node) return CF_OK;
-    hipLaunchKernelGGL(kb_gather_matvec, dim3(t.n_node), dim3(kBlock), 0, s, t, sel);
-    return check_launch_b();
-}
-
-int launch_batch_cg_init(const BatchTables& t, hipStream_t s) {
-    if (!t.n_init) return CF_OK;
-    hipLaunchKernelGGL(kb_cg_init, dim3(t.n_init), dim3(kBlock), 0, s, t);
+    if (!t.n_mv) return CF_OK;
+    hipLaunchKernelGGL(kb_gather_matvec, dim3(t.n_mv), dim3(kBlock), 0, s, t, sel);
     return check_launch_b();
 }
 
@@ -402,6 +433,15 @@ int launch_batch_integrate(const BatchTables& t, hipStream_t s) {
 }
 
 int launch_batch_energy(const BatchTables& t, hipStream_t s) {
+    if (t.n_tile) {
+        switch (t.tile_threads) {
+            case 1024: hipLaunchKernelGGL(kb_tile_energy<1024>, dim3(t.n_tile), dim3(1024), 6 * 1024 * 4, s, t); break;
+            case 512: hipLaunchKernelGGL(kb_tile_energy<512>, dim3(t.n_tile), dim3(512), 6 * 512 * 4, s, t); break;
+            case 128: hipLaunchKernelGGL(kb_tile_energy<128>, dim3(t.n_tile), dim3(128), 6 * 128 * 4, s, t); break;
+            default: hipLaunchKernelGGL(kb_tile_energy<256>, dim3(t.n_tile), dim3(256), 6 * 256 * 4, s, t); break;
+        }
+        CF_TRY(check_launch_b());
+    }
     if (!t.n_elem) return CF_OK;
     hipLaunchKernelGGL(kb_energy, dim3(t.n_elem), dim3(kBlock), 0, s, t);
     return check_launch_b();

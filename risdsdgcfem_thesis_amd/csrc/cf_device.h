@@ -1408,7 +1408,6 @@ __device__ inline void gather_block(uint32_t N, const uint32_t* __restrict__ nod
 // order. Returns the thread's d.q terms, summed in j order. INIT: the CG start
 // instead, r = S(b - q) and d = r written for each node (q is not stored),
 // returning the r.r terms.
-constexpr uint32_t kGatherNpt = 2;
 __device__ inline double cg_start_node(uint32_t n, const float q[3], const NodeTerms& t) {
 #pragma clang fp contract(off)
     const size_t i = 3 * (size_t)n;
@@ -1538,6 +1537,48 @@ __device__ inline double wrp_energy(const float D[9], const float Dm[9], const f
     return 0.5 * (double)V * (double)sH;
 }
 
+// C_V of C_WRP over tile t (ms/fem/TetrahedronElementSet3.cpp:1456-1488): the
+// tile's positions (a.in) and rest positions staged in LDS planes (smem: 6 TB
+// floats) as for K*x, Dm^-1 and V rebuilt from them as k_precompute computes
+// them, one element per thread. Returns the tile's FP64 sum in thread 0 (each
+// wave's terms summed by wave_sum, the waves in order).
+template <int TB>
+__device__ inline double tile_energy_block(const TileArgs& a, uint32_t t, float* smem, double* sh) {
+#pragma clang fp contract(off)
+    const Slots s_x = make_slots(smem, TB), s_r = make_slots(smem + 3 * TB, TB);
+    const uint32_t tid = threadIdx.x;
+    const TileDesc dt = tile_desc(a, t);
+    TileRegs g;
+    load_tile_regs_cold(a, dt, tid, g);
+    if (tid < dt.nn) {
+        s_x.x[tid] = g.v[0]; s_x.y[tid] = g.v[1]; s_x.z[tid] = g.v[2];
+        s_r.x[tid] = g.r[0]; s_r.y[tid] = g.r[1]; s_r.z[tid] = g.r[2];
+    }
+    __syncthreads();
+    double e = 0.0;
+    if (tid < dt.ne) {
+        const uint32_t l0 = g.l[0];
+        float D[9], Dm[9], G[9];
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+            const uint32_t l = g.l[q + 1];
+            D[q] = s_x.x[l] - s_x.x[l0]; D[3 + q] = s_x.y[l] - s_x.y[l0]; D[6 + q] = s_x.z[l] - s_x.z[l0];
+            Dm[q] = s_r.x[l] - s_r.x[l0]; Dm[3 + q] = s_r.y[l] - s_r.y[l0]; Dm[6 + q] = s_r.z[l] - s_r.z[l0];
+        }
+        const float detDm = refmath::det3(Dm);
+        const float V = detDm / 6.0f;
+        refmath::inv3(Dm, detDm, G);
+        e = wrp_energy(D, Dm, g.R, G, V, a.m);
+    }
+    e = wave_sum(e);
+    if ((tid & 63) == 0) sh[tid >> 6] = e;
+    __syncthreads();
+    double sum = 0.0;
+    if (tid == 0)
+        for (int w = 0; w < TB / 64; w++) sum += sh[w];
+    return sum;
+}
+
 // C_V (ms/fem/TetrahedronElementSet3.cpp:1456-1488): 1/2 u^T Ke u = 1/2 V sigma:H
 __device__ inline void energy_block(uint32_t T, const uint32_t* __restrict__ conn, const float* __restrict__ ref,
                                     const float* __restrict__ x, const float* __restrict__ R,
@@ -1664,6 +1705,23 @@ __device__ inline void update_xr_block(uint32_t n, float alpha, float* __restric
     }
     const double s = block_sum(acc, sh);
     if (threadIdx.x == 0) rr_part[blk] = s;
+}
+
+// x += alpha d ; r -= alpha q over chunk c of the leaf CG's chunk layout
+// (rows float4s per thread: float4 (c rows + j) kBlock + thread, j ascending,
+// then the n % 4 tail in chunk 0); returns the thread's r.r terms in the order
+// the leaf's CG step kernel adds them, so the chunk partial is the same.
+__device__ inline double xr_rows_chunk(uint32_t n, uint32_t c, uint32_t rows, float alpha, float* __restrict__ x,
+                                       float* __restrict__ r, const float* __restrict__ d,
+                                       const float* __restrict__ q) {
+    const uint32_t n4 = n / 4, tid = threadIdx.x;
+    double acc = 0.0;
+    for (uint32_t j = 0; j < rows; j++) {
+        const uint32_t i = (c * rows + j) * kBlock + tid;
+        if (i < n4) acc += apply_xr4(load_xr4(x, r, d, q, i), alpha, x, r, i);
+    }
+    if (c == 0) acc = tail_xr(n, alpha, x, r, d, q, tid, kBlock, acc);
+    return acc;
 }
 
 struct D4 { float4 d, r; };

@@ -228,6 +228,34 @@ int launch_gather(const TetDevice& t, int mode, const NodeTerms& nt, int nblk, h
 // ignored) and writes that many d.q partials.
 int launch_gather_skip(const TetDevice& t, int mode, const NodeTerms& nt, int nblk, const int32_t* skip,
                        hipStream_t s);
+// Chunk layouts of the leaf CG, shared by the plain and batched paths so that
+// both sum the same partials in the same order. The matvec gather (and the
+// CG start fused into it) runs in chunks of kGatherNpt kBlock nodes; the CG
+// vector kernels in chunks of kCgVpt kBlock float4s. Where that would make
+// more than kMaxPartials chunks, a chunk takes more nodes (float4s) per thread.
+constexpr uint32_t kGatherNpt = 2;
+constexpr uint32_t kCgVpt = 4;
+inline uint32_t matvec_npt(uint32_t N) {
+    const uint32_t cap = (uint32_t)kMaxPartials * kBlock;
+    const uint32_t need = (N + cap - 1) / cap;
+    return need > kGatherNpt ? need : kGatherNpt;
+}
+inline uint32_t matvec_chunks_of(uint32_t N) {
+    const uint32_t per = matvec_npt(N) * kBlock;
+    const uint32_t c = (N + per - 1) / per;
+    return c ? c : 1u;
+}
+inline uint32_t cg_rows(uint32_t n) {
+    const uint32_t n4 = (n + 3) / 4;
+    const uint32_t cap = (uint32_t)kMaxPartials * kBlock;
+    const uint32_t need = (n4 + cap - 1) / cap;
+    return need > kCgVpt ? need : kCgVpt;
+}
+inline uint32_t cg_chunks(uint32_t n) {
+    const uint32_t per = cg_rows(n) * kBlock;
+    const uint32_t c = ((n + 3) / 4 + per - 1) / per;
+    return c ? c : 1u;
+}
 int matvec_chunks(uint32_t N);
 int launch_energy(const TetDevice& t, const float* x, const float* R, Material m, double* blk, int nblk,
                   double* total, hipStream_t s);
@@ -274,7 +302,10 @@ struct SimDev {
     uint32_t nb_elem;  // node_blocks(T): BeginEvaluation, energy
     uint32_t nb_node;  // node_blocks(N): gathers
     uint32_t nb_init;  // node_blocks(3N): CG init, integration
-    uint32_t nb_vec;   // vec_blocks(3N): CG vector updates
+    uint32_t nb_vec;   // cg_chunks(3N): CG vector updates (the leaf's chunk layout)
+    uint32_t nb_mv;    // matvec_chunks_of(N): matvec gather and CG start chunks
+    uint32_t npt_mv;   // matvec_npt(N): nodes per thread of one matvec chunk
+    uint32_t rows_cg;  // cg_rows(3N): float4s per thread of one CG chunk
     // topology (may be shared by several simulations)
     const uint32_t* conn;
     const uint32_t* tile_elem_ptr;
@@ -292,9 +323,9 @@ struct SimDev {
     // state of this simulation
     float *x, *v, *b, *y, *r, *d, *q, *R, *detF, *partials, *knc_target;
     const uint8_t* knc_mask;  // null without KNC
-    double* part_a;           // nb_node d.q partials
-    double* part_b;           // max(nb_init, nb_vec) r.r partials
-    double* part_e;           // nb_elem energy partials
+    double* part_a;           // nb_mv d.q partials
+    double* part_b;           // max(nb_mv, nb_vec) r.r partials
+    double* part_e;           // max(num_tiles, nb_elem) energy partials
     uint32_t* eval_count;     // 2 nb_elem
     double* eval_sum;         // nb_elem
     float* eval_min;          // nb_elem
@@ -340,6 +371,7 @@ struct BatchTables {
     const BlockRef* node; uint32_t n_node;
     const BlockRef* init; uint32_t n_init;
     const BlockRef* vec; uint32_t n_vec;
+    const BlockRef* mv; uint32_t n_mv;
     const uint32_t* active; uint32_t n_active;  // slots of the stepped simulations
     uint32_t* counter;                          // simulations still iterating
     uint32_t* busy;                             // simulations not idle after a round
@@ -353,7 +385,6 @@ int launch_batch_begin_eval(const BatchTables& t, hipStream_t s);
 int launch_batch_forces(const BatchTables& t, int mode, int sel, hipStream_t s);
 int launch_batch_gather_rhs(const BatchTables& t, hipStream_t s);
 int launch_batch_gather_matvec(const BatchTables& t, int sel, hipStream_t s);
-int launch_batch_cg_init(const BatchTables& t, hipStream_t s);
 int launch_batch_cg_start(const BatchTables& t, hipStream_t s);
 int launch_batch_cg_alpha(const BatchTables& t, hipStream_t s);
 int launch_batch_update_xr(const BatchTables& t, hipStream_t s);

@@ -237,55 +237,20 @@ __global__ __launch_bounds__(kBlock) void k_energy(uint32_t T, const uint32_t* _
     energy_block(T, conn, ref, x, R, dinv, vol, m, blk, blockIdx.x, gridDim.x, sh);
 }
 
-// C_V of C_WRP per tile (ms/fem/TetrahedronElementSet3.cpp:1456-1488): the
-// tile's positions (a.in) and rest positions staged in LDS planes as for K*x,
-// Dm^-1 and V rebuilt from them as k_precompute computes them, one element per
-// thread; part[t] is the tile's FP64 sum (each wave's terms summed by
-// wave_sum, the waves in order).
+// C_V of C_WRP per tile (tile_energy_block), one tile per workgroup with the
+// K*x tiles' XCD mapping; part[t] is tile t's FP64 sum.
 template <int TB>
 __global__ __launch_bounds__(TB) void k_tile_energy(TileArgs a, double* part) {
-#pragma clang fp contract(off)
     extern __shared__ float smem_f[];
     __shared__ double sh[TB / 64];
-    const Slots s_x = make_slots(smem_f, TB), s_r = make_slots(smem_f + 3 * TB, TB);
     const uint32_t xcd = blockIdx.x % 8, j = blockIdx.x / 8;
     const uint32_t per = (a.num_tiles + 7) / 8;
     const uint32_t lo = xcd * per;
     const uint32_t hi = min(a.num_tiles, lo + per);
     if (lo + j >= hi) return;
     const uint32_t t = lo + j;
-    const uint32_t tid = threadIdx.x;
-    const TileDesc dt = tile_desc(a, t);
-    TileRegs g;
-    load_tile_regs_cold(a, dt, tid, g);
-    if (tid < dt.nn) {
-        s_x.x[tid] = g.v[0]; s_x.y[tid] = g.v[1]; s_x.z[tid] = g.v[2];
-        s_r.x[tid] = g.r[0]; s_r.y[tid] = g.r[1]; s_r.z[tid] = g.r[2];
-    }
-    __syncthreads();
-    double e = 0.0;
-    if (tid < dt.ne) {
-        const uint32_t l0 = g.l[0];
-        float D[9], Dm[9], G[9];
-#pragma unroll
-        for (int q = 0; q < 3; q++) {
-            const uint32_t l = g.l[q + 1];
-            D[q] = s_x.x[l] - s_x.x[l0]; D[3 + q] = s_x.y[l] - s_x.y[l0]; D[6 + q] = s_x.z[l] - s_x.z[l0];
-            Dm[q] = s_r.x[l] - s_r.x[l0]; Dm[3 + q] = s_r.y[l] - s_r.y[l0]; Dm[6 + q] = s_r.z[l] - s_r.z[l0];
-        }
-        const float detDm = refmath::det3(Dm);
-        const float V = detDm / 6.0f;
-        refmath::inv3(Dm, detDm, G);
-        e = wrp_energy(D, Dm, g.R, G, V, a.m);
-    }
-    e = wave_sum(e);
-    if ((tid & 63) == 0) sh[tid >> 6] = e;
-    __syncthreads();
-    if (tid == 0) {
-        double sum = 0.0;
-        for (int w = 0; w < TB / 64; w++) sum += sh[w];
-        part[t] = sum;
-    }
+    const double e = tile_energy_block<TB>(a, t, smem_f, sh);
+    if (threadIdx.x == 0) part[t] = e;
 }
 
 // *out = the n partials summed in the one-block order (one workgroup).
@@ -431,7 +396,6 @@ __device__ inline void tail_dx(uint32_t n, float alpha, float beta, bool upd_d, 
 // whichever workgroup computes it, so results do not depend on the grid. A
 // workgroup loads the first kCgVpt rows of its first chunk before it sums the
 // previous kernel's partials (PRE: chunk c's operands are in v).
-constexpr uint32_t kCgVpt = 4;
 struct R4Rows { R4 v[kCgVpt]; };
 struct Dx4Rows { Dx4 v[kCgVpt]; };
 
@@ -751,17 +715,7 @@ int launch_forces(const TetDevice& t, int mode, const float* in, const float* R,
     return launch_forces_skip(t, mode, in, R, m, nullptr, s);
 }
 
-// Nodes per thread of the matvec gather's chunks over N nodes (kGatherNpt, or
-// more where that would make more than kMaxPartials chunks), and the chunk
-// count: the number of d.q partials it writes.
-static uint32_t matvec_npt(uint32_t N) {
-    const uint32_t cap = (uint32_t)kMaxPartials * kBlock;
-    return std::max(kGatherNpt, (N + cap - 1) / cap);
-}
-int matvec_chunks(uint32_t N) {
-    const uint32_t per = matvec_npt(N) * kBlock;
-    return (int)std::max(1u, (N + per - 1) / per);
-}
+int matvec_chunks(uint32_t N) { return (int)matvec_chunks_of(N); }
 
 int launch_gather_skip(const TetDevice& t, int mode, const NodeTerms& nt, int nblk, const int32_t* skip,
                        hipStream_t s) {
@@ -839,19 +793,6 @@ int set_cg_workgroups(uint32_t n) {
     return CF_OK;
 }
 uint32_t get_cg_workgroups() { return g_cg_workgroups ? g_cg_workgroups : kCgWorkgroups; }
-
-// Float4 rows per thread of one CG step chunk and the chunk count over n
-// floats: kCgVpt rows, or more where that would make more than kMaxPartials
-// chunks (the r.r partial buffer's size).
-static uint32_t cg_rows(uint32_t n) {
-    const uint32_t n4 = (n + 3) / 4;
-    const uint32_t cap = (uint32_t)kMaxPartials * kBlock;  // float4 rows of kBlock over all chunks
-    return std::max(kCgVpt, (n4 + cap - 1) / cap);
-}
-static uint32_t cg_chunks(uint32_t n) {
-    const uint32_t per = cg_rows(n) * kBlock;
-    return std::max(1u, ((n + 3) / 4 + per - 1) / per);
-}
 
 static dim3 cg_step_grid(uint32_t nchunk) {
     const uint32_t g = g_cg_workgroups ? g_cg_workgroups : kCgWorkgroups;
