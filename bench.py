@@ -353,7 +353,7 @@ def bench_serve(cf, args):
     """Serving: Poisson arrivals at each rate for a fixed real-time window;
     requests are seeded cube grids of 500-200,000 tets (log-uniform) with 1-60
     steps and config-3 materials, admitted by a 200 GB HBM budget and a cap on
-    the tets stepped per tick."""
+    the tets in flight that adapts to a target round time."""
     from risdsdgcfem_thesis_amd import workloads
     out = []
     for rate in args.serve_rates:
@@ -361,10 +361,12 @@ def bench_serve(cf, args):
         base = workloads.solid_params(1e5, 0.3, args.batch_max_iter)
         res, lat = cf.serve(meshes, reqs, base, hbm_budget_bytes=int(args.serve_budget_gb * 1e9),
                             duration_s=args.serve_seconds, drain=False, max_tick_tets=args.serve_tick_tets,
-                            cg_chunk=args.cg_chunk, use_graphs=not args.no_graphs)
+                            cg_chunk=args.cg_chunk, use_graphs=not args.no_graphs,
+                            target_round_ms=args.serve_target_round_ms)
         log(f"[serve] rate {rate}/s: {res['requests']} requests, {res['admitted']} admitted, {res['completed']} done, "
             f"{res['steps']} steps in {res['wall_s']:.2f} s, p50 {res['step_p50_ms']:.2f} ms, "
-            f"p99 {res['step_p99_ms']:.2f} ms, max active {res['max_active']}")
+            f"p99 {res['step_p99_ms']:.2f} ms, max active {res['max_active']}, round {res['tick_mean_ms']:.2f} ms, "
+            f"cap {res['final_tick_tets'] / 1e6:.1f}M tets")
         out.append({"rate_per_s": rate, "requests": res["requests"], "admitted": res["admitted"],
                     "completed": res["completed"], "steps": res["steps"],
                     "steps_per_s": round(res["steps_per_s"], 1),
@@ -373,9 +375,11 @@ def bench_serve(cf, args):
                     "first_step_p50_ms": round(res["first_p50_ms"], 3),
                     "first_step_p99_ms": round(res["first_p99_ms"], 3),
                     "tick_mean_ms": round(res["tick_mean_ms"], 3), "max_active": res["max_active"],
+                    "final_tick_tets": res["final_tick_tets"],
                     "peak_gb": round(res["peak_bytes"] / 1e9, 3), "window_s": args.serve_seconds})
     return {"workload": "serving: Poisson arrivals, cube grids 500-200,000 tets, 1-60 steps, config-3 materials",
             "hbm_budget_gb": args.serve_budget_gb, "max_tick_tets": args.serve_tick_tets,
+            "target_round_ms": args.serve_target_round_ms,
             "cg_max_iter": args.batch_max_iter, "rates": out}
 
 
@@ -408,10 +412,16 @@ def main():
                     help="batched 3D and serving: launch the step's kernels directly instead of recorded HIP graphs "
                          "(for kernel-trace profiling)")
     ap.add_argument("--batch2d-count", type=int, default=65536)
-    ap.add_argument("--serve-rates", type=float, nargs="+", default=[100.0, 1000.0, 10000.0])
+    ap.add_argument("--serve-rates", type=float, nargs="+", default=[100.0, 300.0, 1000.0, 10000.0],
+                    help="serving: Poisson arrival rates (per second); 100 and 300 are sustainable, 1k and 10k "
+                         "overload the GPU")
     ap.add_argument("--serve-seconds", type=float, default=3.0)
     ap.add_argument("--serve-budget-gb", type=float, default=200.0)
-    ap.add_argument("--serve-tick-tets", type=int, default=16_000_000)
+    ap.add_argument("--serve-tick-tets", type=int, default=16_000_000,
+                    help="serving: tets cap on admission (the starting cap when --serve-target-round-ms > 0)")
+    ap.add_argument("--serve-target-round-ms", type=float, default=10.0,
+                    help="serving: adapt the tets cap so that a round takes about this long (0: fixed cap; "
+                         "swept 5, 10, 20: 10 gave the most steps at 1k arrivals/s)")
     ap.add_argument("--kx-only", type=int, default=0, metavar="REPS",
                     help="profiling mode: one step, then REPS K*x launches and one 256 MiB device copy "
                          "(the copy calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE); prints timings only")

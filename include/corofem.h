@@ -577,6 +577,11 @@ typedef struct cf_serve_config {
     int32_t drain;             /* 1: run until every admitted and waiting request is done */
     uint32_t tile_threads, cg_chunk;
     int32_t use_graphs;        /* 1: record the batched step as HIP graphs */
+    double target_round_ms;    /* > 0: the tets cap adapts from max_tick_tets (the starting cap) so that a
+                                  round takes about this long: after a round that admission hit the cap
+                                  and that took under 0.8 of it the cap grows by 1.25x; after a round
+                                  over 1.25 of it the cap shrinks by 0.8x (not below 1/64 of the start).
+                                  0: max_tick_tets is fixed. */
 } cf_serve_config;
 typedef struct cf_serve_result {
     uint32_t requests, admitted, completed, max_active, ticks, mesh_uploads;
@@ -584,6 +589,7 @@ typedef struct cf_serve_result {
     double wall_s, steps_per_s, tick_mean_ms;
     double step_p50_ms, step_p99_ms, step_mean_ms, step_max_ms;
     double first_p50_ms, first_p99_ms;  /* arrival -> first step result */
+    uint64_t final_tick_tets;           /* the tets cap at the end of the run (0: none) */
 } cf_serve_result;
 /* h_final_pos: NULL or one pointer per request (NULL entries skipped) that
    receives the final positions (3 n_nodes floats) of completed requests.

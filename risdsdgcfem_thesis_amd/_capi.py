@@ -79,7 +79,7 @@ class ServeRequest(C.Structure):
 class ServeConfig(C.Structure):
     _fields_ = [("base", SolidParams), ("hbm_budget_bytes", C.c_uint64), ("max_tick_tets", C.c_uint64),
                 ("duration_s", C.c_double), ("drain", C.c_int32), ("tile_threads", C.c_uint32),
-                ("cg_chunk", C.c_uint32), ("use_graphs", C.c_int32)]
+                ("cg_chunk", C.c_uint32), ("use_graphs", C.c_int32), ("target_round_ms", C.c_double)]
 
 
 class ServeResult(C.Structure):
@@ -89,7 +89,7 @@ class ServeResult(C.Structure):
         ("peak_bytes", C.c_uint64), ("wall_s", C.c_double), ("steps_per_s", C.c_double),
         ("tick_mean_ms", C.c_double), ("step_p50_ms", C.c_double), ("step_p99_ms", C.c_double),
         ("step_mean_ms", C.c_double), ("step_max_ms", C.c_double), ("first_p50_ms", C.c_double),
-        ("first_p99_ms", C.c_double),
+        ("first_p99_ms", C.c_double), ("final_tick_tets", C.c_uint64),
     ]
 
 

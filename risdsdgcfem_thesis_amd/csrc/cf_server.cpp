@@ -1,5 +1,6 @@
 // Real-time serving loop over a batch (cf_serve_trace in include/corofem.h):
-// arrivals, FIFO admission by HBM budget and a cap on the tets in flight,
+// arrivals, FIFO admission by HBM budget and a cap on the tets in flight
+// (fixed, or adapted to a target round time),
 // iteration-level batching (every round runs, in the same launches, the step
 // prologues, a few CG iterations and the step epilogues of all admitted
 // requests, so each step completes in the round its own solve stops),
@@ -92,15 +93,24 @@ extern "C" int cf_serve_trace(const cf_serve_mesh* meshes, uint32_t n_meshes, co
     const clk::time_point t0 = clk::now();
     auto now = [&]() { return std::chrono::duration<double>(clk::now() - t0).count(); };
     double tick_total = 0.0;
+    // the tets cap on admission, adapted to the measured round time when a
+    // target is set
+    uint64_t cap = cfg->max_tick_tets;
+    const uint64_t cap_min = std::max<uint64_t>(1, cap / 64);
+    const double target = cfg->target_round_ms;
 
     while (true) {
         double t = now();
         while (next < eligible && reqs[next].arrival_s <= t) waiting.push_back(next++);
         // admission, in arrival order
+        bool cap_blocked = false;
         while (!waiting.empty()) {
             const cf_serve_request& q = reqs[waiting.front()];
             const cf_serve_mesh& m = meshes[q.mesh];
-            if (cfg->max_tick_tets && !live.empty() && live_tets + m.n_tets > cfg->max_tick_tets) break;
+            if (cap && !live.empty() && live_tets + m.n_tets > cap) {
+                cap_blocked = true;
+                break;
+            }
             auto it = cache.find(q.mesh);
             if (it == cache.end()) {
                 uint32_t mid = 0;
@@ -158,6 +168,11 @@ extern "C" int cf_serve_trace(const cf_serve_mesh* meshes, uint32_t n_meshes, co
         const double te = now();
         tick_total += te - ts;
         r.ticks++;
+        if (target > 0 && cap) {
+            const double ms = (te - ts) * 1e3;
+            if (cap_blocked && ms < 0.8 * target) cap += cap / 4;
+            else if (ms > 1.25 * target) cap = std::max(cap_min, cap - cap / 5);
+        }
         std::unordered_map<uint32_t, uint32_t> done_of;
         for (size_t i = 0; i < ids.size(); i++) done_of[ids[i]] = done[i];
         std::vector<Live> keep;
@@ -197,6 +212,7 @@ extern "C" int cf_serve_trace(const cf_serve_mesh* meshes, uint32_t n_meshes, co
     r.step_mean_ms = lat.empty() ? 0.0 : s / double(lat.size());
     r.first_p50_ms = percentile(first, 0.50);
     r.first_p99_ms = percentile(first, 0.99);
+    r.final_tick_tets = cap;
     if (lat_out) std::copy(lat.begin(), lat.end(), lat_out);
     *out = r;
     return CF_OK;

@@ -623,12 +623,14 @@ class Batch:
 
 
 def serve(meshes, requests, params, hbm_budget_bytes, duration_s, drain=False, max_tick_tets=0, tile_threads=256,
-          cg_chunk=8, final_positions=None, use_graphs=True):
+          cg_chunk=8, final_positions=None, use_graphs=True, target_round_ms=0.0):
     """Runs the real-time serving loop (cf_serve_trace).
 
     meshes: list of (ref_pos, tets); requests: list of dicts with arrival (s),
     mesh (index), steps, E, nu, knc (node ids held at rest), sorted by
     arrival; params: base SolidParams (dt, density, gravity, CG settings).
+    target_round_ms > 0: the tets cap starts at max_tick_tets and adapts so
+    that a round takes about that long.
     final_positions: None or a list (one entry per request, None to skip) of
     float32 arrays that receive the final positions.
     Returns (result dict, per-step latencies in ms)."""
@@ -656,6 +658,7 @@ def serve(meshes, requests, params, hbm_budget_bytes, duration_s, drain=False, m
     cfg.tile_threads = int(tile_threads)
     cfg.cg_chunk = int(cg_chunk)
     cfg.use_graphs = int(bool(use_graphs))
+    cfg.target_round_ms = float(target_round_ms)
     fp = None
     if final_positions is not None:
         fp = (_f * len(requests))(*[(_fp(a) if a is not None else None) for a in final_positions])
