@@ -244,7 +244,7 @@ struct SimLayout {
         mask = l.add<uint8_t>(N);
         pa = l.add<double>(nb_mv);
         pb = l.add<double>(std::max(nb_mv, nb_vec));
-        pe = l.add<double>(std::max(std::max(num_tiles, nb_elem), 1u));
+        pe = l.add<double>(std::max(num_tiles, 1u));
         ec = l.add<uint32_t>(2 * (size_t)nb_elem);
         es = l.add<double>(nb_elem);
         em = l.add<float>(nb_elem);

@@ -243,9 +243,8 @@ __global__ __launch_bounds__(kBlock) void kb_integrate(BatchTables t) {
     integrate_block(3 * s.N, s.x, s.v, s.y, s.dt, s.drag, br.blk, s.nb_init, s.x0);
 }
 
-// V() with the new positions (FixedStep :977): C_WRP per tile as the leaf's
-// tile energy computes it (part_e[tile]); the LCM materials per element block
-// (part_e[block]).
+// V() with the new positions (FixedStep :977), C_WRP per tile as the leaf's
+// tile energy computes it (part_e[tile]); batches hold C_WRP simulations only.
 template <int TB>
 __global__ __launch_bounds__(TB) void kb_tile_energy(BatchTables t) {
     extern __shared__ float smem_f[];
@@ -253,27 +252,17 @@ __global__ __launch_bounds__(TB) void kb_tile_energy(BatchTables t) {
     const BlockRef br = t.tile[blockIdx.x];
     if (br.slot == kIdleSlot || t.cgs[br.slot].phase != kPhaseEnd) return;
     const SimDev& s = t.sims[br.slot];
-    if (!s.compute_energy || s.m.mm != CF_MM_C_WRP) return;
+    if (!s.compute_energy) return;
     const TileArgs a = tile_args(s, s.x, t.max_tn, t.max_te);
     const double e = tile_energy_block<TB>(a, br.blk, smem_f, sh);
     if (threadIdx.x == 0) s.part_e[br.blk] = e;
-}
-
-__global__ __launch_bounds__(kBlock) void kb_energy(BatchTables t) {
-    __shared__ double sh[kBlock / 64];
-    const BlockRef br = t.elem[blockIdx.x];
-    if (br.slot == kIdleSlot || t.cgs[br.slot].phase != kPhaseEnd) return;
-    const SimDev& s = t.sims[br.slot];
-    if (!s.compute_energy || s.m.mm == CF_MM_C_WRP) return;
-    energy_block(s.T, s.conn, s.ref, s.x, s.R, s.dinv, s.vol, s.m, s.part_e, br.blk, s.nb_elem, sh);
 }
 
 // Per-simulation step results, one wavefront per simulation: the
 // BeginEvaluation counts and minimum over the element blocks (any order gives
 // the same values), the sum of degenerate det F in block order (as the leaf's
 // host sum; only when there are degenerate elements, else it is 0), and the
-// energy partials in the leaf's one-block order (tiles for C_WRP, element
-// blocks for the LCM materials).
+// per-tile energies in the leaf's one-block order.
 __global__ __launch_bounds__(kBlock) void kb_finish(BatchTables t) {
     const uint32_t slot = wave_slot(t);
     if (slot == kIdleSlot) return;
@@ -295,7 +284,7 @@ __global__ __launch_bounds__(kBlock) void kb_finish(BatchTables t) {
     }
     double en = 0.0;
     if (s.compute_energy)
-        en = wave_reduce_partials(s.part_e, (int)(s.m.mm == CF_MM_C_WRP ? s.num_tiles : s.nb_elem));
+        en = wave_reduce_partials(s.part_e, (int)s.num_tiles);
     if (lane != 0) return;
     double sd = 0.0;
     if (nd)
@@ -433,17 +422,13 @@ int launch_batch_integrate(const BatchTables& t, hipStream_t s) {
 }
 
 int launch_batch_energy(const BatchTables& t, hipStream_t s) {
-    if (t.n_tile) {
-        switch (t.tile_threads) {
-            case 1024: hipLaunchKernelGGL(kb_tile_energy<1024>, dim3(t.n_tile), dim3(1024), 6 * 1024 * 4, s, t); break;
-            case 512: hipLaunchKernelGGL(kb_tile_energy<512>, dim3(t.n_tile), dim3(512), 6 * 512 * 4, s, t); break;
-            case 128: hipLaunchKernelGGL(kb_tile_energy<128>, dim3(t.n_tile), dim3(128), 6 * 128 * 4, s, t); break;
-            default: hipLaunchKernelGGL(kb_tile_energy<256>, dim3(t.n_tile), dim3(256), 6 * 256 * 4, s, t); break;
-        }
-        CF_TRY(check_launch_b());
+    if (!t.n_tile) return CF_OK;
+    switch (t.tile_threads) {
+        case 1024: hipLaunchKernelGGL(kb_tile_energy<1024>, dim3(t.n_tile), dim3(1024), 6 * 1024 * 4, s, t); break;
+        case 512: hipLaunchKernelGGL(kb_tile_energy<512>, dim3(t.n_tile), dim3(512), 6 * 512 * 4, s, t); break;
+        case 128: hipLaunchKernelGGL(kb_tile_energy<128>, dim3(t.n_tile), dim3(128), 6 * 128 * 4, s, t); break;
+        default: hipLaunchKernelGGL(kb_tile_energy<256>, dim3(t.n_tile), dim3(256), 6 * 256 * 4, s, t); break;
     }
-    if (!t.n_elem) return CF_OK;
-    hipLaunchKernelGGL(kb_energy, dim3(t.n_elem), dim3(kBlock), 0, s, t);
     return check_launch_b();
 }
 

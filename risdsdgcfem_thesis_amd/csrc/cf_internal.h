@@ -325,7 +325,7 @@ struct SimDev {
     const uint8_t* knc_mask;  // null without KNC
     double* part_a;           // nb_mv d.q partials
     double* part_b;           // max(nb_mv, nb_vec) r.r partials
-    double* part_e;           // max(num_tiles, nb_elem) energy partials
+    double* part_e;           // num_tiles energy partials
     uint32_t* eval_count;     // 2 nb_elem
     double* eval_sum;         // nb_elem
     float* eval_min;          // nb_elem
