@@ -409,8 +409,8 @@ def main():
                          "backwards, starting on the tiles the previous K*x touched last")
     ap.add_argument("--no-autotune", action="store_true", help="batched 3D: use --tile-threads/--tile-elems/--cg-chunk")
     ap.add_argument("--no-graphs", action="store_true",
-                    help="batched 3D and serving: launch the step's kernels directly instead of recorded HIP graphs "
-                         "(for kernel-trace profiling)")
+                    help="single mesh, batched 3D and serving: launch the step's kernels directly instead of "
+                         "recorded HIP graphs (for kernel-trace profiling)")
     ap.add_argument("--batch2d-count", type=int, default=65536)
     ap.add_argument("--serve-rates", type=float, nargs="+", default=[100.0, 300.0, 1000.0, 10000.0],
                     help="serving: Poisson arrival rates (per second); 100 and 300 are sustainable, 1k and 10k "
@@ -493,6 +493,7 @@ def bench_single(cf, args):
     reset_tiles(cf)  # the solid keeps its tiles; later workloads build theirs with the library default
     solid.set_state(x0, np.zeros_like(x0))
     solid.set_knc(knc, x0.reshape(-1, 3)[knc].ravel())
+    solid.set_graphs(not args.no_graphs)
     log(f"[bench] setup {time.perf_counter() - t_build:.2f} s; warmup {args.warmup} steps")
     if args.kx_only:
         solid.fixed_step()

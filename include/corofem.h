@@ -339,6 +339,13 @@ int cf_solid_step(cf_solid* s, float dt, cf_step_stats* stats);
 int cf_solid_fixed_step(cf_solid* s, float dt, cf_step_stats* stats);
 /* m_TotalTime: the t handed to BeginEvaluation (:927). */
 int cf_solid_total_time(const cf_solid* s, float* t);
+/* Execution control with no counterpart in the reference. The leaf's CG runs
+   in chunks of eight iterations; with graphs on (the default) each chunk is
+   replayed as a HIP graph recorded on first use and kept until the linear
+   system's coefficients, the iteration cap, the fixed nodes, the tile
+   alternation or the CG grid change; off, the same kernels are launched
+   directly (bit-identical; for kernel-trace profiling). */
+int cf_solid_set_graphs(cf_solid* s, int use_graphs);
 int cf_solid_get_state(const cf_solid* s, float* h_pos, float* h_vel);     /* QueryState (GLeafDSH_Model.h:34-43) */
 int cf_solid_set_state(cf_solid* s, const float* h_pos, const float* h_vel);
 int cf_solid_device_state(cf_solid* s, float** d_pos, float** d_vel);

@@ -164,6 +164,7 @@ SIGNATURES = {
     "cf_solid_device_state": (C.c_int, [_vp, C.POINTER(_vp), C.POINTER(_vp)]),
     "cf_solid_info": (C.c_int, [_vp, _u, _u, _f, _f]),
     "cf_solid_total_time": (C.c_int, [_vp, _f]),
+    "cf_solid_set_graphs": (C.c_int, [_vp, C.c_int]),
     "cf_solid_assemble_rhs": (C.c_int, [_vp, C.c_float, _f, _f]),
     "cf_solid_apply_A": (C.c_int, [_vp, C.c_float, C.c_float, _f, _f]),
     "cf_solid_time_df_x": (C.c_int, [_vp, C.c_uint32, C.POINTER(C.c_double)]),

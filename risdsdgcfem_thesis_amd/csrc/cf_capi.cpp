@@ -158,6 +158,8 @@ struct CgGraphs {
     float fM = 0.f, fK = 0.f;
     uint32_t max_it = 0;
     const uint8_t* knc_mask = nullptr;
+    int alternate = -1;           // the tile-alternation setting recorded
+    uint32_t cg_workgroups = 0;   // the CG step kernels' grid recorded
     void clear() {
         for (hipGraphExec_t g : chunk)
             if (g) (void)hipGraphExecDestroy(g);
@@ -188,7 +190,8 @@ struct cf_solid {
     int32_t ls_total = 0;                 // linear-solve iterations of the last fixed step
     bool nonfinite = false;               // a reduction of the last fixed step was NaN or Inf
     ExactDevice ex;                       // reference_order: the reference-order FP32 step's data
-    CgGraphs graphs;                      // recorded CG chunks (multi-solid step)
+    CgGraphs graphs;                      // recorded CG chunks
+    bool use_graphs = true;               // the leaf solve replays its CG chunks as recorded graphs
 };
 
 struct cf_multi {
@@ -415,9 +418,12 @@ static int cg_enqueue(cf_solid* s, const CgRun& run, uint32_t k0, uint32_t kend)
 // The recorded chunk starting at iteration k0 (recorded on first use).
 static int cg_graph(cf_solid* s, const CgRun& run, uint32_t k0, uint32_t kend, hipGraphExec_t* out) {
     CgGraphs& g = s->graphs;
-    if (g.fM != run.nt.fM || g.fK != run.nt.fK || g.max_it != run.max_it || g.knc_mask != run.nt.knc_mask) {
+    const uint32_t wg = get_cg_workgroups();
+    if (g.fM != run.nt.fM || g.fK != run.nt.fK || g.max_it != run.max_it || g.knc_mask != run.nt.knc_mask ||
+        g.alternate != g_tile_alternate || g.cg_workgroups != wg) {
         g.clear();
         g.fM = run.nt.fM; g.fK = run.nt.fK; g.max_it = run.max_it; g.knc_mask = run.nt.knc_mask;
+        g.alternate = g_tile_alternate; g.cg_workgroups = wg;
     }
     const size_t c = k0 / kCgChunk;
     if (g.chunk.size() <= c) g.chunk.resize(c + 1, nullptr);
@@ -503,23 +509,28 @@ static int cg_finish(cf_solid* s, const CgRun& run, cf_cg_result* res) {
 // The single-leaf solve: chunk c + 1 is queued before the host waits for
 // chunk c, so the device does not idle while the host reads a stop flag and
 // enqueues; a solve that stops inside chunk c leaves chunk c + 1 to run as
-// skipped launches.
+// skipped launches. Chunks are replayed as recorded graphs (one launch instead
+// of 32) unless the solid's graphs are switched off. The host spins on the
+// chunk's event: a blocking wait added 1-2 us per CG iteration on small meshes.
 static int solid_cg(cf_solid* s, float fM, float fK, cf_cg_result* res) {
     CgRun run;
     CF_TRY(cg_begin(s, fM, fK, &run));
     CgBuffers& b = s->cg;
     uint32_t kend[2] = {0, 0};
-    CF_TRY(cg_launch(s, &run));
+    CF_TRY(cg_launch(s, &run, s->use_graphs));
     kend[0] = run.k;
     CF_CHECK_HIP(hipEventRecord(b.ev_chunk[0], s->stream));
     for (uint32_t c = 0;; c++) {
         const uint32_t cur = c & 1, nxt = cur ^ 1;
         if (run.k < run.max_it) {
-            CF_TRY(cg_launch(s, &run));
+            CF_TRY(cg_launch(s, &run, s->use_graphs));
             kend[nxt] = run.k;
             CF_CHECK_HIP(hipEventRecord(b.ev_chunk[nxt], s->stream));
         }
-        CF_CHECK_HIP(hipEventSynchronize(b.ev_chunk[cur]));
+        hipError_t q;
+        while ((q = hipEventQuery(b.ev_chunk[cur])) == hipErrorNotReady) {
+        }
+        CF_CHECK_HIP(q);
         if (cg_chunk_stopped(s, run, kend[cur])) break;
     }
     run.done = true;
@@ -1756,6 +1767,12 @@ int cf_set_tile_threads(uint32_t n) {
 int cf_set_tile_persistent(int persistent) {
     if (persistent < 0 || persistent > 2) { set_error("tile schedule must be 0, 1 or 2"); return CF_ERR_INVALID_ARG; }
     g_tile_persistent = persistent;
+    return CF_OK;
+}
+
+int cf_solid_set_graphs(cf_solid* s, int use_graphs) {
+    if (!s) return CF_ERR_INVALID_ARG;
+    s->use_graphs = use_graphs != 0;
     return CF_OK;
 }
 

@@ -230,11 +230,15 @@ int launch_gather_skip(const TetDevice& t, int mode, const NodeTerms& nt, int nb
                        hipStream_t s);
 // Chunk layouts of the leaf CG, shared by the plain and batched paths so that
 // both sum the same partials in the same order. The matvec gather (and the
-// CG start fused into it) runs in chunks of kGatherNpt kBlock nodes; the CG
-// vector kernels in chunks of kCgVpt kBlock float4s. Where that would make
-// more than kMaxPartials chunks, a chunk takes more nodes (float4s) per thread.
-constexpr uint32_t kGatherNpt = 2;
+// CG start fused into it) runs in chunks of kGatherNpt kBlock nodes. The CG
+// vector kernels run in chunks of kBlock float4 rows of kCgVpt rows each where
+// that still makes at least kCgMinChunks chunks (large meshes: fewer partials
+// to sum, more loads in flight per thread), else of one row (small meshes:
+// more, shorter workgroups; their kernels are latency-bound). Where a layout
+// would make more than kMaxPartials chunks, a chunk takes more nodes (rows).
+constexpr uint32_t kGatherNpt = 1;
 constexpr uint32_t kCgVpt = 4;
+constexpr uint32_t kCgMinChunks = 256;
 inline uint32_t matvec_npt(uint32_t N) {
     const uint32_t cap = (uint32_t)kMaxPartials * kBlock;
     const uint32_t need = (N + cap - 1) / cap;
@@ -249,7 +253,8 @@ inline uint32_t cg_rows(uint32_t n) {
     const uint32_t n4 = (n + 3) / 4;
     const uint32_t cap = (uint32_t)kMaxPartials * kBlock;
     const uint32_t need = (n4 + cap - 1) / cap;
-    return need > kCgVpt ? need : kCgVpt;
+    const uint32_t base = (n4 >= kCgVpt * kBlock * kCgMinChunks) ? kCgVpt : 1u;
+    return need > base ? need : base;
 }
 inline uint32_t cg_chunks(uint32_t n) {
     const uint32_t per = cg_rows(n) * kBlock;

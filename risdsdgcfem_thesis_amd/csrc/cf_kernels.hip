@@ -388,14 +388,13 @@ __device__ inline void tail_dx(uint32_t n, float alpha, float beta, bool upd_d, 
     }
 }
 
-// Chunk c of the two CG step kernels: `rows` float4s per thread (kCgVpt, or
-// more where kCgVpt would make more than kMaxPartials chunks), float4
-// (c rows + j) kBlock + thread for j < rows (each j a coalesced row), plus the
-// n % 4 tail in chunk 0; without float4 access, the floats
+// Chunk c of the two CG step kernels: `rows` float4s per thread (cg_rows),
+// float4 (c rows + j) kBlock + thread for j < rows (each j a coalesced row),
+// plus the n % 4 tail in chunk 0; without float4 access, the floats
 // c kBlock + thread + m nchunk kBlock. Chunk c's r.r partial is the same value
 // whichever workgroup computes it, so results do not depend on the grid. A
-// workgroup loads the first kCgVpt rows of its first chunk before it sums the
-// previous kernel's partials (PRE: chunk c's operands are in v).
+// workgroup loads the first min(rows, kCgVpt) rows of its first chunk before
+// it sums the previous kernel's partials (PRE: chunk c's operands are in v).
 struct R4Rows { R4 v[kCgVpt]; };
 struct Dx4Rows { Dx4 v[kCgVpt]; };
 
@@ -409,7 +408,7 @@ __device__ inline double r_chunk(uint32_t n, uint32_t c, uint32_t nchunk, uint32
 #pragma unroll
         for (uint32_t j = 0; j < kCgVpt; j++) {
             const uint32_t i = (c * rows + j) * kBlock + tid;
-            if (i < n4) {
+            if (j < rows && i < n4) {
                 R4 w;
                 if (PRE) w = v.v[j];
                 else w = load_r4(r, q, i);
@@ -436,7 +435,7 @@ __device__ inline void dx_chunk(uint32_t n, uint32_t c, uint32_t nchunk, uint32_
 #pragma unroll
         for (uint32_t j = 0; j < kCgVpt; j++) {
             const uint32_t i = (c * rows + j) * kBlock + tid;
-            if (i < n4) {
+            if (j < rows && i < n4) {
                 Dx4 w;
                 if (PRE) w = v.v[j];
                 else w = load_dx4(d, r, x, i);
@@ -468,7 +467,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_step_xr(uint32_t n, uint32_t k, C
     for (uint32_t j = 0; j < kCgVpt; j++) {
         const uint32_t i = (c0 * rows + j) * kBlock + threadIdx.x;
         v.v[j] = R4{};
-        if (V4 && blockIdx.x < nchunk && i < n / 4) v.v[j] = load_r4(r, q, i);
+        if (V4 && j < rows && blockIdx.x < nchunk && i < n / 4) v.v[j] = load_r4(r, q, i);
     }
     const double dq = reduce_partials_wide(dq_part, ndq, sh);
     if (fabs(dq) < st->abs_sq) {
@@ -521,7 +520,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_step_d(uint32_t n, uint32_t k, Cg
     for (uint32_t j = 0; j < kCgVpt; j++) {
         const uint32_t i = (c0 * rows + j) * kBlock + threadIdx.x;
         v.v[j] = Dx4{};
-        if (V4 && blockIdx.x < nchunk && i < n / 4) v.v[j] = load_dx4(d, r, x, i);
+        if (V4 && j < rows && blockIdx.x < nchunk && i < n / 4) v.v[j] = load_dx4(d, r, x, i);
     }
     const double dn = reduce_partials_wide(rr_part, (int)nchunk, sh);
     const uint32_t k1 = k + 1;

@@ -461,6 +461,11 @@ class LeafSolid3D:
         check(lib.cf_solid_total_time(self.h, C.byref(t)), "cf_solid_total_time")
         return t.value
 
+    def set_graphs(self, on):
+        """CG chunks replayed as recorded HIP graphs (default) or launched
+        directly (cf_solid_set_graphs); results are the same."""
+        check(lib.cf_solid_set_graphs(self.h, int(bool(on))), "cf_solid_set_graphs")
+
     def info(self):
         nn, nt, m = C.c_uint32(), C.c_uint32(), C.c_float()
         r = np.zeros(2, np.float32)

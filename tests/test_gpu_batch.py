@@ -216,3 +216,16 @@ def test_multichunk_meshes_equal_plain_bitexact(gpu, max_iter):
     for i, (g, p) in enumerate(zip(batched, plain)):
         print(f"\ncase {i}: CG iterations per step {[h[0] for h in p[2]]}")
         assert same(g, p), f"case {i}: batched steps differ from the single-mesh steps"
+
+
+@pytest.mark.slow
+def test_four_row_cg_chunks_equal_plain_bitexact(gpu):
+    """A 70^3-cube mesh (357,911 nodes, 1,715,000 tets): large enough that the
+    CG vector chunks take four float4 rows per thread (263 chunks) - the
+    layout of the large single-mesh workload - batched against alone, 2 steps."""
+    c = box_case(gpu, (71, 71, 71), 2e5, 0.35, (2, -1), max_iter=60)
+    assert 3 * c["N"] >= 4 * 4 * 256 * 256
+    p = run_plain(gpu, c, 2)
+    (g,) = run_batch(gpu, [c], 2)
+    print(f"\nCG iterations per step {[h[0] for h in p[2]]}")
+    assert same(g, p), "batched steps differ from the single-mesh steps"
