@@ -289,46 +289,55 @@ static void fan_weighted_areas(uint32_t N, uint32_t T, const uint32_t* tris, con
     }
 }
 
-int cf_batch2_add_sim(cf_batch2* b, uint32_t N, const float* ref, uint32_t T, const uint32_t* tris,
-                      const cf_solid_params* p, uint32_t n_knc, const uint32_t* knc, uint32_t* sim_id) {
-    if (!b || !ref || !tris || !p || !sim_id || N == 0 || T == 0 || (n_knc && !knc)) {
-        set_error("cf_batch2_add_sim: invalid arguments");
+// Arguments and parameters a 2D solid accepts (C_WRP, QIE-v, truncated
+// differential, ID / MSVD / DAPD rotations; ids in range).
+static int check2(const char* fn, uint32_t N, const float* ref, uint32_t T, const uint32_t* tris,
+                  const cf_solid_params* p, uint32_t n_knc, const uint32_t* knc) {
+    if (!ref || !tris || !p || N == 0 || T == 0 || (n_knc && !knc)) {
+        set_error("%s: invalid arguments", fn);
         return CF_ERR_INVALID_ARG;
     }
     if (p->fem.mm != CF_MM_C_WRP) {
-        set_error("%s: only the C_WRP material is batched", "cf_batch2_add_sim");
+        set_error("%s: only the C_WRP material is batched", fn);
         return CF_ERR_UNSUPPORTED;
     }
     if (p->integrator != CF_IT_QIE_V) {
-        set_error("%s: only the QIE-v integrator is batched", "cf_batch2_add_sim");
+        set_error("%s: only the QIE-v integrator is batched", fn);
         return CF_ERR_UNSUPPORTED;
     }
     if (p->reference_order) {
-        set_error("cf_batch2_add_sim: 2D solids take the resident step kernel (reference_order = 0)");
+        set_error("%s: 2D solids take the resident step kernel (reference_order = 0)", fn);
         return CF_ERR_UNSUPPORTED;
     }
     if (!(p->fixed_dt > 0) || !(p->density > 0) || !(p->time_scale > 0) || !(p->thickness > 0) ||
         !(p->fem.young_modulus > 0) || !(p->fem.poisson_ratio > -1.0f) || !(p->fem.poisson_ratio < 0.5f) ||
         p->fem.mm != CF_MM_C_WRP || p->fem.dm != CF_DM_TRUNCATED ||
         (p->fem.rm != CF_RM_ID && p->fem.rm != CF_RM_MSVD && p->fem.rm != CF_RM_DAPD)) {
-        set_error("cf_batch2_add_sim: unsupported parameters");
+        set_error("%s: unsupported parameters", fn);
         return CF_ERR_INVALID_ARG;
-    }
-    // smallest group of wavefronts that holds the mesh in registers
-    uint32_t gw = 1;
-    while (gw <= 8 && (T > gw * uint32_t(kGroupTris) || N > gw * uint32_t(kGroupNodes))) gw *= 2;
-    if (gw > 8) {
-        set_error("2D mesh of %u triangles / %u nodes exceeds one workgroup (%d / %d)", T, N, 8 * kGroupTris,
-                  8 * kGroupNodes);
-        return CF_ERR_UNSUPPORTED;
     }
     for (size_t i = 0; i < 3 * (size_t)T; i++)
         if (tris[i] >= N) { set_error("triangle %zu references node %u >= %u", i / 3, tris[i], N); return CF_ERR_INVALID_ARG; }
     for (uint32_t i = 0; i < n_knc; i++)
         if (knc[i] >= N) { set_error("KNC node %u out of range", knc[i]); return CF_ERR_INVALID_ARG; }
-    // element precompute (EndEdition, TriangleElementSet2.cpp:2187-2239)
-    std::vector<float> area(T);
-    std::vector<float4> dinv(T);
+    return CF_OK;
+}
+
+// Smallest group of wavefronts that holds the mesh in registers (above 8: the
+// mesh does not fit one workgroup).
+static uint32_t resident_group(uint32_t N, uint32_t T) {
+    uint32_t gw = 1;
+    while (gw <= 8 && (T > gw * uint32_t(kGroupTris) || N > gw * uint32_t(kGroupNodes))) gw *= 2;
+    return gw;
+}
+
+// Element precompute (EndEdition, TriangleElementSet2.cpp:2187-2239) and the
+// lumped masses: Sum(Rcp(3) area) * density * thickness in the mesh's CCW fan
+// order, stored as 1/m (GetMass = 1/(1/m)).
+static int precompute2(uint32_t N, const float* ref, uint32_t T, const uint32_t* tris, const cf_solid_params* p,
+                       std::vector<float>& area, std::vector<float4>& dinv, std::vector<float>& mass) {
+    area.resize(T);
+    dinv.resize(T);
     for (uint32_t e = 0; e < T; e++) {
         const float* r0 = &ref[2 * (size_t)tris[3 * e]];
         const float* r1 = &ref[2 * (size_t)tris[3 * e + 1]];
@@ -340,32 +349,38 @@ int cf_batch2_add_sim(cf_batch2* b, uint32_t N, const float* ref, uint32_t T, co
         const float id = 1.0f / det;
         dinv[e] = make_float4(id * D11, -id * D01, -id * D10, id * D00);
     }
-    // lumped masses: Sum(Rcp(3) area) * density * thickness in the mesh's CCW
-    // fan order, stored as 1/m (GetMass = 1/(1/m))
     const float density_per_area = p->density * p->thickness;
     std::vector<float> wa;
     fan_weighted_areas(N, T, tris, area.data(), wa);
-    // greedy colouring: no two triangles of one colour share a node
-    std::vector<uint32_t> used(N, 0), color(T);
+    mass.resize(N);
+    for (uint32_t n = 0; n < N; n++) {
+        const float inv_mass = 1.0f / (wa[n] * density_per_area);
+        mass[n] = 1.0f / inv_mass;
+    }
+    return CF_OK;
+}
+
+// Greedy colouring, no two triangles of one colour sharing a node; the number
+// of colours, or 33 when 32 do not suffice.
+static uint32_t colour2(uint32_t N, uint32_t T, const uint32_t* tris, std::vector<uint32_t>& color) {
+    std::vector<uint32_t> used(N, 0);
+    color.assign(T, 0);
     uint32_t C = 0;
     for (uint32_t e = 0; e < T; e++) {
         uint32_t mask = used[tris[3 * e]] | used[tris[3 * e + 1]] | used[tris[3 * e + 2]];
         uint32_t c = 0;
         while (c < 32 && (mask >> c & 1u)) c++;
-        if (c >= uint32_t(kMaxColors)) { set_error("2D mesh needs more than %d colours", kMaxColors); return CF_ERR_UNSUPPORTED; }
+        if (c >= 32) return 33;
         color[e] = c;
         for (int j = 0; j < 3; j++) used[tris[3 * e + j]] |= 1u << c;
         C = std::max(C, c + 1);
     }
-    std::vector<uint32_t> order(T);
-    std::iota(order.begin(), order.end(), 0u);
-    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t q) { return color[a] < color[q]; });
-    CF_TRY(pull_state(b));
-    Mesh2Dev m{};
-    m.N = N; m.T = T; m.C = C; m.gw = gw;
-    m.node_off = uint32_t(b->ref.size());
-    m.tri_off = uint32_t(b->area.size());
-    m.color_off = uint32_t(b->cstart.size());
+    return C;
+}
+
+// The solid's constants of the step (material, time step, Rayleigh factors,
+// CG settings, DAPD factors).
+static void consts2(const cf_solid_params* p, Mesh2Dev& m) {
     m.m = material2(p->fem.young_modulus, p->fem.poisson_ratio);
     float c[2];
     rayleigh_coeffs(p->fem.rayleigh_freq1, p->fem.rayleigh_freq2, p->fem.damping_ratio, c);
@@ -387,13 +402,43 @@ int cf_batch2_add_sim(cf_batch2* b, uint32_t N, const float* ref, uint32_t T, co
     m.dapd_l = p->fem.dapd_l_factor;
     m.dapd_nl = p->fem.dapd_nl_factor;
     m.dapd_e = p->fem.dapd_nl_exponent;
+}
+
+int cf_batch2_add_sim(cf_batch2* b, uint32_t N, const float* ref, uint32_t T, const uint32_t* tris,
+                      const cf_solid_params* p, uint32_t n_knc, const uint32_t* knc, uint32_t* sim_id) {
+    if (!b || !sim_id) {
+        set_error("cf_batch2_add_sim: invalid arguments");
+        return CF_ERR_INVALID_ARG;
+    }
+    CF_TRY(check2("cf_batch2_add_sim", N, ref, T, tris, p, n_knc, knc));
+    const uint32_t gw = resident_group(N, T);
+    if (gw > 8) {
+        set_error("2D mesh of %u triangles / %u nodes exceeds one workgroup (%d / %d)", T, N, 8 * kGroupTris,
+                  8 * kGroupNodes);
+        return CF_ERR_UNSUPPORTED;
+    }
+    std::vector<float> area, mass;
+    std::vector<float4> dinv;
+    CF_TRY(precompute2(N, ref, T, tris, p, area, dinv, mass));
+    std::vector<uint32_t> color;
+    const uint32_t C = colour2(N, T, tris, color);
+    if (C > uint32_t(kMaxColors)) { set_error("2D mesh needs more than %d colours", kMaxColors); return CF_ERR_UNSUPPORTED; }
+    std::vector<uint32_t> order(T);
+    std::iota(order.begin(), order.end(), 0u);
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t q) { return color[a] < color[q]; });
+    CF_TRY(pull_state(b));
+    Mesh2Dev m{};
+    m.N = N; m.T = T; m.C = C; m.gw = gw;
+    m.node_off = uint32_t(b->ref.size());
+    m.tri_off = uint32_t(b->area.size());
+    m.color_off = uint32_t(b->cstart.size());
+    consts2(p, m);
     for (uint32_t n = 0; n < N; n++) {
         b->ref.push_back(make_float2(ref[2 * (size_t)n], ref[2 * (size_t)n + 1]));
         b->x.push_back(b->ref.back());
         b->x0.push_back(b->ref.back());  // m_vecPos0 starts at the rest pose
         b->v.push_back(make_float2(0.f, 0.f));
-        const float inv_mass = 1.0f / (wa[n] * density_per_area);
-        b->mass.push_back(1.0f / inv_mass);
+        b->mass.push_back(mass[n]);
         b->knc.push_back(0);
     }
     for (uint32_t i = 0; i < n_knc; i++) b->knc[m.node_off + knc[i]] = 1;
@@ -533,8 +578,210 @@ int cf_batch2_info(const cf_batch2* b, uint32_t* num_sims, uint64_t* tris, uint6
 }
 
 // ---------------------------------------------------------------- single 2D solid
+// A mesh that fits one workgroup is a one-simulation batch (the resident step
+// kernel); a larger one (more triangles, nodes or colours than the resident
+// kernel holds) is stepped by the multi-workgroup kernels on its own arrays.
+}  // extern "C"
+
+namespace {
+
+struct Large2 {
+    Large2Dev d{};
+    uint32_t max_it = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, evc[2] = {nullptr, nullptr};
+    int32_t* h_done = nullptr;  // pinned: done[k + 1] from the last d kernel of each chunk
+    int32_t* d_done = nullptr;
+    std::vector<void*> mem;
+    bool stepped = false;
+    template <class T>
+    int alloc(T** p, size_t n) {
+        void* q = nullptr;
+        CF_CHECK_HIP(hipMalloc(&q, std::max<size_t>(1, n) * sizeof(T)));
+        mem.push_back(q);
+        *p = static_cast<T*>(q);
+        return CF_OK;
+    }
+    template <class T>
+    int upload(const T** p, const std::vector<T>& h) {
+        T* q = nullptr;
+        CF_TRY(alloc(&q, h.size()));
+        if (!h.empty()) CF_CHECK_HIP(hipMemcpy(q, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+        *p = q;
+        return CF_OK;
+    }
+    ~Large2() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        for (void* q : mem) (void)hipFree(q);
+        if (h_done) (void)hipHostFree(h_done);
+        for (hipEvent_t e : {ev0, ev1, evc[0], evc[1]})
+            if (e) (void)hipEventDestroy(e);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+int large2_create(uint32_t N, const float* ref, uint32_t T, const uint32_t* tris, const cf_solid_params* p,
+                  uint32_t n_knc, const uint32_t* knc, Large2** out) {
+    std::vector<float> area, mass;
+    std::vector<float4> dinv;
+    CF_TRY(precompute2(N, ref, T, tris, p, area, dinv, mass));
+    Large2* L = new (std::nothrow) Large2;
+    if (!L) return CF_ERR_OUT_OF_MEMORY;
+    struct Guard {
+        Large2*& L;
+        ~Guard() { delete L; }
+    } guard{L};
+    Large2Dev& a = L->d;
+    a.N = N;
+    a.T = T;
+    a.nblk_tri = std::min<uint32_t>((T + kBlock - 1) / kBlock, kMaxPartials);
+    a.nblk_node = std::min<uint32_t>((N + kBlock - 1) / kBlock, kMaxPartials);
+    a.c = Mesh2Dev{};
+    consts2(p, a.c);
+    L->max_it = a.c.max_iter;
+    if (hipStreamCreateWithFlags(&L->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&L->ev0) != hipSuccess || hipEventCreate(&L->ev1) != hipSuccess ||
+        hipEventCreateWithFlags(&L->evc[0], hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&L->evc[1], hipEventDisableTiming) != hipSuccess) {
+        set_error("large 2D solid: stream/event creation failed");
+        return CF_ERR_HIP;
+    }
+    std::vector<float2> r2(N);
+    for (uint32_t n = 0; n < N; n++) r2[n] = make_float2(ref[2 * (size_t)n], ref[2 * (size_t)n + 1]);
+    std::vector<uint8_t> kn(N, 0), noc(T, uint8_t(kNocNone));
+    for (uint32_t i = 0; i < n_knc; i++) kn[knc[i]] = 1;
+    std::vector<uint32_t> tri(tris, tris + 3 * (size_t)T);
+    // node -> (triangle, vertex) codes 3 e + v, ascending e (counting sort)
+    std::vector<uint32_t> ptr(N + 1, 0), code(3 * (size_t)T);
+    for (size_t i = 0; i < 3 * (size_t)T; i++) ptr[tris[i] + 1]++;
+    for (uint32_t n = 0; n < N; n++) ptr[n + 1] += ptr[n];
+    std::vector<uint32_t> fill(ptr.begin(), ptr.end() - 1);
+    for (size_t i = 0; i < 3 * (size_t)T; i++) code[fill[tris[i]]++] = uint32_t(i);
+    CF_TRY(L->upload(&a.ref, r2));
+    CF_TRY(L->upload(&a.mass, mass));
+    CF_TRY(L->upload(&a.knc, kn));
+    CF_TRY(L->upload(&a.tri, tri));
+    CF_TRY(L->upload(&a.dinv, dinv));
+    CF_TRY(L->upload(&a.area, area));
+    CF_TRY(L->upload(&a.ptr, ptr));
+    CF_TRY(L->upload(&a.code, code));
+    const float2* cx = nullptr;
+    CF_TRY(L->upload(&cx, r2));
+    a.x = const_cast<float2*>(cx);
+    CF_TRY(L->upload(&cx, r2));  // m_vecPos0 starts at the rest pose
+    a.x0 = const_cast<float2*>(cx);
+    const uint8_t* cn = nullptr;
+    CF_TRY(L->upload(&cn, noc));
+    a.noc = const_cast<uint8_t*>(cn);
+    CF_TRY(L->alloc(&a.v, N));
+    CF_CHECK_HIP(hipMemset(a.v, 0, N * sizeof(float2)));
+    CF_TRY(L->alloc(&a.R, T));
+    CF_TRY(L->alloc(&a.tf, 3 * (size_t)T));
+    for (float2** v : {&a.b, &a.y, &a.r, &a.d, &a.q}) CF_TRY(L->alloc(v, N));
+    CF_TRY(L->alloc(&a.st, 1));
+    CF_TRY(L->alloc(&a.part_a, a.nblk_node));
+    CF_TRY(L->alloc(&a.part_b, a.nblk_node));
+    CF_TRY(L->alloc(&a.stat, 4 * (size_t)a.nblk_tri));
+    CF_TRY(L->alloc(&a.stat_min, a.nblk_tri));
+    CF_TRY(L->alloc(&a.out, 1));
+    CF_CHECK_HIP(hipMemset(a.out, 0, sizeof(Sim2Out)));
+    CF_CHECK_HIP(hipHostMalloc(reinterpret_cast<void**>(&L->h_done), (kMaxCgIter + 2) * sizeof(int32_t)));
+    std::memset(L->h_done, 0, (kMaxCgIter + 2) * sizeof(int32_t));
+    CF_CHECK_HIP(hipHostGetDevicePointer(reinterpret_cast<void**>(&L->d_done), L->h_done, 0));
+    *out = L;
+    L = nullptr;  // owned by the caller now
+    return CF_OK;
+}
+
+// Iterations [k0, kend) of the CG, each skipped on the device once the solve
+// has stopped; the last d kernel writes the stop flag to the host.
+int large2_chunk(Large2* L, uint32_t k0, uint32_t kend) {
+    const Large2Dev& a = L->d;
+    for (uint32_t k = k0; k < kend; k++) {
+        const int32_t* skip = &a.st->done[k];
+        CF_TRY(launch_l2_elem(a, false, a.d, skip, L->stream));
+        CF_TRY(launch_l2_matvec(a, false, a.d, skip, L->stream));
+        CF_TRY(launch_l2_xr(a, k, L->stream));
+        CF_TRY(launch_l2_d(a, k, k + 1 == kend ? L->d_done : nullptr, L->stream));
+    }
+    return CF_OK;
+}
+
+// One QIE-v fixed step (Step_QuasiImplicitEuler_v, LeafDSH_Solid2D_FEM.cpp:1287-1403)
+int large2_step(Large2* L, cf_step_stats* stats) {
+    const Large2Dev& a = L->d;
+    hipStream_t st = L->stream;
+    CF_CHECK_HIP(hipEventRecord(L->ev0, st));
+    CF_TRY(launch_l2_begin(a, st));                       // BeginEvaluation
+    CF_TRY(launch_l2_elem(a, true, a.x, nullptr, st));    // f_e(x)
+    CF_TRY(launch_l2_rhs(a, st));                         // b, y, KNC rows
+    CF_TRY(launch_l2_elem(a, false, a.y, nullptr, st));   // A y
+    CF_TRY(launch_l2_matvec(a, true, a.y, nullptr, st));  // r = S(b - A y), d = r
+    CF_TRY(launch_cg_start(a.st, a.part_b, int(a.nblk_node), a.c.rel, a.c.abs_eps, L->max_it, st));
+    // chunks of eight iterations, one queued behind the one the host waits for
+    constexpr uint32_t kChunk = 8;
+    uint32_t k = 0, kend[2] = {0, 0};
+    auto launch = [&](uint32_t slot) -> int {
+        const uint32_t e = std::min(L->max_it, k + kChunk);
+        CF_TRY(large2_chunk(L, k, e));
+        k = e;
+        kend[slot] = e;
+        CF_CHECK_HIP(hipEventRecord(L->evc[slot], st));
+        return CF_OK;
+    };
+    if (L->max_it > 0) {
+        CF_TRY(launch(0));
+        for (uint32_t c = 0;; c++) {
+            const uint32_t cur = c & 1, nxt = cur ^ 1;
+            if (k < L->max_it) CF_TRY(launch(nxt));
+            hipError_t q;
+            while ((q = hipEventQuery(L->evc[cur])) == hipErrorNotReady) {
+            }
+            CF_CHECK_HIP(q);
+            if (kend[cur] >= L->max_it || reinterpret_cast<volatile const int32_t*>(L->h_done)[kend[cur]] != 0) break;
+        }
+    }
+    CF_TRY(launch_l2_integrate(a, st));
+    if (a.c.compute_energy) CF_TRY(launch_l2_energy(a, st));
+    CF_TRY(launch_l2_finish(a, st));
+    CF_CHECK_HIP(hipEventRecord(L->ev1, st));
+    CF_CHECK_HIP(hipEventSynchronize(L->ev1));
+    L->stepped = true;
+    static thread_local CgScalars hs;
+    CF_CHECK_HIP(hipMemcpy(&hs, a.st, sizeof(CgScalars), hipMemcpyDeviceToHost));
+    cf_cg_result res{};
+    cg_result_from(hs, L->max_it, &res);
+    if (stats) {
+        Sim2Out o;
+        CF_CHECK_HIP(hipMemcpy(&o, a.out, sizeof(Sim2Out), hipMemcpyDeviceToHost));
+        float ms = 0.f;
+        CF_CHECK_HIP(hipEventElapsedTime(&ms, L->ev0, L->ev1));
+        *stats = cf_step_stats{};
+        stats->num_fixed_steps = 1;
+        stats->cg_iters = res.iterations;
+        stats->cg_reason = res.reason;
+        stats->cg_rel_residual = res.rel_residual;
+        stats->cg_abs_residual = res.abs_residual;
+        stats->num_degenerate = o.num_degenerate;
+        stats->sum_degenerate_det_f = float(o.sum_degenerate);
+        stats->elastic_energy = float(o.energy);
+        stats->min_det_f = o.min_det;
+        stats->step_ms = ms;
+    }
+    if (res.reason == CF_CG_NONFINITE) {
+        set_error("2D step: a CG reduction is NaN or Inf (non-finite state)");
+        return CF_ERR_NAN;
+    }
+    return CF_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
 struct cf_solid2 {
     cf_batch2* b = nullptr;
+    Large2* large = nullptr;
     float acc_time = 0.f;
     cf_solid_params p{};
 };
@@ -543,12 +790,26 @@ int cf_solid2_create(uint32_t N, const float* ref, uint32_t T, const uint32_t* t
                      uint32_t n_knc, const uint32_t* knc, cf_solid2** out) {
     if (!out || !p) return CF_ERR_INVALID_ARG;
     *out = nullptr;
+    CF_TRY(check2("cf_solid2_create", N, ref, T, tris, p, n_knc, knc));
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n < 1) {
+        (void)hipGetLastError();
+        set_error("no HIP device");
+        return CF_ERR_NO_DEVICE;
+    }
     cf_solid2* s = new (std::nothrow) cf_solid2;
     if (!s) return CF_ERR_OUT_OF_MEMORY;
     s->p = *p;
-    int rc = cf_batch2_create(&s->b);
-    uint32_t id = 0;
-    if (rc == CF_OK) rc = cf_batch2_add_sim(s->b, N, ref, T, tris, p, n_knc, knc, &id);
+    std::vector<uint32_t> color;
+    const bool resident = resident_group(N, T) <= 8 && colour2(N, T, tris, color) <= uint32_t(kMaxColors);
+    int rc;
+    if (resident) {
+        rc = cf_batch2_create(&s->b);
+        uint32_t id = 0;
+        if (rc == CF_OK) rc = cf_batch2_add_sim(s->b, N, ref, T, tris, p, n_knc, knc, &id);
+    } else {
+        rc = large2_create(N, ref, T, tris, p, n_knc, knc, &s->large);
+    }
     if (rc != CF_OK) { cf_solid2_destroy(s); return rc; }
     *out = s;
     return CF_OK;
@@ -556,14 +817,19 @@ int cf_solid2_create(uint32_t N, const float* ref, uint32_t T, const uint32_t* t
 
 int cf_solid2_destroy(cf_solid2* s) {
     if (!s) return CF_OK;
-    cf_batch2_destroy(s->b);
+    if (s->b) cf_batch2_destroy(s->b);
+    delete s->large;
     delete s;
     return CF_OK;
 }
 
+static int solid2_fixed(cf_solid2* s, cf_step_stats* stats) {
+    return s->large ? large2_step(s->large, stats) : cf_batch2_fixed_step(s->b, stats);
+}
+
 int cf_solid2_fixed_step(cf_solid2* s, cf_step_stats* stats) {
     if (!s) return CF_ERR_INVALID_ARG;
-    return cf_batch2_fixed_step(s->b, stats);
+    return solid2_fixed(s, stats);
 }
 
 // Step (LeafDSH_Solid2D_FEM: fixed-dt accumulator with strict '>')
@@ -575,7 +841,7 @@ int cf_solid2_step(cf_solid2* s, float dt, cf_step_stats* stats) {
     uint32_t n = 0;
     int rc = CF_OK;
     while (s->acc_time > fixed) {
-        rc = cf_batch2_fixed_step(s->b, &last);
+        rc = solid2_fixed(s, &last);
         if (rc != CF_OK && rc != CF_ERR_NAN) return rc;
         s->acc_time -= fixed;
         n++;
@@ -587,22 +853,50 @@ int cf_solid2_step(cf_solid2* s, float dt, cf_step_stats* stats) {
 }
 
 int cf_solid2_get_noc(cf_solid2* s, uint8_t* h_noc) {
-    if (!s) return CF_ERR_INVALID_ARG;
+    if (!s || !h_noc) return CF_ERR_INVALID_ARG;
+    if (s->large) {
+        CF_CHECK_HIP(hipStreamSynchronize(s->large->stream));
+        CF_CHECK_HIP(hipMemcpy(h_noc, s->large->d.noc, s->large->d.T, hipMemcpyDeviceToHost));
+        return CF_OK;
+    }
     return cf_batch2_get_noc(s->b, 0, h_noc);
 }
 
 int cf_solid2_fallbacks(cf_solid2* s, uint32_t* num_fallback) {
-    if (!s) return CF_ERR_INVALID_ARG;
+    if (!s || !num_fallback) return CF_ERR_INVALID_ARG;
+    if (s->large) {
+        *num_fallback = 0;
+        if (!s->large->stepped) return CF_OK;
+        Sim2Out o;
+        CF_CHECK_HIP(hipStreamSynchronize(s->large->stream));
+        CF_CHECK_HIP(hipMemcpy(&o, s->large->d.out, sizeof(Sim2Out), hipMemcpyDeviceToHost));
+        *num_fallback = o.num_fallback;
+        return CF_OK;
+    }
     return cf_batch2_fallbacks(s->b, 0, num_fallback);
 }
 
 int cf_solid2_get_state(const cf_solid2* s, float* pos, float* vel) {
     if (!s) return CF_ERR_INVALID_ARG;
+    if (s->large) {
+        const Large2Dev& a = s->large->d;
+        CF_CHECK_HIP(hipStreamSynchronize(s->large->stream));
+        if (pos) CF_CHECK_HIP(hipMemcpy(pos, a.x, a.N * sizeof(float2), hipMemcpyDeviceToHost));
+        if (vel) CF_CHECK_HIP(hipMemcpy(vel, a.v, a.N * sizeof(float2), hipMemcpyDeviceToHost));
+        return CF_OK;
+    }
     return cf_batch2_get_state(s->b, 0, pos, vel);
 }
 
 int cf_solid2_set_state(cf_solid2* s, const float* pos, const float* vel) {
     if (!s) return CF_ERR_INVALID_ARG;
+    if (s->large) {
+        const Large2Dev& a = s->large->d;
+        CF_CHECK_HIP(hipStreamSynchronize(s->large->stream));
+        if (pos) CF_CHECK_HIP(hipMemcpy(a.x, pos, a.N * sizeof(float2), hipMemcpyHostToDevice));
+        if (vel) CF_CHECK_HIP(hipMemcpy(a.v, vel, a.N * sizeof(float2), hipMemcpyHostToDevice));
+        return CF_OK;
+    }
     return cf_batch2_set_state(s->b, 0, pos, vel);
 }
 

@@ -479,7 +479,7 @@ static int cg_poll(cf_solid* s, CgRun* run) {
 // right-hand side, which the reference only asserts against,
 // TetrahedronElementSet3.cpp:253-255) stops the tests at once; it is
 // reported as CF_CG_NONFINITE.
-static void cg_result_from(const CgScalars& hs, uint32_t max_it, cf_cg_result* res) {
+void cf::cg_result_from(const CgScalars& hs, uint32_t max_it, cf_cg_result* res) {
     uint32_t j = 0;
     while (j <= max_it && !hs.done[j]) j++;
     if (j > max_it) j = max_it;

@@ -464,6 +464,46 @@ struct Batch2Args {
 };
 
 int launch_batch2_step(const Batch2Args& a, uint32_t n_work, hipStream_t s);
+
+// A 2D mesh beyond one workgroup (cf_solid2 above the resident kernel's
+// triangle, node or colour limits), stepped by multi-workgroup kernels on its
+// own arrays: triangles in the caller's order, per-(triangle, vertex) forces
+// tf summed per node over its incident entries in ascending triangle order
+// (the reference's serial scatter order), node vectors as float2.
+struct Large2Dev {
+    uint32_t N, T, nblk_tri, nblk_node;
+    Mesh2Dev c;               // constants (material, dt, CG settings); its N, T and offsets are unused
+    const float2* ref;
+    const float* mass;
+    const uint8_t* knc;
+    float2 *x, *v, *x0;       // x0: previous positions (DAPD)
+    const uint32_t* tri;      // 3 node ids per triangle
+    const float4* dinv;       // Dm^-1 row-major
+    const float* area;
+    uint8_t* noc;             // node of collapse per triangle (3 = none)
+    float4* R;                // the step's rotation per triangle
+    float2* tf;               // 3 vertex forces per triangle
+    const uint32_t* ptr;      // N + 1: node n's entries are code[ptr[n] .. ptr[n+1])
+    const uint32_t* code;     // 3 T entries 3 e + vertex, ascending e per node
+    float2 *b, *y, *r, *d, *q;
+    CgScalars* st;
+    double *part_a, *part_b;  // d.q and r.r partials, one per node block (nblk_node)
+    double* stat;             // 4 nblk_tri: degenerate count, degenerate det F sum, fallbacks, energy
+    float* stat_min;          // nblk_tri: minimum det F
+    Sim2Out* out;             // energy and the BeginEvaluation statistics of the step
+};
+int launch_l2_begin(const Large2Dev& a, hipStream_t s);
+int launch_l2_elem(const Large2Dev& a, bool fe, const float2* in, const int32_t* skip, hipStream_t s);
+int launch_l2_rhs(const Large2Dev& a, hipStream_t s);
+int launch_l2_matvec(const Large2Dev& a, bool init, const float2* in, const int32_t* skip, hipStream_t s);
+int launch_l2_xr(const Large2Dev& a, uint32_t k, hipStream_t s);
+int launch_l2_d(const Large2Dev& a, uint32_t k, int32_t* host_done, hipStream_t s);
+int launch_l2_integrate(const Large2Dev& a, hipStream_t s);
+int launch_l2_energy(const Large2Dev& a, hipStream_t s);
+int launch_l2_finish(const Large2Dev& a, hipStream_t s);
+// A solve's result from its scalar record (the iteration that stopped, its
+// reason and residuals; CF_CG_NONFINITE when a reduction was NaN or Inf).
+void cg_result_from(const CgScalars& hs, uint32_t max_it, cf_cg_result* res);
 int launch_rotations2(const float* F, const float* det, uint32_t n, int rm, float thr, float* R, uint32_t* fallback,
                       hipStream_t s);
 }  // namespace cf

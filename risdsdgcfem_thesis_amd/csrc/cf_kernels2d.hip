@@ -779,4 +779,360 @@ int launch_batch2_step(const Batch2Args& a, uint32_t n_work, hipStream_t s) {
     return CF_OK;
 }
 
+// ============================================================ large 2D meshes
+// A mesh beyond one workgroup, one thread per triangle or node per pass, with
+// the resident step's arithmetic: the same rotation, node-of-collapse,
+// gradient and force functions, and the same b / y / q / CG expressions. A
+// node's force is its incident (triangle, vertex) entries summed from 0 in
+// ascending triangle order. FP64 reductions: per block, then the blocks'
+// partials in the fixed one-block order.
+namespace {
+
+__device__ inline Tri l2_tri(const Large2Dev& a, uint32_t e, uint32_t& i0, uint32_t& i1, uint32_t& i2) {
+    i0 = a.tri[3 * (size_t)e]; i1 = a.tri[3 * (size_t)e + 1]; i2 = a.tri[3 * (size_t)e + 2];
+    const float4 g = a.dinv[e];
+    Tri t;
+    t.i01 = 0; t.i2 = 0;  // node ids are passed separately (meshes here may exceed 16-bit ids)
+    t.g00 = g.x; t.g01 = g.y; t.g10 = g.z; t.g11 = g.w;
+    t.A = a.area[e];
+    return t;
+}
+
+__device__ inline float2 l2_node_force(const Large2Dev& a, uint32_t n) {
+#pragma clang fp contract(off)
+    float2 f = make_float2(0.f, 0.f);
+    for (uint32_t c = a.ptr[n]; c < a.ptr[n + 1]; c++) {
+        const float2 t = a.tf[a.code[c]];
+        f.x += t.x; f.y += t.y;
+    }
+    return f;
+}
+
+__device__ inline float block_min(float v, float* sh) {
+    for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o, 64));
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+    __syncthreads();
+    float r = sh[0];
+    for (int i = 1; i < kBlock / 64; i++) r = fminf(r, sh[i]);
+    return r;
+}
+
+// BeginEvaluation: F, det F, node of collapse, R (as the resident step)
+__global__ __launch_bounds__(kBlock) void kl2_begin(Large2Dev a) {
+#pragma clang fp contract(off)
+    __shared__ double sh[kBlock / 64];
+    __shared__ float shm[kBlock / 64];
+    const Mesh2Dev& M = a.c;
+    const bool track = M.rm == CF_RM_DAPD;
+    double ndeg = 0.0, sdeg = 0.0, nfb = 0.0;
+    float mind = 3.0e38f;
+    for (uint32_t e = blockIdx.x * kBlock + threadIdx.x; e < a.T; e += gridDim.x * kBlock) {
+        uint32_t i0, i1, i2;
+        const Tri t = l2_tri(a, e, i0, i1, i2);
+        const float2 p0 = a.x[i0], p1 = a.x[i1], p2 = a.x[i2];
+        const float D00 = p1.x - p0.x, D01 = p2.x - p0.x, D10 = p1.y - p0.y, D11 = p2.y - p0.y;
+        float f00 = 0.0f; f00 += D00 * t.g00; f00 += D01 * t.g10;
+        float f01 = 0.0f; f01 += D00 * t.g01; f01 += D01 * t.g11;
+        float f10 = 0.0f; f10 += D10 * t.g00; f10 += D11 * t.g10;
+        float f11 = 0.0f; f11 += D10 * t.g01; f11 += D11 * t.g11;
+        const float det = f00 * f11 - f10 * f01;
+        uint32_t nc = kNocInvalid;
+        if (track) {
+            const uint32_t old = a.noc[e];
+            nc = (det > M.thr) ? kNocInvalid
+                               : update_noc2(a.x0[i0], a.x0[i1], a.x0[i2], p0, p1, p2, t.g00, t.g01, t.g10, t.g11,
+                                             t.A, M.thr, old);
+            if (nc != old) a.noc[e] = uint8_t(nc);
+        }
+        if (track && !(det > M.thr) && nc != kNocInvalid) {
+            a.R[e] = r_dapd_degenerate2(p0, p1, p2, nc, t.A, t.g00, t.g01, t.g10, t.g11, M.thr, M.dapd_l, M.dapd_nl,
+                                        M.dapd_e);
+        } else {
+            Rot rt;
+            nfb += rotation2(M.rm, M.thr, f00, f01, f10, f11, det, rt);
+            a.R[e] = make_float4(rt.r00, rt.r01, rt.r10, rt.r11);
+        }
+        if (det <= 0.f) { ndeg += 1.0; sdeg += det; }
+        mind = fminf(mind, det);
+    }
+    const double s0 = block_sum(ndeg, sh);
+    const double s1 = block_sum(sdeg, sh);
+    const double s2 = block_sum(nfb, sh);
+    const float mn = block_min(mind, shm);
+    if (threadIdx.x == 0) {
+        a.stat[blockIdx.x] = s0;
+        a.stat[a.nblk_tri + blockIdx.x] = s1;
+        a.stat[2 * a.nblk_tri + blockIdx.x] = s2;
+        a.stat_min[blockIdx.x] = mn;
+    }
+}
+
+// Triangle forces on the vectors `in` (FE: f_e on positions, with the rest
+// positions) into tf
+template <bool FE>
+__global__ __launch_bounds__(kBlock) void kl2_elem(Large2Dev a, const float2* __restrict__ in, const int32_t* skip) {
+    if (skip && *skip) return;
+    for (uint32_t e = blockIdx.x * kBlock + threadIdx.x; e < a.T; e += gridDim.x * kBlock) {
+        uint32_t i0, i1, i2;
+        const Tri t = l2_tri(a, e, i0, i1, i2);
+        const Rot R = rot_of(a.R[e]);
+        float H[4];
+        const float2 z = make_float2(0.f, 0.f);
+        if (FE) tri_gradient<true>(t, R, in[i0], in[i1], in[i2], a.ref[i0], a.ref[i1], a.ref[i2], H);
+        else tri_gradient<false>(t, R, in[i0], in[i1], in[i2], z, z, z, H);
+        float2 f0, f1, f2;
+        tri_forces(t, R, H, a.c.m, f0, f1, f2);
+        a.tf[3 * (size_t)e] = f0;
+        a.tf[3 * (size_t)e + 1] = f1;
+        a.tf[3 * (size_t)e + 2] = f2;
+    }
+}
+
+// b = dt (M g + f_e) + M v ; y = v (warm start) ; KNC rows take the
+// constraint velocity (Step_QuasiImplicitEuler_v, LeafDSH_Solid2D_FEM.cpp:1287-1403)
+__global__ __launch_bounds__(kBlock) void kl2_rhs(Large2Dev a) {
+#pragma clang fp contract(off)
+    const Mesh2Dev& M = a.c;
+    for (uint32_t n = blockIdx.x * kBlock + threadIdx.x; n < a.N; n += gridDim.x * kBlock) {
+        const float2 f = l2_node_force(a, n), vv = a.v[n];
+        const float m = a.mass[n];
+        float2 b;
+        b.x = (m * M.g0 + f.x) * M.dt + m * vv.x;
+        b.y = (m * M.g1 + f.y) * M.dt + m * vv.y;
+        float2 yk = M.warm_start ? vv : make_float2(0.f, 0.f);
+        if (a.knc[n]) {
+            const float2 q = a.ref[n], xx = a.x[n];
+            float vx = (q.x - xx.x) * M.inv_dt, vy = (q.y - xx.y) * M.inv_dt;
+            const float nk = sqrtf(vx * vx + vy * vy);
+            if (nk > 0.001f) {
+                const float inv = 1.0f / nk, sc = fminf(nk, 50.0f);
+                vx = sc * (vx * inv); vy = sc * (vy * inv);
+            } else {
+                vx = 0.f; vy = 0.f;
+            }
+            b = make_float2(vx, vy);
+            yk = b;
+        }
+        a.b[n] = b;
+        a.y[n] = yk;
+    }
+}
+
+// q = S(fM M d - fK f) with per-block d.q; INIT: the CG start r = S(b - q),
+// d = r with per-block r.r instead (q not stored)
+template <bool INIT>
+__global__ __launch_bounds__(kBlock) void kl2_matvec(Large2Dev a, const float2* __restrict__ in, const int32_t* skip) {
+#pragma clang fp contract(off)
+    __shared__ double sh[kBlock / 64];
+    if (skip && *skip) return;
+    const Mesh2Dev& M = a.c;
+    double acc = 0.0;
+    for (uint32_t n = blockIdx.x * kBlock + threadIdx.x; n < a.N; n += gridDim.x * kBlock) {
+        const float2 f = l2_node_force(a, n), dk = in[n];
+        const float fac = a.mass[n] * M.fM;
+        const bool kn = a.knc[n] != 0;
+        float q0 = -M.fK * f.x + fac * dk.x, q1 = -M.fK * f.y + fac * dk.y;
+        if (kn) { q0 = 0.f; q1 = 0.f; }
+        if (INIT) {
+            const float2 bb = a.b[n];
+            float r0 = -1.0f * q0 + bb.x, r1 = -1.0f * q1 + bb.y;
+            if (kn) { r0 = 0.f; r1 = 0.f; }
+            a.r[n] = make_float2(r0, r1);
+            a.d[n] = make_float2(r0, r1);
+            acc += (double)r0 * r0 + (double)r1 * r1;
+        } else {
+            a.q[n] = make_float2(q0, q1);
+            acc += (double)dk.x * q0 + (double)dk.y * q1;
+        }
+    }
+    const double s = block_sum(acc, sh);
+    if (threadIdx.x == 0) (INIT ? a.part_b : a.part_a)[blockIdx.x] = s;
+}
+
+// alpha = delta_k / d.q (break down if |d.q| < abs^2) ; y += alpha d ;
+// r -= alpha q ; per-block r.r
+__global__ __launch_bounds__(kBlock) void kl2_xr(Large2Dev a, uint32_t k) {
+#pragma clang fp contract(off)
+    __shared__ double sh[kBlock / 64];
+    CgScalars* st = a.st;
+    if (st->done[k]) return;
+    const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+    const double dq = reduce_partials(a.part_a, (int)a.nblk_node, sh);
+    if (fabs(dq) < st->abs_sq) {
+        if (lead) {
+            st->dq[k] = dq;
+            st->reason[k + 1] = CF_CG_BREAKDOWN;
+            st->done[k + 1] = 2;
+        }
+        return;
+    }
+    const float alpha = (float)(st->delta[k] / dq);
+    if (lead) {
+        st->dq[k] = dq;
+        st->alpha[k] = alpha;
+    }
+    double acc = 0.0;
+    for (uint32_t n = blockIdx.x * kBlock + threadIdx.x; n < a.N; n += gridDim.x * kBlock) {
+        const float2 dk = a.d[n], qk = a.q[n];
+        float2 yk = a.y[n], rk = a.r[n];
+        yk.x = yk.x + dk.x * alpha; yk.y = yk.y + dk.y * alpha;
+        rk.x = rk.x - qk.x * alpha; rk.y = rk.y - qk.y * alpha;
+        a.y[n] = yk;
+        a.r[n] = rk;
+        acc += (double)rk.x * rk.x + (double)rk.y * rk.y;
+    }
+    const double s = block_sum(acc, sh);
+    if (threadIdx.x == 0) a.part_b[blockIdx.x] = s;
+}
+
+// delta_{k+1} = r.r ; beta ; stop test ; d = S(beta d + r). host_done, when
+// not null, receives done[k + 1].
+__global__ __launch_bounds__(kBlock) void kl2_d(Large2Dev a, uint32_t k, int32_t* host_done) {
+#pragma clang fp contract(off)
+    __shared__ double sh[kBlock / 64];
+    CgScalars* st = a.st;
+    const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+    if (st->done[k]) {
+        if (lead) {
+            st->done[k + 1] = 1;
+            if (host_done) host_done[k + 1] = 1;
+        }
+        return;
+    }
+    if (st->reason[k + 1] == CF_CG_BREAKDOWN) {
+        if (lead && host_done) host_done[k + 1] = 2;
+        return;
+    }
+    const double dn = reduce_partials(a.part_b, (int)a.nblk_node, sh);
+    const uint32_t k1 = k + 1;
+    int reason = 0;
+    if (!(dn > st->abs_sq)) reason = CF_CG_CONVERGED_ABS;
+    else if (!(dn > st->target)) reason = CF_CG_CONVERGED_REL;
+    else if (!(k1 < st->max_iter)) reason = CF_CG_MAX_ITER;
+    const float beta = (float)(dn / st->delta[k]);
+    if (lead) {
+        st->delta[k1] = dn;
+        st->beta[k] = beta;
+        st->reason[k1] = reason;
+        st->done[k1] = reason != 0;
+        if (host_done) host_done[k1] = reason != 0;
+    }
+    if (reason) return;
+    for (uint32_t n = blockIdx.x * kBlock + threadIdx.x; n < a.N; n += gridDim.x * kBlock) {
+        const float2 dk = a.d[n], rk = a.r[n];
+        float d0 = beta * dk.x + rk.x, d1 = beta * dk.y + rk.y;
+        if (a.knc[n]) { d0 = 0.f; d1 = 0.f; }
+        a.d[n] = make_float2(d0, d1);
+    }
+}
+
+// v = y, x = x + dt y (LeafDSH_Solid2D_FEM.cpp:1394-1398); DAPD keeps the
+// previous positions (m_vecPos0)
+__global__ __launch_bounds__(kBlock) void kl2_integrate(Large2Dev a) {
+#pragma clang fp contract(off)
+    const Mesh2Dev& M = a.c;
+    const bool track = M.rm == CF_RM_DAPD;
+    for (uint32_t n = blockIdx.x * kBlock + threadIdx.x; n < a.N; n += gridDim.x * kBlock) {
+        const float2 yy = a.y[n], xx = a.x[n];
+        if (track) a.x0[n] = xx;
+        a.x[n] = make_float2(xx.x + M.dt * yy.x, xx.y + M.dt * yy.y);
+        a.v[n] = (M.drag > 0.f) ? make_float2((1.0f - M.dt * M.drag) * yy.x, (1.0f - M.dt * M.drag) * yy.y) : yy;
+    }
+}
+
+// C_V on the new positions with the step's rotations, per-block sums
+__global__ __launch_bounds__(kBlock) void kl2_energy(Large2Dev a) {
+#pragma clang fp contract(off)
+    __shared__ double sh[kBlock / 64];
+    const Mesh2Dev& M = a.c;
+    double en = 0.0;
+    for (uint32_t e = blockIdx.x * kBlock + threadIdx.x; e < a.T; e += gridDim.x * kBlock) {
+        uint32_t i0, i1, i2;
+        const Tri t = l2_tri(a, e, i0, i1, i2);
+        const Rot R = rot_of(a.R[e]);
+        float H[4];
+        tri_gradient<true>(t, R, a.x[i0], a.x[i1], a.x[i2], a.ref[i0], a.ref[i1], a.ref[i2], H);
+        const float sxx = fmaf(M.m.a_k1, H[0], M.m.a_nu * H[3]);
+        const float syy = fmaf(M.m.a_k1, H[3], M.m.a_nu * H[0]);
+        const float sxy = M.m.a_k2 * (H[1] + H[2]);
+        const float sH = sxx * H[0] + syy * H[3] + sxy * (H[1] + H[2]);
+        en += 0.5 * (double)t.A * (double)sH;
+    }
+    const double s = block_sum(en, sh);
+    if (threadIdx.x == 0) a.stat[3 * a.nblk_tri + blockIdx.x] = s;
+}
+
+// The step's BeginEvaluation statistics and energy (one workgroup)
+__global__ __launch_bounds__(kBlock) void kl2_finish(Large2Dev a, int with_energy) {
+    __shared__ double sh[kBlock / 64];
+    __shared__ float shm[kBlock / 64];
+    const int nb = (int)a.nblk_tri;
+    const double nd = reduce_partials(a.stat, nb, sh);
+    const double sd = reduce_partials(a.stat + nb, nb, sh);
+    const double nf = reduce_partials(a.stat + 2 * nb, nb, sh);
+    const double en = with_energy ? reduce_partials(a.stat + 3 * nb, nb, sh) : 0.0;
+    float mn = 3.0e38f;
+    for (int i = threadIdx.x; i < nb; i += kBlock) mn = fminf(mn, a.stat_min[i]);
+    mn = block_min(mn, shm);
+    if (threadIdx.x == 0) {
+        Sim2Out& o = *a.out;
+        o.energy = en;
+        o.sum_degenerate = sd;
+        o.num_degenerate = uint32_t(nd);
+        o.num_fallback = uint32_t(nf);
+        o.min_det = mn;
+    }
+}
+
+int check_l2() {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        set_error("large 2D kernel launch failed: %s", hipGetErrorString(e));
+        return CF_ERR_HIP;
+    }
+    return CF_OK;
+}
+
+}  // namespace
+
+int launch_l2_begin(const Large2Dev& a, hipStream_t s) {
+    hipLaunchKernelGGL(kl2_begin, dim3(a.nblk_tri), dim3(kBlock), 0, s, a);
+    return check_l2();
+}
+int launch_l2_elem(const Large2Dev& a, bool fe, const float2* in, const int32_t* skip, hipStream_t s) {
+    if (fe) hipLaunchKernelGGL(kl2_elem<true>, dim3(a.nblk_tri), dim3(kBlock), 0, s, a, in, skip);
+    else hipLaunchKernelGGL(kl2_elem<false>, dim3(a.nblk_tri), dim3(kBlock), 0, s, a, in, skip);
+    return check_l2();
+}
+int launch_l2_rhs(const Large2Dev& a, hipStream_t s) {
+    hipLaunchKernelGGL(kl2_rhs, dim3(a.nblk_node), dim3(kBlock), 0, s, a);
+    return check_l2();
+}
+int launch_l2_matvec(const Large2Dev& a, bool init, const float2* in, const int32_t* skip, hipStream_t s) {
+    if (init) hipLaunchKernelGGL(kl2_matvec<true>, dim3(a.nblk_node), dim3(kBlock), 0, s, a, in, skip);
+    else hipLaunchKernelGGL(kl2_matvec<false>, dim3(a.nblk_node), dim3(kBlock), 0, s, a, in, skip);
+    return check_l2();
+}
+int launch_l2_xr(const Large2Dev& a, uint32_t k, hipStream_t s) {
+    hipLaunchKernelGGL(kl2_xr, dim3(a.nblk_node), dim3(kBlock), 0, s, a, k);
+    return check_l2();
+}
+int launch_l2_d(const Large2Dev& a, uint32_t k, int32_t* host_done, hipStream_t s) {
+    hipLaunchKernelGGL(kl2_d, dim3(a.nblk_node), dim3(kBlock), 0, s, a, k, host_done);
+    return check_l2();
+}
+int launch_l2_integrate(const Large2Dev& a, hipStream_t s) {
+    hipLaunchKernelGGL(kl2_integrate, dim3(a.nblk_node), dim3(kBlock), 0, s, a);
+    return check_l2();
+}
+int launch_l2_energy(const Large2Dev& a, hipStream_t s) {
+    hipLaunchKernelGGL(kl2_energy, dim3(a.nblk_tri), dim3(kBlock), 0, s, a);
+    return check_l2();
+}
+int launch_l2_finish(const Large2Dev& a, hipStream_t s) {
+    hipLaunchKernelGGL(kl2_finish, dim3(1), dim3(kBlock), 0, s, a, (int)a.c.compute_energy);
+    return check_l2();
+}
+
 }  // namespace cf

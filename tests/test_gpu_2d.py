@@ -1,5 +1,5 @@
-"""GPU: the 2D twin (workgroup-resident step kernel) against the oracle, and
-batched against alone.
+"""GPU: the 2D twin (workgroup-resident step kernel; the multi-workgroup path
+for meshes beyond one workgroup) against the oracle, and batched against alone.
 
 Tolerances: lumped masses bit-exact (same FP32 host arithmetic); the GPU
 applies Ke in the equivalent Piola form, so steps agree to rounding
@@ -200,11 +200,92 @@ def test_msvd_and_dapd_differ_on_degenerate_elements(gpu):
     assert np.abs(out[0] - out[1]).max() > 1e-3 * np.abs(out[1] - x0).max()
 
 
-def test_too_large_mesh_is_refused(gpu):
+def test_batch_refuses_mesh_beyond_one_workgroup(gpu):
+    """Batches hold meshes that fit the resident kernel; a larger one is a
+    single solid's (the multi-workgroup path below)."""
     pos, tris, knc = grid(gpu, 40)
+    b = gpu.Batch2()
     with pytest.raises(gpu.CorofemError) as e:
-        gpu.LeafSolid2D(pos, tris, params(gpu), knc=knc)
+        b.add_sim(pos, tris, params(gpu), knc=knc)
     assert "status -6" in str(e.value)
+    b.close()
+
+
+@pytest.mark.parametrize("n,E,nu,rm", [(40, 1e5, 0.3, "DAPD"), (64, 3e5, 0.4, "MSVD"), (90, 2e6, 0.25, "DAPD")])
+def test_large_mesh_steps_vs_oracle(gpu, orc, n, E, nu, rm):
+    """Meshes beyond one workgroup (3,200 / 8,192 / 16,200 triangles) take the
+    multi-workgroup path, whose nodal forces sum in the reference's serial
+    order: 3 steps against the oracle with the tolerances of the resident
+    path (positions 1e-4 relative per step, CG iterations +-1, energy 1e-3)."""
+    pos, tris, knc = grid(gpu, n)
+    assert tris.size // 3 > 2048
+    s = gpu.LeafSolid2D(pos, tris, params(gpu, E, nu, rm=getattr(gpu, "RM_" + rm)), knc=knc)
+    o = orc.Solid2(pos, tris, oracle_params(orc, E, nu, rm=getattr(orc, "RM_" + rm)), knc=knc)
+    for step in range(3):
+        sg, so = s.fixed_step(), o.fixed_step()
+        assert abs(sg.cg_iters - so.cg_iters) <= 1, (step, sg.cg_iters, so.cg_iters)
+        xg, vg = s.state()
+        xo, vo = o.state()
+        err = rel_node_err(np.c_[xg.reshape(-1, 2), np.zeros(xg.size // 2)],
+                           np.c_[xo.reshape(-1, 2), np.zeros(xo.size // 2)])
+        print(f"\nn={n} step {step}: CG {sg.cg_iters}/{so.cg_iters}, rel node err {err:.2e}")
+        assert err <= 1e-4 * (step + 1), (step, err)
+        assert abs(sg.elastic_energy - so.elastic_energy) <= 1e-3 * abs(so.elastic_energy) + 1e-12
+        assert sg.num_degenerate == so.num_degenerate
+    assert np.abs(vg.reshape(-1, 2)[knc]).max() == 0
+    s.close()
+
+
+@pytest.mark.parametrize("rm", ["MSVD", "DAPD"])
+def test_large_mesh_degenerate_step_vs_oracle(gpu, orc, rm):
+    """The squashed and partly inverted state on a 40x40 grid (3,200
+    triangles): degenerate counts, nodes of collapse and fallbacks equal to
+    the oracle's, positions within 1e-4 of the displacement (converged CG)."""
+    pos, tris, knc = grid(gpu, 40)
+    x0 = _squashed_state(pos)
+    s = gpu.LeafSolid2D(pos, tris, params(gpu, max_iter=1000, rel=1e-6, rm=getattr(gpu, "RM_" + rm)), knc=knc)
+    o = orc.Solid2(pos, tris, oracle_params(orc, max_iter=1000, rel=1e-6, rm=getattr(orc, "RM_" + rm)), knc=knc)
+    v0 = np.zeros_like(x0)
+    s.set_state(x0, v0)
+    o.set_state(x0, v0)
+    for step in range(2):
+        sg, so = s.fixed_step(), o.fixed_step()
+        assert sg.num_degenerate == so.num_degenerate
+        if step == 0:
+            assert so.num_degenerate > 0
+        if rm == "DAPD":
+            assert np.array_equal(s.noc(), o.noc())
+            assert s.num_fallback() == o.num_fallback()
+        else:
+            assert s.num_fallback() == 0
+    xg, _ = s.state()
+    xo, _ = o.state()
+    d = np.abs(xg - xo).max() / np.abs(xo - x0).max()
+    print(f"\n{rm}: {so.num_degenerate} degenerate, {s.num_fallback()} fallbacks, displacement error {d:.2e}")
+    assert d <= 1e-4, d
+    s.close()
+
+
+def test_large_path_switches_at_the_resident_limits(gpu, orc):
+    """The largest grid that fits one workgroup (2,048 triangles at 32x32)
+    and the next (33x33, 2,178 triangles, multi-workgroup) both follow the
+    oracle; the Step accumulator and set_state work on the large path."""
+    for n in (32, 33):
+        pos, tris, knc = grid(gpu, n)
+        s = gpu.LeafSolid2D(pos, tris, params(gpu), knc=knc)
+        o = orc.Solid2(pos, tris, oracle_params(orc), knc=knc)
+        st = s.step(2.5 / 60.0)  # two fixed steps, the half carried over
+        o.fixed_step()
+        o.fixed_step()
+        assert st.num_fixed_steps == 2
+        xg, _ = s.state()
+        xo, _ = o.state()
+        err = rel_node_err(np.c_[xg.reshape(-1, 2), np.zeros(xg.size // 2)],
+                           np.c_[xo.reshape(-1, 2), np.zeros(xo.size // 2)])
+        assert err <= 2e-4, (n, err)
+        s.set_state(pos, np.zeros_like(pos))
+        assert np.array_equal(s.state()[0], pos) and not s.state()[1].any()
+        s.close()
 
 
 def _pushed_metre_grid(cf):
