@@ -500,10 +500,14 @@ int cf_batch_synchronize(cf_batch* b);
  * kept per triangle and updated from the previous positions as
  * UpdateDegeneration (:583-640) does; without a NoC R = Id (:1100-1105), counted
  * in num_fallback. Previous positions start at the rest pose; set_state leaves them.
- * Each mesh is stepped inside one workgroup by a group of 1, 2, 4 or 8
- * wavefronts (up to 2048 triangles and 1536 nodes; CF_ERR_UNSUPPORTED beyond).
- * A batch steps all its simulations in ONE launch; each simulation's result is
- * independent of the others in the batch. */
+ * A batch steps each mesh inside one workgroup by a group of 1, 2, 4 or 8
+ * wavefronts (up to 2048 triangles, 1536 nodes and 16 colours of a greedy
+ * triangle colouring; cf_batch2_add_sim returns CF_ERR_UNSUPPORTED beyond),
+ * all its simulations in ONE launch; each simulation's result is independent
+ * of the others in the batch. cf_solid2 takes any size: a mesh beyond one
+ * workgroup is stepped by multi-workgroup kernels (one thread per triangle or
+ * node per pass, nodal forces summed in the reference's serial triangle order,
+ * the CG on the device with the 3D leaf's stop flags). */
 typedef struct cf_batch2 cf_batch2;
 /* Rotations of n triangles from their deformation gradients F (4 floats each,
    row-major) and det F, as the 2D step computes them: ms/fem/Rotation.cpp:19-37
@@ -535,7 +539,8 @@ int cf_batch2_get_noc(cf_batch2* b, uint32_t sim_id, uint8_t* h_noc);
 int cf_batch2_info(const cf_batch2* b, uint32_t* num_sims, uint64_t* tris, uint64_t* nodes, uint32_t* workgroups);
 
 /* LeafDSH_Solid2D_FEM drop-in (Create / Step / FixedStep / QueryState): one
-   simulation, the same kernel launched for this mesh alone. */
+   simulation; the batched kernel launched for this mesh alone when it fits
+   one workgroup, the multi-workgroup kernels otherwise. */
 typedef struct cf_solid2 cf_solid2;
 int cf_solid2_create(uint32_t n_nodes, const float* h_ref_pos2, uint32_t n_tris, const uint32_t* h_tris,
                      const cf_solid_params* params, uint32_t n_knc, const uint32_t* h_knc_nids, cf_solid2** out);
