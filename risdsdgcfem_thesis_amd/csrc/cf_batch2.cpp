@@ -45,6 +45,7 @@ struct cf_batch2 {
     float2* d_x0 = nullptr;
     uint8_t* d_noc = nullptr;
     Work2* d_work = nullptr;
+    std::vector<int32_t> last_iters;  // each mesh's CG iterations in the last step (empty before the first)
     Sim2Out* d_out = nullptr;
     std::vector<Sim2Out> h_out;
     int32_t* d_flag = nullptr;  // a mesh's CG met a NaN or Inf
@@ -85,18 +86,35 @@ int pull_state(cf_batch2* b) {
     return CF_OK;
 }
 
-// Work list: meshes of one group size share workgroups (8/gw meshes each),
-// largest first so that the tail is short.
+// Work list: meshes of one group size share workgroups (8/gw meshes each).
+// A workgroup holds its LDS until its slowest mesh's CG has stopped, so once a
+// step has run, meshes are grouped by the iteration count of their last step
+// (most first; the counts change little from one step to the next), then by
+// size; before that by size alone. Largest first, so that the tail is short.
+// A mesh's result does not depend on its group.
 void build_work(cf_batch2* b) {
     std::vector<uint32_t> cls[4];
     auto slot = [](uint32_t gw) { return gw == 1 ? 0 : gw == 2 ? 1 : gw == 4 ? 2 : 3; };
     for (uint32_t i = 0; i < b->meshes.size(); i++) cls[slot(b->meshes[i].gw)].push_back(i);
+    const bool by_iters = b->last_iters.size() == b->meshes.size();
     auto by_size = [&](uint32_t p, uint32_t q) {
         return b->meshes[p].T != b->meshes[q].T ? b->meshes[p].T > b->meshes[q].T : p < q;
     };
+    // counting sort on the iteration count (most first), stable: O(meshes)
+    std::vector<uint32_t> count, sorted;
+    auto sort_by_iters = [&](std::vector<uint32_t>& v) {
+        count.assign(kMaxCgIter + 2, 0);
+        for (uint32_t i : v) count[kMaxCgIter + 1 - std::min<uint32_t>(uint32_t(b->last_iters[i]), kMaxCgIter + 1)]++;
+        uint32_t acc = 0;
+        for (uint32_t& c : count) { const uint32_t t = c; c = acc; acc += t; }
+        sorted.resize(v.size());
+        for (uint32_t i : v) sorted[count[kMaxCgIter + 1 - std::min<uint32_t>(uint32_t(b->last_iters[i]), kMaxCgIter + 1)]++] = i;
+        v.swap(sorted);
+    };
     b->work.clear();
     for (int c = 3; c >= 0; c--) {
-        std::stable_sort(cls[c].begin(), cls[c].end(), by_size);
+        if (by_iters) sort_by_iters(cls[c]);
+        else std::stable_sort(cls[c].begin(), cls[c].end(), by_size);
         const uint32_t gw = 1u << c, per = 8 / gw;
         for (size_t i = 0; i < cls[c].size(); i += per) {
             Work2 w{};
@@ -472,10 +490,16 @@ int cf_batch2_fixed_step(cf_batch2* b, cf_step_stats* stats) {
     CF_CHECK_HIP(hipEventSynchronize(b->ev1));
     int32_t nonfinite = 0;
     CF_CHECK_HIP(hipMemcpy(&nonfinite, b->d_flag, sizeof(int32_t), hipMemcpyDeviceToHost));
+    CF_CHECK_HIP(hipMemcpy(b->h_out.data(), b->d_out, b->h_out.size() * sizeof(Sim2Out), hipMemcpyDeviceToHost));
+    // the next step's workgroups group meshes of similar iteration counts
+    b->last_iters.resize(b->meshes.size());
+    for (size_t i = 0; i < b->h_out.size(); i++) b->last_iters[i] = b->h_out[i].iters;
+    build_work(b);
+    CF_CHECK_HIP(hipMemcpyAsync(b->d_work, b->work.data(), b->work.size() * sizeof(Work2), hipMemcpyHostToDevice,
+                                b->stream));
     if (stats) {
         float ms = 0.f;
         CF_CHECK_HIP(hipEventElapsedTime(&ms, b->ev0, b->ev1));
-        CF_CHECK_HIP(hipMemcpy(b->h_out.data(), b->d_out, b->h_out.size() * sizeof(Sim2Out), hipMemcpyDeviceToHost));
         for (size_t i = 0; i < b->h_out.size(); i++) {
             const Sim2Out& o = b->h_out[i];
             cf_step_stats& s = stats[i];
