@@ -196,7 +196,7 @@ def bench_batch3d(cf, args):
         log(f"[batch3d] autotuned in {tuned['seconds']} s: tiles {tuned['tile_threads']}/{tuned['tile_elems']} "
             f"(ns/tet {tuned['kx_ns_per_tet']}), chunk {tuned['cg_chunk']} (step ms {tuned['step_ms']})")
     b = cf.Batch(tile_threads=tuned["tile_threads"], tile_elems=tuned["tile_elems"], cg_chunk=tuned["cg_chunk"],
-                 use_graphs=not args.no_graphs)
+                 use_graphs=not args.no_graphs, step_mode=args.step_mode)
     mids = b.add_meshes([(m["pos"], m["tets"]) for m in meshes], threads=16)
     log(f"[batch3d] {len(meshes)} meshes, {T} tets, {N} nodes; setup {time.perf_counter() - t0:.1f} s")
     b_alg = 92 * T + 24 * N
@@ -206,7 +206,8 @@ def bench_batch3d(cf, args):
         "meshes": len(meshes), "tets": T, "nodes": N, "steps": args.batch_steps, "warmup": args.warmup,
         "cg_max_iter": args.batch_max_iter,
         "tile": {"threads": tuned["tile_threads"], "elems": tuned["tile_elems"], "cg_chunk": tuned["cg_chunk"],
-                 "graphs": not args.no_graphs},
+                 "graphs": not args.no_graphs,
+                 "step_mode": {0: "host", 1: "device"}[args.step_mode]},
         "autotune": None if args.no_autotune else {k: tuned[k] for k in ("kx_ns_per_tet", "step_ms", "sample",
                                                                           "seconds")},
         "variants": {},
@@ -404,6 +405,9 @@ def main():
                     help="CG iteration cap of the batched and served workloads (high enough that every solve "
                          "reaches the 1e-4 residual)")
     ap.add_argument("--cg-chunk", type=int, default=8)
+    ap.add_argument("--step-mode", type=int, default=1, choices=[0, 1],
+                    help="batched 3D: 0 host-polled chunks, 1 the step submitted at once with device-compacted "
+                         "lists (default)")
     ap.add_argument("--tile-alternate", type=int, default=1, choices=[0, 1],
                     help="single mesh: 0 every CG iteration walks the tiles forwards; 1 odd iterations walk them "
                          "backwards, starting on the tiles the previous K*x touched last")

@@ -435,7 +435,21 @@ typedef struct cf_batch_info {
     uint64_t budget_bytes;
     uint64_t active_tets, active_nodes;
     uint32_t tile_threads, tile_elems, cg_chunk, use_graphs;
+    /* step schedule (CF_BATCH_STEP_*); under the device schedule, the chunks
+       the next step is submitted with, the chunks the last step added after
+       it ran, and the chunk graphs recorded (one per grid class) */
+    uint32_t step_mode, next_chunks, last_continuations, chunk_graphs;
 } cf_batch_info;
+
+/* How cf_batch_fixed_step runs its CG chunks.
+   HOST: the host reads the iterating count after every chunk (one chunk
+   behind) and, once at most half of the simulations still iterate, compacts
+   the tables and launches the rest directly.
+   DEVICE (default): the whole step is submitted at once; before every chunk
+   the device compacts the lists of the simulations still iterating, and the
+   chunk's kernels (grids sized from the previous step's list lengths) walk
+   them; the host reads nothing until the step has run. */
+enum { CF_BATCH_STEP_HOST = 0, CF_BATCH_STEP_DEVICE = 1 };
 
 int cf_batch_create(uint64_t hbm_budget_bytes /* 0: no limit */, cf_batch** out);
 int cf_batch_destroy(cf_batch* b);
@@ -443,6 +457,8 @@ int cf_batch_destroy(cf_batch* b);
 int cf_batch_set_tiles(cf_batch* b, uint32_t tile_threads, uint32_t tile_elems);
 /* CG iterations per recorded chunk (1..1024; default 8) and graph use (default 1). */
 int cf_batch_set_schedule(cf_batch* b, uint32_t cg_chunk, int use_graphs);
+/* CF_BATCH_STEP_* (results are bit-identical under both). */
+int cf_batch_set_step_mode(cf_batch* b, int mode);
 int cf_batch_add_mesh(cf_batch* b, uint32_t n_nodes, const float* h_ref_pos, uint32_t n_tets,
                       const uint32_t* h_tets, uint32_t* mesh_id);
 /* count meshes at once, tiles built on up to `threads` host threads (0: hardware). */

@@ -9,6 +9,7 @@ fails loudly when the library has not been built.
 from ._capi import (  # noqa: F401
     lib, LIB_PATH, CorofemError, check, FemParams, SolidParams, StepStats, CgResult, BatchInfo,
     RM_ID, RM_QR, RM_MSVD, RM_DAPD, MM_C_WRP, MM_C_LCM, MM_C_LCMH, DM_TRUNCATED, DM_EXACT, IT_QIE_V, IT_FIE_DX,
+    BATCH_STEP_HOST, BATCH_STEP_DEVICE,
     CG_CONVERGED_REL, CG_CONVERGED_ABS, CG_MAX_ITER, CG_BREAKDOWN, CG_NONFINITE,
     ERR_INVALID_ARG, ERR_HIP, ERR_OUT_OF_MEMORY, ERR_NAN, ERR_UNSUPPORTED, ERR_NO_DEVICE, ERR_CAPACITY,
 )

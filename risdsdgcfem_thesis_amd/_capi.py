@@ -10,6 +10,7 @@ RM_ID, RM_QR, RM_MSVD, RM_DAPD = 0, 1, 2, 3
 MM_C_WRP, MM_C_LCM, MM_C_LCMH = 1, 2, 3
 DM_TRUNCATED, DM_EXACT = 0, 1
 IT_QIE_V, IT_FIE_DX = 1, 3
+BATCH_STEP_HOST, BATCH_STEP_DEVICE = 0, 1
 CG_CONVERGED_REL, CG_CONVERGED_ABS, CG_MAX_ITER, CG_BREAKDOWN, CG_NONFINITE = 1, 2, 3, 4, 5
 CF_CG_STATIC_MAX_N = 1 << 22  # unknowns of cf_cg_solve_static's static scratch (include/corofem.h)
 ERR_INVALID_ARG, ERR_HIP, ERR_OUT_OF_MEMORY, ERR_NAN, ERR_UNSUPPORTED, ERR_NO_DEVICE, ERR_CAPACITY = (
@@ -62,7 +63,8 @@ class BatchInfo(C.Structure):
         ("bytes_in_use", C.c_uint64), ("bytes_reserved", C.c_uint64), ("budget_bytes", C.c_uint64),
         ("active_tets", C.c_uint64), ("active_nodes", C.c_uint64),
         ("tile_threads", C.c_uint32), ("tile_elems", C.c_uint32), ("cg_chunk", C.c_uint32),
-        ("use_graphs", C.c_uint32),
+        ("use_graphs", C.c_uint32), ("step_mode", C.c_uint32), ("next_chunks", C.c_uint32),
+        ("last_continuations", C.c_uint32), ("chunk_graphs", C.c_uint32),
     ]
 
 
@@ -184,6 +186,7 @@ SIGNATURES = {
     "cf_batch_destroy": (C.c_int, [_vp]),
     "cf_batch_set_tiles": (C.c_int, [_vp, C.c_uint32, C.c_uint32]),
     "cf_batch_set_schedule": (C.c_int, [_vp, C.c_uint32, C.c_int]),
+    "cf_batch_set_step_mode": (C.c_int, [_vp, C.c_int]),
     "cf_batch_add_mesh": (C.c_int, [_vp, C.c_uint32, _f, C.c_uint32, _u, _u]),
     "cf_batch_add_meshes": (C.c_int, [_vp, C.c_uint32, _u, C.POINTER(_f), _u, C.POINTER(_u), C.c_uint32, _u]),
     "cf_batch_remove_mesh": (C.c_int, [_vp, C.c_uint32]),

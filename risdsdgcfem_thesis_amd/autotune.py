@@ -1,11 +1,14 @@
 """Autotuner for batched stepping.
 
 Chooses, for a given set of meshes on the current device:
-  - the tile shape of the element stage (workgroup threads and elements per
-    tile: 128/128, 256/256, 512/512), from the batched K*x time per tet on a
+  - the tile shape of the element stage: workgroup threads and elements per
+    tile (128/128, 256/256, 512/512, and 512-thread tiles of 384 or 256
+    elements: 64, 48 or 32 elements per wavefront, the rest of the threads
+    carrying only tile nodes), from the batched K*x time per tet on a
     stratified sample of the meshes (every k-th mesh by size);
-  - the CG chunk length (iterations per recorded graph between host checks of
-    the convergence counter), from the batched step time on the same sample.
+  - the CG chunk length (iterations per recorded graph, between two device
+    compactions of the lists of the simulations still iterating), from the
+    batched step time on the same sample.
 Results are cached in memory per (device, workload signature), and in a JSON
 file when `cache_path` is given.
 """
@@ -17,7 +20,7 @@ import numpy as np
 
 from .fem import Batch, device_count
 
-TILE_CANDIDATES = [(128, 128), (256, 256), (512, 512)]
+TILE_CANDIDATES = [(128, 128), (256, 256), (512, 512), (512, 384), (512, 256)]
 CHUNK_CANDIDATES = [4, 8, 16, 32]
 _MEMO = {}
 

@@ -14,6 +14,7 @@
 #include <map>
 #include <new>
 #include <thread>
+#include <tuple>
 #include <unordered_map>
 #include <vector>
 
@@ -133,6 +134,14 @@ struct BSim {
     SimDev d{};
 };
 
+// Grid sizes of one chunk's kernels on device-compacted lists (tile, matvec
+// and CG vector kernels). A grid at least as long as its list runs one entry
+// per workgroup, as the full tables do; a shorter one strides.
+struct TailGrid {
+    uint32_t tile = 8, mv = 1, vec = 1;
+    std::tuple<uint32_t, uint32_t, uint32_t> key() const { return std::make_tuple(tile, mv, vec); }
+};
+
 struct TileBuild {
     TileHost th;
     std::vector<float> ref;
@@ -180,8 +189,26 @@ struct cf_batch {
     uint32_t h_cgs_cap = 0;
     uint32_t max_iter = 0;
     uint64_t active_tets = 0, active_nodes = 0;
-    // graphs
+    // lists compacted on the device (CF_BATCH_STEP_DEVICE*): list lengths,
+    // per-simulation offsets, and the tables the tail kernels walk
+    uint32_t* d_dyn = nullptr;
+    uint32_t* d_comp_off = nullptr;
+    size_t comp_off_cap = 0;
+    BatchTables tab_dyn{};
+    CompactOut comp{};
+    uint32_t fill_grid = 1;
+    // step schedule and, for the device schedule, the last step's chunk
+    // count (chunks until no simulation iterated)
+    int step_mode = CF_BATCH_STEP_DEVICE;
+    bool hist_valid = false;
+    uint32_t hist_chunks = 0;
+    uint32_t last_continuations = 0;
+    // grids of each chunk of the next step on device-compacted lists, from
+    // the previous step's list lengths at that chunk (power-of-two classes)
+    std::vector<TailGrid> hist_grids;
+    // graphs; tail chunks recorded per grid class
     hipGraphExec_t g_pro = nullptr, g_chunk = nullptr, g_epi = nullptr, g_round = nullptr;
+    std::map<std::tuple<uint32_t, uint32_t, uint32_t>, hipGraphExec_t> g_tails;
     uint32_t g_chunk_len = 0, g_round_len = 0;
     // steps queued for the next round: (slot, steps)
     std::vector<uint2> pending;
@@ -198,6 +225,8 @@ void free_graphs(cf_batch* b) {
     if (b->g_chunk) (void)hipGraphExecDestroy(b->g_chunk);
     if (b->g_epi) (void)hipGraphExecDestroy(b->g_epi);
     if (b->g_round) (void)hipGraphExecDestroy(b->g_round);
+    for (auto& kv : b->g_tails) (void)hipGraphExecDestroy(kv.second);
+    b->g_tails.clear();
     b->g_pro = b->g_chunk = b->g_epi = b->g_round = nullptr;
 }
 
@@ -443,6 +472,38 @@ void table_concat(const TableLists& L, std::vector<BlockRef>& all) {
     all.insert(all.end(), L.mv.begin(), L.mv.end());
 }
 
+// Tables of the tail chunks (lists compacted on the device, in the
+// compacted-table buffers at fixed offsets: tiles, matvec chunks, CG vector
+// chunks) and their persistent grids: as many workgroups as the CUs hold at
+// once (tiles: by the tile kernel's LDS, a multiple of 8 so that a workgroup
+// stays on one XCD while it strides), never more than the full tables need.
+int dyn_tables(cf_batch* b) {
+    const BatchTables& t = b->tab;
+    int cus = 0;
+    CF_CHECK_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, b->device));
+    const size_t lds = batch_tile_lds_bytes(b->tile_threads);
+    const uint32_t per_cu_tiles = std::max<uint32_t>(1, std::min<uint32_t>(uint32_t(160 * 1024 / lds),
+                                                                             2048 / b->tile_threads));
+    const uint32_t g_tile = std::max<uint32_t>(8, (uint32_t(cus) * per_cu_tiles) & ~7u);
+    const uint32_t g_vec = uint32_t(cus) * 8;
+    BatchTables d = t;
+    d.tile = b->d_tab_cg;
+    d.mv = b->d_tab_cg + t.n_tile;
+    d.vec = b->d_tab_cg + t.n_tile + t.n_mv;
+    d.n_tile = std::min(t.n_tile, g_tile);
+    d.n_mv = std::min(t.n_mv, g_vec);
+    d.n_vec = std::min(t.n_vec, g_vec);
+    d.elem = d.node = d.init = nullptr;
+    d.n_elem = d.n_node = d.n_init = 0;
+    d.active = b->d_active_cg;
+    d.dyn = b->d_dyn;
+    b->tab_dyn = d;
+    b->comp = CompactOut{b->d_tab_cg, b->d_tab_cg + t.n_tile, b->d_tab_cg + t.n_tile + t.n_mv, b->d_active_cg,
+                         b->d_comp_off, b->d_dyn};
+    b->fill_grid = g_vec;
+    return CF_OK;
+}
+
 int rebuild_tables(cf_batch* b) {
     std::vector<uint32_t> ids;
     for (auto& kv : b->sims)
@@ -473,6 +534,8 @@ int rebuild_tables(cf_batch* b) {
         }
         b->h_cgs_cap = b->cap;
     }
+    CF_TRY(ensure_device(reinterpret_cast<void**>(&b->d_comp_off), &b->comp_off_cap, 4 * L.slots.size(),
+                         sizeof(uint32_t)));
     BatchTables& t = b->tab;
     t.sims = b->d_sims;
     t.cgs = b->d_cgs;
@@ -482,6 +545,8 @@ int rebuild_tables(cf_batch* b) {
     t.max_tn = L.max_tn;
     t.max_te = L.max_te;
     t.tile_threads = b->tile_threads;
+    t.dyn = nullptr;
+    CF_TRY(dyn_tables(b));
     b->active_ids.swap(ids);
     b->max_iter = L.max_it;
     b->active_tets = L.tets;
@@ -545,6 +610,17 @@ int enqueue_chunk(cf_batch* b, uint32_t iters, const BatchTables& t) {
     return CF_OK;
 }
 
+// A chunk of CG iterations of the simulations still iterating, on lists the
+// device compacts first: no host work between chunks.
+int enqueue_tail_chunk(cf_batch* b, uint32_t iters, const TailGrid& g) {
+    CF_TRY(launch_batch_compact(b->tab, b->comp, b->fill_grid, b->stream));
+    BatchTables d = b->tab_dyn;
+    d.n_tile = std::min(g.tile, b->tab.n_tile);
+    d.n_mv = std::min(g.mv, b->tab.n_mv);
+    d.n_vec = std::min(g.vec, b->tab.n_vec);
+    return enqueue_chunk(b, iters, d);
+}
+
 int enqueue_epilogue(cf_batch* b) {
     const BatchTables& t = b->tab;
     hipStream_t s = b->stream;
@@ -563,11 +639,12 @@ int enqueue_round(cf_batch* b, uint32_t iters) {
     return launch_batch_count_busy(b->tab, b->stream);
 }
 
-int record(cf_batch* b, int which, uint32_t iters, hipGraphExec_t* exec) {
+int record(cf_batch* b, int which, uint32_t iters, hipGraphExec_t* exec, const TailGrid& grid = TailGrid()) {
     CF_CHECK_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
     int rc = which == 0   ? enqueue_prologue(b, true)
              : which == 1 ? enqueue_chunk(b, iters, b->tab)
              : which == 2 ? enqueue_epilogue(b)
+             : which == 4 ? enqueue_tail_chunk(b, iters, grid)
                           : enqueue_round(b, iters);
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(b->stream, &g);
@@ -629,6 +706,141 @@ void sim_params(BSim& s, const BMesh& m) {
     d.max_iter = std::min<uint32_t>(p.cg_max_iter, kMaxCgIter);
 }
 
+// CG chunks of a step under CF_BATCH_STEP_HOST: the counter and the CG
+// scalars written by each chunk are read while the next one runs. Once the
+// simulations still iterating are at most half of those in the tables, the
+// remaining chunks run on tables of just those, compacted on the host and
+// launched directly.
+int step_chunks_host(cf_batch* b, uint32_t chunk, uint32_t nchunks) {
+    hipStream_t s = b->stream;
+    BatchTables cur = b->tab;
+    bool compacted = false;
+    uint32_t cur_sims = b->tab.n_active;
+    const size_t cg_bytes = b->slots_used * sizeof(SimCg);
+    for (uint32_t i = 0; i < nchunks; i++) {
+        if (b->use_graphs && !compacted) CF_CHECK_HIP(hipGraphLaunch(b->g_chunk, s));
+        else CF_TRY(enqueue_chunk(b, chunk, cur));
+        CF_CHECK_HIP(hipMemcpyAsync(&b->h_counter[i & 1], b->d_counter, 4, hipMemcpyDeviceToHost, s));
+        CF_CHECK_HIP(hipMemcpyAsync(b->h_cgs_pin[i & 1], b->d_cgs, cg_bytes, hipMemcpyDeviceToHost, s));
+        CF_CHECK_HIP(hipEventRecord(b->evc[i & 1], s));
+        if (i >= 1) {
+            CF_CHECK_HIP(hipEventSynchronize(b->evc[(i - 1) & 1]));
+            const uint32_t left = b->h_counter[(i - 1) & 1];
+            if (left == 0) break;
+            if (2 * left <= cur_sims && i + 1 < nchunks) {
+                CF_TRY(compact_tables(b, b->h_cgs_pin[(i - 1) & 1], cur, &cur_sims));
+                compacted = true;
+            }
+        }
+    }
+    return CF_OK;
+}
+
+// The whole step submitted at once under the device schedule: prologue,
+// chunks on lists the device compacts before each chunk, and the epilogue
+// (which only touches simulations whose solve has ended). The chunk count
+// and each chunk's grids come from the previous step; the host reads nothing
+// until the step has run. Simulations still iterating then (a step that
+// needs more chunks than the last one) continue with chunks on grids of the
+// persistent sizes, one host read of the iterating count per chunk, and a
+// second epilogue ends them.
+int launch_tail(cf_batch* b, uint32_t chunk, const TailGrid& g) {
+    if (!b->use_graphs) return enqueue_tail_chunk(b, chunk, g);
+    auto it = b->g_tails.find(g.key());
+    if (it == b->g_tails.end()) {
+        hipGraphExec_t e = nullptr;
+        CF_TRY(record(b, 4, chunk, &e, g));
+        it = b->g_tails.emplace(g.key(), e).first;
+    }
+    CF_CHECK_HIP(hipGraphLaunch(it->second, b->stream));
+    return CF_OK;
+}
+
+TailGrid persistent_grid(const cf_batch* b) {
+    TailGrid g;
+    g.tile = b->tab_dyn.n_tile;
+    g.mv = b->tab_dyn.n_mv;
+    g.vec = b->tab_dyn.n_vec;
+    return g;
+}
+
+int step_device(cf_batch* b, uint32_t chunk, uint32_t nchunks) {
+    hipStream_t s = b->stream;
+    const uint32_t total = b->hist_valid ? std::max(1u, std::min(b->hist_chunks, nchunks)) : 1u;
+    for (uint32_t i = 0; i < total; i++)
+        CF_TRY(launch_tail(b, chunk, (b->hist_valid && i < b->hist_grids.size()) ? b->hist_grids[i]
+                                                                                  : persistent_grid(b)));
+    CF_CHECK_HIP(hipMemcpyAsync(&b->h_counter[0], b->d_counter, 4, hipMemcpyDeviceToHost, s));
+    if (b->use_graphs) CF_CHECK_HIP(hipGraphLaunch(b->g_epi, s));
+    else CF_TRY(enqueue_epilogue(b));
+    CF_CHECK_HIP(hipStreamSynchronize(s));
+    uint32_t done_chunks = total;
+    b->last_continuations = 0;
+    if (b->h_counter[0]) {
+        while (done_chunks < nchunks) {
+            CF_TRY(launch_tail(b, chunk, persistent_grid(b)));
+            done_chunks++;
+            b->last_continuations++;
+            CF_CHECK_HIP(hipMemcpyAsync(&b->h_counter[0], b->d_counter, 4, hipMemcpyDeviceToHost, s));
+            CF_CHECK_HIP(hipStreamSynchronize(s));
+            if (!b->h_counter[0]) break;
+        }
+        if (b->use_graphs) CF_CHECK_HIP(hipGraphLaunch(b->g_epi, s));
+        else CF_TRY(enqueue_epilogue(b));
+    }
+    return CF_OK;
+}
+
+// The chunk count for the next step from this step's CG iterations (chunks
+// until no simulation iterated), and each chunk's grids: the lengths of the lists compacted at the start of
+// chunk i are those of the simulations that ran more than i chunks of
+// iterations, each rounded up to the next m 2^k with 16 <= m < 32 (at most
+// 1/16 above the length, while the recorded tail graphs repeat from step to
+// step).
+uint32_t grid_class(uint32_t v) {
+    if (v <= 32) return std::max(v, 1u);
+    uint32_t k = 0;
+    while ((v >> k) >= 32) k++;
+    const uint32_t m = (v + (1u << k) - 1) >> k;  // in [16, 32]
+    return m << k;
+}
+
+void update_history(cf_batch* b, uint32_t chunk) {
+    std::vector<uint32_t> it;
+    it.reserve(b->active_ids.size());
+    for (uint32_t id : b->active_ids) it.push_back(b->h_cgs[b->sims[id].slot].iters);
+    if (it.empty()) return;
+    const uint32_t most = *std::max_element(it.begin(), it.end());
+    b->hist_chunks = std::max<uint32_t>(1, (most + chunk - 1) / chunk);
+    const std::vector<TailGrid> prev = b->hist_valid ? b->hist_grids : std::vector<TailGrid>();
+    b->hist_grids.assign(b->hist_chunks, TailGrid());
+    // a chunk keeps its previous grid while the new length fits it with at
+    // most a quarter to spare, so that steps replay graphs already recorded
+    auto keep = [](uint32_t old_g, uint32_t len, uint32_t g) { return (len <= old_g && 4 * len >= 3 * old_g) ? old_g : g; };
+    for (uint32_t i = 0; i < b->hist_chunks; i++) {
+        uint64_t tiles = 0, mv = 0, vec = 0;
+        for (size_t k = 0; k < it.size(); k++) {
+            if (it[k] <= i * chunk) continue;
+            const SimDev& d = b->sims[b->active_ids[k]].d;
+            tiles += d.num_tiles;
+            mv += d.nb_mv;
+            vec += d.nb_vec;
+        }
+        TailGrid& g = b->hist_grids[i];
+        // tile grids stay multiples of 8 (a workgroup keeps its XCD while it strides)
+        const uint32_t lt = uint32_t(8 * ((tiles + 7) / 8)), lm = uint32_t(mv), lv = uint32_t(vec);
+        g.tile = std::min(b->tab.n_tile, 8 * grid_class(std::max(1u, lt / 8)));
+        g.mv = std::min(b->tab.n_mv, grid_class(std::max(1u, lm)));
+        g.vec = std::min(b->tab.n_vec, grid_class(std::max(1u, lv)));
+        if (i < prev.size()) {
+            g.tile = std::min(b->tab.n_tile, keep(prev[i].tile, lt, g.tile));
+            g.mv = std::min(b->tab.n_mv, keep(prev[i].mv, lm, g.mv));
+            g.vec = std::min(b->tab.n_vec, keep(prev[i].vec, lv, g.vec));
+        }
+    }
+    b->hist_valid = true;
+}
+
 }  // namespace
 
 extern "C" {
@@ -648,6 +860,7 @@ int cf_batch_create(uint64_t budget, cf_batch** out) {
     (void)hipGetDevice(&b->device);
     if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
         hipMalloc(reinterpret_cast<void**>(&b->d_counter), 16) != hipSuccess ||
+        hipMalloc(reinterpret_cast<void**>(&b->d_dyn), kDynCount * sizeof(uint32_t)) != hipSuccess ||
         hipHostMalloc(reinterpret_cast<void**>(&b->h_counter), 16) != hipSuccess ||
         hipEventCreate(&b->ev0) != hipSuccess || hipEventCreate(&b->ev1) != hipSuccess ||
         hipEventCreateWithFlags(&b->evc[0], hipEventDisableTiming) != hipSuccess ||
@@ -670,6 +883,8 @@ int cf_batch_destroy(cf_batch* b) {
     if (b->d_tab) (void)hipFree(b->d_tab);
     if (b->d_active) (void)hipFree(b->d_active);
     if (b->d_counter) (void)hipFree(b->d_counter);
+    if (b->d_dyn) (void)hipFree(b->d_dyn);
+    if (b->d_comp_off) (void)hipFree(b->d_comp_off);
     if (b->h_counter) (void)hipHostFree(b->h_counter);
     for (auto& p : b->h_cgs_pin)
         if (p) (void)hipHostFree(p);
@@ -703,6 +918,18 @@ int cf_batch_set_schedule(cf_batch* b, uint32_t chunk, int use_graphs) {
     if (!b || chunk < 1 || chunk > kMaxCgIter) return CF_ERR_INVALID_ARG;
     b->cg_chunk = chunk;
     b->use_graphs = use_graphs ? 1 : 0;
+    b->hist_valid = false;  // chunk counts are in chunks of the old length
+    free_graphs(b);
+    return CF_OK;
+}
+
+int cf_batch_set_step_mode(cf_batch* b, int mode) {
+    if (!b || (mode != CF_BATCH_STEP_HOST && mode != CF_BATCH_STEP_DEVICE)) {
+        set_error("cf_batch_set_step_mode: mode must be CF_BATCH_STEP_HOST or CF_BATCH_STEP_DEVICE");
+        return CF_ERR_INVALID_ARG;
+    }
+    b->step_mode = mode;
+    b->hist_valid = false;
     free_graphs(b);
     return CF_OK;
 }
@@ -932,39 +1159,21 @@ int cf_batch_fixed_step(cf_batch* b, cf_step_stats* stats) {
     CF_CHECK_HIP(hipEventRecord(b->ev0, s));
     if (b->use_graphs) CF_CHECK_HIP(hipGraphLaunch(b->g_pro, s));
     else CF_TRY(enqueue_prologue(b, true));
-    // CG chunks: the counter and the CG scalars written by each chunk are read
-    // while the next one runs. Once the simulations still iterating are at
-    // most half of those in the tables, the remaining chunks run on compacted
-    // tables of just those (launched directly).
     const uint32_t chunk = b->use_graphs ? b->g_chunk_len : std::min(b->cg_chunk, std::max<uint32_t>(1, b->max_iter));
     const uint32_t nchunks = (b->max_iter + chunk - 1) / chunk;
-    BatchTables cur = b->tab;
-    bool compacted = false;
-    uint32_t cur_sims = t.n_active;
-    const size_t cg_bytes = b->slots_used * sizeof(SimCg);
-    for (uint32_t i = 0; i < nchunks; i++) {
-        if (b->use_graphs && !compacted) CF_CHECK_HIP(hipGraphLaunch(b->g_chunk, s));
-        else CF_TRY(enqueue_chunk(b, chunk, cur));
-        CF_CHECK_HIP(hipMemcpyAsync(&b->h_counter[i & 1], b->d_counter, 4, hipMemcpyDeviceToHost, s));
-        CF_CHECK_HIP(hipMemcpyAsync(b->h_cgs_pin[i & 1], b->d_cgs, cg_bytes, hipMemcpyDeviceToHost, s));
-        CF_CHECK_HIP(hipEventRecord(b->evc[i & 1], s));
-        if (i >= 1) {
-            CF_CHECK_HIP(hipEventSynchronize(b->evc[(i - 1) & 1]));
-            const uint32_t left = b->h_counter[(i - 1) & 1];
-            if (left == 0) break;
-            if (2 * left <= cur_sims && i + 1 < nchunks) {
-                CF_TRY(compact_tables(b, b->h_cgs_pin[(i - 1) & 1], cur, &cur_sims));
-                compacted = true;
-            }
-        }
+    if (b->step_mode == CF_BATCH_STEP_HOST) {
+        CF_TRY(step_chunks_host(b, chunk, nchunks));
+        if (b->use_graphs) CF_CHECK_HIP(hipGraphLaunch(b->g_epi, s));
+        else CF_TRY(enqueue_epilogue(b));
+    } else {
+        CF_TRY(step_device(b, chunk, nchunks));
     }
-    if (b->use_graphs) CF_CHECK_HIP(hipGraphLaunch(b->g_epi, s));
-    else CF_TRY(enqueue_epilogue(b));
     CF_CHECK_HIP(hipEventRecord(b->ev1, s));
     CF_CHECK_HIP(hipEventSynchronize(b->ev1));
     // every step's CG records: the stats, and the non-finite test
     b->h_cgs.resize(b->slots_used);
     CF_CHECK_HIP(hipMemcpy(b->h_cgs.data(), b->d_cgs, b->slots_used * sizeof(SimCg), hipMemcpyDeviceToHost));
+    if (b->step_mode != CF_BATCH_STEP_HOST) update_history(b, chunk);
     uint32_t nonfinite = 0;
     for (uint32_t id : b->active_ids) nonfinite += sim_cg_nonfinite(b->h_cgs[b->sims[id].slot]) ? 1 : 0;
     if (stats) {
@@ -1066,6 +1275,10 @@ int cf_batch_info_get(const cf_batch* b, cf_batch_info* info) {
     info->tile_elems = b->tile_elems;
     info->cg_chunk = b->cg_chunk;
     info->use_graphs = uint32_t(b->use_graphs);
+    info->step_mode = uint32_t(b->step_mode);
+    info->next_chunks = b->hist_valid ? b->hist_chunks : 1;
+    info->last_continuations = b->last_continuations;
+    info->chunk_graphs = uint32_t(b->g_tails.size());
     return CF_OK;
 }
 

@@ -53,10 +53,15 @@ __device__ inline NodeTerms node_terms(const SimDev& s) {
 }
 
 // The slot of the wavefront's simulation in a per-simulation kernel
-// (4 wavefronts per block), or kIdleSlot.
+// (4 wavefronts per block), or kIdleSlot. The grid covers n_active
+// simulations; a list compacted on the device (t.dyn) holds at most that many.
 __device__ inline uint32_t wave_slot(const BatchTables& t) {
     const uint32_t w = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
-    return (w < t.n_active) ? t.active[w] : kIdleSlot;
+    // the slot is loaded together with the device length (a position below
+    // n_active is inside the list's capacity; past the length it is stale)
+    const uint32_t slot = (w < t.n_active) ? t.active[w] : kIdleSlot;
+    const uint32_t n = t.dyn ? t.dyn[kDynActive] : t.n_active;
+    return (w < n) ? slot : kIdleSlot;
 }
 
 }  // namespace
@@ -85,16 +90,36 @@ __global__ __launch_bounds__(kBlock, 8) void kb_begin_eval(BatchTables t) {
 // warm start, before the solve) for starting simulations, or df_x on d for
 // iterating ones; kSelYAll: df_x on y for every simulation.
 template <int MODE, int TB>
-__global__ __launch_bounds__(TB) void kb_tiles(BatchTables t, int sel) {
-    const BlockRef br = t.tile[blockIdx.x];
+__device__ inline void tiles_entry(const BatchTables& t, int sel, BlockRef br, const TileLds<TB>& L) {
     if (br.slot == kIdleSlot) return;
     if (sel != kSelYAll && t.cgs[br.slot].phase != (sel == kSelD ? kPhaseIter : kPhaseStart)) return;
-    extern __shared__ float smem_f[];
-    const TileLds<TB> L(smem_f);
     const SimDev& s = t.sims[br.slot];
     const float* in = (MODE == kModeFe) ? s.x : (sel == kSelD ? s.d : s.y);
     const TileArgs a = tile_args(s, in, t.max_tn, t.max_te);
     tile_block<MODE>(a, br.blk, L.v, L.r, L.F, L.sp);
+}
+
+// LOOP: a list compacted on the device (t.dyn), walked with a stride of the
+// grid. A workgroup loads its first entry together with the list length (the
+// grid never exceeds the list's capacity, so the load is in bounds; an entry
+// past the length is stale and not used), so the length costs no extra
+// latency. Otherwise: one table entry per workgroup.
+template <int MODE, int TB, bool LOOP>
+__global__ __launch_bounds__(TB) void kb_tiles(BatchTables t, int sel) {
+    extern __shared__ float smem_f[];
+    const TileLds<TB> L(smem_f);
+    const BlockRef br = t.tile[blockIdx.x];
+    if (!LOOP) {
+        tiles_entry<MODE, TB>(t, sel, br, L);
+        return;
+    }
+    const uint32_t n = t.dyn[kDynTile];
+    if (blockIdx.x >= n) return;
+    tiles_entry<MODE, TB>(t, sel, br, L);
+    for (uint32_t i = blockIdx.x + gridDim.x; i < n; i += gridDim.x) {
+        __syncthreads();  // the next entry restages the LDS planes
+        tiles_entry<MODE, TB>(t, sel, t.tile[i], L);
+    }
 }
 
 // b = dt (M g + f_e) + M v, y = v, KNC rows (Step_QuasiImplicitEuler_v,
@@ -117,9 +142,7 @@ __global__ __launch_bounds__(kBlock) void kb_gather_rhs(BatchTables t) {
 // (kSelD) or, on y, for every simulation (kSelYAll); for starting simulations
 // (kSelY) the CG start r = S(b - A y), d = r with per-chunk r.r instead
 // (ILSS_CG.h:55-63), as the leaf's fused start gather computes it.
-__global__ __launch_bounds__(kBlock) void kb_gather_matvec(BatchTables t, int sel) {
-    __shared__ double sh[kBlock / 64];
-    const BlockRef br = t.mv[blockIdx.x];
+__device__ inline void gather_matvec_entry(const BatchTables& t, int sel, BlockRef br, double* sh) {
     if (br.slot == kIdleSlot) return;
     if (sel != kSelYAll && t.cgs[br.slot].phase != (sel == kSelD ? kPhaseIter : kPhaseStart)) return;
     const SimDev& s = t.sims[br.slot];
@@ -136,6 +159,20 @@ __global__ __launch_bounds__(kBlock) void kb_gather_matvec(BatchTables t, int se
         part = block_sum(gather_matvec_chunk<false>(s.N, br.blk, s.npt_mv, s.node_part_ptr, s.partials, nt), sh);
         if (threadIdx.x == 0) s.part_a[br.blk] = part;
     }
+}
+
+template <bool LOOP>
+__global__ __launch_bounds__(kBlock) void kb_gather_matvec(BatchTables t, int sel) {
+    __shared__ double sh[kBlock / 64];
+    const BlockRef br = t.mv[blockIdx.x];
+    if (!LOOP) {
+        gather_matvec_entry(t, sel, br, sh);
+        return;
+    }
+    const uint32_t n = t.dyn[kDynMv];
+    if (blockIdx.x >= n) return;
+    gather_matvec_entry(t, sel, br, sh);
+    for (uint32_t i = blockIdx.x + gridDim.x; i < n; i += gridDim.x) gather_matvec_entry(t, sel, t.mv[i], sh);
 }
 
 // ------------------------------------------------------------ CG (ns/ILSS_CG.h:38-151)
@@ -191,15 +228,27 @@ __global__ __launch_bounds__(kBlock) void kb_cg_alpha(BatchTables t) {
 }
 
 // x += alpha d ; r -= alpha q over the leaf's CG chunks, with per-chunk r.r
-__global__ __launch_bounds__(kBlock) void kb_update_xr(BatchTables t) {
-    __shared__ double sh[kBlock / 64];
-    const BlockRef br = t.vec[blockIdx.x];
+__device__ inline void update_xr_entry(const BatchTables& t, BlockRef br, double* sh) {
     if (br.slot == kIdleSlot) return;
     const SimCg& c = t.cgs[br.slot];
     if (c.phase != kPhaseIter) return;
     const SimDev& s = t.sims[br.slot];
     const double part = block_sum(xr_rows_chunk(3 * s.N, br.blk, s.rows_cg, c.alpha, s.y, s.r, s.d, s.q), sh);
     if (threadIdx.x == 0) s.part_b[br.blk] = part;
+}
+
+template <bool LOOP>
+__global__ __launch_bounds__(kBlock) void kb_update_xr(BatchTables t) {
+    __shared__ double sh[kBlock / 64];
+    const BlockRef br = t.vec[blockIdx.x];
+    if (!LOOP) {
+        update_xr_entry(t, br, sh);
+        return;
+    }
+    const uint32_t n = t.dyn[kDynVec];
+    if (blockIdx.x >= n) return;
+    update_xr_entry(t, br, sh);
+    for (uint32_t i = blockIdx.x + gridDim.x; i < n; i += gridDim.x) update_xr_entry(t, t.vec[i], sh);
 }
 
 // delta' = r.r ; beta = delta'/delta ; stop tests (ILSS_CG.h:86-88,115-133)
@@ -226,13 +275,25 @@ __global__ __launch_bounds__(kBlock) void kb_cg_beta(BatchTables t) {
     }
 }
 
-__global__ __launch_bounds__(kBlock) void kb_update_d(BatchTables t) {
-    const BlockRef br = t.vec[blockIdx.x];
+__device__ inline void update_d_entry(const BatchTables& t, BlockRef br) {
     if (br.slot == kIdleSlot) return;
     const SimCg& c = t.cgs[br.slot];
     if (c.phase != kPhaseIter) return;
     const SimDev& s = t.sims[br.slot];
     update_d_block<true>(3 * s.N, c.beta, s.d, s.r, s.knc_mask, br.blk, s.nb_vec);
+}
+
+template <bool LOOP>
+__global__ __launch_bounds__(kBlock) void kb_update_d(BatchTables t) {
+    const BlockRef br = t.vec[blockIdx.x];
+    if (!LOOP) {
+        update_d_entry(t, br);
+        return;
+    }
+    const uint32_t n = t.dyn[kDynVec];
+    if (blockIdx.x >= n) return;
+    update_d_entry(t, br);
+    for (uint32_t i = blockIdx.x + gridDim.x; i < n; i += gridDim.x) update_d_entry(t, t.vec[i]);
 }
 
 // ------------------------------------------------------------ step epilogue
@@ -331,6 +392,93 @@ __global__ void kb_count_busy(BatchTables t) {
     if (i < t.n_active && t.cgs[t.active[i]].phase != kPhaseIdle) atomicAdd(t.busy, 1u);
 }
 
+// ------------------------------------------------------------ device compaction
+// The CG-loop lists of the simulations still iterating, in the order of the
+// full active list (the order the host compaction uses): per simulation its
+// tiles (0 .. num_tiles), matvec chunks and CG vector chunks, and its slot.
+// The tile list is dealt to the XCDs as the host tables are: natural entry
+// m goes to position (m % per) * 8 + m / per, per = ceil(tiles / 8), so that
+// a workgroup striding by a multiple of 8 stays on one XCD and each XCD walks
+// a contiguous range of tiles; the positions of m in [tiles, 8 per) hold
+// idle entries.
+constexpr uint32_t kScanThreads = 1024;
+
+__device__ inline bool iterating(const BatchTables& t, uint32_t slot) { return t.cgs[slot].phase == kPhaseIter; }
+
+// One workgroup: per-simulation offsets (exclusive prefix sums over the full
+// active list) and the list lengths. Thread k owns a contiguous run of
+// simulations, so the offsets follow the list order.
+__global__ __launch_bounds__(kScanThreads) void kb_compact_scan(BatchTables t, CompactOut o) {
+    __shared__ uint32_t sc[4][kScanThreads];
+    const uint32_t tid = threadIdx.x, n = t.n_active;
+    const uint32_t per = (n + kScanThreads - 1) / kScanThreads;
+    const uint32_t i0 = min(n, tid * per), i1 = min(n, i0 + per);
+    uint32_t c[4] = {0, 0, 0, 0};
+    for (uint32_t i = i0; i < i1; i++) {
+        const uint32_t slot = t.active[i];
+        if (!iterating(t, slot)) continue;
+        const SimDev& s = t.sims[slot];
+        c[0] += s.num_tiles; c[1] += s.nb_mv; c[2] += s.nb_vec; c[3] += 1;
+    }
+#pragma unroll
+    for (int k = 0; k < 4; k++) sc[k][tid] = c[k];
+    __syncthreads();
+    // inclusive Hillis-Steele scan of the four counts
+    for (uint32_t w = 1; w < kScanThreads; w <<= 1) {
+        uint32_t v[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) v[k] = (tid >= w) ? sc[k][tid - w] : 0u;
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < 4; k++) sc[k][tid] += v[k];
+        __syncthreads();
+    }
+    uint32_t run[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) run[k] = sc[k][tid] - c[k];
+    for (uint32_t i = i0; i < i1; i++) {
+        const uint32_t slot = t.active[i];
+#pragma unroll
+        for (int k = 0; k < 4; k++) o.off[4 * i + k] = run[k];
+        if (!iterating(t, slot)) continue;
+        const SimDev& s = t.sims[slot];
+        run[0] += s.num_tiles; run[1] += s.nb_mv; run[2] += s.nb_vec; run[3] += 1;
+    }
+    if (tid == kScanThreads - 1) {
+        const uint32_t tiles = sc[0][tid];
+        const uint32_t tper = (tiles + 7) / 8;
+        o.dyn[kDynTile] = 8 * tper;
+        o.dyn[kDynMv] = sc[1][tid];
+        o.dyn[kDynVec] = sc[2][tid];
+        o.dyn[kDynActive] = sc[3][tid];
+        o.dyn[kDynTilePer] = tper;
+        o.dyn[kDynTiles] = tiles;
+        *t.busy = 0;
+    }
+}
+
+// The entries, one simulation per workgroup at a time (coalesced writes).
+__global__ __launch_bounds__(kBlock) void kb_compact_fill(BatchTables t, CompactOut o) {
+    const uint32_t tper = o.dyn[kDynTilePer], tiles = o.dyn[kDynTiles];
+    for (uint32_t i = blockIdx.x; i < t.n_active; i += gridDim.x) {
+        const uint32_t slot = t.active[i];
+        if (!iterating(t, slot)) continue;
+        const SimDev& s = t.sims[slot];
+        const uint32_t* off = o.off + 4 * i;
+        for (uint32_t j = threadIdx.x; j < s.num_tiles; j += kBlock) {
+            const uint32_t m = off[0] + j;
+            o.tile[(m % tper) * 8 + m / tper] = BlockRef{slot, j};
+        }
+        for (uint32_t j = threadIdx.x; j < s.nb_mv; j += kBlock) o.mv[off[1] + j] = BlockRef{slot, j};
+        for (uint32_t j = threadIdx.x; j < s.nb_vec; j += kBlock) o.vec[off[2] + j] = BlockRef{slot, j};
+        if (threadIdx.x == 0) o.active[off[3]] = slot;
+    }
+    if (blockIdx.x == 0 && threadIdx.x < 8 * tper - tiles) {
+        const uint32_t m = tiles + threadIdx.x;
+        o.tile[(m % tper) * 8 + m / tper] = BlockRef{kIdleSlot, 0};
+    }
+}
+
 __global__ void kb_fill_u32(uint32_t* p, uint32_t value, uint32_t n) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = value;
@@ -360,8 +508,9 @@ int launch_batch_forces(const BatchTables& t, int mode, int sel, hipStream_t s) 
 #define CF_BTILE(TBV)                                                                                     \
     do {                                                                                                  \
         const size_t sm = TileLds<TBV>::kBytes;                                                           \
-        if (mode == kModeFe) hipLaunchKernelGGL((kb_tiles<kModeFe, TBV>), dim3(t.n_tile), dim3(TBV), sm, s, t, sel); \
-        else hipLaunchKernelGGL((kb_tiles<kModeDfx, TBV>), dim3(t.n_tile), dim3(TBV), sm, s, t, sel);      \
+        if (mode == kModeFe) hipLaunchKernelGGL((kb_tiles<kModeFe, TBV, false>), dim3(t.n_tile), dim3(TBV), sm, s, t, sel); \
+        else if (t.dyn) hipLaunchKernelGGL((kb_tiles<kModeDfx, TBV, true>), dim3(t.n_tile), dim3(TBV), sm, s, t, sel); \
+        else hipLaunchKernelGGL((kb_tiles<kModeDfx, TBV, false>), dim3(t.n_tile), dim3(TBV), sm, s, t, sel); \
     } while (0)
     switch (t.tile_threads) {
         case 1024: CF_BTILE(1024); break;
@@ -381,7 +530,8 @@ int launch_batch_gather_rhs(const BatchTables& t, hipStream_t s) {
 
 int launch_batch_gather_matvec(const BatchTables& t, int sel, hipStream_t s) {
     if (!t.n_mv) return CF_OK;
-    hipLaunchKernelGGL(kb_gather_matvec, dim3(t.n_mv), dim3(kBlock), 0, s, t, sel);
+    if (t.dyn) hipLaunchKernelGGL(kb_gather_matvec<true>, dim3(t.n_mv), dim3(kBlock), 0, s, t, sel);
+    else hipLaunchKernelGGL(kb_gather_matvec<false>, dim3(t.n_mv), dim3(kBlock), 0, s, t, sel);
     return check_launch_b();
 }
 
@@ -399,7 +549,8 @@ int launch_batch_cg_alpha(const BatchTables& t, hipStream_t s) {
 
 int launch_batch_update_xr(const BatchTables& t, hipStream_t s) {
     if (!t.n_vec) return CF_OK;
-    hipLaunchKernelGGL(kb_update_xr, dim3(t.n_vec), dim3(kBlock), 0, s, t);
+    if (t.dyn) hipLaunchKernelGGL(kb_update_xr<true>, dim3(t.n_vec), dim3(kBlock), 0, s, t);
+    else hipLaunchKernelGGL(kb_update_xr<false>, dim3(t.n_vec), dim3(kBlock), 0, s, t);
     return check_launch_b();
 }
 
@@ -411,7 +562,8 @@ int launch_batch_cg_beta(const BatchTables& t, hipStream_t s) {
 
 int launch_batch_update_d(const BatchTables& t, hipStream_t s) {
     if (!t.n_vec) return CF_OK;
-    hipLaunchKernelGGL(kb_update_d, dim3(t.n_vec), dim3(kBlock), 0, s, t);
+    if (t.dyn) hipLaunchKernelGGL(kb_update_d<true>, dim3(t.n_vec), dim3(kBlock), 0, s, t);
+    else hipLaunchKernelGGL(kb_update_d<false>, dim3(t.n_vec), dim3(kBlock), 0, s, t);
     return check_launch_b();
 }
 
@@ -453,6 +605,24 @@ int launch_batch_arm(const BatchTables& t, const uint2* list, uint32_t n, hipStr
 int launch_batch_count_busy(const BatchTables& t, hipStream_t s) {
     if (!t.n_active) return CF_OK;
     hipLaunchKernelGGL(kb_count_busy, dim3((t.n_active + 255) / 256), dim3(256), 0, s, t);
+    return check_launch_b();
+}
+
+size_t batch_tile_lds_bytes(uint32_t tile_threads) {
+    switch (tile_threads) {
+        case 1024: return TileLds<1024>::kBytes;
+        case 512: return TileLds<512>::kBytes;
+        case 128: return TileLds<128>::kBytes;
+        default: return TileLds<256>::kBytes;
+    }
+}
+
+int launch_batch_compact(const BatchTables& t, const CompactOut& o, uint32_t fill_grid, hipStream_t s) {
+    if (!t.n_active) return CF_OK;
+    hipLaunchKernelGGL(kb_compact_scan, dim3(1), dim3(kScanThreads), 0, s, t, o);
+    CF_TRY(check_launch_b());
+    const uint32_t g = fill_grid < t.n_active ? fill_grid : t.n_active;
+    hipLaunchKernelGGL(kb_compact_fill, dim3(g ? g : 1), dim3(kBlock), 0, s, t, o);
     return check_launch_b();
 }
 

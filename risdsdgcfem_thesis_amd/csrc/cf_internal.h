@@ -381,6 +381,25 @@ struct BatchTables {
     uint32_t* counter;                          // simulations still iterating
     uint32_t* busy;                             // simulations not idle after a round
     uint32_t max_tn, max_te, tile_threads;
+    // Lists compacted on the device (kb_compact_*): when set, the CG-loop
+    // kernels take their entry counts from dyn (kDyn*) and walk the entries
+    // with a stride of their grid, whose size is then the n_* above.
+    const uint32_t* dyn = nullptr;
+};
+
+// dyn[]: entries of the tile list (padded to a multiple of 8), of the matvec
+// and CG vector lists, the simulations still iterating, the tile list's
+// entries per XCD, and its unpadded length.
+enum DynCount { kDynTile = 0, kDynMv = 1, kDynVec = 2, kDynActive = 3, kDynTilePer = 4, kDynTiles = 5, kDynCount = 8 };
+
+// Where the device compaction writes (capacities of the full tables).
+struct CompactOut {
+    BlockRef* tile;
+    BlockRef* mv;
+    BlockRef* vec;
+    uint32_t* active;
+    uint32_t* off;  // 4 per simulation of the full list: tile, mv, vec and active offsets
+    uint32_t* dyn;
 };
 
 // element-stage input: y in the prologue, d in the CG loop, y for every
@@ -404,6 +423,12 @@ int launch_batch_arm_all(const BatchTables& t, hipStream_t s);
 // (slot, steps) pairs: queue steps; an idle simulation starts at once
 int launch_batch_arm(const BatchTables& t, const uint2* list, uint32_t n, hipStream_t s);
 int launch_batch_count_busy(const BatchTables& t, hipStream_t s);
+// The CG-loop lists of the simulations of t (full tables) still iterating,
+// built on the device: offsets by a one-workgroup scan, then the entries.
+// Also clears the busy counter.
+int launch_batch_compact(const BatchTables& t, const CompactOut& o, uint32_t fill_grid, hipStream_t s);
+// LDS bytes of one batched tile workgroup of the given threads
+size_t batch_tile_lds_bytes(uint32_t tile_threads);
 
 // ---------------------------------------------------------------- 2D resident step
 // A 2D mesh small enough to live in one workgroup is stepped entirely inside

@@ -502,11 +502,14 @@ class Batch:
     topologies, simulations carry their own parameters, KNC and state. Each
     simulation's step equals LeafSolid3D.fixed_step on the same inputs."""
 
-    def __init__(self, hbm_budget_bytes=0, tile_threads=256, tile_elems=0, cg_chunk=8, use_graphs=True):
+    def __init__(self, hbm_budget_bytes=0, tile_threads=256, tile_elems=0, cg_chunk=8, use_graphs=True,
+                 step_mode=None):
         self.h = C.c_void_p()
         check(lib.cf_batch_create(int(hbm_budget_bytes), C.byref(self.h)), "cf_batch_create")
         check(lib.cf_batch_set_tiles(self.h, int(tile_threads), int(tile_elems)), "cf_batch_set_tiles")
         check(lib.cf_batch_set_schedule(self.h, int(cg_chunk), int(bool(use_graphs))), "cf_batch_set_schedule")
+        if step_mode is not None:
+            self.set_step_mode(step_mode)
         self.mesh_nodes = {}
 
     def close(self):
@@ -522,6 +525,10 @@ class Batch:
 
     def set_schedule(self, cg_chunk=8, use_graphs=True):
         check(lib.cf_batch_set_schedule(self.h, int(cg_chunk), int(bool(use_graphs))), "cf_batch_set_schedule")
+
+    def set_step_mode(self, mode):
+        """BATCH_STEP_HOST or BATCH_STEP_DEVICE (default); results are the same."""
+        check(lib.cf_batch_set_step_mode(self.h, int(mode)), "cf_batch_set_step_mode")
 
     def add_mesh(self, ref_pos, tets):
         r = np.ascontiguousarray(ref_pos, np.float32)

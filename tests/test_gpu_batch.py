@@ -101,6 +101,72 @@ def test_schedules_equal(gpu, plain_results, kw):
         assert same(g, p), f"case {i} with {kw}"
 
 
+@pytest.mark.parametrize("mode", ["BATCH_STEP_HOST", "BATCH_STEP_DEVICE"])
+@pytest.mark.parametrize("kw", [dict(), dict(use_graphs=False), dict(cg_chunk=1), dict(cg_chunk=5)])
+def test_step_modes_equal(gpu, plain_results, mode, kw):
+    """Host-polled chunks, and the step submitted at once with the lists
+    compacted on the device before every chunk: the same results."""
+    cs, plain = plain_results
+    got = run_batch(gpu, cs, 3, step_mode=getattr(gpu, mode), **kw)
+    for i, (g, p) in enumerate(zip(got, plain)):
+        assert same(g, p), f"case {i} under {mode} with {kw}"
+
+
+def test_device_schedule_outrun_and_overshoot(gpu):
+    """The device schedule submits a step with the previous step's chunk
+    counts. Steps from rest and from a strongly disturbed state need
+    different CG iteration counts (here the step from rest needs more): the
+    step that needs more chunks than the one before it gets chunks added after
+    the submitted ones (one host read each), the one that needs fewer runs
+    trailing chunks that find no simulation iterating. Every step stays
+    bit-identical to the single-mesh path given the same states."""
+    cs = [box_case(gpu, (5, 5, 9), 1e5, 0.30, (2, -1), max_iter=200),
+          box_case(gpu, (7, 4, 6), 3e6, 0.45, (0, 1), max_iter=200),
+          box_case(gpu, (6, 6, 6), 5e4, 0.25, (1, -1), max_iter=200)]
+    rng = np.random.default_rng(11)
+    kicks = [(c["pos"] + rng.uniform(-2e-3, 2e-3, c["pos"].shape).astype(np.float32),
+              rng.uniform(-0.5, 0.5, c["pos"].shape).astype(np.float32)) for c in cs]
+    plains = []
+    for c, (xk, vk) in zip(cs, kicks):
+        s = gpu.LeafSolid3D(c["pos"], c["tets"], c["params"], knc=c["knc"])
+        st = [s.fixed_step()]
+        s.set_state(xk, vk)
+        st.append(s.fixed_step())
+        s.set_state(c["pos"], np.zeros_like(c["pos"]))
+        st.append(s.fixed_step())
+        x, v = s.state()
+        s.close()
+        plains.append((x, v, [(t.cg_iters, t.cg_reason, t.cg_rel_residual, t.elastic_energy, t.min_det_f)
+                              for t in st]))
+    b = gpu.Batch(cg_chunk=2, step_mode=gpu.BATCH_STEP_DEVICE)
+    mids = b.add_meshes([(c["pos"], c["tets"]) for c in cs], threads=2)
+    sids = [b.add_sim(m, c["params"], knc=c["knc"]) for m, c in zip(mids, cs)]
+    hist = {s: [] for s in sids}
+    conts, chunks = [], []
+    for step in range(3):
+        if step == 1:
+            for sid, (xk, vk) in zip(sids, kicks):
+                b.set_state(sid, xk, vk)
+        if step == 2:
+            for sid, c in zip(sids, cs):
+                b.set_state(sid, c["pos"], np.zeros_like(c["pos"]))
+        submitted = b.info()["next_chunks"]
+        st = b.fixed_step()
+        info = b.info()
+        conts.append(info["last_continuations"])
+        chunks.append((submitted, info["next_chunks"]))
+        for sid in sids:
+            t = st[sid]
+            hist[sid].append((t.cg_iters, t.cg_reason, t.cg_rel_residual, t.elastic_energy, t.min_det_f))
+    print(f"\n(submitted, needed) chunks per step {chunks}, chunks added {conts}")
+    for sid, c, p in zip(sids, cs, plains):
+        x, v = b.state(sid, c["N"])
+        assert same((x, v, hist[sid]), p)
+    b.close()
+    assert any(c > 0 for c in conts[1:]), "no step needed chunks beyond the submitted ones"
+    assert any(sub > need for sub, need in chunks[1:]), "no step needed fewer chunks than submitted"
+
+
 def test_admission_removal_and_inactive(gpu, plain_results):
     cs, plain = plain_results
     b = gpu.Batch()
