@@ -408,7 +408,7 @@ def main():
     ap.add_argument("--step-mode", type=int, default=1, choices=[0, 1],
                     help="batched 3D: 0 host-polled chunks, 1 the step submitted at once with device-compacted "
                          "lists (default)")
-    ap.add_argument("--tile-alternate", type=int, default=1, choices=[0, 1],
+    ap.add_argument("--tile-alternate", type=int, default=0, choices=[0, 1],
                     help="single mesh: 0 every CG iteration walks the tiles forwards; 1 odd iterations walk them "
                          "backwards, starting on the tiles the previous K*x touched last")
     ap.add_argument("--no-autotune", action="store_true", help="batched 3D: use --tile-threads/--tile-elems/--cg-chunk")

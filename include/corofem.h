@@ -396,11 +396,12 @@ int cf_set_tile_threads(uint32_t threads);
    resident workgroups that walk their tiles with the next tile's loads in
    flight; 2: the pipelined kernel of 1 with one tile per workgroup. */
 int cf_set_tile_persistent(int persistent);
-/* 1 (default): the leaf CG's odd iterations walk each XCD's tiles from the
-   end (one tile per workgroup), so a K*x pass starts where the previous one
-   ended, on data the memory-side cache may still hold; 0: always forwards.
-   Results are the same either way. Takes effect for solves enqueued or
-   recorded afterwards. */
+/* 1: the leaf CG's odd iterations walk each XCD's tiles from the end (one
+   tile per workgroup), so a K*x pass starts where the previous one ended, on
+   data the memory-side cache may still hold; 0 (default): always forwards
+   (the element stream is read non-temporally, which leaves the caches to the
+   node data and gains more). Results are the same either way. Takes effect
+   for solves enqueued or recorded afterwards. */
 int cf_set_tile_alternate(int alternate);
 int cf_get_tile_elems(uint32_t* tile_elems);
 /* Workgroups of the two CG iteration kernels of the leaf solvers (alpha with

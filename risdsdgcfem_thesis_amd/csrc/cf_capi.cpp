@@ -39,7 +39,7 @@ Material material_from(float E, float nu, int32_t mm) {
 static uint32_t g_tile_elems = 0;
 static uint32_t g_tile_threads = 256;
 static int g_tile_persistent = 0;
-static int g_tile_alternate = 1;  // odd CG iterations walk the tiles backwards
+static int g_tile_alternate = 0;  // 1: odd CG iterations walk the tiles backwards
 
 template <class T>
 static int dalloc(T** p, size_t count) {

@@ -494,6 +494,18 @@ template <typename V>
 __device__ inline V ld32(const V* base, uint32_t i) {
     return *reinterpret_cast<const V*>(reinterpret_cast<const char*>(base) + i * uint32_t(sizeof(V)));
 }
+// The same load marked non-temporal. The tile stage reads each element's R
+// and packed ids (44 B) once per K*x pass, 231 MB on the 5.24M-tet mesh; read
+// non-temporally they leave the caches to the node vectors (gathered by ~2.5
+// tiles each), the tile-order rest positions and the partials the gather
+// reads next: tile kernel 71.7 -> 65.5 us per K*x, QIE-v step 5.64 -> 5.57 ms.
+// The small static per-tile-node streams and the gather's partials read this
+// way were slower (step 6.1 ms): their 12-byte records share cache lines.
+template <typename V>
+__device__ inline V ld32_nt(const V* base, uint32_t i) {
+    return __builtin_nontemporal_load(reinterpret_cast<const V*>(reinterpret_cast<const char*>(base) +
+                                                                 i * uint32_t(sizeof(V))));
+}
 
 __device__ inline void load_tile_regs(const TileArgs& a, const TileDesc& d, uint32_t gid, uint32_t tid,
                                       TileRegs& g) {
@@ -523,9 +535,9 @@ __device__ inline void load_tile_regs_cold(const TileArgs& a, const TileDesc& d,
     const uint32_t tn = min(tid, d.nn - 1);
     const uint32_t gid = ld32(a.tile_nodes, d.n0 + tn);
     const uint32_t e = d.e0 + min(tid, d.ne - 1);
-    const uint32_t lv = ld32(a.lv, e), rk = ld32(a.rk, e);
+    const uint32_t lv = ld32_nt(a.lv, e), rk = ld32_nt(a.rk, e);
 #pragma unroll
-    for (int k = 0; k < 9; k++) g.R[k] = ld32(a.R + k * (size_t)a.T, e);
+    for (int k = 0; k < 9; k++) g.R[k] = ld32_nt(a.R + k * (size_t)a.T, e);
     g.sp = ld32(a.slot_ptr, d.n0 + min(tid, d.nn)) - 4 * d.e0;
     g.pd = ld32(a.part_dst, d.n0 + tn);
     if (a.ref_tile) {  // static: streamed in tile order, not gathered behind the ids

@@ -282,6 +282,7 @@ def test_leaf_graphs_equal_direct_launches(gpu):
     a = gpu.LeafSolid3D(pos, tets, p, knc=knc)
     b = gpu.LeafSolid3D(pos, tets, p, knc=knc)
     b.set_graphs(False)
+    gpu.set_tile_alternate(1)
     try:
         iters = []
         for k in range(6):
@@ -296,6 +297,6 @@ def test_leaf_graphs_equal_direct_launches(gpu):
         assert any(i % 8 for i in iters)  # some solve stops inside a chunk
         print(f"\nCG iterations {iters}")
     finally:
-        gpu.set_tile_alternate(1)
+        gpu.set_tile_alternate(0)  # the default
         a.close()
         b.close()
