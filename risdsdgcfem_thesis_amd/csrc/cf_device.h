@@ -500,7 +500,8 @@ __device__ inline V ld32(const V* base, uint32_t i) {
 // tiles each), the tile-order rest positions and the partials the gather
 // reads next: tile kernel 71.7 -> 65.5 us per K*x, QIE-v step 5.64 -> 5.57 ms.
 // The small static per-tile-node streams and the gather's partials read this
-// way were slower (step 6.1 ms): their 12-byte records share cache lines.
+// way were slower (step 6.1 ms): their 12-byte records share cache lines; so
+// were the partials stored non-temporally (6.9 ms: the gather reads them next).
 template <typename V>
 __device__ inline V ld32_nt(const V* base, uint32_t i) {
     return __builtin_nontemporal_load(reinterpret_cast<const V*>(reinterpret_cast<const char*>(base) +
@@ -1154,6 +1155,9 @@ __device__ __noinline__ uint32_t degenerate_element(uint32_t e, uint32_t T, cons
     return ok ? 0u : 1u;
 }
 
+// The element streams (connectivity, Dm^-1 in, R out: 88 B per tet) are read
+// and written non-temporally, leaving the caches to the gathered positions
+// (step 5.57 -> 5.53 ms on the 5.24M-tet mesh).
 __device__ inline void begin_eval_block(uint32_t T, const uint32_t* __restrict__ conn, const float* __restrict__ x,
                                         const float* __restrict__ dinv, int rm, float thr, float* __restrict__ R,
                                         float* __restrict__ detF, uint32_t* blk_count, double* blk_sum,
@@ -1167,7 +1171,7 @@ __device__ inline void begin_eval_block(uint32_t T, const uint32_t* __restrict__
     for (uint32_t e = blk * kBlock + threadIdx.x; e < T; e += nblk * kBlock) {
         float xs[4][3];
         for (int j = 0; j < 4; j++) {
-            const uint32_t g = conn[j * (size_t)T + e];
+            const uint32_t g = __builtin_nontemporal_load(conn + j * (size_t)T + e);
             xs[j][0] = x[3 * (size_t)g]; xs[j][1] = x[3 * (size_t)g + 1]; xs[j][2] = x[3 * (size_t)g + 2];
         }
         float Ds[9], G[9], F[9];
@@ -1176,7 +1180,7 @@ __device__ inline void begin_eval_block(uint32_t T, const uint32_t* __restrict__
             Ds[3 * i + 1] = xs[2][i] - xs[0][i];
             Ds[3 * i + 2] = xs[3][i] - xs[0][i];
         }
-        for (int k = 0; k < 9; k++) G[k] = dinv[k * (size_t)T + e];
+        for (int k = 0; k < 9; k++) G[k] = __builtin_nontemporal_load(dinv + k * (size_t)T + e);
         refmath::mul3(Ds, G, F);
         const float d = refmath::det3(F);
         float Rm[9] = {1.f, 0.f, 0.f, 0.f, 1.f, 0.f, 0.f, 0.f, 1.f};
@@ -1204,7 +1208,7 @@ __device__ inline void begin_eval_block(uint32_t T, const uint32_t* __restrict__
             fb++;  // no DoC tracking: R = Id
         }
         if (!r_stored)
-            for (int k = 0; k < 9; k++) R[k * (size_t)T + e] = Rm[k];
+            for (int k = 0; k < 9; k++) __builtin_nontemporal_store(Rm[k], R + k * (size_t)T + e);
         detF[e] = d;
         if (lcm.F) {
             for (int k = 0; k < 9; k++) lcm.F[k * (size_t)T + e] = F[k];
