@@ -239,6 +239,7 @@ def bench_batch3d(cf, args):
         plain_wall = 0.0
         for m in sample:
             s = cf.LeafSolid3D(m["pos"], m["tets"], with_abs(m["params"], abs_eps), knc=m["knc"])
+            s.set_graphs(not args.no_graphs)
             for _ in range(args.warmup):
                 s.fixed_step()
             cf.synchronize()
@@ -580,6 +581,7 @@ def bench_fie(cf, args):
     cf.set_tile_threads(args.tile_threads)
     solid = cf.LeafSolid3D(pos, tets, params)
     reset_tiles(cf)
+    solid.set_graphs(not args.no_graphs)
     solid.set_state(x0, np.zeros_like(x0))
     solid.set_knc(knc, x0.reshape(-1, 3)[knc].ravel())
     solid.fixed_step()
@@ -604,7 +606,7 @@ def bench_fie(cf, args):
                                  poisson_ratio=0.3, damping_ratio=0.0, rm=cf.RM_DAPD, cg_rel_epsilon=1e-4,
                                  cg_abs_epsilon=1e-6, cg_max_iter=50, density=1000.0, nr_max_iter=3)
     solids = [cf.LeafSolid3D(bpos, btets, bp, knc=bknc) for _ in range(16)]
-    multi = cf.MultiSolid3D(solids)
+    multi = cf.MultiSolid3D(solids, use_graphs=not args.no_graphs)
     multi.update(1.0 / 60.0)
     multi.update(1.0 / 60.0)
     cf.synchronize()
