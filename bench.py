@@ -283,6 +283,7 @@ def bench_batch2d(cf, args):
     from risdsdgcfem_thesis_amd import workloads
     meshes = workloads.batch2d(count=args.batch2d_count, cg_max_iter=args.batch_max_iter)
     T = np.array([m["tris"].size // 3 for m in meshes])
+    NN = np.array([m["pos"].size // 2 for m in meshes])
     out = {"workload": "2D batch: 65,536 n x n grids, n in [4,32] (seed 3), QIE-v, CG rel 1e-4, one launch per step",
            "meshes": len(meshes), "steps": args.batch_steps, "warmup": args.warmup,
            "cg_max_iter": args.batch_max_iter, "variants": {}}
@@ -308,6 +309,9 @@ def bench_batch2d(cf, args):
         res = residual_stats(st)
         dev_ms = st[0].step_ms
         tri_matvecs = float(((it + 1) * T).sum())
+        # the 2D K*x convention 48 Ne + 16 Nn bytes per matvec (the data sit in
+        # LDS inside the resident kernel: an HBM-equivalent rate, not HBM traffic)
+        b_alg2 = float(((it + 1) * (48 * T + 16 * NN)).sum())
         batched_sps = len(meshes) * args.batch_steps / wall
         b.close()
         sample = meshes[::64]
@@ -333,12 +337,13 @@ def bench_batch2d(cf, args):
             "batched_ms_per_step": round(wall / args.batch_steps * 1e3, 3),
             "batched_device_ms_per_step": round(float(dev_ms), 3),
             "gtri_matvecs_per_s": round(tri_matvecs / (dev_ms * 1e-3) / 1e9, 2),
+            "kx_hbm_equiv_frac": round(b_alg2 / (dev_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
             "plain_steps_per_s": round(plain_sps, 1),
             "plain_sample": f"{len(sample)} of {len(meshes)} meshes (every 64th)",
             "speedup": round(batched_sps / plain_sps, 1)}
     head = out["variants"][CG_VARIANTS[1][0]]
     out.update({k: head[k] for k in ("batched_steps_per_s", "plain_steps_per_s", "speedup",
-                                     "batched_device_ms_per_step", "gtri_matvecs_per_s")})
+                                     "batched_device_ms_per_step", "gtri_matvecs_per_s", "kx_hbm_equiv_frac")})
     out["headline_variant"] = CG_VARIANTS[1][0]
     if not args.no_cpu_baseline:
         from oracle import orc

@@ -112,6 +112,10 @@ struct cf_cache {
     uint32_t num_degenerate = 0, num_fallback = 0;
     float sum_degenerate = 0.f, min_det = 0.f;
     bool stats_valid = false;
+    // pinned host copies of the BeginEvaluation block statistics
+    uint32_t* h_count = nullptr;
+    double* h_sum = nullptr;
+    float* h_min = nullptr;
     float* F = nullptr;   // LCM / LCMH: 9 planes of F at the evaluation
     float* w = nullptr;   // LCMH: per-element blend weight
     uint8_t* doc = nullptr;  // per-element direction of collapse (ElementCache3::m_DoC)
@@ -186,6 +190,8 @@ struct cf_solid {
     float* scratch_vec = nullptr;
     float *xk = nullptr, *vk = nullptr;  // FIE-dx iterates (m_IE_vec_x_k, m_IE_vec_v_k)
     double* h_part = nullptr;             // pinned: block partials of |b|^2, |y|^2
+    CgScalars* h_cgs = nullptr;           // pinned: the CG record, read once the step has run
+    double* h_energy = nullptr;           // pinned: V() of the step
     cf_cg_result last_ls{};               // last linear solve
     int32_t ls_total = 0;                 // linear-solve iterations of the last fixed step
     bool nonfinite = false;               // a reduction of the last fixed step was NaN or Inf
@@ -332,25 +338,35 @@ static int tetset_bake(cf_tetset* s) {
     return CF_OK;
 }
 
-static int cache_stats_fetch(cf_cache* c) {
-    if (c->stats_valid) return CF_OK;
+// The BeginEvaluation block statistics: copies into pinned memory enqueued on
+// the set's stream, and their sums once the stream has passed them.
+static int cache_stats_enqueue(cf_cache* c) {
     hipStream_t st = c->set->stream;
-    std::vector<uint32_t> cnt(2 * c->nblk);
-    std::vector<double> sum(c->nblk);
-    std::vector<float> mn(c->nblk);
-    CF_CHECK_HIP(hipMemcpyAsync(cnt.data(), c->blk_count, cnt.size() * 4, hipMemcpyDeviceToHost, st));
-    CF_CHECK_HIP(hipMemcpyAsync(sum.data(), c->blk_sum, sum.size() * 8, hipMemcpyDeviceToHost, st));
-    CF_CHECK_HIP(hipMemcpyAsync(mn.data(), c->blk_min, mn.size() * 4, hipMemcpyDeviceToHost, st));
-    CF_CHECK_HIP(hipStreamSynchronize(st));
+    CF_CHECK_HIP(hipMemcpyAsync(c->h_count, c->blk_count, 2 * size_t(c->nblk) * 4, hipMemcpyDeviceToHost, st));
+    CF_CHECK_HIP(hipMemcpyAsync(c->h_sum, c->blk_sum, size_t(c->nblk) * 8, hipMemcpyDeviceToHost, st));
+    CF_CHECK_HIP(hipMemcpyAsync(c->h_min, c->blk_min, size_t(c->nblk) * 4, hipMemcpyDeviceToHost, st));
+    return CF_OK;
+}
+
+static void cache_stats_reduce(cf_cache* c) {
     uint32_t a = 0, f = 0;
     double s = 0;
     float m = 3e38f;
-    for (int i = 0; i < c->nblk; i++) { a += cnt[i]; f += cnt[c->nblk + i]; s += sum[i]; m = std::min(m, mn[i]); }
+    for (int i = 0; i < c->nblk; i++) {
+        a += c->h_count[i]; f += c->h_count[c->nblk + i]; s += c->h_sum[i]; m = std::min(m, c->h_min[i]);
+    }
     c->num_degenerate = a;
     c->num_fallback = f;
     c->sum_degenerate = float(s);
     c->min_det = m;
     c->stats_valid = true;
+}
+
+static int cache_stats_fetch(cf_cache* c) {
+    if (c->stats_valid) return CF_OK;
+    CF_TRY(cache_stats_enqueue(c));
+    CF_CHECK_HIP(hipStreamSynchronize(c->set->stream));
+    cache_stats_reduce(c);
     return CF_OK;
 }
 
@@ -512,7 +528,11 @@ static int cg_finish(cf_solid* s, const CgRun& run, cf_cg_result* res) {
 // skipped launches. Chunks are replayed as recorded graphs (one launch instead
 // of 32) unless the solid's graphs are switched off. The host spins on the
 // chunk's event: a blocking wait added 1-2 us per CG iteration on small meshes.
-static int solid_cg(cf_solid* s, float fM, float fK, cf_cg_result* res) {
+//
+// defer: return as soon as every chunk that can run is queued, with the copy
+// of the scalar record into pinned memory (s->h_cgs) queued behind them; the
+// caller reads the result (cg_result_from) once the stream has passed it.
+static int solid_cg(cf_solid* s, float fM, float fK, cf_cg_result* res, bool defer = false) {
     CgRun run;
     CF_TRY(cg_begin(s, fM, fK, &run));
     CgBuffers& b = s->cg;
@@ -522,6 +542,7 @@ static int solid_cg(cf_solid* s, float fM, float fK, cf_cg_result* res) {
     CF_CHECK_HIP(hipEventRecord(b.ev_chunk[0], s->stream));
     for (uint32_t c = 0;; c++) {
         const uint32_t cur = c & 1, nxt = cur ^ 1;
+        if (defer && run.k >= run.max_it) break;  // the last chunk is queued
         if (run.k < run.max_it) {
             CF_TRY(cg_launch(s, &run, s->use_graphs));
             kend[nxt] = run.k;
@@ -534,6 +555,10 @@ static int solid_cg(cf_solid* s, float fM, float fK, cf_cg_result* res) {
         if (cg_chunk_stopped(s, run, kend[cur])) break;
     }
     run.done = true;
+    if (defer) {
+        CF_CHECK_HIP(hipMemcpyAsync(s->h_cgs, b.st, sizeof(CgScalars), hipMemcpyDeviceToHost, s->stream));
+        return CF_OK;
+    }
     return cg_finish(s, run, res);
 }
 
@@ -917,6 +942,12 @@ int cf_cache_create(cf_tetset* set, const float* inv_mass, cf_cache** out) {
     if (rc == CF_OK) rc = dalloc(&c->blk_min, kMaxPartials);
     if (rc == CF_OK) rc = dalloc(&c->blk_energy, kMaxPartials + 1);  // block partials, then their total
     if (rc == CF_OK) rc = dalloc(&c->doc, std::max<size_t>(1, T));
+    if (rc == CF_OK && (hipHostMalloc(reinterpret_cast<void**>(&c->h_count), 2 * size_t(c->nblk) * 4) != hipSuccess ||
+                        hipHostMalloc(reinterpret_cast<void**>(&c->h_sum), size_t(c->nblk) * 8) != hipSuccess ||
+                        hipHostMalloc(reinterpret_cast<void**>(&c->h_min), size_t(c->nblk) * 4) != hipSuccess)) {
+        set_error("pinned allocation failed");
+        rc = CF_ERR_HIP;
+    }
     if (rc == CF_OK && inv_mass) {
         rc = dalloc(&c->inv_mass, set->dev.N);
         if (rc == CF_OK && hipMemcpy(c->inv_mass, inv_mass, set->dev.N * 4, hipMemcpyHostToDevice) != hipSuccess)
@@ -941,6 +972,9 @@ int cf_cache_destroy(cf_cache* c) {
     if (!c) return CF_OK;
     dfree(c->R); dfree(c->detF); dfree(c->inv_mass); dfree(c->blk_count); dfree(c->blk_sum);
     dfree(c->blk_min); dfree(c->blk_energy); dfree(c->F); dfree(c->w); dfree(c->doc);
+    if (c->h_count) (void)hipHostFree(c->h_count);
+    if (c->h_sum) (void)hipHostFree(c->h_sum);
+    if (c->h_min) (void)hipHostFree(c->h_min);
     delete c;
     return CF_OK;
 }
@@ -1048,14 +1082,20 @@ int cf_df_x(cf_cache* c, const float* dx, float* df) {
     return launch_gather(s->dev, kGatherAccumulate, nt, node_blocks(s->dev.N), s->stream);
 }
 
-int cf_V(cf_cache* c, double* energy) {
-    if (!c || !c->x || !energy) { set_error("cf_V outside Begin/EndEvaluation"); return CF_ERR_INVALID_ARG; }
+// V() on the set's stream, its total copied to h_total (pinned or not) behind it.
+static int cache_energy_enqueue(cf_cache* c, double* h_total) {
     const cf_tetset* s = c->set;
     double* total = c->blk_energy + kMaxPartials;
     CF_TRY(launch_energy(s->dev, c->x, c->R, cache_mat(c), c->blk_energy, c->nblk, total, s->stream));
+    CF_CHECK_HIP(hipMemcpyAsync(h_total, total, sizeof(double), hipMemcpyDeviceToHost, s->stream));
+    return CF_OK;
+}
+
+int cf_V(cf_cache* c, double* energy) {
+    if (!c || !c->x || !energy) { set_error("cf_V outside Begin/EndEvaluation"); return CF_ERR_INVALID_ARG; }
     double h = 0.0;
-    CF_CHECK_HIP(hipMemcpyAsync(&h, total, sizeof(double), hipMemcpyDeviceToHost, s->stream));
-    CF_CHECK_HIP(hipStreamSynchronize(s->stream));
+    CF_TRY(cache_energy_enqueue(c, &h));
+    CF_CHECK_HIP(hipStreamSynchronize(c->set->stream));
     *energy = h;
     return CF_OK;
 }
@@ -1193,6 +1233,8 @@ int cf_solid_destroy(cf_solid* s) {
     dfree(s->knc_mask); dfree(s->knc_target); dfree(s->scratch_vec);
     dfree(s->xk); dfree(s->vk);
     if (s->h_part) (void)hipHostFree(s->h_part);
+    if (s->h_cgs) (void)hipHostFree(s->h_cgs);
+    if (s->h_energy) (void)hipHostFree(s->h_energy);
     cg_free(s->cg);
     if (s->ev0) (void)hipEventDestroy(s->ev0);
     if (s->ev1) (void)hipEventDestroy(s->ev1);
@@ -1237,7 +1279,9 @@ int cf_solid_create(uint32_t N, const float* ref, uint32_t T, const uint32_t* te
     if (rc == CF_OK) rc = cg_alloc(s->cg, uint32_t(n3));
     if (rc == CF_OK) rc = dalloc(&s->xk, n3);
     if (rc == CF_OK) rc = dalloc(&s->vk, n3);
-    if (rc == CF_OK && hipHostMalloc(reinterpret_cast<void**>(&s->h_part), kMaxPartials * sizeof(double)) != hipSuccess) {
+    if (rc == CF_OK && (hipHostMalloc(reinterpret_cast<void**>(&s->h_part), kMaxPartials * sizeof(double)) != hipSuccess ||
+                        hipHostMalloc(reinterpret_cast<void**>(&s->h_cgs), sizeof(CgScalars)) != hipSuccess ||
+                        hipHostMalloc(reinterpret_cast<void**>(&s->h_energy), sizeof(double)) != hipSuccess)) {
         set_error("pinned allocation failed");
         rc = CF_ERR_HIP;
     }
@@ -1438,19 +1482,27 @@ static int solid_fixed_step(cf_solid* s, float dt, cf_step_stats* stats) {
     // A = (1 + dt c0) M + (dt^2 + dt c1) K  (:1300-1302)
     const float fM = 1.0f + dt * s->rayleigh[0];
     const float fK = dt * dt + dt * s->rayleigh[1];
+    // The rest of the step is queued behind the CG without waiting for it;
+    // the CG record, V() and the BeginEvaluation statistics come back through
+    // pinned memory with the step's one synchronisation.
     cf_cg_result res{};
-    CF_TRY(solid_cg(s, fM, fK, &res));
+    CF_TRY(solid_cg(s, fM, fK, &res, true));
     // v = y ; x = x + dt y (:1322-1327) ; air drag (:982-984)
     CF_TRY(launch_integrate(n3, s->x, s->v, s->y, dt, s->air_drag_per_second, st));
-    double energy = 0.0;
-    if (s->p.compute_energy) CF_TRY(cf_V(s->cache, &energy));  // V() with the new x (:977)
-    CF_TRY(cf_end_evaluation(s->cache));
+    *s->h_energy = 0.0;
+    if (s->p.compute_energy) CF_TRY(cache_energy_enqueue(s->cache, s->h_energy));  // V() with the new x (:977)
+    if (stats) CF_TRY(cache_stats_enqueue(s->cache));
     CF_CHECK_HIP(hipEventRecord(s->ev1, st));
     CF_CHECK_HIP(hipEventSynchronize(s->ev1));
+    CF_TRY(cf_end_evaluation(s->cache));
+    cg_result_from(*s->h_cgs, std::min<uint32_t>(s->p.cg_max_iter, kMaxCgIter), &res);
+    if (res.reason == CF_CG_NONFINITE) s->nonfinite = true;
+    s->last_ls = res;
+    const double energy = *s->h_energy;
     if (stats) {
         float ms = 0.f;
         CF_CHECK_HIP(hipEventElapsedTime(&ms, s->ev0, s->ev1));
-        CF_TRY(cache_stats_fetch(s->cache));
+        cache_stats_reduce(s->cache);
         stats->num_fixed_steps = 1;
         stats->cg_iters = res.iterations;
         stats->cg_rel_residual = res.rel_residual;
