@@ -546,6 +546,7 @@ def bench_single(cf, args):
                  "alternate": args.tile_alternate},
         "cg_workgroups": cf.get_cg_workgroups(),
     }
+    pcie = pcie_inclusive(cf, solid, pos, tets, x0, args)
     cpu = None
     if not args.no_cpu_baseline:
         log("[bench] cpu baseline (oracle K*x, one core)")
@@ -561,10 +562,51 @@ def bench_single(cf, args):
         "config": {"workload": "single 64x64x256 cube-grid mesh, plain GPU drop-in (QIE-v, CG rel 1e-4 max 50)",
                    "tets": T, "nodes": N, "rotations": "polar (DAPD)", "parallelism": "single GPU"},
         "steps_per_s": round(args.steps / wall, 3), "cg_iters_mean": round(float(np.mean(iters)), 2),
-        "roofline": roofline, "cpu_baseline": cpu,
+        "roofline": roofline, "cpu_baseline": cpu, "pcie_inclusive": pcie,
     }
     solid.close()
     return out
+
+
+def pcie_inclusive(cf, solid, pos, tets, x0, args, reps=5):
+    """Rates when the caller's arrays live in host memory (not `value`): a K*x
+    through the element-set boundary with the input vector uploaded and the
+    accumulated output copied in and back per call (what the IForce3 adapter
+    does with the leaf's ParticleSystem arrays), and a leaf step followed by a
+    read-back of positions and velocities. Pageable numpy arrays."""
+    N, T = pos.size // 3, tets.size // 4
+    cf.set_tile_elems(args.tile_elems)
+    cf.set_tile_threads(args.tile_threads)
+    ts = cf.TetrahedronElementSet3(pos, tets, cf.default_fem_params(young_modulus=1e5, poisson_ratio=0.3,
+                                                                   rm=cf.RM_DAPD))
+    reset_tiles(cf)
+    dx = cf.DeviceArray.from_host(x0)
+    ts.begin_evaluation(dx)
+    v = np.random.default_rng(1).standard_normal(3 * N).astype(np.float32)
+    out = np.zeros(3 * N, np.float32)
+    dv, dout = cf.DeviceArray(3 * N), cf.DeviceArray(3 * N)
+    ts.df_x(dv, dout)
+    cf.synchronize()
+    t = time.perf_counter()
+    for _ in range(reps):
+        dv.upload(v)
+        dout.upload(out)
+        ts.df_x(dv, dout)
+        out = dout.numpy()
+    kx_ms = (time.perf_counter() - t) / reps * 1e3
+    ts.end_evaluation()
+    ts.close()
+    cf.synchronize()
+    t = time.perf_counter()
+    for _ in range(reps):
+        solid.fixed_step()
+        solid.state()
+    step_ms = (time.perf_counter() - t) / reps * 1e3
+    log(f"[bench] host arrays: K*x {kx_ms:.2f} ms per call, step + state read-back {step_ms:.2f} ms")
+    return {"kx_ms": round(kx_ms, 3), "kx_gtets": round(T / (kx_ms * 1e-3) / 1e9, 2),
+            "step_with_state_readback_ms": round(step_ms, 3),
+            "what": "K*x with the input uploaded and the output copied in and back per call (3 x 13 MB over "
+                    "PCIe, pageable host arrays); a QIE-v step followed by a read-back of x and v (26 MB)"}
 
 
 def bench_fie(cf, args):

@@ -75,6 +75,12 @@ __global__ __launch_bounds__(TB) void k_tile_forces(TileArgs a) {
     uint32_t* s_sp = L.sp;   // slot-run starts
 
     const uint32_t tid = threadIdx.x;
+    // Every argument the prologue and the first loads use, read in one batch
+    // of scalar loads (the compiler would otherwise fetch them in three
+    // dependent rounds, as each branch first needs them).
+    asm volatile("" ::"s"(a.T), "s"(a.num_tiles), "s"(a.reverse), "s"(a.uniform_te), "s"(a.skip),
+                 "s"(a.tile_node_ptr), "s"(a.tile_elem_ptr), "s"(a.tile_nodes), "s"(a.lv), "s"(a.rk), "s"(a.R),
+                 "s"(a.slot_ptr), "s"(a.part_dst), "s"(a.ref_tile), "s"(a.in));
     const uint32_t gx = gridDim.x / 8, xcd = blockIdx.x % 8, j = blockIdx.x / 8;
     const uint32_t per = (a.num_tiles + 7) / 8;
     const uint32_t lo = xcd * per;
@@ -221,6 +227,11 @@ __global__ __launch_bounds__(kBlock) void k_gather(GatherArgs a) {
 template <bool INIT>
 __global__ __launch_bounds__(kBlock) void k_gather_matvec(GatherArgs a, uint32_t npt) {
     __shared__ double sh[kBlock / 64];
+    // the arguments in one batch of scalar loads, before the skip test needs
+    // the first of them (as in the tile kernel)
+    asm volatile("" ::"s"(a.N), "s"(a.node_part_ptr), "s"(a.partials), "s"(a.t.d), "s"(a.t.out), "s"(a.t.mass),
+                 "s"(a.t.fM), "s"(a.t.fK), "s"(a.t.knc_mask), "s"(a.t.mass_array), "s"(a.t.partials), "s"(a.skip),
+                 "s"(npt));
     if (a.skip && *a.skip) return;
     const double s =
         block_sum(gather_matvec_chunk<INIT>(a.N, blockIdx.x, npt, a.node_part_ptr, a.partials, a.t), sh);
