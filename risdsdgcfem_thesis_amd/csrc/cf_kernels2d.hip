@@ -40,15 +40,20 @@ struct Grp {
             __syncthreads();
         }
     }
-    __device__ static double sum(double v, double* red) {
+    // red holds two buffers of 8 wave sums, used alternately (par flips on
+    // every call, the same in every thread): a buffer is written again only
+    // two sums later, after every thread has passed the other sum's barrier
+    // and so has finished reading it. One barrier per sum.
+    __device__ static double sum(double v, double* red, uint32_t& par) {
         v = wave_sum(v);
         if (GW == 1) return v;
-        __syncthreads();
-        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+        double* rb = red + 8 * par;
+        par ^= 1u;
+        if ((threadIdx.x & 63) == 0) rb[threadIdx.x >> 6] = v;
         __syncthreads();
         const uint32_t w0 = group() * GW;
         double r = 0.0;
-        for (int i = 0; i < GW; i++) r += red[w0 + i];
+        for (int i = 0; i < GW; i++) r += rb[w0 + i];
         return r;
     }
     __device__ static float min(float v, float* red) {
@@ -62,14 +67,20 @@ struct Grp {
         for (int i = 1; i < GW; i++) r = fminf(r, red[w0 + i]);
         return r;
     }
-    // whether any group of the workgroup is still active (its own flag for GW == 1)
-    __device__ static bool any(bool active, uint32_t* flags) {
-        if (GW == 1) return active;
-        __syncthreads();
-        if (lane() == 0) flags[group()] = active ? 1u : 0u;
+    // whether any group of the workgroup is still active: its own flag when
+    // the group is alone in its barrier sequence (GW == 1: a wavefront; GW ==
+    // 8: the whole workgroup, where `active` is uniform); else through two
+    // alternating flag buffers, one barrier (as sum). The barrier also orders
+    // the group's LDS writes before the call against the reads after it.
+    static constexpr bool kAlone = GW == 1 || GW == 8;
+    __device__ static bool any(bool active, uint32_t* flags, uint32_t& par) {
+        if (kAlone) return active;
+        uint32_t* fb = flags + 8 * par;
+        par ^= 1u;
+        if (lane() == 0) fb[group()] = active ? 1u : 0u;
         __syncthreads();
         uint32_t f = 0;
-        for (int i = 0; i < 8 / GW; i++) f |= flags[i];
+        for (int i = 0; i < 8 / GW; i++) f |= fb[i];
         return f != 0;
     }
 };
@@ -384,20 +395,25 @@ __device__ inline void tri_forces(const Tri& t, const Rot& R, const float H[4], 
 // at most one); no two triangles of a colour share a node, so the order
 // within a colour does not change a node's sum. `run` false: the group only
 // takes part in the barriers.
-template <int GW, bool FE>
+// ZERO: clear the node forces first (behind a barrier); else the caller has
+// cleared them (each thread its own nodes, as it consumed them) and ordered
+// its LDS writes before the call with a barrier of its own.
+template <int GW, bool FE, bool ZERO = true>
 __device__ inline void accumulate_forces(const Tri (&tr)[kTriRegs], const float4* s_R, uint32_t lane, uint32_t N,
                                          uint32_t T, uint32_t C, uint32_t CL, bool run, const uint16_t* cs,
                                          const float2* ref, const Material& m, const float2* s_v, float2* s_f) {
 #pragma clang fp contract(off)
     constexpr int G = Grp<GW>::G;
-    if (run) {
+    if (ZERO) {
+        if (run) {
 #pragma unroll
-        for (int k = 0; k < kNodeRegs; k++) {
-            const uint32_t n = lane + k * G;
-            if (n < N) s_f[n] = make_float2(0.f, 0.f);
+            for (int k = 0; k < kNodeRegs; k++) {
+                const uint32_t n = lane + k * G;
+                if (n < N) s_f[n] = make_float2(0.f, 0.f);
+            }
         }
+        Grp<GW>::bar();
     }
-    Grp<GW>::bar();
     for (uint32_t c = 0; c < CL; c++) {
         if (run && c < C) {
             const uint32_t lo = cs[c], hi = min(uint32_t(cs[c + 1]), T);
@@ -573,6 +589,7 @@ __device__ void mesh_step(const Batch2Args& a, uint32_t cmax, float2* s_v_all, f
         const uint32_t n = lane + k * G;
         if (n >= N) continue;
         const float2 f = s_f[n], yk = s_y[n];
+        s_f[n] = make_float2(0.f, 0.f);  // cleared for the first iteration's forces
         const float fac = mass[k] * M.fM;
         float q0 = -M.fK * f.x + fac * yk.x, q1 = -M.fK * f.y + fac * yk.y;
         if (kn[k]) { q0 = 0.f; q1 = 0.f; }
@@ -581,7 +598,8 @@ __device__ void mesh_step(const Batch2Args& a, uint32_t cmax, float2* s_v_all, f
         r[k] = make_float2(r0, r1);
         acc += (double)r0 * r0 + (double)r1 * r1;
     }
-    const double delta0 = Gp::sum(acc, red);
+    uint32_t rpar = 0, fpar = 0;  // alternating reduction and flag buffers
+    const double delta0 = Gp::sum(acc, red, rpar);
     double delta = delta0;
     const float abs_sq = M.abs_eps * M.abs_eps;
     const float rel_sq = M.rel * M.rel;
@@ -599,10 +617,10 @@ __device__ void mesh_step(const Batch2Args& a, uint32_t cmax, float2* s_v_all, f
         const uint32_t n = lane + k * G;
         if (n < N) s_v[n] = r[k];  // d = r
     }
-    while (Gp::any(reason == 0, flags)) {
+    while (Gp::any(reason == 0, flags, fpar)) {
         const bool run = reason == 0;
-        Gp::bar();
-        accumulate_forces<GW, false>(tr, s_R, lane, N, T, C, CL, run, cs, ref, M.m, s_v, s_f);
+        if (Gp::kAlone) Gp::bar();  // else any() has just passed a barrier
+        accumulate_forces<GW, false, false>(tr, s_R, lane, N, T, C, CL, run, cs, ref, M.m, s_v, s_f);
         float2 q[kNodeRegs];
         acc = 0.0;
         if (run) {
@@ -611,6 +629,7 @@ __device__ void mesh_step(const Batch2Args& a, uint32_t cmax, float2* s_v_all, f
                 const uint32_t n = lane + k * G;
                 if (n >= N) continue;
                 const float2 f = s_f[n], dk = s_v[n];
+                s_f[n] = make_float2(0.f, 0.f);  // cleared for the next iteration's forces
                 const float fac = mass[k] * M.fM;
                 float q0 = -M.fK * f.x + fac * dk.x, q1 = -M.fK * f.y + fac * dk.y;
                 if (kn[k]) { q0 = 0.f; q1 = 0.f; }
@@ -618,7 +637,7 @@ __device__ void mesh_step(const Batch2Args& a, uint32_t cmax, float2* s_v_all, f
                 acc += (double)dk.x * q0 + (double)dk.y * q1;
             }
         }
-        const double dq = Gp::sum(acc, red);
+        const double dq = Gp::sum(acc, red, rpar);
         if (run && fabs(dq) < abs2) reason = CF_CG_BREAKDOWN;
         const bool upd = run && reason == 0;
         const float alpha = upd ? (float)(delta / dq) : 0.f;
@@ -636,7 +655,7 @@ __device__ void mesh_step(const Batch2Args& a, uint32_t cmax, float2* s_v_all, f
                 acc += (double)r[k].x * r[k].x + (double)r[k].y * r[k].y;
             }
         }
-        const double dn = Gp::sum(acc, red);
+        const double dn = Gp::sum(acc, red, rpar);
         float beta = 0.f;
         if (upd) {
             beta = (float)(dn / delta);
@@ -646,7 +665,10 @@ __device__ void mesh_step(const Batch2Args& a, uint32_t cmax, float2* s_v_all, f
             else if (!(dn > target)) reason = CF_CG_CONVERGED_REL;
             else if (!(it < M.max_iter)) reason = CF_CG_MAX_ITER;
         }
-        Gp::bar();  // every read of s_v (d) is done
+        // Every read of s_v (d) by another thread was in the colour passes,
+        // ordered before this point by their last barrier and the two sums'
+        // barriers; a lone wavefront keeps its wave barrier.
+        if (GW == 1) Gp::bar();
         if (upd && reason == 0) {
 #pragma unroll
             for (int k = 0; k < kNodeRegs; k++) {
@@ -695,10 +717,10 @@ __device__ void mesh_step(const Batch2Args& a, uint32_t cmax, float2* s_v_all, f
             en += 0.5 * (double)t.A * (double)sH;
         }
     }
-    const double energy = Gp::sum(en, red);
-    const double nd = Gp::sum(ndeg, red);
-    const double sd = Gp::sum(sdeg, red);
-    const double nf = Gp::sum(nfb, red);
+    const double energy = Gp::sum(en, red, rpar);
+    const double nd = Gp::sum(ndeg, red, rpar);
+    const double sd = Gp::sum(sdeg, red, rpar);
+    const double nf = Gp::sum(nfb, red, rpar);
     const float md = Gp::min(mind, redm);
     if (real && lane == 0) {
         Sim2Out& o = a.out[sim];
@@ -728,9 +750,9 @@ __global__ __launch_bounds__(k2Threads, 4) void k2_step(Batch2Args a) {
     __shared__ float2 s_f[k2Threads * kNodeRegs];
     __shared__ float2 s_y[k2Threads * kNodeRegs];
     __shared__ float4 s_R[k2Threads * kTriRegs];
-    __shared__ double red[8];
+    __shared__ double red[16];
     __shared__ float redm[8];
-    __shared__ uint32_t flags[8];
+    __shared__ uint32_t flags[16];
     const uint32_t gw = a.work[blockIdx.x].gw, cmax = a.work[blockIdx.x].cmax;
     switch (gw) {
         case 1: mesh_step<1>(a, cmax, s_v, s_f, s_y, s_R, red, redm, flags); break;
