@@ -403,6 +403,15 @@ int cf_set_tile_persistent(int persistent);
    node data and gains more). Results are the same either way. Takes effect
    for solves enqueued or recorded afterwards. */
 int cf_set_tile_alternate(int alternate);
+/* 1 (default): a QIE-v leaf step of a C_WRP solid without the exact rotation
+   differential opens its evaluation with one tile pass that computes each
+   element's F, det F and rotation and, from the rotation in registers, the
+   elastic forces f_e(x) and the CG start's K y, instead of BeginEvaluation
+   followed by two tile passes; 0: the separate passes. Results are the same
+   bit for bit (the sum of the degenerate elements' det F is summed per tile
+   instead of per block, which can move its last bit). Takes effect from the
+   next step. */
+int cf_set_fused_front(int enable);
 int cf_get_tile_elems(uint32_t* tile_elems);
 /* Workgroups of the two CG iteration kernels of the leaf solvers (alpha with
    x, r; beta and the stop test with d), each summing the iteration's FP64

@@ -16,6 +16,6 @@ from ._capi import (  # noqa: F401
 from .fem import (  # noqa: F401
     DeviceArray, device_count, mesh_box, mesh_box2, mesh_box_baked, TetShape, TetrahedronElementSet3, ILSS_CG, LeafSolid3D, MultiSolid3D, Batch,
     Batch2, LeafSolid2D,
-    default_fem_params, default_solid_params, rayleigh_coefficients, serve, cg_solve_static, rotations2, set_tile_elems, set_tile_threads, set_tile_persistent, set_tile_alternate, get_tile_elems, set_cg_workgroups,
+    default_fem_params, default_solid_params, rayleigh_coefficients, serve, cg_solve_static, rotations2, set_tile_elems, set_tile_threads, set_tile_persistent, set_tile_alternate, set_fused_front, get_tile_elems, set_cg_workgroups,
     get_cg_workgroups, synchronize,
 )

@@ -182,6 +182,7 @@ SIGNATURES = {
     "cf_get_cg_workgroups": (C.c_int, [_u]),
     "cf_set_tile_persistent": (C.c_int, [C.c_int]),
     "cf_set_tile_alternate": (C.c_int, [C.c_int]),
+    "cf_set_fused_front": (C.c_int, [C.c_int]),
     "cf_batch_create": (C.c_int, [C.c_uint64, C.POINTER(_vp)]),
     "cf_batch_destroy": (C.c_int, [_vp]),
     "cf_batch_set_tiles": (C.c_int, [_vp, C.c_uint32, C.c_uint32]),

@@ -40,6 +40,7 @@ static uint32_t g_tile_elems = 0;
 static uint32_t g_tile_threads = 256;
 static int g_tile_persistent = 0;
 static int g_tile_alternate = 0;  // 1: odd CG iterations walk the tiles backwards
+static int g_fused_front = 1;     // 1: QIE-v steps fuse BeginEvaluation, f_e and the CG start's K y
 
 template <class T>
 static int dalloc(T** p, size_t count) {
@@ -116,6 +117,9 @@ struct cf_cache {
     uint32_t* h_count = nullptr;
     double* h_sum = nullptr;
     float* h_min = nullptr;
+    // entries of the last evaluation's statistics: one per BeginEvaluation
+    // block (nblk), or one per tile after the fused QIE-v front
+    int stat_n = 1;
     float* F = nullptr;   // LCM / LCMH: 9 planes of F at the evaluation
     float* w = nullptr;   // LCMH: per-element blend weight
     uint8_t* doc = nullptr;  // per-element direction of collapse (ElementCache3::m_DoC)
@@ -192,6 +196,7 @@ struct cf_solid {
     double* h_part = nullptr;             // pinned: block partials of |b|^2, |y|^2
     CgScalars* h_cgs = nullptr;           // pinned: the CG record, read once the step has run
     double* h_energy = nullptr;           // pinned: V() of the step
+    float* ay_part = nullptr;             // K y tile partials of the fused QIE-v front (3 num_parts)
     cf_cg_result last_ls{};               // last linear solve
     int32_t ls_total = 0;                 // linear-solve iterations of the last fixed step
     bool nonfinite = false;               // a reduction of the last fixed step was NaN or Inf
@@ -342,9 +347,10 @@ static int tetset_bake(cf_tetset* s) {
 // the set's stream, and their sums once the stream has passed them.
 static int cache_stats_enqueue(cf_cache* c) {
     hipStream_t st = c->set->stream;
-    CF_CHECK_HIP(hipMemcpyAsync(c->h_count, c->blk_count, 2 * size_t(c->nblk) * 4, hipMemcpyDeviceToHost, st));
-    CF_CHECK_HIP(hipMemcpyAsync(c->h_sum, c->blk_sum, size_t(c->nblk) * 8, hipMemcpyDeviceToHost, st));
-    CF_CHECK_HIP(hipMemcpyAsync(c->h_min, c->blk_min, size_t(c->nblk) * 4, hipMemcpyDeviceToHost, st));
+    const size_t n = size_t(c->stat_n);
+    CF_CHECK_HIP(hipMemcpyAsync(c->h_count, c->blk_count, 2 * n * 4, hipMemcpyDeviceToHost, st));
+    CF_CHECK_HIP(hipMemcpyAsync(c->h_sum, c->blk_sum, n * 8, hipMemcpyDeviceToHost, st));
+    CF_CHECK_HIP(hipMemcpyAsync(c->h_min, c->blk_min, n * 4, hipMemcpyDeviceToHost, st));
     return CF_OK;
 }
 
@@ -352,8 +358,8 @@ static void cache_stats_reduce(cf_cache* c) {
     uint32_t a = 0, f = 0;
     double s = 0;
     float m = 3e38f;
-    for (int i = 0; i < c->nblk; i++) {
-        a += c->h_count[i]; f += c->h_count[c->nblk + i]; s += c->h_sum[i]; m = std::min(m, c->h_min[i]);
+    for (int i = 0; i < c->stat_n; i++) {
+        a += c->h_count[i]; f += c->h_count[c->stat_n + i]; s += c->h_sum[i]; m = std::min(m, c->h_min[i]);
     }
     c->num_degenerate = a;
     c->num_fallback = f;
@@ -389,7 +395,9 @@ struct CgRun {
     bool done = false;
 };
 
-static int cg_begin(cf_solid* s, float fM, float fK, CgRun* run) {
+// ay: the K y tile partials when the fused front computed them (else null:
+// the tile pass runs here).
+static int cg_begin(cf_solid* s, float fM, float fK, CgRun* run, const float* ay = nullptr) {
     const TetDevice& t = s->set->dev;
     hipStream_t st = s->stream;
     CgBuffers& b = s->cg;
@@ -404,8 +412,14 @@ static int cg_begin(cf_solid* s, float fM, float fK, CgRun* run) {
     // r = S(b - A y), d = r and the r.r partials in the gather of A y
     NodeTerms ni = nt;
     ni.d = s->y; ni.b = s->bvec; ni.r_out = b.r; ni.d_out = b.d; ni.partials = b.part_b;
-    CF_TRY(launch_forces(t, kModeDfx, s->y, s->cache->R, cache_mat(s->cache), st));
-    CF_TRY(launch_gather(t, kGatherCgInit, ni, 0, st));
+    if (ay) {
+        TetDevice ta = t;
+        ta.partials = const_cast<float*>(ay);
+        CF_TRY(launch_gather(ta, kGatherCgInit, ni, 0, st));
+    } else {
+        CF_TRY(launch_forces(t, kModeDfx, s->y, s->cache->R, cache_mat(s->cache), st));
+        CF_TRY(launch_gather(t, kGatherCgInit, ni, 0, st));
+    }
     CF_TRY(launch_cg_start(b.st, b.part_b, matvec_chunks(s->N), s->p.cg_rel_epsilon, s->p.cg_abs_epsilon,
                            run->max_it, st));
     nt.partials = b.part_a;
@@ -532,9 +546,10 @@ static int cg_finish(cf_solid* s, const CgRun& run, cf_cg_result* res) {
 // defer: return as soon as every chunk that can run is queued, with the copy
 // of the scalar record into pinned memory (s->h_cgs) queued behind them; the
 // caller reads the result (cg_result_from) once the stream has passed it.
-static int solid_cg(cf_solid* s, float fM, float fK, cf_cg_result* res, bool defer = false) {
+static int solid_cg(cf_solid* s, float fM, float fK, cf_cg_result* res, bool defer = false,
+                    const float* ay = nullptr) {
     CgRun run;
-    CF_TRY(cg_begin(s, fM, fK, &run));
+    CF_TRY(cg_begin(s, fM, fK, &run, ay));
     CgBuffers& b = s->cg;
     uint32_t kend[2] = {0, 0};
     CF_TRY(cg_launch(s, &run, s->use_graphs));
@@ -935,16 +950,20 @@ int cf_cache_create(cf_tetset* set, const float* inv_mass, cf_cache** out) {
     c->set = set;
     const uint32_t T = set->dev.T;
     c->nblk = node_blocks(T);
+    c->stat_n = c->nblk;
+    // statistics entries: per BeginEvaluation block, or per tile (fused front)
+    const size_t ns = std::max<size_t>(kMaxPartials, set->dev.num_tiles);
+    const size_t nh = std::max<size_t>(c->nblk, set->dev.num_tiles);
     int rc = dalloc(&c->R, 9 * (size_t)T);
     if (rc == CF_OK) rc = dalloc(&c->detF, T);
-    if (rc == CF_OK) rc = dalloc(&c->blk_count, 2 * (size_t)kMaxPartials);
-    if (rc == CF_OK) rc = dalloc(&c->blk_sum, kMaxPartials);
-    if (rc == CF_OK) rc = dalloc(&c->blk_min, kMaxPartials);
+    if (rc == CF_OK) rc = dalloc(&c->blk_count, 2 * ns);
+    if (rc == CF_OK) rc = dalloc(&c->blk_sum, ns);
+    if (rc == CF_OK) rc = dalloc(&c->blk_min, ns);
     if (rc == CF_OK) rc = dalloc(&c->blk_energy, kMaxPartials + 1);  // block partials, then their total
     if (rc == CF_OK) rc = dalloc(&c->doc, std::max<size_t>(1, T));
-    if (rc == CF_OK && (hipHostMalloc(reinterpret_cast<void**>(&c->h_count), 2 * size_t(c->nblk) * 4) != hipSuccess ||
-                        hipHostMalloc(reinterpret_cast<void**>(&c->h_sum), size_t(c->nblk) * 8) != hipSuccess ||
-                        hipHostMalloc(reinterpret_cast<void**>(&c->h_min), size_t(c->nblk) * 4) != hipSuccess)) {
+    if (rc == CF_OK && (hipHostMalloc(reinterpret_cast<void**>(&c->h_count), 2 * nh * 4) != hipSuccess ||
+                        hipHostMalloc(reinterpret_cast<void**>(&c->h_sum), nh * 8) != hipSuccess ||
+                        hipHostMalloc(reinterpret_cast<void**>(&c->h_min), nh * 4) != hipSuccess)) {
         set_error("pinned allocation failed");
         rc = CF_ERR_HIP;
     }
@@ -1007,6 +1026,7 @@ int cf_begin_evaluation(cf_cache* c, const float* prev_x, const float* x, const 
     dg.nl_exponent = s->params.dapd_nl_exponent;
     CF_TRY(launch_begin_eval(s->dev, x, s->params.rm, s->params.degenerate_threshold_det_f, c->R, c->detF,
                              c->blk_count, c->blk_sum, c->blk_min, c->nblk, s->stream, lo, dg));
+    c->stat_n = c->nblk;
     c->stats_valid = false;
     return CF_OK;
 }
@@ -1231,7 +1251,7 @@ int cf_solid_destroy(cf_solid* s) {
     if (!s) return CF_OK;
     dfree(s->x); dfree(s->v); dfree(s->x0); dfree(s->bvec); dfree(s->y);
     dfree(s->knc_mask); dfree(s->knc_target); dfree(s->scratch_vec);
-    dfree(s->xk); dfree(s->vk);
+    dfree(s->xk); dfree(s->vk); dfree(s->ay_part);
     if (s->h_part) (void)hipHostFree(s->h_part);
     if (s->h_cgs) (void)hipHostFree(s->h_cgs);
     if (s->h_energy) (void)hipHostFree(s->h_energy);
@@ -1433,6 +1453,47 @@ static int nonfinite_status(cf_solid* s) {
 
 // FixedStep (:898-990). The total time is the caller's: Step advances it once
 // per call (:229-230), a direct fixed step by its own dt.
+// Whether a QIE-v step may open its evaluation with the fused front: C_WRP
+// without the exact rotation differential (nothing else is kept per element).
+static bool front_fusable(const cf_solid* s) {
+    const cf_tetset* set = s->set;
+    const TetDevice& t = set->dev;
+    const uint64_t lim = uint64_t(1) << 32;  // its buffer accesses take 32-bit byte offsets
+    return g_fused_front && s->p.integrator != CF_IT_FIE_DX && t.num_tiles > 0 && set->params.mm == CF_MM_C_WRP &&
+           set->params.dm != CF_DM_EXACT && 36 * uint64_t(t.T) < lim && 12 * uint64_t(t.N) < lim &&
+           12 * uint64_t(t.num_parts) < lim && 8 * uint64_t(t.num_tiles) < lim;
+}
+
+// BeginEvaluation(prev_x = x0, x, v, t) (FixedStep :927) fused with f_e(x)
+// into the tile partials and the CG start's K y into s->ay_part: the same R,
+// det F, DoC and partials as cf_begin_evaluation followed by the two tile
+// passes; y is the right-hand side gather's (yt).
+static int begin_front(cf_solid* s, const NodeTerms& yt) {
+    cf_cache* c = s->cache;
+    const cf_tetset* set = s->set;
+    if (!s->ay_part) CF_TRY(dalloc(&s->ay_part, 3 * (size_t)set->dev.num_parts));
+    c->prev_x = s->x0; c->x = s->x; c->v = s->v;
+    c->timestep = s->total_time - c->time;
+    c->time = s->total_time;
+    FrontOut fo;
+    fo.rm = set->params.rm;
+    fo.thr = set->params.degenerate_threshold_det_f;
+    fo.R = c->R;
+    fo.detF = c->detF;
+    fo.st_count = c->blk_count; fo.st_sum = c->blk_sum; fo.st_min = c->blk_min;
+    fo.dg.prev_x = s->x0;
+    fo.dg.doc = c->doc;
+    fo.dg.vol = set->dev.vol;
+    fo.dg.l_factor = set->params.dapd_l_factor;
+    fo.dg.nl_factor = set->params.dapd_nl_factor;
+    fo.dg.nl_exponent = set->params.dapd_nl_exponent;
+    fo.partials2 = s->ay_part;
+    CF_TRY(launch_qie_front(set->dev, s->x, cache_mat(c), fo, yt, s->stream));
+    c->stat_n = int(set->dev.num_tiles);
+    c->stats_valid = false;
+    return CF_OK;
+}
+
 static int solid_fixed_step(cf_solid* s, float dt, cf_step_stats* stats) {
     if (!s || !(dt >= 0) || (dt == 0 && s->p.integrator != CF_IT_FIE_DX)) return CF_ERR_INVALID_ARG;
     if (s->p.reference_order) return solid_fixed_step_exact(s, dt, stats);
@@ -1442,8 +1503,20 @@ static int solid_fixed_step(cf_solid* s, float dt, cf_step_stats* stats) {
     hipStream_t st = s->stream;
     const uint32_t n3 = 3 * s->N;
     CF_CHECK_HIP(hipEventRecord(s->ev0, st));
-    // BeginEvaluation(prev_x, x, v, t) (FixedStep :927)
-    CF_TRY(cf_begin_evaluation(s->cache, s->x0, s->x, s->v, s->total_time));
+    // b = dt (M g + f_e(x)) + M v ; y = v ; KNC rows (Step_QuasiImplicitEuler_v :1209-1295)
+    NodeTerms nt;
+    nt.out = s->bvec; nt.y = s->y; nt.x = s->x; nt.v = s->v;
+    nt.knc_mask = s->num_knc ? s->knc_mask : nullptr;
+    nt.knc_target = s->knc_target;
+    nt.mass = s->node_mass;
+    nt.dt = dt; nt.inv_dt = 1.0f / dt;
+    nt.g[0] = s->p.gravity[0]; nt.g[1] = s->p.gravity[1]; nt.g[2] = s->p.gravity[2];
+    nt.warm_start = s->p.warm_start;
+    // BeginEvaluation(prev_x, x, v, t) (FixedStep :927), fused with f_e and
+    // the CG start's K y where the material allows
+    const bool fused = front_fusable(s);
+    if (fused) CF_TRY(begin_front(s, nt));
+    else CF_TRY(cf_begin_evaluation(s->cache, s->x0, s->x, s->v, s->total_time));
     // m_vecPos0 <- x (:936-937)
     CF_CHECK_HIP(hipMemcpyAsync(s->x0, s->x, n3 * 4, hipMemcpyDeviceToDevice, st));
     if (s->p.integrator == CF_IT_FIE_DX) {
@@ -1468,16 +1541,7 @@ static int solid_fixed_step(cf_solid* s, float dt, cf_step_stats* stats) {
         }
         return nonfinite_status(s);
     }
-    // b = dt (M g + f_e(x)) + M v ; y = v ; KNC rows (Step_QuasiImplicitEuler_v :1209-1295)
-    CF_TRY(launch_forces(t, kModeFe, s->x, s->cache->R, cache_mat(s->cache), st));
-    NodeTerms nt;
-    nt.out = s->bvec; nt.y = s->y; nt.x = s->x; nt.v = s->v;
-    nt.knc_mask = s->num_knc ? s->knc_mask : nullptr;
-    nt.knc_target = s->knc_target;
-    nt.mass = s->node_mass;
-    nt.dt = dt; nt.inv_dt = 1.0f / dt;
-    nt.g[0] = s->p.gravity[0]; nt.g[1] = s->p.gravity[1]; nt.g[2] = s->p.gravity[2];
-    nt.warm_start = s->p.warm_start;
+    if (!fused) CF_TRY(launch_forces(t, kModeFe, s->x, s->cache->R, cache_mat(s->cache), st));
     CF_TRY(launch_gather(t, kGatherRhs, nt, node_blocks(s->N), st));
     // A = (1 + dt c0) M + (dt^2 + dt c1) K  (:1300-1302)
     const float fM = 1.0f + dt * s->rayleigh[0];
@@ -1486,7 +1550,7 @@ static int solid_fixed_step(cf_solid* s, float dt, cf_step_stats* stats) {
     // the CG record, V() and the BeginEvaluation statistics come back through
     // pinned memory with the step's one synchronisation.
     cf_cg_result res{};
-    CF_TRY(solid_cg(s, fM, fK, &res, true));
+    CF_TRY(solid_cg(s, fM, fK, &res, true, fused ? s->ay_part : nullptr));
     // v = y ; x = x + dt y (:1322-1327) ; air drag (:982-984)
     CF_TRY(launch_integrate(n3, s->x, s->v, s->y, dt, s->air_drag_per_second, st));
     *s->h_energy = 0.0;
@@ -1825,6 +1889,12 @@ int cf_set_tile_persistent(int persistent) {
 int cf_solid_set_graphs(cf_solid* s, int use_graphs) {
     if (!s) return CF_ERR_INVALID_ARG;
     s->use_graphs = use_graphs != 0;
+    return CF_OK;
+}
+
+int cf_set_fused_front(int enable) {
+    if (enable != 0 && enable != 1) { set_error("fused front must be 0 or 1"); return CF_ERR_INVALID_ARG; }
+    g_fused_front = enable;
     return CF_OK;
 }
 

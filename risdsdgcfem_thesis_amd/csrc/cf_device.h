@@ -616,34 +616,15 @@ __device__ inline void sum_slot_run(const Slots& s_F, uint32_t s0, uint32_t s1, 
 }
 
 // C_WRP element force in Piola form: P (column c is the force on vertex c+1).
+// The C_WRP forces of one element from its edge matrix D (columns v1-v0,
+// v2-v0, v3-v0 of the input vectors), rest edges Dm, Dm^-1 (G) and volume V.
 template <int MODE>
-__device__ inline void wrp_forces(const Slots& s_v, const Slots& s_r, const uint32_t* lv, const float* R,
-                                  const Material& m, float P[9]) {
+__device__ inline void wrp_forces_dg(const float D[9], const float Dm[9], const float G[9], float V, const float* R,
+                                     const Material& m, float P[9]) {
     // Every fused multiply-add is written out (fma3); nothing else may be
     // contracted, so the result is fixed by this source in every kernel that
     // inlines it.
 #pragma clang fp contract(off)
-    const float3 x0 = make_float3(s_v.x[lv[0]], s_v.y[lv[0]], s_v.z[lv[0]]);
-    const float3 x1 = make_float3(s_v.x[lv[1]], s_v.y[lv[1]], s_v.z[lv[1]]);
-    const float3 x2 = make_float3(s_v.x[lv[2]], s_v.y[lv[2]], s_v.z[lv[2]]);
-    const float3 x3 = make_float3(s_v.x[lv[3]], s_v.y[lv[3]], s_v.z[lv[3]]);
-    const float3 r0 = make_float3(s_r.x[lv[0]], s_r.y[lv[0]], s_r.z[lv[0]]);
-    const float3 r1 = make_float3(s_r.x[lv[1]], s_r.y[lv[1]], s_r.z[lv[1]]);
-    const float3 r2 = make_float3(s_r.x[lv[2]], s_r.y[lv[2]], s_r.z[lv[2]]);
-    const float3 r3 = make_float3(s_r.x[lv[3]], s_r.y[lv[3]], s_r.z[lv[3]]);
-    // Dm^-1 and V rebuilt from the rest positions staged in LDS with the same
-    // operations as k_precompute (bit-identical to the stored values): 12 B
-    // per tile node instead of 40 B per element from HBM.
-    const float Dm[9] = {r1.x - r0.x, r2.x - r0.x, r3.x - r0.x,
-                         r1.y - r0.y, r2.y - r0.y, r3.y - r0.y,
-                         r1.z - r0.z, r2.z - r0.z, r3.z - r0.z};
-    const float detDm = refmath::det3(Dm);
-    const float V = detDm / 6.0f;
-    float G[9];
-    refmath::inv3(Dm, detDm, G);
-    const float D[9] = {x1.x - x0.x, x2.x - x0.x, x3.x - x0.x,   // columns v1-v0, v2-v0, v3-v0
-                        x1.y - x0.y, x2.y - x0.y, x3.y - x0.y,
-                        x1.z - x0.z, x2.z - x0.z, x3.z - x0.z};
     float RD[9];  // R^T D  (f_e: minus the rest edges Dm, the local displacement differences)
 #pragma unroll
     for (int i3 = 0; i3 < 3; i3++)
@@ -678,6 +659,34 @@ __device__ inline void wrp_forces(const Slots& s_v, const Slots& s_r, const uint
 #pragma unroll
         for (int c = 0; c < 3; c++)
             P[3 * r + c] = fma3(R[3 * r + 0], W[0 + c], R[3 * r + 1], W[3 + c], R[3 * r + 2], W[6 + c]);
+}
+
+template <int MODE>
+__device__ inline void wrp_forces(const Slots& s_v, const Slots& s_r, const uint32_t* lv, const float* R,
+                                  const Material& m, float P[9]) {
+#pragma clang fp contract(off)
+    const float3 x0 = make_float3(s_v.x[lv[0]], s_v.y[lv[0]], s_v.z[lv[0]]);
+    const float3 x1 = make_float3(s_v.x[lv[1]], s_v.y[lv[1]], s_v.z[lv[1]]);
+    const float3 x2 = make_float3(s_v.x[lv[2]], s_v.y[lv[2]], s_v.z[lv[2]]);
+    const float3 x3 = make_float3(s_v.x[lv[3]], s_v.y[lv[3]], s_v.z[lv[3]]);
+    const float3 r0 = make_float3(s_r.x[lv[0]], s_r.y[lv[0]], s_r.z[lv[0]]);
+    const float3 r1 = make_float3(s_r.x[lv[1]], s_r.y[lv[1]], s_r.z[lv[1]]);
+    const float3 r2 = make_float3(s_r.x[lv[2]], s_r.y[lv[2]], s_r.z[lv[2]]);
+    const float3 r3 = make_float3(s_r.x[lv[3]], s_r.y[lv[3]], s_r.z[lv[3]]);
+    // Dm^-1 and V rebuilt from the rest positions staged in LDS with the same
+    // operations as k_precompute (bit-identical to the stored values): 12 B
+    // per tile node instead of 40 B per element from HBM.
+    const float Dm[9] = {r1.x - r0.x, r2.x - r0.x, r3.x - r0.x,
+                         r1.y - r0.y, r2.y - r0.y, r3.y - r0.y,
+                         r1.z - r0.z, r2.z - r0.z, r3.z - r0.z};
+    const float detDm = refmath::det3(Dm);
+    const float V = detDm / 6.0f;
+    float G[9];
+    refmath::inv3(Dm, detDm, G);
+    const float D[9] = {x1.x - x0.x, x2.x - x0.x, x3.x - x0.x,   // columns v1-v0, v2-v0, v3-v0
+                        x1.y - x0.y, x2.y - x0.y, x3.y - x0.y,
+                        x1.z - x0.z, x2.z - x0.z, x3.z - x0.z};
+    wrp_forces_dg<MODE>(D, Dm, G, V, R, m, P);
 }
 
 template <int MODE>
@@ -1114,6 +1123,9 @@ __device__ inline bool r_dapd_degenerate(float (*x)[3], float detF, float thr, u
 // element was undegenerate at prev_x, else the DoC is kept — and, under DAPD,
 // the rotation (ComputeRotations_DAPD :868-892), written to R. Returns 1 when
 // DAPD fell back to R = Id (no DoC, or a collapsed axis).
+// TAG gives each calling kernel its own copy: a shared non-inlined function
+// is allocated for the most registers any caller allows.
+template <int TAG = 0>
 __device__ __noinline__ uint32_t degenerate_element(uint32_t e, uint32_t T, const uint32_t* conn, const float* x,
                                                     const float* prev_x, uint8_t* doc, const float* vol,
                                                     const float* dinv, int rm, float thr, float detF, float fL,

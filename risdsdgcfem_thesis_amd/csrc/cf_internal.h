@@ -215,6 +215,26 @@ int launch_begin_eval(const TetDevice& t, const float* x, int rm, float thr, flo
                       uint32_t* blk_count, double* blk_sum, float* blk_min, int nblk, hipStream_t s,
                       const LcmOut& lcm = LcmOut(), const DegenArgs& dg = DegenArgs());
 int launch_forces(const TetDevice& t, int mode, const float* in, const float* R, Material m, hipStream_t s);
+// BeginEvaluation fused with the front of a QIE-v step, for C_WRP without the
+// exact rotation differential: per tile, x and the rest positions staged once
+// give each element's F, det F and rotation (stored as launch_begin_eval
+// stores them, with the same degeneration update), then f_e(x) into
+// t.partials and K y into partials2, y being the CG start of the right-hand
+// side gather. Statistics per tile: st_count [0, num_tiles) det F <= 0,
+// [num_tiles, 2 num_tiles) DAPD fallbacks; st_sum, st_min.
+struct FrontOut {
+    int rm = 0;
+    float thr = 0.f;
+    float* R = nullptr;
+    float* detF = nullptr;
+    uint32_t* st_count = nullptr;
+    double* st_sum = nullptr;
+    float* st_min = nullptr;
+    DegenArgs dg;
+    float* partials2 = nullptr;
+};
+int launch_qie_front(const TetDevice& t, const float* x, Material m, const FrontOut& fo, const NodeTerms& yt,
+                     hipStream_t s);
 // C_f_d element stage (Rayleigh damping of the velocities v) into the tile partials
 struct DampArgs {
     const float* inv_mass;  // N

@@ -166,6 +166,253 @@ __global__ __launch_bounds__(TB) void k_tile_forces_kind(TileArgs a) {
     tile_block<MODE, KIND>(a, t, L.v, L.r, L.F, L.sp);
 }
 
+// BeginEvaluation fused with the front of a QIE-v step (launch_qie_front):
+// one tile per workgroup, same tile mapping as the tile kernels. The tile's
+// positions and rest positions, staged once, give each element's F (Dm^-1
+// rebuilt from the rest positions as k_precompute builds it), det F and
+// rotation, which stays in registers for f_e(x) and, after y replaces x in
+// LDS, for K y. R, det F, the DoC and the two partial sets carry the same
+// bits as BeginEvaluation, the f_e tile pass and the CG start's K y tile pass
+// run one after another. Every global store goes through a uniform base and
+// a 32-bit offset (st32), so that no per-lane 64-bit address has to live
+// across the rotation's branches and the degenerate branch's call.
+struct FrontArgs {
+    TileArgs a;  // a.in = x, a.partials = f_e
+    NodeTerms yt;
+    const uint32_t* conn;
+    const float* dinv;
+    FrontOut fo;
+    uint32_t num_nodes, num_parts;  // byte sizes below 4 GiB (checked on the host)
+};
+
+// Buffer accesses for the fused front: a resource over n bytes from a uniform
+// base, a 32-bit byte offset per lane, and accesses past n bytes dropped by
+// the hardware. No per-lane 64-bit address lives across the rotation's
+// branches and the degenerate branch's call (a 64-bit store address kept
+// there was once compiled into a register pair that held other values on
+// one path). aux 2: non-temporal.
+using BufRsrc = __amdgpu_buffer_rsrc_t;
+typedef uint32_t Bu2 __attribute__((ext_vector_type(2)));
+__device__ inline BufRsrc buf(const void* p, uint32_t nbytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), 0, int(nbytes), 0x00020000);
+}
+__device__ inline float bld_f(BufRsrc r, uint32_t off) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+}
+__device__ inline uint32_t bld_u8(BufRsrc r, uint32_t off) { return __builtin_amdgcn_raw_buffer_load_b8(r, off, 0, 0); }
+__device__ inline void bst_f(BufRsrc r, uint32_t off, float v) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, off, 0, 0);
+}
+__device__ inline void bst_f_nt(BufRsrc r, uint32_t off, float v) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, off, 0, 2);
+}
+__device__ inline void bst_u(BufRsrc r, uint32_t off, uint32_t v) { __builtin_amdgcn_raw_buffer_store_b32(v, r, off, 0, 0); }
+__device__ inline void bst_u8(BufRsrc r, uint32_t off, uint32_t v) {
+    __builtin_amdgcn_raw_buffer_store_b8(uint8_t(v), r, off, 0, 0);
+}
+__device__ inline void bst_d(BufRsrc r, uint32_t off, double v) {
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(Bu2, v), r, off, 0, 0);
+}
+
+// The CG start y of Step_QuasiImplicitEuler_v (:1209-1295) at node n, the
+// right-hand side gather's y (gather_node<kGatherRhs>) in the same
+// operations: v with the warm start, else 0; a KNC node takes the constraint
+// velocity (target - x) / dt, rescaled to at most 50 and zero below 0.001.
+__device__ inline void front_start_y(uint32_t n, const FrontArgs& f, float y[3]) {
+#pragma clang fp contract(off)
+    const NodeTerms& t = f.yt;
+    const uint32_t N = f.num_nodes;
+    const uint32_t o = 12 * n;
+    if (t.knc_mask && bld_u8(buf(t.knc_mask, N), n)) {
+        const BufRsrc rt = buf(t.knc_target, 12 * N), rx = buf(t.x, 12 * N);
+        float vk[3];
+#pragma unroll
+        for (int c = 0; c < 3; c++) vk[c] = (bld_f(rt, o + 4 * c) - bld_f(rx, o + 4 * c)) * t.inv_dt;
+        const float nk = sqrtf(vk[0] * vk[0] + vk[1] * vk[1] + vk[2] * vk[2]);
+        if (nk > 0.001f) {
+            const float inv = 1.0f / nk, s = fminf(nk, 50.0f);
+#pragma unroll
+            for (int c = 0; c < 3; c++) vk[c] = s * (vk[c] * inv);
+        } else {
+            vk[0] = vk[1] = vk[2] = 0.f;
+        }
+#pragma unroll
+        for (int c = 0; c < 3; c++) y[c] = vk[c];
+    } else {
+        const BufRsrc rv = buf(t.v, 12 * N);
+#pragma unroll
+        for (int c = 0; c < 3; c++) y[c] = t.warm_start ? bld_f(rv, o + 4 * c) : 0.0f;
+    }
+}
+
+// BeginEvaluation's rotation and degeneration update of element e, the
+// sequence of begin_eval_block: the rotation of the method, then
+// UpdateDegeneration (:477-532), which clears an undegenerate element's DoC
+// and, for a degenerate one, updates the DoC and under DAPD writes R itself.
+// R is stored non-temporally (the tile stage reads it in the next pass) and
+// returned in Rm; fb counts DAPD's R = Id fallbacks.
+template <int TAG>
+__device__ inline void front_rotation(const FrontArgs& f, uint32_t e, const float F[9], float d, float Rm[9],
+                                      uint32_t& fb) {
+#pragma clang fp contract(off)
+    const FrontOut& fo = f.fo;
+    const uint32_t T = f.a.T;
+    const BufRsrc rR = buf(fo.R, 36 * T);
+#pragma unroll
+    for (int k = 0; k < 9; k++) Rm[k] = (k % 4 == 0) ? 1.f : 0.f;
+    if (fo.rm == CF_RM_MSVD) {
+        refmath::svd_rotation(F, Rm);
+    } else if (fo.rm == CF_RM_DAPD) {
+        if (d > fo.thr) refmath::polar_newton(F, Rm);
+    }
+    bool r_stored = false;
+    if (fo.dg.doc) {
+        const BufRsrc rd = buf(fo.dg.doc, T);
+        if (d > fo.thr) {
+            if (bld_u8(rd, e) != kDocInvalid) bst_u8(rd, e, kDocInvalid);
+        } else {
+            const float* x = f.a.in;
+            const uint32_t fl = degenerate_element<TAG>(e, T, f.conn, x, fo.dg.prev_x ? fo.dg.prev_x : x, fo.dg.doc,
+                                                        fo.dg.vol, f.dinv, fo.rm, fo.thr, d, fo.dg.l_factor,
+                                                        fo.dg.nl_factor, fo.dg.nl_exponent, fo.R);
+            if (fo.rm == CF_RM_DAPD) {
+                fb += fl;
+                r_stored = true;
+                for (int k = 0; k < 9; k++) Rm[k] = bld_f(rR, 4 * (k * T + e));
+            }
+        }
+    } else if (fo.rm == CF_RM_DAPD && !(d > fo.thr)) {
+        fb++;  // no DoC tracking: R = Id
+    }
+    if (!r_stored)
+        for (int k = 0; k < 9; k++) bst_f_nt(rR, 4 * (k * T + e), Rm[k]);
+}
+
+template <int TB, int W>
+__global__ __launch_bounds__(TB, W) void k_qie_front(FrontArgs f) {
+#pragma clang fp contract(off)
+    extern __shared__ float smem_f[];
+    __shared__ double s_sum[TB / 64];
+    __shared__ float s_min[TB / 64];
+    __shared__ uint32_t s_cnt[TB / 64], s_fb[TB / 64];
+    const TileLds<TB> L(smem_f);
+    const TileArgs& a = f.a;
+    const FrontOut& fo = f.fo;
+    const uint32_t tid = threadIdx.x;
+    const uint32_t xcd = blockIdx.x % 8, j = blockIdx.x / 8;
+    const uint32_t per = (a.num_tiles + 7) / 8;
+    const uint32_t lo = xcd * per;
+    const uint32_t hi = min(a.num_tiles, lo + per);
+    const uint32_t t = lo + j;
+    if (t >= hi) return;
+    const TileDesc dt = tile_desc(a, t);
+    // the cold tile loads without R; lanes past the tile load valid duplicates
+    const uint32_t tn = min(tid, dt.nn - 1);
+    const uint32_t gid = ld32(a.tile_nodes, dt.n0 + tn);
+    const uint32_t e = dt.e0 + min(tid, dt.ne - 1);
+    const uint32_t lv = ld32_nt(a.lv, e), rk = ld32_nt(a.rk, e);
+    const uint32_t sp = ld32(a.slot_ptr, dt.n0 + min(tid, dt.nn)) - 4 * dt.e0;
+    const uint32_t pd = ld32(a.part_dst, dt.n0 + tn);
+    float rp[3] = {0.f, 0.f, 0.f}, xp[3] = {0.f, 0.f, 0.f};
+    if (a.ref_tile) {
+        const uint32_t q = 3 * (dt.n0 + tn);
+        rp[0] = ld32(a.ref_tile, q + 0); rp[1] = ld32(a.ref_tile, q + 1); rp[2] = ld32(a.ref_tile, q + 2);
+    }
+    if (tid < dt.nn) {
+        xp[0] = ld32(a.in, 3 * gid + 0); xp[1] = ld32(a.in, 3 * gid + 1); xp[2] = ld32(a.in, 3 * gid + 2);
+        if (!a.ref_tile) {
+            const BufRsrc rr = buf(a.ref, 12 * f.num_nodes);
+            rp[0] = bld_f(rr, 12 * gid); rp[1] = bld_f(rr, 12 * gid + 4); rp[2] = bld_f(rr, 12 * gid + 8);
+        }
+        L.v.x[tid] = xp[0]; L.v.y[tid] = xp[1]; L.v.z[tid] = xp[2];
+        L.r.x[tid] = rp[0]; L.r.y[tid] = rp[1]; L.r.z[tid] = rp[2];
+    }
+    if (tid <= dt.nn) L.sp[tid] = sp;
+    uint32_t cl[4], fs[4] = {0u, 0u, 0u, 0u};
+#pragma unroll
+    for (int q = 0; q < 4; q++) cl[q] = (lv >> (8 * q)) & 255u;
+    lds_barrier();
+    // BeginEvaluation of the thread's element
+    float Rm[9] = {1.f, 0.f, 0.f, 0.f, 1.f, 0.f, 0.f, 0.f, 1.f};
+    uint32_t cnt = 0, fb = 0;
+    double sdet = 0.0;
+    float mdet = 3.0e38f;
+    if (tid < dt.ne) {
+#pragma unroll
+        for (int q = 0; q < 4; q++) fs[q] = L.sp[cl[q]] + ((rk >> (8 * q)) & 255u);
+        float xs[4][3], rs[4][3];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            xs[q][0] = L.v.x[cl[q]]; xs[q][1] = L.v.y[cl[q]]; xs[q][2] = L.v.z[cl[q]];
+            rs[q][0] = L.r.x[cl[q]]; rs[q][1] = L.r.y[cl[q]]; rs[q][2] = L.r.z[cl[q]];
+        }
+        float Ds[9], Dm[9], G[9], F[9];
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            Ds[3 * i + 0] = xs[1][i] - xs[0][i];
+            Ds[3 * i + 1] = xs[2][i] - xs[0][i];
+            Ds[3 * i + 2] = xs[3][i] - xs[0][i];
+            Dm[3 * i + 0] = rs[1][i] - rs[0][i];
+            Dm[3 * i + 1] = rs[2][i] - rs[0][i];
+            Dm[3 * i + 2] = rs[3][i] - rs[0][i];
+        }
+        refmath::inv3(Dm, refmath::det3(Dm), G);
+        refmath::mul3(Ds, G, F);
+        const float d = refmath::det3(F);
+        bst_f(buf(fo.detF, 4 * a.T), 4 * e, d);
+        if (d <= 0.0f) { cnt++; sdet += d; }
+        mdet = fminf(mdet, d);
+        front_rotation<16 * TB + W>(f, e, F, d, Rm, fb);  // a callee copy per instantiation
+        element_force<kModeFe>(L.v, L.r, L.F, cl, fs, Rm, a.m);
+    }
+    // the tile's statistics: wave sums, then the waves in order
+    sdet = wave_sum(sdet);
+    for (int o = 32; o > 0; o >>= 1) {
+        mdet = fminf(mdet, __shfl_xor(mdet, o, 64));
+        cnt += __shfl_xor(cnt, o, 64);
+        fb += __shfl_xor(fb, o, 64);
+    }
+    if ((tid & 63) == 0) {
+        const uint32_t w = tid >> 6;
+        s_sum[w] = sdet; s_min[w] = mdet; s_cnt[w] = cnt; s_fb[w] = fb;
+    }
+    lds_barrier();
+    if (tid == 0) {
+        double ss = s_sum[0];
+        float mm = s_min[0];
+        uint32_t cc = s_cnt[0], ff = s_fb[0];
+        for (int w = 1; w < TB / 64; w++) { ss += s_sum[w]; mm = fminf(mm, s_min[w]); cc += s_cnt[w]; ff += s_fb[w]; }
+        const BufRsrc rc = buf(fo.st_count, 8 * a.num_tiles);
+        bst_u(rc, 4 * t, cc);
+        bst_u(rc, 4 * (a.num_tiles + t), ff);
+        bst_d(buf(fo.st_sum, 8 * a.num_tiles), 8 * t, ss);
+        bst_f(buf(fo.st_min, 4 * a.num_tiles), 4 * t, mm);
+    }
+    // f_e per tile node
+    if (tid < dt.nn) {
+        float fx, fy, fz;
+        sum_slot_run(L.F, L.sp[tid], L.sp[tid + 1], fx, fy, fz);
+        const BufRsrc rp = buf(a.partials, 12 * f.num_parts);
+        bst_f(rp, 12 * pd, fx); bst_f(rp, 12 * pd + 4, fy); bst_f(rp, 12 * pd + 8, fz);
+    }
+    lds_barrier();
+    // K y with the same rotations (y read now: nothing held across the rotation)
+    if (tid < dt.nn) {
+        float y[3];
+        front_start_y(gid, f, y);
+        L.v.x[tid] = y[0]; L.v.y[tid] = y[1]; L.v.z[tid] = y[2];
+    }
+    lds_barrier();
+    if (tid < dt.ne) element_force<kModeDfx>(L.v, L.r, L.F, cl, fs, Rm, a.m);
+    lds_barrier();
+    if (tid < dt.nn) {
+        float fx, fy, fz;
+        sum_slot_run(L.F, L.sp[tid], L.sp[tid + 1], fx, fy, fz);
+        const BufRsrc rp = buf(fo.partials2, 12 * f.num_parts);
+        bst_f(rp, 12 * pd, fx); bst_f(rp, 12 * pd + 4, fy); bst_f(rp, 12 * pd + 8, fz);
+    }
+}
+
 // C_f_d element stage (TetrahedronElementSet3.cpp:1098-1150): the C_WRP
 // stiffness force of the velocities (a.in) scaled by c1, minus the
 // per-vertex mass term R (c0 m_j R^T v_j); tile node masses staged in LDS.
@@ -618,6 +865,42 @@ int launch_begin_eval(const TetDevice& t, const float* x, int rm, float thr, flo
     return check_launch();
 }
 
+
+int launch_qie_front(const TetDevice& t, const float* x, Material m, const FrontOut& fo, const NodeTerms& yt,
+                     hipStream_t s) {
+    if (t.num_tiles == 0) return CF_OK;
+    FrontArgs f;
+    TileArgs& a = f.a;
+    a.T = t.T; a.num_tiles = t.num_tiles; a.max_te = t.max_tile_elems; a.max_tn = t.max_tile_nodes;
+    a.tile_elem_ptr = t.tile_elem_ptr; a.tile_node_ptr = t.tile_node_ptr; a.tile_nodes = t.tile_nodes;
+    a.part_dst = t.part_dst;
+    a.slot_ptr = t.slot_ptr; a.lv = t.lv; a.rk = t.rk; a.R = fo.R; a.dinv = t.dinv; a.vol = t.vol;
+    a.ref = t.ref;
+    a.ref_tile = t.ref_tile;
+    a.uniform_te = t.uniform_te;
+    a.in = x; a.partials = t.partials; a.m = m; a.skip = nullptr;
+    f.yt = yt;
+    f.conn = t.conn;
+    f.dinv = t.dinv;
+    f.fo = fo;
+    f.num_nodes = t.N;
+    f.num_parts = t.num_parts;
+    const uint32_t g1 = 8 * ((t.num_tiles + 7) / 8);  // one workgroup per tile
+// 6 waves per SIMD (80 VGPRs): the element stage's rotation and forces are
+// ALU- and latency-bound, and 3 workgroups of 512 threads per CU measured
+// 183 us on the 5.24M-tet mesh against 220 (2 per CU, 96 VGPRs), 227 (4, 64
+// VGPRs with spills) and 327 (1, 164 VGPRs).
+#define CF_FRONT(TBV, WV) \
+    hipLaunchKernelGGL((k_qie_front<TBV, WV>), dim3(g1), dim3(TBV), TileLds<TBV>::kBytes, s, f)
+    switch (t.tile_threads) {
+        case 1024: CF_FRONT(1024, 4); break;
+        case 512: CF_FRONT(512, 6); break;
+        case 128: CF_FRONT(128, 6); break;
+        default: CF_FRONT(256, 6); break;
+    }
+#undef CF_FRONT
+    return check_launch();
+}
 
 int launch_forces_skip(const TetDevice& t, int mode, const float* in, const float* R, Material m,
                        const int32_t* skip, hipStream_t s, bool reverse) {

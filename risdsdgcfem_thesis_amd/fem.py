@@ -229,6 +229,11 @@ def set_tile_alternate(level):
     check(lib.cf_set_tile_alternate(int(level)), "cf_set_tile_alternate")
 
 
+def set_fused_front(enable):
+    """1 (default): QIE-v leaf steps open with the fused BeginEvaluation + f_e + K y tile pass; 0: separate passes."""
+    check(lib.cf_set_fused_front(int(enable)), "cf_set_fused_front")
+
+
 def get_tile_elems():
     v = C.c_uint32()
     check(lib.cf_get_tile_elems(C.byref(v)), "cf_get_tile_elems")
