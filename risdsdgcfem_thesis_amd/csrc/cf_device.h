@@ -490,9 +490,13 @@ struct TileRegs {
 // base: the load takes the base in scalar registers and one 32-bit vector
 // offset, with no 64-bit address arithmetic per lane. Every per-mesh array
 // the tile stage reads is below 4 GiB (T, N < 2^28 are checked on the host).
+// The arrays are device allocations, so the load says global: where the base
+// comes from memory (the batch's per-simulation descriptors) rather than from
+// a kernel argument, the compiler would otherwise issue flat loads.
+#define CF_GLOBAL __attribute__((address_space(1)))
 template <typename V>
 __device__ inline V ld32(const V* base, uint32_t i) {
-    return *reinterpret_cast<const V*>(reinterpret_cast<const char*>(base) + i * uint32_t(sizeof(V)));
+    return *(const CF_GLOBAL V*)((const CF_GLOBAL char*)base + i * uint32_t(sizeof(V)));
 }
 // The same load marked non-temporal. The tile stage reads each element's R
 // and packed ids (44 B) once per K*x pass, 231 MB on the 5.24M-tet mesh; read
@@ -504,8 +508,7 @@ __device__ inline V ld32(const V* base, uint32_t i) {
 // were the partials stored non-temporally (6.9 ms: the gather reads them next).
 template <typename V>
 __device__ inline V ld32_nt(const V* base, uint32_t i) {
-    return __builtin_nontemporal_load(reinterpret_cast<const V*>(reinterpret_cast<const char*>(base) +
-                                                                 i * uint32_t(sizeof(V))));
+    return __builtin_nontemporal_load((const CF_GLOBAL V*)((const CF_GLOBAL char*)base + i * uint32_t(sizeof(V))));
 }
 
 __device__ inline void load_tile_regs(const TileArgs& a, const TileDesc& d, uint32_t gid, uint32_t tid,
