@@ -396,8 +396,10 @@ struct CgRun {
 };
 
 // ay: the K y tile partials when the fused front computed them (else null:
-// the tile pass runs here).
-static int cg_begin(cf_solid* s, float fM, float fK, CgRun* run, const float* ay = nullptr) {
+// the tile pass runs here); rhs: the right-hand side gather's terms when it
+// is still to run (after the fused front), fused here with the CG start.
+static int cg_begin(cf_solid* s, float fM, float fK, CgRun* run, const float* ay = nullptr,
+                    const NodeTerms* rhs = nullptr) {
     const TetDevice& t = s->set->dev;
     hipStream_t st = s->stream;
     CgBuffers& b = s->cg;
@@ -412,7 +414,11 @@ static int cg_begin(cf_solid* s, float fM, float fK, CgRun* run, const float* ay
     // r = S(b - A y), d = r and the r.r partials in the gather of A y
     NodeTerms ni = nt;
     ni.d = s->y; ni.b = s->bvec; ni.r_out = b.r; ni.d_out = b.d; ni.partials = b.part_b;
-    if (ay) {
+    if (ay && rhs) {
+        NodeTerms nr = *rhs;
+        nr.fM = fM; nr.fK = fK; nr.r_out = b.r; nr.d_out = b.d; nr.partials = b.part_b;
+        CF_TRY(launch_rhs_start(t, ay, nr, st));
+    } else if (ay) {
         TetDevice ta = t;
         ta.partials = const_cast<float*>(ay);
         CF_TRY(launch_gather(ta, kGatherCgInit, ni, 0, st));
@@ -547,9 +553,9 @@ static int cg_finish(cf_solid* s, const CgRun& run, cf_cg_result* res) {
 // of the scalar record into pinned memory (s->h_cgs) queued behind them; the
 // caller reads the result (cg_result_from) once the stream has passed it.
 static int solid_cg(cf_solid* s, float fM, float fK, cf_cg_result* res, bool defer = false,
-                    const float* ay = nullptr) {
+                    const float* ay = nullptr, const NodeTerms* rhs = nullptr) {
     CgRun run;
-    CF_TRY(cg_begin(s, fM, fK, &run, ay));
+    CF_TRY(cg_begin(s, fM, fK, &run, ay, rhs));
     CgBuffers& b = s->cg;
     uint32_t kend[2] = {0, 0};
     CF_TRY(cg_launch(s, &run, s->use_graphs));
@@ -1541,8 +1547,10 @@ static int solid_fixed_step(cf_solid* s, float dt, cf_step_stats* stats) {
         }
         return nonfinite_status(s);
     }
-    if (!fused) CF_TRY(launch_forces(t, kModeFe, s->x, s->cache->R, cache_mat(s->cache), st));
-    CF_TRY(launch_gather(t, kGatherRhs, nt, node_blocks(s->N), st));
+    if (!fused) {
+        CF_TRY(launch_forces(t, kModeFe, s->x, s->cache->R, cache_mat(s->cache), st));
+        CF_TRY(launch_gather(t, kGatherRhs, nt, node_blocks(s->N), st));
+    }  // fused: the right-hand side gather runs with the CG start
     // A = (1 + dt c0) M + (dt^2 + dt c1) K  (:1300-1302)
     const float fM = 1.0f + dt * s->rayleigh[0];
     const float fK = dt * dt + dt * s->rayleigh[1];
@@ -1550,7 +1558,7 @@ static int solid_fixed_step(cf_solid* s, float dt, cf_step_stats* stats) {
     // the CG record, V() and the BeginEvaluation statistics come back through
     // pinned memory with the step's one synchronisation.
     cf_cg_result res{};
-    CF_TRY(solid_cg(s, fM, fK, &res, true, fused ? s->ay_part : nullptr));
+    CF_TRY(solid_cg(s, fM, fK, &res, true, fused ? s->ay_part : nullptr, fused ? &nt : nullptr));
     // v = y ; x = x + dt y (:1322-1327) ; air drag (:982-984)
     CF_TRY(launch_integrate(n3, s->x, s->v, s->y, dt, s->air_drag_per_second, st));
     *s->h_energy = 0.0;

@@ -235,6 +235,11 @@ struct FrontOut {
 };
 int launch_qie_front(const TetDevice& t, const float* x, Material m, const FrontOut& fo, const NodeTerms& yt,
                      hipStream_t s);
+// After the fused front: b and y (right-hand side gather) and the CG start
+// r = S(b - A y), d = r with per-chunk r.r partials, in one pass; nt carries
+// both gathers' terms (out = b, y, r_out, d_out, partials = the r.r partials)
+// and ay the K y partials.
+int launch_rhs_start(const TetDevice& t, const float* ay, const NodeTerms& nt, hipStream_t s);
 // C_f_d element stage (Rayleigh damping of the velocities v) into the tile partials
 struct DampArgs {
     const float* inv_mass;  // N

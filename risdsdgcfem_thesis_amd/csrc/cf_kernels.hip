@@ -471,6 +471,80 @@ __global__ __launch_bounds__(kBlock) void k_gather(GatherArgs a) {
 
 // The matvec gather in chunks of npt kBlock nodes, one chunk per workgroup;
 // chunk c's d.q partial in t.partials[c].
+// The QIE-v right-hand side and the CG start in one pass, after the fused
+// step front: over the matvec chunks (node (c npt + j) kBlock + thread), b and
+// y of each node from the f_e partials in the operations of
+// gather_node<kGatherRhs>, then q = S(fM m y - fK K y) from the K y partials
+// (ay) and r = S(b - q), d = r in those of gather_matvec_chunk<true>, with the
+// chunk's r.r partial summed in the same order. The outputs carry the bits of
+// the two gathers run one after the other.
+__global__ __launch_bounds__(kBlock) void k_rhs_start(GatherArgs a, const float* __restrict__ ay, uint32_t npt) {
+#pragma clang fp contract(off)
+    __shared__ double sh[kBlock / 64];
+    const NodeTerms& t = a.t;
+    double acc = 0.0;
+    for (uint32_t j = 0; j < npt; j++) {
+        const uint32_t n = (blockIdx.x * npt + j) * kBlock + threadIdx.x;
+        if (n >= a.N) continue;
+        const size_t i = 3 * (size_t)n;
+        const uint32_t p0 = a.node_part_ptr[n], p1 = a.node_part_ptr[n + 1];
+        // right-hand side (gather_node<kGatherRhs>)
+        float fx = 0.f, fy = 0.f, fz = 0.f;
+        for (uint32_t p = p0; p < p1; p++) {
+            const float* pp = a.partials + 3 * (size_t)p;
+            fx += pp[0]; fy += pp[1]; fz += pp[2];
+        }
+        const float mass = t.mass_array ? t.mass_array[n] : t.mass;
+        const float f[3] = {fx, fy, fz};
+        float b[3], y[3];
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const float vc = t.v[i + c];
+            b[c] = (mass * t.g[c] + f[c]) * t.dt + mass * vc;
+            y[c] = t.warm_start ? vc : 0.0f;
+        }
+        const bool knc = t.knc_mask && t.knc_mask[n];
+        if (knc) {
+            float vk[3];
+#pragma unroll
+            for (int c = 0; c < 3; c++) vk[c] = (t.knc_target[i + c] - t.x[i + c]) * t.inv_dt;
+            const float nk = sqrtf(vk[0] * vk[0] + vk[1] * vk[1] + vk[2] * vk[2]);
+            if (nk > 0.001f) {
+                const float inv = 1.0f / nk, sc = fminf(nk, 50.0f);
+#pragma unroll
+                for (int c = 0; c < 3; c++) vk[c] = sc * (vk[c] * inv);
+            } else {
+                vk[0] = vk[1] = vk[2] = 0.f;
+            }
+#pragma unroll
+            for (int c = 0; c < 3; c++) { b[c] = vk[c]; y[c] = vk[c]; }
+        }
+#pragma unroll
+        for (int c = 0; c < 3; c++) { t.out[i + c] = b[c]; t.y[i + c] = y[c]; }
+        // CG start (gather_matvec_chunk<true>, cg_start_node)
+        float kx = 0.f, ky = 0.f, kz = 0.f;
+        for (uint32_t p = p0; p < p1; p++) {
+            const float* pp = ay + 3 * (size_t)p;
+            kx += pp[0]; ky += pp[1]; kz += pp[2];
+        }
+        const float fac = mass * t.fM;
+        float q[3] = {-t.fK * kx + fac * y[0], -t.fK * ky + fac * y[1], -t.fK * kz + fac * y[2]};
+        if (knc) q[0] = q[1] = q[2] = 0.f;
+        double node = 0.0;  // the node's r.r, then the thread's sum (cg_start_node)
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            float ri = -1.0f * q[k] + b[k];
+            if (knc) ri = 0.f;
+            t.r_out[i + k] = ri;
+            t.d_out[i + k] = ri;
+            node += (double)ri * ri;
+        }
+        acc += node;
+    }
+    const double sum = block_sum(acc, sh);
+    if (threadIdx.x == 0) t.partials[blockIdx.x] = sum;
+}
+
 template <bool INIT>
 __global__ __launch_bounds__(kBlock) void k_gather_matvec(GatherArgs a, uint32_t npt) {
     __shared__ double sh[kBlock / 64];
@@ -1025,6 +1099,14 @@ int launch_gather_skip(const TetDevice& t, int mode, const NodeTerms& nt, int nb
         hipLaunchKernelGGL(k_gather<kGatherRhsDx>, dim3(nblk), dim3(kBlock), 0, s, a);
     else
         hipLaunchKernelGGL(k_gather<kGatherRhs>, dim3(nblk), dim3(kBlock), 0, s, a);
+    return check_launch();
+}
+
+int launch_rhs_start(const TetDevice& t, const float* ay, const NodeTerms& nt, hipStream_t s) {
+    if (t.N == 0) return CF_OK;
+    GatherArgs a;
+    a.N = t.N; a.node_part_ptr = t.node_part_ptr; a.partials = t.partials; a.t = nt; a.skip = nullptr;
+    hipLaunchKernelGGL(k_rhs_start, dim3(matvec_chunks(t.N)), dim3(kBlock), 0, s, a, ay, matvec_npt(t.N));
     return check_launch();
 }
 
